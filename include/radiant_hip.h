@@ -1,0 +1,171 @@
+/*
+ * radiant_hip.h -- C ABI of the MI355X (gfx950) backend for Radiant.jl's discrete-ordinates
+ * transport solve.
+ *
+ * The boundary replaces the body of
+ *     compute_flux(cross_sections, geometry, solver::SN, source, electromagnetic_field)
+ * (src/particle_transport/sn_flux.jl:22-234): Radiant's host code keeps building the
+ * quadrature, the Galerkin operators, the closure weights and the cross-section arrays
+ * (sn_flux.jl:27-148); the group loop and the in-group solves (sn_flux.jl:150-224, which call
+ * sn_one_speed.jl:81-196 and sn_inner_pass.jl:212-332) run on the GPU behind one call.
+ *
+ * Conventions
+ *   - Plain pointers and extents only; arrays are Julia column-major as Radiant builds them.
+ *   - The caller owns every host buffer for the duration of a call; the library copies in/out.
+ *   - Device memory is owned by the library (a problem handle or a server handle).
+ *   - Return value 0 on success, a negative RT_E* status on failure; rt_last_error() gives the
+ *     message of the calling thread's last failure (the Julia shim maps it to error()).
+ *   - rt_solve is blocking and may be called from several host threads.
+ */
+#ifndef RADIANT_HIP_H
+#define RADIANT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define RT_OK 0
+#define RT_EINVAL -1      /* invalid argument / unsupported option */
+#define RT_EDEVICE -2     /* HIP runtime error */
+#define RT_ENOMEM -3      /* device memory budget exceeded */
+#define RT_ECANCELLED -4  /* request cancelled (server) */
+#define RT_EPENDING -5    /* request not finished yet (server poll) */
+
+/* acceleration of the in-group iteration (structures/SN.jl:44-63, sn_one_speed.jl:151-181) */
+#define RT_ACCEL_NONE 0
+#define RT_ACCEL_LIVOLANT 1
+#define RT_ACCEL_ANDERSON 2
+#define RT_ACCEL_GMRES 3
+#define RT_ACCEL_BICGSTAB 4
+
+/* precision of the transport passes */
+#define RT_PREC_FP64 0          /* plain FP64 (parity mode) */
+#define RT_PREC_MIXED_FP32 1    /* FP32 sweep + FP32 M/D, FP64 Arnoldi and iterative refinement */
+#define RT_PREC_MIXED_BF16 2    /* as above with BF16 M/D operands (FP32 accumulation) */
+
+/* Krylov state */
+#define RT_STATE_FAITHFUL 0     /* all Np moments + boundary moments, as sn_one_speed.jl:126-134 */
+#define RT_STATE_REDUCED 1      /* the moments that feed back (Legendre order <= XS order) only */
+
+typedef struct rt_problem_desc {
+    /* geometry (Geometry.get_*: dimension, number_of_voxels, voxels_width, material_per_voxel,
+       boundary_conditions) */
+    int32_t ndims, Nx, Ny, Nz;          /* Ny = Nz = 1 below their dimension */
+    const double *dx, *dy, *dz;         /* cell widths per axis */
+    const int64_t *mat;                 /* [Nx*Ny*Nz] 1-based material index, x fastest */
+    const int32_t *bc;                  /* [2*ndims] X-,X+,Y-,Y+,Z-,Z+ : 0 void, 1 reflective, 2 periodic */
+    /* cross sections (sn_flux.jl:69-83) */
+    int32_t Nmat, G;
+    const double *sigt;                 /* Σtot[G, Nmat] */
+    const double *sigs;                 /* Σs[Nmat, G(from), G(to), Ls+1] */
+    int32_t Ls;                         /* Legendre slots - 1 of sigs */
+    /* volume source: moment p=1 of Source.volume_sources (Source.jl:132-137), [G, Nx*Ny*Nz] */
+    const double *qvol;
+    /* angular discretisation (sn_flux.jl:44-48) */
+    int32_t Nd, Np, Nps;
+    const double *mu, *eta, *xi, *w;    /* [Nd] */
+    const double *M;                    /* Mn[Nd, Np]  (moment -> direction) */
+    const double *D;                    /* Dn[Np, Nd]  (direction -> moment) */
+    const int64_t *pl;                  /* [Np] Legendre order of each moment */
+    const double *Msurf[6];             /* Mn_surf[f][Nd+, Nps] per face (X-,X+,Y-,Y+,Z-,Z+) */
+    const double *Dsurf[6];             /* Dn_surf[f][Nps, Nd+] */
+    const int64_t *n_to_np[6];          /* n_to_n⁺[f][Nd], 1-based, 0 = direction not entering f */
+    /* solver (SN.get_acceleration, get_gmres_restart, get_anderson_depth,
+       get_maximum_iteration, get_convergence_criterion) */
+    int32_t accel, restart, depth, maxit;
+    double tol;
+    /* backend options */
+    int32_t precision;                  /* RT_PREC_* */
+    int32_t state_mode;                 /* RT_STATE_* (RT_PREC_MIXED_* implies REDUCED) */
+    int32_t verbose;                    /* print Radiant's per-group convergence lines */
+} rt_problem_desc;
+
+/* Per-group result of the in-group solve (the values sn_one_speed.jl:189-193 prints). */
+typedef struct rt_group_stats {
+    int64_t niter;      /* matvecs / fixed-point applications of the solver */
+    double resid;       /* final relative residual */
+    double rho;         /* spectral-radius estimate (krylov_state.jl:139-144), NaN if undefined */
+    int32_t converged;
+    int32_t passes;     /* transport passes of the group (all precisions) */
+    int32_t passes_lo;  /* of which low-precision passes (mixed modes) */
+    int32_t refinements;/* iterative-refinement steps (mixed modes) */
+} rt_group_stats;
+
+/* Timing of the last solve on a handle (device events). */
+typedef struct rt_solve_info {
+    double ms_device;           /* wall time of the solve on the device stream */
+    int64_t passes;             /* transport passes, all groups */
+    int64_t passes_lo;          /* low-precision passes */
+    double updates_raw;         /* sum over passes of Nc*Nd */
+    double updates_problem;     /* Nc*Nd*G */
+    double ms_sweep, ms_mgemm, ms_dgemm, ms_krylov;  /* per-phase device time (profiling mode) */
+} rt_solve_info;
+
+/* ------------------------------------------------------------------------------------------
+ * One-shot solve: replaces sn_flux.jl:150-224 and returns 𝚽l[G, Np, 1, Nx, Ny, Nz]
+ * (phi_out, caller-allocated, G*Np*Nx*Ny*Nz doubles) and stats_out[G] (may be NULL).
+ * ------------------------------------------------------------------------------------------ */
+int rt_solve(const rt_problem_desc *p, double *phi_out, rt_group_stats *stats_out);
+
+/* Resident problem: upload once, solve many times (inputs stay in HBM). */
+typedef struct rt_problem rt_problem;
+int rt_problem_create(const rt_problem_desc *p, rt_problem **out);
+int rt_problem_solve(rt_problem *h, rt_group_stats *stats_out, rt_solve_info *info);
+int rt_problem_get_flux(rt_problem *h, double *phi_out);
+int rt_problem_destroy(rt_problem *h);
+/* Device bytes a problem needs (admission control). */
+int rt_problem_device_bytes(const rt_problem_desc *p, int64_t *bytes);
+
+/* Kernel-level hook: one transport pass of group g (0-based) on a host state vector laid out
+ * as sn_one_speed.jl:126-134 builds it: [𝚽l_g (Np fastest, then cells) | 𝚽x12 | 𝚽y12 | 𝚽z12]
+ * with 𝚽x12[Nps, 2, Ny, Nz], 𝚽y12[Nps, 2, Nx, Nz], 𝚽z12[Nps, 2, Nx, Ny].
+ * mode 0: T(z) (sn_inner_pass! homogeneous=false), 1: A z (homogeneous=true),
+ * 2: z - A z (the GMRES matvec, sn_one_speed.jl:147).  qlout: out-of-group source
+ * [Np, Nx*Ny*Nz] (Np fastest).  z_out may alias z_in.  Precision follows the handle. */
+int rt_problem_pass(rt_problem *h, int32_t g, int32_t mode, const double *qlout, const double *z_in,
+                    double *z_out);
+/* State length of the faithful Krylov state of a handle. */
+int rt_problem_state_length(rt_problem *h, int64_t *S);
+
+/* Kernel-level test hook (no counterpart in the reference): C[M,N] = A[M,K] . B[K,N], all
+ * row-major FP64 on the host, through the moment->direction (which = 0) or the
+ * direction->moment (which = 1) contraction kernel at precision prec (RT_PREC_*). */
+int rt_test_contraction(int32_t which, int32_t prec, int32_t M, int32_t K, int32_t N, const double *A,
+                        const double *B, double *C);
+
+/* ------------------------------------------------------------------------------------------
+ * Batched solve: n independent problems advanced together (one sweep launch covers every
+ * problem's directions, per-problem Krylov state). phi_out[i] as rt_solve; stats_out[i][G_i].
+ * ------------------------------------------------------------------------------------------ */
+int rt_solve_batch(int32_t n, const rt_problem_desc *const *p, double *const *phi_out,
+                   rt_group_stats *const *stats_out, rt_solve_info *info);
+
+/* ------------------------------------------------------------------------------------------
+ * Serving: continuous batching of fixed-source solves admitted by an HBM budget.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct rt_server rt_server;
+typedef struct rt_server_config {
+    int64_t hbm_budget_bytes;   /* device memory the server may hold for requests */
+    int32_t max_active;         /* maximum number of requests resident at once */
+    int32_t precision;          /* RT_PREC_* applied to every request */
+} rt_server_config;
+int rt_server_create(const rt_server_config *cfg, rt_server **out);
+/* Copies the problem; returns a request id (>= 0) in *req. */
+int rt_server_submit(rt_server *s, const rt_problem_desc *p, int64_t *req);
+/* RT_OK and the results when finished, RT_EPENDING while queued/running, RT_ECANCELLED. */
+int rt_server_poll(rt_server *s, int64_t req, double *phi_out, rt_group_stats *stats_out,
+                   double *latency_ms);
+int rt_server_cancel(rt_server *s, int64_t req);
+int rt_server_destroy(rt_server *s);
+
+const char *rt_last_error(void);
+/* Library/device identification string (e.g. "radiant_hip 0.1 gfx950 ..."). */
+const char *rt_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RADIANT_HIP_H */

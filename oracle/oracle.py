@@ -1,0 +1,187 @@
+"""ctypes wrapper of the CPU oracle (oracle/radiant_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by the tests, by ``__graft_entry__.smoke()`` and by the
+``cpu_baseline`` leg of ``bench.py``, never by the product path.  The oracle restates
+Radiant.jl's ``compute_flux(::SN)`` (src/particle_transport/sn_flux.jl:22-234) in FP64 C++.
+
+A problem is any object with the attributes of ``radiant_hip.problems.Problem`` (Radiant
+layouts: Σtot[G,Nmat], Σs[Nmat,G,G,L+1], mat[Nx,Ny,Nz] 1-based, column-major).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "_build", "libradiant_oracle.so")
+_lib = None
+
+
+class OracleProblem(C.Structure):
+    _fields_ = [
+        ("ndims", C.c_int32), ("Nx", C.c_int32), ("Ny", C.c_int32), ("Nz", C.c_int32),
+        ("dx", C.POINTER(C.c_double)), ("dy", C.POINTER(C.c_double)), ("dz", C.POINTER(C.c_double)),
+        ("mat", C.POINTER(C.c_int64)), ("Nmat", C.c_int32), ("G", C.c_int32),
+        ("sigt", C.POINTER(C.c_double)), ("sigs", C.POINTER(C.c_double)), ("Lxs", C.c_int32),
+        ("qvol", C.POINTER(C.c_double)), ("bc", C.POINTER(C.c_int32)),
+        ("quad_type", C.c_char_p), ("quad_order", C.c_int32), ("qdims", C.c_int32),
+        ("legendre_order", C.c_int32), ("sn_type", C.c_char_p), ("accel", C.c_char_p),
+        ("restart", C.c_int32), ("depth", C.c_int32), ("maxit", C.c_int32), ("tol", C.c_double),
+        ("nthreads", C.c_int32), ("verbose", C.c_int32),
+    ]
+
+
+class OracleStats(C.Structure):
+    _fields_ = [("niter", C.c_int64), ("resid", C.c_double), ("rho", C.c_double),
+                ("converged", C.c_int32), ("passes", C.c_int32)]
+
+
+def build():
+    """Compile the oracle library (g++); called by ``__graft_entry__.build()``."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_SO):
+            build()
+        L = C.CDLL(_SO)
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int64)
+        L.oracle_last_error.restype = C.c_char_p
+        L.oracle_quadrature.argtypes = [C.c_int, C.c_char_p, C.c_int, C.c_int, dp, dp, dp, dp, C.c_int]
+        L.oracle_basis.argtypes = [dp, dp, dp, dp, C.c_int, C.c_int, C.c_char_p, C.c_int, dp, dp, ip, ip, C.c_int]
+        L.oracle_surface_basis.argtypes = [dp, dp, dp, dp, C.c_int, C.c_int, C.c_char_p, C.c_int, C.c_int,
+                                           dp, dp, ip, C.POINTER(C.c_int), C.c_int]
+        L.oracle_real_sh.argtypes = [C.c_int, C.c_double, C.c_double, C.c_int, dp]
+        L.oracle_legendre.argtypes = [C.c_int, C.c_double, C.c_int, dp]
+        L.oracle_problem_sizes.argtypes = [C.POINTER(OracleProblem), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                           C.POINTER(C.c_int), C.POINTER(C.c_int64)]
+        L.oracle_solve.argtypes = [C.POINTER(OracleProblem), dp, C.POINTER(OracleStats)]
+        L.oracle_pass.argtypes = [C.POINTER(OracleProblem), C.c_int, C.c_int, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _check(rc):
+    if rc < 0:
+        raise RuntimeError("oracle: " + lib().oracle_last_error().decode())
+    return rc
+
+
+def quadrature(N, qtype, ndims=1, qdims=None):
+    if qdims is None:
+        qdims = ndims
+    cap = 4 * (N + 2) * (N + 2) + 8
+    mu, eta, xi, w = (np.zeros(cap) for _ in range(4))
+    nd = _check(lib().oracle_quadrature(N, qtype.encode(), ndims, qdims, _p(mu), _p(eta), _p(xi), _p(w), cap))
+    return mu[:nd].copy(), eta[:nd].copy(), xi[:nd].copy(), w[:nd].copy()
+
+
+def basis(mu, eta, xi, w, L, sn_type, qdims):
+    nd = len(w)
+    cap = max(nd, (L + 1) ** 2) + 1
+    M = np.zeros(nd * cap)
+    D = np.zeros(nd * cap)
+    pl = np.zeros(cap, np.int64)
+    pm = np.zeros(cap, np.int64)
+    arr = [np.ascontiguousarray(a, dtype=np.float64) for a in (mu, eta, xi, w)]
+    np_ = _check(lib().oracle_basis(*[_p(a) for a in arr], nd, L, sn_type.encode(), qdims, _p(M), _p(D),
+                                    pl.ctypes.data_as(C.POINTER(C.c_int64)),
+                                    pm.ctypes.data_as(C.POINTER(C.c_int64)), cap))
+    Mm = M[: nd * np_].reshape((np_, nd)).T.copy()   # Nd x Np
+    Dm = D[: nd * np_].reshape((nd, np_)).T.copy()   # Np x Nd
+    return np_, Mm, Dm, pl[:np_].copy(), pm[:np_].copy()
+
+
+def surface_basis(mu, eta, xi, w, L, sn_type, qdims, face):
+    nd = len(w)
+    cap = nd * nd + 1
+    M = np.zeros(cap)
+    D = np.zeros(cap)
+    n2p = np.zeros(nd, np.int64)
+    ndp = C.c_int(0)
+    arr = [np.ascontiguousarray(a, dtype=np.float64) for a in (mu, eta, xi, w)]
+    nps = _check(lib().oracle_surface_basis(*[_p(a) for a in arr], nd, L, sn_type.encode(), qdims, face,
+                                            _p(M), _p(D), n2p.ctypes.data_as(C.POINTER(C.c_int64)),
+                                            C.byref(ndp), cap))
+    k = ndp.value
+    Mm = M[: k * nps].reshape((nps, k)).T.copy()     # Nd+ x Nps
+    Dm = D[: k * nps].reshape((k, nps)).T.copy()     # Nps x Nd+
+    return nps, Mm, Dm, n2p
+
+
+def real_sh(L, mu, phi, half_range=False):
+    R = np.zeros((L + 1) ** 2)
+    _check(lib().oracle_real_sh(L, mu, phi, int(half_range), _p(R)))
+    return R
+
+
+def legendre(L, x, half_range=False):
+    P = np.zeros(L + 1)
+    _check(lib().oracle_legendre(L, x, int(half_range), _p(P)))
+    return P
+
+
+class _Keep:
+    """Holds the numpy arrays referenced by an OracleProblem alive."""
+
+
+def make_problem(prob, nthreads=1, verbose=0, **over):
+    """Build an OracleProblem from a radiant_hip.problems.Problem-like object."""
+    k = _Keep()
+    k.dx = np.ascontiguousarray(prob.dx, np.float64)
+    k.dy = np.ascontiguousarray(prob.dy if prob.ndims >= 2 else [1.0], np.float64)
+    k.dz = np.ascontiguousarray(prob.dz if prob.ndims >= 3 else [1.0], np.float64)
+    k.mat = np.ascontiguousarray(np.asarray(prob.mat).reshape(-1, order="F"), np.int64)
+    k.sigt = np.ascontiguousarray(np.asarray(prob.sigt).reshape(-1, order="F"), np.float64)
+    k.sigs = np.ascontiguousarray(np.asarray(prob.sigs).reshape(-1, order="F"), np.float64)
+    k.qvol = np.ascontiguousarray(np.asarray(prob.qvol).reshape(-1, order="F"), np.float64)
+    k.bc = np.ascontiguousarray(prob.bc, np.int32)
+    k.qt = prob.quad_type.encode()
+    k.st = prob.sn_type.encode()
+    acc = over.get("accel", prob.accel)
+    k.ac = acc.encode()
+    op = OracleProblem(
+        prob.ndims, prob.Nx, prob.Ny, prob.Nz, _p(k.dx), _p(k.dy), _p(k.dz),
+        k.mat.ctypes.data_as(C.POINTER(C.c_int64)), prob.Nmat, prob.G, _p(k.sigt), _p(k.sigs), prob.Lxs,
+        _p(k.qvol), k.bc.ctypes.data_as(C.POINTER(C.c_int32)), k.qt, prob.quad_order, prob.qdims,
+        prob.legendre_order, k.st, k.ac, over.get("restart", prob.restart), over.get("depth", prob.depth),
+        over.get("maxit", prob.maxit), over.get("tol", prob.tol), nthreads, verbose)
+    op._keep = k
+    return op
+
+
+def sizes(prob):
+    op = make_problem(prob)
+    nd, np_, nps = C.c_int(), C.c_int(), C.c_int()
+    S = C.c_int64()
+    _check(lib().oracle_problem_sizes(C.byref(op), C.byref(nd), C.byref(np_), C.byref(nps), C.byref(S)))
+    return nd.value, np_.value, nps.value, S.value
+
+
+def solve(prob, nthreads=1, verbose=0, **over):
+    """Full solve; returns (phi[G,Np,Nc] in Radiant order, stats list of dicts)."""
+    op = make_problem(prob, nthreads=nthreads, verbose=verbose, **over)
+    nd, np_, nps, S = sizes(prob)
+    nc = prob.Nx * prob.Ny * prob.Nz
+    phi = np.zeros(prob.G * np_ * nc)
+    st = (OracleStats * prob.G)()
+    _check(lib().oracle_solve(C.byref(op), _p(phi), st))
+    stats = [dict(niter=s.niter, resid=s.resid, rho=s.rho, converged=bool(s.converged), passes=s.passes) for s in st]
+    return phi.reshape((prob.G, np_, nc), order="F"), stats
+
+
+def transport_pass(prob, g, homogeneous, z, qlout):
+    """One pass T(z) / A z of group g (0-based) on state z (length S); returns the new state."""
+    op = make_problem(prob)
+    zz = np.ascontiguousarray(z, np.float64).copy()
+    q = np.ascontiguousarray(qlout, np.float64)
+    _check(lib().oracle_pass(C.byref(op), g, int(homogeneous), _p(zz), _p(q)))
+    return zz

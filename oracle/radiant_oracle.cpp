@@ -1,0 +1,1415 @@
+// radiant_oracle.cpp -- CPU oracle for the discrete-ordinates (SN) transport solve.
+//
+// TEST INFRASTRUCTURE ONLY.  This file restates, in plain FP64 C++, the algorithm of
+// Radiant.jl's `compute_flux(::SN)` path so that the HIP backend can be checked against it.
+// Only the tests, `__graft_entry__.smoke()` and the `cpu_baseline` leg of `bench.py` may
+// load the library built from it; the product path never does.
+//
+// Every function cites the reference file:line it follows (paths relative to Radiant.jl's
+// repository root).  Loop orders and the order of floating-point operations follow the
+// reference where they influence the result; the compiler is asked not to contract
+// multiply-adds (-ffp-contract=off) so that each product is rounded as in Julia.
+//
+// Scope: Cartesian 1D/2D/3D, BTE (no CSD), diamond-difference order 1 on every axis
+// (`scheme_weights.jl:47-51` with order 1 gives w = [-1, 2]), "standard", "galerkin-d" and
+// "galerkin-m" angular operators, void / reflective / periodic boundaries, volume sources,
+// accelerations none / livolant / anderson / gmres / bicgstab.
+//
+// Parity status of each part is recorded in the test-suite: quadratures, harmonics, M/D,
+// the DD cell solve, the sweep, the Krylov solvers and the group loop are each pinned by a
+// closed form, an invariant, a library routine or a brute-force dense solve.
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include <limits>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+using std::vector;
+using std::string;
+
+namespace {
+
+thread_local string g_err;
+
+[[noreturn]] void fail(const string& m) { throw std::runtime_error(m); }
+
+const double PI = 3.141592653589793;  // Float64(pi)
+const double EPS = std::numeric_limits<double>::epsilon();
+
+// ---------------------------------------------------------------------------------------
+// Quadratures
+// ---------------------------------------------------------------------------------------
+
+// tools/gauss_legendre.jl:18-74 -- Tricomi initial guess, Newton to eps on Bonnet's recursion.
+void gauss_legendre(int N, vector<double>& x, vector<double>& w) {
+    x.assign(N, 0.0);
+    w.assign(N, 0.0);
+    for (int i = 1; i <= N / 2; ++i) {
+        double phi = ((PI / 2) * (double)(4 * i - 1)) / (double)(2 * N + 1);
+        double s = std::sin(phi);
+        double t = (1.0 - (double)(N - 1) / (double)(8 * N * N * N)) -
+                   (1.0 / (double)(384LL * N * N * N * N)) * (39.0 - 28.0 / (s * s));
+        x[i - 1] = t * std::cos(phi);
+        x[N - i] = -x[i - 1];
+    }
+    double dPdx = 0.0;
+    for (int i = 0; i < N; ++i) {
+        double xold = 2.0;
+        while (std::fabs(x[i] - xold) > EPS) {
+            xold = x[i];
+            double P1 = 1.0, P2 = x[i];
+            for (int n = 2; n <= N; ++n) {
+                double Pn = ((double)(2 * n - 1) * x[i] * P2 - (double)(n - 1) * P1) / (double)n;
+                P1 = P2;
+                P2 = Pn;
+            }
+            dPdx = (double)N * (P1 - x[i] * P2) / (1.0 - x[i] * x[i]);
+            x[i] -= P2 / dPdx;
+        }
+        w[i] = 2.0 / ((1.0 - x[i] * x[i]) * (dPdx * dPdx));
+    }
+}
+
+// tools/gauss_legendre_chebychev.jl:19-39 -- product of GL (sorted ascending) in mu and a
+// 2N-point Chebyshev rule in phi; polar axis is x.
+void gauss_legendre_chebychev(int N, vector<double>& mu, vector<double>& eta, vector<double>& xi,
+                              vector<double>& w) {
+    vector<double> mun, wn;
+    gauss_legendre(N, mun, wn);
+    vector<int> idx(N);
+    for (int i = 0; i < N; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return mun[a] < mun[b]; });
+    vector<double> ms(N), ws(N);
+    for (int i = 0; i < N; ++i) { ms[i] = mun[idx[i]]; ws[i] = wn[idx[i]]; }
+    int Nd = 2 * N * N;
+    mu.assign(Nd, 0); eta.assign(Nd, 0); xi.assign(Nd, 0); w.assign(Nd, 0);
+    int i = 0;
+    for (int n = 0; n < N; ++n)
+        for (int m = 1; m <= 2 * N; ++m) {
+            double phi = PI / (double)N * ((double)m - 0.5);
+            mu[i] = ms[n];
+            double st = std::sqrt(1.0 - ms[n] * ms[n]);
+            eta[i] = st * std::cos(phi);
+            xi[i] = st * std::sin(phi);
+            w[i] = ws[n] * PI / (double)N;
+            ++i;
+        }
+}
+
+// tools/newton_bisection.jl:20-75
+template <class F, class DF>
+double newton_bisection(F f, DF dfdx, double x1, double x2, double eps0 = EPS, int Nmax = 100) {
+    double f1 = f(x1), f2 = f(x2);
+    if (f1 * f2 > 0.0) fail("newton_bisection: root not bracketed");
+    if (f1 == 0.0) return x1;
+    if (f2 == 0.0) return x2;
+    double xl, xh;
+    if (f1 < 0) { xl = x1; xh = x2; } else { xl = x2; xh = x1; }
+    double rts = 0.5 * (x1 + x2);
+    double dxold = std::fabs(x2 - x1);
+    double dx = dxold;
+    double frts = f(rts), dfrts = dfdx(rts);
+    for (int j = 1; j <= Nmax; ++j) {
+        if (((rts - xh) * dfrts - frts) * ((rts - xl) * dfrts - frts) > 0.0 ||
+            std::fabs(2.0 * frts) > std::fabs(dxold * dfrts)) {
+            dxold = dx;
+            dx = 0.5 * (xh - xl);
+            rts = xl + dx;
+        } else {
+            dxold = dx;
+            dx = frts / dfrts;
+            rts -= dx;
+        }
+        if (std::fabs(dx) < eps0) return rts;
+        frts = f(rts);
+        dfrts = dfdx(rts);
+        if (frts < 0.0) xl = rts; else xh = rts;
+    }
+    fail("newton_bisection: maximum number of iterations exceeded");
+}
+
+// tools/carlson.jl:20-94 -- Carlson's equal-weight set, N(N+2) directions, octant order
+// (+++, ++-, +-+, +--, -++, -+-, --+, ---).
+void carlson(int N, vector<double>& mu, vector<double>& eta, vector<double>& xi, vector<double>& w) {
+    if (N < 2 || N % 2 != 0) fail("Carlson quadrature available for even N >= 2.");
+    int Nd = N * (N + 2), H = N / 2;
+    mu.assign(Nd, 0); eta.assign(Nd, 0); xi.assign(Nd, 0); w.assign(Nd, 0);
+    vector<double> wn(H), mum(H), mu0(H);
+    for (int n = 1; n <= H; ++n) {
+        wn[n - 1] = 4.0 * (double)(N - 2 * n + 2) / (double)(N * (N + 2));
+        mum[n - 1] = 1.0 - (double)((N - 2 * n + 2) * (N - 2 * n + 4)) / (double)(N * (N + 2));
+        mu0[n - 1] = 1.0 - (double)((N - 2 * n + 2) * (N - 2 * n + 2)) / (double)(N * (N + 2));
+    }
+    vector<double> mun(H);
+    if (N == 2) {
+        mun[0] = 1.0 / std::sqrt(3.0);
+    } else {
+        auto f = [&](double x) {
+            double s = 0.0;
+            for (int n = 0; n < H; ++n) { double t = mu0[n] + x * mum[n]; s += wn[n] * (t * t); }
+            return 1.0 / 3.0 - s;
+        };
+        auto df = [&](double x) {
+            double s = 0.0;
+            for (int n = 0; n < H; ++n) s += 2.0 * wn[n] * mum[n] * (mu0[n] + x * mum[n]);
+            return -s;
+        };
+        double x = newton_bisection(f, df, 1.0, -1.0);
+        for (int n = 0; n < H; ++n) mun[n] = mu0[n] + x * mum[n];
+    }
+    double w0 = 1.0 / (double)(N * (N + 2));
+    double A;
+    if (N == 2) {
+        A = 1.0;
+    } else {
+        auto g = [&](double a) {
+            double s = 0.0;
+            for (int n = 1; n <= H; ++n) {
+                double m = mun[n - 1];
+                s += (PI / 2) * wn[n - 1] * m -
+                     (4 * PI) * w0 / std::sqrt(2.0) * std::sin(PI * a / 4) * std::sqrt(1.0 - m * m) /
+                         std::sin(PI * a / (double)(2 * N - 4 * n + 4));
+            }
+            return s;
+        };
+        auto dg = [&](double a) {
+            double s = 0.0;
+            for (int n = 1; n <= H; ++n) {
+                double m = mun[n - 1];
+                double sq = std::sqrt(1.0 - m * m);
+                double d = (double)(2 * N - 4 * n + 4);
+                double sd = std::sin(PI * a / d);
+                s += -4 * PI * w0 * std::sqrt(2.0) * PI / 8 *
+                     (std::cos(PI * a / 4) * sq / sd -
+                      2 * std::sin(PI * a / 4) * sq * std::cos(PI * a / d) / (sd * sd) / (double)(N - 2 * n + 2));
+            }
+            return s;
+        };
+        A = newton_bisection(g, dg, 0.5, 2.0);
+    }
+    static const int sgn[8][3] = {{1, 1, 1}, {1, 1, -1}, {1, -1, 1}, {1, -1, -1},
+                                  {-1, 1, 1}, {-1, 1, -1}, {-1, -1, 1}, {-1, -1, -1}};
+    int i = 0;
+    for (int t = 0; t < 8; ++t)
+        for (int n = 1; n <= H; ++n)
+            for (int m = 1; m <= H - n + 1; ++m) {
+                double phi = PI / 2 * ((double)(2 * m - 1) / (double)(N - 2 * n + 2) * A + (1.0 - A) / 2);
+                mu[i] = sgn[t][0] * mun[n - 1];
+                double st = std::sqrt(1.0 - mu[i] * mu[i]);
+                eta[i] = sgn[t][1] * st * std::cos(phi);
+                xi[i] = sgn[t][2] * st * std::sin(phi);
+                w[i] = (4 * PI) * w0;
+                ++i;
+            }
+}
+
+// tools/quadrature.jl:19-59 (gauss-legendre, gauss-legendre-chebychev, carlson; the Qdims=2
+// fold onto the upper hemisphere is at :36-54).
+void quadrature(int N, const string& type, int Ndims, int Qdims, vector<double>& mu,
+                vector<double>& eta, vector<double>& xi, vector<double>& w) {
+    if (type == "gauss-legendre" && Ndims == 1) {
+        gauss_legendre(N, mu, w);
+        eta.assign(mu.size(), 0.0);   // sn_flux.jl:45 expands a 1D set to [mu, 0, 0]
+        xi.assign(mu.size(), 0.0);
+    } else if (type == "gauss-legendre-chebychev") {
+        gauss_legendre_chebychev(N, mu, eta, xi, w);
+    } else if (type == "carlson") {
+        carlson(N, mu, eta, xi, w);
+    } else {
+        fail("Unknown " + type + " quadrature in " + std::to_string(Ndims) + "D geometry.");
+    }
+    if (Qdims == 2) {
+        int Nd = (int)w.size();
+        vector<double> w0 = w;
+        for (int i = 0; i < Nd; ++i) {
+            if (xi[i] >= 0) continue;
+            int best = 0;
+            double bv = 0;
+            for (int j = 0; j < Nd; ++j) {
+                double a = std::fabs(mu[i] - mu[j]), b = std::fabs(eta[i] - eta[j]), c = std::fabs(xi[i] + xi[j]);
+                double e = a * a + b * b + c * c;
+                if (j == 0 || e < bv) { bv = e; best = j; }
+            }
+            w0[best] += w0[i];
+        }
+        vector<double> m2, e2, x2, ww;
+        for (int i = 0; i < Nd; ++i)
+            if (xi[i] >= 0) { m2.push_back(mu[i]); e2.push_back(eta[i]); x2.push_back(xi[i]); ww.push_back(w0[i]); }
+        mu = m2; eta = e2; xi = x2; w = ww;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Polynomials and harmonics
+// ---------------------------------------------------------------------------------------
+
+// tools/factorial_factor.jl:17-33 -- exp(sum log) form of factorial ratios.
+double factorial_factor(const vector<int>& num, const vector<int>& den,
+                        const vector<std::pair<double, double>>& ex = {}) {
+    double f = 0.0;
+    for (int n : num) for (int i = 1; i <= n; ++i) f += std::log((double)i);
+    for (int n : den) for (int i = 1; i <= n; ++i) f -= std::log((double)i);
+    for (auto& e : ex) {
+        double x = e.first, m = e.second;
+        if (m == 0.0) continue;
+        if (x == 0.0) return m > 0 ? 0.0 : INFINITY;
+        f += m * std::log(x);
+    }
+    return std::exp(f);
+}
+
+// tools/legendre_polynomials.jl:52-75 -- Bonnet recursion.
+vector<double> legendre_up_to_L(int L, double x) {
+    if (L < 0) fail("Legendre order is greater or equal to zero.");
+    vector<double> P(L + 1, 0.0);
+    if (x == 1.0) {
+        for (int l = 0; l <= L; ++l) P[l] = 1.0;
+    } else if (x == -1.0) {
+        for (int l = 0; l <= L; ++l) P[l] = (l % 2 == 0) ? 1.0 : -1.0;
+    } else {
+        P[0] = 1.0;
+        if (L >= 1) P[1] = x;
+        for (int l = 2; l <= L; ++l)
+            P[l] = ((double)(2 * l - 1) * x * P[l - 1] - (double)(l - 1) * P[l - 2]) / (double)l;
+    }
+    return P;
+}
+
+// tools/legendre_polynomials.jl:91-93
+vector<double> half_range_legendre_up_to_L(int L, double x) {
+    vector<double> P = legendre_up_to_L(L, 2 * x - 1);
+    for (int l = 0; l <= L; ++l) P[l] = std::sqrt((double)(2 * l + 1)) * P[l];
+    return P;
+}
+
+// tools/jacobi_polynomials.jl:16-49 -- three-term recurrence (Darboux asymptotics at l>=125).
+vector<double> jacobi_up_to_L(int L, int a, int b, double x) {
+    vector<double> P(L + 1, 0.0);
+    if (x == 1.0) {
+        for (int l = 0; l <= L; ++l) P[l] = factorial_factor({l + a}, {l, a});
+    } else if (x == -1.0) {
+        for (int l = 0; l <= L; ++l) P[l] = ((l % 2 == 0) ? 1.0 : -1.0) * factorial_factor({l + b}, {l, b});
+    } else {
+        P[0] = 1.0;
+        if (L >= 1) P[1] = (double)(a + 1) + 0.5 * (double)(a + b + 2) * (x - 1.0);
+        for (int l = 2; l <= L; ++l) {
+            if (l < 125) {
+                long s = 2 * l + a + b;
+                double inner = (double)(s * (s - 2)) * x + (double)(a * a) - (double)(b * b);
+                double t1 = (double)(s - 1) * inner * P[l - 1];
+                double t2 = (double)(2L * (l + a - 1) * (l + b - 1) * s) * P[l - 2];
+                P[l] = (t1 - t2) / (double)(2L * l * (l + a + b) * (s - 2));
+            } else {
+                double th = std::acos(x);
+                double kth = 1.0 / (std::sqrt(PI) * std::pow(std::sin(0.5 * th), a + 0.5) *
+                                    std::pow(std::cos(0.5 * th), b + 0.5));
+                double Nn = l + 0.5 * (a + b + 1);
+                double gam = -0.5 * PI * (a + 0.5);
+                P[l] = 1.0 / std::sqrt((double)l) * kth * std::cos(Nn * th + gam);
+            }
+        }
+    }
+    return P;
+}
+
+typedef vector<vector<double>> Harm;  // H[l][l+m], m = -l..l
+
+// tools/real_spherical_harmonics.jl:52-88 -- R_lm = C_lm P_l^|m|(mu) T_m(phi), no
+// Condon-Shortley phase, T_m = cos(m phi) for m>=0 and sin(|m| phi) for m<0.
+Harm real_sh_up_to_L(int L, double mu, double phi) {
+    if (L < 0) fail("Legendre order is greater or equal to zero.");
+    vector<vector<double>> Plm(L + 1);
+    for (int l = 0; l <= L; ++l) Plm[l].assign(l + 1, 0.0);
+    if (mu == -1.0 || mu == 1.0) {
+        vector<double> P = jacobi_up_to_L(L, 0, 0, mu);
+        for (int l = 0; l <= L; ++l) Plm[l][0] = P[l];
+    } else {
+        for (int m = 0; m <= L; ++m) {
+            vector<double> P = jacobi_up_to_L(L - m, m, m, mu);
+            for (int l = m; l <= L; ++l)
+                Plm[l][m] = factorial_factor({l + m}, {l}, {{2.0, (double)(-m)}, {1.0 - mu * mu, m / 2.0}}) * P[l - m];
+        }
+    }
+    Harm R(L + 1);
+    for (int l = 0; l <= L; ++l) {
+        R[l].assign(2 * l + 1, 0.0);
+        for (int m = -l; m <= l; ++m) {
+            int am = std::abs(m);
+            double T = (m >= 0) ? std::cos((double)m * phi) : std::sin((double)am * phi);
+            double C = std::sqrt((double)(2 - (m == 0)) * factorial_factor({l - am}, {l + am}));
+            R[l][l + m] = C * Plm[l][am] * T;
+        }
+    }
+    return R;
+}
+
+// tools/real_spherical_harmonics.jl:104-139 -- half-range harmonics on mu in [0,1].
+Harm real_half_range_sh_up_to_L(int L, double mu, double phi) {
+    if (L < 0) fail("Legendre order is greater or equal to zero.");
+    if (!(0 <= mu && mu <= 1)) fail("Invalid direction cosine (should be between 0 and 1).");
+    vector<vector<double>> Plm(L + 1);
+    for (int l = 0; l <= L; ++l) Plm[l].assign(l + 1, 0.0);
+    if (mu == -1.0 || mu == 1.0) {
+        vector<double> P = jacobi_up_to_L(L, 0, 0, 2 * mu - 1);
+        for (int l = 0; l <= L; ++l) Plm[l][0] = P[l];
+    } else {
+        double x = 2 * mu - 1;
+        for (int m = 0; m <= L; ++m) {
+            vector<double> P = jacobi_up_to_L(L - m, m, m, x);
+            for (int l = m; l <= L; ++l)
+                Plm[l][m] = factorial_factor({l + m}, {l}, {{2.0, (double)(-m)}, {1.0 - x * x, m / 2.0}}) * P[l - m];
+        }
+    }
+    Harm R(L + 1);
+    for (int l = 0; l <= L; ++l) {
+        R[l].assign(2 * l + 1, 0.0);
+        for (int m = -l; m <= l; ++m) {
+            int am = std::abs(m);
+            double T = (m >= 0) ? std::cos((double)m * phi) : std::sin((double)am * phi);
+            double C = std::sqrt((double)(2 - (m == 0)) / (2 * PI) * (double)(2 * l + 1) *
+                                 factorial_factor({l - am}, {l + am}));
+            R[l][l + m] = C * Plm[l][am] * T;
+        }
+    }
+    return R;
+}
+
+// ---------------------------------------------------------------------------------------
+// Dense inverse (LU with partial pivoting), the role of LinearAlgebra.inv in the reference.
+// Column-major n x n.
+// ---------------------------------------------------------------------------------------
+vector<double> invert(const vector<double>& A, int n) {
+    vector<double> LU = A;
+    vector<int> piv(n);
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        double mx = std::fabs(LU[k + n * k]);
+        for (int i = k + 1; i < n; ++i)
+            if (std::fabs(LU[i + n * k]) > mx) { mx = std::fabs(LU[i + n * k]); p = i; }
+        piv[k] = p;
+        if (LU[p + n * k] == 0.0) fail("inv: singular matrix");
+        if (p != k)
+            for (int j = 0; j < n; ++j) std::swap(LU[k + n * j], LU[p + n * j]);
+        for (int i = k + 1; i < n; ++i) {
+            LU[i + n * k] /= LU[k + n * k];
+            double l = LU[i + n * k];
+            for (int j = k + 1; j < n; ++j) LU[i + n * j] -= l * LU[k + n * j];
+        }
+    }
+    vector<double> X(n * n, 0.0);
+    for (int c = 0; c < n; ++c) {
+        vector<double> b(n, 0.0);
+        b[c] = 1.0;
+        for (int k = 0; k < n; ++k) std::swap(b[k], b[piv[k]]);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < i; ++j) b[i] -= LU[i + n * j] * b[j];
+        for (int i = n - 1; i >= 0; --i) {
+            for (int j = i + 1; j < n; ++j) b[i] -= LU[i + n * j] * b[j];
+            b[i] /= LU[i + n * i];
+        }
+        for (int i = 0; i < n; ++i) X[i + n * c] = b[i];
+    }
+    return X;
+}
+
+// ---------------------------------------------------------------------------------------
+// Angular operators
+// ---------------------------------------------------------------------------------------
+
+// particle_transport/angular_polynomial_basis.jl:9-17
+int galerkin_minimum_order(int Nd, int Qdims) {
+    if (Qdims == 1) return Nd - 1;
+    int n = 0;
+    while (true) {
+        int count = (Qdims == 2) ? (n + 1) * (n + 2) / 2 : (n + 1) * (n + 1);
+        if (count >= Nd) return n;
+        ++n;
+    }
+}
+
+struct Basis {
+    int Np = 0;
+    vector<double> M;  // Nd x Np column-major (M[n + Nd*p])
+    vector<double> D;  // Np x Nd column-major (D[p + Np*n])
+    vector<long> pl, pm;
+};
+
+// particle_transport/angular_polynomial_basis.jl:465-548 -- classical Gram-Schmidt selection of
+// independent harmonics (threshold 1e-5 on the residual norm), then M = inv(D) (g_type 1) or
+// D = inv(M) (g_type 2, 3).  Returns false when fewer than Nd harmonics were found.
+bool gram_schmidt(int Nd, int L, const vector<Harm>& R, const vector<double>& w, int Qdims, int gtype,
+                  Basis& B) {
+    B.Np = Nd;
+    B.pl.assign(Nd, 0);
+    B.pm.assign(Nd, 0);
+    B.M.assign((size_t)Nd * Nd, 0.0);
+    B.D.assign((size_t)Nd * Nd, 0.0);
+    vector<double> u((size_t)Nd * Nd, 0.0);  // u[n + Nd*k]
+    auto nrm = [&](const vector<double>& a) {
+        double s = 0;
+        for (double v : a) s += v * v;
+        return std::sqrt(s);
+    };
+    {
+        vector<double> v(Nd);
+        for (int n = 0; n < Nd; ++n) v[n] = R[n][0][0];
+        double nv = nrm(v);
+        for (int n = 0; n < Nd; ++n) {
+            if (gtype == 1) B.D[0 + (size_t)Nd * n] = w[n] * R[n][0][0];
+            else if (gtype == 2) B.M[n + 0] = 1 / (4 * PI) * R[n][0][0];
+            else B.M[n + 0] = R[n][0][0];
+            u[n] = R[n][0][0] / nv;
+        }
+    }
+    int i = 1;  // number of kept harmonics
+    for (int l = 1; l <= L && i < Nd; ++l) {
+        int mmin = (Qdims == 2) ? 0 : -l;
+        for (int m = mmin; m <= l; ++m) {
+            vector<double> v(Nd), S(Nd, 0.0), vk(Nd);
+            for (int n2 = 0; n2 < Nd; ++n2) v[n2] = R[n2][l][m + l];
+            for (int n = 0; n < i; ++n) {
+                double ip = 0;
+                for (int k = 0; k < Nd; ++k) ip += u[k + (size_t)Nd * n] * v[k];
+                for (int n2 = 0; n2 < Nd; ++n2) S[n2] += ip * u[n2 + (size_t)Nd * n];
+            }
+            for (int n = 0; n < Nd; ++n) vk[n] = R[n][l][m + l] - S[n];
+            double nk = nrm(vk);
+            if (nk > 1e-5) {
+                for (int n = 0; n < Nd; ++n) {
+                    u[n + (size_t)Nd * i] = vk[n] / nk;
+                    if (gtype == 1) B.D[i + (size_t)Nd * n] = w[n] * R[n][l][m + l];
+                    else if (gtype == 2) B.M[n + (size_t)Nd * i] = (double)(2 * l + 1) / (4 * PI) * R[n][l][m + l];
+                    else B.M[n + (size_t)Nd * i] = R[n][l][m + l];
+                }
+                B.pl[i] = l;
+                B.pm[i] = m;
+                ++i;
+            }
+            if (i == Nd) break;
+        }
+    }
+    if (i < Nd) return false;
+    if (gtype == 1) B.M = invert(B.D, Nd);
+    else B.D = invert(B.M, Nd);
+    return true;
+}
+
+// particle_transport/angular_polynomial_basis.jl:51-179 (volume basis).
+Basis angular_polynomial_basis(const vector<double>& mu, const vector<double>& eta, const vector<double>& xi,
+                               const vector<double>& w, int L, const string& type, int Qdims) {
+    int Nd = (int)w.size();
+    bool gal = (type == "galerkin-m" || type == "galerkin-d");
+    int Lb = gal ? std::max(L, galerkin_minimum_order(Nd, Qdims)) : L;
+    Basis B;
+    if (Qdims == 1) {
+        vector<vector<double>> Pl(Nd);
+        for (int n = 0; n < Nd; ++n) Pl[n] = legendre_up_to_L(Lb, mu[n]);
+        if (type == "standard") {
+            B.Np = L + 1;
+            B.pl.assign(B.Np, 0); B.pm.assign(B.Np, 0);
+            B.M.assign((size_t)Nd * B.Np, 0); B.D.assign((size_t)B.Np * Nd, 0);
+            for (int l = 0; l <= L; ++l) {
+                B.pl[l] = l;
+                for (int n = 0; n < Nd; ++n) {
+                    B.M[n + (size_t)Nd * l] = (double)(2 * l + 1) / 2 * Pl[n][l];
+                    B.D[l + (size_t)B.Np * n] = w[n] * Pl[n][l];
+                }
+            }
+        } else if (type == "galerkin-m") {
+            B.Np = Nd;
+            B.pl.assign(Nd, 0); B.pm.assign(Nd, 0);
+            B.M.assign((size_t)Nd * Nd, 0);
+            for (int l = 0; l < Nd; ++l) {
+                B.pl[l] = l;
+                for (int n = 0; n < Nd; ++n) B.M[n + (size_t)Nd * l] = (double)(2 * l + 1) / 2 * Pl[n][l];
+            }
+            B.D = invert(B.M, Nd);
+        } else if (type == "galerkin-d") {
+            B.Np = Nd;
+            B.pl.assign(Nd, 0); B.pm.assign(Nd, 0);
+            B.D.assign((size_t)Nd * Nd, 0);
+            for (int l = 0; l < Nd; ++l) {
+                B.pl[l] = l;
+                for (int n = 0; n < Nd; ++n) B.D[l + (size_t)Nd * n] = w[n] * Pl[n][l];
+            }
+            B.M = invert(B.D, Nd);
+        } else {
+            fail("Unknown method.");
+        }
+    } else if (Qdims <= 3) {
+        vector<double> phi(Nd);
+        for (int n = 0; n < Nd; ++n) phi[n] = std::atan2(xi[n], eta[n]);
+        vector<Harm> R(Nd);
+        for (int n = 0; n < Nd; ++n) R[n] = real_sh_up_to_L(Lb, mu[n], phi[n]);
+        if (type == "standard") {
+            B.Np = (Qdims == 2) ? (L + 1) * (L + 2) / 2 : (L + 1) * (L + 1);
+            B.pl.assign(B.Np, 0); B.pm.assign(B.Np, 0);
+            B.M.assign((size_t)Nd * B.Np, 0); B.D.assign((size_t)B.Np * Nd, 0);
+            int li = 0;
+            for (int l = 0; l <= L; ++l)
+                for (int m = -l; m <= l; ++m) {
+                    if (Qdims == 2 && ((l + m) % 2 + 2) % 2 == 1) continue;
+                    B.pl[li] = l; B.pm[li] = m;
+                    for (int n = 0; n < Nd; ++n) {
+                        B.M[n + (size_t)Nd * li] = (double)(2 * l + 1) / (4 * PI) * R[n][l][m + l];
+                        B.D[li + (size_t)B.Np * n] = w[n] * R[n][l][m + l];
+                    }
+                    ++li;
+                }
+        } else if (gal) {
+            int gt = (type == "galerkin-d") ? 1 : 2;
+            int Lg = Lb;
+            bool ok = gram_schmidt(Nd, Lg, R, w, Qdims, gt, B);
+            while (!ok) {
+                ++Lg;
+                if (Lg > Lb + Nd + 10) fail("Galerkin: unable to find enough independent spherical harmonics");
+                for (int n = 0; n < Nd; ++n) R[n] = real_sh_up_to_L(Lg, mu[n], phi[n]);
+                ok = gram_schmidt(Nd, Lg, R, w, Qdims, gt, B);
+            }
+        } else {
+            fail("Unknown method.");
+        }
+    } else {
+        fail("Domain dimension should be either 1, 2 or 3.");
+    }
+    return B;
+}
+
+struct SurfBasis {
+    int Np = 0;
+    vector<double> M, D;      // M: Nd+ x Np, D: Np x Nd+
+    vector<long> n_to_np;     // [Nd] 1-based half-range index or 0
+    int Ndp = 0;
+};
+
+// particle_transport/angular_polynomial_basis.jl:262-436 (half-range basis of one face).
+SurfBasis surface_basis(const vector<double>& mu, const vector<double>& eta, const vector<double>& xi,
+                        const vector<double>& w, int L, const string& type, int Qdims, int face) {
+    int Nd = (int)mu.size();
+    SurfBasis S;
+    vector<int> idx;
+    vector<double> mup, phip;
+    if (Qdims == 1) {
+        if (face > 1) fail("Unknown surface for 1D angular domain.");
+        for (int n = 0; n < Nd; ++n)
+            if (face == 0 ? mu[n] > 0 : mu[n] < 0) { idx.push_back(n); mup.push_back(face == 0 ? mu[n] : -mu[n]); }
+    } else {
+        for (int n = 0; n < Nd; ++n) {
+            double a, ph;
+            switch (face / 2) {
+                case 0: a = mu[n]; ph = std::atan2(xi[n], eta[n]); break;
+                case 1: a = eta[n]; ph = std::atan2(mu[n], xi[n]); break;
+                default: a = xi[n]; ph = std::atan2(eta[n], mu[n]); break;
+            }
+            if (face % 2 == 0 ? a > 0 : a < 0) {
+                idx.push_back(n);
+                mup.push_back(face % 2 == 0 ? a : -a);
+                phip.push_back(ph);
+            }
+        }
+    }
+    int Ndp = (int)idx.size();
+    S.Ndp = Ndp;
+    vector<double> wp(Ndp);
+    for (int i = 0; i < Ndp; ++i) wp[i] = w[idx[i]];
+    S.n_to_np.assign(Nd, 0);
+    for (int i = 0; i < Ndp; ++i) S.n_to_np[idx[i]] = i + 1;
+    bool gal = (type == "galerkin-m" || type == "galerkin-d");
+    int Lb = gal ? std::max(L, galerkin_minimum_order(Ndp, Qdims)) : L;
+    if (Qdims == 1) {
+        vector<vector<double>> Pl(Ndp);
+        for (int i = 0; i < Ndp; ++i) Pl[i] = half_range_legendre_up_to_L(Lb, mup[i]);
+        if (type == "standard") {
+            S.Np = L + 1;
+            S.M.assign((size_t)Ndp * S.Np, 0); S.D.assign((size_t)S.Np * Ndp, 0);
+            for (int l = 0; l <= L; ++l)
+                for (int i = 0; i < Ndp; ++i) {
+                    S.M[i + (size_t)Ndp * l] = Pl[i][l];
+                    S.D[l + (size_t)S.Np * i] = wp[i] * Pl[i][l];
+                }
+        } else if (type == "galerkin-m") {
+            S.Np = Ndp;
+            S.M.assign((size_t)Ndp * Ndp, 0);
+            for (int l = 0; l < Ndp; ++l)
+                for (int i = 0; i < Ndp; ++i) S.M[i + (size_t)Ndp * l] = Pl[i][l];
+            S.D = invert(S.M, Ndp);
+        } else if (type == "galerkin-d") {
+            S.Np = Ndp;
+            S.D.assign((size_t)Ndp * Ndp, 0);
+            for (int l = 0; l < Ndp; ++l)
+                for (int i = 0; i < Ndp; ++i) S.D[l + (size_t)Ndp * i] = wp[i] * Pl[i][l];
+            S.M = invert(S.D, Ndp);
+        } else {
+            fail("Unknown method.");
+        }
+    } else {
+        vector<Harm> R(Ndp);
+        for (int i = 0; i < Ndp; ++i) R[i] = real_half_range_sh_up_to_L(Lb, std::fabs(mup[i]), phip[i]);
+        if (type == "standard") {
+            S.Np = (Qdims == 2) ? (L + 1) * (L + 2) / 2 : (L + 1) * (L + 1);
+            S.M.assign((size_t)Ndp * S.Np, 0); S.D.assign((size_t)S.Np * Ndp, 0);
+            int li = 0;
+            for (int l = 0; l <= L; ++l)
+                for (int m = -l; m <= l; ++m) {
+                    if (Qdims == 2 && ((l + m) % 2 + 2) % 2 == 1) continue;
+                    for (int i = 0; i < Ndp; ++i) {
+                        S.M[i + (size_t)Ndp * li] = R[i][l][m + l];
+                        S.D[li + (size_t)S.Np * i] = wp[i] * R[i][l][m + l];
+                    }
+                    ++li;
+                }
+        } else if (gal) {
+            int gt = (type == "galerkin-d") ? 1 : 3;
+            int Lg = Lb;
+            Basis B;
+            bool ok = gram_schmidt(Ndp, Lg, R, wp, Qdims, gt, B);
+            while (!ok) {
+                ++Lg;
+                if (Lg > Lb + Ndp + 10) fail("Surface Galerkin: unable to find enough independent half-range harmonics");
+                for (int i = 0; i < Ndp; ++i) R[i] = real_half_range_sh_up_to_L(Lg, std::fabs(mup[i]), phip[i]);
+                ok = gram_schmidt(Ndp, Lg, R, wp, Qdims, gt, B);
+            }
+            S.Np = B.Np; S.M = B.M; S.D = B.D;
+        } else {
+            fail("Unknown method.");
+        }
+    }
+    return S;
+}
+
+// ---------------------------------------------------------------------------------------
+// Problem and solver state
+// ---------------------------------------------------------------------------------------
+
+struct Angular {
+    int Nd = 0, Np = 0, Nps = 0;
+    vector<double> mu, eta, xi, w;
+    Basis B;
+    vector<SurfBasis> S;  // one per face
+};
+
+}  // namespace
+
+extern "C" {
+
+struct oracle_problem {
+    int32_t ndims, Nx, Ny, Nz;
+    const double *dx, *dy, *dz;     // cell widths per axis (Nx, Ny, Nz entries)
+    const int64_t* mat;             // [Nx*Ny*Nz], 1-based material index, x fastest
+    int32_t Nmat, G;
+    const double* sigt;             // Radiant Σtot[G, Nmat] column-major
+    const double* sigs;             // Radiant Σs[Nmat, G(from), G(to), Lxs+1] column-major
+    int32_t Lxs;
+    const double* qvol;             // volume source, moment p=1: [G, Nx*Ny*Nz] column-major
+    const int32_t* bc;              // [2*ndims] 0 void, 1 reflective, 2 periodic
+    const char* quad_type;
+    int32_t quad_order, qdims, legendre_order;
+    const char* sn_type;
+    const char* accel;
+    int32_t restart, depth, maxit;
+    double tol;
+    int32_t nthreads;               // 1 = serial (Radiant's execution model); >1 = OpenMP over directions
+    int32_t verbose;
+};
+
+struct oracle_stats {
+    int64_t niter;
+    double resid;
+    double rho;
+    int32_t converged;
+    int32_t passes;
+};
+
+}  // extern "C"
+
+namespace {
+
+struct Solver {
+    const oracle_problem& P;
+    int ndims, Nx, Ny, Nz, Nc, G, Nmat;
+    Angular A;
+    int Ls;                     // max(pl): Σs sized to Ls+1 (sn_flux.jl:79-83)
+    vector<double> sigs;        // [Nmat][G][G][Ls+1] padded copy: sigs[m + Nmat*(gi + G*(gf + G*l))]
+    vector<int> matc;           // 0-based material per cell
+    // state layout: [Φ (Np*Nc, p fastest) | x-bnd | y-bnd | z-bnd]
+    size_t nphi, nbx, nby, nbz, S;
+    vector<int> nxm, nxp, nym, nyp, nzm, nzp;  // n_to_n+ per face (1-based or 0)
+
+    explicit Solver(const oracle_problem& p) : P(p) {
+        ndims = p.ndims;
+        if (ndims < 1 || ndims > 3) fail("Dimension is not 1, 2 or 3.");
+        Nx = p.Nx; Ny = ndims >= 2 ? p.Ny : 1; Nz = ndims >= 3 ? p.Nz : 1;
+        Nc = Nx * Ny * Nz;
+        G = p.G; Nmat = p.Nmat;
+        // sn_flux.jl:38-48 -- quadrature and angular operators
+        string qt = p.quad_type;
+        int Qdims;
+        if (qt == "gauss-legendre") {  // SN.jl:638-659
+            Qdims = 1;
+        } else {
+            Qdims = (ndims == 3) ? 3 : (p.qdims == 2 ? 2 : 3);
+        }
+        quadrature(p.quad_order, qt, ndims, Qdims, A.mu, A.eta, A.xi, A.w);
+        A.Nd = (int)A.w.size();
+        A.B = angular_polynomial_basis(A.mu, A.eta, A.xi, A.w, p.legendre_order, p.sn_type, Qdims);
+        A.Np = A.B.Np;
+        int nf = 2 * ndims;
+        A.S.resize(nf);
+        for (int f = 0; f < nf; ++f) {
+            A.S[f] = surface_basis(A.mu, A.eta, A.xi, A.w, p.legendre_order, p.sn_type, Qdims, f);
+        }
+        A.Nps = A.S[nf - 1].Np;  // the reference keeps the last face's count (angular_polynomial_basis.jl:227)
+        for (int f = 0; f < nf; ++f)
+            if (A.S[f].Np != A.Nps) fail("faces with different half-range basis sizes are not supported");
+        auto conv = [&](int f, vector<int>& v) {
+            v.assign(A.Nd, 0);
+            if (f < nf) for (int n = 0; n < A.Nd; ++n) v[n] = (int)A.S[f].n_to_np[n];
+        };
+        conv(0, nxm); conv(1, nxp); conv(2, nym); conv(3, nyp); conv(4, nzm); conv(5, nzp);
+        // XS (sn_flux.jl:68-83): Σs padded/truncated to Ls = max(pl)
+        Ls = 0;
+        for (long l : A.B.pl) Ls = std::max(Ls, (int)l);
+        sigs.assign((size_t)Nmat * G * G * (Ls + 1), 0.0);
+        for (int l = 0; l <= std::min(Ls, p.Lxs); ++l)
+            for (int gf = 0; gf < G; ++gf)
+                for (int gi = 0; gi < G; ++gi)
+                    for (int m = 0; m < Nmat; ++m)
+                        sigs[m + (size_t)Nmat * (gi + (size_t)G * (gf + (size_t)G * l))] =
+                            p.sigs[m + (size_t)Nmat * (gi + (size_t)G * (gf + (size_t)G * l))];
+        matc.resize(Nc);
+        for (int c = 0; c < Nc; ++c) {
+            long m = p.mat[c];
+            if (m < 1 || m > Nmat) fail("material index out of range");
+            matc[c] = (int)(m - 1);
+        }
+        for (int f = 0; f < nf; ++f) {
+            if (p.bc[f] < 0 || p.bc[f] > 2) fail("Unknown boundary condition type.");
+        }
+        nphi = (size_t)A.Np * Nc;
+        nbx = (size_t)A.Nps * 2 * Ny * Nz;
+        nby = ndims >= 2 ? (size_t)A.Nps * 2 * Nx * Nz : 0;
+        nbz = ndims >= 3 ? (size_t)A.Nps * 2 * Nx * Ny : 0;
+        S = nphi + nbx + nby + nbz;
+    }
+
+    double sig_s(int m, int gi, int gf, int l) const {
+        return sigs[m + (size_t)Nmat * (gi + (size_t)G * (gf + (size_t)G * l))];
+    }
+    double sig_t(int g, int m) const { return P.sigt[g + (size_t)G * m]; }
+
+    // ----- sweeps --------------------------------------------------------------------
+    // DD order-1 cell solve, written once for 1/2/3 axes (sn_1D_bte.jl:150-189,
+    // sn_2D_bte.jl:35-117, sn_3D_bte.jl:42-153 with O=1, C=1, w=[-1,2]):
+    //   S = ((St + 2hx) + 2hy) + 2hz ;  Q = ((Qn + 2hx*fx) + 2hy*fy) + 2hz*fz ;  psi = Q/S
+    //   f_out = -f_in + 2 psi on every axis.
+
+    // sn_sweep_1D.jl:51-124
+    void sweep1(int n, int g, double* phi, const double* Ql, const double* bnd_in, double* tmpx) {
+        const int Np = A.Np, Nps = A.Nps;
+        double mu = A.mu[n];
+        const double* Mrow = nullptr;  // row n of M: M[n + Nd*p]
+        // incoming face value
+        double fx = 0.0;
+        int fm = nxm[n], fp = nxp[n];
+        const SurfBasis* Sb = nullptr;
+        int row = -1;
+        if (fm != 0) { Sb = &A.S[0]; row = fm - 1; } else if (fp != 0) { Sb = &A.S[1]; row = fp - 1; }
+        bool fwd = mu >= 0;
+        int bcin = fwd ? P.bc[0] : P.bc[1];
+        if (bcin != 0 && Sb) {
+            int src_face = fwd ? (bcin == 1 ? 0 : 1) : (bcin == 1 ? 1 : 0);
+            for (int p = 0; p < Nps; ++p)
+                fx += Sb->M[row + (size_t)Sb->Ndp * p] * bnd_in[p + (size_t)Nps * src_face];
+        }
+        (void)Mrow;
+        bool tang = std::fabs(mu) <= 1e-10;
+        double mue = tang ? 0.0 : mu;
+        for (int k = 0; k < Nx; ++k) {
+            int ix = fwd ? k : Nx - 1 - k;
+            int c = ix;
+            double Qn = 0.0;
+            for (int p = 0; p < Np; ++p) Qn += A.B.M[n + (size_t)A.Nd * p] * Ql[p + (size_t)Np * c];
+            double St = sig_t(g, matc[c]);
+            double hx = std::fabs(mue) / P.dx[ix];
+            double Sm = St + 2.0 * hx;
+            double Q = Qn + 2.0 * hx * fx;
+            double psi = Q / Sm;
+            if (!tang) fx = -fx + 2.0 * psi;
+            for (int p = 0; p < Np; ++p) phi[p + (size_t)Np * c] += A.B.D[p + (size_t)Np * n] * psi;
+        }
+        if (Sb) {
+            int of = fwd ? 1 : 0;
+            for (int p = 0; p < Nps; ++p) tmpx[p + (size_t)Nps * of] += Sb->D[p + (size_t)Nps * row] * fx;
+        }
+    }
+
+    // incoming boundary value for a face array b[p + Nps*(f + 2*pos)] (sweep BC branches).
+    double bc_in(int face_bc, bool fwd, const SurfBasis* Sb, int row, const double* b, size_t pos) {
+        if (face_bc == 0 || !Sb) return 0.0;
+        int src = fwd ? (face_bc == 1 ? 0 : 1) : (face_bc == 1 ? 1 : 0);
+        double v = 0.0;
+        for (int p = 0; p < A.Nps; ++p) v += Sb->M[row + (size_t)Sb->Ndp * p] * b[p + (size_t)A.Nps * (src + 2 * pos)];
+        return v;
+    }
+
+    // sn_sweep_2D.jl:54-184 and sn_sweep_3D.jl:57-240 (z collapses to one layer in 2D).
+    void sweep23(int n, int g, double* phi, const double* Ql, const double* bx_in, const double* by_in,
+                 const double* bz_in, double* tx, double* ty, double* tz) {
+        const int Np = A.Np, Nps = A.Nps, Nd = A.Nd;
+        double mu = A.mu[n], eta = A.eta[n], xi = A.xi[n];
+        bool fx_ = mu >= 0, fy_ = eta >= 0, fz_ = xi >= 0;
+        const SurfBasis *Sx = nullptr, *Sy = nullptr, *Sz = nullptr;
+        int rx = -1, ry = -1, rz = -1;
+        if (nxm[n]) { Sx = &A.S[0]; rx = nxm[n] - 1; } else if (nxp[n]) { Sx = &A.S[1]; rx = nxp[n] - 1; }
+        if (nym[n]) { Sy = &A.S[2]; ry = nym[n] - 1; } else if (nyp[n]) { Sy = &A.S[3]; ry = nyp[n] - 1; }
+        if (ndims == 3) {
+            if (nzm[n]) { Sz = &A.S[4]; rz = nzm[n] - 1; } else if (nzp[n]) { Sz = &A.S[5]; rz = nzp[n] - 1; }
+        }
+        int bcx = fx_ ? P.bc[0] : P.bc[1];
+        int bcy = fy_ ? P.bc[2] : P.bc[3];
+        int bcz = ndims == 3 ? (fz_ ? P.bc[4] : P.bc[5]) : 0;
+        vector<double> Fx((size_t)Ny * Nz, 0.0);  // x-face flux per (iy,iz)
+        vector<double> Fy(Nz, 0.0);                // y-face flux per iz (reset per ix)
+        for (int kx = 0; kx < Nx; ++kx) {
+            int ix = fx_ ? kx : Nx - 1 - kx;
+            std::fill(Fy.begin(), Fy.end(), 0.0);
+            for (int ky = 0; ky < Ny; ++ky) {
+                int iy = fy_ ? ky : Ny - 1 - ky;
+                double Fz = 0.0;
+                if (ndims == 3) Fz = bc_in(bcz, fz_, Sz, rz, bz_in, (size_t)ix + (size_t)Nx * iy);
+                for (int kz = 0; kz < Nz; ++kz) {
+                    int iz = fz_ ? kz : Nz - 1 - kz;
+                    if (ky == 0) Fy[iz] = bc_in(bcy, fy_, Sy, ry, by_in, (size_t)ix + (size_t)Nx * iz);
+                    if (kx == 0) Fx[iy + (size_t)Ny * iz] = bc_in(bcx, fx_, Sx, rx, bx_in, (size_t)iy + (size_t)Ny * iz);
+                    size_t c = (size_t)ix + (size_t)Nx * (iy + (size_t)Ny * iz);
+                    double Qn = 0.0;
+                    for (int p = 0; p < Np; ++p) Qn += A.B.M[n + (size_t)Nd * p] * Ql[p + (size_t)Np * c];
+                    double St = sig_t(g, matc[c]);
+                    double hx = std::fabs(mu) / P.dx[ix];
+                    double hy = std::fabs(eta) / P.dy[iy];
+                    double& fxr = Fx[iy + (size_t)Ny * iz];
+                    double& fyr = Fy[iz];
+                    double psi;
+                    if (ndims == 2) {
+                        double Sm = St + 2.0 * hx;
+                        Sm = Sm + 2.0 * hy;
+                        double Q = Qn + 2.0 * hx * fxr;
+                        Q = Q + 2.0 * hy * fyr;
+                        psi = Q / Sm;
+                    } else {
+                        double hz = std::fabs(xi) / P.dz[iz];
+                        double Sm = ((St + 2.0 * hx) + 2.0 * hy) + 2.0 * hz;
+                        double Q = ((Qn + 2.0 * hx * fxr) + 2.0 * hy * fyr) + 2.0 * hz * Fz;
+                        psi = Q / Sm;
+                        Fz = -Fz + 2.0 * psi;
+                    }
+                    fxr = -fxr + 2.0 * psi;
+                    fyr = -fyr + 2.0 * psi;
+                    for (int p = 0; p < Np; ++p) phi[p + (size_t)Np * c] += A.B.D[p + (size_t)Np * n] * psi;
+                    if (kx == Nx - 1 && Sx) {
+                        int of = fx_ ? 1 : 0;
+                        for (int p = 0; p < Nps; ++p)
+                            tx[p + (size_t)Nps * (of + 2 * ((size_t)iy + (size_t)Ny * iz))] += Sx->D[p + (size_t)Nps * rx] * fxr;
+                    }
+                    if (ky == Ny - 1 && Sy) {
+                        int of = fy_ ? 1 : 0;
+                        for (int p = 0; p < Nps; ++p)
+                            ty[p + (size_t)Nps * (of + 2 * ((size_t)ix + (size_t)Nx * iz))] += Sy->D[p + (size_t)Nps * ry] * fyr;
+                    }
+                    if (ndims == 3 && kz == Nz - 1 && Sz) {
+                        int of = fz_ ? 1 : 0;
+                        for (int p = 0; p < Nps; ++p)
+                            tz[p + (size_t)Nps * (of + 2 * ((size_t)ix + (size_t)Nx * iy))] += Sz->D[p + (size_t)Nps * rz] * Fz;
+                    }
+                }
+            }
+        }
+    }
+
+    // sn_inner_pass.jl:212-332 -- one pass T(z) (homogeneous=false) or A z (true), in place on
+    // the state z = [Φ, x-bnd, y-bnd, z-bnd].  With nthreads>1 the direction loop is split over
+    // OpenMP threads with private accumulators, reduced in thread order.
+    void pass(int g, bool homog, vector<double>& z, const vector<double>& Qlout, vector<double>& Ql) {
+        const int Np = A.Np;
+        double* phi = z.data();
+        double* bx = phi + nphi;
+        // Ql = (homog ? 0 : Qlout) + Σs_l Φl   (scattering_source.jl:25-30)
+        for (int c = 0; c < Nc; ++c) {
+            int m = matc[c];
+            for (int p = 0; p < Np; ++p) {
+                double q = homog ? 0.0 : Qlout[p + (size_t)Np * c];
+                q += sig_s(m, g, g, (int)A.B.pl[p]) * phi[p + (size_t)Np * c];
+                Ql[p + (size_t)Np * c] = q;
+            }
+        }
+        vector<double> bin(bx, bx + nbx + nby + nbz);
+        int nt = std::max(1, P.nthreads);
+        if (nt == 1) {
+            std::fill(phi, phi + nphi, 0.0);
+            vector<double> tmp(nbx + nby + nbz, 0.0);
+            for (int n = 0; n < A.Nd; ++n) direction(n, g, phi, Ql.data(), bin.data(), tmp.data());
+            std::copy(tmp.begin(), tmp.end(), bx);
+        } else {
+#ifdef _OPENMP
+            vector<vector<double>> lphi(nt), ltmp(nt);
+#pragma omp parallel num_threads(nt)
+            {
+                int t = omp_get_thread_num();
+                lphi[t].assign(nphi, 0.0);
+                ltmp[t].assign(nbx + nby + nbz, 0.0);
+#pragma omp for schedule(static)
+                for (int n = 0; n < A.Nd; ++n) direction(n, g, lphi[t].data(), Ql.data(), bin.data(), ltmp[t].data());
+            }
+            std::fill(phi, phi + nphi, 0.0);
+            std::fill(bx, bx + nbx + nby + nbz, 0.0);
+            for (int t = 0; t < nt; ++t) {
+                for (size_t i = 0; i < nphi; ++i) phi[i] += lphi[t][i];
+                for (size_t i = 0; i < nbx + nby + nbz; ++i) bx[i] += ltmp[t][i];
+            }
+#else
+            fail("oracle built without OpenMP");
+#endif
+        }
+    }
+
+    void direction(int n, int g, double* phi, const double* Ql, const double* bin, double* tmp) {
+        if (ndims == 1) sweep1(n, g, phi, Ql, bin, tmp);
+        else sweep23(n, g, phi, Ql, bin, bin + nbx, bin + nbx + nby, tmp, tmp + nbx, tmp + nbx + nby);
+    }
+
+    // ----- Krylov state algebra (tools/krylov_state.jl:28-144) ------------------------
+    double dot(const vector<double>& a, const vector<double>& b) const {
+        size_t off[5] = {0, nphi, nphi + nbx, nphi + nbx + nby, S};
+        double s = 0.0;
+        for (int k = 0; k < 4; ++k) {
+            double sk = 0.0;
+            for (size_t i = off[k]; i < off[k + 1]; ++i) sk += a[i] * b[i];
+            s += sk;
+        }
+        return s;
+    }
+    double norm(const vector<double>& a) const { return std::sqrt(dot(a, a)); }
+    static void axpy(double al, const vector<double>& x, vector<double>& y) {
+        for (size_t i = 0; i < y.size(); ++i) y[i] += al * x[i];
+    }
+    static void scale(double al, vector<double>& x) {
+        for (double& v : x) v *= al;
+    }
+    static double rho_est(double r0, double r, long it) {
+        if (it > 1 && r0 > 0.0 && r > 0.0) return std::pow(r / r0, 1.0 / (double)(it - 1));
+        return NAN;
+    }
+
+    // ----- in-group solve (sn_one_speed.jl:81-196) --------------------------------------
+    void one_speed(int g, vector<double>& z_phi_bnd /* state; Φ part holds the group flux */,
+                   const vector<double>& Qlout, oracle_stats& st) {
+        vector<double> Ql(nphi);
+        long passes = 0;
+        bool any = false;
+        for (double v : Qlout) if (v != 0.0) { any = true; break; }
+        if (!any) {  // sn_one_speed.jl:116-120
+            std::fill(z_phi_bnd.begin(), z_phi_bnd.begin() + nphi, 0.0);
+            st.niter = 0; st.resid = 0.0; st.rho = NAN; st.converged = 1; st.passes = 0;
+            return;
+        }
+        vector<double>& work = z_phi_bnd;
+        auto T = [&](vector<double>& out, const vector<double>& zin) {
+            work = zin; pass(g, false, work, Qlout, Ql); out = work; ++passes;
+        };
+        auto matvec = [&](vector<double>& out, const vector<double>& zin) {
+            work = zin; pass(g, true, work, Qlout, Ql); out = work; ++passes;
+            scale(-1.0, out); axpy(1.0, zin, out);
+        };
+        string acc = P.accel;
+        long niter = 0; double resid = 1.0, rho = NAN; bool conv = false;
+        vector<double> z;
+        if (acc == "none" || acc == "livolant") {
+            z = work;
+            livolant(z, T, acc == "none" ? -1 : 3, niter, resid, conv, rho);
+        } else if (acc == "anderson") {
+            z = work;
+            anderson(z, T, niter, resid, conv, rho);
+        } else if (acc == "gmres" || acc == "bicgstab") {
+            std::fill(work.begin(), work.end(), 0.0);
+            pass(g, false, work, Qlout, Ql); ++passes;
+            vector<double> c = work;
+            z.assign(S, 0.0);
+            if (acc == "gmres") gmres(z, matvec, c, niter, resid, conv, rho);
+            else bicgstab(z, matvec, c, niter, resid, conv, rho);
+        } else {
+            fail("Unknown acceleration method: " + acc + ".");
+        }
+        work = z;
+        pass(g, false, work, Qlout, Ql); ++passes;  // reconstruction (sn_one_speed.jl:186)
+        st.niter = niter; st.resid = resid; st.rho = rho; st.converged = conv; st.passes = (int32_t)passes;
+        if (P.verbose)
+            std::printf(">>>Group %d has %sconverged ( eps = %.4E , N = %ld , rho = %.4f )\n", g + 1,
+                        conv ? "" : "not ", resid, niter, rho);
+    }
+
+    // tools/gmres.jl:35-133
+    template <class MV>
+    void gmres(vector<double>& z, MV matvec, const vector<double>& c, long& iter, double& resid, bool& conv,
+               double& rho) {
+        int m = std::max(1, (int)P.restart);
+        long maxit = P.maxit;
+        double tol = P.tol;
+        double bnorm = norm(c);
+        if (bnorm == 0.0) {
+            std::fill(z.begin(), z.end(), 0.0);
+            iter = 0; resid = 0.0; conv = true; rho = NAN;
+            return;
+        }
+        vector<vector<double>> V(m + 1, vector<double>(S, 0.0));
+        vector<double> w(S, 0.0), r(S, 0.0);
+        vector<double> H((size_t)(m + 1) * m, 0.0);  // H[i + (m+1)*j]
+        vector<double> cs(m, 0.0), sn(m, 0.0), gv(m + 1, 0.0);
+        auto h = [&](int i, int j) -> double& { return H[i + (size_t)(m + 1) * j]; };
+        iter = 0; resid = 1.0; double resid0 = NAN; conv = false;
+        while (iter < maxit && !conv) {
+            matvec(r, z); ++iter;
+            scale(-1.0, r); axpy(1.0, c, r);
+            double beta = norm(r);
+            resid = beta / bnorm;
+            if (std::isnan(resid0)) resid0 = resid;
+            if (resid < tol) { conv = true; break; }
+            V[0] = r; scale(1.0 / beta, V[0]);
+            std::fill(gv.begin(), gv.end(), 0.0);
+            gv[0] = beta;
+            int k = 0;
+            for (int j = 0; j < m; ++j) {
+                if (iter >= maxit) break;
+                matvec(w, V[j]); ++iter;
+                for (int i = 0; i <= j; ++i) {
+                    h(i, j) = dot(w, V[i]);
+                    axpy(-h(i, j), V[i], w);
+                }
+                double hjp1 = norm(w);
+                h(j + 1, j) = hjp1;
+                if (hjp1 > 0.0) { V[j + 1] = w; scale(1.0 / hjp1, V[j + 1]); }
+                for (int i = 0; i < j; ++i) {
+                    double t = cs[i] * h(i, j) + sn[i] * h(i + 1, j);
+                    h(i + 1, j) = -sn[i] * h(i, j) + cs[i] * h(i + 1, j);
+                    h(i, j) = t;
+                }
+                double den = std::hypot(h(j, j), h(j + 1, j));
+                if (den == 0.0) { cs[j] = 1.0; sn[j] = 0.0; }
+                else { cs[j] = h(j, j) / den; sn[j] = h(j + 1, j) / den; }
+                h(j, j) = cs[j] * h(j, j) + sn[j] * h(j + 1, j);
+                h(j + 1, j) = 0.0;
+                gv[j + 1] = -sn[j] * gv[j];
+                gv[j] = cs[j] * gv[j];
+                k = j + 1;
+                resid = std::fabs(gv[j + 1]) / bnorm;
+                if (resid < tol) { conv = true; break; }
+                if (hjp1 == 0.0) break;
+            }
+            vector<double> y(k, 0.0);
+            for (int i = k - 1; i >= 0; --i) {
+                double s = gv[i];
+                for (int l = i + 1; l < k; ++l) s -= h(i, l) * y[l];
+                y[i] = h(i, i) != 0.0 ? s / h(i, i) : 0.0;
+            }
+            for (int i = 0; i < k; ++i) axpy(y[i], V[i], z);
+            if (conv) break;
+        }
+        rho = rho_est(resid0, resid, iter);
+    }
+
+    // tools/anderson.jl:38-140 (beta = 1)
+    template <class FP>
+    void anderson(vector<double>& z, FP fixedpoint, long& iter, double& resid, bool& conv, double& rho) {
+        int m = std::max(1, (int)P.depth);
+        double beta = 1.0;
+        vector<double> g(S), f(S), f_old(S, 0.0), g_old(S, 0.0);
+        vector<vector<double>> dF, dG;
+        iter = 0; resid = 1.0; double resid0 = NAN; conv = false;
+        for (long it = 1; it <= P.maxit; ++it) {
+            fixedpoint(g, z); ++iter;
+            f = g; axpy(-1.0, z, f);
+            double fnorm = norm(f);
+            double sc = std::max(std::max(norm(z), norm(g)), 1e-16);
+            resid = fnorm / sc;
+            if (std::isnan(resid0)) resid0 = resid;
+            if (resid < P.tol) { conv = true; break; }
+            if (it == 1) {
+                axpy(beta, f, z);
+            } else {
+                vector<double> a = f; axpy(-1.0, f_old, a);
+                vector<double> b = g; axpy(-1.0, g_old, b);
+                dF.push_back(a); dG.push_back(b);
+                if ((int)dF.size() > m) { dF.erase(dF.begin()); dG.erase(dG.begin()); }
+                int mk = (int)dF.size();
+                vector<double> Nm((size_t)mk * mk), bb(mk);
+                for (int i = 0; i < mk; ++i) {
+                    bb[i] = dot(dF[i], f);
+                    for (int j = 0; j < mk; ++j) Nm[i + (size_t)mk * j] = dot(dF[i], dF[j]);
+                }
+                double tr = 0.0;
+                for (int i = 0; i < mk; ++i) tr += Nm[i + (size_t)mk * i];
+                double lam = 1e-12 * (tr / mk + 1e-300);
+                for (int i = 0; i < mk; ++i) Nm[i + (size_t)mk * i] += lam;
+                vector<double> gam = anderson_solve(Nm, bb, mk);
+                axpy(beta, f, z);
+                for (int i = 0; i < mk; ++i) {
+                    axpy(-gam[i], dG[i], z);
+                    axpy(-(beta - 1.0) * gam[i], dF[i], z);
+                }
+            }
+            f_old = f; g_old = g;
+        }
+        rho = rho_est(resid0, resid, iter);
+    }
+
+    // tools/anderson.jl:109-140
+    static vector<double> anderson_solve(vector<double> M, vector<double> x, int n) {
+        for (int k = 0; k < n; ++k) {
+            int p = k;
+            double mx = std::fabs(M[k + (size_t)n * k]);
+            for (int i = k + 1; i < n; ++i)
+                if (std::fabs(M[i + (size_t)n * k]) > mx) { mx = std::fabs(M[i + (size_t)n * k]); p = i; }
+            if (p != k) {
+                for (int j = k; j < n; ++j) std::swap(M[k + (size_t)n * j], M[p + (size_t)n * j]);
+                std::swap(x[k], x[p]);
+            }
+            double piv = M[k + (size_t)n * k];
+            if (piv == 0.0) continue;
+            for (int i = k + 1; i < n; ++i) {
+                double fct = M[i + (size_t)n * k] / piv;
+                for (int j = k; j < n; ++j) M[i + (size_t)n * j] -= fct * M[k + (size_t)n * j];
+                x[i] -= fct * x[k];
+            }
+        }
+        for (int k = n - 1; k >= 0; --k) {
+            double s = x[k];
+            for (int j = k + 1; j < n; ++j) s -= M[k + (size_t)n * j] * x[j];
+            x[k] = M[k + (size_t)n * k] != 0.0 ? s / M[k + (size_t)n * k] : 0.0;
+        }
+        return x;
+    }
+
+    // tools/livolant.jl:69-81 and :115-155 (period < 0 means no extrapolation: "none")
+    template <class FP>
+    void livolant(vector<double>& z, FP fixedpoint, int period, long& iter, double& resid, bool& conv,
+                  double& rho) {
+        vector<double> g(S), f(S), z1(S, 0.0), z0(S, 0.0);
+        int nhist = 0;
+        iter = 0; resid = 1.0; double resid0 = NAN; conv = false;
+        for (long it = 1; it <= P.maxit; ++it) {
+            fixedpoint(g, z); ++iter;
+            f = g; axpy(-1.0, z, f);
+            double fnorm = norm(f);
+            double sc = std::max(norm(g), 1e-16);
+            resid = fnorm / sc;
+            if (std::isnan(resid0)) resid0 = resid;
+            if (resid < P.tol) { z = g; conv = true; break; }
+            if (period > 0 && it % period == 0 && nhist >= 2) {
+                size_t off[5] = {0, nphi, nphi + nbx, nphi + nbx + nby, S};
+                double se = 0.0, sd = 0.0;
+                for (int k = 0; k < 4; ++k) {
+                    double a = 0.0, b = 0.0;
+                    for (size_t i = off[k]; i < off[k + 1]; ++i) {
+                        double e0 = z1[i] - z0[i];
+                        double e1 = g[i] - z1[i];
+                        double de = e1 - e0;
+                        a += e0 * de;
+                        b += de * de;
+                    }
+                    se += a; sd += b;
+                }
+                double mu = 1.0;
+                if (sd != 0.0) { mu = -se / sd; if (mu <= 0) mu = 1.0; }
+                z = g; scale(mu, z); axpy(1.0 - mu, z1, z);
+            } else {
+                z = g;
+            }
+            z0 = z1; z1 = g; ++nhist;
+        }
+        rho = rho_est(resid0, resid, iter);
+    }
+
+    // tools/bicgstab.jl:30-101
+    template <class MV>
+    void bicgstab(vector<double>& z, MV matvec, const vector<double>& c, long& iter, double& resid, bool& conv,
+                  double& rho) {
+        double bnorm = norm(c);
+        if (bnorm == 0.0) { std::fill(z.begin(), z.end(), 0.0); iter = 0; resid = 0; conv = true; rho = NAN; return; }
+        vector<double> r(S), r0(S), p(S, 0.0), v(S, 0.0), s(S), t(S);
+        matvec(r, z);
+        scale(-1.0, r); axpy(1.0, c, r);
+        iter = 1;
+        resid = norm(r) / bnorm;
+        double resid0 = resid;
+        conv = false;
+        if (resid < P.tol) { conv = true; rho = NAN; return; }
+        r0 = r;
+        double rho_old = 1.0, al = 1.0, om = 1.0, tiny = 1e-300;
+        while (iter < P.maxit) {
+            double rn = dot(r0, r);
+            if (std::fabs(rn) < tiny) break;
+            double be = (rn / rho_old) * (al / om);
+            axpy(-om, v, p); scale(be, p); axpy(1.0, r, p);
+            matvec(v, p); ++iter;
+            double den = dot(r0, v);
+            if (std::fabs(den) < tiny) break;
+            al = rn / den;
+            s = r; axpy(-al, v, s);
+            if (norm(s) / bnorm < P.tol) {
+                axpy(al, p, z);
+                resid = norm(s) / bnorm;
+                conv = true;
+                break;
+            }
+            matvec(t, s); ++iter;
+            double tt = dot(t, t);
+            if (tt < tiny) break;
+            om = dot(t, s) / tt;
+            axpy(al, p, z); axpy(om, s, z);
+            r = s; axpy(-om, t, r);
+            resid = norm(r) / bnorm;
+            if (resid < P.tol) { conv = true; break; }
+            if (std::fabs(om) < tiny) break;
+            rho_old = rn;
+        }
+        rho = rho_est(resid0, resid, iter);
+    }
+
+    // sn_flux.jl:150-232 -- one Gauss-Seidel pass over groups (upscatter sees zero flux).
+    void solve(double* phi_out, oracle_stats* stats) {
+        const int Np = A.Np;
+        vector<double> phi_all((size_t)G * nphi, 0.0);  // [g][p + Np*c]
+        for (int g = 0; g < G; ++g) {
+            vector<double> Qlout(nphi, 0.0);
+            // scattering_source.jl:59-68 (all gi != gf, in order)
+            for (int gi = 0; gi < G; ++gi) {
+                if (gi == g) continue;
+                const double* pg = &phi_all[(size_t)gi * nphi];
+                for (int c = 0; c < Nc; ++c) {
+                    int m = matc[c];
+                    for (int p = 0; p < Np; ++p)
+                        Qlout[p + (size_t)Np * c] += sig_s(m, gi, g, (int)A.B.pl[p]) * pg[p + (size_t)Np * c];
+                }
+            }
+            // sn_flux.jl:179 -- volume source enters moment p=1 only (Source.jl:135)
+            for (int c = 0; c < Nc; ++c) Qlout[(size_t)Np * c] += P.qvol[g + (size_t)G * c];
+            vector<double> z(S, 0.0);
+            std::copy(phi_all.begin() + (size_t)g * nphi, phi_all.begin() + (size_t)(g + 1) * nphi, z.begin());
+            oracle_stats st{};
+            one_speed(g, z, Qlout, st);
+            std::copy(z.begin(), z.begin() + nphi, phi_all.begin() + (size_t)g * nphi);
+            if (stats) stats[g] = st;
+        }
+        // Radiant layout 𝚽l[g, p, 1, ix, iy, iz]
+        for (int c = 0; c < Nc; ++c)
+            for (int p = 0; p < Np; ++p)
+                for (int g = 0; g < G; ++g)
+                    phi_out[g + (size_t)G * (p + (size_t)Np * c)] = phi_all[(size_t)g * nphi + p + (size_t)Np * c];
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+const char* oracle_last_error(void) { return g_err.c_str(); }
+
+// Quadrature: returns Nd (or a negative status); arrays must hold `cap` entries.
+int oracle_quadrature(int N, const char* type, int Ndims, int Qdims, double* mu, double* eta, double* xi,
+                      double* w, int cap) {
+    try {
+        vector<double> a, b, c, d;
+        quadrature(N, type, Ndims, Qdims, a, b, c, d);
+        int Nd = (int)d.size();
+        if (Nd > cap) fail("capacity too small");
+        for (int i = 0; i < Nd; ++i) { mu[i] = a[i]; eta[i] = b[i]; xi[i] = c[i]; w[i] = d[i]; }
+        return Nd;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// Volume basis: M is Nd x Np column-major, D is Np x Nd column-major; returns Np.
+int oracle_basis(const double* mu, const double* eta, const double* xi, const double* w, int Nd, int L,
+                 const char* type, int Qdims, double* M, double* D, int64_t* pl, int64_t* pm, int capNp) {
+    try {
+        vector<double> a(mu, mu + Nd), b(eta, eta + Nd), c(xi, xi + Nd), d(w, w + Nd);
+        Basis B = angular_polynomial_basis(a, b, c, d, L, type, Qdims);
+        if (B.Np > capNp) fail("capacity too small");
+        std::copy(B.M.begin(), B.M.end(), M);
+        std::copy(B.D.begin(), B.D.end(), D);
+        for (int i = 0; i < B.Np; ++i) { pl[i] = B.pl[i]; pm[i] = B.pm[i]; }
+        return B.Np;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// Half-range basis of one face (0=X-,1=X+,2=Y-,3=Y+,4=Z-,5=Z+): returns Np_surf; *Ndp receives
+// the number of directions entering the face.  M is Ndp x Nps, D is Nps x Ndp.
+int oracle_surface_basis(const double* mu, const double* eta, const double* xi, const double* w, int Nd, int L,
+                         const char* type, int Qdims, int face, double* M, double* D, int64_t* n_to_np,
+                         int* Ndp, int cap) {
+    try {
+        vector<double> a(mu, mu + Nd), b(eta, eta + Nd), c(xi, xi + Nd), d(w, w + Nd);
+        SurfBasis S = surface_basis(a, b, c, d, L, type, Qdims, face);
+        if ((int)S.M.size() > cap || (int)S.D.size() > cap) fail("capacity too small");
+        std::copy(S.M.begin(), S.M.end(), M);
+        std::copy(S.D.begin(), S.D.end(), D);
+        for (int n = 0; n < Nd; ++n) n_to_np[n] = S.n_to_np[n];
+        *Ndp = S.Ndp;
+        return S.Np;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// Harmonics (for pinning tests): R has (L+1)^2 entries, R[l*l + l + m].
+int oracle_real_sh(int L, double mu, double phi, int half_range, double* R) {
+    try {
+        Harm H = half_range ? real_half_range_sh_up_to_L(L, mu, phi) : real_sh_up_to_L(L, mu, phi);
+        for (int l = 0; l <= L; ++l)
+            for (int m = -l; m <= l; ++m) R[l * l + l + m] = H[l][l + m];
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+int oracle_legendre(int L, double x, int half_range, double* P) {
+    try {
+        vector<double> v = half_range ? half_range_legendre_up_to_L(L, x) : legendre_up_to_L(L, x);
+        std::copy(v.begin(), v.end(), P);
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// Sizes of a problem: Nd, Np, Np_surf and the Krylov state length S.
+int oracle_problem_sizes(const oracle_problem* p, int* Nd, int* Np, int* Nps, int64_t* S) {
+    try {
+        Solver s(*p);
+        *Nd = s.A.Nd; *Np = s.A.Np; *Nps = s.A.Nps; *S = (int64_t)s.S;
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// Full solve: phi_out in Radiant's 𝚽l[G, Np, 1, Nx, Ny, Nz] layout; stats_out[G].
+int oracle_solve(const oracle_problem* p, double* phi_out, oracle_stats* stats_out) {
+    try {
+        Solver s(*p);
+        s.solve(phi_out, stats_out);
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// One transport pass of group g (0-based) on a state vector z of length S (in place), with
+// out-of-group source Qlout [Np*Nc] (p fastest).  homogeneous=1 gives A z, 0 gives T(z).
+int oracle_pass(const oracle_problem* p, int g, int homogeneous, double* z, const double* Qlout) {
+    try {
+        Solver s(*p);
+        vector<double> zz(z, z + s.S), q(Qlout, Qlout + s.nphi), Ql(s.nphi);
+        s.pass(g, homogeneous != 0, zz, q, Ql);
+        std::copy(zz.begin(), zz.end(), z);
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+}  // extern "C"

@@ -1,0 +1,119 @@
+// abi.cpp -- extern "C" entry points of libradiant_hip.so (declared in include/radiant_hip.h).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "engine.hpp"
+
+namespace {
+thread_local std::string g_last_error;
+
+}  // namespace
+
+void rt::set_last_error(const std::string& m) { g_last_error = m; }
+
+namespace {
+int fail(int code, const std::string& m) {
+    g_last_error = m;
+    return code;
+}
+
+template <class F>
+int guarded(F f) {
+    try {
+        f();
+        return RT_OK;
+    } catch (const rt::Error& e) {
+        return fail(e.code, e.what());
+    } catch (const std::bad_alloc&) {
+        return fail(RT_ENOMEM, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(RT_EDEVICE, e.what());
+    }
+}
+}  // namespace
+
+struct rt_problem {
+    std::unique_ptr<rt::Problem> p;
+};
+
+extern "C" {
+
+const char* rt_last_error(void) { return g_last_error.c_str(); }
+
+const char* rt_version(void) {
+    static std::string v;
+    if (v.empty()) {
+        v = "radiant_hip 0.1";
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) {
+            v += std::string(" ") + prop.gcnArchName + " " + prop.name;
+        } else {
+            (void)hipGetLastError();
+            v += " (no device)";
+        }
+    }
+    return v.c_str();
+}
+
+int rt_problem_device_bytes(const rt_problem_desc* p, int64_t* bytes) {
+    if (!p || !bytes) return fail(RT_EINVAL, "null argument");
+    return guarded([&] { *bytes = rt::Problem::device_bytes(*p); });
+}
+
+int rt_problem_create(const rt_problem_desc* p, rt_problem** out) {
+    if (!p || !out) return fail(RT_EINVAL, "null argument");
+    *out = nullptr;
+    return guarded([&] {
+        auto h = std::make_unique<rt_problem>();
+        h->p = std::make_unique<rt::Problem>(*p);
+        *out = h.release();
+    });
+}
+
+int rt_problem_solve(rt_problem* h, rt_group_stats* stats_out, rt_solve_info* info) {
+    if (!h || !h->p) return fail(RT_EINVAL, "null handle");
+    return guarded([&] { h->p->solve(stats_out, info); });
+}
+
+int rt_problem_get_flux(rt_problem* h, double* phi_out) {
+    if (!h || !h->p || !phi_out) return fail(RT_EINVAL, "null argument");
+    return guarded([&] { h->p->get_flux(phi_out); });
+}
+
+int rt_problem_destroy(rt_problem* h) {
+    if (!h) return RT_OK;
+    return guarded([&] { delete h; });
+}
+
+int rt_problem_pass(rt_problem* h, int32_t g, int32_t mode, const double* qlout, const double* z_in, double* z_out) {
+    if (!h || !h->p || !qlout || !z_in || !z_out) return fail(RT_EINVAL, "null argument");
+    return guarded([&] { h->p->host_pass(g, mode, qlout, z_in, z_out); });
+}
+
+int rt_problem_state_length(rt_problem* h, int64_t* S) {
+    if (!h || !h->p || !S) return fail(RT_EINVAL, "null argument");
+    *S = h->p->faithful_state_length();
+    return RT_OK;
+}
+
+int rt_test_contraction(int32_t which, int32_t prec, int32_t M, int32_t K, int32_t N, const double* A, const double* B,
+                        double* C) {
+    if (!A || !B || !C || M < 1 || K < 1 || N < 1 || which < 0 || which > 1 || prec < 0 || prec > 2)
+        return fail(RT_EINVAL, "invalid contraction test arguments");
+    return guarded([&] { rt::test_contraction(which, prec, M, K, N, A, B, C); });
+}
+
+int rt_solve(const rt_problem_desc* p, double* phi_out, rt_group_stats* stats_out) {
+    if (!p || !phi_out) return fail(RT_EINVAL, "null argument");
+    return guarded([&] {
+        rt::Problem prob(*p);
+        prob.solve(stats_out, nullptr);
+        prob.get_flux(phi_out);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,907 @@
+// engine.cpp -- host orchestration of the SN solve on one device.
+//
+// Follows src/particle_transport/sn_flux.jl:150-224 (group loop), sn_one_speed.jl:81-196
+// (in-group solve) and tools/{gmres,anderson,livolant,bicgstab}.jl, with every state vector
+// resident in HBM and the transport pass of sn_inner_pass.jl:212-332 done by the kernels of
+// kernels.hip.
+#include "engine.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+
+namespace rt {
+
+static inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+template <class T>
+T* Problem::dalloc(size_t n) {
+    void* p = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&p, n * sizeof(T));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(RT_ENOMEM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed: " + hipGetErrorString(e));
+    }
+    allocs.push_back(p);
+    return (T*)p;
+}
+
+// Synchronous upload: the host vectors are temporaries (setup cost only).
+template <class T>
+static void upload(T* d, const std::vector<T>& h, hipStream_t st) {
+    RT_CHECK(hipStreamSynchronize(st));
+    if (!h.empty()) RT_CHECK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+// Legendre orders that carry a scattering source: Σs_l != 0 for some material/group pair.
+static std::vector<int> active_moments(const rt_problem_desc& d) {
+    std::vector<bool> lnz(d.Ls + 1, false);
+    const size_t per_l = (size_t)d.Nmat * d.G * d.G;
+    for (int l = 0; l <= d.Ls; ++l)
+        for (size_t i = 0; i < per_l; ++i)
+            if (d.sigs[i + per_l * l] != 0.0) { lnz[l] = true; break; }
+    std::vector<int> act;
+    for (int p = 0; p < d.Np; ++p) {
+        const int64_t l = d.pl[p];
+        if (l == 0 || (l <= d.Ls && lnz[l])) act.push_back(p);
+    }
+    return act;
+}
+
+static void validate(const rt_problem_desc& d) {
+    auto bad = [](const std::string& m) { throw Error(RT_EINVAL, m); };
+    if (d.ndims < 1 || d.ndims > 3) bad("Dimension is not 1, 2 or 3.");
+    if (d.Nx < 1 || d.Ny < 1 || d.Nz < 1) bad("number of voxels must be positive");
+    if (d.ndims < 2 && d.Ny != 1) bad("Ny must be 1 below 2D");
+    if (d.ndims < 3 && d.Nz != 1) bad("Nz must be 1 below 3D");
+    if (!d.dx || (d.ndims >= 2 && !d.dy) || (d.ndims >= 3 && !d.dz)) bad("missing voxel widths");
+    if (!d.mat || !d.sigt || !d.sigs || !d.qvol || !d.mu || !d.eta || !d.xi || !d.w || !d.M || !d.D || !d.pl)
+        bad("missing input array");
+    if (d.Nmat < 1 || d.G < 1 || d.Ls < 0) bad("invalid cross-section extents");
+    if (d.Nd < 1 || d.Np < 1 || d.Nps < 0) bad("invalid angular extents");
+    if (d.accel < RT_ACCEL_NONE || d.accel > RT_ACCEL_BICGSTAB) bad("Unknown acceleration method.");
+    if (d.precision < RT_PREC_FP64 || d.precision > RT_PREC_MIXED_BF16) bad("unknown precision mode");
+    if (d.state_mode != RT_STATE_FAITHFUL && d.state_mode != RT_STATE_REDUCED) bad("unknown state mode");
+    if (d.precision != RT_PREC_FP64 && d.accel != RT_ACCEL_GMRES)
+        bad("mixed precision requires the gmres acceleration (FP64 Arnoldi, iterative refinement)");
+    if (d.restart < 1 && d.accel == RT_ACCEL_GMRES) bad("gmres restart must be >= 1");
+    if (d.maxit < 1) bad("maximum number of iterations must be >= 1");
+    if (!(d.tol > 0)) bad("convergence criterion must be positive");
+    if (!d.bc) bad("missing boundary conditions");
+    for (int f = 0; f < 2 * d.ndims; ++f) {
+        if (d.bc[f] < 0 || d.bc[f] > 2) bad("Unknown boundary condition type.");
+        if (d.bc[f] != 0) bad("only void boundaries are supported by the GPU path (reflective/periodic: not yet)");
+    }
+    const size_t nc = (size_t)d.Nx * d.Ny * d.Nz;
+    for (size_t c = 0; c < nc; ++c)
+        if (d.mat[c] < 1 || d.mat[c] > d.Nmat) bad("material index out of range at voxel " + std::to_string(c));
+    for (int p = 0; p < d.Np; ++p)
+        if (d.pl[p] < 0) bad("negative Legendre order in pl");
+    if (d.state_mode == RT_STATE_FAITHFUL && d.precision == RT_PREC_FP64) {
+        for (int f = 0; f < 2 * d.ndims; ++f)
+            if (!d.Dsurf[f] || !d.n_to_np[f]) bad("faithful state needs the half-range operators of every face");
+    }
+    if (sweep_pencils_per_thread(d.Ny * d.Nz) < 0) bad("transverse grid too large for the sweep kernel (Ny*Nz > 16384)");
+}
+
+Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
+    validate(d);
+    ndims = d.ndims; Nx = d.Nx; Ny = d.Ny; Nz = d.Nz; Nc = Nx * Ny * Nz;
+    G = d.G; Nmat = d.Nmat; Nd = d.Nd; Np = d.Np; Nps = d.Nps;
+    accel = d.accel; restart = d.restart; depth = d.depth; maxit = d.maxit; tol = d.tol;
+    precision = d.precision;
+    state_mode = (d.precision != RT_PREC_FP64) ? RT_STATE_REDUCED : d.state_mode;
+    verbose = d.verbose;
+    act = active_moments(d);
+    Nact = (int)act.size();
+    npos[0] = (size_t)Ny * Nz;
+    npos[1] = ndims >= 2 ? (size_t)Nx * Nz : 0;
+    npos[2] = ndims >= 3 ? (size_t)Nx * Ny : 0;
+    nb = (size_t)2 * Nps * (npos[0] + npos[1] + npos[2]);
+    S_f = (size_t)Np * Nc + nb;
+    S_r = (size_t)Nact * Nc;
+    if (stream) {
+        st = stream;
+    } else {
+        RT_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        own_stream = true;
+    }
+    try {
+        // ---- geometry
+        std::vector<int> mat(Nc);
+        for (int c = 0; c < Nc; ++c) mat[c] = (int)(d.mat[c] - 1);
+        d_mat = dalloc<int>(Nc); upload(d_mat, mat, st);
+        auto up = [&](const double* src, int n) {
+            double* p = dalloc<double>(n);
+            RT_CHECK(hipMemcpy(p, src, n * sizeof(double), hipMemcpyHostToDevice));
+            return p;
+        };
+        std::vector<double> one(1, 1.0);
+        d_dx = up(d.dx, Nx);
+        d_dy = ndims >= 2 ? up(d.dy, Ny) : up(one.data(), 1);
+        d_dz = ndims >= 3 ? up(d.dz, Nz) : up(one.data(), 1);
+        d_mu = up(d.mu, Nd); d_eta = up(d.eta, Nd); d_xi = up(d.xi, Nd);
+        // ---- cross sections (sn_flux.jl:69-83)
+        std::vector<double> sigt((size_t)G * Nmat);
+        for (int g = 0; g < G; ++g)
+            for (int m = 0; m < Nmat; ++m) sigt[(size_t)g * Nmat + m] = d.sigt[g + (size_t)G * m];
+        d_sigt = dalloc<double>(sigt.size()); upload(d_sigt, sigt, st);
+        auto S = [&](int m, int gi, int gf, int64_t l) -> double {
+            if (l > d.Ls) return 0.0;
+            return d.sigs[m + (size_t)Nmat * (gi + (size_t)G * (gf + (size_t)G * l))];
+        };
+        std::vector<double> sigw((size_t)G * Nmat * Nact), sigo((size_t)G * G * Nmat * Nact);
+        for (int g = 0; g < G; ++g)
+            for (int m = 0; m < Nmat; ++m)
+                for (int a = 0; a < Nact; ++a) {
+                    sigw[((size_t)g * Nmat + m) * Nact + a] = S(m, g, g, d.pl[act[a]]);
+                    for (int gi = 0; gi < G; ++gi)
+                        sigo[(((size_t)g * G + gi) * Nmat + m) * Nact + a] = S(m, gi, g, d.pl[act[a]]);
+                }
+        d_sigw = dalloc<double>(sigw.size()); upload(d_sigw, sigw, st);
+        d_sigo = dalloc<double>(sigo.size()); upload(d_sigo, sigo, st);
+        std::vector<double> qv((size_t)G * Nc);
+        for (int c = 0; c < Nc; ++c)
+            for (int g = 0; g < G; ++g) qv[(size_t)g * Nc + c] = d.qvol[g + (size_t)G * c];
+        d_qvol = dalloc<double>(qv.size()); upload(d_qvol, qv, st);
+        // ---- angular operators (row-major device copies)
+        std::vector<double> Mact((size_t)Nd * Nact), D((size_t)Np * Nd), Dact((size_t)Nact * Nd);
+        for (int n = 0; n < Nd; ++n)
+            for (int a = 0; a < Nact; ++a) Mact[(size_t)n * Nact + a] = d.M[n + (size_t)Nd * act[a]];
+        for (int p = 0; p < Np; ++p)
+            for (int n = 0; n < Nd; ++n) D[(size_t)p * Nd + n] = d.D[p + (size_t)Np * n];
+        for (int a = 0; a < Nact; ++a)
+            for (int n = 0; n < Nd; ++n) Dact[(size_t)a * Nd + n] = d.D[act[a] + (size_t)Np * n];
+        d_Mact64 = dalloc<double>(Mact.size()); upload(d_Mact64, Mact, st);
+        d_D64 = dalloc<double>(D.size()); upload(d_D64, D, st);
+        d_Dact64 = dalloc<double>(Dact.size()); upload(d_Dact64, Dact, st);
+        auto tof = [](const std::vector<double>& v) { return std::vector<float>(v.begin(), v.end()); };
+        d_Mact32 = dalloc<float>(Mact.size()); upload(d_Mact32, tof(Mact), st);
+        d_D32 = dalloc<float>(D.size()); upload(d_D32, tof(D), st);
+        d_Dact32 = dalloc<float>(Dact.size()); upload(d_Dact32, tof(Dact), st);
+        d_act = dalloc<int>(Nact); upload(d_act, act, st);
+        // ---- exit-face half-range D columns (sn_inner_pass.jl:256-317 selection)
+        if (state_mode == RT_STATE_FAITHFUL) {
+            for (int ax = 0; ax < ndims; ++ax) {
+                std::vector<double> Dex((size_t)Nd * Nps, 0.0);
+                std::vector<int> ef(Nd, 0);
+                const double* cos_ = ax == 0 ? d.mu : (ax == 1 ? d.eta : d.xi);
+                const int fm = 2 * ax, fp = 2 * ax + 1;
+                for (int n = 0; n < Nd; ++n) {
+                    ef[n] = cos_[n] >= 0 ? 1 : 0;
+                    int f = -1, col = -1;
+                    if (d.n_to_np[fm][n] != 0) { f = fm; col = (int)d.n_to_np[fm][n] - 1; }
+                    else if (d.n_to_np[fp][n] != 0) { f = fp; col = (int)d.n_to_np[fp][n] - 1; }
+                    if (f >= 0)
+                        for (int p = 0; p < Nps; ++p) Dex[(size_t)n * Nps + p] = d.Dsurf[f][p + (size_t)Nps * col];
+                }
+                d_Dexit[ax] = dalloc<double>(Dex.size()); upload(d_Dexit[ax], Dex, st);
+                d_exitf[ax] = dalloc<int>(Nd); upload(d_exitf[ax], ef, st);
+            }
+        }
+        // ---- fluxes and work buffers
+        d_phi = dalloc<double>((size_t)G * Np * Nc);
+        k_fill(d_phi, 0.0, (size_t)G * Np * Nc, st);
+        d_qlout = dalloc<double>((size_t)Nact * Nc);
+        d_psi = dalloc<double>((size_t)Nd * Nc);
+        for (int ax = 0; ax < 3; ++ax) d_bex[ax] = dalloc<double>(std::max<size_t>(1, (size_t)Nd * (ax < ndims ? npos[ax] : 1)));
+        d_bscratch = dalloc<double>(std::max<size_t>(1, nb));
+        d_pd = dalloc<PassDesc>(1);
+        // ---- work tables
+        auto gemm_tasks = [&](int rows, int& ntask) {
+            std::vector<GemmTask> t;
+            const int tiles = ceil_div(Nc, GEMM_CELLS), rg = ceil_div(rows, GEMM_ROWS);
+            for (int r = 0; r < rg; ++r)
+                for (int i = 0; i < tiles; ++i) t.push_back({0, i, r, 0});
+            ntask = (int)t.size();
+            GemmTask* p = dalloc<GemmTask>(t.size());
+            upload(p, t, st);
+            return p;
+        };
+        d_mg = gemm_tasks(Nd, n_mg);
+        d_dgf = gemm_tasks(Np, n_dgf);
+        d_dga = gemm_tasks(Nact, n_dga);
+        std::vector<SweepTask> sw(Nd);
+        for (int n = 0; n < Nd; ++n) sw[n] = {0, n};
+        d_sw = dalloc<SweepTask>(Nd); upload(d_sw, sw, st);
+        const int P = Ny * Nz;
+        sw_ppt = sweep_pencils_per_thread(P);
+        sw_block = std::min(1024, ((ceil_div(P, sw_ppt) + 63) / 64) * 64);
+        for (int pr = 0; pr < 3; ++pr) {
+            // FP64 passes are always used (residuals, reconstruction); low precision only in mixed modes
+            const bool needed = (pr == P64) || (precision != RT_PREC_FP64);
+            int nbuf = 2;
+            size_t lds = sweep_lds_bytes(pr, Nx, Ny, Nz, Nmat, nbuf);
+            if (lds > 160 * 1024) { nbuf = 1; lds = sweep_lds_bytes(pr, Nx, Ny, Nz, Nmat, nbuf); }
+            if (lds > 160 * 1024 && needed)
+                throw Error(RT_EINVAL, "sweep face buffers exceed the 160 KiB of LDS for this grid");
+            sw_nbuf[pr] = nbuf;
+            sw_lds[pr] = lds;
+        }
+        // ---- reductions
+        rw.partial = dalloc<double>(RED_BLOCKS);
+        rw.counter = dalloc<unsigned>(1);
+        rw.scal = dalloc<double>(128);
+        RT_CHECK(hipMemsetAsync(rw.counter, 0, sizeof(unsigned), st));
+        RT_CHECK(hipHostMalloc((void**)&h_scal, 128 * sizeof(double), hipHostMallocDefault));
+        d_Vptr = dalloc<double*>(64);
+        d_y = dalloc<double>(64);
+        RT_CHECK(hipStreamSynchronize(st));
+    } catch (...) {
+        for (void* p : allocs) (void)hipFree(p);
+        allocs.clear();
+        if (h_scal) (void)hipHostFree(h_scal);
+        if (own_stream) (void)hipStreamDestroy(st);
+        throw;
+    }
+}
+
+Problem::~Problem() {
+    (void)hipStreamSynchronize(st);
+    for (auto& kv : pool) (void)hipFree(kv.second);
+    for (void* p : allocs) (void)hipFree(p);
+    if (h_scal) (void)hipHostFree(h_scal);
+    if (own_stream) (void)hipStreamDestroy(st);
+}
+
+int64_t Problem::device_bytes(const rt_problem_desc& d) {
+    const int64_t Nc = (int64_t)d.Nx * d.Ny * d.Nz;
+    const int Nact_est = d.Np;  // upper bound without scanning the XS
+    int64_t b = 0;
+    b += Nc * 4 + (int64_t)d.G * Nc * 8;                        // mat, qvol
+    b += (int64_t)d.G * d.Np * Nc * 8;                          // fluxes
+    b += (int64_t)Nact_est * Nc * 8;                            // qlout
+    b += (int64_t)d.Nd * Nc * 8;                                // psi
+    b += (int64_t)3 * d.Nd * d.Np * 12;                         // operators
+    const int64_t nposs = (int64_t)d.Ny * d.Nz + (d.ndims >= 2 ? (int64_t)d.Nx * d.Nz : 0) +
+                          (d.ndims >= 3 ? (int64_t)d.Nx * d.Ny : 0);
+    b += (int64_t)d.Nd * nposs * 8 + 2LL * d.Nps * nposs * 8;
+    const bool faithful = d.precision == RT_PREC_FP64 && d.state_mode == RT_STATE_FAITHFUL;
+    const int64_t S = faithful ? (int64_t)d.Np * Nc + 2LL * d.Nps * nposs : (int64_t)Nact_est * Nc;
+    int nvec = 8;
+    if (d.accel == RT_ACCEL_GMRES) nvec = d.restart + 6;
+    if (d.accel == RT_ACCEL_ANDERSON) nvec = 2 * d.depth + 8;
+    if (d.precision != RT_PREC_FP64) nvec += 6;
+    b += (int64_t)nvec * S * 8;
+    return b;
+}
+
+double* Problem::vec(const std::string& name, size_t n) {
+    const std::string key = name + "#" + std::to_string(n);
+    auto it = pool.find(key);
+    if (it != pool.end()) return it->second;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, n * sizeof(double));
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(RT_ENOMEM, "hipMalloc of a Krylov vector (" + std::to_string(n * 8) + " bytes) failed");
+    }
+    pool[key] = (double*)p;
+    return (double*)p;
+}
+
+double Problem::dot(const double* a, const double* b, size_t n) {
+    k_dot(rw, 0, a, b, n, st);
+    RT_CHECK(hipMemcpyAsync(h_scal, rw.scal, sizeof(double), hipMemcpyDeviceToHost, st));
+    RT_CHECK(hipStreamSynchronize(st));
+    return h_scal[0];
+}
+
+// ------------------------------------------------------------------------------------------
+// Transport passes
+// ------------------------------------------------------------------------------------------
+void Problem::fill_geometry(PassDesc& pd, int g) {
+    std::memset(&pd, 0, sizeof(pd));
+    pd.ndims = ndims; pd.Nx = Nx; pd.Ny = Ny; pd.Nz = Nz; pd.Nc = Nc; pd.Nd = Nd; pd.Nmat = Nmat;
+    pd.mat = d_mat; pd.dx = d_dx; pd.dy = d_dy; pd.dz = d_dz; pd.mu = d_mu; pd.eta = d_eta; pd.xi = d_xi;
+    pd.sigt = d_sigt + (size_t)g * Nmat;
+    pd.Nact = Nact;
+    pd.sigw = d_sigw + (size_t)g * Nmat * Nact;
+    pd.psi = d_psi;
+    pd.bx = d_bex[0]; pd.by = d_bex[1]; pd.bz = d_bex[2];
+    pd.Nps = Nps;
+}
+
+void Problem::launch_pass(PassDesc& pd, int prec, bool full_rows) {
+    k_set_pd(pd, d_pd, st);
+    launch_mgemm(prec, d_pd, d_mg, n_mg, st);
+    launch_sweep(prec == PBF16 ? P32 : prec, d_pd, d_sw, Nd, sw_ppt, sw_nbuf[prec], sw_block, sw_lds[prec], st);
+    launch_dgemm(prec, d_pd, full_rows ? d_dgf : d_dga, full_rows ? n_dgf : n_dga, st);
+    if (pd.bout_x) launch_bndmom(d_pd, 1, (int)nb, st);
+    counters.passes += 1;
+    if (prec != P64) counters.passes_lo += 1;
+    counters.updates_raw += (double)Nc * Nd;
+}
+
+// Faithful state [Φ (Np x Nc) | boundary moments] (sn_one_speed.jl:126-134), FP64.
+void Problem::pass_faithful(int g, int prec, int mode, const double* zin, double* zout) {
+    PassDesc pd;
+    fill_geometry(pd, g);
+    pd.Mact = d_Mact64;
+    pd.qlout = (mode == MODE_T) ? d_qlout : nullptr;
+    pd.phi_in = zin;  // null = zero state
+    pd.phi_rows = d_act;
+    pd.Nrows = Np;
+    pd.Drows = d_D64;
+    pd.out = zout;
+    pd.zin = (mode == MODE_IMA) ? zin : nullptr;
+    double* b = zout + (size_t)Np * Nc;
+    pd.bout_x = b;
+    pd.bout_y = ndims >= 2 ? b + 2 * (size_t)Nps * npos[0] : nullptr;
+    pd.bout_z = ndims >= 3 ? b + 2 * (size_t)Nps * (npos[0] + npos[1]) : nullptr;
+    if (mode == MODE_IMA) {
+        const double* bi = zin + (size_t)Np * Nc;
+        pd.bzin_x = bi;
+        pd.bzin_y = ndims >= 2 ? bi + 2 * (size_t)Nps * npos[0] : nullptr;
+        pd.bzin_z = ndims >= 3 ? bi + 2 * (size_t)Nps * (npos[0] + npos[1]) : nullptr;
+    }
+    pd.Dexit_x = d_Dexit[0]; pd.Dexit_y = d_Dexit[1]; pd.Dexit_z = d_Dexit[2];
+    pd.exitf_x = d_exitf[0]; pd.exitf_y = d_exitf[1]; pd.exitf_z = d_exitf[2];
+    if (nb == 0) pd.bout_x = nullptr;
+    (void)prec;
+    launch_pass(pd, P64, true);
+}
+
+// Reduced state [Φ_act (Nact x Nc)]: only the moments that feed back into the next pass.
+void Problem::pass_reduced(int g, int prec, int mode, const double* zin, double* zout) {
+    PassDesc pd;
+    fill_geometry(pd, g);
+    pd.Mact = (prec == P64) ? (const void*)d_Mact64 : (const void*)d_Mact32;
+    pd.qlout = (mode == MODE_T) ? d_qlout : nullptr;
+    pd.phi_in = zin;
+    pd.phi_rows = nullptr;
+    pd.Nrows = Nact;
+    pd.Drows = (prec == P64) ? (const void*)d_Dact64 : (const void*)d_Dact32;
+    pd.out = zout;
+    pd.zin = (mode == MODE_IMA) ? zin : nullptr;
+    launch_pass(pd, prec, false);
+}
+
+// Final reconstruction pass 𝚽l = T(z) from a reduced state, all Np moments in FP64.
+void Problem::pass_reconstruct(int g, const double* zin_reduced) {
+    PassDesc pd;
+    fill_geometry(pd, g);
+    pd.Mact = d_Mact64;
+    pd.qlout = d_qlout;
+    pd.phi_in = zin_reduced;
+    pd.phi_rows = nullptr;
+    pd.Nrows = Np;
+    pd.Drows = d_D64;
+    pd.out = d_phi + (size_t)g * Np * Nc;
+    pd.zin = nullptr;
+    launch_pass(pd, P64, true);
+}
+
+// ------------------------------------------------------------------------------------------
+// Krylov solvers (tools/gmres.jl, anderson.jl, livolant.jl, bicgstab.jl)
+// ------------------------------------------------------------------------------------------
+static double rho_estimate(double r0, double r, int64_t it) {  // krylov_state.jl:139-144
+    if (it > 1 && r0 > 0.0 && r > 0.0) return std::pow(r / r0, 1.0 / (double)(it - 1));
+    return std::numeric_limits<double>::quiet_NaN();
+}
+
+// tools/gmres.jl:35-133 -- restarted GMRES(m), modified Gram-Schmidt, Givens rotations.
+// The MGS chain runs on the device (each kernel reads its coefficient from a scalar slot and
+// reduces the next dot product); the host reads one H column per Arnoldi step.
+template <class MV>
+void Problem::gmres(size_t n, double* z, MV matvec, const double* c, int m, int64_t maxit_, double tol_,
+                    int64_t& iter, double& resid, bool& conv, double& rho, const std::string& tag) {
+    m = std::max(1, m);
+    if (m + 1 > 64) throw Error(RT_EINVAL, "gmres restart too large (max 63)");
+    const double bnorm = norm(c, n);
+    if (bnorm == 0.0) {
+        k_fill(z, 0.0, n, st);
+        iter = 0; resid = 0.0; conv = true; rho = std::numeric_limits<double>::quiet_NaN();
+        return;
+    }
+    std::vector<double*> V(m + 1);
+    for (int i = 0; i <= m; ++i) V[i] = vec(tag + "V" + std::to_string(i), n);
+    double* w = vec(tag + "w", n);
+    double* r = vec(tag + "r", n);
+    RT_CHECK(hipMemcpyAsync(d_Vptr, V.data(), (m + 1) * sizeof(double*), hipMemcpyHostToDevice, st));
+    std::vector<double> H((size_t)(m + 1) * m, 0.0), cs(m, 0.0), sn(m, 0.0), gv(m + 1, 0.0);
+    auto h = [&](int i, int j) -> double& { return H[i + (size_t)(m + 1) * j]; };
+    iter = 0; resid = 1.0; conv = false;
+    double resid0 = std::numeric_limits<double>::quiet_NaN();
+    while (iter < maxit_ && !conv) {
+        matvec(r, z); ++iter;
+        k_neg_add(r, c, n, st);  // r = c - r  (state_scale!(-1) ; state_axpy!(1, c))
+        const double beta = norm(r, n);
+        resid = beta / bnorm;
+        if (std::isnan(resid0)) resid0 = resid;
+        if (resid < tol_) { conv = true; break; }
+        k_copy(V[0], r, n, st);
+        k_scale(V[0], 1.0 / beta, n, st);
+        std::fill(gv.begin(), gv.end(), 0.0);
+        gv[0] = beta;
+        int k = 0;
+        for (int j = 0; j < m; ++j) {
+            if (iter >= maxit_) break;
+            matvec(w, V[j]); ++iter;
+            k_dot(rw, 0, w, V[0], n, st);
+            for (int i = 0; i <= j; ++i) k_mgs(rw, i, i + 1, w, V[i], i < j ? V[i + 1] : nullptr, n, st);
+            k_scale_by_inv_slot(V[j + 1], w, rw.scal, j + 1, n, st);
+            RT_CHECK(hipMemcpyAsync(h_scal, rw.scal, (j + 2) * sizeof(double), hipMemcpyDeviceToHost, st));
+            RT_CHECK(hipStreamSynchronize(st));
+            for (int i = 0; i <= j; ++i) h(i, j) = h_scal[i];
+            const double hjp1 = std::sqrt(h_scal[j + 1]);
+            h(j + 1, j) = hjp1;
+            for (int i = 0; i < j; ++i) {
+                const double t = cs[i] * h(i, j) + sn[i] * h(i + 1, j);
+                h(i + 1, j) = -sn[i] * h(i, j) + cs[i] * h(i + 1, j);
+                h(i, j) = t;
+            }
+            const double den = std::hypot(h(j, j), h(j + 1, j));
+            if (den == 0.0) { cs[j] = 1.0; sn[j] = 0.0; }
+            else { cs[j] = h(j, j) / den; sn[j] = h(j + 1, j) / den; }
+            h(j, j) = cs[j] * h(j, j) + sn[j] * h(j + 1, j);
+            h(j + 1, j) = 0.0;
+            gv[j + 1] = -sn[j] * gv[j];
+            gv[j] = cs[j] * gv[j];
+            k = j + 1;
+            resid = std::fabs(gv[j + 1]) / bnorm;
+            if (resid < tol_) { conv = true; break; }
+            if (hjp1 == 0.0) break;
+        }
+        std::vector<double> y(k, 0.0);
+        for (int i = k - 1; i >= 0; --i) {
+            double s = gv[i];
+            for (int l = i + 1; l < k; ++l) s -= h(i, l) * y[l];
+            y[i] = h(i, i) != 0.0 ? s / h(i, i) : 0.0;
+        }
+        for (int i0 = 0; i0 < k; i0 += 32) {
+            const int kk = std::min(32, k - i0);
+            Vals32 v{};
+            for (int t = 0; t < kk; ++t) v.v[t] = y[i0 + t];
+            k_set_vals(v, d_y, kk, st);
+            k_lincomb(z, (const double* const*)(d_Vptr + i0), d_y, kk, n, st);
+        }
+        if (conv) break;
+    }
+    rho = rho_estimate(resid0, resid, iter);
+}
+
+// tools/anderson.jl:38-140 (beta = 1)
+template <class FP>
+void Problem::anderson(size_t n, double* z, FP fixedpoint, int64_t& iter, double& resid, bool& conv, double& rho) {
+    const int m = std::max(1, depth);
+    double* g = vec("ag", n);
+    double* f = vec("af", n);
+    double* fo = vec("afo", n);
+    double* go = vec("ago", n);
+    std::vector<double*> dF, dG;
+    iter = 0; resid = 1.0; conv = false;
+    double resid0 = std::numeric_limits<double>::quiet_NaN();
+    for (int64_t it = 1; it <= maxit; ++it) {
+        fixedpoint(g, z); ++iter;
+        k_copy(f, g, n, st);
+        k_axpy(-1.0, z, f, n, st);
+        const double fnorm = norm(f, n);
+        const double sc = std::max(std::max(norm(z, n), norm(g, n)), 1e-16);
+        resid = fnorm / sc;
+        if (std::isnan(resid0)) resid0 = resid;
+        if (resid < tol) { conv = true; break; }
+        if (it == 1) {
+            k_axpy(1.0, f, z, n, st);
+        } else {
+            // history of depth m (push!, then popfirst! when longer than m): reuse the oldest buffers
+            double *a, *b;
+            if ((int)dF.size() < m) {
+                a = vec("adF" + std::to_string(dF.size()), n);
+                b = vec("adG" + std::to_string(dG.size()), n);
+            } else {
+                a = dF.front(); b = dG.front();
+                dF.erase(dF.begin()); dG.erase(dG.begin());
+            }
+            dF.push_back(a); dG.push_back(b);
+            k_copy(a, f, n, st); k_axpy(-1.0, fo, a, n, st);
+            k_copy(b, g, n, st); k_axpy(-1.0, go, b, n, st);
+            const int mk = (int)dF.size();
+            std::vector<double> Nm((size_t)mk * mk), bb(mk);
+            for (int i = 0; i < mk; ++i) {
+                bb[i] = dot(dF[i], f, n);
+                for (int j = 0; j < mk; ++j) Nm[i + (size_t)mk * j] = dot(dF[i], dF[j], n);
+            }
+            double tr = 0.0;
+            for (int i = 0; i < mk; ++i) tr += Nm[i + (size_t)mk * i];
+            const double lam = 1e-12 * (tr / mk + 1e-300);
+            for (int i = 0; i < mk; ++i) Nm[i + (size_t)mk * i] += lam;
+            // anderson_solve: Gaussian elimination with partial pivoting (anderson.jl:109-140)
+            std::vector<double> M = Nm, x = bb;
+            for (int kq = 0; kq < mk; ++kq) {
+                int p = kq;
+                double mx = std::fabs(M[kq + (size_t)mk * kq]);
+                for (int i = kq + 1; i < mk; ++i)
+                    if (std::fabs(M[i + (size_t)mk * kq]) > mx) { mx = std::fabs(M[i + (size_t)mk * kq]); p = i; }
+                if (p != kq) {
+                    for (int j = kq; j < mk; ++j) std::swap(M[kq + (size_t)mk * j], M[p + (size_t)mk * j]);
+                    std::swap(x[kq], x[p]);
+                }
+                const double piv = M[kq + (size_t)mk * kq];
+                if (piv == 0.0) continue;
+                for (int i = kq + 1; i < mk; ++i) {
+                    const double fct = M[i + (size_t)mk * kq] / piv;
+                    for (int j = kq; j < mk; ++j) M[i + (size_t)mk * j] -= fct * M[kq + (size_t)mk * j];
+                    x[i] -= fct * x[kq];
+                }
+            }
+            for (int kq = mk - 1; kq >= 0; --kq) {
+                double s = x[kq];
+                for (int j = kq + 1; j < mk; ++j) s -= M[kq + (size_t)mk * j] * x[j];
+                x[kq] = M[kq + (size_t)mk * kq] != 0.0 ? s / M[kq + (size_t)mk * kq] : 0.0;
+            }
+            k_axpy(1.0, f, z, n, st);
+            // z <- z + f - sum gamma_i dG_i  (the (beta-1) dF term vanishes for beta = 1)
+            for (int i = 0; i < mk; ++i) k_axpy(-x[i], dG[i], z, n, st);
+        }
+        k_copy(fo, f, n, st);
+        k_copy(go, g, n, st);
+    }
+    rho = rho_estimate(resid0, resid, iter);
+}
+
+// tools/livolant.jl:115-155 (period < 0: plain source iteration, the "none" option)
+template <class FP>
+void Problem::livolant(size_t n, double* z, FP fixedpoint, int period, int64_t& iter, double& resid, bool& conv,
+                       double& rho) {
+    double* g = vec("lg", n);
+    double* f = vec("lf", n);
+    double* z1 = vec("lz1", n);
+    double* z0 = vec("lz0", n);
+    k_fill(z1, 0.0, n, st);
+    k_fill(z0, 0.0, n, st);
+    int nhist = 0;
+    iter = 0; resid = 1.0; conv = false;
+    double resid0 = std::numeric_limits<double>::quiet_NaN();
+    for (int64_t it = 1; it <= maxit; ++it) {
+        fixedpoint(g, z); ++iter;
+        k_copy(f, g, n, st);
+        k_axpy(-1.0, z, f, n, st);
+        const double fnorm = norm(f, n);
+        const double sc = std::max(norm(g, n), 1e-16);
+        resid = fnorm / sc;
+        if (std::isnan(resid0)) resid0 = resid;
+        if (resid < tol) { k_copy(z, g, n, st); conv = true; break; }
+        if (period > 0 && it % period == 0 && nhist >= 2) {
+            // livolant_factor (livolant.jl:69-81): e0 = z1 - z0, e1 = g - z1, Δe = e1 - e0
+            double* e0 = vec("le0", n);
+            double* de = vec("lde", n);
+            k_sub(e0, z1, z0, n, st);
+            k_sub(de, g, z1, n, st);
+            k_axpy(-1.0, e0, de, n, st);
+            const double se = dot(e0, de, n), sd = dot(de, de, n);
+            double mu = 1.0;
+            if (sd != 0.0) { mu = -se / sd; if (mu <= 0) mu = 1.0; }
+            k_copy(z, g, n, st);
+            k_scale(z, mu, n, st);
+            k_axpy(1.0 - mu, z1, z, n, st);
+        } else {
+            k_copy(z, g, n, st);
+        }
+        k_copy(z0, z1, n, st);
+        k_copy(z1, g, n, st);
+        ++nhist;
+    }
+    rho = rho_estimate(resid0, resid, iter);
+}
+
+// tools/bicgstab.jl:30-101
+template <class MV>
+void Problem::bicgstab(size_t n, double* z, MV matvec, const double* c, int64_t& iter, double& resid, bool& conv,
+                       double& rho) {
+    const double bnorm = norm(c, n);
+    if (bnorm == 0.0) {
+        k_fill(z, 0.0, n, st);
+        iter = 0; resid = 0.0; conv = true; rho = std::numeric_limits<double>::quiet_NaN();
+        return;
+    }
+    double *r = vec("br", n), *r0 = vec("br0", n), *p = vec("bp", n), *v = vec("bv", n), *s = vec("bs", n),
+           *t = vec("bt", n);
+    matvec(r, z);
+    k_neg_add(r, c, n, st);
+    iter = 1;
+    resid = norm(r, n) / bnorm;
+    const double resid0 = resid;
+    conv = false;
+    if (resid < tol) { conv = true; rho = std::numeric_limits<double>::quiet_NaN(); return; }
+    k_copy(r0, r, n, st);
+    double rho_old = 1.0, al = 1.0, om = 1.0;
+    const double tiny = 1e-300;
+    k_fill(p, 0.0, n, st);
+    k_fill(v, 0.0, n, st);
+    while (iter < maxit) {
+        const double rn = dot(r0, r, n);
+        if (std::fabs(rn) < tiny) break;
+        const double be = (rn / rho_old) * (al / om);
+        k_axpy(-om, v, p, n, st);
+        k_scale(p, be, n, st);
+        k_axpy(1.0, r, p, n, st);
+        matvec(v, p); ++iter;
+        const double den = dot(r0, v, n);
+        if (std::fabs(den) < tiny) break;
+        al = rn / den;
+        k_copy(s, r, n, st);
+        k_axpy(-al, v, s, n, st);
+        if (norm(s, n) / bnorm < tol) {
+            k_axpy(al, p, z, n, st);
+            resid = norm(s, n) / bnorm;
+            conv = true;
+            break;
+        }
+        matvec(t, s); ++iter;
+        const double tt = dot(t, t, n);
+        if (tt < tiny) break;
+        om = dot(t, s, n) / tt;
+        k_axpy(al, p, z, n, st);
+        k_axpy(om, s, z, n, st);
+        k_copy(r, s, n, st);
+        k_axpy(-om, t, r, n, st);
+        resid = norm(r, n) / bnorm;
+        if (resid < tol) { conv = true; break; }
+        if (std::fabs(om) < tiny) break;
+        rho_old = rn;
+    }
+    rho = rho_estimate(resid0, resid, iter);
+}
+
+// ------------------------------------------------------------------------------------------
+// In-group solves
+// ------------------------------------------------------------------------------------------
+static void print_group(int g, const rt_group_stats& s) {
+    std::printf(">>>Group %d has %sconverged ( ϵ = %.4E , N = %lld , ρ = %.4f )\n", g + 1, s.converged ? "" : "not ",
+                s.resid, (long long)s.niter, s.rho);
+    std::fflush(stdout);
+}
+
+// sn_one_speed.jl:81-196 with the state of sn_one_speed.jl:126-134 (faithful) or the active
+// moments only (reduced); FP64 passes.
+void Problem::solve_group_plain(int g, rt_group_stats& s) {
+    const bool faithful = state_mode == RT_STATE_FAITHFUL;
+    const size_t n = faithful ? S_f : S_r;
+    const int64_t p0 = counters.passes;
+    auto T = [&](double* out, const double* in) {
+        if (faithful) pass_faithful(g, P64, MODE_T, in, out);
+        else pass_reduced(g, P64, MODE_T, in, out);
+    };
+    auto MV = [&](double* out, const double* in) {
+        if (faithful) pass_faithful(g, P64, MODE_IMA, in, out);
+        else pass_reduced(g, P64, MODE_IMA, in, out);
+    };
+    double* z = vec("z", n);
+    int64_t niter = 0;
+    double resid = 1.0, rho = std::numeric_limits<double>::quiet_NaN();
+    bool conv = false;
+    if (accel == RT_ACCEL_NONE || accel == RT_ACCEL_LIVOLANT || accel == RT_ACCEL_ANDERSON) {
+        // z = state_clone(work): the group flux enters as the initial guess (zero in the single
+        // Gauss-Seidel pass), boundary moments zero.
+        k_fill(z, 0.0, n, st);
+        if (accel == RT_ACCEL_ANDERSON) anderson(n, z, T, niter, resid, conv, rho);
+        else livolant(n, z, T, accel == RT_ACCEL_NONE ? -1 : 3, niter, resid, conv, rho);
+    } else {
+        double* c = vec("c", n);
+        T(c, nullptr);  // c = T(0)
+        k_fill(z, 0.0, n, st);
+        if (accel == RT_ACCEL_GMRES) gmres(n, z, MV, c, restart, maxit, tol, niter, resid, conv, rho, "g");
+        else bicgstab(n, z, MV, c, niter, resid, conv, rho);
+    }
+    // reconstruction pass 𝚽l = T(z) (sn_one_speed.jl:186)
+    if (faithful) {
+        double* out = vec("recon", n);
+        pass_faithful(g, P64, MODE_T, z, out);
+        k_copy(d_phi + (size_t)g * Np * Nc, out, (size_t)Np * Nc, st);
+    } else {
+        pass_reconstruct(g, z);
+    }
+    s.niter = niter; s.resid = resid; s.rho = rho; s.converged = conv ? 1 : 0;
+    s.passes = (int32_t)(counters.passes - p0);
+    s.passes_lo = 0; s.refinements = 0;
+}
+
+// Mixed precision: GMRES with FP64 Arnoldi on low-precision passes, wrapped in FP64 iterative
+// refinement of (I - A) z = c; final 𝚽l = T(z) in FP64.
+void Problem::solve_group_mixed(int g, rt_group_stats& s) {
+    const size_t n = S_r;
+    const int lo = (precision == RT_PREC_MIXED_BF16) ? PBF16 : P32;
+    const double inner_tol = (lo == PBF16) ? 3e-3 : 1e-5;
+    const int max_ref = 12;
+    const int64_t p0 = counters.passes, l0 = counters.passes_lo;
+    double* c = vec("c", n);
+    double* z = vec("z", n);
+    double* r = vec("r", n);
+    double* dz = vec("dz", n);
+    double* t = vec("t", n);
+    pass_reduced(g, P64, MODE_T, nullptr, c);  // c = T(0) in FP64
+    const double bnorm = norm(c, n);
+    int64_t niter = 0;
+    double resid = 0.0;
+    bool conv = true;
+    int refs = 0;
+    k_fill(z, 0.0, n, st);
+    if (bnorm > 0.0) {
+        k_copy(r, c, n, st);
+        resid = 1.0;
+        conv = false;
+        auto MVlo = [&](double* out, const double* in) { pass_reduced(g, lo, MODE_IMA, in, out); };
+        while (true) {
+            if (resid < tol) { conv = true; break; }
+            if (refs >= max_ref || niter >= maxit) break;
+            k_fill(dz, 0.0, n, st);
+            int64_t it = 0;
+            double rr, rh;
+            bool cv;
+            gmres(n, dz, MVlo, r, restart, maxit - niter, inner_tol, it, rr, cv, rh, "m");
+            niter += it;
+            k_axpy(1.0, dz, z, n, st);
+            pass_reduced(g, P64, MODE_IMA, z, t);  // t = z - A z in FP64
+            k_sub(r, c, t, n, st);                  // r = c - (I - A) z
+            resid = norm(r, n) / bnorm;
+            ++refs;
+        }
+    }
+    pass_reconstruct(g, z);
+    s.niter = niter; s.resid = resid; s.rho = std::numeric_limits<double>::quiet_NaN();
+    s.converged = conv ? 1 : 0;
+    s.passes = (int32_t)(counters.passes - p0);
+    s.passes_lo = (int32_t)(counters.passes_lo - l0);
+    s.refinements = refs;
+}
+
+void Problem::solve_group(int g, rt_group_stats& s) {
+    std::memset(&s, 0, sizeof(s));
+    // out-of-group + volume source (sn_flux.jl:175-179)
+    launch_outsrc(d_qlout, d_phi, d_sigo, d_qvol, d_mat, d_act, g, G, Nmat, Nact, Np, Nc, st);
+    // zero-source shortcut (sn_one_speed.jl:116-120)
+    if (dot(d_qlout, d_qlout, (size_t)Nact * Nc) == 0.0) {
+        k_fill(d_phi + (size_t)g * Np * Nc, 0.0, (size_t)Np * Nc, st);
+        s.niter = 0; s.resid = 0.0; s.rho = std::numeric_limits<double>::quiet_NaN(); s.converged = 1; s.passes = 0;
+        if (verbose) print_group(g, s);
+        return;
+    }
+    if (precision == RT_PREC_FP64) solve_group_plain(g, s);
+    else solve_group_mixed(g, s);
+    if (verbose) print_group(g, s);
+}
+
+void Problem::solve(rt_group_stats* stats, rt_solve_info* info) {
+    hipEvent_t e0, e1;
+    RT_CHECK(hipEventCreate(&e0));
+    RT_CHECK(hipEventCreate(&e1));
+    counters = SolveCounters{};
+    k_fill(d_phi, 0.0, (size_t)G * Np * Nc, st);
+    RT_CHECK(hipEventRecord(e0, st));
+    for (int g = 0; g < G; ++g) {
+        rt_group_stats s;
+        solve_group(g, s);
+        if (stats) stats[g] = s;
+    }
+    RT_CHECK(hipEventRecord(e1, st));
+    RT_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    RT_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        info->ms_device = ms;
+        info->passes = counters.passes;
+        info->passes_lo = counters.passes_lo;
+        info->updates_raw = counters.updates_raw;
+        info->updates_problem = (double)Nc * Nd * G;
+    }
+}
+
+void Problem::get_flux(double* phi_out) {
+    const size_t tot = (size_t)G * Np * Nc;
+    double* tmp = vec("flux_out", tot);
+    k_phi_to_radiant(tmp, d_phi, G, Np, Nc, st);
+    RT_CHECK(hipMemcpyAsync(phi_out, tmp, tot * sizeof(double), hipMemcpyDeviceToHost, st));
+    RT_CHECK(hipStreamSynchronize(st));
+}
+
+void Problem::host_pass(int g, int mode, const double* qlout_host, const double* zin, double* zout) {
+    if (g < 0 || g >= G) throw Error(RT_EINVAL, "group out of range");
+    if (mode < 0 || mode > 2) throw Error(RT_EINVAL, "pass mode must be 0 (T), 1 (A) or 2 (I - A)");
+    if (state_mode != RT_STATE_FAITHFUL) throw Error(RT_EINVAL, "rt_problem_pass needs a faithful FP64 handle");
+    const size_t n = S_f;
+    double* a = vec("hp_in", n);
+    double* b = vec("hp_out", n);
+    double* tmp = vec("hp_tmp", n);
+    RT_CHECK(hipMemcpyAsync(tmp, zin, n * sizeof(double), hipMemcpyHostToDevice, st));
+    k_state_moments_in(a, tmp, Np, Nc, st);
+    k_copy(a + (size_t)Np * Nc, tmp + (size_t)Np * Nc, nb, st);
+    // qlout [Np][Nc] host (Np fastest) -> active rows; the other rows must be zero, as the group
+    // loop produces them (their Σs_l vanish and the volume source only feeds p = 1)
+    {
+        std::vector<bool> is_act(Np, false);
+        for (int a2 : act) is_act[a2] = true;
+        for (size_t c = 0; c < (size_t)Nc; ++c)
+            for (int p = 0; p < Np; ++p)
+                if (!is_act[p] && qlout_host[p + (size_t)Np * c] != 0.0)
+                    throw Error(RT_EINVAL, "qlout is nonzero on a moment that carries no scattering source");
+    }
+    double* q = vec("hp_q", (size_t)Np * Nc);
+    RT_CHECK(hipMemcpyAsync(tmp, qlout_host, (size_t)Np * Nc * sizeof(double), hipMemcpyHostToDevice, st));
+    k_state_moments_in(q, tmp, Np, Nc, st);
+    for (int a2 = 0; a2 < Nact; ++a2)
+        RT_CHECK(hipMemcpyAsync(d_qlout + (size_t)a2 * Nc, q + (size_t)act[a2] * Nc, Nc * sizeof(double),
+                                hipMemcpyDeviceToDevice, st));
+    pass_faithful(g, P64, mode == 2 ? MODE_IMA : (mode == 1 ? MODE_A : MODE_T), a, b);
+    k_state_moments_out(tmp, b, Np, Nc, st);
+    k_copy(tmp + (size_t)Np * Nc, b + (size_t)Np * Nc, nb, st);
+    RT_CHECK(hipMemcpyAsync(zout, tmp, n * sizeof(double), hipMemcpyDeviceToHost, st));
+    RT_CHECK(hipStreamSynchronize(st));
+}
+
+void test_contraction(int which, int prec, int M, int K, int N, const double* A, const double* B, double* C) {
+    // which 0: psi = A . B through the moment->direction kernel (B enters as the source operand)
+    // which 1: C = A . B through the direction->moment kernel
+    std::vector<void*> mem;
+    auto al = [&](size_t bytes) {
+        void* p = nullptr;
+        RT_CHECK(hipMalloc(&p, std::max<size_t>(bytes, 8)));
+        mem.push_back(p);
+        return p;
+    };
+    try {
+        const size_t es = prec == P64 ? 8 : 4;
+        void* dA = al((size_t)M * K * es);
+        if (prec == P64) {
+            RT_CHECK(hipMemcpy(dA, A, (size_t)M * K * 8, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> f(A, A + (size_t)M * K);
+            RT_CHECK(hipMemcpy(dA, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+        }
+        double* dC = (double*)al((size_t)M * N * 8);
+        void* dpsi = al((size_t)std::max(M, K) * N * 8);
+        int* dmat = (int*)al((size_t)N * 4);
+        RT_CHECK(hipMemset(dmat, 0, (size_t)N * 4));
+        double* dsig = (double*)al((size_t)K * 8);
+        RT_CHECK(hipMemset(dsig, 0, (size_t)K * 8));
+        PassDesc pd;
+        std::memset(&pd, 0, sizeof(pd));
+        pd.Nc = N; pd.Nmat = 1; pd.mat = dmat; pd.sigw = dsig;
+        std::vector<GemmTask> t;
+        const int rows = which == 0 ? M : M;
+        for (int r = 0; r < (rows + GEMM_ROWS - 1) / GEMM_ROWS; ++r)
+            for (int i = 0; i < (N + GEMM_CELLS - 1) / GEMM_CELLS; ++i) t.push_back({0, i, r, 0});
+        GemmTask* dt = (GemmTask*)al(t.size() * sizeof(GemmTask));
+        RT_CHECK(hipMemcpy(dt, t.data(), t.size() * sizeof(GemmTask), hipMemcpyHostToDevice));
+        PassDesc* dpd = (PassDesc*)al(sizeof(PassDesc));
+        if (which == 0) {
+            double* dB = (double*)al((size_t)K * N * 8);
+            RT_CHECK(hipMemcpy(dB, B, (size_t)K * N * 8, hipMemcpyHostToDevice));
+            pd.Nd = M; pd.Nact = K; pd.Mact = dA; pd.qlout = dB; pd.phi_in = nullptr; pd.psi = dpsi;
+            RT_CHECK(hipMemcpy(dpd, &pd, sizeof(pd), hipMemcpyHostToDevice));
+            launch_mgemm(prec, dpd, dt, (int)t.size(), nullptr);
+            RT_CHECK(hipDeviceSynchronize());
+            if (prec == P64) {
+                RT_CHECK(hipMemcpy(C, dpsi, (size_t)M * N * 8, hipMemcpyDeviceToHost));
+            } else {
+                std::vector<float> f((size_t)M * N);
+                RT_CHECK(hipMemcpy(f.data(), dpsi, f.size() * 4, hipMemcpyDeviceToHost));
+                for (size_t i = 0; i < f.size(); ++i) C[i] = f[i];
+            }
+        } else {
+            if (prec == P64) {
+                RT_CHECK(hipMemcpy(dpsi, B, (size_t)K * N * 8, hipMemcpyHostToDevice));
+            } else {
+                std::vector<float> f(B, B + (size_t)K * N);
+                RT_CHECK(hipMemcpy(dpsi, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+            }
+            pd.Nd = K; pd.Nrows = M; pd.Drows = dA; pd.psi = dpsi; pd.out = dC; pd.zin = nullptr;
+            RT_CHECK(hipMemcpy(dpd, &pd, sizeof(pd), hipMemcpyHostToDevice));
+            launch_dgemm(prec, dpd, dt, (int)t.size(), nullptr);
+            RT_CHECK(hipDeviceSynchronize());
+            RT_CHECK(hipMemcpy(C, dC, (size_t)M * N * 8, hipMemcpyDeviceToHost));
+        }
+    } catch (...) {
+        for (void* p : mem) (void)hipFree(p);
+        throw;
+    }
+    for (void* p : mem) (void)hipFree(p);
+}
+
+}  // namespace rt

@@ -1,0 +1,135 @@
+// engine.hpp -- device-resident SN problem and its solver (host orchestration of the kernels).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/radiant_hip.h"
+#include "kernels.hpp"
+
+namespace rt {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define RT_CHECK(x)                                                                                        \
+    do {                                                                                                   \
+        hipError_t e_ = (x);                                                                               \
+        if (e_ != hipSuccess)                                                                              \
+            throw ::rt::Error(RT_EDEVICE, std::string("HIP error: ") + hipGetErrorString(e_) + " (" #x ") at " + \
+                                              __FILE__ + ":" + std::to_string(__LINE__));                 \
+    } while (0)
+
+// Message returned by rt_last_error() on the calling thread.
+void set_last_error(const std::string& m);
+
+// Runs one contraction kernel in isolation (test hook behind rt_test_contraction).
+void test_contraction(int which, int prec, int M, int K, int N, const double* A, const double* B, double* C);
+
+// Pass kinds of sn_inner_pass! / the GMRES matvec.
+enum PassMode { MODE_T = 0, MODE_A = 1, MODE_IMA = 2 };
+
+struct SolveCounters {
+    int64_t passes = 0, passes_lo = 0;
+    double updates_raw = 0;
+};
+
+class Problem {
+public:
+    explicit Problem(const rt_problem_desc& d, hipStream_t stream = nullptr);
+    ~Problem();
+    Problem(const Problem&) = delete;
+    Problem& operator=(const Problem&) = delete;
+
+    static int64_t device_bytes(const rt_problem_desc& d);
+
+    // Solve all groups (sn_flux.jl:150-224); fluxes stay on the device.
+    void solve(rt_group_stats* stats, rt_solve_info* info);
+    // Copy 𝚽l to the host in Radiant's layout.
+    void get_flux(double* phi_out);
+    // Kernel-level hook: one faithful-state pass on host vectors (see rt_problem_pass).
+    void host_pass(int g, int mode, const double* qlout_host, const double* zin, double* zout);
+    int64_t faithful_state_length() const { return (int64_t)S_f; }
+
+    // ---- pieces used by the batched and served drivers ----
+    int ndims, Nx, Ny, Nz, Nc, G, Nmat, Nd, Np, Nps, Nact;
+    int accel, restart, depth, maxit, precision, state_mode, verbose;
+    double tol;
+    hipStream_t st;
+    bool own_stream = false;
+
+    // per-group solve (used by solve() and by the batch/server schedulers)
+    void solve_group(int g, rt_group_stats& s);
+    SolveCounters counters;
+
+    // device data
+    double* d_phi = nullptr;  // [G][Np][Nc]
+
+private:
+    std::vector<int> act;
+    size_t npos[3] = {0, 0, 0};
+    size_t nb = 0, S_f = 0, S_r = 0;
+    std::vector<void*> allocs;
+    template <class T> T* dalloc(size_t n);
+
+    int* d_mat = nullptr;
+    double *d_dx = nullptr, *d_dy = nullptr, *d_dz = nullptr, *d_mu = nullptr, *d_eta = nullptr, *d_xi = nullptr;
+    double *d_sigt = nullptr, *d_sigw = nullptr, *d_sigo = nullptr, *d_qvol = nullptr;
+    double *d_Mact64 = nullptr, *d_D64 = nullptr, *d_Dact64 = nullptr;
+    float *d_Mact32 = nullptr, *d_D32 = nullptr, *d_Dact32 = nullptr;
+    int* d_act = nullptr;
+    double* d_Dexit[3] = {nullptr, nullptr, nullptr};
+    int* d_exitf[3] = {nullptr, nullptr, nullptr};
+    double* d_qlout = nullptr;
+    void* d_psi = nullptr;
+    void* d_bex[3] = {nullptr, nullptr, nullptr};
+    double* d_bscratch = nullptr;   // boundary part of a reconstruction pass
+    PassDesc* d_pd = nullptr;
+    GemmTask *d_mg = nullptr, *d_dgf = nullptr, *d_dga = nullptr;
+    int n_mg = 0, n_dgf = 0, n_dga = 0;
+    SweepTask* d_sw = nullptr;
+    int sw_ppt = 1, sw_block = 64, sw_nbuf[3] = {2, 2, 2};
+    size_t sw_lds[3] = {0, 0, 0};
+
+    // Krylov work vectors (allocated on first use, keyed by name and length)
+    std::map<std::string, double*> pool;
+    double* vec(const std::string& name, size_t n);
+    RedWork rw{};
+    double* h_scal = nullptr;   // pinned mirror of the scalar slots
+    double** d_Vptr = nullptr;  // device table of Krylov basis pointers
+    double* d_y = nullptr;
+
+    // passes
+    void pass_faithful(int g, int prec, int mode, const double* zin, double* zout);
+    void pass_reduced(int g, int prec, int mode, const double* zin, double* zout);
+    void pass_reconstruct(int g, const double* zin_reduced);
+    void launch_pass(PassDesc& pd, int prec, bool full_rows);
+    void fill_geometry(PassDesc& pd, int g);
+
+    // BLAS-1 helpers
+    double dot(const double* a, const double* b, size_t n);
+    double norm(const double* a, size_t n) { return std::sqrt(dot(a, a, n)); }
+
+    // solvers over a state of length n; pass(out, in) applies the operator
+    template <class MV>
+    void gmres(size_t n, double* z, MV matvec, const double* c, int m, int64_t maxit_, double tol_, int64_t& iter,
+               double& resid, bool& conv, double& rho, const std::string& tag);
+    template <class FP>
+    void anderson(size_t n, double* z, FP fixedpoint, int64_t& iter, double& resid, bool& conv, double& rho);
+    template <class FP>
+    void livolant(size_t n, double* z, FP fixedpoint, int period, int64_t& iter, double& resid, bool& conv,
+                  double& rho);
+    template <class MV>
+    void bicgstab(size_t n, double* z, MV matvec, const double* c, int64_t& iter, double& resid, bool& conv,
+                  double& rho);
+
+    void solve_group_plain(int g, rt_group_stats& s);
+    void solve_group_mixed(int g, rt_group_stats& s);
+};
+
+}  // namespace rt

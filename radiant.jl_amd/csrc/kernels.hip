@@ -1,0 +1,740 @@
+// kernels.hip -- gfx950 kernels of the discrete-ordinates transport pass and its Krylov solver.
+//
+// One transport pass (src/particle_transport/sn_inner_pass.jl:212-332) is three kernels:
+//   mgemm  : psi[n][c] = sum_a M[n][a] * Ql[a][c], with Ql = Qlout + Σs_l Φ computed while the
+//            MFMA operand is loaded (scattering_source.jl:25-30 fused into the contraction);
+//   sweep  : the diamond-difference wavefront of every direction, in place on psi
+//            (sn_sweep_{1,2,3}D.jl with the order-1 DD cell solve of sn_{1,2,3}D_bte.jl);
+//   dgemm  : Φ[r][c] = sum_n D[r][n] * psi[n][c]  (the D accumulation of the sweeps), with the
+//            I - A epilogue of the GMRES matvec (sn_one_speed.jl:147) fused in;
+// plus the half-range boundary moments of the exit faces (bndmom) for the faithful state.
+//
+// Layouts: cells c = ix + Nx*(iy + Ny*iz); moment/direction-major arrays [row][c] so that every
+// contraction streams 16-cell rows (128 B in FP64) per MFMA operand.
+#include "kernels.hpp"
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+namespace rt {
+
+#define RT_HIP(x)                                                                                 \
+    do {                                                                                          \
+        hipError_t e_ = (x);                                                                      \
+        if (e_ != hipSuccess) throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + \
+                                                       " at " + __FILE__ + ":" + std::to_string(__LINE__)); \
+    } while (0)
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef short v4s __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Acc;
+template <> struct Acc<double> { typedef v4d type; };
+template <> struct Acc<float> { typedef v4f type; };
+
+// v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32.  Operand layout (wave64):
+//   A[16x4]: lane l holds A[l%16][l/16];  B[4x16]: lane l holds B[l/16][l%16];
+//   C[16x16]: lane l, element r holds C[row][l%16] with
+//     f32: row = 4*(l/16) + r      f64: row = l/16 + 4*r
+// (measured on gfx950 with rt_test_contraction against a host GEMM).
+__device__ inline v4d mfma4(double a, double b, v4d c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+__device__ inline v4f mfma4(float a, float b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+template <typename T> __device__ inline int acc_row(int lane, int r);
+template <> __device__ inline int acc_row<double>(int lane, int r) { return (lane >> 4) + 4 * r; }
+template <> __device__ inline int acc_row<float>(int lane, int r) { return 4 * (lane >> 4) + r; }
+
+// bf16 round-to-nearest-even of a float, as raw bits.
+__device__ inline short f2bf(float x) {
+    unsigned u = __float_as_uint(x);
+    unsigned r = ((u >> 16) & 1u) + 0x7fffu;
+    return (short)((u + r) >> 16);
+}
+
+// ------------------------------------------------------------------------------------------
+// Moment -> direction contraction with the fused scattering source.
+// ------------------------------------------------------------------------------------------
+template <typename T, int RT>
+__global__ __launch_bounds__(256) void mgemm_kernel(const PassDesc* __restrict__ pds, const GemmTask* __restrict__ tasks) {
+    const GemmTask tk = tasks[blockIdx.x];
+    const PassDesc& d = pds[tk.prob];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int Nc = d.Nc, K = d.Nact, M = d.Nd;
+    const int col = tk.tile * GEMM_CELLS + wave * 16 + (lane & 15);
+    const int kk = lane >> 4;
+    const int row0 = tk.rgroup * GEMM_ROWS;
+    const T* __restrict__ A = (const T*)d.Mact;
+    typename Acc<T>::type acc[RT];
+#pragma unroll
+    for (int t = 0; t < RT; ++t) acc[t] = (typename Acc<T>::type)(0);
+    const bool cin = col < Nc;
+    const int m = cin ? d.mat[col] : 0;
+    for (int k0 = 0; k0 < K; k0 += 4) {
+        const int k = k0 + kk;
+        T b = 0;
+        if (k < K && cin) {
+            const int r = d.phi_rows ? d.phi_rows[k] : k;
+            double q = d.qlout ? d.qlout[(size_t)k * Nc + col] : 0.0;
+            if (d.phi_in) q += d.sigw[m * K + k] * d.phi_in[(size_t)r * Nc + col];
+            b = (T)q;
+        }
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            const int rt0 = row0 + t * 16;
+            if (rt0 < M) {  // wave-uniform
+                const int row = rt0 + (lane & 15);
+                const T a = (k < K && row < M) ? A[(size_t)row * K + k] : (T)0;
+                acc[t] = mfma4(a, b, acc[t]);
+            }
+        }
+    }
+    T* __restrict__ psi = (T*)d.psi;
+    const int ocol = tk.tile * GEMM_CELLS + wave * 16 + (lane & 15);
+    if (ocol < Nc) {
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = row0 + t * 16 + acc_row<T>(lane, r);
+                if (row < M) psi[(size_t)row * Nc + ocol] = acc[t][r];
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Direction -> moment contraction, optional I - A epilogue.
+// ------------------------------------------------------------------------------------------
+template <typename T, int RT>
+__global__ __launch_bounds__(256) void dgemm_kernel(const PassDesc* __restrict__ pds, const GemmTask* __restrict__ tasks) {
+    const GemmTask tk = tasks[blockIdx.x];
+    const PassDesc& d = pds[tk.prob];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int Nc = d.Nc, K = d.Nd, M = d.Nrows;
+    const int col = tk.tile * GEMM_CELLS + wave * 16 + (lane & 15);
+    const int kk = lane >> 4;
+    const int row0 = tk.rgroup * GEMM_ROWS;
+    const T* __restrict__ A = (const T*)d.Drows;
+    const T* __restrict__ B = (const T*)d.psi;
+    typename Acc<T>::type acc[RT];
+#pragma unroll
+    for (int t = 0; t < RT; ++t) acc[t] = (typename Acc<T>::type)(0);
+    const bool cin = col < Nc;
+    for (int k0 = 0; k0 < K; k0 += 4) {
+        const int k = k0 + kk;
+        const T b = (k < K && cin) ? B[(size_t)k * Nc + col] : (T)0;
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            const int rt0 = row0 + t * 16;
+            if (rt0 < M) {
+                const int row = rt0 + (lane & 15);
+                const T a = (k < K && row < M) ? A[(size_t)row * K + k] : (T)0;
+                acc[t] = mfma4(a, b, acc[t]);
+            }
+        }
+    }
+    if (cin) {
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = row0 + t * 16 + acc_row<T>(lane, r);
+                if (row < M) {
+                    double v = (double)acc[t][r];
+                    if (d.zin) v = d.zin[(size_t)row * Nc + col] - v;
+                    d.out[(size_t)row * Nc + col] = v;
+                }
+            }
+        }
+    }
+}
+
+// BF16 operands, FP32 accumulation (v_mfma_f32_16x16x16_bf16): A[16x16] lane l holds
+// A[l%16][4*(l/16)+j], B[16x16] lane l holds B[4*(l/16)+j][l%16], j = 0..3.
+template <int RT>
+__global__ __launch_bounds__(256) void dgemm_bf16_kernel(const PassDesc* __restrict__ pds, const GemmTask* __restrict__ tasks) {
+    const GemmTask tk = tasks[blockIdx.x];
+    const PassDesc& d = pds[tk.prob];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int Nc = d.Nc, K = d.Nd, M = d.Nrows;
+    const int col = tk.tile * GEMM_CELLS + wave * 16 + (lane & 15);
+    const int kq = 4 * (lane >> 4);
+    const int row0 = tk.rgroup * GEMM_ROWS;
+    const float* __restrict__ A = (const float*)d.Drows;
+    const float* __restrict__ B = (const float*)d.psi;
+    v4f acc[RT];
+#pragma unroll
+    for (int t = 0; t < RT; ++t) acc[t] = (v4f)(0.0f);
+    const bool cin = col < Nc;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        v4s b;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = k0 + kq + j;
+            b[j] = f2bf((k < K && cin) ? B[(size_t)k * Nc + col] : 0.0f);
+        }
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            const int rt0 = row0 + t * 16;
+            if (rt0 < M) {
+                const int row = rt0 + (lane & 15);
+                v4s a;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int k = k0 + kq + j;
+                    a[j] = f2bf((k < K && row < M) ? A[(size_t)row * K + k] : 0.0f);
+                }
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+            }
+        }
+    }
+    if (cin) {
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = row0 + t * 16 + 4 * (lane >> 4) + r;
+                if (row < M) {
+                    double v = (double)acc[t][r];
+                    if (d.zin) v = d.zin[(size_t)row * Nc + col] - v;
+                    d.out[(size_t)row * Nc + col] = v;
+                }
+            }
+        }
+    }
+}
+
+template <int RT>
+__global__ __launch_bounds__(256) void mgemm_bf16_kernel(const PassDesc* __restrict__ pds, const GemmTask* __restrict__ tasks) {
+    const GemmTask tk = tasks[blockIdx.x];
+    const PassDesc& d = pds[tk.prob];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int Nc = d.Nc, K = d.Nact, M = d.Nd;
+    const int col = tk.tile * GEMM_CELLS + wave * 16 + (lane & 15);
+    const int kq = 4 * (lane >> 4);
+    const int row0 = tk.rgroup * GEMM_ROWS;
+    const float* __restrict__ A = (const float*)d.Mact;
+    v4f acc[RT];
+#pragma unroll
+    for (int t = 0; t < RT; ++t) acc[t] = (v4f)(0.0f);
+    const bool cin = col < Nc;
+    const int m = cin ? d.mat[col] : 0;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        v4s b;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int k = k0 + kq + j;
+            float v = 0.0f;
+            if (k < K && cin) {
+                const int r = d.phi_rows ? d.phi_rows[k] : k;
+                double q = d.qlout ? d.qlout[(size_t)k * Nc + col] : 0.0;
+                if (d.phi_in) q += d.sigw[m * K + k] * d.phi_in[(size_t)r * Nc + col];
+                v = (float)q;
+            }
+            b[j] = f2bf(v);
+        }
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            const int rt0 = row0 + t * 16;
+            if (rt0 < M) {
+                const int row = rt0 + (lane & 15);
+                v4s a;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int k = k0 + kq + j;
+                    a[j] = f2bf((k < K && row < M) ? A[(size_t)row * K + k] : 0.0f);
+                }
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+            }
+        }
+    }
+    float* __restrict__ psi = (float*)d.psi;
+    if (cin) {
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = row0 + t * 16 + 4 * (lane >> 4) + r;
+                if (row < M) psi[(size_t)row * Nc + col] = acc[t][r];
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Diamond-difference wavefront sweep of one direction per workgroup.
+//
+// Every thread owns PPT pencils (lines of cells along x, indexed in the direction's own sweep
+// frame by (jy, jz)); at step s a pencil processes jx = s - jy - jz.  The incoming x-face flux
+// stays in a register; the y/z face fluxes come from the neighbouring pencils through LDS,
+// double-buffered by step parity (NBUF = 2) or read/written in two barrier phases (NBUF = 1).
+// The angular source of the next PF steps is prefetched into a register ring.
+// Cell solve (order-1 DD, sn_3D_bte.jl:42-153 with O = 1, C = 1, w = [-1, 2]):
+//   psi = (Q + 2hx fx + 2hy fy + 2hz fz) / (St + 2hx + 2hy + 2hz),  f_out = 2 psi - f_in.
+// ------------------------------------------------------------------------------------------
+template <typename T, int PPT, int PF, int NBUF>
+__global__ __launch_bounds__(1024) void sweep_kernel(const PassDesc* __restrict__ pds, const SweepTask* __restrict__ tasks) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const SweepTask tk = tasks[blockIdx.x];
+    const PassDesc& d = pds[tk.prob];
+    const int n = tk.dir;
+    const int Nx = d.Nx, Ny = d.Ny, Nz = d.Nz, Nc = d.Nc, P = Ny * Nz;
+    const int tid = threadIdx.x, bdim = blockDim.x;
+    T* hx2 = (T*)smem;
+    T* hy2 = hx2 + Nx;
+    T* hz2 = hy2 + Ny;
+    T* st = hz2 + Nz;
+    T* Fy = st + d.Nmat;
+    T* Fz = Fy + NBUF * P;
+    const double mu = d.mu[n], eta = d.eta[n], xi = d.xi[n];
+    const bool tang = (d.ndims == 1) && (fabs(mu) <= 1e-10);  // sn_1D_bte.jl:154-155
+    const double amu = tang ? 0.0 : fabs(mu);
+    for (int i = tid; i < Nx; i += bdim) hx2[i] = (T)(2.0 * (amu / d.dx[i]));
+    for (int i = tid; i < Ny; i += bdim) hy2[i] = (T)(d.ndims >= 2 ? 2.0 * (fabs(eta) / d.dy[i]) : 0.0);
+    for (int i = tid; i < Nz; i += bdim) hz2[i] = (T)(d.ndims >= 3 ? 2.0 * (fabs(xi) / d.dz[i]) : 0.0);
+    for (int i = tid; i < d.Nmat; i += bdim) st[i] = (T)d.sigt[i];
+    __syncthreads();
+    const bool fxs = mu >= 0, fys = eta >= 0, fzs = xi >= 0;
+    T* __restrict__ psi = (T*)d.psi + (size_t)n * Nc;
+    T* __restrict__ bx = (T*)d.bx + (size_t)n * P;
+    T* __restrict__ by = (T*)d.by + (size_t)n * Nx * Nz;
+    T* __restrict__ bz = (T*)d.bz + (size_t)n * Nx * Ny;
+    const int* __restrict__ mat = d.mat;
+
+    int jy[PPT], jz[PPT], iy[PPT], iz[PPT], roff[PPT];
+    bool valid[PPT];
+    T fx[PPT];
+    T ring[PPT][PF];
+    int mring[PPT][PF];
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        const int q = tid + k * bdim;
+        valid[k] = q < P;
+        jy[k] = valid[k] ? q % Ny : 0;
+        jz[k] = valid[k] ? q / Ny : 0;
+        iy[k] = fys ? jy[k] : Ny - 1 - jy[k];
+        iz[k] = fzs ? jz[k] : Nz - 1 - jz[k];
+        roff[k] = Nx * (iy[k] + Ny * iz[k]);
+        fx[k] = (T)0;
+#pragma unroll
+        for (int i = 0; i < PF; ++i) {
+            const int jx = i - jy[k] - jz[k];
+            ring[k][i] = (T)0;
+            mring[k][i] = 0;
+            if (valid[k] && jx >= 0 && jx < Nx) {
+                const int ix = fxs ? jx : Nx - 1 - jx;
+                ring[k][i] = psi[roff[k] + ix];
+                mring[k][i] = mat[roff[k] + ix];
+            }
+        }
+    }
+    const int nsteps = Nx + Ny + Nz - 2;
+    for (int s0 = 0; s0 < nsteps; s0 += PF) {
+#pragma unroll
+        for (int i = 0; i < PF; ++i) {
+            const int s = s0 + i;
+            if (s < nsteps) {
+                const int rb = (NBUF == 2) ? ((s + 1) & 1) : 0;
+                const int wb = (NBUF == 2) ? (s & 1) : 0;
+                T ofy[PPT], ofz[PPT];
+                bool act[PPT];
+#pragma unroll
+                for (int k = 0; k < PPT; ++k) {
+                    const int q = tid + k * bdim;
+                    const int jx = s - jy[k] - jz[k];
+                    act[k] = valid[k] && jx >= 0 && jx < Nx;
+                    const T v = ring[k][i];
+                    const int m = mring[k][i];
+                    const int jx2 = jx + PF;
+                    if (valid[k] && jx2 >= 0 && jx2 < Nx) {
+                        const int ix2 = fxs ? jx2 : Nx - 1 - jx2;
+                        ring[k][i] = psi[roff[k] + ix2];
+                        mring[k][i] = mat[roff[k] + ix2];
+                    }
+                    ofy[k] = (T)0;
+                    ofz[k] = (T)0;
+                    if (act[k]) {
+                        const T fy = jy[k] > 0 ? Fy[rb * P + q - 1] : (T)0;
+                        const T fz = jz[k] > 0 ? Fz[rb * P + q - Ny] : (T)0;
+                        const int ix = fxs ? jx : Nx - 1 - jx;
+                        const T hx = hx2[ix], hy = hy2[iy[k]], hz = hz2[iz[k]];
+                        const T S = ((st[m] + hx) + hy) + hz;
+                        const T Q = ((v + hx * fx[k]) + hy * fy) + hz * fz;
+                        const T p = Q / S;
+                        psi[roff[k] + ix] = p;
+                        const T p2 = p + p;
+                        if (!tang) fx[k] = p2 - fx[k];
+                        ofy[k] = p2 - fy;
+                        ofz[k] = p2 - fz;
+                        if (jx == Nx - 1) bx[iy[k] + Ny * iz[k]] = fx[k];
+                        if (d.ndims >= 2 && jy[k] == Ny - 1) by[ix + Nx * iz[k]] = ofy[k];
+                        if (d.ndims >= 3 && jz[k] == Nz - 1) bz[ix + Nx * iy[k]] = ofz[k];
+                        if (NBUF == 2) {
+                            Fy[wb * P + q] = ofy[k];
+                            Fz[wb * P + q] = ofz[k];
+                        }
+                    }
+                }
+                if (NBUF == 1) {
+                    __syncthreads();
+#pragma unroll
+                    for (int k = 0; k < PPT; ++k) {
+                        if (act[k]) {
+                            const int q = tid + k * bdim;
+                            Fy[q] = ofy[k];
+                            Fz[q] = ofz[k];
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Half-range moments of the exit faces (sn_sweep_*D.jl "Save boundary fluxes" + the
+// 𝚽*12_temp accumulation of sn_inner_pass.jl:322-329), summed over directions in index order.
+// One thread per (axis, face, moment, position).
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void bndmom_kernel(const PassDesc* __restrict__ pds, int nprob) {
+    for (int pi = 0; pi < nprob; ++pi) {
+        const PassDesc& d = pds[pi];
+        if (!d.bout_x) continue;
+        const int Nps = d.Nps, Nd = d.Nd;
+        const int npos[3] = {d.Ny * d.Nz, d.Nx * d.Nz, d.Nx * d.Ny};
+        const size_t tot = (size_t)2 * Nps * (npos[0] + (d.ndims >= 2 ? npos[1] : 0) + (d.ndims >= 3 ? npos[2] : 0));
+        for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+            size_t j = i;
+            int ax = 0;
+            while (j >= (size_t)2 * Nps * npos[ax]) { j -= (size_t)2 * Nps * npos[ax]; ++ax; }
+            // j = p + Nps*(f + 2*pos)
+            const int p = (int)(j % Nps);
+            const int f = (int)((j / Nps) % 2);
+            const size_t pos = j / ((size_t)2 * Nps);
+            const double* Dex = ax == 0 ? d.Dexit_x : (ax == 1 ? d.Dexit_y : d.Dexit_z);
+            const int* ef = ax == 0 ? d.exitf_x : (ax == 1 ? d.exitf_y : d.exitf_z);
+            const char* bsrc = (const char*)(ax == 0 ? d.bx : (ax == 1 ? d.by : d.bz));
+            double v = 0.0;
+            for (int n = 0; n < Nd; ++n) {
+                if (ef[n] != f) continue;
+                const double dn = Dex[(size_t)n * Nps + p];
+                const double b = ((const double*)bsrc)[(size_t)n * npos[ax] + pos];
+                v += dn * b;
+            }
+            double* out = ax == 0 ? d.bout_x : (ax == 1 ? d.bout_y : d.bout_z);
+            const double* zin = ax == 0 ? d.bzin_x : (ax == 1 ? d.bzin_y : d.bzin_z);
+            out[j] = zin ? zin[j] - v : v;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Out-of-group + volume source (scattering_source.jl:59-68, sn_flux.jl:179).
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void outsrc_kernel(double* __restrict__ qlout, const double* __restrict__ phi_all,
+                                                     const double* __restrict__ sigo, const double* __restrict__ qvol,
+                                                     const int* __restrict__ mat, const int* __restrict__ act, int g,
+                                                     int G, int Nmat, int Nact, int Np, int Nc) {
+    const size_t tot = (size_t)Nact * Nc;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+        const int a = (int)(i / Nc);
+        const int c = (int)(i % Nc);
+        const int m = mat[c];
+        const int p = act[a];
+        double q = 0.0;
+        for (int gi = 0; gi < g; ++gi)
+            q += sigo[(((size_t)g * G + gi) * Nmat + m) * Nact + a] * phi_all[((size_t)gi * Np + p) * Nc + c];
+        if (p == 0) q += qvol[(size_t)g * Nc + c];
+        qlout[i] = q;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// BLAS-1 with deterministic two-level reductions: fixed grid, fixed tree per block, the last
+// block to finish (atomic ticket after a release fence) sums the block partials in order.
+// ------------------------------------------------------------------------------------------
+__device__ inline double block_sum(double v) {
+    __shared__ double sh[RED_THREADS];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = RED_THREADS / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__device__ inline void finish_reduction(double part, double* partial, unsigned* counter, double* out) {
+    __shared__ bool last;
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = part;
+        __threadfence();
+        const unsigned t = atomicAdd(counter, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (last) {
+        __threadfence();
+        double v = 0.0;
+        for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) v += __builtin_nontemporal_load(&partial[i]);
+        v = block_sum(v);
+        if (threadIdx.x == 0) {
+            *out = v;
+            *counter = 0u;
+        }
+    }
+}
+
+__global__ __launch_bounds__(RED_THREADS) void dot_kernel(const double* __restrict__ a, const double* __restrict__ b, size_t n,
+                                                          double* partial, unsigned* counter, double* out) {
+    double v = 0.0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) v += a[i] * b[i];
+    v = block_sum(v);
+    finish_reduction(v, partial, counter, out);
+}
+
+__global__ __launch_bounds__(RED_THREADS) void mgs_kernel(double* __restrict__ w, const double* __restrict__ v,
+                                                          const double* __restrict__ vnext, size_t n, const double* hptr,
+                                                          double* partial, unsigned* counter, double* out) {
+    const double h = *hptr;
+    double acc = 0.0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double x = w[i] - h * v[i];
+        w[i] = x;
+        acc += x * (vnext ? vnext[i] : x);
+    }
+    acc = block_sum(acc);
+    finish_reduction(acc, partial, counter, out);
+}
+
+__global__ void fill_kernel(double* x, double v, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) x[i] = v;
+}
+__global__ void copy_kernel(double* __restrict__ d, const double* __restrict__ s, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) d[i] = s[i];
+}
+__global__ void scale_kernel(double* x, double a, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) x[i] *= a;
+}
+__global__ void axpy_kernel(double a, const double* __restrict__ x, double* __restrict__ y, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) y[i] += a * x[i];
+}
+__global__ void neg_add_kernel(double* __restrict__ r, const double* __restrict__ c, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) r[i] = -r[i] + c[i];
+}
+__global__ void sub_kernel(double* out, const double* a, const double* b, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) out[i] = a[i] - b[i];
+}
+__global__ void scale_inv_slot_kernel(double* __restrict__ d, const double* __restrict__ s, const double* scal, int slot, size_t n) {
+    const double h = sqrt(scal[slot]);  // the slot holds a squared norm
+    if (!(h > 0.0)) return;
+    const double a = 1.0 / h;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) d[i] = s[i] * a;
+}
+__global__ void lincomb_kernel(double* __restrict__ z, const double* const* __restrict__ V, const double* __restrict__ y, int k,
+                               size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        double v = z[i];
+        for (int t = 0; t < k; ++t) v += y[t] * V[t][i];
+        z[i] = v;
+    }
+}
+__global__ void cast_kernel(float* __restrict__ d, const double* __restrict__ s, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) d[i] = (float)s[i];
+}
+__global__ void scatter_rows_kernel(double* __restrict__ dst, const double* __restrict__ src, int rows, const int* __restrict__ map,
+                                    int Nc) {
+    const size_t tot = (size_t)rows * Nc;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i / Nc), c = (int)(i % Nc);
+        dst[(size_t)map[r] * Nc + c] = src[i];
+    }
+}
+__global__ void set_pd_kernel(PassDesc v, PassDesc* dst) { *dst = v; }
+__global__ void set_vals_kernel(Vals32 v, double* dst, int k) {
+    if ((int)threadIdx.x < k) dst[threadIdx.x] = v.v[threadIdx.x];
+}
+// phi_all [g][p][c] -> Radiant 𝚽l[g + G*(p + Np*c)]
+__global__ void phi_to_radiant_kernel(double* __restrict__ dst, const double* __restrict__ src, int G, int Np, int Nc) {
+    const size_t tot = (size_t)G * Np * Nc;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t g = i % G, r = i / G;
+        const size_t p = r % Np, c = r / Np;
+        dst[i] = src[(g * Np + p) * Nc + c];
+    }
+}
+__global__ void transpose_kernel(double* __restrict__ dst, const double* __restrict__ src, int R, int C) {
+    // dst[c*R + r] = src[r*C + c]
+    const size_t tot = (size_t)R * C;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t r = i % R, c = i / R;
+        dst[i] = src[r * C + c];
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Launchers
+// ------------------------------------------------------------------------------------------
+static inline int grid_for(size_t n, int block = 256, int cap = 2048) {
+    size_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > (size_t)cap) g = cap;
+    return (int)g;
+}
+
+void launch_mgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st) {
+    if (ntasks <= 0) return;
+    if (prec == P64) hipLaunchKernelGGL((mgemm_kernel<double, 8>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    else if (prec == P32) hipLaunchKernelGGL((mgemm_kernel<float, 8>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    else hipLaunchKernelGGL((mgemm_bf16_kernel<8>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    RT_HIP(hipGetLastError());
+}
+
+void launch_dgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st) {
+    if (ntasks <= 0) return;
+    if (prec == P64) hipLaunchKernelGGL((dgemm_kernel<double, 8>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    else if (prec == P32) hipLaunchKernelGGL((dgemm_kernel<float, 8>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    else hipLaunchKernelGGL((dgemm_bf16_kernel<8>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    RT_HIP(hipGetLastError());
+}
+
+int sweep_pencils_per_thread(int P) {
+    for (int ppt : {1, 2, 4, 8, 16})
+        if (P <= 1024 * ppt) return ppt;
+    return -1;
+}
+
+size_t sweep_lds_bytes(int prec, int Nx, int Ny, int Nz, int Nmat, int nbuf) {
+    const size_t es = (prec == P64) ? 8 : 4;
+    return es * ((size_t)Nx + Ny + Nz + Nmat + 2ull * nbuf * Ny * Nz);
+}
+
+template <typename T, int PPT, int NBUF>
+static void sweep_inst(const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int block, size_t lds, hipStream_t st) {
+    constexpr int PF = (PPT <= 4) ? 2 : 1;
+    auto kfn = sweep_kernel<T, PPT, PF, NBUF>;
+    static bool attr_set = false;
+    if (!attr_set) {
+        RT_HIP(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(kfn, dim3(ntasks), dim3(block), lds, st, d_pd, tasks);
+}
+
+template <typename T, int NBUF>
+static void sweep_ppt(int ppt, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int block, size_t lds,
+                      hipStream_t st) {
+    switch (ppt) {
+        case 1: sweep_inst<T, 1, NBUF>(d_pd, tasks, ntasks, block, lds, st); break;
+        case 2: sweep_inst<T, 2, NBUF>(d_pd, tasks, ntasks, block, lds, st); break;
+        case 4: sweep_inst<T, 4, NBUF>(d_pd, tasks, ntasks, block, lds, st); break;
+        case 8: sweep_inst<T, 8, NBUF>(d_pd, tasks, ntasks, block, lds, st); break;
+        case 16: sweep_inst<T, 16, NBUF>(d_pd, tasks, ntasks, block, lds, st); break;
+        default: throw std::runtime_error("sweep: unsupported pencils per thread");
+    }
+}
+
+void launch_sweep(int prec, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int ppt, int nbuf, int block,
+                  size_t lds, hipStream_t st) {
+    if (ntasks <= 0) return;
+    if (prec == P64) {
+        if (nbuf == 2) sweep_ppt<double, 2>(ppt, d_pd, tasks, ntasks, block, lds, st);
+        else sweep_ppt<double, 1>(ppt, d_pd, tasks, ntasks, block, lds, st);
+    } else {
+        if (nbuf == 2) sweep_ppt<float, 2>(ppt, d_pd, tasks, ntasks, block, lds, st);
+        else sweep_ppt<float, 1>(ppt, d_pd, tasks, ntasks, block, lds, st);
+    }
+    RT_HIP(hipGetLastError());
+}
+
+void launch_bndmom(const PassDesc* d_pd, int nprob, int max_items, hipStream_t st) {
+    hipLaunchKernelGGL(bndmom_kernel, dim3(grid_for((size_t)max_items)), dim3(256), 0, st, d_pd, nprob);
+    RT_HIP(hipGetLastError());
+}
+
+void launch_outsrc(double* qlout, const double* phi_all, const double* sigo, const double* qvol, const int* mat,
+                   const int* act, int g, int G, int Nmat, int Nact, int Np, int Nc, hipStream_t st) {
+    hipLaunchKernelGGL(outsrc_kernel, dim3(grid_for((size_t)Nact * Nc)), dim3(256), 0, st, qlout, phi_all, sigo, qvol, mat,
+                       act, g, G, Nmat, Nact, Np, Nc);
+    RT_HIP(hipGetLastError());
+}
+
+void k_dot(const RedWork& rw, int slot, const double* a, const double* b, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(dot_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, a, b, n, rw.partial, rw.counter, rw.scal + slot);
+    RT_HIP(hipGetLastError());
+}
+void k_mgs(const RedWork& rw, int hslot, int nslot, double* w, const double* v, const double* vnext, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(mgs_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, w, v, vnext, n, rw.scal + hslot, rw.partial,
+                       rw.counter, rw.scal + nslot);
+    RT_HIP(hipGetLastError());
+}
+void k_fill(double* x, double v, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(256), 0, st, x, v, n);
+    RT_HIP(hipGetLastError());
+}
+void k_copy(double* dst, const double* src, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(copy_kernel, dim3(grid_for(n)), dim3(256), 0, st, dst, src, n);
+    RT_HIP(hipGetLastError());
+}
+void k_scale(double* x, double a, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n)), dim3(256), 0, st, x, a, n);
+    RT_HIP(hipGetLastError());
+}
+void k_axpy(double a, const double* x, double* y, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(axpy_kernel, dim3(grid_for(n)), dim3(256), 0, st, a, x, y, n);
+    RT_HIP(hipGetLastError());
+}
+void k_neg_add(double* r, const double* c, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(neg_add_kernel, dim3(grid_for(n)), dim3(256), 0, st, r, c, n);
+    RT_HIP(hipGetLastError());
+}
+void k_sub(double* out, const double* a, const double* b, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(sub_kernel, dim3(grid_for(n)), dim3(256), 0, st, out, a, b, n);
+    RT_HIP(hipGetLastError());
+}
+void k_scale_by_inv_slot(double* dst, const double* src, const double* scal, int slot, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(scale_inv_slot_kernel, dim3(grid_for(n)), dim3(256), 0, st, dst, src, scal, slot, n);
+    RT_HIP(hipGetLastError());
+}
+void k_lincomb(double* z, const double* const* V, const double* y, int k, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(lincomb_kernel, dim3(grid_for(n)), dim3(256), 0, st, z, V, y, k, n);
+    RT_HIP(hipGetLastError());
+}
+void k_cast_f64_f32(float* dst, const double* src, size_t n, hipStream_t st) {
+    hipLaunchKernelGGL(cast_kernel, dim3(grid_for(n)), dim3(256), 0, st, dst, src, n);
+    RT_HIP(hipGetLastError());
+}
+void k_scatter_state(double* dst, const double* src, int rows, const int* rowmap, int Nc, hipStream_t st) {
+    hipLaunchKernelGGL(scatter_rows_kernel, dim3(grid_for((size_t)rows * Nc)), dim3(256), 0, st, dst, src, rows, rowmap, Nc);
+    RT_HIP(hipGetLastError());
+}
+void k_set_pd(const PassDesc& v, PassDesc* dst, hipStream_t st) {
+    hipLaunchKernelGGL(set_pd_kernel, dim3(1), dim3(1), 0, st, v, dst);
+    RT_HIP(hipGetLastError());
+}
+void k_set_vals(const Vals32& v, double* dst, int k, hipStream_t st) {
+    hipLaunchKernelGGL(set_vals_kernel, dim3(1), dim3(32), 0, st, v, dst, k);
+    RT_HIP(hipGetLastError());
+}
+void k_phi_to_radiant(double* dst, const double* phi_all, int G, int Np, int Nc, hipStream_t st) {
+    hipLaunchKernelGGL(phi_to_radiant_kernel, dim3(grid_for((size_t)G * Np * Nc)), dim3(256), 0, st, dst, phi_all, G, Np, Nc);
+    RT_HIP(hipGetLastError());
+}
+void k_state_moments_in(double* dst, const double* src, int Np, int Nc, hipStream_t st) {
+    // src [c][p] (p fastest) -> dst [p][c]
+    hipLaunchKernelGGL(transpose_kernel, dim3(grid_for((size_t)Np * Nc)), dim3(256), 0, st, dst, src, Nc, Np);
+    RT_HIP(hipGetLastError());
+}
+void k_state_moments_out(double* dst, const double* src, int Np, int Nc, hipStream_t st) {
+    // src [p][c] -> dst [c][p]
+    hipLaunchKernelGGL(transpose_kernel, dim3(grid_for((size_t)Np * Nc)), dim3(256), 0, st, dst, src, Np, Nc);
+    RT_HIP(hipGetLastError());
+}
+
+}  // namespace rt

@@ -1,0 +1,93 @@
+// kernels.hpp -- device kernels of the SN transport backend (gfx950) and their launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace rt {
+
+// Per-problem arguments of one transport pass.  Lives in device memory (one entry per problem
+// of a batch); the kernels read it through a task table.
+struct PassDesc {
+    // geometry
+    int ndims, Nx, Ny, Nz, Nc, Nd, Nmat;
+    const int* mat;               // [Nc] 0-based material
+    const double *dx, *dy, *dz;   // cell widths
+    const double *mu, *eta, *xi;  // direction cosines [Nd]
+    const double* sigt;           // Σt of the current group [Nmat]
+    // moment -> direction contraction (with the scattering source fused in the operand load)
+    int Nact;                     // moments that can carry a source (Legendre order <= XS order)
+    const void* Mact;             // [Nd][Nact] compute type
+    const double* sigw;           // within-group Σs_l of the current group [Nmat][Nact]
+    const double* qlout;          // out-of-group + volume source [Nact][Nc]; null = homogeneous pass
+    const double* phi_in;         // moments of the input state, row r at phi_in + r*Nc (null = zero state)
+    const int* phi_rows;          // [Nact] row of each active moment in phi_in (null = identity)
+    // angular flux (source in, flux out, in place) and exit-face angular fluxes
+    void* psi;                    // [Nd][Nc] compute type
+    void *bx, *by, *bz;           // [Nd][Ny*Nz], [Nd][Nx*Nz], [Nd][Nx*Ny] compute type
+    // direction -> moment contraction
+    int Nrows;                    // rows of Drows
+    const void* Drows;            // [Nrows][Nd] compute type
+    double* out;                  // [Nrows][Nc]
+    const double* zin;            // null: out = D psi ; else out = zin - D psi (the I - A matvec)
+    // boundary moments (faithful state): [Nps][2][pos] per axis, null = skip
+    int Nps;
+    double *bout_x, *bout_y, *bout_z;
+    const double *bzin_x, *bzin_y, *bzin_z;  // I - A epilogue inputs (null = plain)
+    const double *Dexit_x, *Dexit_y, *Dexit_z;  // [Nd][Nps] half-range D column of the exit face
+    const int *exitf_x, *exitf_y, *exitf_z;      // [Nd] exit face (0 = minus, 1 = plus)
+};
+
+enum Prec { P64 = 0, P32 = 1, PBF16 = 2 };
+
+// Work tiles of the contraction kernels: x = problem, y = 64-cell tile, z = row group.
+struct GemmTask { int prob, tile, rgroup, pad; };
+struct SweepTask { int prob, dir; };
+
+// Sweep geometry needed by the host to size a launch.
+int sweep_pencils_per_thread(int P);
+size_t sweep_lds_bytes(int prec, int Nx, int Ny, int Nz, int Nmat, int nbuf);
+
+void launch_mgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st);
+void launch_sweep(int prec, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int ppt, int nbuf,
+                  int block, size_t lds, hipStream_t st);
+void launch_dgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st);
+void launch_bndmom(const PassDesc* d_pd, int nprob, int max_items, hipStream_t st);
+
+// Row groups of the contraction kernels (rows per work tile).
+constexpr int GEMM_ROWS = 128;   // 8 MFMA row tiles of 16
+constexpr int GEMM_CELLS = 64;   // 4 waves x 16 cells
+
+// Out-of-group + volume source of group g (sn_flux.jl:175-179):
+//   qlout[a][c] = sum_{gi<g} sigo[g][gi][mat c][a] * phi[gi][act a][c] + (p_a == 0) * qvol[g][c]
+void launch_outsrc(double* qlout, const double* phi_all, const double* sigo, const double* qvol, const int* mat,
+                   const int* act, int g, int G, int Nmat, int Nact, int Np, int Nc, hipStream_t st);
+
+// Krylov BLAS-1 with deterministic reductions into device scalar slots.
+constexpr int RED_BLOCKS = 512;
+constexpr int RED_THREADS = 256;
+struct RedWork { double* partial; unsigned* counter; double* scal; };
+void k_dot(const RedWork& rw, int slot, const double* a, const double* b, size_t n, hipStream_t st);
+// w -= scal[hslot]*v ; then scal[nslot] = dot(w, vnext) (vnext null: dot(w, w))
+void k_mgs(const RedWork& rw, int hslot, int nslot, double* w, const double* v, const double* vnext, size_t n,
+           hipStream_t st);
+void k_fill(double* x, double v, size_t n, hipStream_t st);
+void k_copy(double* dst, const double* src, size_t n, hipStream_t st);
+void k_scale(double* x, double a, size_t n, hipStream_t st);
+void k_axpy(double a, const double* x, double* y, size_t n, hipStream_t st);          // y += a x
+void k_neg_add(double* r, const double* c, size_t n, hipStream_t st);                 // r = -r + c
+// dst = src * (1/sqrt(scal[slot])) when scal[slot] > 0 (the slot holds a squared norm)
+void k_scale_by_inv_slot(double* dst, const double* src, const double* scal, int slot, size_t n, hipStream_t st);
+struct Vals32 { double v[32]; };
+void k_set_vals(const Vals32& v, double* dst, int k, hipStream_t st);   // stream-ordered small upload
+void k_set_pd(const PassDesc& v, PassDesc* dst, hipStream_t st);       // stream-ordered PassDesc upload
+void k_lincomb(double* z, const double* const* V, const double* y, int k, size_t n, hipStream_t st);
+void k_sub(double* out, const double* a, const double* b, size_t n, hipStream_t st);   // out = a - b
+void k_cast_f64_f32(float* dst, const double* src, size_t n, hipStream_t st);
+void k_scatter_state(double* dst, const double* src, int rows, const int* rowmap, int Nc, hipStream_t st);
+
+// Layout converters of the host boundary (Radiant order <-> device order).
+void k_phi_to_radiant(double* dst, const double* phi_all, int G, int Np, int Nc, hipStream_t st);
+void k_state_moments_in(double* dst, const double* src, int Np, int Nc, hipStream_t st);   // [c][p] -> [p][c]
+void k_state_moments_out(double* dst, const double* src, int Np, int Nc, hipStream_t st);  // [p][c] -> [c][p]
+
+}  // namespace rt

@@ -1,0 +1,325 @@
+// serve.cpp -- batched solves and the request server (continuous batching under an HBM budget).
+//
+// Every problem keeps its own device-resident state (fluxes, Krylov basis) and its own HIP
+// stream; problems that are resident at the same time run their kernels concurrently on the
+// device.  The server admits queued requests whenever the bytes they need fit the budget and a
+// slot is free, so a request that arrives while others run starts as soon as memory allows.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "engine.hpp"
+
+
+// Deep copy of a problem description (the caller's buffers may go away after submit).
+struct DescCopy {
+    rt_problem_desc d{};
+    std::vector<double> dx, dy, dz, sigt, sigs, qvol, mu, eta, xi, w, M, D, Msurf[6], Dsurf[6];
+    std::vector<int64_t> mat, pl, n2p[6];
+    std::vector<int32_t> bc;
+
+    explicit DescCopy(const rt_problem_desc& s) {
+        d = s;
+        const size_t Nc = (size_t)s.Nx * s.Ny * s.Nz;
+        auto cp = [](auto& v, const auto* p, size_t n) {
+            if (p) v.assign(p, p + n);
+        };
+        cp(dx, s.dx, s.Nx); cp(dy, s.dy, s.Ny); cp(dz, s.dz, s.Nz);
+        cp(mat, s.mat, Nc);
+        cp(bc, s.bc, 2 * s.ndims);
+        cp(sigt, s.sigt, (size_t)s.G * s.Nmat);
+        cp(sigs, s.sigs, (size_t)s.Nmat * s.G * s.G * (s.Ls + 1));
+        cp(qvol, s.qvol, (size_t)s.G * Nc);
+        cp(mu, s.mu, s.Nd); cp(eta, s.eta, s.Nd); cp(xi, s.xi, s.Nd); cp(w, s.w, s.Nd);
+        cp(M, s.M, (size_t)s.Nd * s.Np);
+        cp(D, s.D, (size_t)s.Np * s.Nd);
+        cp(pl, s.pl, s.Np);
+        d.dx = dx.data(); d.dy = s.dy ? dy.data() : nullptr; d.dz = s.dz ? dz.data() : nullptr;
+        d.mat = mat.data(); d.bc = bc.data(); d.sigt = sigt.data(); d.sigs = sigs.data(); d.qvol = qvol.data();
+        d.mu = mu.data(); d.eta = eta.data(); d.xi = xi.data(); d.w = w.data(); d.M = M.data(); d.D = D.data();
+        d.pl = pl.data();
+        for (int f = 0; f < 6; ++f) {
+            d.Msurf[f] = nullptr; d.Dsurf[f] = nullptr; d.n_to_np[f] = nullptr;
+            if (f < 2 * s.ndims) {
+                size_t ndp = 0;
+                if (s.n_to_np[f]) {
+                    n2p[f].assign(s.n_to_np[f], s.n_to_np[f] + s.Nd);
+                    for (int64_t v : n2p[f]) if (v != 0) ++ndp;
+                    d.n_to_np[f] = n2p[f].data();
+                }
+                if (s.Dsurf[f]) { Dsurf[f].assign(s.Dsurf[f], s.Dsurf[f] + (size_t)s.Nps * ndp); d.Dsurf[f] = Dsurf[f].data(); }
+                if (s.Msurf[f]) { Msurf[f].assign(s.Msurf[f], s.Msurf[f] + (size_t)s.Nps * ndp); d.Msurf[f] = Msurf[f].data(); }
+            }
+        }
+    }
+};
+
+extern "C" {
+
+int rt_solve_batch(int32_t n, const rt_problem_desc* const* p, double* const* phi_out, rt_group_stats* const* stats_out,
+                   rt_solve_info* info) {
+    if (n < 0 || (n > 0 && (!p || !phi_out))) {
+        rt::set_last_error("invalid batch arguments");
+        return RT_EINVAL;
+    }
+    try {
+        std::vector<std::unique_ptr<rt::Problem>> probs;
+        for (int i = 0; i < n; ++i) probs.emplace_back(new rt::Problem(*p[i]));
+        hipEvent_t e0, e1;
+        RT_CHECK(hipEventCreate(&e0));
+        RT_CHECK(hipEventCreate(&e1));
+        RT_CHECK(hipDeviceSynchronize());
+        RT_CHECK(hipEventRecord(e0, nullptr));
+        std::vector<std::thread> th;
+        std::vector<std::string> errs(n);
+        std::vector<rt_solve_info> infos(n);
+        for (int i = 0; i < n; ++i)
+            th.emplace_back([&, i] {
+                try {
+                    probs[i]->solve(stats_out ? stats_out[i] : nullptr, &infos[i]);
+                } catch (const std::exception& e) {
+                    errs[i] = e.what();
+                }
+            });
+        for (auto& t : th) t.join();
+        RT_CHECK(hipDeviceSynchronize());
+        RT_CHECK(hipEventRecord(e1, nullptr));
+        RT_CHECK(hipEventSynchronize(e1));
+        float ms = 0;
+        RT_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        for (int i = 0; i < n; ++i)
+            if (!errs[i].empty()) throw rt::Error(RT_EDEVICE, "problem " + std::to_string(i) + ": " + errs[i]);
+        for (int i = 0; i < n; ++i) probs[i]->get_flux(phi_out[i]);
+        if (info) {
+            std::memset(info, 0, sizeof(*info));
+            info->ms_device = ms;
+            for (auto& x : infos) {
+                info->passes += x.passes;
+                info->passes_lo += x.passes_lo;
+                info->updates_raw += x.updates_raw;
+                info->updates_problem += x.updates_problem;
+            }
+        }
+        return RT_OK;
+    } catch (const rt::Error& e) {
+        rt::set_last_error(e.what());
+        return e.code;
+    } catch (const std::exception& e) {
+        rt::set_last_error(e.what());
+        return RT_EDEVICE;
+    }
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// Server
+// ------------------------------------------------------------------------------------------
+struct Request {
+    int64_t id;
+    std::unique_ptr<DescCopy> desc;
+    int64_t bytes = 0;
+    enum State { QUEUED, RUNNING, DONE, FAILED, CANCELLED } state = QUEUED;
+    std::vector<double> phi;
+    std::vector<rt_group_stats> stats;
+    std::string err;
+    int code = 0;
+    std::chrono::steady_clock::time_point t_submit, t_done;
+    std::atomic<bool> cancel{false};
+};
+
+struct rt_server {
+    rt_server_config cfg{};
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<int64_t> queue;
+    std::map<int64_t, std::shared_ptr<Request>> reqs;
+    int64_t next_id = 0;
+    int64_t bytes_in_use = 0;
+    int active = 0;
+    bool stop = false;
+    std::thread sched;
+    std::vector<std::thread> workers;
+
+    void run_request(std::shared_ptr<Request> r) {
+        try {
+            if (!r->cancel.load()) {
+                rt::Problem prob(r->desc->d);
+                std::vector<rt_group_stats> st(r->desc->d.G);
+                prob.solve(st.data(), nullptr);
+                std::vector<double> phi((size_t)prob.G * prob.Np * prob.Nc);
+                prob.get_flux(phi.data());
+                std::lock_guard<std::mutex> lk(mu);
+                r->phi.swap(phi);
+                r->stats.swap(st);
+                r->state = r->cancel.load() ? Request::CANCELLED : Request::DONE;
+            } else {
+                std::lock_guard<std::mutex> lk(mu);
+                r->state = Request::CANCELLED;
+            }
+        } catch (const rt::Error& e) {
+            std::lock_guard<std::mutex> lk(mu);
+            r->state = Request::FAILED; r->err = e.what(); r->code = e.code;
+        } catch (const std::exception& e) {
+            std::lock_guard<std::mutex> lk(mu);
+            r->state = Request::FAILED; r->err = e.what(); r->code = RT_EDEVICE;
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            r->t_done = std::chrono::steady_clock::now();
+            r->desc.reset();
+            bytes_in_use -= r->bytes;
+            --active;
+        }
+        cv.notify_all();
+    }
+
+    void scheduler() {
+        std::unique_lock<std::mutex> lk(mu);
+        while (true) {
+            cv.wait(lk, [&] {
+                if (stop) return true;
+                if (queue.empty()) return false;
+                auto& r = reqs[queue.front()];
+                return r->cancel.load() || (active < cfg.max_active && bytes_in_use + r->bytes <= cfg.hbm_budget_bytes) ||
+                       active == 0;
+            });
+            if (stop) break;
+            // admit in arrival order while the head request fits
+            while (!queue.empty()) {
+                auto r = reqs[queue.front()];
+                if (r->cancel.load()) {
+                    queue.pop_front();
+                    r->state = Request::CANCELLED;
+                    r->t_done = std::chrono::steady_clock::now();
+                    r->desc.reset();
+                    continue;
+                }
+                const bool fits = active < cfg.max_active && bytes_in_use + r->bytes <= cfg.hbm_budget_bytes;
+                if (!fits && active > 0) break;
+                if (!fits && r->bytes > cfg.hbm_budget_bytes) {
+                    queue.pop_front();
+                    r->state = Request::FAILED;
+                    r->code = RT_ENOMEM;
+                    r->err = "request needs " + std::to_string(r->bytes) + " device bytes, above the server budget";
+                    r->t_done = std::chrono::steady_clock::now();
+                    r->desc.reset();
+                    continue;
+                }
+                queue.pop_front();
+                r->state = Request::RUNNING;
+                bytes_in_use += r->bytes;
+                ++active;
+                workers.emplace_back([this, r] { run_request(r); });
+            }
+        }
+    }
+};
+
+extern "C" {
+
+int rt_server_create(const rt_server_config* cfg, rt_server** out) {
+    if (!cfg || !out || cfg->hbm_budget_bytes <= 0 || cfg->max_active < 1) {
+        rt::set_last_error("invalid server configuration");
+        return RT_EINVAL;
+    }
+    auto* s = new rt_server();
+    s->cfg = *cfg;
+    s->sched = std::thread([s] { s->scheduler(); });
+    *out = s;
+    return RT_OK;
+}
+
+int rt_server_submit(rt_server* s, const rt_problem_desc* p, int64_t* req) {
+    if (!s || !p || !req) {
+        rt::set_last_error("null argument");
+        return RT_EINVAL;
+    }
+    auto r = std::make_shared<Request>();
+    try {
+        r->desc.reset(new DescCopy(*p));
+        r->desc->d.precision = s->cfg.precision;
+        if (r->desc->d.precision != RT_PREC_FP64) r->desc->d.accel = RT_ACCEL_GMRES;
+        r->bytes = rt::Problem::device_bytes(r->desc->d);
+    } catch (const std::exception& e) {
+        rt::set_last_error(e.what());
+        return RT_EINVAL;
+    }
+    r->t_submit = std::chrono::steady_clock::now();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        r->id = s->next_id++;
+        s->reqs[r->id] = r;
+        s->queue.push_back(r->id);
+        *req = r->id;
+    }
+    s->cv.notify_all();
+    return RT_OK;
+}
+
+int rt_server_poll(rt_server* s, int64_t req, double* phi_out, rt_group_stats* stats_out, double* latency_ms) {
+    if (!s) {
+        rt::set_last_error("null server");
+        return RT_EINVAL;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(req);
+    if (it == s->reqs.end()) {
+        rt::set_last_error("unknown request id");
+        return RT_EINVAL;
+    }
+    auto& r = it->second;
+    if (r->state == Request::QUEUED || r->state == Request::RUNNING) return RT_EPENDING;
+    if (latency_ms) latency_ms[0] = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+    if (r->state == Request::CANCELLED) return RT_ECANCELLED;
+    if (r->state == Request::FAILED) {
+        rt::set_last_error(r->err);
+        return r->code ? r->code : RT_EDEVICE;
+    }
+    if (phi_out) std::memcpy(phi_out, r->phi.data(), r->phi.size() * sizeof(double));
+    if (stats_out) std::memcpy(stats_out, r->stats.data(), r->stats.size() * sizeof(rt_group_stats));
+    return RT_OK;
+}
+
+int rt_server_cancel(rt_server* s, int64_t req) {
+    if (!s) return RT_EINVAL;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(req);
+        if (it == s->reqs.end()) {
+            rt::set_last_error("unknown request id");
+            return RT_EINVAL;
+        }
+        it->second->cancel.store(true);
+    }
+    s->cv.notify_all();
+    return RT_OK;
+}
+
+int rt_server_destroy(rt_server* s) {
+    if (!s) return RT_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv.notify_all();
+    if (s->sched.joinable()) s->sched.join();
+    for (auto& t : s->workers)
+        if (t.joinable()) t.join();
+    delete s;
+    return RT_OK;
+}
+
+}  // extern "C"

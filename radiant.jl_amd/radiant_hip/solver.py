@@ -1,0 +1,197 @@
+"""Python host of the GPU solve: the role Radiant's `compute_flux(::SN)` setup plays
+(src/particle_transport/sn_flux.jl:27-148) in front of the C ABI.
+
+``compute_flux(problem)`` builds the angular operators (``angular.py``), packs the arrays in
+Radiant's layouts into an ``rt_problem_desc`` and calls ``rt_solve``; it returns 𝚽l as
+``phi[G, Np, Nc]`` (Radiant's 𝚽l[Ng, Np, 1, Nx, Ny, Nz] with the voxels flattened) plus the
+per-group statistics that sn_one_speed.jl prints.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .angular import angular_set, quadrature_dimension
+from ._lib import ACCEL, PREC, STATE, check, lib, ptr
+
+
+class Desc:
+    """An rt_problem_desc together with the numpy arrays it points into."""
+
+    def __init__(self, prob, precision="fp64", state="faithful", accel=None, tol=None, maxit=None, restart=None,
+                 depth=None, verbose=0):
+        qd = quadrature_dimension(prob.quad_type, prob.ndims, prob.qdims)
+        self.ang = ang = angular_set(prob.quad_type, prob.quad_order, prob.ndims, qd, prob.legendre_order,
+                                     prob.sn_type)
+        f64 = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1, order="F"))
+        i64 = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.int64).reshape(-1, order="F"))
+        k = self.keep = {}
+        k["dx"] = f64(prob.dx)
+        k["dy"] = f64(prob.dy if prob.ndims >= 2 else [1.0])
+        k["dz"] = f64(prob.dz if prob.ndims >= 3 else [1.0])
+        k["mat"] = i64(prob.mat)
+        k["bc"] = np.ascontiguousarray(prob.bc, dtype=np.int32)
+        k["sigt"] = f64(prob.sigt)
+        k["sigs"] = f64(prob.sigs)
+        k["qvol"] = f64(prob.qvol)
+        for nm in ("mu", "eta", "xi", "w"):
+            k[nm] = f64(getattr(ang, nm))
+        k["M"] = f64(ang.M)          # Nd x Np, column-major
+        k["D"] = f64(ang.D)          # Np x Nd, column-major
+        k["pl"] = i64(ang.pl)
+        d = self.desc = _lib.ProblemDesc()
+        d.ndims, d.Nx, d.Ny, d.Nz = prob.ndims, prob.Nx, prob.Ny, prob.Nz
+        d.dx, d.dy, d.dz = ptr(k["dx"]), ptr(k["dy"]), ptr(k["dz"])
+        d.mat = ptr(k["mat"], _lib.lp)
+        d.bc = ptr(k["bc"], _lib.ip)
+        d.Nmat, d.G = prob.Nmat, prob.G
+        d.sigt, d.sigs = ptr(k["sigt"]), ptr(k["sigs"])
+        d.Ls = prob.sigs.shape[3] - 1
+        d.qvol = ptr(k["qvol"])
+        d.Nd, d.Np, d.Nps = ang.Nd, ang.Np, ang.Nps
+        d.mu, d.eta, d.xi, d.w = ptr(k["mu"]), ptr(k["eta"]), ptr(k["xi"]), ptr(k["w"])
+        d.M, d.D, d.pl = ptr(k["M"]), ptr(k["D"]), ptr(k["pl"], _lib.lp)
+        for f, (nps, Ms, Ds, n2p) in enumerate(ang.faces):
+            k[f"Ms{f}"] = f64(Ms)
+            k[f"Ds{f}"] = f64(Ds)
+            k[f"n2p{f}"] = i64(n2p)
+            d.Msurf[f] = ptr(k[f"Ms{f}"])
+            d.Dsurf[f] = ptr(k[f"Ds{f}"])
+            d.n_to_np[f] = ptr(k[f"n2p{f}"], _lib.lp)
+        d.accel = ACCEL[accel or prob.accel]
+        d.restart = restart if restart is not None else prob.restart
+        d.depth = depth if depth is not None else prob.depth
+        d.maxit = maxit if maxit is not None else prob.maxit
+        d.tol = tol if tol is not None else prob.tol
+        d.precision = PREC[precision]
+        d.state_mode = STATE[state]
+        d.verbose = verbose
+        self.G, self.Np, self.Nc, self.Nd = prob.G, ang.Np, prob.Nc, ang.Nd
+
+    def byref(self):
+        return C.byref(self.desc)
+
+    def device_bytes(self):
+        b = C.c_int64()
+        check(lib().rt_problem_device_bytes(self.byref(), C.byref(b)))
+        return b.value
+
+
+def _phi(desc, flat):
+    return flat.reshape((desc.G, desc.Np, desc.Nc), order="F")
+
+
+def compute_flux(prob, precision="fp64", state="faithful", **kw):
+    """One-shot solve through rt_solve; returns (phi[G, Np, Nc], stats list)."""
+    d = Desc(prob, precision, state, **kw)
+    phi = np.zeros(d.G * d.Np * d.Nc)
+    st = (_lib.GroupStats * d.G)()
+    check(lib().rt_solve(d.byref(), ptr(phi), st))
+    return _phi(d, phi), [s.as_dict() for s in st]
+
+
+class ResidentProblem:
+    """A problem uploaded once (rt_problem_create); solve() keeps everything in HBM."""
+
+    def __init__(self, prob, precision="fp64", state="faithful", **kw):
+        self.d = Desc(prob, precision, state, **kw)
+        self.h = C.c_void_p()
+        check(lib().rt_problem_create(self.d.byref(), C.byref(self.h)))
+
+    def solve(self):
+        st = (_lib.GroupStats * self.d.G)()
+        info = _lib.SolveInfo()
+        check(lib().rt_problem_solve(self.h, st, C.byref(info)))
+        return [s.as_dict() for s in st], info
+
+    def flux(self):
+        phi = np.zeros(self.d.G * self.d.Np * self.d.Nc)
+        check(lib().rt_problem_get_flux(self.h, ptr(phi)))
+        return _phi(self.d, phi)
+
+    def state_length(self):
+        s = C.c_int64()
+        check(lib().rt_problem_state_length(self.h, C.byref(s)))
+        return s.value
+
+    def transport_pass(self, g, mode, qlout, z):
+        """rt_problem_pass: mode 0 T(z), 1 A z, 2 z - A z on the faithful state layout."""
+        zin = np.ascontiguousarray(z, np.float64)
+        zout = np.zeros_like(zin)
+        q = np.ascontiguousarray(qlout, np.float64)
+        check(lib().rt_problem_pass(self.h, g, mode, ptr(q), ptr(zin), ptr(zout)))
+        return zout
+
+    def close(self):
+        if self.h:
+            lib().rt_problem_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve_batch(probs, precision="mixed-fp32", **kw):
+    """rt_solve_batch over a list of problems; returns ([phi], [stats], info)."""
+    descs = [Desc(p, precision, "reduced" if precision != "fp64" else "faithful", **kw) for p in probs]
+    n = len(descs)
+    PD = C.POINTER(_lib.ProblemDesc)
+    parr = (PD * n)(*[C.pointer(d.desc) for d in descs])
+    phis = [np.zeros(d.G * d.Np * d.Nc) for d in descs]
+    oarr = (_lib.dp * n)(*[ptr(p) for p in phis])
+    stats = [(_lib.GroupStats * d.G)() for d in descs]
+    sarr = (C.POINTER(_lib.GroupStats) * n)(*[C.cast(s, C.POINTER(_lib.GroupStats)) for s in stats])
+    info = _lib.SolveInfo()
+    check(lib().rt_solve_batch(n, parr, oarr, sarr, C.byref(info)))
+    return ([_phi(d, p) for d, p in zip(descs, phis)], [[s.as_dict() for s in st] for st in stats], info)
+
+
+class Server:
+    """rt_server_*: continuous batching of fixed-source solves under an HBM budget."""
+
+    def __init__(self, hbm_budget_bytes, max_active=8, precision="mixed-fp32"):
+        cfg = _lib.ServerConfig(int(hbm_budget_bytes), int(max_active), PREC[precision])
+        self.precision = precision
+        self.h = C.c_void_p()
+        check(lib().rt_server_create(C.byref(cfg), C.byref(self.h)))
+        self.descs = {}
+
+    def submit(self, prob, **kw):
+        d = Desc(prob, self.precision, "reduced" if self.precision != "fp64" else "faithful", **kw)
+        rid = C.c_int64()
+        check(lib().rt_server_submit(self.h, d.byref(), C.byref(rid)))
+        self.descs[rid.value] = (d.G, d.Np, d.Nc)
+        return rid.value
+
+    def poll(self, rid):
+        """Returns None while pending, else (phi, stats, latency_ms); raises on failure/cancel."""
+        G, Np, Nc = self.descs[rid]
+        phi = np.zeros(G * Np * Nc)
+        st = (_lib.GroupStats * G)()
+        lat = C.c_double()
+        rc = lib().rt_server_poll(self.h, rid, ptr(phi), st, C.byref(lat))
+        if rc == _lib.RT_EPENDING:
+            return None
+        check(rc)
+        return phi.reshape((G, Np, Nc), order="F"), [s.as_dict() for s in st], lat.value
+
+    def cancel(self, rid):
+        check(lib().rt_server_cancel(self.h, rid))
+
+    def close(self):
+        if self.h:
+            lib().rt_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def version():
+    return lib().rt_version().decode()
