@@ -127,6 +127,12 @@ int rt_problem_device_bytes(const rt_problem_desc *p, int64_t *bytes);
  * [Np, Nx*Ny*Nz] (Np fastest).  z_out may alias z_in.  Precision follows the handle. */
 int rt_problem_pass(rt_problem *h, int32_t g, int32_t mode, const double *qlout, const double *z_in,
                     double *z_out);
+/* Per-kernel device time of the last solve on a handle, recorded with HIP events on the
+ * handle's stream when the environment has RADIANT_HIP_PROFILE=1 (profiling hook, no
+ * counterpart in the reference).  ms[12] and launches[12] are indexed [precision][kernel] with
+ * precision RT_PREC_* and kernel 0 moment->direction contraction, 1 sweep,
+ * 2 direction->moment contraction, 3 boundary moments. */
+int rt_problem_kernel_times(rt_problem *h, double *ms, int64_t *launches);
 /* State length of the faithful Krylov state of a handle. */
 int rt_problem_state_length(rt_problem *h, int64_t *S);
 

@@ -107,6 +107,16 @@ int rt_test_contraction(int32_t which, int32_t prec, int32_t M, int32_t K, int32
     return guarded([&] { rt::test_contraction(which, prec, M, K, N, A, B, C); });
 }
 
+int rt_problem_kernel_times(rt_problem* h, double* ms, int64_t* launches) {
+    if (!h || !h->p || !ms || !launches) return fail(RT_EINVAL, "null argument");
+    for (int p = 0; p < 3; ++p)
+        for (int k = 0; k < 4; ++k) {
+            ms[p * 4 + k] = h->p->ktimes.ms_p[p][k];
+            launches[p * 4 + k] = h->p->ktimes.n_p[p][k];
+        }
+    return RT_OK;
+}
+
 int rt_solve(const rt_problem_desc* p, double* phi_out, rt_group_stats* stats_out) {
     if (!p || !phi_out) return fail(RT_EINVAL, "null argument");
     return guarded([&] {

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 
@@ -241,6 +242,7 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
 
 Problem::~Problem() {
     (void)hipStreamSynchronize(st);
+    for (hipEvent_t ev : ev_pool) (void)hipEventDestroy(ev);
     for (auto& kv : pool) (void)hipFree(kv.second);
     for (void* p : allocs) (void)hipFree(p);
     if (h_scal) (void)hipHostFree(h_scal);
@@ -305,12 +307,50 @@ void Problem::fill_geometry(PassDesc& pd, int g) {
     pd.Nps = Nps;
 }
 
+hipEvent_t Problem::next_event() {
+    if (ev_next == ev_pool.size()) {
+        hipEvent_t e;
+        RT_CHECK(hipEventCreate(&e));
+        ev_pool.push_back(e);
+    }
+    return ev_pool[ev_next++];
+}
+
+void Problem::collect_kernel_times() {
+    if (ev_used.empty()) return;
+    RT_CHECK(hipStreamSynchronize(st));
+    for (auto& p : ev_used) {
+        float ms = 0.f;
+        RT_CHECK(hipEventElapsedTime(&ms, p.a, p.b));
+        ktimes.ms[p.kernel] += ms;
+        ktimes.launches[p.kernel] += 1;
+        ktimes.ms_p[p.prec][p.kernel] += ms;
+        ktimes.n_p[p.prec][p.kernel] += 1;
+    }
+    ev_used.clear();
+    ev_next = 0;
+}
+
 void Problem::launch_pass(PassDesc& pd, int prec, bool full_rows) {
     k_set_pd(pd, d_pd, st);
-    launch_mgemm(prec, d_pd, d_mg, n_mg, st);
-    launch_sweep(prec == PBF16 ? P32 : prec, d_pd, d_sw, Nd, sw_ppt, sw_nbuf[prec], sw_block, sw_lds[prec], st);
-    launch_dgemm(prec, d_pd, full_rows ? d_dgf : d_dga, full_rows ? n_dgf : n_dga, st);
-    if (pd.bout_x) launch_bndmom(d_pd, 1, (int)nb, st);
+    auto timed = [&](int kernel, auto&& fn) {
+        if (!profiling) {
+            fn();
+            return;
+        }
+        if (ev_next + 2 > 4096) collect_kernel_times();
+        hipEvent_t a = next_event(), b = next_event();
+        RT_CHECK(hipEventRecord(a, st));
+        fn();
+        RT_CHECK(hipEventRecord(b, st));
+        ev_used.push_back({a, b, kernel, prec});
+    };
+    timed(0, [&] { launch_mgemm(prec, d_pd, d_mg, n_mg, st); });
+    timed(1, [&] {
+        launch_sweep(prec == PBF16 ? P32 : prec, d_pd, d_sw, Nd, sw_ppt, sw_nbuf[prec], sw_block, sw_lds[prec], st);
+    });
+    timed(2, [&] { launch_dgemm(prec, d_pd, full_rows ? d_dgf : d_dga, full_rows ? n_dgf : n_dga, st); });
+    if (pd.bout_x) timed(3, [&] { launch_bndmom(d_pd, 1, (int)nb, st); });
     counters.passes += 1;
     if (prec != P64) counters.passes_lo += 1;
     counters.updates_raw += (double)Nc * Nd;
@@ -766,6 +806,11 @@ void Problem::solve_group(int g, rt_group_stats& s) {
 }
 
 void Problem::solve(rt_group_stats* stats, rt_solve_info* info) {
+    const char* pe = std::getenv("RADIANT_HIP_PROFILE");
+    profiling = pe && pe[0] == '1';
+    ktimes = KernelTimes{};
+    ev_used.clear();
+    ev_next = 0;
     hipEvent_t e0, e1;
     RT_CHECK(hipEventCreate(&e0));
     RT_CHECK(hipEventCreate(&e1));
@@ -783,6 +828,7 @@ void Problem::solve(rt_group_stats* stats, rt_solve_info* info) {
     RT_CHECK(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    collect_kernel_times();
     if (info) {
         std::memset(info, 0, sizeof(*info));
         info->ms_device = ms;
@@ -790,6 +836,12 @@ void Problem::solve(rt_group_stats* stats, rt_solve_info* info) {
         info->passes_lo = counters.passes_lo;
         info->updates_raw = counters.updates_raw;
         info->updates_problem = (double)Nc * Nd * G;
+        if (profiling) {
+            info->ms_mgemm = ktimes.ms[0];
+            info->ms_sweep = ktimes.ms[1];
+            info->ms_dgemm = ktimes.ms[2] + ktimes.ms[3];
+            info->ms_krylov = ms - ktimes.ms[0] - ktimes.ms[1] - ktimes.ms[2] - ktimes.ms[3];
+        }
     }
 }
 

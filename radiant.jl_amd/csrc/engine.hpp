@@ -39,6 +39,16 @@ struct SolveCounters {
     double updates_raw = 0;
 };
 
+// Per-kernel device time, recorded with HIP events on the problem's stream when profiling is on
+// (environment RADIANT_HIP_PROFILE=1).  Index: 0 mgemm, 1 sweep, 2 dgemm, 3 bndmom.
+struct KernelTimes {
+    double ms[4] = {0, 0, 0, 0};
+    int64_t launches[4] = {0, 0, 0, 0};
+    // per-precision split of the sweep and contraction kernels [prec][kernel]
+    double ms_p[3][4] = {};
+    int64_t n_p[3][4] = {};
+};
+
 class Problem {
 public:
     explicit Problem(const rt_problem_desc& d, hipStream_t stream = nullptr);
@@ -66,6 +76,8 @@ public:
     // per-group solve (used by solve() and by the batch/server schedulers)
     void solve_group(int g, rt_group_stats& s);
     SolveCounters counters;
+    KernelTimes ktimes;
+    bool profiling = false;
 
     // device data
     double* d_phi = nullptr;  // [G][Np][Nc]
@@ -109,6 +121,12 @@ private:
     void pass_reduced(int g, int prec, int mode, const double* zin, double* zout);
     void pass_reconstruct(int g, const double* zin_reduced);
     void launch_pass(PassDesc& pd, int prec, bool full_rows);
+    struct EvPair { hipEvent_t a, b; int kernel, prec; };
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<EvPair> ev_used;
+    size_t ev_next = 0;
+    hipEvent_t next_event();
+    void collect_kernel_times();
     void fill_geometry(PassDesc& pd, int g);
 
     // BLAS-1 helpers

@@ -109,6 +109,13 @@ class ResidentProblem:
         check(lib().rt_problem_get_flux(self.h, ptr(phi)))
         return _phi(self.d, phi)
 
+    def kernel_times(self):
+        """Per-kernel (ms, launches) of the last solve, [precision][kernel] (RADIANT_HIP_PROFILE=1)."""
+        ms = np.zeros(12)
+        n = np.zeros(12, np.int64)
+        check(lib().rt_problem_kernel_times(self.h, ptr(ms), ptr(n, _lib.lp)))
+        return ms.reshape(3, 4), n.reshape(3, 4)
+
     def state_length(self):
         s = C.c_int64()
         check(lib().rt_problem_state_length(self.h, C.byref(s)))
