@@ -209,6 +209,18 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
         for (int n = 0; n < Nd; ++n) sw[n] = {0, n};
         d_sw = dalloc<SweepTask>(Nd); upload(d_sw, sw, st);
         const int P = Ny * Nz;
+        rows_per_lane = sweep_rows_per_lane(ndims, Ny);
+        for (int pr = 0; pr < 2; ++pr)
+            planes_ok[pr] = ndims == 3 && sweep_planes_shape(pr, Ny, Nz, pl_R[pr], pl_ZW[pr], pl_waves[pr]);
+        scan_cpl = sweep_scan_cpl(Nx);
+        scan_waves = sweep_scan_waves(ndims, Ny, Nz);
+        // RADIANT_HIP_SWEEP selects another sweep kernel for testing: s = scan (default), r = wave
+        // rows / planes, g = pencil kernel
+        sweep_kind = 0;
+        if (const char* ev = std::getenv("RADIANT_HIP_SWEEP")) {
+            if (ev[0] == 'r') sweep_kind = 1;
+            if (ev[0] == 'g') { sweep_kind = 2; rows_per_lane = -1; planes_ok[0] = planes_ok[1] = false; }
+        }
         sw_ppt = sweep_pencils_per_thread(P);
         sw_block = std::min(1024, ((ceil_div(P, sw_ppt) + 63) / 64) * 64);
         for (int pr = 0; pr < 3; ++pr) {
@@ -347,7 +359,17 @@ void Problem::launch_pass(PassDesc& pd, int prec, bool full_rows) {
     };
     timed(0, [&] { launch_mgemm(prec, d_pd, d_mg, n_mg, st); });
     timed(1, [&] {
-        launch_sweep(prec == PBF16 ? P32 : prec, d_pd, d_sw, Nd, sw_ppt, sw_nbuf[prec], sw_block, sw_lds[prec], st);
+        const int sp = prec == PBF16 ? P32 : prec;
+        if (sweep_kind == 0 && scan_cpl > 0 && sweep_scan_lds_bytes(sp, ndims, Nx, Ny, Nz, Nmat) <= 160 * 1024)
+            launch_sweep_scan(sp, d_pd, d_sw, Nd, scan_cpl, scan_waves, sweep_scan_lds_bytes(sp, ndims, Nx, Ny, Nz, Nmat),
+                              st);
+        else if (rows_per_lane > 0 && sweep_rows_lds_bytes(sp, Nx, Nmat) <= 160 * 1024)
+            launch_sweep_rows(sp, d_pd, d_sw, Nd, rows_per_lane, sweep_rows_lds_bytes(sp, Nx, Nmat), st);
+        else if (planes_ok[sp] && sweep_planes_lds_bytes(sp, Nx, Ny, Nz, Nmat, pl_R[sp], pl_waves[sp]) <= 160 * 1024)
+            launch_sweep_planes(sp, d_pd, d_sw, Nd, pl_R[sp], pl_ZW[sp], pl_waves[sp],
+                                sweep_planes_lds_bytes(sp, Nx, Ny, Nz, Nmat, pl_R[sp], pl_waves[sp]), st);
+        else
+            launch_sweep(sp, d_pd, d_sw, Nd, sw_ppt, sw_nbuf[prec], sw_block, sw_lds[prec], st);
     });
     timed(2, [&] { launch_dgemm(prec, d_pd, full_rows ? d_dgf : d_dga, full_rows ? n_dgf : n_dga, st); });
     if (pd.bout_x) timed(3, [&] { launch_bndmom(d_pd, 1, (int)nb, st); });

@@ -106,6 +106,11 @@ private:
     int n_mg = 0, n_dgf = 0, n_dga = 0;
     SweepTask* d_sw = nullptr;
     int sw_ppt = 1, sw_block = 64, sw_nbuf[3] = {2, 2, 2};
+    int rows_per_lane = -1;   // > 0: 1D/2D wave-per-direction sweep
+    int sweep_kind = 0;       // 0 row-scan, 1 rows/planes, 2 pencil
+    int scan_cpl = -1, scan_waves = 1;
+    bool planes_ok[2] = {false, false};   // 3D workgroup-per-direction sweep, per precision
+    int pl_R[2] = {1, 1}, pl_ZW[2] = {1, 1}, pl_waves[2] = {1, 1};
     size_t sw_lds[3] = {0, 0, 0};
 
     // Krylov work vectors (allocated on first use, keyed by name and length)

@@ -47,6 +47,19 @@ template <typename T> __device__ inline int acc_row(int lane, int r);
 template <> __device__ inline int acc_row<double>(int lane, int r) { return (lane >> 4) + 4 * r; }
 template <> __device__ inline int acc_row<float>(int lane, int r) { return 4 * (lane >> 4) + r; }
 
+// Global-memory accessors.  Pointers read from a PassDesc are generic, and generic accesses
+// compile to FLAT instructions, which count against lgkmcnt: every LDS or cross-lane wait would
+// then also wait for the outstanding HBM loads.  Casting to the global address space gives
+// global_load/global_store, tracked by vmcnt only.
+#define RT_GLOBAL __attribute__((address_space(1)))
+template <typename T> __device__ inline T gld(const T* p) { return *(const RT_GLOBAL T*)p; }
+template <typename T> __device__ inline void gst(T* p, T v) { *(RT_GLOBAL T*)p = v; }
+
+// Cell-solve division: IEEE division in FP64 (the parity path); in FP32 the hardware reciprocal
+// (1 ulp) -- the low-precision passes only feed an FP64-refined iteration.
+__device__ inline double dd_div(double q, double s) { return q / s; }
+__device__ inline float dd_div(float q, float s) { return q * __builtin_amdgcn_rcpf(s); }
+
 // bf16 round-to-nearest-even of a float, as raw bits.
 __device__ inline short f2bf(float x) {
     unsigned u = __float_as_uint(x);
@@ -326,8 +339,8 @@ __global__ __launch_bounds__(1024) void sweep_kernel(const PassDesc* __restrict_
             mring[k][i] = 0;
             if (valid[k] && jx >= 0 && jx < Nx) {
                 const int ix = fxs ? jx : Nx - 1 - jx;
-                ring[k][i] = psi[roff[k] + ix];
-                mring[k][i] = mat[roff[k] + ix];
+                ring[k][i] = gld(psi + roff[k] + ix);
+                mring[k][i] = gld(mat + roff[k] + ix);
             }
         }
     }
@@ -351,8 +364,8 @@ __global__ __launch_bounds__(1024) void sweep_kernel(const PassDesc* __restrict_
                     const int jx2 = jx + PF;
                     if (valid[k] && jx2 >= 0 && jx2 < Nx) {
                         const int ix2 = fxs ? jx2 : Nx - 1 - jx2;
-                        ring[k][i] = psi[roff[k] + ix2];
-                        mring[k][i] = mat[roff[k] + ix2];
+                        ring[k][i] = gld(psi + roff[k] + ix2);
+                        mring[k][i] = gld(mat + roff[k] + ix2);
                     }
                     ofy[k] = (T)0;
                     ofz[k] = (T)0;
@@ -363,15 +376,15 @@ __global__ __launch_bounds__(1024) void sweep_kernel(const PassDesc* __restrict_
                         const T hx = hx2[ix], hy = hy2[iy[k]], hz = hz2[iz[k]];
                         const T S = ((st[m] + hx) + hy) + hz;
                         const T Q = ((v + hx * fx[k]) + hy * fy) + hz * fz;
-                        const T p = Q / S;
-                        psi[roff[k] + ix] = p;
+                        const T p = dd_div(Q, S);
+                        gst(psi + roff[k] + ix, p);
                         const T p2 = p + p;
                         if (!tang) fx[k] = p2 - fx[k];
                         ofy[k] = p2 - fy;
                         ofz[k] = p2 - fz;
-                        if (jx == Nx - 1) bx[iy[k] + Ny * iz[k]] = fx[k];
-                        if (d.ndims >= 2 && jy[k] == Ny - 1) by[ix + Nx * iz[k]] = ofy[k];
-                        if (d.ndims >= 3 && jz[k] == Nz - 1) bz[ix + Nx * iy[k]] = ofz[k];
+                        if (jx == Nx - 1) gst(bx + iy[k] + Ny * iz[k], fx[k]);
+                        if (d.ndims >= 2 && jy[k] == Ny - 1) gst(by + ix + Nx * iz[k], ofy[k]);
+                        if (d.ndims >= 3 && jz[k] == Nz - 1) gst(bz + ix + Nx * iy[k], ofz[k]);
                         if (NBUF == 2) {
                             Fy[wb * P + q] = ofy[k];
                             Fz[wb * P + q] = ofz[k];
@@ -392,6 +405,364 @@ __global__ __launch_bounds__(1024) void sweep_kernel(const PassDesc* __restrict_
                 __syncthreads();
             }
         }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// 1D/2D sweep: one wavefront (64 lanes) per direction, R rows per lane (rows l, l+64, ...).
+// Row jy starts at step jy and walks x; the y-face flux of row jy comes from row jy-1 computed
+// one step earlier -- a cross-lane shuffle inside the wave (lane 0 takes the previous slot of
+// lane 63), so the sweep needs no barrier.  The angular source of the next PF steps of every
+// row is prefetched into a register ring.
+// ------------------------------------------------------------------------------------------
+template <typename T, int R, int PF>
+__global__ __launch_bounds__(64) void sweep_rows_kernel(const PassDesc* __restrict__ pds, const SweepTask* __restrict__ tasks) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const SweepTask tk = tasks[blockIdx.x];
+    const PassDesc& d = pds[tk.prob];
+    const int n = tk.dir;
+    const int Nx = d.Nx, Ny = d.Ny, Nc = d.Nc;
+    const int lane = threadIdx.x;
+    T* hx2 = (T*)smem;
+    T* st = hx2 + Nx;
+    const double mu = d.mu[n], eta = d.eta[n];
+    const bool tang = (d.ndims == 1) && (fabs(mu) <= 1e-10);  // sn_1D_bte.jl:154-155
+    const double amu = tang ? 0.0 : fabs(mu);
+    for (int i = lane; i < Nx; i += 64) hx2[i] = (T)(2.0 * (amu / d.dx[i]));
+    for (int i = lane; i < d.Nmat; i += 64) st[i] = (T)d.sigt[i];
+    __syncthreads();
+    const bool fxs = mu >= 0, fys = eta >= 0;
+    T* __restrict__ psi = (T*)d.psi + (size_t)n * Nc;
+    T* __restrict__ bx = (T*)d.bx + (size_t)n * Ny;
+    T* __restrict__ by = (T*)d.by + (size_t)n * Nx;
+    const int* __restrict__ mat = d.mat;
+    int jy[R], roff[R], iyv[R];
+    bool valid[R];
+    T hy[R], fx[R], fyo[R];
+    T ring[R][PF];
+    int mring[R][PF];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        jy[r] = lane + 64 * r;
+        valid[r] = jy[r] < Ny;
+        iyv[r] = valid[r] ? (fys ? jy[r] : Ny - 1 - jy[r]) : 0;
+        roff[r] = Nx * iyv[r];
+        hy[r] = (T)((d.ndims >= 2 && valid[r]) ? 2.0 * (fabs(eta) / d.dy[iyv[r]]) : 0.0);
+        fx[r] = (T)0;
+        fyo[r] = (T)0;
+#pragma unroll
+        for (int i = 0; i < PF; ++i) {
+            const int jx = i - jy[r];
+            ring[r][i] = (T)0;
+            mring[r][i] = 0;
+            if (valid[r] && jx >= 0 && jx < Nx) {
+                const int ix = fxs ? jx : Nx - 1 - jx;
+                ring[r][i] = gld(psi + roff[r] + ix);
+                mring[r][i] = gld(mat + roff[r] + ix);
+            }
+        }
+    }
+    const int nsteps = Nx + Ny - 1;
+    for (int s0 = 0; s0 < nsteps; s0 += PF) {
+#pragma unroll
+        for (int i = 0; i < PF; ++i) {
+            const int s = s0 + i;
+            if (s < nsteps) {
+                T fyin[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const T up = __shfl_up(fyo[r], 1);
+                    const T wrap = (r > 0) ? __shfl(fyo[r > 0 ? r - 1 : 0], 63) : (T)0;
+                    fyin[r] = lane > 0 ? up : wrap;
+                }
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int jx = s - jy[r];
+                    const T v = ring[r][i];
+                    const int m = mring[r][i];
+                    const int jx2 = jx + PF;
+                    if (valid[r] && jx2 >= 0 && jx2 < Nx) {
+                        const int ix2 = fxs ? jx2 : Nx - 1 - jx2;
+                        ring[r][i] = gld(psi + roff[r] + ix2);
+                        mring[r][i] = gld(mat + roff[r] + ix2);
+                    }
+                    if (valid[r] && jx >= 0 && jx < Nx) {
+                        const int ix = fxs ? jx : Nx - 1 - jx;
+                        const T hx = hx2[ix];
+                        const T S = (st[m] + hx) + hy[r];
+                        const T Q = (v + hx * fx[r]) + hy[r] * fyin[r];
+                        const T p = dd_div(Q, S);
+                        gst(psi + roff[r] + ix, p);
+                        const T p2 = p + p;
+                        if (!tang) fx[r] = p2 - fx[r];
+                        fyo[r] = p2 - fyin[r];
+                        if (jx == Nx - 1) gst(bx + iyv[r], fx[r]);
+                        if (d.ndims >= 2 && jy[r] == Ny - 1) gst(by + ix, fyo[r]);
+                    }
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// 3D sweep: one workgroup per direction.  Lane l owns y-rows l + 64 r (r < R); wave w owns the
+// ZW z-levels w*ZW .. w*ZW+ZW-1; every (y, z) pencil walks x with jx = s - jy - jz.
+//   x-face flux: register of the pencil;  y-face: shuffle from lane l-1 (lane 0 takes lane 63's
+//   previous row slot);  z-face: register of the pencil one z-level below in the same lane, or
+//   for the first z-level of a wave the last z-level of the previous wave through LDS
+//   (double-buffered by step parity: one barrier per step).
+// ------------------------------------------------------------------------------------------
+template <typename T, int R, int ZW, int PF, int WMAX>
+__global__ __launch_bounds__(64 * WMAX) void sweep_planes_kernel(const PassDesc* __restrict__ pds, const SweepTask* __restrict__ tasks) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int K = R * ZW;  // pencils per lane
+    const SweepTask tk = tasks[blockIdx.x];
+    const PassDesc& d = pds[tk.prob];
+    const int n = tk.dir;
+    const int Nx = d.Nx, Ny = d.Ny, Nz = d.Nz, Nc = d.Nc;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, W = blockDim.x >> 6;
+    T* hx2 = (T*)smem;
+    T* hy2 = hx2 + Nx;
+    T* hz2 = hy2 + Ny;
+    T* st = hz2 + Nz;
+    T* Fz = st + d.Nmat;  // [2][W][64*R]
+    const double mu = d.mu[n], eta = d.eta[n], xi = d.xi[n];
+    for (int i = tid; i < Nx; i += blockDim.x) hx2[i] = (T)(2.0 * (fabs(mu) / d.dx[i]));
+    for (int i = tid; i < Ny; i += blockDim.x) hy2[i] = (T)(2.0 * (fabs(eta) / d.dy[i]));
+    for (int i = tid; i < Nz; i += blockDim.x) hz2[i] = (T)(2.0 * (fabs(xi) / d.dz[i]));
+    for (int i = tid; i < d.Nmat; i += blockDim.x) st[i] = (T)d.sigt[i];
+    __syncthreads();
+    const bool fxs = mu >= 0, fys = eta >= 0, fzs = xi >= 0;
+    T* __restrict__ psi = (T*)d.psi + (size_t)n * Nc;
+    T* __restrict__ bx = (T*)d.bx + (size_t)n * Ny * Nz;
+    T* __restrict__ by = (T*)d.by + (size_t)n * Nx * Nz;
+    T* __restrict__ bz = (T*)d.bz + (size_t)n * Nx * Ny;
+    const int* __restrict__ mat = d.mat;
+    int js[K], roff[K], iyv[K], izv[K];
+    bool valid[K];
+    T hy[K], hz[K], fx[K], fyo[K], fzo[K];
+    T ring[K][PF];
+    int mring[K][PF];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int r = k / ZW, kz = k % ZW;
+        const int jy = lane + 64 * r, jz = wave * ZW + kz;
+        valid[k] = jy < Ny && jz < Nz;
+        js[k] = jy + jz;
+        iyv[k] = valid[k] ? (fys ? jy : Ny - 1 - jy) : 0;
+        izv[k] = valid[k] ? (fzs ? jz : Nz - 1 - jz) : 0;
+        roff[k] = Nx * (iyv[k] + Ny * izv[k]);
+        hy[k] = hy2[iyv[k]];
+        hz[k] = hz2[izv[k]];
+        fx[k] = (T)0;
+        fyo[k] = (T)0;
+        fzo[k] = (T)0;
+#pragma unroll
+        for (int i = 0; i < PF; ++i) {
+            const int jx = i - js[k];
+            ring[k][i] = (T)0;
+            mring[k][i] = 0;
+            if (valid[k] && jx >= 0 && jx < Nx) {
+                const int ix = fxs ? jx : Nx - 1 - jx;
+                ring[k][i] = gld(psi + roff[k] + ix);
+                mring[k][i] = gld(mat + roff[k] + ix);
+            }
+        }
+    }
+    const int nsteps = Nx + Ny + Nz - 2;
+    const int lrow = 64 * R;
+    for (int s0 = 0; s0 < nsteps; s0 += PF) {
+#pragma unroll
+        for (int i = 0; i < PF; ++i) {
+            const int s = s0 + i;
+            if (s < nsteps) {
+                const int rb = (s + 1) & 1, wb = s & 1;
+                T fyin[K], fzin[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const int r = k / ZW, kz = k % ZW;
+                    const T up = __shfl_up(fyo[k], 1);
+                    const T wrap = (r > 0) ? __shfl(fyo[r > 0 ? k - ZW : 0], 63) : (T)0;
+                    fyin[k] = lane > 0 ? up : wrap;
+                    if (kz > 0) fzin[k] = fzo[kz > 0 ? k - 1 : 0];
+                    else fzin[k] = wave > 0 ? Fz[(rb * W + wave - 1) * lrow + lane + 64 * r] : (T)0;
+                }
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const int jx = s - js[k];
+                    const T v = ring[k][i];
+                    const int m = mring[k][i];
+                    const int jx2 = jx + PF;
+                    if (valid[k] && jx2 >= 0 && jx2 < Nx) {
+                        const int ix2 = fxs ? jx2 : Nx - 1 - jx2;
+                        ring[k][i] = gld(psi + roff[k] + ix2);
+                        mring[k][i] = gld(mat + roff[k] + ix2);
+                    }
+                    if (valid[k] && jx >= 0 && jx < Nx) {
+                        const int ix = fxs ? jx : Nx - 1 - jx;
+                        const T hx = hx2[ix];
+                        const T S = ((st[m] + hx) + hy[k]) + hz[k];
+                        const T Q = ((v + hx * fx[k]) + hy[k] * fyin[k]) + hz[k] * fzin[k];
+                        const T p = dd_div(Q, S);
+                        gst(psi + roff[k] + ix, p);
+                        const T p2 = p + p;
+                        fx[k] = p2 - fx[k];
+                        fyo[k] = p2 - fyin[k];
+                        fzo[k] = p2 - fzin[k];
+                        const int jy = lane + 64 * (k / ZW), jz = wave * ZW + k % ZW;
+                        if (jx == Nx - 1) gst(bx + iyv[k] + Ny * izv[k], fx[k]);
+                        if (jy == Ny - 1) gst(by + ix + Nx * izv[k], fyo[k]);
+                        if (jz == Nz - 1) gst(bz + ix + Nx * iyv[k], fzo[k]);
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < R; ++r) Fz[(wb * W + wave) * lrow + lane + 64 * r] = fzo[r * ZW + ZW - 1];
+                __syncthreads();
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Row-scan sweep (1D/2D/3D): one workgroup per direction, lanes along x.
+//
+// Along a pencil (a row of cells in x) the DD cell solve is an affine map of the incoming
+// x-face flux:  psi_i = c0_i + c1_i fx_i  and  fx_{i+1} = 2 psi_i - fx_i = a_i + b_i fx_i with
+//   c0 = (Q + hy fy + hz fz)/S,  c1 = hx/S,  a = 2 c0,  b = 2 c1 - 1   (|b| < 1),
+// so a whole pencil is solved at once by an exclusive scan of the maps across the lanes (each
+// lane first composes its CPL consecutive cells).  Pencils (jy, jz) advance by diagonals
+// t = jy + jz; the y/z face fluxes of the previous diagonal live in LDS (FY[jz][x], FZ[jy][x]),
+// one barrier per diagonal.  Every global access is a contiguous 64*CPL-cell segment.
+// ------------------------------------------------------------------------------------------
+template <typename T, int CPL, int U>
+__global__ __launch_bounds__(1024) void sweep_scan_kernel(const PassDesc* __restrict__ pds, const SweepTask* __restrict__ tasks) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const SweepTask tk = tasks[blockIdx.x];
+    const PassDesc& d = pds[tk.prob];
+    const int n = tk.dir;
+    const int Nx = d.Nx, Ny = d.Ny, Nz = d.Nz, Nc = d.Nc;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, W = blockDim.x >> 6;
+    T* hy2 = (T*)smem;
+    T* hz2 = hy2 + Ny;
+    T* st = hz2 + Nz;
+    T* FY = st + d.Nmat;                 // [Nz][Nx]
+    T* FZ = FY + (size_t)Nz * Nx;        // [Ny][Nx] (3D only)
+    const double mu = d.mu[n], eta = d.eta[n], xi = d.xi[n];
+    const bool tang = (d.ndims == 1) && (fabs(mu) <= 1e-10);  // sn_1D_bte.jl:154-155
+    const double amu = tang ? 0.0 : fabs(mu);
+    for (int i = tid; i < Ny; i += blockDim.x) hy2[i] = (T)(d.ndims >= 2 ? 2.0 * (fabs(eta) / d.dy[i]) : 0.0);
+    for (int i = tid; i < Nz; i += blockDim.x) hz2[i] = (T)(d.ndims >= 3 ? 2.0 * (fabs(xi) / d.dz[i]) : 0.0);
+    for (int i = tid; i < d.Nmat; i += blockDim.x) st[i] = (T)d.sigt[i];
+    const int nf = Nz * Nx + (d.ndims >= 3 ? Ny * Nx : 0);
+    for (int i = tid; i < nf; i += blockDim.x) FY[i] = (T)0;  // void inflow on the y-/z- faces
+    const bool fxs = mu >= 0, fys = eta >= 0, fzs = xi >= 0;
+    int ixc[CPL];
+    bool inx[CPL];
+    T hx[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+        const int jx = lane * CPL + c;
+        inx[c] = jx < Nx;
+        ixc[c] = inx[c] ? (fxs ? jx : Nx - 1 - jx) : 0;
+        hx[c] = inx[c] ? (T)(2.0 * (amu / d.dx[ixc[c]])) : (T)0;
+    }
+    __syncthreads();
+    T* __restrict__ psi = (T*)d.psi + (size_t)n * Nc;
+    T* __restrict__ bx = (T*)d.bx + (size_t)n * Ny * Nz;
+    T* __restrict__ by = (T*)d.by + (size_t)n * Nx * Nz;
+    T* __restrict__ bz = (T*)d.bz + (size_t)n * Nx * Ny;
+    const int* __restrict__ mat = d.mat;
+    const int ndiag = Ny + Nz - 1;
+    for (int t = 0; t < ndiag; ++t) {
+        const int jz_lo = t - (Ny - 1) > 0 ? t - (Ny - 1) : 0;
+        const int jz_hi = t < Nz - 1 ? t : Nz - 1;
+        const int cnt = jz_hi - jz_lo + 1;
+        for (int q0 = wave * U; q0 < cnt; q0 += W * U) {
+            // U pencils of this diagonal, processed together for instruction-level parallelism
+            T v[U][CPL], fy[U][CPL], fz[U][CPL], c0[U][CPL], c1[U][CPL];
+            int roff[U], jyv[U], jzv[U], iyv[U], izv[U];
+            bool ok[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int q = q0 + u;
+                ok[u] = q < cnt;
+                jzv[u] = jz_lo + (ok[u] ? q : 0);
+                jyv[u] = t - jzv[u];
+                iyv[u] = fys ? jyv[u] : Ny - 1 - jyv[u];
+                izv[u] = fzs ? jzv[u] : Nz - 1 - jzv[u];
+                roff[u] = Nx * (iyv[u] + Ny * izv[u]);
+                const T hy = hy2[iyv[u]], hz = hz2[izv[u]];
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    const int jx = lane * CPL + c;
+                    v[u][c] = (T)0;
+                    fy[u][c] = (T)0;
+                    fz[u][c] = (T)0;
+                    c0[u][c] = (T)0;
+                    c1[u][c] = (T)0;
+                    if (ok[u] && inx[c]) {
+                        v[u][c] = gld(psi + roff[u] + ixc[c]);
+                        const int m = gld(mat + roff[u] + ixc[c]);
+                        fy[u][c] = FY[jzv[u] * Nx + jx];
+                        if (d.ndims >= 3) fz[u][c] = FZ[jyv[u] * Nx + jx];
+                        const T S = ((st[m] + hx[c]) + hy) + hz;
+                        const T rS = dd_div((T)1, S);
+                        c0[u][c] = ((v[u][c] + hy * fy[u][c]) + hz * fz[u][c]) * rS;
+                        c1[u][c] = hx[c] * rS;
+                    }
+                }
+            }
+            // compose each lane's CPL maps, then an exclusive scan of the maps across lanes
+            T A[U], B[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                A[u] = (T)0;
+                B[u] = (T)1;
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    const T a = tang ? (T)0 : c0[u][c] + c0[u][c];
+                    const T b = tang ? (T)1 : (c1[u][c] + c1[u][c]) - (T)1;
+                    A[u] = a + b * A[u];
+                    B[u] = b * B[u];
+                }
+            }
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const T As = __shfl_up(A[u], off);
+                    const T Bs = __shfl_up(B[u], off);
+                    if (lane >= off) {
+                        A[u] = A[u] + B[u] * As;
+                        B[u] = B[u] * Bs;
+                    }
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                T fx = __shfl_up(A[u], 1);  // exclusive prefix applied to the void inflow (0)
+                if (lane == 0) fx = (T)0;
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    const int jx = lane * CPL + c;
+                    if (ok[u] && inx[c]) {
+                        const T p = c0[u][c] + c1[u][c] * fx;
+                        gst(psi + roff[u] + ixc[c], p);
+                        const T p2 = p + p;
+                        if (!tang) fx = p2 - fx;
+                        const T fyo = p2 - fy[u][c];
+                        const T fzo = p2 - fz[u][c];
+                        FY[jzv[u] * Nx + jx] = fyo;
+                        if (d.ndims >= 3) FZ[jyv[u] * Nx + jx] = fzo;
+                        if (jx == Nx - 1) gst(bx + iyv[u] + Ny * izv[u], fx);
+                        if (d.ndims >= 2 && jyv[u] == Ny - 1) gst(by + ixc[c] + Nx * izv[u], fyo);
+                        if (d.ndims >= 3 && jzv[u] == Nz - 1) gst(bz + ixc[c] + Nx * iyv[u], fzo);
+                    }
+                }
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -638,6 +1009,151 @@ static void sweep_ppt(int ppt, const PassDesc* d_pd, const SweepTask* tasks, int
         case 16: sweep_inst<T, 16, NBUF>(d_pd, tasks, ntasks, block, lds, st); break;
         default: throw std::runtime_error("sweep: unsupported pencils per thread");
     }
+}
+
+template <typename T, int R>
+static void sweep_rows_inst(const PassDesc* d_pd, const SweepTask* tasks, int ntasks, size_t lds, hipStream_t st) {
+    constexpr int PF = (sizeof(T) == 8 && R != 2) ? 8 : 16;
+    auto kfn = sweep_rows_kernel<T, R, PF>;
+    static bool attr_set = false;
+    if (!attr_set) {
+        RT_HIP(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(kfn, dim3(ntasks), dim3(64), lds, st, d_pd, tasks);
+}
+
+int sweep_rows_per_lane(int ndims, int Ny) {
+    if (ndims > 2) return -1;
+    for (int r : {1, 2, 4})
+        if (Ny <= 64 * r) return r;
+    return -1;
+}
+
+size_t sweep_rows_lds_bytes(int prec, int Nx, int Nmat) { return (prec == P64 ? 8 : 4) * ((size_t)Nx + Nmat); }
+
+void launch_sweep_rows(int prec, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int rows_per_lane,
+                       size_t lds, hipStream_t st) {
+    if (ntasks <= 0) return;
+    if (prec == P64) {
+        if (rows_per_lane == 1) sweep_rows_inst<double, 1>(d_pd, tasks, ntasks, lds, st);
+        else if (rows_per_lane == 2) sweep_rows_inst<double, 2>(d_pd, tasks, ntasks, lds, st);
+        else sweep_rows_inst<double, 4>(d_pd, tasks, ntasks, lds, st);
+    } else {
+        if (rows_per_lane == 1) sweep_rows_inst<float, 1>(d_pd, tasks, ntasks, lds, st);
+        else if (rows_per_lane == 2) sweep_rows_inst<float, 2>(d_pd, tasks, ntasks, lds, st);
+        else sweep_rows_inst<float, 4>(d_pd, tasks, ntasks, lds, st);
+    }
+    RT_HIP(hipGetLastError());
+}
+
+bool sweep_planes_shape(int prec, int Ny, int Nz, int& R, int& ZW, int& waves) {
+    // pencils per lane K = R*ZW <= 8 with at most 8 waves (256 VGPRs per lane), or, in FP32,
+    // K = 16 with up to 16 waves; otherwise the pencil kernel handles the grid.
+    R = Ny <= 64 ? 1 : (Ny <= 128 ? 2 : -1);
+    if (R < 0) return false;
+    for (int zw : {1, 2, 4, 8}) {
+        const int w = (Nz + zw - 1) / zw;
+        if (w <= 8 && R * zw <= 8) { ZW = zw; waves = w; return true; }
+    }
+    if (prec != P64) {
+        const int w = (Nz + 7) / 8;
+        if (w <= 16) { ZW = 8; waves = w; return true; }
+    }
+    return false;
+}
+
+size_t sweep_planes_lds_bytes(int prec, int Nx, int Ny, int Nz, int Nmat, int R, int waves) {
+    return (prec == P64 ? 8 : 4) * ((size_t)Nx + Ny + Nz + Nmat + 2ull * waves * 64 * R);
+}
+
+template <typename T, int R, int ZW, int WMAX>
+static void planes_inst(const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int waves, size_t lds, hipStream_t st) {
+    constexpr int K = R * ZW;
+    constexpr bool dbl = sizeof(T) == 8;
+    constexpr int PF = K <= 2 ? 8 : (K <= 4 ? 4 : (K <= 8 && !dbl ? 2 : 1));
+    auto kfn = sweep_planes_kernel<T, R, ZW, PF, WMAX>;
+    static bool attr_set = false;
+    if (!attr_set) {
+        RT_HIP(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(kfn, dim3(ntasks), dim3(64 * waves), lds, st, d_pd, tasks);
+}
+
+template <typename T>
+static void planes_dispatch(int R, int ZW, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int waves, size_t lds,
+                            hipStream_t st) {
+    if (R == 1) {
+        if (ZW == 1) planes_inst<T, 1, 1, 8>(d_pd, tasks, ntasks, waves, lds, st);
+        else if (ZW == 2) planes_inst<T, 1, 2, 8>(d_pd, tasks, ntasks, waves, lds, st);
+        else if (ZW == 4) planes_inst<T, 1, 4, 8>(d_pd, tasks, ntasks, waves, lds, st);
+        else if (waves <= 8) planes_inst<T, 1, 8, 8>(d_pd, tasks, ntasks, waves, lds, st);
+        else planes_inst<T, 1, 8, 16>(d_pd, tasks, ntasks, waves, lds, st);
+    } else {
+        if (ZW == 1) planes_inst<T, 2, 1, 8>(d_pd, tasks, ntasks, waves, lds, st);
+        else if (ZW == 2) planes_inst<T, 2, 2, 8>(d_pd, tasks, ntasks, waves, lds, st);
+        else if (ZW == 4) planes_inst<T, 2, 4, 8>(d_pd, tasks, ntasks, waves, lds, st);
+        else planes_inst<T, 2, 8, 16>(d_pd, tasks, ntasks, waves, lds, st);
+    }
+}
+
+void launch_sweep_planes(int prec, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int R, int ZW, int waves,
+                         size_t lds, hipStream_t st) {
+    if (ntasks <= 0) return;
+    if (prec == P64) planes_dispatch<double>(R, ZW, d_pd, tasks, ntasks, waves, lds, st);
+    else planes_dispatch<float>(R, ZW, d_pd, tasks, ntasks, waves, lds, st);
+    RT_HIP(hipGetLastError());
+}
+
+int sweep_scan_cpl(int Nx) {
+    for (int c : {1, 2, 4, 8})
+        if (Nx <= 64 * c) return c;
+    return -1;
+}
+
+size_t sweep_scan_lds_bytes(int prec, int ndims, int Nx, int Ny, int Nz, int Nmat) {
+    return (prec == P64 ? 8 : 4) * ((size_t)Ny + Nz + Nmat + (size_t)Nz * Nx + (ndims >= 3 ? (size_t)Ny * Nx : 0));
+}
+
+int sweep_scan_waves(int ndims, int Ny, int Nz) {
+    if (ndims < 3) return 1;
+    const int diag = Ny < Nz ? Ny : Nz;  // longest diagonal of the (y, z) pencil grid
+    int w = (diag + 1) / 2;              // two pencils per wave per diagonal
+    return w < 1 ? 1 : (w > 16 ? 16 : w);
+}
+
+template <typename T, int CPL>
+static void scan_inst(const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int waves, size_t lds, hipStream_t st) {
+    constexpr int U = CPL >= 4 ? 1 : 2;
+    auto kfn = sweep_scan_kernel<T, CPL, U>;
+    static bool attr_set = false;
+    if (!attr_set) {
+        RT_HIP(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(kfn, dim3(ntasks), dim3(64 * waves), lds, st, d_pd, tasks);
+}
+
+void launch_sweep_scan(int prec, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int cpl, int waves, size_t lds,
+                       hipStream_t st) {
+    if (ntasks <= 0) return;
+    if (prec == P64) {
+        switch (cpl) {
+            case 1: scan_inst<double, 1>(d_pd, tasks, ntasks, waves, lds, st); break;
+            case 2: scan_inst<double, 2>(d_pd, tasks, ntasks, waves, lds, st); break;
+            case 4: scan_inst<double, 4>(d_pd, tasks, ntasks, waves, lds, st); break;
+            default: scan_inst<double, 8>(d_pd, tasks, ntasks, waves, lds, st); break;
+        }
+    } else {
+        switch (cpl) {
+            case 1: scan_inst<float, 1>(d_pd, tasks, ntasks, waves, lds, st); break;
+            case 2: scan_inst<float, 2>(d_pd, tasks, ntasks, waves, lds, st); break;
+            case 4: scan_inst<float, 4>(d_pd, tasks, ntasks, waves, lds, st); break;
+            default: scan_inst<float, 8>(d_pd, tasks, ntasks, waves, lds, st); break;
+        }
+    }
+    RT_HIP(hipGetLastError());
 }
 
 void launch_sweep(int prec, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int ppt, int nbuf, int block,

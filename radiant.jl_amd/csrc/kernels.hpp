@@ -50,6 +50,22 @@ size_t sweep_lds_bytes(int prec, int Nx, int Ny, int Nz, int Nmat, int nbuf);
 void launch_mgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st);
 void launch_sweep(int prec, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int ppt, int nbuf,
                   int block, size_t lds, hipStream_t st);
+// 1D/2D wave-per-direction sweep (rows per lane 1, 2 or 4; -1 when the grid does not fit).
+int sweep_rows_per_lane(int ndims, int Ny);
+size_t sweep_rows_lds_bytes(int prec, int Nx, int Nmat);
+void launch_sweep_rows(int prec, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int rows_per_lane,
+                       size_t lds, hipStream_t st);
+// 3D workgroup-per-direction sweep: lanes = y rows (R per lane), waves = z slabs of ZW levels.
+bool sweep_planes_shape(int prec, int Ny, int Nz, int& R, int& ZW, int& waves);
+size_t sweep_planes_lds_bytes(int prec, int Nx, int Ny, int Nz, int Nmat, int R, int waves);
+void launch_sweep_planes(int prec, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int R, int ZW, int waves,
+                         size_t lds, hipStream_t st);
+// Row-scan sweep (lanes along x, affine scan of the x-recurrence); cells per lane 1..8.
+int sweep_scan_cpl(int Nx);
+size_t sweep_scan_lds_bytes(int prec, int ndims, int Nx, int Ny, int Nz, int Nmat);
+int sweep_scan_waves(int ndims, int Ny, int Nz);
+void launch_sweep_scan(int prec, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int cpl, int waves, size_t lds,
+                       hipStream_t st);
 void launch_dgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st);
 void launch_bndmom(const PassDesc* d_pd, int nprob, int max_items, hipStream_t st);
 

@@ -161,3 +161,19 @@ def test_zero_source_group(oracle):
     phi, st = S.compute_flux(prob)
     assert np.all(phi[0] == 0.0) and st[0]["passes"] == 0 and st[0]["converged"]
     assert rel_err(phi, ref) < 1e-10
+
+
+@pytest.mark.parametrize("kind", ["g", "r"])
+@pytest.mark.parametrize("ndims", [1, 2, 3])
+def test_alternate_sweep_kernel_parity(oracle, ndims, kind, monkeypatch):
+    """The other sweep kernels (the row-scan kernel is the default): RADIANT_HIP_SWEEP=r selects the
+    wave-per-direction rows/planes kernels, g the pencil kernel (fallback for grids the others do
+    not cover)."""
+    monkeypatch.setenv("RADIANT_HIP_SWEEP", kind)
+    P, S = _P(), _S()
+    prob = P.small_problem(ndims, N=5 if ndims == 3 else 9, tol=1e-12)
+    ref, _ = oracle.solve(prob)
+    phi, _ = S.compute_flux(prob)
+    e = rel_err(phi, ref)
+    _say("generic sweep", ndims, e)
+    assert e < 1e-10

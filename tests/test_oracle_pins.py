@@ -1,0 +1,288 @@
+"""Pins of the CPU oracle (and of the numpy host setup) by facts independent of its own formulas.
+
+Radiant ships no numeric tests or fixtures for this path (test/runtests.jl is `@test true`) and
+cannot run here (no Julia), so every part of the oracle is pinned by one of:
+  * library routines     -- numpy leggauss, scipy associated Legendre functions;
+  * closed forms         -- quadrature moment conditions, the 1D Galerkin closed form of M,
+                            the exact solution of a pure absorber;
+  * invariants           -- D M = I, particle balance, mirror symmetry, orthonormality;
+  * brute force          -- (I - A) assembled column by column from oracle passes and solved
+                            with LAPACK, against every in-group solver.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import special
+
+from util import rel_err
+
+PI = math.pi
+
+
+@pytest.mark.parametrize("N", [2, 4, 8, 16, 32])
+def test_gauss_legendre_vs_numpy(oracle, N):
+    mu, _, _, w = oracle.quadrature(N, "gauss-legendre", 1, 1)
+    x, wx = np.polynomial.legendre.leggauss(N)
+    o = np.argsort(mu)
+    assert np.abs(mu[o] - x).max() < 1e-14
+    assert np.abs(w[o] - wx).max() < 1e-14
+    # exactness up to degree 2N-1
+    for k in range(2 * N):
+        exact = 0.0 if k % 2 else 2.0 / (k + 1)
+        assert abs(np.sum(w * mu ** k) - exact) < 1e-13
+
+
+@pytest.mark.parametrize("N", [2, 4, 8, 12])
+def test_glc_moments(oracle, N):
+    mu, eta, xi, w = oracle.quadrature(N, "gauss-legendre-chebychev", 2, 3)
+    assert len(w) == 2 * N * N
+    assert abs(w.sum() - 4 * PI) < 1e-12
+    for a in (mu, eta, xi):
+        assert abs(np.sum(w * a * a) - 4 * PI / 3) < 1e-12
+        assert abs(np.sum(w * a)) < 1e-12
+    assert np.allclose(mu ** 2 + eta ** 2 + xi ** 2, 1.0, atol=1e-14)
+    assert np.all(np.diff(mu) >= 0)  # polar levels ascending (gauss_legendre_chebychev.jl:24)
+
+
+@pytest.mark.parametrize("N", [2, 4, 8, 10, 12, 16])
+def test_carlson_moments(oracle, N):
+    mu, eta, xi, w = oracle.quadrature(N, "carlson", 3, 3)
+    assert len(w) == N * (N + 2)
+    assert np.allclose(w, w[0]) and abs(w.sum() - 4 * PI) < 1e-12
+    for a in (mu, eta, xi):
+        assert abs(np.sum(w * a * a) - 4 * PI / 3) < 1e-12
+        assert abs(np.sum(w * a)) < 1e-12
+    assert np.allclose(mu ** 2 + eta ** 2 + xi ** 2, 1.0, atol=1e-14)
+    # octant symmetry: the 8 octants are sign flips of the first
+    k = len(w) // 8
+    for t in range(8):
+        assert np.allclose(np.abs(mu[t * k:(t + 1) * k]), mu[:k])
+
+
+def _scipy_rlm(l, m, mu, phi):
+    """Radiant's real harmonic from scipy: C_lm P_l^|m| (no Condon-Shortley) T_m(phi)."""
+    am = abs(m)
+    C = math.sqrt((2 - (m == 0)) * math.factorial(l - am) / math.factorial(l + am))
+    P = (-1) ** am * special.lpmv(am, l, mu)  # scipy includes the Condon-Shortley phase
+    T = math.cos(m * phi) if m >= 0 else math.sin(am * phi)
+    return C * P * T
+
+
+def test_real_sh_vs_scipy(oracle):
+    rng = np.random.default_rng(1)
+    L = 10
+    for _ in range(20):
+        mu = rng.uniform(-0.999, 0.999)
+        phi = rng.uniform(-PI, PI)
+        R = oracle.real_sh(L, mu, phi)
+        for l in range(L + 1):
+            for m in range(-l, l + 1):
+                assert abs(R[l * l + l + m] - _scipy_rlm(l, m, mu, phi)) < 1e-12
+
+
+def test_real_sh_addition_theorem(oracle):
+    """sum_m R_lm(a) R_lm(b) = P_l(a . b) with the polar axis along x and phi = atan(xi, eta)."""
+    rng = np.random.default_rng(2)
+    L = 8
+    for _ in range(10):
+        v = rng.standard_normal((2, 3))
+        v /= np.linalg.norm(v, axis=1)[:, None]
+        Ra = oracle.real_sh(L, v[0, 0], math.atan2(v[0, 2], v[0, 1]))
+        Rb = oracle.real_sh(L, v[1, 0], math.atan2(v[1, 2], v[1, 1]))
+        c = float(v[0] @ v[1])
+        for l in range(L + 1):
+            s = sum(Ra[l * l + l + m] * Rb[l * l + l + m] for m in range(-l, l + 1))
+            assert abs(s - special.eval_legendre(l, c)) < 1e-12
+
+
+def test_half_range_sh_orthonormal(oracle):
+    """Half-range harmonics are orthonormal on mu in [0,1], phi in [0, 2pi)."""
+    L = 4
+    x, wx = np.polynomial.legendre.leggauss(24)
+    mus = 0.5 * (x + 1)
+    wm = 0.5 * wx
+    nphi = 32
+    phis = 2 * PI * (np.arange(nphi) + 0.5) / nphi
+    G = np.zeros(((L + 1) ** 2, (L + 1) ** 2))
+    for m_, a in zip(mus, wm):
+        for ph in phis:
+            R = oracle.real_sh(L, m_, ph, half_range=True)
+            G += a * (2 * PI / nphi) * np.outer(R, R)
+    assert np.abs(G - np.eye(len(G))).max() < 1e-12
+
+
+@pytest.mark.parametrize("qt,N,nd,Q,L", [("gauss-legendre", 16, 1, 1, 7), ("gauss-legendre-chebychev", 8, 2, 3, 5),
+                                          ("carlson", 12, 3, 3, 3), ("gauss-legendre-chebychev", 4, 2, 3, 2)])
+def test_galerkin_basis_identities(oracle, qt, N, nd, Q, L):
+    mu, eta, xi, w = oracle.quadrature(N, qt, nd, Q)
+    Np, M, D, pl, pm = oracle.basis(mu, eta, xi, w, L, "galerkin-d", Q)
+    assert Np == len(w)
+    tol = 1e-13 * np.linalg.cond(D)   # the inverse is exact to round-off times cond(D)
+    assert np.abs(D @ M - np.eye(Np)).max() < tol
+    assert np.abs(D[0] - w).max() < 1e-15          # R_00 = 1, so the first row of D is w
+    assert pl[0] == 0 and np.all(np.diff(pl) >= 0)
+    # M D = I as well (square Galerkin)
+    assert np.abs(M @ D - np.eye(Np)).max() < tol
+
+
+def test_galerkin_1d_closed_form(oracle):
+    """For Gauss-Legendre, Galerkin-D gives M[n, l] = (2l+1)/2 P_l(mu_n) (discrete orthogonality)."""
+    mu, eta, xi, w = oracle.quadrature(16, "gauss-legendre", 1, 1)
+    Np, M, D, pl, _ = oracle.basis(mu, eta, xi, w, 7, "galerkin-d", 1)
+    ref = np.array([[(2 * l + 1) / 2 * special.eval_legendre(l, m) for l in range(Np)] for m in mu])
+    assert np.abs(M - ref).max() < 1e-11
+
+
+@pytest.mark.parametrize("qt,N,nd,Q,L", [("gauss-legendre", 16, 1, 1, 7), ("gauss-legendre-chebychev", 8, 2, 3, 5),
+                                          ("carlson", 12, 3, 3, 3)])
+def test_host_setup_matches_oracle(oracle, qt, N, nd, Q, L):
+    """The product's numpy setup (radiant_hip.angular) and the oracle's C++ setup agree."""
+    from radiant_hip import angular
+    A = angular.AngularSet(qt, N, nd, Q if nd == 3 or qt == "gauss-legendre" else 0, L, "galerkin-d")
+    mu, eta, xi, w = oracle.quadrature(N, qt, nd, A.qdims)
+    assert np.abs(A.mu - mu).max() < 1e-15 and np.abs(A.w - w).max() < 1e-15
+    Np, M, D, pl, pm = oracle.basis(mu, eta, xi, w, L, "galerkin-d", A.qdims)
+    assert np.array_equal(pl, A.pl) and np.array_equal(pm, A.pm)
+    assert rel_err(A.D, D) < 1e-13
+    cond = np.linalg.cond(D)
+    assert rel_err(A.M, M) < 1e-15 * cond * 100
+    for f in range(2 * nd):
+        nps, Ms, Ds, n2p = oracle.surface_basis(mu, eta, xi, w, L, "galerkin-d", A.qdims, f)
+        assert nps == A.faces[f][0] and np.array_equal(n2p, A.faces[f][3])
+        assert rel_err(A.faces[f][2], Ds) < 1e-13
+
+
+def _pure_absorber_1d(N, width=4.0, sigt=1.3, q=1.0):
+    from radiant_hip import problems as P
+    x, dx = P._centres(N, width)
+    prob = P.Problem(ndims=1, Nx=N, Ny=1, Nz=1, dx=dx, dy=np.ones(1), dz=np.ones(1),
+                     mat=np.ones((N, 1, 1), np.int64), Nmat=1, G=1, sigt=np.array([[sigt]]),
+                     sigs=np.zeros((1, 1, 1, 1)), Lxs=0, qvol=np.full((1, N), q), bc=np.zeros(2, np.int32),
+                     quad_type="gauss-legendre", quad_order=2, qdims=1, legendre_order=0, tol=1e-13)
+    return prob, x
+
+
+def test_pure_absorber_second_order(oracle):
+    """Uniform source in a pure absorber: psi(x) = q/St (1 - exp(-St s/|mu|)) exactly; the diamond
+    difference error must fall ~4x per mesh halving (second order) at the cell centres (S2 keeps
+    every cell optically thin, the asymptotic regime of the scheme)."""
+    errs = []
+    for N in (40, 80, 160):
+        prob, x = _pure_absorber_1d(N)
+        phi, st = oracle.solve(prob)
+        mu, _, _, w = oracle.quadrature(2, "gauss-legendre", 1, 1)
+        L = 4.0
+        exact = np.zeros(N)
+        for m, ww in zip(mu, w):
+            s = x if m > 0 else L - x
+            # the isotropic source moment q enters each direction as M[n,0] q = q/2 (sum w = 2)
+            exact += ww * (0.5 / 1.3) * (1 - np.exp(-1.3 * s / abs(m)))
+        errs.append(np.abs(phi[0, 0] - exact).max())
+    assert errs[0] / errs[1] > 3.5 and errs[1] / errs[2] > 3.5
+
+
+def test_particle_balance_1d(oracle):
+    """DD is conservative: sum_c dx (St phi0 - Ss0 phi0 - Q0) + net leakage = 0 per group.  The exit
+    currents come from the half-range boundary moments of a final pass on the converged state:
+    b0 = sum w psi_out, b1 = sum w sqrt(3)(2 mu - 1) psi_out  =>  sum w mu psi_out = (b1/sqrt3 + b0)/2."""
+    from radiant_hip import problems as P
+    prob = P.config0()
+    prob.tol = 1e-13
+    phi, st = oracle.solve(prob)
+    nd, Np, nps, S = oracle.sizes(prob)
+    Nc = prob.Nc
+    dx = prob.dx
+    for g in range(prob.G):
+        # out-of-group source of group g from the converged upstream groups
+        q0 = prob.qvol[g].copy()
+        for gi in range(g):
+            q0 += prob.sigs[0, gi, g, 0] * phi[gi, 0]
+        z = np.zeros(S)
+        z[:Np * Nc] = phi[g].reshape(-1, order="F")
+        qlout = np.zeros(Np * Nc)
+        for gi in range(g):
+            for p in range(Np):
+                l = min(p, prob.Lxs)
+                if p <= prob.Lxs:
+                    qlout[p::Np] += prob.sigs[0, gi, g, p] * phi[gi, p]
+        qlout[0::Np] += prob.qvol[g]
+        out = oracle.transport_pass(prob, g, False, z, qlout)
+        b = out[Np * Nc:].reshape(2, nps)   # [face][p]: X- then X+
+        leak = sum((b[f, 1] / math.sqrt(3) + b[f, 0]) / 2 for f in range(2))
+        absorb = np.sum(dx * (prob.sigt[g, 0] - prob.sigs[0, g, g, 0]) * phi[g, 0] - dx * q0)
+        scale = np.sum(dx * q0)
+        assert abs(absorb + leak) / scale < 1e-11, (g, absorb, leak)
+
+
+def test_mirror_symmetry_2d(oracle):
+    """Mirroring material map and source along x mirrors the scalar flux."""
+    from radiant_hip import problems as P
+    prob = P.small_problem(2, N=7, G=2, tol=1e-13)
+    phi, _ = oracle.solve(prob)
+    m = prob.mat[::-1, :, :].copy()
+    q = prob.qvol.reshape(prob.G, prob.Nx, prob.Ny, order="F")[:, ::-1, :].reshape(prob.G, -1, order="F")
+    prob2 = P.Problem(**{**prob.__dict__, "mat": m, "qvol": q.copy()})
+    phi2, _ = oracle.solve(prob2)
+    a = phi[:, 0].reshape(prob.G, prob.Nx, prob.Ny, order="F")
+    b = phi2[:, 0].reshape(prob.G, prob.Nx, prob.Ny, order="F")[:, ::-1, :]
+    assert rel_err(a, b) < 1e-12
+
+
+def _dense_solution(oracle, prob, g, qlout):
+    nd, Np, nps, S = oracle.sizes(prob)
+    c = oracle.transport_pass(prob, g, False, np.zeros(S), qlout)
+    A = np.zeros((S, S))
+    for j in range(S):
+        e = np.zeros(S)
+        e[j] = 1.0
+        A[:, j] = oracle.transport_pass(prob, g, True, e, qlout)
+    z = np.linalg.solve(np.eye(S) - A, c)
+    return oracle.transport_pass(prob, g, False, z, qlout)  # reconstruction T(z)
+
+
+@pytest.mark.parametrize("accel", ["gmres", "anderson", "livolant", "none", "bicgstab"])
+@pytest.mark.parametrize("ndims", [1, 2])
+def test_solvers_vs_dense_direct(oracle, accel, ndims):
+    """Every in-group solver converges to the solution of the dense system (I - A) z = c."""
+    from radiant_hip import problems as P
+    prob = P.small_problem(ndims, N=4, G=1, L=1, quad_order=4 if ndims == 1 else 2, tol=1e-13, accel=accel)
+    prob.maxit = 5000
+    phi, st = oracle.solve(prob)
+    nd, Np, nps, S = oracle.sizes(prob)
+    qlout = np.zeros(Np * prob.Nc)
+    qlout[0::Np] = prob.qvol[0]
+    ref = _dense_solution(oracle, prob, 0, qlout)
+    assert rel_err(phi[0].reshape(-1, order="F"), ref[:Np * prob.Nc]) < 1e-10
+    assert st[0]["converged"]
+
+
+def test_pass_is_affine(oracle):
+    """T(z) = A z + c with A linear: T(a z1 + b z2) = a T(z1) + b T(z2) + (1 - a - b) T(0)."""
+    from radiant_hip import problems as P
+    prob = P.small_problem(3, N=3, G=1, L=1)
+    nd, Np, nps, S = oracle.sizes(prob)
+    rng = np.random.default_rng(4)
+    z1, z2 = rng.standard_normal(S), rng.standard_normal(S)
+    q = np.zeros(Np * prob.Nc)
+    q[0::Np] = 1.0
+    T = lambda z: oracle.transport_pass(prob, 0, False, z, q)
+    lhs = T(0.3 * z1 + 0.5 * z2)
+    rhs = 0.3 * T(z1) + 0.5 * T(z2) + 0.2 * T(np.zeros(S))
+    assert rel_err(lhs, rhs) < 1e-13
+
+
+def test_zero_source_group_shortcut(oracle):
+    from radiant_hip import problems as P
+    prob = P.small_problem(1, N=6, G=3, source_group=1)
+    phi, st = oracle.solve(prob)
+    assert np.all(phi[0] == 0) and st[0]["passes"] == 0 and st[0]["converged"]
+    assert np.abs(phi[1]).max() > 0 and np.abs(phi[2]).max() > 0
+
+
+def test_openmp_variant_matches_serial(oracle):
+    from radiant_hip import problems as P
+    prob = P.small_problem(2, N=8, G=2, tol=1e-13)
+    a, _ = oracle.solve(prob, nthreads=1)
+    b, _ = oracle.solve(prob, nthreads=4)
+    assert rel_err(b, a) < 1e-12
