@@ -149,13 +149,14 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
             for (int g = 0; g < G; ++g) qv[(size_t)g * Nc + c] = d.qvol[g + (size_t)G * c];
         d_qvol = dalloc<double>(qv.size()); upload(d_qvol, qv, st);
         // ---- angular operators (row-major device copies)
+        // k-major device copies: MT[a][n] = M[n, act a], DT[n][p] = D[p, n], DTact[n][a] = D[act a, n]
         std::vector<double> Mact((size_t)Nd * Nact), D((size_t)Np * Nd), Dact((size_t)Nact * Nd);
-        for (int n = 0; n < Nd; ++n)
-            for (int a = 0; a < Nact; ++a) Mact[(size_t)n * Nact + a] = d.M[n + (size_t)Nd * act[a]];
-        for (int p = 0; p < Np; ++p)
-            for (int n = 0; n < Nd; ++n) D[(size_t)p * Nd + n] = d.D[p + (size_t)Np * n];
         for (int a = 0; a < Nact; ++a)
-            for (int n = 0; n < Nd; ++n) Dact[(size_t)a * Nd + n] = d.D[act[a] + (size_t)Np * n];
+            for (int n = 0; n < Nd; ++n) Mact[(size_t)a * Nd + n] = d.M[n + (size_t)Nd * act[a]];
+        for (int n = 0; n < Nd; ++n)
+            for (int p = 0; p < Np; ++p) D[(size_t)n * Np + p] = d.D[p + (size_t)Np * n];
+        for (int n = 0; n < Nd; ++n)
+            for (int a = 0; a < Nact; ++a) Dact[(size_t)n * Nact + a] = d.D[act[a] + (size_t)Np * n];
         d_Mact64 = dalloc<double>(Mact.size()); upload(d_Mact64, Mact, st);
         d_D64 = dalloc<double>(D.size()); upload(d_D64, D, st);
         d_Dact64 = dalloc<double>(Dact.size()); upload(d_Dact64, Dact, st);
@@ -181,6 +182,16 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
                 }
                 d_Dexit[ax] = dalloc<double>(Dex.size()); upload(d_Dexit[ax], Dex, st);
                 d_exitf[ax] = dalloc<int>(Nd); upload(d_exitf[ax], ef, st);
+                // directions leaving through each face of the axis, in index order (the order in
+                // which sn_inner_pass.jl:322-329 accumulates them)
+                std::vector<int> lst(2 * (size_t)Nd, 0);
+                for (int f = 0; f < 2; ++f) {
+                    int c = 0;
+                    for (int n = 0; n < Nd; ++n)
+                        if (ef[n] == f) lst[(size_t)f * Nd + c++] = n;
+                    excnt[ax][f] = c;
+                }
+                d_exlist[ax] = dalloc<int>(lst.size()); upload(d_exlist[ax], lst, st);
             }
         }
         // ---- fluxes and work buffers
@@ -402,6 +413,9 @@ void Problem::pass_faithful(int g, int prec, int mode, const double* zin, double
     }
     pd.Dexit_x = d_Dexit[0]; pd.Dexit_y = d_Dexit[1]; pd.Dexit_z = d_Dexit[2];
     pd.exitf_x = d_exitf[0]; pd.exitf_y = d_exitf[1]; pd.exitf_z = d_exitf[2];
+    pd.exlist_x = d_exlist[0]; pd.exlist_y = d_exlist[1]; pd.exlist_z = d_exlist[2];
+    for (int ax = 0; ax < 3; ++ax)
+        for (int f = 0; f < 2; ++f) pd.excnt[ax][f] = excnt[ax][f];
     if (nb == 0) pd.bout_x = nullptr;
     (void)prec;
     launch_pass(pd, P64, true);
@@ -922,10 +936,13 @@ void test_contraction(int which, int prec, int M, int K, int N, const double* A,
     try {
         const size_t es = prec == P64 ? 8 : 4;
         void* dA = al((size_t)M * K * es);
+        std::vector<double> At((size_t)M * K);  // operators are stored k-major on the device
+        for (int i = 0; i < M; ++i)
+            for (int k = 0; k < K; ++k) At[(size_t)k * M + i] = A[(size_t)i * K + k];
         if (prec == P64) {
-            RT_CHECK(hipMemcpy(dA, A, (size_t)M * K * 8, hipMemcpyHostToDevice));
+            RT_CHECK(hipMemcpy(dA, At.data(), (size_t)M * K * 8, hipMemcpyHostToDevice));
         } else {
-            std::vector<float> f(A, A + (size_t)M * K);
+            std::vector<float> f(At.begin(), At.end());
             RT_CHECK(hipMemcpy(dA, f.data(), f.size() * 4, hipMemcpyHostToDevice));
         }
         double* dC = (double*)al((size_t)M * N * 8);

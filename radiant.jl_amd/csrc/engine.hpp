@@ -97,6 +97,8 @@ private:
     int* d_act = nullptr;
     double* d_Dexit[3] = {nullptr, nullptr, nullptr};
     int* d_exitf[3] = {nullptr, nullptr, nullptr};
+    int* d_exlist[3] = {nullptr, nullptr, nullptr};
+    int excnt[3][2] = {{0, 0}, {0, 0}, {0, 0}};
     double* d_qlout = nullptr;
     void* d_psi = nullptr;
     void* d_bex[3] = {nullptr, nullptr, nullptr};

@@ -68,97 +68,86 @@ __device__ inline short f2bf(float x) {
 }
 
 // ------------------------------------------------------------------------------------------
-// Moment -> direction contraction with the fused scattering source.
+// Contractions on the matrix cores.
+//
+//   mgemm: psi[n][c]  = sum_a MT[a][n] * Ql[a][c],  Ql = Qlout + Σs_l Φ computed on operand load
+//          (scattering_source.jl:25-30 fused into the moment->direction contraction);
+//   dgemm: out[r][c]  = sum_n DT[n][r] * psi[n][c]  (the D accumulation of the sweeps), with the
+//          I - A epilogue of the GMRES matvec (sn_one_speed.jl:147): out = zin - D psi.
+// The operators are stored k-major (MT[K][M]) so that each MFMA operand load is a contiguous
+// 16-element row segment.  A wave computes RT x CT tiles of 16x16 (RT*16 rows, CT*16 cells):
+// every A fragment is reused CT times and every B fragment RT times from registers.
 // ------------------------------------------------------------------------------------------
-template <typename T, int RT>
-__global__ __launch_bounds__(256) void mgemm_kernel(const PassDesc* __restrict__ pds, const GemmTask* __restrict__ tasks) {
+template <typename T, int RT, int CT, bool MG>
+__global__ __launch_bounds__(256) void contract_kernel(const PassDesc* __restrict__ pds, const GemmTask* __restrict__ tasks) {
     const GemmTask tk = tasks[blockIdx.x];
     const PassDesc& d = pds[tk.prob];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int Nc = d.Nc, K = d.Nact, M = d.Nd;
-    const int col = tk.tile * GEMM_CELLS + wave * 16 + (lane & 15);
-    const int kk = lane >> 4;
-    const int row0 = tk.rgroup * GEMM_ROWS;
-    const T* __restrict__ A = (const T*)d.Mact;
-    typename Acc<T>::type acc[RT];
+    const int Nc = d.Nc;
+    const int K = MG ? d.Nact : d.Nd;
+    const int M = MG ? d.Nd : d.Nrows;
+    const T* __restrict__ At = (const T*)(MG ? d.Mact : d.Drows);
+    const int c0 = tk.tile * GEMM_CELLS + wave * (CT * 16);
+    const int r0 = tk.rgroup * GEMM_ROWS;
+    const int kk = lane >> 4, li = lane & 15;
+    typename Acc<T>::type acc[RT][CT];
 #pragma unroll
-    for (int t = 0; t < RT; ++t) acc[t] = (typename Acc<T>::type)(0);
-    const bool cin = col < Nc;
-    const int m = cin ? d.mat[col] : 0;
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+        for (int u = 0; u < CT; ++u) acc[t][u] = (typename Acc<T>::type)(0);
+    int mc[CT];
+    if (MG) {
+#pragma unroll
+        for (int u = 0; u < CT; ++u) {
+            const int col = c0 + 16 * u + li;
+            mc[u] = col < Nc ? gld(d.mat + col) : 0;
+        }
+    }
     for (int k0 = 0; k0 < K; k0 += 4) {
         const int k = k0 + kk;
-        T b = 0;
-        if (k < K && cin) {
-            const int r = d.phi_rows ? d.phi_rows[k] : k;
-            double q = d.qlout ? d.qlout[(size_t)k * Nc + col] : 0.0;
-            if (d.phi_in) q += d.sigw[m * K + k] * d.phi_in[(size_t)r * Nc + col];
-            b = (T)q;
-        }
+        T b[CT], a[RT];
 #pragma unroll
-        for (int t = 0; t < RT; ++t) {
-            const int rt0 = row0 + t * 16;
-            if (rt0 < M) {  // wave-uniform
-                const int row = rt0 + (lane & 15);
-                const T a = (k < K && row < M) ? A[(size_t)row * K + k] : (T)0;
-                acc[t] = mfma4(a, b, acc[t]);
+        for (int u = 0; u < CT; ++u) {
+            const int col = c0 + 16 * u + li;
+            b[u] = (T)0;
+            if (k < K && col < Nc) {
+                if (MG) {
+                    const int r = d.phi_rows ? d.phi_rows[k] : k;
+                    double q = d.qlout ? gld(d.qlout + (size_t)k * Nc + col) : 0.0;
+                    if (d.phi_in) q += d.sigw[mc[u] * K + k] * gld(d.phi_in + (size_t)r * Nc + col);
+                    b[u] = (T)q;
+                } else {
+                    b[u] = gld((const T*)d.psi + (size_t)k * Nc + col);
+                }
             }
         }
-    }
-    T* __restrict__ psi = (T*)d.psi;
-    const int ocol = tk.tile * GEMM_CELLS + wave * 16 + (lane & 15);
-    if (ocol < Nc) {
 #pragma unroll
         for (int t = 0; t < RT; ++t) {
+            const int row = r0 + 16 * t + li;
+            a[t] = (k < K && row < M) ? gld(At + (size_t)k * M + row) : (T)0;
+        }
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+            if (r0 + 16 * t < M)  // wave-uniform
+#pragma unroll
+                for (int u = 0; u < CT; ++u) acc[t][u] = mfma4(a[t], b[u], acc[t][u]);
+    }
+#pragma unroll
+    for (int t = 0; t < RT; ++t) {
+#pragma unroll
+        for (int u = 0; u < CT; ++u) {
+            const int col = c0 + 16 * u + li;
+            if (col >= Nc) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = row0 + t * 16 + acc_row<T>(lane, r);
-                if (row < M) psi[(size_t)row * Nc + ocol] = acc[t][r];
-            }
-        }
-    }
-}
-
-// ------------------------------------------------------------------------------------------
-// Direction -> moment contraction, optional I - A epilogue.
-// ------------------------------------------------------------------------------------------
-template <typename T, int RT>
-__global__ __launch_bounds__(256) void dgemm_kernel(const PassDesc* __restrict__ pds, const GemmTask* __restrict__ tasks) {
-    const GemmTask tk = tasks[blockIdx.x];
-    const PassDesc& d = pds[tk.prob];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int Nc = d.Nc, K = d.Nd, M = d.Nrows;
-    const int col = tk.tile * GEMM_CELLS + wave * 16 + (lane & 15);
-    const int kk = lane >> 4;
-    const int row0 = tk.rgroup * GEMM_ROWS;
-    const T* __restrict__ A = (const T*)d.Drows;
-    const T* __restrict__ B = (const T*)d.psi;
-    typename Acc<T>::type acc[RT];
-#pragma unroll
-    for (int t = 0; t < RT; ++t) acc[t] = (typename Acc<T>::type)(0);
-    const bool cin = col < Nc;
-    for (int k0 = 0; k0 < K; k0 += 4) {
-        const int k = k0 + kk;
-        const T b = (k < K && cin) ? B[(size_t)k * Nc + col] : (T)0;
-#pragma unroll
-        for (int t = 0; t < RT; ++t) {
-            const int rt0 = row0 + t * 16;
-            if (rt0 < M) {
-                const int row = rt0 + (lane & 15);
-                const T a = (k < K && row < M) ? A[(size_t)row * K + k] : (T)0;
-                acc[t] = mfma4(a, b, acc[t]);
-            }
-        }
-    }
-    if (cin) {
-#pragma unroll
-        for (int t = 0; t < RT; ++t) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = row0 + t * 16 + acc_row<T>(lane, r);
-                if (row < M) {
-                    double v = (double)acc[t][r];
-                    if (d.zin) v = d.zin[(size_t)row * Nc + col] - v;
-                    d.out[(size_t)row * Nc + col] = v;
+                const int row = r0 + 16 * t + acc_row<T>(lane, r);
+                if (row >= M) continue;
+                if (MG) {
+                    gst((T*)d.psi + (size_t)row * Nc + col, (T)acc[t][u][r]);
+                } else {
+                    double v = (double)acc[t][u][r];
+                    if (d.zin) v = gld(d.zin + (size_t)row * Nc + col) - v;
+                    gst(d.out + (size_t)row * Nc + col, v);
                 }
             }
         }
@@ -166,112 +155,86 @@ __global__ __launch_bounds__(256) void dgemm_kernel(const PassDesc* __restrict__
 }
 
 // BF16 operands, FP32 accumulation (v_mfma_f32_16x16x16_bf16): A[16x16] lane l holds
-// A[l%16][4*(l/16)+j], B[16x16] lane l holds B[4*(l/16)+j][l%16], j = 0..3.
-template <int RT>
-__global__ __launch_bounds__(256) void dgemm_bf16_kernel(const PassDesc* __restrict__ pds, const GemmTask* __restrict__ tasks) {
+// A[l%16][4*(l/16)+j], B[16x16] lane l holds B[4*(l/16)+j][l%16], j = 0..3; C as the f32 case.
+template <int RT, int CT, bool MG>
+__global__ __launch_bounds__(256) void contract_bf16_kernel(const PassDesc* __restrict__ pds, const GemmTask* __restrict__ tasks) {
     const GemmTask tk = tasks[blockIdx.x];
     const PassDesc& d = pds[tk.prob];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int Nc = d.Nc, K = d.Nd, M = d.Nrows;
-    const int col = tk.tile * GEMM_CELLS + wave * 16 + (lane & 15);
-    const int kq = 4 * (lane >> 4);
-    const int row0 = tk.rgroup * GEMM_ROWS;
-    const float* __restrict__ A = (const float*)d.Drows;
-    const float* __restrict__ B = (const float*)d.psi;
-    v4f acc[RT];
+    const int Nc = d.Nc;
+    const int K = MG ? d.Nact : d.Nd;
+    const int M = MG ? d.Nd : d.Nrows;
+    const float* __restrict__ At = (const float*)(MG ? d.Mact : d.Drows);
+    const int c0 = tk.tile * GEMM_CELLS + wave * (CT * 16);
+    const int r0 = tk.rgroup * GEMM_ROWS;
+    const int kq = 4 * (lane >> 4), li = lane & 15;
+    v4f acc[RT][CT];
 #pragma unroll
-    for (int t = 0; t < RT; ++t) acc[t] = (v4f)(0.0f);
-    const bool cin = col < Nc;
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        v4s b;
+    for (int t = 0; t < RT; ++t)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int k = k0 + kq + j;
-            b[j] = f2bf((k < K && cin) ? B[(size_t)k * Nc + col] : 0.0f);
-        }
+        for (int u = 0; u < CT; ++u) acc[t][u] = (v4f)(0.0f);
+    int mc[CT];
+    if (MG) {
 #pragma unroll
-        for (int t = 0; t < RT; ++t) {
-            const int rt0 = row0 + t * 16;
-            if (rt0 < M) {
-                const int row = rt0 + (lane & 15);
-                v4s a;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int k = k0 + kq + j;
-                    a[j] = f2bf((k < K && row < M) ? A[(size_t)row * K + k] : 0.0f);
-                }
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
-            }
+        for (int u = 0; u < CT; ++u) {
+            const int col = c0 + 16 * u + li;
+            mc[u] = col < Nc ? gld(d.mat + col) : 0;
         }
     }
-    if (cin) {
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        v4s b[CT], a[RT];
+#pragma unroll
+        for (int u = 0; u < CT; ++u) {
+            const int col = c0 + 16 * u + li;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int k = k0 + kq + j;
+                float v = 0.0f;
+                if (k < K && col < Nc) {
+                    if (MG) {
+                        const int r = d.phi_rows ? d.phi_rows[k] : k;
+                        double q = d.qlout ? gld(d.qlout + (size_t)k * Nc + col) : 0.0;
+                        if (d.phi_in) q += d.sigw[mc[u] * K + k] * gld(d.phi_in + (size_t)r * Nc + col);
+                        v = (float)q;
+                    } else {
+                        v = gld((const float*)d.psi + (size_t)k * Nc + col);
+                    }
+                }
+                b[u][j] = f2bf(v);
+            }
+        }
 #pragma unroll
         for (int t = 0; t < RT; ++t) {
+            const int row = r0 + 16 * t + li;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int k = k0 + kq + j;
+                a[t][j] = f2bf((k < K && row < M) ? gld(At + (size_t)k * M + row) : 0.0f);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+            if (r0 + 16 * t < M)
+#pragma unroll
+                for (int u = 0; u < CT; ++u) acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], b[u], acc[t][u], 0, 0, 0);
+    }
+#pragma unroll
+    for (int t = 0; t < RT; ++t) {
+#pragma unroll
+        for (int u = 0; u < CT; ++u) {
+            const int col = c0 + 16 * u + li;
+            if (col >= Nc) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int row = row0 + t * 16 + 4 * (lane >> 4) + r;
-                if (row < M) {
-                    double v = (double)acc[t][r];
-                    if (d.zin) v = d.zin[(size_t)row * Nc + col] - v;
-                    d.out[(size_t)row * Nc + col] = v;
+                const int row = r0 + 16 * t + 4 * (lane >> 4) + r;
+                if (row >= M) continue;
+                if (MG) {
+                    gst((float*)d.psi + (size_t)row * Nc + col, acc[t][u][r]);
+                } else {
+                    double v = (double)acc[t][u][r];
+                    if (d.zin) v = gld(d.zin + (size_t)row * Nc + col) - v;
+                    gst(d.out + (size_t)row * Nc + col, v);
                 }
-            }
-        }
-    }
-}
-
-template <int RT>
-__global__ __launch_bounds__(256) void mgemm_bf16_kernel(const PassDesc* __restrict__ pds, const GemmTask* __restrict__ tasks) {
-    const GemmTask tk = tasks[blockIdx.x];
-    const PassDesc& d = pds[tk.prob];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int Nc = d.Nc, K = d.Nact, M = d.Nd;
-    const int col = tk.tile * GEMM_CELLS + wave * 16 + (lane & 15);
-    const int kq = 4 * (lane >> 4);
-    const int row0 = tk.rgroup * GEMM_ROWS;
-    const float* __restrict__ A = (const float*)d.Mact;
-    v4f acc[RT];
-#pragma unroll
-    for (int t = 0; t < RT; ++t) acc[t] = (v4f)(0.0f);
-    const bool cin = col < Nc;
-    const int m = cin ? d.mat[col] : 0;
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        v4s b;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int k = k0 + kq + j;
-            float v = 0.0f;
-            if (k < K && cin) {
-                const int r = d.phi_rows ? d.phi_rows[k] : k;
-                double q = d.qlout ? d.qlout[(size_t)k * Nc + col] : 0.0;
-                if (d.phi_in) q += d.sigw[m * K + k] * d.phi_in[(size_t)r * Nc + col];
-                v = (float)q;
-            }
-            b[j] = f2bf(v);
-        }
-#pragma unroll
-        for (int t = 0; t < RT; ++t) {
-            const int rt0 = row0 + t * 16;
-            if (rt0 < M) {
-                const int row = rt0 + (lane & 15);
-                v4s a;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int k = k0 + kq + j;
-                    a[j] = f2bf((k < K && row < M) ? A[(size_t)row * K + k] : 0.0f);
-                }
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
-            }
-        }
-    }
-    float* __restrict__ psi = (float*)d.psi;
-    if (cin) {
-#pragma unroll
-        for (int t = 0; t < RT; ++t) {
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = row0 + t * 16 + 4 * (lane >> 4) + r;
-                if (row < M) psi[(size_t)row * Nc + col] = acc[t][r];
             }
         }
     }
@@ -787,18 +750,18 @@ __global__ __launch_bounds__(256) void bndmom_kernel(const PassDesc* __restrict_
             const int f = (int)((j / Nps) % 2);
             const size_t pos = j / ((size_t)2 * Nps);
             const double* Dex = ax == 0 ? d.Dexit_x : (ax == 1 ? d.Dexit_y : d.Dexit_z);
-            const int* ef = ax == 0 ? d.exitf_x : (ax == 1 ? d.exitf_y : d.exitf_z);
-            const char* bsrc = (const char*)(ax == 0 ? d.bx : (ax == 1 ? d.by : d.bz));
+            const int* lst = (ax == 0 ? d.exlist_x : (ax == 1 ? d.exlist_y : d.exlist_z)) + f * Nd;
+            const int cnt = d.excnt[ax][f];
+            const double* bsrc = (const double*)(ax == 0 ? d.bx : (ax == 1 ? d.by : d.bz));
             double v = 0.0;
-            for (int n = 0; n < Nd; ++n) {
-                if (ef[n] != f) continue;
-                const double dn = Dex[(size_t)n * Nps + p];
-                const double b = ((const double*)bsrc)[(size_t)n * npos[ax] + pos];
-                v += dn * b;
+#pragma unroll 8
+            for (int t = 0; t < cnt; ++t) {
+                const int n = lst[t];
+                v += gld(Dex + (size_t)n * Nps + p) * gld(bsrc + (size_t)n * npos[ax] + pos);
             }
             double* out = ax == 0 ? d.bout_x : (ax == 1 ? d.bout_y : d.bout_z);
             const double* zin = ax == 0 ? d.bzin_x : (ax == 1 ? d.bzin_y : d.bzin_z);
-            out[j] = zin ? zin[j] - v : v;
+            gst(out + j, zin ? gld(zin + j) - v : v);
         }
     }
 }
@@ -961,17 +924,17 @@ static inline int grid_for(size_t n, int block = 256, int cap = 2048) {
 
 void launch_mgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st) {
     if (ntasks <= 0) return;
-    if (prec == P64) hipLaunchKernelGGL((mgemm_kernel<double, 8>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
-    else if (prec == P32) hipLaunchKernelGGL((mgemm_kernel<float, 8>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
-    else hipLaunchKernelGGL((mgemm_bf16_kernel<8>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    if (prec == P64) hipLaunchKernelGGL((contract_kernel<double, GEMM_RT, GEMM_CT, true>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    else if (prec == P32) hipLaunchKernelGGL((contract_kernel<float, GEMM_RT, GEMM_CT, true>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    else hipLaunchKernelGGL((contract_bf16_kernel<GEMM_RT, GEMM_CT, true>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
     RT_HIP(hipGetLastError());
 }
 
 void launch_dgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st) {
     if (ntasks <= 0) return;
-    if (prec == P64) hipLaunchKernelGGL((dgemm_kernel<double, 8>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
-    else if (prec == P32) hipLaunchKernelGGL((dgemm_kernel<float, 8>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
-    else hipLaunchKernelGGL((dgemm_bf16_kernel<8>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    if (prec == P64) hipLaunchKernelGGL((contract_kernel<double, GEMM_RT, GEMM_CT, false>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    else if (prec == P32) hipLaunchKernelGGL((contract_kernel<float, GEMM_RT, GEMM_CT, false>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    else hipLaunchKernelGGL((contract_bf16_kernel<GEMM_RT, GEMM_CT, false>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
     RT_HIP(hipGetLastError());
 }
 

@@ -16,7 +16,7 @@ struct PassDesc {
     const double* sigt;           // Σt of the current group [Nmat]
     // moment -> direction contraction (with the scattering source fused in the operand load)
     int Nact;                     // moments that can carry a source (Legendre order <= XS order)
-    const void* Mact;             // [Nd][Nact] compute type
+    const void* Mact;             // M restricted to the active moments, k-major: MT[Nact][Nd], compute type
     const double* sigw;           // within-group Σs_l of the current group [Nmat][Nact]
     const double* qlout;          // out-of-group + volume source [Nact][Nc]; null = homogeneous pass
     const double* phi_in;         // moments of the input state, row r at phi_in + r*Nc (null = zero state)
@@ -26,7 +26,7 @@ struct PassDesc {
     void *bx, *by, *bz;           // [Nd][Ny*Nz], [Nd][Nx*Nz], [Nd][Nx*Ny] compute type
     // direction -> moment contraction
     int Nrows;                    // rows of Drows
-    const void* Drows;            // [Nrows][Nd] compute type
+    const void* Drows;            // D rows, k-major: DT[Nd][Nrows], compute type
     double* out;                  // [Nrows][Nc]
     const double* zin;            // null: out = D psi ; else out = zin - D psi (the I - A matvec)
     // boundary moments (faithful state): [Nps][2][pos] per axis, null = skip
@@ -35,6 +35,8 @@ struct PassDesc {
     const double *bzin_x, *bzin_y, *bzin_z;  // I - A epilogue inputs (null = plain)
     const double *Dexit_x, *Dexit_y, *Dexit_z;  // [Nd][Nps] half-range D column of the exit face
     const int *exitf_x, *exitf_y, *exitf_z;      // [Nd] exit face (0 = minus, 1 = plus)
+    const int *exlist_x, *exlist_y, *exlist_z;   // [2][Nd] directions leaving through face f, in index order
+    int excnt[3][2];                             // lengths of those lists
 };
 
 enum Prec { P64 = 0, P32 = 1, PBF16 = 2 };
@@ -70,8 +72,10 @@ void launch_dgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int nta
 void launch_bndmom(const PassDesc* d_pd, int nprob, int max_items, hipStream_t st);
 
 // Row groups of the contraction kernels (rows per work tile).
-constexpr int GEMM_ROWS = 128;   // 8 MFMA row tiles of 16
-constexpr int GEMM_CELLS = 64;   // 4 waves x 16 cells
+constexpr int GEMM_RT = 4;                      // 16-row MFMA tiles per wave
+constexpr int GEMM_CT = 4;                      // 16-cell MFMA tiles per wave
+constexpr int GEMM_ROWS = 16 * GEMM_RT;         // rows per work tile
+constexpr int GEMM_CELLS = 4 * 16 * GEMM_CT;    // cells per work tile (4 waves)
 
 // Out-of-group + volume source of group g (sn_flux.jl:175-179):
 //   qlout[a][c] = sum_{gi<g} sigo[g][gi][mat c][a] * phi[gi][act a][c] + (p_a == 0) * qvol[g][c]
