@@ -1,0 +1,425 @@
+// batch.cpp -- lockstep solver for many independent problems (mixed precision, reduced state).
+//
+// Every round, each unfinished problem performs exactly one transport pass of its current
+// phase (c = T(0), a low-precision GMRES matvec, an FP64 refinement residual or the FP64
+// reconstruction); the passes of all problems share the same kernel launches (the contraction
+// and sweep task tables carry the problem index).  The Arnoldi step is classical Gram-Schmidt
+// with one re-orthogonalisation (CGS2) in FP64, batched over problems with segmented
+// deterministic reductions.  The host reads every problem's scalars with one copy per round and
+// advances each problem's state machine (the logic of sn_flux.jl:172-205 + sn_one_speed.jl
+// with the iterative refinement of the mixed-precision mode).
+#include "batch.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <map>
+
+namespace rt {
+
+namespace {
+constexpr int SLOTS = 128;  // per problem: [0,40) h1, [40,80) h2, 80 |w|^2, 81 generic norm^2
+constexpr int S_H1 = 0, S_H2 = 40, S_WN = 80, S_NRM = 81;
+constexpr int BPI = 16;     // blocks per segmented dot
+}  // namespace
+
+BatchSolver::BatchSolver(const std::vector<Problem*>& probs, hipStream_t stream) : st(stream) {
+    const int n = (int)probs.size();
+    ps.resize(n);
+    for (int i = 0; i < n; ++i) {
+        Problem* P = probs[i];
+        if (P->precision == RT_PREC_FP64)
+            throw Error(RT_EINVAL, "the batched solver runs the mixed-precision modes");
+        if (P->scan_cpl <= 0 || sweep_scan_lds_bytes(P64, P->ndims, P->Nx, P->Ny, P->Nz, P->Nmat) > 160 * 1024)
+            throw Error(RT_EINVAL, "a batched problem does not fit the row-scan sweep");
+        PS& s = ps[i];
+        s.P = P;
+        s.m = std::min(std::max(1, P->restart), 32);
+        s.lo = (P->precision == RT_PREC_MIXED_BF16) ? PBF16 : P32;
+        s.inner_tol = (s.lo == PBF16) ? 3e-3 : 1e-5;
+        const size_t S = P->S_r;
+        for (int k = 0; k <= s.m; ++k) s.V.push_back(P->vec("bV" + std::to_string(k), S));
+        s.w = P->vec("bw", S);
+        s.c = P->vec("bc", S);
+        s.z = P->vec("bz", S);
+        s.r = P->vec("br", S);
+        s.t = P->vec("bt", S);
+        RT_CHECK(hipMemcpyAsync(P->d_Vptr, s.V.data(), (s.m + 1) * sizeof(double*), hipMemcpyHostToDevice, st));
+        s.stats.assign(P->G, rt_group_stats{});
+    }
+    auto al = [&](size_t bytes) {
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 16));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error(RT_ENOMEM, "batched solver: device allocation failed");
+        }
+        mem.push_back(p);
+        return p;
+    };
+    size_t ng = 0, nsw = 0;
+    for (auto& s : ps) {
+        const Problem* P = s.P;
+        const int ntile = (P->Nc + GEMM_CELLS - 1) / GEMM_CELLS;
+        ng = std::max(ng, (size_t)ntile * ((std::max(P->Nd, P->Np) + GEMM_ROWS - 1) / GEMM_ROWS));
+        nsw = std::max(nsw, (size_t)P->Nd);
+    }
+    cap_gemm = ng * n;
+    cap_sweep = nsw * n;
+    d_pd = (PassDesc*)al(sizeof(PassDesc) * n);
+    d_mg = (GemmTask*)al(sizeof(GemmTask) * cap_gemm);
+    d_dg = (GemmTask*)al(sizeof(GemmTask) * cap_gemm);
+    d_sw = (SweepTask*)al(sizeof(SweepTask) * cap_sweep);
+    d_slots = (double*)al(sizeof(double) * SLOTS * n);
+    d_coef = (double*)al(sizeof(double) * 64 * n);
+    const size_t max_items = 64 * (size_t)n + 8;
+    d_partial = (double*)al(sizeof(double) * BPI * max_items);
+    d_counters = (unsigned*)al(sizeof(unsigned) * max_items);
+    RT_CHECK(hipMemsetAsync(d_counters, 0, sizeof(unsigned) * max_items, st));
+    RT_CHECK(hipHostMalloc((void**)&h_slots, sizeof(double) * SLOTS * n, hipHostMallocDefault));
+    arena_cap = (sizeof(PassDesc) + 3 * 64 * sizeof(DotItem) + 16 * sizeof(VecItem) + 64 * sizeof(double) +
+                 64 * sizeof(int)) * (size_t)n + (1 << 20);
+    d_arena = (unsigned char*)al(arena_cap);
+    RT_CHECK(hipHostMalloc((void**)&h_arena, arena_cap, hipHostMallocDefault));
+    RT_CHECK(hipStreamSynchronize(st));
+}
+
+// Copies host data into the round's staging arena and returns its device address.  The copies
+// are stream-ordered; the arena is reused only after the end-of-round synchronisation.
+template <class T>
+T* BatchSolver::stage(const T* data, size_t count) {
+    const size_t bytes = count * sizeof(T);
+    const size_t off = (arena_off + 255) & ~size_t(255);
+    if (off + bytes > arena_cap) throw Error(RT_ENOMEM, "batched solver: staging arena exhausted");
+    std::memcpy(h_arena + off, data, bytes);
+    RT_CHECK(hipMemcpyAsync(d_arena + off, h_arena + off, bytes, hipMemcpyHostToDevice, st));
+    arena_off = off + bytes;
+    return (T*)(d_arena + off);
+}
+
+BatchSolver::~BatchSolver() {
+    (void)hipStreamSynchronize(st);
+    for (void* p : mem) (void)hipFree(p);
+    if (h_slots) (void)hipHostFree(h_slots);
+    if (h_arena) (void)hipHostFree(h_arena);
+}
+
+// Runs host-built item lists through the device.
+void BatchSolver::run_vecops(const std::vector<VecItem>& items) {
+    if (items.empty()) return;
+    VecItem* d = stage(items.data(), items.size());
+    long nmax = 0;
+    for (auto& it : items) nmax = std::max(nmax, it.n);
+    const int bpi = (int)std::min<long>(256, std::max<long>(1, (nmax + 2047) / 2048));
+    launch_vecops(d, (int)items.size(), bpi, st);
+}
+
+void BatchSolver::run_dots(const std::vector<DotItem>& items) {
+    if (items.empty()) return;
+    DotItem* d = stage(items.data(), items.size());
+    launch_segdot(d, (int)items.size(), BPI, d_partial, d_counters, st);
+}
+
+void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<PassDesc>& pds,
+                                const std::vector<int>& prec, const std::vector<int>& full) {
+    const int n = (int)ps.size();
+    // PassDesc of every problem (index = batch index)
+    std::vector<PassDesc> all(n);
+    for (size_t i = 0; i < who.size(); ++i) all[who[i]] = pds[i];
+    const PassDesc* dpd = stage(all.data(), all.size());
+    for (int pr : {P64, P32, PBF16}) {
+        std::vector<int> mmeta, dmeta;
+        std::map<std::pair<int, int>, std::vector<int>> sweeps;  // (cpl, waves) -> sweep meta
+        int moff = 0, doff = 0;
+        std::map<std::pair<int, int>, int> soff;
+        std::map<std::pair<int, int>, size_t> slds;
+        for (size_t i = 0; i < who.size(); ++i) {
+            if (prec[i] != pr) continue;
+            const Problem* P = ps[who[i]].P;
+            const int ntile = (P->Nc + GEMM_CELLS - 1) / GEMM_CELLS;
+            const int nrg_m = (P->Nd + GEMM_ROWS - 1) / GEMM_ROWS;
+            const int rows = full[i] ? P->Np : P->Nact;
+            const int nrg_d = (rows + GEMM_ROWS - 1) / GEMM_ROWS;
+            mmeta.insert(mmeta.end(), {who[i], ntile, nrg_m, moff});
+            moff += ntile * nrg_m;
+            dmeta.insert(dmeta.end(), {who[i], ntile, nrg_d, doff});
+            doff += ntile * nrg_d;
+            const std::pair<int, int> key(P->scan_cpl, P->scan_waves);
+            auto& v = sweeps[key];
+            v.insert(v.end(), {who[i], P->Nd, soff[key]});
+            soff[key] += P->Nd;
+            const int sp = pr == PBF16 ? P32 : pr;
+            slds[key] = std::max(slds[key], sweep_scan_lds_bytes(sp, P->ndims, P->Nx, P->Ny, P->Nz, P->Nmat));
+        }
+        if (mmeta.empty()) continue;
+        // task tables expanded on the device from the small per-problem metadata
+        const int nm = (int)mmeta.size() / 4;
+        launch_expand_gemm(stage(mmeta.data(), mmeta.size()), nm, d_mg, st);
+        launch_expand_gemm(stage(dmeta.data(), dmeta.size()), nm, d_dg, st);
+        launch_mgemm(pr, dpd, d_mg, moff, st);
+        SweepTask* swbase = d_sw;
+        for (auto& kv : sweeps) {
+            const std::vector<int>& sm = kv.second;
+            launch_expand_sweep(stage(sm.data(), sm.size()), (int)sm.size() / 3, swbase, st);
+            const int ntask = soff[kv.first];
+            launch_sweep_scan(pr == PBF16 ? P32 : pr, dpd, swbase, ntask, kv.first.first, kv.first.second,
+                              slds[kv.first], st);
+            swbase += ntask;
+        }
+        launch_dgemm(pr, dpd, d_dg, doff, st);
+    }
+    for (size_t i = 0; i < who.size(); ++i) {
+        Problem* P = ps[who[i]].P;
+        P->counters.passes += 1;
+        if (prec[i] != P64) P->counters.passes_lo += 1;
+        P->counters.updates_raw += (double)P->Nc * P->Nd;
+    }
+}
+
+void BatchSolver::run(rt_solve_info* info) {
+    const int n = (int)ps.size();
+    hipEvent_t e0, e1;
+    RT_CHECK(hipEventCreate(&e0));
+    RT_CHECK(hipEventCreate(&e1));
+    for (auto& s : ps) {
+        s.P->counters = SolveCounters{};
+        k_fill(s.P->d_phi, 0.0, (size_t)s.P->G * s.P->Np * s.P->Nc, st);
+        s.phase = START;
+        s.g = 0;
+    }
+    RT_CHECK(hipEventRecord(e0, st));
+    std::vector<VecItem> pre;  // vector ops to run before the next round's passes
+    rounds = 0;
+    while (true) {
+        // ---- group starts: out-of-group + volume source (sn_flux.jl:175-179)
+        for (auto& s : ps) {
+            if (s.phase != START) continue;
+            Problem* P = s.P;
+            launch_outsrc(P->d_qlout, P->d_phi, P->d_sigo, P->d_qvol, P->d_mat, P->d_act, s.g, P->G, P->Nmat, P->Nact,
+                          P->Np, P->Nc, st);
+            s.phase = CPASS;
+            s.niter = 0;
+            s.refs = 0;
+            s.p0 = P->counters.passes;
+            s.l0 = P->counters.passes_lo;
+        }
+        arena_off = 0;
+        run_vecops(pre);
+        pre.clear();
+        // ---- one pass per active problem
+        std::vector<int> who, prec, full;
+        std::vector<PassDesc> pds;
+        for (int i = 0; i < n; ++i) {
+            PS& s = ps[i];
+            if (s.phase == DONE) continue;
+            Problem* P = s.P;
+            PassDesc pd;
+            switch (s.phase) {
+                case CPASS: P->desc_reduced(pd, s.g, P64, MODE_T, nullptr, s.c); prec.push_back(P64); full.push_back(0); break;
+                case ARNOLDI: P->desc_reduced(pd, s.g, s.lo, MODE_IMA, s.V[s.j], s.w); prec.push_back(s.lo); full.push_back(0); break;
+                case RESID: P->desc_reduced(pd, s.g, P64, MODE_IMA, s.z, s.t); prec.push_back(P64); full.push_back(0); break;
+                case RECON: P->desc_reconstruct(pd, s.g, s.z); prec.push_back(P64); full.push_back(1); break;
+                default: continue;
+            }
+            who.push_back(i);
+            pds.push_back(pd);
+        }
+        if (who.empty()) break;
+        ++rounds;
+        launch_passes(who, pds, prec, full);
+        // ---- Krylov work after the pass
+        std::vector<DotItem> d1;
+        std::vector<VecItem> v1;
+        for (int i : who) {
+            PS& s = ps[i];
+            double* slots = d_slots + (size_t)SLOTS * i;
+            const long S = (long)s.P->S_r;
+            if (s.phase == CPASS) d1.push_back({s.c, nullptr, S, slots + S_NRM});
+            if (s.phase == RESID) {
+                VecItem it{};
+                it.op = VOP_SUB; it.n = S; it.y = s.r; it.x = s.c; it.x2 = s.t;
+                v1.push_back(it);
+                d1.push_back({s.r, nullptr, S, slots + S_NRM});
+            }
+            if (s.phase == ARNOLDI)
+                for (int k = 0; k <= s.j; ++k) d1.push_back({s.V[k], s.w, S, slots + S_H1 + k});
+        }
+        run_vecops(v1);
+        run_dots(d1);
+        // CGS2: w -= V h1 ; h2 = V^T w ; w -= V h2 ; |w|^2 ; V[j+1] = w/|w|
+        std::vector<VecItem> v2, v3, v4;
+        std::vector<DotItem> d2, d3;
+        for (int i : who) {
+            PS& s = ps[i];
+            if (s.phase != ARNOLDI) continue;
+            double* slots = d_slots + (size_t)SLOTS * i;
+            const long S = (long)s.P->S_r;
+            VecItem it{};
+            it.op = VOP_SUBMULTI; it.n = S; it.y = s.w; it.V = (const double* const*)s.P->d_Vptr; it.k = s.j + 1;
+            it.coef = slots + S_H1;
+            v2.push_back(it);
+            for (int k = 0; k <= s.j; ++k) d2.push_back({s.V[k], s.w, S, slots + S_H2 + k});
+            it.coef = slots + S_H2;
+            v3.push_back(it);
+            d3.push_back({s.w, nullptr, S, slots + S_WN});
+            VecItem nm{};
+            nm.op = VOP_NORMALIZE; nm.n = S; nm.y = s.V[s.j + 1]; nm.x = s.w; nm.slot = slots + S_WN;
+            v4.push_back(nm);
+        }
+        run_vecops(v2);
+        run_dots(d2);
+        run_vecops(v3);
+        run_dots(d3);
+        run_vecops(v4);
+        RT_CHECK(hipMemcpyAsync(h_slots, d_slots, sizeof(double) * SLOTS * n, hipMemcpyDeviceToHost, st));
+        RT_CHECK(hipStreamSynchronize(st));
+        // ---- host state machines
+        std::vector<double> coefs(64 * (size_t)n, 0.0);
+        std::vector<VecItem> adds;
+        for (int i : who) advance(i, pre, adds, coefs);
+        if (!adds.empty()) {
+            // z += sum y V, with the coefficients staged into d_coef first (the arena is free again
+            // after the synchronisation above)
+            arena_off = 0;
+            const double* dc = stage(coefs.data(), coefs.size());
+            RT_CHECK(hipMemcpyAsync(d_coef, dc, coefs.size() * sizeof(double), hipMemcpyDeviceToDevice, st));
+            run_vecops(adds);
+            RT_CHECK(hipStreamSynchronize(st));
+        }
+    }
+    RT_CHECK(hipEventRecord(e1, st));
+    RT_CHECK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    RT_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        info->ms_device = ms;
+        for (auto& s : ps) {
+            info->passes += s.P->counters.passes;
+            info->passes_lo += s.P->counters.passes_lo;
+            info->updates_raw += s.P->counters.updates_raw;
+            info->updates_problem += (double)s.P->Nc * s.P->Nd * s.P->G;
+        }
+    }
+}
+
+// One step of problem i's state machine after its pass of this round.
+void BatchSolver::advance(int i, std::vector<VecItem>& pre, std::vector<VecItem>& adds, std::vector<double>& coefs) {
+    PS& s = ps[i];
+    Problem* P = s.P;
+    const double* slots = h_slots + (size_t)SLOTS * i;
+    const long S = (long)P->S_r;
+    auto item = [&](int op, double* y, const double* x, double alpha) {
+        VecItem it{};
+        it.op = op; it.n = S; it.y = y; it.x = x; it.alpha = alpha;
+        return it;
+    };
+    auto finish_group = [&](bool conv) {
+        rt_group_stats& g = s.stats[s.g];
+        g.niter = s.niter;
+        g.resid = s.resid;
+        g.rho = std::numeric_limits<double>::quiet_NaN();
+        g.converged = conv ? 1 : 0;
+        g.passes = (int32_t)(P->counters.passes - s.p0);
+        g.passes_lo = (int32_t)(P->counters.passes_lo - s.l0);
+        g.refinements = s.refs;
+        ++s.g;
+        s.phase = (s.g >= P->G) ? DONE : START;
+    };
+    switch (s.phase) {
+        case CPASS: {
+            s.bnorm = std::sqrt(slots[S_NRM]);
+            if (s.bnorm == 0.0) {  // no source anywhere in the group (sn_one_speed.jl:116-120)
+                pre.push_back([&] {
+                    VecItem it{};
+                    it.op = VOP_FILL; it.n = (long)P->Np * P->Nc; it.y = P->d_phi + (size_t)s.g * P->Np * P->Nc;
+                    return it;
+                }());
+                s.resid = 0.0;
+                finish_group(true);
+                break;
+            }
+            s.resid = 1.0;
+            pre.push_back(item(VOP_FILL, s.z, nullptr, 0.0));
+            pre.push_back(item(VOP_COPY, s.r, s.c, 0.0));
+            pre.push_back(item(VOP_SCALE_COPY, s.V[0], s.c, 1.0 / s.bnorm));
+            s.beta = s.bnorm;
+            start_arnoldi(s);
+            break;
+        }
+        case ARNOLDI: {
+            const int j = s.j;
+            ++s.niter;
+            for (int k = 0; k <= j; ++k) s.H[k + 33 * j] = slots[S_H1 + k] + slots[S_H2 + k];
+            const double hjp1 = std::sqrt(std::max(0.0, slots[S_WN]));
+            s.H[j + 1 + 33 * j] = hjp1;
+            auto h = [&](int a, int b) -> double& { return s.H[a + 33 * b]; };
+            for (int k = 0; k < j; ++k) {
+                const double t = s.cs[k] * h(k, j) + s.sn[k] * h(k + 1, j);
+                h(k + 1, j) = -s.sn[k] * h(k, j) + s.cs[k] * h(k + 1, j);
+                h(k, j) = t;
+            }
+            const double den = std::hypot(h(j, j), h(j + 1, j));
+            if (den == 0.0) { s.cs[j] = 1.0; s.sn[j] = 0.0; }
+            else { s.cs[j] = h(j, j) / den; s.sn[j] = h(j + 1, j) / den; }
+            h(j, j) = s.cs[j] * h(j, j) + s.sn[j] * h(j + 1, j);
+            h(j + 1, j) = 0.0;
+            s.gv[j + 1] = -s.sn[j] * s.gv[j];
+            s.gv[j] = s.cs[j] * s.gv[j];
+            const double ri = std::fabs(s.gv[j + 1]) / s.beta;
+            const bool stop = ri < s.inner_tol || j + 1 == s.m || hjp1 == 0.0 || s.niter >= P->maxit;
+            if (!stop) {
+                ++s.j;
+                break;
+            }
+            const int k = j + 1;
+            std::vector<double> y(k, 0.0);
+            for (int a = k - 1; a >= 0; --a) {
+                double v = s.gv[a];
+                for (int l = a + 1; l < k; ++l) v -= h(a, l) * y[l];
+                y[a] = h(a, a) != 0.0 ? v / h(a, a) : 0.0;
+            }
+            for (int a = 0; a < k; ++a) coefs[64 * (size_t)i + a] = y[a];
+            VecItem it{};
+            it.op = VOP_ADDMULTI; it.n = S; it.y = s.z; it.V = (const double* const*)P->d_Vptr; it.k = k;
+            it.coef = d_coef + 64 * (size_t)i;
+            adds.push_back(it);
+            s.phase = RESID;
+            ++s.refs;
+            break;
+        }
+        case RESID: {
+            const double rn = std::sqrt(slots[S_NRM]);
+            s.resid = rn / s.bnorm;
+            if (s.resid < P->tol || s.refs >= 12 || s.niter >= P->maxit) {
+                s.conv = s.resid < P->tol;
+                s.phase = RECON;
+                break;
+            }
+            s.beta = rn;
+            pre.push_back(item(VOP_SCALE_COPY, s.V[0], s.r, 1.0 / rn));
+            start_arnoldi(s);
+            break;
+        }
+        case RECON:
+            finish_group(s.conv);
+            break;
+        default:
+            break;
+    }
+}
+
+void BatchSolver::start_arnoldi(PS& s) {
+    s.phase = ARNOLDI;
+    s.j = 0;
+    std::fill(std::begin(s.H), std::end(s.H), 0.0);
+    std::fill(std::begin(s.cs), std::end(s.cs), 0.0);
+    std::fill(std::begin(s.sn), std::end(s.sn), 0.0);
+    std::fill(std::begin(s.gv), std::end(s.gv), 0.0);
+    s.gv[0] = s.beta;
+}
+
+}  // namespace rt

@@ -1,0 +1,52 @@
+// batch.hpp -- lockstep solver for many independent problems in one set of launches.
+#pragma once
+#include <vector>
+
+#include "engine.hpp"
+
+namespace rt {
+
+class BatchSolver {
+public:
+    BatchSolver(const std::vector<Problem*>& probs, hipStream_t stream);
+    ~BatchSolver();
+    void run(rt_solve_info* info);
+    const std::vector<rt_group_stats>& stats(int i) const { return ps[i].stats; }
+    int rounds = 0;
+
+private:
+    enum Phase { START, CPASS, ARNOLDI, RESID, RECON, DONE };
+    struct PS {
+        Problem* P = nullptr;
+        int g = 0, phase = START, j = 0, m = 30, lo = P32, refs = 0;
+        int64_t niter = 0, p0 = 0, l0 = 0;
+        double bnorm = 0, beta = 0, resid = 1, inner_tol = 1e-5;
+        bool conv = false;
+        double H[33 * 32] = {}, cs[32] = {}, sn[32] = {}, gv[33] = {};
+        std::vector<double*> V;
+        double *w = nullptr, *c = nullptr, *z = nullptr, *r = nullptr, *t = nullptr;
+        std::vector<rt_group_stats> stats;
+    };
+    std::vector<PS> ps;
+    hipStream_t st;
+    std::vector<void*> mem;
+    PassDesc* d_pd = nullptr;
+    GemmTask *d_mg = nullptr, *d_dg = nullptr;
+    SweepTask* d_sw = nullptr;
+    size_t cap_gemm = 0, cap_sweep = 0;
+    double *d_slots = nullptr, *h_slots = nullptr, *d_coef = nullptr, *d_partial = nullptr;
+    unsigned* d_counters = nullptr;
+    // staging arena: host (pinned) and device copies at equal offsets, reset once per round
+    unsigned char *h_arena = nullptr, *d_arena = nullptr;
+    size_t arena_cap = 0, arena_off = 0;
+    template <class T> T* stage(const T* data, size_t count);
+
+    void launch_passes(const std::vector<int>& who, const std::vector<PassDesc>& pds, const std::vector<int>& prec,
+                       const std::vector<int>& full);
+    void run_vecops(const std::vector<VecItem>& items);
+    void run_dots(const std::vector<DotItem>& items);
+    void advance(int i, std::vector<VecItem>& pre, std::vector<VecItem>& adds, std::vector<double>& coefs);
+    void start_arnoldi(PS& s);
+};
+
+}  // namespace rt

@@ -422,8 +422,7 @@ void Problem::pass_faithful(int g, int prec, int mode, const double* zin, double
 }
 
 // Reduced state [Φ_act (Nact x Nc)]: only the moments that feed back into the next pass.
-void Problem::pass_reduced(int g, int prec, int mode, const double* zin, double* zout) {
-    PassDesc pd;
+void Problem::desc_reduced(PassDesc& pd, int g, int prec, int mode, const double* zin, double* zout) {
     fill_geometry(pd, g);
     pd.Mact = (prec == P64) ? (const void*)d_Mact64 : (const void*)d_Mact32;
     pd.qlout = (mode == MODE_T) ? d_qlout : nullptr;
@@ -433,12 +432,16 @@ void Problem::pass_reduced(int g, int prec, int mode, const double* zin, double*
     pd.Drows = (prec == P64) ? (const void*)d_Dact64 : (const void*)d_Dact32;
     pd.out = zout;
     pd.zin = (mode == MODE_IMA) ? zin : nullptr;
+}
+
+void Problem::pass_reduced(int g, int prec, int mode, const double* zin, double* zout) {
+    PassDesc pd;
+    desc_reduced(pd, g, prec, mode, zin, zout);
     launch_pass(pd, prec, false);
 }
 
 // Final reconstruction pass 𝚽l = T(z) from a reduced state, all Np moments in FP64.
-void Problem::pass_reconstruct(int g, const double* zin_reduced) {
-    PassDesc pd;
+void Problem::desc_reconstruct(PassDesc& pd, int g, const double* zin_reduced) {
     fill_geometry(pd, g);
     pd.Mact = d_Mact64;
     pd.qlout = d_qlout;
@@ -448,6 +451,11 @@ void Problem::pass_reconstruct(int g, const double* zin_reduced) {
     pd.Drows = d_D64;
     pd.out = d_phi + (size_t)g * Np * Nc;
     pd.zin = nullptr;
+}
+
+void Problem::pass_reconstruct(int g, const double* zin_reduced) {
+    PassDesc pd;
+    desc_reconstruct(pd, g, zin_reduced);
     launch_pass(pd, P64, true);
 }
 

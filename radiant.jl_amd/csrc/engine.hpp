@@ -49,7 +49,11 @@ struct KernelTimes {
     int64_t n_p[3][4] = {};
 };
 
+class BatchSolver;
+
 class Problem {
+    friend class BatchSolver;
+
 public:
     explicit Problem(const rt_problem_desc& d, hipStream_t stream = nullptr);
     ~Problem();
@@ -127,6 +131,8 @@ private:
     void pass_faithful(int g, int prec, int mode, const double* zin, double* zout);
     void pass_reduced(int g, int prec, int mode, const double* zin, double* zout);
     void pass_reconstruct(int g, const double* zin_reduced);
+    void desc_reduced(PassDesc& pd, int g, int prec, int mode, const double* zin, double* zout);
+    void desc_reconstruct(PassDesc& pd, int g, const double* zin_reduced);
     void launch_pass(PassDesc& pd, int prec, bool full_rows);
     struct EvPair { hipEvent_t a, b; int kernel, prec; };
     std::vector<hipEvent_t> ev_pool;

@@ -1216,4 +1216,103 @@ void k_state_moments_out(double* dst, const double* src, int Np, int Nc, hipStre
     RT_HIP(hipGetLastError());
 }
 
+// ------------------------------------------------------------------------------------------
+// Batched helpers: task-table expansion, segmented dot products, vector-op lists.
+// ------------------------------------------------------------------------------------------
+__global__ void expand_gemm_kernel(const int* __restrict__ meta, int n, GemmTask* __restrict__ out) {
+    // meta: [prob, ntile, nrg, offset] per entry
+    const int e = blockIdx.x;
+    if (e >= n) return;
+    const int prob = meta[4 * e], ntile = meta[4 * e + 1], nrg = meta[4 * e + 2], off = meta[4 * e + 3];
+    for (int i = threadIdx.x; i < ntile * nrg; i += blockDim.x) out[off + i] = GemmTask{prob, i % ntile, i / ntile, 0};
+}
+
+__global__ void expand_sweep_kernel(const int* __restrict__ meta, int n, SweepTask* __restrict__ out) {
+    // meta: [prob, ndir, offset] per entry
+    const int e = blockIdx.x;
+    if (e >= n) return;
+    const int prob = meta[3 * e], nd = meta[3 * e + 1], off = meta[3 * e + 2];
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) out[off + i] = SweepTask{prob, i};
+}
+
+__global__ __launch_bounds__(RED_THREADS) void segdot_kernel(const DotItem* __restrict__ items, double* partial,
+                                                             unsigned* counters) {
+    const DotItem it = items[blockIdx.y];
+    const int BPI = gridDim.x;
+    double v = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < it.n; i += (long)BPI * blockDim.x) {
+        const double a = it.a[i];
+        v += a * (it.b ? it.b[i] : a);
+    }
+    v = block_sum(v);
+    __shared__ bool last;
+    if (threadIdx.x == 0) {
+        partial[(size_t)blockIdx.y * BPI + blockIdx.x] = v;
+        __threadfence();
+        const unsigned t = atomicAdd(&counters[blockIdx.y], 1u);
+        last = (t == (unsigned)BPI - 1);
+    }
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        __threadfence();
+        double s = 0.0;
+        for (int b = 0; b < BPI; ++b) s += __builtin_nontemporal_load(&partial[(size_t)blockIdx.y * BPI + b]);
+        *it.out = s;
+        counters[blockIdx.y] = 0u;
+    }
+}
+
+__global__ __launch_bounds__(256) void vecop_kernel(const VecItem* __restrict__ items) {
+    const VecItem it = items[blockIdx.y];
+    double a = it.alpha;
+    if (it.op == VOP_NORMALIZE) {
+        const double h = *it.slot;
+        if (!(h > 0.0)) return;
+        a = 1.0 / sqrt(h);
+    }
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < it.n; i += (long)gridDim.x * blockDim.x) {
+        switch (it.op) {
+            case VOP_FILL: it.y[i] = a; break;
+            case VOP_COPY: it.y[i] = it.x[i]; break;
+            case VOP_SCALE_COPY:
+            case VOP_NORMALIZE: it.y[i] = it.x[i] * a; break;
+            case VOP_SUB: it.y[i] = it.x[i] - it.x2[i]; break;
+            case VOP_SUBMULTI: {
+                double v = it.y[i];
+                for (int t = 0; t < it.k; ++t) v -= it.coef[t] * it.V[t][i];
+                it.y[i] = v;
+                break;
+            }
+            case VOP_ADDMULTI: {
+                double v = it.y[i];
+                for (int t = 0; t < it.k; ++t) v += it.coef[t] * it.V[t][i];
+                it.y[i] = v;
+                break;
+            }
+            default: break;
+        }
+    }
+}
+
+void launch_expand_gemm(const int* meta, int n, GemmTask* out, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(expand_gemm_kernel, dim3(n), dim3(256), 0, st, meta, n, out);
+    RT_HIP(hipGetLastError());
+}
+void launch_expand_sweep(const int* meta, int n, SweepTask* out, hipStream_t st) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(expand_sweep_kernel, dim3(n), dim3(256), 0, st, meta, n, out);
+    RT_HIP(hipGetLastError());
+}
+void launch_segdot(const DotItem* items, int nitems, int bpi, double* partial, unsigned* counters, hipStream_t st) {
+    if (nitems <= 0) return;
+    hipLaunchKernelGGL(segdot_kernel, dim3(bpi, nitems), dim3(RED_THREADS), 0, st, items, partial, counters);
+    RT_HIP(hipGetLastError());
+}
+void launch_vecops(const VecItem* items, int nitems, int blocks_per_item, hipStream_t st) {
+    if (nitems <= 0) return;
+    hipLaunchKernelGGL(vecop_kernel, dim3(blocks_per_item, nitems), dim3(256), 0, st, items);
+    RT_HIP(hipGetLastError());
+}
+
 }  // namespace rt

@@ -105,6 +105,26 @@ void k_sub(double* out, const double* a, const double* b, size_t n, hipStream_t 
 void k_cast_f64_f32(float* dst, const double* src, size_t n, hipStream_t st);
 void k_scatter_state(double* dst, const double* src, int rows, const int* rowmap, int Nc, hipStream_t st);
 
+// Batched helpers (one launch serves every problem of a batch).
+struct DotItem { const double* a; const double* b; long n; double* out; };   // *out = a.b (b null: a.a)
+enum VecOpCode { VOP_FILL = 0, VOP_COPY = 1, VOP_SCALE_COPY = 2, VOP_NORMALIZE = 3, VOP_SUB = 4, VOP_SUBMULTI = 5,
+                 VOP_ADDMULTI = 6 };
+struct VecItem {
+    int op, k;
+    long n;
+    double* y;
+    const double* x;
+    const double* x2;
+    const double* const* V;  // device table of k vectors
+    const double* coef;      // device array of k coefficients
+    const double* slot;      // squared norm for VOP_NORMALIZE
+    double alpha;
+};
+void launch_expand_gemm(const int* meta, int n, GemmTask* out, hipStream_t st);   // meta [prob, ntile, nrg, off]
+void launch_expand_sweep(const int* meta, int n, SweepTask* out, hipStream_t st); // meta [prob, ndir, off]
+void launch_segdot(const DotItem* items, int nitems, int bpi, double* partial, unsigned* counters, hipStream_t st);
+void launch_vecops(const VecItem* items, int nitems, int blocks_per_item, hipStream_t st);
+
 // Layout converters of the host boundary (Radiant order <-> device order).
 void k_phi_to_radiant(double* dst, const double* phi_all, int G, int Np, int Nc, hipStream_t st);
 void k_state_moments_in(double* dst, const double* src, int Np, int Nc, hipStream_t st);   // [c][p] -> [p][c]

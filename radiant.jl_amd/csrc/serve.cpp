@@ -18,6 +18,7 @@
 #include <thread>
 #include <vector>
 
+#include "batch.hpp"
 #include "engine.hpp"
 
 
@@ -73,6 +74,45 @@ int rt_solve_batch(int32_t n, const rt_problem_desc* const* p, double* const* ph
         return RT_EINVAL;
     }
     try {
+        // Lockstep path: every problem in the mixed-precision modes, one shared stream, one set of
+        // launches per round for all problems.
+        bool lockstep = n > 0;
+        for (int i = 0; i < n; ++i)
+            if (p[i]->precision == RT_PREC_FP64) lockstep = false;
+        if (lockstep) {
+            hipStream_t st;
+            RT_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            struct StreamGuard {
+                hipStream_t s;
+                ~StreamGuard() { (void)hipStreamDestroy(s); }
+            } guard{st};
+            std::vector<std::unique_ptr<rt::Problem>> probs;
+            std::vector<rt::Problem*> raw;
+            for (int i = 0; i < n; ++i) {
+                probs.emplace_back(new rt::Problem(*p[i], st));
+                raw.push_back(probs.back().get());
+            }
+            bool ok = true;
+            std::unique_ptr<rt::BatchSolver> bs;
+            try {
+                bs.reset(new rt::BatchSolver(raw, st));
+            } catch (const rt::Error& e) {
+                if (e.code != RT_EINVAL) throw;
+                ok = false;  // a problem the row-scan sweep cannot take: per-problem streams below
+            }
+            if (ok) {
+                bs->run(info);
+                for (int i = 0; i < n; ++i) {
+                    if (stats_out && stats_out[i]) {
+                        const auto& s = bs->stats(i);
+                        std::copy(s.begin(), s.end(), stats_out[i]);
+                    }
+                    probs[i]->get_flux(phi_out[i]);
+                }
+                bs.reset();
+                return RT_OK;
+            }
+        }
         std::vector<std::unique_ptr<rt::Problem>> probs;
         for (int i = 0; i < n; ++i) probs.emplace_back(new rt::Problem(*p[i]));
         hipEvent_t e0, e1;

@@ -1,0 +1,94 @@
+"""Batched (lockstep) and served solves: every result within 1e-6 of the FP64 solve, and the same
+alone and in the mix (the batched kernels never combine numbers of different problems)."""
+import sys
+import time
+
+import numpy as np
+import pytest
+
+from util import rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _mix():
+    from radiant_hip import problems as P
+    return [P.config_2d(N=24, G=3, L=2, quad_order=4, seed=11),
+            P.config_2d(N=40, G=4, L=3, quad_order=6, block=5, seed=12),
+            P.small_problem(3, N=6, G=3, L=2, tol=1e-10),
+            P.small_problem(1, N=30, G=2, L=3, tol=1e-10),
+            P.config_3d(N=10, width=1.0, G=3, L=3, quad_order=6, seed=13)]
+
+
+@pytest.mark.parametrize("precision", ["mixed-fp32", "mixed-bf16"])
+def test_batch_matches_fp64_and_alone(precision):
+    from radiant_hip import solver as S
+    probs = _mix()
+    _say("batch", precision)
+    phis, stats, info = S.solve_batch(probs, precision=precision)
+    for i, p in enumerate(probs):
+        ref, _ = S.compute_flux(p)
+        e = rel_err(phis[i], ref)
+        _say("  problem", i, "err vs fp64", e, [s["refinements"] for s in stats[i]])
+        assert e < 1e-6
+        assert all(s["converged"] for s in stats[i])
+    # alone vs in the mix: identical
+    for i in (0, 2):
+        alone, _, _ = S.solve_batch([probs[i]], precision=precision)
+        d = float(np.abs(alone[0] - phis[i]).max())
+        _say("  alone vs mix", i, d)
+        assert d == 0.0
+
+
+def test_server_continuous_batching():
+    from radiant_hip import solver as S
+    from radiant_hip import problems as P
+    probs = _mix() + [P.config_2d(N=32, G=3, L=2, quad_order=4, seed=14)]
+    srv = S.Server(hbm_budget_bytes=2 * 1024 ** 3, max_active=3, precision="mixed-fp32")
+    try:
+        ids = []
+        for k, p in enumerate(probs):
+            ids.append(srv.submit(p))
+            time.sleep(0.01)  # requests arrive while earlier ones run
+        res = {}
+        t0 = time.time()
+        while len(res) < len(ids) and time.time() - t0 < 90:
+            for rid in ids:
+                if rid not in res:
+                    r = srv.poll(rid)
+                    if r is not None:
+                        res[rid] = r
+            time.sleep(0.005)
+        assert len(res) == len(ids)
+        for rid, p in zip(ids, probs):
+            phi, st, lat = res[rid]
+            ref, _ = S.compute_flux(p)
+            e = rel_err(phi, ref)
+            _say("  request", rid, "latency ms", round(lat, 2), "err", e)
+            assert e < 1e-6 and lat > 0
+    finally:
+        srv.close()
+
+
+def test_server_budget_refusal_and_cancel():
+    from radiant_hip import solver as S, _lib
+    from radiant_hip import problems as P
+    srv = S.Server(hbm_budget_bytes=1024, max_active=2, precision="mixed-fp32")
+    try:
+        rid = srv.submit(P.small_problem(2, N=6, G=2))
+        t0 = time.time()
+        while True:
+            try:
+                r = srv.poll(rid)
+            except _lib.RTError as e:
+                assert e.code == _lib.RT_ENOMEM
+                break
+            assert r is None and time.time() - t0 < 30
+            time.sleep(0.01)
+    finally:
+        srv.close()
