@@ -149,6 +149,14 @@ int rt_test_contraction(int32_t which, int32_t prec, int32_t M, int32_t K, int32
 int rt_solve_batch(int32_t n, const rt_problem_desc *const *p, double *const *phi_out,
                    rt_group_stats *const *stats_out, rt_solve_info *info);
 
+/* Resident batch: n problems uploaded once and solved together (all RT_PREC_MIXED_*: one
+ * lockstep launch stream; otherwise one stream per problem). */
+typedef struct rt_batch rt_batch;
+int rt_batch_create(int32_t n, const rt_problem_desc *const *p, rt_batch **out);
+int rt_batch_solve(rt_batch *b, rt_group_stats *const *stats_out, rt_solve_info *info);
+int rt_batch_get_flux(rt_batch *b, int32_t i, double *phi_out);
+int rt_batch_destroy(rt_batch *b);
+
 /* ------------------------------------------------------------------------------------------
  * Serving: continuous batching of fixed-source solves admitted by an HBM budget.
  * ------------------------------------------------------------------------------------------ */

@@ -65,93 +65,100 @@ struct DescCopy {
     }
 };
 
-extern "C" {
-
-int rt_solve_batch(int32_t n, const rt_problem_desc* const* p, double* const* phi_out, rt_group_stats* const* stats_out,
-                   rt_solve_info* info) {
-    if (n < 0 || (n > 0 && (!p || !phi_out))) {
-        rt::set_last_error("invalid batch arguments");
-        return RT_EINVAL;
+// A resident batch: the problems, their shared stream and (for the mixed-precision modes) the
+// lockstep solver; otherwise each problem keeps its own stream and a host thread drives it.
+struct rt_batch {
+    hipStream_t st = nullptr;
+    std::vector<std::unique_ptr<rt::Problem>> probs;
+    std::unique_ptr<rt::BatchSolver> bs;
+    std::vector<std::vector<rt_group_stats>> stats;
+    ~rt_batch() {
+        bs.reset();
+        probs.clear();
+        if (st) (void)hipStreamDestroy(st);
     }
-    try {
-        // Lockstep path: every problem in the mixed-precision modes, one shared stream, one set of
-        // launches per round for all problems.
-        bool lockstep = n > 0;
-        for (int i = 0; i < n; ++i)
-            if (p[i]->precision == RT_PREC_FP64) lockstep = false;
-        if (lockstep) {
-            hipStream_t st;
-            RT_CHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-            struct StreamGuard {
-                hipStream_t s;
-                ~StreamGuard() { (void)hipStreamDestroy(s); }
-            } guard{st};
-            std::vector<std::unique_ptr<rt::Problem>> probs;
-            std::vector<rt::Problem*> raw;
-            for (int i = 0; i < n; ++i) {
-                probs.emplace_back(new rt::Problem(*p[i], st));
-                raw.push_back(probs.back().get());
-            }
-            bool ok = true;
-            std::unique_ptr<rt::BatchSolver> bs;
+};
+
+static rt_batch* make_batch(int32_t n, const rt_problem_desc* const* p) {
+    std::unique_ptr<rt_batch> b(new rt_batch());
+    bool lockstep = n > 0;
+    for (int i = 0; i < n; ++i)
+        if (p[i]->precision == RT_PREC_FP64) lockstep = false;
+    if (lockstep) {
+        RT_CHECK(hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
+        std::vector<rt::Problem*> raw;
+        for (int i = 0; i < n; ++i) {
+            b->probs.emplace_back(new rt::Problem(*p[i], b->st));
+            raw.push_back(b->probs.back().get());
+        }
+        try {
+            b->bs.reset(new rt::BatchSolver(raw, b->st));
+        } catch (const rt::Error& e) {
+            if (e.code != RT_EINVAL) throw;
+            b->bs.reset();  // a problem the row-scan sweep cannot take: per-problem streams
+        }
+        if (!b->bs) {
+            b->probs.clear();
+            (void)hipStreamDestroy(b->st);
+            b->st = nullptr;
+        }
+    }
+    if (!b->bs)
+        for (int i = 0; i < n; ++i) b->probs.emplace_back(new rt::Problem(*p[i]));
+    b->stats.resize(n);
+    for (int i = 0; i < n; ++i) b->stats[i].assign(b->probs[i]->G, rt_group_stats{});
+    return b.release();
+}
+
+static void solve_batch(rt_batch* b, rt_solve_info* info) {
+    const int n = (int)b->probs.size();
+    if (b->bs) {
+        b->bs->run(info);
+        for (int i = 0; i < n; ++i) b->stats[i] = b->bs->stats(i);
+        return;
+    }
+    hipEvent_t e0, e1;
+    RT_CHECK(hipEventCreate(&e0));
+    RT_CHECK(hipEventCreate(&e1));
+    RT_CHECK(hipDeviceSynchronize());
+    RT_CHECK(hipEventRecord(e0, nullptr));
+    std::vector<std::thread> th;
+    std::vector<std::string> errs(n);
+    std::vector<rt_solve_info> infos(n);
+    for (int i = 0; i < n; ++i)
+        th.emplace_back([&, i] {
             try {
-                bs.reset(new rt::BatchSolver(raw, st));
-            } catch (const rt::Error& e) {
-                if (e.code != RT_EINVAL) throw;
-                ok = false;  // a problem the row-scan sweep cannot take: per-problem streams below
+                b->probs[i]->solve(b->stats[i].data(), &infos[i]);
+            } catch (const std::exception& e) {
+                errs[i] = e.what();
             }
-            if (ok) {
-                bs->run(info);
-                for (int i = 0; i < n; ++i) {
-                    if (stats_out && stats_out[i]) {
-                        const auto& s = bs->stats(i);
-                        std::copy(s.begin(), s.end(), stats_out[i]);
-                    }
-                    probs[i]->get_flux(phi_out[i]);
-                }
-                bs.reset();
-                return RT_OK;
-            }
+        });
+    for (auto& t : th) t.join();
+    RT_CHECK(hipDeviceSynchronize());
+    RT_CHECK(hipEventRecord(e1, nullptr));
+    RT_CHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    RT_CHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    for (int i = 0; i < n; ++i)
+        if (!errs[i].empty()) throw rt::Error(RT_EDEVICE, "problem " + std::to_string(i) + ": " + errs[i]);
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        info->ms_device = ms;
+        for (auto& x : infos) {
+            info->passes += x.passes;
+            info->passes_lo += x.passes_lo;
+            info->updates_raw += x.updates_raw;
+            info->updates_problem += x.updates_problem;
         }
-        std::vector<std::unique_ptr<rt::Problem>> probs;
-        for (int i = 0; i < n; ++i) probs.emplace_back(new rt::Problem(*p[i]));
-        hipEvent_t e0, e1;
-        RT_CHECK(hipEventCreate(&e0));
-        RT_CHECK(hipEventCreate(&e1));
-        RT_CHECK(hipDeviceSynchronize());
-        RT_CHECK(hipEventRecord(e0, nullptr));
-        std::vector<std::thread> th;
-        std::vector<std::string> errs(n);
-        std::vector<rt_solve_info> infos(n);
-        for (int i = 0; i < n; ++i)
-            th.emplace_back([&, i] {
-                try {
-                    probs[i]->solve(stats_out ? stats_out[i] : nullptr, &infos[i]);
-                } catch (const std::exception& e) {
-                    errs[i] = e.what();
-                }
-            });
-        for (auto& t : th) t.join();
-        RT_CHECK(hipDeviceSynchronize());
-        RT_CHECK(hipEventRecord(e1, nullptr));
-        RT_CHECK(hipEventSynchronize(e1));
-        float ms = 0;
-        RT_CHECK(hipEventElapsedTime(&ms, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        for (int i = 0; i < n; ++i)
-            if (!errs[i].empty()) throw rt::Error(RT_EDEVICE, "problem " + std::to_string(i) + ": " + errs[i]);
-        for (int i = 0; i < n; ++i) probs[i]->get_flux(phi_out[i]);
-        if (info) {
-            std::memset(info, 0, sizeof(*info));
-            info->ms_device = ms;
-            for (auto& x : infos) {
-                info->passes += x.passes;
-                info->passes_lo += x.passes_lo;
-                info->updates_raw += x.updates_raw;
-                info->updates_problem += x.updates_problem;
-            }
-        }
+    }
+}
+
+template <class F>
+static int guard(F f) {
+    try {
+        f();
         return RT_OK;
     } catch (const rt::Error& e) {
         rt::set_last_error(e.what());
@@ -160,6 +167,59 @@ int rt_solve_batch(int32_t n, const rt_problem_desc* const* p, double* const* ph
         rt::set_last_error(e.what());
         return RT_EDEVICE;
     }
+}
+
+extern "C" {
+
+int rt_batch_create(int32_t n, const rt_problem_desc* const* p, rt_batch** out) {
+    if (n < 1 || !p || !out) {
+        rt::set_last_error("invalid batch arguments");
+        return RT_EINVAL;
+    }
+    *out = nullptr;
+    return guard([&] { *out = make_batch(n, p); });
+}
+
+int rt_batch_solve(rt_batch* b, rt_group_stats* const* stats_out, rt_solve_info* info) {
+    if (!b) {
+        rt::set_last_error("null batch");
+        return RT_EINVAL;
+    }
+    return guard([&] {
+        solve_batch(b, info);
+        if (stats_out)
+            for (size_t i = 0; i < b->probs.size(); ++i)
+                if (stats_out[i]) std::copy(b->stats[i].begin(), b->stats[i].end(), stats_out[i]);
+    });
+}
+
+int rt_batch_get_flux(rt_batch* b, int32_t i, double* phi_out) {
+    if (!b || i < 0 || i >= (int32_t)b->probs.size() || !phi_out) {
+        rt::set_last_error("invalid batch flux request");
+        return RT_EINVAL;
+    }
+    return guard([&] { b->probs[i]->get_flux(phi_out); });
+}
+
+int rt_batch_destroy(rt_batch* b) {
+    return guard([&] { delete b; });
+}
+
+int rt_solve_batch(int32_t n, const rt_problem_desc* const* p, double* const* phi_out, rt_group_stats* const* stats_out,
+                   rt_solve_info* info) {
+    if (n < 0 || (n > 0 && (!p || !phi_out))) {
+        rt::set_last_error("invalid batch arguments");
+        return RT_EINVAL;
+    }
+    if (n == 0) return RT_OK;
+    return guard([&] {
+        std::unique_ptr<rt_batch> b(make_batch(n, p));
+        solve_batch(b.get(), info);
+        for (int i = 0; i < n; ++i) {
+            if (stats_out && stats_out[i]) std::copy(b->stats[i].begin(), b->stats[i].end(), stats_out[i]);
+            b->probs[i]->get_flux(phi_out[i]);
+        }
+    });
 }
 
 }  // extern "C"

@@ -156,6 +156,43 @@ def solve_batch(probs, precision="mixed-fp32", **kw):
     return ([_phi(d, p) for d, p in zip(descs, phis)], [[s.as_dict() for s in st] for st in stats], info)
 
 
+class ResidentBatch:
+    """rt_batch_*: problems uploaded once, solved together (lockstep in the mixed modes)."""
+
+    def __init__(self, probs, precision="mixed-fp32", **kw):
+        self.descs = [Desc(p, precision, "reduced" if precision != "fp64" else "faithful", **kw) for p in probs]
+        n = len(self.descs)
+        PD = C.POINTER(_lib.ProblemDesc)
+        parr = (PD * n)(*[C.pointer(d.desc) for d in self.descs])
+        self.h = C.c_void_p()
+        check(lib().rt_batch_create(n, parr, C.byref(self.h)))
+
+    def solve(self):
+        n = len(self.descs)
+        stats = [(_lib.GroupStats * d.G)() for d in self.descs]
+        sarr = (C.POINTER(_lib.GroupStats) * n)(*[C.cast(s, C.POINTER(_lib.GroupStats)) for s in stats])
+        info = _lib.SolveInfo()
+        check(lib().rt_batch_solve(self.h, sarr, C.byref(info)))
+        return [[s.as_dict() for s in st] for st in stats], info
+
+    def flux(self, i):
+        d = self.descs[i]
+        phi = np.zeros(d.G * d.Np * d.Nc)
+        check(lib().rt_batch_get_flux(self.h, i, ptr(phi)))
+        return _phi(d, phi)
+
+    def close(self):
+        if self.h:
+            lib().rt_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Server:
     """rt_server_*: continuous batching of fixed-source solves under an HBM budget."""
 
