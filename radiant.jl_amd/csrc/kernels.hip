@@ -637,13 +637,58 @@ __global__ __launch_bounds__(1024) void sweep_scan_kernel(const PassDesc* __rest
     T* __restrict__ bz = (T*)d.bz + (size_t)n * Nx * Ny;
     const int* __restrict__ mat = d.mat;
     const int ndiag = Ny + Nz - 1;
+    // Work items of this wave: (diagonal t, first pencil q0) for q0 = wave*U, wave*U + W*U, ...
+    // The angular source and material of the next item are loaded while the current one is solved.
+    auto diag_lo = [&](int t) { return t - (Ny - 1) > 0 ? t - (Ny - 1) : 0; };
+    auto diag_cnt = [&](int t) { return (t < Nz - 1 ? t : Nz - 1) - diag_lo(t) + 1; };
+    auto next_item = [&](int t, int q0, int& tn, int& qn) {
+        if (q0 + W * U < diag_cnt(t)) { tn = t; qn = q0 + W * U; return; }
+        tn = t + 1;
+        while (tn < ndiag && wave * U >= diag_cnt(tn)) ++tn;
+        qn = wave * U;
+    };
+    T vb[U][CPL];
+    int mb[U][CPL];
+    auto load_item = [&](int t, int q0) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int q = q0 + u;
+            const bool ok = t < ndiag && q < diag_cnt(t);
+            const int jz = diag_lo(t) + (ok ? q : 0), jy = t - jz;
+            const int iy = fys ? jy : Ny - 1 - jy, iz = fzs ? jz : Nz - 1 - jz;
+            const int ro = Nx * (iy + Ny * iz);
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                vb[u][c] = (T)0;
+                mb[u][c] = 0;
+                if (ok && inx[c]) {
+                    vb[u][c] = gld(psi + ro + ixc[c]);
+                    mb[u][c] = gld(mat + ro + ixc[c]);
+                }
+            }
+        }
+    };
+    {
+        int t0 = 0;
+        while (t0 < ndiag && wave * U >= diag_cnt(t0)) ++t0;
+        load_item(t0, wave * U);
+    }
     for (int t = 0; t < ndiag; ++t) {
-        const int jz_lo = t - (Ny - 1) > 0 ? t - (Ny - 1) : 0;
-        const int jz_hi = t < Nz - 1 ? t : Nz - 1;
-        const int cnt = jz_hi - jz_lo + 1;
+        const int jz_lo = diag_lo(t);
+        const int cnt = diag_cnt(t);
         for (int q0 = wave * U; q0 < cnt; q0 += W * U) {
             // U pencils of this diagonal, processed together for instruction-level parallelism
             T v[U][CPL], fy[U][CPL], fz[U][CPL], c0[U][CPL], c1[U][CPL];
+            int mm[U][CPL];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) { v[u][c] = vb[u][c]; mm[u][c] = mb[u][c]; }
+            {
+                int tn, qn;
+                next_item(t, q0, tn, qn);
+                load_item(tn, qn);
+            }
             int roff[U], jyv[U], jzv[U], iyv[U], izv[U];
             bool ok[U];
 #pragma unroll
@@ -659,17 +704,14 @@ __global__ __launch_bounds__(1024) void sweep_scan_kernel(const PassDesc* __rest
 #pragma unroll
                 for (int c = 0; c < CPL; ++c) {
                     const int jx = lane * CPL + c;
-                    v[u][c] = (T)0;
                     fy[u][c] = (T)0;
                     fz[u][c] = (T)0;
                     c0[u][c] = (T)0;
                     c1[u][c] = (T)0;
                     if (ok[u] && inx[c]) {
-                        v[u][c] = gld(psi + roff[u] + ixc[c]);
-                        const int m = gld(mat + roff[u] + ixc[c]);
                         fy[u][c] = FY[jzv[u] * Nx + jx];
                         if (d.ndims >= 3) fz[u][c] = FZ[jyv[u] * Nx + jx];
-                        const T S = ((st[m] + hx[c]) + hy) + hz;
+                        const T S = ((st[mm[u][c]] + hx[c]) + hy) + hz;
                         const T rS = dd_div((T)1, S);
                         c0[u][c] = ((v[u][c] + hy * fy[u][c]) + hz * fz[u][c]) * rS;
                         c1[u][c] = hx[c] * rS;
