@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_bf16.h>
 #include <cstdio>
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 
@@ -588,6 +589,53 @@ __global__ __launch_bounds__(64 * WMAX) void sweep_planes_kernel(const PassDesc*
 }
 
 // ------------------------------------------------------------------------------------------
+// Cross-lane affine-map scan.  Maps x -> A + B x are composed "later after earlier":
+// (A2, B2) o (A1, B1) = (A2 + B2 A1, B2 B1).  The DPP version builds the inclusive scan with
+// row_shr:1,2,4,8 inside each 16-lane row, then row_bcast:15 / row_bcast:31 across rows (out-of-
+// range and masked lanes receive the identity (0, 1)); the shuffle version is a Hillis-Steele scan
+// through ds_bpermute.
+// ------------------------------------------------------------------------------------------
+template <int CTRL, int RMASK>
+__device__ inline float dpp_move(float old, float src) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(src), CTRL, RMASK, 0xF, false));
+}
+template <int CTRL, int RMASK>
+__device__ inline double dpp_move(double old, double src) {
+    const long long o = __double_as_longlong(old), v = __double_as_longlong(src);
+    const int lo = __builtin_amdgcn_update_dpp((int)(o & 0xffffffff), (int)(v & 0xffffffff), CTRL, RMASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(v >> 32), CTRL, RMASK, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <int CTRL, int RMASK, typename T>
+__device__ inline void dpp_compose(T& A, T& B) {
+    const T As = dpp_move<CTRL, RMASK>((T)0, A);
+    const T Bs = dpp_move<CTRL, RMASK>((T)1, B);
+    A = A + B * As;
+    B = B * Bs;
+}
+template <bool DPP, typename T>
+__device__ inline void affine_scan(T& A, T& B, int lane) {
+    if (DPP) {
+        dpp_compose<0x111, 0xF>(A, B);  // row_shr:1
+        dpp_compose<0x112, 0xF>(A, B);  // row_shr:2
+        dpp_compose<0x114, 0xF>(A, B);  // row_shr:4
+        dpp_compose<0x118, 0xF>(A, B);  // row_shr:8
+        dpp_compose<0x142, 0xA>(A, B);  // row_bcast:15 into rows 1 and 3
+        dpp_compose<0x143, 0xC>(A, B);  // row_bcast:31 into rows 2 and 3
+    } else {
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const T As = __shfl_up(A, off);
+            const T Bs = __shfl_up(B, off);
+            if (lane >= off) {
+                A = A + B * As;
+                B = B * Bs;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
 // Row-scan sweep (1D/2D/3D): one workgroup per direction, lanes along x.
 //
 // Along a pencil (a row of cells in x) the DD cell solve is an affine map of the incoming
@@ -598,7 +646,7 @@ __global__ __launch_bounds__(64 * WMAX) void sweep_planes_kernel(const PassDesc*
 // t = jy + jz; the y/z face fluxes of the previous diagonal live in LDS (FY[jz][x], FZ[jy][x]),
 // one barrier per diagonal.  Every global access is a contiguous 64*CPL-cell segment.
 // ------------------------------------------------------------------------------------------
-template <typename T, int CPL, int U>
+template <typename T, int CPL, int U, bool DPP>
 __global__ __launch_bounds__(1024) void sweep_scan_kernel(const PassDesc* __restrict__ pds, const SweepTask* __restrict__ tasks) {
     extern __shared__ __align__(16) unsigned char smem[];
     const SweepTask tk = tasks[blockIdx.x];
@@ -733,17 +781,7 @@ __global__ __launch_bounds__(1024) void sweep_scan_kernel(const PassDesc* __rest
                 }
             }
 #pragma unroll
-            for (int off = 1; off < 64; off <<= 1) {
-#pragma unroll
-                for (int u = 0; u < U; ++u) {
-                    const T As = __shfl_up(A[u], off);
-                    const T Bs = __shfl_up(B[u], off);
-                    if (lane >= off) {
-                        A[u] = A[u] + B[u] * As;
-                        B[u] = B[u] * Bs;
-                    }
-                }
-            }
+            for (int u = 0; u < U; ++u) affine_scan<DPP>(A[u], B[u], lane);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 T fx = __shfl_up(A[u], 1);  // exclusive prefix applied to the void inflow (0)
@@ -1128,16 +1166,28 @@ int sweep_scan_waves(int ndims, int Ny, int Nz) {
     return w < 1 ? 1 : (w > 16 ? 16 : w);
 }
 
-template <typename T, int CPL>
-static void scan_inst(const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int waves, size_t lds, hipStream_t st) {
+static bool scan_use_dpp() {
+    // RADIANT_HIP_SCAN=shfl selects the ds_bpermute scan (kept for cross-checking the DPP one)
+    const char* e = std::getenv("RADIANT_HIP_SCAN");
+    return !(e && e[0] == 's');
+}
+
+template <typename T, int CPL, bool DPP>
+static void scan_inst2(const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int waves, size_t lds, hipStream_t st) {
     constexpr int U = CPL >= 4 ? 1 : 2;
-    auto kfn = sweep_scan_kernel<T, CPL, U>;
+    auto kfn = sweep_scan_kernel<T, CPL, U, DPP>;
     static bool attr_set = false;
     if (!attr_set) {
         RT_HIP(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         attr_set = true;
     }
     hipLaunchKernelGGL(kfn, dim3(ntasks), dim3(64 * waves), lds, st, d_pd, tasks);
+}
+
+template <typename T, int CPL>
+static void scan_inst(const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int waves, size_t lds, hipStream_t st) {
+    if (scan_use_dpp()) scan_inst2<T, CPL, true>(d_pd, tasks, ntasks, waves, lds, st);
+    else scan_inst2<T, CPL, false>(d_pd, tasks, ntasks, waves, lds, st);
 }
 
 void launch_sweep_scan(int prec, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int cpl, int waves, size_t lds,

@@ -177,3 +177,17 @@ def test_alternate_sweep_kernel_parity(oracle, ndims, kind, monkeypatch):
     e = rel_err(phi, ref)
     _say("generic sweep", ndims, e)
     assert e < 1e-10
+
+
+@pytest.mark.parametrize("ndims", [1, 2, 3])
+def test_scan_shuffle_variant_parity(oracle, ndims, monkeypatch):
+    """The ds_bpermute scan of the row-scan sweep (the DPP scan is the default), selected by
+    RADIANT_HIP_SCAN=shfl; both must match the oracle."""
+    monkeypatch.setenv("RADIANT_HIP_SCAN", "shfl")
+    P, S = _P(), _S()
+    prob = P.small_problem(ndims, N=5 if ndims == 3 else 70, tol=1e-12)
+    ref, _ = oracle.solve(prob)
+    phi, _ = S.compute_flux(prob)
+    e = rel_err(phi, ref)
+    _say("shuffle scan", ndims, e)
+    assert e < 1e-10
