@@ -110,6 +110,46 @@ def run_batch(probs, precision, steps, warmup, label):
     return out, phis
 
 
+def run_serve(nreq, rate, budget_gib, max_active, precision):
+    """Served stream: the first `nreq` requests of the 3D Carlson stream (N~U{32..96} per axis,
+    G~U{10..40}, Carlson 8/10/12, P3, 3 random materials) submitted at Poisson arrival times
+    (rate per second) to rt_server; latency = submit -> result ready on the host."""
+    from radiant_hip import problems, solver
+    t_arr = problems.poisson_arrivals(nreq, rate=rate)
+    srv = solver.Server(hbm_budget_bytes=int(budget_gib * 1024 ** 3), max_active=max_active, precision=precision)
+    ids, nupd, lat, done = [], {}, {}, 0
+    t0 = time.perf_counter()
+    try:
+        for i in range(nreq):
+            p = problems.config4_request(i)
+            wait = t_arr[i] - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            rid = srv.submit(p)
+            ids.append(rid)
+            nupd[rid] = p.Nc * srv.descs[rid][1] * p.G
+            log(f"[serve] submit {i} (N={p.Nx}, G={p.G}, Nd={srv.descs[rid][1]}) at {time.perf_counter() - t0:.2f}s")
+            del p
+        while len(lat) < len(ids):
+            for rid in ids:
+                if rid in lat:
+                    continue
+                r = srv.poll(rid)
+                if r is not None:
+                    lat[rid] = r[2]
+                    log(f"[serve] request {rid} done: {r[2]:.0f} ms")
+                    del r
+            time.sleep(0.002)
+        wall = time.perf_counter() - t0
+    finally:
+        srv.close()
+    ls = np.array([lat[r] for r in ids])
+    return dict(requests=nreq, rate_per_s=rate, wall_s=wall, requests_per_s=nreq / wall,
+                updates_per_s=sum(nupd.values()) / wall, p50_ms=float(np.percentile(ls, 50)),
+                p99_ms=float(np.percentile(ls, 99)), budget_gib=budget_gib, max_active=max_active,
+                precision=precision)
+
+
 def kernel_roofline(prob, res, nact):
     """Achieved rate of the dominant kernel from its event-timed launches.
 
@@ -182,6 +222,11 @@ def main():
     ap.add_argument("--cpu-groups", type=int, default=2)
     ap.add_argument("--skip-bf16", action="store_true")
     ap.add_argument("--batch", type=int, default=16, help="problems per lockstep batch (1 = no batched mode)")
+    ap.add_argument("--serve", type=int, default=0,
+                    help="instead of the solve benchmark, serve this many requests of the 3D request stream")
+    ap.add_argument("--rate", type=float, default=50.0, help="served stream: Poisson arrivals per second")
+    ap.add_argument("--budget-gib", type=float, default=200.0, help="served stream: server HBM budget")
+    ap.add_argument("--max-active", type=int, default=8, help="served stream: resident requests at most")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only "
@@ -190,6 +235,18 @@ def main():
         sys.exit(2)
     from radiant_hip import problems, solver
     log(solver.version())
+    if args.serve > 0:
+        r = run_serve(args.serve, args.rate, args.budget_gib, args.max_active, "mixed-fp32")
+        print(json.dumps({
+            "metric": "served requests/s and p50/p99 latency", "value": r["requests_per_s"], "unit": "requests/s",
+            "n_gpus": 1, "steps": args.serve, "warmup": 0, "ms_per_step": r["wall_s"] / args.serve * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "fp32 sweep+M/D, fp64 Arnoldi/refinement", "data": "synthetic (seeded splitmix64)",
+            "config": {"workload": f"first {args.serve} requests of the 3D Carlson request stream "
+                                   f"(Poisson {args.rate}/s)", "budget_gib": args.budget_gib,
+                       "max_active": args.max_active},
+            "p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"], "updates_per_s": r["updates_per_s"]}))
+        return
     prob = problems.config1()
     steps, warm = args.steps, args.warmup
 
@@ -251,7 +308,7 @@ def main():
         "fp64_plain": summary(r64), "mixed_fp32": summary(rmx),
         "mixed_fp32_batched": summary(rbt) if rbt else None,
         "mixed_bf16": summary(rbf) if rbf else None,
-        "kernels_headline": kernel_table(head), "kernels_fp64": kernel_table(r64),
+        "kernels_single_mixed": kernel_table(single), "kernels_fp64": kernel_table(r64),
         "roofline": roof, "roofline_fp64": roof64,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (head["updates_per_s"] / cpu["value"]) if cpu else None,

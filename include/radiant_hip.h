@@ -169,7 +169,10 @@ typedef struct rt_server_config {
 int rt_server_create(const rt_server_config *cfg, rt_server **out);
 /* Copies the problem; returns a request id (>= 0) in *req. */
 int rt_server_submit(rt_server *s, const rt_problem_desc *p, int64_t *req);
-/* RT_OK and the results when finished, RT_EPENDING while queued/running, RT_ECANCELLED. */
+/* RT_EPENDING while queued/running; otherwise the final status: RT_OK with the results,
+ * RT_ECANCELLED, or the error of a failed request.  The poll that reports a final status
+ * releases the request id, except a finished request polled with phi_out == NULL (a status
+ * query that fills stats_out and latency_ms and keeps the flux for a later poll). */
 int rt_server_poll(rt_server *s, int64_t req, double *phi_out, rt_group_stats *stats_out,
                    double *latency_ms);
 int rt_server_cancel(rt_server *s, int64_t req);

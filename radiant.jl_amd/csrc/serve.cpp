@@ -251,7 +251,8 @@ struct rt_server {
     int active = 0;
     bool stop = false;
     std::thread sched;
-    std::vector<std::thread> workers;
+    std::map<int64_t, std::thread> workers;  // running requests
+    std::vector<int64_t> finished;           // workers to join
 
     void run_request(std::shared_ptr<Request> r) {
         try {
@@ -282,20 +283,38 @@ struct rt_server {
             r->desc.reset();
             bytes_in_use -= r->bytes;
             --active;
+            finished.push_back(r->id);
         }
         cv.notify_all();
+    }
+
+    // Joins the worker threads that have ended (called with mu held; they no longer take it).
+    void reap(std::unique_lock<std::mutex>& lk) {
+        std::vector<std::thread> done;
+        for (int64_t id : finished) {
+            auto it = workers.find(id);
+            if (it != workers.end()) {
+                done.push_back(std::move(it->second));
+                workers.erase(it);
+            }
+        }
+        finished.clear();
+        lk.unlock();
+        for (auto& t : done) t.join();
+        lk.lock();
     }
 
     void scheduler() {
         std::unique_lock<std::mutex> lk(mu);
         while (true) {
             cv.wait(lk, [&] {
-                if (stop) return true;
+                if (stop || !finished.empty()) return true;
                 if (queue.empty()) return false;
                 auto& r = reqs[queue.front()];
                 return r->cancel.load() || (active < cfg.max_active && bytes_in_use + r->bytes <= cfg.hbm_budget_bytes) ||
                        active == 0;
             });
+            if (!finished.empty()) reap(lk);
             if (stop) break;
             // admit in arrival order while the head request fits
             while (!queue.empty()) {
@@ -322,7 +341,7 @@ struct rt_server {
                 r->state = Request::RUNNING;
                 bytes_in_use += r->bytes;
                 ++active;
-                workers.emplace_back([this, r] { run_request(r); });
+                workers.emplace(r->id, std::thread([this, r] { run_request(r); }));
             }
         }
     }
@@ -380,15 +399,21 @@ int rt_server_poll(rt_server* s, int64_t req, double* phi_out, rt_group_stats* s
         rt::set_last_error("unknown request id");
         return RT_EINVAL;
     }
-    auto& r = it->second;
+    std::shared_ptr<Request> r = it->second;
     if (r->state == Request::QUEUED || r->state == Request::RUNNING) return RT_EPENDING;
     if (latency_ms) latency_ms[0] = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+    if (r->state == Request::DONE && !phi_out) {
+        // status query: the result stays until a poll collects the flux
+        if (stats_out) std::memcpy(stats_out, r->stats.data(), r->stats.size() * sizeof(rt_group_stats));
+        return RT_OK;
+    }
+    s->reqs.erase(it);  // a final status is reported once; the id is released here
     if (r->state == Request::CANCELLED) return RT_ECANCELLED;
     if (r->state == Request::FAILED) {
         rt::set_last_error(r->err);
         return r->code ? r->code : RT_EDEVICE;
     }
-    if (phi_out) std::memcpy(phi_out, r->phi.data(), r->phi.size() * sizeof(double));
+    std::memcpy(phi_out, r->phi.data(), r->phi.size() * sizeof(double));
     if (stats_out) std::memcpy(stats_out, r->stats.data(), r->stats.size() * sizeof(rt_group_stats));
     return RT_OK;
 }
@@ -416,8 +441,8 @@ int rt_server_destroy(rt_server* s) {
     }
     s->cv.notify_all();
     if (s->sched.joinable()) s->sched.join();
-    for (auto& t : s->workers)
-        if (t.joinable()) t.join();
+    for (auto& kv : s->workers)
+        if (kv.second.joinable()) kv.second.join();
     delete s;
     return RT_OK;
 }

@@ -213,12 +213,17 @@ class Server:
     def poll(self, rid):
         """Returns None while pending, else (phi, stats, latency_ms); raises on failure/cancel."""
         G, Np, Nc = self.descs[rid]
-        phi = np.zeros(G * Np * Nc)
         st = (_lib.GroupStats * G)()
         lat = C.c_double()
-        rc = lib().rt_server_poll(self.h, rid, ptr(phi), st, C.byref(lat))
+        rc = lib().rt_server_poll(self.h, rid, None, st, C.byref(lat))  # status query
         if rc == _lib.RT_EPENDING:
             return None
+        if rc != _lib.RT_OK:
+            del self.descs[rid]
+            check(rc)
+        phi = np.empty(G * Np * Nc)
+        rc = lib().rt_server_poll(self.h, rid, ptr(phi), st, C.byref(lat))  # collects and releases
+        del self.descs[rid]
         check(rc)
         return phi.reshape((G, Np, Nc), order="F"), [s.as_dict() for s in st], lat.value
 
