@@ -636,38 +636,51 @@ __device__ inline void affine_scan(T& A, T& B, int lane) {
 }
 
 // ------------------------------------------------------------------------------------------
-// Row-scan sweep (1D/2D/3D): one workgroup per direction, lanes along x.
+// Row-scan sweeps: lanes along x.
 //
 // Along a pencil (a row of cells in x) the DD cell solve is an affine map of the incoming
 // x-face flux:  psi_i = c0_i + c1_i fx_i  and  fx_{i+1} = 2 psi_i - fx_i = a_i + b_i fx_i with
 //   c0 = (Q + hy fy + hz fz)/S,  c1 = hx/S,  a = 2 c0,  b = 2 c1 - 1   (|b| < 1),
 // so a whole pencil is solved at once by an exclusive scan of the maps across the lanes (each
-// lane first composes its CPL consecutive cells).  Pencils (jy, jz) advance by diagonals
-// t = jy + jz; the y/z face fluxes of the previous diagonal live in LDS (FY[jz][x], FZ[jy][x]),
-// one barrier per diagonal.  Every global access is a contiguous 64*CPL-cell segment.
+// lane first composes its CPL consecutive cells).
+//
+// Every global load and store is issued unconditionally: loads of inactive lanes or items read
+// a clamped in-range address and their values are discarded; stores of inactive lanes go to a
+// trash line.  With no memory operation under a branch the wait counters stay exact, so the
+// loads issued for later pencils remain in flight while the current pencil is solved (a
+// branch around any of them makes the compiler wait for all outstanding loads instead).
 // ------------------------------------------------------------------------------------------
-template <typename T, int CPL, int U, bool DPP>
-__global__ __launch_bounds__(1024) void sweep_scan_kernel(const PassDesc* __restrict__ pds, const SweepTask* __restrict__ tasks) {
+__device__ __align__(256) double g_sweep_trash[64 * 12];
+
+// Slot k of a lane; slots of one lane are 64 elements apart so that stores to consecutive
+// slots stay separate instructions (the count of stores in flight is part of the design).
+template <typename T>
+__device__ inline T* trash_slot(int lane, int k) { return (T*)g_sweep_trash + k * 64 + lane; }
+
+// 1D/2D: one wave per direction; the rows y = 0..Ny-1 are solved in order with the y-face
+// fluxes in registers (each lane owns its cells).  The angular source and material of row
+// y + PD are loaded while row y is solved.
+template <typename T, int CPL, int PD, bool DPP>
+__global__ __launch_bounds__(64) void sweep_line_kernel(const PassDesc* __restrict__ pds,
+                                                        const SweepTask* __restrict__ tasks) {
     extern __shared__ __align__(16) unsigned char smem[];
     const SweepTask tk = tasks[blockIdx.x];
     const PassDesc& d = pds[tk.prob];
     const int n = tk.dir;
-    const int Nx = d.Nx, Ny = d.Ny, Nz = d.Nz, Nc = d.Nc;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, W = blockDim.x >> 6;
-    T* hy2 = (T*)smem;
-    T* hz2 = hy2 + Ny;
-    T* st = hz2 + Nz;
-    T* FY = st + d.Nmat;                 // [Nz][Nx]
-    T* FZ = FY + (size_t)Nz * Nx;        // [Ny][Nx] (3D only)
-    const double mu = d.mu[n], eta = d.eta[n], xi = d.xi[n];
-    const bool tang = (d.ndims == 1) && (fabs(mu) <= 1e-10);  // sn_1D_bte.jl:154-155
+    const int ndims = d.ndims, Nx = d.Nx, Ny = d.Ny, Nc = d.Nc, Nmat = d.Nmat;
+    const int lane = threadIdx.x;
+    T* hy2 = (T*)smem;   // [Ny]
+    T* st = hy2 + Ny;    // [Nmat]
+    const double mu = d.mu[n], eta = ndims >= 2 ? d.eta[n] : 0.0;
+    const bool tang = (ndims == 1) && (fabs(mu) <= 1e-10);  // sn_1D_bte.jl:154-155
     const double amu = tang ? 0.0 : fabs(mu);
-    for (int i = tid; i < Ny; i += blockDim.x) hy2[i] = (T)(d.ndims >= 2 ? 2.0 * (fabs(eta) / d.dy[i]) : 0.0);
-    for (int i = tid; i < Nz; i += blockDim.x) hz2[i] = (T)(d.ndims >= 3 ? 2.0 * (fabs(xi) / d.dz[i]) : 0.0);
-    for (int i = tid; i < d.Nmat; i += blockDim.x) st[i] = (T)d.sigt[i];
-    const int nf = Nz * Nx + (d.ndims >= 3 ? Ny * Nx : 0);
-    for (int i = tid; i < nf; i += blockDim.x) FY[i] = (T)0;  // void inflow on the y-/z- faces
-    const bool fxs = mu >= 0, fys = eta >= 0, fzs = xi >= 0;
+    for (int i = lane; i < Ny; i += 64) hy2[i] = (T)(ndims >= 2 ? 2.0 * (fabs(eta) / d.dy[i]) : 0.0);
+    for (int i = lane; i < Nmat; i += 64) st[i] = (T)d.sigt[i];
+    const bool fxs = mu >= 0, fys = eta >= 0;
+    T* __restrict__ psi = (T*)d.psi + (size_t)n * Nc;
+    T* __restrict__ bx = (T*)d.bx + (size_t)n * Ny;
+    T* __restrict__ by = (T*)d.by + (size_t)n * Nx;
+    const int* __restrict__ mat = d.mat;
     int ixc[CPL];
     bool inx[CPL];
     T hx[CPL];
@@ -678,6 +691,119 @@ __global__ __launch_bounds__(1024) void sweep_scan_kernel(const PassDesc* __rest
         ixc[c] = inx[c] ? (fxs ? jx : Nx - 1 - jx) : 0;
         hx[c] = inx[c] ? (T)(2.0 * (amu / d.dx[ixc[c]])) : (T)0;
     }
+    const bool last_lane = lane == (Nx - 1) / CPL;
+    __syncthreads();
+    auto row_of = [&](int jy) {
+        const int j = jy < Ny ? jy : Ny - 1;
+        return fys ? j : Ny - 1 - j;
+    };
+    T vr[PD][CPL];
+    int mr[PD][CPL];
+#pragma unroll
+    for (int i = 0; i < PD; ++i) {
+        // issued in slot order, so that slot 0 is the oldest load when the loop starts
+        __builtin_amdgcn_sched_barrier(0);
+        const int ro = Nx * row_of(i);
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            vr[i][c] = gld(psi + ro + ixc[c]);
+            mr[i][c] = gld(mat + ro + ixc[c]);
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    T fy[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) fy[c] = (T)0;
+    for (int r0 = 0; r0 < Ny; r0 += PD) {
+#pragma unroll
+        for (int i = 0; i < PD; ++i) {
+            const int jy = r0 + i;
+            const bool rv = jy < Ny;
+            const int iy = row_of(jy);
+            const int ro = Nx * iy;
+            const T hy = hy2[iy];
+            T c0[CPL], c1[CPL];
+            T A = (T)0, B = (T)1;
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                const T S = (st[mr[i][c]] + hx[c]) + hy;
+                const T rS = dd_div((T)1, S);
+                c0[c] = inx[c] ? (vr[i][c] + hy * fy[c]) * rS : (T)0;
+                c1[c] = hx[c] * rS;
+                const T a = tang ? (T)0 : c0[c] + c0[c];
+                const T b = (tang || !inx[c]) ? (T)1 : (c1[c] + c1[c]) - (T)1;
+                A = a + b * A;
+                B = b * B;
+            }
+            {
+                // Row jy + PD goes into the ring slot just freed.  The loads stay here: the
+                // scheduler would otherwise sink them towards their first use, and a slot whose
+                // old and new values overlap is rotated through copies that wait for the loads.
+                __builtin_amdgcn_sched_barrier(0);
+                const int ron = Nx * row_of(jy + PD);
+#pragma unroll
+                for (int c = 0; c < CPL; ++c) {
+                    vr[i][c] = gld(psi + ron + ixc[c]);
+                    mr[i][c] = gld(mat + ron + ixc[c]);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            affine_scan<DPP>(A, B, lane);
+            T fx = __shfl_up(A, 1);  // exclusive prefix applied to the void inflow (0)
+            if (lane == 0) fx = (T)0;
+#pragma unroll
+            for (int c = 0; c < CPL; ++c) {
+                const T p = c0[c] + c1[c] * fx;
+                gst((rv && inx[c]) ? psi + ro + ixc[c] : trash_slot<T>(lane, c), p);
+                const T p2 = p + p;
+                fx = (tang || !inx[c]) ? fx : p2 - fx;
+                fy[c] = rv ? p2 - fy[c] : fy[c];
+            }
+            gst((rv && last_lane) ? bx + iy : trash_slot<T>(lane, 8), fx);
+        }
+    }
+    if (ndims >= 2) {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c)
+            if (inx[c]) gst(by + ixc[c], fy[c]);  // exit y-face after the last row
+    }
+}
+
+// 3D: one workgroup per direction.  Pencils (jy, jz) advance by diagonals t = jy + jz; the
+// y/z face fluxes of the previous diagonal live in LDS (FY[jz][x], FZ[jy][x]), one barrier per
+// diagonal.  After the last diagonal FY and FZ hold the exit y- and z-faces, which are written
+// out at the end with coalesced stores.
+template <typename T, int CPL, int U, bool DPP>
+__global__ __launch_bounds__(1024) void sweep_scan_kernel(const PassDesc* __restrict__ pds, const SweepTask* __restrict__ tasks) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const SweepTask tk = tasks[blockIdx.x];
+    const PassDesc& d = pds[tk.prob];
+    const int n = tk.dir;
+    const int Nx = d.Nx, Ny = d.Ny, Nz = d.Nz, Nc = d.Nc, Nmat = d.Nmat;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, W = blockDim.x >> 6;
+    T* hy2 = (T*)smem;
+    T* hz2 = hy2 + Ny;
+    T* st = hz2 + Nz;
+    T* FY = st + Nmat;                   // [Nz][Nx]
+    T* FZ = FY + (size_t)Nz * Nx;        // [Ny][Nx]
+    T* FT = FZ + (size_t)Ny * Nx;        // [64*CPL] trash for inactive lanes
+    const double mu = d.mu[n], eta = d.eta[n], xi = d.xi[n];
+    for (int i = tid; i < Ny; i += blockDim.x) hy2[i] = (T)(2.0 * (fabs(eta) / d.dy[i]));
+    for (int i = tid; i < Nz; i += blockDim.x) hz2[i] = (T)(2.0 * (fabs(xi) / d.dz[i]));
+    for (int i = tid; i < Nmat; i += blockDim.x) st[i] = (T)d.sigt[i];
+    for (int i = tid; i < (Nz + Ny) * Nx; i += blockDim.x) FY[i] = (T)0;  // void inflow on the y-/z- faces
+    const bool fxs = mu >= 0, fys = eta >= 0, fzs = xi >= 0;
+    int ixc[CPL];
+    bool inx[CPL];
+    T hx[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+        const int jx = lane * CPL + c;
+        inx[c] = jx < Nx;
+        ixc[c] = inx[c] ? (fxs ? jx : Nx - 1 - jx) : 0;
+        hx[c] = inx[c] ? (T)(2.0 * (fabs(mu) / d.dx[ixc[c]])) : (T)0;
+    }
+    const bool last_lane = lane == (Nx - 1) / CPL;
     __syncthreads();
     T* __restrict__ psi = (T*)d.psi + (size_t)n * Nc;
     T* __restrict__ bx = (T*)d.bx + (size_t)n * Ny * Nz;
@@ -698,87 +824,78 @@ __global__ __launch_bounds__(1024) void sweep_scan_kernel(const PassDesc* __rest
     T vb[U][CPL];
     int mb[U][CPL];
     auto load_item = [&](int t, int q0) {
+        const int tt = t < ndiag ? t : ndiag - 1;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int q = q0 + u;
-            const bool ok = t < ndiag && q < diag_cnt(t);
-            const int jz = diag_lo(t) + (ok ? q : 0), jy = t - jz;
+            const bool ok = q < diag_cnt(tt);
+            const int jz = diag_lo(tt) + (ok ? q : 0), jy = tt - jz;
             const int iy = fys ? jy : Ny - 1 - jy, iz = fzs ? jz : Nz - 1 - jz;
             const int ro = Nx * (iy + Ny * iz);
 #pragma unroll
             for (int c = 0; c < CPL; ++c) {
-                vb[u][c] = (T)0;
-                mb[u][c] = 0;
-                if (ok && inx[c]) {
-                    vb[u][c] = gld(psi + ro + ixc[c]);
-                    mb[u][c] = gld(mat + ro + ixc[c]);
-                }
+                vb[u][c] = gld(psi + ro + ixc[c]);
+                mb[u][c] = gld(mat + ro + ixc[c]);
             }
         }
     };
     {
         int t0 = 0;
         while (t0 < ndiag && wave * U >= diag_cnt(t0)) ++t0;
+        __builtin_amdgcn_sched_barrier(0);
         load_item(t0, wave * U);
+        // As many stores as an item issues after its prefetch, so that the loop is entered with
+        // the same memory operations in flight as it repeats with (exact wait counts).
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int k = 0; k < U * (CPL + 1); ++k) gst(trash_slot<T>(lane, k), (T)0);
+        __builtin_amdgcn_sched_barrier(0);
     }
     for (int t = 0; t < ndiag; ++t) {
         const int jz_lo = diag_lo(t);
         const int cnt = diag_cnt(t);
         for (int q0 = wave * U; q0 < cnt; q0 += W * U) {
             // U pencils of this diagonal, processed together for instruction-level parallelism
-            T v[U][CPL], fy[U][CPL], fz[U][CPL], c0[U][CPL], c1[U][CPL];
-            int mm[U][CPL];
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-#pragma unroll
-                for (int c = 0; c < CPL; ++c) { v[u][c] = vb[u][c]; mm[u][c] = mb[u][c]; }
-            {
-                int tn, qn;
-                next_item(t, q0, tn, qn);
-                load_item(tn, qn);
-            }
-            int roff[U], jyv[U], jzv[U], iyv[U], izv[U];
+            T fy[U][CPL], fz[U][CPL], c0[U][CPL], c1[U][CPL];
+            int roff[U], fyo_i[U], fzo_i[U], iyv[U], izv[U];
             bool ok[U];
+            T A[U], B[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int q = q0 + u;
                 ok[u] = q < cnt;
-                jzv[u] = jz_lo + (ok[u] ? q : 0);
-                jyv[u] = t - jzv[u];
-                iyv[u] = fys ? jyv[u] : Ny - 1 - jyv[u];
-                izv[u] = fzs ? jzv[u] : Nz - 1 - jzv[u];
+                const int jz = jz_lo + (ok[u] ? q : 0), jy = t - jz;
+                iyv[u] = fys ? jy : Ny - 1 - jy;
+                izv[u] = fzs ? jz : Nz - 1 - jz;
                 roff[u] = Nx * (iyv[u] + Ny * izv[u]);
+                fyo_i[u] = jz * Nx;
+                fzo_i[u] = jy * Nx;
                 const T hy = hy2[iyv[u]], hz = hz2[izv[u]];
-#pragma unroll
-                for (int c = 0; c < CPL; ++c) {
-                    const int jx = lane * CPL + c;
-                    fy[u][c] = (T)0;
-                    fz[u][c] = (T)0;
-                    c0[u][c] = (T)0;
-                    c1[u][c] = (T)0;
-                    if (ok[u] && inx[c]) {
-                        fy[u][c] = FY[jzv[u] * Nx + jx];
-                        if (d.ndims >= 3) fz[u][c] = FZ[jyv[u] * Nx + jx];
-                        const T S = ((st[mm[u][c]] + hx[c]) + hy) + hz;
-                        const T rS = dd_div((T)1, S);
-                        c0[u][c] = ((v[u][c] + hy * fy[u][c]) + hz * fz[u][c]) * rS;
-                        c1[u][c] = hx[c] * rS;
-                    }
-                }
-            }
-            // compose each lane's CPL maps, then an exclusive scan of the maps across lanes
-            T A[U], B[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
                 A[u] = (T)0;
                 B[u] = (T)1;
 #pragma unroll
                 for (int c = 0; c < CPL; ++c) {
-                    const T a = tang ? (T)0 : c0[u][c] + c0[u][c];
-                    const T b = tang ? (T)1 : (c1[u][c] + c1[u][c]) - (T)1;
+                    const int jx = inx[c] ? lane * CPL + c : 0;
+                    fy[u][c] = FY[fyo_i[u] + jx];
+                    fz[u][c] = FZ[fzo_i[u] + jx];
+                    const T S = ((st[mb[u][c]] + hx[c]) + hy) + hz;
+                    const T rS = dd_div((T)1, S);
+                    c0[u][c] = inx[c] ? ((vb[u][c] + hy * fy[u][c]) + hz * fz[u][c]) * rS : (T)0;
+                    c1[u][c] = hx[c] * rS;
+                    // compose this lane's maps (inactive cells: identity)
+                    const T a = c0[u][c] + c0[u][c];
+                    const T b = inx[c] ? (c1[u][c] + c1[u][c]) - (T)1 : (T)1;
                     A[u] = a + b * A[u];
                     B[u] = b * B[u];
                 }
+            }
+            {
+                // the next item goes into the buffers just consumed (see sweep_line_kernel)
+                __builtin_amdgcn_sched_barrier(0);
+                int tn, qn;
+                next_item(t, q0, tn, qn);
+                load_item(tn, qn);
+                __builtin_amdgcn_sched_barrier(0);
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) affine_scan<DPP>(A[u], B[u], lane);
@@ -788,24 +905,30 @@ __global__ __launch_bounds__(1024) void sweep_scan_kernel(const PassDesc* __rest
                 if (lane == 0) fx = (T)0;
 #pragma unroll
                 for (int c = 0; c < CPL; ++c) {
+                    const bool act = ok[u] && inx[c];
                     const int jx = lane * CPL + c;
-                    if (ok[u] && inx[c]) {
-                        const T p = c0[u][c] + c1[u][c] * fx;
-                        gst(psi + roff[u] + ixc[c], p);
-                        const T p2 = p + p;
-                        if (!tang) fx = p2 - fx;
-                        const T fyo = p2 - fy[u][c];
-                        const T fzo = p2 - fz[u][c];
-                        FY[jzv[u] * Nx + jx] = fyo;
-                        if (d.ndims >= 3) FZ[jyv[u] * Nx + jx] = fzo;
-                        if (jx == Nx - 1) gst(bx + iyv[u] + Ny * izv[u], fx);
-                        if (d.ndims >= 2 && jyv[u] == Ny - 1) gst(by + ixc[c] + Nx * izv[u], fyo);
-                        if (d.ndims >= 3 && jzv[u] == Nz - 1) gst(bz + ixc[c] + Nx * iyv[u], fzo);
-                    }
+                    const T p = c0[u][c] + c1[u][c] * fx;
+                    gst(act ? psi + roff[u] + ixc[c] : trash_slot<T>(lane, c), p);
+                    const T p2 = p + p;
+                    fx = inx[c] ? p2 - fx : fx;
+                    *(act ? FY + fyo_i[u] + jx : FT + jx) = p2 - fy[u][c];
+                    *(act ? FZ + fzo_i[u] + jx : FT + jx) = p2 - fz[u][c];
                 }
+                gst((ok[u] && last_lane) ? bx + iyv[u] + Ny * izv[u] : trash_slot<T>(lane, 8 + u), fx);
             }
         }
         __syncthreads();
+    }
+    // exit faces: FY[jz][jx] -> by[ix + Nx iz], FZ[jy][jx] -> bz[ix + Nx iy]
+    for (int i = tid; i < Nz * Nx; i += blockDim.x) {
+        const int jz = i / Nx, jx = i - jz * Nx;
+        const int ix = fxs ? jx : Nx - 1 - jx, iz = fzs ? jz : Nz - 1 - jz;
+        gst(by + ix + Nx * iz, FY[i]);
+    }
+    for (int i = tid; i < Ny * Nx; i += blockDim.x) {
+        const int jy = i / Nx, jx = i - jy * Nx;
+        const int ix = fxs ? jx : Nx - 1 - jx, iy = fys ? jy : Ny - 1 - jy;
+        gst(bz + ix + Nx * iy, FZ[i]);
     }
 }
 
@@ -1156,11 +1279,14 @@ int sweep_scan_cpl(int Nx) {
 }
 
 size_t sweep_scan_lds_bytes(int prec, int ndims, int Nx, int Ny, int Nz, int Nmat) {
-    return (prec == P64 ? 8 : 4) * ((size_t)Ny + Nz + Nmat + (size_t)Nz * Nx + (ndims >= 3 ? (size_t)Ny * Nx : 0));
+    const size_t es = prec == P64 ? 8 : 4;
+    if (ndims < 3) return es * ((size_t)Ny + Nmat);
+    const int cpl = sweep_scan_cpl(Nx);
+    return es * ((size_t)Ny + Nz + Nmat + ((size_t)Nz + Ny) * Nx + 64 * (size_t)(cpl > 0 ? cpl : 8));
 }
 
 int sweep_scan_waves(int ndims, int Ny, int Nz) {
-    if (ndims < 3) return 1;
+    if (ndims < 3) return 0;              // 1D/2D: the line kernel (one wave per direction)
     const int diag = Ny < Nz ? Ny : Nz;  // longest diagonal of the (y, z) pencil grid
     int w = (diag + 1) / 2;              // two pencils per wave per diagonal
     return w < 1 ? 1 : (w > 16 ? 16 : w);
@@ -1174,6 +1300,18 @@ static bool scan_use_dpp() {
 
 template <typename T, int CPL, bool DPP>
 static void scan_inst2(const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int waves, size_t lds, hipStream_t st) {
+    if (waves == 0) {
+        // rows in flight: keeps the outstanding loads and stores of PD rows within vmcnt's 63
+        constexpr int PD = CPL <= 2 ? 8 : (CPL == 4 ? 4 : 2);
+        auto lfn = sweep_line_kernel<T, CPL, PD, DPP>;
+        static bool lattr_set = false;
+        if (!lattr_set) {
+            RT_HIP(hipFuncSetAttribute((const void*)lfn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            lattr_set = true;
+        }
+        hipLaunchKernelGGL(lfn, dim3(ntasks), dim3(64), lds, st, d_pd, tasks);
+        return;
+    }
     constexpr int U = CPL >= 4 ? 1 : 2;
     auto kfn = sweep_scan_kernel<T, CPL, U, DPP>;
     static bool attr_set = false;
