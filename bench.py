@@ -287,7 +287,7 @@ def main():
         log(f"[cpu] {cpu['value']:.3e} updates/s ({cpu['wall_s']:.1f} s)")
 
     def summary(r):
-        return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()
+        return {k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in r.items()
                 if k in ("updates_per_s", "solves_per_s", "p50_ms", "p99_ms", "raw_updates_per_s", "passes",
                          "passes_lo", "niter", "refinements", "converged", "err_vs_fp64", "batch")}
 

@@ -98,6 +98,8 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
     verbose = d.verbose;
     act = active_moments(d);
     Nact = (int)act.size();
+    if (Nact > CONTRACT_KMAX)
+        throw Error(RT_EINVAL, "more than " + std::to_string(CONTRACT_KMAX) + " moments carry a scattering source");
     npos[0] = (size_t)Ny * Nz;
     npos[1] = ndims >= 2 ? (size_t)Nx * Nz : 0;
     npos[2] = ndims >= 3 ? (size_t)Nx * Ny : 0;

@@ -78,9 +78,17 @@ __device__ inline short f2bf(float x) {
 // The operators are stored k-major (MT[K][M]) so that each MFMA operand load is a contiguous
 // 16-element row segment.  A wave computes RT x CT tiles of 16x16 (RT*16 rows, CT*16 cells):
 // every A fragment is reused CT times and every B fragment RT times from registers.
+//
+// The k loop is software-pipelined over two register stages: the operands of step s+1 are in
+// flight while the MFMAs of step s run.  Every operand load is unconditional (padding rows,
+// cells and k read clamped in-range addresses, absent inputs read a zero line), so the wait
+// counts stay exact and the loads of the next stage are never waited for early.
 // ------------------------------------------------------------------------------------------
+__device__ __align__(256) double g_zero[64];  // zero-initialised, never written
+
 template <typename T, int RT, int CT, bool MG>
 __global__ __launch_bounds__(256) void contract_kernel(const PassDesc* __restrict__ pds, const GemmTask* __restrict__ tasks) {
+    __shared__ int rows_s[MG ? CONTRACT_KMAX : 1];
     const GemmTask tk = tasks[blockIdx.x];
     const PassDesc& d = pds[tk.prob];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -88,50 +96,102 @@ __global__ __launch_bounds__(256) void contract_kernel(const PassDesc* __restric
     const int K = MG ? d.Nact : d.Nd;
     const int M = MG ? d.Nd : d.Nrows;
     const T* __restrict__ At = (const T*)(MG ? d.Mact : d.Drows);
+    const T* __restrict__ psi_in = (const T*)d.psi;
+    const double* __restrict__ qlout = d.qlout;
+    const double* __restrict__ phi_in = d.phi_in;
+    const double* __restrict__ sigw = d.sigw;
+    const bool hasq = qlout != nullptr, hasp = phi_in != nullptr;
     const int c0 = tk.tile * GEMM_CELLS + wave * (CT * 16);
     const int r0 = tk.rgroup * GEMM_ROWS;
     const int kk = lane >> 4, li = lane & 15;
+    int colc[CT], rowc[RT];
+    bool colok[CT], rowok[RT];
+#pragma unroll
+    for (int u = 0; u < CT; ++u) {
+        const int col = c0 + 16 * u + li;
+        colok[u] = col < Nc;
+        colc[u] = colok[u] ? col : Nc - 1;
+    }
+#pragma unroll
+    for (int t = 0; t < RT; ++t) {
+        const int row = r0 + 16 * t + li;
+        rowok[t] = row < M;
+        rowc[t] = rowok[t] ? row : M - 1;
+    }
+    // Absent inputs are selected here, once, as address arithmetic: a missing Qlout or Φ is
+    // read from the zero line with stride 0 (selects inside the k loop become branches).
+    int qcol[CT], pcol[CT], wrow[CT];
+    const double *qb = g_zero, *pb = g_zero, *wb = g_zero;
+    size_t qks = 0, pks = 0;
+    int wks = 0;
+    if (MG) {
+        const int* __restrict__ rows = d.phi_rows;
+        for (int i = threadIdx.x; i < K; i += blockDim.x) rows_s[i] = rows ? rows[i] : i;
+        if (hasq) { qb = qlout; qks = (size_t)Nc; }
+        if (hasp) { pb = phi_in; pks = (size_t)Nc; wb = sigw; wks = 1; }
+#pragma unroll
+        for (int u = 0; u < CT; ++u) {
+            const int mc = gld(d.mat + colc[u]);
+            qcol[u] = hasq ? colc[u] : li;
+            pcol[u] = hasp ? colc[u] : li;
+            wrow[u] = hasp ? mc * K : li;
+        }
+        __syncthreads();
+    }
     typename Acc<T>::type acc[RT][CT];
 #pragma unroll
     for (int t = 0; t < RT; ++t)
 #pragma unroll
         for (int u = 0; u < CT; ++u) acc[t][u] = (typename Acc<T>::type)(0);
-    int mc[CT];
-    if (MG) {
+    // A stage holds the raw loads of one k step; padding (k >= K, rows >= M, cells >= Nc) reads
+    // clamped in-range data and is zeroed by a 0/1 factor when the MFMA operands are formed (a
+    // select would let the compiler move a load into a branch; forming the operands at load
+    // time would wait for the loads right after issuing them).
+    struct Stage { T a[RT]; double q[CT], w[CT], p[CT]; T b[CT]; };
+    auto load = [&](int s, Stage& g) {
+        const int k = 4 * s + kk;
+        const int kc = k < K ? k : K - 1;
+#pragma unroll
+        for (int t = 0; t < RT; ++t) g.a[t] = gld(At + (size_t)kc * M + rowc[t]);
 #pragma unroll
         for (int u = 0; u < CT; ++u) {
-            const int col = c0 + 16 * u + li;
-            mc[u] = col < Nc ? gld(d.mat + col) : 0;
-        }
-    }
-    for (int k0 = 0; k0 < K; k0 += 4) {
-        const int k = k0 + kk;
-        T b[CT], a[RT];
-#pragma unroll
-        for (int u = 0; u < CT; ++u) {
-            const int col = c0 + 16 * u + li;
-            b[u] = (T)0;
-            if (k < K && col < Nc) {
-                if (MG) {
-                    const int r = d.phi_rows ? d.phi_rows[k] : k;
-                    double q = d.qlout ? gld(d.qlout + (size_t)k * Nc + col) : 0.0;
-                    if (d.phi_in) q += d.sigw[mc[u] * K + k] * gld(d.phi_in + (size_t)r * Nc + col);
-                    b[u] = (T)q;
-                } else {
-                    b[u] = gld((const T*)d.psi + (size_t)k * Nc + col);
-                }
+            if (MG) {
+                g.q[u] = gld(qb + kc * qks + qcol[u]);
+                g.w[u] = gld(wb + kc * wks + wrow[u]);
+                g.p[u] = gld(pb + (size_t)rows_s[kc] * pks + pcol[u]);
+            } else {
+                g.b[u] = gld(psi_in + (size_t)kc * Nc + colc[u]);
             }
         }
+    };
+    auto mma = [&](int s, const Stage& g) {
+        const bool kok = 4 * s + kk < K;
+        T a[RT], b[CT];
 #pragma unroll
-        for (int t = 0; t < RT; ++t) {
-            const int row = r0 + 16 * t + li;
-            a[t] = (k < K && row < M) ? gld(At + (size_t)k * M + row) : (T)0;
+        for (int t = 0; t < RT; ++t) a[t] = g.a[t] * ((kok && rowok[t]) ? (T)1 : (T)0);
+#pragma unroll
+        for (int u = 0; u < CT; ++u) {
+            if (MG) b[u] = (T)((g.q[u] + g.w[u] * g.p[u]) * ((kok && colok[u]) ? 1.0 : 0.0));
+            else b[u] = g.b[u] * ((kok && colok[u]) ? (T)1 : (T)0);
         }
 #pragma unroll
         for (int t = 0; t < RT; ++t)
             if (r0 + 16 * t < M)  // wave-uniform
 #pragma unroll
                 for (int u = 0; u < CT; ++u) acc[t][u] = mfma4(a[t], b[u], acc[t][u]);
+    };
+    const int nsteps = (K + 3) >> 2;
+    Stage g0, g1;
+    load(0, g0);
+    for (int s = 0; s < nsteps; s += 2) {
+        __builtin_amdgcn_sched_barrier(0);
+        load(s + 1, g1);
+        __builtin_amdgcn_sched_barrier(0);
+        mma(s, g0);
+        __builtin_amdgcn_sched_barrier(0);
+        load(s + 2, g0);
+        __builtin_amdgcn_sched_barrier(0);
+        mma(s + 1, g1);  // all zero when s + 1 == nsteps
     }
 #pragma unroll
     for (int t = 0; t < RT; ++t) {
@@ -157,8 +217,10 @@ __global__ __launch_bounds__(256) void contract_kernel(const PassDesc* __restric
 
 // BF16 operands, FP32 accumulation (v_mfma_f32_16x16x16_bf16): A[16x16] lane l holds
 // A[l%16][4*(l/16)+j], B[16x16] lane l holds B[4*(l/16)+j][l%16], j = 0..3; C as the f32 case.
+// Pipelined as contract_kernel; the operands are staged in FP32 and rounded to BF16 when used.
 template <int RT, int CT, bool MG>
 __global__ __launch_bounds__(256) void contract_bf16_kernel(const PassDesc* __restrict__ pds, const GemmTask* __restrict__ tasks) {
+    __shared__ int rows_s[MG ? CONTRACT_KMAX : 1];
     const GemmTask tk = tasks[blockIdx.x];
     const PassDesc& d = pds[tk.prob];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -166,58 +228,106 @@ __global__ __launch_bounds__(256) void contract_bf16_kernel(const PassDesc* __re
     const int K = MG ? d.Nact : d.Nd;
     const int M = MG ? d.Nd : d.Nrows;
     const float* __restrict__ At = (const float*)(MG ? d.Mact : d.Drows);
+    const float* __restrict__ psi_in = (const float*)d.psi;
+    const double* __restrict__ qlout = d.qlout;
+    const double* __restrict__ phi_in = d.phi_in;
+    const double* __restrict__ sigw = d.sigw;
+    const bool hasq = qlout != nullptr, hasp = phi_in != nullptr;
     const int c0 = tk.tile * GEMM_CELLS + wave * (CT * 16);
     const int r0 = tk.rgroup * GEMM_ROWS;
     const int kq = 4 * (lane >> 4), li = lane & 15;
+    int colc[CT], rowc[RT];
+    bool colok[CT], rowok[RT];
+#pragma unroll
+    for (int u = 0; u < CT; ++u) {
+        const int col = c0 + 16 * u + li;
+        colok[u] = col < Nc;
+        colc[u] = colok[u] ? col : Nc - 1;
+    }
+#pragma unroll
+    for (int t = 0; t < RT; ++t) {
+        const int row = r0 + 16 * t + li;
+        rowok[t] = row < M;
+        rowc[t] = rowok[t] ? row : M - 1;
+    }
+    // Absent inputs are selected here, once, as address arithmetic: a missing Qlout or Φ is
+    // read from the zero line with stride 0 (selects inside the k loop become branches).
+    int qcol[CT], pcol[CT], wrow[CT];
+    const double *qb = g_zero, *pb = g_zero, *wb = g_zero;
+    size_t qks = 0, pks = 0;
+    int wks = 0;
+    if (MG) {
+        const int* __restrict__ rows = d.phi_rows;
+        for (int i = threadIdx.x; i < K; i += blockDim.x) rows_s[i] = rows ? rows[i] : i;
+        if (hasq) { qb = qlout; qks = (size_t)Nc; }
+        if (hasp) { pb = phi_in; pks = (size_t)Nc; wb = sigw; wks = 1; }
+#pragma unroll
+        for (int u = 0; u < CT; ++u) {
+            const int mc = gld(d.mat + colc[u]);
+            qcol[u] = hasq ? colc[u] : li;
+            pcol[u] = hasp ? colc[u] : li;
+            wrow[u] = hasp ? mc * K : li;
+        }
+        __syncthreads();
+    }
     v4f acc[RT][CT];
 #pragma unroll
     for (int t = 0; t < RT; ++t)
 #pragma unroll
         for (int u = 0; u < CT; ++u) acc[t][u] = (v4f)(0.0f);
-    int mc[CT];
-    if (MG) {
+    struct Stage { float a[RT][4]; double q[CT][4], w[CT][4], p[CT][4]; float b[CT][4]; };
+    auto load = [&](int s, Stage& g) {
 #pragma unroll
-        for (int u = 0; u < CT; ++u) {
-            const int col = c0 + 16 * u + li;
-            mc[u] = col < Nc ? gld(d.mat + col) : 0;
-        }
-    }
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        v4s b[CT], a[RT];
+        for (int j = 0; j < 4; ++j) {
+            const int k = 16 * s + kq + j;
+            const int kc = k < K ? k : K - 1;
 #pragma unroll
-        for (int u = 0; u < CT; ++u) {
-            const int col = c0 + 16 * u + li;
+            for (int t = 0; t < RT; ++t) g.a[t][j] = gld(At + (size_t)kc * M + rowc[t]);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int k = k0 + kq + j;
-                float v = 0.0f;
-                if (k < K && col < Nc) {
-                    if (MG) {
-                        const int r = d.phi_rows ? d.phi_rows[k] : k;
-                        double q = d.qlout ? gld(d.qlout + (size_t)k * Nc + col) : 0.0;
-                        if (d.phi_in) q += d.sigw[mc[u] * K + k] * gld(d.phi_in + (size_t)r * Nc + col);
-                        v = (float)q;
-                    } else {
-                        v = gld((const float*)d.psi + (size_t)k * Nc + col);
-                    }
+            for (int u = 0; u < CT; ++u) {
+                if (MG) {
+                    g.q[u][j] = gld(qb + kc * qks + qcol[u]);
+                    g.w[u][j] = gld(wb + kc * wks + wrow[u]);
+                    g.p[u][j] = gld(pb + (size_t)rows_s[kc] * pks + pcol[u]);
+                } else {
+                    g.b[u][j] = gld(psi_in + (size_t)kc * Nc + colc[u]);
                 }
-                b[u][j] = f2bf(v);
             }
         }
+    };
+    auto mma = [&](int s, const Stage& g) {
+        v4s ab[RT], bb[CT];
 #pragma unroll
-        for (int t = 0; t < RT; ++t) {
-            const int row = r0 + 16 * t + li;
+        for (int j = 0; j < 4; ++j) {
+            const bool kok = 16 * s + kq + j < K;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int k = k0 + kq + j;
-                a[t][j] = f2bf((k < K && row < M) ? gld(At + (size_t)k * M + row) : 0.0f);
+            for (int t = 0; t < RT; ++t) ab[t][j] = f2bf(g.a[t][j] * ((kok && rowok[t]) ? 1.0f : 0.0f));
+#pragma unroll
+            for (int u = 0; u < CT; ++u) {
+                const float v = MG ? (float)((g.q[u][j] + g.w[u][j] * g.p[u][j]) * ((kok && colok[u]) ? 1.0 : 0.0))
+                                   : g.b[u][j] * ((kok && colok[u]) ? 1.0f : 0.0f);
+                bb[u][j] = f2bf(v);
             }
         }
 #pragma unroll
         for (int t = 0; t < RT; ++t)
             if (r0 + 16 * t < M)
 #pragma unroll
-                for (int u = 0; u < CT; ++u) acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], b[u], acc[t][u], 0, 0, 0);
+                for (int u = 0; u < CT; ++u)
+                    acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ab[t], bb[u], acc[t][u], 0, 0, 0);
+    };
+    const int nsteps = (K + 15) >> 4;
+    Stage g0, g1;
+    load(0, g0);
+    for (int s = 0; s < nsteps; s += 2) {
+        __builtin_amdgcn_sched_barrier(0);
+        load(s + 1, g1);
+        __builtin_amdgcn_sched_barrier(0);
+        mma(s, g0);
+        __builtin_amdgcn_sched_barrier(0);
+        load(s + 2, g0);
+        __builtin_amdgcn_sched_barrier(0);
+        mma(s + 1, g1);
     }
 #pragma unroll
     for (int t = 0; t < RT; ++t) {

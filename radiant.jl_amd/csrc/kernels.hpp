@@ -76,6 +76,7 @@ constexpr int GEMM_RT = 4;                      // 16-row MFMA tiles per wave
 constexpr int GEMM_CT = 4;                      // 16-cell MFMA tiles per wave
 constexpr int GEMM_ROWS = 16 * GEMM_RT;         // rows per work tile
 constexpr int GEMM_CELLS = 4 * 16 * GEMM_CT;    // cells per work tile (4 waves)
+constexpr int CONTRACT_KMAX = 1024;             // active moments at most (row map held in LDS)
 
 // Out-of-group + volume source of group g (sn_flux.jl:175-179):
 //   qlout[a][c] = sum_{gi<g} sigo[g][gi][mat c][a] * phi[gi][act a][c] + (p_a == 0) * qvol[g][c]

@@ -255,31 +255,35 @@ struct rt_server {
     std::vector<int64_t> finished;           // workers to join
 
     void run_request(std::shared_ptr<Request> r) {
+        // The outcome is published in one critical section (state, results and completion time
+        // together), after the device memory of the solve has been released.
+        Request::State state = Request::CANCELLED;
+        std::vector<double> phi;
+        std::vector<rt_group_stats> st;
+        std::string err;
+        int code = 0;
         try {
             if (!r->cancel.load()) {
                 rt::Problem prob(r->desc->d);
-                std::vector<rt_group_stats> st(r->desc->d.G);
+                st.resize(r->desc->d.G);
                 prob.solve(st.data(), nullptr);
-                std::vector<double> phi((size_t)prob.G * prob.Np * prob.Nc);
+                phi.resize((size_t)prob.G * prob.Np * prob.Nc);
                 prob.get_flux(phi.data());
-                std::lock_guard<std::mutex> lk(mu);
-                r->phi.swap(phi);
-                r->stats.swap(st);
-                r->state = r->cancel.load() ? Request::CANCELLED : Request::DONE;
-            } else {
-                std::lock_guard<std::mutex> lk(mu);
-                r->state = Request::CANCELLED;
+                state = r->cancel.load() ? Request::CANCELLED : Request::DONE;
             }
         } catch (const rt::Error& e) {
-            std::lock_guard<std::mutex> lk(mu);
-            r->state = Request::FAILED; r->err = e.what(); r->code = e.code;
+            state = Request::FAILED; err = e.what(); code = e.code;
         } catch (const std::exception& e) {
-            std::lock_guard<std::mutex> lk(mu);
-            r->state = Request::FAILED; r->err = e.what(); r->code = RT_EDEVICE;
+            state = Request::FAILED; err = e.what(); code = RT_EDEVICE;
         }
         {
             std::lock_guard<std::mutex> lk(mu);
+            r->phi.swap(phi);
+            r->stats.swap(st);
+            r->err = err;
+            r->code = code;
             r->t_done = std::chrono::steady_clock::now();
+            r->state = state;
             r->desc.reset();
             bytes_in_use -= r->bytes;
             --active;
