@@ -4,7 +4,8 @@
 // phase (c = T(0), a low-precision GMRES matvec, an FP64 refinement residual or the FP64
 // reconstruction); the passes of all problems share the same kernel launches (the contraction
 // and sweep task tables carry the problem index).  The Arnoldi step is classical Gram-Schmidt
-// with one re-orthogonalisation (CGS2) in FP64, batched over problems with segmented
+// with one re-orthogonalisation (CGS2) in FP64: three fused passes over the basis (h1 = V^T w;
+// w -= V h1 with h2 = V^T w; w -= V h2 with |w|^2), batched over problems with per-problem
 // deterministic reductions.  The host reads every problem's scalars with one copy per round and
 // advances each problem's state machine (the logic of sn_flux.jl:172-205 + sn_one_speed.jl
 // with the iterative refinement of the mixed-precision mode).
@@ -21,7 +22,6 @@ namespace rt {
 namespace {
 constexpr int SLOTS = 128;  // per problem: [0,40) h1, [40,80) h2, 80 |w|^2, 81 generic norm^2
 constexpr int S_H1 = 0, S_H2 = 40, S_WN = 80, S_NRM = 81;
-constexpr int BPI = 16;     // blocks per segmented dot
 }  // namespace
 
 BatchSolver::BatchSolver(const std::vector<Problem*>& probs, hipStream_t stream) : st(stream) {
@@ -72,30 +72,31 @@ BatchSolver::BatchSolver(const std::vector<Problem*>& probs, hipStream_t stream)
     d_dg = (GemmTask*)al(sizeof(GemmTask) * cap_gemm);
     d_sw = (SweepTask*)al(sizeof(SweepTask) * cap_sweep);
     d_slots = (double*)al(sizeof(double) * SLOTS * n);
-    d_coef = (double*)al(sizeof(double) * 64 * n);
-    const size_t max_items = 64 * (size_t)n + 8;
-    d_partial = (double*)al(sizeof(double) * BPI * max_items);
-    d_counters = (unsigned*)al(sizeof(unsigned) * max_items);
-    RT_CHECK(hipMemsetAsync(d_counters, 0, sizeof(unsigned) * max_items, st));
+    d_partial = (double*)al(sizeof(double) * GS_BPI_MAX * GS_KMAX * (size_t)n);
     RT_CHECK(hipHostMalloc((void**)&h_slots, sizeof(double) * SLOTS * n, hipHostMallocDefault));
-    arena_cap = (sizeof(PassDesc) + 3 * 64 * sizeof(DotItem) + 16 * sizeof(VecItem) + 64 * sizeof(double) +
+    arena_cap = (sizeof(PassDesc) + 4 * sizeof(GsItem) + 16 * sizeof(VecItem) + 64 * sizeof(double) +
                  64 * sizeof(int)) * (size_t)n + (1 << 20);
     d_arena = (unsigned char*)al(arena_cap);
     RT_CHECK(hipHostMalloc((void**)&h_arena, arena_cap, hipHostMallocDefault));
     RT_CHECK(hipStreamSynchronize(st));
 }
 
-// Copies host data into the round's staging arena and returns its device address.  The copies
-// are stream-ordered; the arena is reused only after the end-of-round synchronisation.
+// Copies host data into the staging arena and returns the device address it will have after
+// the next flush().  The arena is reused only after the end-of-round synchronisation.
 template <class T>
 T* BatchSolver::stage(const T* data, size_t count) {
     const size_t bytes = count * sizeof(T);
     const size_t off = (arena_off + 255) & ~size_t(255);
     if (off + bytes > arena_cap) throw Error(RT_ENOMEM, "batched solver: staging arena exhausted");
     std::memcpy(h_arena + off, data, bytes);
-    RT_CHECK(hipMemcpyAsync(d_arena + off, h_arena + off, bytes, hipMemcpyHostToDevice, st));
     arena_off = off + bytes;
     return (T*)(d_arena + off);
+}
+
+void BatchSolver::flush() {
+    if (arena_off > 0) RT_CHECK(hipMemcpyAsync(d_arena, h_arena, arena_off, hipMemcpyHostToDevice, st));
+    for (auto& f : plan) f();
+    plan.clear();
 }
 
 BatchSolver::~BatchSolver() {
@@ -112,13 +113,23 @@ void BatchSolver::run_vecops(const std::vector<VecItem>& items) {
     long nmax = 0;
     for (auto& it : items) nmax = std::max(nmax, it.n);
     const int bpi = (int)std::min<long>(256, std::max<long>(1, (nmax + 2047) / 2048));
-    launch_vecops(d, (int)items.size(), bpi, st);
+    const int ni = (int)items.size();
+    hipStream_t s = st;
+    plan.push_back([=] { launch_vecops(d, ni, bpi, s); });
 }
 
-void BatchSolver::run_dots(const std::vector<DotItem>& items) {
+void BatchSolver::run_gs(const std::vector<GsItem>& items, bool update, bool norm) {
     if (items.empty()) return;
-    DotItem* d = stage(items.data(), items.size());
-    launch_segdot(d, (int)items.size(), BPI, d_partial, d_counters, st);
+    int kmax = 1, bmax = 1;
+    for (auto& it : items) {
+        kmax = std::max(kmax, it.k);
+        bmax = std::max(bmax, it.bpi);
+    }
+    GsItem* d = stage(items.data(), items.size());
+    const int ni = (int)items.size();
+    double* part = d_partial;
+    hipStream_t s = st;
+    plan.push_back([=] { launch_gs(d, ni, kmax, bmax, update, norm, part, s); });
 }
 
 void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<PassDesc>& pds,
@@ -155,19 +166,32 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
         if (mmeta.empty()) continue;
         // task tables expanded on the device from the small per-problem metadata
         const int nm = (int)mmeta.size() / 4;
-        launch_expand_gemm(stage(mmeta.data(), mmeta.size()), nm, d_mg, st);
-        launch_expand_gemm(stage(dmeta.data(), dmeta.size()), nm, d_dg, st);
-        launch_mgemm(pr, dpd, d_mg, moff, st);
+        const int* dmm = stage(mmeta.data(), mmeta.size());
+        const int* ddm = stage(dmeta.data(), dmeta.size());
+        GemmTask *mg = d_mg, *dg = d_dg;
         SweepTask* swbase = d_sw;
+        hipStream_t s = st;
+        const int ps_ = pr;
+        plan.push_back([=] {
+            launch_expand_gemm(dmm, nm, mg, s);
+            launch_expand_gemm(ddm, nm, dg, s);
+            launch_mgemm(ps_, dpd, mg, moff, s);
+        });
         for (auto& kv : sweeps) {
             const std::vector<int>& sm = kv.second;
-            launch_expand_sweep(stage(sm.data(), sm.size()), (int)sm.size() / 3, swbase, st);
+            const int* dsm = stage(sm.data(), sm.size());
+            const int nsm = (int)sm.size() / 3;
             const int ntask = soff[kv.first];
-            launch_sweep_scan(pr == PBF16 ? P32 : pr, dpd, swbase, ntask, kv.first.first, kv.first.second,
-                              slds[kv.first], st);
+            const int cpl = kv.first.first, waves = kv.first.second;
+            const size_t lds = slds[kv.first];
+            SweepTask* sb = swbase;
+            plan.push_back([=] {
+                launch_expand_sweep(dsm, nsm, sb, s);
+                launch_sweep_scan(ps_ == PBF16 ? P32 : ps_, dpd, sb, ntask, cpl, waves, lds, s);
+            });
             swbase += ntask;
         }
-        launch_dgemm(pr, dpd, d_dg, doff, st);
+        plan.push_back([=] { launch_dgemm(ps_, dpd, dg, doff, s); });
     }
     for (size_t i = 0; i < who.size(); ++i) {
         Problem* P = ps[who[i]].P;
@@ -225,66 +249,65 @@ void BatchSolver::run(rt_solve_info* info) {
             who.push_back(i);
             pds.push_back(pd);
         }
-        if (who.empty()) break;
+        if (who.empty()) {
+            flush();  // vector operations recorded for problems that have just finished
+            break;
+        }
         ++rounds;
         launch_passes(who, pds, prec, full);
-        // ---- Krylov work after the pass
-        std::vector<DotItem> d1;
-        std::vector<VecItem> v1;
+        // ---- Krylov work after the pass: norms of c and r; CGS2 for the Arnoldi steps
+        std::vector<VecItem> v1, v4;
+        std::vector<GsItem> gn, g1, g2, g3;
         for (int i : who) {
             PS& s = ps[i];
             double* slots = d_slots + (size_t)SLOTS * i;
             const long S = (long)s.P->S_r;
-            if (s.phase == CPASS) d1.push_back({s.c, nullptr, S, slots + S_NRM});
+            const int bpi = gs_blocks(S);
+            auto gs = [&](double* y, int k, const double* coef, double* out) {
+                GsItem it{};
+                it.V = (const double* const*)s.P->d_Vptr; it.coef = coef; it.y = y; it.out = out;
+                it.n = S; it.k = k; it.bpi = bpi;
+                return it;
+            };
+            if (s.phase == CPASS) gn.push_back(gs(s.c, 0, nullptr, slots + S_NRM));
             if (s.phase == RESID) {
                 VecItem it{};
                 it.op = VOP_SUB; it.n = S; it.y = s.r; it.x = s.c; it.x2 = s.t;
                 v1.push_back(it);
-                d1.push_back({s.r, nullptr, S, slots + S_NRM});
+                gn.push_back(gs(s.r, 0, nullptr, slots + S_NRM));
             }
-            if (s.phase == ARNOLDI)
-                for (int k = 0; k <= s.j; ++k) d1.push_back({s.V[k], s.w, S, slots + S_H1 + k});
+            if (s.phase == ARNOLDI) {
+                const int k = s.j + 1;
+                g1.push_back(gs(s.w, k, nullptr, slots + S_H1));        // h1 = V^T w
+                g2.push_back(gs(s.w, k, slots + S_H1, slots + S_H2));   // w -= V h1 ; h2 = V^T w
+                g3.push_back(gs(s.w, k, slots + S_H2, slots + S_WN));   // w -= V h2 ; |w|^2
+                VecItem nm{};
+                nm.op = VOP_NORMALIZE; nm.n = S; nm.y = s.V[s.j + 1]; nm.x = s.w; nm.slot = slots + S_WN;
+                v4.push_back(nm);
+            }
         }
         run_vecops(v1);
-        run_dots(d1);
-        // CGS2: w -= V h1 ; h2 = V^T w ; w -= V h2 ; |w|^2 ; V[j+1] = w/|w|
-        std::vector<VecItem> v2, v3, v4;
-        std::vector<DotItem> d2, d3;
-        for (int i : who) {
-            PS& s = ps[i];
-            if (s.phase != ARNOLDI) continue;
-            double* slots = d_slots + (size_t)SLOTS * i;
-            const long S = (long)s.P->S_r;
-            VecItem it{};
-            it.op = VOP_SUBMULTI; it.n = S; it.y = s.w; it.V = (const double* const*)s.P->d_Vptr; it.k = s.j + 1;
-            it.coef = slots + S_H1;
-            v2.push_back(it);
-            for (int k = 0; k <= s.j; ++k) d2.push_back({s.V[k], s.w, S, slots + S_H2 + k});
-            it.coef = slots + S_H2;
-            v3.push_back(it);
-            d3.push_back({s.w, nullptr, S, slots + S_WN});
-            VecItem nm{};
-            nm.op = VOP_NORMALIZE; nm.n = S; nm.y = s.V[s.j + 1]; nm.x = s.w; nm.slot = slots + S_WN;
-            v4.push_back(nm);
-        }
-        run_vecops(v2);
-        run_dots(d2);
-        run_vecops(v3);
-        run_dots(d3);
+        run_gs(gn, false, true);
+        run_gs(g1, false, false);
+        run_gs(g2, true, false);
+        run_gs(g3, true, true);
         run_vecops(v4);
+        flush();
         RT_CHECK(hipMemcpyAsync(h_slots, d_slots, sizeof(double) * SLOTS * n, hipMemcpyDeviceToHost, st));
         RT_CHECK(hipStreamSynchronize(st));
         // ---- host state machines
-        std::vector<double> coefs(64 * (size_t)n, 0.0);
+        std::vector<double> coefs;
         std::vector<VecItem> adds;
-        for (int i : who) advance(i, pre, adds, coefs);
+        std::vector<int> add_who;
+        for (int i : who) advance(i, pre, adds, add_who, coefs);
         if (!adds.empty()) {
-            // z += sum y V, with the coefficients staged into d_coef first (the arena is free again
-            // after the synchronisation above)
+            // z += sum y V with the coefficients in the staging arena (free again after the
+            // synchronisation above)
             arena_off = 0;
             const double* dc = stage(coefs.data(), coefs.size());
-            RT_CHECK(hipMemcpyAsync(d_coef, dc, coefs.size() * sizeof(double), hipMemcpyDeviceToDevice, st));
+            for (size_t a = 0; a < adds.size(); ++a) adds[a].coef = dc + 64 * a;
             run_vecops(adds);
+            flush();
             RT_CHECK(hipStreamSynchronize(st));
         }
     }
@@ -307,7 +330,8 @@ void BatchSolver::run(rt_solve_info* info) {
 }
 
 // One step of problem i's state machine after its pass of this round.
-void BatchSolver::advance(int i, std::vector<VecItem>& pre, std::vector<VecItem>& adds, std::vector<double>& coefs) {
+void BatchSolver::advance(int i, std::vector<VecItem>& pre, std::vector<VecItem>& adds, std::vector<int>& add_who,
+                          std::vector<double>& coefs) {
     PS& s = ps[i];
     Problem* P = s.P;
     const double* slots = h_slots + (size_t)SLOTS * i;
@@ -382,11 +406,13 @@ void BatchSolver::advance(int i, std::vector<VecItem>& pre, std::vector<VecItem>
                 for (int l = a + 1; l < k; ++l) v -= h(a, l) * y[l];
                 y[a] = h(a, a) != 0.0 ? v / h(a, a) : 0.0;
             }
-            for (int a = 0; a < k; ++a) coefs[64 * (size_t)i + a] = y[a];
+            // coefficients: 64 per add item, staged by the caller (coef set there)
+            coefs.resize(64 * (adds.size() + 1), 0.0);
+            for (int a = 0; a < k; ++a) coefs[64 * adds.size() + a] = y[a];
             VecItem it{};
             it.op = VOP_ADDMULTI; it.n = S; it.y = s.z; it.V = (const double* const*)P->d_Vptr; it.k = k;
-            it.coef = d_coef + 64 * (size_t)i;
             adds.push_back(it);
+            add_who.push_back(i);
             s.phase = RESID;
             ++s.refs;
             break;

@@ -1,5 +1,6 @@
 // batch.hpp -- lockstep solver for many independent problems in one set of launches.
 #pragma once
+#include <functional>
 #include <vector>
 
 #include "engine.hpp"
@@ -34,18 +35,21 @@ private:
     GemmTask *d_mg = nullptr, *d_dg = nullptr;
     SweepTask* d_sw = nullptr;
     size_t cap_gemm = 0, cap_sweep = 0;
-    double *d_slots = nullptr, *h_slots = nullptr, *d_coef = nullptr, *d_partial = nullptr;
-    unsigned* d_counters = nullptr;
-    // staging arena: host (pinned) and device copies at equal offsets, reset once per round
+    double *d_slots = nullptr, *h_slots = nullptr, *d_partial = nullptr;
+    // Staging arena: host (pinned) and device copies at equal offsets.  A round stages its
+    // inputs and records its launches; flush() uploads the arena with one copy, then launches.
     unsigned char *h_arena = nullptr, *d_arena = nullptr;
     size_t arena_cap = 0, arena_off = 0;
+    std::vector<std::function<void()>> plan;
     template <class T> T* stage(const T* data, size_t count);
+    void flush();
 
     void launch_passes(const std::vector<int>& who, const std::vector<PassDesc>& pds, const std::vector<int>& prec,
                        const std::vector<int>& full);
     void run_vecops(const std::vector<VecItem>& items);
-    void run_dots(const std::vector<DotItem>& items);
-    void advance(int i, std::vector<VecItem>& pre, std::vector<VecItem>& adds, std::vector<double>& coefs);
+    void run_gs(const std::vector<GsItem>& items, bool update, bool norm);
+    void advance(int i, std::vector<VecItem>& pre, std::vector<VecItem>& adds, std::vector<int>& add_who,
+                 std::vector<double>& coefs);
     void start_arnoldi(PS& s);
 };
 

@@ -249,9 +249,7 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
         }
         // ---- reductions
         rw.partial = dalloc<double>(RED_BLOCKS);
-        rw.counter = dalloc<unsigned>(1);
         rw.scal = dalloc<double>(128);
-        RT_CHECK(hipMemsetAsync(rw.counter, 0, sizeof(unsigned), st));
         RT_CHECK(hipHostMalloc((void**)&h_scal, 128 * sizeof(double), hipHostMallocDefault));
         d_Vptr = dalloc<double*>(64);
         d_y = dalloc<double>(64);

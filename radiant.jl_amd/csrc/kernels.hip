@@ -79,14 +79,44 @@ __device__ inline short f2bf(float x) {
 // 16-element row segment.  A wave computes RT x CT tiles of 16x16 (RT*16 rows, CT*16 cells):
 // every A fragment is reused CT times and every B fragment RT times from registers.
 //
-// The k loop is software-pipelined over two register stages: the operands of step s+1 are in
-// flight while the MFMAs of step s run.  Every operand load is unconditional (padding rows,
+// The k loop is software-pipelined over NS register stages: the operands of steps s+1 .. s+NS-1
+// are in flight while the MFMAs of step s run.  Every operand load is unconditional (padding rows,
 // cells and k read clamped in-range addresses, absent inputs read a zero line), so the wait
 // counts stay exact and the loads of the next stage are never waited for early.
 // ------------------------------------------------------------------------------------------
 __device__ __align__(256) double g_zero[64];  // zero-initialised, never written
 
-template <typename T, int RT, int CT, bool MG>
+template <typename T, int RT, int CT, typename V>
+__device__ inline void epilogue_dgemm(const PassDesc& d, const V (&acc)[RT][CT], int c0, int r0, int li, int lane,
+                                      int M, int Nc) {
+    const bool hasz = d.zin != nullptr;
+    const double* __restrict__ zb = hasz ? d.zin : g_zero;
+    const size_t zrs = hasz ? (size_t)Nc : 0;
+    const double zs = hasz ? 1.0 : 0.0, sg = hasz ? -1.0 : 1.0;
+    double z[RT][CT][4];
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+        for (int u = 0; u < CT; ++u)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = c0 + 16 * u + li, row = r0 + 16 * t + acc_row<T>(lane, r);
+                const int cc = col < Nc ? col : Nc - 1, rc = row < M ? row : M - 1;
+                z[t][u][r] = gld(zb + (size_t)rc * zrs + (hasz ? cc : li));
+            }
+#pragma unroll
+    for (int t = 0; t < RT; ++t)
+#pragma unroll
+        for (int u = 0; u < CT; ++u)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int col = c0 + 16 * u + li, row = r0 + 16 * t + acc_row<T>(lane, r);
+                const double v = zs * z[t][u][r] + sg * (double)acc[t][u][r];
+                if (col < Nc && row < M) gst(d.out + (size_t)row * Nc + col, v);
+            }
+}
+
+template <typename T, int RT, int CT, bool MG, int NS>
 __global__ __launch_bounds__(256) void contract_kernel(const PassDesc* __restrict__ pds, const GemmTask* __restrict__ tasks) {
     __shared__ int rows_s[MG ? CONTRACT_KMAX : 1];
     const GemmTask tk = tasks[blockIdx.x];
@@ -180,38 +210,37 @@ __global__ __launch_bounds__(256) void contract_kernel(const PassDesc* __restric
 #pragma unroll
                 for (int u = 0; u < CT; ++u) acc[t][u] = mfma4(a[t], b[u], acc[t][u]);
     };
+    // NS stages: step s + NS - 1 is loaded into the stage that step s - 1 has just freed.
     const int nsteps = (K + 3) >> 2;
-    Stage g0, g1;
-    load(0, g0);
-    for (int s = 0; s < nsteps; s += 2) {
+    Stage g[NS];
+#pragma unroll
+    for (int i = 0; i < NS - 1; ++i) {
         __builtin_amdgcn_sched_barrier(0);
-        load(s + 1, g1);
-        __builtin_amdgcn_sched_barrier(0);
-        mma(s, g0);
-        __builtin_amdgcn_sched_barrier(0);
-        load(s + 2, g0);
-        __builtin_amdgcn_sched_barrier(0);
-        mma(s + 1, g1);  // all zero when s + 1 == nsteps
+        load(i, g[i]);
     }
+    for (int s = 0; s < nsteps; s += NS) {
 #pragma unroll
-    for (int t = 0; t < RT; ++t) {
-#pragma unroll
-        for (int u = 0; u < CT; ++u) {
-            const int col = c0 + 16 * u + li;
-            if (col >= Nc) continue;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = r0 + 16 * t + acc_row<T>(lane, r);
-                if (row >= M) continue;
-                if (MG) {
-                    gst((T*)d.psi + (size_t)row * Nc + col, (T)acc[t][u][r]);
-                } else {
-                    double v = (double)acc[t][u][r];
-                    if (d.zin) v = gld(d.zin + (size_t)row * Nc + col) - v;
-                    gst(d.out + (size_t)row * Nc + col, v);
-                }
-            }
+        for (int i = 0; i < NS; ++i) {
+            __builtin_amdgcn_sched_barrier(0);
+            load(s + i + NS - 1, g[(i + NS - 1) % NS]);
+            __builtin_amdgcn_sched_barrier(0);
+            if (s + i < nsteps) mma(s + i, g[i]);  // wave-uniform
         }
+    }
+    if (MG) {
+#pragma unroll
+        for (int t = 0; t < RT; ++t)
+#pragma unroll
+            for (int u = 0; u < CT; ++u)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int col = c0 + 16 * u + li, row = r0 + 16 * t + acc_row<T>(lane, r);
+                    if (col < Nc && row < M) gst((T*)d.psi + (size_t)row * Nc + col, (T)acc[t][u][r]);
+                }
+    } else {
+        // out = zin - D psi (I - A matvec) or D psi: every zin value is loaded before the first
+        // store, unconditionally, and combined as zs*z + sg*v (a select would branch the loads).
+        epilogue_dgemm<T, RT, CT>(d, acc, c0, r0, li, lane, M, Nc);
     }
 }
 
@@ -327,27 +356,20 @@ __global__ __launch_bounds__(256) void contract_bf16_kernel(const PassDesc* __re
         __builtin_amdgcn_sched_barrier(0);
         load(s + 2, g0);
         __builtin_amdgcn_sched_barrier(0);
-        mma(s + 1, g1);
+        if (s + 1 < nsteps) mma(s + 1, g1);  // wave-uniform
     }
+    if (MG) {
 #pragma unroll
-    for (int t = 0; t < RT; ++t) {
+        for (int t = 0; t < RT; ++t)
 #pragma unroll
-        for (int u = 0; u < CT; ++u) {
-            const int col = c0 + 16 * u + li;
-            if (col >= Nc) continue;
+            for (int u = 0; u < CT; ++u)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = r0 + 16 * t + 4 * (lane >> 4) + r;
-                if (row >= M) continue;
-                if (MG) {
-                    gst((float*)d.psi + (size_t)row * Nc + col, acc[t][u][r]);
-                } else {
-                    double v = (double)acc[t][u][r];
-                    if (d.zin) v = gld(d.zin + (size_t)row * Nc + col) - v;
-                    gst(d.out + (size_t)row * Nc + col, v);
+                for (int r = 0; r < 4; ++r) {
+                    const int col = c0 + 16 * u + li, row = r0 + 16 * t + acc_row<float>(lane, r);
+                    if (col < Nc && row < M) gst((float*)d.psi + (size_t)row * Nc + col, acc[t][u][r]);
                 }
-            }
-        }
+    } else {
+        epilogue_dgemm<float, RT, CT>(d, acc, c0, r0, li, lane, M, Nc);
     }
 }
 
@@ -1101,8 +1123,10 @@ __global__ __launch_bounds__(256) void outsrc_kernel(double* __restrict__ qlout,
 }
 
 // ------------------------------------------------------------------------------------------
-// BLAS-1 with deterministic two-level reductions: fixed grid, fixed tree per block, the last
-// block to finish (atomic ticket after a release fence) sums the block partials in order.
+// BLAS-1 with deterministic two-level reductions: a fixed grid writes one partial per block
+// (fixed tree inside the block), then a one-block kernel sums the partials in a fixed order.
+// A last-block ticket instead would need a device-scope release fence in every block, which on
+// this GPU writes back the XCD's L2 each time -- far costlier than the second launch.
 // ------------------------------------------------------------------------------------------
 __device__ inline double block_sum(double v) {
     __shared__ double sh[RED_THREADS];
@@ -1117,47 +1141,37 @@ __device__ inline double block_sum(double v) {
     return r;
 }
 
-__device__ inline void finish_reduction(double part, double* partial, unsigned* counter, double* out) {
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-        partial[blockIdx.x] = part;
-        __threadfence();
-        const unsigned t = atomicAdd(counter, 1u);
-        last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (last) {
-        __threadfence();
-        double v = 0.0;
-        for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) v += __builtin_nontemporal_load(&partial[i]);
-        v = block_sum(v);
-        if (threadIdx.x == 0) {
-            *out = v;
-            *counter = 0u;
-        }
-    }
+__global__ __launch_bounds__(RED_THREADS) void reduce_finish_kernel(const double* __restrict__ partial, int np,
+                                                                    double* out) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) v += partial[i];
+    v = block_sum(v);
+    if (threadIdx.x == 0) *out = v;
 }
 
 __global__ __launch_bounds__(RED_THREADS) void dot_kernel(const double* __restrict__ a, const double* __restrict__ b, size_t n,
-                                                          double* partial, unsigned* counter, double* out) {
+                                                          double* partial) {
     double v = 0.0;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) v += a[i] * b[i];
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        v += gld(a + i) * gld(b + i);
     v = block_sum(v);
-    finish_reduction(v, partial, counter, out);
+    if (threadIdx.x == 0) partial[blockIdx.x] = v;
 }
 
+// w -= h v, then the partial of w . vnext (NEXT) or w . w
+template <bool NEXT>
 __global__ __launch_bounds__(RED_THREADS) void mgs_kernel(double* __restrict__ w, const double* __restrict__ v,
                                                           const double* __restrict__ vnext, size_t n, const double* hptr,
-                                                          double* partial, unsigned* counter, double* out) {
+                                                          double* partial) {
     const double h = *hptr;
     double acc = 0.0;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const double x = w[i] - h * v[i];
-        w[i] = x;
-        acc += x * (vnext ? vnext[i] : x);
+        const double x = gld(w + i) - h * gld(v + i);
+        gst(w + i, x);
+        acc += x * (NEXT ? gld(vnext + i) : x);
     }
     acc = block_sum(acc);
-    finish_reduction(acc, partial, counter, out);
+    if (threadIdx.x == 0) partial[blockIdx.x] = acc;
 }
 
 __global__ void fill_kernel(double* x, double v, size_t n) {
@@ -1237,16 +1251,16 @@ static inline int grid_for(size_t n, int block = 256, int cap = 2048) {
 
 void launch_mgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st) {
     if (ntasks <= 0) return;
-    if (prec == P64) hipLaunchKernelGGL((contract_kernel<double, GEMM_RT, GEMM_CT, true>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
-    else if (prec == P32) hipLaunchKernelGGL((contract_kernel<float, GEMM_RT, GEMM_CT, true>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    if (prec == P64) hipLaunchKernelGGL((contract_kernel<double, GEMM_RT, GEMM_CT, true, 4>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    else if (prec == P32) hipLaunchKernelGGL((contract_kernel<float, GEMM_RT, GEMM_CT, true, 4>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
     else hipLaunchKernelGGL((contract_bf16_kernel<GEMM_RT, GEMM_CT, true>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
     RT_HIP(hipGetLastError());
 }
 
 void launch_dgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st) {
     if (ntasks <= 0) return;
-    if (prec == P64) hipLaunchKernelGGL((contract_kernel<double, GEMM_RT, GEMM_CT, false>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
-    else if (prec == P32) hipLaunchKernelGGL((contract_kernel<float, GEMM_RT, GEMM_CT, false>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    if (prec == P64) hipLaunchKernelGGL((contract_kernel<double, GEMM_RT, GEMM_CT, false, 4>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    else if (prec == P32) hipLaunchKernelGGL((contract_kernel<float, GEMM_RT, GEMM_CT, false, 4>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
     else hipLaunchKernelGGL((contract_bf16_kernel<GEMM_RT, GEMM_CT, false>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
     RT_HIP(hipGetLastError());
 }
@@ -1485,12 +1499,18 @@ void launch_outsrc(double* qlout, const double* phi_all, const double* sigo, con
 }
 
 void k_dot(const RedWork& rw, int slot, const double* a, const double* b, size_t n, hipStream_t st) {
-    hipLaunchKernelGGL(dot_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, a, b, n, rw.partial, rw.counter, rw.scal + slot);
+    hipLaunchKernelGGL(dot_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, a, b, n, rw.partial);
+    hipLaunchKernelGGL(reduce_finish_kernel, dim3(1), dim3(RED_THREADS), 0, st, rw.partial, RED_BLOCKS, rw.scal + slot);
     RT_HIP(hipGetLastError());
 }
 void k_mgs(const RedWork& rw, int hslot, int nslot, double* w, const double* v, const double* vnext, size_t n, hipStream_t st) {
-    hipLaunchKernelGGL(mgs_kernel, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, w, v, vnext, n, rw.scal + hslot, rw.partial,
-                       rw.counter, rw.scal + nslot);
+    if (vnext)
+        hipLaunchKernelGGL(mgs_kernel<true>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, w, v, vnext, n, rw.scal + hslot,
+                           rw.partial);
+    else
+        hipLaunchKernelGGL(mgs_kernel<false>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, w, v, vnext, n,
+                           rw.scal + hslot, rw.partial);
+    hipLaunchKernelGGL(reduce_finish_kernel, dim3(1), dim3(RED_THREADS), 0, st, rw.partial, RED_BLOCKS, rw.scal + nslot);
     RT_HIP(hipGetLastError());
 }
 void k_fill(double* x, double v, size_t n, hipStream_t st) {
@@ -1575,31 +1595,142 @@ __global__ void expand_sweep_kernel(const int* __restrict__ meta, int n, SweepTa
     for (int i = threadIdx.x; i < nd; i += blockDim.x) out[off + i] = SweepTask{prob, i};
 }
 
-__global__ __launch_bounds__(RED_THREADS) void segdot_kernel(const DotItem* __restrict__ items, double* partial,
-                                                             unsigned* counters) {
-    const DotItem it = items[blockIdx.y];
-    const int BPI = gridDim.x;
-    double v = 0.0;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < it.n; i += (long)BPI * blockDim.x) {
-        const double a = it.a[i];
-        v += a * (it.b ? it.b[i] : a);
+// UPD: y -= sum_i coef_i V_i first (in place); NRM: the output is y . y, else V_i . y.  A block
+// streams its contiguous chunk in batches of UE elements per thread, two batches in flight: the
+// loads of batch b+1 are issued before batch b is worked, and every load and store is
+// unconditional (padding lanes read a clamped element and store to the trash line), so the
+// waits never cover the next batch.  Slots i >= k alias y with coefficient 0 (x - 0*v == x).
+template <int KM, bool UPD, bool NRM>
+__global__ __launch_bounds__(256) void gs_kernel(const GsItem* __restrict__ items, double* partial) {
+    constexpr int UE = KM <= 8 ? 2 : 1;
+    const GsItem it = items[blockIdx.y];
+    if ((int)blockIdx.x >= it.bpi) return;
+    const int k = it.k;
+    const double* __restrict__ Vp[KM];
+    double c[KM], acc[KM];
+#pragma unroll
+    for (int i = 0; i < KM; ++i) {
+        Vp[i] = i < k ? it.V[i] : it.y;
+        c[i] = (UPD && i < k) ? it.coef[i] : 0.0;
+        acc[i] = 0.0;
     }
-    v = block_sum(v);
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-        partial[(size_t)blockIdx.y * BPI + blockIdx.x] = v;
-        __threadfence();
-        const unsigned t = atomicAdd(&counters[blockIdx.y], 1u);
-        last = (t == (unsigned)BPI - 1);
+    double* __restrict__ y = it.y;
+    const long chunk = (it.n + it.bpi - 1) / it.bpi;
+    const long lo = (long)blockIdx.x * chunk;
+    const long hi = lo + chunk < it.n ? lo + chunk : it.n;
+    const long nb = hi > lo ? (hi - lo + 256 * UE - 1) / (256 * UE) : 0;
+    const int tid = threadIdx.x;
+    struct Batch { double v[UE][KM]; double x[UE]; };
+    auto load = [&](long b, Batch& B) {
+#pragma unroll
+        for (int u = 0; u < UE; ++u) {
+            const long e = lo + (b * UE + u) * 256 + tid;
+            const long ec = e < hi ? e : hi - 1;
+#pragma unroll
+            for (int i = 0; i < KM; ++i) B.v[u][i] = gld(Vp[i] + ec);
+            B.x[u] = gld(y + ec);
+        }
+    };
+    auto work = [&](long b, const Batch& B) {
+#pragma unroll
+        for (int u = 0; u < UE; ++u) {
+            const long e = lo + (b * UE + u) * 256 + tid;
+            const double m = e < hi ? 1.0 : 0.0;
+            double x = B.x[u];
+            if (UPD) {
+#pragma unroll
+                for (int i = 0; i < KM; ++i) x -= c[i] * B.v[u][i];
+                gst(e < hi ? y + e : (double*)trash_slot<double>(tid & 63, u), x);
+            }
+            const double xm = x * m;
+            if (NRM) {
+                acc[0] += xm * x;
+            } else {
+#pragma unroll
+                for (int i = 0; i < KM; ++i) acc[i] += B.v[u][i] * xm;
+            }
+        }
+    };
+    if (nb > 0) {
+        Batch A, B;
+        load(0, A);
+        for (long b = 0; b < nb; b += 2) {
+            __builtin_amdgcn_sched_barrier(0);
+            load(b + 1, B);
+            __builtin_amdgcn_sched_barrier(0);
+            work(b, A);
+            __builtin_amdgcn_sched_barrier(0);
+            load(b + 2, A);
+            __builtin_amdgcn_sched_barrier(0);
+            work(b + 1, B);  // past the chunk when b + 1 == nb: masked, stores to the trash line
+        }
+    }
+    // block partials: butterfly sums within each wave, then the 4 waves in order
+    const int nout = NRM ? 1 : k;
+    __shared__ double red[4][KM];
+    const int lane = tid & 63, wv = tid >> 6;
+    auto block_reduce = [&](double (&val)[KM]) {
+#pragma unroll
+        for (int i = 0; i < KM; ++i) {
+            double s = val[i];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+            if (lane == 0) red[wv][i] = s;
+        }
+        __syncthreads();
+    };
+    block_reduce(acc);
+    double* part = partial + ((size_t)blockIdx.y * GS_BPI_MAX + blockIdx.x) * KM;
+    for (int i = tid; i < nout; i += blockDim.x) part[i] = ((red[0][i] + red[1][i]) + red[2][i]) + red[3][i];
+}
+
+// One block per item: thread b loads the partials of block b (all at once), then a fixed-order
+// block sum of each output.
+template <int KM, bool NRM>
+__global__ __launch_bounds__(256) void gs_finish_kernel(const GsItem* __restrict__ items, const double* __restrict__ partial) {
+    const GsItem it = items[blockIdx.x];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int nout = NRM ? 1 : it.k;
+    const int b = tid < it.bpi ? tid : it.bpi - 1;
+    const double keep = tid < it.bpi ? 1.0 : 0.0;
+    __shared__ double red[4][KM];
+#pragma unroll
+    for (int i = 0; i < KM; ++i) {
+        double s = partial[((size_t)blockIdx.x * GS_BPI_MAX + b) * KM + i] * keep;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+        if (lane == 0) red[wv][i] = s;
     }
     __syncthreads();
-    if (last && threadIdx.x == 0) {
-        __threadfence();
-        double s = 0.0;
-        for (int b = 0; b < BPI; ++b) s += __builtin_nontemporal_load(&partial[(size_t)blockIdx.y * BPI + b]);
-        *it.out = s;
-        counters[blockIdx.y] = 0u;
-    }
+    for (int i = tid; i < nout; i += blockDim.x) it.out[i] = ((red[0][i] + red[1][i]) + red[2][i]) + red[3][i];
+}
+
+template <int KM>
+static void gs_launch(dim3 grid, const GsItem* items, bool update, bool norm, double* partial, hipStream_t st) {
+    if (update && norm) hipLaunchKernelGGL((gs_kernel<KM, true, true>), grid, dim3(256), 0, st, items, partial);
+    else if (update) hipLaunchKernelGGL((gs_kernel<KM, true, false>), grid, dim3(256), 0, st, items, partial);
+    else if (norm) hipLaunchKernelGGL((gs_kernel<KM, false, true>), grid, dim3(256), 0, st, items, partial);
+    else hipLaunchKernelGGL((gs_kernel<KM, false, false>), grid, dim3(256), 0, st, items, partial);
+    if (norm) hipLaunchKernelGGL((gs_finish_kernel<KM, true>), dim3(grid.y), dim3(256), 0, st, items, partial);
+    else hipLaunchKernelGGL((gs_finish_kernel<KM, false>), dim3(grid.y), dim3(256), 0, st, items, partial);
+}
+
+int gs_blocks(long n) {
+    const long b = (n + 4095) / 4096;
+    return (int)(b < 8 ? 8 : (b > GS_BPI_MAX ? GS_BPI_MAX : b));
+}
+
+void launch_gs(const GsItem* items, int nitems, int kmax, int bpi_max, bool update, bool norm, double* partial,
+               hipStream_t st) {
+    if (nitems <= 0) return;
+    if (kmax > GS_KMAX || bpi_max > GS_BPI_MAX) throw std::runtime_error("launch_gs: item too large");
+    const dim3 grid(bpi_max, nitems);
+    if (kmax <= 2) gs_launch<2>(grid, items, update, norm, partial, st);
+    else if (kmax <= 4) gs_launch<4>(grid, items, update, norm, partial, st);
+    else if (kmax <= 8) gs_launch<8>(grid, items, update, norm, partial, st);
+    else if (kmax <= 16) gs_launch<16>(grid, items, update, norm, partial, st);
+    else gs_launch<GS_KMAX>(grid, items, update, norm, partial, st);
+    RT_HIP(hipGetLastError());
 }
 
 __global__ __launch_bounds__(256) void vecop_kernel(const VecItem* __restrict__ items) {
@@ -1642,11 +1773,6 @@ void launch_expand_gemm(const int* meta, int n, GemmTask* out, hipStream_t st) {
 void launch_expand_sweep(const int* meta, int n, SweepTask* out, hipStream_t st) {
     if (n <= 0) return;
     hipLaunchKernelGGL(expand_sweep_kernel, dim3(n), dim3(256), 0, st, meta, n, out);
-    RT_HIP(hipGetLastError());
-}
-void launch_segdot(const DotItem* items, int nitems, int bpi, double* partial, unsigned* counters, hipStream_t st) {
-    if (nitems <= 0) return;
-    hipLaunchKernelGGL(segdot_kernel, dim3(bpi, nitems), dim3(RED_THREADS), 0, st, items, partial, counters);
     RT_HIP(hipGetLastError());
 }
 void launch_vecops(const VecItem* items, int nitems, int blocks_per_item, hipStream_t st) {

@@ -84,9 +84,9 @@ void launch_outsrc(double* qlout, const double* phi_all, const double* sigo, con
                    const int* act, int g, int G, int Nmat, int Nact, int Np, int Nc, hipStream_t st);
 
 // Krylov BLAS-1 with deterministic reductions into device scalar slots.
-constexpr int RED_BLOCKS = 512;
+constexpr int RED_BLOCKS = 1024;
 constexpr int RED_THREADS = 256;
-struct RedWork { double* partial; unsigned* counter; double* scal; };
+struct RedWork { double* partial; double* scal; };   // partial: RED_BLOCKS doubles
 void k_dot(const RedWork& rw, int slot, const double* a, const double* b, size_t n, hipStream_t st);
 // w -= scal[hslot]*v ; then scal[nslot] = dot(w, vnext) (vnext null: dot(w, w))
 void k_mgs(const RedWork& rw, int hslot, int nslot, double* w, const double* v, const double* vnext, size_t n,
@@ -107,7 +107,6 @@ void k_cast_f64_f32(float* dst, const double* src, size_t n, hipStream_t st);
 void k_scatter_state(double* dst, const double* src, int rows, const int* rowmap, int Nc, hipStream_t st);
 
 // Batched helpers (one launch serves every problem of a batch).
-struct DotItem { const double* a; const double* b; long n; double* out; };   // *out = a.b (b null: a.a)
 enum VecOpCode { VOP_FILL = 0, VOP_COPY = 1, VOP_SCALE_COPY = 2, VOP_NORMALIZE = 3, VOP_SUB = 4, VOP_SUBMULTI = 5,
                  VOP_ADDMULTI = 6 };
 struct VecItem {
@@ -123,7 +122,22 @@ struct VecItem {
 };
 void launch_expand_gemm(const int* meta, int n, GemmTask* out, hipStream_t st);   // meta [prob, ntile, nrg, off]
 void launch_expand_sweep(const int* meta, int n, SweepTask* out, hipStream_t st); // meta [prob, ndir, off]
-void launch_segdot(const DotItem* items, int nitems, int bpi, double* partial, unsigned* counters, hipStream_t st);
+// Batched Gram-Schmidt step, one item per problem: optionally (update) y -= sum_i coef_i V_i in
+// place, then out_i = V_i . y for i < k, or out_0 = y . y (norm).  Every item is reduced over its
+// own bpi blocks in a fixed order, so its result does not depend on the other items.
+constexpr int GS_KMAX = 33;      // vectors per item at most
+constexpr int GS_BPI_MAX = 256;  // blocks per item at most
+struct GsItem {
+    const double* const* V;  // device table of k vectors
+    const double* coef;      // k coefficients (update only)
+    double* y;
+    double* out;
+    long n;
+    int k, bpi;
+};
+int gs_blocks(long n);   // the block count of an item of length n
+void launch_gs(const GsItem* items, int nitems, int kmax, int bpi_max, bool update, bool norm, double* partial,
+               hipStream_t st);
 void launch_vecops(const VecItem* items, int nitems, int blocks_per_item, hipStream_t st);
 
 // Layout converters of the host boundary (Radiant order <-> device order).
