@@ -3,16 +3,21 @@
 // Every round, each unfinished problem performs exactly one transport pass of its current
 // phase (c = T(0), a low-precision GMRES matvec, an FP64 refinement residual or the FP64
 // reconstruction); the passes of all problems share the same kernel launches (the contraction
-// and sweep task tables carry the problem index).  The Arnoldi step is classical Gram-Schmidt
-// with one re-orthogonalisation (CGS2) in FP64: three fused passes over the basis (h1 = V^T w;
-// w -= V h1 with h2 = V^T w; w -= V h2 with |w|^2), batched over problems with per-problem
-// deterministic reductions.  The host reads every problem's scalars with one copy per round and
+// and sweep task tables carry the problem index).  The Arnoldi step is classical Gram-Schmidt in
+// FP64: h = V^T w, then w -= V h with |w|^2 in the same pass over the basis, batched over
+// problems with per-problem deterministic reductions.  The inner GMRES only has to reach 1e-5
+// (3e-3 with BF16) and its result is corrected by FP64 residuals, so the loss of orthogonality
+// of single-pass CGS (of order cond * eps) is immaterial; RADIANT_HIP_CGS2=1 re-orthogonalises
+// once more (CGS2) for cross-checks.  The host reads every problem's scalars with one copy per round and
 // advances each problem's state machine (the logic of sn_flux.jl:172-205 + sn_one_speed.jl
 // with the iterative refinement of the mixed-precision mode).
 #include "batch.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <map>
@@ -68,9 +73,18 @@ BatchSolver::BatchSolver(const std::vector<Problem*>& probs, hipStream_t stream)
     cap_gemm = ng * n;
     cap_sweep = nsw * n;
     d_pd = (PassDesc*)al(sizeof(PassDesc) * n);
-    d_mg = (GemmTask*)al(sizeof(GemmTask) * cap_gemm);
-    d_dg = (GemmTask*)al(sizeof(GemmTask) * cap_gemm);
-    d_sw = (SweepTask*)al(sizeof(SweepTask) * cap_sweep);
+    for (int pr = 0; pr < 3; ++pr) {
+        d_mg[pr] = (GemmTask*)al(sizeof(GemmTask) * cap_gemm);
+        d_dg[pr] = (GemmTask*)al(sizeof(GemmTask) * cap_gemm);
+        d_sw[pr] = (SweepTask*)al(sizeof(SweepTask) * cap_sweep);
+    }
+    RT_CHECK(hipStreamCreateWithFlags(&st64, hipStreamNonBlocking));
+    {
+        const char* e = std::getenv("RADIANT_HIP_CGS2");  // re-orthogonalise every Arnoldi step
+        reorth = e && e[0] == '1';
+    }
+    RT_CHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    RT_CHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     d_slots = (double*)al(sizeof(double) * SLOTS * n);
     d_partial = (double*)al(sizeof(double) * GS_BPI_MAX * GS_KMAX * (size_t)n);
     RT_CHECK(hipHostMalloc((void**)&h_slots, sizeof(double) * SLOTS * n, hipHostMallocDefault));
@@ -101,6 +115,12 @@ void BatchSolver::flush() {
 
 BatchSolver::~BatchSolver() {
     (void)hipStreamSynchronize(st);
+    if (st64) {
+        (void)hipStreamSynchronize(st64);
+        (void)hipStreamDestroy(st64);
+    }
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
     for (void* p : mem) (void)hipFree(p);
     if (h_slots) (void)hipHostFree(h_slots);
     if (h_arena) (void)hipHostFree(h_arena);
@@ -139,6 +159,20 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
     std::vector<PassDesc> all(n);
     for (size_t i = 0; i < who.size(); ++i) all[who[i]] = pds[i];
     const PassDesc* dpd = stage(all.data(), all.size());
+    // The FP64 passes (refinement residuals, reconstructions, c = T(0)) of this round run on st64,
+    // concurrently with the low-precision Arnoldi passes on st; st waits for them before the
+    // Krylov work.  Every group has its own task tables.
+    bool has64 = false, haslo = false;
+    for (int p : prec) (p == P64 ? has64 : haslo) = true;
+    const bool fork = has64 && haslo;
+    if (fork) {
+        hipStream_t s0 = st, s1 = st64;
+        hipEvent_t ef = ev_fork;
+        plan.push_back([=] {
+            RT_CHECK(hipEventRecord(ef, s0));
+            RT_CHECK(hipStreamWaitEvent(s1, ef, 0));
+        });
+    }
     for (int pr : {P64, P32, PBF16}) {
         std::vector<int> mmeta, dmeta;
         std::map<std::pair<int, int>, std::vector<int>> sweeps;  // (cpl, waves) -> sweep meta
@@ -168,9 +202,9 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
         const int nm = (int)mmeta.size() / 4;
         const int* dmm = stage(mmeta.data(), mmeta.size());
         const int* ddm = stage(dmeta.data(), dmeta.size());
-        GemmTask *mg = d_mg, *dg = d_dg;
-        SweepTask* swbase = d_sw;
-        hipStream_t s = st;
+        GemmTask *mg = d_mg[pr], *dg = d_dg[pr];
+        SweepTask* swbase = d_sw[pr];
+        hipStream_t s = (fork && pr == P64) ? st64 : st;
         const int ps_ = pr;
         plan.push_back([=] {
             launch_expand_gemm(dmm, nm, mg, s);
@@ -192,6 +226,14 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
             swbase += ntask;
         }
         plan.push_back([=] { launch_dgemm(ps_, dpd, dg, doff, s); });
+    }
+    if (fork) {
+        hipStream_t s0 = st, s1 = st64;
+        hipEvent_t ej = ev_join;
+        plan.push_back([=] {
+            RT_CHECK(hipEventRecord(ej, s1));
+            RT_CHECK(hipStreamWaitEvent(s0, ej, 0));
+        });
     }
     for (size_t i = 0; i < who.size(); ++i) {
         Problem* P = ps[who[i]].P;
@@ -215,6 +257,10 @@ void BatchSolver::run(rt_solve_info* info) {
     RT_CHECK(hipEventRecord(e0, st));
     std::vector<VecItem> pre;  // vector ops to run before the next round's passes
     rounds = 0;
+    // host-side time per phase of a round (RADIANT_HIP_HOSTPROF=1 prints the totals)
+    using clk = std::chrono::steady_clock;
+    auto us = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::micro>(b - a).count(); };
+    double host_plan = 0, host_submit = 0, host_wait = 0, host_advance = 0;
     while (true) {
         // ---- group starts: out-of-group + volume source (sn_flux.jl:175-179)
         for (auto& s : ps) {
@@ -228,6 +274,7 @@ void BatchSolver::run(rt_solve_info* info) {
             s.p0 = P->counters.passes;
             s.l0 = P->counters.passes_lo;
         }
+        const auto t0 = clk::now();
         arena_off = 0;
         run_vecops(pre);
         pre.clear();
@@ -279,8 +326,12 @@ void BatchSolver::run(rt_solve_info* info) {
             if (s.phase == ARNOLDI) {
                 const int k = s.j + 1;
                 g1.push_back(gs(s.w, k, nullptr, slots + S_H1));        // h1 = V^T w
-                g2.push_back(gs(s.w, k, slots + S_H1, slots + S_H2));   // w -= V h1 ; h2 = V^T w
-                g3.push_back(gs(s.w, k, slots + S_H2, slots + S_WN));   // w -= V h2 ; |w|^2
+                if (reorth) {
+                    g2.push_back(gs(s.w, k, slots + S_H1, slots + S_H2));   // w -= V h1 ; h2 = V^T w
+                    g3.push_back(gs(s.w, k, slots + S_H2, slots + S_WN));   // w -= V h2 ; |w|^2
+                } else {
+                    g3.push_back(gs(s.w, k, slots + S_H1, slots + S_WN));   // w -= V h1 ; |w|^2
+                }
                 VecItem nm{};
                 nm.op = VOP_NORMALIZE; nm.n = S; nm.y = s.V[s.j + 1]; nm.x = s.w; nm.slot = slots + S_WN;
                 v4.push_back(nm);
@@ -292,14 +343,22 @@ void BatchSolver::run(rt_solve_info* info) {
         run_gs(g2, true, false);
         run_gs(g3, true, true);
         run_vecops(v4);
+        const auto t1 = clk::now();
         flush();
         RT_CHECK(hipMemcpyAsync(h_slots, d_slots, sizeof(double) * SLOTS * n, hipMemcpyDeviceToHost, st));
+        const auto t2 = clk::now();
         RT_CHECK(hipStreamSynchronize(st));
+        const auto t3 = clk::now();
         // ---- host state machines
         std::vector<double> coefs;
         std::vector<VecItem> adds;
         std::vector<int> add_who;
         for (int i : who) advance(i, pre, adds, add_who, coefs);
+        const auto t4 = clk::now();
+        host_plan += us(t0, t1);
+        host_submit += us(t1, t2);
+        host_wait += us(t2, t3);
+        host_advance += us(t3, t4);
         if (!adds.empty()) {
             // z += sum y V with the coefficients in the staging arena (free again after the
             // synchronisation above)
@@ -310,6 +369,12 @@ void BatchSolver::run(rt_solve_info* info) {
             flush();
             RT_CHECK(hipStreamSynchronize(st));
         }
+    }
+    {
+        const char* e = std::getenv("RADIANT_HIP_HOSTPROF");
+        if (e && e[0] == '1')
+            std::fprintf(stderr, "[batch host] rounds %d plan %.0f us submit %.0f us wait %.0f us advance %.0f us\n",
+                         rounds, host_plan, host_submit, host_wait, host_advance);
     }
     RT_CHECK(hipEventRecord(e1, st));
     RT_CHECK(hipEventSynchronize(e1));
@@ -377,7 +442,7 @@ void BatchSolver::advance(int i, std::vector<VecItem>& pre, std::vector<VecItem>
         case ARNOLDI: {
             const int j = s.j;
             ++s.niter;
-            for (int k = 0; k <= j; ++k) s.H[k + 33 * j] = slots[S_H1 + k] + slots[S_H2 + k];
+            for (int k = 0; k <= j; ++k) s.H[k + 33 * j] = slots[S_H1 + k] + (reorth ? slots[S_H2 + k] : 0.0);
             const double hjp1 = std::sqrt(std::max(0.0, slots[S_WN]));
             s.H[j + 1 + 33 * j] = hjp1;
             auto h = [&](int a, int b) -> double& { return s.H[a + 33 * b]; };

@@ -30,10 +30,14 @@ private:
     };
     std::vector<PS> ps;
     hipStream_t st;
+    bool reorth = false;  // CGS2 instead of CGS
     std::vector<void*> mem;
     PassDesc* d_pd = nullptr;
-    GemmTask *d_mg = nullptr, *d_dg = nullptr;
-    SweepTask* d_sw = nullptr;
+    // task tables per precision group (the FP64 group runs on st64, concurrently with the others)
+    GemmTask *d_mg[3] = {}, *d_dg[3] = {};
+    SweepTask* d_sw[3] = {};
+    hipStream_t st64 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     size_t cap_gemm = 0, cap_sweep = 0;
     double *d_slots = nullptr, *h_slots = nullptr, *d_partial = nullptr;
     // Staging arena: host (pinned) and device copies at equal offsets.  A round stages its
