@@ -101,11 +101,20 @@ def run_batch(probs, precision, steps, warmup, label):
     wall = time.perf_counter() - t0
     nupd = sum(p.Nc * d.Nd * p.G for p, d in zip(probs, rb.descs))
     phis = [rb.flux(0), rb.flux(1)]
+    # profiled solve: per-kernel HIP events on the streams the launches run on
+    os.environ["RADIANT_HIP_PROFILE"] = "1"
+    try:
+        _, pinfo = rb.solve()
+        kms, kn, kupd, kfl = rb.kernel_times()
+    finally:
+        os.environ["RADIANT_HIP_PROFILE"] = "0"
     out = dict(wall_s=wall, steps=steps, batch=len(probs), p50_ms=float(np.percentile(lat, 50) * 1e3),
                p99_ms=float(np.percentile(lat, 99) * 1e3), updates_per_s=nupd * steps / wall,
                solves_per_s=len(probs) * steps / wall, raw_updates_per_s=info.updates_raw * steps / wall,
                passes=int(info.passes), passes_lo=int(info.passes_lo),
-               converged=all(all(s["converged"] for s in st) for st in stats))
+               converged=all(all(s["converged"] for s in st) for st in stats),
+               kernel_ms=kms, kernel_launches=kn, kernel_updates=kupd, kernel_flops=kfl,
+               profiled_ms=pinfo.ms_device, Nd=rb.descs[0].Nd)
     rb.close()
     return out, phis
 
@@ -150,36 +159,54 @@ def run_serve(nreq, rate, budget_gib, max_active, precision):
                 precision=precision)
 
 
-def kernel_roofline(prob, res, nact):
-    """Achieved rate of the dominant kernel from its event-timed launches.
+def kernel_work(prob, res, nact):
+    """(updates, flops) processed by each [precision][kernel] class over the profiled solve."""
+    if "kernel_updates" in res:
+        return res["kernel_updates"], res["kernel_flops"]
+    kn = np.asarray(res["kernel_launches"], float)
+    upd = kn * prob.Nc * res["Nd"]
+    fl = np.zeros_like(upd)
+    fl[:, 0] = 2.0 * nact * upd[:, 0]
+    rows = res["Np"] if res.get("faithful") else nact
+    fl[:, 2] = 2.0 * rows * upd[:, 2]
+    return upd, fl
 
-    Algorithmic work per launch (one launch = one pass over Nd directions x Nc cells):
-      sweep : bytes = Nd*Nc*2*es + 4*Nc     (angular source in, angular flux out, material map)
-      dgemm : flops = 2*Nrows*Nd*Nc          (Nrows = Np for the full D, Nact for the reduced one)
-      mgemm : flops = 2*Nd*Nact*Nc
+
+def kernel_roofline(prob, res, nact):
+    """Achieved rate of the dominant kernel (most device time) from its event-timed launches.
+
+    Algorithmic work, per cell*direction update u of a launch:
+      sweep : 2*es bytes (angular source in, angular flux out; es = 8 FP64, 4 FP32) -> HBM roofline
+      mgemm : 2*K flops with K = active moments; bytes es + 8*K/Nd (angular source out, moments in)
+      dgemm : 2*R flops with R = output rows;    bytes es + 16*R/Nd (angular flux in, z in, Az out)
+    A contraction is reported against HBM when its arithmetic intensity is below the ridge point
+    of its MFMA precision, else against MFMA.
     """
-    kms, kn = res["kernel_ms"], res["kernel_launches"]
-    best = None
-    for p in range(3):
-        for k in range(3):
-            if kn[p][k] > 0 and (best is None or kms[p][k] > kms[best[0]][best[1]]):
-                best = (p, k)
-    p, k = best
-    avg_s = kms[p][k] / kn[p][k] / 1e3
-    Nc, Nd = prob.Nc, res["Nd"]
-    es = 8 if p == 0 else 4
+    kms, kn = np.asarray(res["kernel_ms"]), np.asarray(res["kernel_launches"])
+    upd, fl = kernel_work(prob, res, nact)
+    p, k = np.unravel_index(np.argmax(kms[:, :3]), kms[:, :3].shape)
+    t = kms[p][k] / 1e3
+    es = 8 if p == 0 else (4 if p == 1 else 2)
+    Nd = res["Nd"]
+    name = f"{KNAMES[k]}_{PNAMES[p]}"
+    base = dict(kernel=name, avg_launch_us=kms[p][k] / kn[p][k] * 1e3, launches=int(kn[p][k]),
+                share_of_solve=float(kms[p][k] / res["profiled_ms"]))
     if k == 1:
-        by = Nd * Nc * 2 * es + 4 * Nc
-        ach = by / avg_s / 1e9
-        return dict(kernel=f"sweep_{PNAMES[p]}", bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=ach / HBM_PEAK_GBS, traffic=None, avg_launch_us=avg_s * 1e6, launches=int(kn[p][k]),
-                    algorithmic_bytes_per_launch=by, share_of_solve=kms[p][k] / res["profiled_ms"])
-    rows = res["Np"] if (k == 2 and p == 0 and res.get("faithful")) else nact
-    fl = 2.0 * (rows if k == 2 else nact) * Nd * Nc
-    ach = fl / avg_s / 1e12
-    return dict(kernel=f"{KNAMES[k]}_{PNAMES[p]}", bound="mfma", achieved=ach, peak=MFMA_PEAK_TFLOPS[p],
-                unit="TFLOP/s", frac=ach / MFMA_PEAK_TFLOPS[p], traffic=None, avg_launch_us=avg_s * 1e6,
-                launches=int(kn[p][k]), algorithmic_flops_per_launch=fl, share_of_solve=kms[p][k] / res["profiled_ms"])
+        by = upd[p][k] * 2 * (8 if p == 0 else 4)
+        ach = by / t / 1e9
+        return dict(base, bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS,
+                    traffic=None, algorithmic_bytes_per_launch=by / kn[p][k])
+    K = fl[p][k] / (2.0 * upd[p][k])
+    by = upd[p][k] * (es + (8.0 * K / Nd if k == 0 else 16.0 * K / Nd))
+    ai = fl[p][k] / by
+    if ai < MFMA_PEAK_TFLOPS[p] * 1e12 / (HBM_PEAK_GBS * 1e9):
+        ach = by / t / 1e9
+        return dict(base, bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS,
+                    traffic=None, algorithmic_bytes_per_launch=by / kn[p][k], flop_per_byte=ai)
+    ach = fl[p][k] / t / 1e12
+    return dict(base, bound="mfma", achieved=ach, peak=MFMA_PEAK_TFLOPS[p], unit="TFLOP/s",
+                frac=ach / MFMA_PEAK_TFLOPS[p], traffic=None, algorithmic_flops_per_launch=fl[p][k] / kn[p][k],
+                flop_per_byte=ai)
 
 
 def kernel_table(res):
@@ -222,6 +249,8 @@ def main():
     ap.add_argument("--cpu-groups", type=int, default=2)
     ap.add_argument("--skip-bf16", action="store_true")
     ap.add_argument("--batch", type=int, default=16, help="problems per lockstep batch (1 = no batched mode)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="run only the batched headline mode (for kernel profiling of exactly that mode)")
     ap.add_argument("--serve", type=int, default=0,
                     help="instead of the solve benchmark, serve this many requests of the 3D request stream")
     ap.add_argument("--rate", type=float, default=50.0, help="served stream: Poisson arrivals per second")
@@ -250,35 +279,49 @@ def main():
     prob = problems.config1()
     steps, warm = args.steps, args.warmup
 
-    r64, phi64 = run_mode(prob, "fp64", "faithful", steps, warm, "fp64")
-    r64["faithful"] = True
-    rmx, phim = run_mode(prob, "mixed-fp32", "reduced", steps, warm, "mixed-fp32")
-    rmx["err_vs_fp64"] = rel_err(phim, phi64)
-    log(f"[mixed-fp32] rel flux-moment err vs fp64 = {rmx['err_vs_fp64']:.3e}")
-    rbf = None
-    if not args.skip_bf16:
-        rbf, phib = run_mode(prob, "mixed-bf16", "reduced", max(1, steps // 2), warm, "mixed-bf16")
-        rbf["err_vs_fp64"] = rel_err(phib, phi64)
-        log(f"[mixed-bf16] rel flux-moment err vs fp64 = {rbf['err_vs_fp64']:.3e}")
+    from radiant_hip.solver import compute_flux
+    if args.headline_only:
+        phi64 = None  # no FP64 reference solve: its kernels would mix into a kernel profile
+        r64 = rmx = rbf = None
+    else:
+        r64, phi64 = run_mode(prob, "fp64", "faithful", steps, warm, "fp64")
+        r64["faithful"] = True
+        rmx, phim = run_mode(prob, "mixed-fp32", "reduced", steps, warm, "mixed-fp32")
+        rmx["err_vs_fp64"] = rel_err(phim, phi64)
+        log(f"[mixed-fp32] rel flux-moment err vs fp64 = {rmx['err_vs_fp64']:.3e}")
+        rbf = None
+        if not args.skip_bf16:
+            rbf, phib = run_mode(prob, "mixed-bf16", "reduced", max(1, steps // 2), warm, "mixed-bf16")
+            rbf["err_vs_fp64"] = rel_err(phib, phi64)
+            log(f"[mixed-bf16] rel flux-moment err vs fp64 = {rbf['err_vs_fp64']:.3e}")
 
     rbt = None
     if args.batch > 1:
         bprobs = [prob] + [problems.config_2d(seed=20260102 + i) for i in range(1, args.batch)]
         rbt, bphis = run_batch(bprobs, "mixed-fp32", steps, warm, f"batch{args.batch}-fp32")
-        from radiant_hip.solver import compute_flux
-        ref1, _ = compute_flux(bprobs[1])
-        rbt["err_vs_fp64"] = max(rel_err(bphis[0], phi64), rel_err(bphis[1], ref1))
-        log(f"[batch] rel flux-moment err vs fp64 = {rbt['err_vs_fp64']:.3e}")
+        if phi64 is not None:
+            ref1, _ = compute_flux(bprobs[1])
+            rbt["err_vs_fp64"] = max(rel_err(bphis[0], phi64), rel_err(bphis[1], ref1))
+            log(f"[batch] rel flux-moment err vs fp64 = {rbt['err_vs_fp64']:.3e}")
+        else:
+            rbt["err_vs_fp64"] = None
 
     from radiant_hip.angular import angular_set
     ang = angular_set(prob.quad_type, prob.quad_order, 2, 3, prob.legendre_order, prob.sn_type)
     nact = int(np.sum(ang.pl <= prob.Lxs))
 
-    single = rmx if rmx["err_vs_fp64"] <= 1e-6 else r64
-    head = rbt if (rbt is not None and rbt["err_vs_fp64"] <= 1e-6 and rbt["converged"]) else single
-    head_name = f"mixed-fp32 batched x{args.batch}" if head is rbt else ("mixed-fp32" if single is rmx else "fp64")
-    roof = kernel_roofline(prob, single, nact)
-    roof64 = kernel_roofline(prob, r64, nact)
+    if args.headline_only:
+        if rbt is None:
+            raise SystemExit("--headline-only needs --batch > 1")
+        single, head, head_name = None, rbt, f"mixed-fp32 batched x{args.batch}"
+        roof, roof64 = kernel_roofline(prob, head, nact), None
+    else:
+        single = rmx if rmx["err_vs_fp64"] <= 1e-6 else r64
+        head = rbt if (rbt is not None and rbt["err_vs_fp64"] <= 1e-6 and rbt["converged"]) else single
+        head_name = (f"mixed-fp32 batched x{args.batch}" if head is rbt else
+                     ("mixed-fp32" if single is rmx else "fp64"))
+        roof = kernel_roofline(prob, head, nact)
+        roof64 = kernel_roofline(prob, r64, nact)
 
     cpu = None
     if not args.no_cpu:
@@ -295,7 +338,7 @@ def main():
         "metric": METRIC, "value": head["updates_per_s"], "unit": "cell·dir·group updates/s",
         "n_gpus": 1, "steps": steps, "warmup": warm, "ms_per_step": head["wall_s"] / steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32 sweep+M/D, fp64 Arnoldi/refinement" if head is not r64 else "f64",
+        "dtype": "f32 (sweep, M/D contractions) + f64 (Krylov, refinement)" if head is not r64 else "f64",
         "data": "synthetic (seeded splitmix64 cross-sections and geometry)",
         "config": {"workload": "2D 128x128 cells, 10x10 cm checkerboard of 2 materials (16x16-cell blocks), "
                                "20 groups, P5 downscatter, GLC-8 (128 directions), DD, GMRES(30) tol 1e-10, "
@@ -303,12 +346,13 @@ def main():
                                "(batched mode: one lockstep solve of `batch` such problems with distinct seeds)",
                    "mode": head_name, "batch": head.get("batch", 1)},
         "solves_per_s": head["solves_per_s"], "p50_ms": head["p50_ms"], "p99_ms": head["p99_ms"],
-        "rel_flux_err_vs_fp64": head.get("err_vs_fp64", 0.0),
-        "speedup_vs_fp64_plain": head["updates_per_s"] / r64["updates_per_s"],
-        "fp64_plain": summary(r64), "mixed_fp32": summary(rmx),
+        "rel_flux_err_vs_fp64": head.get("err_vs_fp64"),
+        "speedup_vs_fp64_plain": head["updates_per_s"] / r64["updates_per_s"] if r64 else None,
+        "fp64_plain": summary(r64) if r64 else None, "mixed_fp32": summary(rmx) if rmx else None,
         "mixed_fp32_batched": summary(rbt) if rbt else None,
         "mixed_bf16": summary(rbf) if rbf else None,
-        "kernels_single_mixed": kernel_table(single), "kernels_fp64": kernel_table(r64),
+        "kernels_headline": kernel_table(head), "kernels_single_mixed": kernel_table(single) if single else None,
+        "kernels_fp64": kernel_table(r64) if r64 else None,
         "roofline": roof, "roofline_fp64": roof64,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (head["updates_per_s"] / cpu["value"]) if cpu else None,

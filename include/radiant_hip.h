@@ -156,6 +156,11 @@ int rt_batch_create(int32_t n, const rt_problem_desc *const *p, rt_batch **out);
 int rt_batch_solve(rt_batch *b, rt_group_stats *const *stats_out, rt_solve_info *info);
 int rt_batch_get_flux(rt_batch *b, int32_t i, double *phi_out);
 int rt_batch_destroy(rt_batch *b);
+/* Profiling hook (no counterpart in the reference): per-kernel device time of the last
+ * rt_batch_solve run with RADIANT_HIP_PROFILE=1, lockstep batches only.  Arrays of 12 indexed
+ * [precision][kernel] as rt_problem_kernel_times; updates[] holds the cell*direction updates the
+ * launches processed and flops[] the contraction flops. */
+int rt_batch_kernel_times(rt_batch *b, double *ms, int64_t *launches, double *updates, double *flops);
 
 /* ------------------------------------------------------------------------------------------
  * Serving: continuous batching of fixed-source solves admitted by an HBM budget.

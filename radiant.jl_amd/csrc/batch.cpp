@@ -113,8 +113,40 @@ void BatchSolver::flush() {
     plan.clear();
 }
 
+std::function<void()> BatchSolver::timed(int kernel, int prec, double upd, double flop, hipStream_t s,
+                                         std::function<void()> fn) {
+    if (!profiling) return fn;
+    while (ev_pool.size() < ev_next + 2) {
+        hipEvent_t e;
+        RT_CHECK(hipEventCreate(&e));
+        ev_pool.push_back(e);
+    }
+    hipEvent_t a = ev_pool[ev_next], b = ev_pool[ev_next + 1];
+    ev_next += 2;
+    ev_used.push_back({a, b, kernel, prec, upd, flop});
+    return [=] {
+        RT_CHECK(hipEventRecord(a, s));
+        fn();
+        RT_CHECK(hipEventRecord(b, s));
+    };
+}
+
+void BatchSolver::collect_times() {
+    for (auto& r : ev_used) {
+        float ms = 0.f;
+        RT_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
+        timing.ms[r.prec][r.kernel] += ms;
+        timing.n[r.prec][r.kernel] += 1;
+        timing.upd[r.prec][r.kernel] += r.upd;
+        timing.flop[r.prec][r.kernel] += r.flop;
+    }
+    ev_used.clear();
+    ev_next = 0;
+}
+
 BatchSolver::~BatchSolver() {
     (void)hipStreamSynchronize(st);
+    for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     if (st64) {
         (void)hipStreamSynchronize(st64);
         (void)hipStreamDestroy(st64);
@@ -179,6 +211,8 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
         int moff = 0, doff = 0;
         std::map<std::pair<int, int>, int> soff;
         std::map<std::pair<int, int>, size_t> slds;
+        std::map<std::pair<int, int>, double> s_upd;
+        double g_upd = 0, g_mflop = 0, g_dflop = 0;
         for (size_t i = 0; i < who.size(); ++i) {
             if (prec[i] != pr) continue;
             const Problem* P = ps[who[i]].P;
@@ -194,6 +228,11 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
             auto& v = sweeps[key];
             v.insert(v.end(), {who[i], P->Nd, soff[key]});
             soff[key] += P->Nd;
+            const double upd = (double)P->Nc * P->Nd;
+            g_upd += upd;
+            g_mflop += 2.0 * P->Nact * upd;
+            g_dflop += 2.0 * rows * upd;
+            s_upd[key] += upd;
             const int sp = pr == PBF16 ? P32 : pr;
             slds[key] = std::max(slds[key], sweep_scan_lds_bytes(sp, P->ndims, P->Nx, P->Ny, P->Nz, P->Nmat));
         }
@@ -209,8 +248,8 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
         plan.push_back([=] {
             launch_expand_gemm(dmm, nm, mg, s);
             launch_expand_gemm(ddm, nm, dg, s);
-            launch_mgemm(ps_, dpd, mg, moff, s);
         });
+        plan.push_back(timed(0, pr, g_upd, g_mflop, s, [=] { launch_mgemm(ps_, dpd, mg, moff, s); }));
         for (auto& kv : sweeps) {
             const std::vector<int>& sm = kv.second;
             const int* dsm = stage(sm.data(), sm.size());
@@ -219,13 +258,13 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
             const int cpl = kv.first.first, waves = kv.first.second;
             const size_t lds = slds[kv.first];
             SweepTask* sb = swbase;
-            plan.push_back([=] {
-                launch_expand_sweep(dsm, nsm, sb, s);
+            plan.push_back([=] { launch_expand_sweep(dsm, nsm, sb, s); });
+            plan.push_back(timed(1, pr, s_upd[kv.first], 0.0, s, [=] {
                 launch_sweep_scan(ps_ == PBF16 ? P32 : ps_, dpd, sb, ntask, cpl, waves, lds, s);
-            });
+            }));
             swbase += ntask;
         }
-        plan.push_back([=] { launch_dgemm(ps_, dpd, dg, doff, s); });
+        plan.push_back(timed(2, pr, g_upd, g_dflop, s, [=] { launch_dgemm(ps_, dpd, dg, doff, s); }));
     }
     if (fork) {
         hipStream_t s0 = st, s1 = st64;
@@ -254,6 +293,11 @@ void BatchSolver::run(rt_solve_info* info) {
         s.phase = START;
         s.g = 0;
     }
+    {
+        const char* e = std::getenv("RADIANT_HIP_PROFILE");
+        profiling = e && e[0] == '1';
+    }
+    timing = Timing{};
     RT_CHECK(hipEventRecord(e0, st));
     std::vector<VecItem> pre;  // vector ops to run before the next round's passes
     rounds = 0;
@@ -298,6 +342,8 @@ void BatchSolver::run(rt_solve_info* info) {
         }
         if (who.empty()) {
             flush();  // vector operations recorded for problems that have just finished
+            RT_CHECK(hipStreamSynchronize(st));
+            collect_times();
             break;
         }
         ++rounds;
@@ -349,6 +395,7 @@ void BatchSolver::run(rt_solve_info* info) {
         const auto t2 = clk::now();
         RT_CHECK(hipStreamSynchronize(st));
         const auto t3 = clk::now();
+        collect_times();
         // ---- host state machines
         std::vector<double> coefs;
         std::vector<VecItem> adds;

@@ -14,6 +14,14 @@ public:
     void run(rt_solve_info* info);
     const std::vector<rt_group_stats>& stats(int i) const { return ps[i].stats; }
     int rounds = 0;
+    // Per-kernel device time of the last run() when RADIANT_HIP_PROFILE=1 (HIP events on the stream
+    // each launch runs on), [precision][mgemm, sweep, dgemm, -], with the work the launches did:
+    // cell*direction updates and contraction flops.
+    struct Timing {
+        double ms[3][4] = {}, upd[3][4] = {}, flop[3][4] = {};
+        int64_t n[3][4] = {};
+    };
+    Timing timing;
 
 private:
     enum Phase { START, CPASS, ARNOLDI, RESID, RECON, DONE };
@@ -55,6 +63,15 @@ private:
     void advance(int i, std::vector<VecItem>& pre, std::vector<VecItem>& adds, std::vector<int>& add_who,
                  std::vector<double>& coefs);
     void start_arnoldi(PS& s);
+
+    bool profiling = false;
+    struct EvRec { hipEvent_t a, b; int kernel, prec; double upd, flop; };
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_next = 0;
+    std::vector<EvRec> ev_used;
+    // wraps a launch closure with start/stop events when profiling
+    std::function<void()> timed(int kernel, int prec, double upd, double flop, hipStream_t s, std::function<void()> fn);
+    void collect_times();  // after the end-of-round synchronisation
 };
 
 }  // namespace rt

@@ -201,6 +201,22 @@ int rt_batch_get_flux(rt_batch* b, int32_t i, double* phi_out) {
     return guard([&] { b->probs[i]->get_flux(phi_out); });
 }
 
+int rt_batch_kernel_times(rt_batch* b, double* ms, int64_t* launches, double* updates, double* flops) {
+    if (!b || !b->bs) {
+        rt::set_last_error("kernel times are recorded by lockstep (mixed-precision) batches only");
+        return RT_EINVAL;
+    }
+    const auto& t = b->bs->timing;
+    for (int p = 0; p < 3; ++p)
+        for (int k = 0; k < 4; ++k) {
+            if (ms) ms[4 * p + k] = t.ms[p][k];
+            if (launches) launches[4 * p + k] = t.n[p][k];
+            if (updates) updates[4 * p + k] = t.upd[p][k];
+            if (flops) flops[4 * p + k] = t.flop[p][k];
+        }
+    return RT_OK;
+}
+
 int rt_batch_destroy(rt_batch* b) {
     return guard([&] { delete b; });
 }

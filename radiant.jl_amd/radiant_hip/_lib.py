@@ -62,7 +62,7 @@ EXPORTS = ["rt_solve", "rt_problem_create", "rt_problem_solve", "rt_problem_get_
            "rt_problem_device_bytes", "rt_problem_pass", "rt_problem_state_length", "rt_solve_batch",
            "rt_server_create", "rt_server_submit", "rt_server_poll", "rt_server_cancel", "rt_server_destroy",
            "rt_last_error", "rt_version", "rt_test_contraction", "rt_problem_kernel_times",
-           "rt_batch_create", "rt_batch_solve", "rt_batch_get_flux", "rt_batch_destroy"]
+           "rt_batch_create", "rt_batch_solve", "rt_batch_get_flux", "rt_batch_destroy", "rt_batch_kernel_times"]
 
 _lib = None
 
@@ -95,6 +95,7 @@ def lib():
     L.rt_batch_solve.argtypes = [H, C.POINTER(GS), C.POINTER(SolveInfo)]
     L.rt_batch_get_flux.argtypes = [H, C.c_int32, dp]
     L.rt_batch_destroy.argtypes = [H]
+    L.rt_batch_kernel_times.argtypes = [H, dp, lp, dp, dp]
     L.rt_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(H)]
     L.rt_server_submit.argtypes = [H, PD, lp]
     L.rt_server_poll.argtypes = [H, C.c_int64, dp, GS, dp]
