@@ -248,7 +248,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-groups", type=int, default=2)
     ap.add_argument("--skip-bf16", action="store_true")
-    ap.add_argument("--batch", type=int, default=16, help="problems per lockstep batch (1 = no batched mode)")
+    ap.add_argument("--batch", type=int, default=32, help="problems per lockstep batch (1 = no batched mode)")
     ap.add_argument("--headline-only", action="store_true",
                     help="run only the batched headline mode (for kernel profiling of exactly that mode)")
     ap.add_argument("--serve", type=int, default=0,

@@ -1251,8 +1251,8 @@ static inline int grid_for(size_t n, int block = 256, int cap = 2048) {
 
 void launch_mgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st) {
     if (ntasks <= 0) return;
-    if (prec == P64) hipLaunchKernelGGL((contract_kernel<double, GEMM_RT, GEMM_CT, true, 4>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
-    else if (prec == P32) hipLaunchKernelGGL((contract_kernel<float, GEMM_RT, GEMM_CT, true, 4>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    if (prec == P64) hipLaunchKernelGGL((contract_kernel<double, GEMM_RT, GEMM_CT, true, 2>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    else if (prec == P32) hipLaunchKernelGGL((contract_kernel<float, GEMM_RT, GEMM_CT, true, 2>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
     else hipLaunchKernelGGL((contract_bf16_kernel<GEMM_RT, GEMM_CT, true>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
     RT_HIP(hipGetLastError());
 }

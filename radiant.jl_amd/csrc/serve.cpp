@@ -6,9 +6,11 @@
 // slot is free, so a request that arrives while others run starts as soon as memory allows.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -65,19 +67,30 @@ struct DescCopy {
     }
 };
 
-// A resident batch: the problems, their shared stream and (for the mixed-precision modes) the
-// lockstep solver; otherwise each problem keeps its own stream and a host thread drives it.
+// A resident batch: the problems and (for the mixed-precision modes) lockstep solvers.  The
+// problems are dealt round-robin to a few sub-batches, each a BatchSolver with its own streams
+// driven by its own host thread, so that one sub-batch's transport passes overlap another's
+// Arnoldi work and host turnaround.  Without the lockstep solver each problem keeps its own
+// stream and a host thread drives it.
 struct rt_batch {
-    hipStream_t st = nullptr;
     std::vector<std::unique_ptr<rt::Problem>> probs;
-    std::unique_ptr<rt::BatchSolver> bs;
+    std::vector<hipStream_t> sts;                      // one per sub-batch
+    std::vector<std::unique_ptr<rt::BatchSolver>> bss;  // sub-batch solvers
+    std::vector<std::pair<int, int>> where;             // problem -> (sub-batch, index in it)
     std::vector<std::vector<rt_group_stats>> stats;
     ~rt_batch() {
-        bs.reset();
+        bss.clear();
         probs.clear();
-        if (st) (void)hipStreamDestroy(st);
+        for (hipStream_t s : sts) (void)hipStreamDestroy(s);
     }
 };
+
+static int sub_batches(int n) {
+    const char* e = std::getenv("RADIANT_HIP_SUBBATCH");
+    int k = e ? std::atoi(e) : 4;
+    k = std::max(1, std::min(k, 4));
+    return n >= 2 * k ? k : 1;
+}
 
 static rt_batch* make_batch(int32_t n, const rt_problem_desc* const* p) {
     std::unique_ptr<rt_batch> b(new rt_batch());
@@ -85,25 +98,30 @@ static rt_batch* make_batch(int32_t n, const rt_problem_desc* const* p) {
     for (int i = 0; i < n; ++i)
         if (p[i]->precision == RT_PREC_FP64) lockstep = false;
     if (lockstep) {
-        RT_CHECK(hipStreamCreateWithFlags(&b->st, hipStreamNonBlocking));
-        std::vector<rt::Problem*> raw;
+        const int nsub = sub_batches(n);
+        b->sts.resize(nsub, nullptr);
+        for (int s = 0; s < nsub; ++s) RT_CHECK(hipStreamCreateWithFlags(&b->sts[s], hipStreamNonBlocking));
+        std::vector<std::vector<rt::Problem*>> raw(nsub);
         for (int i = 0; i < n; ++i) {
-            b->probs.emplace_back(new rt::Problem(*p[i], b->st));
-            raw.push_back(b->probs.back().get());
+            const int s = i % nsub;
+            b->probs.emplace_back(new rt::Problem(*p[i], b->sts[s]));
+            b->where.push_back({s, (int)raw[s].size()});
+            raw[s].push_back(b->probs.back().get());
         }
         try {
-            b->bs.reset(new rt::BatchSolver(raw, b->st));
+            for (int s = 0; s < nsub; ++s) b->bss.emplace_back(new rt::BatchSolver(raw[s], b->sts[s]));
         } catch (const rt::Error& e) {
             if (e.code != RT_EINVAL) throw;
-            b->bs.reset();  // a problem the row-scan sweep cannot take: per-problem streams
+            b->bss.clear();  // a problem the row-scan sweep cannot take: per-problem streams
         }
-        if (!b->bs) {
+        if (b->bss.empty()) {
             b->probs.clear();
-            (void)hipStreamDestroy(b->st);
-            b->st = nullptr;
+            b->where.clear();
+            for (hipStream_t s : b->sts) (void)hipStreamDestroy(s);
+            b->sts.clear();
         }
     }
-    if (!b->bs)
+    if (b->bss.empty())
         for (int i = 0; i < n; ++i) b->probs.emplace_back(new rt::Problem(*p[i]));
     b->stats.resize(n);
     for (int i = 0; i < n; ++i) b->stats[i].assign(b->probs[i]->G, rt_group_stats{});
@@ -112,9 +130,35 @@ static rt_batch* make_batch(int32_t n, const rt_problem_desc* const* p) {
 
 static void solve_batch(rt_batch* b, rt_solve_info* info) {
     const int n = (int)b->probs.size();
-    if (b->bs) {
-        b->bs->run(info);
-        for (int i = 0; i < n; ++i) b->stats[i] = b->bs->stats(i);
+    if (!b->bss.empty()) {
+        const int nsub = (int)b->bss.size();
+        std::vector<rt_solve_info> infos(nsub);
+        std::vector<std::string> errs(nsub);
+        const auto t0 = std::chrono::steady_clock::now();
+        std::vector<std::thread> th;
+        for (int s = 0; s < nsub; ++s)
+            th.emplace_back([&, s] {
+                try {
+                    b->bss[s]->run(&infos[s]);
+                } catch (const std::exception& e) {
+                    errs[s] = e.what();
+                }
+            });
+        for (auto& t : th) t.join();
+        const double wall = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        for (int s = 0; s < nsub; ++s)
+            if (!errs[s].empty()) throw rt::Error(RT_EDEVICE, "sub-batch " + std::to_string(s) + ": " + errs[s]);
+        for (int i = 0; i < n; ++i) b->stats[i] = b->bss[b->where[i].first]->stats(b->where[i].second);
+        if (info) {
+            std::memset(info, 0, sizeof(*info));
+            info->ms_device = wall;
+            for (auto& x : infos) {
+                info->passes += x.passes;
+                info->passes_lo += x.passes_lo;
+                info->updates_raw += x.updates_raw;
+                info->updates_problem += x.updates_problem;
+            }
+        }
         return;
     }
     hipEvent_t e0, e1;
@@ -202,17 +246,24 @@ int rt_batch_get_flux(rt_batch* b, int32_t i, double* phi_out) {
 }
 
 int rt_batch_kernel_times(rt_batch* b, double* ms, int64_t* launches, double* updates, double* flops) {
-    if (!b || !b->bs) {
+    if (!b || b->bss.empty()) {
         rt::set_last_error("kernel times are recorded by lockstep (mixed-precision) batches only");
         return RT_EINVAL;
     }
-    const auto& t = b->bs->timing;
     for (int p = 0; p < 3; ++p)
         for (int k = 0; k < 4; ++k) {
-            if (ms) ms[4 * p + k] = t.ms[p][k];
-            if (launches) launches[4 * p + k] = t.n[p][k];
-            if (updates) updates[4 * p + k] = t.upd[p][k];
-            if (flops) flops[4 * p + k] = t.flop[p][k];
+            double t_ms = 0, t_upd = 0, t_fl = 0;
+            int64_t t_n = 0;
+            for (auto& bs : b->bss) {
+                t_ms += bs->timing.ms[p][k];
+                t_n += bs->timing.n[p][k];
+                t_upd += bs->timing.upd[p][k];
+                t_fl += bs->timing.flop[p][k];
+            }
+            if (ms) ms[4 * p + k] = t_ms;
+            if (launches) launches[4 * p + k] = t_n;
+            if (updates) updates[4 * p + k] = t_upd;
+            if (flops) flops[4 * p + k] = t_fl;
         }
     return RT_OK;
 }
