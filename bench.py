@@ -82,7 +82,13 @@ def run_mode(prob, precision, state, steps, warmup, label):
 
 
 def run_batch(probs, precision, steps, warmup, label):
-    """One step = one lockstep solve of the whole batch (inputs resident in HBM)."""
+    """One step = one lockstep solve of the whole batch (inputs resident in HBM).
+
+    The library splits a batch into concurrent sub-batches.  Kernel durations are measured on one
+    extra solve of the same problems run as a single sub-batch (RADIANT_HIP_SUBBATCH=1), where each
+    kernel runs alone: under concurrency a kernel's interval also holds its wait for compute units
+    taken by the other sub-batches.  HIP events attached to each dispatch (hipExtLaunchKernel,
+    RADIANT_HIP_PROFILE=1) give the per-kernel times."""
     from radiant_hip.solver import ResidentBatch
     t0 = time.perf_counter()
     rb = ResidentBatch(probs, precision=precision)
@@ -101,62 +107,32 @@ def run_batch(probs, precision, steps, warmup, label):
     wall = time.perf_counter() - t0
     nupd = sum(p.Nc * d.Nd * p.G for p, d in zip(probs, rb.descs))
     phis = [rb.flux(0), rb.flux(1)]
-    # profiled solve: per-kernel HIP events on the streams the launches run on
-    os.environ["RADIANT_HIP_PROFILE"] = "1"
+    rb.close()
+    # kernel timing: the same problems as one sub-batch, one solve
+    saved = {k: os.environ.get(k) for k in ("RADIANT_HIP_SUBBATCH", "RADIANT_HIP_PROFILE")}
+    os.environ["RADIANT_HIP_SUBBATCH"] = "1"
     try:
-        _, pinfo = rb.solve()
-        kms, kn, kupd, kfl = rb.kernel_times()
+        rs = ResidentBatch(probs, precision=precision)
+        os.environ["RADIANT_HIP_PROFILE"] = "1"
+        _, pinfo = rs.solve()
+        kms, kn, kupd, kfl = rs.kernel_times()
+        rs.close()
     finally:
-        os.environ["RADIANT_HIP_PROFILE"] = "0"
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    log(f"[{label}] kernel-timing solve (one sub-batch): {pinfo.ms_device:.1f} ms")
     out = dict(wall_s=wall, steps=steps, batch=len(probs), p50_ms=float(np.percentile(lat, 50) * 1e3),
                p99_ms=float(np.percentile(lat, 99) * 1e3), updates_per_s=nupd * steps / wall,
                solves_per_s=len(probs) * steps / wall, raw_updates_per_s=info.updates_raw * steps / wall,
                passes=int(info.passes), passes_lo=int(info.passes_lo),
                converged=all(all(s["converged"] for s in st) for st in stats),
                kernel_ms=kms, kernel_launches=kn, kernel_updates=kupd, kernel_flops=kfl,
-               profiled_ms=pinfo.ms_device, Nd=rb.descs[0].Nd)
-    rb.close()
+               profiled_ms=pinfo.ms_device, Nd=rb.descs[0].Nd,
+               kernel_timing="one extra solve of the same batch as a single sub-batch (kernels run alone)")
     return out, phis
-
-
-def run_serve(nreq, rate, budget_gib, max_active, precision):
-    """Served stream: the first `nreq` requests of the 3D Carlson stream (N~U{32..96} per axis,
-    G~U{10..40}, Carlson 8/10/12, P3, 3 random materials) submitted at Poisson arrival times
-    (rate per second) to rt_server; latency = submit -> result ready on the host."""
-    from radiant_hip import problems, solver
-    t_arr = problems.poisson_arrivals(nreq, rate=rate)
-    srv = solver.Server(hbm_budget_bytes=int(budget_gib * 1024 ** 3), max_active=max_active, precision=precision)
-    ids, nupd, lat, done = [], {}, {}, 0
-    t0 = time.perf_counter()
-    try:
-        for i in range(nreq):
-            p = problems.config4_request(i)
-            wait = t_arr[i] - (time.perf_counter() - t0)
-            if wait > 0:
-                time.sleep(wait)
-            rid = srv.submit(p)
-            ids.append(rid)
-            nupd[rid] = p.Nc * srv.descs[rid][1] * p.G
-            log(f"[serve] submit {i} (N={p.Nx}, G={p.G}, Nd={srv.descs[rid][1]}) at {time.perf_counter() - t0:.2f}s")
-            del p
-        while len(lat) < len(ids):
-            for rid in ids:
-                if rid in lat:
-                    continue
-                r = srv.poll(rid)
-                if r is not None:
-                    lat[rid] = r[2]
-                    log(f"[serve] request {rid} done: {r[2]:.0f} ms")
-                    del r
-            time.sleep(0.002)
-        wall = time.perf_counter() - t0
-    finally:
-        srv.close()
-    ls = np.array([lat[r] for r in ids])
-    return dict(requests=nreq, rate_per_s=rate, wall_s=wall, requests_per_s=nreq / wall,
-                updates_per_s=sum(nupd.values()) / wall, p50_ms=float(np.percentile(ls, 50)),
-                p99_ms=float(np.percentile(ls, 99)), budget_gib=budget_gib, max_active=max_active,
-                precision=precision)
 
 
 def kernel_work(prob, res, nact):
@@ -190,7 +166,8 @@ def kernel_roofline(prob, res, nact):
     Nd = res["Nd"]
     name = f"{KNAMES[k]}_{PNAMES[p]}"
     base = dict(kernel=name, avg_launch_us=kms[p][k] / kn[p][k] * 1e3, launches=int(kn[p][k]),
-                share_of_solve=float(kms[p][k] / res["profiled_ms"]))
+                share_of_solve=float(kms[p][k] / res["profiled_ms"]),
+                measured_on=res.get("kernel_timing", "one extra solve, event-timed"))
     if k == 1:
         by = upd[p][k] * 2 * (8 if p == 0 else 4)
         ach = by / t / 1e9

@@ -82,6 +82,10 @@ BatchSolver::BatchSolver(const std::vector<Problem*>& probs, hipStream_t stream)
     {
         const char* e = std::getenv("RADIANT_HIP_CGS2");  // re-orthogonalise every Arnoldi step
         reorth = e && e[0] == '1';
+        // 1: FP64 passes on a second stream beside the low-precision ones.  Off by default: the
+        // sub-batches of a batch already overlap on the device, and measured faster without it.
+        const char* f = std::getenv("RADIANT_HIP_FORK");
+        fork_streams = f && f[0] == '1';
     }
     RT_CHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
     RT_CHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
@@ -124,10 +128,11 @@ std::function<void()> BatchSolver::timed(int kernel, int prec, double upd, doubl
     hipEvent_t a = ev_pool[ev_next], b = ev_pool[ev_next + 1];
     ev_next += 2;
     ev_used.push_back({a, b, kernel, prec, upd, flop});
+    (void)s;
     return [=] {
-        RT_CHECK(hipEventRecord(a, s));
+        set_launch_events(a, b);  // recorded at the kernel's own dispatch start and end
         fn();
-        RT_CHECK(hipEventRecord(b, s));
+        set_launch_events(nullptr, nullptr);
     };
 }
 
@@ -191,12 +196,12 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
     std::vector<PassDesc> all(n);
     for (size_t i = 0; i < who.size(); ++i) all[who[i]] = pds[i];
     const PassDesc* dpd = stage(all.data(), all.size());
-    // The FP64 passes (refinement residuals, reconstructions, c = T(0)) of this round run on st64,
-    // concurrently with the low-precision Arnoldi passes on st; st waits for them before the
-    // Krylov work.  Every group has its own task tables.
+    // With fork_streams the FP64 passes (refinement residuals, reconstructions, c = T(0)) of this
+    // round run on st64, concurrently with the low-precision Arnoldi passes on st; st waits for
+    // them before the Krylov work.  Every group has its own task tables.
     bool has64 = false, haslo = false;
     for (int p : prec) (p == P64 ? has64 : haslo) = true;
-    const bool fork = has64 && haslo;
+    const bool fork = has64 && haslo && fork_streams;
     if (fork) {
         hipStream_t s0 = st, s1 = st64;
         hipEvent_t ef = ev_fork;

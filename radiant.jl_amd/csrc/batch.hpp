@@ -38,7 +38,8 @@ private:
     };
     std::vector<PS> ps;
     hipStream_t st;
-    bool reorth = false;  // CGS2 instead of CGS
+    bool reorth = false;        // CGS2 instead of CGS
+    bool fork_streams = false;  // FP64 passes on st64 beside the low-precision ones
     std::vector<void*> mem;
     PassDesc* d_pd = nullptr;
     // task tables per precision group (the FP64 group runs on st64, concurrently with the others)

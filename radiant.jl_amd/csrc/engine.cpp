@@ -363,9 +363,9 @@ void Problem::launch_pass(PassDesc& pd, int prec, bool full_rows) {
         }
         if (ev_next + 2 > 4096) collect_kernel_times();
         hipEvent_t a = next_event(), b = next_event();
-        RT_CHECK(hipEventRecord(a, st));
+        set_launch_events(a, b);  // recorded at the kernel's own dispatch start and end
         fn();
-        RT_CHECK(hipEventRecord(b, st));
+        set_launch_events(nullptr, nullptr);
         ev_used.push_back({a, b, kernel, prec});
     };
     timed(0, [&] { launch_mgemm(prec, d_pd, d_mg, n_mg, st); });

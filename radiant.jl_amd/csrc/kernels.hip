@@ -14,6 +14,7 @@
 #include "kernels.hpp"
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <hip/hip_bf16.h>
 #include <cstdio>
 #include <cstdlib>
@@ -28,6 +29,27 @@ namespace rt {
         if (e_ != hipSuccess) throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + \
                                                        " at " + __FILE__ + ":" + std::to_string(__LINE__)); \
     } while (0)
+
+// Per-thread pair of events that the next timed launch records at its own dispatch start and end
+// (hipExtLaunchKernel): the kernel's execution interval -- what a kernel trace reports -- even while
+// kernels of other streams occupy the device (events recorded on the stream around the launch
+// would also count the wait for those).
+static thread_local hipEvent_t tl_ev_start = nullptr, tl_ev_stop = nullptr;
+void set_launch_events(hipEvent_t start, hipEvent_t stop) {
+    tl_ev_start = start;
+    tl_ev_stop = stop;
+}
+template <typename... KArgs, typename... Args>
+static void launch_timed(void (*kernel)(KArgs...), dim3 grid, dim3 block, uint32_t shmem, hipStream_t st,
+                         Args... args) {
+    if (tl_ev_start) {
+        hipEvent_t a = tl_ev_start, b = tl_ev_stop;
+        tl_ev_start = tl_ev_stop = nullptr;
+        hipExtLaunchKernelGGL(kernel, grid, block, shmem, st, a, b, 0u, static_cast<KArgs>(args)...);
+    } else {
+        hipLaunchKernelGGL(kernel, grid, block, shmem, st, static_cast<KArgs>(args)...);
+    }
+}
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -1251,17 +1273,17 @@ static inline int grid_for(size_t n, int block = 256, int cap = 2048) {
 
 void launch_mgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st) {
     if (ntasks <= 0) return;
-    if (prec == P64) hipLaunchKernelGGL((contract_kernel<double, GEMM_RT, GEMM_CT, true, 2>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
-    else if (prec == P32) hipLaunchKernelGGL((contract_kernel<float, GEMM_RT, GEMM_CT, true, 2>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
-    else hipLaunchKernelGGL((contract_bf16_kernel<GEMM_RT, GEMM_CT, true>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    if (prec == P64) launch_timed(contract_kernel<double, GEMM_RT, GEMM_CT, true, 2>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    else if (prec == P32) launch_timed(contract_kernel<float, GEMM_RT, GEMM_CT, true, 2>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    else launch_timed(contract_bf16_kernel<GEMM_RT, GEMM_CT, true>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
     RT_HIP(hipGetLastError());
 }
 
 void launch_dgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st) {
     if (ntasks <= 0) return;
-    if (prec == P64) hipLaunchKernelGGL((contract_kernel<double, GEMM_RT, GEMM_CT, false, 4>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
-    else if (prec == P32) hipLaunchKernelGGL((contract_kernel<float, GEMM_RT, GEMM_CT, false, 4>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
-    else hipLaunchKernelGGL((contract_bf16_kernel<GEMM_RT, GEMM_CT, false>), dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    if (prec == P64) launch_timed(contract_kernel<double, GEMM_RT, GEMM_CT, false, 4>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    else if (prec == P32) launch_timed(contract_kernel<float, GEMM_RT, GEMM_CT, false, 4>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    else launch_timed(contract_bf16_kernel<GEMM_RT, GEMM_CT, false>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
     RT_HIP(hipGetLastError());
 }
 
@@ -1285,7 +1307,7 @@ static void sweep_inst(const PassDesc* d_pd, const SweepTask* tasks, int ntasks,
         RT_HIP(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         attr_set = true;
     }
-    hipLaunchKernelGGL(kfn, dim3(ntasks), dim3(block), lds, st, d_pd, tasks);
+    launch_timed(kfn, dim3(ntasks), dim3(block), lds, st, d_pd, tasks);
 }
 
 template <typename T, int NBUF>
@@ -1310,7 +1332,7 @@ static void sweep_rows_inst(const PassDesc* d_pd, const SweepTask* tasks, int nt
         RT_HIP(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         attr_set = true;
     }
-    hipLaunchKernelGGL(kfn, dim3(ntasks), dim3(64), lds, st, d_pd, tasks);
+    launch_timed(kfn, dim3(ntasks), dim3(64), lds, st, d_pd, tasks);
 }
 
 int sweep_rows_per_lane(int ndims, int Ny) {
@@ -1368,7 +1390,7 @@ static void planes_inst(const PassDesc* d_pd, const SweepTask* tasks, int ntasks
         RT_HIP(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         attr_set = true;
     }
-    hipLaunchKernelGGL(kfn, dim3(ntasks), dim3(64 * waves), lds, st, d_pd, tasks);
+    launch_timed(kfn, dim3(ntasks), dim3(64 * waves), lds, st, d_pd, tasks);
 }
 
 template <typename T>
@@ -1433,7 +1455,7 @@ static void scan_inst2(const PassDesc* d_pd, const SweepTask* tasks, int ntasks,
             RT_HIP(hipFuncSetAttribute((const void*)lfn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
             lattr_set = true;
         }
-        hipLaunchKernelGGL(lfn, dim3(ntasks), dim3(64), lds, st, d_pd, tasks);
+        launch_timed(lfn, dim3(ntasks), dim3(64), lds, st, d_pd, tasks);
         return;
     }
     constexpr int U = CPL >= 4 ? 1 : 2;
@@ -1443,7 +1465,7 @@ static void scan_inst2(const PassDesc* d_pd, const SweepTask* tasks, int ntasks,
         RT_HIP(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         attr_set = true;
     }
-    hipLaunchKernelGGL(kfn, dim3(ntasks), dim3(64 * waves), lds, st, d_pd, tasks);
+    launch_timed(kfn, dim3(ntasks), dim3(64 * waves), lds, st, d_pd, tasks);
 }
 
 template <typename T, int CPL>
@@ -1487,7 +1509,7 @@ void launch_sweep(int prec, const PassDesc* d_pd, const SweepTask* tasks, int nt
 }
 
 void launch_bndmom(const PassDesc* d_pd, int nprob, int max_items, hipStream_t st) {
-    hipLaunchKernelGGL(bndmom_kernel, dim3(grid_for((size_t)max_items)), dim3(256), 0, st, d_pd, nprob);
+    launch_timed(bndmom_kernel, dim3(grid_for((size_t)max_items)), dim3(256), 0, st, d_pd, nprob);
     RT_HIP(hipGetLastError());
 }
 

@@ -41,6 +41,10 @@ struct PassDesc {
 
 enum Prec { P64 = 0, P32 = 1, PBF16 = 2 };
 
+// The next launch of a pass kernel on this host thread (mgemm, sweep, dgemm, bndmom) records start
+// and stop at its own dispatch (kernel execution interval); null, null: untimed.
+void set_launch_events(hipEvent_t start, hipEvent_t stop);
+
 // Work tiles of the contraction kernels: x = problem, y = 64-cell tile, z = row group.
 struct GemmTask { int prob, tile, rgroup, pad; };
 struct SweepTask { int prob, dir; };
