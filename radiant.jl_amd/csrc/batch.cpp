@@ -199,9 +199,18 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
     // With fork_streams the FP64 passes (refinement residuals, reconstructions, c = T(0)) of this
     // round run on st64, concurrently with the low-precision Arnoldi passes on st; st waits for
     // them before the Krylov work.  Every group has its own task tables.
-    bool has64 = false, haslo = false;
-    for (int p : prec) (p == P64 ? has64 : haslo) = true;
+    bool has64 = false, haslo = false, has32 = false;
+    for (int p : prec) {
+        (p == P64 ? has64 : haslo) = true;
+        if (p == P32) has32 = true;
+    }
     const bool fork = has64 && haslo && fork_streams;
+    // Without the fork, the FP64 and FP32 passes of a round share one launch of each kernel
+    // (every workgroup takes its problem's precision): a sweep costs the latency of its 128-ish
+    // dependent rows whatever the number of problems in it.  Such launches are recorded in the
+    // FP32 row of the kernel timing, with FP64 updates counted twice (FP32-equivalent bytes).
+    const bool mix = has64 && has32 && !fork;
+    auto group_of = [&](int p) { return (mix && (p == P64 || p == P32)) ? (int)PMIX : p; };
     if (fork) {
         hipStream_t s0 = st, s1 = st64;
         hipEvent_t ef = ev_fork;
@@ -210,7 +219,8 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
             RT_CHECK(hipStreamWaitEvent(s1, ef, 0));
         });
     }
-    for (int pr : {P64, P32, PBF16}) {
+    for (int pr : {(int)PMIX, (int)P64, (int)P32, (int)PBF16}) {
+        const int tix = pr == PMIX ? (int)P32 : pr;  // task tables and timing row of the group
         std::vector<int> mmeta, dmeta;
         std::map<std::pair<int, int>, std::vector<int>> sweeps;  // (cpl, waves) -> sweep meta
         int moff = 0, doff = 0;
@@ -219,7 +229,7 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
         std::map<std::pair<int, int>, double> s_upd;
         double g_upd = 0, g_mflop = 0, g_dflop = 0;
         for (size_t i = 0; i < who.size(); ++i) {
-            if (prec[i] != pr) continue;
+            if (group_of(prec[i]) != pr) continue;
             const Problem* P = ps[who[i]].P;
             const int ntile = (P->Nc + GEMM_CELLS - 1) / GEMM_CELLS;
             const int nrg_m = (P->Nd + GEMM_ROWS - 1) / GEMM_ROWS;
@@ -233,12 +243,12 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
             auto& v = sweeps[key];
             v.insert(v.end(), {who[i], P->Nd, soff[key]});
             soff[key] += P->Nd;
-            const double upd = (double)P->Nc * P->Nd;
+            const double upd = (double)P->Nc * P->Nd * ((pr == PMIX && prec[i] == P64) ? 2.0 : 1.0);
             g_upd += upd;
             g_mflop += 2.0 * P->Nact * upd;
             g_dflop += 2.0 * rows * upd;
             s_upd[key] += upd;
-            const int sp = pr == PBF16 ? P32 : pr;
+            const int sp = pr == PBF16 ? (int)P32 : (pr == PMIX ? (int)P64 : pr);
             slds[key] = std::max(slds[key], sweep_scan_lds_bytes(sp, P->ndims, P->Nx, P->Ny, P->Nz, P->Nmat));
         }
         if (mmeta.empty()) continue;
@@ -246,15 +256,15 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
         const int nm = (int)mmeta.size() / 4;
         const int* dmm = stage(mmeta.data(), mmeta.size());
         const int* ddm = stage(dmeta.data(), dmeta.size());
-        GemmTask *mg = d_mg[pr], *dg = d_dg[pr];
-        SweepTask* swbase = d_sw[pr];
+        GemmTask *mg = d_mg[tix], *dg = d_dg[tix];
+        SweepTask* swbase = d_sw[tix];
         hipStream_t s = (fork && pr == P64) ? st64 : st;
         const int ps_ = pr;
         plan.push_back([=] {
             launch_expand_gemm(dmm, nm, mg, s);
             launch_expand_gemm(ddm, nm, dg, s);
         });
-        plan.push_back(timed(0, pr, g_upd, g_mflop, s, [=] { launch_mgemm(ps_, dpd, mg, moff, s); }));
+        plan.push_back(timed(0, tix, g_upd, g_mflop, s, [=] { launch_mgemm(ps_, dpd, mg, moff, s); }));
         for (auto& kv : sweeps) {
             const std::vector<int>& sm = kv.second;
             const int* dsm = stage(sm.data(), sm.size());
@@ -264,12 +274,12 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
             const size_t lds = slds[kv.first];
             SweepTask* sb = swbase;
             plan.push_back([=] { launch_expand_sweep(dsm, nsm, sb, s); });
-            plan.push_back(timed(1, pr, s_upd[kv.first], 0.0, s, [=] {
+            plan.push_back(timed(1, tix, s_upd[kv.first], 0.0, s, [=] {
                 launch_sweep_scan(ps_ == PBF16 ? P32 : ps_, dpd, sb, ntask, cpl, waves, lds, s);
             }));
             swbase += ntask;
         }
-        plan.push_back(timed(2, pr, g_upd, g_dflop, s, [=] { launch_dgemm(ps_, dpd, dg, doff, s); }));
+        plan.push_back(timed(2, tix, g_upd, g_dflop, s, [=] { launch_dgemm(ps_, dpd, dg, doff, s); }));
     }
     if (fork) {
         hipStream_t s0 = st, s1 = st64;

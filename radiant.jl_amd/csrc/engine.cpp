@@ -424,6 +424,7 @@ void Problem::pass_faithful(int g, int prec, int mode, const double* zin, double
 // Reduced state [Φ_act (Nact x Nc)]: only the moments that feed back into the next pass.
 void Problem::desc_reduced(PassDesc& pd, int g, int prec, int mode, const double* zin, double* zout) {
     fill_geometry(pd, g);
+    pd.prec = prec;
     pd.Mact = (prec == P64) ? (const void*)d_Mact64 : (const void*)d_Mact32;
     pd.qlout = (mode == MODE_T) ? d_qlout : nullptr;
     pd.phi_in = zin;

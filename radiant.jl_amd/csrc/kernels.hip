@@ -139,10 +139,7 @@ __device__ inline void epilogue_dgemm(const PassDesc& d, const V (&acc)[RT][CT],
 }
 
 template <typename T, int RT, int CT, bool MG, int NS>
-__global__ __launch_bounds__(256) void contract_kernel(const PassDesc* __restrict__ pds, const GemmTask* __restrict__ tasks) {
-    __shared__ int rows_s[MG ? CONTRACT_KMAX : 1];
-    const GemmTask tk = tasks[blockIdx.x];
-    const PassDesc& d = pds[tk.prob];
+__device__ __forceinline__ void contract_body(const PassDesc& d, const GemmTask tk, int* rows_s) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int Nc = d.Nc;
     const int K = MG ? d.Nact : d.Nd;
@@ -264,6 +261,24 @@ __global__ __launch_bounds__(256) void contract_kernel(const PassDesc* __restric
         // store, unconditionally, and combined as zs*z + sg*v (a select would branch the loads).
         epilogue_dgemm<T, RT, CT>(d, acc, c0, r0, li, lane, M, Nc);
     }
+}
+
+template <typename T, int RT, int CT, bool MG, int NS>
+__global__ __launch_bounds__(256) void contract_kernel(const PassDesc* __restrict__ pds, const GemmTask* __restrict__ tasks) {
+    __shared__ int rows_s[MG ? CONTRACT_KMAX : 1];
+    const GemmTask tk = tasks[blockIdx.x];
+    contract_body<T, RT, CT, MG, NS>(pds[tk.prob], tk, rows_s);
+}
+
+// FP64 and FP32 passes of different problems in one launch (PassDesc::prec per workgroup).
+template <int RT, int CT, bool MG, int NS>
+__global__ __launch_bounds__(256) void contract_mixed_kernel(const PassDesc* __restrict__ pds,
+                                                             const GemmTask* __restrict__ tasks) {
+    __shared__ int rows_s[MG ? CONTRACT_KMAX : 1];
+    const GemmTask tk = tasks[blockIdx.x];
+    const PassDesc& d = pds[tk.prob];
+    if (d.prec == P64) contract_body<double, RT, CT, MG, NS>(d, tk, rows_s);
+    else contract_body<float, RT, CT, MG, NS>(d, tk, rows_s);
 }
 
 // BF16 operands, FP32 accumulation (v_mfma_f32_16x16x16_bf16): A[16x16] lane l holds
@@ -815,12 +830,7 @@ __device__ inline T* trash_slot(int lane, int k) { return (T*)g_sweep_trash + k 
 // fluxes in registers (each lane owns its cells).  The angular source and material of row
 // y + PD are loaded while row y is solved.
 template <typename T, int CPL, int PD, bool DPP>
-__global__ __launch_bounds__(64) void sweep_line_kernel(const PassDesc* __restrict__ pds,
-                                                        const SweepTask* __restrict__ tasks) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const SweepTask tk = tasks[blockIdx.x];
-    const PassDesc& d = pds[tk.prob];
-    const int n = tk.dir;
+__device__ __forceinline__ void sweep_line_body(const PassDesc& d, const int n, unsigned char* smem) {
     const int ndims = d.ndims, Nx = d.Nx, Ny = d.Ny, Nc = d.Nc, Nmat = d.Nmat;
     const int lane = threadIdx.x;
     T* hy2 = (T*)smem;   // [Ny]
@@ -923,16 +933,32 @@ __global__ __launch_bounds__(64) void sweep_line_kernel(const PassDesc* __restri
     }
 }
 
+template <typename T, int CPL, int PD, bool DPP>
+__global__ __launch_bounds__(64) void sweep_line_kernel(const PassDesc* __restrict__ pds,
+                                                        const SweepTask* __restrict__ tasks) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const SweepTask tk = tasks[blockIdx.x];
+    sweep_line_body<T, CPL, PD, DPP>(pds[tk.prob], tk.dir, smem);
+}
+
+// FP64 and FP32 passes of different problems in one launch: each workgroup takes the precision of
+// its problem's pass (PassDesc::prec; BF16 passes sweep in FP32).
+template <int CPL, int PD, bool DPP>
+__global__ __launch_bounds__(64) void sweep_line_mixed_kernel(const PassDesc* __restrict__ pds,
+                                                              const SweepTask* __restrict__ tasks) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const SweepTask tk = tasks[blockIdx.x];
+    const PassDesc& d = pds[tk.prob];
+    if (d.prec == P64) sweep_line_body<double, CPL, PD, DPP>(d, tk.dir, smem);
+    else sweep_line_body<float, CPL, PD, DPP>(d, tk.dir, smem);
+}
+
 // 3D: one workgroup per direction.  Pencils (jy, jz) advance by diagonals t = jy + jz; the
 // y/z face fluxes of the previous diagonal live in LDS (FY[jz][x], FZ[jy][x]), one barrier per
 // diagonal.  After the last diagonal FY and FZ hold the exit y- and z-faces, which are written
 // out at the end with coalesced stores.
 template <typename T, int CPL, int U, bool DPP>
-__global__ __launch_bounds__(1024) void sweep_scan_kernel(const PassDesc* __restrict__ pds, const SweepTask* __restrict__ tasks) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    const SweepTask tk = tasks[blockIdx.x];
-    const PassDesc& d = pds[tk.prob];
-    const int n = tk.dir;
+__device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, unsigned char* smem) {
     const int Nx = d.Nx, Ny = d.Ny, Nz = d.Nz, Nc = d.Nc, Nmat = d.Nmat;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, W = blockDim.x >> 6;
     T* hy2 = (T*)smem;
@@ -1084,6 +1110,23 @@ __global__ __launch_bounds__(1024) void sweep_scan_kernel(const PassDesc* __rest
         const int ix = fxs ? jx : Nx - 1 - jx, iy = fys ? jy : Ny - 1 - jy;
         gst(bz + ix + Nx * iy, FZ[i]);
     }
+}
+
+template <typename T, int CPL, int U, bool DPP>
+__global__ __launch_bounds__(1024) void sweep_scan_kernel(const PassDesc* __restrict__ pds, const SweepTask* __restrict__ tasks) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const SweepTask tk = tasks[blockIdx.x];
+    sweep_scan_body<T, CPL, U, DPP>(pds[tk.prob], tk.dir, smem);
+}
+
+template <int CPL, int U, bool DPP>
+__global__ __launch_bounds__(1024) void sweep_scan_mixed_kernel(const PassDesc* __restrict__ pds,
+                                                                const SweepTask* __restrict__ tasks) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const SweepTask tk = tasks[blockIdx.x];
+    const PassDesc& d = pds[tk.prob];
+    if (d.prec == P64) sweep_scan_body<double, CPL, U, DPP>(d, tk.dir, smem);
+    else sweep_scan_body<float, CPL, U, DPP>(d, tk.dir, smem);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1273,7 +1316,8 @@ static inline int grid_for(size_t n, int block = 256, int cap = 2048) {
 
 void launch_mgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st) {
     if (ntasks <= 0) return;
-    if (prec == P64) launch_timed(contract_kernel<double, GEMM_RT, GEMM_CT, true, 2>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    if (prec == PMIX) launch_timed(contract_mixed_kernel<GEMM_RT, GEMM_CT, true, 2>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    else if (prec == P64) launch_timed(contract_kernel<double, GEMM_RT, GEMM_CT, true, 2>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
     else if (prec == P32) launch_timed(contract_kernel<float, GEMM_RT, GEMM_CT, true, 2>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
     else launch_timed(contract_bf16_kernel<GEMM_RT, GEMM_CT, true>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
     RT_HIP(hipGetLastError());
@@ -1281,7 +1325,8 @@ void launch_mgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int nta
 
 void launch_dgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st) {
     if (ntasks <= 0) return;
-    if (prec == P64) launch_timed(contract_kernel<double, GEMM_RT, GEMM_CT, false, 4>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    if (prec == PMIX) launch_timed(contract_mixed_kernel<GEMM_RT, GEMM_CT, false, 4>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    else if (prec == P64) launch_timed(contract_kernel<double, GEMM_RT, GEMM_CT, false, 4>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
     else if (prec == P32) launch_timed(contract_kernel<float, GEMM_RT, GEMM_CT, false, 4>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
     else launch_timed(contract_bf16_kernel<GEMM_RT, GEMM_CT, false>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
     RT_HIP(hipGetLastError());
@@ -1468,16 +1513,54 @@ static void scan_inst2(const PassDesc* d_pd, const SweepTask* tasks, int ntasks,
     launch_timed(kfn, dim3(ntasks), dim3(64 * waves), lds, st, d_pd, tasks);
 }
 
+template <int CPL, bool DPP>
+static void scan_mixed_inst2(const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int waves, size_t lds,
+                             hipStream_t st) {
+    if (waves == 0) {
+        constexpr int PD = CPL <= 2 ? 8 : (CPL == 4 ? 4 : 2);
+        auto lfn = sweep_line_mixed_kernel<CPL, PD, DPP>;
+        static bool lattr_set = false;
+        if (!lattr_set) {
+            RT_HIP(hipFuncSetAttribute((const void*)lfn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            lattr_set = true;
+        }
+        launch_timed(lfn, dim3(ntasks), dim3(64), lds, st, d_pd, tasks);
+        return;
+    }
+    constexpr int U = CPL >= 4 ? 1 : 2;
+    auto kfn = sweep_scan_mixed_kernel<CPL, U, DPP>;
+    static bool attr_set = false;
+    if (!attr_set) {
+        RT_HIP(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr_set = true;
+    }
+    launch_timed(kfn, dim3(ntasks), dim3(64 * waves), lds, st, d_pd, tasks);
+}
+
 template <typename T, int CPL>
 static void scan_inst(const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int waves, size_t lds, hipStream_t st) {
     if (scan_use_dpp()) scan_inst2<T, CPL, true>(d_pd, tasks, ntasks, waves, lds, st);
     else scan_inst2<T, CPL, false>(d_pd, tasks, ntasks, waves, lds, st);
 }
 
+template <int CPL>
+static void scan_mixed_inst(const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int waves, size_t lds,
+                            hipStream_t st) {
+    if (scan_use_dpp()) scan_mixed_inst2<CPL, true>(d_pd, tasks, ntasks, waves, lds, st);
+    else scan_mixed_inst2<CPL, false>(d_pd, tasks, ntasks, waves, lds, st);
+}
+
 void launch_sweep_scan(int prec, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int cpl, int waves, size_t lds,
                        hipStream_t st) {
     if (ntasks <= 0) return;
-    if (prec == P64) {
+    if (prec == PMIX) {
+        switch (cpl) {
+            case 1: scan_mixed_inst<1>(d_pd, tasks, ntasks, waves, lds, st); break;
+            case 2: scan_mixed_inst<2>(d_pd, tasks, ntasks, waves, lds, st); break;
+            case 4: scan_mixed_inst<4>(d_pd, tasks, ntasks, waves, lds, st); break;
+            default: scan_mixed_inst<8>(d_pd, tasks, ntasks, waves, lds, st); break;
+        }
+    } else if (prec == P64) {
         switch (cpl) {
             case 1: scan_inst<double, 1>(d_pd, tasks, ntasks, waves, lds, st); break;
             case 2: scan_inst<double, 2>(d_pd, tasks, ntasks, waves, lds, st); break;

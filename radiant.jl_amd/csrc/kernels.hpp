@@ -37,9 +37,11 @@ struct PassDesc {
     const int *exitf_x, *exitf_y, *exitf_z;      // [Nd] exit face (0 = minus, 1 = plus)
     const int *exlist_x, *exlist_y, *exlist_z;   // [2][Nd] directions leaving through face f, in index order
     int excnt[3][2];                             // lengths of those lists
+    int prec;                                    // Prec of this pass (operators, psi, bx/by/bz)
 };
 
-enum Prec { P64 = 0, P32 = 1, PBF16 = 2 };
+enum Prec { P64 = 0, P32 = 1, PBF16 = 2,
+            PMIX = 3 };  // launch level: FP64 and FP32 passes of different problems in one launch
 
 // The next launch of a pass kernel on this host thread (mgemm, sweep, dgemm, bndmom) records start
 // and stop at its own dispatch (kernel execution interval); null, null: untimed.
