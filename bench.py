@@ -135,6 +135,46 @@ def run_batch(probs, precision, steps, warmup, label):
     return out, phis
 
 
+def run_serve(nreq, rate, budget_gib, max_active, precision):
+    """Served stream: the first `nreq` requests of the 3D Carlson stream (N~U{32..96} per axis,
+    G~U{10..40}, Carlson 8/10/12, P3, 3 random materials) submitted at Poisson arrival times
+    (rate per second) to rt_server; latency = submit -> result ready on the host."""
+    from radiant_hip import problems, solver
+    t_arr = problems.poisson_arrivals(nreq, rate=rate)
+    srv = solver.Server(hbm_budget_bytes=int(budget_gib * 1024 ** 3), max_active=max_active, precision=precision)
+    ids, nupd, lat = [], {}, {}
+    t0 = time.perf_counter()
+    try:
+        for i in range(nreq):
+            p = problems.config4_request(i)
+            wait = t_arr[i] - (time.perf_counter() - t0)
+            if wait > 0:
+                time.sleep(wait)
+            rid = srv.submit(p)
+            ids.append(rid)
+            nupd[rid] = p.Nc * srv.descs[rid][1] * p.G
+            log(f"[serve] submit {i} (N={p.Nx}, G={p.G}, Nd={srv.descs[rid][1]}) at {time.perf_counter() - t0:.2f}s")
+            del p
+        while len(lat) < len(ids):
+            for rid in ids:
+                if rid in lat:
+                    continue
+                r = srv.poll(rid)
+                if r is not None:
+                    lat[rid] = r[2]
+                    log(f"[serve] request {rid} done: {r[2]:.0f} ms")
+                    del r
+            time.sleep(0.002)
+        wall = time.perf_counter() - t0
+    finally:
+        srv.close()
+    ls = np.array([lat[r] for r in ids])
+    return dict(requests=nreq, rate_per_s=rate, wall_s=wall, requests_per_s=nreq / wall,
+                updates_per_s=sum(nupd.values()) / wall, p50_ms=float(np.percentile(ls, 50)),
+                p99_ms=float(np.percentile(ls, 99)), budget_gib=budget_gib, max_active=max_active,
+                precision=precision)
+
+
 def kernel_work(prob, res, nact):
     """(updates, flops) processed by each [precision][kernel] class over the profiled solve."""
     if "kernel_updates" in res:
@@ -228,6 +268,9 @@ def main():
     ap.add_argument("--batch", type=int, default=32, help="problems per lockstep batch (1 = no batched mode)")
     ap.add_argument("--headline-only", action="store_true",
                     help="run only the batched headline mode (for kernel profiling of exactly that mode)")
+    ap.add_argument("--c3", type=int, default=0,
+                    help="instead of the C1 benchmark, solve the first N problems of the uneven 2D batch "
+                         "(N~U{32..160} per axis, G~U{8..32}, P1..P5, GLC 8/10) in one lockstep batch")
     ap.add_argument("--serve", type=int, default=0,
                     help="instead of the solve benchmark, serve this many requests of the 3D request stream")
     ap.add_argument("--rate", type=float, default=50.0, help="served stream: Poisson arrivals per second")
@@ -241,6 +284,24 @@ def main():
         sys.exit(2)
     from radiant_hip import problems, solver
     log(solver.version())
+    if args.c3 > 0:
+        # GLC 12/14/16 members are numerically singular Galerkin-D operators (cond(D) ~ 1e18): the
+        # measured batch draws its quadrature orders from GLC 8 and 10.
+        probs = problems.config3(args.c3, quad_orders=(8, 10))
+        t0 = time.perf_counter()
+        r, _ = run_batch(probs, "mixed-fp32", args.steps, args.warmup, f"c3x{args.c3}")
+        log(f"[c3] total {time.perf_counter() - t0:.1f} s")
+        print(json.dumps({
+            "metric": "cell·dir·group updates/s (uneven batch)", "value": r["updates_per_s"],
+            "unit": "cell·dir·group updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": r["wall_s"] / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32 (sweep, M/D contractions) + f64 (Krylov, refinement)",
+            "data": "synthetic (seeded splitmix64)",
+            "config": {"workload": f"first {args.c3} problems of the uneven 2D batch (GLC 8/10)", "batch": args.c3},
+            "solves_per_s": r["solves_per_s"], "raw_updates_per_s": r["raw_updates_per_s"],
+            "passes": r["passes"], "converged": r["converged"]},
+            default=lambda o: o.tolist() if hasattr(o, "tolist") else str(o)))
+        return
     if args.serve > 0:
         r = run_serve(args.serve, args.rate, args.budget_gib, args.max_active, "mixed-fp32")
         print(json.dumps({

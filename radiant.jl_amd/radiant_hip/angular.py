@@ -194,8 +194,10 @@ def quadrature(N, quad_type, ndims=1, qdims=None):
 
 
 # ----------------------------------------------------------------------------- harmonics
-def _ff(num, den, ex=()):
-    """tools/factorial_factor.jl:17-33."""
+@functools.lru_cache(maxsize=None)
+def _ff_int(num, den):
+    """The integer part of factorial_factor: the same sequential sum of logs, computed once per
+    (num, den)."""
     f = 0.0
     for n in num:
         for i in range(1, n + 1):
@@ -203,6 +205,12 @@ def _ff(num, den, ex=()):
     for n in den:
         for i in range(1, n + 1):
             f -= math.log(i)
+    return f
+
+
+def _ff(num, den, ex=()):
+    """tools/factorial_factor.jl:17-33."""
+    f = _ff_int(tuple(num), tuple(den))
     for x, m in ex:
         if m == 0:
             continue
@@ -496,5 +504,21 @@ class AngularSet:
 
 
 @functools.lru_cache(maxsize=64)
-def angular_set(quad_type, quad_order, ndims, qdims, L, sn_type):
+def _angular_set(quad_type, quad_order, ndims, qdims, L, sn_type):
     return AngularSet(quad_type, quad_order, ndims, qdims, L, sn_type)
+
+
+def angular_set(quad_type, quad_order, ndims, qdims, L, sn_type):
+    """Cached AngularSet.  For the Galerkin types the bases are built to max(L, minimum order)
+    (angular_polynomial_basis.jl:64 and the surface twin), so every L up to the smallest of those
+    minima gives the same set; such L share one cache entry."""
+    if sn_type in ("galerkin-m", "galerkin-d"):
+        Q = quadrature_dimension(quad_type, ndims, qdims)
+        mu, eta, xi, w = quadrature(quad_order, quad_type, ndims, Q)
+        mo = galerkin_minimum_order(len(w), Q)
+        for f in range(2 * ndims):  # directions entering each face, as surface_basis selects them
+            a = (mu, eta, xi)[f // 2] if Q > 1 else mu
+            ndp = int(np.count_nonzero(a > 0 if f % 2 == 0 else a < 0))
+            mo = min(mo, galerkin_minimum_order(ndp, Q))
+        L = max(L, mo)
+    return _angular_set(quad_type, quad_order, ndims, qdims, L, sn_type)

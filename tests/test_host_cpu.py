@@ -73,3 +73,16 @@ def test_descriptor_packing():
     assert D.sigt[1] == p.sigt[1, 0] and D.sigt[3] == p.sigt[0, 1]
     assert D.sigs[1] == p.sigs[1, 0, 0, 0] and D.sigs[2] == p.sigs[0, 1, 0, 0]
     assert D.M[1] == d.ang.M[1, 0] and D.M[d.ang.Nd] == d.ang.M[0, 1]
+
+
+def test_angular_cache_key_equivalence():
+    """Every Legendre order below the Galerkin minimum orders shares one cached set; the shared set
+    equals the one built directly for that order (angular_polynomial_basis.jl:64)."""
+    import numpy as np
+    from radiant_hip import angular as A
+    direct = A.AngularSet("gauss-legendre-chebychev", 4, 2, 3, 2, "galerkin-d")
+    cached = A.angular_set("gauss-legendre-chebychev", 4, 2, 3, 2, "galerkin-d")
+    assert np.array_equal(direct.M, cached.M) and np.array_equal(direct.D, cached.D)
+    assert np.array_equal(direct.pl, cached.pl)
+    for a, b in zip(direct.faces, cached.faces):
+        assert a[0] == b[0] and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
