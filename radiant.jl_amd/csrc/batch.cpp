@@ -30,29 +30,20 @@ constexpr int S_H1 = 0, S_H2 = 40, S_WN = 80, S_NRM = 81;
 }  // namespace
 
 BatchSolver::BatchSolver(const std::vector<Problem*>& probs, hipStream_t stream) : st(stream) {
-    const int n = (int)probs.size();
+    init_device((int)probs.size());
+    for (Problem* P : probs) add(P);
+    RT_CHECK(hipStreamSynchronize(st));
+}
+
+BatchSolver::BatchSolver(int capacity, hipStream_t stream) : st(stream) {
+    if (capacity < 1) throw Error(RT_EINVAL, "batched solver: capacity must be positive");
+    init_device(capacity);
+    RT_CHECK(hipStreamSynchronize(st));
+}
+
+void BatchSolver::init_device(int n) {
     ps.resize(n);
-    for (int i = 0; i < n; ++i) {
-        Problem* P = probs[i];
-        if (P->precision == RT_PREC_FP64)
-            throw Error(RT_EINVAL, "the batched solver runs the mixed-precision modes");
-        if (P->scan_cpl <= 0 || sweep_scan_lds_bytes(P64, P->ndims, P->Nx, P->Ny, P->Nz, P->Nmat) > 160 * 1024)
-            throw Error(RT_EINVAL, "a batched problem does not fit the row-scan sweep");
-        PS& s = ps[i];
-        s.P = P;
-        s.m = std::min(std::max(1, P->restart), 32);
-        s.lo = (P->precision == RT_PREC_MIXED_BF16) ? PBF16 : P32;
-        s.inner_tol = (s.lo == PBF16) ? 3e-3 : 1e-5;
-        const size_t S = P->S_r;
-        for (int k = 0; k <= s.m; ++k) s.V.push_back(P->vec("bV" + std::to_string(k), S));
-        s.w = P->vec("bw", S);
-        s.c = P->vec("bc", S);
-        s.z = P->vec("bz", S);
-        s.r = P->vec("br", S);
-        s.t = P->vec("bt", S);
-        RT_CHECK(hipMemcpyAsync(P->d_Vptr, s.V.data(), (s.m + 1) * sizeof(double*), hipMemcpyHostToDevice, st));
-        s.stats.assign(P->G, rt_group_stats{});
-    }
+    for (auto& s : ps) s.phase = DONE;
     auto al = [&](size_t bytes) {
         void* p = nullptr;
         hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 16));
@@ -63,21 +54,7 @@ BatchSolver::BatchSolver(const std::vector<Problem*>& probs, hipStream_t stream)
         mem.push_back(p);
         return p;
     };
-    size_t ng = 0, nsw = 0;
-    for (auto& s : ps) {
-        const Problem* P = s.P;
-        const int ntile = (P->Nc + GEMM_CELLS - 1) / GEMM_CELLS;
-        ng = std::max(ng, (size_t)ntile * ((std::max(P->Nd, P->Np) + GEMM_ROWS - 1) / GEMM_ROWS));
-        nsw = std::max(nsw, (size_t)P->Nd);
-    }
-    cap_gemm = ng * n;
-    cap_sweep = nsw * n;
     d_pd = (PassDesc*)al(sizeof(PassDesc) * n);
-    for (int pr = 0; pr < 3; ++pr) {
-        d_mg[pr] = (GemmTask*)al(sizeof(GemmTask) * cap_gemm);
-        d_dg[pr] = (GemmTask*)al(sizeof(GemmTask) * cap_gemm);
-        d_sw[pr] = (SweepTask*)al(sizeof(SweepTask) * cap_sweep);
-    }
     RT_CHECK(hipStreamCreateWithFlags(&st64, hipStreamNonBlocking));
     {
         const char* e = std::getenv("RADIANT_HIP_CGS2");  // re-orthogonalise every Arnoldi step
@@ -96,6 +73,106 @@ BatchSolver::BatchSolver(const std::vector<Problem*>& probs, hipStream_t stream)
                  64 * sizeof(int)) * (size_t)n + (1 << 20);
     d_arena = (unsigned char*)al(arena_cap);
     RT_CHECK(hipHostMalloc((void**)&h_arena, arena_cap, hipHostMallocDefault));
+}
+
+void BatchSolver::ensure_task_capacity() {
+    size_t ng = 0, nsw = 0;
+    for (auto& s : ps) {
+        if (!s.P) continue;
+        const Problem* P = s.P;
+        const size_t ntile = (P->Nc + GEMM_CELLS - 1) / GEMM_CELLS;
+        ng += ntile * ((std::max(P->Nd, P->Np) + GEMM_ROWS - 1) / GEMM_ROWS);
+        nsw += (size_t)P->Nd;
+    }
+    if (ng <= cap_gemm && nsw <= cap_sweep) return;
+    // the tables are rewritten every round: after the stream drains they can be replaced
+    RT_CHECK(hipStreamSynchronize(st));
+    if (st64) RT_CHECK(hipStreamSynchronize(st64));
+    for (void*& p : task_mem)
+        if (p) {
+            (void)hipFree(p);
+            p = nullptr;
+        }
+    cap_gemm = std::max(cap_gemm, ng + ng / 2);
+    cap_sweep = std::max(cap_sweep, nsw + nsw / 2);
+    auto al = [&](size_t bytes) {
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 16));
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error(RT_ENOMEM, "batched solver: device allocation failed");
+        }
+        return p;
+    };
+    for (int pr = 0; pr < 3; ++pr) {
+        task_mem[3 * pr] = d_mg[pr] = (GemmTask*)al(sizeof(GemmTask) * cap_gemm);
+        task_mem[3 * pr + 1] = d_dg[pr] = (GemmTask*)al(sizeof(GemmTask) * cap_gemm);
+        task_mem[3 * pr + 2] = d_sw[pr] = (SweepTask*)al(sizeof(SweepTask) * cap_sweep);
+    }
+}
+
+int BatchSolver::add(Problem* P) {
+    if (P->precision == RT_PREC_FP64)
+        throw Error(RT_EINVAL, "the batched solver runs the mixed-precision modes");
+    if (P->scan_cpl <= 0 || sweep_scan_lds_bytes(P64, P->ndims, P->Nx, P->Ny, P->Nz, P->Nmat) > 160 * 1024)
+        throw Error(RT_EINVAL, "a batched problem does not fit the row-scan sweep");
+    int slot = -1;
+    for (int i = 0; i < (int)ps.size(); ++i)
+        if (!ps[i].P) {
+            slot = i;
+            break;
+        }
+    if (slot < 0) throw Error(RT_ENOMEM, "batched solver: no free slot");
+    PS& s = ps[slot];
+    s = PS{};
+    s.P = P;
+    s.m = std::min(std::max(1, P->restart), 32);
+    s.lo = (P->precision == RT_PREC_MIXED_BF16) ? PBF16 : P32;
+    s.inner_tol = (s.lo == PBF16) ? 3e-3 : 1e-5;
+    const size_t S = P->S_r;
+    for (int k = 0; k <= s.m; ++k) s.V.push_back(P->vec("bV" + std::to_string(k), S));
+    s.w = P->vec("bw", S);
+    s.c = P->vec("bc", S);
+    s.z = P->vec("bz", S);
+    s.r = P->vec("br", S);
+    s.t = P->vec("bt", S);
+    RT_CHECK(hipMemcpy(P->d_Vptr, s.V.data(), (s.m + 1) * sizeof(double*), hipMemcpyHostToDevice));
+    s.stats.assign(P->G, rt_group_stats{});
+    try {
+        ensure_task_capacity();
+    } catch (...) {
+        s = PS{};
+        s.phase = DONE;
+        throw;
+    }
+    start(slot);
+    return slot;
+}
+
+void BatchSolver::start(int slot) {
+    PS& s = ps[slot];
+    s.P->counters = SolveCounters{};
+    k_fill(s.P->d_phi, 0.0, (size_t)s.P->G * s.P->Np * s.P->Nc, st);
+    s.phase = START;
+    s.g = 0;
+    std::fill(s.stats.begin(), s.stats.end(), rt_group_stats{});
+}
+
+// Vector operations recorded for the next round may write into the slot's memory: they run
+// before the problem is released.
+void BatchSolver::remove(int slot) {
+    drain_pre();
+    PS& s = ps[slot];
+    s = PS{};
+    s.phase = DONE;
+}
+
+void BatchSolver::drain_pre() {
+    if (pre.empty()) return;
+    arena_off = 0;
+    run_vecops(pre);
+    pre.clear();
+    flush();
     RT_CHECK(hipStreamSynchronize(st));
 }
 
@@ -152,6 +229,8 @@ void BatchSolver::collect_times() {
 BatchSolver::~BatchSolver() {
     (void)hipStreamSynchronize(st);
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+    for (void* p : task_mem)
+        if (p) (void)hipFree(p);
     if (st64) {
         (void)hipStreamSynchronize(st64);
         (void)hipStreamDestroy(st64);
@@ -297,140 +376,151 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
     }
 }
 
-void BatchSolver::run(rt_solve_info* info) {
+// One round: every unfinished problem does one pass of its current phase, then its Krylov work;
+// the host reads the scalars and advances each problem's state machine.
+bool BatchSolver::round() {
+    using clk = std::chrono::steady_clock;
+    auto us = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::micro>(b - a).count(); };
     const int n = (int)ps.size();
+    // ---- group starts: out-of-group + volume source (sn_flux.jl:175-179)
+    for (auto& s : ps) {
+        if (s.phase != START) continue;
+        Problem* P = s.P;
+        launch_outsrc(P->d_qlout, P->d_phi, P->d_sigo, P->d_qvol, P->d_mat, P->d_act, s.g, P->G, P->Nmat, P->Nact,
+                      P->Np, P->Nc, st);
+        s.phase = CPASS;
+        s.niter = 0;
+        s.refs = 0;
+        s.p0 = P->counters.passes;
+        s.l0 = P->counters.passes_lo;
+    }
+    const auto t0 = clk::now();
+    arena_off = 0;
+    run_vecops(pre);
+    pre.clear();
+    // ---- one pass per active problem
+    std::vector<int> who, prec, full;
+    std::vector<PassDesc> pds;
+    for (int i = 0; i < n; ++i) {
+        PS& s = ps[i];
+        if (s.phase == DONE) continue;
+        Problem* P = s.P;
+        PassDesc pd;
+        switch (s.phase) {
+            case CPASS: P->desc_reduced(pd, s.g, P64, MODE_T, nullptr, s.c); prec.push_back(P64); full.push_back(0); break;
+            case ARNOLDI: P->desc_reduced(pd, s.g, s.lo, MODE_IMA, s.V[s.j], s.w); prec.push_back(s.lo); full.push_back(0); break;
+            case RESID: P->desc_reduced(pd, s.g, P64, MODE_IMA, s.z, s.t); prec.push_back(P64); full.push_back(0); break;
+            case RECON: P->desc_reconstruct(pd, s.g, s.z); prec.push_back(P64); full.push_back(1); break;
+            default: continue;
+        }
+        who.push_back(i);
+        pds.push_back(pd);
+    }
+    if (who.empty()) {
+        flush();  // vector operations recorded for problems that have just finished
+        RT_CHECK(hipStreamSynchronize(st));
+        collect_times();
+        return false;
+    }
+    ++rounds;
+    launch_passes(who, pds, prec, full);
+    // ---- Krylov work after the pass: norms of c and r; CGS2 for the Arnoldi steps
+    std::vector<VecItem> v1, v4;
+    std::vector<GsItem> gn, g1, g2, g3;
+    for (int i : who) {
+        PS& s = ps[i];
+        double* slots = d_slots + (size_t)SLOTS * i;
+        const long S = (long)s.P->S_r;
+        const int bpi = gs_blocks(S);
+        auto gs = [&](double* y, int k, const double* coef, double* out) {
+            GsItem it{};
+            it.V = (const double* const*)s.P->d_Vptr; it.coef = coef; it.y = y; it.out = out;
+            it.n = S; it.k = k; it.bpi = bpi;
+            return it;
+        };
+        if (s.phase == CPASS) gn.push_back(gs(s.c, 0, nullptr, slots + S_NRM));
+        if (s.phase == RESID) {
+            VecItem it{};
+            it.op = VOP_SUB; it.n = S; it.y = s.r; it.x = s.c; it.x2 = s.t;
+            v1.push_back(it);
+            gn.push_back(gs(s.r, 0, nullptr, slots + S_NRM));
+        }
+        if (s.phase == ARNOLDI) {
+            const int k = s.j + 1;
+            g1.push_back(gs(s.w, k, nullptr, slots + S_H1));        // h1 = V^T w
+            if (reorth) {
+                g2.push_back(gs(s.w, k, slots + S_H1, slots + S_H2));   // w -= V h1 ; h2 = V^T w
+                g3.push_back(gs(s.w, k, slots + S_H2, slots + S_WN));   // w -= V h2 ; |w|^2
+            } else {
+                g3.push_back(gs(s.w, k, slots + S_H1, slots + S_WN));   // w -= V h1 ; |w|^2
+            }
+            VecItem nm{};
+            nm.op = VOP_NORMALIZE; nm.n = S; nm.y = s.V[s.j + 1]; nm.x = s.w; nm.slot = slots + S_WN;
+            v4.push_back(nm);
+        }
+    }
+    run_vecops(v1);
+    run_gs(gn, false, true);
+    run_gs(g1, false, false);
+    run_gs(g2, true, false);
+    run_gs(g3, true, true);
+    run_vecops(v4);
+    const auto t1 = clk::now();
+    flush();
+    RT_CHECK(hipMemcpyAsync(h_slots, d_slots, sizeof(double) * SLOTS * n, hipMemcpyDeviceToHost, st));
+    const auto t2 = clk::now();
+    RT_CHECK(hipStreamSynchronize(st));
+    const auto t3 = clk::now();
+    collect_times();
+    // ---- host state machines
+    std::vector<double> coefs;
+    std::vector<VecItem> adds;
+    std::vector<int> add_who;
+    for (int i : who) advance(i, pre, adds, add_who, coefs);
+    const auto t4 = clk::now();
+    host_plan += us(t0, t1);
+    host_submit += us(t1, t2);
+    host_wait += us(t2, t3);
+    host_advance += us(t3, t4);
+    // A problem that has just finished may have a flux fill pending (a group without source):
+    // the pending operations run now, so that its flux is final when round() returns.
+    bool fin = false;
+    for (int i : who) fin = fin || ps[i].phase == DONE;
+    if (!adds.empty() || (fin && !pre.empty())) {
+        // z += sum y V with the coefficients in the staging arena (free again after the
+        // synchronisation above)
+        arena_off = 0;
+        if (!adds.empty()) {
+            const double* dc = stage(coefs.data(), coefs.size());
+            for (size_t a = 0; a < adds.size(); ++a) adds[a].coef = dc + 64 * a;
+            run_vecops(adds);
+        }
+        if (fin) {
+            run_vecops(pre);
+            pre.clear();
+        }
+        flush();
+        RT_CHECK(hipStreamSynchronize(st));
+    }
+    return true;
+}
+
+void BatchSolver::run(rt_solve_info* info) {
     hipEvent_t e0, e1;
     RT_CHECK(hipEventCreate(&e0));
     RT_CHECK(hipEventCreate(&e1));
-    for (auto& s : ps) {
-        s.P->counters = SolveCounters{};
-        k_fill(s.P->d_phi, 0.0, (size_t)s.P->G * s.P->Np * s.P->Nc, st);
-        s.phase = START;
-        s.g = 0;
-    }
+    for (int i = 0; i < (int)ps.size(); ++i)
+        if (ps[i].P) start(i);
     {
         const char* e = std::getenv("RADIANT_HIP_PROFILE");
         profiling = e && e[0] == '1';
     }
     timing = Timing{};
     RT_CHECK(hipEventRecord(e0, st));
-    std::vector<VecItem> pre;  // vector ops to run before the next round's passes
+    pre.clear();
     rounds = 0;
-    // host-side time per phase of a round (RADIANT_HIP_HOSTPROF=1 prints the totals)
-    using clk = std::chrono::steady_clock;
-    auto us = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::micro>(b - a).count(); };
-    double host_plan = 0, host_submit = 0, host_wait = 0, host_advance = 0;
-    while (true) {
-        // ---- group starts: out-of-group + volume source (sn_flux.jl:175-179)
-        for (auto& s : ps) {
-            if (s.phase != START) continue;
-            Problem* P = s.P;
-            launch_outsrc(P->d_qlout, P->d_phi, P->d_sigo, P->d_qvol, P->d_mat, P->d_act, s.g, P->G, P->Nmat, P->Nact,
-                          P->Np, P->Nc, st);
-            s.phase = CPASS;
-            s.niter = 0;
-            s.refs = 0;
-            s.p0 = P->counters.passes;
-            s.l0 = P->counters.passes_lo;
-        }
-        const auto t0 = clk::now();
-        arena_off = 0;
-        run_vecops(pre);
-        pre.clear();
-        // ---- one pass per active problem
-        std::vector<int> who, prec, full;
-        std::vector<PassDesc> pds;
-        for (int i = 0; i < n; ++i) {
-            PS& s = ps[i];
-            if (s.phase == DONE) continue;
-            Problem* P = s.P;
-            PassDesc pd;
-            switch (s.phase) {
-                case CPASS: P->desc_reduced(pd, s.g, P64, MODE_T, nullptr, s.c); prec.push_back(P64); full.push_back(0); break;
-                case ARNOLDI: P->desc_reduced(pd, s.g, s.lo, MODE_IMA, s.V[s.j], s.w); prec.push_back(s.lo); full.push_back(0); break;
-                case RESID: P->desc_reduced(pd, s.g, P64, MODE_IMA, s.z, s.t); prec.push_back(P64); full.push_back(0); break;
-                case RECON: P->desc_reconstruct(pd, s.g, s.z); prec.push_back(P64); full.push_back(1); break;
-                default: continue;
-            }
-            who.push_back(i);
-            pds.push_back(pd);
-        }
-        if (who.empty()) {
-            flush();  // vector operations recorded for problems that have just finished
-            RT_CHECK(hipStreamSynchronize(st));
-            collect_times();
-            break;
-        }
-        ++rounds;
-        launch_passes(who, pds, prec, full);
-        // ---- Krylov work after the pass: norms of c and r; CGS2 for the Arnoldi steps
-        std::vector<VecItem> v1, v4;
-        std::vector<GsItem> gn, g1, g2, g3;
-        for (int i : who) {
-            PS& s = ps[i];
-            double* slots = d_slots + (size_t)SLOTS * i;
-            const long S = (long)s.P->S_r;
-            const int bpi = gs_blocks(S);
-            auto gs = [&](double* y, int k, const double* coef, double* out) {
-                GsItem it{};
-                it.V = (const double* const*)s.P->d_Vptr; it.coef = coef; it.y = y; it.out = out;
-                it.n = S; it.k = k; it.bpi = bpi;
-                return it;
-            };
-            if (s.phase == CPASS) gn.push_back(gs(s.c, 0, nullptr, slots + S_NRM));
-            if (s.phase == RESID) {
-                VecItem it{};
-                it.op = VOP_SUB; it.n = S; it.y = s.r; it.x = s.c; it.x2 = s.t;
-                v1.push_back(it);
-                gn.push_back(gs(s.r, 0, nullptr, slots + S_NRM));
-            }
-            if (s.phase == ARNOLDI) {
-                const int k = s.j + 1;
-                g1.push_back(gs(s.w, k, nullptr, slots + S_H1));        // h1 = V^T w
-                if (reorth) {
-                    g2.push_back(gs(s.w, k, slots + S_H1, slots + S_H2));   // w -= V h1 ; h2 = V^T w
-                    g3.push_back(gs(s.w, k, slots + S_H2, slots + S_WN));   // w -= V h2 ; |w|^2
-                } else {
-                    g3.push_back(gs(s.w, k, slots + S_H1, slots + S_WN));   // w -= V h1 ; |w|^2
-                }
-                VecItem nm{};
-                nm.op = VOP_NORMALIZE; nm.n = S; nm.y = s.V[s.j + 1]; nm.x = s.w; nm.slot = slots + S_WN;
-                v4.push_back(nm);
-            }
-        }
-        run_vecops(v1);
-        run_gs(gn, false, true);
-        run_gs(g1, false, false);
-        run_gs(g2, true, false);
-        run_gs(g3, true, true);
-        run_vecops(v4);
-        const auto t1 = clk::now();
-        flush();
-        RT_CHECK(hipMemcpyAsync(h_slots, d_slots, sizeof(double) * SLOTS * n, hipMemcpyDeviceToHost, st));
-        const auto t2 = clk::now();
-        RT_CHECK(hipStreamSynchronize(st));
-        const auto t3 = clk::now();
-        collect_times();
-        // ---- host state machines
-        std::vector<double> coefs;
-        std::vector<VecItem> adds;
-        std::vector<int> add_who;
-        for (int i : who) advance(i, pre, adds, add_who, coefs);
-        const auto t4 = clk::now();
-        host_plan += us(t0, t1);
-        host_submit += us(t1, t2);
-        host_wait += us(t2, t3);
-        host_advance += us(t3, t4);
-        if (!adds.empty()) {
-            // z += sum y V with the coefficients in the staging arena (free again after the
-            // synchronisation above)
-            arena_off = 0;
-            const double* dc = stage(coefs.data(), coefs.size());
-            for (size_t a = 0; a < adds.size(); ++a) adds[a].coef = dc + 64 * a;
-            run_vecops(adds);
-            flush();
-            RT_CHECK(hipStreamSynchronize(st));
-        }
+    host_plan = host_submit = host_wait = host_advance = 0;
+    while (round()) {
     }
     {
         const char* e = std::getenv("RADIANT_HIP_HOSTPROF");
@@ -448,6 +538,7 @@ void BatchSolver::run(rt_solve_info* info) {
         std::memset(info, 0, sizeof(*info));
         info->ms_device = ms;
         for (auto& s : ps) {
+            if (!s.P) continue;
             info->passes += s.P->counters.passes;
             info->passes_lo += s.P->counters.passes_lo;
             info->updates_raw += s.P->counters.updates_raw;

@@ -9,9 +9,24 @@ namespace rt {
 
 class BatchSolver {
 public:
+    // A fixed batch: slot i holds probs[i].
     BatchSolver(const std::vector<Problem*>& probs, hipStream_t stream);
+    // Empty slots for continuous batching (add / remove between rounds).
+    BatchSolver(int capacity, hipStream_t stream);
     ~BatchSolver();
+    // Solves every problem in the slots from group 0 to the end.
     void run(rt_solve_info* info);
+    // Continuous batching.  add() puts a problem into a free slot; it starts from group 0 at the
+    // next round.  round() advances every unfinished problem by one pass (plus its Krylov work)
+    // and returns false when none is left; the flux of a problem that is done() when round()
+    // returns is final.  A finished problem keeps its slot until remove(), which may also take an
+    // unfinished problem out (cancellation).  Problems in the slots are used on the solver's
+    // stream and must outlive their slot.
+    int add(Problem* P);
+    void remove(int slot);
+    bool done(int slot) const { return ps[slot].phase == DONE; }
+    bool round();
+    int capacity() const { return (int)ps.size(); }
     const std::vector<rt_group_stats>& stats(int i) const { return ps[i].stats; }
     int rounds = 0;
     // Per-kernel device time of the last run() when RADIANT_HIP_PROFILE=1 (HIP events on the stream
@@ -38,6 +53,11 @@ private:
     };
     std::vector<PS> ps;
     hipStream_t st;
+    std::vector<VecItem> pre;  // vector operations to run before the next round's passes
+    void init_device(int capacity);
+    void ensure_task_capacity();  // grows the task tables to the problems in the slots
+    void start(int slot);         // resets a slot's problem to group 0
+    void drain_pre();             // runs the vector operations recorded for the next round
     bool reorth = false;        // CGS2 instead of CGS
     bool fork_streams = false;  // FP64 passes on st64 beside the low-precision ones
     std::vector<void*> mem;
@@ -48,6 +68,7 @@ private:
     hipStream_t st64 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     size_t cap_gemm = 0, cap_sweep = 0;
+    void* task_mem[9] = {};
     double *d_slots = nullptr, *h_slots = nullptr, *d_partial = nullptr;
     // Staging arena: host (pinned) and device copies at equal offsets.  A round stages its
     // inputs and records its launches; flush() uploads the arena with one copy, then launches.
@@ -65,6 +86,7 @@ private:
                  std::vector<double>& coefs);
     void start_arnoldi(PS& s);
 
+    double host_plan = 0, host_submit = 0, host_wait = 0, host_advance = 0;  // RADIANT_HIP_HOSTPROF
     bool profiling = false;
     struct EvRec { hipEvent_t a, b; int kernel, prec; double upd, flop; };
     std::vector<hipEvent_t> ev_pool;

@@ -1,9 +1,13 @@
 // serve.cpp -- batched solves and the request server (continuous batching under an HBM budget).
 //
-// Every problem keeps its own device-resident state (fluxes, Krylov basis) and its own HIP
-// stream; problems that are resident at the same time run their kernels concurrently on the
-// device.  The server admits queued requests whenever the bytes they need fit the budget and a
-// slot is free, so a request that arrives while others run starts as soon as memory allows.
+// Every problem keeps its own device-resident state (fluxes, Krylov basis).  The server admits
+// queued requests in arrival order whenever the bytes they need fit the budget and a slot is
+// free, so a request that arrives while others run starts as soon as memory allows.  In the
+// mixed-precision modes admitted requests join the slots of one lockstep BatchSolver at the next
+// round boundary (continuous batching: each round launches one pass per resident problem, and
+// a finished problem leaves its slot at the end of the round that finished it).  FP64 requests,
+// and problems the row-scan sweep cannot take, are solved on their own streams by worker
+// threads, concurrently on the device.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -315,15 +319,40 @@ struct rt_server {
     std::map<int64_t, std::shared_ptr<Request>> reqs;
     int64_t next_id = 0;
     int64_t bytes_in_use = 0;
-    int active = 0;
+    int active = 0;    // admitted requests (lockstep slots and workers)
+    int in_batch = 0;  // of which in lockstep slots (scheduler thread only)
     bool stop = false;
     std::thread sched;
-    std::map<int64_t, std::thread> workers;  // running requests
+    std::map<int64_t, std::thread> workers;  // requests solved on their own stream
     std::vector<int64_t> finished;           // workers to join
 
+    // Lockstep engine of the mixed-precision servers: one BatchSolver whose slots take admitted
+    // requests at round boundaries.
+    hipStream_t est = nullptr;
+    std::unique_ptr<rt::BatchSolver> bs;
+    struct Slot {
+        std::shared_ptr<Request> r;
+        std::unique_ptr<rt::Problem> P;
+    };
+    std::vector<Slot> slots;
+
+    // Publishes a request's outcome (state, results and completion time together) and releases
+    // its admission; called with mu held.
+    void publish(Request& r, Request::State state, std::vector<double>& phi, std::vector<rt_group_stats>& st,
+                 const std::string& err, int code) {
+        r.phi.swap(phi);
+        r.stats.swap(st);
+        r.err = err;
+        r.code = code;
+        r.t_done = std::chrono::steady_clock::now();
+        r.state = state;
+        r.desc.reset();
+        bytes_in_use -= r.bytes;
+        --active;
+    }
+
     void run_request(std::shared_ptr<Request> r) {
-        // The outcome is published in one critical section (state, results and completion time
-        // together), after the device memory of the solve has been released.
+        // The outcome is published after the device memory of the solve has been released.
         Request::State state = Request::CANCELLED;
         std::vector<double> phi;
         std::vector<rt_group_stats> st;
@@ -345,15 +374,7 @@ struct rt_server {
         }
         {
             std::lock_guard<std::mutex> lk(mu);
-            r->phi.swap(phi);
-            r->stats.swap(st);
-            r->err = err;
-            r->code = code;
-            r->t_done = std::chrono::steady_clock::now();
-            r->state = state;
-            r->desc.reset();
-            bytes_in_use -= r->bytes;
-            --active;
+            publish(*r, state, phi, st, err, code);
             finished.push_back(r->id);
         }
         cv.notify_all();
@@ -375,45 +396,194 @@ struct rt_server {
         lk.lock();
     }
 
+    bool head_ready() {
+        if (queue.empty()) return false;
+        auto& r = reqs[queue.front()];
+        return r->cancel.load() || (active < cfg.max_active && bytes_in_use + r->bytes <= cfg.hbm_budget_bytes) ||
+               active == 0;
+    }
+
+    // Takes queued requests in arrival order while the head request fits the budget and a slot;
+    // called with mu held.  Returns the requests admitted to the lockstep engine.
+    std::vector<std::shared_ptr<Request>> admit() {
+        std::vector<std::shared_ptr<Request>> out;
+        while (!queue.empty()) {
+            auto r = reqs[queue.front()];
+            if (r->cancel.load()) {
+                queue.pop_front();
+                r->state = Request::CANCELLED;
+                r->t_done = std::chrono::steady_clock::now();
+                r->desc.reset();
+                continue;
+            }
+            const bool fits = active < cfg.max_active && bytes_in_use + r->bytes <= cfg.hbm_budget_bytes;
+            if (!fits && active > 0) break;
+            if (!fits && r->bytes > cfg.hbm_budget_bytes) {
+                queue.pop_front();
+                r->state = Request::FAILED;
+                r->code = RT_ENOMEM;
+                r->err = "request needs " + std::to_string(r->bytes) + " device bytes, above the server budget";
+                r->t_done = std::chrono::steady_clock::now();
+                r->desc.reset();
+                continue;
+            }
+            queue.pop_front();
+            r->state = Request::RUNNING;
+            bytes_in_use += r->bytes;
+            ++active;
+            if (bs) out.push_back(r);
+            else workers.emplace(r->id, std::thread([this, r] { run_request(r); }));
+        }
+        return out;
+    }
+
+    // Puts admitted requests into free slots of the lockstep engine (mu not held).  A problem the
+    // row-scan sweep cannot take is solved on its own stream instead.
+    void place(std::vector<std::shared_ptr<Request>>& rs) {
+        for (auto& r : rs) {
+            std::unique_ptr<rt::Problem> P;
+            std::string err;
+            int code = 0;
+            bool own = false;
+            try {
+                P.reset(new rt::Problem(r->desc->d, est));
+                try {
+                    const int k = bs->add(P.get());
+                    slots[k].r = r;
+                    slots[k].P = std::move(P);
+                    ++in_batch;
+                    continue;
+                } catch (const rt::Error& e) {
+                    if (e.code != RT_EINVAL) throw;
+                    own = true;
+                }
+            } catch (const rt::Error& e) {
+                err = e.what(); code = e.code;
+            } catch (const std::exception& e) {
+                err = e.what(); code = RT_EDEVICE;
+            }
+            P.reset();
+            std::lock_guard<std::mutex> lk(mu);
+            if (own) {
+                workers.emplace(r->id, std::thread([this, r] { run_request(r); }));
+            } else {
+                std::vector<double> phi;
+                std::vector<rt_group_stats> st;
+                publish(*r, Request::FAILED, phi, st, err, code);
+            }
+        }
+    }
+
+    // One lockstep round, then the finished and cancelled requests leave their slots (mu not
+    // held).  A device error fails every request in the slots and restarts the engine.
+    void step() {
+        struct Out {
+            std::shared_ptr<Request> r;
+            Request::State state;
+            std::vector<double> phi;
+            std::vector<rt_group_stats> st;
+            std::string err;
+            int code;
+        };
+        std::vector<Out> outs;
+        std::string err;
+        int code = 0;
+        try {
+            bs->round();
+            for (int k = 0; k < (int)slots.size(); ++k) {
+                Slot& sl = slots[k];
+                if (!sl.r) continue;
+                const bool cancelled = sl.r->cancel.load();
+                if (!cancelled && !bs->done(k)) continue;
+                Out o{sl.r, Request::CANCELLED, {}, {}, {}, 0};
+                if (!cancelled) {
+                    o.state = Request::DONE;
+                    o.st = bs->stats(k);
+                    o.phi.resize((size_t)sl.P->G * sl.P->Np * sl.P->Nc);
+                    sl.P->get_flux(o.phi.data());
+                }
+                bs->remove(k);
+                sl.P.reset();
+                sl.r.reset();
+                --in_batch;
+                outs.push_back(std::move(o));
+            }
+        } catch (const rt::Error& e) {
+            err = e.what(); code = e.code;
+        } catch (const std::exception& e) {
+            err = e.what(); code = RT_EDEVICE;
+        }
+        if (!err.empty()) {
+            for (auto& sl : slots) {
+                if (!sl.r) continue;
+                outs.push_back(Out{sl.r, Request::FAILED, {}, {}, err, code});
+                sl.r.reset();
+            }
+            in_batch = 0;
+            restart_engine();
+        }
+        if (outs.empty()) return;
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            for (auto& o : outs) publish(*o.r, o.state, o.phi, o.st, o.err, o.code);
+        }
+        cv.notify_all();
+    }
+
+    void restart_engine() {
+        try {
+            if (est) (void)hipStreamSynchronize(est);
+            bs.reset();
+            for (auto& sl : slots) sl.P.reset();
+            bs.reset(new rt::BatchSolver(cfg.max_active, est));
+        } catch (const std::exception&) {
+            bs.reset();  // later requests are solved on their own streams
+        }
+    }
+
     void scheduler() {
+        if (cfg.precision != RT_PREC_FP64) {
+            try {
+                RT_CHECK(hipStreamCreateWithFlags(&est, hipStreamNonBlocking));
+                slots.resize(cfg.max_active);
+                bs.reset(new rt::BatchSolver(cfg.max_active, est));
+            } catch (const std::exception&) {
+                bs.reset();
+            }
+        }
         std::unique_lock<std::mutex> lk(mu);
         while (true) {
-            cv.wait(lk, [&] {
-                if (stop || !finished.empty()) return true;
-                if (queue.empty()) return false;
-                auto& r = reqs[queue.front()];
-                return r->cancel.load() || (active < cfg.max_active && bytes_in_use + r->bytes <= cfg.hbm_budget_bytes) ||
-                       active == 0;
-            });
+            cv.wait(lk, [&] { return stop || !finished.empty() || in_batch > 0 || head_ready(); });
             if (!finished.empty()) reap(lk);
             if (stop) break;
-            // admit in arrival order while the head request fits
-            while (!queue.empty()) {
-                auto r = reqs[queue.front()];
-                if (r->cancel.load()) {
-                    queue.pop_front();
-                    r->state = Request::CANCELLED;
-                    r->t_done = std::chrono::steady_clock::now();
-                    r->desc.reset();
-                    continue;
-                }
-                const bool fits = active < cfg.max_active && bytes_in_use + r->bytes <= cfg.hbm_budget_bytes;
-                if (!fits && active > 0) break;
-                if (!fits && r->bytes > cfg.hbm_budget_bytes) {
-                    queue.pop_front();
-                    r->state = Request::FAILED;
-                    r->code = RT_ENOMEM;
-                    r->err = "request needs " + std::to_string(r->bytes) + " device bytes, above the server budget";
-                    r->t_done = std::chrono::steady_clock::now();
-                    r->desc.reset();
-                    continue;
-                }
-                queue.pop_front();
-                r->state = Request::RUNNING;
-                bytes_in_use += r->bytes;
-                ++active;
-                workers.emplace(r->id, std::thread([this, r] { run_request(r); }));
+            auto adm = admit();
+            lk.unlock();
+            if (!adm.empty()) place(adm);
+            if (in_batch > 0) step();
+            lk.lock();
+        }
+        // shutdown: requests still in the slots end as cancelled
+        lk.unlock();
+        std::vector<std::shared_ptr<Request>> left;
+        for (int k = 0; k < (int)slots.size(); ++k) {
+            if (!slots[k].r) continue;
+            left.push_back(slots[k].r);
+            try {
+                if (bs) bs->remove(k);
+            } catch (const std::exception&) {
             }
+            slots[k].P.reset();
+            slots[k].r.reset();
+        }
+        in_batch = 0;
+        bs.reset();
+        if (est) (void)hipStreamDestroy(est);
+        est = nullptr;
+        lk.lock();
+        for (auto& r : left) {
+            std::vector<double> phi;
+            std::vector<rt_group_stats> st;
+            publish(*r, Request::CANCELLED, phi, st, std::string(), 0);
         }
     }
 };

@@ -69,8 +69,76 @@ def test_server_continuous_batching():
             phi, st, lat = res[rid]
             ref, _ = S.compute_flux(p)
             e = rel_err(phi, ref)
-            _say("  request", rid, "latency ms", round(lat, 2), "err", e)
+            alone, _, _ = S.solve_batch([p], precision="mixed-fp32")
+            d = float(np.abs(alone[0] - phi).max())
+            _say("  request", rid, "latency ms", round(lat, 2), "err", e, "alone vs served", d)
             assert e < 1e-6 and lat > 0
+            assert d == 0.0  # the same alone and in the served mix
+    finally:
+        srv.close()
+
+
+def test_server_cancel_running_then_serve():
+    """A request cancelled while it runs leaves its slot; the engine goes on serving."""
+    from radiant_hip import solver as S, _lib
+    from radiant_hip import problems as P
+    srv = S.Server(hbm_budget_bytes=2 * 1024 ** 3, max_active=2, precision="mixed-fp32")
+    try:
+        big = srv.submit(P.config_2d(N=64, G=8, L=3, quad_order=8, seed=21))
+        time.sleep(0.05)
+        srv.cancel(big)
+        t0 = time.time()
+        while True:
+            try:
+                r = srv.poll(big)
+                if r is not None:
+                    _say("  cancelled request finished first")
+                    break
+            except _lib.RTError as e:
+                assert e.code == _lib.RT_ECANCELLED
+                _say("  cancelled")
+                break
+            assert time.time() - t0 < 60
+            time.sleep(0.005)
+        p = P.config_2d(N=24, G=3, L=2, quad_order=4, seed=22)
+        rid = srv.submit(p)
+        t0 = time.time()
+        r = None
+        while r is None:
+            assert time.time() - t0 < 60
+            r = srv.poll(rid)
+            time.sleep(0.005)
+        ref, _ = S.compute_flux(p)
+        e = rel_err(r[0], ref)
+        _say("  after cancel: err", e)
+        assert e < 1e-6
+    finally:
+        srv.close()
+
+
+def test_server_fp64_workers():
+    """FP64 servers solve each request on its own stream: results equal the one-shot solve."""
+    from radiant_hip import solver as S
+    from radiant_hip import problems as P
+    probs = [P.config_2d(N=24, G=3, L=2, quad_order=4, seed=31), P.small_problem(1, N=30, G=2, L=3, tol=1e-10)]
+    srv = S.Server(hbm_budget_bytes=2 * 1024 ** 3, max_active=2, precision="fp64")
+    try:
+        ids = [srv.submit(p) for p in probs]
+        res = {}
+        t0 = time.time()
+        while len(res) < len(ids):
+            assert time.time() - t0 < 60
+            for rid in ids:
+                if rid not in res:
+                    r = srv.poll(rid)
+                    if r is not None:
+                        res[rid] = r
+            time.sleep(0.005)
+        for rid, p in zip(ids, probs):
+            ref, _ = S.compute_flux(p)
+            e = rel_err(res[rid][0], ref)
+            _say("  fp64 request", rid, "err", e)
+            assert e < 1e-10
     finally:
         srv.close()
 
