@@ -177,7 +177,10 @@ int rt_server_submit(rt_server *s, const rt_problem_desc *p, int64_t *req);
 /* RT_EPENDING while queued/running; otherwise the final status: RT_OK with the results,
  * RT_ECANCELLED, or the error of a failed request.  The poll that reports a final status
  * releases the request id, except a finished request polled with phi_out == NULL (a status
- * query that fills stats_out and latency_ms and keeps the flux for a later poll). */
+ * query that fills stats_out and latency_ms and keeps the flux for a later poll).
+ * A finished request keeps its flux moments in device memory, counted against the server's
+ * budget, until a poll with phi_out collects them (copied out on a stream of that call).
+ * latency_ms is submit -> flux final in device memory. */
 int rt_server_poll(rt_server *s, int64_t req, double *phi_out, rt_group_stats *stats_out,
                    double *latency_ms);
 int rt_server_cancel(rt_server *s, int64_t req);

@@ -274,7 +274,7 @@ Problem::~Problem() {
 
 int64_t Problem::device_bytes(const rt_problem_desc& d) {
     const int64_t Nc = (int64_t)d.Nx * d.Ny * d.Nz;
-    const int Nact_est = d.Np;  // upper bound without scanning the XS
+    const int Nact_est = (d.sigs && d.pl && d.Ls >= 0) ? (int)active_moments(d).size() : d.Np;
     int64_t b = 0;
     b += Nc * 4 + (int64_t)d.G * Nc * 8;                        // mat, qvol
     b += (int64_t)d.G * d.Np * Nc * 8;                          // fluxes
@@ -291,6 +291,7 @@ int64_t Problem::device_bytes(const rt_problem_desc& d) {
     if (d.accel == RT_ACCEL_ANDERSON) nvec = 2 * d.depth + 8;
     if (d.precision != RT_PREC_FP64) nvec += 6;
     b += (int64_t)nvec * S * 8;
+    b += (int64_t)std::min<int64_t>((int64_t)d.G * d.Np * Nc, std::max<int64_t>(FLUX_STAGE, (int64_t)d.G * d.Np)) * 8;
     return b;
 }
 
@@ -890,12 +891,20 @@ void Problem::solve(rt_group_stats* stats, rt_solve_info* info) {
     }
 }
 
-void Problem::get_flux(double* phi_out) {
-    const size_t tot = (size_t)G * Np * Nc;
-    double* tmp = vec("flux_out", tot);
-    k_phi_to_radiant(tmp, d_phi, G, Np, Nc, st);
-    RT_CHECK(hipMemcpyAsync(phi_out, tmp, tot * sizeof(double), hipMemcpyDeviceToHost, st));
-    RT_CHECK(hipStreamSynchronize(st));
+// Radiant's layout [G, Np, Nc] has the cells outermost, so a range of cells is one contiguous
+// piece of the output: the flux is reordered and copied out one range at a time through a
+// staging buffer of at most FLUX_STAGE doubles.
+void Problem::get_flux(double* phi_out, hipStream_t s) {
+    if (!s) s = st;
+    const size_t per_cell = (size_t)G * Np;
+    const size_t cells = std::max<size_t>(1, std::min<size_t>(Nc, FLUX_STAGE / per_cell));
+    double* tmp = vec("flux_stage", cells * per_cell);
+    for (size_t c0 = 0; c0 < (size_t)Nc; c0 += cells) {
+        const size_t nc = std::min(cells, (size_t)Nc - c0);
+        k_phi_to_radiant(tmp, d_phi, G, Np, Nc, (int)c0, (int)nc, s);
+        RT_CHECK(hipMemcpyAsync(phi_out + c0 * per_cell, tmp, nc * per_cell * sizeof(double), hipMemcpyDeviceToHost, s));
+    }
+    RT_CHECK(hipStreamSynchronize(s));
 }
 
 void Problem::host_pass(int g, int mode, const double* qlout_host, const double* zin, double* zout) {

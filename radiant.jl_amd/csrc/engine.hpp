@@ -65,7 +65,9 @@ public:
     // Solve all groups (sn_flux.jl:150-224); fluxes stay on the device.
     void solve(rt_group_stats* stats, rt_solve_info* info);
     // Copy 𝚽l to the host in Radiant's layout.
-    void get_flux(double* phi_out);
+    // Flux moments in Radiant's layout into host memory, on stream s (default: the problem's).
+    void get_flux(double* phi_out, hipStream_t s = nullptr);
+    static constexpr size_t FLUX_STAGE = (size_t)8 << 20;  // doubles staged per copy-out step
     // Kernel-level hook: one faithful-state pass on host vectors (see rt_problem_pass).
     void host_pass(int g, int mode, const double* qlout_host, const double* zin, double* zout);
     int64_t faithful_state_length() const { return (int64_t)S_f; }

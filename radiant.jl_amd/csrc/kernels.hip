@@ -957,10 +957,11 @@ __global__ __launch_bounds__(64) void sweep_line_mixed_kernel(const PassDesc* __
 // y/z face fluxes of the previous diagonal live in LDS (FY[jz][x], FZ[jy][x]), one barrier per
 // diagonal.  After the last diagonal FY and FZ hold the exit y- and z-faces, which are written
 // out at the end with coalesced stores.
-template <typename T, int CPL, int U, bool DPP>
+template <typename T, int CPL, int U, int PF, bool DPP>
 __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, unsigned char* smem) {
     const int Nx = d.Nx, Ny = d.Ny, Nz = d.Nz, Nc = d.Nc, Nmat = d.Nmat;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, W = blockDim.x >> 6;
+    // the wave index is uniform: in a scalar register, the item cursors and barrier counts are too
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6), W = blockDim.x >> 6;
     T* hy2 = (T*)smem;
     T* hz2 = hy2 + Ny;
     T* st = hz2 + Nz;
@@ -991,19 +992,24 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
     T* __restrict__ bz = (T*)d.bz + (size_t)n * Nx * Ny;
     const int* __restrict__ mat = d.mat;
     const int ndiag = Ny + Nz - 1;
-    // Work items of this wave: (diagonal t, first pencil q0) for q0 = wave*U, wave*U + W*U, ...
-    // The angular source and material of the next item are loaded while the current one is solved.
+    // Work items of this wave, in order: (diagonal t, first pencil q) for q = wave*U, wave*U + W*U,
+    // ... below the diagonal's pencil count, diagonal by diagonal (diagonals where the wave has no
+    // pencil are skipped).  The angular source and material of the item PF places ahead are
+    // loaded while the current one is solved, into a ring of PF register slots.
     auto diag_lo = [&](int t) { return t - (Ny - 1) > 0 ? t - (Ny - 1) : 0; };
-    auto diag_cnt = [&](int t) { return (t < Nz - 1 ? t : Nz - 1) - diag_lo(t) + 1; };
-    auto next_item = [&](int t, int q0, int& tn, int& qn) {
-        if (q0 + W * U < diag_cnt(t)) { tn = t; qn = q0 + W * U; return; }
-        tn = t + 1;
-        while (tn < ndiag && wave * U >= diag_cnt(tn)) ++tn;
-        qn = wave * U;
+    auto diag_cnt = [&](int t) { return (t < Nz - 1 ? t : Nz - 1) - diag_lo(t) + 1; };  // <= 0 past the end
+    auto advance = [&](int& t, int& q) {
+        if (q + W * U < diag_cnt(t)) {
+            q += W * U;
+            return;
+        }
+        ++t;
+        while (t < ndiag && wave * U >= diag_cnt(t)) ++t;
+        q = wave * U;
     };
-    T vb[U][CPL];
-    int mb[U][CPL];
-    auto load_item = [&](int t, int q0) {
+    T vb[PF][U][CPL];
+    int mb[PF][U][CPL];
+    auto load_item = [&](T (&v)[U][CPL], int (&m)[U][CPL], int t, int q0) {
         const int tt = t < ndiag ? t : ndiag - 1;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -1014,16 +1020,19 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
             const int ro = Nx * (iy + Ny * iz);
 #pragma unroll
             for (int c = 0; c < CPL; ++c) {
-                vb[u][c] = gld(psi + ro + ixc[c]);
-                mb[u][c] = gld(mat + ro + ixc[c]);
+                v[u][c] = gld(psi + ro + ixc[c]);
+                m[u][c] = gld(mat + ro + ixc[c]);
             }
         }
     };
-    {
-        int t0 = 0;
-        while (t0 < ndiag && wave * U >= diag_cnt(t0)) ++t0;
-        __builtin_amdgcn_sched_barrier(0);
-        load_item(t0, wave * U);
+    int ct = 0, cq = wave * U;  // item being solved
+    while (ct < ndiag && wave * U >= diag_cnt(ct)) ++ct;
+    int lt = ct, lq = cq;       // next item to load
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int sl = 0; sl < PF; ++sl) {
+        load_item(vb[sl], mb[sl], lt, lq);
+        advance(lt, lq);
         // As many stores as an item issues after its prefetch, so that the loop is entered with
         // the same memory operations in flight as it repeats with (exact wait counts).
         __builtin_amdgcn_sched_barrier(0);
@@ -1031,19 +1040,27 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
         for (int k = 0; k < U * (CPL + 1); ++k) gst(trash_slot<T>(lane, k), (T)0);
         __builtin_amdgcn_sched_barrier(0);
     }
-    for (int t = 0; t < ndiag; ++t) {
-        const int jz_lo = diag_lo(t);
-        const int cnt = diag_cnt(t);
-        for (int q0 = wave * U; q0 < cnt; q0 += W * U) {
-            // U pencils of this diagonal, processed together for instruction-level parallelism
+    int bt = 0;  // barriers passed = diagonals completed by every wave
+    while (ct < ndiag) {
+#pragma unroll
+        for (int sl = 0; sl < PF; ++sl) {
+            // An item of diagonal t needs every item of diagonals < t done.  Items past the end
+            // (at most PF - 1 per wave) solve a clamped pencil and store only to trash.
+            const int tb = ct < ndiag ? ct : ndiag;
+            while (bt < tb) {
+                __syncthreads();
+                ++bt;
+            }
+            const int t = ct < ndiag ? ct : ndiag - 1;
+            const int cnt = diag_cnt(t), jz_lo = diag_lo(t);
             T fy[U][CPL], fz[U][CPL], c0[U][CPL], c1[U][CPL];
             int roff[U], fyo_i[U], fzo_i[U], iyv[U], izv[U];
             bool ok[U];
             T A[U], B[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int q = q0 + u;
-                ok[u] = q < cnt;
+                const int q = cq + u;
+                ok[u] = ct < ndiag && q < cnt;
                 const int jz = jz_lo + (ok[u] ? q : 0), jy = t - jz;
                 iyv[u] = fys ? jy : Ny - 1 - jy;
                 izv[u] = fzs ? jz : Nz - 1 - jz;
@@ -1058,9 +1075,9 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
                     const int jx = inx[c] ? lane * CPL + c : 0;
                     fy[u][c] = FY[fyo_i[u] + jx];
                     fz[u][c] = FZ[fzo_i[u] + jx];
-                    const T S = ((st[mb[u][c]] + hx[c]) + hy) + hz;
+                    const T S = ((st[mb[sl][u][c]] + hx[c]) + hy) + hz;
                     const T rS = dd_div((T)1, S);
-                    c0[u][c] = inx[c] ? ((vb[u][c] + hy * fy[u][c]) + hz * fz[u][c]) * rS : (T)0;
+                    c0[u][c] = inx[c] ? ((vb[sl][u][c] + hy * fy[u][c]) + hz * fz[u][c]) * rS : (T)0;
                     c1[u][c] = hx[c] * rS;
                     // compose this lane's maps (inactive cells: identity)
                     const T a = c0[u][c] + c0[u][c];
@@ -1069,12 +1086,16 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
                     B[u] = b * B[u];
                 }
             }
+            // The composed maps depend on every use of the slot's values: pinning them here keeps
+            // that arithmetic ahead of the slot's reload (instruction selection would otherwise
+            // place it after the loads, and the ring would rotate through copies that wait).
+#pragma unroll
+            for (int u = 0; u < U; ++u) asm volatile("" ::"v"(A[u]), "v"(B[u]));
             {
-                // the next item goes into the buffers just consumed (see sweep_line_kernel)
+                // the item PF ahead goes into the slot just consumed (see sweep_line_kernel)
                 __builtin_amdgcn_sched_barrier(0);
-                int tn, qn;
-                next_item(t, q0, tn, qn);
-                load_item(tn, qn);
+                load_item(vb[sl], mb[sl], lt, lq);
+                advance(lt, lq);
                 __builtin_amdgcn_sched_barrier(0);
             }
 #pragma unroll
@@ -1096,8 +1117,12 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
                 }
                 gst((ok[u] && last_lane) ? bx + iyv[u] + Ny * izv[u] : trash_slot<T>(lane, 8 + u), fx);
             }
+            advance(ct, cq);
         }
+    }
+    while (bt < ndiag) {
         __syncthreads();
+        ++bt;
     }
     // exit faces: FY[jz][jx] -> by[ix + Nx iz], FZ[jy][jx] -> bz[ix + Nx iy]
     for (int i = tid; i < Nz * Nx; i += blockDim.x) {
@@ -1112,21 +1137,21 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
     }
 }
 
-template <typename T, int CPL, int U, bool DPP>
-__global__ __launch_bounds__(1024) void sweep_scan_kernel(const PassDesc* __restrict__ pds, const SweepTask* __restrict__ tasks) {
+template <typename T, int CPL, int U, int PF, int MAXT, bool DPP>
+__global__ __launch_bounds__(MAXT) void sweep_scan_kernel(const PassDesc* __restrict__ pds, const SweepTask* __restrict__ tasks) {
     extern __shared__ __align__(16) unsigned char smem[];
     const SweepTask tk = tasks[blockIdx.x];
-    sweep_scan_body<T, CPL, U, DPP>(pds[tk.prob], tk.dir, smem);
+    sweep_scan_body<T, CPL, U, PF, DPP>(pds[tk.prob], tk.dir, smem);
 }
 
-template <int CPL, int U, bool DPP>
-__global__ __launch_bounds__(1024) void sweep_scan_mixed_kernel(const PassDesc* __restrict__ pds,
+template <int CPL, int U, int PF, int MAXT, bool DPP>
+__global__ __launch_bounds__(MAXT) void sweep_scan_mixed_kernel(const PassDesc* __restrict__ pds,
                                                                 const SweepTask* __restrict__ tasks) {
     extern __shared__ __align__(16) unsigned char smem[];
     const SweepTask tk = tasks[blockIdx.x];
     const PassDesc& d = pds[tk.prob];
-    if (d.prec == P64) sweep_scan_body<double, CPL, U, DPP>(d, tk.dir, smem);
-    else sweep_scan_body<float, CPL, U, DPP>(d, tk.dir, smem);
+    if (d.prec == P64) sweep_scan_body<double, CPL, U, PF, DPP>(d, tk.dir, smem);
+    else sweep_scan_body<float, CPL, U, PF, DPP>(d, tk.dir, smem);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1287,11 +1312,13 @@ __global__ void set_vals_kernel(Vals32 v, double* dst, int k) {
     if ((int)threadIdx.x < k) dst[threadIdx.x] = v.v[threadIdx.x];
 }
 // phi_all [g][p][c] -> Radiant 𝚽l[g + G*(p + Np*c)]
-__global__ void phi_to_radiant_kernel(double* __restrict__ dst, const double* __restrict__ src, int G, int Np, int Nc) {
-    const size_t tot = (size_t)G * Np * Nc;
+// dst[g + G (p + Np (c - c0))] = src[(g Np + p) Nc + c] for the cells c0 <= c < c0 + nc
+__global__ void phi_to_radiant_kernel(double* __restrict__ dst, const double* __restrict__ src, int G, int Np, int Nc,
+                                      int c0, int nc) {
+    const size_t tot = (size_t)G * Np * nc;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
         const size_t g = i % G, r = i / G;
-        const size_t p = r % Np, c = r / Np;
+        const size_t p = r % Np, c = c0 + r / Np;
         dst[i] = src[(g * Np + p) * Nc + c];
     }
 }
@@ -1489,91 +1516,91 @@ static bool scan_use_dpp() {
     return !(e && e[0] == 's');
 }
 
-template <typename T, int CPL, bool DPP>
-static void scan_inst2(const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int waves, size_t lds, hipStream_t st) {
-    if (waves == 0) {
-        // rows in flight: keeps the outstanding loads and stores of PD rows within vmcnt's 63
-        constexpr int PD = CPL <= 2 ? 8 : (CPL == 4 ? 4 : 2);
-        auto lfn = sweep_line_kernel<T, CPL, PD, DPP>;
-        static bool lattr_set = false;
-        if (!lattr_set) {
-            RT_HIP(hipFuncSetAttribute((const void*)lfn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-            lattr_set = true;
-        }
-        launch_timed(lfn, dim3(ntasks), dim3(64), lds, st, d_pd, tasks);
-        return;
+// 3D shapes (pencils per item U, prefetch depth PF, threads per workgroup at most MAXT): shape 0
+// (U 1, PF 4) is the default; RADIANT_HIP_SCAN3D=1 selects U 2 with a one-item prefetch, for
+// cross-checking.  Measured on 48^3..96^3 grids the two are within 10% of each other (the kernel
+// is bound by its instruction issue, not by the latency of its loads).
+static int scan3d_shape() {
+    const char* e = std::getenv("RADIANT_HIP_SCAN3D");
+    return e && e[0] == '1' ? 1 : 0;
+}
+
+template <int CPL, int U, int PF, int MAXT, bool DPP>
+static void scan3d_launch(int kind, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int waves, size_t lds,
+                          hipStream_t st) {
+    const void* fn = kind == PMIX ? (const void*)sweep_scan_mixed_kernel<CPL, U, PF, MAXT, DPP>
+                     : kind == P64 ? (const void*)sweep_scan_kernel<double, CPL, U, PF, MAXT, DPP>
+                                   : (const void*)sweep_scan_kernel<float, CPL, U, PF, MAXT, DPP>;
+    static bool attr_set[3] = {};
+    const int ai = kind == PMIX ? 0 : (kind == P64 ? 1 : 2);
+    if (!attr_set[ai]) {
+        RT_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr_set[ai] = true;
     }
-    constexpr int U = CPL >= 4 ? 1 : 2;
-    auto kfn = sweep_scan_kernel<T, CPL, U, DPP>;
-    static bool attr_set = false;
-    if (!attr_set) {
-        RT_HIP(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        attr_set = true;
-    }
-    launch_timed(kfn, dim3(ntasks), dim3(64 * waves), lds, st, d_pd, tasks);
+    waves = waves < MAXT / 64 ? waves : MAXT / 64;
+    if (kind == PMIX)
+        launch_timed(sweep_scan_mixed_kernel<CPL, U, PF, MAXT, DPP>, dim3(ntasks), dim3(64 * waves), lds, st, d_pd, tasks);
+    else if (kind == P64)
+        launch_timed(sweep_scan_kernel<double, CPL, U, PF, MAXT, DPP>, dim3(ntasks), dim3(64 * waves), lds, st, d_pd, tasks);
+    else
+        launch_timed(sweep_scan_kernel<float, CPL, U, PF, MAXT, DPP>, dim3(ntasks), dim3(64 * waves), lds, st, d_pd, tasks);
 }
 
 template <int CPL, bool DPP>
-static void scan_mixed_inst2(const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int waves, size_t lds,
-                             hipStream_t st) {
-    if (waves == 0) {
-        constexpr int PD = CPL <= 2 ? 8 : (CPL == 4 ? 4 : 2);
-        auto lfn = sweep_line_mixed_kernel<CPL, PD, DPP>;
-        static bool lattr_set = false;
-        if (!lattr_set) {
-            RT_HIP(hipFuncSetAttribute((const void*)lfn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-            lattr_set = true;
-        }
-        launch_timed(lfn, dim3(ntasks), dim3(64), lds, st, d_pd, tasks);
-        return;
+static void scan3d_inst(int kind, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int waves, size_t lds,
+                        hipStream_t st) {
+    constexpr int U2 = CPL >= 4 ? 1 : 2;
+    if constexpr (CPL >= 8 || !DPP) {  // 8 cells per lane: 512 threads, so that the registers do not spill
+        scan3d_launch<CPL, 1, 2, 512, DPP>(kind, d_pd, tasks, ntasks, waves, lds, st);
+    } else {
+        constexpr int PF = CPL >= 4 ? 2 : 4;  // 4 cells per lane: a deeper ring would spill at 1024 threads
+        if (scan3d_shape() == 1) scan3d_launch<CPL, U2, 1, 1024, DPP>(kind, d_pd, tasks, ntasks, waves, lds, st);
+        else scan3d_launch<CPL, 1, PF, 1024, DPP>(kind, d_pd, tasks, ntasks, waves, lds, st);
     }
-    constexpr int U = CPL >= 4 ? 1 : 2;
-    auto kfn = sweep_scan_mixed_kernel<CPL, U, DPP>;
-    static bool attr_set = false;
-    if (!attr_set) {
-        RT_HIP(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        attr_set = true;
-    }
-    launch_timed(kfn, dim3(ntasks), dim3(64 * waves), lds, st, d_pd, tasks);
 }
 
-template <typename T, int CPL>
-static void scan_inst(const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int waves, size_t lds, hipStream_t st) {
-    if (scan_use_dpp()) scan_inst2<T, CPL, true>(d_pd, tasks, ntasks, waves, lds, st);
-    else scan_inst2<T, CPL, false>(d_pd, tasks, ntasks, waves, lds, st);
+template <int CPL, bool DPP>
+static void line_inst(int kind, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, size_t lds, hipStream_t st) {
+    // rows in flight: keeps the outstanding loads and stores of PD rows within vmcnt's 63
+    constexpr int PD = CPL <= 2 ? 8 : (CPL == 4 ? 4 : 2);
+    const void* fn = kind == PMIX ? (const void*)sweep_line_mixed_kernel<CPL, PD, DPP>
+                     : kind == P64 ? (const void*)sweep_line_kernel<double, CPL, PD, DPP>
+                                   : (const void*)sweep_line_kernel<float, CPL, PD, DPP>;
+    static bool attr_set[3] = {};
+    const int ai = kind == PMIX ? 0 : (kind == P64 ? 1 : 2);
+    if (!attr_set[ai]) {
+        RT_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr_set[ai] = true;
+    }
+    if (kind == PMIX)
+        launch_timed(sweep_line_mixed_kernel<CPL, PD, DPP>, dim3(ntasks), dim3(64), lds, st, d_pd, tasks);
+    else if (kind == P64)
+        launch_timed(sweep_line_kernel<double, CPL, PD, DPP>, dim3(ntasks), dim3(64), lds, st, d_pd, tasks);
+    else
+        launch_timed(sweep_line_kernel<float, CPL, PD, DPP>, dim3(ntasks), dim3(64), lds, st, d_pd, tasks);
 }
 
 template <int CPL>
-static void scan_mixed_inst(const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int waves, size_t lds,
-                            hipStream_t st) {
-    if (scan_use_dpp()) scan_mixed_inst2<CPL, true>(d_pd, tasks, ntasks, waves, lds, st);
-    else scan_mixed_inst2<CPL, false>(d_pd, tasks, ntasks, waves, lds, st);
+static void scan_any(int kind, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int waves, size_t lds,
+                     hipStream_t st) {
+    if (scan_use_dpp()) {
+        if (waves == 0) line_inst<CPL, true>(kind, d_pd, tasks, ntasks, lds, st);
+        else scan3d_inst<CPL, true>(kind, d_pd, tasks, ntasks, waves, lds, st);
+    } else {
+        if (waves == 0) line_inst<CPL, false>(kind, d_pd, tasks, ntasks, lds, st);
+        else scan3d_inst<CPL, false>(kind, d_pd, tasks, ntasks, waves, lds, st);
+    }
 }
 
 void launch_sweep_scan(int prec, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int cpl, int waves, size_t lds,
                        hipStream_t st) {
     if (ntasks <= 0) return;
-    if (prec == PMIX) {
-        switch (cpl) {
-            case 1: scan_mixed_inst<1>(d_pd, tasks, ntasks, waves, lds, st); break;
-            case 2: scan_mixed_inst<2>(d_pd, tasks, ntasks, waves, lds, st); break;
-            case 4: scan_mixed_inst<4>(d_pd, tasks, ntasks, waves, lds, st); break;
-            default: scan_mixed_inst<8>(d_pd, tasks, ntasks, waves, lds, st); break;
-        }
-    } else if (prec == P64) {
-        switch (cpl) {
-            case 1: scan_inst<double, 1>(d_pd, tasks, ntasks, waves, lds, st); break;
-            case 2: scan_inst<double, 2>(d_pd, tasks, ntasks, waves, lds, st); break;
-            case 4: scan_inst<double, 4>(d_pd, tasks, ntasks, waves, lds, st); break;
-            default: scan_inst<double, 8>(d_pd, tasks, ntasks, waves, lds, st); break;
-        }
-    } else {
-        switch (cpl) {
-            case 1: scan_inst<float, 1>(d_pd, tasks, ntasks, waves, lds, st); break;
-            case 2: scan_inst<float, 2>(d_pd, tasks, ntasks, waves, lds, st); break;
-            case 4: scan_inst<float, 4>(d_pd, tasks, ntasks, waves, lds, st); break;
-            default: scan_inst<float, 8>(d_pd, tasks, ntasks, waves, lds, st); break;
-        }
+    const int kind = prec == PMIX ? PMIX : (prec == P64 ? P64 : P32);
+    switch (cpl) {
+        case 1: scan_any<1>(kind, d_pd, tasks, ntasks, waves, lds, st); break;
+        case 2: scan_any<2>(kind, d_pd, tasks, ntasks, waves, lds, st); break;
+        case 4: scan_any<4>(kind, d_pd, tasks, ntasks, waves, lds, st); break;
+        default: scan_any<8>(kind, d_pd, tasks, ntasks, waves, lds, st); break;
     }
     RT_HIP(hipGetLastError());
 }
@@ -1666,8 +1693,9 @@ void k_set_vals(const Vals32& v, double* dst, int k, hipStream_t st) {
     hipLaunchKernelGGL(set_vals_kernel, dim3(1), dim3(32), 0, st, v, dst, k);
     RT_HIP(hipGetLastError());
 }
-void k_phi_to_radiant(double* dst, const double* phi_all, int G, int Np, int Nc, hipStream_t st) {
-    hipLaunchKernelGGL(phi_to_radiant_kernel, dim3(grid_for((size_t)G * Np * Nc)), dim3(256), 0, st, dst, phi_all, G, Np, Nc);
+void k_phi_to_radiant(double* dst, const double* phi_all, int G, int Np, int Nc, int c0, int nc, hipStream_t st) {
+    hipLaunchKernelGGL(phi_to_radiant_kernel, dim3(grid_for((size_t)G * Np * nc)), dim3(256), 0, st, dst, phi_all, G, Np,
+                       Nc, c0, nc);
     RT_HIP(hipGetLastError());
 }
 void k_state_moments_in(double* dst, const double* src, int Np, int Nc, hipStream_t st) {

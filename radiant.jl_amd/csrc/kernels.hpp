@@ -147,7 +147,7 @@ void launch_gs(const GsItem* items, int nitems, int kmax, int bpi_max, bool upda
 void launch_vecops(const VecItem* items, int nitems, int blocks_per_item, hipStream_t st);
 
 // Layout converters of the host boundary (Radiant order <-> device order).
-void k_phi_to_radiant(double* dst, const double* phi_all, int G, int Np, int Nc, hipStream_t st);
+void k_phi_to_radiant(double* dst, const double* phi_all, int G, int Np, int Nc, int c0, int nc, hipStream_t st);
 void k_state_moments_in(double* dst, const double* src, int Np, int Nc, hipStream_t st);   // [c][p] -> [p][c]
 void k_state_moments_out(double* dst, const double* src, int Np, int Nc, hipStream_t st);  // [p][c] -> [c][p]
 

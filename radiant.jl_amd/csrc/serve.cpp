@@ -303,7 +303,7 @@ struct Request {
     std::unique_ptr<DescCopy> desc;
     int64_t bytes = 0;
     enum State { QUEUED, RUNNING, DONE, FAILED, CANCELLED } state = QUEUED;
-    std::vector<double> phi;
+    std::unique_ptr<rt::Problem> result;  // a finished solve: flux moments resident until collected
     std::vector<rt_group_stats> stats;
     std::string err;
     int code = 0;
@@ -337,34 +337,33 @@ struct rt_server {
     std::vector<Slot> slots;
 
     // Publishes a request's outcome (state, results and completion time together) and releases
-    // its admission; called with mu held.
-    void publish(Request& r, Request::State state, std::vector<double>& phi, std::vector<rt_group_stats>& st,
+    // its slot; called with mu held.  A finished solve keeps its device memory, and its bytes stay
+    // counted against the budget, until a poll collects the flux.
+    void publish(Request& r, Request::State state, std::unique_ptr<rt::Problem>& P, std::vector<rt_group_stats>& st,
                  const std::string& err, int code) {
-        r.phi.swap(phi);
+        if (state == Request::DONE) r.result = std::move(P);
+        else bytes_in_use -= r.bytes;
         r.stats.swap(st);
         r.err = err;
         r.code = code;
         r.t_done = std::chrono::steady_clock::now();
         r.state = state;
         r.desc.reset();
-        bytes_in_use -= r.bytes;
         --active;
     }
 
     void run_request(std::shared_ptr<Request> r) {
         // The outcome is published after the device memory of the solve has been released.
         Request::State state = Request::CANCELLED;
-        std::vector<double> phi;
+        std::unique_ptr<rt::Problem> prob;
         std::vector<rt_group_stats> st;
         std::string err;
         int code = 0;
         try {
             if (!r->cancel.load()) {
-                rt::Problem prob(r->desc->d);
+                prob.reset(new rt::Problem(r->desc->d));
                 st.resize(r->desc->d.G);
-                prob.solve(st.data(), nullptr);
-                phi.resize((size_t)prob.G * prob.Np * prob.Nc);
-                prob.get_flux(phi.data());
+                prob->solve(st.data(), nullptr);
                 state = r->cancel.load() ? Request::CANCELLED : Request::DONE;
             }
         } catch (const rt::Error& e) {
@@ -372,9 +371,10 @@ struct rt_server {
         } catch (const std::exception& e) {
             state = Request::FAILED; err = e.what(); code = RT_EDEVICE;
         }
+        if (state != Request::DONE) prob.reset();
         {
             std::lock_guard<std::mutex> lk(mu);
-            publish(*r, state, phi, st, err, code);
+            publish(*r, state, prob, st, err, code);
             finished.push_back(r->id);
         }
         cv.notify_all();
@@ -467,9 +467,9 @@ struct rt_server {
             if (own) {
                 workers.emplace(r->id, std::thread([this, r] { run_request(r); }));
             } else {
-                std::vector<double> phi;
+                std::unique_ptr<rt::Problem> none;
                 std::vector<rt_group_stats> st;
-                publish(*r, Request::FAILED, phi, st, err, code);
+                publish(*r, Request::FAILED, none, st, err, code);
             }
         }
     }
@@ -480,7 +480,7 @@ struct rt_server {
         struct Out {
             std::shared_ptr<Request> r;
             Request::State state;
-            std::vector<double> phi;
+            std::unique_ptr<rt::Problem> P;
             std::vector<rt_group_stats> st;
             std::string err;
             int code;
@@ -495,14 +495,13 @@ struct rt_server {
                 if (!sl.r) continue;
                 const bool cancelled = sl.r->cancel.load();
                 if (!cancelled && !bs->done(k)) continue;
-                Out o{sl.r, Request::CANCELLED, {}, {}, {}, 0};
+                Out o{sl.r, Request::CANCELLED, nullptr, {}, {}, 0};
                 if (!cancelled) {
                     o.state = Request::DONE;
                     o.st = bs->stats(k);
-                    o.phi.resize((size_t)sl.P->G * sl.P->Np * sl.P->Nc);
-                    sl.P->get_flux(o.phi.data());
                 }
                 bs->remove(k);
+                if (!cancelled) o.P = std::move(sl.P);  // the flux stays in device memory
                 sl.P.reset();
                 sl.r.reset();
                 --in_batch;
@@ -516,7 +515,7 @@ struct rt_server {
         if (!err.empty()) {
             for (auto& sl : slots) {
                 if (!sl.r) continue;
-                outs.push_back(Out{sl.r, Request::FAILED, {}, {}, err, code});
+                outs.push_back(Out{sl.r, Request::FAILED, nullptr, {}, err, code});
                 sl.r.reset();
             }
             in_batch = 0;
@@ -525,7 +524,7 @@ struct rt_server {
         if (outs.empty()) return;
         {
             std::lock_guard<std::mutex> lk(mu);
-            for (auto& o : outs) publish(*o.r, o.state, o.phi, o.st, o.err, o.code);
+            for (auto& o : outs) publish(*o.r, o.state, o.P, o.st, o.err, o.code);
         }
         cv.notify_all();
     }
@@ -581,9 +580,9 @@ struct rt_server {
         est = nullptr;
         lk.lock();
         for (auto& r : left) {
-            std::vector<double> phi;
+            std::unique_ptr<rt::Problem> none;
             std::vector<rt_group_stats> st;
-            publish(*r, Request::CANCELLED, phi, st, std::string(), 0);
+            publish(*r, Request::CANCELLED, none, st, std::string(), 0);
         }
     }
 };
@@ -634,29 +633,47 @@ int rt_server_poll(rt_server* s, int64_t req, double* phi_out, rt_group_stats* s
         rt::set_last_error("null server");
         return RT_EINVAL;
     }
-    std::lock_guard<std::mutex> lk(s->mu);
-    auto it = s->reqs.find(req);
-    if (it == s->reqs.end()) {
-        rt::set_last_error("unknown request id");
-        return RT_EINVAL;
+    std::shared_ptr<Request> r;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(req);
+        if (it == s->reqs.end()) {
+            rt::set_last_error("unknown request id");
+            return RT_EINVAL;
+        }
+        r = it->second;
+        if (r->state == Request::QUEUED || r->state == Request::RUNNING) return RT_EPENDING;
+        if (latency_ms) latency_ms[0] = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
+        if (stats_out && r->state == Request::DONE)
+            std::memcpy(stats_out, r->stats.data(), r->stats.size() * sizeof(rt_group_stats));
+        if (r->state == Request::DONE && !phi_out) return RT_OK;  // status query: the flux stays
+        s->reqs.erase(it);  // a final status is reported once; the id is released here
+        if (r->state == Request::CANCELLED) return RT_ECANCELLED;
+        if (r->state == Request::FAILED) {
+            rt::set_last_error(r->err);
+            return r->code ? r->code : RT_EDEVICE;
+        }
     }
-    std::shared_ptr<Request> r = it->second;
-    if (r->state == Request::QUEUED || r->state == Request::RUNNING) return RT_EPENDING;
-    if (latency_ms) latency_ms[0] = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();
-    if (r->state == Request::DONE && !phi_out) {
-        // status query: the result stays until a poll collects the flux
-        if (stats_out) std::memcpy(stats_out, r->stats.data(), r->stats.size() * sizeof(rt_group_stats));
-        return RT_OK;
+    // Collection: the flux is copied out of device memory on a stream of this call, outside the
+    // server's lock, then the request's memory returns to the budget.
+    int rc = guard([&] {
+        hipStream_t cs = nullptr;
+        RT_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+        try {
+            r->result->get_flux(phi_out, cs);
+        } catch (...) {
+            (void)hipStreamDestroy(cs);
+            throw;
+        }
+        RT_CHECK(hipStreamDestroy(cs));
+    });
+    r->result.reset();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->bytes_in_use -= r->bytes;
     }
-    s->reqs.erase(it);  // a final status is reported once; the id is released here
-    if (r->state == Request::CANCELLED) return RT_ECANCELLED;
-    if (r->state == Request::FAILED) {
-        rt::set_last_error(r->err);
-        return r->code ? r->code : RT_EDEVICE;
-    }
-    std::memcpy(phi_out, r->phi.data(), r->phi.size() * sizeof(double));
-    if (stats_out) std::memcpy(stats_out, r->stats.data(), r->stats.size() * sizeof(rt_group_stats));
-    return RT_OK;
+    s->cv.notify_all();
+    return rc;
 }
 
 int rt_server_cancel(rt_server* s, int64_t req) {

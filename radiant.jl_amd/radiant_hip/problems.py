@@ -227,27 +227,27 @@ def poisson_arrivals(n, rate=50.0, seed=20260106):
 
 
 def small_problem(ndims, N=6, G=3, L=2, quad_type=None, quad_order=None, seed=7, Nmat=2, accel="gmres",
-                  tol=1e-12, bc=None, source_group=0):
-    """Tiny seeded problem for unit tests (oracle runs in milliseconds)."""
+                  tol=1e-12, bc=None, source_group=0, dims=None):
+    """Tiny seeded problem for unit tests (oracle runs in milliseconds): N cells of width 1 per
+    axis, or dims = (Nx, Ny, Nz) cells for a non-cubic grid."""
     rng = SplitMix64(seed)
     if quad_type is None:
         quad_type = {1: "gauss-legendre", 2: "gauss-legendre-chebychev", 3: "carlson"}[ndims]
     if quad_order is None:
         quad_order = {1: 8, 2: 4, 3: 4}[ndims]
-    width = 1.0 * N
+    n_ax = [N] * ndims if dims is None else [int(v) for v in dims[:ndims]]
     axes = []
     ws = []
-    for _ in range(ndims):
-        x, d = _centres(N, width)
+    for a in range(ndims):
+        x, d = _centres(n_ax[a], 1.0 * n_ax[a])
         axes.append(x)
         ws.append(d)
     mats = [multigroup_xs(rng, G, L) for _ in range(Nmat)]
     sigt, sigs = _pack_xs(mats, G, L)
-    shape = [N] * ndims + [1] * (3 - ndims)
+    shape = n_ax + [1] * (3 - ndims)
     mat = (1 + np.minimum(np.floor(rng.uniform(int(np.prod(shape))) * Nmat), Nmat - 1)).astype(np.int64)
     mat = mat.reshape(shape, order="F")
-    lo, hi = 0.3 * width, 0.7 * width
-    q = _box_source(G, source_group, axes, [(lo, hi)] * ndims)
+    q = _box_source(G, source_group, axes, [(0.3 * n, 0.7 * n) for n in n_ax])
     return Problem(ndims=ndims, Nx=shape[0], Ny=shape[1], Nz=shape[2], dx=ws[0],
                    dy=ws[1] if ndims >= 2 else np.ones(1), dz=ws[2] if ndims >= 3 else np.ones(1),
                    mat=mat, Nmat=Nmat, G=G, sigt=sigt, sigs=sigs, Lxs=L, qvol=q,

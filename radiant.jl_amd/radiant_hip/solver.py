@@ -7,6 +7,7 @@ Radiant's layouts into an ``rt_problem_desc`` and calls ``rt_solve``; it returns
 per-group statistics that sn_one_speed.jl prints.
 """
 import ctypes as C
+import time
 
 import numpy as np
 
@@ -210,6 +211,7 @@ class Server:
         self.h = C.c_void_p()
         check(lib().rt_server_create(C.byref(cfg), C.byref(self.h)))
         self.descs = {}
+        self.last_collect_ms = 0.0
 
     def submit(self, prob, **kw):
         d = Desc(prob, self.precision, "reduced" if self.precision != "fp64" else "faithful", **kw)
@@ -219,7 +221,10 @@ class Server:
         return rid.value
 
     def poll(self, rid):
-        """Returns None while pending, else (phi, stats, latency_ms); raises on failure/cancel."""
+        """Returns None while pending, else (phi, stats, latency_ms); raises on failure/cancel.
+
+        latency_ms is submit -> flux final in device memory; the device -> host copy of the
+        collection that follows takes ``self.last_collect_ms``."""
         G, Np, Nc = self.descs[rid]
         st = (_lib.GroupStats * G)()
         lat = C.c_double()
@@ -229,8 +234,10 @@ class Server:
         if rc != _lib.RT_OK:
             del self.descs[rid]
             check(rc)
+        t0 = time.perf_counter()
         phi = np.empty(G * Np * Nc)
         rc = lib().rt_server_poll(self.h, rid, ptr(phi), st, C.byref(lat))  # collects and releases
+        self.last_collect_ms = (time.perf_counter() - t0) * 1e3
         del self.descs[rid]
         check(rc)
         return phi.reshape((G, Np, Nc), order="F"), [s.as_dict() for s in st], lat.value

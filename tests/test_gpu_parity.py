@@ -191,3 +191,27 @@ def test_scan_shuffle_variant_parity(oracle, ndims, monkeypatch):
     e = rel_err(phi, ref)
     _say("shuffle scan", ndims, e)
     assert e < 1e-10
+
+
+@pytest.mark.parametrize("shape", [0, 1])
+def test_scan3d_shapes_parity(oracle, shape, monkeypatch):
+    """Both 3D row-scan shapes (pencils per item, prefetch depth), selected by RADIANT_HIP_SCAN3D,
+    match the oracle: on 5^3, where some waves own no pencil of most diagonals, on 12^3, where a
+    wave's items wrap over several diagonals, and on long-x grids with 2, 4 and 8 cells per lane;
+    the mixed FP32 solve stays within 1e-6 of the oracle."""
+    monkeypatch.setenv("RADIANT_HIP_SCAN3D", str(shape))
+    P, S = _P(), _S()
+    probs = [P.small_problem(3, N=5, tol=1e-12), P.config_3d(N=12, width=1.2, G=2, quad_order=4, seed=17)]
+    # 2, 4 and 8 cells per lane along x
+    probs += [P.small_problem(3, dims=d, G=2, tol=1e-12, seed=s) for d, s in (((70, 4, 3), 21), ((130, 3, 4), 22),
+                                                                               ((300, 3, 3), 23))]
+    for prob in probs:
+        ref, _ = oracle.solve(prob)
+        phi, _ = S.compute_flux(prob)
+        e = rel_err(phi, ref)
+        _say("scan3d shape", shape, prob.Nx, e)
+        assert e < 1e-10
+        mx, _ = S.compute_flux(prob, precision="mixed-fp32", state="reduced")
+        e = rel_err(mx, ref)
+        _say("scan3d shape", shape, prob.Nx, "mixed-fp32", e)
+        assert e < 1e-6
