@@ -29,6 +29,13 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cell·dir·group updates/s + solves/s, p50/p99 latency, at ≤1e-6 rel flux err"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the dominant kernel of the default batched workload (32 problems),
+# from rocprofv3 --pmc runs over that workload: FETCH_SIZE and WRITE_SIZE (KiB per dispatch,
+# averaged over the kernel's dispatches), each scaled by what the counter reads for a plain 1 GiB
+# device copy (FETCH_SIZE reads 0.5 GiB for it: factor 2.000; WRITE_SIZE 1 GiB: factor 1.000).
+MEASURED_TRAFFIC = {("sweep_fp32", 32): dict(fetch=378.65e6, write=348.67e6,
+                                             kernel="sweep_line_mixed_kernel<2, 8, true>",
+                                             summary="profiles/sweep_line_mixed_c1_batch32.md")}
 MFMA_PEAK_TFLOPS = {0: 78.6, 1: 157.3, 2: 2500.0}
 KNAMES = ["mgemm", "sweep", "dgemm", "bndmom"]
 PNAMES = ["fp64", "fp32", "bf16"]
@@ -294,7 +301,20 @@ def main():
     ap.add_argument("--rate", type=float, default=50.0, help="served stream: Poisson arrivals per second")
     ap.add_argument("--budget-gib", type=float, default=200.0, help="served stream: server HBM budget")
     ap.add_argument("--max-active", type=int, default=8, help="served stream: resident requests at most")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run 3 device-to-device copies of BYTES (the reference for scaling the HBM "
+                         "byte counters FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_calibration > 0:
+        import torch  # device memory plumbing: a plain copy of known size
+        n = args.copy_calibration // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": 4 * n, "copies": 3}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only "
                                    "(independent problems shard with no exchange; one large problem would use "
@@ -384,6 +404,11 @@ def main():
                      ("mixed-fp32" if single is rmx else "fp64"))
         roof = kernel_roofline(prob, head, nact)
         roof64 = kernel_roofline(prob, r64, nact)
+    mt = MEASURED_TRAFFIC.get((roof.get("kernel"), args.batch)) if head is rbt else None
+    if mt is not None:
+        roof["traffic"] = mt["fetch"] + mt["write"]
+        roof["traffic_source"] = (f"rocprofv3 --pmc FETCH_SIZE x2.000 + WRITE_SIZE x1.000 per dispatch of "
+                                  f"{mt['kernel']} over this workload ({mt['summary']})")
 
     cpu = None
     if not args.no_cpu:
