@@ -72,9 +72,17 @@ static void validate(const rt_problem_desc& d) {
     if (d.maxit < 1) bad("maximum number of iterations must be >= 1");
     if (!(d.tol > 0)) bad("convergence criterion must be positive");
     if (!d.bc) bad("missing boundary conditions");
+    bool nonvoid = false;
     for (int f = 0; f < 2 * d.ndims; ++f) {
         if (d.bc[f] < 0 || d.bc[f] > 2) bad("Unknown boundary condition type.");
-        if (d.bc[f] != 0) bad("only void boundaries are supported by the GPU path (reflective/periodic: not yet)");
+        nonvoid = nonvoid || d.bc[f] != 0;
+    }
+    if (nonvoid) {
+        if (d.precision != RT_PREC_FP64 || d.state_mode != RT_STATE_FAITHFUL)
+            bad("reflective/periodic boundaries need the FP64 faithful state (RT_PREC_FP64, RT_STATE_FAITHFUL)");
+        for (int f = 0; f < 2 * d.ndims; ++f)
+            if (!d.Msurf[f] || !d.Dsurf[f] || !d.n_to_np[f])
+                bad("reflective/periodic boundaries need the half-range operators of every face");
     }
     const size_t nc = (size_t)d.Nx * d.Ny * d.Nz;
     for (size_t c = 0; c < nc; ++c)
@@ -195,6 +203,39 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
                 }
                 d_exlist[ax] = dalloc<int>(lst.size()); upload(d_exlist[ax], lst, st);
             }
+            // ---- reflective / periodic faces (sn_sweep_2D.jl:70-99 and the x/z analogues): the
+            // entry half-range M row of every direction and the face whose exit moments feed it
+            for (int f = 0; f < 2 * ndims; ++f) has_bc = has_bc || d.bc[f] != 0;
+            if (has_bc) {
+                const size_t npf[3] = {(size_t)Ny * Nz, (size_t)Nx * Nz, (size_t)Nx * Ny};
+                for (int ax = 0; ax < 3; ++ax) {
+                    std::vector<double> Me((size_t)Nd * std::max(Nps, 1), 0.0);
+                    std::vector<int> src(Nd, -1);
+                    if (ax < ndims) {
+                        const double* cos_ = ax == 0 ? d.mu : (ax == 1 ? d.eta : d.xi);
+                        const int fm = 2 * ax, fp = 2 * ax + 1;
+                        int ndp[2] = {0, 0};
+                        for (int k = 0; k < 2; ++k)
+                            for (int n = 0; n < Nd; ++n) ndp[k] += d.n_to_np[2 * ax + k][n] != 0;
+                        for (int n = 0; n < Nd; ++n) {
+                            const bool fwd = cos_[n] >= 0;  // enters through the minus face
+                            const int code = d.bc[fwd ? fm : fp];
+                            int f = -1, row = -1;
+                            if (d.n_to_np[fm][n] != 0) { f = fm; row = (int)d.n_to_np[fm][n] - 1; }
+                            else if (d.n_to_np[fp][n] != 0) { f = fp; row = (int)d.n_to_np[fp][n] - 1; }
+                            if (code == 0 || f < 0) continue;
+                            src[n] = fwd ? (code == 1 ? 0 : 1) : (code == 1 ? 1 : 0);
+                            const int nd_f = ndp[f - fm];
+                            for (int p = 0; p < Nps; ++p) Me[(size_t)n * Nps + p] = d.Msurf[f][row + (size_t)nd_f * p];
+                        }
+                    }
+                    d_Mentry[ax] = dalloc<double>(Me.size()); upload(d_Mentry[ax], Me, st);
+                    d_insrc[ax] = dalloc<int>(Nd); upload(d_insrc[ax], src, st);
+                    d_bin[ax] = dalloc<double>((size_t)Nd * npf[ax]);
+                    k_fill(d_bin[ax], 0.0, (size_t)Nd * npf[ax], st);  // axes beyond ndims stay zero
+                    bin_total += ax < ndims ? (int64_t)Nd * (int64_t)npf[ax] : 0;
+                }
+            }
         }
         // ---- fluxes and work buffers
         d_phi = dalloc<double>((size_t)G * Np * Nc);
@@ -234,6 +275,10 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
             if (ev[0] == 'r') sweep_kind = 1;
             if (ev[0] == 'g') { sweep_kind = 2; rows_per_lane = -1; planes_ok[0] = planes_ok[1] = false; }
         }
+        if (has_bc && !(sweep_kind == 0 && scan_cpl > 0 &&
+                        sweep_scan_lds_bytes(P64, ndims, Nx, Ny, Nz, Nmat) <= 160 * 1024))
+            throw Error(RT_EINVAL, "reflective/periodic boundaries need the row-scan sweep (Nx <= 512 and its "
+                                   "LDS face buffers within 160 KiB)");
         sw_ppt = sweep_pencils_per_thread(P);
         sw_block = std::min(1024, ((ceil_div(P, sw_ppt) + 63) / 64) * 64);
         for (int pr = 0; pr < 3; ++pr) {
@@ -357,6 +402,7 @@ void Problem::collect_kernel_times() {
 
 void Problem::launch_pass(PassDesc& pd, int prec, bool full_rows) {
     k_set_pd(pd, d_pd, st);
+    if (pd.bin_x) launch_bndin(d_pd, (int)bin_total, st);  // incoming fluxes of non-void faces
     auto timed = [&](int kernel, auto&& fn) {
         if (!profiling) {
             fn();
@@ -374,7 +420,7 @@ void Problem::launch_pass(PassDesc& pd, int prec, bool full_rows) {
         const int sp = prec == PBF16 ? P32 : prec;
         if (sweep_kind == 0 && scan_cpl > 0 && sweep_scan_lds_bytes(sp, ndims, Nx, Ny, Nz, Nmat) <= 160 * 1024)
             launch_sweep_scan(sp, d_pd, d_sw, Nd, scan_cpl, scan_waves, sweep_scan_lds_bytes(sp, ndims, Nx, Ny, Nz, Nmat),
-                              st);
+                              st, pd.bin_x != nullptr);
         else if (rows_per_lane > 0 && sweep_rows_lds_bytes(sp, Nx, Nmat) <= 160 * 1024)
             launch_sweep_rows(sp, d_pd, d_sw, Nd, rows_per_lane, sweep_rows_lds_bytes(sp, Nx, Nmat), st);
         else if (planes_ok[sp] && sweep_planes_lds_bytes(sp, Nx, Ny, Nz, Nmat, pl_R[sp], pl_waves[sp]) <= 160 * 1024)
@@ -418,6 +464,17 @@ void Problem::pass_faithful(int g, int prec, int mode, const double* zin, double
     for (int ax = 0; ax < 3; ++ax)
         for (int f = 0; f < 2; ++f) pd.excnt[ax][f] = excnt[ax][f];
     if (nb == 0) pd.bout_x = nullptr;
+    if (has_bc) {
+        pd.Mentry_x = d_Mentry[0]; pd.Mentry_y = d_Mentry[1]; pd.Mentry_z = d_Mentry[2];
+        pd.insrc_x = d_insrc[0]; pd.insrc_y = d_insrc[1]; pd.insrc_z = d_insrc[2];
+        pd.bin_x = d_bin[0]; pd.bin_y = d_bin[1]; pd.bin_z = d_bin[2];
+        if (zin) {  // the boundary part of the input state; none for T(0)
+            const double* bi = zin + (size_t)Np * Nc;
+            pd.bsin_x = bi;
+            pd.bsin_y = ndims >= 2 ? bi + 2 * (size_t)Nps * npos[0] : nullptr;
+            pd.bsin_z = ndims >= 3 ? bi + 2 * (size_t)Nps * (npos[0] + npos[1]) : nullptr;
+        }
+    }
     (void)prec;
     launch_pass(pd, P64, true);
 }

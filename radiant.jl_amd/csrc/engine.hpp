@@ -105,6 +105,12 @@ private:
     int* d_exitf[3] = {nullptr, nullptr, nullptr};
     int* d_exlist[3] = {nullptr, nullptr, nullptr};
     int excnt[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+    // reflective / periodic faces (faithful FP64): entry M rows, inflow source faces, inflows
+    bool has_bc = false;
+    double* d_Mentry[3] = {nullptr, nullptr, nullptr};
+    int* d_insrc[3] = {nullptr, nullptr, nullptr};
+    double* d_bin[3] = {nullptr, nullptr, nullptr};
+    int64_t bin_total = 0;
     double* d_qlout = nullptr;
     void* d_psi = nullptr;
     void* d_bex[3] = {nullptr, nullptr, nullptr};

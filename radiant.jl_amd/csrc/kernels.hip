@@ -828,8 +828,9 @@ __device__ inline T* trash_slot(int lane, int k) { return (T*)g_sweep_trash + k 
 
 // 1D/2D: one wave per direction; the rows y = 0..Ny-1 are solved in order with the y-face
 // fluxes in registers (each lane owns its cells).  The angular source and material of row
-// y + PD are loaded while row y is solved.
-template <typename T, int CPL, int PD, bool DPP>
+// y + PD are loaded while row y is solved.  BIN: the x- and y-face inflows come from
+// PassDesc::bin_x / bin_y (non-void boundaries) instead of zero.
+template <typename T, int CPL, int PD, bool DPP, bool BIN = false>
 __device__ __forceinline__ void sweep_line_body(const PassDesc& d, const int n, unsigned char* smem) {
     const int ndims = d.ndims, Nx = d.Nx, Ny = d.Ny, Nc = d.Nc, Nmat = d.Nmat;
     const int lane = threadIdx.x;
@@ -861,8 +862,13 @@ __device__ __forceinline__ void sweep_line_body(const PassDesc& d, const int n, 
         const int j = jy < Ny ? jy : Ny - 1;
         return fys ? j : Ny - 1 - j;
     };
+    const size_t npx = (size_t)Ny * d.Nz, npy = (size_t)Nx * d.Nz;  // face positions (Nz = 1 here)
+    T fy[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) fy[c] = BIN ? (T)gld(d.bin_y + n * npy + ixc[c]) : (T)0;
     T vr[PD][CPL];
     int mr[PD][CPL];
+    T xr[PD];  // x-face inflow of the row in each slot (BIN)
 #pragma unroll
     for (int i = 0; i < PD; ++i) {
         // issued in slot order, so that slot 0 is the oldest load when the loop starts
@@ -873,11 +879,9 @@ __device__ __forceinline__ void sweep_line_body(const PassDesc& d, const int n, 
             vr[i][c] = gld(psi + ro + ixc[c]);
             mr[i][c] = gld(mat + ro + ixc[c]);
         }
+        xr[i] = BIN ? (T)gld(d.bin_x + n * npx + row_of(i)) : (T)0;
     }
     __builtin_amdgcn_sched_barrier(0);
-    T fy[CPL];
-#pragma unroll
-    for (int c = 0; c < CPL; ++c) fy[c] = (T)0;
     for (int r0 = 0; r0 < Ny; r0 += PD) {
 #pragma unroll
         for (int i = 0; i < PD; ++i) {
@@ -913,8 +917,13 @@ __device__ __forceinline__ void sweep_line_body(const PassDesc& d, const int n, 
                 __builtin_amdgcn_sched_barrier(0);
             }
             affine_scan<DPP>(A, B, lane);
-            T fx = __shfl_up(A, 1);  // exclusive prefix applied to the void inflow (0)
+            T fx = __shfl_up(A, 1);  // exclusive prefix applied to the row's inflow (0 if void)
             if (lane == 0) fx = (T)0;
+            if constexpr (BIN) {
+                T bb = __shfl_up(B, 1);
+                if (lane == 0) bb = (T)1;
+                fx = fx + bb * xr[i];
+            }
 #pragma unroll
             for (int c = 0; c < CPL; ++c) {
                 const T p = c0[c] + c1[c] * fx;
@@ -924,6 +933,8 @@ __device__ __forceinline__ void sweep_line_body(const PassDesc& d, const int n, 
                 fy[c] = rv ? p2 - fy[c] : fy[c];
             }
             gst((rv && last_lane) ? bx + iy : trash_slot<T>(lane, 8), fx);
+            // the slot's next inflow is loaded after its last use (see the ring above)
+            if constexpr (BIN) xr[i] = (T)gld(d.bin_x + n * npx + row_of(jy + PD));
         }
     }
     if (ndims >= 2) {
@@ -933,12 +944,12 @@ __device__ __forceinline__ void sweep_line_body(const PassDesc& d, const int n, 
     }
 }
 
-template <typename T, int CPL, int PD, bool DPP>
+template <typename T, int CPL, int PD, bool DPP, bool BIN = false>
 __global__ __launch_bounds__(64) void sweep_line_kernel(const PassDesc* __restrict__ pds,
                                                         const SweepTask* __restrict__ tasks) {
     extern __shared__ __align__(16) unsigned char smem[];
     const SweepTask tk = tasks[blockIdx.x];
-    sweep_line_body<T, CPL, PD, DPP>(pds[tk.prob], tk.dir, smem);
+    sweep_line_body<T, CPL, PD, DPP, BIN>(pds[tk.prob], tk.dir, smem);
 }
 
 // FP64 and FP32 passes of different problems in one launch: each workgroup takes the precision of
@@ -957,7 +968,7 @@ __global__ __launch_bounds__(64) void sweep_line_mixed_kernel(const PassDesc* __
 // y/z face fluxes of the previous diagonal live in LDS (FY[jz][x], FZ[jy][x]), one barrier per
 // diagonal.  After the last diagonal FY and FZ hold the exit y- and z-faces, which are written
 // out at the end with coalesced stores.
-template <typename T, int CPL, int U, int PF, bool DPP>
+template <typename T, int CPL, int U, int PF, bool DPP, bool BIN = false>
 __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, unsigned char* smem) {
     const int Nx = d.Nx, Ny = d.Ny, Nz = d.Nz, Nc = d.Nc, Nmat = d.Nmat;
     // the wave index is uniform: in a scalar register, the item cursors and barrier counts are too
@@ -972,8 +983,24 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
     for (int i = tid; i < Ny; i += blockDim.x) hy2[i] = (T)(2.0 * (fabs(eta) / d.dy[i]));
     for (int i = tid; i < Nz; i += blockDim.x) hz2[i] = (T)(2.0 * (fabs(xi) / d.dz[i]));
     for (int i = tid; i < Nmat; i += blockDim.x) st[i] = (T)d.sigt[i];
-    for (int i = tid; i < (Nz + Ny) * Nx; i += blockDim.x) FY[i] = (T)0;  // void inflow on the y-/z- faces
     const bool fxs = mu >= 0, fys = eta >= 0, fzs = xi >= 0;
+    const size_t npx = (size_t)Ny * Nz, npy = (size_t)Nx * Nz, npz = (size_t)Nx * Ny;  // face positions
+    for (int i = tid; i < (Nz + Ny) * Nx; i += blockDim.x) {
+        // inflow on the entry y- and z-faces: zero for void boundaries
+        T v = (T)0;
+        if constexpr (BIN) {
+            if (i < Nz * Nx) {
+                const int jz = i / Nx, jx = i - jz * Nx;
+                const int ix = fxs ? jx : Nx - 1 - jx, iz = fzs ? jz : Nz - 1 - jz;
+                v = (T)gld(d.bin_y + n * npy + ix + (size_t)Nx * iz);
+            } else {
+                const int k = i - Nz * Nx, jy = k / Nx, jx = k - jy * Nx;
+                const int ix = fxs ? jx : Nx - 1 - jx, iy = fys ? jy : Ny - 1 - jy;
+                v = (T)gld(d.bin_z + n * npz + ix + (size_t)Nx * iy);
+            }
+        }
+        FY[i] = v;
+    }
     int ixc[CPL];
     bool inx[CPL];
     T hx[CPL];
@@ -1009,6 +1036,18 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
     };
     T vb[PF][U][CPL];
     int mb[PF][U][CPL];
+    T xb[PF][U];  // x-face inflow of each pencil of the item in a slot (BIN)
+    auto load_inflow = [&](T (&x)[U], int t, int q0) {
+        const int tt = t < ndiag ? t : ndiag - 1;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int q = q0 + u;
+            const bool ok = q < diag_cnt(tt);
+            const int jz = diag_lo(tt) + (ok ? q : 0), jy = tt - jz;
+            const int iy = fys ? jy : Ny - 1 - jy, iz = fzs ? jz : Nz - 1 - jz;
+            x[u] = (T)gld(d.bin_x + n * npx + iy + (size_t)Ny * iz);
+        }
+    };
     auto load_item = [&](T (&v)[U][CPL], int (&m)[U][CPL], int t, int q0) {
         const int tt = t < ndiag ? t : ndiag - 1;
 #pragma unroll
@@ -1032,6 +1071,7 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
 #pragma unroll
     for (int sl = 0; sl < PF; ++sl) {
         load_item(vb[sl], mb[sl], lt, lq);
+        if constexpr (BIN) load_inflow(xb[sl], lt, lq);
         advance(lt, lq);
         // As many stores as an item issues after its prefetch, so that the loop is entered with
         // the same memory operations in flight as it repeats with (exact wait counts).
@@ -1091,6 +1131,7 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
             // place it after the loads, and the ring would rotate through copies that wait).
 #pragma unroll
             for (int u = 0; u < U; ++u) asm volatile("" ::"v"(A[u]), "v"(B[u]));
+            const int xt = lt, xq = lq;  // the item whose inflow goes into this slot after its last use
             {
                 // the item PF ahead goes into the slot just consumed (see sweep_line_kernel)
                 __builtin_amdgcn_sched_barrier(0);
@@ -1102,8 +1143,13 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
             for (int u = 0; u < U; ++u) affine_scan<DPP>(A[u], B[u], lane);
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                T fx = __shfl_up(A[u], 1);  // exclusive prefix applied to the void inflow (0)
+                T fx = __shfl_up(A[u], 1);  // exclusive prefix applied to the pencil's inflow (0 if void)
                 if (lane == 0) fx = (T)0;
+                if constexpr (BIN) {
+                    T bb = __shfl_up(B[u], 1);
+                    if (lane == 0) bb = (T)1;
+                    fx = fx + bb * xb[sl][u];
+                }
 #pragma unroll
                 for (int c = 0; c < CPL; ++c) {
                     const bool act = ok[u] && inx[c];
@@ -1117,6 +1163,7 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
                 }
                 gst((ok[u] && last_lane) ? bx + iyv[u] + Ny * izv[u] : trash_slot<T>(lane, 8 + u), fx);
             }
+            if constexpr (BIN) load_inflow(xb[sl], xt, xq);
             advance(ct, cq);
         }
     }
@@ -1137,11 +1184,11 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
     }
 }
 
-template <typename T, int CPL, int U, int PF, int MAXT, bool DPP>
+template <typename T, int CPL, int U, int PF, int MAXT, bool DPP, bool BIN = false>
 __global__ __launch_bounds__(MAXT) void sweep_scan_kernel(const PassDesc* __restrict__ pds, const SweepTask* __restrict__ tasks) {
     extern __shared__ __align__(16) unsigned char smem[];
     const SweepTask tk = tasks[blockIdx.x];
-    sweep_scan_body<T, CPL, U, PF, DPP>(pds[tk.prob], tk.dir, smem);
+    sweep_scan_body<T, CPL, U, PF, DPP, BIN>(pds[tk.prob], tk.dir, smem);
 }
 
 template <int CPL, int U, int PF, int MAXT, bool DPP>
@@ -1188,6 +1235,37 @@ __global__ __launch_bounds__(256) void bndmom_kernel(const PassDesc* __restrict_
             const double* zin = ax == 0 ? d.bzin_x : (ax == 1 ? d.bzin_y : d.bzin_z);
             gst(out + j, zin ? gld(zin + j) - v : v);
         }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Incoming face fluxes of reflective / periodic boundaries (the boundary branches of
+// sn_sweep_{1,2,3}D.jl, e.g. sn_sweep_2D.jl:70-99): a direction entering through a face with a
+// non-void condition receives sum_p M_surf[n, p] * (exit half-range moments of the input state on
+// the same face for reflective, on the opposite face for periodic), summed in moment order.
+// One thread per (axis, direction, position).
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void bndin_kernel(const PassDesc* __restrict__ pds) {
+    const PassDesc& d = pds[0];
+    const int Nd = d.Nd, Nps = d.Nps;
+    const size_t np[3] = {(size_t)d.Ny * d.Nz, (size_t)d.Nx * d.Nz, (size_t)d.Nx * d.Ny};
+    size_t tot = 0;
+    for (int ax = 0; ax < d.ndims; ++ax) tot += (size_t)Nd * np[ax];
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+        size_t j = i;
+        int ax = 0;
+        while (j >= (size_t)Nd * np[ax]) { j -= (size_t)Nd * np[ax]; ++ax; }
+        const int n = (int)(j / np[ax]);
+        const size_t pos = j - (size_t)n * np[ax];
+        const int* insrc = ax == 0 ? d.insrc_x : (ax == 1 ? d.insrc_y : d.insrc_z);
+        const double* Me = ax == 0 ? d.Mentry_x : (ax == 1 ? d.Mentry_y : d.Mentry_z);
+        const double* bs = ax == 0 ? d.bsin_x : (ax == 1 ? d.bsin_y : d.bsin_z);
+        double* out = ax == 0 ? d.bin_x : (ax == 1 ? d.bin_y : d.bin_z);
+        const int src = insrc[n];
+        double v = 0.0;
+        if (src >= 0 && bs)
+            for (int p = 0; p < Nps; ++p) v += Me[(size_t)n * Nps + p] * bs[p + (size_t)Nps * (src + 2 * pos)];
+        out[j] = v;
     }
 }
 
@@ -1525,9 +1603,20 @@ static int scan3d_shape() {
     return e && e[0] == '1' ? 1 : 0;
 }
 
-template <int CPL, int U, int PF, int MAXT, bool DPP>
+template <int CPL, int U, int PF, int MAXT, bool DPP, bool BIN = false>
 static void scan3d_launch(int kind, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int waves, size_t lds,
                           hipStream_t st) {
+    waves = waves < MAXT / 64 ? waves : MAXT / 64;
+    if constexpr (BIN) {  // FP64 passes only
+        auto fn = sweep_scan_kernel<double, CPL, U, PF, MAXT, DPP, true>;
+        static bool battr = false;
+        if (!battr) {
+            RT_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            battr = true;
+        }
+        launch_timed(fn, dim3(ntasks), dim3(64 * waves), lds, st, d_pd, tasks);
+        return;
+    }
     const void* fn = kind == PMIX ? (const void*)sweep_scan_mixed_kernel<CPL, U, PF, MAXT, DPP>
                      : kind == P64 ? (const void*)sweep_scan_kernel<double, CPL, U, PF, MAXT, DPP>
                                    : (const void*)sweep_scan_kernel<float, CPL, U, PF, MAXT, DPP>;
@@ -1537,7 +1626,6 @@ static void scan3d_launch(int kind, const PassDesc* d_pd, const SweepTask* tasks
         RT_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         attr_set[ai] = true;
     }
-    waves = waves < MAXT / 64 ? waves : MAXT / 64;
     if (kind == PMIX)
         launch_timed(sweep_scan_mixed_kernel<CPL, U, PF, MAXT, DPP>, dim3(ntasks), dim3(64 * waves), lds, st, d_pd, tasks);
     else if (kind == P64)
@@ -1546,12 +1634,14 @@ static void scan3d_launch(int kind, const PassDesc* d_pd, const SweepTask* tasks
         launch_timed(sweep_scan_kernel<float, CPL, U, PF, MAXT, DPP>, dim3(ntasks), dim3(64 * waves), lds, st, d_pd, tasks);
 }
 
-template <int CPL, bool DPP>
+template <int CPL, bool DPP, bool BIN>
 static void scan3d_inst(int kind, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int waves, size_t lds,
                         hipStream_t st) {
     constexpr int U2 = CPL >= 4 ? 1 : 2;
     if constexpr (CPL >= 8 || !DPP) {  // 8 cells per lane: 512 threads, so that the registers do not spill
-        scan3d_launch<CPL, 1, 2, 512, DPP>(kind, d_pd, tasks, ntasks, waves, lds, st);
+        scan3d_launch<CPL, 1, 2, 512, DPP, BIN>(kind, d_pd, tasks, ntasks, waves, lds, st);
+    } else if constexpr (BIN) {
+        scan3d_launch<CPL, 1, 2, 1024, DPP, BIN>(kind, d_pd, tasks, ntasks, waves, lds, st);
     } else {
         constexpr int PF = CPL >= 4 ? 2 : 4;  // 4 cells per lane: a deeper ring would spill at 1024 threads
         if (scan3d_shape() == 1) scan3d_launch<CPL, U2, 1, 1024, DPP>(kind, d_pd, tasks, ntasks, waves, lds, st);
@@ -1559,10 +1649,20 @@ static void scan3d_inst(int kind, const PassDesc* d_pd, const SweepTask* tasks, 
     }
 }
 
-template <int CPL, bool DPP>
+template <int CPL, bool DPP, bool BIN>
 static void line_inst(int kind, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, size_t lds, hipStream_t st) {
     // rows in flight: keeps the outstanding loads and stores of PD rows within vmcnt's 63
     constexpr int PD = CPL <= 2 ? 8 : (CPL == 4 ? 4 : 2);
+    if constexpr (BIN) {  // FP64 passes only
+        auto fn = sweep_line_kernel<double, CPL, PD, DPP, true>;
+        static bool battr = false;
+        if (!battr) {
+            RT_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            battr = true;
+        }
+        launch_timed(fn, dim3(ntasks), dim3(64), lds, st, d_pd, tasks);
+        return;
+    }
     const void* fn = kind == PMIX ? (const void*)sweep_line_mixed_kernel<CPL, PD, DPP>
                      : kind == P64 ? (const void*)sweep_line_kernel<double, CPL, PD, DPP>
                                    : (const void*)sweep_line_kernel<float, CPL, PD, DPP>;
@@ -1580,28 +1680,44 @@ static void line_inst(int kind, const PassDesc* d_pd, const SweepTask* tasks, in
         launch_timed(sweep_line_kernel<float, CPL, PD, DPP>, dim3(ntasks), dim3(64), lds, st, d_pd, tasks);
 }
 
-template <int CPL>
+template <int CPL, bool BIN>
 static void scan_any(int kind, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int waves, size_t lds,
                      hipStream_t st) {
     if (scan_use_dpp()) {
-        if (waves == 0) line_inst<CPL, true>(kind, d_pd, tasks, ntasks, lds, st);
-        else scan3d_inst<CPL, true>(kind, d_pd, tasks, ntasks, waves, lds, st);
+        if (waves == 0) line_inst<CPL, true, BIN>(kind, d_pd, tasks, ntasks, lds, st);
+        else scan3d_inst<CPL, true, BIN>(kind, d_pd, tasks, ntasks, waves, lds, st);
     } else {
-        if (waves == 0) line_inst<CPL, false>(kind, d_pd, tasks, ntasks, lds, st);
-        else scan3d_inst<CPL, false>(kind, d_pd, tasks, ntasks, waves, lds, st);
+        if (waves == 0) line_inst<CPL, false, BIN>(kind, d_pd, tasks, ntasks, lds, st);
+        else scan3d_inst<CPL, false, BIN>(kind, d_pd, tasks, ntasks, waves, lds, st);
     }
 }
 
 void launch_sweep_scan(int prec, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int cpl, int waves, size_t lds,
-                       hipStream_t st) {
+                       hipStream_t st, bool bin) {
     if (ntasks <= 0) return;
     const int kind = prec == PMIX ? PMIX : (prec == P64 ? P64 : P32);
-    switch (cpl) {
-        case 1: scan_any<1>(kind, d_pd, tasks, ntasks, waves, lds, st); break;
-        case 2: scan_any<2>(kind, d_pd, tasks, ntasks, waves, lds, st); break;
-        case 4: scan_any<4>(kind, d_pd, tasks, ntasks, waves, lds, st); break;
-        default: scan_any<8>(kind, d_pd, tasks, ntasks, waves, lds, st); break;
+    if (bin && kind != P64) throw std::runtime_error("boundary inflow is supported by FP64 passes only");
+    if (bin) {
+        switch (cpl) {
+            case 1: scan_any<1, true>(kind, d_pd, tasks, ntasks, waves, lds, st); break;
+            case 2: scan_any<2, true>(kind, d_pd, tasks, ntasks, waves, lds, st); break;
+            case 4: scan_any<4, true>(kind, d_pd, tasks, ntasks, waves, lds, st); break;
+            default: scan_any<8, true>(kind, d_pd, tasks, ntasks, waves, lds, st); break;
+        }
+    } else {
+        switch (cpl) {
+            case 1: scan_any<1, false>(kind, d_pd, tasks, ntasks, waves, lds, st); break;
+            case 2: scan_any<2, false>(kind, d_pd, tasks, ntasks, waves, lds, st); break;
+            case 4: scan_any<4, false>(kind, d_pd, tasks, ntasks, waves, lds, st); break;
+            default: scan_any<8, false>(kind, d_pd, tasks, ntasks, waves, lds, st); break;
+        }
     }
+    RT_HIP(hipGetLastError());
+}
+
+void launch_bndin(const PassDesc* d_pd, int total, hipStream_t st) {
+    if (total <= 0) return;
+    hipLaunchKernelGGL(bndin_kernel, dim3(grid_for((size_t)total)), dim3(256), 0, st, d_pd);
     RT_HIP(hipGetLastError());
 }
 

@@ -38,6 +38,13 @@ struct PassDesc {
     const int *exlist_x, *exlist_y, *exlist_z;   // [2][Nd] directions leaving through face f, in index order
     int excnt[3][2];                             // lengths of those lists
     int prec;                                    // Prec of this pass (operators, psi, bx/by/bz)
+    // Non-void boundaries (FP64 faithful passes; null = every face void).  The incoming face flux
+    // of direction n at position pos of axis a is bin_a[n*npos_a + pos] = sum_p Mentry_a[n][p] *
+    // bsin_a[p + Nps*(insrc_a[n] + 2*pos)] (0 when insrc_a[n] < 0), with npos = Ny*Nz, Nx*Nz, Nx*Ny.
+    const double *Mentry_x, *Mentry_y, *Mentry_z;  // [Nd][Nps] half-range M row of each direction
+    const int *insrc_x, *insrc_y, *insrc_z;        // [Nd] face whose exit moments come back in (-1: none)
+    const double *bsin_x, *bsin_y, *bsin_z;        // boundary moments of the input state (null: zero)
+    double *bin_x, *bin_y, *bin_z;                 // incoming face fluxes [Nd][npos]
 };
 
 enum Prec { P64 = 0, P32 = 1, PBF16 = 2,
@@ -72,8 +79,11 @@ void launch_sweep_planes(int prec, const PassDesc* d_pd, const SweepTask* tasks,
 int sweep_scan_cpl(int Nx);
 size_t sweep_scan_lds_bytes(int prec, int ndims, int Nx, int Ny, int Nz, int Nmat);
 int sweep_scan_waves(int ndims, int Ny, int Nz);
+// bin: the passes take their inflow from PassDesc::bin_* (FP64 only)
 void launch_sweep_scan(int prec, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int cpl, int waves, size_t lds,
-                       hipStream_t st);
+                       hipStream_t st, bool bin = false);
+// incoming face fluxes of non-void boundaries (PassDesc::bin_*), one problem
+void launch_bndin(const PassDesc* d_pd, int total, hipStream_t st);
 void launch_dgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st);
 void launch_bndmom(const PassDesc* d_pd, int nprob, int max_items, hipStream_t st);
 
