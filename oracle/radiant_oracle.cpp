@@ -77,6 +77,122 @@ void gauss_legendre(int N, vector<double>& x, vector<double>& w) {
     }
 }
 
+// tools/gauss_lobatto.jl:18-76 -- nodes +-1 and the roots of P'_{N-1}, by the same iteration from
+// x = cos(pi i/(N-1)); weights 2/(N(N-1) P_{N-1}(x)^2); the centre node of an odd rule is 0.
+void gauss_lobatto(int N, vector<double>& x, vector<double>& w) {
+    if (N < 2) fail("Gauss-Lobatto needs at least 2 points");
+    x.assign(N, 0.0);
+    w.assign(N, 0.0);
+    for (int i = 0; i < N; ++i) x[i] = std::cos(PI * (double)i / (double)(N - 1));
+    w[0] = w[N - 1] = 2.0 / ((double)N * (double)(N - 1));
+    for (int i = 1; i < N - 1; ++i) {
+        double xold = 2.0, P2 = 0.0;
+        for (int it = 0; it < 200 && std::fabs(x[i] - xold) > EPS; ++it) {
+            xold = x[i];
+            double P1 = 1.0;
+            P2 = x[i];
+            for (int n = 2; n < N; ++n) {
+                double Pn = ((double)(2 * n - 1) * x[i] * P2 - (double)(n - 1) * P1) / (double)n;
+                P1 = P2;
+                P2 = Pn;
+            }
+            double d1 = (double)N * (P1 - x[i] * P2) / (1.0 - x[i] * x[i]);
+            double d2 = (2.0 * x[i] * d1 - (double)N * (double)(N + 1) * P2) / (1.0 - x[i] * x[i]);
+            x[i] -= d1 / d2;
+        }
+        w[i] = 2.0 / ((double)N * (double)(N - 1) * P2 * P2);
+    }
+    if (N % 2 == 1) x[(N + 1) / 2 - 1] = 0.0;
+    for (int i = 0; i < N; ++i)
+        for (int j = i + 1; j < N; ++j)
+            if (x[i] == x[j]) fail("Unable to build Lobatto quadrature at order N = " + std::to_string(N) + ".");
+}
+
+// tools/lebedev.jl for the orders whose rules are known in closed form (3, 5, 7, 9, 11).  Each
+// rule is a list of orbits of the octahedral group O_h with one weight (fraction of the sphere):
+// 1 = (1,0,0) 6 points, 2 = (0,a,a) a=1/sqrt2 12, 3 = (a,a,a) a=1/sqrt3 8, 4 = (a,a,b)
+// b=sqrt(1-2a^2) 24, 5 = (a,b,0) b=sqrt(1-a^2) 24.  Weights are scaled to 4 pi.
+static void oh_orbit(int code, double a, double v, vector<double>& mu, vector<double>& eta, vector<double>& xi,
+                     vector<double>& w) {
+    const int s2[4][2] = {{1, 1}, {1, -1}, {-1, 1}, {-1, -1}};
+    auto push = [&](double x, double y, double z) {
+        mu.push_back(x); eta.push_back(y); xi.push_back(z); w.push_back(4.0 * PI * v);
+    };
+    if (code == 1) {
+        for (int ax = 0; ax < 3; ++ax)
+            for (double sg : {1.0, -1.0}) {
+                double p[3] = {0.0, 0.0, 0.0};
+                p[ax] = sg;
+                push(p[0], p[1], p[2]);
+            }
+    } else if (code == 2) {
+        double c = 1.0 / std::sqrt(2.0);
+        for (int z = 0; z < 3; ++z) {
+            int o0 = z == 0 ? 1 : 0, o1 = z == 2 ? 1 : 2;
+            for (auto& s : s2) {
+                double p[3] = {0.0, 0.0, 0.0};
+                p[o0] = s[0] * c; p[o1] = s[1] * c;
+                push(p[0], p[1], p[2]);
+            }
+        }
+    } else if (code == 3) {
+        double c = 1.0 / std::sqrt(3.0);
+        for (auto& s : s2)
+            for (int s3 : {1, -1}) push(s[0] * c, s[1] * c, s3 * c);
+    } else if (code == 4) {
+        double b = std::sqrt(1.0 - 2.0 * a * a);
+        for (int bi = 0; bi < 3; ++bi)
+            for (auto& s : s2)
+                for (int s3 : {1, -1}) {
+                    int sg[3] = {s[0], s[1], s3};
+                    double p[3] = {sg[0] * a, sg[1] * a, sg[2] * a};
+                    p[bi] = sg[bi] * b;
+                    push(p[0], p[1], p[2]);
+                }
+    } else if (code == 5) {
+        double b = std::sqrt(1.0 - a * a);
+        for (int z = 0; z < 3; ++z) {
+            int o0 = z == 0 ? 1 : 0, o1 = z == 2 ? 1 : 2;
+            const double pq[2][2] = {{a, b}, {b, a}};
+            for (auto& t : pq)
+                for (auto& s : s2) {
+                    double p[3] = {0.0, 0.0, 0.0};
+                    p[o0] = s[0] * t[0]; p[o1] = s[1] * t[1];
+                    push(p[0], p[1], p[2]);
+                }
+        }
+    }
+}
+
+void lebedev(int N, vector<double>& mu, vector<double>& eta, vector<double>& xi, vector<double>& w) {
+    mu.clear(); eta.clear(); xi.clear(); w.clear();
+    switch (N) {
+        case 3: oh_orbit(1, 0.0, 1.0 / 6.0, mu, eta, xi, w); break;
+        case 5:
+            oh_orbit(1, 0.0, 1.0 / 15.0, mu, eta, xi, w);
+            oh_orbit(3, 0.0, 3.0 / 40.0, mu, eta, xi, w);
+            break;
+        case 7:
+            oh_orbit(1, 0.0, 1.0 / 21.0, mu, eta, xi, w);
+            oh_orbit(2, 0.0, 4.0 / 105.0, mu, eta, xi, w);
+            oh_orbit(3, 0.0, 9.0 / 280.0, mu, eta, xi, w);
+            break;
+        case 9:
+            oh_orbit(1, 0.0, 1.0 / 105.0, mu, eta, xi, w);
+            oh_orbit(3, 0.0, 9.0 / 280.0, mu, eta, xi, w);
+            oh_orbit(5, std::sqrt((1.0 - 1.0 / std::sqrt(3.0)) / 2.0), 1.0 / 35.0, mu, eta, xi, w);
+            break;
+        case 11:
+            oh_orbit(1, 0.0, 4.0 / 315.0, mu, eta, xi, w);
+            oh_orbit(2, 0.0, 64.0 / 2835.0, mu, eta, xi, w);
+            oh_orbit(3, 0.0, 27.0 / 1280.0, mu, eta, xi, w);
+            oh_orbit(4, 1.0 / std::sqrt(11.0), 14641.0 / 725760.0, mu, eta, xi, w);
+            break;
+        default:
+            fail("Lebedev quadrature is not available at order " + std::to_string(N) + " (available: 3, 5, 7, 9, 11).");
+    }
+}
+
 // tools/gauss_legendre_chebychev.jl:19-39 -- product of GL (sorted ascending) in mu and a
 // 2N-point Chebyshev rule in phi; polar axis is x.
 void gauss_legendre_chebychev(int N, vector<double>& mu, vector<double>& eta, vector<double>& xi,
@@ -210,14 +326,20 @@ void carlson(int N, vector<double>& mu, vector<double>& eta, vector<double>& xi,
             }
 }
 
-// tools/quadrature.jl:19-59 (gauss-legendre, gauss-legendre-chebychev, carlson; the Qdims=2
-// fold onto the upper hemisphere is at :36-54).
+// tools/quadrature.jl:19-59 (gauss-legendre, gauss-lobatto, gauss-legendre-chebychev, lebedev,
+// carlson; the Qdims=2 fold onto the upper hemisphere is at :36-54).
 void quadrature(int N, const string& type, int Ndims, int Qdims, vector<double>& mu,
                 vector<double>& eta, vector<double>& xi, vector<double>& w) {
     if (type == "gauss-legendre" && Ndims == 1) {
         gauss_legendre(N, mu, w);
         eta.assign(mu.size(), 0.0);   // sn_flux.jl:45 expands a 1D set to [mu, 0, 0]
         xi.assign(mu.size(), 0.0);
+    } else if (type == "gauss-lobatto" && Ndims == 1) {
+        gauss_lobatto(N, mu, w);
+        eta.assign(mu.size(), 0.0);
+        xi.assign(mu.size(), 0.0);
+    } else if (type == "lebedev") {
+        lebedev(N, mu, eta, xi, w);
     } else if (type == "gauss-legendre-chebychev") {
         gauss_legendre_chebychev(N, mu, eta, xi, w);
     } else if (type == "carlson") {
@@ -751,7 +873,7 @@ struct Solver {
         // sn_flux.jl:38-48 -- quadrature and angular operators
         string qt = p.quad_type;
         int Qdims;
-        if (qt == "gauss-legendre") {  // SN.jl:638-659
+        if (qt == "gauss-legendre" || qt == "gauss-lobatto") {  // SN.jl:638-659
             Qdims = 1;
         } else {
             Qdims = (ndims == 3) ? 3 : (p.qdims == 2 ? 2 : 3);

@@ -42,6 +42,106 @@ def gauss_legendre(N):
     return x, w
 
 
+def gauss_lobatto(N, nmax=200):
+    """tools/gauss_lobatto.jl:18-76: nodes +-1 and the roots of P'_{N-1}, found by the same
+    iteration from x = cos(pi i/(N-1)); weights 2/(N(N-1) P_{N-1}(x)^2); the centre node of an
+    odd rule is set to 0."""
+    if N < 2:
+        raise ValueError("Gauss-Lobatto needs at least 2 points")
+    x = np.cos(PI * np.arange(N) / (N - 1))
+    w = np.zeros(N)
+    w[0] = w[-1] = 2.0 / (N * (N - 1))
+    for i in range(1, N - 1):
+        xi = float(x[i])
+        xold = 2.0
+        P2 = 0.0
+        it = 0
+        while abs(xi - xold) > EPS and it < nmax:
+            xold = xi
+            P1, P2 = 1.0, xi
+            for n in range(2, N):
+                P1, P2 = P2, ((2 * n - 1) * xi * P2 - (n - 1) * P1) / n
+            d1 = N * (P1 - xi * P2) / (1 - xi * xi)
+            d2 = (2 * xi * d1 - N * (N + 1) * P2) / (1 - xi * xi)
+            xi -= d1 / d2
+            it += 1
+        x[i] = xi
+        w[i] = 2.0 / (N * (N - 1) * P2 * P2)
+    if N % 2 == 1:
+        x[(N + 1) // 2 - 1] = 0.0
+    if len(set(x.tolist())) != N:
+        raise ValueError(f"Unable to build Lobatto quadrature at order N = {N}.")
+    return x, w
+
+
+# Lebedev rules available in closed form (order: [(symmetry class, parameter, weight)]); the
+# weights are fractions of the sphere and the classes are those of the octahedral group O_h:
+# 1 = (1,0,0) 6 points, 2 = (0,a,a) a=1/sqrt2 12, 3 = (a,a,a) a=1/sqrt3 8, 4 = (a,a,b)
+# b=sqrt(1-2a^2) 24, 5 = (a,b,0) b=sqrt(1-a^2) 24.  Each rule integrates every spherical
+# harmonic of degree <= order exactly.
+LEBEDEV_RULES = {
+    3: [(1, 0.0, 1 / 6)],
+    5: [(1, 0.0, 1 / 15), (3, 0.0, 3 / 40)],
+    7: [(1, 0.0, 1 / 21), (2, 0.0, 4 / 105), (3, 0.0, 9 / 280)],
+    9: [(1, 0.0, 1 / 105), (3, 0.0, 9 / 280), (5, math.sqrt((1 - 1 / math.sqrt(3)) / 2), 1 / 35)],
+    11: [(1, 0.0, 4 / 315), (2, 0.0, 64 / 2835), (3, 0.0, 27 / 1280), (4, 1 / math.sqrt(11), 14641 / 725760)],
+}
+
+
+def _oh_orbit(code, a):
+    s2 = ((1, 1), (1, -1), (-1, 1), (-1, -1))
+    pts = []
+    if code == 1:
+        for ax in range(3):
+            for sg in (1.0, -1.0):
+                v = [0.0, 0.0, 0.0]
+                v[ax] = sg
+                pts.append(v)
+    elif code == 2:
+        c = 1 / math.sqrt(2)
+        for z in range(3):
+            o = [k for k in range(3) if k != z]
+            for s1, s_2 in s2:
+                v = [0.0, 0.0, 0.0]
+                v[o[0]], v[o[1]] = s1 * c, s_2 * c
+                pts.append(v)
+    elif code == 3:
+        c = 1 / math.sqrt(3)
+        for s1, s_2 in s2:
+            for s3 in (1, -1):
+                pts.append([s1 * c, s_2 * c, s3 * c])
+    elif code == 4:
+        b = math.sqrt(1 - 2 * a * a)
+        for bi in range(3):
+            for s1, s_2 in s2:
+                for s3 in (1, -1):
+                    v = [s1 * a, s_2 * a, s3 * a]
+                    v[bi] = (s1, s_2, s3)[bi] * b
+                    pts.append(v)
+    elif code == 5:
+        b = math.sqrt(1 - a * a)
+        for z in range(3):
+            o = [k for k in range(3) if k != z]
+            for p_, q_ in ((a, b), (b, a)):
+                for s1, s_2 in s2:
+                    v = [0.0, 0.0, 0.0]
+                    v[o[0]], v[o[1]] = s1 * p_, s_2 * q_
+                    pts.append(v)
+    return pts
+
+
+def lebedev(N):
+    """tools/lebedev.jl:21-4751 for the orders whose rules are known in closed form (3, 5, 7, 9,
+    11); weights scaled to 4 pi as Radiant does."""
+    if N not in LEBEDEV_RULES:
+        raise ValueError(f"Lebedev quadrature is not available at order {N} (available: 3, 5, 7, 9, 11).")
+    mu, eta, xi, w = [], [], [], []
+    for code, a, v in LEBEDEV_RULES[N]:
+        for p in _oh_orbit(code, a):
+            mu.append(p[0]); eta.append(p[1]); xi.append(p[2]); w.append(4 * PI * v)
+    return np.array(mu), np.array(eta), np.array(xi), np.array(w)
+
+
 def gauss_legendre_chebychev(N):
     """tools/gauss_legendre_chebychev.jl:19-39: GL in mu (ascending) x 2N Chebyshev azimuths."""
     mun, wn = gauss_legendre(N)
@@ -174,8 +274,14 @@ def quadrature(N, quad_type, ndims=1, qdims=None):
         mu, w = gauss_legendre(N)
         eta = np.zeros_like(mu)
         xi = np.zeros_like(mu)
+    elif quad_type == "gauss-lobatto" and ndims == 1:
+        mu, w = gauss_lobatto(N)
+        eta = np.zeros_like(mu)
+        xi = np.zeros_like(mu)
     elif quad_type == "gauss-legendre-chebychev":
         mu, eta, xi, w = gauss_legendre_chebychev(N)
+    elif quad_type == "lebedev":
+        mu, eta, xi, w = lebedev(N)
     elif quad_type == "carlson":
         mu, eta, xi, w = carlson(N)
     else:
