@@ -81,6 +81,15 @@ typedef struct rt_problem_desc {
     int32_t precision;                  /* RT_PREC_* */
     int32_t state_mode;                 /* RT_STATE_* (RT_PREC_MIXED_* implies REDUCED) */
     int32_t verbose;                    /* print Radiant's per-group convergence lines */
+    /* Surface sources (surface_source.jl:22-408, accumulated as Source.jl:152-190 does and applied
+     * in the sweeps' boundary blocks, e.g. sn_sweep_2D.jl:70-99): per face f (x-, x+, y-, y+, z-,
+     * z+), NULL or the half-range moments [G][Np_source][npos_f], row-major with the position
+     * fastest (npos = Ny*Nz for x faces, Nx*Nz for y faces, Nx*Ny for z faces; positions ordered
+     * like the boundary moments: iy + Ny*iz, ix + Nx*iz, ix + Nx*iy).  The first Np_source
+     * half-range moments of each face carry the source.  FP64 faithful state only; callers that
+     * do not use surface sources leave these fields zero. */
+    const double *ssrc[6];
+    int32_t Np_source;
 } rt_problem_desc;
 
 /* Per-group result of the in-group solve (the values sn_one_speed.jl:189-193 prints). */

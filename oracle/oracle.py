@@ -29,6 +29,7 @@ class OracleProblem(C.Structure):
         ("legendre_order", C.c_int32), ("sn_type", C.c_char_p), ("accel", C.c_char_p),
         ("restart", C.c_int32), ("depth", C.c_int32), ("maxit", C.c_int32), ("tol", C.c_double),
         ("nthreads", C.c_int32), ("verbose", C.c_int32),
+        ("ssrc", C.POINTER(C.c_double) * 6), ("Np_source", C.c_int32),
     ]
 
 
@@ -154,6 +155,13 @@ def make_problem(prob, nthreads=1, verbose=0, **over):
         _p(k.qvol), k.bc.ctypes.data_as(C.POINTER(C.c_int32)), k.qt, prob.quad_order, prob.qdims,
         prob.legendre_order, k.st, k.ac, over.get("restart", prob.restart), over.get("depth", prob.depth),
         over.get("maxit", prob.maxit), over.get("tol", prob.tol), nthreads, verbose)
+    ss = getattr(prob, "ssrc", None)
+    if ss:
+        k.ss = {}
+        for f, arr in ss.items():
+            k.ss[f] = np.ascontiguousarray(arr, np.float64).ravel()
+            op.ssrc[f] = _p(k.ss[f])
+        op.Np_source = int(prob.Np_source)
     op._keep = k
     return op
 

@@ -839,6 +839,9 @@ struct oracle_problem {
     double tol;
     int32_t nthreads;               // 1 = serial (Radiant's execution model); >1 = OpenMP over directions
     int32_t verbose;
+    // surface sources (surface_source.jl): per face, NULL or [G][Np_source][npos_f], position fastest
+    const double* ssrc[6];
+    int32_t Np_source;
 };
 
 struct oracle_stats {
@@ -860,6 +863,7 @@ struct Solver {
     int Ls;                     // max(pl): Σs sized to Ls+1 (sn_flux.jl:79-83)
     vector<double> sigs;        // [Nmat][G][G][Ls+1] padded copy: sigs[m + Nmat*(gi + G*(gf + G*l))]
     vector<int> matc;           // 0-based material per cell
+    const double* cur_src[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // this pass's surface sources
     // state layout: [Φ (Np*Nc, p fastest) | x-bnd | y-bnd | z-bnd]
     size_t nphi, nbx, nby, nbz, S;
     vector<int> nxm, nxp, nym, nyp, nzm, nzp;  // n_to_n+ per face (1-based or 0)
@@ -913,6 +917,8 @@ struct Solver {
         }
         for (int f = 0; f < nf; ++f) {
             if (p.bc[f] < 0 || p.bc[f] > 2) fail("Unknown boundary condition type.");
+            if (p.ssrc[f] && (p.Np_source < 0 || p.Np_source > A.Nps))
+                fail("surface-source moments exceed the half-range basis of the face");
         }
         nphi = (size_t)A.Np * Nc;
         nbx = (size_t)A.Nps * 2 * Ny * Nz;
@@ -945,6 +951,9 @@ struct Solver {
         if (fm != 0) { Sb = &A.S[0]; row = fm - 1; } else if (fp != 0) { Sb = &A.S[1]; row = fp - 1; }
         bool fwd = mu >= 0;
         int bcin = fwd ? P.bc[0] : P.bc[1];
+        // surface source of the entry face (sn_sweep_1D.jl boundary block: the source term first)
+        if (Sb && cur_src[fwd ? 0 : 1])
+            for (int p = 0; p < P.Np_source; ++p) fx += Sb->M[row + (size_t)Sb->Ndp * p] * cur_src[fwd ? 0 : 1][p];
         if (bcin != 0 && Sb) {
             int src_face = fwd ? (bcin == 1 ? 0 : 1) : (bcin == 1 ? 1 : 0);
             for (int p = 0; p < Nps; ++p)
@@ -973,11 +982,16 @@ struct Solver {
     }
 
     // incoming boundary value for a face array b[p + Nps*(f + 2*pos)] (sweep BC branches).
-    double bc_in(int face_bc, bool fwd, const SurfBasis* Sb, int row, const double* b, size_t pos) {
-        if (face_bc == 0 || !Sb) return 0.0;
-        int src = fwd ? (face_bc == 1 ? 0 : 1) : (face_bc == 1 ? 1 : 0);
+    // src: the entry face's surface source of the current group ([Np_source][npos]) or null; its
+    // term comes first, as in sn_sweep_2D.jl:70-99.
+    double bc_in(int face_bc, bool fwd, const SurfBasis* Sb, int row, const double* b, size_t pos,
+                 const double* src = nullptr, size_t npos = 0) {
         double v = 0.0;
-        for (int p = 0; p < A.Nps; ++p) v += Sb->M[row + (size_t)Sb->Ndp * p] * b[p + (size_t)A.Nps * (src + 2 * pos)];
+        if (src && Sb)
+            for (int p = 0; p < P.Np_source; ++p) v += Sb->M[row + (size_t)Sb->Ndp * p] * src[(size_t)p * npos + pos];
+        if (face_bc == 0 || !Sb) return v;
+        int src_face = fwd ? (face_bc == 1 ? 0 : 1) : (face_bc == 1 ? 1 : 0);
+        for (int p = 0; p < A.Nps; ++p) v += Sb->M[row + (size_t)Sb->Ndp * p] * b[p + (size_t)A.Nps * (src_face + 2 * pos)];
         return v;
     }
 
@@ -1005,11 +1019,17 @@ struct Solver {
             for (int ky = 0; ky < Ny; ++ky) {
                 int iy = fy_ ? ky : Ny - 1 - ky;
                 double Fz = 0.0;
-                if (ndims == 3) Fz = bc_in(bcz, fz_, Sz, rz, bz_in, (size_t)ix + (size_t)Nx * iy);
+                if (ndims == 3)
+                    Fz = bc_in(bcz, fz_, Sz, rz, bz_in, (size_t)ix + (size_t)Nx * iy, cur_src[fz_ ? 4 : 5],
+                               (size_t)Nx * Ny);
                 for (int kz = 0; kz < Nz; ++kz) {
                     int iz = fz_ ? kz : Nz - 1 - kz;
-                    if (ky == 0) Fy[iz] = bc_in(bcy, fy_, Sy, ry, by_in, (size_t)ix + (size_t)Nx * iz);
-                    if (kx == 0) Fx[iy + (size_t)Ny * iz] = bc_in(bcx, fx_, Sx, rx, bx_in, (size_t)iy + (size_t)Ny * iz);
+                    if (ky == 0)
+                        Fy[iz] = bc_in(bcy, fy_, Sy, ry, by_in, (size_t)ix + (size_t)Nx * iz, cur_src[fy_ ? 2 : 3],
+                                       (size_t)Nx * Nz);
+                    if (kx == 0)
+                        Fx[iy + (size_t)Ny * iz] = bc_in(bcx, fx_, Sx, rx, bx_in, (size_t)iy + (size_t)Ny * iz,
+                                                         cur_src[fx_ ? 0 : 1], (size_t)Ny * Nz);
                     size_t c = (size_t)ix + (size_t)Nx * (iy + (size_t)Ny * iz);
                     double Qn = 0.0;
                     for (int p = 0; p < Np; ++p) Qn += A.B.M[n + (size_t)Nd * p] * Ql[p + (size_t)Np * c];
@@ -1062,6 +1082,12 @@ struct Solver {
         const int Np = A.Np;
         double* phi = z.data();
         double* bx = phi + nphi;
+        // surface sources enter the inhomogeneous passes only (sn_inner_pass.jl: sources with
+        // homogeneous = false)
+        const size_t npf[6] = {(size_t)Ny * Nz, (size_t)Ny * Nz, (size_t)Nx * Nz, (size_t)Nx * Nz,
+                               (size_t)Nx * Ny, (size_t)Nx * Ny};
+        for (int f = 0; f < 6; ++f)
+            cur_src[f] = (!homog && f < 2 * ndims && P.ssrc[f]) ? P.ssrc[f] + (size_t)g * P.Np_source * npf[f] : nullptr;
         // Ql = (homog ? 0 : Qlout) + Σs_l Φl   (scattering_source.jl:25-30)
         for (int c = 0; c < Nc; ++c) {
             int m = matc[c];
@@ -1136,6 +1162,15 @@ struct Solver {
         long passes = 0;
         bool any = false;
         for (double v : Qlout) if (v != 0.0) { any = true; break; }
+        // the group's surface sources count too (sn_one_speed.jl:116: `sources`)
+        const size_t npf[6] = {(size_t)Ny * Nz, (size_t)Ny * Nz, (size_t)Nx * Nz, (size_t)Nx * Nz,
+                               (size_t)Nx * Ny, (size_t)Nx * Ny};
+        for (int f = 0; f < 2 * ndims && !any; ++f) {
+            if (!P.ssrc[f]) continue;
+            const double* q = P.ssrc[f] + (size_t)g * P.Np_source * npf[f];
+            for (size_t i = 0; i < (size_t)P.Np_source * npf[f]; ++i)
+                if (q[i] != 0.0) { any = true; break; }
+        }
         if (!any) {  // sn_one_speed.jl:116-120
             std::fill(z_phi_bnd.begin(), z_phi_bnd.begin() + nphi, 0.0);
             st.niter = 0; st.resid = 0.0; st.rho = NAN; st.converged = 1; st.passes = 0;

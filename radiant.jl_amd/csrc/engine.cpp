@@ -77,13 +77,18 @@ static void validate(const rt_problem_desc& d) {
         if (d.bc[f] < 0 || d.bc[f] > 2) bad("Unknown boundary condition type.");
         nonvoid = nonvoid || d.bc[f] != 0;
     }
-    if (nonvoid) {
+    bool surf = false;
+    for (int f = 0; f < 2 * d.ndims; ++f) surf = surf || d.ssrc[f] != nullptr;
+    if (nonvoid || surf) {
         if (d.precision != RT_PREC_FP64 || d.state_mode != RT_STATE_FAITHFUL)
-            bad("reflective/periodic boundaries need the FP64 faithful state (RT_PREC_FP64, RT_STATE_FAITHFUL)");
+            bad("reflective/periodic boundaries and surface sources need the FP64 faithful state (RT_PREC_FP64, "
+                "RT_STATE_FAITHFUL)");
         for (int f = 0; f < 2 * d.ndims; ++f)
             if (!d.Msurf[f] || !d.Dsurf[f] || !d.n_to_np[f])
-                bad("reflective/periodic boundaries need the half-range operators of every face");
+                bad("reflective/periodic boundaries and surface sources need the half-range operators of every face");
     }
+    if (surf && (d.Np_source < 1 || d.Np_source > d.Nps))
+        bad("Np_source must lie in 1..Nps when a surface source is given");
     const size_t nc = (size_t)d.Nx * d.Ny * d.Nz;
     for (size_t c = 0; c < nc; ++c)
         if (d.mat[c] < 1 || d.mat[c] > d.Nmat) bad("material index out of range at voxel " + std::to_string(c));
@@ -205,12 +210,15 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
             }
             // ---- reflective / periodic faces (sn_sweep_2D.jl:70-99 and the x/z analogues): the
             // entry half-range M row of every direction and the face whose exit moments feed it
-            for (int f = 0; f < 2 * ndims; ++f) has_bc = has_bc || d.bc[f] != 0;
-            if (has_bc) {
+            for (int f = 0; f < 2 * ndims; ++f) {
+                has_bc = has_bc || d.bc[f] != 0;
+                has_ss = has_ss || d.ssrc[f] != nullptr;
+            }
+            if (has_bc || has_ss) {
                 const size_t npf[3] = {(size_t)Ny * Nz, (size_t)Nx * Nz, (size_t)Nx * Ny};
                 for (int ax = 0; ax < 3; ++ax) {
                     std::vector<double> Me((size_t)Nd * std::max(Nps, 1), 0.0);
-                    std::vector<int> src(Nd, -1);
+                    std::vector<int> src(Nd, -1), sface(Nd, -1);
                     if (ax < ndims) {
                         const double* cos_ = ax == 0 ? d.mu : (ax == 1 ? d.eta : d.xi);
                         const int fm = 2 * ax, fp = 2 * ax + 1;
@@ -223,12 +231,36 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
                             int f = -1, row = -1;
                             if (d.n_to_np[fm][n] != 0) { f = fm; row = (int)d.n_to_np[fm][n] - 1; }
                             else if (d.n_to_np[fp][n] != 0) { f = fp; row = (int)d.n_to_np[fp][n] - 1; }
-                            if (code == 0 || f < 0) continue;
-                            src[n] = fwd ? (code == 1 ? 0 : 1) : (code == 1 ? 1 : 0);
+                            if (f < 0) continue;
+                            if (code != 0) src[n] = fwd ? (code == 1 ? 0 : 1) : (code == 1 ? 1 : 0);
+                            if (d.ssrc[fwd ? fm : fp]) sface[n] = fwd ? 0 : 1;
                             const int nd_f = ndp[f - fm];
                             for (int p = 0; p < Nps; ++p) Me[(size_t)n * Nps + p] = d.Msurf[f][row + (size_t)nd_f * p];
                         }
+                        // surface sources of the axis: [2 faces][G][Nps][npos], moments beyond
+                        // Np_source zero
+                        if (d.ssrc[fm] || d.ssrc[fp]) {
+                            const size_t npa = npf[ax];
+                            grp_ss.resize(G, 0);
+                            for (int k = 0; k < 2; ++k)
+                                if (d.ssrc[fm + k])
+                                    for (int g = 0; g < G; ++g)
+                                        for (size_t i = 0; i < (size_t)d.Np_source * npa && !grp_ss[g]; ++i)
+                                            grp_ss[g] = d.ssrc[fm + k][(size_t)g * d.Np_source * npa + i] != 0.0;
+                            std::vector<double> q((size_t)2 * G * Nps * npa, 0.0);
+                            for (int k = 0; k < 2; ++k) {
+                                const double* in = d.ssrc[fm + k];
+                                if (!in) continue;
+                                for (int g = 0; g < G; ++g)
+                                    for (int p = 0; p < d.Np_source; ++p)
+                                        for (size_t ps = 0; ps < npa; ++ps)
+                                            q[(((size_t)k * G + g) * Nps + p) * npa + ps] =
+                                                in[((size_t)g * d.Np_source + p) * npa + ps];
+                            }
+                            d_ssrc[ax] = dalloc<double>(q.size()); upload(d_ssrc[ax], q, st);
+                        }
                     }
+                    d_sface[ax] = dalloc<int>(Nd); upload(d_sface[ax], sface, st);
                     d_Mentry[ax] = dalloc<double>(Me.size()); upload(d_Mentry[ax], Me, st);
                     d_insrc[ax] = dalloc<int>(Nd); upload(d_insrc[ax], src, st);
                     d_bin[ax] = dalloc<double>((size_t)Nd * npf[ax]);
@@ -275,10 +307,10 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
             if (ev[0] == 'r') sweep_kind = 1;
             if (ev[0] == 'g') { sweep_kind = 2; rows_per_lane = -1; planes_ok[0] = planes_ok[1] = false; }
         }
-        if (has_bc && !(sweep_kind == 0 && scan_cpl > 0 &&
+        if ((has_bc || has_ss) && !(sweep_kind == 0 && scan_cpl > 0 &&
                         sweep_scan_lds_bytes(P64, ndims, Nx, Ny, Nz, Nmat) <= 160 * 1024))
-            throw Error(RT_EINVAL, "reflective/periodic boundaries need the row-scan sweep (Nx <= 512 and its "
-                                   "LDS face buffers within 160 KiB)");
+            throw Error(RT_EINVAL, "reflective/periodic boundaries and surface sources need the row-scan sweep "
+                                   "(Nx <= 512 and its LDS face buffers within 160 KiB)");
         sw_ppt = sweep_pencils_per_thread(P);
         sw_block = std::min(1024, ((ceil_div(P, sw_ppt) + 63) / 64) * 64);
         for (int pr = 0; pr < 3; ++pr) {
@@ -464,7 +496,7 @@ void Problem::pass_faithful(int g, int prec, int mode, const double* zin, double
     for (int ax = 0; ax < 3; ++ax)
         for (int f = 0; f < 2; ++f) pd.excnt[ax][f] = excnt[ax][f];
     if (nb == 0) pd.bout_x = nullptr;
-    if (has_bc) {
+    if (has_bc || has_ss) {
         pd.Mentry_x = d_Mentry[0]; pd.Mentry_y = d_Mentry[1]; pd.Mentry_z = d_Mentry[2];
         pd.insrc_x = d_insrc[0]; pd.insrc_y = d_insrc[1]; pd.insrc_z = d_insrc[2];
         pd.bin_x = d_bin[0]; pd.bin_y = d_bin[1]; pd.bin_z = d_bin[2];
@@ -473,6 +505,15 @@ void Problem::pass_faithful(int g, int prec, int mode, const double* zin, double
             pd.bsin_x = bi;
             pd.bsin_y = ndims >= 2 ? bi + 2 * (size_t)Nps * npos[0] : nullptr;
             pd.bsin_z = ndims >= 3 ? bi + 2 * (size_t)Nps * (npos[0] + npos[1]) : nullptr;
+        }
+        if (has_ss && mode == MODE_T) {  // surface sources enter the inhomogeneous passes only
+            const size_t npf[3] = {(size_t)Ny * Nz, (size_t)Nx * Nz, (size_t)Nx * Ny};
+            const double** ps[3] = {&pd.ssrc_x, &pd.ssrc_y, &pd.ssrc_z};
+            for (int ax = 0; ax < 3; ++ax) {
+                *ps[ax] = d_ssrc[ax] ? d_ssrc[ax] + (size_t)g * Nps * npf[ax] : nullptr;
+                pd.ssrc_fs[ax] = (long)((size_t)G * Nps * npf[ax]);
+            }
+            pd.sface_x = d_sface[0]; pd.sface_y = d_sface[1]; pd.sface_z = d_sface[2];
         }
     }
     (void)prec;
@@ -896,8 +937,9 @@ void Problem::solve_group(int g, rt_group_stats& s) {
     std::memset(&s, 0, sizeof(s));
     // out-of-group + volume source (sn_flux.jl:175-179)
     launch_outsrc(d_qlout, d_phi, d_sigo, d_qvol, d_mat, d_act, g, G, Nmat, Nact, Np, Nc, st);
-    // zero-source shortcut (sn_one_speed.jl:116-120)
-    if (dot(d_qlout, d_qlout, (size_t)Nact * Nc) == 0.0) {
+    // zero-source shortcut (sn_one_speed.jl:116-120): no volume or scattering source and no
+    // surface source in the group
+    if (!(has_ss && grp_ss[g]) && dot(d_qlout, d_qlout, (size_t)Nact * Nc) == 0.0) {
         k_fill(d_phi + (size_t)g * Np * Nc, 0.0, (size_t)Np * Nc, st);
         s.niter = 0; s.resid = 0.0; s.rho = std::numeric_limits<double>::quiet_NaN(); s.converged = 1; s.passes = 0;
         if (verbose) print_group(g, s);

@@ -106,7 +106,10 @@ private:
     int* d_exlist[3] = {nullptr, nullptr, nullptr};
     int excnt[3][2] = {{0, 0}, {0, 0}, {0, 0}};
     // reflective / periodic faces (faithful FP64): entry M rows, inflow source faces, inflows
-    bool has_bc = false;
+    bool has_bc = false, has_ss = false;  // non-void faces; surface sources
+    double* d_ssrc[3] = {nullptr, nullptr, nullptr};   // per axis [2][G][Nps][npos]
+    int* d_sface[3] = {nullptr, nullptr, nullptr};     // per axis [Nd] entry face with a source, or -1
+    std::vector<char> grp_ss;                          // per group: some surface source is nonzero
     double* d_Mentry[3] = {nullptr, nullptr, nullptr};
     int* d_insrc[3] = {nullptr, nullptr, nullptr};
     double* d_bin[3] = {nullptr, nullptr, nullptr};

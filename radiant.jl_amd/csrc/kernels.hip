@@ -1239,10 +1239,11 @@ __global__ __launch_bounds__(256) void bndmom_kernel(const PassDesc* __restrict_
 }
 
 // ------------------------------------------------------------------------------------------
-// Incoming face fluxes of reflective / periodic boundaries (the boundary branches of
-// sn_sweep_{1,2,3}D.jl, e.g. sn_sweep_2D.jl:70-99): a direction entering through a face with a
-// non-void condition receives sum_p M_surf[n, p] * (exit half-range moments of the input state on
-// the same face for reflective, on the opposite face for periodic), summed in moment order.
+// Incoming face fluxes (the boundary branches of sn_sweep_{1,2,3}D.jl, e.g. sn_sweep_2D.jl:70-99):
+// a direction entering through a face receives sum_p M_surf[n, p] * (the face's surface-source
+// moments, in inhomogeneous passes) and then, for a non-void condition, sum_p M_surf[n, p] * (exit
+// half-range moments of the input state on the same face for reflective, on the opposite face for
+// periodic), each summed in moment order.
 // One thread per (axis, direction, position).
 // ------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void bndin_kernel(const PassDesc* __restrict__ pds) {
@@ -1262,7 +1263,13 @@ __global__ __launch_bounds__(256) void bndin_kernel(const PassDesc* __restrict__
         const double* bs = ax == 0 ? d.bsin_x : (ax == 1 ? d.bsin_y : d.bsin_z);
         double* out = ax == 0 ? d.bin_x : (ax == 1 ? d.bin_y : d.bin_z);
         const int src = insrc[n];
+        const double* ss = ax == 0 ? d.ssrc_x : (ax == 1 ? d.ssrc_y : d.ssrc_z);
+        const int* sf = ax == 0 ? d.sface_x : (ax == 1 ? d.sface_y : d.sface_z);
         double v = 0.0;
+        if (ss && sf[n] >= 0) {  // surface source of the entry face
+            const double* q = ss + (size_t)sf[n] * d.ssrc_fs[ax] + pos;
+            for (int p = 0; p < Nps; ++p) v += Me[(size_t)n * Nps + p] * q[(size_t)p * np[ax]];
+        }
         if (src >= 0 && bs)
             for (int p = 0; p < Nps; ++p) v += Me[(size_t)n * Nps + p] * bs[p + (size_t)Nps * (src + 2 * pos)];
         out[j] = v;

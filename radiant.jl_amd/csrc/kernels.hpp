@@ -45,6 +45,12 @@ struct PassDesc {
     const int *insrc_x, *insrc_y, *insrc_z;        // [Nd] face whose exit moments come back in (-1: none)
     const double *bsin_x, *bsin_y, *bsin_z;        // boundary moments of the input state (null: zero)
     double *bin_x, *bin_y, *bin_z;                 // incoming face fluxes [Nd][npos]
+    // surface sources of the current group (inhomogeneous passes; null = none): per axis the
+    // moments [2 faces][Nps][npos] at face stride ssrc_fs, and the face each direction enters
+    // through when that face carries a source (-1: none).  Their term comes before the boundary one.
+    const double *ssrc_x, *ssrc_y, *ssrc_z;
+    const int *sface_x, *sface_y, *sface_z;
+    long ssrc_fs[3];
 };
 
 enum Prec { P64 = 0, P32 = 1, PBF16 = 2,

@@ -31,7 +31,7 @@
 // Deep copy of a problem description (the caller's buffers may go away after submit).
 struct DescCopy {
     rt_problem_desc d{};
-    std::vector<double> dx, dy, dz, sigt, sigs, qvol, mu, eta, xi, w, M, D, Msurf[6], Dsurf[6];
+    std::vector<double> dx, dy, dz, sigt, sigs, qvol, mu, eta, xi, w, M, D, Msurf[6], Dsurf[6], ssrc[6];
     std::vector<int64_t> mat, pl, n2p[6];
     std::vector<int32_t> bc;
 
@@ -66,6 +66,12 @@ struct DescCopy {
                 }
                 if (s.Dsurf[f]) { Dsurf[f].assign(s.Dsurf[f], s.Dsurf[f] + (size_t)s.Nps * ndp); d.Dsurf[f] = Dsurf[f].data(); }
                 if (s.Msurf[f]) { Msurf[f].assign(s.Msurf[f], s.Msurf[f] + (size_t)s.Nps * ndp); d.Msurf[f] = Msurf[f].data(); }
+            }
+            d.ssrc[f] = nullptr;
+            if (f < 2 * s.ndims && s.ssrc[f] && s.Np_source > 0) {
+                const size_t npf = f < 2 ? (size_t)s.Ny * s.Nz : (f < 4 ? (size_t)s.Nx * s.Nz : (size_t)s.Nx * s.Ny);
+                ssrc[f].assign(s.ssrc[f], s.ssrc[f] + (size_t)s.G * s.Np_source * npf);
+                d.ssrc[f] = ssrc[f].data();
             }
         }
     }

@@ -36,6 +36,7 @@ class ProblemDesc(C.Structure):
         ("accel", C.c_int32), ("restart", C.c_int32), ("depth", C.c_int32), ("maxit", C.c_int32),
         ("tol", C.c_double),
         ("precision", C.c_int32), ("state_mode", C.c_int32), ("verbose", C.c_int32),
+        ("ssrc", dp * 6), ("Np_source", C.c_int32),
     ]
 
 

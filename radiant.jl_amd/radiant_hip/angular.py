@@ -530,9 +530,15 @@ def angular_polynomial_basis(mu, eta, xi, w, L, sn_type, qdims):
             raise RuntimeError("Galerkin: unable to find enough independent spherical harmonics")
 
 
-def surface_basis(mu, eta, xi, w, L, sn_type, qdims, face):
+def surface_basis(mu, eta, xi, w, L, sn_type, qdims, face, with_indices=False):
     """angular_polynomial_basis.jl:262-436 for face 0..5 (X-,X+,Y-,Y+,Z-,Z+).
-    Returns (Nps, M[Nd+,Nps], D[Nps,Nd+], n_to_np[Nd] (1-based, 0 = not entering))."""
+    Returns (Nps, M[Nd+,Nps], D[Nps,Nd+], n_to_np[Nd] (1-based, 0 = not entering)), and with
+    with_indices also the degree and order (pl, pm) of each half-range moment."""
+    out = _surface_basis(mu, eta, xi, w, L, sn_type, qdims, face)
+    return out if with_indices else out[:4]
+
+
+def _surface_basis(mu, eta, xi, w, L, sn_type, qdims, face):
     Nd = len(w)
     if qdims == 1:
         if face > 1:
@@ -556,7 +562,7 @@ def surface_basis(mu, eta, xi, w, L, sn_type, qdims, face):
         if sn_type == "standard":
             M = Pl[:, : L + 1].copy()
             D = (wp[:, None] * Pl[:, : L + 1]).T.copy()
-            return L + 1, M, D, n2p
+            return L + 1, M, D, n2p, np.arange(L + 1), np.zeros(L + 1, np.int64)
         if sn_type == "galerkin-m":
             M = Pl[:, :ndp].copy()
             D = np.linalg.inv(M)
@@ -565,17 +571,19 @@ def surface_basis(mu, eta, xi, w, L, sn_type, qdims, face):
             M = np.linalg.inv(D)
         else:
             raise ValueError("Unknown method.")
-        return ndp, M, D, n2p
+        return ndp, M, D, n2p, np.arange(ndp), np.zeros(ndp, np.int64)
     if sn_type == "standard":
         R = [real_half_range_sh_up_to_L(L, abs(mup[i]), php[i]) for i in range(ndp)]
-        rows = []
+        rows, lm = [], []
         for l in range(L + 1):
             for m in range(-l, l + 1):
                 if qdims == 2 and (l + m) % 2 == 1:
                     continue
                 rows.append([R[i][l][m + l] for i in range(ndp)])
+                lm.append((l, m))
         H = np.array(rows)
-        return len(rows), H.T.copy(), H * wp[None, :], n2p
+        return (len(rows), H.T.copy(), H * wp[None, :], n2p, np.array([a for a, _ in lm]),
+                np.array([b for _, b in lm]))
     if not gal:
         raise ValueError("Unknown method.")
     gt = 1 if sn_type == "galerkin-d" else 3
@@ -584,8 +592,8 @@ def surface_basis(mu, eta, xi, w, L, sn_type, qdims, face):
         R = [real_half_range_sh_up_to_L(Lg, abs(mup[i]), php[i]) for i in range(ndp)]
         out = _gram_schmidt(ndp, Lg, R, wp, qdims, gt)
         if out is not None:
-            M, D, _, _ = out
-            return ndp, M, D, n2p
+            M, D, pl, pm = out
+            return ndp, M, D, n2p, np.asarray(pl), np.asarray(pm)
         Lg += 1
         if Lg > Lb + ndp + 10:
             raise RuntimeError("Surface Galerkin: unable to find enough independent half-range harmonics")
@@ -602,8 +610,11 @@ class AngularSet:
         self.Np, self.M, self.D, self.pl, self.pm = angular_polynomial_basis(
             self.mu, self.eta, self.xi, self.w, L, sn_type, Q)
         self.faces = []
+        self.face_lm = []  # (pl, pm) of each face's half-range moments
         for f in range(2 * ndims):
-            self.faces.append(surface_basis(self.mu, self.eta, self.xi, self.w, L, sn_type, Q, f))
+            sb = surface_basis(self.mu, self.eta, self.xi, self.w, L, sn_type, Q, f, with_indices=True)
+            self.faces.append(sb[:4])
+            self.face_lm.append(sb[4:])
         self.Nps = self.faces[-1][0]
         if any(f[0] != self.Nps for f in self.faces):
             raise ValueError("faces with different half-range basis sizes are not supported")

@@ -82,6 +82,10 @@ class Problem:
     tol: float = 1e-10
     name: str = ""
     meta: dict = field(default_factory=dict)
+    # surface sources: face (0..5 = X-, X+, Y-, Y+, Z-, Z+) -> moments [G, Np_source, npos]
+    # (radiant_hip.sources.add_surface_source builds them)
+    ssrc: dict = None
+    Np_source: int = 0
 
     @property
     def Nc(self):

@@ -59,6 +59,11 @@ class Desc:
             d.Msurf[f] = ptr(k[f"Ms{f}"])
             d.Dsurf[f] = ptr(k[f"Ds{f}"])
             d.n_to_np[f] = ptr(k[f"n2p{f}"], _lib.lp)
+        if getattr(prob, "ssrc", None):  # surface sources [G, Np_source, npos] per face, position fastest
+            for f, arr in prob.ssrc.items():
+                k[f"ssrc{f}"] = np.ascontiguousarray(arr, np.float64).ravel()
+                d.ssrc[f] = ptr(k[f"ssrc{f}"])
+            d.Np_source = int(prob.Np_source)
         d.accel = ACCEL[accel or prob.accel]
         d.restart = restart if restart is not None else prob.restart
         d.depth = depth if depth is not None else prob.depth
