@@ -90,6 +90,13 @@ typedef struct rt_problem_desc {
      * do not use surface sources leave these fields zero. */
     const double *ssrc[6];
     int32_t Np_source;
+    /* Continuous slowing down (SN solver type "BCSD" with the DD scheme of order 1 in energy;
+     * sn_{1,2,3}D_bfp.jl, sn_flux.jl:86-97,166,183-188): 0 = off.  Sb: stopping powers at the group
+     * boundaries [G+1][Nmat] column-major (cross_sections.get_boundary_stopping_powers), dE: group
+     * widths [G] (get_energy_width).  Single-problem solves only (not the lockstep batch). */
+    int32_t csd;
+    const double *Sb;
+    const double *dE;
 } rt_problem_desc;
 
 /* Per-group result of the in-group solve (the values sn_one_speed.jl:189-193 prints). */

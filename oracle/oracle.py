@@ -30,6 +30,7 @@ class OracleProblem(C.Structure):
         ("restart", C.c_int32), ("depth", C.c_int32), ("maxit", C.c_int32), ("tol", C.c_double),
         ("nthreads", C.c_int32), ("verbose", C.c_int32),
         ("ssrc", C.POINTER(C.c_double) * 6), ("Np_source", C.c_int32),
+        ("csd", C.c_int32), ("Sb", C.POINTER(C.c_double)), ("dE", C.POINTER(C.c_double)),
     ]
 
 
@@ -162,6 +163,10 @@ def make_problem(prob, nthreads=1, verbose=0, **over):
             k.ss[f] = np.ascontiguousarray(arr, np.float64).ravel()
             op.ssrc[f] = _p(k.ss[f])
         op.Np_source = int(prob.Np_source)
+    if getattr(prob, "csd", False):
+        k.Sb = np.ascontiguousarray(np.asarray(prob.Sb, np.float64).reshape(-1, order="F"))
+        k.dE = np.ascontiguousarray(prob.dE, np.float64)
+        op.csd, op.Sb, op.dE = 1, _p(k.Sb), _p(k.dE)
     op._keep = k
     return op
 

@@ -842,6 +842,11 @@ struct oracle_problem {
     // surface sources (surface_source.jl): per face, NULL or [G][Np_source][npos_f], position fastest
     const double* ssrc[6];
     int32_t Np_source;
+    // continuous slowing down (SN solver type "BCSD", DD of order 1 in energy): 0 = off; Sb the
+    // stopping powers at the group boundaries [G+1, Nmat] (column-major), dE the group widths [G]
+    int32_t csd;
+    const double* Sb;
+    const double* dE;
 };
 
 struct oracle_stats {
@@ -864,6 +869,10 @@ struct Solver {
     vector<double> sigs;        // [Nmat][G][G][Ls+1] padded copy: sigs[m + Nmat*(gi + G*(gf + G*l))]
     vector<int> matc;           // 0-based material per cell
     const double* cur_src[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // this pass's surface sources
+    // continuous slowing down: incoming / outgoing energy-face angular flux [n*Nc + c] (𝚽E12 of
+    // sn_flux.jl:166), the current group's S⁻ and S⁺ per material (stopping powers / ΔE)
+    vector<double> psiE_in, psiE_out, Sminus, Splus;
+    const double* curE = nullptr;  // psiE_in in inhomogeneous passes, null (zero) in homogeneous ones
     // state layout: [Φ (Np*Nc, p fastest) | x-bnd | y-bnd | z-bnd]
     size_t nphi, nbx, nby, nbz, S;
     vector<int> nxm, nxp, nym, nyp, nzm, nzp;  // n_to_n+ per face (1-based or 0)
@@ -917,6 +926,11 @@ struct Solver {
         }
         for (int f = 0; f < nf; ++f) {
             if (p.bc[f] < 0 || p.bc[f] > 2) fail("Unknown boundary condition type.");
+            if (f == 0 && p.csd) {
+                if (!p.Sb || !p.dE) fail("continuous slowing down needs the boundary stopping powers and group widths");
+                for (int g = 0; g < p.G; ++g)
+                    if (!(p.dE[g] > 0)) fail("energy group widths must be positive");
+            }
             if (p.ssrc[f] && (p.Np_source < 0 || p.Np_source > A.Nps))
                 fail("surface-source moments exceed the half-range basis of the face");
         }
@@ -971,7 +985,15 @@ struct Solver {
             double hx = std::fabs(mue) / P.dx[ix];
             double Sm = St + 2.0 * hx;
             double Q = Qn + 2.0 * hx * fx;
+            double Ein = 0.0;
+            if (P.csd) {  // flux_1D_BFP with O = 1 in space and energy (sn_1D_bfp.jl)
+                const int m = matc[c];
+                Ein = curE ? curE[(size_t)n * Nc + c] : 0.0;
+                Sm = Sm + 2.0 * Splus[m];
+                Q = Q + (Splus[m] + Sminus[m]) * Ein;
+            }
             double psi = Q / Sm;
+            if (P.csd) psiE_out[(size_t)n * Nc + c] = 2.0 * psi - Ein;
             if (!tang) fx = -fx + 2.0 * psi;
             for (int p = 0; p < Np; ++p) phi[p + (size_t)Np * c] += A.B.D[p + (size_t)Np * n] * psi;
         }
@@ -1039,19 +1061,30 @@ struct Solver {
                     double& fxr = Fx[iy + (size_t)Ny * iz];
                     double& fyr = Fy[iz];
                     double psi;
+                    // flux_2D_BFP / flux_3D_BFP with O = 1: the energy terms come after the spatial ones
+                    const double Ein = (P.csd && curE) ? curE[(size_t)n * Nc + c] : 0.0;
                     if (ndims == 2) {
                         double Sm = St + 2.0 * hx;
                         Sm = Sm + 2.0 * hy;
                         double Q = Qn + 2.0 * hx * fxr;
                         Q = Q + 2.0 * hy * fyr;
+                        if (P.csd) {
+                            Sm = Sm + 2.0 * Splus[matc[c]];
+                            Q = Q + (Splus[matc[c]] + Sminus[matc[c]]) * Ein;
+                        }
                         psi = Q / Sm;
                     } else {
                         double hz = std::fabs(xi) / P.dz[iz];
                         double Sm = ((St + 2.0 * hx) + 2.0 * hy) + 2.0 * hz;
                         double Q = ((Qn + 2.0 * hx * fxr) + 2.0 * hy * fyr) + 2.0 * hz * Fz;
+                        if (P.csd) {
+                            Sm = Sm + 2.0 * Splus[matc[c]];
+                            Q = Q + (Splus[matc[c]] + Sminus[matc[c]]) * Ein;
+                        }
                         psi = Q / Sm;
                         Fz = -Fz + 2.0 * psi;
                     }
+                    if (P.csd) psiE_out[(size_t)n * Nc + c] = 2.0 * psi - Ein;
                     fxr = -fxr + 2.0 * psi;
                     fyr = -fyr + 2.0 * psi;
                     for (int p = 0; p < Np; ++p) phi[p + (size_t)Np * c] += A.B.D[p + (size_t)Np * n] * psi;
@@ -1088,6 +1121,17 @@ struct Solver {
                                (size_t)Nx * Ny, (size_t)Nx * Ny};
         for (int f = 0; f < 6; ++f)
             cur_src[f] = (!homog && f < 2 * ndims && P.ssrc[f]) ? P.ssrc[f] + (size_t)g * P.Np_source * npf[f] : nullptr;
+        if (P.csd) {  // sn_flux.jl:183-188 (S / ΔE); the energy inflow is a fixed source (sn_inner_pass.jl:38)
+            Sminus.assign(Nmat, 0.0);
+            Splus.assign(Nmat, 0.0);
+            for (int m = 0; m < Nmat; ++m) {
+                Sminus[m] = P.Sb[g + (size_t)(G + 1) * m] / P.dE[g];
+                Splus[m] = P.Sb[(g + 1) + (size_t)(G + 1) * m] / P.dE[g];
+            }
+            if (psiE_in.size() != (size_t)A.Nd * Nc) psiE_in.assign((size_t)A.Nd * Nc, 0.0);
+            psiE_out.assign((size_t)A.Nd * Nc, 0.0);
+            curE = homog ? nullptr : psiE_in.data();
+        }
         // Ql = (homog ? 0 : Qlout) + Σs_l Φl   (scattering_source.jl:25-30)
         for (int c = 0; c < Nc; ++c) {
             int m = matc[c];
@@ -1171,7 +1215,10 @@ struct Solver {
             for (size_t i = 0; i < (size_t)P.Np_source * npf[f]; ++i)
                 if (q[i] != 0.0) { any = true; break; }
         }
+        if (P.csd && !any)  // the incoming energy flux is a source too
+            for (double v : psiE_in) if (v != 0.0) { any = true; break; }
         if (!any) {  // sn_one_speed.jl:116-120
+            if (P.csd) psiE_out.assign((size_t)A.Nd * Nc, 0.0);
             std::fill(z_phi_bnd.begin(), z_phi_bnd.begin() + nphi, 0.0);
             st.niter = 0; st.resid = 0.0; st.rho = NAN; st.converged = 1; st.passes = 0;
             return;
@@ -1458,6 +1505,11 @@ struct Solver {
             for (int c = 0; c < Nc; ++c) Qlout[(size_t)Np * c] += P.qvol[g + (size_t)G * c];
             vector<double> z(S, 0.0);
             std::copy(phi_all.begin() + (size_t)g * nphi, phi_all.begin() + (size_t)(g + 1) * nphi, z.begin());
+            if (P.csd) {  // sn_flux.jl:166,183: zero for the first group, then rescaled by ΔE_g/ΔE_{g-1}
+                if (g == 0) psiE_in.assign((size_t)A.Nd * Nc, 0.0);
+                else
+                    for (size_t i = 0; i < psiE_in.size(); ++i) psiE_in[i] = psiE_out[i] * P.dE[g] / P.dE[g - 1];
+            }
             oracle_stats st{};
             one_speed(g, z, Qlout, st);
             std::copy(z.begin(), z.begin() + nphi, phi_all.begin() + (size_t)g * nphi);

@@ -116,6 +116,7 @@ int BatchSolver::add(Problem* P) {
         throw Error(RT_EINVAL, "the batched solver runs the mixed-precision modes");
     if (P->scan_cpl <= 0 || sweep_scan_lds_bytes(P64, P->ndims, P->Nx, P->Ny, P->Nz, P->Nmat) > 160 * 1024)
         throw Error(RT_EINVAL, "a batched problem does not fit the row-scan sweep");
+    if (P->csd) throw Error(RT_EINVAL, "continuous slowing down is solved one problem at a time");
     int slot = -1;
     for (int i = 0; i < (int)ps.size(); ++i)
         if (!ps[i].P) {

@@ -89,6 +89,11 @@ static void validate(const rt_problem_desc& d) {
     }
     if (surf && (d.Np_source < 1 || d.Np_source > d.Nps))
         bad("Np_source must lie in 1..Nps when a surface source is given");
+    if (d.csd) {
+        if (!d.Sb || !d.dE) bad("continuous slowing down needs the boundary stopping powers and group widths");
+        for (int g = 0; g < d.G; ++g)
+            if (!(d.dE[g] > 0)) bad("energy group widths must be positive");
+    }
     const size_t nc = (size_t)d.Nx * d.Ny * d.Nz;
     for (size_t c = 0; c < nc; ++c)
         if (d.mat[c] < 1 || d.mat[c] > d.Nmat) bad("material index out of range at voxel " + std::to_string(c));
@@ -144,6 +149,21 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
         std::vector<double> sigt((size_t)G * Nmat);
         for (int g = 0; g < G; ++g)
             for (int m = 0; m < Nmat; ++m) sigt[(size_t)g * Nmat + m] = d.sigt[g + (size_t)G * m];
+        csd = d.csd != 0;
+        if (csd) {
+            // continuous slowing down (sn_flux.jl:86-97,183-188): S⁻ = Sb(g)/ΔE, S⁺ = Sb(g+1)/ΔE; the
+            // cell removal gains 2 S⁺ and the energy inflow enters with (S⁺ + S⁻)
+            std::vector<double> K((size_t)G * Nmat);
+            for (int g = 0; g < G; ++g)
+                for (int m = 0; m < Nmat; ++m) {
+                    const double Sm = d.Sb[g + (size_t)(G + 1) * m] / d.dE[g];
+                    const double Sp = d.Sb[(g + 1) + (size_t)(G + 1) * m] / d.dE[g];
+                    sigt[(size_t)g * Nmat + m] += 2.0 * Sp;
+                    K[(size_t)g * Nmat + m] = Sp + Sm;
+                }
+            d_csdK = dalloc<double>(K.size()); upload(d_csdK, K, st);
+            dE.assign(d.dE, d.dE + G);
+        }
         d_sigt = dalloc<double>(sigt.size()); upload(d_sigt, sigt, st);
         auto S = [&](int m, int gi, int gf, int64_t l) -> double {
             if (l > d.Ls) return 0.0;
@@ -274,6 +294,12 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
         k_fill(d_phi, 0.0, (size_t)G * Np * Nc, st);
         d_qlout = dalloc<double>((size_t)Nact * Nc);
         d_psi = dalloc<double>((size_t)Nd * Nc);
+        if (csd) {
+            d_psiE_in = dalloc<double>((size_t)Nd * Nc);
+            d_psiE_out = dalloc<double>((size_t)Nd * Nc);
+            k_fill(d_psiE_in, 0.0, (size_t)Nd * Nc, st);
+            k_fill(d_psiE_out, 0.0, (size_t)Nd * Nc, st);
+        }
         for (int ax = 0; ax < 3; ++ax) d_bex[ax] = dalloc<double>(std::max<size_t>(1, (size_t)Nd * (ax < ndims ? npos[ax] : 1)));
         d_bscratch = dalloc<double>(std::max<size_t>(1, nb));
         d_pd = dalloc<PassDesc>(1);
@@ -448,6 +474,7 @@ void Problem::launch_pass(PassDesc& pd, int prec, bool full_rows) {
         ev_used.push_back({a, b, kernel, prec});
     };
     timed(0, [&] { launch_mgemm(prec, d_pd, d_mg, n_mg, st); });
+    if (pd.psiE_in) launch_csd_source(d_pd, (long)Nd * Nc, st);  // energy inflow (FP64 passes)
     timed(1, [&] {
         const int sp = prec == PBF16 ? P32 : prec;
         if (sweep_kind == 0 && scan_cpl > 0 && sweep_scan_lds_bytes(sp, ndims, Nx, Ny, Nz, Nmat) <= 160 * 1024)
@@ -461,6 +488,7 @@ void Problem::launch_pass(PassDesc& pd, int prec, bool full_rows) {
         else
             launch_sweep(sp, d_pd, d_sw, Nd, sw_ppt, sw_nbuf[prec], sw_block, sw_lds[prec], st);
     });
+    if (pd.psiE_out) launch_csd_out(d_pd, (long)Nd * Nc, st);
     timed(2, [&] { launch_dgemm(prec, d_pd, full_rows ? d_dgf : d_dga, full_rows ? n_dgf : n_dga, st); });
     if (pd.bout_x) timed(3, [&] { launch_bndmom(d_pd, 1, (int)nb, st); });
     counters.passes += 1;
@@ -480,6 +508,7 @@ void Problem::pass_faithful(int g, int prec, int mode, const double* zin, double
     pd.Drows = d_D64;
     pd.out = zout;
     pd.zin = (mode == MODE_IMA) ? zin : nullptr;
+    if (mode == MODE_T) set_csd(pd, g);
     double* b = zout + (size_t)Np * Nc;
     pd.bout_x = b;
     pd.bout_y = ndims >= 2 ? b + 2 * (size_t)Nps * npos[0] : nullptr;
@@ -532,6 +561,7 @@ void Problem::desc_reduced(PassDesc& pd, int g, int prec, int mode, const double
     pd.Drows = (prec == P64) ? (const void*)d_Dact64 : (const void*)d_Dact32;
     pd.out = zout;
     pd.zin = (mode == MODE_IMA) ? zin : nullptr;
+    if (mode == MODE_T && prec == P64) set_csd(pd, g);
 }
 
 void Problem::pass_reduced(int g, int prec, int mode, const double* zin, double* zout) {
@@ -551,6 +581,16 @@ void Problem::desc_reconstruct(PassDesc& pd, int g, const double* zin_reduced) {
     pd.Drows = d_D64;
     pd.out = d_phi + (size_t)g * Np * Nc;
     pd.zin = nullptr;
+    set_csd(pd, g);
+}
+
+// Continuous slowing down in an inhomogeneous pass: the energy inflow psiE_in is a fixed source
+// (zero in homogeneous passes, sn_inner_pass.jl:38) and the outgoing energy flux is kept.
+void Problem::set_csd(PassDesc& pd, int g) {
+    if (!csd) return;
+    pd.psiE_in = d_psiE_in;
+    pd.psiE_out = d_psiE_out;
+    pd.csdK = d_csdK + (size_t)g * Nmat;
 }
 
 void Problem::pass_reconstruct(int g, const double* zin_reduced) {
@@ -937,9 +977,18 @@ void Problem::solve_group(int g, rt_group_stats& s) {
     std::memset(&s, 0, sizeof(s));
     // out-of-group + volume source (sn_flux.jl:175-179)
     launch_outsrc(d_qlout, d_phi, d_sigo, d_qvol, d_mat, d_act, g, G, Nmat, Nact, Np, Nc, st);
-    // zero-source shortcut (sn_one_speed.jl:116-120): no volume or scattering source and no
-    // surface source in the group
-    if (!(has_ss && grp_ss[g]) && dot(d_qlout, d_qlout, (size_t)Nact * Nc) == 0.0) {
+    if (csd) {  // the energy flux enters group g rescaled by ΔE_g/ΔE_{g-1}; zero for the first (sn_flux.jl:166,183)
+        if (g == 0) k_fill(d_psiE_in, 0.0, (size_t)Nd * Nc, st);
+        else {
+            k_copy(d_psiE_in, d_psiE_out, (size_t)Nd * Nc, st);
+            k_scale(d_psiE_in, dE[g] / dE[g - 1], (size_t)Nd * Nc, st);
+        }
+    }
+    // zero-source shortcut (sn_one_speed.jl:116-120): no volume or scattering source, no surface
+    // source and no incoming energy flux in the group
+    if (!(has_ss && grp_ss[g]) && !(csd && dot(d_psiE_in, d_psiE_in, (size_t)Nd * Nc) != 0.0) &&
+        dot(d_qlout, d_qlout, (size_t)Nact * Nc) == 0.0) {
+        if (csd) k_fill(d_psiE_out, 0.0, (size_t)Nd * Nc, st);
         k_fill(d_phi + (size_t)g * Np * Nc, 0.0, (size_t)Np * Nc, st);
         s.niter = 0; s.resid = 0.0; s.rho = std::numeric_limits<double>::quiet_NaN(); s.converged = 1; s.passes = 0;
         if (verbose) print_group(g, s);

@@ -110,6 +110,12 @@ private:
     double* d_ssrc[3] = {nullptr, nullptr, nullptr};   // per axis [2][G][Nps][npos]
     int* d_sface[3] = {nullptr, nullptr, nullptr};     // per axis [Nd] entry face with a source, or -1
     std::vector<char> grp_ss;                          // per group: some surface source is nonzero
+    // continuous slowing down: (S⁻ + S⁺) per group and material, energy-face angular fluxes, ΔE
+    bool csd = false;
+    double* d_csdK = nullptr;
+    double *d_psiE_in = nullptr, *d_psiE_out = nullptr;
+    std::vector<double> dE;
+    void set_csd(PassDesc& pd, int g);
     double* d_Mentry[3] = {nullptr, nullptr, nullptr};
     int* d_insrc[3] = {nullptr, nullptr, nullptr};
     double* d_bin[3] = {nullptr, nullptr, nullptr};

@@ -1277,6 +1277,28 @@ __global__ __launch_bounds__(256) void bndin_kernel(const PassDesc* __restrict__
 }
 
 // ------------------------------------------------------------------------------------------
+// Continuous slowing down with the DD scheme of order 1 in energy (flux_{1,2,3}D_BFP of
+// sn_{1,2,3}D_bfp.jl with O = 1): the cell solve gains 2 S⁺ in its removal (folded into the
+// group's Σt table by the host) and (S⁺ + S⁻) psiE_in in its source, which enters the angular
+// source of the sweep here; the energy face leaves with psiE_out = 2 psi - psiE_in.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void csd_source_kernel(const PassDesc* __restrict__ pds, long n) {
+    const PassDesc& d = pds[0];
+    double* psi = (double*)d.psi;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const long c = i % d.Nc;
+        psi[i] += d.csdK[d.mat[c]] * d.psiE_in[i];
+    }
+}
+
+__global__ __launch_bounds__(256) void csd_out_kernel(const PassDesc* __restrict__ pds, long n) {
+    const PassDesc& d = pds[0];
+    const double* psi = (const double*)d.psi;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        d.psiE_out[i] = 2.0 * psi[i] - d.psiE_in[i];
+}
+
+// ------------------------------------------------------------------------------------------
 // Out-of-group + volume source (scattering_source.jl:59-68, sn_flux.jl:179).
 // ------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void outsrc_kernel(double* __restrict__ qlout, const double* __restrict__ phi_all,
@@ -1719,6 +1741,15 @@ void launch_sweep_scan(int prec, const PassDesc* d_pd, const SweepTask* tasks, i
             default: scan_any<8, false>(kind, d_pd, tasks, ntasks, waves, lds, st); break;
         }
     }
+    RT_HIP(hipGetLastError());
+}
+
+void launch_csd_source(const PassDesc* d_pd, long n, hipStream_t st) {
+    hipLaunchKernelGGL(csd_source_kernel, dim3(grid_for((size_t)n)), dim3(256), 0, st, d_pd, n);
+    RT_HIP(hipGetLastError());
+}
+void launch_csd_out(const PassDesc* d_pd, long n, hipStream_t st) {
+    hipLaunchKernelGGL(csd_out_kernel, dim3(grid_for((size_t)n)), dim3(256), 0, st, d_pd, n);
     RT_HIP(hipGetLastError());
 }
 

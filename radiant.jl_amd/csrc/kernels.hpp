@@ -51,6 +51,11 @@ struct PassDesc {
     const double *ssrc_x, *ssrc_y, *ssrc_z;
     const int *sface_x, *sface_y, *sface_z;
     long ssrc_fs[3];
+    // continuous slowing down (FP64 inhomogeneous passes; null = off): the incoming and outgoing
+    // energy-face angular fluxes [Nd][Nc] and the group's (S⁻ + S⁺) per material
+    const double* psiE_in;
+    double* psiE_out;
+    const double* csdK;
 };
 
 enum Prec { P64 = 0, P32 = 1, PBF16 = 2,
@@ -90,6 +95,10 @@ void launch_sweep_scan(int prec, const PassDesc* d_pd, const SweepTask* tasks, i
                        hipStream_t st, bool bin = false);
 // incoming face fluxes of non-void boundaries (PassDesc::bin_*), one problem
 void launch_bndin(const PassDesc* d_pd, int total, hipStream_t st);
+// continuous slowing down, one problem: psi += K[mat] * psiE_in before the sweep; psiE_out = 2 psi -
+// psiE_in after it (FP64 angular fluxes [Nd][Nc])
+void launch_csd_source(const PassDesc* d_pd, long n, hipStream_t st);
+void launch_csd_out(const PassDesc* d_pd, long n, hipStream_t st);
 void launch_dgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st);
 void launch_bndmom(const PassDesc* d_pd, int nprob, int max_items, hipStream_t st);
 

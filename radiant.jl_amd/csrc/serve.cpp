@@ -31,7 +31,7 @@
 // Deep copy of a problem description (the caller's buffers may go away after submit).
 struct DescCopy {
     rt_problem_desc d{};
-    std::vector<double> dx, dy, dz, sigt, sigs, qvol, mu, eta, xi, w, M, D, Msurf[6], Dsurf[6], ssrc[6];
+    std::vector<double> dx, dy, dz, sigt, sigs, qvol, mu, eta, xi, w, M, D, Msurf[6], Dsurf[6], ssrc[6], Sb, dE;
     std::vector<int64_t> mat, pl, n2p[6];
     std::vector<int32_t> bc;
 
@@ -55,6 +55,12 @@ struct DescCopy {
         d.mat = mat.data(); d.bc = bc.data(); d.sigt = sigt.data(); d.sigs = sigs.data(); d.qvol = qvol.data();
         d.mu = mu.data(); d.eta = eta.data(); d.xi = xi.data(); d.w = w.data(); d.M = M.data(); d.D = D.data();
         d.pl = pl.data();
+        if (s.csd) {
+            cp(Sb, s.Sb, (size_t)(s.G + 1) * s.Nmat);
+            cp(dE, s.dE, s.G);
+            d.Sb = s.Sb ? Sb.data() : nullptr;
+            d.dE = s.dE ? dE.data() : nullptr;
+        }
         for (int f = 0; f < 6; ++f) {
             d.Msurf[f] = nullptr; d.Dsurf[f] = nullptr; d.n_to_np[f] = nullptr;
             if (f < 2 * s.ndims) {

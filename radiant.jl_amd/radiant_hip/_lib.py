@@ -37,6 +37,7 @@ class ProblemDesc(C.Structure):
         ("tol", C.c_double),
         ("precision", C.c_int32), ("state_mode", C.c_int32), ("verbose", C.c_int32),
         ("ssrc", dp * 6), ("Np_source", C.c_int32),
+        ("csd", C.c_int32), ("Sb", dp), ("dE", dp),
     ]
 
 

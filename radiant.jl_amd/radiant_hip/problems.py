@@ -86,6 +86,11 @@ class Problem:
     # (radiant_hip.sources.add_surface_source builds them)
     ssrc: dict = None
     Np_source: int = 0
+    # continuous slowing down (SN solver "BCSD", DD of order 1 in energy): stopping powers at the
+    # group boundaries Sb [G+1, Nmat] and group widths dE [G]
+    csd: bool = False
+    Sb: np.ndarray = None
+    dE: np.ndarray = None
 
     @property
     def Nc(self):

@@ -64,6 +64,10 @@ class Desc:
                 k[f"ssrc{f}"] = np.ascontiguousarray(arr, np.float64).ravel()
                 d.ssrc[f] = ptr(k[f"ssrc{f}"])
             d.Np_source = int(prob.Np_source)
+        if getattr(prob, "csd", False):  # continuous slowing down: Sb [G+1, Nmat], dE [G]
+            k["Sb"] = f64(prob.Sb)
+            k["dE"] = f64(prob.dE)
+            d.csd, d.Sb, d.dE = 1, ptr(k["Sb"]), ptr(k["dE"])
         d.accel = ACCEL[accel or prob.accel]
         d.restart = restart if restart is not None else prob.restart
         d.depth = depth if depth is not None else prob.depth
