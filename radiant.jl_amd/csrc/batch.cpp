@@ -137,6 +137,7 @@ int BatchSolver::add(Problem* P) {
     s.z = P->vec("bz", S);
     s.r = P->vec("br", S);
     s.t = P->vec("bt", S);
+    s.psiC = P->vec("bpsiC", (size_t)P->Nd * P->Nc);
     RT_CHECK(hipMemcpy(P->d_Vptr, s.V.data(), (s.m + 1) * sizeof(double*), hipMemcpyHostToDevice));
     s.stats.assign(P->G, rt_group_stats{});
     try {
@@ -307,7 +308,7 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
         std::map<std::pair<int, int>, int> soff;
         std::map<std::pair<int, int>, size_t> slds;
         std::map<std::pair<int, int>, double> s_upd;
-        double g_upd = 0, g_mflop = 0, g_dflop = 0;
+        double g_upd = 0, g_mflop = 0, g_dflop = 0, d_upd = 0;
         for (size_t i = 0; i < who.size(); ++i) {
             if (group_of(prec[i]) != pr) continue;
             const Problem* P = ps[who[i]].P;
@@ -315,36 +316,38 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
             const int nrg_m = (P->Nd + GEMM_ROWS - 1) / GEMM_ROWS;
             const int rows = full[i] ? P->Np : P->Nact;
             const int nrg_d = (rows + GEMM_ROWS - 1) / GEMM_ROWS;
-            mmeta.insert(mmeta.end(), {who[i], ntile, nrg_m, moff});
-            moff += ntile * nrg_m;
             dmeta.insert(dmeta.end(), {who[i], ntile, nrg_d, doff});
             doff += ntile * nrg_d;
+            const double upd = (double)P->Nc * P->Nd * ((pr == PMIX && prec[i] == P64) ? 2.0 : 1.0);
+            d_upd += upd;
+            g_dflop += 2.0 * rows * upd;
+            if (full[i] == 2) continue;  // contraction only: no moment->direction contraction, no sweep
+            mmeta.insert(mmeta.end(), {who[i], ntile, nrg_m, moff});
+            moff += ntile * nrg_m;
             const std::pair<int, int> key(P->scan_cpl, P->scan_waves);
             auto& v = sweeps[key];
             v.insert(v.end(), {who[i], P->Nd, soff[key]});
             soff[key] += P->Nd;
-            const double upd = (double)P->Nc * P->Nd * ((pr == PMIX && prec[i] == P64) ? 2.0 : 1.0);
             g_upd += upd;
             g_mflop += 2.0 * P->Nact * upd;
-            g_dflop += 2.0 * rows * upd;
             s_upd[key] += upd;
             const int sp = pr == PBF16 ? (int)P32 : (pr == PMIX ? (int)P64 : pr);
             slds[key] = std::max(slds[key], sweep_scan_lds_bytes(sp, P->ndims, P->Nx, P->Ny, P->Nz, P->Nmat));
         }
-        if (mmeta.empty()) continue;
+        if (dmeta.empty()) continue;
         // task tables expanded on the device from the small per-problem metadata
-        const int nm = (int)mmeta.size() / 4;
-        const int* dmm = stage(mmeta.data(), mmeta.size());
+        const int nm = (int)mmeta.size() / 4, nd = (int)dmeta.size() / 4;
+        const int* dmm = nm ? stage(mmeta.data(), mmeta.size()) : nullptr;
         const int* ddm = stage(dmeta.data(), dmeta.size());
         GemmTask *mg = d_mg[tix], *dg = d_dg[tix];
         SweepTask* swbase = d_sw[tix];
         hipStream_t s = (fork && pr == P64) ? st64 : st;
         const int ps_ = pr;
         plan.push_back([=] {
-            launch_expand_gemm(dmm, nm, mg, s);
-            launch_expand_gemm(ddm, nm, dg, s);
+            if (nm) launch_expand_gemm(dmm, nm, mg, s);
+            launch_expand_gemm(ddm, nd, dg, s);
         });
-        plan.push_back(timed(0, tix, g_upd, g_mflop, s, [=] { launch_mgemm(ps_, dpd, mg, moff, s); }));
+        if (nm) plan.push_back(timed(0, tix, g_upd, g_mflop, s, [=] { launch_mgemm(ps_, dpd, mg, moff, s); }));
         for (auto& kv : sweeps) {
             const std::vector<int>& sm = kv.second;
             const int* dsm = stage(sm.data(), sm.size());
@@ -359,7 +362,7 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
             }));
             swbase += ntask;
         }
-        plan.push_back(timed(2, tix, g_upd, g_dflop, s, [=] { launch_dgemm(ps_, dpd, dg, doff, s); }));
+        plan.push_back(timed(2, tix, d_upd, g_dflop, s, [=] { launch_dgemm(ps_, dpd, dg, doff, s); }));
     }
     if (fork) {
         hipStream_t s0 = st, s1 = st64;
@@ -373,7 +376,7 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
         Problem* P = ps[who[i]].P;
         P->counters.passes += 1;
         if (prec[i] != P64) P->counters.passes_lo += 1;
-        P->counters.updates_raw += (double)P->Nc * P->Nd;
+        if (full[i] != 2) P->counters.updates_raw += (double)P->Nc * P->Nd;  // sweeps only
     }
 }
 
@@ -408,10 +411,16 @@ bool BatchSolver::round() {
         Problem* P = s.P;
         PassDesc pd;
         switch (s.phase) {
-            case CPASS: P->desc_reduced(pd, s.g, P64, MODE_T, nullptr, s.c); prec.push_back(P64); full.push_back(0); break;
+            case CPASS:  // c = T(0); its angular flux stays in psiC for the reconstruction
+                P->desc_reduced(pd, s.g, P64, MODE_T, nullptr, s.c);
+                pd.psi = s.psiC;
+                prec.push_back(P64); full.push_back(0);
+                break;
             case ARNOLDI: P->desc_reduced(pd, s.g, s.lo, MODE_IMA, s.V[s.j], s.w); prec.push_back(s.lo); full.push_back(0); break;
             case RESID: P->desc_reduced(pd, s.g, P64, MODE_IMA, s.z, s.t); prec.push_back(P64); full.push_back(0); break;
-            case RECON: P->desc_reconstruct(pd, s.g, s.z); prec.push_back(P64); full.push_back(1); break;
+            // T(z) = A z + T(0): the angular flux of A z is still in psi from the residual pass on the
+            // same z, psiC was added to it before this round, so only the full-row D contraction runs
+            case RECON: P->desc_reconstruct(pd, s.g, s.z); prec.push_back(P64); full.push_back(2); break;
             default: continue;
         }
         who.push_back(i);
@@ -642,6 +651,11 @@ void BatchSolver::advance(int i, std::vector<VecItem>& pre, std::vector<VecItem>
             if (s.resid < P->tol || s.refs >= 12 || s.niter >= P->maxit) {
                 s.conv = s.resid < P->tol;
                 s.phase = RECON;
+                // psi (angular flux of A z from this residual pass) += psiC (of c = T(0))
+                VecItem it{};
+                it.op = VOP_ADD; it.n = (long)P->Nd * P->Nc; it.y = (double*)P->d_psi; it.x = (double*)P->d_psi;
+                it.x2 = s.psiC;
+                pre.push_back(it);
                 break;
             }
             s.beta = rn;

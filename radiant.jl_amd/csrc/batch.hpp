@@ -49,6 +49,7 @@ private:
         double H[33 * 32] = {}, cs[32] = {}, sn[32] = {}, gv[33] = {};
         std::vector<double*> V;
         double *w = nullptr, *c = nullptr, *z = nullptr, *r = nullptr, *t = nullptr;
+        double* psiC = nullptr;  // angular flux of the group's c = T(0) pass (FP64), kept for the reconstruction
         std::vector<rt_group_stats> stats;
     };
     std::vector<PS> ps;

@@ -394,6 +394,7 @@ int64_t Problem::device_bytes(const rt_problem_desc& d) {
     if (d.accel == RT_ACCEL_ANDERSON) nvec = 2 * d.depth + 8;
     if (d.precision != RT_PREC_FP64) nvec += 6;
     b += (int64_t)nvec * S * 8;
+    if (d.precision != RT_PREC_FP64) b += (int64_t)d.Nd * Nc * 8;  // angular flux of c = T(0) kept by batches
     b += (int64_t)std::min<int64_t>((int64_t)d.G * d.Np * Nc, std::max<int64_t>(FLUX_STAGE, (int64_t)d.G * d.Np)) * 8;
     return b;
 }

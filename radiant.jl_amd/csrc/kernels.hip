@@ -2035,6 +2035,7 @@ __global__ __launch_bounds__(256) void vecop_kernel(const VecItem* __restrict__ 
             case VOP_SCALE_COPY:
             case VOP_NORMALIZE: it.y[i] = it.x[i] * a; break;
             case VOP_SUB: it.y[i] = it.x[i] - it.x2[i]; break;
+            case VOP_ADD: it.y[i] = it.x[i] + it.x2[i]; break;
             case VOP_SUBMULTI: {
                 double v = it.y[i];
                 for (int t = 0; t < it.k; ++t) v -= it.coef[t] * it.V[t][i];

@@ -139,7 +139,7 @@ void k_scatter_state(double* dst, const double* src, int rows, const int* rowmap
 
 // Batched helpers (one launch serves every problem of a batch).
 enum VecOpCode { VOP_FILL = 0, VOP_COPY = 1, VOP_SCALE_COPY = 2, VOP_NORMALIZE = 3, VOP_SUB = 4, VOP_SUBMULTI = 5,
-                 VOP_ADDMULTI = 6 };
+                 VOP_ADDMULTI = 6, VOP_ADD = 7 };  // VOP_ADD: y = x + x2
 struct VecItem {
     int op, k;
     long n;
