@@ -97,6 +97,17 @@ typedef struct rt_problem_desc {
     int32_t csd;
     const double *Sb;
     const double *dE;
+    /* SN solver type (SN.jl get_solver_type; sn_flux.jl:54): 1 BTE, 2 BFP, 3 BCSD, 4 FP, 5 CSD,
+     * 6 BFP-EF; 0 = BTE, or BCSD when csd is set.  Types 2-6 need csd = 1.  The caller passes Σtot
+     * and Σs as sn_flux.jl:69-112 prepares them (absorption and zero Σs for FP/CSD, + T λ₀ for
+     * BFP/FP, minus the l = 0 self-scattering for BFP-EF); the solve leaves the in-group scattering
+     * out for types 4-6 and the out-of-group scattering for types 4-5 (sn_inner_pass.jl:22,
+     * sn_flux.jl:176).  Types 2 and 4 add the angular Fokker-Planck source T[g, mat] ℳ 𝚽l
+     * (fokker_planck_source.jl:26-36): T the momentum transfer [G, Nmat] column-major, Mfp = ℳ
+     * [Np, Np] column-major (fokker_planck_scattering_matrix.jl). */
+    int32_t solver_type;
+    const double *T;
+    const double *Mfp;
 } rt_problem_desc;
 
 /* Per-group result of the in-group solve (the values sn_one_speed.jl:189-193 prints). */

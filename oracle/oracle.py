@@ -31,7 +31,12 @@ class OracleProblem(C.Structure):
         ("nthreads", C.c_int32), ("verbose", C.c_int32),
         ("ssrc", C.POINTER(C.c_double) * 6), ("Np_source", C.c_int32),
         ("csd", C.c_int32), ("Sb", C.POINTER(C.c_double)), ("dE", C.POINTER(C.c_double)),
+        ("solver_type", C.c_int32), ("T", C.POINTER(C.c_double)), ("siga", C.POINTER(C.c_double)),
+        ("fokker_planck", C.c_char_p),
     ]
+
+# SN.jl get_solver_type
+SOLVER_TYPES = {"BTE": 1, "BFP": 2, "BCSD": 3, "FP": 4, "CSD": 5, "BFP-EF": 6}
 
 
 class OracleStats(C.Structure):
@@ -63,6 +68,7 @@ def lib():
                                            C.POINTER(C.c_int), C.POINTER(C.c_int64)]
         L.oracle_solve.argtypes = [C.POINTER(OracleProblem), dp, C.POINTER(OracleStats)]
         L.oracle_pass.argtypes = [C.POINTER(OracleProblem), C.c_int, C.c_int, dp, dp]
+        L.oracle_fokker_planck.argtypes = [C.POINTER(OracleProblem), dp, dp, C.c_int]
         _lib = L
     return _lib
 
@@ -167,8 +173,31 @@ def make_problem(prob, nthreads=1, verbose=0, **over):
         k.Sb = np.ascontiguousarray(np.asarray(prob.Sb, np.float64).reshape(-1, order="F"))
         k.dE = np.ascontiguousarray(prob.dE, np.float64)
         op.csd, op.Sb, op.dE = 1, _p(k.Sb), _p(k.dE)
+    st = getattr(prob, "solver_type", None)
+    if st:
+        op.solver_type = SOLVER_TYPES[st.upper()]
+    for nm in ("T", "siga"):
+        v = getattr(prob, nm, None)
+        if v is not None:
+            setattr(k, nm, np.ascontiguousarray(np.asarray(v, np.float64).reshape(-1, order="F")))
+            setattr(op, nm, _p(getattr(k, nm)))
+    fp = getattr(prob, "fokker_planck", None)
+    if fp:
+        k.fp = fp.encode()
+        op.fokker_planck = k.fp
     op._keep = k
     return op
+
+
+def fokker_planck(prob):
+    """(ℳ [Np, Np], λ₀) of a BFP / FP problem as the oracle builds it."""
+    op = make_problem(prob)
+    nd, np_, nps, S = sizes(prob)
+    M = np.zeros(np_ * np_)
+    lam = C.c_double()
+    r = lib().oracle_fokker_planck(C.byref(op), _p(M), C.byref(lam), np_)
+    _check(r)
+    return M.reshape((np_, np_), order="F"), lam.value
 
 
 def sizes(prob):

@@ -10,7 +10,8 @@
 // reference where they influence the result; the compiler is asked not to contract
 // multiply-adds (-ffp-contract=off) so that each product is rounded as in Julia.
 //
-// Scope: Cartesian 1D/2D/3D, BTE (no CSD), diamond-difference order 1 on every axis
+// Scope: Cartesian 1D/2D/3D, solver types BTE, BFP, BCSD, FP, CSD and BFP-EF (slowing down by
+// the DD scheme of order 1 in energy), diamond-difference order 1 on every axis
 // (`scheme_weights.jl:47-51` with order 1 gives w = [-1, 2]), "standard", "galerkin-d" and
 // "galerkin-m" angular operators, void / reflective / periodic boundaries, volume sources,
 // accelerations none / livolant / anderson / gmres / bicgstab.
@@ -810,6 +811,179 @@ SurfBasis surface_basis(const vector<double>& mu, const vector<double>& eta, con
 // Problem and solver state
 // ---------------------------------------------------------------------------------------
 
+// ---------------------------------------------------------------------------------------
+// Angular Fokker-Planck operator in moment space (fokker_planck_scattering_matrix.jl:29-40)
+// ---------------------------------------------------------------------------------------
+
+// C = A B for column-major A (r x k) and B (k x c).
+vector<double> matmul(const vector<double>& A, const vector<double>& B, int r, int k, int c) {
+    vector<double> C((size_t)r * c, 0.0);
+    for (int j = 0; j < c; ++j)
+        for (int q = 0; q < k; ++q) {
+            const double b = B[q + (size_t)k * j];
+            for (int i = 0; i < r; ++i) C[i + (size_t)r * j] += A[i + (size_t)r * q] * b;
+        }
+    return C;
+}
+
+// ℳ_p = Dn ℳ_n Mn for a direction-space operator ℳ_n (Nd x Nd).
+vector<double> to_moments(const Basis& B, const vector<double>& Mn_dir, int Nd) {
+    vector<double> t = matmul(B.D, Mn_dir, B.Np, Nd, Nd);
+    return matmul(t, B.M, B.Np, Nd, B.Np);
+}
+
+// particle_transport/fokker_planck_galerkin.jl:27-51 -- diag(-l(l+1)) mapped to directions,
+// shifted by λ₀ = max(0, max_n -ℳ_n[n,n]) on the diagonal, and mapped back.
+vector<double> fp_galerkin(const Basis& B, int Nd, double& lam0) {
+    const int P = B.Np;
+    vector<double> Mp0((size_t)P * P, 0.0);
+    for (int p = 0; p < P; ++p) Mp0[p + (size_t)P * p] = -(double)(B.pl[p] * (B.pl[p] + 1));
+    vector<double> Mn_dir = matmul(matmul(B.M, Mp0, Nd, P, P), B.D, Nd, P, Nd);
+    lam0 = 0.0;
+    for (int n = 0; n < Nd; ++n)
+        if (Mn_dir[n + (size_t)Nd * n] < -lam0) lam0 = -Mn_dir[n + (size_t)Nd * n];
+    for (int n = 0; n < Nd; ++n) Mn_dir[n + (size_t)Nd * n] += lam0;
+    return to_moments(B, Mn_dir, Nd);
+}
+
+// particle_transport/fokker_planck_finite_difference.jl, Qdims = 1 branch: Morel's weighted
+// three-point scheme on the directions sorted by mu (β_{1/2} = 0, β_{n+1/2} = β_{n-1/2} - 2 w_n mu_n),
+// with λ₀ = max_n C_n added on the diagonal.
+vector<double> fp_fd_1d(const vector<double>& mu, const vector<double>& w, double& lam0) {
+    const int N = (int)mu.size();
+    vector<int> idx(N);
+    for (int i = 0; i < N; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return mu[a] < mu[b]; });
+    vector<double> m(N), ww(N);
+    for (int i = 0; i < N; ++i) { m[i] = mu[idx[i]]; ww[i] = w[idx[i]]; }
+    vector<double> beta(N + 1, 0.0), Cm(N, 0.0), Cp(N, 0.0), C(N);
+    for (int n = 1; n < N; ++n) beta[n] = beta[n - 1] - 2.0 * ww[n - 1] * m[n - 1];
+    for (int n = 1; n < N; ++n) Cm[n] = beta[n] / (ww[n] * (m[n] - m[n - 1]));
+    for (int n = 0; n < N - 1; ++n) Cp[n] = beta[n + 1] / (ww[n] * (m[n + 1] - m[n]));
+    lam0 = -std::numeric_limits<double>::infinity();
+    for (int n = 0; n < N; ++n) { C[n] = Cm[n] + Cp[n]; lam0 = std::max(lam0, C[n]); }
+    vector<double> Mn_dir((size_t)N * N, 0.0);
+    for (int n = 0; n < N; ++n) {
+        Mn_dir[idx[n] + (size_t)N * idx[n]] = lam0 - C[n];
+        if (n > 0) Mn_dir[idx[n] + (size_t)N * idx[n - 1]] = Cm[n];
+        if (n < N - 1) Mn_dir[idx[n] + (size_t)N * idx[n + 1]] = Cp[n];
+    }
+    return Mn_dir;
+}
+
+// particle_transport/fokker_planck_finite_difference.jl, Qdims = 2 and 3 branches for the
+// gauss-legendre-chebychev product set: Morel's polar operator on the mu levels and a periodic
+// (Qdims = 3) or one-sided (Qdims = 2, upper half of the azimuth) three-point azimuthal operator
+// scaled by γ_n; directions are ordered azimuth fastest.  λ₀ = max |diag| is added on the diagonal.
+vector<double> fp_fd_glc(int N, int Qdims, double& lam0) {
+    vector<double> mun, wn;
+    gauss_legendre(N, mun, wn);
+    vector<int> idx(N);
+    for (int i = 0; i < N; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return mun[a] < mun[b]; });
+    vector<double> mu(N), wi(N);
+    for (int i = 0; i < N; ++i) { mu[i] = mun[idx[i]]; wi[i] = wn[idx[i]]; }
+    const int Na = Qdims == 2 ? N : 2 * N;  // azimuthal points per mu level
+    const double wj = PI / N;
+    vector<double> om(Na);
+    for (int i = 0; i < Na; ++i) om[i] = PI * ((double)(i + 1) - 0.5) / N;
+    vector<double> beta(N + 1, 0.0), d12(N + 1, 0.0), gam(N, 0.0);
+    for (int n = 1; n < N; ++n) {
+        beta[n] = beta[n - 1] - 2.0 * wi[n - 1] * mu[n - 1];
+        d12[n] = (std::sqrt(1.0 - mu[n] * mu[n]) - std::sqrt(1.0 - mu[n - 1] * mu[n - 1])) / (mu[n] - mu[n - 1]);
+    }
+    for (int n = 0; n < N; ++n) {
+        const double cn = (beta[n + 1] * d12[n + 1] - beta[n] * d12[n]) / wi[n];
+        const double Kn = 2.0 * (1.0 - mu[n] * mu[n]) + cn * std::sqrt(1.0 - mu[n] * mu[n]);
+        gam[n] = (PI * PI * Kn) / (2.0 * N * N * (1.0 - std::cos(PI / N)));
+    }
+    const int Nd = N * Na;
+    vector<double> Mn_dir((size_t)Nd * Nd, 0.0);
+    auto at = [&](int n, int i, int m, int j) -> double& {
+        return Mn_dir[(i + (size_t)Na * n) + (size_t)Nd * (j + (size_t)Na * m)];
+    };
+    for (int n = 0; n < N; ++n) {
+        const double a = 1.0 / (1.0 - mu[n] * mu[n]) * gam[n] / wj;  // azimuthal factor of level n
+        const double pm = n > 0 ? 1.0 / wi[n] * beta[n] / (mu[n] - mu[n - 1]) : 0.0;
+        const double pp = n < N - 1 ? 1.0 / wi[n] * beta[n + 1] / (mu[n + 1] - mu[n]) : 0.0;
+        double polar;
+        if (n == 0) polar = 1.0 / wi[n] * beta[n + 1] / (mu[n + 1] - mu[n]);
+        else if (n == N - 1) polar = 1.0 / wi[n] * beta[n] / (mu[n] - mu[n - 1]);
+        else polar = 1.0 / wi[n] * (beta[n + 1] / (mu[n + 1] - mu[n]) + beta[n] / (mu[n] - mu[n - 1]));
+        for (int i = 0; i < Na; ++i) {
+            double& dg = at(n, i, n, i);
+            dg -= polar;
+            if (Qdims == 2) {
+                if (i == 0) dg -= a / (om[i + 1] - om[i]);
+                else if (i == Na - 1) dg -= a / (om[i] - om[i - 1]);
+                else dg -= a / (om[i] - om[i - 1]) + a / (om[i + 1] - om[i]);
+                if (i > 0) at(n, i, n, i - 1) += a / (om[i] - om[i - 1]);
+                if (i < Na - 1) at(n, i, n, i + 1) += a / (om[i + 1] - om[i]);
+            } else {
+                const double lo = i == 0 ? om[i] + 2.0 * PI - om[Na - 1] : om[i] - om[i - 1];
+                const double hi = i == Na - 1 ? om[0] + 2.0 * PI - om[i] : om[i + 1] - om[i];
+                dg -= a / lo + a / hi;
+                at(n, i, n, i == 0 ? Na - 1 : i - 1) += a / lo;
+                at(n, i, n, i == Na - 1 ? 0 : i + 1) += a / hi;
+            }
+            if (n > 0) at(n, i, n - 1, i) += pm;
+            if (n < N - 1) at(n, i, n + 1, i) += pp;
+        }
+    }
+    lam0 = 0.0;
+    for (int d = 0; d < Nd; ++d) lam0 = std::max(lam0, std::fabs(Mn_dir[d + (size_t)Nd * d]));
+    for (int d = 0; d < Nd; ++d) Mn_dir[d + (size_t)Nd * d] += lam0;
+    return Mn_dir;
+}
+
+// particle_transport/fokker_planck_differential_quadrature.jl (Qdims = 1): (1 - mu²) W² - 2 mu W
+// with W the Lagrange differentiation matrix on the nodes (diagonal = minus the row sum), shifted
+// by λ₀ = max(0, max_n -ℳ[n,n]).
+vector<double> fp_dq_1d(const vector<double>& mu, double& lam0) {
+    const int N = (int)mu.size();
+    vector<double> Pi(N, 1.0), W((size_t)N * N, 0.0);
+    for (int n = 0; n < N; ++n)
+        for (int m = 0; m < N; ++m)
+            if (n != m) Pi[n] *= (mu[n] - mu[m]);
+    for (int n = 0; n < N; ++n)
+        for (int m = 0; m < N; ++m)
+            if (n != m) W[n + (size_t)N * m] = Pi[n] / ((mu[n] - mu[m]) * Pi[m]);
+    for (int n = 0; n < N; ++n)
+        for (int m = 0; m < N; ++m)
+            if (n != m) W[n + (size_t)N * n] -= W[n + (size_t)N * m];
+    vector<double> W2 = matmul(W, W, N, N, N);
+    vector<double> Mn_dir((size_t)N * N);
+    for (int n = 0; n < N; ++n)
+        for (int m = 0; m < N; ++m)
+            Mn_dir[n + (size_t)N * m] = (1.0 - mu[n] * mu[n]) * W2[n + (size_t)N * m] - 2.0 * mu[n] * W[n + (size_t)N * m];
+    lam0 = 0.0;
+    for (int n = 0; n < N; ++n)
+        if (Mn_dir[n + (size_t)N * n] < -lam0) lam0 = -Mn_dir[n + (size_t)N * n];
+    for (int n = 0; n < N; ++n) Mn_dir[n + (size_t)N * n] += lam0;
+    return Mn_dir;
+}
+
+// fokker_planck_scattering_matrix.jl:29-40 -- ℳ (Np x Np column-major) and λ₀ for one method.
+vector<double> fokker_planck_matrix(const string& method, int N, const string& qt, int Qdims,
+                                    const vector<double>& mu, const vector<double>& w, const Basis& B,
+                                    double& lam0) {
+    const int Nd = (int)w.size();
+    if (method == "galerkin") return fp_galerkin(B, Nd, lam0);
+    vector<double> Mn_dir;
+    if (method == "finite-difference") {
+        if (Qdims == 1) Mn_dir = fp_fd_1d(mu, w, lam0);
+        else if (qt == "gauss-legendre-chebychev") Mn_dir = fp_fd_glc(N, Qdims, lam0);
+        else fail("finite-difference Fokker-Planck on the " + qt + " quadrature (Voronoi weights) is not supported");
+        if ((int)(Mn_dir.size()) != Nd * Nd) fail("Fokker-Planck operator does not match the quadrature");
+    } else if (method == "differential-quadrature") {
+        if (Qdims != 1) fail("Differential quadrature unavailable in multidimensional geometry.");
+        Mn_dir = fp_dq_1d(mu, lam0);
+    } else {
+        fail("Unknown method to treat the Fokker-Planck term.");
+    }
+    return to_moments(B, Mn_dir, Nd);
+}
+
 struct Angular {
     int Nd = 0, Np = 0, Nps = 0;
     vector<double> mu, eta, xi, w;
@@ -847,6 +1021,15 @@ struct oracle_problem {
     int32_t csd;
     const double* Sb;
     const double* dE;
+    // SN solver type (SN.jl get_solver_type: 1 BTE, 2 BFP, 3 BCSD, 4 FP, 5 CSD, 6 BFP-EF; 0 = BTE,
+    // or BCSD when csd is set); types 2-6 carry the slowing-down term (csd = 1).  T: momentum
+    // transfer [G, Nmat] (types 2, 4), siga: absorption [G, Nmat] (types 4, 5), fokker_planck: the
+    // angular Fokker-Planck discretisation ("finite-difference" when NULL, "galerkin",
+    // "differential-quadrature").
+    int32_t solver_type;
+    const double* T;
+    const double* siga;
+    const char* fokker_planck;
 };
 
 struct oracle_stats {
@@ -868,6 +1051,10 @@ struct Solver {
     int Ls;                     // max(pl): Σs sized to Ls+1 (sn_flux.jl:79-83)
     vector<double> sigs;        // [Nmat][G][G][Ls+1] padded copy: sigs[m + Nmat*(gi + G*(gf + G*l))]
     vector<int> matc;           // 0-based material per cell
+    int solver = 1;             // SN solver type (1 BTE .. 6 BFP-EF)
+    vector<double> sigt_eff;    // Σtot as sn_flux.jl:69-112 prepares it [g + G*m]
+    vector<double> Mfp;         // Fokker-Planck ℳ [Np x Np] column-major (types 2, 4)
+    double lam0 = 0.0;          // its λ₀
     const double* cur_src[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // this pass's surface sources
     // continuous slowing down: incoming / outgoing energy-face angular flux [n*Nc + c] (𝚽E12 of
     // sn_flux.jl:166), the current group's S⁻ and S⁺ per material (stopping powers / ΔE)
@@ -934,6 +1121,27 @@ struct Solver {
             if (p.ssrc[f] && (p.Np_source < 0 || p.Np_source > A.Nps))
                 fail("surface-source moments exceed the half-range basis of the face");
         }
+        // sn_flux.jl:69-112 -- solver type, removal cross section, Fokker-Planck operator
+        solver = p.solver_type ? p.solver_type : (p.csd ? 3 : 1);
+        if (solver < 1 || solver > 6) fail("Unknown type of solver.");
+        if ((solver >= 2) != (p.csd != 0))
+            fail("solver types BFP, BCSD, FP, CSD and BFP-EF carry the slowing-down term (csd), BTE does not");
+        const double* base = p.sigt;
+        if (solver == 4 || solver == 5) {  // absorption removal, no Boltzmann scattering
+            if (!p.siga) fail("solver types FP and CSD need the absorption cross sections");
+            base = p.siga;
+            std::fill(sigs.begin(), sigs.end(), 0.0);
+        }
+        sigt_eff.assign(base, base + (size_t)G * Nmat);
+        if (solver == 2 || solver == 4) {
+            if (!p.T) fail("solver types BFP and FP need the momentum transfer");
+            const string meth = p.fokker_planck ? p.fokker_planck : "finite-difference";
+            Mfp = fokker_planck_matrix(meth, p.quad_order, qt, Qdims, A.mu, A.w, A.B, lam0);
+            for (size_t i = 0; i < sigt_eff.size(); ++i) sigt_eff[i] += p.T[i] * lam0;
+        }
+        if (solver == 6)  // elastic-free: the self-scattering moment l = 0 leaves the removal
+            for (int m = 0; m < Nmat; ++m)
+                for (int g = 0; g < G; ++g) sigt_eff[g + (size_t)G * m] -= sig_s(m, g, g, 0);
         nphi = (size_t)A.Np * Nc;
         nbx = (size_t)A.Nps * 2 * Ny * Nz;
         nby = ndims >= 2 ? (size_t)A.Nps * 2 * Nx * Nz : 0;
@@ -944,7 +1152,7 @@ struct Solver {
     double sig_s(int m, int gi, int gf, int l) const {
         return sigs[m + (size_t)Nmat * (gi + (size_t)G * (gf + (size_t)G * l))];
     }
-    double sig_t(int g, int m) const { return P.sigt[g + (size_t)G * m]; }
+    double sig_t(int g, int m) const { return sigt_eff[g + (size_t)G * m]; }
 
     // ----- sweeps --------------------------------------------------------------------
     // DD order-1 cell solve, written once for 1/2/3 axes (sn_1D_bte.jl:150-189,
@@ -1132,13 +1340,24 @@ struct Solver {
             psiE_out.assign((size_t)A.Nd * Nc, 0.0);
             curE = homog ? nullptr : psiE_in.data();
         }
-        // Ql = (homog ? 0 : Qlout) + Σs_l Φl   (scattering_source.jl:25-30)
+        // Ql = (homog ? 0 : Qlout) + Σs_l Φl   (scattering_source.jl:25-30; sn_inner_pass.jl:22
+        // leaves the in-group scattering out for solver types 4, 5 and 6)
+        const bool within = solver != 4 && solver != 5 && solver != 6;
         for (int c = 0; c < Nc; ++c) {
             int m = matc[c];
             for (int p = 0; p < Np; ++p) {
                 double q = homog ? 0.0 : Qlout[p + (size_t)Np * c];
-                q += sig_s(m, g, g, (int)A.B.pl[p]) * phi[p + (size_t)Np * c];
+                if (within) q += sig_s(m, g, g, (int)A.B.pl[p]) * phi[p + (size_t)Np * c];
                 Ql[p + (size_t)Np * c] = q;
+            }
+        }
+        // Ql += T ℳ Φl   (fokker_planck_source.jl:26-36, solver types 2 and 4)
+        if (solver == 2 || solver == 4) {
+            for (int c = 0; c < Nc; ++c) {
+                const double Tm = P.T[g + (size_t)G * matc[c]];
+                for (int n = 0; n < Np; ++n)
+                    for (int k = 0; k < Np; ++k)
+                        Ql[n + (size_t)Np * c] += Tm * Mfp[n + (size_t)Np * k] * phi[k + (size_t)Np * c];
             }
         }
         vector<double> bin(bx, bx + nbx + nby + nbz);
@@ -1491,9 +1710,10 @@ struct Solver {
         vector<double> phi_all((size_t)G * nphi, 0.0);  // [g][p + Np*c]
         for (int g = 0; g < G; ++g) {
             vector<double> Qlout(nphi, 0.0);
-            // scattering_source.jl:59-68 (all gi != gf, in order)
+            // scattering_source.jl:59-68 (all gi != gf, in order; none for solver types 4 and 5,
+            // sn_flux.jl:176)
             for (int gi = 0; gi < G; ++gi) {
-                if (gi == g) continue;
+                if (gi == g || solver == 4 || solver == 5) continue;
                 const double* pg = &phi_all[(size_t)gi * nphi];
                 for (int c = 0; c < Nc; ++c) {
                     int m = matc[c];
@@ -1588,6 +1808,19 @@ int oracle_legendre(int L, double x, int half_range, double* P) {
         vector<double> v = half_range ? half_range_legendre_up_to_L(L, x) : legendre_up_to_L(L, x);
         std::copy(v.begin(), v.end(), P);
         return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// Fokker-Planck operator of a problem (types 2 and 4): ℳ [Np x Np] column-major and λ₀;
+// returns Np.
+int oracle_fokker_planck(const oracle_problem* p, double* Mfp, double* lam0, int capNp) {
+    try {
+        Solver s(*p);
+        if (s.Mfp.empty()) fail("the problem's solver type has no Fokker-Planck term");
+        if (s.A.Np > capNp) fail("capacity too small");
+        std::copy(s.Mfp.begin(), s.Mfp.end(), Mfp);
+        *lam0 = s.lam0;
+        return s.A.Np;
     } catch (std::exception& e) { g_err = e.what(); return -1; }
 }
 

@@ -37,17 +37,33 @@ static void upload(T* d, const std::vector<T>& h, hipStream_t st) {
     if (!h.empty()) RT_CHECK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
 }
 
-// Legendre orders that carry a scattering source: Σs_l != 0 for some material/group pair.
+// SN solver type (SN.jl get_solver_type): 0 in the description means BTE, or BCSD with csd.
+static int solver_type_of(const rt_problem_desc& d) { return d.solver_type ? d.solver_type : (d.csd ? 3 : 1); }
+static bool has_fp(const rt_problem_desc& d) {
+    const int t = solver_type_of(d);
+    return (t == 2 || t == 4) && d.T && d.Mfp;
+}
+
+// Moments that carry a source: the Legendre orders with Σs_l != 0 for some material/group pair,
+// and with the Fokker-Planck term every moment that ℳ reads or writes.
 static std::vector<int> active_moments(const rt_problem_desc& d) {
     std::vector<bool> lnz(d.Ls + 1, false);
     const size_t per_l = (size_t)d.Nmat * d.G * d.G;
     for (int l = 0; l <= d.Ls; ++l)
         for (size_t i = 0; i < per_l; ++i)
             if (d.sigs[i + per_l * l] != 0.0) { lnz[l] = true; break; }
+    std::vector<bool> fp(d.Np, false);
+    if (has_fp(d)) {
+        bool tnz = false;
+        for (size_t i = 0; i < (size_t)d.G * d.Nmat; ++i) tnz = tnz || d.T[i] != 0.0;
+        for (int p = 0; p < d.Np && tnz; ++p)
+            for (int q = 0; q < d.Np; ++q)
+                if (d.Mfp[p + (size_t)d.Np * q] != 0.0) fp[p] = fp[q] = true;
+    }
     std::vector<int> act;
     for (int p = 0; p < d.Np; ++p) {
         const int64_t l = d.pl[p];
-        if (l == 0 || (l <= d.Ls && lnz[l])) act.push_back(p);
+        if (l == 0 || (l <= d.Ls && lnz[l]) || fp[p]) act.push_back(p);
     }
     return act;
 }
@@ -89,6 +105,12 @@ static void validate(const rt_problem_desc& d) {
     }
     if (surf && (d.Np_source < 1 || d.Np_source > d.Nps))
         bad("Np_source must lie in 1..Nps when a surface source is given");
+    if (d.solver_type < 0 || d.solver_type > 6) bad("Unknown type of solver.");
+    const int stype = solver_type_of(d);
+    if ((stype >= 2) != (d.csd != 0))
+        bad("solver types BFP, BCSD, FP, CSD and BFP-EF carry the slowing-down term (csd), BTE does not");
+    if ((stype == 2 || stype == 4) && (!d.T || !d.Mfp))
+        bad("solver types BFP and FP need the momentum transfer T and the Fokker-Planck matrix Mfp");
     if (d.csd) {
         if (!d.Sb || !d.dE) bad("continuous slowing down needs the boundary stopping powers and group widths");
         for (int g = 0; g < d.G; ++g)
@@ -169,14 +191,31 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
             if (l > d.Ls) return 0.0;
             return d.sigs[m + (size_t)Nmat * (gi + (size_t)G * (gf + (size_t)G * l))];
         };
+        // solver types 4-6 have no in-group scattering source, 4-5 none from other groups
+        // (sn_inner_pass.jl:22, sn_flux.jl:176)
+        const int stype = solver_type_of(d);
+        const bool within = stype < 4, across = stype != 4 && stype != 5;
         std::vector<double> sigw((size_t)G * Nmat * Nact), sigo((size_t)G * G * Nmat * Nact);
         for (int g = 0; g < G; ++g)
             for (int m = 0; m < Nmat; ++m)
                 for (int a = 0; a < Nact; ++a) {
-                    sigw[((size_t)g * Nmat + m) * Nact + a] = S(m, g, g, d.pl[act[a]]);
+                    sigw[((size_t)g * Nmat + m) * Nact + a] = within ? S(m, g, g, d.pl[act[a]]) : 0.0;
                     for (int gi = 0; gi < G; ++gi)
-                        sigo[(((size_t)g * G + gi) * Nmat + m) * Nact + a] = S(m, gi, g, d.pl[act[a]]);
+                        sigo[(((size_t)g * G + gi) * Nmat + m) * Nact + a] = across ? S(m, gi, g, d.pl[act[a]]) : 0.0;
                 }
+        if (has_fp(d)) {
+            // angular Fokker-Planck source T[g, mat] ℳ Φ (fokker_planck_source.jl:26-36) on the
+            // active moments: ℳ row-major [Nact][Nact], T [G][Nmat]
+            fp = true;
+            std::vector<double> Ma((size_t)Nact * Nact), Tg((size_t)G * Nmat);
+            for (int k = 0; k < Nact; ++k)
+                for (int j = 0; j < Nact; ++j) Ma[(size_t)k * Nact + j] = d.Mfp[act[k] + (size_t)Np * act[j]];
+            for (int g = 0; g < G; ++g)
+                for (int m = 0; m < Nmat; ++m) Tg[(size_t)g * Nmat + m] = d.T[g + (size_t)G * m];
+            d_fpM = dalloc<double>(Ma.size()); upload(d_fpM, Ma, st);
+            d_fpT = dalloc<double>(Tg.size()); upload(d_fpT, Tg, st);
+            d_fpq = dalloc<double>((size_t)Nact * Nc);
+        }
         d_sigw = dalloc<double>(sigw.size()); upload(d_sigw, sigw, st);
         d_sigo = dalloc<double>(sigo.size()); upload(d_sigo, sigo, st);
         std::vector<double> qv((size_t)G * Nc);
@@ -395,6 +434,8 @@ int64_t Problem::device_bytes(const rt_problem_desc& d) {
     if (d.precision != RT_PREC_FP64) nvec += 6;
     b += (int64_t)nvec * S * 8;
     if (d.precision != RT_PREC_FP64) b += (int64_t)d.Nd * Nc * 8;  // angular flux of c = T(0) kept by batches
+    if (d.csd) b += 2LL * d.Nd * Nc * 8;                              // energy-face angular fluxes
+    if (has_fp(d)) b += (int64_t)Nact_est * (Nc + Nact_est) * 8;     // Fokker-Planck source, ℳ
     b += (int64_t)std::min<int64_t>((int64_t)d.G * d.Np * Nc, std::max<int64_t>(FLUX_STAGE, (int64_t)d.G * d.Np)) * 8;
     return b;
 }
@@ -433,6 +474,7 @@ void Problem::fill_geometry(PassDesc& pd, int g) {
     pd.psi = d_psi;
     pd.bx = d_bex[0]; pd.by = d_bex[1]; pd.bz = d_bex[2];
     pd.Nps = Nps;
+    if (fp) { pd.fpT = d_fpT + (size_t)g * Nmat; pd.fpM = d_fpM; }
 }
 
 hipEvent_t Problem::next_event() {
@@ -460,8 +502,12 @@ void Problem::collect_kernel_times() {
 }
 
 void Problem::launch_pass(PassDesc& pd, int prec, bool full_rows) {
+    // Fokker-Planck source of a non-zero state: the contraction reads qlout + T ℳ Φ
+    const bool fpsrc = pd.fpT && pd.phi_in;
+    if (fpsrc) { pd.fp_qin = pd.qlout; pd.fpq = d_fpq; pd.qlout = d_fpq; }
     k_set_pd(pd, d_pd, st);
     if (pd.bin_x) launch_bndin(d_pd, (int)bin_total, st);  // incoming fluxes of non-void faces
+    if (fpsrc) launch_fp_source(d_pd, (long)Nact * Nc, st);
     auto timed = [&](int kernel, auto&& fn) {
         if (!profiling) {
             fn();

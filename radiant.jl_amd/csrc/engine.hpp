@@ -116,6 +116,10 @@ private:
     double *d_psiE_in = nullptr, *d_psiE_out = nullptr;
     std::vector<double> dE;
     void set_csd(PassDesc& pd, int g);
+    // angular Fokker-Planck term (solver types BFP, FP): ℳ on the active moments, T per group and
+    // material, and the source qlout + T ℳ Φ the contraction reads
+    bool fp = false;
+    double *d_fpM = nullptr, *d_fpT = nullptr, *d_fpq = nullptr;
     double* d_Mentry[3] = {nullptr, nullptr, nullptr};
     int* d_insrc[3] = {nullptr, nullptr, nullptr};
     double* d_bin[3] = {nullptr, nullptr, nullptr};

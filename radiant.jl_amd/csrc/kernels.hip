@@ -1298,6 +1298,21 @@ __global__ __launch_bounds__(256) void csd_out_kernel(const PassDesc* __restrict
         d.psiE_out[i] = 2.0 * psi[i] - d.psiE_in[i];
 }
 
+// Angular Fokker-Planck source (fokker_planck_source.jl:26-36) added to the out-of-group source
+// before the moment -> direction contraction: one thread per (moment k, cell c), cells fastest.
+__global__ __launch_bounds__(256) void fp_source_kernel(const PassDesc* __restrict__ pds, long n) {
+    const PassDesc& d = pds[0];
+    const int K = d.Nact, Nc = d.Nc;
+    const int* __restrict__ rows = d.phi_rows;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(i / Nc), c = (int)(i - (long)k * Nc);
+        const double* __restrict__ Mk = d.fpM + (size_t)k * K;
+        double s = 0.0;
+        for (int j = 0; j < K; ++j) s += Mk[j] * d.phi_in[(size_t)(rows ? rows[j] : j) * Nc + c];
+        d.fpq[i] = (d.fp_qin ? d.fp_qin[i] : 0.0) + d.fpT[d.mat[c]] * s;
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // Out-of-group + volume source (scattering_source.jl:59-68, sn_flux.jl:179).
 // ------------------------------------------------------------------------------------------
@@ -1750,6 +1765,10 @@ void launch_csd_source(const PassDesc* d_pd, long n, hipStream_t st) {
 }
 void launch_csd_out(const PassDesc* d_pd, long n, hipStream_t st) {
     hipLaunchKernelGGL(csd_out_kernel, dim3(grid_for((size_t)n)), dim3(256), 0, st, d_pd, n);
+    RT_HIP(hipGetLastError());
+}
+void launch_fp_source(const PassDesc* d_pd, long n, hipStream_t st) {
+    hipLaunchKernelGGL(fp_source_kernel, dim3(grid_for((size_t)n)), dim3(256), 0, st, d_pd, n);
     RT_HIP(hipGetLastError());
 }
 

@@ -56,6 +56,13 @@ struct PassDesc {
     const double* psiE_in;
     double* psiE_out;
     const double* csdK;
+    // angular Fokker-Planck term (null = none): T of the group per material, ℳ on the active
+    // moments [Nact][Nact], and for a pass with a non-zero state the source it builds,
+    // fpq = fp_qin + T[mat] ℳ Φ [Nact][Nc] (fp_qin: the out-of-group source, null = zero)
+    const double* fpT;
+    const double* fpM;
+    const double* fp_qin;
+    double* fpq;
 };
 
 enum Prec { P64 = 0, P32 = 1, PBF16 = 2,
@@ -99,6 +106,9 @@ void launch_bndin(const PassDesc* d_pd, int total, hipStream_t st);
 // psiE_in after it (FP64 angular fluxes [Nd][Nc])
 void launch_csd_source(const PassDesc* d_pd, long n, hipStream_t st);
 void launch_csd_out(const PassDesc* d_pd, long n, hipStream_t st);
+// angular Fokker-Planck source, one problem: fpq[k][c] = fp_qin[k][c] + T[mat c] sum_j ℳ[k][j]
+// phi_in[row j][c], n = Nact * Nc
+void launch_fp_source(const PassDesc* d_pd, long n, hipStream_t st);
 void launch_dgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st);
 void launch_bndmom(const PassDesc* d_pd, int nprob, int max_items, hipStream_t st);
 

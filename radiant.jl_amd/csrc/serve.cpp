@@ -31,7 +31,7 @@
 // Deep copy of a problem description (the caller's buffers may go away after submit).
 struct DescCopy {
     rt_problem_desc d{};
-    std::vector<double> dx, dy, dz, sigt, sigs, qvol, mu, eta, xi, w, M, D, Msurf[6], Dsurf[6], ssrc[6], Sb, dE;
+    std::vector<double> dx, dy, dz, sigt, sigs, qvol, mu, eta, xi, w, M, D, Msurf[6], Dsurf[6], ssrc[6], Sb, dE, T, Mfp;
     std::vector<int64_t> mat, pl, n2p[6];
     std::vector<int32_t> bc;
 
@@ -61,6 +61,10 @@ struct DescCopy {
             d.Sb = s.Sb ? Sb.data() : nullptr;
             d.dE = s.dE ? dE.data() : nullptr;
         }
+        cp(T, s.T, (size_t)s.G * s.Nmat);
+        cp(Mfp, s.Mfp, (size_t)s.Np * s.Np);
+        d.T = s.T ? T.data() : nullptr;
+        d.Mfp = s.Mfp ? Mfp.data() : nullptr;
         for (int f = 0; f < 6; ++f) {
             d.Msurf[f] = nullptr; d.Dsurf[f] = nullptr; d.n_to_np[f] = nullptr;
             if (f < 2 * s.ndims) {

@@ -38,6 +38,7 @@ class ProblemDesc(C.Structure):
         ("precision", C.c_int32), ("state_mode", C.c_int32), ("verbose", C.c_int32),
         ("ssrc", dp * 6), ("Np_source", C.c_int32),
         ("csd", C.c_int32), ("Sb", dp), ("dE", dp),
+        ("solver_type", C.c_int32), ("T", dp), ("Mfp", dp),
     ]
 
 

@@ -91,6 +91,14 @@ class Problem:
     csd: bool = False
     Sb: np.ndarray = None
     dE: np.ndarray = None
+    # SN solver type (SN.set_solver_type): None = "BTE", or "BCSD" when csd is set; "BFP", "FP",
+    # "CSD" and "BFP-EF" carry the slowing-down term too.  T: momentum transfer [G, Nmat] (BFP,
+    # FP), siga: absorption [G, Nmat] (the removal of FP and CSD), fokker_planck: the angular
+    # Fokker-Planck discretisation (SN.set_angular_fokker_planck)
+    solver_type: str = None
+    T: np.ndarray = None
+    siga: np.ndarray = None
+    fokker_planck: str = "finite-difference"
 
     @property
     def Nc(self):

@@ -11,7 +11,7 @@ import time
 
 import numpy as np
 
-from . import _lib
+from . import _lib, fokker_planck
 from .angular import angular_set, quadrature_dimension
 from ._lib import ACCEL, PREC, STATE, check, lib, ptr
 
@@ -32,8 +32,10 @@ class Desc:
         k["dz"] = f64(prob.dz if prob.ndims >= 3 else [1.0])
         k["mat"] = i64(prob.mat)
         k["bc"] = np.ascontiguousarray(prob.bc, dtype=np.int32)
-        k["sigt"] = f64(prob.sigt)
-        k["sigs"] = f64(prob.sigs)
+        # removal, scattering and Fokker-Planck operator of the solver type (sn_flux.jl:69-112)
+        stype, sigt, sigs, T, Mfp = fokker_planck.prepare(prob, ang, qd)
+        k["sigt"] = f64(sigt)
+        k["sigs"] = f64(sigs)
         k["qvol"] = f64(prob.qvol)
         for nm in ("mu", "eta", "xi", "w"):
             k[nm] = f64(getattr(ang, nm))
@@ -68,6 +70,11 @@ class Desc:
             k["Sb"] = f64(prob.Sb)
             k["dE"] = f64(prob.dE)
             d.csd, d.Sb, d.dE = 1, ptr(k["Sb"]), ptr(k["dE"])
+        d.solver_type = stype
+        if Mfp is not None:  # T [G, Nmat], ℳ [Np, Np], both column-major
+            k["T"] = f64(T)
+            k["Mfp"] = f64(Mfp)
+            d.T, d.Mfp = ptr(k["T"]), ptr(k["Mfp"])
         d.accel = ACCEL[accel or prob.accel]
         d.restart = restart if restart is not None else prob.restart
         d.depth = depth if depth is not None else prob.depth
