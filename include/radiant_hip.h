@@ -108,6 +108,15 @@ typedef struct rt_problem_desc {
     int32_t solver_type;
     const double *T;
     const double *Mfp;
+    /* Spatial closure along x (SN.set_scheme("x", type, order); scheme_weights.jl:25-90,
+     * sn_1D_bte.jl:27-64): order_x = 𝒪x spatial Legendre moments per cell (0 or 1: first order, at
+     * most 4) and omega_x[𝒪x + 1] the closure weights ω of the x axis (NULL: diamond difference of
+     * order 1).  Any order or closure other than diamond difference of order 1 is for the 1D BTE
+     * (csd = 0), single problems, every precision and state.  The flux moments then carry Nm5 = 𝒪x
+     * spatial moments per cell: phi_out is 𝚽l[G, Np, Nm5, Nx] and the qlout of rt_problem_pass is
+     * [Np, Nm5 * Nx]; qvol keeps one value per cell (spatial moment 1, Source.jl:135). */
+    int32_t order_x;
+    const double *omega_x;
 } rt_problem_desc;
 
 /* Per-group result of the in-group solve (the values sn_one_speed.jl:189-193 prints). */

@@ -32,7 +32,7 @@ class OracleProblem(C.Structure):
         ("ssrc", C.POINTER(C.c_double) * 6), ("Np_source", C.c_int32),
         ("csd", C.c_int32), ("Sb", C.POINTER(C.c_double)), ("dE", C.POINTER(C.c_double)),
         ("solver_type", C.c_int32), ("T", C.POINTER(C.c_double)), ("siga", C.POINTER(C.c_double)),
-        ("fokker_planck", C.c_char_p),
+        ("fokker_planck", C.c_char_p), ("scheme", C.c_char_p), ("scheme_order", C.c_int32),
     ]
 
 # SN.jl get_solver_type
@@ -69,6 +69,7 @@ def lib():
         L.oracle_solve.argtypes = [C.POINTER(OracleProblem), dp, C.POINTER(OracleStats)]
         L.oracle_pass.argtypes = [C.POINTER(OracleProblem), C.c_int, C.c_int, dp, dp]
         L.oracle_fokker_planck.argtypes = [C.POINTER(OracleProblem), dp, dp, C.c_int]
+        L.oracle_cell_1d.argtypes = [C.c_char_p, C.c_int, C.c_double, C.c_double, C.c_double, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -185,6 +186,9 @@ def make_problem(prob, nthreads=1, verbose=0, **over):
     if fp:
         k.fp = fp.encode()
         op.fokker_planck = k.fp
+    k.scheme = (getattr(prob, "scheme", None) or "DD").encode()
+    op.scheme = k.scheme
+    op.scheme_order = int(getattr(prob, "scheme_order", 1) or 1)
     op._keep = k
     return op
 
@@ -200,6 +204,20 @@ def fokker_planck(prob):
     return M.reshape((np_, np_), order="F"), lam.value
 
 
+def spatial_moments(prob):
+    """Spatial Legendre moments per cell (the 1D scheme order; 1 otherwise)."""
+    return max(1, int(getattr(prob, "scheme_order", 1) or 1)) if prob.ndims == 1 else 1
+
+
+def cell_1d(scheme, K, mu, St, dx, Qn, fx_in):
+    """One 1D cell of the given closure and order: (spatial moments [K], outgoing face flux)."""
+    q = np.ascontiguousarray(Qn, np.float64)
+    psi = np.zeros(K)
+    fx = C.c_double(fx_in)
+    _check(lib().oracle_cell_1d(scheme.encode(), K, mu, St, dx, _p(q), C.byref(fx), _p(psi)))
+    return psi, fx.value
+
+
 def sizes(prob):
     op = make_problem(prob)
     nd, np_, nps = C.c_int(), C.c_int(), C.c_int()
@@ -212,7 +230,7 @@ def solve(prob, nthreads=1, verbose=0, **over):
     """Full solve; returns (phi[G,Np,Nc] in Radiant order, stats list of dicts)."""
     op = make_problem(prob, nthreads=nthreads, verbose=verbose, **over)
     nd, np_, nps, S = sizes(prob)
-    nc = prob.Nx * prob.Ny * prob.Nz
+    nc = prob.Nx * prob.Ny * prob.Nz * spatial_moments(prob)
     phi = np.zeros(prob.G * np_ * nc)
     st = (OracleStats * prob.G)()
     _check(lib().oracle_solve(C.byref(op), _p(phi), st))

@@ -984,6 +984,89 @@ vector<double> fokker_planck_matrix(const string& method, int N, const string& q
     return to_moments(B, Mn_dir, Nd);
 }
 
+// ---------------------------------------------------------------------------------------
+// Spatial closure of the 1D cell (scheme_weights.jl:25-90, sn_1D_bte.jl:27-64)
+// ---------------------------------------------------------------------------------------
+
+// scheme_weights.jl, Ndims = 1 without CSD: ω[Mx+2] for order K = Mx+1.
+vector<double> scheme_weights_1d(const string& scheme, int K) {
+    const int Mx = K - 1;
+    vector<double> w(Mx + 2, 0.0);
+    auto pw = [](int e) { return (e % 2 == 0) ? 1.0 : -1.0; };  // (-1)^e, e >= 0
+    if (scheme == "DD") {
+        w[0] = pw(Mx + 1);
+        for (int n = 0; n <= Mx; ++n) w[n + 1] = 1.0 - pw(Mx + 1 - n);
+    } else if (scheme == "DG") {
+        w[0] = 0.0;
+        for (int n = 0; n <= Mx; ++n) w[n + 1] = 1.0;
+    } else if (scheme == "DG-" || scheme == "DG+") {
+        if (Mx < 1) fail(scheme + " scheme require at least a 2nd-order p expansion.");
+        w[0] = 0.0;
+        for (int n = 0; n < Mx; ++n) w[n + 1] = 1.0;
+        w[Mx + 1] = scheme == "DG-" ? (double)Mx / (2.0 * Mx + 1.0) : 1001.0;
+    } else {
+        fail("Closure relation not implemented yet.");
+    }
+    return w;
+}
+
+// Dense solve S x = b (K x K, row-major S), LU with partial pivoting: the role of Julia's `\`.
+vector<double> lu_solve(vector<double> S, vector<double> b, int K) {
+    for (int k = 0; k < K; ++k) {
+        int piv = k;
+        for (int i = k + 1; i < K; ++i)
+            if (std::fabs(S[i * K + k]) > std::fabs(S[piv * K + k])) piv = i;
+        if (piv != k) {
+            for (int j = 0; j < K; ++j) std::swap(S[k * K + j], S[piv * K + j]);
+            std::swap(b[k], b[piv]);
+        }
+        if (S[k * K + k] == 0.0) fail("singular cell system");
+        for (int i = k + 1; i < K; ++i) {
+            const double f = S[i * K + k] / S[k * K + k];
+            for (int j = k; j < K; ++j) S[i * K + j] -= f * S[k * K + j];
+            b[i] -= f * b[k];
+        }
+    }
+    vector<double> x(K);
+    for (int i = K - 1; i >= 0; --i) {
+        double v = b[i];
+        for (int j = i + 1; j < K; ++j) v -= S[i * K + j] * x[j];
+        x[i] = v / S[i * K + i];
+    }
+    return x;
+}
+
+// sn_1D_bte.jl:27-64 (non-adaptive): the K spatial Legendre moments of the cell's angular flux
+// and the outgoing face flux.  C[i] = sqrt(2i-1) (scheme_weights.jl:409-410).
+void flux_1d_bte(double mu, double St, double dx, const double* Qn, double& fx, int K, const double* om,
+                 double* psi) {
+    double sx = mu > 0 ? 1.0 : (mu < 0 ? -1.0 : 0.0);
+    const bool tang = std::fabs(mu) <= 1e-10;
+    if (tang) { sx = 1.0; mu = 0.0; }
+    const double hx = std::fabs(mu) / dx;
+    auto C = [](int i) { return std::sqrt(2.0 * i - 1.0); };  // 1-based
+    auto spow = [](double s, int e) { double v = 1.0; for (int k = 0; k < e; ++k) v *= s; return v; };
+    vector<double> S((size_t)K * K, 0.0), Q(K, 0.0);
+    for (int ix = 1; ix <= K; ++ix)
+        for (int jx = 1; jx <= K; ++jx) {
+            double& e = S[(ix - 1) * K + (jx - 1)];
+            if (ix == jx) e += St;
+            if (ix >= jx + 1) e -= C(ix) * hx * sx * C(jx) * (1.0 - spow(-1.0, ix - jx));
+            e += C(ix) * hx * spow(sx, ix - 1) * C(jx) * spow(sx, jx - 1) * om[jx];
+        }
+    for (int jx = 1; jx <= K; ++jx) {
+        Q[jx - 1] += Qn[jx - 1];
+        Q[jx - 1] -= C(jx) * hx * (spow(sx, jx - 1) * om[0] - spow(-sx, jx - 1)) * fx;
+    }
+    vector<double> x = lu_solve(S, Q, K);
+    for (int j = 0; j < K; ++j) psi[j] = x[j];
+    if (!tang) {
+        double f = om[0] * fx;
+        for (int jx = 1; jx <= K; ++jx) f += C(jx) * spow(sx, jx - 1) * om[jx] * x[jx - 1];
+        fx = f;
+    }
+}
+
 struct Angular {
     int Nd = 0, Np = 0, Nps = 0;
     vector<double> mu, eta, xi, w;
@@ -1030,6 +1113,10 @@ struct oracle_problem {
     const double* T;
     const double* siga;
     const char* fokker_planck;
+    // spatial closure along x (SN.set_scheme("x", scheme, order)): "DD" when NULL, "DG", "DG-",
+    // "DG+"; order 0 or 1 = first order.  Orders above 1 and closures other than DD: 1D BTE only.
+    const char* scheme;
+    int32_t scheme_order;
 };
 
 struct oracle_stats {
@@ -1055,6 +1142,10 @@ struct Solver {
     vector<double> sigt_eff;    // Σtot as sn_flux.jl:69-112 prepares it [g + G*m]
     vector<double> Mfp;         // Fokker-Planck ℳ [Np x Np] column-major (types 2, 4)
     double lam0 = 0.0;          // its λ₀
+    int K = 1;                  // spatial moments per cell (1D scheme order; 1 elsewhere)
+    bool ho = false;            // the general 1D cell solve (order > 1 or a closure other than DD)
+    vector<double> omx;         // its ω
+    int Ncol = 0;               // moment columns: K * Nc (column = spatial moment + K * cell)
     const double* cur_src[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // this pass's surface sources
     // continuous slowing down: incoming / outgoing energy-face angular flux [n*Nc + c] (𝚽E12 of
     // sn_flux.jl:166), the current group's S⁻ and S⁺ per material (stopping powers / ΔE)
@@ -1142,7 +1233,17 @@ struct Solver {
         if (solver == 6)  // elastic-free: the self-scattering moment l = 0 leaves the removal
             for (int m = 0; m < Nmat; ++m)
                 for (int g = 0; g < G; ++g) sigt_eff[g + (size_t)G * m] -= sig_s(m, g, g, 0);
-        nphi = (size_t)A.Np * Nc;
+        // spatial closure (SN.get_schemes, scheme_weights.jl)
+        const string scheme = p.scheme ? p.scheme : "DD";
+        K = p.scheme_order > 1 ? p.scheme_order : 1;
+        if (K > 1 || scheme != "DD") {
+            if (ndims != 1 || p.csd)
+                fail("closures other than diamond difference of order 1 are supported for the 1D BTE only");
+            omx = scheme_weights_1d(scheme, K);
+            ho = true;
+        }
+        Ncol = K * Nc;
+        nphi = (size_t)A.Np * Ncol;
         nbx = (size_t)A.Nps * 2 * Ny * Nz;
         nby = ndims >= 2 ? (size_t)A.Nps * 2 * Nx * Nz : 0;
         nbz = ndims >= 3 ? (size_t)A.Nps * 2 * Nx * Ny : 0;
@@ -1184,6 +1285,21 @@ struct Solver {
         (void)Mrow;
         bool tang = std::fabs(mu) <= 1e-10;
         double mue = tang ? 0.0 : mu;
+        if (ho) {  // sn_sweep_1D.jl:93-110 with the general cell solve
+            vector<double> Qn(K), psi(K);
+            for (int k = 0; k < Nx; ++k) {
+                int ix = fwd ? k : Nx - 1 - k;
+                for (int is = 0; is < K; ++is) {
+                    double q = 0.0;
+                    for (int p = 0; p < Np; ++p) q += A.B.M[n + (size_t)A.Nd * p] * Ql[p + (size_t)Np * (is + (size_t)K * ix)];
+                    Qn[is] = q;
+                }
+                flux_1d_bte(mu, sig_t(g, matc[ix]), P.dx[ix], Qn.data(), fx, K, omx.data(), psi.data());
+                for (int is = 0; is < K; ++is)
+                    for (int p = 0; p < Np; ++p)
+                        phi[p + (size_t)Np * (is + (size_t)K * ix)] += A.B.D[p + (size_t)Np * n] * psi[is];
+            }
+        } else
         for (int k = 0; k < Nx; ++k) {
             int ix = fwd ? k : Nx - 1 - k;
             int c = ix;
@@ -1343,8 +1459,8 @@ struct Solver {
         // Ql = (homog ? 0 : Qlout) + Σs_l Φl   (scattering_source.jl:25-30; sn_inner_pass.jl:22
         // leaves the in-group scattering out for solver types 4, 5 and 6)
         const bool within = solver != 4 && solver != 5 && solver != 6;
-        for (int c = 0; c < Nc; ++c) {
-            int m = matc[c];
+        for (int c = 0; c < Ncol; ++c) {  // moment columns (spatial moment + K * cell)
+            int m = matc[c / K];
             for (int p = 0; p < Np; ++p) {
                 double q = homog ? 0.0 : Qlout[p + (size_t)Np * c];
                 if (within) q += sig_s(m, g, g, (int)A.B.pl[p]) * phi[p + (size_t)Np * c];
@@ -1353,8 +1469,8 @@ struct Solver {
         }
         // Ql += T ℳ Φl   (fokker_planck_source.jl:26-36, solver types 2 and 4)
         if (solver == 2 || solver == 4) {
-            for (int c = 0; c < Nc; ++c) {
-                const double Tm = P.T[g + (size_t)G * matc[c]];
+            for (int c = 0; c < Ncol; ++c) {
+                const double Tm = P.T[g + (size_t)G * matc[c / K]];
                 for (int n = 0; n < Np; ++n)
                     for (int k = 0; k < Np; ++k)
                         Ql[n + (size_t)Np * c] += Tm * Mfp[n + (size_t)Np * k] * phi[k + (size_t)Np * c];
@@ -1715,14 +1831,14 @@ struct Solver {
             for (int gi = 0; gi < G; ++gi) {
                 if (gi == g || solver == 4 || solver == 5) continue;
                 const double* pg = &phi_all[(size_t)gi * nphi];
-                for (int c = 0; c < Nc; ++c) {
-                    int m = matc[c];
+                for (int c = 0; c < Ncol; ++c) {
+                    int m = matc[c / K];
                     for (int p = 0; p < Np; ++p)
                         Qlout[p + (size_t)Np * c] += sig_s(m, gi, g, (int)A.B.pl[p]) * pg[p + (size_t)Np * c];
                 }
             }
-            // sn_flux.jl:179 -- volume source enters moment p=1 only (Source.jl:135)
-            for (int c = 0; c < Nc; ++c) Qlout[(size_t)Np * c] += P.qvol[g + (size_t)G * c];
+            // sn_flux.jl:179 -- volume source enters moment p=1 and spatial moment 1 only (Source.jl:135)
+            for (int c = 0; c < Nc; ++c) Qlout[(size_t)Np * K * c] += P.qvol[g + (size_t)G * c];
             vector<double> z(S, 0.0);
             std::copy(phi_all.begin() + (size_t)g * nphi, phi_all.begin() + (size_t)(g + 1) * nphi, z.begin());
             if (P.csd) {  // sn_flux.jl:166,183: zero for the first group, then rescaled by ΔE_g/ΔE_{g-1}
@@ -1735,8 +1851,8 @@ struct Solver {
             std::copy(z.begin(), z.begin() + nphi, phi_all.begin() + (size_t)g * nphi);
             if (stats) stats[g] = st;
         }
-        // Radiant layout 𝚽l[g, p, 1, ix, iy, iz]
-        for (int c = 0; c < Nc; ++c)
+        // Radiant layout 𝚽l[g, p, is, ix, iy, iz]
+        for (int c = 0; c < Ncol; ++c)
             for (int p = 0; p < Np; ++p)
                 for (int g = 0; g < G; ++g)
                     phi_out[g + (size_t)G * (p + (size_t)Np * c)] = phi_all[(size_t)g * nphi + p + (size_t)Np * c];
@@ -1821,6 +1937,17 @@ int oracle_fokker_planck(const oracle_problem* p, double* Mfp, double* lam0, int
         std::copy(s.Mfp.begin(), s.Mfp.end(), Mfp);
         *lam0 = s.lam0;
         return s.A.Np;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// One 1D cell of scheme (DD, DG, DG-, DG+) and order K: psi[K] receives the spatial moments of the
+// angular flux, *fx enters as the incoming face flux and leaves as the outgoing one.
+int oracle_cell_1d(const char* scheme, int K, double mu, double St, double dx, const double* Qn, double* fx,
+                   double* psi) {
+    try {
+        vector<double> om = scheme_weights_1d(scheme, K);
+        flux_1d_bte(mu, St, dx, Qn, *fx, K, om.data(), psi);
+        return 0;
     } catch (std::exception& e) { g_err = e.what(); return -1; }
 }
 

@@ -117,6 +117,7 @@ int BatchSolver::add(Problem* P) {
     if (P->scan_cpl <= 0 || sweep_scan_lds_bytes(P64, P->ndims, P->Nx, P->Ny, P->Nz, P->Nmat) > 160 * 1024)
         throw Error(RT_EINVAL, "a batched problem does not fit the row-scan sweep");
     if (P->csd) throw Error(RT_EINVAL, "continuous slowing down is solved one problem at a time");
+    if (P->ho_K) throw Error(RT_EINVAL, "general 1D closures are solved one problem at a time");
     int slot = -1;
     for (int i = 0; i < (int)ps.size(); ++i)
         if (!ps[i].P) {
@@ -376,7 +377,7 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
         Problem* P = ps[who[i]].P;
         P->counters.passes += 1;
         if (prec[i] != P64) P->counters.passes_lo += 1;
-        if (full[i] != 2) P->counters.updates_raw += (double)P->Nc * P->Nd;  // sweeps only
+        if (full[i] != 2) P->counters.updates_raw += (double)P->Ncell * P->Nd;  // sweeps only
     }
 }
 
@@ -552,7 +553,7 @@ void BatchSolver::run(rt_solve_info* info) {
             info->passes += s.P->counters.passes;
             info->passes_lo += s.P->counters.passes_lo;
             info->updates_raw += s.P->counters.updates_raw;
-            info->updates_problem += (double)s.P->Nc * s.P->Nd * s.P->G;
+            info->updates_problem += (double)s.P->Ncell * s.P->Nd * s.P->G;
         }
     }
 }

@@ -39,6 +39,14 @@ static void upload(T* d, const std::vector<T>& h, hipStream_t st) {
 
 // SN solver type (SN.jl get_solver_type): 0 in the description means BTE, or BCSD with csd.
 static int solver_type_of(const rt_problem_desc& d) { return d.solver_type ? d.solver_type : (d.csd ? 3 : 1); }
+// Spatial moments per cell of the general 1D closure (SN.set_scheme): 0 for diamond difference of
+// order 1, which the order-1 kernels take.
+static int ho_order(const rt_problem_desc& d) {
+    const int K = d.order_x > 1 ? d.order_x : 1;
+    if (K == 1 && (!d.omega_x || (d.omega_x[0] == -1.0 && d.omega_x[1] == 2.0))) return 0;
+    return K;
+}
+
 static bool has_fp(const rt_problem_desc& d) {
     const int t = solver_type_of(d);
     return (t == 2 || t == 4) && d.T && d.Mfp;
@@ -105,6 +113,11 @@ static void validate(const rt_problem_desc& d) {
     }
     if (surf && (d.Np_source < 1 || d.Np_source > d.Nps))
         bad("Np_source must lie in 1..Nps when a surface source is given");
+    if (d.order_x < 0 || d.order_x > HO_KMAX)
+        bad("spatial closure order must lie in 1.." + std::to_string(HO_KMAX));
+    if (d.order_x > 1 && !d.omega_x) bad("closures of order above 1 need their weights omega_x");
+    if (ho_order(d) && (d.ndims != 1 || d.csd))
+        bad("closures other than diamond difference of order 1 are supported for the 1D BTE only");
     if (d.solver_type < 0 || d.solver_type > 6) bad("Unknown type of solver.");
     const int stype = solver_type_of(d);
     if ((stype >= 2) != (d.csd != 0))
@@ -130,7 +143,9 @@ static void validate(const rt_problem_desc& d) {
 
 Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
     validate(d);
-    ndims = d.ndims; Nx = d.Nx; Ny = d.Ny; Nz = d.Nz; Nc = Nx * Ny * Nz;
+    ndims = d.ndims; Nx = d.Nx; Ny = d.Ny; Nz = d.Nz; Ncell = Nx * Ny * Nz;
+    ho_K = ho_order(d);
+    Nc = Ncell * (ho_K ? ho_K : 1);  // moment columns: spatial moment + K * cell
     G = d.G; Nmat = d.Nmat; Nd = d.Nd; Np = d.Np; Nps = d.Nps;
     accel = d.accel; restart = d.restart; depth = d.depth; maxit = d.maxit; tol = d.tol;
     precision = d.precision;
@@ -154,9 +169,15 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
     }
     try {
         // ---- geometry
-        std::vector<int> mat(Nc);
-        for (int c = 0; c < Nc; ++c) mat[c] = (int)(d.mat[c] - 1);
+        const int Kc = Nc / Ncell;
+        std::vector<int> mat(Nc);  // per moment column
+        for (int c = 0; c < Nc; ++c) mat[c] = (int)(d.mat[c / Kc] - 1);
         d_mat = dalloc<int>(Nc); upload(d_mat, mat, st);
+        if (ho_K) {
+            std::vector<double> om(d.omega_x ? d.omega_x : nullptr, d.omega_x ? d.omega_x + ho_K + 1 : nullptr);
+            if (om.empty()) om = {-1.0, 2.0};
+            d_omx = dalloc<double>(om.size()); upload(d_omx, om, st);
+        }
         auto up = [&](const double* src, int n) {
             double* p = dalloc<double>(n);
             RT_CHECK(hipMemcpy(p, src, n * sizeof(double), hipMemcpyHostToDevice));
@@ -218,9 +239,9 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
         }
         d_sigw = dalloc<double>(sigw.size()); upload(d_sigw, sigw, st);
         d_sigo = dalloc<double>(sigo.size()); upload(d_sigo, sigo, st);
-        std::vector<double> qv((size_t)G * Nc);
-        for (int c = 0; c < Nc; ++c)
-            for (int g = 0; g < G; ++g) qv[(size_t)g * Nc + c] = d.qvol[g + (size_t)G * c];
+        std::vector<double> qv((size_t)G * Nc, 0.0);  // spatial moment 1 of each cell
+        for (int c = 0; c < Ncell; ++c)
+            for (int g = 0; g < G; ++g) qv[(size_t)g * Nc + (size_t)Kc * c] = d.qvol[g + (size_t)G * c];
         d_qvol = dalloc<double>(qv.size()); upload(d_qvol, qv, st);
         // ---- angular operators (row-major device copies)
         // k-major device copies: MT[a][n] = M[n, act a], DT[n][p] = D[p, n], DTact[n][a] = D[act a, n]
@@ -415,7 +436,7 @@ Problem::~Problem() {
 }
 
 int64_t Problem::device_bytes(const rt_problem_desc& d) {
-    const int64_t Nc = (int64_t)d.Nx * d.Ny * d.Nz;
+    const int64_t Nc = (int64_t)d.Nx * d.Ny * d.Nz * (ho_order(d) ? ho_order(d) : 1);
     const int Nact_est = (d.sigs && d.pl && d.Ls >= 0) ? (int)active_moments(d).size() : d.Np;
     int64_t b = 0;
     b += Nc * 4 + (int64_t)d.G * Nc * 8;                        // mat, qvol
@@ -475,6 +496,8 @@ void Problem::fill_geometry(PassDesc& pd, int g) {
     pd.bx = d_bex[0]; pd.by = d_bex[1]; pd.bz = d_bex[2];
     pd.Nps = Nps;
     if (fp) { pd.fpT = d_fpT + (size_t)g * Nmat; pd.fpM = d_fpM; }
+    pd.ho_K = ho_K;
+    pd.ho_om = d_omx;
 }
 
 hipEvent_t Problem::next_event() {
@@ -524,7 +547,9 @@ void Problem::launch_pass(PassDesc& pd, int prec, bool full_rows) {
     if (pd.psiE_in) launch_csd_source(d_pd, (long)Nd * Nc, st);  // energy inflow (FP64 passes)
     timed(1, [&] {
         const int sp = prec == PBF16 ? P32 : prec;
-        if (sweep_kind == 0 && scan_cpl > 0 && sweep_scan_lds_bytes(sp, ndims, Nx, Ny, Nz, Nmat) <= 160 * 1024)
+        if (ho_K)
+            launch_sweep_1d_ho(sp, d_pd, Nd, ho_K, st);
+        else if (sweep_kind == 0 && scan_cpl > 0 && sweep_scan_lds_bytes(sp, ndims, Nx, Ny, Nz, Nmat) <= 160 * 1024)
             launch_sweep_scan(sp, d_pd, d_sw, Nd, scan_cpl, scan_waves, sweep_scan_lds_bytes(sp, ndims, Nx, Ny, Nz, Nmat),
                               st, pd.bin_x != nullptr);
         else if (rows_per_lane > 0 && sweep_rows_lds_bytes(sp, Nx, Nmat) <= 160 * 1024)
@@ -540,7 +565,7 @@ void Problem::launch_pass(PassDesc& pd, int prec, bool full_rows) {
     if (pd.bout_x) timed(3, [&] { launch_bndmom(d_pd, 1, (int)nb, st); });
     counters.passes += 1;
     if (prec != P64) counters.passes_lo += 1;
-    counters.updates_raw += (double)Nc * Nd;
+    counters.updates_raw += (double)Ncell * Nd;
 }
 
 // Faithful state [Φ (Np x Nc) | boundary moments] (sn_one_speed.jl:126-134), FP64.
@@ -1076,7 +1101,7 @@ void Problem::solve(rt_group_stats* stats, rt_solve_info* info) {
         info->passes = counters.passes;
         info->passes_lo = counters.passes_lo;
         info->updates_raw = counters.updates_raw;
-        info->updates_problem = (double)Nc * Nd * G;
+        info->updates_problem = (double)Ncell * Nd * G;
         if (profiling) {
             info->ms_mgemm = ktimes.ms[0];
             info->ms_sweep = ktimes.ms[1];

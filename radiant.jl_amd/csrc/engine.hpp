@@ -74,6 +74,8 @@ public:
 
     // ---- pieces used by the batched and served drivers ----
     int ndims, Nx, Ny, Nz, Nc, G, Nmat, Nd, Np, Nps, Nact;
+    int Ncell = 0;   // cells; Nc counts moment columns (Ncell times the spatial moments per cell)
+    int ho_K = 0;    // spatial moments of the general 1D closure (0: diamond difference, order 1)
     int accel, restart, depth, maxit, precision, state_mode, verbose;
     double tol;
     hipStream_t st;
@@ -120,6 +122,7 @@ private:
     // material, and the source qlout + T ℳ Φ the contraction reads
     bool fp = false;
     double *d_fpM = nullptr, *d_fpT = nullptr, *d_fpq = nullptr;
+    double* d_omx = nullptr;  // closure weights of the general 1D cell
     double* d_Mentry[3] = {nullptr, nullptr, nullptr};
     int* d_insrc[3] = {nullptr, nullptr, nullptr};
     double* d_bin[3] = {nullptr, nullptr, nullptr};

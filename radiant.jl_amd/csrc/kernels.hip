@@ -1314,6 +1314,141 @@ __global__ __launch_bounds__(256) void fp_source_kernel(const PassDesc* __restri
 }
 
 // ------------------------------------------------------------------------------------------
+// 1D sweep with K spatial Legendre moments per cell and a general closure (sn_sweep_1D.jl:51-124
+// with flux_1D_BTE, sn_1D_bte.jl:27-64): one wave per direction.  A cell's K x K system
+//   S_ij = St d_ij - [i > j] C_i hx sx C_j (1 - (-1)^(i-j)) + C_i hx sx^(i-1) C_j sx^(j-1) w_(j+1)
+// is solved for two right-hand sides, the source moments Q and r_j = C_j hx (sx^(j-1) w_0 -
+// (-sx)^(j-1)), so that psi = S^-1 Q - f_in S^-1 r and the outgoing face flux
+// f_out = w_0 f_in + sum_j C_j sx^(j-1) w_(j+1) psi_j is an affine map a f_in + b of the incoming
+// one (a = 1, b = 0 for a tangential direction, which passes the face flux through).  The 64 lanes
+// take CPL consecutive cells each, compose their maps, and scan them across the wave; one step
+// covers 64 * CPL cells.  Source and flux moments: psi[n][j + K * cell] in the pass's precision.
+// ------------------------------------------------------------------------------------------
+template <typename T, int K>
+__global__ __launch_bounds__(64) void sweep_1d_ho_kernel(const PassDesc* __restrict__ pds) {
+    constexpr int CPL = 4;
+    const PassDesc& d = pds[0];
+    const int n = blockIdx.x, lane = threadIdx.x;
+    const int Nx = d.Nx, Nc = d.Nc;
+    double mu = d.mu[n];
+    const bool fwd = mu >= 0.0;
+    const bool tang = fabs(mu) <= 1e-10;
+    const T sx = (tang || mu > 0.0) ? T(1) : T(-1);
+    if (tang) mu = 0.0;
+    T om[K + 1], Cc[K], sp[K], sn[K];   // w, C_j, sx^(j-1), (-sx)^(j-1)
+#pragma unroll
+    for (int i = 0; i <= K; ++i) om[i] = (T)d.ho_om[i];
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        Cc[i] = (T)sqrt(2.0 * i + 1.0);
+        sp[i] = i == 0 ? T(1) : sp[i - 1] * sx;
+        sn[i] = i == 0 ? T(1) : -sn[i - 1] * sx;
+    }
+    T* __restrict__ psi = (T*)d.psi + (size_t)n * Nc;
+    T f = d.bin_x ? (T)d.bin_x[n] : T(0);  // incoming face flux of the current step
+    for (int base = 0; base < Nx; base += 64 * CPL) {
+        T a[CPL], b[CPL], xq[CPL][K], xr[CPL][K];
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            const int k = base + lane * CPL + c;
+            a[c] = T(1);
+            b[c] = T(0);
+#pragma unroll
+            for (int j = 0; j < K; ++j) { xq[c][j] = T(0); xr[c][j] = T(0); }
+            if (k < Nx) {
+                const int ix = fwd ? k : Nx - 1 - k;
+                const T hx = (T)(fabs(mu) / d.dx[ix]);
+                const T St = (T)d.sigt[d.mat[(size_t)K * ix]];
+                T S[K][K];
+#pragma unroll
+                for (int i = 0; i < K; ++i) {
+#pragma unroll
+                    for (int j = 0; j < K; ++j) {
+                        T e = i == j ? St : T(0);
+                        if (i > j && ((i - j) & 1)) e -= T(2) * Cc[i] * hx * sx * Cc[j];
+                        e += Cc[i] * hx * sp[i] * Cc[j] * sp[j] * om[j + 1];
+                        S[i][j] = e;
+                    }
+                    xq[c][i] = psi[i + (size_t)K * ix];
+                    xr[c][i] = Cc[i] * hx * (sp[i] * om[0] - sn[i]);
+                }
+                // Gaussian elimination with partial pivoting (row swaps as selects)
+#pragma unroll
+                for (int p = 0; p < K; ++p) {
+#pragma unroll
+                    for (int i = p + 1; i < K; ++i) {
+                        const bool sw = fabs(S[i][p]) > fabs(S[p][p]);
+#pragma unroll
+                        for (int j = 0; j < K; ++j) {
+                            const T t = S[p][j];
+                            S[p][j] = sw ? S[i][j] : t;
+                            S[i][j] = sw ? t : S[i][j];
+                        }
+                        const T tq = xq[c][p], tr = xr[c][p];
+                        xq[c][p] = sw ? xq[c][i] : tq;
+                        xq[c][i] = sw ? tq : xq[c][i];
+                        xr[c][p] = sw ? xr[c][i] : tr;
+                        xr[c][i] = sw ? tr : xr[c][i];
+                    }
+#pragma unroll
+                    for (int i = p + 1; i < K; ++i) {
+                        const T fct = S[i][p] / S[p][p];
+#pragma unroll
+                        for (int j = p + 1; j < K; ++j) S[i][j] -= fct * S[p][j];
+                        xq[c][i] -= fct * xq[c][p];
+                        xr[c][i] -= fct * xr[c][p];
+                    }
+                }
+#pragma unroll
+                for (int i = K - 1; i >= 0; --i) {
+                    T vq = xq[c][i], vr = xr[c][i];
+#pragma unroll
+                    for (int j = i + 1; j < K; ++j) { vq -= S[i][j] * xq[c][j]; vr -= S[i][j] * xr[c][j]; }
+                    xq[c][i] = vq / S[i][i];
+                    xr[c][i] = vr / S[i][i];
+                }
+                if (!tang) {
+                    T ca = om[0], cb = T(0);
+#pragma unroll
+                    for (int j = 0; j < K; ++j) {
+                        const T cj = Cc[j] * sp[j] * om[j + 1];
+                        ca -= cj * xr[c][j];
+                        cb += cj * xq[c][j];
+                    }
+                    a[c] = ca;
+                    b[c] = cb;
+                }
+            }
+        }
+        // the lane's map over its cells, then an inclusive scan of the maps across the wave
+        T A = a[0], B = b[0];
+#pragma unroll
+        for (int c = 1; c < CPL; ++c) { B = a[c] * B + b[c]; A = a[c] * A; }
+#pragma unroll
+        for (int dl = 1; dl < 64; dl <<= 1) {
+            const T Ap = __shfl_up(A, dl), Bp = __shfl_up(B, dl);
+            if (lane >= dl) { B = A * Bp + B; A = A * Ap; }
+        }
+        T Ae = __shfl_up(A, 1), Be = __shfl_up(B, 1);
+        if (lane == 0) { Ae = T(1); Be = T(0); }
+        const T fnext = __shfl(A, 63) * f + __shfl(B, 63);
+        T fin = Ae * f + Be;
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            const int k = base + lane * CPL + c;
+            if (k < Nx) {
+                const int ix = fwd ? k : Nx - 1 - k;
+#pragma unroll
+                for (int j = 0; j < K; ++j) psi[j + (size_t)K * ix] = xq[c][j] - xr[c][j] * fin;
+                fin = a[c] * fin + b[c];
+            }
+        }
+        f = fnext;
+    }
+    if (lane == 0 && d.bx) ((T*)d.bx)[n] = f;  // exit face flux (one position in 1D)
+}
+
+// ------------------------------------------------------------------------------------------
 // Out-of-group + volume source (scattering_source.jl:59-68, sn_flux.jl:179).
 // ------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void outsrc_kernel(double* __restrict__ qlout, const double* __restrict__ phi_all,
@@ -1767,6 +1902,23 @@ void launch_csd_out(const PassDesc* d_pd, long n, hipStream_t st) {
     hipLaunchKernelGGL(csd_out_kernel, dim3(grid_for((size_t)n)), dim3(256), 0, st, d_pd, n);
     RT_HIP(hipGetLastError());
 }
+template <typename T>
+static void ho_launch(const PassDesc* d_pd, int Nd, int K, hipStream_t st) {
+    switch (K) {
+        case 1: hipLaunchKernelGGL((sweep_1d_ho_kernel<T, 1>), dim3(Nd), dim3(64), 0, st, d_pd); break;
+        case 2: hipLaunchKernelGGL((sweep_1d_ho_kernel<T, 2>), dim3(Nd), dim3(64), 0, st, d_pd); break;
+        case 3: hipLaunchKernelGGL((sweep_1d_ho_kernel<T, 3>), dim3(Nd), dim3(64), 0, st, d_pd); break;
+        case 4: hipLaunchKernelGGL((sweep_1d_ho_kernel<T, 4>), dim3(Nd), dim3(64), 0, st, d_pd); break;
+        default: throw std::runtime_error("1D closure order out of range");
+    }
+}
+void launch_sweep_1d_ho(int prec, const PassDesc* d_pd, int Nd, int K, hipStream_t st) {
+    if (Nd <= 0) return;
+    if (prec == P64) ho_launch<double>(d_pd, Nd, K, st);
+    else ho_launch<float>(d_pd, Nd, K, st);
+    RT_HIP(hipGetLastError());
+}
+
 void launch_fp_source(const PassDesc* d_pd, long n, hipStream_t st) {
     hipLaunchKernelGGL(fp_source_kernel, dim3(grid_for((size_t)n)), dim3(256), 0, st, d_pd, n);
     RT_HIP(hipGetLastError());

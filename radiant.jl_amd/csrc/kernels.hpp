@@ -63,6 +63,10 @@ struct PassDesc {
     const double* fpM;
     const double* fp_qin;
     double* fpq;
+    // 1D cells with ho_K spatial Legendre moments and the closure weights ho_om[ho_K + 1] (0 = the
+    // diamond-difference kernels of order 1); moment columns are then spatial moment + ho_K * cell
+    int ho_K;
+    const double* ho_om;
 };
 
 enum Prec { P64 = 0, P32 = 1, PBF16 = 2,
@@ -109,6 +113,10 @@ void launch_csd_out(const PassDesc* d_pd, long n, hipStream_t st);
 // angular Fokker-Planck source, one problem: fpq[k][c] = fp_qin[k][c] + T[mat c] sum_j ℳ[k][j]
 // phi_in[row j][c], n = Nact * Nc
 void launch_fp_source(const PassDesc* d_pd, long n, hipStream_t st);
+// 1D sweep of a general closure with K = PassDesc::ho_K (1..HO_KMAX) spatial moments per cell, one
+// problem, one wave per direction
+constexpr int HO_KMAX = 4;
+void launch_sweep_1d_ho(int prec, const PassDesc* d_pd, int Nd, int K, hipStream_t st);
 void launch_dgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st);
 void launch_bndmom(const PassDesc* d_pd, int nprob, int max_items, hipStream_t st);
 

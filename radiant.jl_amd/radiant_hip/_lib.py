@@ -39,6 +39,7 @@ class ProblemDesc(C.Structure):
         ("ssrc", dp * 6), ("Np_source", C.c_int32),
         ("csd", C.c_int32), ("Sb", dp), ("dE", dp),
         ("solver_type", C.c_int32), ("T", dp), ("Mfp", dp),
+        ("order_x", C.c_int32), ("omega_x", dp),
     ]
 
 

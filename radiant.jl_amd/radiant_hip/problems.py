@@ -99,6 +99,11 @@ class Problem:
     T: np.ndarray = None
     siga: np.ndarray = None
     fokker_planck: str = "finite-difference"
+    # spatial closure along x (SN.set_scheme("x", scheme, order)): "DD", "DG", "DG-" or "DG+" and
+    # the number of spatial Legendre moments per cell; orders above 1 and closures other than DD
+    # are 1D BTE only, and the flux then holds scheme_order spatial moments per cell
+    scheme: str = "DD"
+    scheme_order: int = 1
 
     @property
     def Nc(self):

@@ -1,0 +1,40 @@
+"""Spatial closure weights of the 1D cell: scheme_weights.jl:25-90 (Ndims = 1, no slowing down),
+the part of Radiant's set-up that produces ω for ``flux_1D_BTE`` (sn_1D_bte.jl:27-64).
+
+For an order K (spatial Legendre moments per cell) the closure relates the outgoing face flux to
+the incoming one and the cell moments: f_out = ω[0] f_in + sum_j C_j sx^(j-1) ω[j] psi_j with
+C_j = sqrt(2j - 1).  ``DD`` is the high-order diamond difference (exact on polynomials of degree
+K), ``DG`` the upwind discontinuous Galerkin closure (exact to degree K - 1), ``DG-`` / ``DG+``
+change its last weight.  The adaptive ``AWD`` closure makes the sweep non-linear and is not
+provided.
+"""
+import numpy as np
+
+SCHEMES = ("DD", "DG", "DG-", "DG+")
+
+
+def weights_1d(scheme, K):
+    """ω [K + 1] of the x axis."""
+    scheme = scheme.upper()
+    if scheme not in SCHEMES:
+        raise ValueError("Closure relation not implemented yet.")
+    Mx = K - 1
+    w = np.zeros(K + 1)
+    n = np.arange(K)
+    if scheme == "DD":
+        w[0] = (-1.0) ** (Mx + 1)
+        w[1:] = 1.0 - (-1.0) ** (Mx + 1 - n)
+    else:
+        if scheme != "DG" and Mx < 1:
+            raise ValueError(f"{scheme} scheme require at least a 2nd-order p expansion.")
+        w[1:] = 1.0
+        if scheme == "DG-":
+            w[K] = Mx / (2.0 * Mx + 1.0)
+        elif scheme == "DG+":
+            w[K] = 1001.0
+    return w
+
+
+def spatial_moments(prob):
+    """Spatial moments per cell of a problem (its 1D scheme order; 1 otherwise)."""
+    return max(1, int(getattr(prob, "scheme_order", 1) or 1)) if prob.ndims == 1 else 1

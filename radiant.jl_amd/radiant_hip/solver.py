@@ -11,7 +11,7 @@ import time
 
 import numpy as np
 
-from . import _lib, fokker_planck
+from . import _lib, fokker_planck, schemes
 from .angular import angular_set, quadrature_dimension
 from ._lib import ACCEL, PREC, STATE, check, lib, ptr
 
@@ -71,6 +71,11 @@ class Desc:
             k["dE"] = f64(prob.dE)
             d.csd, d.Sb, d.dE = 1, ptr(k["Sb"]), ptr(k["dE"])
         d.solver_type = stype
+        K = schemes.spatial_moments(prob)
+        scheme = (getattr(prob, "scheme", None) or "DD").upper()
+        if K > 1 or scheme != "DD":  # closure weights of the x axis (scheme_weights.jl)
+            k["omega_x"] = np.ascontiguousarray(schemes.weights_1d(scheme, K))
+            d.order_x, d.omega_x = K, ptr(k["omega_x"])
         if Mfp is not None:  # T [G, Nmat], ℳ [Np, Np], both column-major
             k["T"] = f64(T)
             k["Mfp"] = f64(Mfp)
@@ -83,7 +88,8 @@ class Desc:
         d.precision = PREC[precision]
         d.state_mode = STATE[state]
         d.verbose = verbose
-        self.G, self.Np, self.Nc, self.Nd = prob.G, ang.Np, prob.Nc, ang.Nd
+        # moment columns: spatial moment + K * cell
+        self.G, self.Np, self.Nc, self.Nd = prob.G, ang.Np, prob.Nc * K, ang.Nd
 
     def byref(self):
         return C.byref(self.desc)
