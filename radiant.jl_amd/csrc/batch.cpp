@@ -118,6 +118,8 @@ int BatchSolver::add(Problem* P) {
         throw Error(RT_EINVAL, "a batched problem does not fit the row-scan sweep");
     if (P->csd) throw Error(RT_EINVAL, "continuous slowing down is solved one problem at a time");
     if (P->ho_K) throw Error(RT_EINVAL, "general 1D closures are solved one problem at a time");
+    if (P->has_bc || P->has_ss)
+        throw Error(RT_EINVAL, "non-void faces and surface sources are solved one problem at a time");
     int slot = -1;
     for (int i = 0; i < (int)ps.size(); ++i)
         if (!ps[i].P) {

@@ -104,9 +104,6 @@ static void validate(const rt_problem_desc& d) {
     bool surf = false;
     for (int f = 0; f < 2 * d.ndims; ++f) surf = surf || d.ssrc[f] != nullptr;
     if (nonvoid || surf) {
-        if (d.precision != RT_PREC_FP64 || d.state_mode != RT_STATE_FAITHFUL)
-            bad("reflective/periodic boundaries and surface sources need the FP64 faithful state (RT_PREC_FP64, "
-                "RT_STATE_FAITHFUL)");
         for (int f = 0; f < 2 * d.ndims; ++f)
             if (!d.Msurf[f] || !d.Dsurf[f] || !d.n_to_np[f])
                 bad("reflective/periodic boundaries and surface sources need the half-range operators of every face");
@@ -159,8 +156,13 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
     npos[1] = ndims >= 2 ? (size_t)Nx * Nz : 0;
     npos[2] = ndims >= 3 ? (size_t)Nx * Ny : 0;
     nb = (size_t)2 * Nps * (npos[0] + npos[1] + npos[2]);
+    for (int f = 0; f < 2 * ndims; ++f) {
+        has_bc = has_bc || d.bc[f] != 0;
+        has_ss = has_ss || d.ssrc[f] != nullptr;
+    }
     S_f = (size_t)Np * Nc + nb;
-    S_r = (size_t)Nact * Nc;
+    // the reduced state carries the boundary moments when they feed back (reflective, periodic)
+    S_r = (size_t)Nact * Nc + (has_bc ? nb : 0);
     if (stream) {
         st = stream;
     } else {
@@ -261,7 +263,7 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
         d_Dact32 = dalloc<float>(Dact.size()); upload(d_Dact32, tof(Dact), st);
         d_act = dalloc<int>(Nact); upload(d_act, act, st);
         // ---- exit-face half-range D columns (sn_inner_pass.jl:256-317 selection)
-        if (state_mode == RT_STATE_FAITHFUL) {
+        if (state_mode == RT_STATE_FAITHFUL || has_bc || has_ss) {
             for (int ax = 0; ax < ndims; ++ax) {
                 std::vector<double> Dex((size_t)Nd * Nps, 0.0);
                 std::vector<int> ef(Nd, 0);
@@ -290,10 +292,6 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
             }
             // ---- reflective / periodic faces (sn_sweep_2D.jl:70-99 and the x/z analogues): the
             // entry half-range M row of every direction and the face whose exit moments feed it
-            for (int f = 0; f < 2 * ndims; ++f) {
-                has_bc = has_bc || d.bc[f] != 0;
-                has_ss = has_ss || d.ssrc[f] != nullptr;
-            }
             if (has_bc || has_ss) {
                 const size_t npf[3] = {(size_t)Ny * Nz, (size_t)Nx * Nz, (size_t)Nx * Ny};
                 for (int ax = 0; ax < 3; ++ax) {
@@ -393,7 +391,7 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
             if (ev[0] == 'r') sweep_kind = 1;
             if (ev[0] == 'g') { sweep_kind = 2; rows_per_lane = -1; planes_ok[0] = planes_ok[1] = false; }
         }
-        if ((has_bc || has_ss) && !(sweep_kind == 0 && scan_cpl > 0 &&
+        if ((has_bc || has_ss) && !ho_K && !(sweep_kind == 0 && scan_cpl > 0 &&
                         sweep_scan_lds_bytes(P64, ndims, Nx, Ny, Nz, Nmat) <= 160 * 1024))
             throw Error(RT_EINVAL, "reflective/periodic boundaries and surface sources need the row-scan sweep "
                                    "(Nx <= 512 and its LDS face buffers within 160 KiB)");
@@ -448,7 +446,10 @@ int64_t Problem::device_bytes(const rt_problem_desc& d) {
                           (d.ndims >= 3 ? (int64_t)d.Nx * d.Ny : 0);
     b += (int64_t)d.Nd * nposs * 8 + 2LL * d.Nps * nposs * 8;
     const bool faithful = d.precision == RT_PREC_FP64 && d.state_mode == RT_STATE_FAITHFUL;
-    const int64_t S = faithful ? (int64_t)d.Np * Nc + 2LL * d.Nps * nposs : (int64_t)Nact_est * Nc;
+    bool nonvoid = false;
+    for (int f = 0; f < 2 * d.ndims && d.bc; ++f) nonvoid = nonvoid || d.bc[f] != 0;
+    const int64_t S = faithful ? (int64_t)d.Np * Nc + 2LL * d.Nps * nposs
+                               : (int64_t)Nact_est * Nc + (nonvoid ? 2LL * d.Nps * nposs : 0);
     int nvec = 8;
     if (d.accel == RT_ACCEL_GMRES) nvec = d.restart + 6;
     if (d.accel == RT_ACCEL_ANDERSON) nvec = 2 * d.depth + 8;
@@ -581,32 +582,38 @@ void Problem::pass_faithful(int g, int prec, int mode, const double* zin, double
     pd.out = zout;
     pd.zin = (mode == MODE_IMA) ? zin : nullptr;
     if (mode == MODE_T) set_csd(pd, g);
-    double* b = zout + (size_t)Np * Nc;
-    pd.bout_x = b;
-    pd.bout_y = ndims >= 2 ? b + 2 * (size_t)Nps * npos[0] : nullptr;
-    pd.bout_z = ndims >= 3 ? b + 2 * (size_t)Nps * (npos[0] + npos[1]) : nullptr;
-    if (mode == MODE_IMA) {
-        const double* bi = zin + (size_t)Np * Nc;
-        pd.bzin_x = bi;
-        pd.bzin_y = ndims >= 2 ? bi + 2 * (size_t)Nps * npos[0] : nullptr;
-        pd.bzin_z = ndims >= 3 ? bi + 2 * (size_t)Nps * (npos[0] + npos[1]) : nullptr;
+    set_boundary(pd, g, mode, zin, zout, (size_t)Np * Nc);
+    (void)prec;
+    launch_pass(pd, P64, true);
+}
+
+// Boundary part of a pass: the exit-face moments written into zout's boundary part at offset off
+// (I - A epilogue from zin's in MODE_IMA; zout null: none), and for non-void faces or surface
+// sources the incoming face fluxes, formed from zin's boundary part (zin null: zero) and, in
+// inhomogeneous passes, the group's surface sources.
+void Problem::set_boundary(PassDesc& pd, int g, int mode, const double* zin, double* zout, size_t off) {
+    auto split = [&](const double* b, const double** x, const double** y, const double** z) {
+        *x = b;
+        *y = ndims >= 2 ? b + 2 * (size_t)Nps * npos[0] : nullptr;
+        *z = ndims >= 3 ? b + 2 * (size_t)Nps * (npos[0] + npos[1]) : nullptr;
+    };
+    if (zout && nb) {
+        double* b = zout + off;
+        pd.bout_x = b;
+        pd.bout_y = ndims >= 2 ? b + 2 * (size_t)Nps * npos[0] : nullptr;
+        pd.bout_z = ndims >= 3 ? b + 2 * (size_t)Nps * (npos[0] + npos[1]) : nullptr;
+        if (mode == MODE_IMA) split(zin + off, &pd.bzin_x, &pd.bzin_y, &pd.bzin_z);
+        pd.Dexit_x = d_Dexit[0]; pd.Dexit_y = d_Dexit[1]; pd.Dexit_z = d_Dexit[2];
+        pd.exitf_x = d_exitf[0]; pd.exitf_y = d_exitf[1]; pd.exitf_z = d_exitf[2];
+        pd.exlist_x = d_exlist[0]; pd.exlist_y = d_exlist[1]; pd.exlist_z = d_exlist[2];
+        for (int ax = 0; ax < 3; ++ax)
+            for (int f = 0; f < 2; ++f) pd.excnt[ax][f] = excnt[ax][f];
     }
-    pd.Dexit_x = d_Dexit[0]; pd.Dexit_y = d_Dexit[1]; pd.Dexit_z = d_Dexit[2];
-    pd.exitf_x = d_exitf[0]; pd.exitf_y = d_exitf[1]; pd.exitf_z = d_exitf[2];
-    pd.exlist_x = d_exlist[0]; pd.exlist_y = d_exlist[1]; pd.exlist_z = d_exlist[2];
-    for (int ax = 0; ax < 3; ++ax)
-        for (int f = 0; f < 2; ++f) pd.excnt[ax][f] = excnt[ax][f];
-    if (nb == 0) pd.bout_x = nullptr;
     if (has_bc || has_ss) {
         pd.Mentry_x = d_Mentry[0]; pd.Mentry_y = d_Mentry[1]; pd.Mentry_z = d_Mentry[2];
         pd.insrc_x = d_insrc[0]; pd.insrc_y = d_insrc[1]; pd.insrc_z = d_insrc[2];
         pd.bin_x = d_bin[0]; pd.bin_y = d_bin[1]; pd.bin_z = d_bin[2];
-        if (zin) {  // the boundary part of the input state; none for T(0)
-            const double* bi = zin + (size_t)Np * Nc;
-            pd.bsin_x = bi;
-            pd.bsin_y = ndims >= 2 ? bi + 2 * (size_t)Nps * npos[0] : nullptr;
-            pd.bsin_z = ndims >= 3 ? bi + 2 * (size_t)Nps * (npos[0] + npos[1]) : nullptr;
-        }
+        if (zin) split(zin + off, &pd.bsin_x, &pd.bsin_y, &pd.bsin_z);  // none for T(0)
         if (has_ss && mode == MODE_T) {  // surface sources enter the inhomogeneous passes only
             const size_t npf[3] = {(size_t)Ny * Nz, (size_t)Nx * Nz, (size_t)Nx * Ny};
             const double** ps[3] = {&pd.ssrc_x, &pd.ssrc_y, &pd.ssrc_z};
@@ -617,8 +624,6 @@ void Problem::pass_faithful(int g, int prec, int mode, const double* zin, double
             pd.sface_x = d_sface[0]; pd.sface_y = d_sface[1]; pd.sface_z = d_sface[2];
         }
     }
-    (void)prec;
-    launch_pass(pd, P64, true);
 }
 
 // Reduced state [Φ_act (Nact x Nc)]: only the moments that feed back into the next pass.
@@ -634,6 +639,9 @@ void Problem::desc_reduced(PassDesc& pd, int g, int prec, int mode, const double
     pd.out = zout;
     pd.zin = (mode == MODE_IMA) ? zin : nullptr;
     if (mode == MODE_T && prec == P64) set_csd(pd, g);
+    // non-void faces: the state's boundary moments follow its Nact x Nc moments
+    if (has_bc || has_ss)
+        set_boundary(pd, g, mode, has_bc ? zin : nullptr, has_bc ? zout : nullptr, (size_t)Nact * Nc);
 }
 
 void Problem::pass_reduced(int g, int prec, int mode, const double* zin, double* zout) {
@@ -654,6 +662,7 @@ void Problem::desc_reconstruct(PassDesc& pd, int g, const double* zin_reduced) {
     pd.out = d_phi + (size_t)g * Np * Nc;
     pd.zin = nullptr;
     set_csd(pd, g);
+    if (has_bc || has_ss) set_boundary(pd, g, MODE_T, has_bc ? zin_reduced : nullptr, nullptr, (size_t)Nact * Nc);
 }
 
 // Continuous slowing down in an inhomogeneous pass: the energy inflow psiE_in is a fixed source

@@ -157,6 +157,7 @@ private:
     void pass_reconstruct(int g, const double* zin_reduced);
     void desc_reduced(PassDesc& pd, int g, int prec, int mode, const double* zin, double* zout);
     void desc_reconstruct(PassDesc& pd, int g, const double* zin_reduced);
+    void set_boundary(PassDesc& pd, int g, int mode, const double* zin, double* zout, size_t off);
     void launch_pass(PassDesc& pd, int prec, bool full_rows);
     struct EvPair { hipEvent_t a, b; int kernel, prec; };
     std::vector<hipEvent_t> ev_pool;

@@ -1224,12 +1224,23 @@ __global__ __launch_bounds__(256) void bndmom_kernel(const PassDesc* __restrict_
             const double* Dex = ax == 0 ? d.Dexit_x : (ax == 1 ? d.Dexit_y : d.Dexit_z);
             const int* lst = (ax == 0 ? d.exlist_x : (ax == 1 ? d.exlist_y : d.exlist_z)) + f * Nd;
             const int cnt = d.excnt[ax][f];
-            const double* bsrc = (const double*)(ax == 0 ? d.bx : (ax == 1 ? d.by : d.bz));
+            // exit-face fluxes in the pass's precision
+            const void* bsrc = ax == 0 ? d.bx : (ax == 1 ? d.by : d.bz);
             double v = 0.0;
+            if (d.prec == P64) {
+                const double* b64 = (const double*)bsrc;
 #pragma unroll 8
-            for (int t = 0; t < cnt; ++t) {
-                const int n = lst[t];
-                v += gld(Dex + (size_t)n * Nps + p) * gld(bsrc + (size_t)n * npos[ax] + pos);
+                for (int t = 0; t < cnt; ++t) {
+                    const int n = lst[t];
+                    v += gld(Dex + (size_t)n * Nps + p) * gld(b64 + (size_t)n * npos[ax] + pos);
+                }
+            } else {
+                const float* b32 = (const float*)bsrc;
+#pragma unroll 8
+                for (int t = 0; t < cnt; ++t) {
+                    const int n = lst[t];
+                    v += gld(Dex + (size_t)n * Nps + p) * (double)gld(b32 + (size_t)n * npos[ax] + pos);
+                }
             }
             double* out = ax == 0 ? d.bout_x : (ax == 1 ? d.bout_y : d.bout_z);
             const double* zin = ax == 0 ? d.bzin_x : (ax == 1 ? d.bzin_y : d.bzin_z);
@@ -1786,14 +1797,18 @@ template <int CPL, int U, int PF, int MAXT, bool DPP, bool BIN = false>
 static void scan3d_launch(int kind, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int waves, size_t lds,
                           hipStream_t st) {
     waves = waves < MAXT / 64 ? waves : MAXT / 64;
-    if constexpr (BIN) {  // FP64 passes only
-        auto fn = sweep_scan_kernel<double, CPL, U, PF, MAXT, DPP, true>;
-        static bool battr = false;
-        if (!battr) {
-            RT_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-            battr = true;
+    if constexpr (BIN) {  // FP64 and FP32 passes of one problem
+        auto f64 = sweep_scan_kernel<double, CPL, U, PF, MAXT, DPP, true>;
+        auto f32 = sweep_scan_kernel<float, CPL, U, PF, MAXT, DPP, true>;
+        static bool battr[2] = {false, false};
+        const int bi = kind == P64 ? 0 : 1;
+        if (!battr[bi]) {
+            RT_HIP(hipFuncSetAttribute(bi == 0 ? (const void*)f64 : (const void*)f32,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            battr[bi] = true;
         }
-        launch_timed(fn, dim3(ntasks), dim3(64 * waves), lds, st, d_pd, tasks);
+        if (bi == 0) launch_timed(f64, dim3(ntasks), dim3(64 * waves), lds, st, d_pd, tasks);
+        else launch_timed(f32, dim3(ntasks), dim3(64 * waves), lds, st, d_pd, tasks);
         return;
     }
     const void* fn = kind == PMIX ? (const void*)sweep_scan_mixed_kernel<CPL, U, PF, MAXT, DPP>
@@ -1832,14 +1847,18 @@ template <int CPL, bool DPP, bool BIN>
 static void line_inst(int kind, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, size_t lds, hipStream_t st) {
     // rows in flight: keeps the outstanding loads and stores of PD rows within vmcnt's 63
     constexpr int PD = CPL <= 2 ? 8 : (CPL == 4 ? 4 : 2);
-    if constexpr (BIN) {  // FP64 passes only
-        auto fn = sweep_line_kernel<double, CPL, PD, DPP, true>;
-        static bool battr = false;
-        if (!battr) {
-            RT_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-            battr = true;
+    if constexpr (BIN) {  // FP64 and FP32 passes of one problem
+        auto f64 = sweep_line_kernel<double, CPL, PD, DPP, true>;
+        auto f32 = sweep_line_kernel<float, CPL, PD, DPP, true>;
+        static bool battr[2] = {false, false};
+        const int bi = kind == P64 ? 0 : 1;
+        if (!battr[bi]) {
+            RT_HIP(hipFuncSetAttribute(bi == 0 ? (const void*)f64 : (const void*)f32,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            battr[bi] = true;
         }
-        launch_timed(fn, dim3(ntasks), dim3(64), lds, st, d_pd, tasks);
+        if (bi == 0) launch_timed(f64, dim3(ntasks), dim3(64), lds, st, d_pd, tasks);
+        else launch_timed(f32, dim3(ntasks), dim3(64), lds, st, d_pd, tasks);
         return;
     }
     const void* fn = kind == PMIX ? (const void*)sweep_line_mixed_kernel<CPL, PD, DPP>
@@ -1875,7 +1894,7 @@ void launch_sweep_scan(int prec, const PassDesc* d_pd, const SweepTask* tasks, i
                        hipStream_t st, bool bin) {
     if (ntasks <= 0) return;
     const int kind = prec == PMIX ? PMIX : (prec == P64 ? P64 : P32);
-    if (bin && kind != P64) throw std::runtime_error("boundary inflow is supported by FP64 passes only");
+    if (bin && kind == PMIX) throw std::runtime_error("boundary inflow is supported by single-problem passes only");
     if (bin) {
         switch (cpl) {
             case 1: scan_any<1, true>(kind, d_pd, tasks, ntasks, waves, lds, st); break;

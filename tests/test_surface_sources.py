@@ -176,9 +176,17 @@ def test_surface_source_gpu_parity(oracle, case):
 
 
 @pytest.mark.gpu
-def test_surface_source_needs_fp64_faithful():
-    from radiant_hip import solver as S, _lib
-    p = _prob(2)
-    with pytest.raises(_lib.RTError) as ei:
-        S.compute_flux(p, precision="mixed-fp32", state="reduced")
-    assert ei.value.code == _lib.RT_EINVAL
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_surface_source_reduced_and_mixed(oracle, case):
+    nd, faces, bc = CASES[case]
+    from radiant_hip import solver as S
+    p = _prob(nd, faces=faces, bc=bc, qvol=(case % 2 == 1))
+    ref, _ = oracle.solve(p)
+    phi, st = S.compute_flux(p, state="reduced")
+    e = rel_err(phi, ref)
+    _say("surface source reduced fp64", nd, faces, bc, e)
+    assert e < 1e-10 and all(s["converged"] for s in st)
+    mx, st = S.compute_flux(p, precision="mixed-fp32", state="reduced")
+    e = rel_err(mx, ref)
+    _say("surface source mixed fp32", nd, faces, bc, e)
+    assert e < 1e-6 and all(s["converged"] for s in st)
