@@ -86,8 +86,9 @@ typedef struct rt_problem_desc {
      * z+), NULL or the half-range moments [G][Np_source][npos_f], row-major with the position
      * fastest (npos = Ny*Nz for x faces, Nx*Nz for y faces, Nx*Ny for z faces; positions ordered
      * like the boundary moments: iy + Ny*iz, ix + Nx*iz, ix + Nx*iy).  The first Np_source
-     * half-range moments of each face carry the source.  FP64 faithful state only; callers that
-     * do not use surface sources leave these fields zero. */
+     * half-range moments of each face carry the source.  Every precision and state; lockstep
+     * batches solve such problems on their per-problem path.  Callers that do not use surface
+     * sources leave these fields zero. */
     const double *ssrc[6];
     int32_t Np_source;
     /* Continuous slowing down (SN solver type "BCSD" with the DD scheme of order 1 in energy;
