@@ -251,6 +251,31 @@ def kernel_roofline(prob, res, nact):
                 flop_per_byte=ai)
 
 
+def contraction_rooflines(prob, res, nact):
+    """Both rooflines of the M and D contractions of a solve, per precision class that ran: the
+    HBM rate of their algorithmic bytes and the MFMA rate of their algorithmic flops (work per
+    update as in kernel_roofline), from the same event-timed launches."""
+    kms, kn = np.asarray(res["kernel_ms"]), np.asarray(res["kernel_launches"])
+    upd, fl = kernel_work(prob, res, nact)
+    Nd = res["Nd"]
+    out = {}
+    for p in range(3):
+        es = 8 if p == 0 else (4 if p == 1 else 2)
+        for k in (0, 2):
+            if not kn[p][k]:
+                continue
+            t = kms[p][k] / 1e3
+            K = fl[p][k] / (2.0 * upd[p][k])
+            by = upd[p][k] * (es + (8.0 * K / Nd if k == 0 else 16.0 * K / Nd))
+            gbs, tfs = by / t / 1e9, fl[p][k] / t / 1e12
+            out[f"{KNAMES[k]}_{PNAMES[p]}"] = dict(
+                avg_launch_us=round(float(kms[p][k] / kn[p][k] * 1e3), 2), launches=int(kn[p][k]),
+                hbm_gbs=round(float(gbs), 1), hbm_frac=round(float(gbs / HBM_PEAK_GBS), 4),
+                mfma_tflops=round(float(tfs), 3), mfma_frac=round(float(tfs / MFMA_PEAK_TFLOPS[p]), 5),
+                flop_per_byte=round(float(fl[p][k] / by), 3), rows_per_update=round(float(K), 2))
+    return out
+
+
 def kernel_table(res):
     kms, kn = res["kernel_ms"], res["kernel_launches"]
     t = {}
@@ -441,6 +466,7 @@ def main():
         "kernels_headline": kernel_table(head), "kernels_single_mixed": kernel_table(single) if single else None,
         "kernels_fp64": kernel_table(r64) if r64 else None,
         "roofline": roof, "roofline_fp64": roof64,
+        "contractions_headline": contraction_rooflines(prob, head, nact),
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (head["updates_per_s"] / cpu["value"]) if cpu else None,
     }
