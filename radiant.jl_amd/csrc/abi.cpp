@@ -1,6 +1,7 @@
 // abi.cpp -- extern "C" entry points of libradiant_hip.so (declared in include/radiant_hip.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -105,6 +106,22 @@ int rt_test_contraction(int32_t which, int32_t prec, int32_t M, int32_t K, int32
     if (!A || !B || !C || M < 1 || K < 1 || N < 1 || which < 0 || which > 1 || prec < 0 || prec > 2)
         return fail(RT_EINVAL, "invalid contraction test arguments");
     return guarded([&] { rt::test_contraction(which, prec, M, K, N, A, B, C); });
+}
+
+int rt_autotune(int32_t mode) {
+    if (mode < 0 || mode > 2) return fail(RT_EINVAL, "autotune mode must be 0, 1 or 2");
+    rt::set_autotune_mode(mode);
+    return RT_OK;
+}
+
+int64_t rt_autotune_report(char* buf, int64_t cap) {
+    const std::string r = rt::autotune_report();
+    if (buf && cap > 0) {
+        const size_t n = std::min<size_t>(r.size(), (size_t)cap - 1);
+        std::memcpy(buf, r.data(), n);
+        buf[n] = 0;
+    }
+    return (int64_t)r.size();
 }
 
 int rt_problem_kernel_times(rt_problem* h, double* ms, int64_t* launches) {

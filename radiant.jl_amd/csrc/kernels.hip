@@ -16,10 +16,16 @@
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
 #include <hip/hip_bf16.h>
+#include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstdlib>
+#include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <tuple>
+#include <vector>
 
 namespace rt {
 
@@ -49,6 +55,141 @@ static void launch_timed(void (*kernel)(KArgs...), dim3 grid, dim3 block, uint32
     } else {
         hipLaunchKernelGGL(kernel, grid, block, shmem, st, static_cast<KArgs>(args)...);
     }
+}
+
+// ------------------------------------------------------------------------------------------
+// Launch-shape autotuner.  A sweep launch has variants that compute bit-identical results and
+// differ only in their schedule: the rows in flight of the line kernel (1D/2D), and for the 3D
+// scan kernel the pencils per item with their prefetch depth and the waves per workgroup.  In
+// mode 1 the first launches of each key (launch class, kernel, cells per lane, default waves,
+// inflow flag, log2 of the task count) cycle through the candidates with dispatch start/stop
+// events; once every candidate has TUNE_REPS completed timings, the one with the lowest median
+// serves the rest of the process.  Mode 0 keeps the first (default) candidate; mode 2 moves to
+// the next candidate on every launch (for tests of the variants' identity).
+// ------------------------------------------------------------------------------------------
+namespace {
+constexpr int TUNE_REPS = 3;
+struct TuneKey {
+    int kind, kernel, cpl, waves, bin, lg;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(kind, kernel, cpl, waves, bin, lg) < std::tie(o.kind, o.kernel, o.cpl, o.waves, o.bin, o.lg);
+    }
+};
+struct TunePending { int cand; hipEvent_t a, b; };
+struct TuneState {
+    int ncand = 0, best = -1, next = 0;
+    std::vector<std::vector<float>> ms;
+    std::vector<TunePending> pending;
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneState> g_tune;
+std::atomic<int> g_tune_mode{-1};
+
+int tune_mode() {
+    int m = g_tune_mode.load();
+    if (m < 0) {
+        const char* e = std::getenv("RADIANT_HIP_AUTOTUNE");
+        m = (e && (e[0] == '0' || e[0] == '2')) ? e[0] - '0' : 1;
+        g_tune_mode.store(m);
+    }
+    return m;
+}
+
+int ilog2(int n) {
+    int l = 0;
+    while ((1 << (l + 1)) <= n) ++l;
+    return l;
+}
+
+float median(std::vector<float> v) {
+    std::sort(v.begin(), v.end());
+    return v[v.size() / 2];
+}
+
+// The candidate for this launch; when the launch is to be timed, a and b receive fresh events
+// (to be handed to launch_timed through set_launch_events).
+int tune_choose(const TuneKey& key, int ncand, hipEvent_t& a, hipEvent_t& b) {
+    a = b = nullptr;
+    const int mode = tune_mode();
+    if (ncand <= 1 || mode == 0) return 0;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    TuneState& s = g_tune[key];
+    if (s.ncand == 0) { s.ncand = ncand; s.ms.assign(ncand, {}); }
+    if (mode == 2) { const int c = s.next; s.next = (s.next + 1) % ncand; return c; }
+    if (s.best >= 0) return s.best;
+    if (tl_ev_start) return 0;  // the caller times this launch itself
+    for (size_t i = 0; i < s.pending.size();) {
+        TunePending& p = s.pending[i];
+        const hipError_t q = hipEventQuery(p.b);
+        if (q == hipErrorNotReady) { ++i; continue; }
+        float ms = 0.f;
+        if (q == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) s.ms[p.cand].push_back(ms);
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+        s.pending.erase(s.pending.begin() + i);
+    }
+    bool done = true;
+    for (auto& v : s.ms) done = done && v.size() >= (size_t)TUNE_REPS;
+    if (done) {
+        int best = 0;
+        for (int c = 1; c < ncand; ++c)
+            if (median(s.ms[c]) < median(s.ms[best])) best = c;
+        s.best = best;
+        return best;
+    }
+    // the next candidate still short of samples (counting those in flight)
+    for (int t = 0; t < ncand; ++t) {
+        const int c = (s.next + t) % ncand;
+        size_t inflight = 0;
+        for (auto& p : s.pending) inflight += p.cand == c;
+        if (s.ms[c].size() + inflight < (size_t)TUNE_REPS) {
+            s.next = (c + 1) % ncand;
+            if (hipEventCreate(&a) != hipSuccess) { a = nullptr; return c; }
+            if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); a = nullptr; return c; }
+            s.pending.push_back({c, a, b});
+            return c;
+        }
+    }
+    return 0;  // every candidate has its timings in flight
+}
+}  // namespace
+
+void set_autotune_mode(int mode) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_mode.store(mode);
+    for (auto& kv : g_tune)
+        for (auto& p : kv.second.pending) {
+            (void)hipEventSynchronize(p.b);
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+    g_tune.clear();
+}
+
+std::string autotune_report() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    static const char* kinds[] = {"fp64", "fp32", "bf16", "mixed"};
+    std::string r;
+    for (auto& kv : g_tune) {
+        const TuneKey& k = kv.first;
+        const TuneState& st = kv.second;
+        char line[256];
+        std::snprintf(line, sizeof line, "%s %s cpl=%d waves=%d inflow=%d tasks~2^%d: ", kinds[k.kind & 3],
+                      k.kernel == 0 ? "line" : "scan3d", k.cpl, k.waves, k.bin, k.lg);
+        r += line;
+        if (st.best < 0) r += "tuning";
+        else {
+            std::snprintf(line, sizeof line, "variant %d of %d;", st.best, st.ncand);
+            r += line;
+            for (int c = 0; c < st.ncand; ++c) {
+                std::snprintf(line, sizeof line, " %.1f", 1e3 * (st.ms[c].empty() ? 0.f : median(st.ms[c])));
+                r += line;
+            }
+            r += " us";
+        }
+        r += "\n";
+    }
+    return r;
 }
 
 typedef double v4d __attribute__((ext_vector_type(4)));
@@ -1832,21 +1973,29 @@ template <int CPL, bool DPP, bool BIN>
 static void scan3d_inst(int kind, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int waves, size_t lds,
                         hipStream_t st) {
     constexpr int U2 = CPL >= 4 ? 1 : 2;
+    // candidates: item shapes (one unless DPP, fewer than 8 cells per lane and no inflow) times
+    // waves per workgroup (the default and half of it)
+    constexpr int NS = (CPL >= 8 || !DPP || BIN) ? 1 : 2;
+    const int NW = waves >= 2 ? 2 : 1;
+    hipEvent_t a, b;
+    int v = tune_choose({kind, 1, CPL, waves, BIN ? 1 : 0, ilog2(ntasks)}, NS * NW, a, b);
+    if (scan3d_shape() == 1 && NS == 2 && v % 2 == 0) v += 1;  // RADIANT_HIP_SCAN3D=1 forces shape 1
+    if (a) set_launch_events(a, b);
+    const int shape = v % NS, w = (v / NS) ? (waves + 1) / 2 : waves;
     if constexpr (CPL >= 8 || !DPP) {  // 8 cells per lane: 512 threads, so that the registers do not spill
-        scan3d_launch<CPL, 1, 2, 512, DPP, BIN>(kind, d_pd, tasks, ntasks, waves, lds, st);
+        scan3d_launch<CPL, 1, 2, 512, DPP, BIN>(kind, d_pd, tasks, ntasks, w, lds, st);
     } else if constexpr (BIN) {
-        scan3d_launch<CPL, 1, 2, 1024, DPP, BIN>(kind, d_pd, tasks, ntasks, waves, lds, st);
+        scan3d_launch<CPL, 1, 2, 1024, DPP, BIN>(kind, d_pd, tasks, ntasks, w, lds, st);
     } else {
         constexpr int PF = CPL >= 4 ? 2 : 4;  // 4 cells per lane: a deeper ring would spill at 1024 threads
-        if (scan3d_shape() == 1) scan3d_launch<CPL, U2, 1, 1024, DPP>(kind, d_pd, tasks, ntasks, waves, lds, st);
-        else scan3d_launch<CPL, 1, PF, 1024, DPP>(kind, d_pd, tasks, ntasks, waves, lds, st);
+        if (shape == 1) scan3d_launch<CPL, U2, 1, 1024, DPP>(kind, d_pd, tasks, ntasks, w, lds, st);
+        else scan3d_launch<CPL, 1, PF, 1024, DPP>(kind, d_pd, tasks, ntasks, w, lds, st);
     }
+    (void)shape;
 }
 
-template <int CPL, bool DPP, bool BIN>
-static void line_inst(int kind, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, size_t lds, hipStream_t st) {
-    // rows in flight: keeps the outstanding loads and stores of PD rows within vmcnt's 63
-    constexpr int PD = CPL <= 2 ? 8 : (CPL == 4 ? 4 : 2);
+template <int CPL, int PD, bool DPP, bool BIN>
+static void line_launch(int kind, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, size_t lds, hipStream_t st) {
     if constexpr (BIN) {  // FP64 and FP32 passes of one problem
         auto f64 = sweep_line_kernel<double, CPL, PD, DPP, true>;
         auto f32 = sweep_line_kernel<float, CPL, PD, DPP, true>;
@@ -1876,6 +2025,28 @@ static void line_inst(int kind, const PassDesc* d_pd, const SweepTask* tasks, in
         launch_timed(sweep_line_kernel<double, CPL, PD, DPP>, dim3(ntasks), dim3(64), lds, st, d_pd, tasks);
     else
         launch_timed(sweep_line_kernel<float, CPL, PD, DPP>, dim3(ntasks), dim3(64), lds, st, d_pd, tasks);
+}
+
+template <int CPL, bool DPP, bool BIN>
+static void line_inst(int kind, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, size_t lds, hipStream_t st) {
+    // rows in flight: at most PD0 keeps the outstanding loads and stores of PD rows within vmcnt's
+    // 63; fewer rows in flight are the autotuner's alternatives (same arithmetic)
+    constexpr int PD0 = CPL <= 2 ? 8 : (CPL == 4 ? 4 : 2);
+    constexpr int NC = CPL <= 2 ? 3 : (CPL == 4 ? 2 : 1);
+    hipEvent_t a, b;
+    const int v = tune_choose({kind, 0, CPL, 0, BIN ? 1 : 0, ilog2(ntasks)}, NC, a, b);
+    if (a) set_launch_events(a, b);
+    const int pd = PD0 >> v;
+    if constexpr (CPL <= 2) {
+        if (pd == 8) return line_launch<CPL, 8, DPP, BIN>(kind, d_pd, tasks, ntasks, lds, st);
+        if (pd == 4) return line_launch<CPL, 4, DPP, BIN>(kind, d_pd, tasks, ntasks, lds, st);
+        return line_launch<CPL, 2, DPP, BIN>(kind, d_pd, tasks, ntasks, lds, st);
+    } else if constexpr (CPL == 4) {
+        if (pd == 4) return line_launch<CPL, 4, DPP, BIN>(kind, d_pd, tasks, ntasks, lds, st);
+        return line_launch<CPL, 2, DPP, BIN>(kind, d_pd, tasks, ntasks, lds, st);
+    } else {
+        return line_launch<CPL, 2, DPP, BIN>(kind, d_pd, tasks, ntasks, lds, st);
+    }
 }
 
 template <int CPL, bool BIN>

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <string>
 
 namespace rt {
 
@@ -75,6 +76,13 @@ enum Prec { P64 = 0, P32 = 1, PBF16 = 2,
 // The next launch of a pass kernel on this host thread (mgemm, sweep, dgemm, bndmom) records start
 // and stop at its own dispatch (kernel execution interval); null, null: untimed.
 void set_launch_events(hipEvent_t start, hipEvent_t stop);
+
+// Autotuner of the sweep launch shapes (variants with bit-identical results): mode 0 default
+// variants, 1 online tuning (the default; environment RADIANT_HIP_AUTOTUNE), 2 a new variant on
+// every launch.  Setting the mode forgets every tuned key.  The report lists each key's choice
+// and the median time of every candidate.
+void set_autotune_mode(int mode);
+std::string autotune_report();
 
 // Work tiles of the contraction kernels: x = problem, y = 64-cell tile, z = row group.
 struct GemmTask { int prob, tile, rgroup, pad; };

@@ -279,5 +279,18 @@ class Server:
             pass
 
 
+def autotune(mode):
+    """rt_autotune: 0 default launch shapes, 1 online tuning (default), 2 a new variant every launch."""
+    check(lib().rt_autotune(int(mode)))
+
+
+def autotune_report():
+    """rt_autotune_report: one line per tuned sweep launch shape."""
+    n = lib().rt_autotune_report(None, 0)
+    buf = C.create_string_buffer(int(n) + 1)
+    lib().rt_autotune_report(buf, n + 1)
+    return buf.value.decode()
+
+
 def version():
     return lib().rt_version().decode()

@@ -8,7 +8,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def header_symbols():
     txt = open(os.path.join(ROOT, "include", "radiant_hip.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*\*?\s*(rt_\w+)\s*\(", txt, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int64_t|int|const char \*)\s*\*?\s*(rt_\w+)\s*\(", txt, re.M)))
 
 
 def test_header_matches_binding():
