@@ -467,6 +467,7 @@ def main():
         "kernels_fp64": kernel_table(r64) if r64 else None,
         "roofline": roof, "roofline_fp64": roof64,
         "contractions_headline": contraction_rooflines(prob, head, nact),
+        "autotune": solver.autotune_report().splitlines(),
         "cpu_baseline": cpu,
         "gpu_vs_cpu": (head["updates_per_s"] / cpu["value"]) if cpu else None,
     }

@@ -33,6 +33,7 @@ class OracleProblem(C.Structure):
         ("csd", C.c_int32), ("Sb", C.POINTER(C.c_double)), ("dE", C.POINTER(C.c_double)),
         ("solver_type", C.c_int32), ("T", C.POINTER(C.c_double)), ("siga", C.POINTER(C.c_double)),
         ("fokker_planck", C.c_char_p), ("scheme", C.c_char_p), ("scheme_order", C.c_int32),
+        ("scheme_y", C.c_char_p), ("scheme_order_y", C.c_int32), ("partial_coupling", C.c_int32),
     ]
 
 # SN.jl get_solver_type
@@ -70,6 +71,8 @@ def lib():
         L.oracle_pass.argtypes = [C.POINTER(OracleProblem), C.c_int, C.c_int, dp, dp]
         L.oracle_fokker_planck.argtypes = [C.POINTER(OracleProblem), dp, dp, C.c_int]
         L.oracle_cell_1d.argtypes = [C.c_char_p, C.c_int, C.c_double, C.c_double, C.c_double, dp, dp, dp]
+        L.oracle_cell_2d.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                     C.c_double, C.c_double, C.c_double, dp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -189,6 +192,10 @@ def make_problem(prob, nthreads=1, verbose=0, **over):
     k.scheme = (getattr(prob, "scheme", None) or "DD").encode()
     op.scheme = k.scheme
     op.scheme_order = int(getattr(prob, "scheme_order", 1) or 1)
+    k.scheme_y = (getattr(prob, "scheme_y", None) or "DD").encode()
+    op.scheme_y = k.scheme_y
+    op.scheme_order_y = int(getattr(prob, "scheme_order_y", 1) or 1)
+    op.partial_coupling = 0 if getattr(prob, "full_coupling", True) else 1
     op._keep = k
     return op
 
@@ -205,8 +212,15 @@ def fokker_planck(prob):
 
 
 def spatial_moments(prob):
-    """Spatial Legendre moments per cell (the 1D scheme order; 1 otherwise)."""
-    return max(1, int(getattr(prob, "scheme_order", 1) or 1)) if prob.ndims == 1 else 1
+    """Spatial Legendre moments per cell: the 1D scheme order; Ox*Oy (or Ox + Oy - 1 without full
+    coupling) in 2D; 1 in 3D."""
+    ox = max(1, int(getattr(prob, "scheme_order", 1) or 1))
+    if prob.ndims == 1:
+        return ox
+    if prob.ndims == 2:
+        oy = max(1, int(getattr(prob, "scheme_order_y", 1) or 1))
+        return ox * oy if getattr(prob, "full_coupling", True) else ox + oy - 1
+    return 1
 
 
 def cell_1d(scheme, K, mu, St, dx, Qn, fx_in):
@@ -216,6 +230,19 @@ def cell_1d(scheme, K, mu, St, dx, Qn, fx_in):
     fx = C.c_double(fx_in)
     _check(lib().oracle_cell_1d(scheme.encode(), K, mu, St, dx, _p(q), C.byref(fx), _p(psi)))
     return psi, fx.value
+
+
+def cell_2d(schemes, orders, full_coupling, mu, eta, St, dx, dy, Qn, fx_in, fy_in):
+    """One 2D cell: (spatial moments, outgoing x-face moments [Oy], outgoing y-face moments [Ox])."""
+    ox, oy = orders
+    nm = ox * oy if full_coupling else ox + oy - 1
+    q = np.ascontiguousarray(Qn, np.float64)
+    fx = np.ascontiguousarray(fx_in, np.float64).copy()
+    fy = np.ascontiguousarray(fy_in, np.float64).copy()
+    psi = np.zeros(nm)
+    _check(lib().oracle_cell_2d(schemes[0].encode(), schemes[1].encode(), ox, oy, int(not full_coupling), mu, eta,
+                                St, dx, dy, _p(q), _p(fx), _p(fy), _p(psi)))
+    return psi, fx, fy
 
 
 def sizes(prob):

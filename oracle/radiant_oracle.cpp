@@ -1067,6 +1067,61 @@ void flux_1d_bte(double mu, double St, double dx, const double* Qn, double& fx, 
     }
 }
 
+// sn_2D_bte.jl:27-116 (non-adaptive): the Nm spatial moments of the cell (Ox*Oy with full coupling,
+// Ox + Oy - 1 without) and the outgoing face moments fx[Oy], fy[Ox] (in: incoming).  The closure
+// weights of each axis are those of scheme_weights_1d (scheme_weights.jl fills the diagonal only).
+struct Cell2 {
+    int Ox, Oy, Nm;
+    bool fc;
+    // basis index of (ix, iy) (0-based), or -1 when the partial coupling drops it
+    int idx(int ix, int iy) const {
+        if (fc) return Ox * iy + ix;
+        if (ix > 0 && iy > 0) return -1;
+        return iy > 0 ? Ox - 1 + iy : ix;
+    }
+};
+void flux_2d_bte(double mu, double eta, double St, double dx, double dy, const double* Qn, double* fx, double* fy,
+                 const Cell2& cl, const double* wx, const double* wy, double* psi) {
+    const double sx = mu > 0 ? 1.0 : (mu < 0 ? -1.0 : 0.0), sy = eta > 0 ? 1.0 : (eta < 0 ? -1.0 : 0.0);
+    const double hx = std::fabs(mu) / dx, hy = std::fabs(eta) / dy;
+    const int Ox = cl.Ox, Oy = cl.Oy, Nm = cl.Nm;
+    auto C = [](int i) { return std::sqrt(2.0 * i - 1.0); };  // 1-based
+    auto spow = [](double s, int e) { double v = 1.0; for (int k = 0; k < e; ++k) v *= s; return v; };
+    vector<double> S((size_t)Nm * Nm, 0.0), Q(Nm, 0.0);
+    for (int ix = 1; ix <= Ox; ++ix)
+        for (int jx = 1; jx <= Ox; ++jx)
+            for (int iy = 1; iy <= Oy; ++iy)
+                for (int jy = 1; jy <= Oy; ++jy) {
+                    const int i = cl.idx(ix - 1, iy - 1), j = cl.idx(jx - 1, jy - 1);
+                    if (i < 0 || j < 0) continue;
+                    double& e = S[(size_t)i * Nm + j];
+                    if (i == j) e += St;
+                    if (iy == jy && ix >= jx + 1) e -= C(ix) * hx * sx * C(jx) * (1.0 - spow(-1.0, ix - jx));
+                    if (jy == iy) e += C(ix) * hx * spow(sx, ix - 1) * C(jx) * spow(sx, jx - 1) * wx[jx];
+                    if (ix == jx && iy >= jy + 1) e -= C(iy) * hy * sy * C(jy) * (1.0 - spow(-1.0, iy - jy));
+                    if (jx == ix) e += C(iy) * hy * spow(sy, iy - 1) * C(jy) * spow(sy, jy - 1) * wy[jy];
+                }
+    for (int jx = 1; jx <= Ox; ++jx)
+        for (int jy = 1; jy <= Oy; ++jy) {
+            const int j = cl.idx(jx - 1, jy - 1);
+            if (j < 0) continue;
+            Q[j] += Qn[j];
+            Q[j] -= C(jx) * hx * (spow(sx, jx - 1) * wx[0] - spow(-sx, jx - 1)) * fx[jy - 1];
+            Q[j] -= C(jy) * hy * (spow(sy, jy - 1) * wy[0] - spow(-sy, jy - 1)) * fy[jx - 1];
+        }
+    vector<double> x = lu_solve(S, Q, Nm);
+    for (int j = 0; j < Nm; ++j) psi[j] = x[j];
+    for (int jx = 1; jx <= Ox; ++jx)
+        for (int jy = 1; jy <= Oy; ++jy) {
+            const int j = cl.idx(jx - 1, jy - 1);
+            if (j < 0) continue;
+            if (jx == 1) fx[jy - 1] = wx[0] * fx[jy - 1];
+            if (jy == 1) fy[jx - 1] = wy[0] * fy[jx - 1];
+            fx[jy - 1] += C(jx) * spow(sx, jx - 1) * wx[jx] * x[j];
+            fy[jx - 1] += C(jy) * spow(sy, jy - 1) * wy[jy] * x[j];
+        }
+}
+
 struct Angular {
     int Nd = 0, Np = 0, Nps = 0;
     vector<double> mu, eta, xi, w;
@@ -1117,6 +1172,11 @@ struct oracle_problem {
     // "DG+"; order 0 or 1 = first order.  Orders above 1 and closures other than DD: 1D BTE only.
     const char* scheme;
     int32_t scheme_order;
+    // the y axis of 2D problems likewise, and the coupling of the high-order moments
+    // (SN.set_is_full_coupling: 0 = full, the default; 1 = Ox + Oy - 1 moments per cell)
+    const char* scheme_y;
+    int32_t scheme_order_y;
+    int32_t partial_coupling;
 };
 
 struct oracle_stats {
@@ -1144,7 +1204,10 @@ struct Solver {
     double lam0 = 0.0;          // its λ₀
     int K = 1;                  // spatial moments per cell (1D scheme order; 1 elsewhere)
     bool ho = false;            // the general 1D cell solve (order > 1 or a closure other than DD)
-    vector<double> omx;         // its ω
+    vector<double> omx, omy;    // its ω per axis
+    bool ho2 = false;           // the general 2D cell solve
+    Cell2 cl2{1, 1, 1, true};
+    int Nm1 = 1, Nm2 = 1;       // moments of the x-face and y-face fluxes
     int Ncol = 0;               // moment columns: K * Nc (column = spatial moment + K * cell)
     const double* cur_src[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // this pass's surface sources
     // continuous slowing down: incoming / outgoing energy-face angular flux [n*Nc + c] (𝚽E12 of
@@ -1235,17 +1298,32 @@ struct Solver {
                 for (int g = 0; g < G; ++g) sigt_eff[g + (size_t)G * m] -= sig_s(m, g, g, 0);
         // spatial closure (SN.get_schemes, scheme_weights.jl)
         const string scheme = p.scheme ? p.scheme : "DD";
+        const string scheme_y = p.scheme_y ? p.scheme_y : "DD";
         K = p.scheme_order > 1 ? p.scheme_order : 1;
-        if (K > 1 || scheme != "DD") {
-            if (ndims != 1 || p.csd)
-                fail("closures other than diamond difference of order 1 are supported for the 1D BTE only");
+        const int Ky = p.scheme_order_y > 1 ? p.scheme_order_y : 1;
+        if (ndims == 1 && (K > 1 || scheme != "DD")) {
+            if (p.csd) fail("closures other than diamond difference of order 1 are supported for the BTE only");
             omx = scheme_weights_1d(scheme, K);
             ho = true;
+        } else if (ndims == 2 && (K > 1 || Ky > 1 || scheme != "DD" || scheme_y != "DD")) {
+            if (p.csd) fail("closures other than diamond difference of order 1 are supported for the BTE only");
+            for (const string* sc : {&scheme, &scheme_y})
+                if (*sc != "DD" && *sc != "DG") fail("Closure relation not implemented yet.");
+            omx = scheme_weights_1d(scheme, K);
+            omy = scheme_weights_1d(scheme_y, Ky);
+            const bool fc = p.partial_coupling == 0;
+            cl2 = Cell2{K, Ky, fc ? K * Ky : K + Ky - 1, fc};
+            Nm1 = Ky;  // x faces carry the y moments (sn_flux.jl get_schemes: Nm[1])
+            Nm2 = K;
+            K = cl2.Nm;
+            ho2 = true;
+        } else if (K > 1 || Ky > 1 || scheme != "DD" || scheme_y != "DD") {
+            fail("closures other than diamond difference of order 1 are supported in 1D and 2D only");
         }
         Ncol = K * Nc;
         nphi = (size_t)A.Np * Ncol;
-        nbx = (size_t)A.Nps * 2 * Ny * Nz;
-        nby = ndims >= 2 ? (size_t)A.Nps * 2 * Nx * Nz : 0;
+        nbx = (size_t)A.Nps * Nm1 * 2 * Ny * Nz;
+        nby = ndims >= 2 ? (size_t)A.Nps * Nm2 * 2 * Nx * Nz : 0;
         nbz = ndims >= 3 ? (size_t)A.Nps * 2 * Nx * Ny : 0;
         S = nphi + nbx + nby + nbz;
     }
@@ -1324,6 +1402,63 @@ struct Solver {
         if (Sb) {
             int of = fwd ? 1 : 0;
             for (int p = 0; p < Nps; ++p) tmpx[p + (size_t)Nps * of] += Sb->D[p + (size_t)Nps * row] * fx;
+        }
+    }
+
+    // sn_sweep_2D.jl:54-200 with the general cell: columns (x) outer, rows (y) inner.  The x-face
+    // moments of every row are carried across the columns, the y-face moments along a column;
+    // face arrays are b[p + Nps*(is + Nm*(f + 2*pos))], surface sources enter moment 1 only.
+    void sweep2_ho(int n, int g, double* phi, const double* Ql, const double* bx_in, const double* by_in, double* tx,
+                   double* ty) {
+        const int Np = A.Np, Nps = A.Nps, Nd = A.Nd;
+        const double mu = A.mu[n], eta = A.eta[n];
+        const bool fx_ = mu >= 0, fy_ = eta >= 0;
+        const SurfBasis *Sx = nullptr, *Sy = nullptr;
+        int rx = -1, ry = -1;
+        if (nxm[n]) { Sx = &A.S[0]; rx = nxm[n] - 1; } else if (nxp[n]) { Sx = &A.S[1]; rx = nxp[n] - 1; }
+        if (nym[n]) { Sy = &A.S[2]; ry = nym[n] - 1; } else if (nyp[n]) { Sy = &A.S[3]; ry = nyp[n] - 1; }
+        auto inflow = [&](int face_bc, bool fwd, const SurfBasis* Sb, int row, const double* b, int Nm, size_t pos,
+                          const double* src, size_t npos, double* f) {
+            if (src && Sb)
+                for (int p = 0; p < P.Np_source; ++p) f[0] += Sb->M[row + (size_t)Sb->Ndp * p] * src[(size_t)p * npos + pos];
+            if (face_bc == 0 || !Sb) return;
+            const int sf = fwd ? (face_bc == 1 ? 0 : 1) : (face_bc == 1 ? 1 : 0);
+            for (int p = 0; p < Nps; ++p)
+                for (int is = 0; is < Nm; ++is)
+                    f[is] += Sb->M[row + (size_t)Sb->Ndp * p] * b[p + (size_t)Nps * (is + (size_t)Nm * (sf + 2 * pos))];
+        };
+        vector<double> fxm((size_t)Nm1 * Ny, 0.0), fym(Nm2), Qn(K), psi(K);
+        for (int kx = 0; kx < Nx; ++kx) {
+            const int ix = fx_ ? kx : Nx - 1 - kx;
+            std::fill(fym.begin(), fym.end(), 0.0);
+            inflow(fy_ ? P.bc[2] : P.bc[3], fy_, Sy, ry, by_in, Nm2, ix, cur_src[fy_ ? 2 : 3], (size_t)Nx, fym.data());
+            for (int ky = 0; ky < Ny; ++ky) {
+                const int iy = fy_ ? ky : Ny - 1 - ky;
+                double* fx = &fxm[(size_t)Nm1 * iy];
+                if (kx == 0)
+                    inflow(fx_ ? P.bc[0] : P.bc[1], fx_, Sx, rx, bx_in, Nm1, iy, cur_src[fx_ ? 0 : 1], (size_t)Ny, fx);
+                const int c = ix + Nx * iy;
+                for (int is = 0; is < K; ++is) {
+                    double q = 0.0;
+                    for (int p = 0; p < Np; ++p) q += A.B.M[n + (size_t)Nd * p] * Ql[p + (size_t)Np * (is + (size_t)K * c)];
+                    Qn[is] = q;
+                }
+                flux_2d_bte(mu, eta, sig_t(g, matc[c]), P.dx[ix], P.dy[iy], Qn.data(), fx, fym.data(), cl2, omx.data(),
+                            omy.data(), psi.data());
+                for (int is = 0; is < K; ++is)
+                    for (int p = 0; p < Np; ++p)
+                        phi[p + (size_t)Np * (is + (size_t)K * c)] += A.B.D[p + (size_t)Np * n] * psi[is];
+                if (kx == Nx - 1 && Sx)
+                    for (int p = 0; p < Nps; ++p)
+                        for (int is = 0; is < Nm1; ++is)
+                            tx[p + (size_t)Nps * (is + (size_t)Nm1 * ((fx_ ? 1 : 0) + 2 * iy))] +=
+                                Sx->D[p + (size_t)Nps * rx] * fx[is];
+                if (ky == Ny - 1 && Sy)
+                    for (int p = 0; p < Nps; ++p)
+                        for (int is = 0; is < Nm2; ++is)
+                            ty[p + (size_t)Nps * (is + (size_t)Nm2 * ((fy_ ? 1 : 0) + 2 * ix))] +=
+                                Sy->D[p + (size_t)Nps * ry] * fym[is];
+            }
         }
     }
 
@@ -1508,6 +1643,7 @@ struct Solver {
 
     void direction(int n, int g, double* phi, const double* Ql, const double* bin, double* tmp) {
         if (ndims == 1) sweep1(n, g, phi, Ql, bin, tmp);
+        else if (ho2) sweep2_ho(n, g, phi, Ql, bin, bin + nbx, tmp, tmp + nbx);
         else sweep23(n, g, phi, Ql, bin, bin + nbx, bin + nbx + nby, tmp, tmp + nbx, tmp + nbx + nby);
     }
 
@@ -1947,6 +2083,19 @@ int oracle_cell_1d(const char* scheme, int K, double mu, double St, double dx, c
     try {
         vector<double> om = scheme_weights_1d(scheme, K);
         flux_1d_bte(mu, St, dx, Qn, *fx, K, om.data(), psi);
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// One 2D cell of closures (DD or DG per axis) and orders (Ox, Oy), fully coupled or not: psi receives
+// the spatial moments, fx[Oy] and fy[Ox] enter as the incoming face moments and leave as the outgoing.
+int oracle_cell_2d(const char* sx, const char* sy, int Ox, int Oy, int partial, double mu, double eta, double St,
+                   double dx, double dy, const double* Qn, double* fx, double* fy, double* psi) {
+    try {
+        vector<double> wx = scheme_weights_1d(sx, Ox), wy = scheme_weights_1d(sy, Oy);
+        const bool fc = partial == 0;
+        Cell2 cl{Ox, Oy, fc ? Ox * Oy : Ox + Oy - 1, fc};
+        flux_2d_bte(mu, eta, St, dx, dy, Qn, fx, fy, cl, wx.data(), wy.data(), psi);
         return 0;
     } catch (std::exception& e) { g_err = e.what(); return -1; }
 }

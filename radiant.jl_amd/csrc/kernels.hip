@@ -76,10 +76,14 @@ struct TuneKey {
     }
 };
 struct TunePending { int cand; hipEvent_t a, b; };
+// A launch chosen for timing: its events join the key's pending list only once the launch that
+// records them has been issued (tune_commit), so no other thread can query them before.
+struct TuneTicket { TuneKey key; int cand; hipEvent_t a = nullptr, b = nullptr; };
 struct TuneState {
     int ncand = 0, best = -1, next = 0;
     std::vector<std::vector<float>> ms;
     std::vector<TunePending> pending;
+    std::vector<int> issuing;  // tickets handed out and not yet committed, per candidate
 };
 std::mutex g_tune_mu;
 std::map<TuneKey, TuneState> g_tune;
@@ -106,18 +110,9 @@ float median(std::vector<float> v) {
     return v[v.size() / 2];
 }
 
-// The candidate for this launch; when the launch is to be timed, a and b receive fresh events
-// (to be handed to launch_timed through set_launch_events).
-int tune_choose(const TuneKey& key, int ncand, hipEvent_t& a, hipEvent_t& b) {
-    a = b = nullptr;
-    const int mode = tune_mode();
-    if (ncand <= 1 || mode == 0) return 0;
-    std::lock_guard<std::mutex> lk(g_tune_mu);
-    TuneState& s = g_tune[key];
-    if (s.ncand == 0) { s.ncand = ncand; s.ms.assign(ncand, {}); }
-    if (mode == 2) { const int c = s.next; s.next = (s.next + 1) % ncand; return c; }
-    if (s.best >= 0) return s.best;
-    if (tl_ev_start) return 0;  // the caller times this launch itself
+// Completed timings of a key's issued launches move into its samples (lock held).  The queries'
+// status codes are cleared from the thread's last-error state.
+void tune_harvest(TuneState& s) {
     for (size_t i = 0; i < s.pending.size();) {
         TunePending& p = s.pending[i];
         const hipError_t q = hipEventQuery(p.b);
@@ -128,6 +123,23 @@ int tune_choose(const TuneKey& key, int ncand, hipEvent_t& a, hipEvent_t& b) {
         (void)hipEventDestroy(p.b);
         s.pending.erase(s.pending.begin() + i);
     }
+    (void)hipGetLastError();
+}
+
+// The candidate for this launch; when the launch is to be timed, t.a and t.b receive fresh events
+// (to be handed to launch_timed through set_launch_events, and to tune_commit after the launch).
+int tune_choose(const TuneKey& key, int ncand, TuneTicket& t) {
+    t.key = key;
+    t.a = t.b = nullptr;
+    const int mode = tune_mode();
+    if (ncand <= 1 || mode == 0) return t.cand = 0;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    TuneState& s = g_tune[key];
+    if (s.ncand == 0) { s.ncand = ncand; s.ms.assign(ncand, {}); s.issuing.assign(ncand, 0); }
+    if (mode == 2) { const int c = s.next; s.next = (s.next + 1) % ncand; return t.cand = c; }
+    if (!s.pending.empty()) tune_harvest(s);
+    if (s.best >= 0) return t.cand = s.best;
+    if (tl_ev_start) return t.cand = 0;  // the caller times this launch itself
     bool done = true;
     for (auto& v : s.ms) done = done && v.size() >= (size_t)TUNE_REPS;
     if (done) {
@@ -135,22 +147,45 @@ int tune_choose(const TuneKey& key, int ncand, hipEvent_t& a, hipEvent_t& b) {
         for (int c = 1; c < ncand; ++c)
             if (median(s.ms[c]) < median(s.ms[best])) best = c;
         s.best = best;
-        return best;
+        return t.cand = best;
     }
     // the next candidate still short of samples (counting those in flight)
-    for (int t = 0; t < ncand; ++t) {
-        const int c = (s.next + t) % ncand;
+    for (int k = 0; k < ncand; ++k) {
+        const int c = (s.next + k) % ncand;
         size_t inflight = 0;
         for (auto& p : s.pending) inflight += p.cand == c;
-        if (s.ms[c].size() + inflight < (size_t)TUNE_REPS) {
+        if (s.ms[c].size() + inflight + s.issuing[c] < (size_t)TUNE_REPS) {
             s.next = (c + 1) % ncand;
-            if (hipEventCreate(&a) != hipSuccess) { a = nullptr; return c; }
-            if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); a = nullptr; return c; }
-            s.pending.push_back({c, a, b});
+            t.cand = c;
+            if (hipEventCreate(&t.a) != hipSuccess) { t.a = nullptr; (void)hipGetLastError(); return c; }
+            if (hipEventCreate(&t.b) != hipSuccess) {
+                (void)hipEventDestroy(t.a);
+                t.a = t.b = nullptr;
+                (void)hipGetLastError();
+                return c;
+            }
+            s.issuing[c] += 1;
             return c;
         }
     }
-    return 0;  // every candidate has its timings in flight
+    return t.cand = 0;  // every candidate has its timings in flight
+}
+
+// After the timed launch of a ticket has been issued: its events join the pending list.
+void tune_commit(const TuneTicket& t) {
+    if (!t.a) return;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    TuneState& s = g_tune[t.key];
+    if (s.ncand == 0) {  // the key was forgotten meanwhile (set_autotune_mode): drop the timing
+        g_tune.erase(t.key);
+        (void)hipEventSynchronize(t.b);
+        (void)hipEventDestroy(t.a);
+        (void)hipEventDestroy(t.b);
+        (void)hipGetLastError();
+        return;
+    }
+    s.issuing[t.cand] -= 1;
+    s.pending.push_back({t.cand, t.a, t.b});
 }
 }  // namespace
 
@@ -1977,10 +2012,10 @@ static void scan3d_inst(int kind, const PassDesc* d_pd, const SweepTask* tasks, 
     // waves per workgroup (the default and half of it)
     constexpr int NS = (CPL >= 8 || !DPP || BIN) ? 1 : 2;
     const int NW = waves >= 2 ? 2 : 1;
-    hipEvent_t a, b;
-    int v = tune_choose({kind, 1, CPL, waves, BIN ? 1 : 0, ilog2(ntasks)}, NS * NW, a, b);
+    TuneTicket t;
+    int v = tune_choose({kind, 1, CPL, waves, BIN ? 1 : 0, ilog2(ntasks)}, NS * NW, t);
     if (scan3d_shape() == 1 && NS == 2 && v % 2 == 0) v += 1;  // RADIANT_HIP_SCAN3D=1 forces shape 1
-    if (a) set_launch_events(a, b);
+    if (t.a) set_launch_events(t.a, t.b);
     const int shape = v % NS, w = (v / NS) ? (waves + 1) / 2 : waves;
     if constexpr (CPL >= 8 || !DPP) {  // 8 cells per lane: 512 threads, so that the registers do not spill
         scan3d_launch<CPL, 1, 2, 512, DPP, BIN>(kind, d_pd, tasks, ntasks, w, lds, st);
@@ -1992,6 +2027,7 @@ static void scan3d_inst(int kind, const PassDesc* d_pd, const SweepTask* tasks, 
         else scan3d_launch<CPL, 1, PF, 1024, DPP>(kind, d_pd, tasks, ntasks, w, lds, st);
     }
     (void)shape;
+    tune_commit(t);
 }
 
 template <int CPL, int PD, bool DPP, bool BIN>
@@ -2033,20 +2069,21 @@ static void line_inst(int kind, const PassDesc* d_pd, const SweepTask* tasks, in
     // 63; fewer rows in flight are the autotuner's alternatives (same arithmetic)
     constexpr int PD0 = CPL <= 2 ? 8 : (CPL == 4 ? 4 : 2);
     constexpr int NC = CPL <= 2 ? 3 : (CPL == 4 ? 2 : 1);
-    hipEvent_t a, b;
-    const int v = tune_choose({kind, 0, CPL, 0, BIN ? 1 : 0, ilog2(ntasks)}, NC, a, b);
-    if (a) set_launch_events(a, b);
+    TuneTicket t;
+    const int v = tune_choose({kind, 0, CPL, 0, BIN ? 1 : 0, ilog2(ntasks)}, NC, t);
+    if (t.a) set_launch_events(t.a, t.b);
     const int pd = PD0 >> v;
     if constexpr (CPL <= 2) {
-        if (pd == 8) return line_launch<CPL, 8, DPP, BIN>(kind, d_pd, tasks, ntasks, lds, st);
-        if (pd == 4) return line_launch<CPL, 4, DPP, BIN>(kind, d_pd, tasks, ntasks, lds, st);
-        return line_launch<CPL, 2, DPP, BIN>(kind, d_pd, tasks, ntasks, lds, st);
+        if (pd == 8) line_launch<CPL, 8, DPP, BIN>(kind, d_pd, tasks, ntasks, lds, st);
+        else if (pd == 4) line_launch<CPL, 4, DPP, BIN>(kind, d_pd, tasks, ntasks, lds, st);
+        else line_launch<CPL, 2, DPP, BIN>(kind, d_pd, tasks, ntasks, lds, st);
     } else if constexpr (CPL == 4) {
-        if (pd == 4) return line_launch<CPL, 4, DPP, BIN>(kind, d_pd, tasks, ntasks, lds, st);
-        return line_launch<CPL, 2, DPP, BIN>(kind, d_pd, tasks, ntasks, lds, st);
+        if (pd == 4) line_launch<CPL, 4, DPP, BIN>(kind, d_pd, tasks, ntasks, lds, st);
+        else line_launch<CPL, 2, DPP, BIN>(kind, d_pd, tasks, ntasks, lds, st);
     } else {
-        return line_launch<CPL, 2, DPP, BIN>(kind, d_pd, tasks, ntasks, lds, st);
+        line_launch<CPL, 2, DPP, BIN>(kind, d_pd, tasks, ntasks, lds, st);
     }
+    tune_commit(t);
 }
 
 template <int CPL, bool BIN>

@@ -104,6 +104,11 @@ class Problem:
     # are 1D BTE only, and the flux then holds scheme_order spatial moments per cell
     scheme: str = "DD"
     scheme_order: int = 1
+    # the y axis of 2D problems ("DD" or "DG") and the coupling of the high-order moments
+    # (SN.set_is_full_coupling; without it a cell holds Ox + Oy - 1 moments instead of Ox * Oy)
+    scheme_y: str = "DD"
+    scheme_order_y: int = 1
+    full_coupling: bool = True
 
     @property
     def Nc(self):

@@ -73,6 +73,9 @@ class Desc:
         d.solver_type = stype
         K = schemes.spatial_moments(prob)
         scheme = (getattr(prob, "scheme", None) or "DD").upper()
+        if prob.ndims >= 2 and (int(getattr(prob, "scheme_order_y", 1) or 1) > 1
+                                or (getattr(prob, "scheme_y", None) or "DD").upper() != "DD"):
+            raise ValueError("the GPU path takes closures other than diamond difference of order 1 in 1D only")
         if K > 1 or scheme != "DD":  # closure weights of the x axis (scheme_weights.jl)
             k["omega_x"] = np.ascontiguousarray(schemes.weights_1d(scheme, K))
             d.order_x, d.omega_x = K, ptr(k["omega_x"])

@@ -58,3 +58,22 @@ def test_autotune_modes_agree_bitwise():
         assert "line" in rep and "scan3d" in rep and "variant" in rep
     finally:
         S.autotune(1)
+
+
+def test_autotune_concurrent_sub_batches():
+    """Eight problems run as four sub-batches on their own host threads while the tuner times
+    their first launches: the results equal those of the default shapes."""
+    from radiant_hip import problems as P, solver as S
+    probs = [P.small_problem(2, dims=(40 + 8 * i, 6), G=2, L=2, seed=320 + i) for i in range(8)]
+    try:
+        S.autotune(0)
+        ref, _, _ = S.solve_batch(probs, precision="mixed-fp32")
+        S.autotune(1)
+        for _ in range(2):
+            got, _, _ = S.solve_batch(probs, precision="mixed-fp32")
+            for a, b in zip(ref, got):
+                assert np.array_equal(a, b)
+        _say(S.autotune_report())
+    finally:
+        S.autotune(1)
+
