@@ -118,6 +118,14 @@ typedef struct rt_problem_desc {
      * [Np, Nm5 * Nx]; qvol keeps one value per cell (spatial moment 1, Source.jl:135). */
     int32_t order_x;
     const double *omega_x;
+    /* The y axis of 2D problems likewise (order_y at most 3, omega_y[𝒪y + 1]; with order_x at most 3),
+     * and the coupling of the high-order moments (SN.set_is_full_coupling; 0 = full, the default:
+     * Nm5 = 𝒪x*𝒪y moments per cell; 1 = 𝒪x + 𝒪y - 1).  The x-face boundary moments then carry 𝒪y
+     * moments per position and the y faces 𝒪x (sn_one_speed.jl:92-106: 𝚽x12[Nps, Nm1, 2, Ny], ...),
+     * and the faithful state and rt_problem_pass follow that layout. */
+    int32_t order_y;
+    const double *omega_y;
+    int32_t partial_coupling;
 } rt_problem_desc;
 
 /* Per-group result of the in-group solve (the values sn_one_speed.jl:189-193 prints). */
@@ -234,7 +242,7 @@ const char *rt_version(void);
  * launches of each shape time every candidate and the fastest serves the rest of the process;
  * 0: default candidates only; 2: a new candidate on every launch (identity tests).  Setting the
  * mode forgets what was tuned.  The report writes one line per launch shape (its choice and the
- * median time of each candidate), NUL-terminated and truncated to cap bytes; returns the full
+ * fastest time of each candidate), NUL-terminated and truncated to cap bytes; returns the full
  * length. */
 int rt_autotune(int32_t mode);
 int64_t rt_autotune_report(char *buf, int64_t cap);

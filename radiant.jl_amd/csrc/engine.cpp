@@ -39,12 +39,15 @@ static void upload(T* d, const std::vector<T>& h, hipStream_t st) {
 
 // SN solver type (SN.jl get_solver_type): 0 in the description means BTE, or BCSD with csd.
 static int solver_type_of(const rt_problem_desc& d) { return d.solver_type ? d.solver_type : (d.csd ? 3 : 1); }
-// Spatial moments per cell of the general 1D closure (SN.set_scheme): 0 for diamond difference of
-// order 1, which the order-1 kernels take.
+// Spatial moments per cell of a general closure (SN.set_scheme, set_is_full_coupling): 0 for
+// diamond difference of order 1 on every axis, which the order-1 kernels take; in 1D the x order,
+// in 2D Kx*Ky with full coupling and Kx + Ky - 1 without.
 static int ho_order(const rt_problem_desc& d) {
-    const int K = d.order_x > 1 ? d.order_x : 1;
-    if (K == 1 && (!d.omega_x || (d.omega_x[0] == -1.0 && d.omega_x[1] == 2.0))) return 0;
-    return K;
+    const int Kx = d.order_x > 1 ? d.order_x : 1, Ky = d.ndims >= 2 && d.order_y > 1 ? d.order_y : 1;
+    auto dd1 = [](const double* w) { return !w || (w[0] == -1.0 && w[1] == 2.0); };
+    if (Kx == 1 && Ky == 1 && dd1(d.omega_x) && (d.ndims < 2 || dd1(d.omega_y))) return 0;
+    if (d.ndims == 1) return Kx;
+    return d.partial_coupling ? Kx + Ky - 1 : Kx * Ky;
 }
 
 static bool has_fp(const rt_problem_desc& d) {
@@ -110,11 +113,13 @@ static void validate(const rt_problem_desc& d) {
     }
     if (surf && (d.Np_source < 1 || d.Np_source > d.Nps))
         bad("Np_source must lie in 1..Nps when a surface source is given");
-    if (d.order_x < 0 || d.order_x > HO_KMAX)
-        bad("spatial closure order must lie in 1.." + std::to_string(HO_KMAX));
+    const int kmax = d.ndims == 1 ? HO_KMAX : HO2_KMAX;
+    if (d.order_x < 0 || d.order_x > kmax || d.order_y < 0 || d.order_y > kmax)
+        bad("spatial closure orders must lie in 1.." + std::to_string(kmax) + " in " + std::to_string(d.ndims) + "D");
     if (d.order_x > 1 && !d.omega_x) bad("closures of order above 1 need their weights omega_x");
-    if (ho_order(d) && (d.ndims != 1 || d.csd))
-        bad("closures other than diamond difference of order 1 are supported for the 1D BTE only");
+    if (d.ndims >= 2 && d.order_y > 1 && !d.omega_y) bad("closures of order above 1 need their weights omega_y");
+    if (ho_order(d) && (d.ndims == 3 || d.csd))
+        bad("closures other than diamond difference of order 1 are supported for the 1D and 2D BTE only");
     if (d.solver_type < 0 || d.solver_type > 6) bad("Unknown type of solver.");
     const int stype = solver_type_of(d);
     if ((stype >= 2) != (d.csd != 0))
@@ -142,6 +147,13 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
     validate(d);
     ndims = d.ndims; Nx = d.Nx; Ny = d.Ny; Nz = d.Nz; Ncell = Nx * Ny * Nz;
     ho_K = ho_order(d);
+    if (ho_K && ndims == 2) {
+        ho_Kx = d.order_x > 1 ? d.order_x : 1;
+        ho_Ky = d.order_y > 1 ? d.order_y : 1;
+        ho_fc = d.partial_coupling == 0;
+        fm[0] = ho_Ky;  // x faces carry the y moments, y faces the x moments (SN.get_schemes: Nm)
+        fm[1] = ho_Kx;
+    }
     Nc = Ncell * (ho_K ? ho_K : 1);  // moment columns: spatial moment + K * cell
     G = d.G; Nmat = d.Nmat; Nd = d.Nd; Np = d.Np; Nps = d.Nps;
     accel = d.accel; restart = d.restart; depth = d.depth; maxit = d.maxit; tol = d.tol;
@@ -152,8 +164,9 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
     Nact = (int)act.size();
     if (Nact > CONTRACT_KMAX)
         throw Error(RT_EINVAL, "more than " + std::to_string(CONTRACT_KMAX) + " moments carry a scattering source");
-    npos[0] = (size_t)Ny * Nz;
-    npos[1] = ndims >= 2 ? (size_t)Nx * Nz : 0;
+    // boundary positions times face moments per axis
+    npos[0] = (size_t)Ny * Nz * fm[0];
+    npos[1] = ndims >= 2 ? (size_t)Nx * Nz * fm[1] : 0;
     npos[2] = ndims >= 3 ? (size_t)Nx * Ny : 0;
     nb = (size_t)2 * Nps * (npos[0] + npos[1] + npos[2]);
     for (int f = 0; f < 2 * ndims; ++f) {
@@ -175,10 +188,18 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
         std::vector<int> mat(Nc);  // per moment column
         for (int c = 0; c < Nc; ++c) mat[c] = (int)(d.mat[c / Kc] - 1);
         d_mat = dalloc<int>(Nc); upload(d_mat, mat, st);
-        if (ho_K) {
-            std::vector<double> om(d.omega_x ? d.omega_x : nullptr, d.omega_x ? d.omega_x + ho_K + 1 : nullptr);
+        if (ho_K) {  // closure weights per axis (diamond difference of order 1 when absent)
+            const int kx = ndims == 2 ? ho_Kx : ho_K;
+            std::vector<double> om(d.omega_x ? d.omega_x : nullptr, d.omega_x ? d.omega_x + kx + 1 : nullptr);
             if (om.empty()) om = {-1.0, 2.0};
             d_omx = dalloc<double>(om.size()); upload(d_omx, om, st);
+            if (ndims == 2) {
+                std::vector<double> oy(d.omega_y ? d.omega_y : nullptr, d.omega_y ? d.omega_y + ho_Ky + 1 : nullptr);
+                if (oy.empty()) oy = {-1.0, 2.0};
+                d_omy = dalloc<double>(oy.size()); upload(d_omy, oy, st);
+                if (sweep_2d_ho_lds_bytes(P64, Nx, Ny, ho_Kx, ho_Ky) > 160 * 1024)
+                    throw Error(RT_EINVAL, "the face moments of the 2D closure exceed the 160 KiB LDS of a workgroup");
+            }
         }
         auto up = [&](const double* src, int n) {
             double* p = dalloc<double>(n);
@@ -341,9 +362,9 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
                     d_sface[ax] = dalloc<int>(Nd); upload(d_sface[ax], sface, st);
                     d_Mentry[ax] = dalloc<double>(Me.size()); upload(d_Mentry[ax], Me, st);
                     d_insrc[ax] = dalloc<int>(Nd); upload(d_insrc[ax], src, st);
-                    d_bin[ax] = dalloc<double>((size_t)Nd * npf[ax]);
-                    k_fill(d_bin[ax], 0.0, (size_t)Nd * npf[ax], st);  // axes beyond ndims stay zero
-                    bin_total += ax < ndims ? (int64_t)Nd * (int64_t)npf[ax] : 0;
+                    d_bin[ax] = dalloc<double>((size_t)Nd * npf[ax] * fm[ax]);
+                    k_fill(d_bin[ax], 0.0, (size_t)Nd * npf[ax] * fm[ax], st);  // axes beyond ndims stay zero
+                    bin_total += ax < ndims ? (int64_t)Nd * (int64_t)npf[ax] * fm[ax] : 0;
                 }
             }
         }
@@ -442,7 +463,8 @@ int64_t Problem::device_bytes(const rt_problem_desc& d) {
     b += (int64_t)Nact_est * Nc * 8;                            // qlout
     b += (int64_t)d.Nd * Nc * 8;                                // psi
     b += (int64_t)3 * d.Nd * d.Np * 12;                         // operators
-    const int64_t nposs = (int64_t)d.Ny * d.Nz + (d.ndims >= 2 ? (int64_t)d.Nx * d.Nz : 0) +
+    const int64_t fmax = ho_order(d) && d.ndims == 2 ? std::max(std::max(d.order_x, d.order_y), 1) : 1;  // face moments
+    const int64_t nposs = fmax * ((int64_t)d.Ny * d.Nz + (d.ndims >= 2 ? (int64_t)d.Nx * d.Nz : 0)) +
                           (d.ndims >= 3 ? (int64_t)d.Nx * d.Ny : 0);
     b += (int64_t)d.Nd * nposs * 8 + 2LL * d.Nps * nposs * 8;
     const bool faithful = d.precision == RT_PREC_FP64 && d.state_mode == RT_STATE_FAITHFUL;
@@ -499,6 +521,8 @@ void Problem::fill_geometry(PassDesc& pd, int g) {
     if (fp) { pd.fpT = d_fpT + (size_t)g * Nmat; pd.fpM = d_fpM; }
     pd.ho_K = ho_K;
     pd.ho_om = d_omx;
+    pd.ho_Kx = ho_Kx; pd.ho_Ky = ho_Ky; pd.ho_fc = ho_fc ? 1 : 0; pd.ho_omy = d_omy;
+    pd.fm_x = fm[0]; pd.fm_y = fm[1];
 }
 
 hipEvent_t Problem::next_event() {
@@ -548,7 +572,9 @@ void Problem::launch_pass(PassDesc& pd, int prec, bool full_rows) {
     if (pd.psiE_in) launch_csd_source(d_pd, (long)Nd * Nc, st);  // energy inflow (FP64 passes)
     timed(1, [&] {
         const int sp = prec == PBF16 ? P32 : prec;
-        if (ho_K)
+        if (ho_K && ndims == 2)
+            launch_sweep_2d_ho(sp, d_pd, Nd, ho_Kx, ho_Ky, ho_fc, sweep_2d_ho_lds_bytes(sp, Nx, Ny, ho_Kx, ho_Ky), st);
+        else if (ho_K)
             launch_sweep_1d_ho(sp, d_pd, Nd, ho_K, st);
         else if (sweep_kind == 0 && scan_cpl > 0 && sweep_scan_lds_bytes(sp, ndims, Nx, Ny, Nz, Nmat) <= 160 * 1024)
             launch_sweep_scan(sp, d_pd, d_sw, Nd, scan_cpl, scan_waves, sweep_scan_lds_bytes(sp, ndims, Nx, Ny, Nz, Nmat),

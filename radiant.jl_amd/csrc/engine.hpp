@@ -75,7 +75,10 @@ public:
     // ---- pieces used by the batched and served drivers ----
     int ndims, Nx, Ny, Nz, Nc, G, Nmat, Nd, Np, Nps, Nact;
     int Ncell = 0;   // cells; Nc counts moment columns (Ncell times the spatial moments per cell)
-    int ho_K = 0;    // spatial moments of the general 1D closure (0: diamond difference, order 1)
+    int ho_K = 0;    // spatial moments per cell of a general closure (0: diamond difference, order 1)
+    int ho_Kx = 1, ho_Ky = 1;  // 2D closure orders
+    bool ho_fc = true;         // full coupling of the 2D moments
+    int fm[3] = {1, 1, 1};     // moments of the face fluxes per axis (Ky, Kx for a 2D closure)
     int accel, restart, depth, maxit, precision, state_mode, verbose;
     double tol;
     hipStream_t st;
@@ -122,7 +125,7 @@ private:
     // material, and the source qlout + T ℳ Φ the contraction reads
     bool fp = false;
     double *d_fpM = nullptr, *d_fpT = nullptr, *d_fpq = nullptr;
-    double* d_omx = nullptr;  // closure weights of the general 1D cell
+    double *d_omx = nullptr, *d_omy = nullptr;  // closure weights of the general cells per axis
     double* d_Mentry[3] = {nullptr, nullptr, nullptr};
     int* d_insrc[3] = {nullptr, nullptr, nullptr};
     double* d_bin[3] = {nullptr, nullptr, nullptr};

@@ -63,12 +63,12 @@ static void launch_timed(void (*kernel)(KArgs...), dim3 grid, dim3 block, uint32
 // scan kernel the pencils per item with their prefetch depth and the waves per workgroup.  In
 // mode 1 the first launches of each key (launch class, kernel, cells per lane, default waves,
 // inflow flag, log2 of the task count) cycle through the candidates with dispatch start/stop
-// events; once every candidate has TUNE_REPS completed timings, the one with the lowest median
-// serves the rest of the process.  Mode 0 keeps the first (default) candidate; mode 2 moves to
+// events; once every candidate has TUNE_REPS completed timings, the one with the fastest timing
+// serves the rest of the process (the fastest, as other streams' work can only slow a launch).  Mode 0 keeps the first (default) candidate; mode 2 moves to
 // the next candidate on every launch (for tests of the variants' identity).
 // ------------------------------------------------------------------------------------------
 namespace {
-constexpr int TUNE_REPS = 3;
+constexpr int TUNE_REPS = 5;
 struct TuneKey {
     int kind, kernel, cpl, waves, bin, lg;
     bool operator<(const TuneKey& o) const {
@@ -105,10 +105,8 @@ int ilog2(int n) {
     return l;
 }
 
-float median(std::vector<float> v) {
-    std::sort(v.begin(), v.end());
-    return v[v.size() / 2];
-}
+// the fastest of a candidate's timings: launches of concurrent streams only ever slow one down
+float best_of(const std::vector<float>& v) { return v.empty() ? 0.f : *std::min_element(v.begin(), v.end()); }
 
 // Completed timings of a key's issued launches move into its samples (lock held).  The queries'
 // status codes are cleared from the thread's last-error state.
@@ -145,7 +143,7 @@ int tune_choose(const TuneKey& key, int ncand, TuneTicket& t) {
     if (done) {
         int best = 0;
         for (int c = 1; c < ncand; ++c)
-            if (median(s.ms[c]) < median(s.ms[best])) best = c;
+            if (best_of(s.ms[c]) < best_of(s.ms[best])) best = c;
         s.best = best;
         return t.cand = best;
     }
@@ -217,7 +215,7 @@ std::string autotune_report() {
             std::snprintf(line, sizeof line, "variant %d of %d;", st.best, st.ncand);
             r += line;
             for (int c = 0; c < st.ncand; ++c) {
-                std::snprintf(line, sizeof line, " %.1f", 1e3 * (st.ms[c].empty() ? 0.f : median(st.ms[c])));
+                std::snprintf(line, sizeof line, " %.1f", 1e3 * best_of(st.ms[c]));
                 r += line;
             }
             r += " us";
@@ -1387,16 +1385,19 @@ __global__ __launch_bounds__(256) void bndmom_kernel(const PassDesc* __restrict_
         const PassDesc& d = pds[pi];
         if (!d.bout_x) continue;
         const int Nps = d.Nps, Nd = d.Nd;
-        const int npos[3] = {d.Ny * d.Nz, d.Nx * d.Nz, d.Nx * d.Ny};
+        // positions times face moments per axis (general 2D closures; 1 otherwise)
+        const int fm[3] = {d.fm_x > 1 ? d.fm_x : 1, d.fm_y > 1 ? d.fm_y : 1, 1};
+        const int npos[3] = {d.Ny * d.Nz * fm[0], d.Nx * d.Nz * fm[1], d.Nx * d.Ny};
         const size_t tot = (size_t)2 * Nps * (npos[0] + (d.ndims >= 2 ? npos[1] : 0) + (d.ndims >= 3 ? npos[2] : 0));
         for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
             size_t j = i;
             int ax = 0;
             while (j >= (size_t)2 * Nps * npos[ax]) { j -= (size_t)2 * Nps * npos[ax]; ++ax; }
-            // j = p + Nps*(f + 2*pos)
+            // j = p + Nps*(m + fm*(f + 2*pos)); the exit fluxes are [Nd][pos][m]
             const int p = (int)(j % Nps);
-            const int f = (int)((j / Nps) % 2);
-            const size_t pos = j / ((size_t)2 * Nps);
+            const int m = (int)((j / Nps) % fm[ax]);
+            const int f = (int)((j / ((size_t)Nps * fm[ax])) % 2);
+            const size_t pos = (j / ((size_t)2 * Nps * fm[ax])) * fm[ax] + m;
             const double* Dex = ax == 0 ? d.Dexit_x : (ax == 1 ? d.Dexit_y : d.Dexit_z);
             const int* lst = (ax == 0 ? d.exlist_x : (ax == 1 ? d.exlist_y : d.exlist_z)) + f * Nd;
             const int cnt = d.excnt[ax][f];
@@ -1436,15 +1437,19 @@ __global__ __launch_bounds__(256) void bndmom_kernel(const PassDesc* __restrict_
 __global__ __launch_bounds__(256) void bndin_kernel(const PassDesc* __restrict__ pds) {
     const PassDesc& d = pds[0];
     const int Nd = d.Nd, Nps = d.Nps;
+    // positions times face moments per axis (the moments of a general 2D closure; 1 otherwise)
+    const int fm[3] = {d.fm_x > 1 ? d.fm_x : 1, d.fm_y > 1 ? d.fm_y : 1, 1};
     const size_t np[3] = {(size_t)d.Ny * d.Nz, (size_t)d.Nx * d.Nz, (size_t)d.Nx * d.Ny};
     size_t tot = 0;
-    for (int ax = 0; ax < d.ndims; ++ax) tot += (size_t)Nd * np[ax];
+    for (int ax = 0; ax < d.ndims; ++ax) tot += (size_t)Nd * np[ax] * fm[ax];
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
         size_t j = i;
         int ax = 0;
-        while (j >= (size_t)Nd * np[ax]) { j -= (size_t)Nd * np[ax]; ++ax; }
-        const int n = (int)(j / np[ax]);
-        const size_t pos = j - (size_t)n * np[ax];
+        while (j >= (size_t)Nd * np[ax] * fm[ax]) { j -= (size_t)Nd * np[ax] * fm[ax]; ++ax; }
+        const int n = (int)(j / (np[ax] * fm[ax]));
+        const size_t pm = j - (size_t)n * np[ax] * fm[ax];
+        const size_t pos = pm / fm[ax];
+        const int m = (int)(pm - pos * fm[ax]);
         const int* insrc = ax == 0 ? d.insrc_x : (ax == 1 ? d.insrc_y : d.insrc_z);
         const double* Me = ax == 0 ? d.Mentry_x : (ax == 1 ? d.Mentry_y : d.Mentry_z);
         const double* bs = ax == 0 ? d.bsin_x : (ax == 1 ? d.bsin_y : d.bsin_z);
@@ -1453,12 +1458,13 @@ __global__ __launch_bounds__(256) void bndin_kernel(const PassDesc* __restrict__
         const double* ss = ax == 0 ? d.ssrc_x : (ax == 1 ? d.ssrc_y : d.ssrc_z);
         const int* sf = ax == 0 ? d.sface_x : (ax == 1 ? d.sface_y : d.sface_z);
         double v = 0.0;
-        if (ss && sf[n] >= 0) {  // surface source of the entry face
+        if (ss && sf[n] >= 0 && m == 0) {  // surface source of the entry face (first face moment)
             const double* q = ss + (size_t)sf[n] * d.ssrc_fs[ax] + pos;
             for (int p = 0; p < Nps; ++p) v += Me[(size_t)n * Nps + p] * q[(size_t)p * np[ax]];
         }
         if (src >= 0 && bs)
-            for (int p = 0; p < Nps; ++p) v += Me[(size_t)n * Nps + p] * bs[p + (size_t)Nps * (src + 2 * pos)];
+            for (int p = 0; p < Nps; ++p)
+                v += Me[(size_t)n * Nps + p] * bs[p + (size_t)Nps * (m + (size_t)fm[ax] * (src + 2 * pos))];
         out[j] = v;
     }
 }
@@ -1633,6 +1639,159 @@ __global__ __launch_bounds__(64) void sweep_1d_ho_kernel(const PassDesc* __restr
         f = fnext;
     }
     if (lane == 0 && d.bx) ((T*)d.bx)[n] = f;  // exit face flux (one position in 1D)
+}
+
+// ------------------------------------------------------------------------------------------
+// 2D sweep with a general closure of orders KX x KY (sn_sweep_2D.jl:54-200 with flux_2D_BTE,
+// sn_2D_bte.jl:27-116): one workgroup per direction walks the anti-diagonals of the cell grid in
+// sweep order; the cells of a diagonal are independent, one per thread.  A cell reads the x-face
+// moments of its row (FX[iy][KY]) and the y-face moments of its column (FY[ix][KX]) from LDS,
+// solves its NM x NM system (NM = KX*KY with full coupling, KX + KY - 1 without; basis (ix, iy)
+// at KX*iy + ix, or ix for iy = 0 and KX - 1 + iy for ix = 0)
+//   S_ij = St d_ij - [iy=jy, ix>jx] C_ix hx sx C_jx (1 - (-1)^(ix-jx)) + [iy=jy] C_ix hx sx^ix C_jx sx^jx wx_(jx+1)
+//          - [ix=jx, iy>jy] C_iy hy sy C_jy (1 - (-1)^(iy-jy)) + [ix=jx] C_iy hy sy^iy C_jy sy^jy wy_(jy+1)
+//   Q_j = Qn_j - C_jx hx (sx^jx wx_0 - (-sx)^jx) fx_jy - C_jy hy (sy^jy wy_0 - (-sy)^jy) fy_jx
+// (0-based orders, C_k = sqrt(2k+1)), and writes back its outgoing face moments
+//   fx_jy <- wx_0 fx_jy + sum_jx C_jx sx^jx wx_(jx+1) psi_(jx,jy),  fy_jx likewise.
+// Face inflows come from bin_x / bin_y ([Nd][pos][moment]); the exit faces leave in bx / by.
+// ------------------------------------------------------------------------------------------
+template <int KX, int KY, bool FC>
+__device__ __forceinline__ int ho2_idx(int ix, int iy) {
+    if constexpr (FC) return KX * iy + ix;
+    return iy > 0 ? KX - 1 + iy : ix;
+}
+
+template <typename T, int KX, int KY, bool FC>
+__global__ __launch_bounds__(256) void sweep_2d_ho_kernel(const PassDesc* __restrict__ pds) {
+    constexpr int NM = FC ? KX * KY : KX + KY - 1;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const PassDesc& d = pds[0];
+    const int n = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    const int Nx = d.Nx, Ny = d.Ny, Nc = d.Nc;
+    T* FX = (T*)smem;                  // [Ny][KY]
+    T* FY = FX + (size_t)Ny * KY;      // [Nx][KX]
+    const double mu = d.mu[n], eta = d.eta[n];
+    const bool fxs = mu >= 0, fys = eta >= 0;
+    const T sx = mu > 0 ? T(1) : (mu < 0 ? T(-1) : T(0)), sy = eta > 0 ? T(1) : (eta < 0 ? T(-1) : T(0));
+    T wx[KX + 1], wy[KY + 1], Cx[KX], Cy[KY], px[KX], nx[KX], py[KY], ny[KY];
+#pragma unroll
+    for (int k = 0; k <= KX; ++k) wx[k] = (T)d.ho_om[k];
+#pragma unroll
+    for (int k = 0; k <= KY; ++k) wy[k] = (T)d.ho_omy[k];
+#pragma unroll
+    for (int k = 0; k < KX; ++k) {
+        Cx[k] = (T)sqrt(2.0 * k + 1.0);
+        px[k] = k == 0 ? T(1) : px[k - 1] * sx;
+        nx[k] = k == 0 ? T(1) : -nx[k - 1] * sx;
+    }
+#pragma unroll
+    for (int k = 0; k < KY; ++k) {
+        Cy[k] = (T)sqrt(2.0 * k + 1.0);
+        py[k] = k == 0 ? T(1) : py[k - 1] * sy;
+        ny[k] = k == 0 ? T(1) : -ny[k - 1] * sy;
+    }
+    T* __restrict__ psi = (T*)d.psi + (size_t)n * Nc;
+    for (int i = tid; i < Ny * KY; i += nt) FX[i] = d.bin_x ? (T)d.bin_x[(size_t)n * Ny * KY + i] : T(0);
+    for (int i = tid; i < Nx * KX; i += nt) FY[i] = d.bin_y ? (T)d.bin_y[(size_t)n * Nx * KX + i] : T(0);
+    __syncthreads();
+    for (int dg = 0; dg < Nx + Ny - 1; ++dg) {
+        const int k0 = dg - (Ny - 1) > 0 ? dg - (Ny - 1) : 0, k1 = dg < Nx - 1 ? dg : Nx - 1;
+        for (int kx = k0 + tid; kx <= k1; kx += nt) {
+            const int ky = dg - kx;
+            const int ix = fxs ? kx : Nx - 1 - kx, iy = fys ? ky : Ny - 1 - ky;
+            const size_t c = (size_t)ix + (size_t)Nx * iy;
+            const T hx = (T)(fabs(mu) / d.dx[ix]), hy = (T)(fabs(eta) / d.dy[iy]);
+            const T St = (T)d.sigt[d.mat[c * NM]];
+            T* fx = FX + (size_t)iy * KY;
+            T* fy = FY + (size_t)ix * KX;
+            T S[NM][NM], q[NM];
+#pragma unroll
+            for (int i = 0; i < NM; ++i)
+#pragma unroll
+                for (int j = 0; j < NM; ++j) S[i][j] = i == j ? St : T(0);
+#pragma unroll
+            for (int ax = 0; ax < KX; ++ax)
+#pragma unroll
+                for (int ay = 0; ay < KY; ++ay) {
+                    if (!FC && ax > 0 && ay > 0) continue;
+                    const int i = ho2_idx<KX, KY, FC>(ax, ay);
+#pragma unroll
+                    for (int bx = 0; bx < KX; ++bx)
+#pragma unroll
+                        for (int by = 0; by < KY; ++by) {
+                            if (!FC && bx > 0 && by > 0) continue;
+                            const int j = ho2_idx<KX, KY, FC>(bx, by);
+                            if (ay == by) {
+                                if (ax > bx && ((ax - bx) & 1)) S[i][j] -= T(2) * Cx[ax] * hx * sx * Cx[bx];
+                                S[i][j] += Cx[ax] * hx * px[ax] * Cx[bx] * px[bx] * wx[bx + 1];
+                            }
+                            if (ax == bx) {
+                                if (ay > by && ((ay - by) & 1)) S[i][j] -= T(2) * Cy[ay] * hy * sy * Cy[by];
+                                S[i][j] += Cy[ay] * hy * py[ay] * Cy[by] * py[by] * wy[by + 1];
+                            }
+                        }
+                    q[i] = psi[(size_t)i + NM * c] - Cx[ax] * hx * (px[ax] * wx[0] - nx[ax]) * fx[ay] -
+                           Cy[ay] * hy * (py[ay] * wy[0] - ny[ay]) * fy[ax];
+                }
+            // Gaussian elimination with partial pivoting (row swaps as selects)
+#pragma unroll
+            for (int p = 0; p < NM; ++p) {
+#pragma unroll
+                for (int i = p + 1; i < NM; ++i) {
+                    const bool sw = fabs(S[i][p]) > fabs(S[p][p]);
+#pragma unroll
+                    for (int j = 0; j < NM; ++j) {
+                        const T t = S[p][j];
+                        S[p][j] = sw ? S[i][j] : t;
+                        S[i][j] = sw ? t : S[i][j];
+                    }
+                    const T tq = q[p];
+                    q[p] = sw ? q[i] : tq;
+                    q[i] = sw ? tq : q[i];
+                }
+#pragma unroll
+                for (int i = p + 1; i < NM; ++i) {
+                    const T f = S[i][p] / S[p][p];
+#pragma unroll
+                    for (int j = p + 1; j < NM; ++j) S[i][j] -= f * S[p][j];
+                    q[i] -= f * q[p];
+                }
+            }
+#pragma unroll
+            for (int i = NM - 1; i >= 0; --i) {
+                T v = q[i];
+#pragma unroll
+                for (int j = i + 1; j < NM; ++j) v -= S[i][j] * q[j];
+                q[i] = v / S[i][i];
+            }
+#pragma unroll
+            for (int i = 0; i < NM; ++i) psi[(size_t)i + NM * c] = q[i];
+            // outgoing face moments
+            T ox[KY], oy[KX];
+#pragma unroll
+            for (int ay = 0; ay < KY; ++ay) ox[ay] = wx[0] * fx[ay];
+#pragma unroll
+            for (int ax = 0; ax < KX; ++ax) oy[ax] = wy[0] * fy[ax];
+#pragma unroll
+            for (int ax = 0; ax < KX; ++ax)
+#pragma unroll
+                for (int ay = 0; ay < KY; ++ay) {
+                    if (!FC && ax > 0 && ay > 0) continue;
+                    const T v = q[ho2_idx<KX, KY, FC>(ax, ay)];
+                    ox[ay] += Cx[ax] * px[ax] * wx[ax + 1] * v;
+                    oy[ax] += Cy[ay] * py[ay] * wy[ay + 1] * v;
+                }
+#pragma unroll
+            for (int ay = 0; ay < KY; ++ay) fx[ay] = ox[ay];
+#pragma unroll
+            for (int ax = 0; ax < KX; ++ax) fy[ax] = oy[ax];
+        }
+        __syncthreads();
+    }
+    if (d.bx)
+        for (int i = tid; i < Ny * KY; i += nt) ((T*)d.bx)[(size_t)n * Ny * KY + i] = FX[i];
+    if (d.by)
+        for (int i = tid; i < Nx * KX; i += nt) ((T*)d.by)[(size_t)n * Nx * KX + i] = FY[i];
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2139,6 +2298,47 @@ static void ho_launch(const PassDesc* d_pd, int Nd, int K, hipStream_t st) {
         default: throw std::runtime_error("1D closure order out of range");
     }
 }
+size_t sweep_2d_ho_lds_bytes(int prec, int Nx, int Ny, int Kx, int Ky) {
+    return (prec == P64 ? 8 : 4) * ((size_t)Ny * Ky + (size_t)Nx * Kx);
+}
+
+template <typename T, int KX, int KY, bool FC>
+static void ho2_launch(const PassDesc* d_pd, int Nd, size_t lds, hipStream_t st) {
+    auto fn = sweep_2d_ho_kernel<T, KX, KY, FC>;
+    static bool attr = false;
+    if (!attr) {
+        RT_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr = true;
+    }
+    hipLaunchKernelGGL(fn, dim3(Nd), dim3(256), lds, st, d_pd);
+}
+
+template <typename T>
+static void ho2_dispatch(const PassDesc* d_pd, int Nd, int Kx, int Ky, bool fc, size_t lds, hipStream_t st) {
+    // partial coupling differs from full coupling only when both orders exceed 1
+    const bool full = fc || Kx == 1 || Ky == 1;
+    switch (Kx * 10 + Ky) {
+        case 11: return ho2_launch<T, 1, 1, true>(d_pd, Nd, lds, st);
+        case 12: return ho2_launch<T, 1, 2, true>(d_pd, Nd, lds, st);
+        case 13: return ho2_launch<T, 1, 3, true>(d_pd, Nd, lds, st);
+        case 21: return ho2_launch<T, 2, 1, true>(d_pd, Nd, lds, st);
+        case 31: return ho2_launch<T, 3, 1, true>(d_pd, Nd, lds, st);
+        case 22: return full ? ho2_launch<T, 2, 2, true>(d_pd, Nd, lds, st) : ho2_launch<T, 2, 2, false>(d_pd, Nd, lds, st);
+        case 23: return full ? ho2_launch<T, 2, 3, true>(d_pd, Nd, lds, st) : ho2_launch<T, 2, 3, false>(d_pd, Nd, lds, st);
+        case 32: return full ? ho2_launch<T, 3, 2, true>(d_pd, Nd, lds, st) : ho2_launch<T, 3, 2, false>(d_pd, Nd, lds, st);
+        case 33: return full ? ho2_launch<T, 3, 3, true>(d_pd, Nd, lds, st) : ho2_launch<T, 3, 3, false>(d_pd, Nd, lds, st);
+        default: throw std::runtime_error("2D closure orders out of range");
+    }
+}
+
+void launch_sweep_2d_ho(int prec, const PassDesc* d_pd, int Nd, int Kx, int Ky, bool fc, size_t lds, hipStream_t st) {
+    if (Nd <= 0) return;
+    if (lds > 160 * 1024) throw std::runtime_error("2D closure face moments exceed the LDS");
+    if (prec == P64) ho2_dispatch<double>(d_pd, Nd, Kx, Ky, fc, lds, st);
+    else ho2_dispatch<float>(d_pd, Nd, Kx, Ky, fc, lds, st);
+    RT_HIP(hipGetLastError());
+}
+
 void launch_sweep_1d_ho(int prec, const PassDesc* d_pd, int Nd, int K, hipStream_t st) {
     if (Nd <= 0) return;
     if (prec == P64) ho_launch<double>(d_pd, Nd, K, st);

@@ -68,6 +68,13 @@ struct PassDesc {
     // diamond-difference kernels of order 1); moment columns are then spatial moment + ho_K * cell
     int ho_K;
     const double* ho_om;
+    // 2D cells of orders ho_Kx x ho_Ky (ho_K = Kx*Ky with full coupling, Kx + Ky - 1 without) and the
+    // y-axis closure weights ho_omy[ho_Ky + 1]; the x faces then carry fm_x = Ky moments per position
+    // and the y faces fm_y = Kx (face arrays [Nd][npos][fm], boundary moments p + Nps*(m + fm*(f +
+    // 2*pos))).  fm 0 stands for 1.
+    int ho_Kx, ho_Ky, ho_fc;
+    const double* ho_omy;
+    int fm_x, fm_y;
 };
 
 enum Prec { P64 = 0, P32 = 1, PBF16 = 2,
@@ -80,7 +87,7 @@ void set_launch_events(hipEvent_t start, hipEvent_t stop);
 // Autotuner of the sweep launch shapes (variants with bit-identical results): mode 0 default
 // variants, 1 online tuning (the default; environment RADIANT_HIP_AUTOTUNE), 2 a new variant on
 // every launch.  Setting the mode forgets every tuned key.  The report lists each key's choice
-// and the median time of every candidate.
+// and the fastest time of every candidate.
 void set_autotune_mode(int mode);
 std::string autotune_report();
 
@@ -125,6 +132,11 @@ void launch_fp_source(const PassDesc* d_pd, long n, hipStream_t st);
 // problem, one wave per direction
 constexpr int HO_KMAX = 4;
 void launch_sweep_1d_ho(int prec, const PassDesc* d_pd, int Nd, int K, hipStream_t st);
+// 2D sweep of a general closure of orders Kx, Ky (1..HO2_KMAX each), one problem, one workgroup per
+// direction walking the anti-diagonals; lds = face moments of every row and column
+constexpr int HO2_KMAX = 3;
+size_t sweep_2d_ho_lds_bytes(int prec, int Nx, int Ny, int Kx, int Ky);
+void launch_sweep_2d_ho(int prec, const PassDesc* d_pd, int Nd, int Kx, int Ky, bool fc, size_t lds, hipStream_t st);
 void launch_dgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st);
 void launch_bndmom(const PassDesc* d_pd, int nprob, int max_items, hipStream_t st);
 

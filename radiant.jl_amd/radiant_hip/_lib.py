@@ -40,6 +40,7 @@ class ProblemDesc(C.Structure):
         ("csd", C.c_int32), ("Sb", dp), ("dE", dp),
         ("solver_type", C.c_int32), ("T", dp), ("Mfp", dp),
         ("order_x", C.c_int32), ("omega_x", dp),
+        ("order_y", C.c_int32), ("omega_y", dp), ("partial_coupling", C.c_int32),
     ]
 
 

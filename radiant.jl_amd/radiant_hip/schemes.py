@@ -1,5 +1,6 @@
-"""Spatial closure weights of the 1D cell: scheme_weights.jl:25-90 (Ndims = 1, no slowing down),
-the part of Radiant's set-up that produces ω for ``flux_1D_BTE`` (sn_1D_bte.jl:27-64).
+"""Spatial closure weights of the cells: scheme_weights.jl:25-140 (no slowing down), the part of
+Radiant's set-up that produces ω for ``flux_1D_BTE`` and ``flux_2D_BTE`` (sn_1D_bte.jl:27-64,
+sn_2D_bte.jl:27-116; in 2D the weights of each axis fill the diagonal of ω and DD / DG apply).
 
 For an order K (spatial Legendre moments per cell) the closure relates the outgoing face flux to
 the incoming one and the cell moments: f_out = ω[0] f_in + sum_j C_j sx^(j-1) ω[j] psi_j with
@@ -36,5 +37,12 @@ def weights_1d(scheme, K):
 
 
 def spatial_moments(prob):
-    """Spatial moments per cell of a problem (its 1D scheme order; 1 otherwise)."""
-    return max(1, int(getattr(prob, "scheme_order", 1) or 1)) if prob.ndims == 1 else 1
+    """Spatial moments per cell of a problem: its 1D scheme order; in 2D Ox*Oy with full coupling
+    and Ox + Oy - 1 without (SN.get_schemes: Nm[5]); 1 in 3D."""
+    ox = max(1, int(getattr(prob, "scheme_order", 1) or 1))
+    if prob.ndims == 1:
+        return ox
+    if prob.ndims == 2:
+        oy = max(1, int(getattr(prob, "scheme_order_y", 1) or 1))
+        return ox * oy if getattr(prob, "full_coupling", True) else ox + oy - 1
+    return 1

@@ -73,10 +73,20 @@ class Desc:
         d.solver_type = stype
         K = schemes.spatial_moments(prob)
         scheme = (getattr(prob, "scheme", None) or "DD").upper()
-        if prob.ndims >= 2 and (int(getattr(prob, "scheme_order_y", 1) or 1) > 1
-                                or (getattr(prob, "scheme_y", None) or "DD").upper() != "DD"):
-            raise ValueError("the GPU path takes closures other than diamond difference of order 1 in 1D only")
-        if K > 1 or scheme != "DD":  # closure weights of the x axis (scheme_weights.jl)
+        ox = max(1, int(getattr(prob, "scheme_order", 1) or 1))
+        if prob.ndims == 2:
+            scheme_y = (getattr(prob, "scheme_y", None) or "DD").upper()
+            oy = max(1, int(getattr(prob, "scheme_order_y", 1) or 1))
+            if ox > 1 or oy > 1 or scheme != "DD" or scheme_y != "DD":
+                for sc in (scheme, scheme_y):
+                    if sc not in ("DD", "DG"):
+                        raise ValueError("Closure relation not implemented yet.")
+                k["omega_x"] = np.ascontiguousarray(schemes.weights_1d(scheme, ox))
+                k["omega_y"] = np.ascontiguousarray(schemes.weights_1d(scheme_y, oy))
+                d.order_x, d.omega_x = ox, ptr(k["omega_x"])
+                d.order_y, d.omega_y = oy, ptr(k["omega_y"])
+                d.partial_coupling = 0 if getattr(prob, "full_coupling", True) else 1
+        elif K > 1 or scheme != "DD":  # closure weights of the x axis (scheme_weights.jl)
             k["omega_x"] = np.ascontiguousarray(schemes.weights_1d(scheme, K))
             d.order_x, d.omega_x = K, ptr(k["omega_x"])
         if Mfp is not None:  # T [G, Nmat], ℳ [Np, Np], both column-major
