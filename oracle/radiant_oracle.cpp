@@ -11,8 +11,9 @@
 // multiply-adds (-ffp-contract=off) so that each product is rounded as in Julia.
 //
 // Scope: Cartesian 1D/2D/3D, solver types BTE, BFP, BCSD, FP, CSD and BFP-EF (slowing down by
-// the DD scheme of order 1 in energy), diamond-difference order 1 on every axis
-// (`scheme_weights.jl:47-51` with order 1 gives w = [-1, 2]), "standard", "galerkin-d" and
+// the DD scheme of order 1 in energy), diamond difference of order 1 on every axis
+// (`scheme_weights.jl:47-51` with order 1 gives w = [-1, 2]) and, for the BTE in 1D and 2D, the
+// DD / DG closures of higher orders (`sn_1D_bte.jl`, `sn_2D_bte.jl`), "standard", "galerkin-d" and
 // "galerkin-m" angular operators, void / reflective / periodic boundaries, volume sources,
 // accelerations none / livolant / anderson / gmres / bicgstab.
 //
