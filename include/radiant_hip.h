@@ -235,10 +235,10 @@ const char *rt_last_error(void);
 /* Library/device identification string (e.g. "radiant_hip 0.1 gfx950 ..."). */
 const char *rt_version(void);
 
-/* Launch-shape autotuner of the sweep kernels (no counterpart in the reference; a backend tuning
- * hook).  The candidates of a launch compute bit-identical results and differ only in their
- * schedule (rows in flight of the 1D/2D line kernel; pencils per item and waves per workgroup of
- * the 3D scan kernel).  mode 1 (the default, or environment RADIANT_HIP_AUTOTUNE): the first
+/* Launch-shape autotuner of the sweep and contraction kernels (no counterpart in the reference; a
+ * backend tuning hook).  The candidates of a launch compute bit-identical results and differ only
+ * in their schedule (rows in flight of the 1D/2D line kernel; pencils per item and waves per
+ * workgroup of the 3D scan kernel; k steps in flight of the M and D contractions).  mode 1 (the default, or environment RADIANT_HIP_AUTOTUNE): the first
  * launches of each shape time every candidate and the fastest serves the rest of the process;
  * 0: default candidates only; 2: a new candidate on every launch (identity tests).  Setting the
  * mode forgets what was tuned.  The report writes one line per launch shape (its choice and the

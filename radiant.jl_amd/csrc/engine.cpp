@@ -386,8 +386,11 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
         auto gemm_tasks = [&](int rows, int& ntask) {
             std::vector<GemmTask> t;
             const int tiles = ceil_div(Nc, GEMM_CELLS), rg = ceil_div(rows, GEMM_ROWS);
-            for (int r = 0; r < rg; ++r)
-                for (int i = 0; i < tiles; ++i) t.push_back({0, i, r, 0});
+            for (int i = 0; i < tiles * rg; ++i) {
+                int tile, r;
+                gemm_task_order(i, tiles, rg, tile, r);
+                t.push_back({0, tile, r, 0});
+            }
             ntask = (int)t.size();
             GemmTask* p = dalloc<GemmTask>(t.size());
             upload(p, t, st);

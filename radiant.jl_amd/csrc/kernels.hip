@@ -58,9 +58,10 @@ static void launch_timed(void (*kernel)(KArgs...), dim3 grid, dim3 block, uint32
 }
 
 // ------------------------------------------------------------------------------------------
-// Launch-shape autotuner.  A sweep launch has variants that compute bit-identical results and
-// differ only in their schedule: the rows in flight of the line kernel (1D/2D), and for the 3D
-// scan kernel the pencils per item with their prefetch depth and the waves per workgroup.  In
+// Launch-shape autotuner.  A sweep or contraction launch has variants that compute bit-identical
+// results and differ only in their schedule: the rows in flight of the line kernel (1D/2D), for
+// the 3D scan kernel the pencils per item with their prefetch depth and the waves per workgroup,
+// and the pipeline depth (k steps in flight) of the M and D contractions.  In
 // mode 1 the first launches of each key (launch class, kernel, cells per lane, default waves,
 // inflow flag, log2 of the task count) cycle through the candidates with dispatch start/stop
 // events; once every candidate has TUNE_REPS completed timings, the one with the fastest timing
@@ -207,8 +208,9 @@ std::string autotune_report() {
         const TuneKey& k = kv.first;
         const TuneState& st = kv.second;
         char line[256];
+        static const char* knames[] = {"line", "scan3d", "mgemm", "dgemm"};
         std::snprintf(line, sizeof line, "%s %s cpl=%d waves=%d inflow=%d tasks~2^%d: ", kinds[k.kind & 3],
-                      k.kernel == 0 ? "line" : "scan3d", k.cpl, k.waves, k.bin, k.lg);
+                      knames[k.kernel & 3], k.cpl, k.waves, k.bin, k.lg);
         r += line;
         if (st.best < 0) r += "tuning";
         else {
@@ -1944,21 +1946,42 @@ static inline int grid_for(size_t n, int block = 256, int cap = 2048) {
     return (int)g;
 }
 
+template <bool MG, int NS>
+static void contract_launch(int prec, const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st) {
+    if (prec == PMIX) launch_timed(contract_mixed_kernel<GEMM_RT, GEMM_CT, MG, NS>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    else if (prec == P64) launch_timed(contract_kernel<double, GEMM_RT, GEMM_CT, MG, NS>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    else launch_timed(contract_kernel<float, GEMM_RT, GEMM_CT, MG, NS>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+}
+
+// Pipeline depth (k steps in flight) of the contractions: the default first, then the autotuner's
+// alternative (same accumulation order, so the same bits); BF16 operands keep their one shape.
 void launch_mgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st) {
     if (ntasks <= 0) return;
-    if (prec == PMIX) launch_timed(contract_mixed_kernel<GEMM_RT, GEMM_CT, true, 2>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
-    else if (prec == P64) launch_timed(contract_kernel<double, GEMM_RT, GEMM_CT, true, 2>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
-    else if (prec == P32) launch_timed(contract_kernel<float, GEMM_RT, GEMM_CT, true, 2>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
-    else launch_timed(contract_bf16_kernel<GEMM_RT, GEMM_CT, true>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    if (prec == PBF16) {
+        launch_timed(contract_bf16_kernel<GEMM_RT, GEMM_CT, true>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    } else {
+        TuneTicket t;
+        const int v = tune_choose({prec, 2, 0, 0, 0, ilog2(ntasks)}, 2, t);
+        if (t.a) set_launch_events(t.a, t.b);
+        if (v == 0) contract_launch<true, 2>(prec, d_pd, tasks, ntasks, st);
+        else contract_launch<true, 3>(prec, d_pd, tasks, ntasks, st);
+        tune_commit(t);
+    }
     RT_HIP(hipGetLastError());
 }
 
 void launch_dgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st) {
     if (ntasks <= 0) return;
-    if (prec == PMIX) launch_timed(contract_mixed_kernel<GEMM_RT, GEMM_CT, false, 4>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
-    else if (prec == P64) launch_timed(contract_kernel<double, GEMM_RT, GEMM_CT, false, 4>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
-    else if (prec == P32) launch_timed(contract_kernel<float, GEMM_RT, GEMM_CT, false, 4>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
-    else launch_timed(contract_bf16_kernel<GEMM_RT, GEMM_CT, false>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    if (prec == PBF16) {
+        launch_timed(contract_bf16_kernel<GEMM_RT, GEMM_CT, false>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    } else {
+        TuneTicket t;
+        const int v = tune_choose({prec, 3, 0, 0, 0, ilog2(ntasks)}, 2, t);
+        if (t.a) set_launch_events(t.a, t.b);
+        if (v == 0) contract_launch<false, 4>(prec, d_pd, tasks, ntasks, st);
+        else contract_launch<false, 2>(prec, d_pd, tasks, ntasks, st);
+        tune_commit(t);
+    }
     RT_HIP(hipGetLastError());
 }
 
@@ -2469,7 +2492,11 @@ __global__ void expand_gemm_kernel(const int* __restrict__ meta, int n, GemmTask
     const int e = blockIdx.x;
     if (e >= n) return;
     const int prob = meta[4 * e], ntile = meta[4 * e + 1], nrg = meta[4 * e + 2], off = meta[4 * e + 3];
-    for (int i = threadIdx.x; i < ntile * nrg; i += blockDim.x) out[off + i] = GemmTask{prob, i % ntile, i / ntile, 0};
+    for (int i = threadIdx.x; i < ntile * nrg; i += blockDim.x) {
+        int tile, rg;
+        gemm_task_order(i, ntile, nrg, tile, rg);
+        out[off + i] = GemmTask{prob, tile, rg, 0};
+    }
 }
 
 __global__ void expand_sweep_kernel(const int* __restrict__ meta, int n, SweepTask* __restrict__ out) {

@@ -84,7 +84,7 @@ enum Prec { P64 = 0, P32 = 1, PBF16 = 2,
 // and stop at its own dispatch (kernel execution interval); null, null: untimed.
 void set_launch_events(hipEvent_t start, hipEvent_t stop);
 
-// Autotuner of the sweep launch shapes (variants with bit-identical results): mode 0 default
+// Autotuner of the sweep and contraction launch shapes (variants with bit-identical results): mode 0 default
 // variants, 1 online tuning (the default; environment RADIANT_HIP_AUTOTUNE), 2 a new variant on
 // every launch.  Setting the mode forgets every tuned key.  The report lists each key's choice
 // and the fastest time of every candidate.
@@ -139,6 +139,24 @@ size_t sweep_2d_ho_lds_bytes(int prec, int Nx, int Ny, int Kx, int Ky);
 void launch_sweep_2d_ho(int prec, const PassDesc* d_pd, int Nd, int Kx, int Ky, bool fc, size_t lds, hipStream_t st);
 void launch_dgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st);
 void launch_bndmom(const PassDesc* d_pd, int nprob, int max_items, hipStream_t st);
+
+// Order of the work tiles of one contraction: task i -> (cell tile, row group).  The row groups of
+// a cell tile sit XCD_WAYS task indices apart, within a group of XCD_WAYS consecutive tiles, so that
+// the workgroups reading the same cells' operands run close in time on the same XCD (workgroups are
+// dealt to the XCDs round-robin by index) and share its L2.
+constexpr int XCD_WAYS = 8;
+__host__ __device__ inline void gemm_task_order(int i, int ntile, int nrg, int& tile, int& rg) {
+    const int full = (ntile / XCD_WAYS) * XCD_WAYS;
+    if (i < full * nrg) {
+        const int grp = i / (XCD_WAYS * nrg), k = i - grp * XCD_WAYS * nrg;
+        rg = k / XCD_WAYS;
+        tile = grp * XCD_WAYS + (k - rg * XCD_WAYS);
+    } else {
+        const int j = i - full * nrg, w = ntile - full;
+        rg = j / w;
+        tile = full + (j - rg * w);
+    }
+}
 
 // Row groups of the contraction kernels (rows per work tile).
 constexpr int GEMM_RT = 4;                      // 16-row MFMA tiles per wave
