@@ -126,6 +126,11 @@ typedef struct rt_problem_desc {
     int32_t order_y;
     const double *omega_y;
     int32_t partial_coupling;
+    /* The z axis of 3D problems likewise (orders at most 2 per axis in 3D; Nm5 = 𝒪x*𝒪y*𝒪z moments,
+     * or 𝒪x + 𝒪y + 𝒪z - 2 without full coupling; a face carries the moments over the other two
+     * axes). */
+    int32_t order_z;
+    const double *omega_z;
 } rt_problem_desc;
 
 /* Per-group result of the in-group solve (the values sn_one_speed.jl:189-193 prints). */

@@ -34,6 +34,7 @@ class OracleProblem(C.Structure):
         ("solver_type", C.c_int32), ("T", C.POINTER(C.c_double)), ("siga", C.POINTER(C.c_double)),
         ("fokker_planck", C.c_char_p), ("scheme", C.c_char_p), ("scheme_order", C.c_int32),
         ("scheme_y", C.c_char_p), ("scheme_order_y", C.c_int32), ("partial_coupling", C.c_int32),
+        ("scheme_z", C.c_char_p), ("scheme_order_z", C.c_int32),
     ]
 
 # SN.jl get_solver_type
@@ -71,6 +72,8 @@ def lib():
         L.oracle_pass.argtypes = [C.POINTER(OracleProblem), C.c_int, C.c_int, dp, dp]
         L.oracle_fokker_planck.argtypes = [C.POINTER(OracleProblem), dp, dp, C.c_int]
         L.oracle_cell_1d.argtypes = [C.c_char_p, C.c_int, C.c_double, C.c_double, C.c_double, dp, dp, dp]
+        L.oracle_cell_3d.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp,
+                                     C.c_double, dp, dp, dp, dp, dp]
         L.oracle_cell_2d.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                      C.c_double, C.c_double, C.c_double, dp, dp, dp, dp]
         _lib = L
@@ -196,6 +199,9 @@ def make_problem(prob, nthreads=1, verbose=0, **over):
     op.scheme_y = k.scheme_y
     op.scheme_order_y = int(getattr(prob, "scheme_order_y", 1) or 1)
     op.partial_coupling = 0 if getattr(prob, "full_coupling", True) else 1
+    k.scheme_z = (getattr(prob, "scheme_z", None) or "DD").encode()
+    op.scheme_z = k.scheme_z
+    op.scheme_order_z = int(getattr(prob, "scheme_order_z", 1) or 1)
     op._keep = k
     return op
 
@@ -212,15 +218,17 @@ def fokker_planck(prob):
 
 
 def spatial_moments(prob):
-    """Spatial Legendre moments per cell: the 1D scheme order; Ox*Oy (or Ox + Oy - 1 without full
-    coupling) in 2D; 1 in 3D."""
+    """Spatial Legendre moments per cell: the 1D scheme order; Ox*Oy (Ox + Oy - 1 without full
+    coupling) in 2D; Ox*Oy*Oz (Ox + Oy + Oz - 2) in 3D."""
     ox = max(1, int(getattr(prob, "scheme_order", 1) or 1))
     if prob.ndims == 1:
         return ox
+    oy = max(1, int(getattr(prob, "scheme_order_y", 1) or 1))
+    fc = getattr(prob, "full_coupling", True)
     if prob.ndims == 2:
-        oy = max(1, int(getattr(prob, "scheme_order_y", 1) or 1))
-        return ox * oy if getattr(prob, "full_coupling", True) else ox + oy - 1
-    return 1
+        return ox * oy if fc else ox + oy - 1
+    oz = max(1, int(getattr(prob, "scheme_order_z", 1) or 1))
+    return ox * oy * oz if fc else ox + oy + oz - 2
 
 
 def cell_1d(scheme, K, mu, St, dx, Qn, fx_in):
@@ -243,6 +251,18 @@ def cell_2d(schemes, orders, full_coupling, mu, eta, St, dx, dy, Qn, fx_in, fy_i
     _check(lib().oracle_cell_2d(schemes[0].encode(), schemes[1].encode(), ox, oy, int(not full_coupling), mu, eta,
                                 St, dx, dy, _p(q), _p(fx), _p(fy), _p(psi)))
     return psi, fx, fy
+
+
+def cell_3d(schemes, orders, full_coupling, om, h, St, Qn, fx_in, fy_in, fz_in):
+    """One 3D cell: (spatial moments, outgoing x-, y- and z-face moments)."""
+    ox, oy, oz = orders
+    nm = ox * oy * oz if full_coupling else ox + oy + oz - 2
+    arr = lambda v: np.ascontiguousarray(v, np.float64).copy()
+    q, o, hh, fx, fy, fz = arr(Qn), arr(om), arr(h), arr(fx_in), arr(fy_in), arr(fz_in)
+    psi = np.zeros(nm)
+    _check(lib().oracle_cell_3d(schemes[0].encode(), schemes[1].encode(), schemes[2].encode(), ox, oy, oz,
+                                int(not full_coupling), _p(o), _p(hh), St, _p(q), _p(fx), _p(fy), _p(fz), _p(psi)))
+    return psi, fx, fy, fz
 
 
 def sizes(prob):

@@ -1123,6 +1123,90 @@ void flux_2d_bte(double mu, double eta, double St, double dx, double dy, const d
         }
 }
 
+// sn_3D_bte.jl:34-152 (non-adaptive): Nm = Ox*Oy*Oz moments with full coupling, Ox + Oy + Oz - 2
+// without; the x, y and z faces carry the moments over (y, z), (x, z) and (x, y).
+struct Cell3 {
+    int O[3];
+    bool fc;
+    int Nm, F[3];  // cell moments, face moments per axis
+    Cell3(int ox, int oy, int oz, bool full) : O{ox, oy, oz}, fc(full) {
+        Nm = fc ? ox * oy * oz : ox + oy + oz - 2;
+        F[0] = fc ? oy * oz : oy + oz - 1;
+        F[1] = fc ? ox * oz : ox + oz - 1;
+        F[2] = fc ? ox * oy : ox + oy - 1;
+    }
+    bool valid(int a, int b, int c) const { return fc || (a > 0) + (b > 0) + (c > 0) <= 1; }
+    int idx(int a, int b, int c) const {
+        if (fc) return O[0] * O[1] * c + O[0] * b + a;
+        return a + b + c + (b > 0 ? O[0] - 1 : 0) + (c > 0 ? O[0] - 1 + O[1] - 1 : 0);
+    }
+    int fx(int b, int c) const { return fc ? O[1] * c + b : b + c + (c > 0 ? O[1] - 1 : 0); }
+    int fy(int a, int c) const { return fc ? O[0] * c + a : a + c + (c > 0 ? O[0] - 1 : 0); }
+    int fz(int a, int b) const { return fc ? O[0] * b + a : a + b + (b > 0 ? O[0] - 1 : 0); }
+};
+void flux_3d_bte(const double* om, const double* h, double St, const double* Qn, double* fx, double* fy, double* fz,
+                 const Cell3& cl, const double* const* w, double* psi) {
+    double s[3], hh[3];
+    for (int a = 0; a < 3; ++a) {
+        s[a] = om[a] > 0 ? 1.0 : (om[a] < 0 ? -1.0 : 0.0);
+        hh[a] = std::fabs(om[a]) / h[a];
+    }
+    auto C = [](int i) { return std::sqrt(2.0 * i + 1.0); };  // 0-based
+    auto spow = [](double v, int e) { double r = 1.0; for (int k = 0; k < e; ++k) r *= v; return r; };
+    const int Nm = cl.Nm;
+    vector<double> S((size_t)Nm * Nm, 0.0), Q(Nm, 0.0);
+    for (int ax = 0; ax < cl.O[0]; ++ax)
+        for (int ay = 0; ay < cl.O[1]; ++ay)
+            for (int az = 0; az < cl.O[2]; ++az) {
+                if (!cl.valid(ax, ay, az)) continue;
+                const int i = cl.idx(ax, ay, az);
+                const int ia[3] = {ax, ay, az};
+                for (int bx = 0; bx < cl.O[0]; ++bx)
+                    for (int by = 0; by < cl.O[1]; ++by)
+                        for (int bz = 0; bz < cl.O[2]; ++bz) {
+                            if (!cl.valid(bx, by, bz)) continue;
+                            const int j = cl.idx(bx, by, bz);
+                            const int jb[3] = {bx, by, bz};
+                            double& e = S[(size_t)i * Nm + j];
+                            if (i == j) e += St;
+                            for (int d = 0; d < 3; ++d) {
+                                const int o1 = (d + 1) % 3, o2 = (d + 2) % 3;
+                                if (ia[o1] != jb[o1] || ia[o2] != jb[o2]) continue;
+                                const int u = ia[d], v = jb[d];
+                                if (u >= v + 1) e -= C(u) * hh[d] * s[d] * C(v) * (1.0 - spow(-1.0, u - v));
+                                e += C(u) * hh[d] * spow(s[d], u) * C(v) * spow(s[d], v) * w[d][v + 1];
+                            }
+                        }
+            }
+    for (int bx = 0; bx < cl.O[0]; ++bx)
+        for (int by = 0; by < cl.O[1]; ++by)
+            for (int bz = 0; bz < cl.O[2]; ++bz) {
+                if (!cl.valid(bx, by, bz)) continue;
+                const int j = cl.idx(bx, by, bz);
+                Q[j] = Qn[j];
+                Q[j] -= C(bx) * hh[0] * (spow(s[0], bx) * w[0][0] - spow(-s[0], bx)) * fx[cl.fx(by, bz)];
+                Q[j] -= C(by) * hh[1] * (spow(s[1], by) * w[1][0] - spow(-s[1], by)) * fy[cl.fy(bx, bz)];
+                Q[j] -= C(bz) * hh[2] * (spow(s[2], bz) * w[2][0] - spow(-s[2], bz)) * fz[cl.fz(bx, by)];
+            }
+    vector<double> x = lu_solve(S, Q, Nm);
+    for (int j = 0; j < Nm; ++j) psi[j] = x[j];
+    for (int bx = 0; bx < cl.O[0]; ++bx)
+        for (int by = 0; by < cl.O[1]; ++by)
+            for (int bz = 0; bz < cl.O[2]; ++bz) {
+                if (!cl.valid(bx, by, bz)) continue;
+                const int j = cl.idx(bx, by, bz);
+                double& Fx = fx[cl.fx(by, bz)];
+                double& Fy = fy[cl.fy(bx, bz)];
+                double& Fz = fz[cl.fz(bx, by)];
+                if (bx == 0) Fx = w[0][0] * Fx;
+                if (by == 0) Fy = w[1][0] * Fy;
+                if (bz == 0) Fz = w[2][0] * Fz;
+                Fx += C(bx) * spow(s[0], bx) * w[0][bx + 1] * x[j];
+                Fy += C(by) * spow(s[1], by) * w[1][by + 1] * x[j];
+                Fz += C(bz) * spow(s[2], bz) * w[2][bz + 1] * x[j];
+            }
+}
+
 struct Angular {
     int Nd = 0, Np = 0, Nps = 0;
     vector<double> mu, eta, xi, w;
@@ -1178,6 +1262,8 @@ struct oracle_problem {
     const char* scheme_y;
     int32_t scheme_order_y;
     int32_t partial_coupling;
+    const char* scheme_z;       // the z axis of 3D problems
+    int32_t scheme_order_z;
 };
 
 struct oracle_stats {
@@ -1208,7 +1294,10 @@ struct Solver {
     vector<double> omx, omy;    // its ω per axis
     bool ho2 = false;           // the general 2D cell solve
     Cell2 cl2{1, 1, 1, true};
-    int Nm1 = 1, Nm2 = 1;       // moments of the x-face and y-face fluxes
+    bool ho3 = false;           // the general 3D cell solve
+    Cell3 cl3{1, 1, 1, true};
+    vector<double> omz;
+    int Nm1 = 1, Nm2 = 1, Nm3 = 1;  // moments of the x-, y- and z-face fluxes
     int Ncol = 0;               // moment columns: K * Nc (column = spatial moment + K * cell)
     const double* cur_src[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // this pass's surface sources
     // continuous slowing down: incoming / outgoing energy-face angular flux [n*Nc + c] (𝚽E12 of
@@ -1318,14 +1407,27 @@ struct Solver {
             Nm2 = K;
             K = cl2.Nm;
             ho2 = true;
-        } else if (K > 1 || Ky > 1 || scheme != "DD" || scheme_y != "DD") {
-            fail("closures other than diamond difference of order 1 are supported in 1D and 2D only");
+        } else if (ndims == 3) {
+            const string scheme_z = p.scheme_z ? p.scheme_z : "DD";
+            const int Kz = p.scheme_order_z > 1 ? p.scheme_order_z : 1;
+            if (K > 1 || Ky > 1 || Kz > 1 || scheme != "DD" || scheme_y != "DD" || scheme_z != "DD") {
+                if (p.csd) fail("closures other than diamond difference of order 1 are supported for the BTE only");
+                for (const string* sc : {&scheme, &scheme_y, &scheme_z})
+                    if (*sc != "DD" && *sc != "DG") fail("Closure relation not implemented yet.");
+                omx = scheme_weights_1d(scheme, K);
+                omy = scheme_weights_1d(scheme_y, Ky);
+                omz = scheme_weights_1d(scheme_z, Kz);
+                cl3 = Cell3(K, Ky, Kz, p.partial_coupling == 0);
+                Nm1 = cl3.F[0]; Nm2 = cl3.F[1]; Nm3 = cl3.F[2];
+                K = cl3.Nm;
+                ho3 = true;
+            }
         }
         Ncol = K * Nc;
         nphi = (size_t)A.Np * Ncol;
         nbx = (size_t)A.Nps * Nm1 * 2 * Ny * Nz;
         nby = ndims >= 2 ? (size_t)A.Nps * Nm2 * 2 * Nx * Nz : 0;
-        nbz = ndims >= 3 ? (size_t)A.Nps * 2 * Nx * Ny : 0;
+        nbz = ndims >= 3 ? (size_t)A.Nps * Nm3 * 2 * Nx * Ny : 0;
         S = nphi + nbx + nby + nbz;
     }
 
@@ -1459,6 +1561,84 @@ struct Solver {
                         for (int is = 0; is < Nm2; ++is)
                             ty[p + (size_t)Nps * (is + (size_t)Nm2 * ((fy_ ? 1 : 0) + 2 * ix))] +=
                                 Sy->D[p + (size_t)Nps * ry] * fym[is];
+            }
+        }
+    }
+
+    // sn_sweep_3D.jl:57-240 with the general cell: x outer, y, z inner.  x-face moments per (y, z)
+    // are carried across the columns, y-face moments per z along a column, z-face moments along a
+    // pillar; face arrays b[p + Nps*(m + Nm*(f + 2*pos))], surface sources enter moment 1 only.
+    void sweep3_ho(int n, int g, double* phi, const double* Ql, const double* bx_in, const double* by_in,
+                   const double* bz_in, double* tx, double* ty, double* tz) {
+        const int Np = A.Np, Nps = A.Nps, Nd = A.Nd;
+        const double om[3] = {A.mu[n], A.eta[n], A.xi[n]};
+        const bool fx_ = om[0] >= 0, fy_ = om[1] >= 0, fz_ = om[2] >= 0;
+        const SurfBasis *Sx = nullptr, *Sy = nullptr, *Sz = nullptr;
+        int rx = -1, ry = -1, rz = -1;
+        if (nxm[n]) { Sx = &A.S[0]; rx = nxm[n] - 1; } else if (nxp[n]) { Sx = &A.S[1]; rx = nxp[n] - 1; }
+        if (nym[n]) { Sy = &A.S[2]; ry = nym[n] - 1; } else if (nyp[n]) { Sy = &A.S[3]; ry = nyp[n] - 1; }
+        if (nzm[n]) { Sz = &A.S[4]; rz = nzm[n] - 1; } else if (nzp[n]) { Sz = &A.S[5]; rz = nzp[n] - 1; }
+        auto add_src = [&](const SurfBasis* Sb, int row, const double* src, size_t npos, size_t pos, double* f) {
+            if (src && Sb)
+                for (int p = 0; p < P.Np_source; ++p) f[0] += Sb->M[row + (size_t)Sb->Ndp * p] * src[(size_t)p * npos + pos];
+        };
+        auto add_bc = [&](int face_bc, bool fwd, const SurfBasis* Sb, int row, const double* b, int Nm, size_t pos,
+                          double* f) {
+            if (face_bc == 0 || !Sb) return;
+            const int sf = fwd ? (face_bc == 1 ? 0 : 1) : (face_bc == 1 ? 1 : 0);
+            for (int p = 0; p < Nps; ++p)
+                for (int is = 0; is < Nm; ++is)
+                    f[is] += Sb->M[row + (size_t)Sb->Ndp * p] * b[p + (size_t)Nps * (is + (size_t)Nm * (sf + 2 * pos))];
+        };
+        const double* w[3] = {omx.data(), omy.data(), omz.data()};
+        vector<double> FX((size_t)Nm1 * Ny * Nz, 0.0), FY((size_t)Nm2 * Nz), FZ(Nm3), Qn(K), psi(K);
+        for (int kx = 0; kx < Nx; ++kx) {
+            const int ix = fx_ ? kx : Nx - 1 - kx;
+            std::fill(FY.begin(), FY.end(), 0.0);
+            for (int ky = 0; ky < Ny; ++ky) {
+                const int iy = fy_ ? ky : Ny - 1 - ky;
+                std::fill(FZ.begin(), FZ.end(), 0.0);
+                add_src(Sz, rz, cur_src[fz_ ? 4 : 5], (size_t)Nx * Ny, (size_t)ix + (size_t)Nx * iy, FZ.data());
+                add_bc(fz_ ? P.bc[4] : P.bc[5], fz_, Sz, rz, bz_in, Nm3, (size_t)ix + (size_t)Nx * iy, FZ.data());
+                for (int kz = 0; kz < Nz; ++kz) {
+                    const int iz = fz_ ? kz : Nz - 1 - kz;
+                    double* fy = &FY[(size_t)Nm2 * iz];
+                    double* fx = &FX[(size_t)Nm1 * (iy + (size_t)Ny * iz)];
+                    if (ky == 0) {
+                        add_src(Sy, ry, cur_src[fy_ ? 2 : 3], (size_t)Nx * Nz, (size_t)ix + (size_t)Nx * iz, fy);
+                        add_bc(fy_ ? P.bc[2] : P.bc[3], fy_, Sy, ry, by_in, Nm2, (size_t)ix + (size_t)Nx * iz, fy);
+                    }
+                    if (kx == 0) {
+                        add_src(Sx, rx, cur_src[fx_ ? 0 : 1], (size_t)Ny * Nz, (size_t)iy + (size_t)Ny * iz, fx);
+                        add_bc(fx_ ? P.bc[0] : P.bc[1], fx_, Sx, rx, bx_in, Nm1, (size_t)iy + (size_t)Ny * iz, fx);
+                    }
+                    const int c = ix + Nx * (iy + Ny * iz);
+                    for (int is = 0; is < K; ++is) {
+                        double q = 0.0;
+                        for (int p = 0; p < Np; ++p) q += A.B.M[n + (size_t)Nd * p] * Ql[p + (size_t)Np * (is + (size_t)K * c)];
+                        Qn[is] = q;
+                    }
+                    const double h[3] = {P.dx[ix], P.dy[iy], P.dz[iz]};
+                    flux_3d_bte(om, h, sig_t(g, matc[c]), Qn.data(), fx, fy, FZ.data(), cl3, w, psi.data());
+                    for (int is = 0; is < K; ++is)
+                        for (int p = 0; p < Np; ++p)
+                            phi[p + (size_t)Np * (is + (size_t)K * c)] += A.B.D[p + (size_t)Np * n] * psi[is];
+                    if (kx == Nx - 1 && Sx)
+                        for (int p = 0; p < Nps; ++p)
+                            for (int is = 0; is < Nm1; ++is)
+                                tx[p + (size_t)Nps * (is + (size_t)Nm1 * ((fx_ ? 1 : 0) + 2 * (iy + (size_t)Ny * iz)))] +=
+                                    Sx->D[p + (size_t)Nps * rx] * fx[is];
+                    if (ky == Ny - 1 && Sy)
+                        for (int p = 0; p < Nps; ++p)
+                            for (int is = 0; is < Nm2; ++is)
+                                ty[p + (size_t)Nps * (is + (size_t)Nm2 * ((fy_ ? 1 : 0) + 2 * (ix + (size_t)Nx * iz)))] +=
+                                    Sy->D[p + (size_t)Nps * ry] * fy[is];
+                    if (kz == Nz - 1 && Sz)
+                        for (int p = 0; p < Nps; ++p)
+                            for (int is = 0; is < Nm3; ++is)
+                                tz[p + (size_t)Nps * (is + (size_t)Nm3 * ((fz_ ? 1 : 0) + 2 * (ix + (size_t)Nx * iy)))] +=
+                                    Sz->D[p + (size_t)Nps * rz] * FZ[is];
+                }
             }
         }
     }
@@ -1645,6 +1825,7 @@ struct Solver {
     void direction(int n, int g, double* phi, const double* Ql, const double* bin, double* tmp) {
         if (ndims == 1) sweep1(n, g, phi, Ql, bin, tmp);
         else if (ho2) sweep2_ho(n, g, phi, Ql, bin, bin + nbx, tmp, tmp + nbx);
+        else if (ho3) sweep3_ho(n, g, phi, Ql, bin, bin + nbx, bin + nbx + nby, tmp, tmp + nbx, tmp + nbx + nby);
         else sweep23(n, g, phi, Ql, bin, bin + nbx, bin + nbx + nby, tmp, tmp + nbx, tmp + nbx + nby);
     }
 
@@ -2097,6 +2278,20 @@ int oracle_cell_2d(const char* sx, const char* sy, int Ox, int Oy, int partial, 
         const bool fc = partial == 0;
         Cell2 cl{Ox, Oy, fc ? Ox * Oy : Ox + Oy - 1, fc};
         flux_2d_bte(mu, eta, St, dx, dy, Qn, fx, fy, cl, wx.data(), wy.data(), psi);
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// One 3D cell of closures (DD or DG per axis) and orders: psi receives the spatial moments; fx, fy,
+// fz enter as the incoming face moments (over (y, z), (x, z), (x, y)) and leave as the outgoing.
+int oracle_cell_3d(const char* sx, const char* sy, const char* sz, int Ox, int Oy, int Oz, int partial,
+                   const double* om, const double* h, double St, const double* Qn, double* fx, double* fy, double* fz,
+                   double* psi) {
+    try {
+        vector<double> wx = scheme_weights_1d(sx, Ox), wy = scheme_weights_1d(sy, Oy), wz = scheme_weights_1d(sz, Oz);
+        const double* w[3] = {wx.data(), wy.data(), wz.data()};
+        Cell3 cl(Ox, Oy, Oz, partial == 0);
+        flux_3d_bte(om, h, St, Qn, fx, fy, fz, cl, w, psi);
         return 0;
     } catch (std::exception& e) { g_err = e.what(); return -1; }
 }

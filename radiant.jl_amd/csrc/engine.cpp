@@ -44,10 +44,14 @@ static int solver_type_of(const rt_problem_desc& d) { return d.solver_type ? d.s
 // in 2D Kx*Ky with full coupling and Kx + Ky - 1 without.
 static int ho_order(const rt_problem_desc& d) {
     const int Kx = d.order_x > 1 ? d.order_x : 1, Ky = d.ndims >= 2 && d.order_y > 1 ? d.order_y : 1;
+    const int Kz = d.ndims >= 3 && d.order_z > 1 ? d.order_z : 1;
     auto dd1 = [](const double* w) { return !w || (w[0] == -1.0 && w[1] == 2.0); };
-    if (Kx == 1 && Ky == 1 && dd1(d.omega_x) && (d.ndims < 2 || dd1(d.omega_y))) return 0;
+    if (Kx == 1 && Ky == 1 && Kz == 1 && dd1(d.omega_x) && (d.ndims < 2 || dd1(d.omega_y)) &&
+        (d.ndims < 3 || dd1(d.omega_z)))
+        return 0;
     if (d.ndims == 1) return Kx;
-    return d.partial_coupling ? Kx + Ky - 1 : Kx * Ky;
+    if (d.ndims == 2) return d.partial_coupling ? Kx + Ky - 1 : Kx * Ky;
+    return d.partial_coupling ? Kx + Ky + Kz - 2 : Kx * Ky * Kz;
 }
 
 static bool has_fp(const rt_problem_desc& d) {
@@ -113,13 +117,14 @@ static void validate(const rt_problem_desc& d) {
     }
     if (surf && (d.Np_source < 1 || d.Np_source > d.Nps))
         bad("Np_source must lie in 1..Nps when a surface source is given");
-    const int kmax = d.ndims == 1 ? HO_KMAX : HO2_KMAX;
-    if (d.order_x < 0 || d.order_x > kmax || d.order_y < 0 || d.order_y > kmax)
+    const int kmax = d.ndims == 1 ? HO_KMAX : (d.ndims == 2 ? HO2_KMAX : HO3_KMAX);
+    if (d.order_x < 0 || d.order_x > kmax || d.order_y < 0 || d.order_y > kmax || d.order_z < 0 || d.order_z > kmax)
         bad("spatial closure orders must lie in 1.." + std::to_string(kmax) + " in " + std::to_string(d.ndims) + "D");
     if (d.order_x > 1 && !d.omega_x) bad("closures of order above 1 need their weights omega_x");
     if (d.ndims >= 2 && d.order_y > 1 && !d.omega_y) bad("closures of order above 1 need their weights omega_y");
-    if (ho_order(d) && (d.ndims == 3 || d.csd))
-        bad("closures other than diamond difference of order 1 are supported for the 1D and 2D BTE only");
+    if (d.ndims >= 3 && d.order_z > 1 && !d.omega_z) bad("closures of order above 1 need their weights omega_z");
+    if (ho_order(d) && d.csd)
+        bad("closures other than diamond difference of order 1 are supported for the BTE only");
     if (d.solver_type < 0 || d.solver_type > 6) bad("Unknown type of solver.");
     const int stype = solver_type_of(d);
     if ((stype >= 2) != (d.csd != 0))
@@ -147,12 +152,20 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
     validate(d);
     ndims = d.ndims; Nx = d.Nx; Ny = d.Ny; Nz = d.Nz; Ncell = Nx * Ny * Nz;
     ho_K = ho_order(d);
-    if (ho_K && ndims == 2) {
+    if (ho_K && ndims >= 2) {
         ho_Kx = d.order_x > 1 ? d.order_x : 1;
         ho_Ky = d.order_y > 1 ? d.order_y : 1;
+        ho_Kz = ndims == 3 && d.order_z > 1 ? d.order_z : 1;
         ho_fc = d.partial_coupling == 0;
-        fm[0] = ho_Ky;  // x faces carry the y moments, y faces the x moments (SN.get_schemes: Nm)
-        fm[1] = ho_Kx;
+        // a face carries the moments over the other axes (SN.get_schemes: Nm[1..3])
+        if (ndims == 2) {
+            fm[0] = ho_Ky;
+            fm[1] = ho_Kx;
+        } else {
+            fm[0] = ho_fc ? ho_Ky * ho_Kz : ho_Ky + ho_Kz - 1;
+            fm[1] = ho_fc ? ho_Kx * ho_Kz : ho_Kx + ho_Kz - 1;
+            fm[2] = ho_fc ? ho_Kx * ho_Ky : ho_Kx + ho_Ky - 1;
+        }
     }
     Nc = Ncell * (ho_K ? ho_K : 1);  // moment columns: spatial moment + K * cell
     G = d.G; Nmat = d.Nmat; Nd = d.Nd; Np = d.Np; Nps = d.Nps;
@@ -167,7 +180,7 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
     // boundary positions times face moments per axis
     npos[0] = (size_t)Ny * Nz * fm[0];
     npos[1] = ndims >= 2 ? (size_t)Nx * Nz * fm[1] : 0;
-    npos[2] = ndims >= 3 ? (size_t)Nx * Ny : 0;
+    npos[2] = ndims >= 3 ? (size_t)Nx * Ny * fm[2] : 0;
     nb = (size_t)2 * Nps * (npos[0] + npos[1] + npos[2]);
     for (int f = 0; f < 2 * ndims; ++f) {
         has_bc = has_bc || d.bc[f] != 0;
@@ -189,7 +202,7 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
         for (int c = 0; c < Nc; ++c) mat[c] = (int)(d.mat[c / Kc] - 1);
         d_mat = dalloc<int>(Nc); upload(d_mat, mat, st);
         if (ho_K) {  // closure weights per axis (diamond difference of order 1 when absent)
-            const int kx = ndims == 2 ? ho_Kx : ho_K;
+            const int kx = ndims >= 2 ? ho_Kx : ho_K;
             std::vector<double> om(d.omega_x ? d.omega_x : nullptr, d.omega_x ? d.omega_x + kx + 1 : nullptr);
             if (om.empty()) om = {-1.0, 2.0};
             d_omx = dalloc<double>(om.size()); upload(d_omx, om, st);
@@ -199,6 +212,16 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
                 d_omy = dalloc<double>(oy.size()); upload(d_omy, oy, st);
                 if (sweep_2d_ho_lds_bytes(P64, Nx, Ny, ho_Kx, ho_Ky) > 160 * 1024)
                     throw Error(RT_EINVAL, "the face moments of the 2D closure exceed the 160 KiB LDS of a workgroup");
+            }
+            if (ndims == 3) {
+                std::vector<double> oy(d.omega_y ? d.omega_y : nullptr, d.omega_y ? d.omega_y + ho_Ky + 1 : nullptr);
+                std::vector<double> oz(d.omega_z ? d.omega_z : nullptr, d.omega_z ? d.omega_z + ho_Kz + 1 : nullptr);
+                if (oy.empty()) oy = {-1.0, 2.0};
+                if (oz.empty()) oz = {-1.0, 2.0};
+                d_omy = dalloc<double>(oy.size()); upload(d_omy, oy, st);
+                d_omz = dalloc<double>(oz.size()); upload(d_omz, oz, st);
+                if (sweep_3d_ho_lds_bytes(P64, Nx, Ny, Nz, ho_Kx, ho_Ky, ho_Kz, ho_fc) > 160 * 1024)
+                    throw Error(RT_EINVAL, "the face moments of the 3D closure exceed the 160 KiB LDS of a workgroup");
             }
         }
         auto up = [&](const double* src, int n) {
@@ -466,9 +489,12 @@ int64_t Problem::device_bytes(const rt_problem_desc& d) {
     b += (int64_t)Nact_est * Nc * 8;                            // qlout
     b += (int64_t)d.Nd * Nc * 8;                                // psi
     b += (int64_t)3 * d.Nd * d.Np * 12;                         // operators
-    const int64_t fmax = ho_order(d) && d.ndims == 2 ? std::max(std::max(d.order_x, d.order_y), 1) : 1;  // face moments
-    const int64_t nposs = fmax * ((int64_t)d.Ny * d.Nz + (d.ndims >= 2 ? (int64_t)d.Nx * d.Nz : 0)) +
-                          (d.ndims >= 3 ? (int64_t)d.Nx * d.Ny : 0);
+    // face moments: at most the product of the other two orders
+    const int64_t fmax = ho_order(d) && d.ndims >= 2
+                             ? std::max<int64_t>(1, (int64_t)std::max(d.order_x, 1) * std::max(d.order_y, 1) * std::max(d.order_z, 1))
+                             : 1;
+    const int64_t nposs = fmax * ((int64_t)d.Ny * d.Nz + (d.ndims >= 2 ? (int64_t)d.Nx * d.Nz : 0) +
+                                  (d.ndims >= 3 ? (int64_t)d.Nx * d.Ny : 0));
     b += (int64_t)d.Nd * nposs * 8 + 2LL * d.Nps * nposs * 8;
     const bool faithful = d.precision == RT_PREC_FP64 && d.state_mode == RT_STATE_FAITHFUL;
     bool nonvoid = false;
@@ -526,6 +552,7 @@ void Problem::fill_geometry(PassDesc& pd, int g) {
     pd.ho_om = d_omx;
     pd.ho_Kx = ho_Kx; pd.ho_Ky = ho_Ky; pd.ho_fc = ho_fc ? 1 : 0; pd.ho_omy = d_omy;
     pd.fm_x = fm[0]; pd.fm_y = fm[1];
+    pd.ho_Kz = ho_Kz; pd.ho_omz = d_omz; pd.fm_z = fm[2];
 }
 
 hipEvent_t Problem::next_event() {
@@ -575,7 +602,10 @@ void Problem::launch_pass(PassDesc& pd, int prec, bool full_rows) {
     if (pd.psiE_in) launch_csd_source(d_pd, (long)Nd * Nc, st);  // energy inflow (FP64 passes)
     timed(1, [&] {
         const int sp = prec == PBF16 ? P32 : prec;
-        if (ho_K && ndims == 2)
+        if (ho_K && ndims == 3)
+            launch_sweep_3d_ho(sp, d_pd, Nd, ho_Kx, ho_Ky, ho_Kz, ho_fc,
+                               sweep_3d_ho_lds_bytes(sp, Nx, Ny, Nz, ho_Kx, ho_Ky, ho_Kz, ho_fc), st);
+        else if (ho_K && ndims == 2)
             launch_sweep_2d_ho(sp, d_pd, Nd, ho_Kx, ho_Ky, ho_fc, sweep_2d_ho_lds_bytes(sp, Nx, Ny, ho_Kx, ho_Ky), st);
         else if (ho_K)
             launch_sweep_1d_ho(sp, d_pd, Nd, ho_K, st);

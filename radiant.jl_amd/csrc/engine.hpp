@@ -76,7 +76,7 @@ public:
     int ndims, Nx, Ny, Nz, Nc, G, Nmat, Nd, Np, Nps, Nact;
     int Ncell = 0;   // cells; Nc counts moment columns (Ncell times the spatial moments per cell)
     int ho_K = 0;    // spatial moments per cell of a general closure (0: diamond difference, order 1)
-    int ho_Kx = 1, ho_Ky = 1;  // 2D closure orders
+    int ho_Kx = 1, ho_Ky = 1, ho_Kz = 1;  // 2D / 3D closure orders
     bool ho_fc = true;         // full coupling of the 2D moments
     int fm[3] = {1, 1, 1};     // moments of the face fluxes per axis (Ky, Kx for a 2D closure)
     int accel, restart, depth, maxit, precision, state_mode, verbose;
@@ -125,7 +125,7 @@ private:
     // material, and the source qlout + T ℳ Φ the contraction reads
     bool fp = false;
     double *d_fpM = nullptr, *d_fpT = nullptr, *d_fpq = nullptr;
-    double *d_omx = nullptr, *d_omy = nullptr;  // closure weights of the general cells per axis
+    double *d_omx = nullptr, *d_omy = nullptr, *d_omz = nullptr;  // closure weights of the general cells per axis
     double* d_Mentry[3] = {nullptr, nullptr, nullptr};
     int* d_insrc[3] = {nullptr, nullptr, nullptr};
     double* d_bin[3] = {nullptr, nullptr, nullptr};

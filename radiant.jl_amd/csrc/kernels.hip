@@ -1388,8 +1388,8 @@ __global__ __launch_bounds__(256) void bndmom_kernel(const PassDesc* __restrict_
         if (!d.bout_x) continue;
         const int Nps = d.Nps, Nd = d.Nd;
         // positions times face moments per axis (general 2D closures; 1 otherwise)
-        const int fm[3] = {d.fm_x > 1 ? d.fm_x : 1, d.fm_y > 1 ? d.fm_y : 1, 1};
-        const int npos[3] = {d.Ny * d.Nz * fm[0], d.Nx * d.Nz * fm[1], d.Nx * d.Ny};
+        const int fm[3] = {d.fm_x > 1 ? d.fm_x : 1, d.fm_y > 1 ? d.fm_y : 1, d.fm_z > 1 ? d.fm_z : 1};
+        const int npos[3] = {d.Ny * d.Nz * fm[0], d.Nx * d.Nz * fm[1], d.Nx * d.Ny * fm[2]};
         const size_t tot = (size_t)2 * Nps * (npos[0] + (d.ndims >= 2 ? npos[1] : 0) + (d.ndims >= 3 ? npos[2] : 0));
         for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
             size_t j = i;
@@ -1440,7 +1440,7 @@ __global__ __launch_bounds__(256) void bndin_kernel(const PassDesc* __restrict__
     const PassDesc& d = pds[0];
     const int Nd = d.Nd, Nps = d.Nps;
     // positions times face moments per axis (the moments of a general 2D closure; 1 otherwise)
-    const int fm[3] = {d.fm_x > 1 ? d.fm_x : 1, d.fm_y > 1 ? d.fm_y : 1, 1};
+    const int fm[3] = {d.fm_x > 1 ? d.fm_x : 1, d.fm_y > 1 ? d.fm_y : 1, d.fm_z > 1 ? d.fm_z : 1};
     const size_t np[3] = {(size_t)d.Ny * d.Nz, (size_t)d.Nx * d.Nz, (size_t)d.Nx * d.Ny};
     size_t tot = 0;
     for (int ax = 0; ax < d.ndims; ++ax) tot += (size_t)Nd * np[ax] * fm[ax];
@@ -1794,6 +1794,179 @@ __global__ __launch_bounds__(256) void sweep_2d_ho_kernel(const PassDesc* __rest
         for (int i = tid; i < Ny * KY; i += nt) ((T*)d.bx)[(size_t)n * Ny * KY + i] = FX[i];
     if (d.by)
         for (int i = tid; i < Nx * KX; i += nt) ((T*)d.by)[(size_t)n * Nx * KX + i] = FY[i];
+}
+
+// ------------------------------------------------------------------------------------------
+// 3D sweep with a general closure of orders KX x KY x KZ (sn_sweep_3D.jl:57-240 with flux_3D_BTE,
+// sn_3D_bte.jl:34-152): one workgroup per direction walks the diagonal planes kx + ky + kz = d in
+// sweep order, one cell per thread.  Cell moments (a, b, c): KX*KY*c + KX*b + a with full coupling;
+// without it only one of a, b, c is non-zero, at a + b + c (+ KX-1 when b > 0, + KX-1+KY-1 when
+// c > 0).  The x, y, z faces carry the moments over (b, c), (a, c), (a, b) with the same rule, held
+// in LDS per pencil: FX[Ny*Nz][F0], FY[Nx*Nz][F1], FZ[Nx*Ny][F2].  The cell system and the face
+// updates are those of the 2D kernel with a third axis; an axis's terms couple moments that agree on
+// the other two axes.
+// ------------------------------------------------------------------------------------------
+template <int KX, int KY, int KZ, bool FC>
+struct Ho3 {
+    static constexpr int NM = FC ? KX * KY * KZ : KX + KY + KZ - 2;
+    static constexpr int F0 = FC ? KY * KZ : KY + KZ - 1, F1 = FC ? KX * KZ : KX + KZ - 1, F2 = FC ? KX * KY : KX + KY - 1;
+    __device__ static constexpr bool valid(int a, int b, int c) { return FC || (a > 0) + (b > 0) + (c > 0) <= 1; }
+    __device__ static constexpr int idx(int a, int b, int c) {
+        return FC ? KX * KY * c + KX * b + a : a + b + c + (b > 0 ? KX - 1 : 0) + (c > 0 ? KX - 1 + KY - 1 : 0);
+    }
+    __device__ static constexpr int fx(int b, int c) { return FC ? KY * c + b : b + c + (c > 0 ? KY - 1 : 0); }
+    __device__ static constexpr int fy(int a, int c) { return FC ? KX * c + a : a + c + (c > 0 ? KX - 1 : 0); }
+    __device__ static constexpr int fz(int a, int b) { return FC ? KX * b + a : a + b + (b > 0 ? KX - 1 : 0); }
+};
+
+template <typename T, int KX, int KY, int KZ, bool FC>
+__global__ __launch_bounds__(256) void sweep_3d_ho_kernel(const PassDesc* __restrict__ pds) {
+    using H = Ho3<KX, KY, KZ, FC>;
+    constexpr int NM = H::NM, F0 = H::F0, F1 = H::F1, F2 = H::F2;
+    constexpr int KM = KX > KY ? (KX > KZ ? KX : KZ) : (KY > KZ ? KY : KZ);
+    extern __shared__ __align__(16) unsigned char smem[];
+    const PassDesc& d = pds[0];
+    const int n = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    const int Nx = d.Nx, Ny = d.Ny, Nz = d.Nz, Nc = d.Nc;
+    T* FX = (T*)smem;                        // [Ny*Nz][F0]
+    T* FY = FX + (size_t)Ny * Nz * F0;       // [Nx*Nz][F1]
+    T* FZ = FY + (size_t)Nx * Nz * F1;       // [Nx*Ny][F2]
+    const double om[3] = {d.mu[n], d.eta[n], d.xi[n]};
+    const int K3[3] = {KX, KY, KZ};
+    T s[3], w[3][KM + 1], Cc[KM], sp[3][KM], sn[3][KM];
+    const double* ws[3] = {d.ho_om, d.ho_omy, d.ho_omz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        s[a] = om[a] > 0 ? T(1) : (om[a] < 0 ? T(-1) : T(0));
+#pragma unroll
+        for (int k = 0; k <= KM; ++k) w[a][k] = k <= K3[a] ? (T)ws[a][k] : T(0);
+#pragma unroll
+        for (int k = 0; k < KM; ++k) {
+            sp[a][k] = k == 0 ? T(1) : sp[a][k - 1] * s[a];
+            sn[a][k] = k == 0 ? T(1) : -sn[a][k - 1] * s[a];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < KM; ++k) Cc[k] = (T)sqrt(2.0 * k + 1.0);
+    T* __restrict__ psi = (T*)d.psi + (size_t)n * Nc;
+    for (int i = tid; i < Ny * Nz * F0; i += nt) FX[i] = d.bin_x ? (T)d.bin_x[(size_t)n * Ny * Nz * F0 + i] : T(0);
+    for (int i = tid; i < Nx * Nz * F1; i += nt) FY[i] = d.bin_y ? (T)d.bin_y[(size_t)n * Nx * Nz * F1 + i] : T(0);
+    for (int i = tid; i < Nx * Ny * F2; i += nt) FZ[i] = d.bin_z ? (T)d.bin_z[(size_t)n * Nx * Ny * F2 + i] : T(0);
+    __syncthreads();
+    const bool fw[3] = {om[0] >= 0, om[1] >= 0, om[2] >= 0};
+    for (int dg = 0; dg < Nx + Ny + Nz - 2; ++dg) {
+        for (int i = tid; i < Nx * Ny; i += nt) {
+            const int kx = i % Nx, ky = i / Nx, kz = dg - kx - ky;
+            if (kz < 0 || kz >= Nz) continue;
+            const int ix = fw[0] ? kx : Nx - 1 - kx, iy = fw[1] ? ky : Ny - 1 - ky, iz = fw[2] ? kz : Nz - 1 - kz;
+            const size_t c = (size_t)ix + (size_t)Nx * (iy + (size_t)Ny * iz);
+            const T h[3] = {(T)(fabs(om[0]) / d.dx[ix]), (T)(fabs(om[1]) / d.dy[iy]), (T)(fabs(om[2]) / d.dz[iz])};
+            const T St = (T)d.sigt[d.mat[c * NM]];
+            T* fx = FX + ((size_t)iy + (size_t)Ny * iz) * F0;
+            T* fy = FY + ((size_t)ix + (size_t)Nx * iz) * F1;
+            T* fz = FZ + ((size_t)ix + (size_t)Nx * iy) * F2;
+            T S[NM][NM], q[NM];
+#pragma unroll
+            for (int i2 = 0; i2 < NM; ++i2)
+#pragma unroll
+                for (int j = 0; j < NM; ++j) S[i2][j] = i2 == j ? St : T(0);
+#pragma unroll
+            for (int a = 0; a < KX; ++a)
+#pragma unroll
+                for (int b = 0; b < KY; ++b)
+#pragma unroll
+                    for (int cz = 0; cz < KZ; ++cz) {
+                        if (!H::valid(a, b, cz)) continue;
+                        const int ii = H::idx(a, b, cz);
+                        const int ia[3] = {a, b, cz};
+#pragma unroll
+                        for (int a2 = 0; a2 < KX; ++a2)
+#pragma unroll
+                            for (int b2 = 0; b2 < KY; ++b2)
+#pragma unroll
+                                for (int c2 = 0; c2 < KZ; ++c2) {
+                                    if (!H::valid(a2, b2, c2)) continue;
+                                    const int jj = H::idx(a2, b2, c2);
+                                    const int jb[3] = {a2, b2, c2};
+#pragma unroll
+                                    for (int ax = 0; ax < 3; ++ax) {
+                                        const int o1 = (ax + 1) % 3, o2 = (ax + 2) % 3;
+                                        if (ia[o1] != jb[o1] || ia[o2] != jb[o2]) continue;
+                                        const int u = ia[ax], v = jb[ax];
+                                        if (u > v && ((u - v) & 1)) S[ii][jj] -= T(2) * Cc[u] * h[ax] * s[ax] * Cc[v];
+                                        S[ii][jj] += Cc[u] * h[ax] * sp[ax][u] * Cc[v] * sp[ax][v] * w[ax][v + 1];
+                                    }
+                                }
+                        q[ii] = psi[(size_t)ii + NM * c] - Cc[a] * h[0] * (sp[0][a] * w[0][0] - sn[0][a]) * fx[H::fx(b, cz)] -
+                                Cc[b] * h[1] * (sp[1][b] * w[1][0] - sn[1][b]) * fy[H::fy(a, cz)] -
+                                Cc[cz] * h[2] * (sp[2][cz] * w[2][0] - sn[2][cz]) * fz[H::fz(a, b)];
+                    }
+#pragma unroll
+            for (int p = 0; p < NM; ++p) {
+#pragma unroll
+                for (int i2 = p + 1; i2 < NM; ++i2) {
+                    const bool sw = fabs(S[i2][p]) > fabs(S[p][p]);
+#pragma unroll
+                    for (int j = 0; j < NM; ++j) {
+                        const T t = S[p][j];
+                        S[p][j] = sw ? S[i2][j] : t;
+                        S[i2][j] = sw ? t : S[i2][j];
+                    }
+                    const T tq = q[p];
+                    q[p] = sw ? q[i2] : tq;
+                    q[i2] = sw ? tq : q[i2];
+                }
+#pragma unroll
+                for (int i2 = p + 1; i2 < NM; ++i2) {
+                    const T f = S[i2][p] / S[p][p];
+#pragma unroll
+                    for (int j = p + 1; j < NM; ++j) S[i2][j] -= f * S[p][j];
+                    q[i2] -= f * q[p];
+                }
+            }
+#pragma unroll
+            for (int i2 = NM - 1; i2 >= 0; --i2) {
+                T v = q[i2];
+#pragma unroll
+                for (int j = i2 + 1; j < NM; ++j) v -= S[i2][j] * q[j];
+                q[i2] = v / S[i2][i2];
+            }
+#pragma unroll
+            for (int i2 = 0; i2 < NM; ++i2) psi[(size_t)i2 + NM * c] = q[i2];
+            T ox[F0], oy[F1], oz[F2];
+#pragma unroll
+            for (int k = 0; k < F0; ++k) ox[k] = w[0][0] * fx[k];
+#pragma unroll
+            for (int k = 0; k < F1; ++k) oy[k] = w[1][0] * fy[k];
+#pragma unroll
+            for (int k = 0; k < F2; ++k) oz[k] = w[2][0] * fz[k];
+#pragma unroll
+            for (int a = 0; a < KX; ++a)
+#pragma unroll
+                for (int b = 0; b < KY; ++b)
+#pragma unroll
+                    for (int cz = 0; cz < KZ; ++cz) {
+                        if (!H::valid(a, b, cz)) continue;
+                        const T v = q[H::idx(a, b, cz)];
+                        ox[H::fx(b, cz)] += Cc[a] * sp[0][a] * w[0][a + 1] * v;
+                        oy[H::fy(a, cz)] += Cc[b] * sp[1][b] * w[1][b + 1] * v;
+                        oz[H::fz(a, b)] += Cc[cz] * sp[2][cz] * w[2][cz + 1] * v;
+                    }
+#pragma unroll
+            for (int k = 0; k < F0; ++k) fx[k] = ox[k];
+#pragma unroll
+            for (int k = 0; k < F1; ++k) fy[k] = oy[k];
+#pragma unroll
+            for (int k = 0; k < F2; ++k) fz[k] = oz[k];
+        }
+        __syncthreads();
+    }
+    if (d.bx)
+        for (int i = tid; i < Ny * Nz * F0; i += nt) ((T*)d.bx)[(size_t)n * Ny * Nz * F0 + i] = FX[i];
+    if (d.by)
+        for (int i = tid; i < Nx * Nz * F1; i += nt) ((T*)d.by)[(size_t)n * Nx * Nz * F1 + i] = FY[i];
+    if (d.bz)
+        for (int i = tid; i < Nx * Ny * F2; i += nt) ((T*)d.bz)[(size_t)n * Nx * Ny * F2 + i] = FZ[i];
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2359,6 +2532,55 @@ void launch_sweep_2d_ho(int prec, const PassDesc* d_pd, int Nd, int Kx, int Ky, 
     if (lds > 160 * 1024) throw std::runtime_error("2D closure face moments exceed the LDS");
     if (prec == P64) ho2_dispatch<double>(d_pd, Nd, Kx, Ky, fc, lds, st);
     else ho2_dispatch<float>(d_pd, Nd, Kx, Ky, fc, lds, st);
+    RT_HIP(hipGetLastError());
+}
+
+size_t sweep_3d_ho_lds_bytes(int prec, int Nx, int Ny, int Nz, int Kx, int Ky, int Kz, bool fc) {
+    const size_t f0 = fc ? Ky * Kz : Ky + Kz - 1, f1 = fc ? Kx * Kz : Kx + Kz - 1, f2 = fc ? Kx * Ky : Kx + Ky - 1;
+    return (prec == P64 ? 8 : 4) * ((size_t)Ny * Nz * f0 + (size_t)Nx * Nz * f1 + (size_t)Nx * Ny * f2);
+}
+
+template <typename T, int KX, int KY, int KZ, bool FC>
+static void ho3_launch(const PassDesc* d_pd, int Nd, size_t lds, hipStream_t st) {
+    auto fn = sweep_3d_ho_kernel<T, KX, KY, KZ, FC>;
+    static bool attr = false;
+    if (!attr) {
+        RT_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr = true;
+    }
+    hipLaunchKernelGGL(fn, dim3(Nd), dim3(256), lds, st, d_pd);
+}
+
+template <typename T, int KX, int KY, int KZ>
+static void ho3_fc(bool fc, const PassDesc* d_pd, int Nd, size_t lds, hipStream_t st) {
+    // partial coupling differs from full coupling only when two orders or more exceed 1
+    if constexpr ((KX > 1) + (KY > 1) + (KZ > 1) >= 2) {
+        if (!fc) return ho3_launch<T, KX, KY, KZ, false>(d_pd, Nd, lds, st);
+    }
+    ho3_launch<T, KX, KY, KZ, true>(d_pd, Nd, lds, st);
+}
+
+template <typename T>
+static void ho3_dispatch(const PassDesc* d_pd, int Nd, int Kx, int Ky, int Kz, bool fc, size_t lds, hipStream_t st) {
+    switch (Kx * 100 + Ky * 10 + Kz) {
+        case 111: return ho3_fc<T, 1, 1, 1>(fc, d_pd, Nd, lds, st);
+        case 211: return ho3_fc<T, 2, 1, 1>(fc, d_pd, Nd, lds, st);
+        case 121: return ho3_fc<T, 1, 2, 1>(fc, d_pd, Nd, lds, st);
+        case 112: return ho3_fc<T, 1, 1, 2>(fc, d_pd, Nd, lds, st);
+        case 221: return ho3_fc<T, 2, 2, 1>(fc, d_pd, Nd, lds, st);
+        case 212: return ho3_fc<T, 2, 1, 2>(fc, d_pd, Nd, lds, st);
+        case 122: return ho3_fc<T, 1, 2, 2>(fc, d_pd, Nd, lds, st);
+        case 222: return ho3_fc<T, 2, 2, 2>(fc, d_pd, Nd, lds, st);
+        default: throw std::runtime_error("3D closure orders out of range");
+    }
+}
+
+void launch_sweep_3d_ho(int prec, const PassDesc* d_pd, int Nd, int Kx, int Ky, int Kz, bool fc, size_t lds,
+                        hipStream_t st) {
+    if (Nd <= 0) return;
+    if (lds > 160 * 1024) throw std::runtime_error("3D closure face moments exceed the LDS");
+    if (prec == P64) ho3_dispatch<double>(d_pd, Nd, Kx, Ky, Kz, fc, lds, st);
+    else ho3_dispatch<float>(d_pd, Nd, Kx, Ky, Kz, fc, lds, st);
     RT_HIP(hipGetLastError());
 }
 

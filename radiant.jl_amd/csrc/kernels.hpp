@@ -75,6 +75,10 @@ struct PassDesc {
     int ho_Kx, ho_Ky, ho_fc;
     const double* ho_omy;
     int fm_x, fm_y;
+    // 3D: the z order and weights; faces carry the moments over the other two axes (fm_z: x, y)
+    int ho_Kz;
+    const double* ho_omz;
+    int fm_z;
 };
 
 enum Prec { P64 = 0, P32 = 1, PBF16 = 2,
@@ -137,6 +141,12 @@ void launch_sweep_1d_ho(int prec, const PassDesc* d_pd, int Nd, int K, hipStream
 constexpr int HO2_KMAX = 3;
 size_t sweep_2d_ho_lds_bytes(int prec, int Nx, int Ny, int Kx, int Ky);
 void launch_sweep_2d_ho(int prec, const PassDesc* d_pd, int Nd, int Kx, int Ky, bool fc, size_t lds, hipStream_t st);
+// 3D sweep of a general closure of orders up to HO3_KMAX per axis: one workgroup per direction
+// walking the diagonal planes; lds = face moments of every pencil of the three axes
+constexpr int HO3_KMAX = 2;
+size_t sweep_3d_ho_lds_bytes(int prec, int Nx, int Ny, int Nz, int Kx, int Ky, int Kz, bool fc);
+void launch_sweep_3d_ho(int prec, const PassDesc* d_pd, int Nd, int Kx, int Ky, int Kz, bool fc, size_t lds,
+                        hipStream_t st);
 void launch_dgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st);
 void launch_bndmom(const PassDesc* d_pd, int nprob, int max_items, hipStream_t st);
 

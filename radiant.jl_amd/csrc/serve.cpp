@@ -31,7 +31,7 @@
 // Deep copy of a problem description (the caller's buffers may go away after submit).
 struct DescCopy {
     rt_problem_desc d{};
-    std::vector<double> dx, dy, dz, sigt, sigs, qvol, mu, eta, xi, w, M, D, Msurf[6], Dsurf[6], ssrc[6], Sb, dE, T, Mfp, omega_x, omega_y;
+    std::vector<double> dx, dy, dz, sigt, sigs, qvol, mu, eta, xi, w, M, D, Msurf[6], Dsurf[6], ssrc[6], Sb, dE, T, Mfp, omega_x, omega_y, omega_z;
     std::vector<int64_t> mat, pl, n2p[6];
     std::vector<int32_t> bc;
 
@@ -69,6 +69,8 @@ struct DescCopy {
         d.omega_x = s.omega_x ? omega_x.data() : nullptr;
         cp(omega_y, s.omega_y, (size_t)(s.order_y > 1 ? s.order_y : 1) + 1);
         d.omega_y = s.omega_y ? omega_y.data() : nullptr;
+        cp(omega_z, s.omega_z, (size_t)(s.order_z > 1 ? s.order_z : 1) + 1);
+        d.omega_z = s.omega_z ? omega_z.data() : nullptr;
         for (int f = 0; f < 6; ++f) {
             d.Msurf[f] = nullptr; d.Dsurf[f] = nullptr; d.n_to_np[f] = nullptr;
             if (f < 2 * s.ndims) {

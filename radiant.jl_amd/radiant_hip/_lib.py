@@ -41,6 +41,7 @@ class ProblemDesc(C.Structure):
         ("solver_type", C.c_int32), ("T", dp), ("Mfp", dp),
         ("order_x", C.c_int32), ("omega_x", dp),
         ("order_y", C.c_int32), ("omega_y", dp), ("partial_coupling", C.c_int32),
+        ("order_z", C.c_int32), ("omega_z", dp),
     ]
 
 

@@ -109,6 +109,8 @@ class Problem:
     scheme_y: str = "DD"
     scheme_order_y: int = 1
     full_coupling: bool = True
+    scheme_z: str = "DD"     # the z axis of 3D problems
+    scheme_order_z: int = 1
 
     @property
     def Nc(self):

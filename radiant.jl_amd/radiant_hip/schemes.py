@@ -38,11 +38,13 @@ def weights_1d(scheme, K):
 
 def spatial_moments(prob):
     """Spatial moments per cell of a problem: its 1D scheme order; in 2D Ox*Oy with full coupling
-    and Ox + Oy - 1 without (SN.get_schemes: Nm[5]); 1 in 3D."""
+    and Ox + Oy - 1 without, in 3D Ox*Oy*Oz and Ox + Oy + Oz - 2 (SN.get_schemes: Nm[5])."""
     ox = max(1, int(getattr(prob, "scheme_order", 1) or 1))
     if prob.ndims == 1:
         return ox
+    oy = max(1, int(getattr(prob, "scheme_order_y", 1) or 1))
+    fc = getattr(prob, "full_coupling", True)
     if prob.ndims == 2:
-        oy = max(1, int(getattr(prob, "scheme_order_y", 1) or 1))
-        return ox * oy if getattr(prob, "full_coupling", True) else ox + oy - 1
-    return 1
+        return ox * oy if fc else ox + oy - 1
+    oz = max(1, int(getattr(prob, "scheme_order_z", 1) or 1))
+    return ox * oy * oz if fc else ox + oy + oz - 2

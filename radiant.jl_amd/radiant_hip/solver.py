@@ -74,17 +74,22 @@ class Desc:
         K = schemes.spatial_moments(prob)
         scheme = (getattr(prob, "scheme", None) or "DD").upper()
         ox = max(1, int(getattr(prob, "scheme_order", 1) or 1))
-        if prob.ndims == 2:
+        if prob.ndims >= 2:
             scheme_y = (getattr(prob, "scheme_y", None) or "DD").upper()
             oy = max(1, int(getattr(prob, "scheme_order_y", 1) or 1))
-            if ox > 1 or oy > 1 or scheme != "DD" or scheme_y != "DD":
-                for sc in (scheme, scheme_y):
+            scheme_z = (getattr(prob, "scheme_z", None) or "DD").upper() if prob.ndims == 3 else "DD"
+            oz = max(1, int(getattr(prob, "scheme_order_z", 1) or 1)) if prob.ndims == 3 else 1
+            if ox > 1 or oy > 1 or oz > 1 or scheme != "DD" or scheme_y != "DD" or scheme_z != "DD":
+                for sc in (scheme, scheme_y, scheme_z):
                     if sc not in ("DD", "DG"):
                         raise ValueError("Closure relation not implemented yet.")
                 k["omega_x"] = np.ascontiguousarray(schemes.weights_1d(scheme, ox))
                 k["omega_y"] = np.ascontiguousarray(schemes.weights_1d(scheme_y, oy))
                 d.order_x, d.omega_x = ox, ptr(k["omega_x"])
                 d.order_y, d.omega_y = oy, ptr(k["omega_y"])
+                if prob.ndims == 3:
+                    k["omega_z"] = np.ascontiguousarray(schemes.weights_1d(scheme_z, oz))
+                    d.order_z, d.omega_z = oz, ptr(k["omega_z"])
                 d.partial_coupling = 0 if getattr(prob, "full_coupling", True) else 1
         elif K > 1 or scheme != "DD":  # closure weights of the x axis (scheme_weights.jl)
             k["omega_x"] = np.ascontiguousarray(schemes.weights_1d(scheme, K))

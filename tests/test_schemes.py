@@ -115,19 +115,20 @@ def test_infinite_medium_has_no_slopes(oracle, scheme, K):
             assert np.abs(mom).max() < 1e-10 * np.abs(ref).max()
 
 
-def test_higher_order_differs_and_refuses_3d(oracle):
+def test_higher_order_differs_and_refusals(oracle):
     P, _ = _S()
     a, _ = oracle.solve(_slab("DD", 1))
     b, _ = oracle.solve(_slab("DG", 2))
     assert rel_err(b[:, :, 0::2], a) > 1e-4
-    q = P.small_problem(3, N=3, G=2)
-    q.scheme, q.scheme_order = "DG", 2
-    with pytest.raises(RuntimeError, match="1D and 2D only"):
+    for nd in (2, 3):  # the modified DG closures exist in 1D only
+        r = P.small_problem(nd, N=3, G=2)
+        r.scheme, r.scheme_order = "DG-", 2
+        with pytest.raises(RuntimeError, match="not implemented"):
+            oracle.solve(r)
+    q = _slab("DG", 2)   # with slowing down the energy closure couples in: not provided
+    q.csd, q.Sb, q.dE = True, np.ones((q.G + 1, q.Nmat)), np.ones(q.G)
+    with pytest.raises(RuntimeError, match="BTE only"):
         oracle.solve(q)
-    r = P.small_problem(2, N=3, G=2)
-    r.scheme, r.scheme_order = "DG-", 2
-    with pytest.raises(RuntimeError, match="not implemented"):
-        oracle.solve(r)
 
 
 # ------------------------------------------------------------------------------------------
