@@ -12,8 +12,8 @@
 //
 // Scope: Cartesian 1D/2D/3D, solver types BTE, BFP, BCSD, FP, CSD and BFP-EF (slowing down by
 // the DD scheme of order 1 in energy), diamond difference of order 1 on every axis
-// (`scheme_weights.jl:47-51` with order 1 gives w = [-1, 2]) and, for the BTE in 1D and 2D, the
-// DD / DG closures of higher orders (`sn_1D_bte.jl`, `sn_2D_bte.jl`), "standard", "galerkin-d" and
+// (`scheme_weights.jl:47-51` with order 1 gives w = [-1, 2]) and, for the BTE, the DD / DG
+// closures of higher orders (`sn_1D_bte.jl`, `sn_2D_bte.jl`, `sn_3D_bte.jl`), "standard", "galerkin-d" and
 // "galerkin-m" angular operators, void / reflective / periodic boundaries, volume sources,
 // accelerations none / livolant / anderson / gmres / bicgstab.
 //
