@@ -89,6 +89,9 @@ struct TuneState {
 std::mutex g_tune_mu;
 std::map<TuneKey, TuneState> g_tune;
 std::atomic<int> g_tune_mode{-1};
+// Launches issued from this host thread may be timed (false on threads whose launches share the
+// device with other concurrent streams: their timings would measure the neighbours' work too).
+thread_local bool tl_tune_measure = true;
 
 int tune_mode() {
     int m = g_tune_mode.load();
@@ -138,7 +141,7 @@ int tune_choose(const TuneKey& key, int ncand, TuneTicket& t) {
     if (mode == 2) { const int c = s.next; s.next = (s.next + 1) % ncand; return t.cand = c; }
     if (!s.pending.empty()) tune_harvest(s);
     if (s.best >= 0) return t.cand = s.best;
-    if (tl_ev_start) return t.cand = 0;  // the caller times this launch itself
+    if (tl_ev_start || !tl_tune_measure) return t.cand = 0;  // timed by the caller, or not timed alone
     bool done = true;
     for (auto& v : s.ms) done = done && v.size() >= (size_t)TUNE_REPS;
     if (done) {
@@ -187,6 +190,8 @@ void tune_commit(const TuneTicket& t) {
     s.pending.push_back({t.cand, t.a, t.b});
 }
 }  // namespace
+
+void set_autotune_measuring(bool on) { tl_tune_measure = on; }
 
 void set_autotune_mode(int mode) {
     std::lock_guard<std::mutex> lk(g_tune_mu);

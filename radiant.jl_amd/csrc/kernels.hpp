@@ -93,6 +93,10 @@ void set_launch_events(hipEvent_t start, hipEvent_t stop);
 // every launch.  Setting the mode forgets every tuned key.  The report lists each key's choice
 // and the fastest time of every candidate.
 void set_autotune_mode(int mode);
+// Whether the calling host thread's launches may be timed for tuning (default true); threads whose
+// launches run beside other streams' work (concurrent sub-batches, served requests) turn it off
+// and use the shapes tuned elsewhere or the defaults.
+void set_autotune_measuring(bool on);
 std::string autotune_report();
 
 // Work tiles of the contraction kernels: x = problem, y = 64-cell tile, z = row group.

@@ -164,6 +164,7 @@ static void solve_batch(rt_batch* b, rt_solve_info* info) {
         std::vector<std::thread> th;
         for (int s = 0; s < nsub; ++s)
             th.emplace_back([&, s] {
+                rt::set_autotune_measuring(nsub == 1);  // concurrent sub-batches are not timed
                 try {
                     b->bss[s]->run(&infos[s]);
                 } catch (const std::exception& e) {
@@ -197,6 +198,7 @@ static void solve_batch(rt_batch* b, rt_solve_info* info) {
     std::vector<rt_solve_info> infos(n);
     for (int i = 0; i < n; ++i)
         th.emplace_back([&, i] {
+            rt::set_autotune_measuring(n == 1);  // concurrent problems are not timed
             try {
                 b->probs[i]->solve(b->stats[i].data(), &infos[i]);
             } catch (const std::exception& e) {
@@ -375,6 +377,7 @@ struct rt_server {
     }
 
     void run_request(std::shared_ptr<Request> r) {
+        rt::set_autotune_measuring(false);  // served requests overlap on the device
         // The outcome is published after the device memory of the solve has been released.
         Request::State state = Request::CANCELLED;
         std::unique_ptr<rt::Problem> prob;
@@ -618,7 +621,10 @@ int rt_server_create(const rt_server_config* cfg, rt_server** out) {
     }
     auto* s = new rt_server();
     s->cfg = *cfg;
-    s->sched = std::thread([s] { s->scheduler(); });
+    s->sched = std::thread([s] {
+        rt::set_autotune_measuring(false);  // served requests overlap on the device
+        s->scheduler();
+    });
     *out = s;
     return RT_OK;
 }
