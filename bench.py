@@ -33,7 +33,7 @@ HBM_PEAK_GBS = 8000.0
 # from rocprofv3 --pmc runs over that workload: FETCH_SIZE and WRITE_SIZE (KiB per dispatch,
 # averaged over the kernel's dispatches), each scaled by what the counter reads for a plain 1 GiB
 # device copy (FETCH_SIZE reads 0.5 GiB for it: factor 2.000; WRITE_SIZE 1 GiB: factor 1.000).
-MEASURED_TRAFFIC = {("sweep_fp32", 32): dict(fetch=335.50e6, write=307.81e6,
+MEASURED_TRAFFIC = {("sweep_fp32", 32): dict(fetch=336.36e6, write=308.34e6,
                                              kernel="sweep_line_mixed_kernel<2, 8, true>",
                                              summary="profiles/sweep_line_mixed_c1_batch32.md")}
 MFMA_PEAK_TFLOPS = {0: 78.6, 1: 157.3, 2: 2500.0}
