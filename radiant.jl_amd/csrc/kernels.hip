@@ -1125,12 +1125,231 @@ __device__ __forceinline__ void sweep_line_body(const PassDesc& d, const int n, 
     }
 }
 
+// ------------------------------------------------------------------------------------------
+// Row-scan line sweep, buffer-addressed form (the DPP scan's kernels).
+//
+// Each row is reached through a raw buffer resource whose base is the row and whose
+// num_records is its byte length, so a lane's offsets stay fixed for the whole direction and
+// the row address is scalar work.  Loads beyond num_records return 0 and stores beyond it are
+// dropped by the hardware: a cell past Nx gets the offset BUF_OOB, a row past Ny gets
+// num_records 0, and neither needs a select, a branch or a trash line.
+//
+// A cell past Nx is given the identity map (a = 0, b = 1): its source loads as 0, its y-face
+// flux starts and stays 0, its hx is 0 (so c0 = c1 = 0) with 1 in place of hx in the removal
+// (no 0/0 in a void material), and b = 2 c1 + bias with bias = 1 instead of -1.  The exit
+// x-face of a row is then the flux after a lane's last cell, whatever the lane holds.
+// ------------------------------------------------------------------------------------------
+constexpr unsigned BUF_OOB = 0x80000000u;  // beyond every num_records used here
+__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes, 0x00020000);
+}
+template <typename T> __device__ inline T bld(__amdgpu_buffer_rsrc_t r, unsigned off);
+template <> __device__ inline float bld<float>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+template <> __device__ inline int bld<int>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return (int)__builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0);
+}
+template <> __device__ inline double bld<double>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+typedef unsigned rt_u32x2 __attribute__((ext_vector_type(2)));
+__device__ inline void bst(__amdgpu_buffer_rsrc_t r, unsigned off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)off, 0, 0);
+}
+__device__ inline void bst(__amdgpu_buffer_rsrc_t r, unsigned off, double v) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(rt_u32x2, v), r, (int)off, 0, 0);
+}
+
+// Inclusive affine scan of one FP32 map per lane with the cross-lane read fused into the
+// arithmetic (v_fmac_f32_dpp: A += A[src] * B; v_mul_f32_dpp: B *= B[src]).  bound_ctrl is off,
+// so a lane whose source lane is out of range, or whose row the row mask leaves out, is not
+// written and keeps its map.  A DPP read of a VGPR needs two wait states after a VALU write of
+// it: the s_nop 1 in front covers the compiler's last writes of A and B, the s_nop 0 between
+// steps the fmac -> next fmac distance, and the s_nop 1 at the end the compiler's next DPP read
+// (it does not count instructions inside an asm statement as VALU writes).
+__device__ inline void affine_scan_f32_fused(float& A, float& B) {
+    asm volatile(
+        "s_nop 1\n\t"
+        "v_fmac_f32_dpp %0, %0, %1 row_shr:1 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mul_f32_dpp %1, %1, %1 row_shr:1 row_mask:0xf bank_mask:0xf\n\t"
+        "s_nop 0\n\t"
+        "v_fmac_f32_dpp %0, %0, %1 row_shr:2 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mul_f32_dpp %1, %1, %1 row_shr:2 row_mask:0xf bank_mask:0xf\n\t"
+        "s_nop 0\n\t"
+        "v_fmac_f32_dpp %0, %0, %1 row_shr:4 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mul_f32_dpp %1, %1, %1 row_shr:4 row_mask:0xf bank_mask:0xf\n\t"
+        "s_nop 0\n\t"
+        "v_fmac_f32_dpp %0, %0, %1 row_shr:8 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mul_f32_dpp %1, %1, %1 row_shr:8 row_mask:0xf bank_mask:0xf\n\t"
+        "s_nop 0\n\t"
+        "v_fmac_f32_dpp %0, %0, %1 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+        "v_mul_f32_dpp %1, %1, %1 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+        "s_nop 0\n\t"
+        "v_fmac_f32_dpp %0, %0, %1 row_bcast:31 row_mask:0xc bank_mask:0xf\n\t"
+        "v_mul_f32_dpp %1, %1, %1 row_bcast:31 row_mask:0xc bank_mask:0xf\n\t"
+        "s_nop 1"
+        : "+v"(A), "+v"(B));
+}
+template <typename T>
+__device__ inline void affine_scan_dpp(T& A, T& B) {
+    if constexpr (sizeof(T) == 4) affine_scan_f32_fused(A, B);
+    else affine_scan<true>(A, B, 0);
+}
+
+// 1D/2D, one wave per direction, rows in order with the y-face fluxes in registers; the source
+// and material of row y + PD are loaded while row y is solved.  BIN: the x- and y-face inflows
+// come from PassDesc::bin_x / bin_y.
+template <typename T, int CPL, int PD, bool BIN = false>
+__device__ __forceinline__ void sweep_line_fast_body(const PassDesc& d, const int n, unsigned char* smem) {
+    const int ndims = d.ndims, Nx = d.Nx, Ny = d.Ny, Nc = d.Nc, Nmat = d.Nmat;
+    const int lane = threadIdx.x;
+    T* hy2 = (T*)smem;   // [Ny]
+    T* st = hy2 + Ny;    // [Nmat]
+    const double mu = d.mu[n], eta = ndims >= 2 ? d.eta[n] : 0.0;
+    // direction-uniform: the row order and the tangential 1D case (sn_1D_bte.jl:154-155)
+    const int fys = __builtin_amdgcn_readfirstlane(eta >= 0 ? 1 : 0);
+    const int tang = __builtin_amdgcn_readfirstlane((ndims == 1 && fabs(mu) <= 1e-10) ? 1 : 0);
+    const bool fxs = mu >= 0;
+    const double amu = fabs(mu);
+    for (int i = lane; i < Ny; i += 64) hy2[i] = (T)(ndims >= 2 ? 2.0 * (fabs(eta) / d.dy[i]) : 0.0);
+    for (int i = lane; i < Nmat; i += 64) st[i] = (T)d.sigt[i];
+    const T* psi = (const T*)d.psi + (size_t)n * Nc;
+    const int* mat = d.mat;
+    unsigned offT[CPL], off4[CPL];
+    T hx[CPL], hxs[CPL], bias[CPL];
+    int ixc[CPL];
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+        const int jx = lane * CPL + c;
+        const bool in = jx < Nx;
+        const int ix = in ? (fxs ? jx : Nx - 1 - jx) : 0;
+        ixc[c] = in ? ix : -1;
+        offT[c] = in ? (unsigned)ix * (unsigned)sizeof(T) : BUF_OOB;
+        off4[c] = in ? (unsigned)ix * 4u : BUF_OOB;
+        hx[c] = in ? (T)(2.0 * (amu / d.dx[ix])) : (T)0;
+        hxs[c] = in ? hx[c] : (T)1;
+        bias[c] = in ? (T)-1 : (T)1;
+    }
+    const unsigned rowT = (unsigned)Nx * (unsigned)sizeof(T), row4 = (unsigned)Nx * 4u;
+    const unsigned bxoff = lane == (Nx - 1) / CPL ? 0u : BUF_OOB;  // the lane with the row's last cell
+    T* bx = (T*)d.bx + (size_t)n * Ny;
+    const size_t npx = (size_t)Ny * d.Nz, npy = (size_t)Nx * d.Nz;  // face positions (Nz = 1 here)
+    __syncthreads();
+    if (tang) {  // psi = Q / Σt; the x-face flux passes through
+        const auto rp = buf_rsrc(psi, rowT), rm = buf_rsrc(mat, row4);
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            const T v = bld<T>(rp, offT[c]);
+            const int m = bld<int>(rm, off4[c]);
+            bst(rp, offT[c], dd_div(v, st[m]));
+        }
+        const T fx0 = BIN ? (T)gld(d.bin_x + n * npx) : (T)0;
+        bst(buf_rsrc(bx, (unsigned)sizeof(T)), bxoff, fx0);
+        return;
+    }
+    auto row_of = [&](int jy) {  // scalar; rows past the grid map to the last one
+        const int j = jy < Ny ? jy : Ny - 1;
+        return fys ? j : Ny - 1 - j;
+    };
+    T fy[CPL];
+    if constexpr (BIN) {
+        const auto rb = buf_rsrc(d.bin_y + n * npy, (unsigned)Nx * 8u);
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) fy[c] = (T)bld<double>(rb, ixc[c] >= 0 ? (unsigned)ixc[c] * 8u : BUF_OOB);
+    } else {
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) fy[c] = (T)0;
+    }
+    T vr[PD][CPL];
+    int mr[PD][CPL];
+    T xr[PD];  // x-face inflow of the row in each slot (BIN)
+    auto load_row = [&](int i, int jy) __attribute__((always_inline)) {
+        const int iy = row_of(jy);
+        const bool live = jy < Ny;  // rows past the grid load zeros and move no bytes
+        const auto rp = buf_rsrc(psi + (size_t)Nx * iy, live ? rowT : 0u);
+        const auto rm = buf_rsrc(mat + (size_t)Nx * iy, live ? row4 : 0u);
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            vr[i][c] = bld<T>(rp, offT[c]);
+            mr[i][c] = bld<int>(rm, off4[c]);
+        }
+    };
+#pragma unroll
+    for (int i = 0; i < PD; ++i) {
+        __builtin_amdgcn_sched_barrier(0);  // issued in slot order: slot 0 is the oldest load
+        load_row(i, i);
+        if constexpr (BIN) xr[i] = (T)gld(d.bin_x + n * npx + row_of(i));
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    // one row of slot i; rv is false only for the rows past the grid in the last block
+    auto solve_row = [&](int i, int jy, bool rv) __attribute__((always_inline)) {
+        const int iy = row_of(jy);
+        const T hy = hy2[iy];
+        T c0[CPL], c1[CPL], a[CPL], b[CPL];
+        T A = (T)0, B = (T)1;
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            const T S = (st[mr[i][c]] + hxs[c]) + hy;
+            const T rS = dd_div((T)1, S);
+            c0[c] = (vr[i][c] + hy * fy[c]) * rS;
+            c1[c] = hx[c] * rS;
+            a[c] = c0[c] + c0[c];
+            b[c] = (c1[c] + c1[c]) + bias[c];
+            if (c == 0) {
+                A = a[c];
+                B = b[c];
+            } else {
+                A = a[c] + b[c] * A;
+                B = b[c] * B;
+            }
+        }
+        {
+            // row jy + PD goes into the ring slot just freed (kept here: the scheduler would
+            // otherwise sink the loads towards their first use)
+            __builtin_amdgcn_sched_barrier(0);
+            load_row(i, jy + PD);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        affine_scan_dpp<T>(A, B);
+        T fx = dpp_move<0x138, 0xF>((T)0, A);  // wave_shr:1: the exclusive prefix, 0 in lane 0
+        if constexpr (BIN) fx = fx + dpp_move<0x138, 0xF>((T)1, B) * xr[i];
+        const auto rp = buf_rsrc(psi + (size_t)Nx * iy, rv ? rowT : 0u);
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+            const T p = c0[c] + c1[c] * fx;
+            bst(rp, offT[c], p);
+            const T fyn = (p + p) - fy[c];
+            fy[c] = rv ? fyn : fy[c];
+            fx = a[c] + b[c] * fx;
+        }
+        bst(buf_rsrc(bx + iy, rv ? (unsigned)sizeof(T) : 0u), bxoff, fx);
+        if constexpr (BIN) xr[i] = (T)gld(d.bin_x + n * npx + row_of(jy + PD));
+    };
+    int r0 = 0;
+    for (; r0 + PD <= Ny; r0 += PD) {
+#pragma unroll
+        for (int i = 0; i < PD; ++i) solve_row(i, r0 + i, true);
+    }
+    if (r0 < Ny) {
+#pragma unroll
+        for (int i = 0; i < PD; ++i) solve_row(i, r0 + i, r0 + i < Ny);
+    }
+    if (ndims >= 2) {
+        T* by = (T*)d.by + (size_t)n * Nx;
+#pragma unroll
+        for (int c = 0; c < CPL; ++c)
+            if (ixc[c] >= 0) gst(by + ixc[c], fy[c]);  // exit y-face after the last row
+    }
+}
+
 template <typename T, int CPL, int PD, bool DPP, bool BIN = false>
 __global__ __launch_bounds__(64) void sweep_line_kernel(const PassDesc* __restrict__ pds,
                                                         const SweepTask* __restrict__ tasks) {
     extern __shared__ __align__(16) unsigned char smem[];
     const SweepTask tk = tasks[blockIdx.x];
-    sweep_line_body<T, CPL, PD, DPP, BIN>(pds[tk.prob], tk.dir, smem);
+    if constexpr (DPP) sweep_line_fast_body<T, CPL, PD, BIN>(pds[tk.prob], tk.dir, smem);
+    else sweep_line_body<T, CPL, PD, DPP, BIN>(pds[tk.prob], tk.dir, smem);
 }
 
 // FP64 and FP32 passes of different problems in one launch: each workgroup takes the precision of
@@ -1141,8 +1360,13 @@ __global__ __launch_bounds__(64) void sweep_line_mixed_kernel(const PassDesc* __
     extern __shared__ __align__(16) unsigned char smem[];
     const SweepTask tk = tasks[blockIdx.x];
     const PassDesc& d = pds[tk.prob];
-    if (d.prec == P64) sweep_line_body<double, CPL, PD, DPP>(d, tk.dir, smem);
-    else sweep_line_body<float, CPL, PD, DPP>(d, tk.dir, smem);
+    if constexpr (DPP) {
+        if (d.prec == P64) sweep_line_fast_body<double, CPL, PD>(d, tk.dir, smem);
+        else sweep_line_fast_body<float, CPL, PD>(d, tk.dir, smem);
+    } else {
+        if (d.prec == P64) sweep_line_body<double, CPL, PD, DPP>(d, tk.dir, smem);
+        else sweep_line_body<float, CPL, PD, DPP>(d, tk.dir, smem);
+    }
 }
 
 // 3D: one workgroup per direction.  Pencils (jy, jz) advance by diagonals t = jy + jz; the
