@@ -33,9 +33,15 @@ HBM_PEAK_GBS = 8000.0
 # from rocprofv3 --pmc runs over that workload: FETCH_SIZE and WRITE_SIZE (KiB per dispatch,
 # averaged over the kernel's dispatches), each scaled by what the counter reads for a plain 1 GiB
 # device copy (FETCH_SIZE reads 0.5 GiB for it: factor 2.000; WRITE_SIZE 1 GiB: factor 1.000).
-MEASURED_TRAFFIC = {("sweep_fp32", 32): dict(fetch=336.36e6, write=308.34e6,
+MEASURED_TRAFFIC = {("sweep_fp32", 32): dict(fetch=329.06e6, write=309.04e6,
                                              kernel="sweep_line_mixed_kernel<2, 8, true>",
-                                             summary="profiles/sweep_line_mixed_c1_batch32.md")}
+                                             summary="profiles/sweep_line_mixed_c1_batch32.md"),
+                    ("dgemm_fp32", 32): dict(fetch=482.74e6, write=174.70e6,
+                                             kernel="contract_mixed_kernel<4, 4, false, 4>",
+                                             summary="profiles/contractions_mfma_c1_batch32.md"),
+                    ("mgemm_fp32", 32): dict(fetch=142.58e6, write=304.57e6,
+                                             kernel="contract_mixed_kernel<4, 4, true, 2>",
+                                             summary="profiles/contractions_mfma_c1_batch32.md")}
 MFMA_PEAK_TFLOPS = {0: 78.6, 1: 157.3, 2: 2500.0}
 KNAMES = ["mgemm", "sweep", "dgemm", "bndmom"]
 PNAMES = ["fp64", "fp32", "bf16"]
@@ -122,7 +128,7 @@ def run_batch(probs, precision, steps, warmup, label):
         rs = ResidentBatch(probs, precision=precision)
         os.environ["RADIANT_HIP_PROFILE"] = "1"
         _, pinfo = rs.solve()
-        kms, kn, kupd, kfl = rs.kernel_times()
+        kms, kn, kupd, kfl, kby = rs.kernel_times()
         rs.close()
     finally:
         for k, v in saved.items():
@@ -136,7 +142,7 @@ def run_batch(probs, precision, steps, warmup, label):
                solves_per_s=len(probs) * steps / wall, raw_updates_per_s=info.updates_raw * steps / wall,
                passes=int(info.passes), passes_lo=int(info.passes_lo),
                converged=all(all(s["converged"] for s in st) for st in stats),
-               kernel_ms=kms, kernel_launches=kn, kernel_updates=kupd, kernel_flops=kfl,
+               kernel_ms=kms, kernel_launches=kn, kernel_updates=kupd, kernel_flops=kfl, kernel_bytes=kby,
                profiled_ms=pinfo.ms_device, Nd=rb.descs[0].Nd,
                kernel_timing="one extra solve of the same batch as a single sub-batch (kernels run alone)")
     return out, phis
@@ -220,11 +226,15 @@ def kernel_roofline(prob, res, nact):
       sweep : 2*es bytes (angular source in, angular flux out; es = 8 FP64, 4 FP32) -> HBM roofline
       mgemm : 2*K flops with K = active moments; bytes es + 8*K/Nd (angular source out, moments in)
       dgemm : 2*R flops with R = output rows;    bytes es + 16*R/Nd (angular flux in, z in, Az out)
+    For a lockstep batch the library counts the bytes per problem pass exactly instead (the
+    precision of each pass's angular flux; D's input rows only for an I - A pass; FP64 passes in
+    a shared launch once): rt_batch_kernel_times.
     A contraction is reported against HBM when its arithmetic intensity is below the ridge point
     of its MFMA precision, else against MFMA.
     """
     kms, kn = np.asarray(res["kernel_ms"]), np.asarray(res["kernel_launches"])
     upd, fl = kernel_work(prob, res, nact)
+    kby = res.get("kernel_bytes")  # exact per-pass bytes (lockstep batches)
     p, k = np.unravel_index(np.argmax(kms[:, :3]), kms[:, :3].shape)
     t = kms[p][k] / 1e3
     es = 8 if p == 0 else (4 if p == 1 else 2)
@@ -234,12 +244,12 @@ def kernel_roofline(prob, res, nact):
                 share_of_solve=float(kms[p][k] / res["profiled_ms"]),
                 measured_on=res.get("kernel_timing", "one extra solve, event-timed"))
     if k == 1:
-        by = upd[p][k] * 2 * (8 if p == 0 else 4)
+        by = kby[p][k] if kby is not None else upd[p][k] * 2 * (8 if p == 0 else 4)
         ach = by / t / 1e9
         return dict(base, bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS,
                     traffic=None, algorithmic_bytes_per_launch=by / kn[p][k])
     K = fl[p][k] / (2.0 * upd[p][k])
-    by = upd[p][k] * (es + (8.0 * K / Nd if k == 0 else 16.0 * K / Nd))
+    by = upd[p][k] * (es + (8.0 * K / Nd if k == 0 else 16.0 * K / Nd)) if kby is None else kby[p][k]
     ai = fl[p][k] / by
     if ai < MFMA_PEAK_TFLOPS[p] * 1e12 / (HBM_PEAK_GBS * 1e9):
         ach = by / t / 1e9
@@ -257,6 +267,7 @@ def contraction_rooflines(prob, res, nact):
     update as in kernel_roofline), from the same event-timed launches."""
     kms, kn = np.asarray(res["kernel_ms"]), np.asarray(res["kernel_launches"])
     upd, fl = kernel_work(prob, res, nact)
+    kby = res.get("kernel_bytes")
     Nd = res["Nd"]
     out = {}
     for p in range(3):
@@ -266,7 +277,7 @@ def contraction_rooflines(prob, res, nact):
                 continue
             t = kms[p][k] / 1e3
             K = fl[p][k] / (2.0 * upd[p][k])
-            by = upd[p][k] * (es + (8.0 * K / Nd if k == 0 else 16.0 * K / Nd))
+            by = upd[p][k] * (es + (8.0 * K / Nd if k == 0 else 16.0 * K / Nd)) if kby is None else kby[p][k]
             gbs, tfs = by / t / 1e9, fl[p][k] / t / 1e12
             out[f"{KNAMES[k]}_{PNAMES[p]}"] = dict(
                 avg_launch_us=round(float(kms[p][k] / kn[p][k] * 1e3), 2), launches=int(kn[p][k]),

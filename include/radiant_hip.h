@@ -152,6 +152,8 @@ typedef struct rt_solve_info {
     double updates_raw;         /* sum over passes of Nc*Nd */
     double updates_problem;     /* Nc*Nd*G */
     double ms_sweep, ms_mgemm, ms_dgemm, ms_krylov;  /* per-phase device time (profiling mode) */
+    int64_t graph_launches;       /* lockstep rounds run as HIP graph launches (RADIANT_HIP_GRAPHS=1) */
+    int64_t graph_instantiations; /* of which needed a new executable graph (the rest updated one) */
 } rt_solve_info;
 
 /* ------------------------------------------------------------------------------------------
@@ -209,8 +211,11 @@ int rt_batch_destroy(rt_batch *b);
 /* Profiling hook (no counterpart in the reference): per-kernel device time of the last
  * rt_batch_solve run with RADIANT_HIP_PROFILE=1, lockstep batches only.  Arrays of 12 indexed
  * [precision][kernel] as rt_problem_kernel_times; updates[] holds the cell*direction updates the
- * launches processed and flops[] the contraction flops. */
-int rt_batch_kernel_times(rt_batch *b, double *ms, int64_t *launches, double *updates, double *flops);
+ * launches processed, flops[] the contraction flops and bytes[] their algorithmic HBM bytes
+ * (angular flux/source es = 8 or 4 B per value, moments 8 B: M es*Nd*Nc + 8*Nact*Nc, sweep
+ * 2*es*Nd*Nc, D es*Nd*Nc + 8*rows*Nc, twice that last term for an I - A pass, per problem pass). */
+int rt_batch_kernel_times(rt_batch *b, double *ms, int64_t *launches, double *updates, double *flops,
+                          double *bytes);
 
 /* ------------------------------------------------------------------------------------------
  * Serving: continuous batching of fixed-source solves admitted by an HBM budget.

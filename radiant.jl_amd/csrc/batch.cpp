@@ -63,6 +63,9 @@ void BatchSolver::init_device(int n) {
         // sub-batches of a batch already overlap on the device, and measured faster without it.
         const char* f = std::getenv("RADIANT_HIP_FORK");
         fork_streams = f && f[0] == '1';
+        // 1: every round's launches are captured into a HIP graph and run as one graph launch
+        const char* gr = std::getenv("RADIANT_HIP_GRAPHS");
+        use_graphs = gr && gr[0] == '1';
     }
     RT_CHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
     RT_CHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
@@ -194,13 +197,92 @@ T* BatchSolver::stage(const T* data, size_t count) {
 }
 
 void BatchSolver::flush() {
+    if (use_graphs && !profiling && !fork_streams) {
+        flush_graph();
+        return;
+    }
     if (arena_off > 0) RT_CHECK(hipMemcpyAsync(d_arena, h_arena, arena_off, hipMemcpyHostToDevice, st));
     for (auto& f : plan) f();
     plan.clear();
 }
 
-std::function<void()> BatchSolver::timed(int kernel, int prec, double upd, double flop, hipStream_t s,
-                                         std::function<void()> fn) {
+// Graph form of flush(): the arena upload and the recorded launches are captured from the stream
+// into a graph.  Rounds of one shape (same node kinds and kernels in the same order: the same
+// phase mix of the same problems) reuse an executable graph, updated in place with the new
+// launch parameters; any other shape is instantiated and cached.  The captured work has no
+// memset nodes (buffers are cleared by fill kernels) and no events (timing and the autotuner's
+// measurements are off while capturing; tuned or default shapes are used).
+void BatchSolver::flush_graph() {
+    if (arena_off == 0 && plan.empty()) return;
+    const bool meas = autotune_measuring();
+    set_autotune_measuring(false);
+    RT_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    hipGraph_t g = nullptr;
+    try {
+        if (arena_off > 0) RT_CHECK(hipMemcpyAsync(d_arena, h_arena, arena_off, hipMemcpyHostToDevice, st));
+        for (auto& f : plan) f();
+    } catch (...) {
+        (void)hipStreamEndCapture(st, &g);
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        set_autotune_measuring(meas);
+        plan.clear();
+        throw;
+    }
+    set_autotune_measuring(meas);
+    plan.clear();
+    RT_CHECK(hipStreamEndCapture(st, &g));
+    // shape of the captured graph: node kinds, and the kernel of each kernel node, in order
+    size_t nn = 0;
+    RT_CHECK(hipGraphGetNodes(g, nullptr, &nn));
+    std::vector<hipGraphNode_t> nodes(nn);
+    if (nn) RT_CHECK(hipGraphGetNodes(g, nodes.data(), &nn));
+    std::vector<uintptr_t> sig;
+    sig.reserve(2 * nn);
+    for (hipGraphNode_t nd : nodes) {
+        hipGraphNodeType ty;
+        RT_CHECK(hipGraphNodeGetType(nd, &ty));
+        sig.push_back((uintptr_t)ty);
+        if (ty == hipGraphNodeTypeKernel) {
+            hipKernelNodeParams kp{};
+            RT_CHECK(hipGraphKernelNodeGetParams(nd, &kp));
+            sig.push_back((uintptr_t)kp.func);
+        }
+    }
+    hipGraphExec_t ex = nullptr;
+    for (size_t c = 0; c < gcache.size(); ++c) {
+        if (gcache[c].sig != sig) continue;
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult ur = hipGraphExecUpdateError;
+        if (hipGraphExecUpdate(gcache[c].exec, g, &bad, &ur) == hipSuccess && ur == hipGraphExecUpdateSuccess) {
+            ex = gcache[c].exec;
+            ++graph_updates;
+        } else {  // not updatable after all: dropped, instantiated afresh below
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(gcache[c].exec);
+            gcache.erase(gcache.begin() + (long)c);
+        }
+        break;
+    }
+    if (!ex) {
+        hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            throw Error(RT_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+        if (gcache.size() >= GRAPH_CACHE) {
+            (void)hipGraphExecDestroy(gcache.front().exec);
+            gcache.erase(gcache.begin());
+        }
+        gcache.push_back({sig, ex});
+        ++graph_instantiations;
+    }
+    (void)hipGraphDestroy(g);
+    RT_CHECK(hipGraphLaunch(ex, st));
+}
+
+std::function<void()> BatchSolver::timed(int kernel, int prec, double upd, double flop, double bytes,
+                                         hipStream_t s, std::function<void()> fn) {
     if (!profiling) return fn;
     while (ev_pool.size() < ev_next + 2) {
         hipEvent_t e;
@@ -209,7 +291,7 @@ std::function<void()> BatchSolver::timed(int kernel, int prec, double upd, doubl
     }
     hipEvent_t a = ev_pool[ev_next], b = ev_pool[ev_next + 1];
     ev_next += 2;
-    ev_used.push_back({a, b, kernel, prec, upd, flop});
+    ev_used.push_back({a, b, kernel, prec, upd, flop, bytes});
     (void)s;
     return [=] {
         set_launch_events(a, b);  // recorded at the kernel's own dispatch start and end
@@ -226,6 +308,7 @@ void BatchSolver::collect_times() {
         timing.n[r.prec][r.kernel] += 1;
         timing.upd[r.prec][r.kernel] += r.upd;
         timing.flop[r.prec][r.kernel] += r.flop;
+        timing.bytes[r.prec][r.kernel] += r.bytes;
     }
     ev_used.clear();
     ev_next = 0;
@@ -233,6 +316,7 @@ void BatchSolver::collect_times() {
 
 BatchSolver::~BatchSolver() {
     (void)hipStreamSynchronize(st);
+    for (auto& c : gcache) (void)hipGraphExecDestroy(c.exec);
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     for (void* p : task_mem)
         if (p) (void)hipFree(p);
@@ -312,6 +396,11 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
         std::map<std::pair<int, int>, size_t> slds;
         std::map<std::pair<int, int>, double> s_upd;
         double g_upd = 0, g_mflop = 0, g_dflop = 0, d_upd = 0;
+        // algorithmic bytes: M reads the Nact source moments (8 B) and writes the angular source
+        // (es); the sweep reads and writes the angular flux; D reads it, writes its output rows
+        // and, for an I - A pass, reads the input state's rows
+        double g_mby = 0, d_by = 0;
+        std::map<std::pair<int, int>, double> s_by;
         for (size_t i = 0; i < who.size(); ++i) {
             if (group_of(prec[i]) != pr) continue;
             const Problem* P = ps[who[i]].P;
@@ -324,6 +413,8 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
             const double upd = (double)P->Nc * P->Nd * ((pr == PMIX && prec[i] == P64) ? 2.0 : 1.0);
             d_upd += upd;
             g_dflop += 2.0 * rows * upd;
+            const double es = prec[i] == P64 ? 8.0 : 4.0, ndc = (double)P->Nd * P->Nc;
+            d_by += es * ndc + 8.0 * rows * (double)P->Nc * (pds[i].zin ? 2.0 : 1.0);
             if (full[i] == 2) continue;  // contraction only: no moment->direction contraction, no sweep
             mmeta.insert(mmeta.end(), {who[i], ntile, nrg_m, moff});
             moff += ntile * nrg_m;
@@ -334,6 +425,8 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
             g_upd += upd;
             g_mflop += 2.0 * P->Nact * upd;
             s_upd[key] += upd;
+            g_mby += es * ndc + 8.0 * P->Nact * (double)P->Nc;
+            s_by[key] += 2.0 * es * ndc;
             const int sp = pr == PBF16 ? (int)P32 : (pr == PMIX ? (int)P64 : pr);
             slds[key] = std::max(slds[key], sweep_scan_lds_bytes(sp, P->ndims, P->Nx, P->Ny, P->Nz, P->Nmat));
         }
@@ -350,7 +443,7 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
             if (nm) launch_expand_gemm(dmm, nm, mg, s);
             launch_expand_gemm(ddm, nd, dg, s);
         });
-        if (nm) plan.push_back(timed(0, tix, g_upd, g_mflop, s, [=] { launch_mgemm(ps_, dpd, mg, moff, s); }));
+        if (nm) plan.push_back(timed(0, tix, g_upd, g_mflop, g_mby, s, [=] { launch_mgemm(ps_, dpd, mg, moff, s); }));
         for (auto& kv : sweeps) {
             const std::vector<int>& sm = kv.second;
             const int* dsm = stage(sm.data(), sm.size());
@@ -360,12 +453,12 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
             const size_t lds = slds[kv.first];
             SweepTask* sb = swbase;
             plan.push_back([=] { launch_expand_sweep(dsm, nsm, sb, s); });
-            plan.push_back(timed(1, tix, s_upd[kv.first], 0.0, s, [=] {
+            plan.push_back(timed(1, tix, s_upd[kv.first], 0.0, s_by[kv.first], s, [=] {
                 launch_sweep_scan(ps_ == PBF16 ? P32 : ps_, dpd, sb, ntask, cpl, waves, lds, s);
             }));
             swbase += ntask;
         }
-        plan.push_back(timed(2, tix, d_upd, g_dflop, s, [=] { launch_dgemm(ps_, dpd, dg, doff, s); }));
+        plan.push_back(timed(2, tix, d_upd, g_dflop, d_by, s, [=] { launch_dgemm(ps_, dpd, dg, doff, s); }));
     }
     if (fork) {
         hipStream_t s0 = st, s1 = st64;
@@ -532,6 +625,7 @@ void BatchSolver::run(rt_solve_info* info) {
     RT_CHECK(hipEventRecord(e0, st));
     pre.clear();
     rounds = 0;
+    const int64_t graph0 = graph_updates + graph_instantiations, inst0 = graph_instantiations;
     host_plan = host_submit = host_wait = host_advance = 0;
     while (round()) {
     }
@@ -557,6 +651,8 @@ void BatchSolver::run(rt_solve_info* info) {
             info->updates_raw += s.P->counters.updates_raw;
             info->updates_problem += (double)s.P->Ncell * s.P->Nd * s.P->G;
         }
+        info->graph_launches = graph_updates + graph_instantiations - graph0;
+        info->graph_instantiations = graph_instantiations - inst0;
     }
 }
 

@@ -31,9 +31,9 @@ public:
     int rounds = 0;
     // Per-kernel device time of the last run() when RADIANT_HIP_PROFILE=1 (HIP events on the stream
     // each launch runs on), [precision][mgemm, sweep, dgemm, -], with the work the launches did:
-    // cell*direction updates and contraction flops.
+    // cell*direction updates, contraction flops and algorithmic HBM bytes.
     struct Timing {
-        double ms[3][4] = {}, upd[3][4] = {}, flop[3][4] = {};
+        double ms[3][4] = {}, upd[3][4] = {}, flop[3][4] = {}, bytes[3][4] = {};
         int64_t n[3][4] = {};
     };
     Timing timing;
@@ -78,6 +78,20 @@ private:
     std::vector<std::function<void()>> plan;
     template <class T> T* stage(const T* data, size_t count);
     void flush();
+    // RADIANT_HIP_GRAPHS=1: rounds run as captured HIP graphs, executable graphs cached by shape
+    bool use_graphs = false;
+    struct GraphEntry {
+        std::vector<uintptr_t> sig;
+        hipGraphExec_t exec;
+    };
+    static constexpr size_t GRAPH_CACHE = 32;
+    std::vector<GraphEntry> gcache;
+    void flush_graph();
+
+public:
+    int64_t graph_updates = 0, graph_instantiations = 0;  // rounds run by updating a cached graph / new ones
+
+private:
 
     void launch_passes(const std::vector<int>& who, const std::vector<PassDesc>& pds, const std::vector<int>& prec,
                        const std::vector<int>& full);
@@ -89,12 +103,13 @@ private:
 
     double host_plan = 0, host_submit = 0, host_wait = 0, host_advance = 0;  // RADIANT_HIP_HOSTPROF
     bool profiling = false;
-    struct EvRec { hipEvent_t a, b; int kernel, prec; double upd, flop; };
+    struct EvRec { hipEvent_t a, b; int kernel, prec; double upd, flop, bytes; };
     std::vector<hipEvent_t> ev_pool;
     size_t ev_next = 0;
     std::vector<EvRec> ev_used;
     // wraps a launch closure with start/stop events when profiling
-    std::function<void()> timed(int kernel, int prec, double upd, double flop, hipStream_t s, std::function<void()> fn);
+    std::function<void()> timed(int kernel, int prec, double upd, double flop, double bytes, hipStream_t s,
+                                std::function<void()> fn);
     void collect_times();  // after the end-of-round synchronisation
 };
 

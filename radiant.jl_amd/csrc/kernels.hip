@@ -192,6 +192,7 @@ void tune_commit(const TuneTicket& t) {
 }  // namespace
 
 void set_autotune_measuring(bool on) { tl_tune_measure = on; }
+bool autotune_measuring() { return tl_tune_measure; }
 
 void set_autotune_mode(int mode) {
     std::lock_guard<std::mutex> lk(g_tune_mu);

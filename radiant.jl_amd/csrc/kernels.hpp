@@ -97,6 +97,7 @@ void set_autotune_mode(int mode);
 // launches run beside other streams' work (concurrent sub-batches, served requests) turn it off
 // and use the shapes tuned elsewhere or the defaults.
 void set_autotune_measuring(bool on);
+bool autotune_measuring();
 std::string autotune_report();
 
 // Work tiles of the contraction kernels: x = problem, y = 64-cell tile, z = row group.

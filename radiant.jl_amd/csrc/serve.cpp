@@ -184,6 +184,8 @@ static void solve_batch(rt_batch* b, rt_solve_info* info) {
                 info->passes_lo += x.passes_lo;
                 info->updates_raw += x.updates_raw;
                 info->updates_problem += x.updates_problem;
+                info->graph_launches += x.graph_launches;
+                info->graph_instantiations += x.graph_instantiations;
             }
         }
         return;
@@ -223,6 +225,8 @@ static void solve_batch(rt_batch* b, rt_solve_info* info) {
             info->passes_lo += x.passes_lo;
             info->updates_raw += x.updates_raw;
             info->updates_problem += x.updates_problem;
+            info->graph_launches += x.graph_launches;
+            info->graph_instantiations += x.graph_instantiations;
         }
     }
 }
@@ -273,25 +277,28 @@ int rt_batch_get_flux(rt_batch* b, int32_t i, double* phi_out) {
     return guard([&] { b->probs[i]->get_flux(phi_out); });
 }
 
-int rt_batch_kernel_times(rt_batch* b, double* ms, int64_t* launches, double* updates, double* flops) {
+int rt_batch_kernel_times(rt_batch* b, double* ms, int64_t* launches, double* updates, double* flops,
+                          double* bytes) {
     if (!b || b->bss.empty()) {
         rt::set_last_error("kernel times are recorded by lockstep (mixed-precision) batches only");
         return RT_EINVAL;
     }
     for (int p = 0; p < 3; ++p)
         for (int k = 0; k < 4; ++k) {
-            double t_ms = 0, t_upd = 0, t_fl = 0;
+            double t_ms = 0, t_upd = 0, t_fl = 0, t_by = 0;
             int64_t t_n = 0;
             for (auto& bs : b->bss) {
                 t_ms += bs->timing.ms[p][k];
                 t_n += bs->timing.n[p][k];
                 t_upd += bs->timing.upd[p][k];
                 t_fl += bs->timing.flop[p][k];
+                t_by += bs->timing.bytes[p][k];
             }
             if (ms) ms[4 * p + k] = t_ms;
             if (launches) launches[4 * p + k] = t_n;
             if (updates) updates[4 * p + k] = t_upd;
             if (flops) flops[4 * p + k] = t_fl;
+            if (bytes) bytes[4 * p + k] = t_by;
         }
     return RT_OK;
 }

@@ -57,7 +57,8 @@ class GroupStats(C.Structure):
 class SolveInfo(C.Structure):
     _fields_ = [("ms_device", C.c_double), ("passes", C.c_int64), ("passes_lo", C.c_int64),
                 ("updates_raw", C.c_double), ("updates_problem", C.c_double), ("ms_sweep", C.c_double),
-                ("ms_mgemm", C.c_double), ("ms_dgemm", C.c_double), ("ms_krylov", C.c_double)]
+                ("ms_mgemm", C.c_double), ("ms_dgemm", C.c_double), ("ms_krylov", C.c_double),
+                ("graph_launches", C.c_int64), ("graph_instantiations", C.c_int64)]
 
 
 class ServerConfig(C.Structure):
@@ -102,7 +103,7 @@ def lib():
     L.rt_batch_solve.argtypes = [H, C.POINTER(GS), C.POINTER(SolveInfo)]
     L.rt_batch_get_flux.argtypes = [H, C.c_int32, dp]
     L.rt_batch_destroy.argtypes = [H]
-    L.rt_batch_kernel_times.argtypes = [H, dp, lp, dp, dp]
+    L.rt_batch_kernel_times.argtypes = [H, dp, lp, dp, dp, dp]
     L.rt_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(H)]
     L.rt_server_submit.argtypes = [H, PD, lp]
     L.rt_server_poll.argtypes = [H, C.c_int64, dp, GS, dp]

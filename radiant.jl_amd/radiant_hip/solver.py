@@ -217,12 +217,12 @@ class ResidentBatch:
         return [[s.as_dict() for s in st] for st in stats], info
 
     def kernel_times(self):
-        """Per-kernel (ms, launches, updates, flops) of the last solve, each [precision][kernel]
-        (RADIANT_HIP_PROFILE=1; lockstep batches)."""
-        ms, upd, fl = np.zeros(12), np.zeros(12), np.zeros(12)
+        """Per-kernel (ms, launches, updates, flops, algorithmic bytes) of the last solve, each
+        [precision][kernel] (RADIANT_HIP_PROFILE=1; lockstep batches)."""
+        ms, upd, fl, by = np.zeros(12), np.zeros(12), np.zeros(12), np.zeros(12)
         n = np.zeros(12, np.int64)
-        check(lib().rt_batch_kernel_times(self.h, ptr(ms), ptr(n, _lib.lp), ptr(upd), ptr(fl)))
-        return ms.reshape(3, 4), n.reshape(3, 4), upd.reshape(3, 4), fl.reshape(3, 4)
+        check(lib().rt_batch_kernel_times(self.h, ptr(ms), ptr(n, _lib.lp), ptr(upd), ptr(fl), ptr(by)))
+        return ms.reshape(3, 4), n.reshape(3, 4), upd.reshape(3, 4), fl.reshape(3, 4), by.reshape(3, 4)
 
     def flux(self, i):
         d = self.descs[i]
