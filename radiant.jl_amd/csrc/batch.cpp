@@ -135,7 +135,7 @@ int BatchSolver::add(Problem* P) {
     s.P = P;
     s.m = std::min(std::max(1, P->restart), 32);
     s.lo = (P->precision == RT_PREC_MIXED_BF16) ? PBF16 : P32;
-    s.inner_tol = (s.lo == PBF16) ? 3e-3 : 1e-5;
+    s.inner_tol = inner_tol_for(s.lo);
     const size_t S = P->S_r;
     for (int k = 0; k <= s.m; ++k) s.V.push_back(P->vec("bV" + std::to_string(k), S));
     s.w = P->vec("bw", S);

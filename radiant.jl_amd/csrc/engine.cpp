@@ -17,6 +17,16 @@ namespace rt {
 
 static inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
+double inner_tol_for(int prec) {
+    if (prec == PBF16) return 3e-3;
+    static const double t = [] {
+        const char* e = std::getenv("RADIANT_HIP_INNER_TOL");
+        const double v = e ? std::atof(e) : 0.0;
+        return (v > 0.0 && v < 1.0) ? v : 1e-5;
+    }();
+    return t;
+}
+
 template <class T>
 T* Problem::dalloc(size_t n) {
     void* p = nullptr;
@@ -1069,7 +1079,7 @@ void Problem::solve_group_plain(int g, rt_group_stats& s) {
 void Problem::solve_group_mixed(int g, rt_group_stats& s) {
     const size_t n = S_r;
     const int lo = (precision == RT_PREC_MIXED_BF16) ? PBF16 : P32;
-    const double inner_tol = (lo == PBF16) ? 3e-3 : 1e-5;
+    const double inner_tol = inner_tol_for(lo);
     const int max_ref = 12;
     const int64_t p0 = counters.passes, l0 = counters.passes_lo;
     double* c = vec("c", n);

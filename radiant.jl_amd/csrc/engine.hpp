@@ -28,6 +28,10 @@ struct Error : std::runtime_error {
 // Message returned by rt_last_error() on the calling thread.
 void set_last_error(const std::string& m);
 
+// Relative residual at which the low-precision inner GMRES of an iterative-refinement step stops
+// (FP32: 1e-5, BF16 operators: 3e-3); RADIANT_HIP_INNER_TOL overrides the FP32 value.
+double inner_tol_for(int prec);
+
 // Runs one contraction kernel in isolation (test hook behind rt_test_contraction).
 void test_contraction(int which, int prec, int M, int K, int N, const double* A, const double* B, double* C);
 
