@@ -297,27 +297,33 @@ __device__ inline void epilogue_dgemm(const PassDesc& d, const V (&acc)[RT][CT],
     const double* __restrict__ zb = hasz ? d.zin : g_zero;
     const size_t zrs = hasz ? (size_t)Nc : 0;
     const double zs = hasz ? 1.0 : 0.0, sg = hasz ? -1.0 : 1.0;
-    double z[RT][CT][4];
-#pragma unroll
-    for (int t = 0; t < RT; ++t)
+    // one row tile at a time (the zin values of a whole wave tile would hold 2*RT*CT*4 VGPRs
+    // and halve the waves a SIMD can hold); the loads of tile t + 1 are issued before the
+    // stores of tile t
+    double z[2][CT][4];
+    auto zload = [&](int t, double (&zt)[CT][4]) {
 #pragma unroll
         for (int u = 0; u < CT; ++u)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int col = c0 + 16 * u + li, row = r0 + 16 * t + acc_row<T>(lane, r);
                 const int cc = col < Nc ? col : Nc - 1, rc = row < M ? row : M - 1;
-                z[t][u][r] = gld(zb + (size_t)rc * zrs + (hasz ? cc : li));
+                zt[u][r] = gld(zb + (size_t)rc * zrs + (hasz ? cc : li));
             }
+    };
+    zload(0, z[0]);
 #pragma unroll
-    for (int t = 0; t < RT; ++t)
+    for (int t = 0; t < RT; ++t) {
+        if (t + 1 < RT) zload(t + 1, z[(t + 1) & 1]);
 #pragma unroll
         for (int u = 0; u < CT; ++u)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int col = c0 + 16 * u + li, row = r0 + 16 * t + acc_row<T>(lane, r);
-                const double v = zs * z[t][u][r] + sg * (double)acc[t][u][r];
+                const double v = zs * z[t & 1][u][r] + sg * (double)acc[t][u][r];
                 if (col < Nc && row < M) gst(d.out + (size_t)row * Nc + col, v);
             }
+    }
 }
 
 template <typename T, int RT, int CT, bool MG, int NS>
@@ -446,7 +452,7 @@ __device__ __forceinline__ void contract_body(const PassDesc& d, const GemmTask 
 }
 
 template <typename T, int RT, int CT, bool MG, int NS>
-__global__ __launch_bounds__(256) void contract_kernel(const PassDesc* __restrict__ pds, const GemmTask* __restrict__ tasks) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void contract_kernel(const PassDesc* __restrict__ pds, const GemmTask* __restrict__ tasks) {
     __shared__ int rows_s[MG ? CONTRACT_KMAX : 1];
     const GemmTask tk = tasks[blockIdx.x];
     contract_body<T, RT, CT, MG, NS>(pds[tk.prob], tk, rows_s);
@@ -454,12 +460,14 @@ __global__ __launch_bounds__(256) void contract_kernel(const PassDesc* __restric
 
 // FP64 and FP32 passes of different problems in one launch (PassDesc::prec per workgroup).
 template <int RT, int CT, bool MG, int NS>
-__global__ __launch_bounds__(256) void contract_mixed_kernel(const PassDesc* __restrict__ pds,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void contract_mixed_kernel(const PassDesc* __restrict__ pds,
                                                              const GemmTask* __restrict__ tasks) {
     __shared__ int rows_s[MG ? CONTRACT_KMAX : 1];
     const GemmTask tk = tasks[blockIdx.x];
     const PassDesc& d = pds[tk.prob];
-    if (d.prec == P64) contract_body<double, RT, CT, MG, NS>(d, tk, rows_s);
+    // the FP64 body keeps at most 2 stages in flight: with 4, its registers (FP64 operands and
+    // accumulators) would leave room for one wave per SIMD for the FP32 passes of the launch too
+    if (d.prec == P64) contract_body<double, RT, CT, MG, (NS > 2 ? 2 : NS)>(d, tk, rows_s);
     else contract_body<float, RT, CT, MG, NS>(d, tk, rows_s);
 }
 
