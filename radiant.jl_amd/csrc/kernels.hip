@@ -1200,11 +1200,58 @@ __device__ inline void affine_scan_f32_fused(float& A, float& B) {
         "s_nop 1"
         : "+v"(A), "+v"(B));
 }
+// One row_shr step of the FP64 scan.  There is no 64-bit DPP ALU form for row_shr, so the
+// halves move separately; a lane whose source is out of range reads 0 (bound_ctrl), which is
+// the identity of A's term and the low word of B's identity 1.0, so only B's high word needs an
+// old value (0x3ff00000).
+template <int CTRL>
+__device__ inline void dpp_compose_shr_f64(double& A, double& B) {
+    const long long a = __double_as_longlong(A), b = __double_as_longlong(B);
+    const int alo = __builtin_amdgcn_mov_dpp((int)(a & 0xffffffff), CTRL, 0xF, 0xF, true);
+    const int ahi = __builtin_amdgcn_mov_dpp((int)(a >> 32), CTRL, 0xF, 0xF, true);
+    const int blo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffff), CTRL, 0xF, 0xF, true);
+    const int bhi = __builtin_amdgcn_update_dpp(0x3ff00000, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    const double As = __longlong_as_double(((long long)ahi << 32) | (unsigned)alo);
+    const double Bs = __longlong_as_double(((long long)bhi << 32) | (unsigned)blo);
+    A = A + B * As;
+    B = B * Bs;
+}
 template <typename T>
 __device__ inline void affine_scan_dpp(T& A, T& B) {
-    if constexpr (sizeof(T) == 4) affine_scan_f32_fused(A, B);
-    else affine_scan<true>(A, B, 0);
+    if constexpr (sizeof(T) == 4) {
+        affine_scan_f32_fused(A, B);
+    } else {
+        dpp_compose_shr_f64<0x111>(A, B);  // row_shr:1
+        dpp_compose_shr_f64<0x112>(A, B);  // row_shr:2
+        dpp_compose_shr_f64<0x114>(A, B);  // row_shr:4
+        dpp_compose_shr_f64<0x118>(A, B);  // row_shr:8
+        dpp_compose<0x142, 0xA>(A, B);     // row_bcast:15 into rows 1 and 3
+        dpp_compose<0x143, 0xC>(A, B);     // row_bcast:31 into rows 2 and 3
+    }
 }
+// wave_shr:1 with lane 0 reading 0: the exclusive prefix of A
+template <typename T>
+__device__ inline T wave_shr_zero(T v) {
+    if constexpr (sizeof(T) == 4) {
+        return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xF, 0xF, true));
+    } else {
+        const long long a = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_mov_dpp((int)(a & 0xffffffff), 0x138, 0xF, 0xF, true);
+        const int hi = __builtin_amdgcn_mov_dpp((int)(a >> 32), 0x138, 0xF, 0xF, true);
+        return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+    }
+}
+// 1 / s for the FP64 line sweep: the hardware reciprocal (about 2^-23 relative) refined by two
+// Newton steps, r <- r + r (1 - s r), each doubling the correct bits; within about 1 ulp of the
+// correctly rounded quotient, in 5 instructions instead of the 10 of an IEEE division.
+__device__ inline double rcp_nr(double s) {
+    double r = __builtin_amdgcn_rcp(s);
+    double e = fma(-s, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-s, r, 1.0);
+    return fma(r, e, r);
+}
+__device__ inline float rcp_nr(float s) { return __builtin_amdgcn_rcpf(s); }
 
 // 1D/2D, one wave per direction, rows in order with the y-face fluxes in registers; the source
 // and material of row y + PD are loaded while row y is solved.  BIN: the x- and y-face inflows
@@ -1300,7 +1347,7 @@ __device__ __forceinline__ void sweep_line_fast_body(const PassDesc& d, const in
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
             const T S = (st[mr[i][c]] + hxs[c]) + hy;
-            const T rS = dd_div((T)1, S);
+            const T rS = rcp_nr(S);
             c0[c] = (vr[i][c] + hy * fy[c]) * rS;
             c1[c] = hx[c] * rS;
             a[c] = c0[c] + c0[c];
@@ -1321,7 +1368,7 @@ __device__ __forceinline__ void sweep_line_fast_body(const PassDesc& d, const in
             __builtin_amdgcn_sched_barrier(0);
         }
         affine_scan_dpp<T>(A, B);
-        T fx = dpp_move<0x138, 0xF>((T)0, A);  // wave_shr:1: the exclusive prefix, 0 in lane 0
+        T fx = wave_shr_zero(A);  // the exclusive prefix, 0 in lane 0
         if constexpr (BIN) fx = fx + dpp_move<0x138, 0xF>((T)1, B) * xr[i];
         const auto rp = buf_rsrc(psi + (size_t)Nx * iy, rv ? rowT : 0u);
 #pragma unroll
