@@ -440,6 +440,9 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
         for (int pr = 0; pr < 2; ++pr)
             planes_ok[pr] = ndims == 3 && sweep_planes_shape(pr, Ny, Nz, pl_R[pr], pl_ZW[pr], pl_waves[pr]);
         scan_cpl = sweep_scan_cpl(Nx);
+        // the 1D/2D row-scan kernel addresses a direction's angular flux through one buffer
+        // resource, whose byte count is 32-bit
+        if (ndims <= 2 && (uint64_t)Nc * 8u > 0xffffffffull) scan_cpl = 0;
         scan_waves = sweep_scan_waves(ndims, Ny, Nz);
         // RADIANT_HIP_SWEEP selects another sweep kernel for testing: s = scan (default), r = wave
         // rows / planes, g = pencil kernel

@@ -1169,6 +1169,25 @@ __device__ inline void bst(__amdgpu_buffer_rsrc_t r, unsigned off, float v) {
 __device__ inline void bst(__amdgpu_buffer_rsrc_t r, unsigned off, double v) {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(rt_u32x2, v), r, (int)off, 0, 0);
 }
+// With a scalar (wave-uniform) byte offset soff added to the address.  Every access made this
+// way keeps voff + soff below num_records when it is meant to land and puts voff at BUF_OOB
+// when it is not, so it is masked the same whether or not the range check includes soff.
+template <typename T> __device__ inline T bld_s(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff);
+template <> __device__ inline float bld_s<float>(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, (int)soff, 0));
+}
+template <> __device__ inline int bld_s<int>(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff) {
+    return (int)__builtin_amdgcn_raw_buffer_load_b32(r, (int)off, (int)soff, 0);
+}
+template <> __device__ inline double bld_s<double>(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, (int)soff, 0));
+}
+__device__ inline void bst_s(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)off, (int)soff, 0);
+}
+__device__ inline void bst_s(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff, double v) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(rt_u32x2, v), r, (int)off, (int)soff, 0);
+}
 
 // Inclusive affine scan of one FP32 map per lane with the cross-lane read fused into the
 // arithmetic (v_fmac_f32_dpp: A += A[src] * B; v_mul_f32_dpp: B *= B[src]).  bound_ctrl is off,
@@ -1320,15 +1339,19 @@ __device__ __forceinline__ void sweep_line_fast_body(const PassDesc& d, const in
     T vr[PD][CPL];
     int mr[PD][CPL];
     T xr[PD];  // x-face inflow of the row in each slot (BIN)
+    // one resource per array for the whole direction (a row is a scalar byte offset), and one
+    // with num_records 0 for the rows past the grid, which load zeros and move no bytes
+    const auto rpsi = buf_rsrc(psi, (unsigned)Nc * (unsigned)sizeof(T)), rpsi0 = buf_rsrc(psi, 0u);
+    const auto rmat = buf_rsrc(mat, (unsigned)Nc * 4u), rmat0 = buf_rsrc(mat, 0u);
+    const auto rbx = buf_rsrc(bx, (unsigned)Ny * (unsigned)sizeof(T)), rbx0 = buf_rsrc(bx, 0u);
     auto load_row = [&](int i, int jy) __attribute__((always_inline)) {
         const int iy = row_of(jy);
-        const bool live = jy < Ny;  // rows past the grid load zeros and move no bytes
-        const auto rp = buf_rsrc(psi + (size_t)Nx * iy, live ? rowT : 0u);
-        const auto rm = buf_rsrc(mat + (size_t)Nx * iy, live ? row4 : 0u);
+        const bool live = jy < Ny;
+        const unsigned so = (unsigned)(Nx * iy);
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
-            vr[i][c] = bld<T>(rp, offT[c]);
-            mr[i][c] = bld<int>(rm, off4[c]);
+            vr[i][c] = bld_s<T>(live ? rpsi : rpsi0, offT[c], so * (unsigned)sizeof(T));
+            mr[i][c] = bld_s<int>(live ? rmat : rmat0, off4[c], so * 4u);
         }
     };
 #pragma unroll
@@ -1370,16 +1393,16 @@ __device__ __forceinline__ void sweep_line_fast_body(const PassDesc& d, const in
         affine_scan_dpp<T>(A, B);
         T fx = wave_shr_zero(A);  // the exclusive prefix, 0 in lane 0
         if constexpr (BIN) fx = fx + dpp_move<0x138, 0xF>((T)1, B) * xr[i];
-        const auto rp = buf_rsrc(psi + (size_t)Nx * iy, rv ? rowT : 0u);
+        const unsigned so = (unsigned)(Nx * iy) * (unsigned)sizeof(T);
 #pragma unroll
         for (int c = 0; c < CPL; ++c) {
             const T p = c0[c] + c1[c] * fx;
-            bst(rp, offT[c], p);
+            bst_s(rv ? rpsi : rpsi0, offT[c], so, p);
             const T fyn = (p + p) - fy[c];
             fy[c] = rv ? fyn : fy[c];
             fx = a[c] + b[c] * fx;
         }
-        bst(buf_rsrc(bx + iy, rv ? (unsigned)sizeof(T) : 0u), bxoff, fx);
+        bst_s(rv ? rbx : rbx0, bxoff, (unsigned)iy * (unsigned)sizeof(T), fx);
         if constexpr (BIN) xr[i] = (T)gld(d.bin_x + n * npx + row_of(jy + PD));
     };
     int r0 = 0;
