@@ -14,6 +14,7 @@
 #include "batch.hpp"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -121,8 +122,7 @@ int BatchSolver::add(Problem* P) {
         throw Error(RT_EINVAL, "a batched problem does not fit the row-scan sweep");
     if (P->csd) throw Error(RT_EINVAL, "continuous slowing down is solved one problem at a time");
     if (P->ho_K) throw Error(RT_EINVAL, "general 1D closures are solved one problem at a time");
-    if (P->has_bc || P->has_ss)
-        throw Error(RT_EINVAL, "non-void faces and surface sources are solved one problem at a time");
+    if (P->fp) throw Error(RT_EINVAL, "the angular Fokker-Planck term is solved one problem at a time");
     int slot = -1;
     for (int i = 0; i < (int)ps.size(); ++i)
         if (!ps[i].P) {
@@ -379,6 +379,24 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
     // FP32 row of the kernel timing, with FP64 updates counted twice (FP32-equivalent bytes).
     const bool mix = has64 && has32 && !fork;
     auto group_of = [&](int p) { return (mix && (p == P64 || p == P32)) ? (int)PMIX : p; };
+    // Non-void faces and surface sources: each such problem's incoming face fluxes are formed
+    // from its state's boundary moments (and sources) before the sweeps, and after the last D
+    // contraction one launch writes every problem's exit-face moments.
+    bool any_bout = false;
+    int max_nb = 0;
+    for (size_t i = 0; i < who.size(); ++i) {
+        const Problem* P = ps[who[i]].P;
+        if (pds[i].bin_x && full[i] != 2) {
+            const PassDesc* one = dpd + who[i];
+            const int tot = (int)P->bin_total;
+            hipStream_t s0 = st;
+            plan.push_back([=] { launch_bndin(one, tot, s0); });
+        }
+        if (pds[i].bout_x) {
+            any_bout = true;
+            max_nb = std::max(max_nb, (int)P->nb);
+        }
+    }
     if (fork) {
         hipStream_t s0 = st, s1 = st64;
         hipEvent_t ef = ev_fork;
@@ -390,17 +408,20 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
     for (int pr : {(int)PMIX, (int)P64, (int)P32, (int)PBF16}) {
         const int tix = pr == PMIX ? (int)P32 : pr;  // task tables and timing row of the group
         std::vector<int> mmeta, dmeta;
-        std::map<std::pair<int, int>, std::vector<int>> sweeps;  // (cpl, waves) -> sweep meta
+        // (cells per lane, waves, precision of an inflow-taking sweep or -1) -> sweep meta: the
+        // sweeps that take boundary inflows run in their own launches, one per precision
+        using SKey = std::array<int, 3>;
+        std::map<SKey, std::vector<int>> sweeps;
         int moff = 0, doff = 0;
-        std::map<std::pair<int, int>, int> soff;
-        std::map<std::pair<int, int>, size_t> slds;
-        std::map<std::pair<int, int>, double> s_upd;
+        std::map<SKey, int> soff;
+        std::map<SKey, size_t> slds;
+        std::map<SKey, double> s_upd;
         double g_upd = 0, g_mflop = 0, g_dflop = 0, d_upd = 0;
         // algorithmic bytes: M reads the Nact source moments (8 B) and writes the angular source
         // (es); the sweep reads and writes the angular flux; D reads it, writes its output rows
         // and, for an I - A pass, reads the input state's rows
         double g_mby = 0, d_by = 0;
-        std::map<std::pair<int, int>, double> s_by;
+        std::map<SKey, double> s_by;
         for (size_t i = 0; i < who.size(); ++i) {
             if (group_of(prec[i]) != pr) continue;
             const Problem* P = ps[who[i]].P;
@@ -418,7 +439,8 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
             if (full[i] == 2) continue;  // contraction only: no moment->direction contraction, no sweep
             mmeta.insert(mmeta.end(), {who[i], ntile, nrg_m, moff});
             moff += ntile * nrg_m;
-            const std::pair<int, int> key(P->scan_cpl, P->scan_waves);
+            const int bprec = pds[i].bin_x ? (prec[i] == P64 ? (int)P64 : (int)P32) : -1;
+            const SKey key{P->scan_cpl, P->scan_waves, bprec};
             auto& v = sweeps[key];
             v.insert(v.end(), {who[i], P->Nd, soff[key]});
             soff[key] += P->Nd;
@@ -427,7 +449,7 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
             s_upd[key] += upd;
             g_mby += es * ndc + 8.0 * P->Nact * (double)P->Nc;
             s_by[key] += 2.0 * es * ndc;
-            const int sp = pr == PBF16 ? (int)P32 : (pr == PMIX ? (int)P64 : pr);
+            const int sp = bprec >= 0 ? bprec : (pr == PBF16 ? (int)P32 : (pr == PMIX ? (int)P64 : pr));
             slds[key] = std::max(slds[key], sweep_scan_lds_bytes(sp, P->ndims, P->Nx, P->Ny, P->Nz, P->Nmat));
         }
         if (dmeta.empty()) continue;
@@ -449,12 +471,13 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
             const int* dsm = stage(sm.data(), sm.size());
             const int nsm = (int)sm.size() / 3;
             const int ntask = soff[kv.first];
-            const int cpl = kv.first.first, waves = kv.first.second;
+            const int cpl = kv.first[0], waves = kv.first[1], bprec = kv.first[2];
             const size_t lds = slds[kv.first];
             SweepTask* sb = swbase;
             plan.push_back([=] { launch_expand_sweep(dsm, nsm, sb, s); });
             plan.push_back(timed(1, tix, s_upd[kv.first], 0.0, s_by[kv.first], s, [=] {
-                launch_sweep_scan(ps_ == PBF16 ? P32 : ps_, dpd, sb, ntask, cpl, waves, lds, s);
+                const int kind = bprec >= 0 ? bprec : (ps_ == PBF16 ? (int)P32 : ps_);
+                launch_sweep_scan(kind, dpd, sb, ntask, cpl, waves, lds, s, bprec >= 0);
             }));
             swbase += ntask;
         }
@@ -467,6 +490,11 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
             RT_CHECK(hipEventRecord(ej, s1));
             RT_CHECK(hipStreamWaitEvent(s0, ej, 0));
         });
+    }
+    if (any_bout) {
+        const int nall = n;
+        hipStream_t s0 = st;
+        plan.push_back([=] { launch_bndmom(dpd, nall, max_nb, s0); });
     }
     for (size_t i = 0; i < who.size(); ++i) {
         Problem* P = ps[who[i]].P;

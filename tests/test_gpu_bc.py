@@ -136,7 +136,7 @@ def test_bc_bf16_and_batch(oracle):
     e = rel_err(phi, ref)
     _say("bc mixed bf16", e)
     assert e < 1e-6
-    # a lockstep batch hands problems with non-void faces to its per-problem path
+    # in a lockstep batch (boundary moments in the reduced state)
     phis, _, _ = S.solve_batch([prob, prob], precision="mixed-fp32")
     for ph in phis:
         assert rel_err(ph, ref) < 1e-6
