@@ -120,7 +120,6 @@ int BatchSolver::add(Problem* P) {
         throw Error(RT_EINVAL, "the batched solver runs the mixed-precision modes");
     if (P->scan_cpl <= 0 || sweep_scan_lds_bytes(P64, P->ndims, P->Nx, P->Ny, P->Nz, P->Nmat) > 160 * 1024)
         throw Error(RT_EINVAL, "a batched problem does not fit the row-scan sweep");
-    if (P->csd) throw Error(RT_EINVAL, "continuous slowing down is solved one problem at a time");
     if (P->ho_K) throw Error(RT_EINVAL, "general 1D closures are solved one problem at a time");
     if (P->fp) throw Error(RT_EINVAL, "the angular Fokker-Planck term is solved one problem at a time");
     int slot = -1;
@@ -466,6 +465,16 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
             launch_expand_gemm(ddm, nd, dg, s);
         });
         if (nm) plan.push_back(timed(0, tix, g_upd, g_mflop, g_mby, s, [=] { launch_mgemm(ps_, dpd, mg, moff, s); }));
+        // slowing down (FP64 inhomogeneous passes): the energy inflow joins the angular source
+        // before the sweep, and the energy outflow 2 psi - psiE_in is formed after it
+        std::vector<std::pair<const PassDesc*, long>> csd_here;
+        for (size_t i = 0; i < who.size(); ++i)
+            if (group_of(prec[i]) == pr && pds[i].psiE_in && full[i] != 2)
+                csd_here.push_back({dpd + who[i], (long)ps[who[i]].P->Nd * ps[who[i]].P->Nc});
+        if (!csd_here.empty())
+            plan.push_back([=] {
+                for (auto& c : csd_here) launch_csd_source(c.first, c.second, s);
+            });
         for (auto& kv : sweeps) {
             const std::vector<int>& sm = kv.second;
             const int* dsm = stage(sm.data(), sm.size());
@@ -481,6 +490,10 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
             }));
             swbase += ntask;
         }
+        if (!csd_here.empty())
+            plan.push_back([=] {
+                for (auto& c : csd_here) launch_csd_out(c.first, c.second, s);
+            });
         plan.push_back(timed(2, tix, d_upd, g_dflop, d_by, s, [=] { launch_dgemm(ps_, dpd, dg, doff, s); }));
     }
     if (fork) {
@@ -491,6 +504,15 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
             RT_CHECK(hipStreamWaitEvent(s0, ej, 0));
         });
     }
+    // the reconstruction's angular flux is T(z)'s (the residual pass's plus c = T(0)'s, summed
+    // before this round): its energy outflow is the group's
+    for (size_t i = 0; i < who.size(); ++i)
+        if (full[i] == 2 && pds[i].psiE_out) {
+            const PassDesc* one = dpd + who[i];
+            const long ne = (long)ps[who[i]].P->Nd * ps[who[i]].P->Nc;
+            hipStream_t s0 = st;
+            plan.push_back([=] { launch_csd_out(one, ne, s0); });
+        }
     if (any_bout) {
         const int nall = n;
         hipStream_t s0 = st;
@@ -516,6 +538,15 @@ bool BatchSolver::round() {
         Problem* P = s.P;
         launch_outsrc(P->d_qlout, P->d_phi, P->d_sigo, P->d_qvol, P->d_mat, P->d_act, s.g, P->G, P->Nmat, P->Nact,
                       P->Np, P->Nc, st);
+        if (P->csd) {  // the energy flux enters group g rescaled by ΔE_g/ΔE_{g-1}; zero for the first (sn_flux.jl:166,183)
+            const size_t ne = (size_t)P->Nd * P->Nc;
+            if (s.g == 0) {
+                k_fill(P->d_psiE_in, 0.0, ne, st);
+            } else {
+                k_copy(P->d_psiE_in, P->d_psiE_out, ne, st);
+                k_scale(P->d_psiE_in, P->dE[s.g] / P->dE[s.g - 1], ne, st);
+            }
+        }
         s.phase = CPASS;
         s.niter = 0;
         s.refs = 0;

@@ -121,7 +121,40 @@ def test_csd_other_accelerations_and_batch(oracle):
         r, _ = oracle.solve(q)
         phi, _ = S.compute_flux(q)
         assert rel_err(phi, r) < 1e-10, acc
-    # a lockstep batch hands a slowing-down problem to its per-problem path
     phis, _, _ = S.solve_batch([p, p], precision="mixed-fp32")
     for ph in phis:
         assert rel_err(ph, ref) < 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["mixed-fp32", "mixed-bf16"])
+def test_csd_in_lockstep_batch(oracle, precision):
+    """Slowing-down problems run in the lockstep solver (the energy inflow joins the angular
+    source of each c = T(0) pass, the energy outflow of T(z) is formed at the reconstruction and
+    carried to the next group), mixed with BTE problems and reflective/periodic faces: within 1e-6
+    of the oracle, bit-identical alone and in the mix."""
+    import os
+    from radiant_hip import solver as S
+    P = _P()
+    probs = [_csd(P.small_problem(2, N=7, G=3, L=2, tol=1e-12, seed=91), seed=11),
+             _csd(P.small_problem(1, N=12, G=4, L=3, tol=1e-12, seed=92), seed=12),
+             _csd(P.small_problem(3, N=5, G=3, L=2, tol=1e-12, seed=93, bc=[1, 2, 0, 1, 2, 0]), seed=13),
+             P.small_problem(2, N=8, G=3, L=2, tol=1e-12, seed=94),
+             _csd(P.small_problem(2, dims=(20, 6), G=3, L=2, tol=1e-12, seed=95, bc=[1, 1, 2, 2]), seed=14)]
+    phis, stats, _ = S.solve_batch(probs, precision=precision)
+    for i, p in enumerate(probs):
+        ref, _ = oracle.solve(p)
+        e = rel_err(phis[i], ref)
+        _say("csd batch", precision, i, e)
+        assert e < 1e-6 and all(s["converged"] for s in stats[i])
+    alone, _, _ = S.solve_batch([probs[0]], precision=precision)
+    assert np.array_equal(alone[0], phis[0])
+    os.environ["RADIANT_HIP_PROFILE"] = "1"
+    try:  # kernel times are recorded by lockstep solvers only
+        rb = S.ResidentBatch(probs[:3], precision=precision)
+        rb.solve()
+        _, n, _, _, _ = rb.kernel_times()
+        rb.close()
+    finally:
+        os.environ.pop("RADIANT_HIP_PROFILE", None)
+    assert n[:, 1].sum() > 0
