@@ -121,7 +121,6 @@ int BatchSolver::add(Problem* P) {
     if (P->scan_cpl <= 0 || sweep_scan_lds_bytes(P64, P->ndims, P->Nx, P->Ny, P->Nz, P->Nmat) > 160 * 1024)
         throw Error(RT_EINVAL, "a batched problem does not fit the row-scan sweep");
     if (P->ho_K) throw Error(RT_EINVAL, "general 1D closures are solved one problem at a time");
-    if (P->fp) throw Error(RT_EINVAL, "the angular Fokker-Planck term is solved one problem at a time");
     int slot = -1;
     for (int i = 0; i < (int)ps.size(); ++i)
         if (!ps[i].P) {
@@ -359,9 +358,22 @@ void BatchSolver::run_gs(const std::vector<GsItem>& items, bool update, bool nor
 void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<PassDesc>& pds,
                                 const std::vector<int>& prec, const std::vector<int>& full) {
     const int n = (int)ps.size();
-    // PassDesc of every problem (index = batch index)
+    // PassDesc of every problem (index = batch index).  A pass of a problem with the angular
+    // Fokker-Planck term on a non-zero state reads its source from the problem's fpq buffer,
+    // qlout + T ℳ Φ, formed by fp_source_kernel at the start of the round (a reconstruction
+    // runs no M contraction: its angular flux already holds the term).
     std::vector<PassDesc> all(n);
-    for (size_t i = 0; i < who.size(); ++i) all[who[i]] = pds[i];
+    std::vector<int> fpw;
+    for (size_t i = 0; i < who.size(); ++i) {
+        all[who[i]] = pds[i];
+        PassDesc& a = all[who[i]];
+        if (a.fpT && a.phi_in && full[i] != 2) {
+            a.fp_qin = a.qlout;
+            a.fpq = ps[who[i]].P->d_fpq;
+            a.qlout = a.fpq;
+            fpw.push_back(who[i]);
+        }
+    }
     const PassDesc* dpd = stage(all.data(), all.size());
     // With fork_streams the FP64 passes (refinement residuals, reconstructions, c = T(0)) of this
     // round run on st64, concurrently with the low-precision Arnoldi passes on st; st waits for
@@ -381,6 +393,12 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
     // Non-void faces and surface sources: each such problem's incoming face fluxes are formed
     // from its state's boundary moments (and sources) before the sweeps, and after the last D
     // contraction one launch writes every problem's exit-face moments.
+    for (int w : fpw) {
+        const PassDesc* one = dpd + w;
+        const long na = (long)ps[w].P->Nact * ps[w].P->Nc;
+        hipStream_t s0 = st;
+        plan.push_back([=] { launch_fp_source(one, na, s0); });
+    }
     bool any_bout = false;
     int max_nb = 0;
     for (size_t i = 0; i < who.size(); ++i) {

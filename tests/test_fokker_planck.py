@@ -292,3 +292,39 @@ def test_fp_gpu_other_accelerations_and_batch(oracle):
     phis, _, _ = S.solve_batch([p, p], precision="mixed-fp32")
     for ph in phis:
         assert rel_err(ph, ref) < 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["mixed-fp32", "mixed-bf16"])
+def test_fp_in_lockstep_batch(oracle, precision):
+    """Problems with the angular Fokker-Planck term in the lockstep solver (their source
+    qlout + T ℳ Φ formed at the start of each round), mixed with each other, with BTE problems and
+    with reflective/periodic faces: within 1e-6 of the oracle, bit-identical alone and in the mix."""
+    import os
+    from radiant_hip import solver as S
+    *_, P = _mods()
+    probs = []
+    for k, (nd, stype, meth, qt, qo, qd, bc) in enumerate(GPU_CASES[:7]):
+        p = P.small_problem(nd, N=5 if nd == 3 else 7, G=3, L=2, tol=1e-12, seed=170 + k, bc=bc,
+                            quad_type=qt, quad_order=qo)
+        if qd:
+            p.qdims = qd
+        probs.append(_bfp(p, stype=stype, method=meth, seed=120 + k))
+    probs.append(P.small_problem(2, N=8, G=3, L=2, tol=1e-12, seed=180))
+    phis, stats, _ = S.solve_batch(probs, precision=precision)
+    for i, p in enumerate(probs):
+        ref, _ = oracle.solve(p)
+        e = rel_err(phis[i], ref)
+        _say("fp batch", precision, i, e)
+        assert e < 1e-6 and all(s["converged"] for s in stats[i])
+    alone, _, _ = S.solve_batch([probs[3]], precision=precision)
+    assert np.array_equal(alone[0], phis[3])
+    os.environ["RADIANT_HIP_PROFILE"] = "1"
+    try:  # kernel times are recorded by lockstep solvers only
+        rb = S.ResidentBatch(probs[:3], precision=precision)
+        rb.solve()
+        _, n, _, _, _ = rb.kernel_times()
+        rb.close()
+    finally:
+        os.environ.pop("RADIANT_HIP_PROFILE", None)
+    assert n[:, 1].sum() > 0
