@@ -118,9 +118,8 @@ void BatchSolver::ensure_task_capacity() {
 int BatchSolver::add(Problem* P) {
     if (P->precision == RT_PREC_FP64)
         throw Error(RT_EINVAL, "the batched solver runs the mixed-precision modes");
-    if (P->scan_cpl <= 0 || sweep_scan_lds_bytes(P64, P->ndims, P->Nx, P->Ny, P->Nz, P->Nmat) > 160 * 1024)
+    if (!P->ho_K && (P->scan_cpl <= 0 || sweep_scan_lds_bytes(P64, P->ndims, P->Nx, P->Ny, P->Nz, P->Nmat) > 160 * 1024))
         throw Error(RT_EINVAL, "a batched problem does not fit the row-scan sweep");
-    if (P->ho_K) throw Error(RT_EINVAL, "general 1D closures are solved one problem at a time");
     int slot = -1;
     for (int i = 0; i < (int)ps.size(); ++i)
         if (!ps[i].P) {
@@ -439,6 +438,7 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
         // and, for an I - A pass, reads the input state's rows
         double g_mby = 0, d_by = 0;
         std::map<SKey, double> s_by;
+        std::vector<size_t> ho_here;  // problems of this group with a closure above order 1
         for (size_t i = 0; i < who.size(); ++i) {
             if (group_of(prec[i]) != pr) continue;
             const Problem* P = ps[who[i]].P;
@@ -456,6 +456,13 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
             if (full[i] == 2) continue;  // contraction only: no moment->direction contraction, no sweep
             mmeta.insert(mmeta.end(), {who[i], ntile, nrg_m, moff});
             moff += ntile * nrg_m;
+            if (P->ho_K) {  // a closure above order 1: its own sweep kernel, in its pass's precision
+                ho_here.push_back(i);
+                g_upd += upd;
+                g_mflop += 2.0 * P->Nact * upd;
+                g_mby += es * ndc + 8.0 * P->Nact * (double)P->Nc;
+                continue;
+            }
             const int bprec = pds[i].bin_x ? (prec[i] == P64 ? (int)P64 : (int)P32) : -1;
             const SKey key{P->scan_cpl, P->scan_waves, bprec};
             auto& v = sweeps[key];
@@ -507,6 +514,22 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
                 launch_sweep_scan(kind, dpd, sb, ntask, cpl, waves, lds, s, bprec >= 0);
             }));
             swbase += ntask;
+        }
+        for (size_t i : ho_here) {
+            const Problem* P = ps[who[i]].P;
+            const PassDesc* one = dpd + who[i];
+            const int sp = prec[i] == P64 ? (int)P64 : (int)P32;
+            const int nd_ = P->Nd, K = P->ho_K, kx = P->ho_Kx, ky = P->ho_Ky, kz = P->ho_Kz, dims = P->ndims;
+            const bool fc = P->ho_fc;
+            const size_t lds = dims == 3 ? sweep_3d_ho_lds_bytes(sp, P->Nx, P->Ny, P->Nz, kx, ky, kz, fc)
+                             : (dims == 2 ? sweep_2d_ho_lds_bytes(sp, P->Nx, P->Ny, kx, ky) : 0);
+            const double hu = (double)P->Ncell * P->Nd * ((pr == PMIX && prec[i] == P64) ? 2.0 : 1.0);
+            const double hb = 2.0 * (prec[i] == P64 ? 8.0 : 4.0) * (double)P->Nd * P->Nc;
+            plan.push_back(timed(1, tix, hu, 0.0, hb, s, [=] {
+                if (dims == 3) launch_sweep_3d_ho(sp, one, nd_, kx, ky, kz, fc, lds, s);
+                else if (dims == 2) launch_sweep_2d_ho(sp, one, nd_, kx, ky, fc, lds, s);
+                else launch_sweep_1d_ho(sp, one, nd_, K, s);
+            }));
         }
         if (!csd_here.empty())
             plan.push_back([=] {

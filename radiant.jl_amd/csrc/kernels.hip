@@ -2795,10 +2795,10 @@ void launch_csd_out(const PassDesc* d_pd, long n, hipStream_t st) {
 template <typename T>
 static void ho_launch(const PassDesc* d_pd, int Nd, int K, hipStream_t st) {
     switch (K) {
-        case 1: hipLaunchKernelGGL((sweep_1d_ho_kernel<T, 1>), dim3(Nd), dim3(64), 0, st, d_pd); break;
-        case 2: hipLaunchKernelGGL((sweep_1d_ho_kernel<T, 2>), dim3(Nd), dim3(64), 0, st, d_pd); break;
-        case 3: hipLaunchKernelGGL((sweep_1d_ho_kernel<T, 3>), dim3(Nd), dim3(64), 0, st, d_pd); break;
-        case 4: hipLaunchKernelGGL((sweep_1d_ho_kernel<T, 4>), dim3(Nd), dim3(64), 0, st, d_pd); break;
+        case 1: launch_timed(sweep_1d_ho_kernel<T, 1>, dim3(Nd), dim3(64), 0, st, d_pd); break;
+        case 2: launch_timed(sweep_1d_ho_kernel<T, 2>, dim3(Nd), dim3(64), 0, st, d_pd); break;
+        case 3: launch_timed(sweep_1d_ho_kernel<T, 3>, dim3(Nd), dim3(64), 0, st, d_pd); break;
+        case 4: launch_timed(sweep_1d_ho_kernel<T, 4>, dim3(Nd), dim3(64), 0, st, d_pd); break;
         default: throw std::runtime_error("1D closure order out of range");
     }
 }
@@ -2814,7 +2814,7 @@ static void ho2_launch(const PassDesc* d_pd, int Nd, size_t lds, hipStream_t st)
         RT_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         attr = true;
     }
-    hipLaunchKernelGGL(fn, dim3(Nd), dim3(256), lds, st, d_pd);
+    launch_timed(fn, dim3(Nd), dim3(256), (uint32_t)lds, st, d_pd);  // with the caller's timing events
 }
 
 template <typename T>
@@ -2856,7 +2856,7 @@ static void ho3_launch(const PassDesc* d_pd, int Nd, size_t lds, hipStream_t st)
         RT_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         attr = true;
     }
-    hipLaunchKernelGGL(fn, dim3(Nd), dim3(256), lds, st, d_pd);
+    launch_timed(fn, dim3(Nd), dim3(256), (uint32_t)lds, st, d_pd);  // with the caller's timing events
 }
 
 template <typename T, int KX, int KY, int KZ>
