@@ -794,8 +794,8 @@ void BatchSolver::advance(int i, std::vector<VecItem>& pre, std::vector<VecItem>
                 break;
             }
             s.resid = 1.0;
+            // (r is first read after the residual pass writes r = c - (I - A) z)
             pre.push_back(item(VOP_FILL, s.z, nullptr, 0.0));
-            pre.push_back(item(VOP_COPY, s.r, s.c, 0.0));
             pre.push_back(item(VOP_SCALE_COPY, s.V[0], s.c, 1.0 / s.bnorm));
             s.beta = s.bnorm;
             start_arnoldi(s);
