@@ -452,7 +452,7 @@ __device__ __forceinline__ void contract_body(const PassDesc& d, const GemmTask 
 }
 
 template <typename T, int RT, int CT, bool MG, int NS>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void contract_kernel(const PassDesc* __restrict__ pds, const GemmTask* __restrict__ tasks) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((sizeof(T) == 4 && !MG) ? 3 : 2))) void contract_kernel(const PassDesc* __restrict__ pds, const GemmTask* __restrict__ tasks) {
     __shared__ int rows_s[MG ? CONTRACT_KMAX : 1];
     const GemmTask tk = tasks[blockIdx.x];
     contract_body<T, RT, CT, MG, NS>(pds[tk.prob], tk, rows_s);
