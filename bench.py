@@ -33,13 +33,13 @@ HBM_PEAK_GBS = 8000.0
 # from rocprofv3 --pmc runs over that workload: FETCH_SIZE and WRITE_SIZE (KiB per dispatch,
 # averaged over the kernel's dispatches), each scaled by what the counter reads for a plain 1 GiB
 # device copy (FETCH_SIZE reads 0.5 GiB for it: factor 2.000; WRITE_SIZE 1 GiB: factor 1.000).
-MEASURED_TRAFFIC = {("sweep_fp32", 32): dict(fetch=337.04e6, write=308.96e6,
+MEASURED_TRAFFIC = {("sweep_fp32", 32): dict(fetch=346.30e6, write=309.41e6,
                                              kernel="sweep_line_mixed_kernel<2, 8, true>",
                                              summary="profiles/sweep_line_mixed_c1_batch32.md"),
-                    ("dgemm_fp32", 32): dict(fetch=486.18e6, write=174.70e6,
+                    ("dgemm_fp32", 32): dict(fetch=482.76e6, write=174.70e6,
                                              kernel="contract_mixed_kernel<4, 4, false, 4>",
                                              summary="profiles/contractions_mfma_c1_batch32.md"),
-                    ("mgemm_fp32", 32): dict(fetch=145.64e6, write=304.57e6,
+                    ("mgemm_fp32", 32): dict(fetch=141.22e6, write=303.39e6,
                                              kernel="contract_mixed_kernel<4, 4, true, 2>",
                                              summary="profiles/contractions_mfma_c1_batch32.md")}
 MFMA_PEAK_TFLOPS = {0: 78.6, 1: 157.3, 2: 2500.0}
