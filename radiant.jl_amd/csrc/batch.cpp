@@ -157,7 +157,9 @@ int BatchSolver::add(Problem* P) {
 void BatchSolver::start(int slot) {
     PS& s = ps[slot];
     s.P->counters = SolveCounters{};
-    k_fill(s.P->d_phi, 0.0, (size_t)s.P->G * s.P->Np * s.P->Nc, st);
+    // no clearing of the flux: a group's out-of-group source reads only the groups before it
+    // (outsrc_kernel), and every group's moments are written whole before that (the
+    // reconstruction's full-row contraction, or the fill of a group without source)
     s.phase = START;
     s.g = 0;
     std::fill(s.stats.begin(), s.stats.end(), rt_group_stats{});
