@@ -86,15 +86,15 @@ typedef struct rt_problem_desc {
      * z+), NULL or the half-range moments [G][Np_source][npos_f], row-major with the position
      * fastest (npos = Ny*Nz for x faces, Nx*Nz for y faces, Nx*Ny for z faces; positions ordered
      * like the boundary moments: iy + Ny*iz, ix + Nx*iz, ix + Nx*iy).  The first Np_source
-     * half-range moments of each face carry the source.  Every precision and state; lockstep
-     * batches solve such problems on their per-problem path.  Callers that do not use surface
+     * half-range moments of each face carry the source.  Every precision and state, single
+     * problems, lockstep batches and served requests alike.  Callers that do not use surface
      * sources leave these fields zero. */
     const double *ssrc[6];
     int32_t Np_source;
     /* Continuous slowing down (SN solver type "BCSD" with the DD scheme of order 1 in energy;
      * sn_{1,2,3}D_bfp.jl, sn_flux.jl:86-97,166,183-188): 0 = off.  Sb: stopping powers at the group
      * boundaries [G+1][Nmat] column-major (cross_sections.get_boundary_stopping_powers), dE: group
-     * widths [G] (get_energy_width).  Single-problem solves only (not the lockstep batch). */
+     * widths [G] (get_energy_width).  Single problems, lockstep batches and served requests. */
     int32_t csd;
     const double *Sb;
     const double *dE;
@@ -112,8 +112,9 @@ typedef struct rt_problem_desc {
     /* Spatial closure along x (SN.set_scheme("x", type, order); scheme_weights.jl:25-90,
      * sn_1D_bte.jl:27-64): order_x = 𝒪x spatial Legendre moments per cell (0 or 1: first order, at
      * most 4) and omega_x[𝒪x + 1] the closure weights ω of the x axis (NULL: diamond difference of
-     * order 1).  Any order or closure other than diamond difference of order 1 is for the 1D BTE
-     * (csd = 0), single problems, every precision and state.  The flux moments then carry Nm5 = 𝒪x
+     * order 1).  Any order or closure other than diamond difference of order 1 is for the BTE
+     * (csd = 0), in single problems, lockstep batches and served requests, every precision and
+     * state.  The flux moments then carry Nm5 = 𝒪x
      * spatial moments per cell: phi_out is 𝚽l[G, Np, Nm5, Nx] and the qlout of rt_problem_pass is
      * [Np, Nm5 * Nx]; qvol keeps one value per cell (spatial moment 1, Source.jl:135). */
     int32_t order_x;
@@ -162,12 +163,21 @@ typedef struct rt_solve_info {
  * ------------------------------------------------------------------------------------------ */
 int rt_solve(const rt_problem_desc *p, double *phi_out, rt_group_stats *stats_out);
 
+/* One-shot solve that also returns the flux at the cutoff energy of the slowing-down solver
+ * types (sn_flux.jl:151,213-218, stored by flux.add_flux_cutoff at sn_flux.jl:229):
+ * 𝚽cutoff[p, 1, Nx, Ny, Nz] = sum_n Dn[p, n] 𝚽E12[n, 1, Nx, Ny, Nz], the energy-face angular flux
+ * leaving the last group.  cutoff_out (caller-allocated, Np*Nx*Ny*Nz doubles, may be NULL) is
+ * zero-filled when continuous slowing down is off (csd = 0). */
+int rt_solve_cutoff(const rt_problem_desc *p, double *phi_out, double *cutoff_out, rt_group_stats *stats_out);
+
 /* Resident problem: upload once, solve many times (inputs stay in HBM). */
 typedef struct rt_problem rt_problem;
 int rt_problem_create(const rt_problem_desc *p, rt_problem **out);
 int rt_problem_solve(rt_problem *h, rt_group_stats *stats_out, rt_solve_info *info);
 int rt_problem_get_flux(rt_problem *h, double *phi_out);
 int rt_problem_destroy(rt_problem *h);
+/* 𝚽cutoff of the last solve on a handle, as rt_solve_cutoff returns it. */
+int rt_problem_get_flux_cutoff(rt_problem *h, double *cutoff_out);
 /* Device bytes a problem needs (admission control). */
 int rt_problem_device_bytes(const rt_problem_desc *p, int64_t *bytes);
 
@@ -207,6 +217,8 @@ typedef struct rt_batch rt_batch;
 int rt_batch_create(int32_t n, const rt_problem_desc *const *p, rt_batch **out);
 int rt_batch_solve(rt_batch *b, rt_group_stats *const *stats_out, rt_solve_info *info);
 int rt_batch_get_flux(rt_batch *b, int32_t i, double *phi_out);
+/* 𝚽cutoff of problem i after the last rt_batch_solve, as rt_solve_cutoff returns it. */
+int rt_batch_get_flux_cutoff(rt_batch *b, int32_t i, double *cutoff_out);
 int rt_batch_destroy(rt_batch *b);
 /* Profiling hook (no counterpart in the reference): per-kernel device time of the last
  * rt_batch_solve run with RADIANT_HIP_PROFILE=1, lockstep batches only.  Arrays of 12 indexed
@@ -238,6 +250,9 @@ int rt_server_submit(rt_server *s, const rt_problem_desc *p, int64_t *req);
  * latency_ms is submit -> flux final in device memory. */
 int rt_server_poll(rt_server *s, int64_t req, double *phi_out, rt_group_stats *stats_out,
                    double *latency_ms);
+/* rt_server_poll that also collects 𝚽cutoff (as rt_solve_cutoff) when cutoff_out is not NULL. */
+int rt_server_poll_cutoff(rt_server *s, int64_t req, double *phi_out, double *cutoff_out,
+                          rt_group_stats *stats_out, double *latency_ms);
 int rt_server_cancel(rt_server *s, int64_t req);
 int rt_server_destroy(rt_server *s);
 

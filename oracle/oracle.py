@@ -69,6 +69,7 @@ def lib():
         L.oracle_problem_sizes.argtypes = [C.POINTER(OracleProblem), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                            C.POINTER(C.c_int), C.POINTER(C.c_int64)]
         L.oracle_solve.argtypes = [C.POINTER(OracleProblem), dp, C.POINTER(OracleStats)]
+        L.oracle_solve_cutoff.argtypes = [C.POINTER(OracleProblem), dp, dp, C.POINTER(OracleStats)]
         L.oracle_pass.argtypes = [C.POINTER(OracleProblem), C.c_int, C.c_int, dp, dp]
         L.oracle_fokker_planck.argtypes = [C.POINTER(OracleProblem), dp, dp, C.c_int]
         L.oracle_cell_1d.argtypes = [C.c_char_p, C.c_int, C.c_double, C.c_double, C.c_double, dp, dp, dp]
@@ -283,6 +284,20 @@ def solve(prob, nthreads=1, verbose=0, **over):
     _check(lib().oracle_solve(C.byref(op), _p(phi), st))
     stats = [dict(niter=s.niter, resid=s.resid, rho=s.rho, converged=bool(s.converged), passes=s.passes) for s in st]
     return phi.reshape((prob.G, np_, nc), order="F"), stats
+
+
+def solve_cutoff(prob, nthreads=1, verbose=0, **over):
+    """Full solve that also returns 𝚽cutoff (sn_flux.jl:213-218): (phi[G,Np,Nc], cutoff[Np,Nc],
+    stats).  cutoff is zero unless continuous slowing down is on."""
+    op = make_problem(prob, nthreads=nthreads, verbose=verbose, **over)
+    nd, np_, nps, S = sizes(prob)
+    nc = prob.Nx * prob.Ny * prob.Nz * spatial_moments(prob)
+    phi = np.zeros(prob.G * np_ * nc)
+    cut = np.zeros(np_ * nc)
+    st = (OracleStats * prob.G)()
+    _check(lib().oracle_solve_cutoff(C.byref(op), _p(phi), _p(cut), st))
+    stats = [dict(niter=s.niter, resid=s.resid, rho=s.rho, converged=bool(s.converged), passes=s.passes) for s in st]
+    return phi.reshape((prob.G, np_, nc), order="F"), cut.reshape((np_, nc), order="F"), stats
 
 
 def transport_pass(prob, g, homogeneous, z, qlout):

@@ -2139,7 +2139,7 @@ struct Solver {
     }
 
     // sn_flux.jl:150-232 -- one Gauss-Seidel pass over groups (upscatter sees zero flux).
-    void solve(double* phi_out, oracle_stats* stats) {
+    void solve(double* phi_out, oracle_stats* stats, double* cutoff_out = nullptr) {
         const int Np = A.Np;
         vector<double> phi_all((size_t)G * nphi, 0.0);  // [g][p + Np*c]
         for (int g = 0; g < G; ++g) {
@@ -2168,6 +2168,17 @@ struct Solver {
             one_speed(g, z, Qlout, st);
             std::copy(z.begin(), z.begin() + nphi, phi_all.begin() + (size_t)g * nphi);
             if (stats) stats[g] = st;
+        }
+        // sn_flux.jl:213-218 -- flux at the cutoff energy: the energy-face angular flux leaving the
+        // last group (not rescaled), projected on the moments with Dn, accumulated over n outermost.
+        // Layout 𝚽cutoff[p, is, ix, iy, iz] (is = 1: the slowing-down types carry one spatial moment).
+        if (cutoff_out) {
+            std::fill(cutoff_out, cutoff_out + (size_t)Np * Ncol, 0.0);
+            if (P.csd)
+                for (int n = 0; n < A.Nd; ++n)
+                    for (int c = 0; c < Nc; ++c)
+                        for (int p = 0; p < Np; ++p)
+                            cutoff_out[p + (size_t)Np * c] += A.B.D[p + (size_t)Np * n] * psiE_out[(size_t)n * Nc + c];
         }
         // Radiant layout 𝚽l[g, p, is, ix, iy, iz]
         for (int c = 0; c < Ncol; ++c)
@@ -2310,6 +2321,16 @@ int oracle_solve(const oracle_problem* p, double* phi_out, oracle_stats* stats_o
     try {
         Solver s(*p);
         s.solve(phi_out, stats_out);
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// Full solve that also returns 𝚽cutoff[Np, 1, Nx, Ny, Nz] (sn_flux.jl:151,213-218,229) in
+// cutoff_out (zeros unless continuous slowing down is on).
+int oracle_solve_cutoff(const oracle_problem* p, double* phi_out, double* cutoff_out, oracle_stats* stats_out) {
+    try {
+        Solver s(*p);
+        s.solve(phi_out, stats_out, cutoff_out);
         return 0;
     } catch (std::exception& e) { g_err = e.what(); return -1; }
 }

@@ -85,6 +85,11 @@ int rt_problem_get_flux(rt_problem* h, double* phi_out) {
     return guarded([&] { h->p->get_flux(phi_out); });
 }
 
+int rt_problem_get_flux_cutoff(rt_problem* h, double* cutoff_out) {
+    if (!h || !h->p || !cutoff_out) return fail(RT_EINVAL, "null argument");
+    return guarded([&] { h->p->get_flux_cutoff(cutoff_out); });
+}
+
 int rt_problem_destroy(rt_problem* h) {
     if (!h) return RT_OK;
     return guarded([&] { delete h; });
@@ -140,6 +145,16 @@ int rt_solve(const rt_problem_desc* p, double* phi_out, rt_group_stats* stats_ou
         rt::Problem prob(*p);
         prob.solve(stats_out, nullptr);
         prob.get_flux(phi_out);
+    });
+}
+
+int rt_solve_cutoff(const rt_problem_desc* p, double* phi_out, double* cutoff_out, rt_group_stats* stats_out) {
+    if (!p || !phi_out) return fail(RT_EINVAL, "null argument");
+    return guarded([&] {
+        rt::Problem prob(*p);
+        prob.solve(stats_out, nullptr);
+        prob.get_flux(phi_out);
+        if (cutoff_out) prob.get_flux_cutoff(cutoff_out);
     });
 }
 

@@ -158,8 +158,9 @@ void BatchSolver::start(int slot) {
     PS& s = ps[slot];
     s.P->counters = SolveCounters{};
     // no clearing of the flux: a group's out-of-group source reads only the groups before it
-    // (outsrc_kernel), and every group's moments are written whole before that (the
-    // reconstruction's full-row contraction, or the fill of a group without source)
+    // (outsrc_kernel), and every group's moments are written whole before that: by the
+    // reconstruction's full-row contraction, or by the fill of a group without source, which
+    // round() issues ahead of the next group's start
     s.phase = START;
     s.g = 0;
     std::fill(s.stats.begin(), s.stats.end(), rt_group_stats{});
@@ -575,31 +576,37 @@ bool BatchSolver::round() {
     using clk = std::chrono::steady_clock;
     auto us = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double, std::micro>(b - a).count(); };
     const int n = (int)ps.size();
+    const auto t0 = clk::now();
+    // Vector operations recorded for this round run first: among them is the flux fill of a
+    // group without source, whose moments the next group's out-of-group source reads.
+    arena_off = 0;
+    run_vecops(pre);
+    pre.clear();
     // ---- group starts: out-of-group + volume source (sn_flux.jl:175-179)
     for (auto& s : ps) {
         if (s.phase != START) continue;
         Problem* P = s.P;
-        launch_outsrc(P->d_qlout, P->d_phi, P->d_sigo, P->d_qvol, P->d_mat, P->d_act, s.g, P->G, P->Nmat, P->Nact,
-                      P->Np, P->Nc, st);
-        if (P->csd) {  // the energy flux enters group g rescaled by ΔE_g/ΔE_{g-1}; zero for the first (sn_flux.jl:166,183)
-            const size_t ne = (size_t)P->Nd * P->Nc;
-            if (s.g == 0) {
-                k_fill(P->d_psiE_in, 0.0, ne, st);
-            } else {
-                k_copy(P->d_psiE_in, P->d_psiE_out, ne, st);
-                k_scale(P->d_psiE_in, P->dE[s.g] / P->dE[s.g - 1], ne, st);
+        const int g = s.g;
+        hipStream_t s0 = st;
+        plan.push_back([=] {
+            launch_outsrc(P->d_qlout, P->d_phi, P->d_sigo, P->d_qvol, P->d_mat, P->d_act, g, P->G, P->Nmat, P->Nact,
+                          P->Np, P->Nc, s0);
+            if (P->csd) {  // the energy flux enters group g rescaled by ΔE_g/ΔE_{g-1}; zero for the first (sn_flux.jl:166,183)
+                const size_t ne = (size_t)P->Nd * P->Nc;
+                if (g == 0) {
+                    k_fill(P->d_psiE_in, 0.0, ne, s0);
+                } else {
+                    k_copy(P->d_psiE_in, P->d_psiE_out, ne, s0);
+                    k_scale(P->d_psiE_in, P->dE[g] / P->dE[g - 1], ne, s0);
+                }
             }
-        }
+        });
         s.phase = CPASS;
         s.niter = 0;
         s.refs = 0;
         s.p0 = P->counters.passes;
         s.l0 = P->counters.passes_lo;
     }
-    const auto t0 = clk::now();
-    arena_off = 0;
-    run_vecops(pre);
-    pre.clear();
     // ---- one pass per active problem
     std::vector<int> who, prec, full;
     std::vector<PassDesc> pds;
@@ -791,6 +798,11 @@ void BatchSolver::advance(int i, std::vector<VecItem>& pre, std::vector<VecItem>
                     it.op = VOP_FILL; it.n = (long)P->Np * P->Nc; it.y = P->d_phi + (size_t)s.g * P->Np * P->Nc;
                     return it;
                 }());
+                if (P->csd) {  // and no energy flux leaves the group (sn_one_speed.jl:119)
+                    VecItem it{};
+                    it.op = VOP_FILL; it.n = (long)P->Nd * P->Nc; it.y = P->d_psiE_out;
+                    pre.push_back(it);
+                }
                 s.resid = 0.0;
                 finish_group(true);
                 break;

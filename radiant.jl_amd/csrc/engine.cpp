@@ -1208,6 +1208,19 @@ void Problem::get_flux(double* phi_out, hipStream_t s) {
     RT_CHECK(hipStreamSynchronize(s));
 }
 
+void Problem::get_flux_cutoff(double* out, hipStream_t s) {
+    if (!s) s = st;
+    const size_t n = (size_t)Np * Nc;
+    if (!csd) {
+        std::fill(out, out + n, 0.0);
+        return;
+    }
+    double* tmp = vec("cutoff", n);
+    k_cutoff(tmp, d_D64, d_psiE_out, Np, Nd, Nc, s);
+    RT_CHECK(hipMemcpyAsync(out, tmp, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    RT_CHECK(hipStreamSynchronize(s));
+}
+
 void Problem::host_pass(int g, int mode, const double* qlout_host, const double* zin, double* zout) {
     if (g < 0 || g >= G) throw Error(RT_EINVAL, "group out of range");
     if (mode < 0 || mode > 2) throw Error(RT_EINVAL, "pass mode must be 0 (T), 1 (A) or 2 (I - A)");

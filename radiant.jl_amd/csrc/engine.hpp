@@ -72,6 +72,9 @@ public:
     // Flux moments in Radiant's layout into host memory, on stream s (default: the problem's).
     void get_flux(double* phi_out, hipStream_t s = nullptr);
     static constexpr size_t FLUX_STAGE = (size_t)8 << 20;  // doubles staged per copy-out step
+    // 𝚽cutoff[Np, 1, Nx, Ny, Nz] (sn_flux.jl:213-218): the energy-face flux leaving the last group
+    // of the last solve, projected with Dn; zeros unless continuous slowing down is on.
+    void get_flux_cutoff(double* out, hipStream_t s = nullptr);
     // Kernel-level hook: one faithful-state pass on host vectors (see rt_problem_pass).
     void host_pass(int g, int mode, const double* qlout_host, const double* zin, double* zout);
     int64_t faithful_state_length() const { return (int64_t)S_f; }

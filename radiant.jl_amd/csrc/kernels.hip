@@ -2408,6 +2408,18 @@ __global__ void phi_to_radiant_kernel(double* __restrict__ dst, const double* __
         dst[i] = src[(g * Np + p) * Nc + c];
     }
 }
+// Flux at the cutoff energy (sn_flux.jl:213-218): out[p + Np*c] = sum_n D[p, n] psiE[n][c], the
+// directions summed in increasing order; DT is the k-major operator DT[n][p] = D[p, n].
+__global__ void cutoff_kernel(double* __restrict__ out, const double* __restrict__ DT, const double* __restrict__ psiE,
+                              int Np, int Nd, int Nc) {
+    const size_t tot = (size_t)Np * Nc;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
+        const size_t p = i % Np, c = i / Np;
+        double acc = 0.0;
+        for (int n = 0; n < Nd; ++n) acc = fma(DT[(size_t)n * Np + p], psiE[(size_t)n * Nc + c], acc);
+        out[i] = acc;
+    }
+}
 __global__ void transpose_kernel(double* __restrict__ dst, const double* __restrict__ src, int R, int C) {
     // dst[c*R + r] = src[r*C + c]
     const size_t tot = (size_t)R * C;
@@ -3001,6 +3013,10 @@ void k_set_vals(const Vals32& v, double* dst, int k, hipStream_t st) {
 void k_phi_to_radiant(double* dst, const double* phi_all, int G, int Np, int Nc, int c0, int nc, hipStream_t st) {
     hipLaunchKernelGGL(phi_to_radiant_kernel, dim3(grid_for((size_t)G * Np * nc)), dim3(256), 0, st, dst, phi_all, G, Np,
                        Nc, c0, nc);
+    RT_HIP(hipGetLastError());
+}
+void k_cutoff(double* out, const double* DT, const double* psiE, int Np, int Nd, int Nc, hipStream_t st) {
+    hipLaunchKernelGGL(cutoff_kernel, dim3(grid_for((size_t)Np * Nc)), dim3(256), 0, st, out, DT, psiE, Np, Nd, Nc);
     RT_HIP(hipGetLastError());
 }
 void k_state_moments_in(double* dst, const double* src, int Np, int Nc, hipStream_t st) {

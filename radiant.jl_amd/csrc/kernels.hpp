@@ -244,6 +244,8 @@ void launch_vecops(const VecItem* items, int nitems, int blocks_per_item, hipStr
 
 // Layout converters of the host boundary (Radiant order <-> device order).
 void k_phi_to_radiant(double* dst, const double* phi_all, int G, int Np, int Nc, int c0, int nc, hipStream_t st);
+// out[p + Np*c] = sum_n DT[n][p] psiE[n][c] (the cutoff-energy flux moments, Radiant's layout)
+void k_cutoff(double* out, const double* DT, const double* psiE, int Np, int Nd, int Nc, hipStream_t st);
 void k_state_moments_in(double* dst, const double* src, int Np, int Nc, hipStream_t st);   // [c][p] -> [p][c]
 void k_state_moments_out(double* dst, const double* src, int Np, int Nc, hipStream_t st);  // [p][c] -> [c][p]
 

@@ -277,6 +277,14 @@ int rt_batch_get_flux(rt_batch* b, int32_t i, double* phi_out) {
     return guard([&] { b->probs[i]->get_flux(phi_out); });
 }
 
+int rt_batch_get_flux_cutoff(rt_batch* b, int32_t i, double* cutoff_out) {
+    if (!b || i < 0 || i >= (int32_t)b->probs.size() || !cutoff_out) {
+        rt::set_last_error("invalid batch cutoff-flux request");
+        return RT_EINVAL;
+    }
+    return guard([&] { b->probs[i]->get_flux_cutoff(cutoff_out); });
+}
+
 int rt_batch_kernel_times(rt_batch* b, double* ms, int64_t* launches, double* updates, double* flops,
                           double* bytes) {
     if (!b || b->bss.empty()) {
@@ -608,14 +616,29 @@ struct rt_server {
         }
         in_batch = 0;
         bs.reset();
-        if (est) (void)hipStreamDestroy(est);
-        est = nullptr;
         lk.lock();
         for (auto& r : left) {
             std::unique_ptr<rt::Problem> none;
             std::vector<rt_group_stats> st;
             publish(*r, Request::CANCELLED, none, st, std::string(), 0);
         }
+        // Finished results of the lockstep engine that no poll collected were solved on est and
+        // synchronise it when released: release them while the stream still exists.  Nothing can
+        // collect them any more (the server is being destroyed).
+        std::vector<std::unique_ptr<rt::Problem>> uncollected;
+        for (auto& kv : reqs) {
+            Request& r = *kv.second;
+            if (r.state == Request::DONE && r.result && r.result->st == est) {
+                bytes_in_use -= r.bytes;
+                uncollected.push_back(std::move(r.result));
+                r.state = Request::CANCELLED;
+            }
+        }
+        lk.unlock();
+        uncollected.clear();
+        if (est) (void)hipStreamDestroy(est);
+        est = nullptr;
+        lk.lock();
     }
 };
 
@@ -664,6 +687,11 @@ int rt_server_submit(rt_server* s, const rt_problem_desc* p, int64_t* req) {
 }
 
 int rt_server_poll(rt_server* s, int64_t req, double* phi_out, rt_group_stats* stats_out, double* latency_ms) {
+    return rt_server_poll_cutoff(s, req, phi_out, nullptr, stats_out, latency_ms);
+}
+
+int rt_server_poll_cutoff(rt_server* s, int64_t req, double* phi_out, double* cutoff_out, rt_group_stats* stats_out,
+                          double* latency_ms) {
     if (!s) {
         rt::set_last_error("null server");
         return RT_EINVAL;
@@ -696,6 +724,7 @@ int rt_server_poll(rt_server* s, int64_t req, double* phi_out, rt_group_stats* s
         RT_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
         try {
             r->result->get_flux(phi_out, cs);
+            if (cutoff_out) r->result->get_flux_cutoff(cutoff_out, cs);
         } catch (...) {
             (void)hipStreamDestroy(cs);
             throw;

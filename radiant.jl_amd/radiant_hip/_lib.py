@@ -70,7 +70,8 @@ EXPORTS = ["rt_solve", "rt_problem_create", "rt_problem_solve", "rt_problem_get_
            "rt_server_create", "rt_server_submit", "rt_server_poll", "rt_server_cancel", "rt_server_destroy",
            "rt_last_error", "rt_version", "rt_test_contraction", "rt_problem_kernel_times",
            "rt_batch_create", "rt_batch_solve", "rt_batch_get_flux", "rt_batch_destroy", "rt_batch_kernel_times",
-           "rt_autotune", "rt_autotune_report"]
+           "rt_autotune", "rt_autotune_report", "rt_solve_cutoff", "rt_problem_get_flux_cutoff",
+           "rt_batch_get_flux_cutoff", "rt_server_poll_cutoff"]
 
 _lib = None
 
@@ -109,6 +110,10 @@ def lib():
     L.rt_server_poll.argtypes = [H, C.c_int64, dp, GS, dp]
     L.rt_server_cancel.argtypes = [H, C.c_int64]
     L.rt_server_destroy.argtypes = [H]
+    L.rt_solve_cutoff.argtypes = [PD, dp, dp, GS]
+    L.rt_problem_get_flux_cutoff.argtypes = [H, dp]
+    L.rt_batch_get_flux_cutoff.argtypes = [H, C.c_int32, dp]
+    L.rt_server_poll_cutoff.argtypes = [H, C.c_int64, dp, dp, GS, dp]
     L.rt_autotune.argtypes = [C.c_int32]
     L.rt_autotune_report.argtypes = [C.c_char_p, C.c_int64]
     L.rt_autotune_report.restype = C.c_int64

@@ -131,6 +131,17 @@ def compute_flux(prob, precision="fp64", state="faithful", **kw):
     return _phi(d, phi), [s.as_dict() for s in st]
 
 
+def compute_flux_cutoff(prob, precision="fp64", state="faithful", **kw):
+    """One-shot solve through rt_solve_cutoff; returns (phi[G, Np, Nc], cutoff[Np, Nc], stats):
+    𝚽cutoff of the slowing-down solver types (sn_flux.jl:213-218), zeros without slowing down."""
+    d = Desc(prob, precision, state, **kw)
+    phi = np.zeros(d.G * d.Np * d.Nc)
+    cut = np.zeros(d.Np * d.Nc)
+    st = (_lib.GroupStats * d.G)()
+    check(lib().rt_solve_cutoff(d.byref(), ptr(phi), ptr(cut), st))
+    return _phi(d, phi), cut.reshape((d.Np, d.Nc), order="F"), [s.as_dict() for s in st]
+
+
 class ResidentProblem:
     """A problem uploaded once (rt_problem_create); solve() keeps everything in HBM."""
 
@@ -149,6 +160,12 @@ class ResidentProblem:
         phi = np.zeros(self.d.G * self.d.Np * self.d.Nc)
         check(lib().rt_problem_get_flux(self.h, ptr(phi)))
         return _phi(self.d, phi)
+
+    def flux_cutoff(self):
+        """𝚽cutoff[Np, Nc] of the last solve (rt_problem_get_flux_cutoff)."""
+        cut = np.zeros(self.d.Np * self.d.Nc)
+        check(lib().rt_problem_get_flux_cutoff(self.h, ptr(cut)))
+        return cut.reshape((self.d.Np, self.d.Nc), order="F")
 
     def kernel_times(self):
         """Per-kernel (ms, launches) of the last solve, [precision][kernel] (RADIANT_HIP_PROFILE=1)."""
@@ -230,6 +247,13 @@ class ResidentBatch:
         check(lib().rt_batch_get_flux(self.h, i, ptr(phi)))
         return _phi(d, phi)
 
+    def flux_cutoff(self, i):
+        """𝚽cutoff[Np, Nc] of problem i after the last solve (rt_batch_get_flux_cutoff)."""
+        d = self.descs[i]
+        cut = np.zeros(d.Np * d.Nc)
+        check(lib().rt_batch_get_flux_cutoff(self.h, i, ptr(cut)))
+        return cut.reshape((d.Np, d.Nc), order="F")
+
     def close(self):
         if self.h:
             lib().rt_batch_destroy(self.h)
@@ -260,8 +284,9 @@ class Server:
         self.descs[rid.value] = (d.G, d.Np, d.Nc)
         return rid.value
 
-    def poll(self, rid):
-        """Returns None while pending, else (phi, stats, latency_ms); raises on failure/cancel.
+    def poll(self, rid, cutoff=False):
+        """Returns None while pending, else (phi, stats, latency_ms), or with cutoff=True
+        (phi, stats, latency_ms, cutoff[Np, Nc]); raises on failure/cancel.
 
         latency_ms is submit -> flux final in device memory; the device -> host copy of the
         collection that follows takes ``self.last_collect_ms``."""
@@ -276,11 +301,14 @@ class Server:
             check(rc)
         t0 = time.perf_counter()
         phi = np.empty(G * Np * Nc)
-        rc = lib().rt_server_poll(self.h, rid, ptr(phi), st, C.byref(lat))  # collects and releases
+        cut = np.empty(Np * Nc) if cutoff else None
+        # collects and releases
+        rc = lib().rt_server_poll_cutoff(self.h, rid, ptr(phi), ptr(cut) if cutoff else None, st, C.byref(lat))
         self.last_collect_ms = (time.perf_counter() - t0) * 1e3
         del self.descs[rid]
         check(rc)
-        return phi.reshape((G, Np, Nc), order="F"), [s.as_dict() for s in st], lat.value
+        out = (phi.reshape((G, Np, Nc), order="F"), [s.as_dict() for s in st], lat.value)
+        return out + (cut.reshape((Np, Nc), order="F"),) if cutoff else out
 
     def cancel(self, rid):
         check(lib().rt_server_cancel(self.h, rid))

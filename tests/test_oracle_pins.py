@@ -286,3 +286,78 @@ def test_openmp_variant_matches_serial(oracle):
     a, _ = oracle.solve(prob, nthreads=1)
     b, _ = oracle.solve(prob, nthreads=4)
     assert rel_err(b, a) < 1e-12
+
+
+def _independent_selection(mu, eta, xi, qdims, thr=1e-5):
+    """Greedy rank selection of Radiant's real harmonics (scipy, unweighted point values) in the
+    traversal order of angular_polynomial_basis.jl:490-527 (l = 1, 2, ...; m from -l, or 0 on the
+    half sphere Qdims = 2, to l): a candidate is kept when the norm of its component orthogonal to
+    the kept ones exceeds thr.  The projection uses an orthonormal basis kept with two passes of
+    re-orthogonalisation (a different algorithm from the reference's one-pass classical
+    Gram-Schmidt).  Returns the kept (l, m) and every candidate's residual norm."""
+    Nd = len(mu)
+    phi = np.arctan2(xi, eta)
+    rl = lambda l, m: np.array([_scipy_rlm(l, m, a, b) for a, b in zip(mu, phi)])
+    v = rl(0, 0)
+    Q = (v / np.linalg.norm(v))[:, None]
+    kept, res = [(0, 0)], []
+    l = 0
+    while len(kept) < Nd and l < 4 * Nd:
+        l += 1
+        for m in range(0 if qdims == 2 else -l, l + 1):
+            v = rl(l, m)
+            r = v - Q @ (Q.T @ v)
+            r = r - Q @ (Q.T @ r)
+            nr = float(np.linalg.norm(r))
+            res.append((l, m, nr))
+            if nr > thr:
+                kept.append((l, m))
+                Q = np.hstack([Q, (r / nr)[:, None]])
+            if len(kept) == Nd:
+                break
+    return kept, res
+
+
+@pytest.mark.parametrize("qt,N,nd,Q", [("gauss-legendre-chebychev", 4, 2, 3), ("gauss-legendre-chebychev", 8, 2, 3),
+                                       ("gauss-legendre-chebychev", 10, 2, 3), ("carlson", 8, 3, 3),
+                                       ("carlson", 10, 3, 3), ("carlson", 12, 3, 3),
+                                       ("gauss-legendre-chebychev", 8, 2, 2)])
+def test_galerkin_selection_independent(oracle, qt, N, nd, Q):
+    """The oracle's Gram-Schmidt selection (pl, pm) equals an independent greedy rank selection.
+    Residual norms within 10x of the 1e-5 threshold would make the comparison depend on rounding:
+    the cases here have none, except GLC-10, whose (l, m) = (14, -14) candidate lies at 2.3e-5
+    and is kept by both."""
+    mu, eta, xi, w = oracle.quadrature(N, qt, nd, Q)
+    Np, M, D, pl, pm = oracle.basis(mu, eta, xi, w, 3, "galerkin-d", Q)
+    kept, res = _independent_selection(mu, eta, xi, Q)
+    assert [(int(a), int(b)) for a, b in zip(pl, pm)] == kept
+    near = [(l, m) for l, m, r in res if 1e-6 < r < 1e-4]
+    assert near == ([(14, -14)] if (qt, N) == ("gauss-legendre-chebychev", 10) else [])
+
+
+@pytest.mark.parametrize("N", [4, 8, 10])
+def test_galerkin_selection_natural_prefix(oracle, N):
+    """GLC-N on the full sphere (N Gauss-Legendre levels in mu, 2N equally spaced azimuths)
+    integrates every product R_lm R_l'm' with l, l' <= N-1 exactly (degree <= 2N-2 in mu,
+    azimuthal frequency <= 2N-2 < 2N), so those N^2 harmonics are independent on the points and
+    the selection keeps all of them, in the natural (l, m) order, before any harmonic of order N."""
+    mu, eta, xi, w = oracle.quadrature(N, "gauss-legendre-chebychev", 2, 3)
+    Np, M, D, pl, pm = oracle.basis(mu, eta, xi, w, 3, "galerkin-d", 3)
+    nat = [(l, m) for l in range(N) for m in range(-l, l + 1)]
+    assert [(int(a), int(b)) for a, b in zip(pl[:N * N], pm[:N * N])] == nat
+    assert pl[N * N] == N
+    # the exactness itself: the weighted Gram matrix of those harmonics is diag(4 pi / (2l + 1))
+    phi = np.arctan2(xi, eta)
+    H = np.array([[_scipy_rlm(l, m, a, b) for a, b in zip(mu, phi)] for l, m in nat])
+    G = (H * w[None, :]) @ H.T
+    assert np.abs(G - np.diag([4 * PI / (2 * l + 1) for l, _ in nat])).max() < 1e-11
+
+
+@pytest.mark.parametrize("N", [12, 14, 16])
+def test_galerkin_glc_high_orders_singular(oracle, N):
+    """Galerkin-D on the 2D full sphere with GLC 12, 14 and 16: D is numerically singular
+    (cond(D) > 1e15), so M = inv(D) is not determined in FP64 -- the reason the measured uneven
+    2D batch draws its quadrature orders from GLC 8 and 10 only."""
+    mu, eta, xi, w = oracle.quadrature(N, "gauss-legendre-chebychev", 2, 3)
+    Np, M, D, pl, pm = oracle.basis(mu, eta, xi, w, 5, "galerkin-d", 3)
+    assert np.linalg.cond(D) > 1e15
