@@ -246,13 +246,19 @@ int rt_server_submit(rt_server *s, const rt_problem_desc *p, int64_t *req);
  * releases the request id, except a finished request polled with phi_out == NULL (a status
  * query that fills stats_out and latency_ms and keeps the flux for a later poll).
  * A finished request keeps its flux moments in device memory, counted against the server's
- * budget, until a poll with phi_out collects them (copied out on a stream of that call).
+ * budget, until a poll with phi_out collects them (copied out on a stream of that call) or
+ * rt_server_cancel drops them.  Queued requests are admitted in arrival order when a slot is
+ * free and their bytes fit beside the running requests and the uncollected results; a request
+ * larger than the whole budget fails with RT_ENOMEM.
  * latency_ms is submit -> flux final in device memory. */
 int rt_server_poll(rt_server *s, int64_t req, double *phi_out, rt_group_stats *stats_out,
                    double *latency_ms);
 /* rt_server_poll that also collects 𝚽cutoff (as rt_solve_cutoff) when cutoff_out is not NULL. */
 int rt_server_poll_cutoff(rt_server *s, int64_t req, double *phi_out, double *cutoff_out,
                           rt_group_stats *stats_out, double *latency_ms);
+/* Cancels a queued or running request; a finished request that no poll has collected releases
+ * its flux moments (their bytes return to the budget at once).  The next poll of the id reports
+ * RT_ECANCELLED and releases it. */
 int rt_server_cancel(rt_server *s, int64_t req);
 int rt_server_destroy(rt_server *s);
 

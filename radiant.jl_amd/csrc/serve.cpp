@@ -439,8 +439,8 @@ struct rt_server {
     bool head_ready() {
         if (queue.empty()) return false;
         auto& r = reqs[queue.front()];
-        return r->cancel.load() || (active < cfg.max_active && bytes_in_use + r->bytes <= cfg.hbm_budget_bytes) ||
-               active == 0;
+        return r->cancel.load() || r->bytes > cfg.hbm_budget_bytes ||
+               (active < cfg.max_active && bytes_in_use + r->bytes <= cfg.hbm_budget_bytes);
     }
 
     // Takes queued requests in arrival order while the head request fits the budget and a slot;
@@ -456,9 +456,7 @@ struct rt_server {
                 r->desc.reset();
                 continue;
             }
-            const bool fits = active < cfg.max_active && bytes_in_use + r->bytes <= cfg.hbm_budget_bytes;
-            if (!fits && active > 0) break;
-            if (!fits && r->bytes > cfg.hbm_budget_bytes) {
+            if (r->bytes > cfg.hbm_budget_bytes) {  // can never fit: refused at once
                 queue.pop_front();
                 r->state = Request::FAILED;
                 r->code = RT_ENOMEM;
@@ -467,6 +465,9 @@ struct rt_server {
                 r->desc.reset();
                 continue;
             }
+            // Otherwise it waits for a slot and for the bytes held by running requests and by
+            // finished results not yet collected (or dropped with rt_server_cancel).
+            if (active >= cfg.max_active || bytes_in_use + r->bytes > cfg.hbm_budget_bytes) break;
             queue.pop_front();
             r->state = Request::RUNNING;
             bytes_in_use += r->bytes;
@@ -742,6 +743,7 @@ int rt_server_poll_cutoff(rt_server* s, int64_t req, double* phi_out, double* cu
 
 int rt_server_cancel(rt_server* s, int64_t req) {
     if (!s) return RT_EINVAL;
+    std::unique_ptr<rt::Problem> dropped;  // a finished, uncollected result: released here
     {
         std::lock_guard<std::mutex> lk(s->mu);
         auto it = s->reqs.find(req);
@@ -749,10 +751,17 @@ int rt_server_cancel(rt_server* s, int64_t req) {
             rt::set_last_error("unknown request id");
             return RT_EINVAL;
         }
-        it->second->cancel.store(true);
+        Request& r = *it->second;
+        r.cancel.store(true);
+        if (r.state == Request::DONE && r.result) {
+            dropped = std::move(r.result);
+            r.state = Request::CANCELLED;
+            s->bytes_in_use -= r.bytes;
+        }
     }
+    int rc = guard([&] { dropped.reset(); });
     s->cv.notify_all();
-    return RT_OK;
+    return rc;
 }
 
 int rt_server_destroy(rt_server* s) {

@@ -310,6 +310,37 @@ class Server:
         out = (phi.reshape((G, Np, Nc), order="F"), [s.as_dict() for s in st], lat.value)
         return out + (cut.reshape((Np, Nc), order="F"),) if cutoff else out
 
+    def finished(self, rid):
+        """Status query (rt_server_poll with no flux buffer): True once the request's flux is final
+        in device memory; the result stays there for a later poll.  Raises on failure/cancel."""
+        G = self.descs[rid][0]
+        st = (_lib.GroupStats * G)()
+        lat = C.c_double()
+        rc = lib().rt_server_poll(self.h, rid, None, st, C.byref(lat))
+        if rc == _lib.RT_EPENDING:
+            return False
+        if rc != _lib.RT_OK:
+            del self.descs[rid]
+            check(rc)
+        return True
+
+    def discard(self, rid):
+        """Drops a finished request's result without copying it out (rt_server_cancel releases its
+        device memory), then releases the id; returns its latency in ms (submit -> flux final)."""
+        G = self.descs[rid][0]
+        st = (_lib.GroupStats * G)()
+        lat = C.c_double()
+        rc = lib().rt_server_poll(self.h, rid, None, st, C.byref(lat))
+        if rc != _lib.RT_OK:
+            del self.descs[rid]
+            check(rc)
+        check(lib().rt_server_cancel(self.h, rid))
+        rc = lib().rt_server_poll(self.h, rid, None, st, None)
+        del self.descs[rid]
+        if rc != _lib.RT_ECANCELLED:
+            check(rc)
+        return lat.value, [s.as_dict() for s in st]
+
     def cancel(self, rid):
         check(lib().rt_server_cancel(self.h, rid))
 

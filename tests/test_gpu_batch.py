@@ -160,3 +160,75 @@ def test_server_budget_refusal_and_cancel():
             time.sleep(0.01)
     finally:
         srv.close()
+
+
+def test_server_budget_gated_wait():
+    """The HBM budget, not the slot count, makes a request wait: with room for one request's
+    bytes, the second stays queued while the first one's finished flux is resident and
+    uncollected, and is admitted and solved once a poll collects the first (or cancel drops it)."""
+    from radiant_hip import solver as S
+    from radiant_hip import problems as P
+    p1 = P.config_2d(N=48, G=4, L=3, quad_order=8, seed=41)
+    p2 = P.config_2d(N=40, G=3, L=2, quad_order=8, seed=42)
+    p3 = P.config_2d(N=32, G=3, L=2, quad_order=4, seed=43)
+    b = [S.Desc(p, "mixed-fp32", "reduced").device_bytes() for p in (p1, p2, p3)]
+    budget = max(b[0] + b[1], b[1] + b[2]) - 1
+    assert budget >= max(b) and b[0] + b[1] > budget and b[1] + b[2] > budget
+    srv = S.Server(hbm_budget_bytes=budget, max_active=8, precision="mixed-fp32")
+    try:
+        r1 = srv.submit(p1)
+        r2 = srv.submit(p2)
+        t0 = time.time()
+        while not srv.finished(r1):
+            assert time.time() - t0 < 60
+            time.sleep(0.005)
+        time.sleep(0.5)
+        assert not srv.finished(r2), "admitted although the budget holds the first result"
+        phi1 = srv.poll(r1)[0]  # collecting frees the budget
+        t0 = time.time()
+        r = None
+        while r is None:
+            assert time.time() - t0 < 60
+            r = srv.poll(r2)
+            time.sleep(0.005)
+        _say("  budget-gated request latency ms", round(r[2], 1))
+        for phi, p in ((phi1, p1), (r[0], p2)):
+            ref, _ = S.compute_flux(p)
+            assert rel_err(phi, ref) < 1e-6
+        # a finished result dropped with cancel also returns its bytes to the budget
+        r3 = srv.submit(p3)
+        r4 = srv.submit(p2)
+        t0 = time.time()
+        while not srv.finished(r3):
+            assert time.time() - t0 < 60
+            time.sleep(0.005)
+        time.sleep(0.3)
+        assert not srv.finished(r4)
+        lat3, st3 = srv.discard(r3)
+        assert lat3 > 0 and all(s["converged"] for s in st3)
+        t0 = time.time()
+        r = None
+        while r is None:
+            assert time.time() - t0 < 60
+            r = srv.poll(r4)
+            time.sleep(0.005)
+        assert rel_err(r[0], S.compute_flux(p2)[0]) < 1e-6
+    finally:
+        srv.close()
+
+
+def test_server_destroy_with_uncollected_results():
+    """Destroying a server that still holds finished, uncollected results releases them before
+    the engine's stream goes; the library keeps working afterwards."""
+    from radiant_hip import solver as S
+    from radiant_hip import problems as P
+    probs = [P.config_2d(N=24, G=3, L=2, quad_order=4, seed=51 + k) for k in range(3)]
+    srv = S.Server(hbm_budget_bytes=2 * 1024 ** 3, max_active=3, precision="mixed-fp32")
+    ids = [srv.submit(p) for p in probs]
+    t0 = time.time()
+    while not all(srv.finished(r) for r in ids):
+        assert time.time() - t0 < 60
+        time.sleep(0.005)
+    srv.close()
+    phi, _ = S.compute_flux(probs[0])
+    assert np.isfinite(phi).all()
