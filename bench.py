@@ -1,17 +1,20 @@
 """Benchmark of the SN transport solve on one MI355X.
 
-Workload (a "step" = one complete fixed-source solve, every group): 2D 128x128 cells on a
-10x10 cm two-material checkerboard, 20 groups, P5 downscatter, Gauss-Legendre-Chebyshev 8
-(128 directions), diamond difference, GMRES(30), tol 1e-10, void boundaries.
+Headline workload (a "step" = one complete fixed-source solve, every group): C2, 3D 64^3 cells,
+3 materials on a seeded random voxel map, 40 groups, P3 downscatter, Carlson S12 (168
+directions), diamond difference, GMRES(30), tol 1e-10, void faces.
 
 Reported:
-  * plain FP64 solve (faithful Krylov state, the parity mode) -- the reference point;
-  * mixed precision (FP32 sweep + FP32 M/D, FP64 Arnoldi and FP64 iterative refinement) --
-    the headline ``value`` when its flux-moment error vs the FP64 solve is <= 1e-6;
-  * the BF16-M/D variant (reported beside it);
-  * p50/p99 latency per solve, solves/s, problem-normalised cell·direction·group updates/s;
+  * mixed precision (FP32 sweep + FP32 M/D, FP64 Arnoldi, FP64 iterative refinement and
+    reconstruction) -- the headline ``value`` (problem-normalised cell*direction*group updates/s),
+    with its flux-moment error against the FP64 solve;
+  * the plain FP64 solve (faithful Krylov state, the parity mode) -- the reference point -- and
+    the FP64 solve on the reduced state; the BF16-M/D variant;
+  * p50/p99 latency per solve, solves/s;
   * the roofline of the dominant kernel from HIP events recorded on the solve's stream;
-  * a CPU baseline: the FP64 oracle on this host, on a bounded sample (the first groups).
+  * C3 (256 uneven 2D problems as one lockstep batch, against the same batch in FP64) and C4
+    (the Poisson 50/s stream of 3D requests through the server, p50/p99 latency);
+  * a CPU baseline: the FP64 oracle on this host, one timed transport pass per case.
 
 Inputs are uploaded once (rt_problem_create) and stay in HBM during the timed region.
 """
@@ -29,19 +32,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "cell·dir·group updates/s + solves/s, p50/p99 latency, at ≤1e-6 rel flux err"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the dominant kernel of the default batched workload (32 problems),
-# from rocprofv3 --pmc runs over that workload: FETCH_SIZE and WRITE_SIZE (KiB per dispatch,
-# averaged over the kernel's dispatches), each scaled by what the counter reads for a plain 1 GiB
-# device copy (FETCH_SIZE reads 0.5 GiB for it: factor 2.000; WRITE_SIZE 1 GiB: factor 1.000).
-MEASURED_TRAFFIC = {("sweep_fp32", 32): dict(fetch=346.30e6, write=309.41e6,
-                                             kernel="sweep_line_mixed_kernel<2, 8, true>",
-                                             summary="profiles/sweep_line_mixed_c1_batch32.md"),
-                    ("dgemm_fp32", 32): dict(fetch=482.76e6, write=174.70e6,
-                                             kernel="contract_mixed_kernel<4, 4, false, 4>",
-                                             summary="profiles/contractions_mfma_c1_batch32.md"),
-                    ("mgemm_fp32", 32): dict(fetch=141.22e6, write=303.39e6,
-                                             kernel="contract_mixed_kernel<4, 4, true, 2>",
-                                             summary="profiles/contractions_mfma_c1_batch32.md")}
+# HBM bytes per launch of the dominant kernel of the headline workload, from rocprofv3 --pmc runs over
+# that workload (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, averaged over the kernel's
+# dispatches), each scaled by what the counter reads for a plain 1 GiB device copy.  Keyed by
+# (workload, kernel class); reported only when the library sources match the measured digest.
+MEASURED_TRAFFIC = {}
 MFMA_PEAK_TFLOPS = {0: 78.6, 1: 157.3, 2: 2500.0}
 KNAMES = ["mgemm", "sweep", "dgemm", "bndmom"]
 PNAMES = ["fp64", "fp32", "bf16"]
@@ -94,7 +89,7 @@ def run_mode(prob, precision, state, steps, warmup, label):
     return out, phi
 
 
-def run_batch(probs, precision, steps, warmup, label):
+def run_batch(probs, precision, steps, warmup, label, kernel_timing=True, keep=(0, 1)):
     """One step = one lockstep solve of the whole batch (inputs resident in HBM).
 
     The library splits a batch into concurrent sub-batches.  Kernel durations are measured on one
@@ -119,8 +114,16 @@ def run_batch(probs, precision, steps, warmup, label):
         log(f"[{label}] step {i + 1}/{steps}: {lat[-1] * 1e3:.1f} ms")
     wall = time.perf_counter() - t0
     nupd = sum(p.Nc * d.Nd * p.G for p, d in zip(probs, rb.descs))
-    phis = [rb.flux(0), rb.flux(1)]
+    phis = [rb.flux(i) for i in keep]
+    Nd0 = rb.descs[0].Nd
     rb.close()
+    out = dict(wall_s=wall, steps=steps, batch=len(probs), p50_ms=float(np.percentile(lat, 50) * 1e3),
+               p99_ms=float(np.percentile(lat, 99) * 1e3), updates_per_s=nupd * steps / wall,
+               solves_per_s=len(probs) * steps / wall, raw_updates_per_s=info.updates_raw * steps / wall,
+               passes=int(info.passes), passes_lo=int(info.passes_lo),
+               converged=all(all(s["converged"] for s in st) for st in stats), Nd=Nd0)
+    if not kernel_timing:
+        return out, phis
     # kernel timing: the same problems as one sub-batch, one solve
     saved = {k: os.environ.get(k) for k in ("RADIANT_HIP_SUBBATCH", "RADIANT_HIP_PROFILE")}
     os.environ["RADIANT_HIP_SUBBATCH"] = "1"
@@ -137,73 +140,79 @@ def run_batch(probs, precision, steps, warmup, label):
             else:
                 os.environ[k] = v
     log(f"[{label}] kernel-timing solve (one sub-batch): {pinfo.ms_device:.1f} ms")
-    out = dict(wall_s=wall, steps=steps, batch=len(probs), p50_ms=float(np.percentile(lat, 50) * 1e3),
-               p99_ms=float(np.percentile(lat, 99) * 1e3), updates_per_s=nupd * steps / wall,
-               solves_per_s=len(probs) * steps / wall, raw_updates_per_s=info.updates_raw * steps / wall,
-               passes=int(info.passes), passes_lo=int(info.passes_lo),
-               converged=all(all(s["converged"] for s in st) for st in stats),
-               kernel_ms=kms, kernel_launches=kn, kernel_updates=kupd, kernel_flops=kfl, kernel_bytes=kby,
-               profiled_ms=pinfo.ms_device, Nd=rb.descs[0].Nd,
+    out.update(kernel_ms=kms, kernel_launches=kn, kernel_updates=kupd, kernel_flops=kfl, kernel_bytes=kby,
+               profiled_ms=pinfo.ms_device,
                kernel_timing="one extra solve of the same batch as a single sub-batch (kernels run alone)")
     return out, phis
 
 
-def run_serve(nreq, rate, budget_gib, max_active, precision, ncheck=2):
+def run_serve(nreq, rate, budget_gib, max_active, precision, sample=(2, 5)):
     """Served stream: the first `nreq` requests of the 3D Carlson stream (N~U{32..96} per axis,
     G~U{10..40}, Carlson 8/10/12, P3, 3 random materials) submitted at Poisson arrival times
     (rate per second) to rt_server.  Latency = submit -> flux final in device memory (the server
-    keeps results resident); the host collection that follows (device -> host copy of the full
-    flux moments) is timed separately.  The first `ncheck` results collected are checked against
-    FP64 solves of the same requests."""
+    keeps results resident).  The results of the `sample` requests are collected (device -> host
+    copy of the full flux moments, timed separately) and checked against FP64 solves of the same
+    requests, which also give the plain FP64 reference rate (one request at a time); every other
+    result is dropped in device memory once final (rt_server_cancel), so that the stream's cost is
+    the solves, not the host copies."""
     from radiant_hip import problems, solver
     t_arr = problems.poisson_arrivals(nreq, rate=rate)
     srv = solver.Server(hbm_budget_bytes=int(budget_gib * 1024 ** 3), max_active=max_active, precision=precision)
-    ids, nupd, lat, t_sub, t_col, col_ms, keep = [], {}, {}, {}, {}, {}, {}
+    ids, nupd, lat, t_sub, col_ms, keep = [], {}, {}, {}, {}, {}
     t0 = time.perf_counter()
+    last = t0
     try:
-        for i in range(nreq):
-            p = problems.config4_request(i)
-            wait = t_arr[i] - (time.perf_counter() - t0)
-            if wait > 0:
-                time.sleep(wait)
-            t_sub[i] = time.perf_counter() - t0
-            rid = srv.submit(p)
-            ids.append(rid)
-            nupd[rid] = p.Nc * srv.descs[rid][1] * p.G
-            log(f"[serve] submit {i} (N={p.Nx}, G={p.G}, Nd={srv.descs[rid][1]}) at {t_sub[i]:.2f}s")
-            del p
-        while len(lat) < len(ids):
+        k_next = 0
+        while len(lat) < nreq:
+            now = time.perf_counter() - t0
+            while k_next < nreq and t_arr[k_next] <= now:
+                p = problems.config4_request(k_next)
+                t_sub[k_next] = time.perf_counter() - t0
+                rid = srv.submit(p)
+                ids.append(rid)
+                nupd[rid] = p.Nc * srv.descs[rid][1] * p.G
+                del p
+                k_next += 1
             for k, rid in enumerate(ids):
-                if rid in lat:
+                if rid in lat or not srv.finished(rid):
                     continue
-                r = srv.poll(rid)
-                if r is not None:
+                if k in sample:
+                    r = srv.poll(rid)
                     lat[rid] = r[2]
                     col_ms[rid] = srv.last_collect_ms
-                    t_col[rid] = time.perf_counter() - t0 - t_sub[k]
-                    log(f"[serve] request {rid} done: {r[2]:.0f} ms in device memory, collected after "
-                        f"{t_col[rid] * 1e3:.0f} ms (copy-out {col_ms[rid]:.0f} ms)")
-                    if len(keep) < ncheck:
-                        keep[k] = r[0]
+                    keep[k] = r[0]
                     del r
+                else:
+                    lat[rid], _ = srv.discard(rid)
+            if time.perf_counter() - last > 10:
+                last = time.perf_counter()
+                log(f"[serve] {len(lat)}/{nreq} served, {k_next} submitted at {last - t0:.1f}s")
             time.sleep(0.002)
         wall = time.perf_counter() - t0
     finally:
         srv.close()
     ls = np.array([lat[r] for r in ids])
-    lc = np.array([t_col[r] * 1e3 for r in ids])
     t_hbm = max(t_sub[k] + lat[rid] * 1e-3 for k, rid in enumerate(ids))  # last result final in HBM
-    errs = []
+    errs, f_upd, f_t = [], 0.0, 0.0
     for k, phi in keep.items():
-        ref, _ = solver.compute_flux(problems.config4_request(k))
+        p = problems.config4_request(k)
+        rp = solver.ResidentProblem(p, precision="fp64", state="faithful")
+        t1 = time.perf_counter()
+        rp.solve()
+        f_t += time.perf_counter() - t1
+        f_upd += p.Nc * rp.d.Nd * p.G
+        ref = rp.flux()
+        rp.close()
         errs.append(float(np.linalg.norm(phi - ref) / np.linalg.norm(ref)))
-        log(f"[serve] request {k}: flux err vs FP64 {errs[-1]:.3g}")
+        log(f"[serve] request {k} (N={p.Nx}, G={p.G}): flux err vs FP64 {errs[-1]:.3g}")
+        del ref
     return dict(requests=nreq, rate_per_s=rate, wall_s=wall, t_hbm_s=t_hbm, requests_per_s=nreq / t_hbm,
                 updates_per_s=sum(nupd.values()) / t_hbm, p50_ms=float(np.percentile(ls, 50)),
-                p99_ms=float(np.percentile(ls, 99)), collected_requests_per_s=nreq / wall,
-                collected_p50_ms=float(np.percentile(lc, 50)), collected_p99_ms=float(np.percentile(lc, 99)),
-                copy_out_ms_total=float(sum(col_ms.values())), flux_err_checked=errs, budget_gib=budget_gib,
-                max_active=max_active, precision=precision)
+                p99_ms=float(np.percentile(ls, 99)), latency="submit -> flux final in device memory",
+                collected={str(k): round(col_ms[ids[k]], 1) for k in keep}, flux_err_vs_fp64=errs,
+                fp64_one_at_a_time_updates_per_s=(f_upd / f_t) if f_t else None,
+                fp64_reference=f"FP64 faithful solves of requests {list(keep)}, one at a time",
+                budget_gib=budget_gib, max_active=max_active, precision=precision)
 
 
 def kernel_work(prob, res, nact):
@@ -299,23 +308,70 @@ def kernel_table(res):
     return t
 
 
-def cpu_baseline(prob, groups, nthreads):
-    """FP64 oracle on this host: the first `groups` groups of the same workload."""
-    import copy
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads():
+    """Host threads of this process's CPU share (OMP_NUM_THREADS where set, else every core)."""
+    n = os.cpu_count() or 1
+    e = os.environ.get("OMP_NUM_THREADS")
+    return max(1, min(n, int(e))) if e and e.isdigit() else n
+
+
+def cpu_baseline(G, passes_per_solve, cases):
+    """The FP64 oracle on this host: one timed transport pass T(0) of group 1 of the same 3D path
+    (Carlson 12, P3, 3 random materials, 40 groups) per (grid N, threads) case.  The pass rate is
+    scaled to problem-normalised updates/s with the pass count of the FP64 solve of the workload
+    (the oracle's solve makes the same passes; its Krylov work, left out, only adds CPU time)."""
     from oracle import oracle as O
-    sub = copy.deepcopy(prob)
-    sub.G = groups
-    sub.sigt = prob.sigt[:groups].copy()
-    sub.sigs = prob.sigs[:, :groups, :groups, :].copy()
-    sub.qvol = prob.qvol[:groups].copy()
-    t0 = time.perf_counter()
-    O.solve(sub, nthreads=nthreads)
-    wall = time.perf_counter() - t0
-    nd, _, _, _ = O.sizes(sub)
-    return dict(value=prob.Nc * nd * groups / wall, unit="cell·dir·group updates/s", cores=nthreads,
-                kind="port", sample=f"first {groups} of {prob.G} groups of the same 2D 128x128 GLC-8 P5 solve "
-                f"(FP64 oracle, {nthreads} thread{'s' if nthreads > 1 else ''}), {wall:.1f} s wall",
-                wall_s=wall)
+    from radiant_hip import problems
+    out = []
+    for N, nth in cases:
+        sub = problems.config_3d(N=N, width=0.1 * N, G=G, L=3, quad_order=12, Nmat=3, seed=20260103)
+        nd, np_, _, S = O.sizes(sub)
+        z = np.zeros(S)
+        q = np.zeros(np_ * sub.Nc)
+        q[::np_] = sub.qvol[0]
+        _, sec = O.timed_pass(sub, 0, 0, z, q, nthreads=nth)
+        raw = sub.Nc * nd / sec
+        out.append(dict(value=raw * G / passes_per_solve, unit="cell·dir·group updates/s (problem-normalised)",
+                        cores=nth, kind="port", cpu=cpu_model(), raw_pass_updates_per_s=raw, pass_s=sec,
+                        sample=f"one FP64 oracle transport pass (T(0), group 1) of the C2 path on {N}^3 cells x "
+                               f"{nd} directions, {nth} thread{'s' if nth > 1 else ''}, {sec:.1f} s; scaled by "
+                               f"{G}/{passes_per_solve} (groups / passes of the FP64 solve)"))
+        log(f"[cpu] {N}^3 on {nth} threads: pass {sec:.2f} s, {out[-1]['value']:.3e} updates/s")
+    return out
+
+
+def source_digest():
+    """sha256 of the library sources (first 16 hex digits): measured HBM traffic is reported only
+    for the code it was measured on."""
+    import hashlib
+    h = hashlib.sha256()
+    d = os.path.join(ROOT, "radiant.jl_amd", "csrc")
+    for f in sorted(os.listdir(d)):
+        h.update(f.encode())
+        with open(os.path.join(d, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def summary(r):
+    return {k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in r.items()
+            if k in ("updates_per_s", "solves_per_s", "p50_ms", "p99_ms", "raw_updates_per_s", "passes",
+                     "passes_lo", "niter", "refinements", "converged", "err_vs_fp64", "batch", "ms_per_solve")}
+
+
+C2_WORKLOAD = ("C2: 3D 64x64x64 cells on a 6.4 cm cube, 3 materials on a seeded random voxel map, 40 groups, "
+               "P3 downscatter, Carlson S12 (168 directions, Galerkin-D), DD, GMRES(30) tol 1e-10, void "
+               "faces, unit source in the central 16^3 block of group 1; one step = one full solve")
 
 
 def main():
@@ -324,19 +380,18 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-groups", type=int, default=2)
     ap.add_argument("--skip-bf16", action="store_true")
-    ap.add_argument("--batch", type=int, default=32, help="problems per lockstep batch (1 = no batched mode)")
     ap.add_argument("--headline-only", action="store_true",
-                    help="run only the batched headline mode (for kernel profiling of exactly that mode)")
-    ap.add_argument("--c3", type=int, default=0,
-                    help="instead of the C1 benchmark, solve the first N problems of the uneven 2D batch "
-                         "(N~U{32..160} per axis, G~U{8..32}, P1..P5, GLC 8/10) in one lockstep batch")
-    ap.add_argument("--serve", type=int, default=0,
-                    help="instead of the solve benchmark, serve this many requests of the 3D request stream")
+                    help="run only the headline mode (C2 mixed FP32), for kernel profiling of exactly that mode")
+    ap.add_argument("--c3", type=int, default=256,
+                    help="problems of the uneven 2D batch (N~U{32..160} per axis, G~U{8..32}, P1..P5, GLC 8/10) "
+                         "solved as one lockstep batch beside the headline (0: skip)")
+    ap.add_argument("--c4", type=int, default=200, help="requests of the 3D served stream (0: skip)")
     ap.add_argument("--rate", type=float, default=50.0, help="served stream: Poisson arrivals per second")
     ap.add_argument("--budget-gib", type=float, default=200.0, help="served stream: server HBM budget")
     ap.add_argument("--max-active", type=int, default=8, help="served stream: resident requests at most")
+    ap.add_argument("--c1-batch", type=int, default=0,
+                    help="also run the 2D C1 workload as a lockstep batch of this many problems")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 3 device-to-device copies of BYTES (the reference for scaling the HBM "
                          "byte counters FETCH_SIZE / WRITE_SIZE)")
@@ -357,130 +412,109 @@ def main():
                                    "spatial KBA with point-to-point face exchange)", "n_gpus": args.gpus}))
         sys.exit(2)
     from radiant_hip import problems, solver
-    log(solver.version())
-    if args.c3 > 0:
-        # GLC 12/14/16 members are numerically singular Galerkin-D operators (cond(D) ~ 1e18): the
-        # measured batch draws its quadrature orders from GLC 8 and 10.
-        probs = problems.config3(args.c3, quad_orders=(8, 10))
-        t0 = time.perf_counter()
-        r, _ = run_batch(probs, "mixed-fp32", args.steps, args.warmup, f"c3x{args.c3}")
-        log(f"[c3] total {time.perf_counter() - t0:.1f} s")
-        print(json.dumps({
-            "metric": "cell·dir·group updates/s (uneven batch)", "value": r["updates_per_s"],
-            "unit": "cell·dir·group updates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": r["wall_s"] / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": "f32 (sweep, M/D contractions) + f64 (Krylov, refinement)",
-            "data": "synthetic (seeded splitmix64)",
-            "config": {"workload": f"first {args.c3} problems of the uneven 2D batch (GLC 8/10)", "batch": args.c3},
-            "solves_per_s": r["solves_per_s"], "raw_updates_per_s": r["raw_updates_per_s"],
-            "passes": r["passes"], "converged": r["converged"]},
-            default=lambda o: o.tolist() if hasattr(o, "tolist") else str(o)))
-        return
-    if args.serve > 0:
-        r = run_serve(args.serve, args.rate, args.budget_gib, args.max_active, "mixed-fp32")
-        print(json.dumps({
-            "metric": "served requests/s and p50/p99 latency", "value": r["requests_per_s"], "unit": "requests/s",
-            "n_gpus": 1, "steps": args.serve, "warmup": 0, "ms_per_step": r["t_hbm_s"] / args.serve * 1e3,
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": "fp32 sweep+M/D, fp64 Arnoldi/refinement", "data": "synthetic (seeded splitmix64)",
-            "config": {"workload": f"first {args.serve} requests of the 3D Carlson request stream "
-                                   f"(Poisson {args.rate}/s)", "budget_gib": args.budget_gib,
-                       "max_active": args.max_active},
-            "p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"], "updates_per_s": r["updates_per_s"],
-            "latency": "submit -> flux final in device memory",
-            "host_collected": {"requests_per_s": r["collected_requests_per_s"], "p50_ms": r["collected_p50_ms"],
-                               "p99_ms": r["collected_p99_ms"], "copy_out_ms_total": r["copy_out_ms_total"]},
-            "flux_err_checked": r["flux_err_checked"]}))
-        return
-    prob = problems.config1()
-    steps, warm = args.steps, args.warmup
-
-    from radiant_hip.solver import compute_flux
-    # The batched headline runs first, in a fresh process state: it then sees the same device
-    # state as it does alone (runs measured after the single-problem modes were ~10% slower).
-    rbt = None
-    if args.batch > 1:
-        bprobs = [prob] + [problems.config_2d(seed=20260102 + i) for i in range(1, args.batch)]
-        rbt, bphis = run_batch(bprobs, "mixed-fp32", steps, warm, f"batch{args.batch}-fp32")
-    if args.headline_only:
-        phi64 = None  # no FP64 reference solve: its kernels would mix into a kernel profile
-        r64 = rmx = rbf = None
-    else:
-        r64, phi64 = run_mode(prob, "fp64", "faithful", steps, warm, "fp64")
-        r64["faithful"] = True
-        rmx, phim = run_mode(prob, "mixed-fp32", "reduced", steps, warm, "mixed-fp32")
-        rmx["err_vs_fp64"] = rel_err(phim, phi64)
-        log(f"[mixed-fp32] rel flux-moment err vs fp64 = {rmx['err_vs_fp64']:.3e}")
-        rbf = None
-        if not args.skip_bf16:
-            rbf, phib = run_mode(prob, "mixed-bf16", "reduced", max(1, steps // 2), warm, "mixed-bf16")
-            rbf["err_vs_fp64"] = rel_err(phib, phi64)
-            log(f"[mixed-bf16] rel flux-moment err vs fp64 = {rbf['err_vs_fp64']:.3e}")
-    if rbt is not None:
-        if phi64 is not None:
-            ref1, _ = compute_flux(bprobs[1])
-            rbt["err_vs_fp64"] = max(rel_err(bphis[0], phi64), rel_err(bphis[1], ref1))
-            log(f"[batch] rel flux-moment err vs fp64 = {rbt['err_vs_fp64']:.3e}")
-        else:
-            rbt["err_vs_fp64"] = None
-
     from radiant_hip.angular import angular_set
-    ang = angular_set(prob.quad_type, prob.quad_order, 2, 3, prob.legendre_order, prob.sn_type)
+    log(solver.version())
+    steps, warm = args.steps, args.warmup
+    prob = problems.config2()
+    ang = angular_set(prob.quad_type, prob.quad_order, 3, 3, prob.legendre_order, prob.sn_type)
     nact = int(np.sum(ang.pl <= prob.Lxs))
 
-    if args.headline_only:
-        if rbt is None:
-            raise SystemExit("--headline-only needs --batch > 1")
-        single, head, head_name = None, rbt, f"mixed-fp32 batched x{args.batch}"
-        roof, roof64 = kernel_roofline(prob, head, nact), None
-    else:
-        single = rmx if rmx["err_vs_fp64"] <= 1e-6 else r64
-        head = rbt if (rbt is not None and rbt["err_vs_fp64"] <= 1e-6 and rbt["converged"]) else single
-        head_name = (f"mixed-fp32 batched x{args.batch}" if head is rbt else
-                     ("mixed-fp32" if single is rmx else "fp64"))
-        roof = kernel_roofline(prob, head, nact)
-        roof64 = kernel_roofline(prob, r64, nact)
-    mt = MEASURED_TRAFFIC.get((roof.get("kernel"), args.batch)) if head is rbt else None
-    if mt is not None:
+    # ---- C2: the headline (mixed FP32) first, in a fresh process state
+    rmx, phim = run_mode(prob, "mixed-fp32", "reduced", steps, warm, "c2 mixed-fp32")
+    roof = kernel_roofline(prob, rmx, nact)
+    digest = source_digest()
+    mt = MEASURED_TRAFFIC.get(("C2 mixed-fp32", roof["kernel"]))
+    if mt is not None and mt["digest"] == digest:
         roof["traffic"] = mt["fetch"] + mt["write"]
-        roof["traffic_source"] = (f"rocprofv3 --pmc FETCH_SIZE x2.000 + WRITE_SIZE x1.000 per dispatch of "
-                                  f"{mt['kernel']} over this workload ({mt['summary']})")
+        roof["traffic_source"] = (f"rocprofv3 --pmc FETCH_SIZE x{mt['fetch_scale']:.3f} + WRITE_SIZE "
+                                  f"x{mt['write_scale']:.3f} per dispatch of {mt['name']} over this workload "
+                                  f"({mt['summary']}, sources {digest})")
+    elif mt is not None:
+        roof["traffic_source"] = f"not reported: measured on sources {mt['digest']}, these are {digest}"
+    if args.headline_only:
+        print(json.dumps({"metric": METRIC, "value": rmx["updates_per_s"], "roofline": roof,
+                          "kernels": kernel_table(rmx)}, default=lambda o: o.tolist() if hasattr(o, "tolist") else str(o)))
+        return
+    r64, phi64 = run_mode(prob, "fp64", "faithful", steps, warm, "c2 fp64")
+    r64["faithful"] = True
+    rmx["err_vs_fp64"] = rel_err(phim, phi64)
+    del phim
+    log(f"[c2 mixed-fp32] rel flux-moment err vs fp64 = {rmx['err_vs_fp64']:.3e}")
+    r64r, phir = run_mode(prob, "fp64", "reduced", steps, warm, "c2 fp64-reduced")
+    r64r["err_vs_fp64"] = rel_err(phir, phi64)
+    del phir
+    rbf = None
+    if not args.skip_bf16:
+        rbf, phib = run_mode(prob, "mixed-bf16", "reduced", max(1, steps // 2), warm, "c2 mixed-bf16")
+        rbf["err_vs_fp64"] = rel_err(phib, phi64)
+        del phib
+        log(f"[c2 mixed-bf16] rel flux-moment err vs fp64 = {rbf['err_vs_fp64']:.3e}")
+    del phi64
+    roof64 = kernel_roofline(prob, r64, nact)
+    for r in (rmx, r64, r64r, rbf):
+        if r is not None:
+            r["ms_per_solve"] = r["wall_s"] / r["steps"] * 1e3
 
-    cpu = None
+    # ---- C3: the uneven 2D batch, lockstep mixed FP32 vs the same batch in FP64
+    c3 = None
+    if args.c3 > 0:
+        probs = problems.config3(args.c3, quad_orders=(8, 10))
+        r3, ph3 = run_batch(probs, "mixed-fp32", 2, 1, f"c3x{args.c3} mixed-fp32", kernel_timing=False)
+        r3f, ph3f = run_batch(probs, "fp64", 1, 0, f"c3x{args.c3} fp64", kernel_timing=False)
+        r3["err_vs_fp64"] = max(rel_err(a, b) for a, b in zip(ph3, ph3f))
+        del ph3, ph3f
+        c3 = dict(workload=f"C3: the first {args.c3} problems of the uneven 2D batch (N~U{{32..160}} per axis, "
+                           f"G~U{{8..32}}, P1..P5, GLC 8/10: GLC 12/14/16 have cond(D) > 1e15)",
+                  mixed_fp32_lockstep=summary(r3), fp64_batched=summary(r3f),
+                  speedup_vs_fp64_batched=r3["updates_per_s"] / r3f["updates_per_s"],
+                  speedup_vs_c2_fp64_plain=r3["updates_per_s"] / r64["updates_per_s"],
+                  err_checked="problems 0 and 1 vs their FP64 solves in the FP64 batch")
+        log(f"[c3] {r3['updates_per_s']:.3e} updates/s mixed lockstep, {r3f['updates_per_s']:.3e} FP64 batched")
+
+    # ---- C1 as a lockstep batch (optional)
+    c1b = None
+    if args.c1_batch > 1:
+        p1 = problems.config1()
+        bprobs = [p1] + [problems.config_2d(seed=20260102 + i) for i in range(1, args.c1_batch)]
+        rb1, _ = run_batch(bprobs, "mixed-fp32", steps, warm, f"c1 batch{args.c1_batch}", kernel_timing=False)
+        c1b = summary(rb1)
+
+    # ---- C4: the served 3D stream
+    c4 = None
+    if args.c4 > 0:
+        c4 = run_serve(args.c4, args.rate, args.budget_gib, args.max_active, "mixed-fp32")
+        if c4.get("fp64_one_at_a_time_updates_per_s"):
+            c4["speedup_vs_fp64_one_at_a_time"] = c4["updates_per_s"] / c4["fp64_one_at_a_time_updates_per_s"]
+
+    cpu = cpu_all = None
     if not args.no_cpu:
         log("[cpu] oracle baseline ...")
-        cpu = cpu_baseline(prob, args.cpu_groups, 1)
-        log(f"[cpu] {cpu['value']:.3e} updates/s ({cpu['wall_s']:.1f} s)")
+        nth = host_threads()
+        cpu, cpu_all = cpu_baseline(prob.G, r64["passes"], [(40, 1), (64, nth)])
 
-    def summary(r):
-        return {k: (float(f"{v:.5g}") if isinstance(v, float) else v) for k, v in r.items()
-                if k in ("updates_per_s", "solves_per_s", "p50_ms", "p99_ms", "raw_updates_per_s", "passes",
-                         "passes_lo", "niter", "refinements", "converged", "err_vs_fp64", "batch")}
-
+    head = rmx
     out = {
         "metric": METRIC, "value": head["updates_per_s"], "unit": "cell·dir·group updates/s",
         "n_gpus": 1, "steps": steps, "warmup": warm, "ms_per_step": head["wall_s"] / steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 (sweep, M/D contractions) + f64 (Krylov, refinement)" if head is not r64 else "f64",
-        "data": "synthetic (seeded splitmix64 cross-sections and geometry)",
-        "config": {"workload": "2D 128x128 cells, 10x10 cm checkerboard of 2 materials (16x16-cell blocks), "
-                               "20 groups, P5 downscatter, GLC-8 (128 directions), DD, GMRES(30) tol 1e-10, "
-                               "void BCs, unit source in the central 32x32 block; one step = one full solve "
-                               "(batched mode: one lockstep solve of `batch` such problems with distinct seeds)",
-                   "mode": head_name, "batch": head.get("batch", 1)},
+        "dtype": "f32 (sweep, M/D contractions) + f64 (Krylov, refinement, reconstruction)",
+        "data": "synthetic (seeded splitmix64 cross-sections and voxel map)",
+        "config": {"workload": C2_WORKLOAD, "mode": "mixed FP32, reduced Krylov state, FP64 refinement",
+                   "batch": 1},
         "solves_per_s": head["solves_per_s"], "p50_ms": head["p50_ms"], "p99_ms": head["p99_ms"],
-        "rel_flux_err_vs_fp64": head.get("err_vs_fp64"),
-        "speedup_vs_fp64_plain": head["updates_per_s"] / r64["updates_per_s"] if r64 else None,
-        "fp64_plain": summary(r64) if r64 else None, "mixed_fp32": summary(rmx) if rmx else None,
-        "mixed_fp32_batched": summary(rbt) if rbt else None,
+        "rel_flux_err_vs_fp64": head["err_vs_fp64"],
+        "speedup_vs_fp64_plain": head["updates_per_s"] / r64["updates_per_s"],
+        "speedup_vs_fp64_reduced": head["updates_per_s"] / r64r["updates_per_s"],
+        "fp64_plain": summary(r64), "fp64_reduced_state": summary(r64r), "mixed_fp32": summary(rmx),
         "mixed_bf16": summary(rbf) if rbf else None,
-        "kernels_headline": kernel_table(head), "kernels_single_mixed": kernel_table(single) if single else None,
-        "kernels_fp64": kernel_table(r64) if r64 else None,
+        "kernels_headline": kernel_table(head), "kernels_fp64": kernel_table(r64),
         "roofline": roof, "roofline_fp64": roof64,
         "contractions_headline": contraction_rooflines(prob, head, nact),
+        "c3_batch": c3, "c4_served": c4, "c1_batch": c1b,
         "autotune": solver.autotune_report().splitlines(),
-        "cpu_baseline": cpu,
+        "cpu_baseline": cpu, "cpu_baseline_all_cores": cpu_all,
         "gpu_vs_cpu": (head["updates_per_s"] / cpu["value"]) if cpu else None,
+        "source_digest": digest,
     }
     print(json.dumps(out, default=lambda o: o.tolist() if hasattr(o, "tolist") else str(o)))
 

@@ -71,6 +71,7 @@ def lib():
         L.oracle_solve.argtypes = [C.POINTER(OracleProblem), dp, C.POINTER(OracleStats)]
         L.oracle_solve_cutoff.argtypes = [C.POINTER(OracleProblem), dp, dp, C.POINTER(OracleStats)]
         L.oracle_pass.argtypes = [C.POINTER(OracleProblem), C.c_int, C.c_int, dp, dp]
+        L.oracle_pass_timed.argtypes = [C.POINTER(OracleProblem), C.c_int, C.c_int, dp, dp, dp]
         L.oracle_fokker_planck.argtypes = [C.POINTER(OracleProblem), dp, dp, C.c_int]
         L.oracle_cell_1d.argtypes = [C.c_char_p, C.c_int, C.c_double, C.c_double, C.c_double, dp, dp, dp]
         L.oracle_cell_3d.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, dp, dp,
@@ -300,10 +301,20 @@ def solve_cutoff(prob, nthreads=1, verbose=0, **over):
     return phi.reshape((prob.G, np_, nc), order="F"), cut.reshape((np_, nc), order="F"), stats
 
 
-def transport_pass(prob, g, homogeneous, z, qlout):
+def transport_pass(prob, g, homogeneous, z, qlout, nthreads=1):
     """One pass T(z) / A z of group g (0-based) on state z (length S); returns the new state."""
-    op = make_problem(prob)
+    op = make_problem(prob, nthreads=nthreads)
     zz = np.ascontiguousarray(z, np.float64).copy()
     q = np.ascontiguousarray(qlout, np.float64)
     _check(lib().oracle_pass(C.byref(op), g, int(homogeneous), _p(zz), _p(q)))
     return zz
+
+
+def timed_pass(prob, g, homogeneous, z, qlout, nthreads=1):
+    """transport_pass that also returns the wall time (s) of the pass alone, set-up excluded."""
+    op = make_problem(prob, nthreads=nthreads)
+    zz = np.ascontiguousarray(z, np.float64).copy()
+    q = np.ascontiguousarray(qlout, np.float64)
+    sec = C.c_double()
+    _check(lib().oracle_pass_timed(C.byref(op), g, int(homogeneous), _p(zz), _p(q), C.byref(sec)))
+    return zz, sec.value

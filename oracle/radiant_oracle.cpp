@@ -2347,4 +2347,17 @@ int oracle_pass(const oracle_problem* p, int g, int homogeneous, double* z, cons
     } catch (std::exception& e) { g_err = e.what(); return -1; }
 }
 
+// oracle_pass timed: the wall time of the pass alone (the set-up of the solver excluded) in *sec.
+int oracle_pass_timed(const oracle_problem* p, int g, int homogeneous, double* z, const double* Qlout, double* sec) {
+    try {
+        Solver s(*p);
+        vector<double> zz(z, z + s.S), q(Qlout, Qlout + s.nphi), Ql(s.nphi);
+        const double t0 = omp_get_wtime();
+        s.pass(g, homogeneous != 0, zz, q, Ql);
+        *sec = omp_get_wtime() - t0;
+        std::copy(zz.begin(), zz.end(), z);
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
 }  // extern "C"

@@ -439,11 +439,11 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
         rows_per_lane = sweep_rows_per_lane(ndims, Ny);
         for (int pr = 0; pr < 2; ++pr)
             planes_ok[pr] = ndims == 3 && sweep_planes_shape(pr, Ny, Nz, pl_R[pr], pl_ZW[pr], pl_waves[pr]);
-        scan_cpl = sweep_scan_cpl(Nx);
+        scan_cpl = ndims == 3 ? sweep_scan_cpl3d(Nx) : sweep_scan_cpl(Nx);
         // the 1D/2D row-scan kernel addresses a direction's angular flux through one buffer
         // resource, whose byte count is 32-bit
         if (ndims <= 2 && (uint64_t)Nc * 8u > 0xffffffffull) scan_cpl = 0;
-        scan_waves = sweep_scan_waves(ndims, Ny, Nz);
+        scan_waves = sweep_scan_waves(ndims, Ny, Nz, Nx);
         // RADIANT_HIP_SWEEP selects another sweep kernel for testing: s = scan (default), r = wave
         // rows / planes, g = pencil kernel
         sweep_kind = 0;
@@ -716,9 +716,10 @@ void Problem::desc_reduced(PassDesc& pd, int g, int prec, int mode, const double
         set_boundary(pd, g, mode, has_bc ? zin : nullptr, has_bc ? zout : nullptr, (size_t)Nact * Nc);
 }
 
-void Problem::pass_reduced(int g, int prec, int mode, const double* zin, double* zout) {
+void Problem::pass_reduced(int g, int prec, int mode, const double* zin, double* zout, void* psi) {
     PassDesc pd;
     desc_reduced(pd, g, prec, mode, zin, zout);
+    if (psi) pd.psi = psi;  // the pass's angular flux goes there instead of d_psi
     launch_pass(pd, prec, false);
 }
 
@@ -752,6 +753,32 @@ void Problem::pass_reconstruct(int g, const double* zin_reduced) {
     launch_pass(pd, P64, true);
 }
 
+// The sweep is linear in its angular source and inflows, so the angular flux of T(z) is that of
+// A z (the FP64 residual pass on the same z: its inflows come from z's boundary moments) plus
+// that of c = T(0) (volume, surface and energy-inflow sources).  𝚽l = D ψ on all rows, and the
+// energy outflow of the slowing-down types is 2 ψ - ψE_in.
+void Problem::reconstruct_sweep_free(int g, const double* zin_reduced, const double* psi_c, bool have_az) {
+    PassDesc pd;
+    desc_reconstruct(pd, g, zin_reduced);
+    const size_t n = (size_t)Nd * Nc;
+    if (have_az) k_axpy(1.0, psi_c, (double*)d_psi, n, st);
+    else k_copy((double*)d_psi, psi_c, n, st);
+    pd.psi = d_psi;
+    k_set_pd(pd, d_pd, st);
+    if (pd.psiE_out) launch_csd_out(d_pd, (long)n, st);
+    counters.passes += 1;  // a pass without sweep
+    if (profiling) {
+        if (ev_next + 2 > 4096) collect_kernel_times();
+        hipEvent_t a = next_event(), b = next_event();
+        set_launch_events(a, b);
+        launch_dgemm(P64, d_pd, d_dgf, n_dgf, st);
+        set_launch_events(nullptr, nullptr);
+        ev_used.push_back({a, b, 2, P64});
+    } else {
+        launch_dgemm(P64, d_pd, d_dgf, n_dgf, st);
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // Krylov solvers (tools/gmres.jl, anderson.jl, livolant.jl, bicgstab.jl)
 // ------------------------------------------------------------------------------------------
@@ -765,7 +792,7 @@ static double rho_estimate(double r0, double r, int64_t it) {  // krylov_state.j
 // reduces the next dot product); the host reads one H column per Arnoldi step.
 template <class MV>
 void Problem::gmres(size_t n, double* z, MV matvec, const double* c, int m, int64_t maxit_, double tol_,
-                    int64_t& iter, double& resid, bool& conv, double& rho, const std::string& tag) {
+                    int64_t& iter, double& resid, bool& conv, double& rho, const std::string& tag, bool zero_guess) {
     m = std::max(1, m);
     if (m + 1 > 64) throw Error(RT_EINVAL, "gmres restart too large (max 63)");
     const double bnorm = norm(c, n);
@@ -783,10 +810,17 @@ void Problem::gmres(size_t n, double* z, MV matvec, const double* c, int m, int6
     auto h = [&](int i, int j) -> double& { return H[i + (size_t)(m + 1) * j]; };
     iter = 0; resid = 1.0; conv = false;
     double resid0 = std::numeric_limits<double>::quiet_NaN();
+    bool first = true;
     while (iter < maxit_ && !conv) {
-        matvec(r, z); ++iter;
-        k_neg_add(r, c, n, st);  // r = c - r  (state_scale!(-1) ; state_axpy!(1, c))
-        const double beta = norm(r, n);
+        const bool skip = first && zero_guess;  // z = 0: the initial residual is c, no pass needed
+        first = false;
+        if (skip) {
+            k_copy(r, c, n, st);
+        } else {
+            matvec(r, z); ++iter;
+            k_neg_add(r, c, n, st);  // r = c - r  (state_scale!(-1) ; state_axpy!(1, c))
+        }
+        const double beta = skip ? bnorm : norm(r, n);
         resid = beta / bnorm;
         if (std::isnan(resid0)) resid0 = resid;
         if (resid < tol_) { conv = true; break; }
@@ -1090,12 +1124,14 @@ void Problem::solve_group_mixed(int g, rt_group_stats& s) {
     double* r = vec("r", n);
     double* dz = vec("dz", n);
     double* t = vec("t", n);
-    pass_reduced(g, P64, MODE_T, nullptr, c);  // c = T(0) in FP64
+    double* psi_c = vec("psi_c", (size_t)Nd * Nc);
+    pass_reduced(g, P64, MODE_T, nullptr, c, psi_c);  // c = T(0) in FP64, its angular flux kept
     const double bnorm = norm(c, n);
     int64_t niter = 0;
     double resid = 0.0;
     bool conv = true;
     int refs = 0;
+    bool have_az = false;  // d_psi holds the angular flux of A z at the current z
     k_fill(z, 0.0, n, st);
     if (bnorm > 0.0) {
         k_copy(r, c, n, st);
@@ -1109,16 +1145,17 @@ void Problem::solve_group_mixed(int g, rt_group_stats& s) {
             int64_t it = 0;
             double rr, rh;
             bool cv;
-            gmres(n, dz, MVlo, r, restart, maxit - niter, inner_tol, it, rr, cv, rh, "m");
+            gmres(n, dz, MVlo, r, restart, maxit - niter, inner_tol, it, rr, cv, rh, "m", true);
             niter += it;
             k_axpy(1.0, dz, z, n, st);
             pass_reduced(g, P64, MODE_IMA, z, t);  // t = z - A z in FP64
+            have_az = true;
             k_sub(r, c, t, n, st);                  // r = c - (I - A) z
             resid = norm(r, n) / bnorm;
             ++refs;
         }
     }
-    pass_reconstruct(g, z);
+    reconstruct_sweep_free(g, z, psi_c, have_az);
     s.niter = niter; s.resid = resid; s.rho = std::numeric_limits<double>::quiet_NaN();
     s.converged = conv ? 1 : 0;
     s.passes = (int32_t)(counters.passes - p0);

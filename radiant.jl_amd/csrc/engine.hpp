@@ -163,7 +163,7 @@ private:
 
     // passes
     void pass_faithful(int g, int prec, int mode, const double* zin, double* zout);
-    void pass_reduced(int g, int prec, int mode, const double* zin, double* zout);
+    void pass_reduced(int g, int prec, int mode, const double* zin, double* zout, void* psi = nullptr);
     void pass_reconstruct(int g, const double* zin_reduced);
     void desc_reduced(PassDesc& pd, int g, int prec, int mode, const double* zin, double* zout);
     void desc_reconstruct(PassDesc& pd, int g, const double* zin_reduced);
@@ -184,7 +184,10 @@ private:
     // solvers over a state of length n; pass(out, in) applies the operator
     template <class MV>
     void gmres(size_t n, double* z, MV matvec, const double* c, int m, int64_t maxit_, double tol_, int64_t& iter,
-               double& resid, bool& conv, double& rho, const std::string& tag);
+               double& resid, bool& conv, double& rho, const std::string& tag, bool zero_guess = false);
+    // T(z) = A z + T(0) from the angular fluxes of the last A z pass (in d_psi) and of c = T(0)
+    // (psi_c, or psi_c alone when no A z pass ran): one full-row D contraction, no sweep.
+    void reconstruct_sweep_free(int g, const double* zin_reduced, const double* psi_c, bool have_az);
     template <class FP>
     void anderson(size_t n, double* z, FP fixedpoint, int64_t& iter, double& resid, bool& conv, double& rho);
     template <class FP>

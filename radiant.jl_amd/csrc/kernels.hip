@@ -972,6 +972,19 @@ __device__ inline void dpp_compose(T& A, T& B) {
     A = A + B * As;
     B = B * Bs;
 }
+// Shuffle scan within segments of SEG lanes (ls: the lane's index in its segment).
+template <int SEG, typename T>
+__device__ inline void affine_scan_seg(T& A, T& B, int ls) {
+#pragma unroll
+    for (int off = 1; off < SEG; off <<= 1) {
+        const T As = __shfl_up(A, off);
+        const T Bs = __shfl_up(B, off);
+        if (ls >= off) {
+            A = A + B * As;
+            B = B * Bs;
+        }
+    }
+}
 template <bool DPP, typename T>
 __device__ inline void affine_scan(T& A, T& B, int lane) {
     if (DPP) {
@@ -1248,6 +1261,40 @@ __device__ inline void affine_scan_dpp(T& A, T& B) {
         dpp_compose<0x143, 0xC>(A, B);     // row_bcast:31 into rows 2 and 3
     }
 }
+// The same scan within each half of the wave (lanes 0-31 and 32-63 scan separately): the row
+// steps, then row_bcast:15 into rows 1 and 3 only.
+__device__ inline void affine_scan_f32_fused32(float& A, float& B) {
+    asm volatile(
+        "s_nop 1\n\t"
+        "v_fmac_f32_dpp %0, %0, %1 row_shr:1 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mul_f32_dpp %1, %1, %1 row_shr:1 row_mask:0xf bank_mask:0xf\n\t"
+        "s_nop 0\n\t"
+        "v_fmac_f32_dpp %0, %0, %1 row_shr:2 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mul_f32_dpp %1, %1, %1 row_shr:2 row_mask:0xf bank_mask:0xf\n\t"
+        "s_nop 0\n\t"
+        "v_fmac_f32_dpp %0, %0, %1 row_shr:4 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mul_f32_dpp %1, %1, %1 row_shr:4 row_mask:0xf bank_mask:0xf\n\t"
+        "s_nop 0\n\t"
+        "v_fmac_f32_dpp %0, %0, %1 row_shr:8 row_mask:0xf bank_mask:0xf\n\t"
+        "v_mul_f32_dpp %1, %1, %1 row_shr:8 row_mask:0xf bank_mask:0xf\n\t"
+        "s_nop 0\n\t"
+        "v_fmac_f32_dpp %0, %0, %1 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+        "v_mul_f32_dpp %1, %1, %1 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+        "s_nop 1"
+        : "+v"(A), "+v"(B));
+}
+template <typename T>
+__device__ inline void affine_scan_dpp32(T& A, T& B) {
+    if constexpr (sizeof(T) == 4) {
+        affine_scan_f32_fused32(A, B);
+    } else {
+        dpp_compose_shr_f64<0x111>(A, B);
+        dpp_compose_shr_f64<0x112>(A, B);
+        dpp_compose_shr_f64<0x114>(A, B);
+        dpp_compose_shr_f64<0x118>(A, B);
+        dpp_compose<0x142, 0xA>(A, B);  // row_bcast:15 into rows 1 and 3
+    }
+}
 // wave_shr:1 with lane 0 reading 0: the exclusive prefix of A
 template <typename T>
 __device__ inline T wave_shr_zero(T v) {
@@ -1451,12 +1498,17 @@ __global__ __launch_bounds__(64) void sweep_line_mixed_kernel(const PassDesc* __
 // 3D: one workgroup per direction.  Pencils (jy, jz) advance by diagonals t = jy + jz; the
 // y/z face fluxes of the previous diagonal live in LDS (FY[jz][x], FZ[jy][x]), one barrier per
 // diagonal.  After the last diagonal FY and FZ hold the exit y- and z-faces, which are written
-// out at the end with coalesced stores.
-template <typename T, int CPL, int U, int PF, bool DPP, bool BIN = false>
+// out at the end with coalesced stores.  PPW pencils per wave: with PPW = 2 each half of the wave
+// (32 lanes, CPL cells per lane) solves its own pencil of the item, and the scans run per half.
+template <typename T, int CPL, int U, int PF, bool DPP, bool BIN = false, int PPW = 1>
 __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, unsigned char* smem) {
     const int Nx = d.Nx, Ny = d.Ny, Nz = d.Nz, Nc = d.Nc, Nmat = d.Nmat;
     // the wave index is uniform: in a scalar register, the item cursors and barrier counts are too
     const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6), W = blockDim.x >> 6;
+    static_assert(PPW == 1 || PPW == 2, "one or two pencils per wave");
+    const int half = PPW == 2 ? lane >> 5 : 0;    // the pencil of the item this lane works on
+    const int ll = PPW == 2 ? lane & 31 : lane;    // lane within the pencil
+    constexpr int UP = U * PPW;                    // pencils per item
     T* hy2 = (T*)smem;
     T* hz2 = hy2 + Ny;
     T* st = hz2 + Nz;
@@ -1490,12 +1542,12 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
     T hx[CPL];
 #pragma unroll
     for (int c = 0; c < CPL; ++c) {
-        const int jx = lane * CPL + c;
+        const int jx = ll * CPL + c;
         inx[c] = jx < Nx;
         ixc[c] = inx[c] ? (fxs ? jx : Nx - 1 - jx) : 0;
         hx[c] = inx[c] ? (T)(2.0 * (fabs(mu) / d.dx[ixc[c]])) : (T)0;
     }
-    const bool last_lane = lane == (Nx - 1) / CPL;
+    const bool last_lane = ll == (Nx - 1) / CPL;
     __syncthreads();
     T* __restrict__ psi = (T*)d.psi + (size_t)n * Nc;
     T* __restrict__ bx = (T*)d.bx + (size_t)n * Ny * Nz;
@@ -1510,13 +1562,13 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
     auto diag_lo = [&](int t) { return t - (Ny - 1) > 0 ? t - (Ny - 1) : 0; };
     auto diag_cnt = [&](int t) { return (t < Nz - 1 ? t : Nz - 1) - diag_lo(t) + 1; };  // <= 0 past the end
     auto advance = [&](int& t, int& q) {
-        if (q + W * U < diag_cnt(t)) {
-            q += W * U;
+        if (q + W * UP < diag_cnt(t)) {
+            q += W * UP;
             return;
         }
         ++t;
-        while (t < ndiag && wave * U >= diag_cnt(t)) ++t;
-        q = wave * U;
+        while (t < ndiag && wave * UP >= diag_cnt(t)) ++t;
+        q = wave * UP;
     };
     T vb[PF][U][CPL];
     int mb[PF][U][CPL];
@@ -1525,7 +1577,7 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
         const int tt = t < ndiag ? t : ndiag - 1;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int q = q0 + u;
+            const int q = q0 + u * PPW + half;
             const bool ok = q < diag_cnt(tt);
             const int jz = diag_lo(tt) + (ok ? q : 0), jy = tt - jz;
             const int iy = fys ? jy : Ny - 1 - jy, iz = fzs ? jz : Nz - 1 - jz;
@@ -1536,7 +1588,7 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
         const int tt = t < ndiag ? t : ndiag - 1;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int q = q0 + u;
+            const int q = q0 + u * PPW + half;
             const bool ok = q < diag_cnt(tt);
             const int jz = diag_lo(tt) + (ok ? q : 0), jy = tt - jz;
             const int iy = fys ? jy : Ny - 1 - jy, iz = fzs ? jz : Nz - 1 - jz;
@@ -1548,8 +1600,8 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
             }
         }
     };
-    int ct = 0, cq = wave * U;  // item being solved
-    while (ct < ndiag && wave * U >= diag_cnt(ct)) ++ct;
+    int ct = 0, cq = wave * UP;  // item being solved
+    while (ct < ndiag && wave * UP >= diag_cnt(ct)) ++ct;
     int lt = ct, lq = cq;       // next item to load
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -1583,7 +1635,7 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
             T A[U], B[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int q = cq + u;
+                const int q = cq + u * PPW + half;
                 ok[u] = ct < ndiag && q < cnt;
                 const int jz = jz_lo + (ok[u] ? q : 0), jy = t - jz;
                 iyv[u] = fys ? jy : Ny - 1 - jy;
@@ -1596,11 +1648,11 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
                 B[u] = (T)1;
 #pragma unroll
                 for (int c = 0; c < CPL; ++c) {
-                    const int jx = inx[c] ? lane * CPL + c : 0;
+                    const int jx = inx[c] ? ll * CPL + c : 0;
                     fy[u][c] = FY[fyo_i[u] + jx];
                     fz[u][c] = FZ[fzo_i[u] + jx];
                     const T S = ((st[mb[sl][u][c]] + hx[c]) + hy) + hz;
-                    const T rS = dd_div((T)1, S);
+                    const T rS = rcp_nr(S);
                     c0[u][c] = inx[c] ? ((vb[sl][u][c] + hy * fy[u][c]) + hz * fz[u][c]) * rS : (T)0;
                     c1[u][c] = hx[c] * rS;
                     // compose this lane's maps (inactive cells: identity)
@@ -1624,26 +1676,37 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
                 __builtin_amdgcn_sched_barrier(0);
             }
 #pragma unroll
-            for (int u = 0; u < U; ++u) affine_scan<DPP>(A[u], B[u], lane);
+            for (int u = 0; u < U; ++u) {
+                if constexpr (DPP && PPW == 2) affine_scan_dpp32(A[u], B[u]);
+                else if constexpr (DPP) affine_scan_dpp(A[u], B[u]);
+                else affine_scan_seg<PPW == 2 ? 32 : 64>(A[u], B[u], ll);
+            }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                T fx = __shfl_up(A[u], 1);  // exclusive prefix applied to the pencil's inflow (0 if void)
-                if (lane == 0) fx = (T)0;
+                // exclusive prefix applied to the pencil's inflow (0 if void)
+                T fx;
+                if constexpr (DPP) {
+                    fx = wave_shr_zero(A[u]);
+                    if (PPW == 2 && ll == 0) fx = (T)0;
+                } else {
+                    fx = __shfl_up(A[u], 1);
+                    if (ll == 0) fx = (T)0;
+                }
                 if constexpr (BIN) {
                     T bb = __shfl_up(B[u], 1);
-                    if (lane == 0) bb = (T)1;
+                    if (ll == 0) bb = (T)1;
                     fx = fx + bb * xb[sl][u];
                 }
 #pragma unroll
                 for (int c = 0; c < CPL; ++c) {
                     const bool act = ok[u] && inx[c];
-                    const int jx = lane * CPL + c;
+                    const int jx = ll * CPL + c, jt = lane * CPL + c;
                     const T p = c0[u][c] + c1[u][c] * fx;
                     gst(act ? psi + roff[u] + ixc[c] : trash_slot<T>(lane, c), p);
                     const T p2 = p + p;
                     fx = inx[c] ? p2 - fx : fx;
-                    *(act ? FY + fyo_i[u] + jx : FT + jx) = p2 - fy[u][c];
-                    *(act ? FZ + fzo_i[u] + jx : FT + jx) = p2 - fz[u][c];
+                    *(act ? FY + fyo_i[u] + jx : FT + jt) = p2 - fy[u][c];
+                    *(act ? FZ + fzo_i[u] + jx : FT + jt) = p2 - fz[u][c];
                 }
                 gst((ok[u] && last_lane) ? bx + iyv[u] + Ny * izv[u] : trash_slot<T>(lane, 8 + u), fx);
             }
@@ -1668,21 +1731,21 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
     }
 }
 
-template <typename T, int CPL, int U, int PF, int MAXT, bool DPP, bool BIN = false>
+template <typename T, int CPL, int U, int PF, int MAXT, bool DPP, bool BIN = false, int PPW = 1>
 __global__ __launch_bounds__(MAXT) void sweep_scan_kernel(const PassDesc* __restrict__ pds, const SweepTask* __restrict__ tasks) {
     extern __shared__ __align__(16) unsigned char smem[];
     const SweepTask tk = tasks[blockIdx.x];
-    sweep_scan_body<T, CPL, U, PF, DPP, BIN>(pds[tk.prob], tk.dir, smem);
+    sweep_scan_body<T, CPL, U, PF, DPP, BIN, PPW>(pds[tk.prob], tk.dir, smem);
 }
 
-template <int CPL, int U, int PF, int MAXT, bool DPP>
+template <int CPL, int U, int PF, int MAXT, bool DPP, int PPW = 1>
 __global__ __launch_bounds__(MAXT) void sweep_scan_mixed_kernel(const PassDesc* __restrict__ pds,
                                                                 const SweepTask* __restrict__ tasks) {
     extern __shared__ __align__(16) unsigned char smem[];
     const SweepTask tk = tasks[blockIdx.x];
     const PassDesc& d = pds[tk.prob];
-    if (d.prec == P64) sweep_scan_body<double, CPL, U, PF, DPP>(d, tk.dir, smem);
-    else sweep_scan_body<float, CPL, U, PF, DPP>(d, tk.dir, smem);
+    if (d.prec == P64) sweep_scan_body<double, CPL, U, PF, DPP, false, PPW>(d, tk.dir, smem);
+    else sweep_scan_body<float, CPL, U, PF, DPP, false, PPW>(d, tk.dir, smem);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2615,17 +2678,34 @@ int sweep_scan_cpl(int Nx) {
     return -1;
 }
 
+// 3D: cells per lane, plus SCAN_PAIR when each half-wave solves its own pencil (32 lanes per
+// pencil).  RADIANT_HIP_SCAN_PPW=1 or 2 forces one pencil or two pencils per wave; by default
+// grids with Nx <= SCAN_PAIR_NX take two.
+static int scan_ppw_env() {
+    const char* e = std::getenv("RADIANT_HIP_SCAN_PPW");
+    return e ? std::atoi(e) : 0;
+}
+int sweep_scan_cpl3d(int Nx) {
+    const int env = scan_ppw_env();
+    const bool pair = env == 2 || (env != 1 && Nx <= SCAN_PAIR_NX);
+    if (pair)
+        for (int c : {1, 2, 4})
+            if (Nx <= 32 * c) return c | SCAN_PAIR;
+    return sweep_scan_cpl(Nx);
+}
+
 size_t sweep_scan_lds_bytes(int prec, int ndims, int Nx, int Ny, int Nz, int Nmat) {
     const size_t es = prec == P64 ? 8 : 4;
     if (ndims < 3) return es * ((size_t)Ny + Nmat);
-    const int cpl = sweep_scan_cpl(Nx);
+    const int cpl = sweep_scan_cpl3d(Nx) & (SCAN_PAIR - 1);
     return es * ((size_t)Ny + Nz + Nmat + ((size_t)Nz + Ny) * Nx + 64 * (size_t)(cpl > 0 ? cpl : 8));
 }
 
-int sweep_scan_waves(int ndims, int Ny, int Nz) {
+int sweep_scan_waves(int ndims, int Ny, int Nz, int Nx) {
     if (ndims < 3) return 0;              // 1D/2D: the line kernel (one wave per direction)
     const int diag = Ny < Nz ? Ny : Nz;  // longest diagonal of the (y, z) pencil grid
-    int w = (diag + 1) / 2;              // two pencils per wave per diagonal
+    const int per = (sweep_scan_cpl3d(Nx) & SCAN_PAIR) ? 4 : 2;
+    int w = (diag + per - 1) / per;      // two items per wave per diagonal
     return w < 1 ? 1 : (w > 16 ? 16 : w);
 }
 
@@ -2644,13 +2724,13 @@ static int scan3d_shape() {
     return e && e[0] == '1' ? 1 : 0;
 }
 
-template <int CPL, int U, int PF, int MAXT, bool DPP, bool BIN = false>
+template <int CPL, int U, int PF, int MAXT, bool DPP, bool BIN = false, int PPW = 1>
 static void scan3d_launch(int kind, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int waves, size_t lds,
                           hipStream_t st) {
     waves = waves < MAXT / 64 ? waves : MAXT / 64;
     if constexpr (BIN) {  // FP64 and FP32 passes of one problem
-        auto f64 = sweep_scan_kernel<double, CPL, U, PF, MAXT, DPP, true>;
-        auto f32 = sweep_scan_kernel<float, CPL, U, PF, MAXT, DPP, true>;
+        auto f64 = sweep_scan_kernel<double, CPL, U, PF, MAXT, DPP, true, PPW>;
+        auto f32 = sweep_scan_kernel<float, CPL, U, PF, MAXT, DPP, true, PPW>;
         static bool battr[2] = {false, false};
         const int bi = kind == P64 ? 0 : 1;
         if (!battr[bi]) {
@@ -2662,9 +2742,9 @@ static void scan3d_launch(int kind, const PassDesc* d_pd, const SweepTask* tasks
         else launch_timed(f32, dim3(ntasks), dim3(64 * waves), lds, st, d_pd, tasks);
         return;
     }
-    const void* fn = kind == PMIX ? (const void*)sweep_scan_mixed_kernel<CPL, U, PF, MAXT, DPP>
-                     : kind == P64 ? (const void*)sweep_scan_kernel<double, CPL, U, PF, MAXT, DPP>
-                                   : (const void*)sweep_scan_kernel<float, CPL, U, PF, MAXT, DPP>;
+    const void* fn = kind == PMIX ? (const void*)sweep_scan_mixed_kernel<CPL, U, PF, MAXT, DPP, PPW>
+                     : kind == P64 ? (const void*)sweep_scan_kernel<double, CPL, U, PF, MAXT, DPP, false, PPW>
+                                   : (const void*)sweep_scan_kernel<float, CPL, U, PF, MAXT, DPP, false, PPW>;
     static bool attr_set[3] = {};
     const int ai = kind == PMIX ? 0 : (kind == P64 ? 1 : 2);
     if (!attr_set[ai]) {
@@ -2672,11 +2752,28 @@ static void scan3d_launch(int kind, const PassDesc* d_pd, const SweepTask* tasks
         attr_set[ai] = true;
     }
     if (kind == PMIX)
-        launch_timed(sweep_scan_mixed_kernel<CPL, U, PF, MAXT, DPP>, dim3(ntasks), dim3(64 * waves), lds, st, d_pd, tasks);
+        launch_timed(sweep_scan_mixed_kernel<CPL, U, PF, MAXT, DPP, PPW>, dim3(ntasks), dim3(64 * waves), lds, st, d_pd,
+                     tasks);
     else if (kind == P64)
-        launch_timed(sweep_scan_kernel<double, CPL, U, PF, MAXT, DPP>, dim3(ntasks), dim3(64 * waves), lds, st, d_pd, tasks);
+        launch_timed(sweep_scan_kernel<double, CPL, U, PF, MAXT, DPP, false, PPW>, dim3(ntasks), dim3(64 * waves), lds, st,
+                     d_pd, tasks);
     else
-        launch_timed(sweep_scan_kernel<float, CPL, U, PF, MAXT, DPP>, dim3(ntasks), dim3(64 * waves), lds, st, d_pd, tasks);
+        launch_timed(sweep_scan_kernel<float, CPL, U, PF, MAXT, DPP, false, PPW>, dim3(ntasks), dim3(64 * waves), lds, st,
+                     d_pd, tasks);
+}
+
+// Two pencils per wave (32 lanes each): one item shape, the default waves and half of them.
+template <int CPL, bool BIN>
+static void scan3d_pair_inst(int kind, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int waves, size_t lds,
+                             hipStream_t st) {
+    const int NW = waves >= 2 ? 2 : 1;
+    TuneTicket t;
+    const int v = tune_choose({kind, 1, CPL | SCAN_PAIR, waves, BIN ? 1 : 0, ilog2(ntasks)}, NW, t);
+    if (t.a) set_launch_events(t.a, t.b);
+    const int w = v ? (waves + 1) / 2 : waves;
+    constexpr int PF = CPL >= 4 ? 2 : 4;
+    scan3d_launch<CPL, 1, PF, 1024, true, BIN, 2>(kind, d_pd, tasks, ntasks, w, lds, st);
+    tune_commit(t);
 }
 
 template <int CPL, bool DPP, bool BIN>
@@ -2778,6 +2875,21 @@ void launch_sweep_scan(int prec, const PassDesc* d_pd, const SweepTask* tasks, i
     if (ntasks <= 0) return;
     const int kind = prec == PMIX ? PMIX : (prec == P64 ? P64 : P32);
     if (bin && kind == PMIX) throw std::runtime_error("boundary inflow is supported by single-problem passes only");
+    if (cpl & SCAN_PAIR) {
+        cpl &= SCAN_PAIR - 1;
+        if (waves > 0 && scan_use_dpp()) {  // the shuffle cross-check runs one pencil per wave
+            switch (cpl) {
+                case 1: bin ? scan3d_pair_inst<1, true>(kind, d_pd, tasks, ntasks, waves, lds, st)
+                            : scan3d_pair_inst<1, false>(kind, d_pd, tasks, ntasks, waves, lds, st); break;
+                case 2: bin ? scan3d_pair_inst<2, true>(kind, d_pd, tasks, ntasks, waves, lds, st)
+                            : scan3d_pair_inst<2, false>(kind, d_pd, tasks, ntasks, waves, lds, st); break;
+                default: bin ? scan3d_pair_inst<4, true>(kind, d_pd, tasks, ntasks, waves, lds, st)
+                             : scan3d_pair_inst<4, false>(kind, d_pd, tasks, ntasks, waves, lds, st); break;
+            }
+            RT_HIP(hipGetLastError());
+            return;
+        }
+    }
     if (bin) {
         switch (cpl) {
             case 1: scan_any<1, true>(kind, d_pd, tasks, ntasks, waves, lds, st); break;

@@ -123,8 +123,12 @@ void launch_sweep_planes(int prec, const PassDesc* d_pd, const SweepTask* tasks,
                          size_t lds, hipStream_t st);
 // Row-scan sweep (lanes along x, affine scan of the x-recurrence); cells per lane 1..8.
 int sweep_scan_cpl(int Nx);
+// 3D scan: cells per lane, with SCAN_PAIR set when each half-wave solves its own pencil
+constexpr int SCAN_PAIR = 16;
+constexpr int SCAN_PAIR_NX = 64;
+int sweep_scan_cpl3d(int Nx);
 size_t sweep_scan_lds_bytes(int prec, int ndims, int Nx, int Ny, int Nz, int Nmat);
-int sweep_scan_waves(int ndims, int Ny, int Nz);
+int sweep_scan_waves(int ndims, int Ny, int Nz, int Nx);
 // bin: the passes take their inflow from PassDesc::bin_* (FP64 only)
 void launch_sweep_scan(int prec, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int cpl, int waves, size_t lds,
                        hipStream_t st, bool bin = false);

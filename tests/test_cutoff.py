@@ -194,7 +194,8 @@ def test_zero_source_first_group_in_batch(oracle):
     phis, stats, _ = S.solve_batch(probs, precision="mixed-fp32")
     for i in range(len(probs)):
         assert rel_err(phis[i], refs[i]) < 1e-6 and np.all(phis[i][0] == 0.0)
-        assert stats[i][0]["passes"] == 0
+        # the lockstep solver learns that the group has no source from its c = T(0) pass
+        assert stats[i][0]["niter"] == 0 and stats[i][0]["converged"]
     rb = S.ResidentBatch(probs, precision="mixed-fp32")
     for rep in range(2):
         rb.solve()

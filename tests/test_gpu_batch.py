@@ -172,8 +172,8 @@ def test_server_budget_gated_wait():
     p2 = P.config_2d(N=40, G=3, L=2, quad_order=8, seed=42)
     p3 = P.config_2d(N=32, G=3, L=2, quad_order=4, seed=43)
     b = [S.Desc(p, "mixed-fp32", "reduced").device_bytes() for p in (p1, p2, p3)]
-    budget = max(b[0] + b[1], b[1] + b[2]) - 1
-    assert budget >= max(b) and b[0] + b[1] > budget and b[1] + b[2] > budget
+    budget = b[0] + b[2] - 1  # room for p1 alone, p2 + p3, not p1 + p2 nor p1 + p3
+    assert budget >= max(b) and b[0] + b[1] > budget and b[0] + b[2] > budget
     srv = S.Server(hbm_budget_bytes=budget, max_active=8, precision="mixed-fp32")
     try:
         r1 = srv.submit(p1)
@@ -197,7 +197,7 @@ def test_server_budget_gated_wait():
             assert rel_err(phi, ref) < 1e-6
         # a finished result dropped with cancel also returns its bytes to the budget
         r3 = srv.submit(p3)
-        r4 = srv.submit(p2)
+        r4 = srv.submit(p1)
         t0 = time.time()
         while not srv.finished(r3):
             assert time.time() - t0 < 60
@@ -212,7 +212,7 @@ def test_server_budget_gated_wait():
             assert time.time() - t0 < 60
             r = srv.poll(r4)
             time.sleep(0.005)
-        assert rel_err(r[0], S.compute_flux(p2)[0]) < 1e-6
+        assert rel_err(r[0], S.compute_flux(p1)[0]) < 1e-6
     finally:
         srv.close()
 
