@@ -36,7 +36,9 @@ HBM_PEAK_GBS = 8000.0
 # that workload (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, averaged over the kernel's
 # dispatches), each scaled by what the counter reads for a plain 1 GiB device copy.  Keyed by
 # (workload, kernel class); reported only when the library sources match the measured digest.
-MEASURED_TRAFFIC = {}
+MEASURED_TRAFFIC = {("C2 mixed-fp32", "sweep_fp32"): dict(
+    fetch=200.883e6, write=184.729e6, fetch_scale=2.0, write_scale=1.0, digest="5b2ff9f4569567a5",
+    name="sweep_scan_kernel<float, 2, 1, 4, 1024, true, false, 2>", summary="profiles/sweep_scan_c2_mixed.md")}
 MFMA_PEAK_TFLOPS = {0: 78.6, 1: 157.3, 2: 2500.0}
 KNAMES = ["mgemm", "sweep", "dgemm", "bndmom"]
 PNAMES = ["fp64", "fp32", "bf16"]
@@ -351,12 +353,12 @@ def cpu_baseline(G, passes_per_solve, cases):
 
 
 def source_digest():
-    """sha256 of the library sources (first 16 hex digits): measured HBM traffic is reported only
-    for the code it was measured on."""
+    """sha256 of the kernel sources (kernels.hip, kernels.hpp; first 16 hex digits): measured HBM
+    traffic is reported only for the kernels it was measured on."""
     import hashlib
     h = hashlib.sha256()
     d = os.path.join(ROOT, "radiant.jl_amd", "csrc")
-    for f in sorted(os.listdir(d)):
+    for f in ("kernels.hip", "kernels.hpp"):
         h.update(f.encode())
         with open(os.path.join(d, f), "rb") as fh:
             h.update(fh.read())

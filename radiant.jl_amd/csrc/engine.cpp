@@ -1199,7 +1199,8 @@ void Problem::solve(rt_group_stats* stats, rt_solve_info* info) {
     RT_CHECK(hipEventCreate(&e0));
     RT_CHECK(hipEventCreate(&e1));
     counters = SolveCounters{};
-    k_fill(d_phi, 0.0, (size_t)G * Np * Nc, st);
+    // no clearing of the flux: every group's moments are written whole (reconstruction, or the
+    // fill of a group without source) before a later group's out-of-group source reads them
     RT_CHECK(hipEventRecord(e0, st));
     for (int g = 0; g < G; ++g) {
         rt_group_stats s;

@@ -356,6 +356,25 @@ class Server:
             pass
 
 
+def serve_alone(prob, precision="mixed-fp32", timeout_s=120.0):
+    """The flux a server computes for this request when it is the only one (the reference for
+    "the same alone and in the mix")."""
+    srv = Server(hbm_budget_bytes=max(1, Desc(prob, precision, "reduced" if precision != "fp64" else "faithful")
+                                      .device_bytes()), max_active=1, precision=precision)
+    try:
+        rid = srv.submit(prob)
+        t0 = time.perf_counter()
+        while True:
+            r = srv.poll(rid)
+            if r is not None:
+                return r[0]
+            if time.perf_counter() - t0 > timeout_s:
+                raise TimeoutError("request not served in time")
+            time.sleep(0.002)
+    finally:
+        srv.close()
+
+
 def autotune(mode):
     """rt_autotune: 0 default launch shapes, 1 online tuning (default), 2 a new variant every launch."""
     check(lib().rt_autotune(int(mode)))

@@ -69,8 +69,8 @@ def test_server_continuous_batching():
             phi, st, lat = res[rid]
             ref, _ = S.compute_flux(p)
             e = rel_err(phi, ref)
-            alone, _, _ = S.solve_batch([p], precision="mixed-fp32")
-            d = float(np.abs(alone[0] - phi).max())
+            alone = S.serve_alone(p)  # the same request alone in a server
+            d = float(np.abs(alone - phi).max())
             _say("  request", rid, "latency ms", round(lat, 2), "err", e, "alone vs served", d)
             assert e < 1e-6 and lat > 0
             assert d == 0.0  # the same alone and in the served mix

@@ -95,8 +95,8 @@ def test_server_with_boundaries(oracle):
         for k, rid in enumerate(ids):
             phi, _, lat = res[rid]
             ref, _ = oracle.solve(probs[k])
-            alone, _, _ = S.solve_batch([probs[k]], precision="mixed-fp32")
-            e, d = rel_err(phi, ref), float(np.abs(alone[0] - phi).max())
+            alone = S.serve_alone(probs[k])  # the same request alone in a server
+            e, d = rel_err(phi, ref), float(np.abs(alone - phi).max())
             _say("  served", k, "bc", probs[k].bc, "err", e, "vs alone", d)
             assert e < 1e-6 and d == 0.0
     finally:

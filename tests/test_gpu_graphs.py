@@ -59,7 +59,7 @@ def test_graph_rounds_served(monkeypatch):
     from radiant_hip import solver as S
     probs = _mix()
     monkeypatch.delenv("RADIANT_HIP_GRAPHS", raising=False)
-    alone = [S.solve_batch([p], precision="mixed-fp32")[0][0] for p in probs]
+    alone = [S.serve_alone(p) for p in probs]  # each request alone in a server, eager rounds
     monkeypatch.setenv("RADIANT_HIP_GRAPHS", "1")
     srv = S.Server(hbm_budget_bytes=2 * 1024 ** 3, max_active=3, precision="mixed-fp32")
     try:
