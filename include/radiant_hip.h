@@ -189,6 +189,17 @@ int rt_problem_device_bytes(const rt_problem_desc *p, int64_t *bytes);
  * [Np, Nx*Ny*Nz] (Np fastest).  z_out may alias z_in.  Precision follows the handle. */
 int rt_problem_pass(rt_problem *h, int32_t g, int32_t mode, const double *qlout, const double *z_in,
                     double *z_out);
+/* Allocation-free variant of rt_problem_pass.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc), e.g. an external Krylov solver applying the transport operator to vectors
+ * it keeps on the device.  qlout, z_in and z_out are DEVICE pointers (caller-allocated) in the
+ * layouts of rt_problem_pass; the rows of qlout for moments without a scattering source are not
+ * read.  Its temporaries live in a fixed device array of 64 MiB per GPU, held until the process
+ * ends: 2*S + Np*Nx*Ny*Nz doubles (S = rt_problem_state_length) must fit in 8,388,608, else
+ * RT_EINVAL.  Every call on a GPU shares that array, so calls that overlap on one GPU race.
+ * Blocking (the handle's stream is synchronised); needs a faithful FP64 handle. */
+int rt_problem_pass_device(rt_problem *h, int32_t g, int32_t mode, const double *qlout, const double *z_in,
+                           double *z_out);
 /* Per-kernel device time of the last solve on a handle, recorded with HIP events on the
  * handle's stream when the environment has RADIANT_HIP_PROFILE=1 (profiling hook, no
  * counterpart in the reference).  ms[12] and launches[12] are indexed [precision][kernel] with

@@ -100,6 +100,16 @@ int rt_problem_pass(rt_problem* h, int32_t g, int32_t mode, const double* qlout,
     return guarded([&] { h->p->host_pass(g, mode, qlout, z_in, z_out); });
 }
 
+int rt_problem_pass_device(rt_problem* h, int32_t g, int32_t mode, const double* qlout, const double* z_in,
+                           double* z_out) {
+    if (!h || !h->p || !qlout || !z_in || !z_out) return fail(RT_EINVAL, "null argument");
+    return guarded([&] {
+        size_t cap = 0;
+        double* scr = rt::noalloc_scratch(&cap);
+        h->p->device_pass(g, mode, qlout, z_in, z_out, scr, cap);
+    });
+}
+
 int rt_problem_state_length(rt_problem* h, int64_t* S) {
     if (!h || !h->p || !S) return fail(RT_EINVAL, "null argument");
     *S = h->p->faithful_state_length();

@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <mutex>
 
 namespace rt {
 
@@ -1291,6 +1292,74 @@ void Problem::host_pass(int g, int mode, const double* qlout_host, const double*
     k_copy(tmp + (size_t)Np * Nc, b + (size_t)Np * Nc, nb, st);
     RT_CHECK(hipMemcpyAsync(zout, tmp, n * sizeof(double), hipMemcpyDeviceToHost, st));
     RT_CHECK(hipStreamSynchronize(st));
+}
+
+// ---- allocation-free pass (rt_problem_pass_device) ----
+// Scratch of the allocation-free pass: the input and output states in the engine's moment-major
+// layout and the out-of-group source, 2 S + Np Nc doubles at most NOALLOC_DOUBLES.  Every device
+// holds its own copy until the process ends; every call on a device shares it, so calls that
+// overlap on one device race.  Host code reaches it only through the address hipGetSymbolAddress
+// gives for the device (the array's name is a host-side stand-in, not a device address).
+static constexpr size_t NOALLOC_DOUBLES = (size_t)8 << 20;  // 64 MiB
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+double* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+}  // namespace
+
+double* noalloc_scratch(size_t* cap) {
+    int dev = 0;
+    RT_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) throw Error(RT_EINVAL, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            throw Error(RT_EDEVICE, std::string("hipGetSymbolAddress of the pass scratch failed: ") +
+                                        hipGetErrorString(e));
+        }
+        g_noalloc_ptr[dev] = (double*)p;
+    }
+    *cap = NOALLOC_DOUBLES;
+    return g_noalloc_ptr[dev];
+}
+
+void Problem::device_pass(int g, int mode, const double* qlout_dev, const double* zin_dev, double* zout_dev,
+                          double* scr, size_t cap) {
+    if (g < 0 || g >= G) throw Error(RT_EINVAL, "group out of range");
+    if (mode < 0 || mode > 2) throw Error(RT_EINVAL, "pass mode must be 0 (T), 1 (A) or 2 (I - A)");
+    if (state_mode != RT_STATE_FAITHFUL) throw Error(RT_EINVAL, "rt_problem_pass_device needs a faithful FP64 handle");
+    const size_t n = S_f, nphi = (size_t)Np * Nc, need = 2 * n + nphi;
+    if (need > cap)
+        throw Error(RT_EINVAL, "rt_problem_pass_device: the pass needs " + std::to_string(need) +
+                                   " doubles of scratch (2 x state + Np x cells), above its maximum of " +
+                                   std::to_string(cap) + "; use rt_problem_pass");
+    double* a = scr;
+    double* b = scr + n;
+    double* q = scr + 2 * n;
+    const bool meas = autotune_measuring();
+    set_autotune_measuring(false);  // no timing events: tuned or default launch shapes
+    try {
+        k_state_moments_in(a, zin_dev, Np, Nc, st);
+        k_copy(a + nphi, zin_dev + nphi, nb, st);
+        // active rows of the out-of-group source; the other rows are not read
+        k_state_moments_in(q, qlout_dev, Np, Nc, st);
+        for (int a2 = 0; a2 < Nact; ++a2)
+            RT_CHECK(hipMemcpyAsync(d_qlout + (size_t)a2 * Nc, q + (size_t)act[a2] * Nc, Nc * sizeof(double),
+                                    hipMemcpyDeviceToDevice, st));
+        pass_faithful(g, P64, mode == 2 ? MODE_IMA : (mode == 1 ? MODE_A : MODE_T), a, b);
+        k_state_moments_out(zout_dev, b, Np, Nc, st);
+        k_copy(zout_dev + nphi, b + nphi, nb, st);
+        RT_CHECK(hipStreamSynchronize(st));
+    } catch (...) {
+        set_autotune_measuring(meas);
+        throw;
+    }
+    set_autotune_measuring(meas);
 }
 
 void test_contraction(int which, int prec, int M, int K, int N, const double* A, const double* B, double* C) {

@@ -77,6 +77,10 @@ public:
     void get_flux_cutoff(double* out, hipStream_t s = nullptr);
     // Kernel-level hook: one faithful-state pass on host vectors (see rt_problem_pass).
     void host_pass(int g, int mode, const double* qlout_host, const double* zin, double* zout);
+    // The same pass on device vectors, with every temporary in the caller-given scratch of cap
+    // doubles (rt_problem_pass_device): no device memory is allocated.
+    void device_pass(int g, int mode, const double* qlout_dev, const double* zin_dev, double* zout_dev,
+                     double* scratch, size_t cap);
     int64_t faithful_state_length() const { return (int64_t)S_f; }
 
     // ---- pieces used by the batched and served drivers ----
@@ -200,5 +204,9 @@ private:
     void solve_group_plain(int g, rt_group_stats& s);
     void solve_group_mixed(int g, rt_group_stats& s);
 };
+
+// Address of this device's scratch array of the allocation-free pass (taken once per device) and
+// its capacity in doubles.
+double* noalloc_scratch(size_t* cap);
 
 }  // namespace rt

@@ -71,7 +71,7 @@ EXPORTS = ["rt_solve", "rt_problem_create", "rt_problem_solve", "rt_problem_get_
            "rt_last_error", "rt_version", "rt_test_contraction", "rt_problem_kernel_times",
            "rt_batch_create", "rt_batch_solve", "rt_batch_get_flux", "rt_batch_destroy", "rt_batch_kernel_times",
            "rt_autotune", "rt_autotune_report", "rt_solve_cutoff", "rt_problem_get_flux_cutoff",
-           "rt_batch_get_flux_cutoff", "rt_server_poll_cutoff"]
+           "rt_batch_get_flux_cutoff", "rt_server_poll_cutoff", "rt_problem_pass_device"]
 
 _lib = None
 
@@ -97,6 +97,7 @@ def lib():
     L.rt_problem_device_bytes.argtypes = [PD, lp]
     L.rt_problem_pass.argtypes = [H, C.c_int32, C.c_int32, dp, dp, dp]
     L.rt_problem_state_length.argtypes = [H, lp]
+    L.rt_problem_pass_device.argtypes = [H, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
     L.rt_solve_batch.argtypes = [C.c_int32, C.POINTER(PD), C.POINTER(dp), C.POINTER(GS), C.POINTER(SolveInfo)]
     L.rt_problem_kernel_times.argtypes = [H, dp, lp]
     L.rt_test_contraction.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32, dp, dp, dp]
