@@ -35,6 +35,8 @@ class OracleProblem(C.Structure):
         ("fokker_planck", C.c_char_p), ("scheme", C.c_char_p), ("scheme_order", C.c_int32),
         ("scheme_y", C.c_char_p), ("scheme_order_y", C.c_int32), ("partial_coupling", C.c_int32),
         ("scheme_z", C.c_char_p), ("scheme_order_z", C.c_int32),
+        ("quad_mu", C.POINTER(C.c_double)), ("quad_eta", C.POINTER(C.c_double)),
+        ("quad_xi", C.POINTER(C.c_double)), ("quad_w", C.POINTER(C.c_double)), ("quad_n", C.c_int32),
     ]
 
 # SN.jl get_solver_type
@@ -194,6 +196,13 @@ def make_problem(prob, nthreads=1, verbose=0, **over):
     if fp:
         k.fp = fp.encode()
         op.fokker_planck = k.fp
+    if prob.quad_type == "lebedev" and prob.quad_order > 11:
+        # rules above order 11 come point by point from the host's orbit generation (pinned by
+        # exactness in tests/test_quadratures.py)
+        from radiant_hip.angular import lebedev
+        k.q = [np.ascontiguousarray(a, np.float64) for a in lebedev(prob.quad_order)]
+        op.quad_mu, op.quad_eta, op.quad_xi, op.quad_w = (_p(a) for a in k.q)
+        op.quad_n = len(k.q[3])
     k.scheme = (getattr(prob, "scheme", None) or "DD").encode()
     op.scheme = k.scheme
     op.scheme_order = int(getattr(prob, "scheme_order", 1) or 1)

@@ -328,6 +328,8 @@ void carlson(int N, vector<double>& mu, vector<double>& eta, vector<double>& xi,
             }
 }
 
+void fold_upper(vector<double>& mu, vector<double>& eta, vector<double>& xi, vector<double>& w);
+
 // tools/quadrature.jl:19-59 (gauss-legendre, gauss-lobatto, gauss-legendre-chebychev, lebedev,
 // carlson; the Qdims=2 fold onto the upper hemisphere is at :36-54).
 void quadrature(int N, const string& type, int Ndims, int Qdims, vector<double>& mu,
@@ -349,7 +351,13 @@ void quadrature(int N, const string& type, int Ndims, int Qdims, vector<double>&
     } else {
         fail("Unknown " + type + " quadrature in " + std::to_string(Ndims) + "D geometry.");
     }
-    if (Qdims == 2) {
+    if (Qdims == 2) fold_upper(mu, eta, xi, w);
+}
+
+// quadrature.jl:36-54: the unit-sphere set folded onto the upper hemisphere (xi >= 0), each
+// lower point's weight added to its mirror image.
+void fold_upper(vector<double>& mu, vector<double>& eta, vector<double>& xi, vector<double>& w) {
+    {
         int Nd = (int)w.size();
         vector<double> w0 = w;
         for (int i = 0; i < Nd; ++i) {
@@ -1264,6 +1272,11 @@ struct oracle_problem {
     int32_t partial_coupling;
     const char* scheme_z;       // the z axis of 3D problems
     int32_t scheme_order_z;
+    // A rule given point by point (quad_n > 0): the Lebedev rules above order 11, generated from
+    // their published orbit parameters by the caller and pinned there by exactness; the
+    // upper-hemisphere fold of quadrature.jl:36-54 still applies.
+    const double *quad_mu, *quad_eta, *quad_xi, *quad_w;
+    int32_t quad_n;
 };
 
 struct oracle_stats {
@@ -1322,7 +1335,15 @@ struct Solver {
         } else {
             Qdims = (ndims == 3) ? 3 : (p.qdims == 2 ? 2 : 3);
         }
-        quadrature(p.quad_order, qt, ndims, Qdims, A.mu, A.eta, A.xi, A.w);
+        if (p.quad_n > 0) {
+            A.mu.assign(p.quad_mu, p.quad_mu + p.quad_n);
+            A.eta.assign(p.quad_eta, p.quad_eta + p.quad_n);
+            A.xi.assign(p.quad_xi, p.quad_xi + p.quad_n);
+            A.w.assign(p.quad_w, p.quad_w + p.quad_n);
+            if (Qdims == 2) fold_upper(A.mu, A.eta, A.xi, A.w);
+        } else {
+            quadrature(p.quad_order, qt, ndims, Qdims, A.mu, A.eta, A.xi, A.w);
+        }
         A.Nd = (int)A.w.size();
         A.B = angular_polynomial_basis(A.mu, A.eta, A.xi, A.w, p.legendre_order, p.sn_type, Qdims);
         A.Np = A.B.Np;

@@ -472,7 +472,10 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
         // ---- reductions
         rw.partial = dalloc<double>(RED_BLOCKS);
         rw.scal = dalloc<double>(128);
-        RT_CHECK(hipHostMalloc((void**)&h_scal, 128 * sizeof(double), hipHostMallocDefault));
+        // pinned: 128 scalar slots, then the two Gram-Schmidt items of a classical Arnoldi step
+        RT_CHECK(hipHostMalloc((void**)&h_scal, 128 * sizeof(double) + 2 * sizeof(GsItem), hipHostMallocDefault));
+        d_gsi = dalloc<GsItem>(2);
+        d_gspart = dalloc<double>((size_t)GS_BPI_MAX * GS_KMAX);
         d_Vptr = dalloc<double*>(64);
         d_y = dalloc<double>(64);
         RT_CHECK(hipStreamSynchronize(st));
@@ -790,10 +793,14 @@ static double rho_estimate(double r0, double r, int64_t it) {  // krylov_state.j
 
 // tools/gmres.jl:35-133 -- restarted GMRES(m), modified Gram-Schmidt, Givens rotations.
 // The MGS chain runs on the device (each kernel reads its coefficient from a scalar slot and
-// reduces the next dot product); the host reads one H column per Arnoldi step.
+// reduces the next dot product); the host reads one H column per Arnoldi step.  Two options
+// serve the inner solves of the mixed-precision refinement, which need no parity with
+// gmres.jl's iteration counts: zero_guess (z = 0, so the first residual is c without a pass) and
+// cgs (classical Gram-Schmidt in two passes over the basis instead of the MGS chain).
 template <class MV>
 void Problem::gmres(size_t n, double* z, MV matvec, const double* c, int m, int64_t maxit_, double tol_,
-                    int64_t& iter, double& resid, bool& conv, double& rho, const std::string& tag, bool zero_guess) {
+                    int64_t& iter, double& resid, bool& conv, double& rho, const std::string& tag, bool zero_guess,
+                    bool cgs) {
     m = std::max(1, m);
     if (m + 1 > 64) throw Error(RT_EINVAL, "gmres restart too large (max 63)");
     const double bnorm = norm(c, n);
@@ -833,8 +840,20 @@ void Problem::gmres(size_t n, double* z, MV matvec, const double* c, int m, int6
         for (int j = 0; j < m; ++j) {
             if (iter >= maxit_) break;
             matvec(w, V[j]); ++iter;
-            k_dot(rw, 0, w, V[0], n, st);
-            for (int i = 0; i <= j; ++i) k_mgs(rw, i, i + 1, w, V[i], i < j ? V[i + 1] : nullptr, n, st);
+            if (cgs) {
+                // classical Gram-Schmidt in two passes over the basis: h = V^T w, then
+                // w -= V h with |w|^2 (the item is reused after the step's synchronisation)
+                GsItem* h_gsi = (GsItem*)(h_scal + 128);
+                const int bpi = gs_blocks((long)n);
+                h_gsi[0] = GsItem{(const double* const*)d_Vptr, nullptr, w, rw.scal, (long)n, j + 1, bpi};
+                h_gsi[1] = GsItem{(const double* const*)d_Vptr, rw.scal, w, rw.scal + j + 1, (long)n, j + 1, bpi};
+                RT_CHECK(hipMemcpyAsync(d_gsi, h_gsi, 2 * sizeof(GsItem), hipMemcpyHostToDevice, st));
+                launch_gs(d_gsi, 1, j + 1, bpi, false, false, d_gspart, st);
+                launch_gs(d_gsi + 1, 1, j + 1, bpi, true, true, d_gspart, st);
+            } else {
+                k_dot(rw, 0, w, V[0], n, st);
+                for (int i = 0; i <= j; ++i) k_mgs(rw, i, i + 1, w, V[i], i < j ? V[i + 1] : nullptr, n, st);
+            }
             k_scale_by_inv_slot(V[j + 1], w, rw.scal, j + 1, n, st);
             RT_CHECK(hipMemcpyAsync(h_scal, rw.scal, (j + 2) * sizeof(double), hipMemcpyDeviceToHost, st));
             RT_CHECK(hipStreamSynchronize(st));
@@ -1146,7 +1165,7 @@ void Problem::solve_group_mixed(int g, rt_group_stats& s) {
             int64_t it = 0;
             double rr, rh;
             bool cv;
-            gmres(n, dz, MVlo, r, restart, maxit - niter, inner_tol, it, rr, cv, rh, "m", true);
+            gmres(n, dz, MVlo, r, restart, maxit - niter, inner_tol, it, rr, cv, rh, "m", true, true);
             niter += it;
             k_axpy(1.0, dz, z, n, st);
             pass_reduced(g, P64, MODE_IMA, z, t);  // t = z - A z in FP64

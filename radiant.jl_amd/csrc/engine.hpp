@@ -162,6 +162,8 @@ private:
     double* vec(const std::string& name, size_t n);
     RedWork rw{};
     double* h_scal = nullptr;   // pinned mirror of the scalar slots
+    GsItem* d_gsi = nullptr;    // the two Gram-Schmidt items of a classical Arnoldi step
+    double* d_gspart = nullptr; // their block partials
     double** d_Vptr = nullptr;  // device table of Krylov basis pointers
     double* d_y = nullptr;
 
@@ -188,7 +190,8 @@ private:
     // solvers over a state of length n; pass(out, in) applies the operator
     template <class MV>
     void gmres(size_t n, double* z, MV matvec, const double* c, int m, int64_t maxit_, double tol_, int64_t& iter,
-               double& resid, bool& conv, double& rho, const std::string& tag, bool zero_guess = false);
+               double& resid, bool& conv, double& rho, const std::string& tag, bool zero_guess = false,
+               bool cgs = false);
     // T(z) = A z + T(0) from the angular fluxes of the last A z pass (in d_psi) and of c = T(0)
     // (psi_c, or psi_c alone when no A z pass ran): one full-row D contraction, no sweep.
     void reconstruct_sweep_free(int g, const double* zin_reduced, const double* psi_c, bool have_az);

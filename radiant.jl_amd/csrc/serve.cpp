@@ -14,6 +14,7 @@
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <deque>
@@ -125,6 +126,16 @@ static bool lockstep_request(const rt_problem_desc& d) {
         return e && e[0] == 'l';
     }();
     return d.ndims < 3 || all;
+}
+
+// RADIANT_HIP_SERVE_LOG=1: one line per request solved on a worker stream (queueing, set-up and
+// solve times) on stderr.
+static bool serve_log() {
+    static const bool on = [] {
+        const char* e = std::getenv("RADIANT_HIP_SERVE_LOG");
+        return e && e[0] == '1';
+    }();
+    return on;
 }
 
 static int sub_batches(int n) {
@@ -417,10 +428,18 @@ struct rt_server {
         int code = 0;
         try {
             if (!r->cancel.load()) {
+                const auto t0 = std::chrono::steady_clock::now();
                 prob.reset(new rt::Problem(r->desc->d));
+                const auto t1 = std::chrono::steady_clock::now();
                 st.resize(r->desc->d.G);
                 prob->solve(st.data(), nullptr);
                 state = r->cancel.load() ? Request::CANCELLED : Request::DONE;
+                if (serve_log()) {
+                    const auto t2 = std::chrono::steady_clock::now();
+                    auto ms = [](auto a, auto b) { return std::chrono::duration<double, std::milli>(b - a).count(); };
+                    std::fprintf(stderr, "[serve] request %lld: queued %.1f ms, set-up %.1f ms, solve %.1f ms\n",
+                                 (long long)r->id, ms(r->t_submit, t0), ms(t0, t1), ms(t1, t2));
+                }
             }
         } catch (const rt::Error& e) {
             state = Request::FAILED; err = e.what(); code = e.code;

@@ -130,15 +130,56 @@ def _oh_orbit(code, a):
     return pts
 
 
+def _signed(p, zero_free):
+    """The sign patterns of a point, the first coordinate's sign outermost, zeros left unsigned."""
+    out = [[]]
+    for k, x in enumerate(p):
+        out = [q + [s * x] for q in out for s in ((1, -1) if x != 0.0 else (1,))]
+    return out
+
+
+def _oh_orbit_ab(code, a, b):
+    """Orbit of the octahedral group with its sign changes, for the table-based rules: type 4
+    (a, a, b) with b = sqrt(1 - 2 a^2), 5 (a, b, 0) with b = sqrt(1 - a^2), 6 (a, b, c) with
+    c = sqrt(1 - a^2 - b^2); types 1-3 as the closed-form rules.  The coordinate permutations are
+    taken in the order of tools/lebedev.jl gen_oh!."""
+    if code in (1, 2, 3):
+        return _oh_orbit(code, a)
+    if code == 4:
+        c = math.sqrt(1 - 2 * a * a)
+        perms = [(a, a, c), (a, c, a), (c, a, a)]
+    elif code == 5:
+        c = math.sqrt(1 - a * a)
+        perms = [(a, c, 0.0), (c, a, 0.0), (a, 0.0, c), (c, 0.0, a), (0.0, a, c), (0.0, c, a)]
+    elif code == 6:
+        c = math.sqrt(1 - a * a - b * b)
+        perms = [(a, b, c), (a, c, b), (b, a, c), (b, c, a), (c, a, b), (c, b, a)]
+    else:
+        raise ValueError("orbit type must be 1..6")
+    pts = []
+    for p in perms:
+        pts.extend(_signed(p, True))
+    return pts
+
+
 def lebedev(N):
-    """tools/lebedev.jl:21-4751 for the orders whose rules are known in closed form (3, 5, 7, 9,
-    11); weights scaled to 4 pi as Radiant does."""
-    if N not in LEBEDEV_RULES:
-        raise ValueError(f"Lebedev quadrature is not available at order {N} (available: 3, 5, 7, 9, 11).")
+    """tools/lebedev.jl:21-4751: the Lebedev-Laikov rule of algebraic order N (31 rules, 6 to 5294
+    points), weights scaled to 4 pi as Radiant does.  Orders 3-11 are written from their closed
+    forms; higher orders generate their orbits from the published parameters (lebedev_rules)."""
+    from .lebedev_rules import ORDER_POINTS, RULES
+    if N not in ORDER_POINTS:
+        raise ValueError(f"Lebedev quadrature is not available at order {N} "
+                         f"(available: {', '.join(str(k) for k in sorted(ORDER_POINTS))}).")
     mu, eta, xi, w = [], [], [], []
-    for code, a, v in LEBEDEV_RULES[N]:
-        for p in _oh_orbit(code, a):
+    if N in LEBEDEV_RULES:
+        orbits = [(code, a, v, _oh_orbit(code, a)) for code, a, v in LEBEDEV_RULES[N]]
+    else:
+        orbits = [(code, a, v, _oh_orbit_ab(code, a, b)) for code, a, b, v in RULES[ORDER_POINTS[N]]]
+    for code, a, v, pts in orbits:
+        for p in pts:
             mu.append(p[0]); eta.append(p[1]); xi.append(p[2]); w.append(4 * PI * v)
+    if len(w) != ORDER_POINTS[N]:
+        raise RuntimeError("Lebedev rule with a wrong number of points")
     return np.array(mu), np.array(eta), np.array(xi), np.array(w)
 
 

@@ -1,5 +1,5 @@
-"""Gauss-Lobatto (tools/gauss_lobatto.jl) and Lebedev (tools/lebedev.jl, orders 3-11 in closed
-form) quadratures.
+"""Gauss-Lobatto (tools/gauss_lobatto.jl) and Lebedev (tools/lebedev.jl: orders 3-11 in closed
+form, orders 13-125 from the published Lebedev-Laikov orbit parameters) quadratures.
 
 CPU: nodes and weights against numpy (Lobatto: +-1 and the roots of P'_{N-1}, weights
 2/(N(N-1) P_{N-1}^2)), exactness (Lobatto: monomials up to degree 2N-3; Lebedev: every spherical
@@ -60,6 +60,56 @@ def test_lebedev_exactness(oracle, order):
     assert np.abs(c - xi).max() < 1e-15 and np.abs(d - w).max() < 1e-15
 
 
+def _lebedev_orders():
+    from radiant_hip.lebedev_rules import ORDER_POINTS
+    return sorted(o for o in ORDER_POINTS if o > 11)
+
+
+@pytest.mark.parametrize("order", _lebedev_orders())
+def test_lebedev_table_rules_exact(order):
+    """Every table-based rule (13 to 125, 74 to 5294 points): unit directions, weights summing to
+    4 pi, the point count of the rule, invariance under the octahedral group (every coordinate
+    permutation and sign change maps the set onto itself with equal weights), and exactness for the
+    spherical harmonics up to its order (every (l, m) up to order 35; above, a spread subset of
+    degrees and orders up to the rule's order)."""
+    from radiant_hip import angular
+    from radiant_hip.lebedev_rules import ORDER_POINTS
+    mu, eta, xi, w = angular.lebedev(order)
+    assert len(w) == ORDER_POINTS[order]
+    assert np.abs(mu ** 2 + eta ** 2 + xi ** 2 - 1).max() < 1e-15
+    assert abs(w.sum() - 4 * PI) < 1e-12
+    pts = np.c_[mu, eta, xi]
+    key = {tuple(np.round(p, 12)): wi for p, wi in zip(pts, w)}
+    for perm in ((1, 0, 2), (2, 1, 0), (0, 2, 1)):
+        for sg in ((-1, 1, 1), (1, -1, 1), (1, 1, -1)):
+            q = pts[:, perm] * np.array(sg)
+            for p, wi in zip(q, w):
+                assert abs(key[tuple(np.round(p, 12))] - wi) < 1e-15
+    th = np.arccos(np.clip(xi, -1, 1))
+    ph = np.arctan2(eta, mu)
+    ls = range(1, order + 1) if order <= 35 else sorted(set(range(1, order + 1, 9)) | {order - 1, order})
+    for l in ls:
+        ms = range(0, l + 1) if order <= 35 else sorted(set(range(0, l + 1, 7)) | {l})
+        for m in ms:
+            assert abs((w * _ylm(l, m, th, ph)).sum()) < 1e-12, (l, m)
+    if order <= 35:  # the rule is of this order, not higher
+        worst = max(abs((w * _ylm(order + 1, m, th, ph)).sum()) for m in range(order + 2))
+        assert worst > 1e-4
+
+
+@pytest.mark.parametrize("N", [13, 17, 21])
+def test_lebedev_table_galerkin_and_oracle_setup(oracle, N):
+    """Galerkin-D on the table-based rules: D M = I, and the oracle (given the rule point by point)
+    builds the same basis as the host setup."""
+    from radiant_hip import angular
+    A = angular.AngularSet("lebedev", N, 3, 3, 3, "galerkin-d")
+    Np, M, D, pl, pm = oracle.basis(A.mu, A.eta, A.xi, A.w, 3, "galerkin-d", 3)
+    assert np.array_equal(pl, A.pl) and np.array_equal(pm, A.pm)
+    cond = np.linalg.cond(D)
+    assert cond < 1e4 and np.abs(D @ M - np.eye(Np)).max() < 1e-13 * cond
+    assert rel_err(A.D, D) < 1e-13
+
+
 @pytest.mark.parametrize("qt,N,nd,Q,L", [("gauss-lobatto", 8, 1, 1, 7), ("gauss-lobatto", 7, 1, 1, 6),
                                           ("lebedev", 7, 3, 3, 3), ("lebedev", 11, 3, 3, 3),
                                           ("lebedev", 9, 2, 3, 3)])
@@ -112,7 +162,8 @@ def _say(*a):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("ndims,qt,N", [(1, "gauss-lobatto", 8), (1, "gauss-lobatto", 7), (2, "lebedev", 7),
-                                        (3, "lebedev", 5), (3, "lebedev", 11)])
+                                        (3, "lebedev", 5), (3, "lebedev", 11), (3, "lebedev", 13),
+                                        (3, "lebedev", 17), (2, "lebedev", 21)])
 def test_new_quadratures_solve_parity(oracle, ndims, qt, N):
     from radiant_hip import problems as P, solver as S
     prob = P.small_problem(ndims, N=6 if ndims == 3 else 8, G=2, L=2, quad_type=qt, quad_order=N, tol=1e-12,
