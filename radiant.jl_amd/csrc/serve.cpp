@@ -389,6 +389,30 @@ struct rt_server {
     int in_batch = 0;  // of which in lockstep slots (scheduler thread only)
     bool stop = false;
     std::thread sched;
+    // Results dropped with rt_server_cancel are released on this thread: freeing device memory
+    // waits for the device's outstanding work, which would stall the caller.  Their bytes return
+    // to the budget once released.
+    std::thread reaper;
+    std::deque<std::pair<std::unique_ptr<rt::Problem>, int64_t>> trash;
+    std::condition_variable rcv;
+    bool reaper_stop = false;
+    void reap_results() {
+        std::unique_lock<std::mutex> lk(mu);
+        while (true) {
+            rcv.wait(lk, [&] { return reaper_stop || !trash.empty(); });
+            if (trash.empty() && reaper_stop) break;
+            auto item = std::move(trash.front());
+            trash.pop_front();
+            lk.unlock();
+            try {
+                item.first.reset();
+            } catch (const std::exception&) {
+            }
+            lk.lock();
+            bytes_in_use -= item.second;
+            cv.notify_all();
+        }
+    }
     std::map<int64_t, std::thread> workers;  // requests solved on their own stream
     std::vector<int64_t> finished;           // workers to join
 
@@ -670,6 +694,10 @@ struct rt_server {
                 r.state = Request::CANCELLED;
             }
         }
+        // results dropped with rt_server_cancel are released by the reaper, which is still running:
+        // wait for it, since those solved on est synchronise it when released
+        rcv.notify_all();
+        cv.wait(lk, [&] { return trash.empty(); });
         lk.unlock();
         uncollected.clear();
         if (est) (void)hipStreamDestroy(est);
@@ -687,6 +715,7 @@ int rt_server_create(const rt_server_config* cfg, rt_server** out) {
     }
     auto* s = new rt_server();
     s->cfg = *cfg;
+    s->reaper = std::thread([s] { s->reap_results(); });
     s->sched = std::thread([s] {
         rt::set_autotune_measuring(false);  // served requests overlap on the device
         s->scheduler();
@@ -778,7 +807,6 @@ int rt_server_poll_cutoff(rt_server* s, int64_t req, double* phi_out, double* cu
 
 int rt_server_cancel(rt_server* s, int64_t req) {
     if (!s) return RT_EINVAL;
-    std::unique_ptr<rt::Problem> dropped;  // a finished, uncollected result: released here
     {
         std::lock_guard<std::mutex> lk(s->mu);
         auto it = s->reqs.find(req);
@@ -788,15 +816,14 @@ int rt_server_cancel(rt_server* s, int64_t req) {
         }
         Request& r = *it->second;
         r.cancel.store(true);
-        if (r.state == Request::DONE && r.result) {
-            dropped = std::move(r.result);
+        if (r.state == Request::DONE && r.result) {  // a finished, uncollected result: released by the reaper
+            s->trash.emplace_back(std::move(r.result), r.bytes);
             r.state = Request::CANCELLED;
-            s->bytes_in_use -= r.bytes;
         }
     }
-    int rc = guard([&] { dropped.reset(); });
+    s->rcv.notify_all();
     s->cv.notify_all();
-    return rc;
+    return RT_OK;
 }
 
 int rt_server_destroy(rt_server* s) {
@@ -807,6 +834,12 @@ int rt_server_destroy(rt_server* s) {
     }
     s->cv.notify_all();
     if (s->sched.joinable()) s->sched.join();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->reaper_stop = true;
+    }
+    s->rcv.notify_all();
+    if (s->reaper.joinable()) s->reaper.join();
     for (auto& kv : s->workers)
         if (kv.second.joinable()) kv.second.join();
     delete s;

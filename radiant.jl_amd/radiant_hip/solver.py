@@ -277,8 +277,13 @@ class Server:
         self.descs = {}
         self.last_collect_ms = 0.0
 
+    def describe(self, prob, **kw):
+        """The problem description a request of this server carries (built on the host)."""
+        return Desc(prob, self.precision, "reduced" if self.precision != "fp64" else "faithful", **kw)
+
     def submit(self, prob, **kw):
-        d = Desc(prob, self.precision, "reduced" if self.precision != "fp64" else "faithful", **kw)
+        """Submits a problem, or a description built with describe(); returns the request id."""
+        d = prob if isinstance(prob, Desc) else self.describe(prob, **kw)
         rid = C.c_int64()
         check(lib().rt_server_submit(self.h, d.byref(), C.byref(rid)))
         self.descs[rid.value] = (d.G, d.Np, d.Nc)
