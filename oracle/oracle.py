@@ -37,6 +37,7 @@ class OracleProblem(C.Structure):
         ("scheme_z", C.c_char_p), ("scheme_order_z", C.c_int32),
         ("quad_mu", C.POINTER(C.c_double)), ("quad_eta", C.POINTER(C.c_double)),
         ("quad_xi", C.POINTER(C.c_double)), ("quad_w", C.POINTER(C.c_double)), ("quad_n", C.c_int32),
+        ("scheme_e", C.c_char_p), ("scheme_order_e", C.c_int32),
     ]
 
 # SN.jl get_solver_type
@@ -80,6 +81,10 @@ def lib():
                                      C.c_double, dp, dp, dp, dp, dp]
         L.oracle_cell_2d.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                      C.c_double, C.c_double, C.c_double, dp, dp, dp, dp]
+        L.oracle_cell_bfp.argtypes = [C.c_char_p] * 4 + [C.c_int] * 6 + [dp, dp, C.c_double, C.c_double, C.c_double,
+                                                                       dp, dp, dp, dp, dp, dp]
+        L.oracle_legendre_triple.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.oracle_legendre_triple.restype = C.c_double
         _lib = L
     return _lib
 
@@ -213,6 +218,9 @@ def make_problem(prob, nthreads=1, verbose=0, **over):
     k.scheme_z = (getattr(prob, "scheme_z", None) or "DD").encode()
     op.scheme_z = k.scheme_z
     op.scheme_order_z = int(getattr(prob, "scheme_order_z", 1) or 1)
+    k.scheme_e = (getattr(prob, "scheme_e", None) or "DD").encode()
+    op.scheme_e = k.scheme_e
+    op.scheme_order_e = int(getattr(prob, "scheme_order_e", 1) or 1)
     op._keep = k
     return op
 
@@ -229,17 +237,19 @@ def fokker_planck(prob):
 
 
 def spatial_moments(prob):
-    """Spatial Legendre moments per cell: the 1D scheme order; Ox*Oy (Ox + Oy - 1 without full
-    coupling) in 2D; Ox*Oy*Oz (Ox + Oy + Oz - 2) in 3D."""
-    ox = max(1, int(getattr(prob, "scheme_order", 1) or 1))
-    if prob.ndims == 1:
-        return ox
-    oy = max(1, int(getattr(prob, "scheme_order_y", 1) or 1))
-    fc = getattr(prob, "full_coupling", True)
-    if prob.ndims == 2:
-        return ox * oy if fc else ox + oy - 1
-    oz = max(1, int(getattr(prob, "scheme_order_z", 1) or 1))
-    return ox * oy * oz if fc else ox + oy + oz - 2
+    """Legendre moments per cell (SN.get_schemes Nm[5]): over x[, y[, z]] and, for the slowing-down
+    types, the energy axis; the product of the orders with full coupling, 1 + the sum of
+    (order - 1) without."""
+    orders = [max(1, int(getattr(prob, "scheme_order", 1) or 1))]
+    if prob.ndims >= 2:
+        orders.append(max(1, int(getattr(prob, "scheme_order_y", 1) or 1)))
+    if prob.ndims >= 3:
+        orders.append(max(1, int(getattr(prob, "scheme_order_z", 1) or 1)))
+    if getattr(prob, "csd", False):
+        orders.append(max(1, int(getattr(prob, "scheme_order_e", 1) or 1)))
+    if getattr(prob, "full_coupling", True):
+        return int(np.prod(orders))
+    return 1 + sum(o - 1 for o in orders)
 
 
 def cell_1d(scheme, K, mu, St, dx, Qn, fx_in):
@@ -274,6 +284,26 @@ def cell_3d(schemes, orders, full_coupling, om, h, St, Qn, fx_in, fy_in, fz_in):
     _check(lib().oracle_cell_3d(schemes[0].encode(), schemes[1].encode(), schemes[2].encode(), ox, oy, oz,
                                 int(not full_coupling), _p(o), _p(hh), St, _p(q), _p(fx), _p(fy), _p(fz), _p(psi)))
     return psi, fx, fy, fz
+
+
+def cell_bfp(schemes, orders, full_coupling, ndims, om, h, St, Sm, Sp, Qn, fe_in, fx_in, fy_in, fz_in):
+    """One slowing-down cell: schemes/orders over (E, x, y, z), Sm/Sp the stopping powers at the
+    group's upper/lower energy per unit group width; returns (moments, outgoing energy, x, y and z
+    face moments)."""
+    oe, ox, oy, oz = orders
+    nm = oe * ox * oy * oz if full_coupling else oe + ox + oy + oz - 3
+    arr = lambda v: np.ascontiguousarray(v, np.float64).copy()
+    q, o, hh = arr(Qn), arr(om), arr(h)
+    fe, fx, fy, fz = arr(fe_in), arr(fx_in), arr(fy_in), arr(fz_in)
+    psi = np.zeros(nm)
+    _check(lib().oracle_cell_bfp(*(s.encode() for s in schemes), oe, ox, oy, oz, int(not full_coupling), ndims,
+                                 _p(o), _p(hh), St, Sm, Sp, _p(q), _p(fe), _p(fx), _p(fy), _p(fz), _p(psi)))
+    return psi, fe, fx, fy, fz
+
+
+def legendre_triple(a, b, c):
+    """½ ∫ P_a P_b P_c dx as the oracle computes it."""
+    return lib().oracle_legendre_triple(a, b, c)
 
 
 def sizes(prob):

@@ -1215,6 +1215,137 @@ void flux_3d_bte(const double* om, const double* h, double St, const double* Qn,
             }
 }
 
+// ½ ∫_{-1}^{1} P_a P_b P_c dx (the 𝒲 constants of scheme_weights.jl:411-412, integrals.jl:254),
+// from the closed form of the Wigner 3j symbol with zero projections:
+//   (a b c; 0 0 0)² = (J-2a)!(J-2b)!(J-2c)!/(J+1)! · [g!/((g-a)!(g-b)!(g-c)!)]²,  J = a+b+c = 2g,
+// zero for odd J or outside the triangle |a-b| <= c <= a+b.
+double legendre_triple(int a, int b, int c) {
+    const int J = a + b + c;
+    if (J % 2 || c > a + b || a > b + c || b > a + c) return 0.0;
+    const int g = J / 2;
+    auto lf = [](int n) { return std::lgamma(n + 1.0); };
+    const double l1 = lf(J - 2 * a) + lf(J - 2 * b) + lf(J - 2 * c) - lf(J + 1);
+    const double l2 = lf(g) - lf(g - a) - lf(g - b) - lf(g - c);
+    return std::exp(l1 + 2.0 * l2);
+}
+
+// The in-group stopping-power moments of sn_flux.jl:92-96 from S⁻/ΔE and S⁺/ΔE (the stopping
+// powers at the group's upper and lower energy): (S⁻+S⁺)/2, and (S⁻-S⁺)/(2√3) for Ke > 1.
+void stopping_moments(double Sm, double Sp, int Ke, double* Sv) {
+    for (int k = 0; k < Ke; ++k) Sv[k] = 0.0;
+    Sv[0] = (Sm + Sp) / 2.0;
+    if (Ke > 1) Sv[1] = (Sm - Sp) / (2.0 * std::sqrt(3.0));
+}
+
+// flux_1D_BFP / flux_2D_BFP / flux_3D_BFP (sn_1D_bfp.jl:44-125, sn_2D_bfp.jl:43-154,
+// sn_3D_bfp.jl:51-191; non-adaptive) written once over the axes (E, x, y, z) with orders
+// O = (OE, Ox, Oy, Oz); the axes a problem lacks have order 1 and no streaming.  A cell moment is
+// a multi-index (e, a, b, c): with full coupling at e + OE*(a + Ox*(b + Oy*c)) (energy fastest),
+// without it only one index is non-zero, at that index plus the (O-1) of the axes before it.  The
+// face of an axis carries the moments over the other axes by the same rule.
+struct Cell4 {
+    int O[4];
+    bool fc;
+    int Nm, F[4];  // cell moments; face moments per axis (F[0]: the energy face)
+    Cell4() : O{1, 1, 1, 1}, fc(true), Nm(1), F{1, 1, 1, 1} {}
+    Cell4(int oe, int ox, int oy, int oz, bool full) : O{oe, ox, oy, oz}, fc(full) {
+        Nm = fc ? oe * ox * oy * oz : oe + ox + oy + oz - 3;
+        for (int a = 0; a < 4; ++a) {
+            int prod = 1, sum = 0;
+            for (int k = 0; k < 4; ++k)
+                if (k != a) { prod *= O[k]; sum += O[k] - 1; }
+            F[a] = fc ? prod : 1 + sum;
+        }
+    }
+    bool valid(const int* m) const { return fc || (m[0] > 0) + (m[1] > 0) + (m[2] > 0) + (m[3] > 0) <= 1; }
+    // index over the axes of `m` except `skip` (-1: all four)
+    int index(const int* m, int skip) const {
+        int r = 0, stride = 1, off = 0;
+        for (int k = 0; k < 4; ++k) {
+            if (k == skip) continue;
+            if (fc) { r += m[k] * stride; stride *= O[k]; }
+            else { if (m[k] > 0) r = m[k] + off; off += O[k] - 1; }
+        }
+        return r;
+    }
+};
+
+// One cell of the slowing-down closure.  om: direction cosines (μ, η, ξ), zero on the axes the
+// problem lacks; h: cell widths; Sm, Sp: S⁻/ΔE, S⁺/ΔE of the cell's material; Sv[OE]: the
+// in-group stopping-power moments / ΔE (sn_flux.jl:92-96); W[OE³]: legendre_triple; f[0..3]: the
+// incoming energy, x, y and z face moments, overwritten with the outgoing ones; w[0..3]: the
+// closure weights of each axis (scheme_weights_1d).  tang1d: the 1D tangential guard
+// (|μ| <= 1e-10: no streaming, the x face passes through, sn_1D_bfp.jl:52-54).
+void flux_bfp(const double* om, const double* h, double St, double Sm, double Sp, const double* Sv,
+              const double* W, const double* Qn, double* const* f, const Cell4& cl, const double* const* w,
+              bool tang1d, double* psi) {
+    double s[4] = {-1.0, 0, 0, 0}, hh[4] = {0, 0, 0, 0};
+    for (int a = 1; a < 4; ++a) {
+        s[a] = om[a - 1] > 0 ? 1.0 : (om[a - 1] < 0 ? -1.0 : 0.0);
+        hh[a] = h[a - 1] > 0 ? std::fabs(om[a - 1]) / h[a - 1] : 0.0;
+    }
+    if (tang1d) { s[1] = 1.0; hh[1] = 0.0; }
+    auto C = [](int i) { return std::sqrt(2.0 * i + 1.0); };  // 0-based
+    auto spow = [](double v, int e) { double r = 1.0; for (int k = 0; k < e; ++k) r *= v; return r; };
+    const int OE = cl.O[0], Nm = cl.Nm;
+    vector<int> mi;  // valid multi-indices in the reference's loop order (x outer ... E inner)
+    for (int a = 0; a < cl.O[1]; ++a)
+        for (int b = 0; b < cl.O[2]; ++b)
+            for (int c = 0; c < cl.O[3]; ++c)
+                for (int e = 0; e < OE; ++e) {
+                    const int m[4] = {e, a, b, c};
+                    if (cl.valid(m)) { mi.push_back(e); mi.push_back(a); mi.push_back(b); mi.push_back(c); }
+                }
+    const int nv = (int)mi.size() / 4;
+    vector<double> S((size_t)Nm * Nm, 0.0), Q(Nm, 0.0);
+    for (int u = 0; u < nv; ++u) {
+        const int* iu = &mi[4 * u];
+        const int i = cl.index(iu, -1);
+        for (int v = 0; v < nv; ++v) {
+            const int* jv = &mi[4 * v];
+            const int j = cl.index(jv, -1);
+            double& e = S[(size_t)i * Nm + j];
+            if (i == j) e += St;
+            for (int d = 1; d < 4; ++d) {  // streaming along x, y, z: moments equal on the other axes
+                bool same = true;
+                for (int k = 0; k < 4; ++k) if (k != d && iu[k] != jv[k]) same = false;
+                if (!same) continue;
+                const int p = iu[d], q = jv[d];
+                if (p >= q + 1) e -= C(p) * hh[d] * s[d] * C(q) * (1.0 - spow(-1.0, p - q));
+                e += C(p) * hh[d] * spow(s[d], p) * C(q) * spow(s[d], q) * w[d][q + 1];
+            }
+            if (iu[1] == jv[1] && iu[2] == jv[2] && iu[3] == jv[3]) {  // slowing down
+                const int p = iu[0], q = jv[0];
+                for (int k = 0; k < p; ++k)
+                    for (int ww = 0; ww < OE; ++ww)
+                        e += C(p) * C(q) * C(k) * C(ww) * (1.0 - spow(-1.0, p - k)) * Sv[ww] *
+                             W[q + OE * (k + OE * ww)];
+                e += C(p) * Sp * spow(-1.0, p) * C(q) * spow(-1.0, q) * w[0][q + 1];
+            }
+        }
+    }
+    for (int v = 0; v < nv; ++v) {
+        const int* jv = &mi[4 * v];
+        const int j = cl.index(jv, -1);
+        Q[j] += Qn[j];
+        for (int d = 1; d < 4; ++d)
+            Q[j] -= C(jv[d]) * hh[d] * (spow(s[d], jv[d]) * w[d][0] - spow(-s[d], jv[d])) * f[d][cl.index(jv, d)];
+        Q[j] -= C(jv[0]) * (spow(-1.0, jv[0]) * Sp * w[0][0] - Sm) * f[0][cl.index(jv, 0)];
+    }
+    vector<double> x = lu_solve(S, Q, Nm);
+    for (int j = 0; j < Nm; ++j) psi[j] = x[j];
+    for (int v = 0; v < nv; ++v) {
+        const int* jv = &mi[4 * v];
+        const int j = cl.index(jv, -1);
+        for (int d = 0; d < 4; ++d) {
+            if (d == 1 && tang1d) continue;
+            double& F = f[d][cl.index(jv, d)];
+            if (jv[d] == 0) F = w[d][0] * F;
+            F += C(jv[d]) * (d == 0 ? spow(-1.0, jv[0]) : spow(s[d], jv[d])) * w[d][jv[d] + 1] * x[j];
+        }
+    }
+}
+
 struct Angular {
     int Nd = 0, Np = 0, Nps = 0;
     vector<double> mu, eta, xi, w;
@@ -1277,6 +1408,10 @@ struct oracle_problem {
     // upper-hemisphere fold of quadrature.jl:36-54 still applies.
     const double *quad_mu, *quad_eta, *quad_xi, *quad_w;
     int32_t quad_n;
+    // the energy closure of the slowing-down types (SN.set_scheme("E", scheme, order)): "DD" when
+    // NULL, or "DG"; order 0 or 1 = first order
+    const char* scheme_e;
+    int32_t scheme_order_e;
 };
 
 struct oracle_stats {
@@ -1311,10 +1446,15 @@ struct Solver {
     Cell3 cl3{1, 1, 1, true};
     vector<double> omz;
     int Nm1 = 1, Nm2 = 1, Nm3 = 1;  // moments of the x-, y- and z-face fluxes
+    bool bfp = false;           // the general slowing-down cell (any order in space or energy)
+    Cell4 cl4;
+    vector<double> ome, Wt;     // energy closure weights, legendre_triple table [OE³]
+    int NmE = 1;                // moments of the energy-face flux (over the spatial axes)
+    vector<double> Svec;        // in-group stopping-power moments / ΔE [OE * Nmat]
     int Ncol = 0;               // moment columns: K * Nc (column = spatial moment + K * cell)
     const double* cur_src[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};  // this pass's surface sources
-    // continuous slowing down: incoming / outgoing energy-face angular flux [n*Nc + c] (𝚽E12 of
-    // sn_flux.jl:166), the current group's S⁻ and S⁺ per material (stopping powers / ΔE)
+    // continuous slowing down: incoming / outgoing energy-face angular flux [is + NmE*(c + Nc*n)]
+    // (𝚽E12 of sn_flux.jl:166), the current group's S⁻ and S⁺ per material (stopping powers / ΔE)
     vector<double> psiE_in, psiE_out, Sminus, Splus;
     const double* curE = nullptr;  // psiE_in in inhomogeneous passes, null (zero) in homogeneous ones
     // state layout: [Φ (Np*Nc, p fastest) | x-bnd | y-bnd | z-bnd]
@@ -1412,12 +1552,36 @@ struct Solver {
         const string scheme_y = p.scheme_y ? p.scheme_y : "DD";
         K = p.scheme_order > 1 ? p.scheme_order : 1;
         const int Ky = p.scheme_order_y > 1 ? p.scheme_order_y : 1;
-        if (ndims == 1 && (K > 1 || scheme != "DD")) {
-            if (p.csd) fail("closures other than diamond difference of order 1 are supported for the BTE only");
+        const string scheme_e = p.scheme_e ? p.scheme_e : "DD";
+        const int Ke = p.scheme_order_e > 1 ? p.scheme_order_e : 1;
+        if (!p.csd && (Ke > 1 || scheme_e != "DD"))
+            fail("an energy closure needs the slowing-down term (csd)");
+        if (Ke > 2) fail("energy closures of order above 2 are not supported");
+        if (p.csd) {  // scheme_weights.jl with isCSD: axes E, x[, y[, z]], DD or DG each
+            const string scheme_z = p.scheme_z ? p.scheme_z : "DD";
+            const int Kz = p.scheme_order_z > 1 ? p.scheme_order_z : 1;
+            const string sy = ndims >= 2 ? scheme_y : "DD", sz = ndims >= 3 ? scheme_z : "DD";
+            const int oy = ndims >= 2 ? Ky : 1, oz = ndims >= 3 ? Kz : 1;
+            if (Ke > 1 || K > 1 || oy > 1 || oz > 1 || scheme_e != "DD" || scheme != "DD" || sy != "DD" || sz != "DD") {
+                for (const string* sc : {&scheme_e, &scheme, &sy, &sz})
+                    if (*sc != "DD" && *sc != "DG") fail("Closure relation not implemented yet.");
+                ome = scheme_weights_1d(scheme_e, Ke);
+                omx = scheme_weights_1d(scheme, K);
+                omy = scheme_weights_1d(sy, oy);
+                omz = scheme_weights_1d(sz, oz);
+                cl4 = Cell4(Ke, K, oy, oz, p.partial_coupling == 0);
+                NmE = cl4.F[0]; Nm1 = cl4.F[1]; Nm2 = cl4.F[2]; Nm3 = cl4.F[3];
+                K = cl4.Nm;
+                Wt.assign((size_t)Ke * Ke * Ke, 0.0);
+                for (int a = 0; a < Ke; ++a)
+                    for (int b = 0; b < Ke; ++b)
+                        for (int c = 0; c < Ke; ++c) Wt[a + Ke * (b + Ke * c)] = legendre_triple(a, b, c);
+                bfp = true;
+            }
+        } else if (ndims == 1 && (K > 1 || scheme != "DD")) {
             omx = scheme_weights_1d(scheme, K);
             ho = true;
         } else if (ndims == 2 && (K > 1 || Ky > 1 || scheme != "DD" || scheme_y != "DD")) {
-            if (p.csd) fail("closures other than diamond difference of order 1 are supported for the BTE only");
             for (const string* sc : {&scheme, &scheme_y})
                 if (*sc != "DD" && *sc != "DG") fail("Closure relation not implemented yet.");
             omx = scheme_weights_1d(scheme, K);
@@ -1432,7 +1596,6 @@ struct Solver {
             const string scheme_z = p.scheme_z ? p.scheme_z : "DD";
             const int Kz = p.scheme_order_z > 1 ? p.scheme_order_z : 1;
             if (K > 1 || Ky > 1 || Kz > 1 || scheme != "DD" || scheme_y != "DD" || scheme_z != "DD") {
-                if (p.csd) fail("closures other than diamond difference of order 1 are supported for the BTE only");
                 for (const string* sc : {&scheme, &scheme_y, &scheme_z})
                     if (*sc != "DD" && *sc != "DG") fail("Closure relation not implemented yet.");
                 omx = scheme_weights_1d(scheme, K);
@@ -1664,6 +1827,99 @@ struct Solver {
         }
     }
 
+    // sn_sweep_1D.jl:51-124, sn_sweep_2D.jl:54-200 and sn_sweep_3D.jl:57-240 with the
+    // slowing-down cell of any order (flux_bfp): x outer, y, z inner, as sweep3_ho; the axes a
+    // problem lacks have one position.  Face arrays b[p + Nps*(m + Nm*(f + 2*pos))]; surface
+    // sources enter face moment 1 only.  The energy face of each cell comes in from psiE (the
+    // previous group's outflow, zero in homogeneous passes) and leaves into psiE_out.
+    void sweep_bfp(int n, int g, double* phi, const double* Ql, const double* bx_in, const double* by_in,
+                   const double* bz_in, double* tx, double* ty, double* tz) {
+        const int Np = A.Np, Nps = A.Nps, Nd = A.Nd;
+        const double om[3] = {A.mu[n], ndims >= 2 ? A.eta[n] : 0.0, ndims >= 3 ? A.xi[n] : 0.0};
+        const bool fx_ = om[0] >= 0, fy_ = om[1] >= 0, fz_ = om[2] >= 0;
+        const bool tang = ndims == 1 && std::fabs(om[0]) <= 1e-10;
+        const SurfBasis *Sx = nullptr, *Sy = nullptr, *Sz = nullptr;
+        int rx = -1, ry = -1, rz = -1;
+        if (nxm[n]) { Sx = &A.S[0]; rx = nxm[n] - 1; } else if (nxp[n]) { Sx = &A.S[1]; rx = nxp[n] - 1; }
+        if (ndims >= 2) {
+            if (nym[n]) { Sy = &A.S[2]; ry = nym[n] - 1; } else if (nyp[n]) { Sy = &A.S[3]; ry = nyp[n] - 1; }
+        }
+        if (ndims >= 3) {
+            if (nzm[n]) { Sz = &A.S[4]; rz = nzm[n] - 1; } else if (nzp[n]) { Sz = &A.S[5]; rz = nzp[n] - 1; }
+        }
+        auto add_src = [&](const SurfBasis* Sb, int row, const double* src, size_t npos, size_t pos, double* f) {
+            if (src && Sb)
+                for (int p = 0; p < P.Np_source; ++p) f[0] += Sb->M[row + (size_t)Sb->Ndp * p] * src[(size_t)p * npos + pos];
+        };
+        auto add_bc = [&](int face_bc, bool fwd, const SurfBasis* Sb, int row, const double* b, int Nm, size_t pos,
+                          double* f) {
+            if (face_bc == 0 || !Sb) return;
+            const int sf = fwd ? (face_bc == 1 ? 0 : 1) : (face_bc == 1 ? 1 : 0);
+            for (int p = 0; p < Nps; ++p)
+                for (int is = 0; is < Nm; ++is)
+                    f[is] += Sb->M[row + (size_t)Sb->Ndp * p] * b[p + (size_t)Nps * (is + (size_t)Nm * (sf + 2 * pos))];
+        };
+        const double* w[4] = {ome.data(), omx.data(), omy.data(), omz.data()};
+        vector<double> FX((size_t)Nm1 * Ny * Nz, 0.0), FY((size_t)Nm2 * Nz), FZ(Nm3), FE(NmE), Qn(K), psi(K);
+        for (int kx = 0; kx < Nx; ++kx) {
+            const int ix = fx_ ? kx : Nx - 1 - kx;
+            std::fill(FY.begin(), FY.end(), 0.0);
+            for (int ky = 0; ky < Ny; ++ky) {
+                const int iy = fy_ ? ky : Ny - 1 - ky;
+                std::fill(FZ.begin(), FZ.end(), 0.0);
+                if (ndims >= 3) {
+                    add_src(Sz, rz, cur_src[fz_ ? 4 : 5], (size_t)Nx * Ny, (size_t)ix + (size_t)Nx * iy, FZ.data());
+                    add_bc(fz_ ? P.bc[4] : P.bc[5], fz_, Sz, rz, bz_in, Nm3, (size_t)ix + (size_t)Nx * iy, FZ.data());
+                }
+                for (int kz = 0; kz < Nz; ++kz) {
+                    const int iz = fz_ ? kz : Nz - 1 - kz;
+                    double* fy = &FY[(size_t)Nm2 * iz];
+                    double* fx = &FX[(size_t)Nm1 * (iy + (size_t)Ny * iz)];
+                    if (ky == 0 && ndims >= 2) {
+                        add_src(Sy, ry, cur_src[fy_ ? 2 : 3], (size_t)Nx * Nz, (size_t)ix + (size_t)Nx * iz, fy);
+                        add_bc(fy_ ? P.bc[2] : P.bc[3], fy_, Sy, ry, by_in, Nm2, (size_t)ix + (size_t)Nx * iz, fy);
+                    }
+                    if (kx == 0) {
+                        add_src(Sx, rx, cur_src[fx_ ? 0 : 1], (size_t)Ny * Nz, (size_t)iy + (size_t)Ny * iz, fx);
+                        add_bc(fx_ ? P.bc[0] : P.bc[1], fx_, Sx, rx, bx_in, Nm1, (size_t)iy + (size_t)Ny * iz, fx);
+                    }
+                    const int c = ix + Nx * (iy + Ny * iz);
+                    for (int is = 0; is < K; ++is) {
+                        double q = 0.0;
+                        for (int p = 0; p < Np; ++p) q += A.B.M[n + (size_t)Nd * p] * Ql[p + (size_t)Np * (is + (size_t)K * c)];
+                        Qn[is] = q;
+                    }
+                    double* fe_out = &psiE_out[(size_t)NmE * (c + (size_t)Nc * n)];
+                    for (int is = 0; is < NmE; ++is) FE[is] = curE ? curE[is + (size_t)NmE * (c + (size_t)Nc * n)] : 0.0;
+                    const double h[3] = {P.dx[ix], ndims >= 2 ? P.dy[iy] : 0.0, ndims >= 3 ? P.dz[iz] : 0.0};
+                    const int m = matc[c];
+                    double* f[4] = {FE.data(), fx, fy, FZ.data()};
+                    flux_bfp(om, h, sig_t(g, m), Sminus[m], Splus[m], &Svec[(size_t)cl4.O[0] * m], Wt.data(), Qn.data(),
+                             f, cl4, w, tang, psi.data());
+                    for (int is = 0; is < NmE; ++is) fe_out[is] = FE[is];
+                    for (int is = 0; is < K; ++is)
+                        for (int p = 0; p < Np; ++p)
+                            phi[p + (size_t)Np * (is + (size_t)K * c)] += A.B.D[p + (size_t)Np * n] * psi[is];
+                    if (kx == Nx - 1 && Sx)
+                        for (int p = 0; p < Nps; ++p)
+                            for (int is = 0; is < Nm1; ++is)
+                                tx[p + (size_t)Nps * (is + (size_t)Nm1 * ((fx_ ? 1 : 0) + 2 * (iy + (size_t)Ny * iz)))] +=
+                                    Sx->D[p + (size_t)Nps * rx] * fx[is];
+                    if (ky == Ny - 1 && Sy)
+                        for (int p = 0; p < Nps; ++p)
+                            for (int is = 0; is < Nm2; ++is)
+                                ty[p + (size_t)Nps * (is + (size_t)Nm2 * ((fy_ ? 1 : 0) + 2 * (ix + (size_t)Nx * iz)))] +=
+                                    Sy->D[p + (size_t)Nps * ry] * fy[is];
+                    if (kz == Nz - 1 && Sz)
+                        for (int p = 0; p < Nps; ++p)
+                            for (int is = 0; is < Nm3; ++is)
+                                tz[p + (size_t)Nps * (is + (size_t)Nm3 * ((fz_ ? 1 : 0) + 2 * (ix + (size_t)Nx * iy)))] +=
+                                    Sz->D[p + (size_t)Nps * rz] * FZ[is];
+                }
+            }
+        }
+    }
+
     // incoming boundary value for a face array b[p + Nps*(f + 2*pos)] (sweep BC branches).
     // src: the entry face's surface source of the current group ([Np_source][npos]) or null; its
     // term comes first, as in sn_sweep_2D.jl:70-99.
@@ -1789,8 +2045,14 @@ struct Solver {
                 Sminus[m] = P.Sb[g + (size_t)(G + 1) * m] / P.dE[g];
                 Splus[m] = P.Sb[(g + 1) + (size_t)(G + 1) * m] / P.dE[g];
             }
-            if (psiE_in.size() != (size_t)A.Nd * Nc) psiE_in.assign((size_t)A.Nd * Nc, 0.0);
-            psiE_out.assign((size_t)A.Nd * Nc, 0.0);
+            if (bfp) {  // sn_flux.jl:92-96: S = [(S⁻+S⁺)/2, (S⁻-S⁺)/(2√3), 0...] / ΔE
+                const int Ke = cl4.O[0];
+                Svec.assign((size_t)Ke * Nmat, 0.0);
+                for (int m = 0; m < Nmat; ++m) stopping_moments(Sminus[m], Splus[m], Ke, &Svec[(size_t)Ke * m]);
+            }
+            const size_t ne = (size_t)A.Nd * Nc * NmE;
+            if (psiE_in.size() != ne) psiE_in.assign(ne, 0.0);
+            psiE_out.assign(ne, 0.0);
             curE = homog ? nullptr : psiE_in.data();
         }
         // Ql = (homog ? 0 : Qlout) + Σs_l Φl   (scattering_source.jl:25-30; sn_inner_pass.jl:22
@@ -1844,7 +2106,8 @@ struct Solver {
     }
 
     void direction(int n, int g, double* phi, const double* Ql, const double* bin, double* tmp) {
-        if (ndims == 1) sweep1(n, g, phi, Ql, bin, tmp);
+        if (bfp) sweep_bfp(n, g, phi, Ql, bin, bin + nbx, bin + nbx + nby, tmp, tmp + nbx, tmp + nbx + nby);
+        else if (ndims == 1) sweep1(n, g, phi, Ql, bin, tmp);
         else if (ho2) sweep2_ho(n, g, phi, Ql, bin, bin + nbx, tmp, tmp + nbx);
         else if (ho3) sweep3_ho(n, g, phi, Ql, bin, bin + nbx, bin + nbx + nby, tmp, tmp + nbx, tmp + nbx + nby);
         else sweep23(n, g, phi, Ql, bin, bin + nbx, bin + nbx + nby, tmp, tmp + nbx, tmp + nbx + nby);
@@ -1892,7 +2155,7 @@ struct Solver {
         if (P.csd && !any)  // the incoming energy flux is a source too
             for (double v : psiE_in) if (v != 0.0) { any = true; break; }
         if (!any) {  // sn_one_speed.jl:116-120
-            if (P.csd) psiE_out.assign((size_t)A.Nd * Nc, 0.0);
+            if (P.csd) psiE_out.assign((size_t)A.Nd * Nc * NmE, 0.0);
             std::fill(z_phi_bnd.begin(), z_phi_bnd.begin() + nphi, 0.0);
             st.niter = 0; st.resid = 0.0; st.rho = NAN; st.converged = 1; st.passes = 0;
             return;
@@ -2181,7 +2444,7 @@ struct Solver {
             vector<double> z(S, 0.0);
             std::copy(phi_all.begin() + (size_t)g * nphi, phi_all.begin() + (size_t)(g + 1) * nphi, z.begin());
             if (P.csd) {  // sn_flux.jl:166,183: zero for the first group, then rescaled by ΔE_g/ΔE_{g-1}
-                if (g == 0) psiE_in.assign((size_t)A.Nd * Nc, 0.0);
+                if (g == 0) psiE_in.assign((size_t)A.Nd * Nc * NmE, 0.0);
                 else
                     for (size_t i = 0; i < psiE_in.size(); ++i) psiE_in[i] = psiE_out[i] * P.dE[g] / P.dE[g - 1];
             }
@@ -2192,14 +2455,17 @@ struct Solver {
         }
         // sn_flux.jl:213-218 -- flux at the cutoff energy: the energy-face angular flux leaving the
         // last group (not rescaled), projected on the moments with Dn, accumulated over n outermost.
-        // Layout 𝚽cutoff[p, is, ix, iy, iz] (is = 1: the slowing-down types carry one spatial moment).
+        // Layout 𝚽cutoff[p, is, ix, iy, iz] with is over the cell moments; the energy face's NmE
+        // spatial moments fill is = 1..NmE (sn_flux.jl:216), the others stay zero.
         if (cutoff_out) {
             std::fill(cutoff_out, cutoff_out + (size_t)Np * Ncol, 0.0);
             if (P.csd)
                 for (int n = 0; n < A.Nd; ++n)
                     for (int c = 0; c < Nc; ++c)
-                        for (int p = 0; p < Np; ++p)
-                            cutoff_out[p + (size_t)Np * c] += A.B.D[p + (size_t)Np * n] * psiE_out[(size_t)n * Nc + c];
+                        for (int is = 0; is < NmE; ++is)
+                            for (int p = 0; p < Np; ++p)
+                                cutoff_out[p + (size_t)Np * (is + (size_t)K * c)] +=
+                                    A.B.D[p + (size_t)Np * n] * psiE_out[is + (size_t)NmE * (c + (size_t)Nc * n)];
         }
         // Radiant layout 𝚽l[g, p, is, ix, iy, iz]
         for (int c = 0; c < Ncol; ++c)
@@ -2327,6 +2593,35 @@ int oracle_cell_3d(const char* sx, const char* sy, const char* sz, int Ox, int O
         return 0;
     } catch (std::exception& e) { g_err = e.what(); return -1; }
 }
+
+// One slowing-down cell (flux_bfp) of the given closures and orders: axes E, x, y, z; ndims = 1
+// applies the 1D tangential guard.  fe/fx/fy/fz: incoming face moments, overwritten with the
+// outgoing ones; Sm, Sp: S⁻/ΔE and S⁺/ΔE (the in-group moments follow from them).
+int oracle_cell_bfp(const char* se, const char* sx, const char* sy, const char* sz, int Oe, int Ox, int Oy, int Oz,
+                    int partial, int ndims, const double* om, const double* h, double St, double Sm, double Sp,
+                    const double* Qn, double* fe, double* fx, double* fy, double* fz, double* psi) {
+    try {
+        if (Oe < 1 || Oe > 2) fail("energy closures of order above 2 are not supported");
+        double Sv[2];
+        stopping_moments(Sm, Sp, Oe, Sv);
+        for (const char* sc : {se, sx, sy, sz})
+            if (string(sc) != "DD" && string(sc) != "DG") fail("Closure relation not implemented yet.");
+        vector<double> we = scheme_weights_1d(se, Oe), wx = scheme_weights_1d(sx, Ox), wy = scheme_weights_1d(sy, Oy),
+                       wz = scheme_weights_1d(sz, Oz);
+        const double* w[4] = {we.data(), wx.data(), wy.data(), wz.data()};
+        vector<double> W((size_t)Oe * Oe * Oe);
+        for (int a = 0; a < Oe; ++a)
+            for (int b = 0; b < Oe; ++b)
+                for (int c = 0; c < Oe; ++c) W[a + Oe * (b + Oe * c)] = legendre_triple(a, b, c);
+        Cell4 cl(Oe, Ox, Oy, Oz, partial == 0);
+        double* f[4] = {fe, fx, fy, fz};
+        flux_bfp(om, h, St, Sm, Sp, Sv, W.data(), Qn, f, cl, w, ndims == 1 && std::fabs(om[0]) <= 1e-10, psi);
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// ½ ∫ P_a P_b P_c dx (legendre_triple).
+double oracle_legendre_triple(int a, int b, int c) { return legendre_triple(a, b, c); }
 
 // Sizes of a problem: Nd, Np, Np_surf and the Krylov state length S.
 int oracle_problem_sizes(const oracle_problem* p, int* Nd, int* Np, int* Nps, int64_t* S) {

@@ -99,9 +99,8 @@ class Problem:
     T: np.ndarray = None
     siga: np.ndarray = None
     fokker_planck: str = "finite-difference"
-    # spatial closure along x (SN.set_scheme("x", scheme, order)): "DD", "DG", "DG-" or "DG+" and
-    # the number of spatial Legendre moments per cell; orders above 1 and closures other than DD
-    # are 1D BTE only, and the flux then holds scheme_order spatial moments per cell
+    # spatial closure along x (SN.set_scheme("x", scheme, order)): "DD", "DG", "DG-" or "DG+" (the
+    # last two 1D BTE only) and the number of spatial Legendre moments per cell along x
     scheme: str = "DD"
     scheme_order: int = 1
     # the y axis of 2D problems ("DD" or "DG") and the coupling of the high-order moments
@@ -111,6 +110,10 @@ class Problem:
     full_coupling: bool = True
     scheme_z: str = "DD"     # the z axis of 3D problems
     scheme_order_z: int = 1
+    # the energy axis of the slowing-down types (SN.set_scheme("E", scheme, order)): "DD" or "DG";
+    # a cell then holds the moments over the spatial axes and energy, energy fastest
+    scheme_e: str = "DD"
+    scheme_order_e: int = 1
 
     @property
     def Nc(self):

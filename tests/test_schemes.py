@@ -125,9 +125,9 @@ def test_higher_order_differs_and_refusals(oracle):
         r.scheme, r.scheme_order = "DG-", 2
         with pytest.raises(RuntimeError, match="not implemented"):
             oracle.solve(r)
-    q = _slab("DG", 2)   # with slowing down the energy closure couples in: not provided
+    q = _slab("DG-", 2)   # with slowing down the closures are DD or DG (scheme_weights.jl, isCSD)
     q.csd, q.Sb, q.dE = True, np.ones((q.G + 1, q.Nmat)), np.ones(q.G)
-    with pytest.raises(RuntimeError, match="BTE only"):
+    with pytest.raises(RuntimeError, match="not implemented"):
         oracle.solve(q)
 
 
