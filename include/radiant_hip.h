@@ -112,9 +112,9 @@ typedef struct rt_problem_desc {
     /* Spatial closure along x (SN.set_scheme("x", type, order); scheme_weights.jl:25-90,
      * sn_1D_bte.jl:27-64): order_x = 𝒪x spatial Legendre moments per cell (0 or 1: first order, at
      * most 4) and omega_x[𝒪x + 1] the closure weights ω of the x axis (NULL: diamond difference of
-     * order 1).  Any order or closure other than diamond difference of order 1 is for the BTE
-     * (csd = 0), in single problems, lockstep batches and served requests, every precision and
-     * state.  The flux moments then carry Nm5 = 𝒪x
+     * order 1).  Any order or closure other than diamond difference of order 1 works in single
+     * problems, lockstep batches and served requests, every precision and state; with the
+     * slowing-down term (csd = 1) see order_e below.  The flux moments then carry Nm5 = 𝒪x
      * spatial moments per cell: phi_out is 𝚽l[G, Np, Nm5, Nx] and the qlout of rt_problem_pass is
      * [Np, Nm5 * Nx]; qvol keeps one value per cell (spatial moment 1, Source.jl:135). */
     int32_t order_x;
@@ -132,6 +132,18 @@ typedef struct rt_problem_desc {
      * axes). */
     int32_t order_z;
     const double *omega_z;
+    /* The energy axis of the slowing-down types (csd = 1; SN.set_scheme("E", type, order),
+     * scheme_weights.jl with isCSD, flux_{1,2,3}D_BFP of sn_{1,2,3}D_bfp.jl): order_e = 𝒪E energy
+     * Legendre moments per cell (0 or 1: first order, at most 2) and omega_e[𝒪E + 1] its closure
+     * weights (NULL: diamond difference of order 1).  With any closure other than diamond
+     * difference of order 1 on every axis (space or energy), the slowing-down types take the
+     * general cell: spatial orders up to 4 in 1D and 2 per axis in 2D/3D, DD or DG on each axis,
+     * at most 8 moments per cell.  A cell then holds Nm5 moments over (E, x[, y[, z]]), energy
+     * fastest (𝒪E*𝒪x*... with full coupling, 𝒪E + 𝒪x + ... - (ndims) without); the faces carry the
+     * energy moments too (Nm1 = 𝒪E*𝒪y*𝒪z, ...), and the cutoff flux fills the first
+     * Nm4 = 𝒪x*𝒪y*𝒪z moments of each cell (sn_flux.jl:216). */
+    int32_t order_e;
+    const double *omega_e;
 } rt_problem_desc;
 
 /* Per-group result of the in-group solve (the values sn_one_speed.jl:189-193 prints). */

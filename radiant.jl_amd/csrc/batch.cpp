@@ -525,7 +525,7 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
         // before the sweep, and the energy outflow 2 psi - psiE_in is formed after it
         std::vector<std::pair<const PassDesc*, long>> csd_here;
         for (size_t i = 0; i < who.size(); ++i)
-            if (group_of(prec[i]) == pr && pds[i].psiE_in && full[i] != 2)
+            if (group_of(prec[i]) == pr && pds[i].psiE_in && full[i] != 2 && !ps[who[i]].P->bfp)
                 csd_here.push_back({dpd + who[i], (long)ps[who[i]].P->Nd * ps[who[i]].P->Nc});
         if (!csd_here.empty())
             plan.push_back([=] {
@@ -551,13 +551,16 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
             const PassDesc* one = dpd + who[i];
             const int sp = prec[i] == P64 ? (int)P64 : (int)P32;
             const int nd_ = P->Nd, K = P->ho_K, kx = P->ho_Kx, ky = P->ho_Ky, kz = P->ho_Kz, dims = P->ndims;
-            const bool fc = P->ho_fc;
-            const size_t lds = dims == 3 ? sweep_3d_ho_lds_bytes(sp, P->Nx, P->Ny, P->Nz, kx, ky, kz, fc)
+            const bool fc = P->ho_fc, bfp = P->bfp;
+            const int ke = P->ho_KE;
+            const size_t lds = bfp ? sweep_bfp_lds_bytes(sp, dims, P->Nx, P->Ny, P->Nz, ke, kx, ky, kz, fc)
+                             : dims == 3 ? sweep_3d_ho_lds_bytes(sp, P->Nx, P->Ny, P->Nz, kx, ky, kz, fc)
                              : (dims == 2 ? sweep_2d_ho_lds_bytes(sp, P->Nx, P->Ny, kx, ky) : 0);
             const double hu = (double)P->Ncell * P->Nd * ((pr == PMIX && prec[i] == P64) ? 2.0 : 1.0);
             const double hb = 2.0 * (prec[i] == P64 ? 8.0 : 4.0) * (double)P->Nd * P->Nc;
             plan.push_back(timed(1, tix, hu, 0.0, hb, s, [=] {
-                if (dims == 3) launch_sweep_3d_ho(sp, one, nd_, kx, ky, kz, fc, lds, s);
+                if (bfp) launch_sweep_bfp(sp, one, nd_, ke, kx, ky, kz, fc, lds, s);
+                else if (dims == 3) launch_sweep_3d_ho(sp, one, nd_, kx, ky, kz, fc, lds, s);
                 else if (dims == 2) launch_sweep_2d_ho(sp, one, nd_, kx, ky, fc, lds, s);
                 else launch_sweep_1d_ho(sp, one, nd_, K, s);
             }));
@@ -581,9 +584,14 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
     for (size_t i = 0; i < who.size(); ++i)
         if (full[i] == 2 && pds[i].psiE_out) {
             const PassDesc* one = dpd + who[i];
-            const long ne = (long)ps[who[i]].P->Nd * ps[who[i]].P->Nc;
+            const Problem* P = ps[who[i]].P;
+            const long ne = P->bfp ? (long)P->nE() : (long)P->Nd * P->Nc;
+            const bool bfp = P->bfp;
             hipStream_t s0 = st;
-            plan.push_back([=] { launch_csd_out(one, ne, s0); });
+            plan.push_back([=] {
+                if (bfp) launch_bfp_out(one, ne, s0);
+                else launch_csd_out(one, ne, s0);
+            });
         }
     if (any_bout) {
         const int nall = n;
@@ -620,7 +628,7 @@ bool BatchSolver::round() {
             launch_outsrc(P->d_qlout, P->d_phi, P->d_sigo, P->d_qvol, P->d_mat, P->d_act, g, P->G, P->Nmat, P->Nact,
                           P->Np, P->Nc, s0);
             if (P->csd) {  // the energy flux enters group g rescaled by ΔE_g/ΔE_{g-1}; zero for the first (sn_flux.jl:166,183)
-                const size_t ne = (size_t)P->Nd * P->Nc;
+                const size_t ne = P->nE();
                 if (g == 0) {
                     k_fill(P->d_psiE_in, 0.0, ne, s0);
                 } else {
@@ -828,7 +836,7 @@ void BatchSolver::advance(int i, std::vector<VecItem>& pre, std::vector<VecItem>
                 }());
                 if (P->csd) {  // and no energy flux leaves the group (sn_one_speed.jl:119)
                     VecItem it{};
-                    it.op = VOP_FILL; it.n = (long)P->Nd * P->Nc; it.y = P->d_psiE_out;
+                    it.op = VOP_FILL; it.n = (long)P->nE(); it.y = P->d_psiE_out;
                     pre.push_back(it);
                 }
                 s.resid = 0.0;

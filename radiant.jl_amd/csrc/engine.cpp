@@ -57,10 +57,12 @@ static int solver_type_of(const rt_problem_desc& d) { return d.solver_type ? d.s
 static int ho_order(const rt_problem_desc& d) {
     const int Kx = d.order_x > 1 ? d.order_x : 1, Ky = d.ndims >= 2 && d.order_y > 1 ? d.order_y : 1;
     const int Kz = d.ndims >= 3 && d.order_z > 1 ? d.order_z : 1;
+    const int Ke = d.csd && d.order_e > 1 ? d.order_e : 1;
     auto dd1 = [](const double* w) { return !w || (w[0] == -1.0 && w[1] == 2.0); };
-    if (Kx == 1 && Ky == 1 && Kz == 1 && dd1(d.omega_x) && (d.ndims < 2 || dd1(d.omega_y)) &&
-        (d.ndims < 3 || dd1(d.omega_z)))
+    if (Kx == 1 && Ky == 1 && Kz == 1 && Ke == 1 && dd1(d.omega_x) && (d.ndims < 2 || dd1(d.omega_y)) &&
+        (d.ndims < 3 || dd1(d.omega_z)) && (!d.csd || dd1(d.omega_e)))
         return 0;
+    if (d.csd) return bfp_cell_moments(Ke, Kx, Ky, Kz, d.partial_coupling == 0);  // moments over (E, x, y, z)
     if (d.ndims == 1) return Kx;
     if (d.ndims == 2) return d.partial_coupling ? Kx + Ky - 1 : Kx * Ky;
     return d.partial_coupling ? Kx + Ky + Kz - 2 : Kx * Ky * Kz;
@@ -135,8 +137,15 @@ static void validate(const rt_problem_desc& d) {
     if (d.order_x > 1 && !d.omega_x) bad("closures of order above 1 need their weights omega_x");
     if (d.ndims >= 2 && d.order_y > 1 && !d.omega_y) bad("closures of order above 1 need their weights omega_y");
     if (d.ndims >= 3 && d.order_z > 1 && !d.omega_z) bad("closures of order above 1 need their weights omega_z");
-    if (ho_order(d) && d.csd)
-        bad("closures other than diamond difference of order 1 are supported for the BTE only");
+    if (!d.csd && d.order_e > 1) bad("an energy closure needs the slowing-down term (csd)");
+    if (d.csd && ho_order(d)) {
+        const int Ke = d.order_e > 1 ? d.order_e : 1;
+        if (Ke > 1 && !d.omega_e) bad("energy closures of order above 1 need their weights omega_e");
+        if (!sweep_bfp_supported(d.ndims, Ke, std::max(1, d.order_x), d.ndims >= 2 ? std::max(1, d.order_y) : 1,
+                                 d.ndims >= 3 ? std::max(1, d.order_z) : 1, d.partial_coupling == 0))
+            bad("slowing-down closures: energy order at most 2, spatial orders up to 4 in 1D and 2 per axis in "
+                "2D/3D, at most " + std::to_string(SWEEP_BFP_NM) + " moments per cell");
+    }
     if (d.solver_type < 0 || d.solver_type > 6) bad("Unknown type of solver.");
     const int stype = solver_type_of(d);
     if ((stype >= 2) != (d.csd != 0))
@@ -164,7 +173,16 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
     validate(d);
     ndims = d.ndims; Nx = d.Nx; Ny = d.Ny; Nz = d.Nz; Ncell = Nx * Ny * Nz;
     ho_K = ho_order(d);
-    if (ho_K && ndims >= 2) {
+    bfp = d.csd && ho_K;
+    if (bfp) {
+        ho_KE = d.order_e > 1 ? d.order_e : 1;
+        ho_Kx = d.order_x > 1 ? d.order_x : 1;
+        ho_Ky = ndims >= 2 && d.order_y > 1 ? d.order_y : 1;
+        ho_Kz = ndims >= 3 && d.order_z > 1 ? d.order_z : 1;
+        ho_fc = d.partial_coupling == 0;
+        for (int a = 0; a < 3; ++a) fm[a] = bfp_face_moments(a + 1, ho_KE, ho_Kx, ho_Ky, ho_Kz, ho_fc);
+        fm_e = bfp_face_moments(0, ho_KE, ho_Kx, ho_Ky, ho_Kz, ho_fc);
+    } else if (ho_K && ndims >= 2) {
         ho_Kx = d.order_x > 1 ? d.order_x : 1;
         ho_Ky = d.order_y > 1 ? d.order_y : 1;
         ho_Kz = ndims == 3 && d.order_z > 1 ? d.order_z : 1;
@@ -213,7 +231,45 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
         std::vector<int> mat(Nc);  // per moment column
         for (int c = 0; c < Nc; ++c) mat[c] = (int)(d.mat[c / Kc] - 1);
         d_mat = dalloc<int>(Nc); upload(d_mat, mat, st);
-        if (ho_K) {  // closure weights per axis (diamond difference of order 1 when absent)
+        if (bfp) {  // closure weights of every axis and of energy, the energy-face moment map
+            auto weights = [&](const double* w, int K) {
+                std::vector<double> v(w ? w : nullptr, w ? w + K + 1 : nullptr);
+                if (v.empty()) v = {-1.0, 2.0};
+                double* p = dalloc<double>(v.size());
+                upload(p, v, st);
+                return p;
+            };
+            d_omx = weights(d.omega_x, ho_Kx);
+            d_omy = weights(ndims >= 2 ? d.omega_y : nullptr, ho_Ky);
+            d_omz = weights(ndims >= 3 ? d.omega_z : nullptr, ho_Kz);
+            d_ome = weights(d.omega_e, ho_KE);
+            // cell index of (e, a, b, c): energy fastest; without full coupling one index is
+            // non-zero, after the (order - 1) of the axes before it
+            const int O[4] = {ho_KE, ho_Kx, ho_Ky, ho_Kz};
+            auto index = [&](const int* m, int skip) {
+                int r = 0, stride = 1, off = 0;
+                for (int k = 0; k < 4; ++k) {
+                    if (k == skip) continue;
+                    if (ho_fc) { r += m[k] * stride; stride *= O[k]; }
+                    else { if (m[k] > 0) r = m[k] + off; off += O[k] - 1; }
+                }
+                return r;
+            };
+            std::vector<int> emap((size_t)ho_KE * fm_e, -1);
+            for (int a = 0; a < ho_Kx; ++a)
+                for (int b = 0; b < ho_Ky; ++b)
+                    for (int c = 0; c < ho_Kz; ++c) {
+                        if (!ho_fc && (a > 0) + (b > 0) + (c > 0) > 1) continue;
+                        for (int e = 0; e < ho_KE; ++e) {
+                            const int m[4] = {e, a, b, c};
+                            if (!ho_fc && (e > 0) + (a > 0) + (b > 0) + (c > 0) > 1) continue;
+                            emap[(size_t)e * fm_e + index(m, 0)] = index(m, -1);
+                        }
+                    }
+            d_emap = dalloc<int>(emap.size()); upload(d_emap, emap, st);
+            if (sweep_bfp_lds_bytes(P64, ndims, Nx, Ny, Nz, ho_KE, ho_Kx, ho_Ky, ho_Kz, ho_fc) > 160 * 1024)
+                throw Error(RT_EINVAL, "the face moments of the slowing-down closure exceed the 160 KiB LDS of a workgroup");
+        } else if (ho_K) {  // closure weights per axis (diamond difference of order 1 when absent)
             const int kx = ndims >= 2 ? ho_Kx : ho_K;
             std::vector<double> om(d.omega_x ? d.omega_x : nullptr, d.omega_x ? d.omega_x + kx + 1 : nullptr);
             if (om.empty()) om = {-1.0, 2.0};
@@ -252,17 +308,21 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
             for (int m = 0; m < Nmat; ++m) sigt[(size_t)g * Nmat + m] = d.sigt[g + (size_t)G * m];
         csd = d.csd != 0;
         if (csd) {
-            // continuous slowing down (sn_flux.jl:86-97,183-188): S⁻ = Sb(g)/ΔE, S⁺ = Sb(g+1)/ΔE; the
-            // cell removal gains 2 S⁺ and the energy inflow enters with (S⁺ + S⁻)
-            std::vector<double> K((size_t)G * Nmat);
+            // continuous slowing down (sn_flux.jl:86-97,183-188): S⁻ = Sb(g)/ΔE, S⁺ = Sb(g+1)/ΔE.  In
+            // the order-1 cell the removal gains 2 S⁺ and the energy inflow enters with (S⁺ + S⁻);
+            // the general cell takes S⁻ and S⁺ themselves.
+            std::vector<double> K((size_t)G * Nmat), SS((size_t)G * Nmat * 2);
             for (int g = 0; g < G; ++g)
                 for (int m = 0; m < Nmat; ++m) {
                     const double Sm = d.Sb[g + (size_t)(G + 1) * m] / d.dE[g];
                     const double Sp = d.Sb[(g + 1) + (size_t)(G + 1) * m] / d.dE[g];
-                    sigt[(size_t)g * Nmat + m] += 2.0 * Sp;
+                    if (!bfp) sigt[(size_t)g * Nmat + m] += 2.0 * Sp;
                     K[(size_t)g * Nmat + m] = Sp + Sm;
+                    SS[((size_t)g * Nmat + m) * 2] = Sm;
+                    SS[((size_t)g * Nmat + m) * 2 + 1] = Sp;
                 }
-            d_csdK = dalloc<double>(K.size()); upload(d_csdK, K, st);
+            if (bfp) { d_csdS = dalloc<double>(SS.size()); upload(d_csdS, SS, st); }
+            else { d_csdK = dalloc<double>(K.size()); upload(d_csdK, K, st); }
             dE.assign(d.dE, d.dE + G);
         }
         d_sigt = dalloc<double>(sigt.size()); upload(d_sigt, sigt, st);
@@ -409,10 +469,10 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
         d_qlout = dalloc<double>((size_t)Nact * Nc);
         d_psi = dalloc<double>((size_t)Nd * Nc);
         if (csd) {
-            d_psiE_in = dalloc<double>((size_t)Nd * Nc);
-            d_psiE_out = dalloc<double>((size_t)Nd * Nc);
-            k_fill(d_psiE_in, 0.0, (size_t)Nd * Nc, st);
-            k_fill(d_psiE_out, 0.0, (size_t)Nd * Nc, st);
+            d_psiE_in = dalloc<double>(nE());
+            d_psiE_out = dalloc<double>(nE());
+            k_fill(d_psiE_in, 0.0, nE(), st);
+            k_fill(d_psiE_out, 0.0, nE(), st);
         }
         for (int ax = 0; ax < 3; ++ax) d_bex[ax] = dalloc<double>(std::max<size_t>(1, (size_t)Nd * (ax < ndims ? npos[ax] : 1)));
         d_bscratch = dalloc<double>(std::max<size_t>(1, nb));
@@ -508,8 +568,9 @@ int64_t Problem::device_bytes(const rt_problem_desc& d) {
     b += (int64_t)d.Nd * Nc * 8;                                // psi
     b += (int64_t)3 * d.Nd * d.Np * 12;                         // operators
     // face moments: at most the product of the other two orders
-    const int64_t fmax = ho_order(d) && d.ndims >= 2
-                             ? std::max<int64_t>(1, (int64_t)std::max(d.order_x, 1) * std::max(d.order_y, 1) * std::max(d.order_z, 1))
+    const int64_t ke = d.csd ? std::max(d.order_e, 1) : 1;
+    const int64_t fmax = ho_order(d) && (d.ndims >= 2 || ke > 1)
+                             ? std::max<int64_t>(1, ke * std::max(d.order_x, 1) * std::max(d.order_y, 1) * std::max(d.order_z, 1))
                              : 1;
     const int64_t nposs = fmax * ((int64_t)d.Ny * d.Nz + (d.ndims >= 2 ? (int64_t)d.Nx * d.Nz : 0) +
                                   (d.ndims >= 3 ? (int64_t)d.Nx * d.Ny : 0));
@@ -525,7 +586,7 @@ int64_t Problem::device_bytes(const rt_problem_desc& d) {
     if (d.precision != RT_PREC_FP64) nvec += 6;
     b += (int64_t)nvec * S * 8;
     if (d.precision != RT_PREC_FP64) b += (int64_t)d.Nd * Nc * 8;  // angular flux of c = T(0) kept by batches
-    if (d.csd) b += 2LL * d.Nd * Nc * 8;                              // energy-face angular fluxes
+    if (d.csd) b += 2LL * d.Nd * Nc * 8;                              // energy-face angular fluxes (at most Nc per direction)
     if (has_fp(d)) b += (int64_t)Nact_est * (Nc + Nact_est) * 8;     // Fokker-Planck source, ℳ
     b += (int64_t)std::min<int64_t>((int64_t)d.G * d.Np * Nc, std::max<int64_t>(FLUX_STAGE, (int64_t)d.G * d.Np)) * 8;
     return b;
@@ -571,6 +632,10 @@ void Problem::fill_geometry(PassDesc& pd, int g) {
     pd.ho_Kx = ho_Kx; pd.ho_Ky = ho_Ky; pd.ho_fc = ho_fc ? 1 : 0; pd.ho_omy = d_omy;
     pd.fm_x = fm[0]; pd.fm_y = fm[1];
     pd.ho_Kz = ho_Kz; pd.ho_omz = d_omz; pd.fm_z = fm[2];
+    if (bfp) {  // the general slowing-down cell needs S⁻, S⁺ in every pass (homogeneous ones too)
+        pd.ho_KE = ho_KE; pd.ho_ome = d_ome; pd.fm_e = fm_e; pd.emap = d_emap;
+        pd.csdS = d_csdS + (size_t)g * Nmat * 2;
+    }
 }
 
 hipEvent_t Problem::next_event() {
@@ -617,10 +682,13 @@ void Problem::launch_pass(PassDesc& pd, int prec, bool full_rows) {
         ev_used.push_back({a, b, kernel, prec});
     };
     timed(0, [&] { launch_mgemm(prec, d_pd, d_mg, n_mg, st); });
-    if (pd.psiE_in) launch_csd_source(d_pd, (long)Nd * Nc, st);  // energy inflow (FP64 passes)
+    if (pd.psiE_in && !bfp) launch_csd_source(d_pd, (long)Nd * Nc, st);  // energy inflow (FP64 passes)
     timed(1, [&] {
         const int sp = prec == PBF16 ? P32 : prec;
-        if (ho_K && ndims == 3)
+        if (bfp)
+            launch_sweep_bfp(sp, d_pd, Nd, ho_KE, ho_Kx, ho_Ky, ho_Kz, ho_fc,
+                             sweep_bfp_lds_bytes(sp, ndims, Nx, Ny, Nz, ho_KE, ho_Kx, ho_Ky, ho_Kz, ho_fc), st);
+        else if (ho_K && ndims == 3)
             launch_sweep_3d_ho(sp, d_pd, Nd, ho_Kx, ho_Ky, ho_Kz, ho_fc,
                                sweep_3d_ho_lds_bytes(sp, Nx, Ny, Nz, ho_Kx, ho_Ky, ho_Kz, ho_fc), st);
         else if (ho_K && ndims == 2)
@@ -638,7 +706,7 @@ void Problem::launch_pass(PassDesc& pd, int prec, bool full_rows) {
         else
             launch_sweep(sp, d_pd, d_sw, Nd, sw_ppt, sw_nbuf[prec], sw_block, sw_lds[prec], st);
     });
-    if (pd.psiE_out) launch_csd_out(d_pd, (long)Nd * Nc, st);
+    if (pd.psiE_out && !bfp) launch_csd_out(d_pd, (long)Nd * Nc, st);
     timed(2, [&] { launch_dgemm(prec, d_pd, full_rows ? d_dgf : d_dga, full_rows ? n_dgf : n_dga, st); });
     if (pd.bout_x) timed(3, [&] { launch_bndmom(d_pd, 1, (int)nb, st); });
     counters.passes += 1;
@@ -749,7 +817,8 @@ void Problem::set_csd(PassDesc& pd, int g) {
     if (!csd) return;
     pd.psiE_in = d_psiE_in;
     pd.psiE_out = d_psiE_out;
-    pd.csdK = d_csdK + (size_t)g * Nmat;
+    if (bfp) pd.csdS = d_csdS + (size_t)g * Nmat * 2;
+    else pd.csdK = d_csdK + (size_t)g * Nmat;
 }
 
 void Problem::pass_reconstruct(int g, const double* zin_reduced) {
@@ -770,7 +839,10 @@ void Problem::reconstruct_sweep_free(int g, const double* zin_reduced, const dou
     else k_copy((double*)d_psi, psi_c, n, st);
     pd.psi = d_psi;
     k_set_pd(pd, d_pd, st);
-    if (pd.psiE_out) launch_csd_out(d_pd, (long)n, st);
+    if (pd.psiE_out) {
+        if (bfp) launch_bfp_out(d_pd, (long)nE(), st);
+        else launch_csd_out(d_pd, (long)n, st);
+    }
     counters.passes += 1;  // a pass without sweep
     if (profiling) {
         if (ev_next + 2 > 4096) collect_kernel_times();
@@ -1189,17 +1261,17 @@ void Problem::solve_group(int g, rt_group_stats& s) {
     // out-of-group + volume source (sn_flux.jl:175-179)
     launch_outsrc(d_qlout, d_phi, d_sigo, d_qvol, d_mat, d_act, g, G, Nmat, Nact, Np, Nc, st);
     if (csd) {  // the energy flux enters group g rescaled by ΔE_g/ΔE_{g-1}; zero for the first (sn_flux.jl:166,183)
-        if (g == 0) k_fill(d_psiE_in, 0.0, (size_t)Nd * Nc, st);
+        if (g == 0) k_fill(d_psiE_in, 0.0, nE(), st);
         else {
-            k_copy(d_psiE_in, d_psiE_out, (size_t)Nd * Nc, st);
-            k_scale(d_psiE_in, dE[g] / dE[g - 1], (size_t)Nd * Nc, st);
+            k_copy(d_psiE_in, d_psiE_out, nE(), st);
+            k_scale(d_psiE_in, dE[g] / dE[g - 1], nE(), st);
         }
     }
     // zero-source shortcut (sn_one_speed.jl:116-120): no volume or scattering source, no surface
     // source and no incoming energy flux in the group
-    if (!(has_ss && grp_ss[g]) && !(csd && dot(d_psiE_in, d_psiE_in, (size_t)Nd * Nc) != 0.0) &&
+    if (!(has_ss && grp_ss[g]) && !(csd && dot(d_psiE_in, d_psiE_in, nE()) != 0.0) &&
         dot(d_qlout, d_qlout, (size_t)Nact * Nc) == 0.0) {
-        if (csd) k_fill(d_psiE_out, 0.0, (size_t)Nd * Nc, st);
+        if (csd) k_fill(d_psiE_out, 0.0, nE(), st);
         k_fill(d_phi + (size_t)g * Np * Nc, 0.0, (size_t)Np * Nc, st);
         s.niter = 0; s.resid = 0.0; s.rho = std::numeric_limits<double>::quiet_NaN(); s.converged = 1; s.passes = 0;
         if (verbose) print_group(g, s);
@@ -1339,7 +1411,7 @@ void Problem::get_flux_cutoff(double* out, hipStream_t s) {
         return;
     }
     double* tmp = vec("cutoff", n);
-    k_cutoff(tmp, d_D64, d_psiE_out, Np, Nd, Nc, s);
+    k_cutoff(tmp, d_D64, d_psiE_out, Np, Nd, Ncell, Nc / Ncell, fm_e, s);
     RT_CHECK(hipMemcpyAsync(out, tmp, n * sizeof(double), hipMemcpyDeviceToHost, s));
     RT_CHECK(hipStreamSynchronize(s));
 }

@@ -90,6 +90,12 @@ public:
     int ho_Kx = 1, ho_Ky = 1, ho_Kz = 1;  // 2D / 3D closure orders
     bool ho_fc = true;         // full coupling of the 2D moments
     int fm[3] = {1, 1, 1};     // moments of the face fluxes per axis (Ky, Kx for a 2D closure)
+    // slowing-down cells of a general closure (csd with any order or closure other than diamond
+    // difference of order 1): energy order, moments of the energy face; the cell moments ho_K
+    // then span (E, x, y, z)
+    bool bfp = false;
+    int ho_KE = 1, fm_e = 1;
+    size_t nE() const { return (size_t)Nd * Ncell * fm_e; }  // energy-face angular flux values
     int accel, restart, depth, maxit, precision, state_mode, verbose;
     double tol;
     hipStream_t st;
@@ -137,6 +143,9 @@ private:
     bool fp = false;
     double *d_fpM = nullptr, *d_fpT = nullptr, *d_fpq = nullptr;
     double *d_omx = nullptr, *d_omy = nullptr, *d_omz = nullptr;  // closure weights of the general cells per axis
+    double* d_ome = nullptr;   // energy closure weights (slowing-down cells)
+    double* d_csdS = nullptr;  // S⁻/ΔE, S⁺/ΔE per group and material [G][Nmat][2] (slowing-down cells)
+    int* d_emap = nullptr;     // cell moment of (energy moment, energy-face moment), -1 when dropped
     double* d_Mentry[3] = {nullptr, nullptr, nullptr};
     int* d_insrc[3] = {nullptr, nullptr, nullptr};
     double* d_bin[3] = {nullptr, nullptr, nullptr};

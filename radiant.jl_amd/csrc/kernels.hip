@@ -45,6 +45,11 @@ void set_launch_events(hipEvent_t start, hipEvent_t stop) {
     tl_ev_start = start;
     tl_ev_stop = stop;
 }
+void take_launch_events(hipEvent_t* start, hipEvent_t* stop) {
+    *start = tl_ev_start;
+    *stop = tl_ev_stop;
+    tl_ev_start = tl_ev_stop = nullptr;
+}
 template <typename... KArgs, typename... Args>
 static void launch_timed(void (*kernel)(KArgs...), dim3 grid, dim3 block, uint32_t shmem, hipStream_t st,
                          Args... args) {
@@ -2474,12 +2479,15 @@ __global__ void phi_to_radiant_kernel(double* __restrict__ dst, const double* __
 // Flux at the cutoff energy (sn_flux.jl:213-218): out[p + Np*c] = sum_n D[p, n] psiE[n][c], the
 // directions summed in increasing order; DT is the k-major operator DT[n][p] = D[p, n].
 __global__ void cutoff_kernel(double* __restrict__ out, const double* __restrict__ DT, const double* __restrict__ psiE,
-                              int Np, int Nd, int Nc) {
-    const size_t tot = (size_t)Np * Nc;
+                              int Np, int Nd, int Ncell, int K, int FE) {
+    // out[p + Np*(is + K*cell)] = sum_n D[p, n] psiE[n][cell][is] for is < FE, zero above
+    const size_t tot = (size_t)Np * Ncell * K;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < tot; i += (size_t)gridDim.x * blockDim.x) {
-        const size_t p = i % Np, c = i / Np;
+        const size_t p = i % Np, col = i / Np, cell = col / K, is = col % K;
         double acc = 0.0;
-        for (int n = 0; n < Nd; ++n) acc = fma(DT[(size_t)n * Np + p], psiE[(size_t)n * Nc + c], acc);
+        if (is < (size_t)FE)
+            for (int n = 0; n < Nd; ++n)
+                acc = fma(DT[(size_t)n * Np + p], psiE[((size_t)n * Ncell + cell) * FE + is], acc);
         out[i] = acc;
     }
 }
@@ -3127,8 +3135,10 @@ void k_phi_to_radiant(double* dst, const double* phi_all, int G, int Np, int Nc,
                        Nc, c0, nc);
     RT_HIP(hipGetLastError());
 }
-void k_cutoff(double* out, const double* DT, const double* psiE, int Np, int Nd, int Nc, hipStream_t st) {
-    hipLaunchKernelGGL(cutoff_kernel, dim3(grid_for((size_t)Np * Nc)), dim3(256), 0, st, out, DT, psiE, Np, Nd, Nc);
+void k_cutoff(double* out, const double* DT, const double* psiE, int Np, int Nd, int Ncell, int K, int FE,
+              hipStream_t st) {
+    hipLaunchKernelGGL(cutoff_kernel, dim3(grid_for((size_t)Np * Ncell * K)), dim3(256), 0, st, out, DT, psiE, Np, Nd,
+                       Ncell, K, FE);
     RT_HIP(hipGetLastError());
 }
 void k_state_moments_in(double* dst, const double* src, int Np, int Nc, hipStream_t st) {

@@ -79,6 +79,17 @@ struct PassDesc {
     int ho_Kz;
     const double* ho_omz;
     int fm_z;
+    // Slowing-down cells of a general closure (ho_KE > 0; csdK then unused): energy order ho_KE and
+    // its weights ho_ome[ho_KE + 1]; a cell holds the moments over (E, x, y, z), energy fastest, and
+    // the x, y, z faces carry the energy moments too.  The energy face carries fm_e moments over
+    // the spatial axes; psiE_in / psiE_out are then [Nd][cells][fm_e].  csdS: the group's S⁻/ΔE and
+    // S⁺/ΔE per material [Nmat][2].  emap[e * fm_e + m]: the cell moment of energy moment e and
+    // energy-face moment m (-1 when the partial coupling drops it).
+    int ho_KE;
+    const double* ho_ome;
+    int fm_e;
+    const double* csdS;
+    const int* emap;
 };
 
 enum Prec { P64 = 0, P32 = 1, PBF16 = 2,
@@ -154,6 +165,21 @@ void launch_sweep_2d_ho(int prec, const PassDesc* d_pd, int Nd, int Kx, int Ky, 
 // walking the diagonal planes; lds = face moments of every pencil of the three axes
 constexpr int HO3_KMAX = 2;
 size_t sweep_3d_ho_lds_bytes(int prec, int Nx, int Ny, int Nz, int Kx, int Ky, int Kz, bool fc);
+// Slowing-down sweep with a general closure in space and energy (sweep_bfp.hip): one workgroup per
+// direction walks the diagonal planes; orders Ke 1..2, Kx 1..4 in 1D and 1..2 per axis in 2D/3D,
+// at most SWEEP_BFP_NM moments per cell.
+constexpr int SWEEP_BFP_NM = 8;
+int bfp_cell_moments(int Ke, int Kx, int Ky, int Kz, bool fc);
+int bfp_face_moments(int axis, int Ke, int Kx, int Ky, int Kz, bool fc);  // axis 0 = E, 1..3 = x, y, z
+bool sweep_bfp_supported(int ndims, int Ke, int Kx, int Ky, int Kz, bool fc);
+size_t sweep_bfp_lds_bytes(int prec, int ndims, int Nx, int Ny, int Nz, int Ke, int Kx, int Ky, int Kz, bool fc);
+void launch_sweep_bfp(int prec, const PassDesc* d_pd, int Nd, int Ke, int Kx, int Ky, int Kz, bool fc, size_t lds,
+                      hipStream_t st);
+// energy outflow of the slowing-down closure from a pass's angular flux (sweep-free reconstruction):
+// psiE_out = w_0 psiE_in + sum_e C_e (-1)^e w_(e+1) psi_(e, m), n = Nd * cells * fm_e entries
+void launch_bfp_out(const PassDesc* d_pd, long n, hipStream_t st);
+// the timing events set for the next launch (set_launch_events), taken and cleared
+void take_launch_events(hipEvent_t* start, hipEvent_t* stop);
 void launch_sweep_3d_ho(int prec, const PassDesc* d_pd, int Nd, int Kx, int Ky, int Kz, bool fc, size_t lds,
                         hipStream_t st);
 void launch_dgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st);
@@ -249,7 +275,10 @@ void launch_vecops(const VecItem* items, int nitems, int blocks_per_item, hipStr
 // Layout converters of the host boundary (Radiant order <-> device order).
 void k_phi_to_radiant(double* dst, const double* phi_all, int G, int Np, int Nc, int c0, int nc, hipStream_t st);
 // out[p + Np*c] = sum_n DT[n][p] psiE[n][c] (the cutoff-energy flux moments, Radiant's layout)
-void k_cutoff(double* out, const double* DT, const double* psiE, int Np, int Nd, int Nc, hipStream_t st);
+// 𝚽cutoff [Np][K*cells] from the energy-face angular flux psiE [Nd][cells][FE]: the first FE of each
+// cell's K moments, the others zero
+void k_cutoff(double* out, const double* DT, const double* psiE, int Np, int Nd, int Ncell, int K, int FE,
+              hipStream_t st);
 void k_state_moments_in(double* dst, const double* src, int Np, int Nc, hipStream_t st);   // [c][p] -> [p][c]
 void k_state_moments_out(double* dst, const double* src, int Np, int Nc, hipStream_t st);  // [p][c] -> [c][p]
 

@@ -37,14 +37,16 @@ def weights_1d(scheme, K):
 
 
 def spatial_moments(prob):
-    """Spatial moments per cell of a problem: its 1D scheme order; in 2D Ox*Oy with full coupling
-    and Ox + Oy - 1 without, in 3D Ox*Oy*Oz and Ox + Oy + Oz - 2 (SN.get_schemes: Nm[5])."""
-    ox = max(1, int(getattr(prob, "scheme_order", 1) or 1))
-    if prob.ndims == 1:
-        return ox
-    oy = max(1, int(getattr(prob, "scheme_order_y", 1) or 1))
-    fc = getattr(prob, "full_coupling", True)
-    if prob.ndims == 2:
-        return ox * oy if fc else ox + oy - 1
-    oz = max(1, int(getattr(prob, "scheme_order_z", 1) or 1))
-    return ox * oy * oz if fc else ox + oy + oz - 2
+    """Legendre moments per cell of a problem (SN.get_schemes: Nm[5]) over its spatial axes and,
+    for the slowing-down types, energy: the product of the orders with full coupling, 1 + the sum
+    of (order - 1) without (1D: the x order; 2D Ox*Oy or Ox + Oy - 1; ...)."""
+    orders = [max(1, int(getattr(prob, "scheme_order", 1) or 1))]
+    if prob.ndims >= 2:
+        orders.append(max(1, int(getattr(prob, "scheme_order_y", 1) or 1)))
+    if prob.ndims >= 3:
+        orders.append(max(1, int(getattr(prob, "scheme_order_z", 1) or 1)))
+    if getattr(prob, "csd", False):
+        orders.append(max(1, int(getattr(prob, "scheme_order_e", 1) or 1)))
+    if getattr(prob, "full_coupling", True):
+        return int(np.prod(orders))
+    return 1 + sum(o - 1 for o in orders)

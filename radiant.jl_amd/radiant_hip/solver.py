@@ -91,9 +91,20 @@ class Desc:
                     k["omega_z"] = np.ascontiguousarray(schemes.weights_1d(scheme_z, oz))
                     d.order_z, d.omega_z = oz, ptr(k["omega_z"])
                 d.partial_coupling = 0 if getattr(prob, "full_coupling", True) else 1
-        elif K > 1 or scheme != "DD":  # closure weights of the x axis (scheme_weights.jl)
-            k["omega_x"] = np.ascontiguousarray(schemes.weights_1d(scheme, K))
-            d.order_x, d.omega_x = K, ptr(k["omega_x"])
+        elif ox > 1 or scheme != "DD":  # closure weights of the x axis (scheme_weights.jl)
+            k["omega_x"] = np.ascontiguousarray(schemes.weights_1d(scheme, ox))
+            d.order_x, d.omega_x = ox, ptr(k["omega_x"])
+        scheme_e = (getattr(prob, "scheme_e", None) or "DD").upper()
+        oe = max(1, int(getattr(prob, "scheme_order_e", 1) or 1))
+        if oe > 1 or scheme_e != "DD":  # the energy closure of the slowing-down types
+            if not getattr(prob, "csd", False):
+                raise ValueError("an energy closure needs the slowing-down term (csd)")
+            if scheme_e not in ("DD", "DG"):
+                raise ValueError("Closure relation not implemented yet.")
+            k["omega_e"] = np.ascontiguousarray(schemes.weights_1d(scheme_e, oe))
+            d.order_e, d.omega_e = oe, ptr(k["omega_e"])
+        if getattr(prob, "csd", False):
+            d.partial_coupling = 0 if getattr(prob, "full_coupling", True) else 1
         if Mfp is not None:  # T [G, Nmat], ℳ [Np, Np], both column-major
             k["T"] = f64(T)
             k["Mfp"] = f64(Mfp)

@@ -261,3 +261,118 @@ def test_refusals(oracle):
     q = _bfp(copy.deepcopy(p), (2, 2, 1, 1), schemes=("DD", "DG-", "DD", "DD"))
     with pytest.raises(RuntimeError, match="Closure relation"):
         oracle.solve(q)
+
+
+# ------------------------------------------------------------------------------------------
+# GPU parity (sweep_bfp_kernel, bfp_out_kernel, the cutoff flux of the general cell)
+# ------------------------------------------------------------------------------------------
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+# (ndims, orders (E, x, y, z), schemes (E, x, y, z), full coupling, boundary conditions)
+GPU_CASES = [
+    (1, (2, 2, 1, 1), ("DG", "DG", "DD", "DD"), True, None),
+    (1, (2, 1, 1, 1), ("DG", "DD", "DD", "DD"), True, [1, 2]),
+    (1, (1, 3, 1, 1), ("DD", "DD", "DD", "DD"), True, None),
+    (1, (2, 3, 1, 1), ("DD", "DG", "DD", "DD"), False, [2, 2]),
+    (1, (2, 4, 1, 1), ("DG", "DD", "DD", "DD"), True, None),
+    (2, (2, 2, 1, 1), ("DG", "DD", "DD", "DD"), True, [1, 0, 2, 2]),
+    (2, (2, 2, 2, 1), ("DG", "DG", "DG", "DD"), True, None),
+    (2, (2, 2, 2, 1), ("DD", "DD", "DD", "DD"), False, [1, 1, 1, 1]),
+    (3, (2, 2, 1, 1), ("DG", "DD", "DD", "DD"), True, None),
+    (3, (1, 2, 2, 2), ("DD", "DG", "DG", "DG"), True, [1, 1, 0, 0, 2, 2]),
+    (3, (2, 2, 2, 2), ("DG", "DD", "DD", "DD"), False, None),
+]
+
+
+def _gpu_problem(case):
+    nd, orders, schemes, fc, bc = GPU_CASES[case]
+    P = _P()
+    dims = {1: (9,), 2: (5, 4), 3: (3, 4, 3)}[nd]
+    p = P.small_problem(nd, dims=dims if nd > 1 else None, N=dims[0], G=3, L=2, tol=1e-12, seed=120 + case, bc=bc)
+    return _bfp(p, orders, schemes=schemes, fc=fc, seed=130 + case, scale=0.8)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", range(len(GPU_CASES)))
+def test_bfp_closure_gpu_parity(oracle, case):
+    from radiant_hip import solver as S
+    p = _gpu_problem(case)
+    _say("bfp closure", GPU_CASES[case])
+    ref, rcut, rst = oracle.solve_cutoff(p)
+    assert all(s["converged"] for s in rst)
+    K = oracle.spatial_moments(p)
+    h = S.ResidentProblem(p)
+    try:
+        nd_, np_, nps, Sl = oracle.sizes(p)
+        assert Sl == h.state_length()
+        rng = np.random.default_rng(600 + case)
+        z = rng.standard_normal(Sl)
+        q = rng.standard_normal(np_ * p.Nc * K)
+        q.reshape(-1, np_)[:, [i for i in range(np_) if h.d.ang.pl[i] > p.Lxs]] = 0.0
+        for mode in (0, 1, 2):
+            want = oracle.transport_pass(p, 0, 0 if mode == 0 else 1, z, q)
+            if mode == 2:
+                want = z - want
+            e = rel_err(h.transport_pass(0, mode, q, z), want)
+            _say("  pass mode", mode, e)
+            assert e < 1e-12
+    finally:
+        h.close()
+    phi, cut, st = S.compute_flux_cutoff(p)
+    e, ec = rel_err(phi, ref), rel_err(cut, rcut)
+    _say("  solve", e, "cutoff", ec, [s["niter"] for s in st], [s["niter"] for s in rst])
+    assert e < 1e-10 and ec < 1e-10 and all(s["converged"] for s in st)
+    assert np.abs(rcut).max() > 0
+    mx, _ = S.compute_flux(p, precision="mixed-fp32", state="reduced")
+    e = rel_err(mx, ref)
+    _say("  mixed-fp32", e)
+    assert e < 1e-6
+
+
+@pytest.mark.gpu
+def test_bfp_closure_fokker_planck_and_accelerations(oracle):
+    """BFP (Galerkin ℳ) with DG 2 in energy and space, through GMRES, Anderson and BiCGStab."""
+    from radiant_hip import solver as S
+    P = _P()
+    p = _bfp(P.small_problem(1, N=10, G=3, L=3, tol=1e-12, seed=140), (2, 2, 1, 1), schemes=("DG", "DG", "DD", "DD"),
+             seed=141)
+    rng = np.random.default_rng(142)
+    p.solver_type = "BFP"
+    p.T = rng.uniform(0.2, 1.0, (p.G, p.Nmat))
+    p.fokker_planck = "galerkin"
+    for acc in ("gmres", "anderson", "bicgstab"):
+        q = copy.deepcopy(p)
+        q.accel = acc
+        ref, _ = oracle.solve(q)
+        phi, st = S.compute_flux(q)
+        e = rel_err(phi, ref)
+        _say("bfp fp", acc, e)
+        assert e < 1e-10 and all(s["converged"] for s in st)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["mixed-fp32", "mixed-bf16"])
+def test_bfp_closure_in_lockstep_batch(oracle, precision):
+    """General slowing-down cells in the lockstep solver (their own sweep launch per round, the
+    energy outflow of the sweep-free reconstruction from the closure), mixed with an order-1
+    slowing-down problem and a BTE problem: within 1e-6 of the oracle, bit-identical alone and in
+    the mix, and through the server."""
+    from radiant_hip import solver as S
+    P = _P()
+    probs = [_gpu_problem(0), _gpu_problem(5), _gpu_problem(8),
+             _bfp(P.small_problem(2, N=6, G=3, L=2, tol=1e-12, seed=150), (1, 1, 1, 1), seed=151),
+             P.small_problem(2, N=7, G=3, L=2, tol=1e-12, seed=152)]
+    phis, stats, _ = S.solve_batch(probs, precision=precision)
+    for i, p in enumerate(probs):
+        ref, _ = oracle.solve(p)
+        e = rel_err(phis[i], ref)
+        _say("bfp batch", precision, i, e)
+        assert e < 1e-6 and all(s["converged"] for s in stats[i])
+    alone, _, _ = S.solve_batch([probs[1]], precision=precision)
+    assert np.array_equal(alone[0], phis[1])
+    served = S.serve_alone(probs[0], precision=precision)
+    assert rel_err(served, phis[0]) < 1e-6
