@@ -553,9 +553,7 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
             const int nd_ = P->Nd, K = P->ho_K, kx = P->ho_Kx, ky = P->ho_Ky, kz = P->ho_Kz, dims = P->ndims;
             const bool fc = P->ho_fc, bfp = P->bfp;
             const int ke = P->ho_KE;
-            const size_t lds = bfp ? sweep_bfp_lds_bytes(sp, dims, P->Nx, P->Ny, P->Nz, ke, kx, ky, kz, fc)
-                             : dims == 3 ? sweep_3d_ho_lds_bytes(sp, P->Nx, P->Ny, P->Nz, kx, ky, kz, fc)
-                             : (dims == 2 ? sweep_2d_ho_lds_bytes(sp, P->Nx, P->Ny, kx, ky) : 0);
+            const size_t lds = P->ho_lds(sp);
             const double hu = (double)P->Ncell * P->Nd * ((pr == PMIX && prec[i] == P64) ? 2.0 : 1.0);
             const double hb = 2.0 * (prec[i] == P64 ? 8.0 : 4.0) * (double)P->Nd * P->Nc;
             plan.push_back(timed(1, tix, hu, 0.0, hb, s, [=] {

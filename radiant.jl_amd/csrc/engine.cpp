@@ -267,8 +267,8 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
                         }
                     }
             d_emap = dalloc<int>(emap.size()); upload(d_emap, emap, st);
-            if (sweep_bfp_lds_bytes(P64, ndims, Nx, Ny, Nz, ho_KE, ho_Kx, ho_Ky, ho_Kz, ho_fc) > 160 * 1024)
-                throw Error(RT_EINVAL, "the face moments of the slowing-down closure exceed the 160 KiB LDS of a workgroup");
+            const size_t fb = sweep_bfp_lds_bytes(P64, ndims, Nx, Ny, Nz, ho_KE, ho_Kx, ho_Ky, ho_Kz, ho_fc);
+            if (fb > 160 * 1024) d_hoface = dalloc<double>((size_t)Nd * fb / sizeof(double));
         } else if (ho_K) {  // closure weights per axis (diamond difference of order 1 when absent)
             const int kx = ndims >= 2 ? ho_Kx : ho_K;
             std::vector<double> om(d.omega_x ? d.omega_x : nullptr, d.omega_x ? d.omega_x + kx + 1 : nullptr);
@@ -288,8 +288,8 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
                 if (oz.empty()) oz = {-1.0, 2.0};
                 d_omy = dalloc<double>(oy.size()); upload(d_omy, oy, st);
                 d_omz = dalloc<double>(oz.size()); upload(d_omz, oz, st);
-                if (sweep_3d_ho_lds_bytes(P64, Nx, Ny, Nz, ho_Kx, ho_Ky, ho_Kz, ho_fc) > 160 * 1024)
-                    throw Error(RT_EINVAL, "the face moments of the 3D closure exceed the 160 KiB LDS of a workgroup");
+                const size_t fb = sweep_3d_ho_lds_bytes(P64, Nx, Ny, Nz, ho_Kx, ho_Ky, ho_Kz, ho_fc);
+                if (fb > 160 * 1024) d_hoface = dalloc<double>((size_t)Nd * fb / sizeof(double));
             }
         }
         auto up = [&](const double* src, int n) {
@@ -575,6 +575,7 @@ int64_t Problem::device_bytes(const rt_problem_desc& d) {
     const int64_t nposs = fmax * ((int64_t)d.Ny * d.Nz + (d.ndims >= 2 ? (int64_t)d.Nx * d.Nz : 0) +
                                   (d.ndims >= 3 ? (int64_t)d.Nx * d.Ny : 0));
     b += (int64_t)d.Nd * nposs * 8 + 2LL * d.Nps * nposs * 8;
+    if (ho_order(d) && d.ndims >= 2 && nposs * 8 > 160 * 1024) b += (int64_t)d.Nd * nposs * 8;  // walker faces in HBM
     const bool faithful = d.precision == RT_PREC_FP64 && d.state_mode == RT_STATE_FAITHFUL;
     bool nonvoid = false;
     for (int f = 0; f < 2 * d.ndims && d.bc; ++f) nonvoid = nonvoid || d.bc[f] != 0;
@@ -632,10 +633,19 @@ void Problem::fill_geometry(PassDesc& pd, int g) {
     pd.ho_Kx = ho_Kx; pd.ho_Ky = ho_Ky; pd.ho_fc = ho_fc ? 1 : 0; pd.ho_omy = d_omy;
     pd.fm_x = fm[0]; pd.fm_y = fm[1];
     pd.ho_Kz = ho_Kz; pd.ho_omz = d_omz; pd.fm_z = fm[2];
+    pd.hoface = d_hoface;
     if (bfp) {  // the general slowing-down cell needs S⁻, S⁺ in every pass (homogeneous ones too)
         pd.ho_KE = ho_KE; pd.ho_ome = d_ome; pd.fm_e = fm_e; pd.emap = d_emap;
         pd.csdS = d_csdS + (size_t)g * Nmat * 2;
     }
+}
+
+size_t Problem::ho_lds(int sp) const {
+    if (d_hoface) return 0;  // faces in device memory
+    if (bfp) return sweep_bfp_lds_bytes(sp, ndims, Nx, Ny, Nz, ho_KE, ho_Kx, ho_Ky, ho_Kz, ho_fc);
+    if (ho_K && ndims == 3) return sweep_3d_ho_lds_bytes(sp, Nx, Ny, Nz, ho_Kx, ho_Ky, ho_Kz, ho_fc);
+    if (ho_K && ndims == 2) return sweep_2d_ho_lds_bytes(sp, Nx, Ny, ho_Kx, ho_Ky);
+    return 0;
 }
 
 hipEvent_t Problem::next_event() {
@@ -686,11 +696,9 @@ void Problem::launch_pass(PassDesc& pd, int prec, bool full_rows) {
     timed(1, [&] {
         const int sp = prec == PBF16 ? P32 : prec;
         if (bfp)
-            launch_sweep_bfp(sp, d_pd, Nd, ho_KE, ho_Kx, ho_Ky, ho_Kz, ho_fc,
-                             sweep_bfp_lds_bytes(sp, ndims, Nx, Ny, Nz, ho_KE, ho_Kx, ho_Ky, ho_Kz, ho_fc), st);
+            launch_sweep_bfp(sp, d_pd, Nd, ho_KE, ho_Kx, ho_Ky, ho_Kz, ho_fc, ho_lds(sp), st);
         else if (ho_K && ndims == 3)
-            launch_sweep_3d_ho(sp, d_pd, Nd, ho_Kx, ho_Ky, ho_Kz, ho_fc,
-                               sweep_3d_ho_lds_bytes(sp, Nx, Ny, Nz, ho_Kx, ho_Ky, ho_Kz, ho_fc), st);
+            launch_sweep_3d_ho(sp, d_pd, Nd, ho_Kx, ho_Ky, ho_Kz, ho_fc, ho_lds(sp), st);
         else if (ho_K && ndims == 2)
             launch_sweep_2d_ho(sp, d_pd, Nd, ho_Kx, ho_Ky, ho_fc, sweep_2d_ho_lds_bytes(sp, Nx, Ny, ho_Kx, ho_Ky), st);
         else if (ho_K)

@@ -95,6 +95,10 @@ public:
     // then span (E, x, y, z)
     bool bfp = false;
     int ho_KE = 1, fm_e = 1;
+    // the 3D and slowing-down walkers keep their face moments in device memory when they exceed
+    // the LDS (d_hoface, [Nd][faces] doubles); dynamic LDS of their launch in precision sp
+    double* d_hoface = nullptr;
+    size_t ho_lds(int sp) const;
     size_t nE() const { return (size_t)Nd * Ncell * fm_e; }  // energy-face angular flux values
     int accel, restart, depth, maxit, precision, state_mode, verbose;
     double tol;

@@ -2204,7 +2204,10 @@ __global__ __launch_bounds__(256) void sweep_3d_ho_kernel(const PassDesc* __rest
     const PassDesc& d = pds[0];
     const int n = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
     const int Nx = d.Nx, Ny = d.Ny, Nz = d.Nz, Nc = d.Nc;
-    T* FX = (T*)smem;                        // [Ny*Nz][F0]
+    // face moments in LDS, or in this direction's slice of hoface (the same walk: the waves of the
+    // workgroup share one CU, so __syncthreads orders their device-memory writes and reads too)
+    const size_t nface = (size_t)Ny * Nz * F0 + (size_t)Nx * Nz * F1 + (size_t)Nx * Ny * F2;
+    T* FX = d.hoface ? (T*)d.hoface + (size_t)n * nface : (T*)smem;  // [Ny*Nz][F0]
     T* FY = FX + (size_t)Ny * Nz * F0;       // [Nx*Nz][F1]
     T* FZ = FY + (size_t)Nx * Nz * F1;       // [Nx*Ny][F2]
     const double om[3] = {d.mu[n], d.eta[n], d.xi[n]};

@@ -90,6 +90,10 @@ struct PassDesc {
     int fm_e;
     const double* csdS;
     const int* emap;
+    // Face moments of the 3D and slowing-down walkers in device memory instead of LDS (grids whose
+    // faces exceed the 160 KiB of a workgroup): [Nd][faces] in the pass's compute type, launched
+    // with no dynamic LDS.  Null: LDS.
+    void* hoface;
 };
 
 enum Prec { P64 = 0, P32 = 1, PBF16 = 2,

@@ -85,7 +85,10 @@ __global__ __launch_bounds__(256) void sweep_bfp_kernel(const PassDesc* __restri
     const int n = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
     const int nd = d.ndims, Nx = d.Nx, Ny = d.Ny, Nz = d.Nz, Nc = d.Nc;
     const size_t Ncell = (size_t)Nx * Ny * Nz;
-    T* FX = (T*)smem;                                     // [Ny*Nz][F0]
+    // face moments in LDS, or in this direction's slice of hoface (the waves of the workgroup share
+    // one CU, so __syncthreads orders their device-memory writes and reads as it does for LDS)
+    const size_t nface = (size_t)Ny * Nz * F0 + (nd >= 2 ? (size_t)Nx * Nz * F1 : 0) + (nd >= 3 ? (size_t)Nx * Ny * F2 : 0);
+    T* FX = d.hoface ? (T*)d.hoface + (size_t)n * nface : (T*)smem;  // [Ny*Nz][F0]
     T* FY = FX + (size_t)Ny * Nz * F0;                    // [Nx*Nz][F1] (2D, 3D)
     T* FZ = FY + (nd >= 2 ? (size_t)Nx * Nz * F1 : 0);    // [Nx*Ny][F2] (3D)
     const double om[3] = {d.mu[n], nd >= 2 ? d.eta[n] : 0.0, nd >= 3 ? d.xi[n] : 0.0};

@@ -376,3 +376,41 @@ def test_bfp_closure_in_lockstep_batch(oracle, precision):
     assert np.array_equal(alone[0], phis[1])
     served = S.serve_alone(probs[0], precision=precision)
     assert rel_err(served, phis[0]) < 1e-6
+
+
+@pytest.mark.gpu
+def test_bfp_faces_in_device_memory(oracle):
+    """3D slowing-down cells whose face moments exceed the LDS (72 x 72 x 2, energy DG 2 and
+    spatial 2 x 2 x 1 with full coupling: 341 KiB in FP64) keep them in device memory: pass, FP64
+    solve with its cutoff flux, and mixed FP32 against the oracle."""
+    import os
+    from radiant_hip import solver as S
+    P = _P()
+    p = _bfp(P.small_problem(3, dims=(72, 72, 2), G=2, L=1, tol=1e-12, seed=170, bc=[1, 0, 2, 2, 0, 1]),
+             (2, 2, 2, 1), schemes=("DG", "DD", "DG", "DD"), seed=171, scale=0.6)
+    p.qvol = np.zeros_like(p.qvol)
+    p.qvol[0, : p.Nx * 30] = 1.0   # a source in the first 30 rows of both layers
+    assert 8 * (72 * 2 * 4 + 72 * 2 * 4 + 72 * 72 * 8) > 160 * 1024
+    nth = max(1, min(16, os.cpu_count() or 1))
+    ref, rcut, rst = oracle.solve_cutoff(p, nthreads=nth)
+    assert all(s["converged"] for s in rst)
+    K = oracle.spatial_moments(p)
+    h = S.ResidentProblem(p)
+    try:
+        nd, np_, nps, Sl = oracle.sizes(p)
+        rng = np.random.default_rng(172)
+        z = rng.standard_normal(Sl)
+        q = rng.standard_normal(np_ * p.Nc * K)
+        q.reshape(-1, np_)[:, [i for i in range(np_) if h.d.ang.pl[i] > p.Lxs]] = 0.0
+        want = oracle.transport_pass(p, 0, 1, z, q, nthreads=nth)
+        e = rel_err(h.transport_pass(0, 1, q, z), want)
+        _say("bfp faces in HBM: pass", e)
+        assert e < 1e-12
+    finally:
+        h.close()
+    phi, cut, st = S.compute_flux_cutoff(p)
+    e, ec = rel_err(phi, ref), rel_err(cut, rcut)
+    mx, _ = S.compute_flux(p, precision="mixed-fp32", state="reduced")
+    em = rel_err(mx, ref)
+    _say("bfp faces in HBM: solve", e, "cutoff", ec, "mixed-fp32", em)
+    assert e < 1e-10 and ec < 1e-10 and em < 1e-6 and all(s["converged"] for s in st)

@@ -152,3 +152,41 @@ def test_scheme3d_gpu_parity(oracle, case):
     e = rel_err(mx, ref)
     _say("  mixed-fp32", e)
     assert e < 1e-6
+
+
+@pytest.mark.gpu
+def test_scheme3d_faces_in_device_memory(oracle):
+    """A grid whose face moments exceed the 160 KiB LDS of a workgroup (72 x 72 x 2 at 2x2x2 with
+    full coupling: 175 KiB in FP64) keeps them in device memory; passes, FP64 and mixed solves
+    match the oracle as in LDS."""
+    import os
+    from radiant_hip import problems as P, solver as S
+    p = _set(P.small_problem(3, dims=(72, 72, 2), G=2, L=1, tol=1e-12, seed=160, bc=[1, 0, 0, 2, 0, 0]),
+             "DD", "DG", "DD", 2, 2, 2, True)
+    p.qvol = np.zeros_like(p.qvol)
+    p.qvol[0, : p.Nx * 30] = 1.0   # a source in the first 30 rows of both layers
+    faces = 8 * (72 * 2 * 4 + 72 * 2 * 4 + 72 * 72 * 4)
+    assert faces > 160 * 1024
+    nth = max(1, min(16, os.cpu_count() or 1))
+    ref, rst = oracle.solve(p, nthreads=nth)
+    assert all(s["converged"] for s in rst)
+    K = oracle.spatial_moments(p)
+    h = S.ResidentProblem(p)
+    try:
+        nd, np_, nps, Sl = oracle.sizes(p)
+        rng = np.random.default_rng(161)
+        z = rng.standard_normal(Sl)
+        q = rng.standard_normal(np_ * p.Nc * K)
+        q.reshape(-1, np_)[:, [i for i in range(np_) if h.d.ang.pl[i] > p.Lxs]] = 0.0
+        want = oracle.transport_pass(p, 1, 0, z, q, nthreads=nth)
+        e = rel_err(h.transport_pass(1, 0, q, z), want)
+        _say("faces in HBM: pass", e)
+        assert e < 1e-12
+    finally:
+        h.close()
+    phi, st = S.compute_flux(p)
+    e = rel_err(phi, ref)
+    mx, _ = S.compute_flux(p, precision="mixed-fp32", state="reduced")
+    em = rel_err(mx, ref)
+    _say("faces in HBM: solve", e, "mixed-fp32", em)
+    assert e < 1e-10 and em < 1e-6 and all(s["converged"] for s in st)
