@@ -281,6 +281,19 @@ void BatchSolver::flush_graph() {
     RT_CHECK(hipGraphLaunch(ex, st));
 }
 
+// The contractions of a round with FP64 and FP32 passes run as one launch of the mixed-precision
+// kernel (each workgroup takes its problem's precision).  RADIANT_HIP_SPLIT_GEMM=1 runs them as
+// an FP64 launch and then an FP32 launch, each kernel at its own occupancy instead; the results
+// are the same bits, and on the 256-problem 2D batch the single mixed launch measured faster
+// (1021 vs 1037 ms per batch): the FP64 launch alone leaves the device underfilled.
+static bool mixed_gemm() {
+    static const bool on = [] {
+        const char* e = std::getenv("RADIANT_HIP_SPLIT_GEMM");
+        return !(e && e[0] == '1');
+    }();
+    return on;
+}
+
 std::function<void()> BatchSolver::timed(int kernel, int prec, double upd, double flop, double bytes,
                                          hipStream_t s, std::function<void()> fn) {
     if (!profiling) return fn;
@@ -451,8 +464,16 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
         double g_mby = 0, d_by = 0;
         std::map<SKey, double> s_by;
         std::vector<size_t> ho_here;  // problems of this group with a closure above order 1
+        // In a round with FP64 and FP32 passes the sweeps share one launch (latency-bound); with
+        // the split the contractions run as an FP64 launch and then an FP32 launch, the FP64
+        // problems' tasks first in the task tables.
+        const bool split = pr == PMIX && !mixed_gemm();
+        int moff64 = 0, doff64 = 0;
+        for (int part = 0; part < (split ? 2 : 1); ++part) {
+        if (part == 1) { moff64 = moff; doff64 = doff; }
         for (size_t i = 0; i < who.size(); ++i) {
             if (group_of(prec[i]) != pr) continue;
+            if (split && (prec[i] == P64) != (part == 0)) continue;
             const Problem* P = ps[who[i]].P;
             const int ntile = (P->Nc + GEMM_CELLS - 1) / GEMM_CELLS;
             const int nrg_m = (P->Nd + GEMM_ROWS - 1) / GEMM_ROWS;
@@ -488,6 +509,7 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
             const int sp = bprec >= 0 ? bprec : (pr == PBF16 ? (int)P32 : (pr == PMIX ? (int)P64 : pr));
             slds[key] = std::max(slds[key], sweep_scan_lds_bytes(sp, P->ndims, P->Nx, P->Ny, P->Nz, P->Nmat));
         }
+        }
         if (dmeta.empty()) continue;
         // Largest problems first in every sweep launch: workgroups are dispatched in task order,
         // so the directions of the problems with the most cells start first and the launch does
@@ -520,7 +542,24 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
             if (nm) launch_expand_gemm(dmm, nm, mg, s);
             launch_expand_gemm(ddm, nd, dg, s);
         });
-        if (nm) plan.push_back(timed(0, tix, g_upd, g_mflop, g_mby, s, [=] { launch_mgemm(ps_, dpd, mg, moff, s); }));
+        // the split launches of a mixed round: FP64 tasks [0, n64), FP32 tasks [n64, n); the
+        // timing events of the pair start at the first launch and stop at the last
+        auto gemm2 = [split](void (*fn)(int, const PassDesc*, const GemmTask*, int, hipStream_t), int prec_,
+                             const PassDesc* pd, GemmTask* t, int n64, int n, hipStream_t s_) {
+            if (!split || n64 == 0 || n64 == n) {
+                fn(split ? (n64 == n ? (int)P64 : (int)P32) : prec_, pd, t, n, s_);
+                return;
+            }
+            hipEvent_t a, b;
+            take_launch_events(&a, &b);
+            set_launch_events(a, nullptr);
+            fn(P64, pd, t, n64, s_);
+            set_launch_events(nullptr, b);
+            fn(P32, pd, t + n64, n - n64, s_);
+            set_launch_events(nullptr, nullptr);
+        };
+        if (nm)
+            plan.push_back(timed(0, tix, g_upd, g_mflop, g_mby, s, [=] { gemm2(launch_mgemm, ps_, dpd, mg, moff64, moff, s); }));
         // slowing down (FP64 inhomogeneous passes): the energy inflow joins the angular source
         // before the sweep, and the energy outflow 2 psi - psiE_in is formed after it
         std::vector<std::pair<const PassDesc*, long>> csd_here;
@@ -567,7 +606,7 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
             plan.push_back([=] {
                 for (auto& c : csd_here) launch_csd_out(c.first, c.second, s);
             });
-        plan.push_back(timed(2, tix, d_upd, g_dflop, d_by, s, [=] { launch_dgemm(ps_, dpd, dg, doff, s); }));
+        plan.push_back(timed(2, tix, d_upd, g_dflop, d_by, s, [=] { gemm2(launch_dgemm, ps_, dpd, dg, doff64, doff, s); }));
     }
     if (fork) {
         hipStream_t s0 = st, s1 = st64;

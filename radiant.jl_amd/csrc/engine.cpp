@@ -835,6 +835,16 @@ void Problem::pass_reconstruct(int g, const double* zin_reduced) {
     launch_pass(pd, P64, true);
 }
 
+// RADIANT_HIP_RECON_FOLD=0: the reconstruction sums the angular fluxes with an axpy before its D
+// contraction instead of inside it (for comparisons; the results agree to rounding).
+static bool fold_off() {
+    static const bool off = [] {
+        const char* e = std::getenv("RADIANT_HIP_RECON_FOLD");
+        return e && e[0] == '0';
+    }();
+    return off;
+}
+
 // The sweep is linear in its angular source and inflows, so the angular flux of T(z) is that of
 // A z (the FP64 residual pass on the same z: its inflows come from z's boundary moments) plus
 // that of c = T(0) (volume, surface and energy-inflow sources).  𝚽l = D ψ on all rows, and the
@@ -843,24 +853,37 @@ void Problem::reconstruct_sweep_free(int g, const double* zin_reduced, const dou
     PassDesc pd;
     desc_reconstruct(pd, g, zin_reduced);
     const size_t n = (size_t)Nd * Nc;
-    if (have_az) k_axpy(1.0, psi_c, (double*)d_psi, n, st);
-    else k_copy((double*)d_psi, psi_c, n, st);
-    pd.psi = d_psi;
+    // Without an energy outflow to form, the D contraction sums the two angular fluxes in its
+    // operand load (one pass over psi_c instead of an axpy over both); with one, the summed
+    // angular flux is formed first, as the outflow needs it.
+    const bool fold = have_az && !pd.psiE_out && !fold_off();
+    if (fold) {
+        pd.psi = d_psi;
+        pd.psi_add = psi_c;
+    } else {
+        if (have_az) k_axpy(1.0, psi_c, (double*)d_psi, n, st);
+        else k_copy((double*)d_psi, psi_c, n, st);
+        pd.psi = d_psi;
+    }
     k_set_pd(pd, d_pd, st);
     if (pd.psiE_out) {
         if (bfp) launch_bfp_out(d_pd, (long)nE(), st);
         else launch_csd_out(d_pd, (long)n, st);
     }
     counters.passes += 1;  // a pass without sweep
+    auto dg = [&] {
+        if (fold) launch_dgemm_add(d_pd, d_dgf, n_dgf, st);
+        else launch_dgemm(P64, d_pd, d_dgf, n_dgf, st);
+    };
     if (profiling) {
         if (ev_next + 2 > 4096) collect_kernel_times();
         hipEvent_t a = next_event(), b = next_event();
         set_launch_events(a, b);
-        launch_dgemm(P64, d_pd, d_dgf, n_dgf, st);
+        dg();
         set_launch_events(nullptr, nullptr);
         ev_used.push_back({a, b, 2, P64});
     } else {
-        launch_dgemm(P64, d_pd, d_dgf, n_dgf, st);
+        dg();
     }
 }
 

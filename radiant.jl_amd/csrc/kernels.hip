@@ -53,7 +53,7 @@ void take_launch_events(hipEvent_t* start, hipEvent_t* stop) {
 template <typename... KArgs, typename... Args>
 static void launch_timed(void (*kernel)(KArgs...), dim3 grid, dim3 block, uint32_t shmem, hipStream_t st,
                          Args... args) {
-    if (tl_ev_start) {
+    if (tl_ev_start || tl_ev_stop) {  // either may be null (a timed pair of launches)
         hipEvent_t a = tl_ev_start, b = tl_ev_stop;
         tl_ev_start = tl_ev_stop = nullptr;
         hipExtLaunchKernelGGL(kernel, grid, block, shmem, st, a, b, 0u, static_cast<KArgs>(args)...);
@@ -146,7 +146,7 @@ int tune_choose(const TuneKey& key, int ncand, TuneTicket& t) {
     if (mode == 2) { const int c = s.next; s.next = (s.next + 1) % ncand; return t.cand = c; }
     if (!s.pending.empty()) tune_harvest(s);
     if (s.best >= 0) return t.cand = s.best;
-    if (tl_ev_start || !tl_tune_measure) return t.cand = 0;  // timed by the caller, or not timed alone
+    if (tl_ev_start || tl_ev_stop || !tl_tune_measure) return t.cand = 0;  // timed by the caller, or not timed alone
     bool done = true;
     for (auto& v : s.ms) done = done && v.size() >= (size_t)TUNE_REPS;
     if (done) {
@@ -331,7 +331,7 @@ __device__ inline void epilogue_dgemm(const PassDesc& d, const V (&acc)[RT][CT],
     }
 }
 
-template <typename T, int RT, int CT, bool MG, int NS>
+template <typename T, int RT, int CT, bool MG, int NS, bool ADD = false>
 __device__ __forceinline__ void contract_body(const PassDesc& d, const GemmTask tk, int* rows_s) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int Nc = d.Nc;
@@ -339,6 +339,7 @@ __device__ __forceinline__ void contract_body(const PassDesc& d, const GemmTask 
     const int M = MG ? d.Nd : d.Nrows;
     const T* __restrict__ At = (const T*)(MG ? d.Mact : d.Drows);
     const T* __restrict__ psi_in = (const T*)d.psi;
+    const T* __restrict__ psi_ad = ADD ? (const T*)d.psi_add : nullptr;
     const double* __restrict__ qlout = d.qlout;
     const double* __restrict__ phi_in = d.phi_in;
     const double* __restrict__ sigw = d.sigw;
@@ -389,7 +390,7 @@ __device__ __forceinline__ void contract_body(const PassDesc& d, const GemmTask 
     // clamped in-range data and is zeroed by a 0/1 factor when the MFMA operands are formed (a
     // select would let the compiler move a load into a branch; forming the operands at load
     // time would wait for the loads right after issuing them).
-    struct Stage { T a[RT]; double q[CT], w[CT], p[CT]; T b[CT]; };
+    struct Stage { T a[RT]; double q[CT], w[CT], p[CT]; T b[CT], b2[ADD ? CT : 1]; };
     auto load = [&](int s, Stage& g) {
         const int k = 4 * s + kk;
         const int kc = k < K ? k : K - 1;
@@ -403,6 +404,7 @@ __device__ __forceinline__ void contract_body(const PassDesc& d, const GemmTask 
                 g.p[u] = gld(pb + (size_t)rows_s[kc] * pks + pcol[u]);
             } else {
                 g.b[u] = gld(psi_in + (size_t)kc * Nc + colc[u]);
+                if constexpr (ADD) g.b2[u] = gld(psi_ad + (size_t)kc * Nc + colc[u]);
             }
         }
     };
@@ -414,6 +416,7 @@ __device__ __forceinline__ void contract_body(const PassDesc& d, const GemmTask 
 #pragma unroll
         for (int u = 0; u < CT; ++u) {
             if (MG) b[u] = (T)((g.q[u] + g.w[u] * g.p[u]) * ((kok && colok[u]) ? 1.0 : 0.0));
+            else if constexpr (ADD) b[u] = (g.b[u] + g.b2[u]) * ((kok && colok[u]) ? (T)1 : (T)0);
             else b[u] = g.b[u] * ((kok && colok[u]) ? (T)1 : (T)0);
         }
 #pragma unroll
@@ -456,11 +459,11 @@ __device__ __forceinline__ void contract_body(const PassDesc& d, const GemmTask 
     }
 }
 
-template <typename T, int RT, int CT, bool MG, int NS>
+template <typename T, int RT, int CT, bool MG, int NS, bool ADD = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((sizeof(T) == 4 && !MG) ? 3 : 2))) void contract_kernel(const PassDesc* __restrict__ pds, const GemmTask* __restrict__ tasks) {
     __shared__ int rows_s[MG ? CONTRACT_KMAX : 1];
     const GemmTask tk = tasks[blockIdx.x];
-    contract_body<T, RT, CT, MG, NS>(pds[tk.prob], tk, rows_s);
+    contract_body<T, RT, CT, MG, NS, ADD>(pds[tk.prob], tk, rows_s);
 }
 
 // FP64 and FP32 passes of different problems in one launch (PassDesc::prec per workgroup).
@@ -2534,6 +2537,17 @@ void launch_mgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int nta
         else contract_launch<true, 3>(prec, d_pd, tasks, ntasks, st);
         tune_commit(t);
     }
+    RT_HIP(hipGetLastError());
+}
+
+void launch_dgemm_add(const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st) {
+    if (ntasks <= 0) return;
+    TuneTicket t;
+    const int v = tune_choose({P64, 4, 0, 0, 0, ilog2(ntasks)}, 2, t);
+    if (t.a) set_launch_events(t.a, t.b);
+    if (v == 0) launch_timed(contract_kernel<double, GEMM_RT, GEMM_CT, false, 4, true>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    else launch_timed(contract_kernel<double, GEMM_RT, GEMM_CT, false, 2, true>, dim3(ntasks), dim3(256), 0, st, d_pd, tasks);
+    tune_commit(t);
     RT_HIP(hipGetLastError());
 }
 

@@ -90,6 +90,9 @@ struct PassDesc {
     int fm_e;
     const double* csdS;
     const int* emap;
+    // D contraction of launch_dgemm_add: the angular flux is psi + psi_add (FP64), so a
+    // reconstruction T(z) = A z + T(0) sums the two passes' angular fluxes in its operand load
+    const double* psi_add;
     // Face moments of the 3D and slowing-down walkers in device memory instead of LDS (grids whose
     // faces exceed the 160 KiB of a workgroup): [Nd][faces] in the pass's compute type, launched
     // with no dynamic LDS.  Null: LDS.
@@ -124,6 +127,8 @@ int sweep_pencils_per_thread(int P);
 size_t sweep_lds_bytes(int prec, int Nx, int Ny, int Nz, int Nmat, int nbuf);
 
 void launch_mgemm(int prec, const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st);
+// FP64 D contraction of (psi + psi_add), one problem (PassDesc::psi_add non-null)
+void launch_dgemm_add(const PassDesc* d_pd, const GemmTask* tasks, int ntasks, hipStream_t st);
 void launch_sweep(int prec, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int ppt, int nbuf,
                   int block, size_t lds, hipStream_t st);
 // 1D/2D wave-per-direction sweep (rows per lane 1, 2 or 4; -1 when the grid does not fit).
