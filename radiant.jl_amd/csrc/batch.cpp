@@ -46,12 +46,7 @@ void BatchSolver::init_device(int n) {
     ps.resize(n);
     for (auto& s : ps) s.phase = DONE;
     auto al = [&](size_t bytes) {
-        void* p = nullptr;
-        hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 16));
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            throw Error(RT_ENOMEM, "batched solver: device allocation failed");
-        }
+        void* p = dev_alloc(std::max<size_t>(bytes, 16));
         mem.push_back(p);
         return p;
     };
@@ -94,20 +89,12 @@ void BatchSolver::ensure_task_capacity() {
     if (st64) RT_CHECK(hipStreamSynchronize(st64));
     for (void*& p : task_mem)
         if (p) {
-            (void)hipFree(p);
+            dev_free(p);
             p = nullptr;
         }
     cap_gemm = std::max(cap_gemm, ng + ng / 2);
     cap_sweep = std::max(cap_sweep, nsw + nsw / 2);
-    auto al = [&](size_t bytes) {
-        void* p = nullptr;
-        hipError_t e = hipMalloc(&p, std::max<size_t>(bytes, 16));
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            throw Error(RT_ENOMEM, "batched solver: device allocation failed");
-        }
-        return p;
-    };
+    auto al = [&](size_t bytes) { return dev_alloc(std::max<size_t>(bytes, 16)); };
     for (int pr = 0; pr < 3; ++pr) {
         task_mem[3 * pr] = d_mg[pr] = (GemmTask*)al(sizeof(GemmTask) * cap_gemm);
         task_mem[3 * pr + 1] = d_dg[pr] = (GemmTask*)al(sizeof(GemmTask) * cap_gemm);
@@ -329,17 +316,18 @@ void BatchSolver::collect_times() {
 
 BatchSolver::~BatchSolver() {
     (void)hipStreamSynchronize(st);
+    if (st64) (void)hipStreamSynchronize(st64);  // no pending work uses the blocks given back below
     for (auto& c : gcache) (void)hipGraphExecDestroy(c.exec);
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     for (void* p : task_mem)
-        if (p) (void)hipFree(p);
+        if (p) dev_free(p);
     if (st64) {
         (void)hipStreamSynchronize(st64);
         (void)hipStreamDestroy(st64);
     }
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
-    for (void* p : mem) (void)hipFree(p);
+    for (void* p : mem) dev_free(p);
     if (h_slots) (void)hipHostFree(h_slots);
     if (h_arena) (void)hipHostFree(h_arena);
 }

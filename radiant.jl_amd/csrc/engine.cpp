@@ -31,13 +31,8 @@ double inner_tol_for(int prec) {
 
 template <class T>
 T* Problem::dalloc(size_t n) {
-    void* p = nullptr;
     if (n == 0) n = 1;
-    hipError_t e = hipMalloc(&p, n * sizeof(T));
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        throw Error(RT_ENOMEM, "hipMalloc of " + std::to_string(n * sizeof(T)) + " bytes failed: " + hipGetErrorString(e));
-    }
+    void* p = dev_alloc(n * sizeof(T));
     allocs.push_back(p);
     return (T*)p;
 }
@@ -541,7 +536,8 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
         d_y = dalloc<double>(64);
         RT_CHECK(hipStreamSynchronize(st));
     } catch (...) {
-        for (void* p : allocs) (void)hipFree(p);
+        (void)hipStreamSynchronize(st);
+        for (void* p : allocs) dev_free(p);
         allocs.clear();
         if (h_scal) (void)hipHostFree(h_scal);
         if (own_stream) (void)hipStreamDestroy(st);
@@ -552,8 +548,8 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
 Problem::~Problem() {
     (void)hipStreamSynchronize(st);
     for (hipEvent_t ev : ev_pool) (void)hipEventDestroy(ev);
-    for (auto& kv : pool) (void)hipFree(kv.second);
-    for (void* p : allocs) (void)hipFree(p);
+    for (auto& kv : pool) dev_free(kv.second);
+    for (void* p : allocs) dev_free(p);
     if (h_scal) (void)hipHostFree(h_scal);
     if (own_stream) (void)hipStreamDestroy(st);
 }
@@ -597,12 +593,7 @@ double* Problem::vec(const std::string& name, size_t n) {
     const std::string key = name + "#" + std::to_string(n);
     auto it = pool.find(key);
     if (it != pool.end()) return it->second;
-    void* p = nullptr;
-    hipError_t e = hipMalloc(&p, n * sizeof(double));
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        throw Error(RT_ENOMEM, "hipMalloc of a Krylov vector (" + std::to_string(n * 8) + " bytes) failed");
-    }
+    void* p = dev_alloc(n * sizeof(double));
     pool[key] = (double*)p;
     return (double*)p;
 }

@@ -25,6 +25,13 @@ struct Error : std::runtime_error {
                                               __FILE__ + ":" + std::to_string(__LINE__));                 \
     } while (0)
 
+// Device allocations through the cache of pool.cpp: dev_free takes a block back once no pending
+// device work uses it (it is kept mapped for the next allocation of a fitting size).
+void* dev_alloc(size_t bytes);
+void dev_free(void* p);
+size_t dev_cached_bytes();
+void dev_trim();
+
 // Message returned by rt_last_error() on the calling thread.
 void set_last_error(const std::string& m);
 
