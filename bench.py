@@ -37,7 +37,7 @@ HBM_PEAK_GBS = 8000.0
 # dispatches), each scaled by what the counter reads for a plain 1 GiB device copy.  Keyed by
 # (workload, kernel class); reported only when the library sources match the measured digest.
 MEASURED_TRAFFIC = {("C2 mixed-fp32", "sweep_fp32"): dict(
-    fetch=200.883e6, write=184.729e6, fetch_scale=2.0, write_scale=1.0, digest="5b2ff9f4569567a5",
+    fetch=200.849e6, write=184.785e6, fetch_scale=2.0, write_scale=1.0, digest="baf6ed87c5bd709b",
     name="sweep_scan_kernel<float, 2, 1, 4, 1024, true, false, 2>", summary="profiles/sweep_scan_c2_mixed.md")}
 MFMA_PEAK_TFLOPS = {0: 78.6, 1: 157.3, 2: 2500.0}
 KNAMES = ["mgemm", "sweep", "dgemm", "bndmom"]
@@ -398,7 +398,7 @@ def main():
     ap.add_argument("--c3", type=int, default=256,
                     help="problems of the uneven 2D batch (N~U{32..160} per axis, G~U{8..32}, P1..P5, GLC 8/10) "
                          "solved as one lockstep batch beside the headline (0: skip)")
-    ap.add_argument("--c4", type=int, default=200, help="requests of the 3D served stream (0: skip)")
+    ap.add_argument("--c4", type=int, default=500, help="requests of the 3D served stream (0: skip)")
     ap.add_argument("--rate", type=float, default=50.0, help="served stream: Poisson arrivals per second")
     ap.add_argument("--budget-gib", type=float, default=200.0, help="served stream: server HBM budget")
     ap.add_argument("--max-active", type=int, default=8, help="served stream: resident requests at most")
