@@ -395,6 +395,9 @@ def main():
     ap.add_argument("--skip-bf16", action="store_true")
     ap.add_argument("--headline-only", action="store_true",
                     help="run only the headline mode (C2 mixed FP32), for kernel profiling of exactly that mode")
+    ap.add_argument("--only", choices=["c3", "c4"], default=None,
+                    help="run only the C3 mixed lockstep batch (one solve after a warmup) or only the C4 served "
+                         "stream, for profiling that workload alone; prints its result object")
     ap.add_argument("--c3", type=int, default=256,
                     help="problems of the uneven 2D batch (N~U{32..160} per axis, G~U{8..32}, P1..P5, GLC 8/10) "
                          "solved as one lockstep batch beside the headline (0: skip)")
@@ -426,6 +429,15 @@ def main():
     from radiant_hip import problems, solver
     from radiant_hip.angular import angular_set
     log(solver.version())
+    if args.only == "c3":
+        probs = problems.config3(args.c3 or 256, quad_orders=(8, 10))
+        r3, _ = run_batch(probs, "mixed-fp32", 1, 1, f"c3x{len(probs)} mixed-fp32", kernel_timing=False, keep=())
+        print(json.dumps(summary(r3), default=lambda o: o.tolist() if hasattr(o, "tolist") else str(o)))
+        return
+    if args.only == "c4":
+        c4 = run_serve(args.c4, args.rate, args.budget_gib, args.max_active, "mixed-fp32")
+        print(json.dumps(c4, default=lambda o: o.tolist() if hasattr(o, "tolist") else str(o)))
+        return
     steps, warm = args.steps, args.warmup
     prob = problems.config2()
     ang = angular_set(prob.quad_type, prob.quad_order, 3, 3, prob.legendre_order, prob.sn_type)

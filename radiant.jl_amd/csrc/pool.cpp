@@ -11,9 +11,9 @@
 //
 // Sizes are rounded to classes (powers of two below 1 MiB; above, steps of 1/8 of the power of two
 // below, at least 2 MiB), and an allocation takes the smallest cached block of at least its class
-// and at most twice it.  Cached bytes above a cap (RADIANT_HIP_POOL_GIB, default 48 GiB per device)
-// are freed, largest first, and when hipMalloc fails every cached block is freed before the one
-// retry.  RADIANT_HIP_POOL=0 turns the cache off (plain hipMalloc / hipFree).
+// and at most twice it.  Cached bytes above a cap (RADIANT_HIP_POOL_GIB, default 64 GiB per device)
+// are freed, least recently cached first, and when hipMalloc fails every cached block is freed
+// before the one retry.  RADIANT_HIP_POOL=0 turns the cache off (plain hipMalloc / hipFree).
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
@@ -30,9 +30,10 @@ namespace {
 
 struct DevCache {
     std::mutex mu;
-    std::multimap<size_t, void*> free_;          // cached blocks by size
-    std::unordered_map<void*, size_t> live;      // allocated blocks and their sizes
+    std::multimap<size_t, std::pair<uint64_t, void*>> free_;  // cached blocks by size: (release tick, block)
+    std::unordered_map<void*, size_t> live;                   // allocated blocks and their sizes
     size_t cached = 0;
+    uint64_t tick = 0;
 };
 
 DevCache& cache_of(int dev) {
@@ -55,7 +56,7 @@ bool pool_on() {
 size_t pool_cap() {
     static const size_t cap = [] {
         const char* e = std::getenv("RADIANT_HIP_POOL_GIB");
-        const double g = e ? std::atof(e) : 48.0;
+        const double g = e ? std::atof(e) : 64.0;
         return (size_t)((g > 0 ? g : 0.0) * 1073741824.0);
     }();
     return cap;
@@ -81,13 +82,16 @@ int current_device() {
     return d;
 }
 
-// frees cached blocks, largest first, until at most `keep` bytes stay cached (lock held)
+// frees cached blocks, least recently released first, until at most `keep` bytes stay cached
+// (lock held)
 void shrink(DevCache& c, size_t keep) {
     while (c.cached > keep && !c.free_.empty()) {
-        auto it = std::prev(c.free_.end());
-        c.cached -= it->first;
-        (void)hipFree(it->second);
-        c.free_.erase(it);
+        auto old = c.free_.begin();
+        for (auto it = c.free_.begin(); it != c.free_.end(); ++it)
+            if (it->second.first < old->second.first) old = it;
+        c.cached -= old->first;
+        (void)hipFree(old->second.second);
+        c.free_.erase(old);
     }
 }
 
@@ -108,7 +112,7 @@ void* dev_alloc(size_t bytes) {
     std::lock_guard<std::mutex> lk(c.mu);
     auto it = c.free_.lower_bound(cls);
     if (it != c.free_.end() && it->first <= 2 * cls) {
-        void* p = it->second;
+        void* p = it->second.second;
         c.live[p] = it->first;
         c.cached -= it->first;
         c.free_.erase(it);
@@ -144,7 +148,7 @@ void dev_free(void* p) {
     }
     const size_t sz = it->second;
     c.live.erase(it);
-    c.free_.emplace(sz, p);
+    c.free_.emplace(sz, std::make_pair(++c.tick, p));
     c.cached += sz;
     if (c.cached > pool_cap()) shrink(c, pool_cap());
 }

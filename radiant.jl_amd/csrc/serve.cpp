@@ -713,6 +713,10 @@ int rt_server_create(const rt_server_config* cfg, rt_server** out) {
         rt::set_last_error("invalid server configuration");
         return RT_EINVAL;
     }
+    // a serving session starts from an empty device-memory cache: blocks cached for earlier work of
+    // other sizes would only fill the cache's cap and be evicted (each eviction frees, which
+    // synchronises the device) while the requests allocate their own
+    rt::dev_trim();
     auto* s = new rt_server();
     s->cfg = *cfg;
     s->reaper = std::thread([s] { s->reap_results(); });
