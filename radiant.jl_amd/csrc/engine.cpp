@@ -63,6 +63,20 @@ static int ho_order(const rt_problem_desc& d) {
     return d.partial_coupling ? Kx + Ky + Kz - 2 : Kx * Ky * Kz;
 }
 
+// The 3D order-1 scan sweep with two pencils per wave can split each direction's pencils by z
+// over several workgroups (kernels.hip, sweep_scan_body): offered when the handed rows fit its
+// courier (Nx <= 64) and the grid has layers for two slabs at least.
+static bool split_scan_offered(const rt_problem_desc& d) {
+    if (d.ndims != 3 || ho_order(d) || d.Nx > 64 || d.Nz < 4) return false;
+    if (!(sweep_scan_cpl3d(d.Nx) & SCAN_PAIR)) return false;
+    const char* e = std::getenv("RADIANT_HIP_SCAN_SPLIT");
+    return !(e && e[0] == '1' && e[1] == 0);
+}
+// words of the handed rows: [Nd][SCAN_SPLIT_MAX - 1][Ny][Nx][2]
+static size_t split_scan_words(const rt_problem_desc& d) {
+    return (size_t)d.Nd * (SCAN_SPLIT_MAX - 1) * d.Ny * d.Nx * 2;
+}
+
 static bool has_fp(const rt_problem_desc& d) {
     const int t = solver_type_of(d);
     return (t == 2 || t == 4) && d.T && d.Mfp;
@@ -501,6 +515,12 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
         // resource, whose byte count is 32-bit
         if (ndims <= 2 && (uint64_t)Nc * 8u > 0xffffffffull) scan_cpl = 0;
         scan_waves = sweep_scan_waves(ndims, Ny, Nz, Nx);
+        if (split_scan_offered(d)) {
+            d_sclaim = dalloc<unsigned long long>(Nd);
+            d_shand = dalloc<unsigned long long>(split_scan_words(d));
+            RT_CHECK(hipMemsetAsync(d_sclaim, 0, (size_t)Nd * 8, st));
+            RT_CHECK(hipMemsetAsync(d_shand, 0, split_scan_words(d) * 8, st));
+        }
         // RADIANT_HIP_SWEEP selects another sweep kernel for testing: s = scan (default), r = wave
         // rows / planes, g = pencil kernel
         sweep_kind = 0;
@@ -586,6 +606,7 @@ int64_t Problem::device_bytes(const rt_problem_desc& d) {
     if (d.csd) b += 2LL * d.Nd * Nc * 8;                              // energy-face angular fluxes (at most Nc per direction)
     if (has_fp(d)) b += (int64_t)Nact_est * (Nc + Nact_est) * 8;     // Fokker-Planck source, ℳ
     b += (int64_t)std::min<int64_t>((int64_t)d.G * d.Np * Nc, std::max<int64_t>(FLUX_STAGE, (int64_t)d.G * d.Np)) * 8;
+    if (split_scan_offered(d)) b += ((int64_t)d.Nd + (int64_t)split_scan_words(d)) * 8;
     return b;
 }
 
@@ -625,6 +646,7 @@ void Problem::fill_geometry(PassDesc& pd, int g) {
     pd.fm_x = fm[0]; pd.fm_y = fm[1];
     pd.ho_Kz = ho_Kz; pd.ho_omz = d_omz; pd.fm_z = fm[2];
     pd.hoface = d_hoface;
+    pd.sclaim = d_sclaim; pd.shand = d_shand;
     if (bfp) {  // the general slowing-down cell needs S⁻, S⁺ in every pass (homogeneous ones too)
         pd.ho_KE = ho_KE; pd.ho_ome = d_ome; pd.fm_e = fm_e; pd.emap = d_emap;
         pd.csdS = d_csdS + (size_t)g * Nmat * 2;
@@ -695,8 +717,11 @@ void Problem::launch_pass(PassDesc& pd, int prec, bool full_rows) {
         else if (ho_K)
             launch_sweep_1d_ho(sp, d_pd, Nd, ho_K, st);
         else if (sweep_kind == 0 && scan_cpl > 0 && sweep_scan_lds_bytes(sp, ndims, Nx, Ny, Nz, Nmat) <= 160 * 1024)
+        {
+            const ScanSplitGeom sg{Nx, Ny, Nz};
             launch_sweep_scan(sp, d_pd, d_sw, Nd, scan_cpl, scan_waves, sweep_scan_lds_bytes(sp, ndims, Nx, Ny, Nz, Nmat),
-                              st, pd.bin_x != nullptr);
+                              st, pd.bin_x != nullptr, d_sclaim ? &sg : nullptr);
+        }
         else if (rows_per_lane > 0 && sweep_rows_lds_bytes(sp, Nx, Nmat) <= 160 * 1024)
             launch_sweep_rows(sp, d_pd, d_sw, Nd, rows_per_lane, sweep_rows_lds_bytes(sp, Nx, Nmat), st);
         else if (planes_ok[sp] && sweep_planes_lds_bytes(sp, Nx, Ny, Nz, Nmat, pl_R[sp], pl_waves[sp]) <= 160 * 1024)

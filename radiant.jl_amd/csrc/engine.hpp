@@ -105,6 +105,9 @@ public:
     // the 3D and slowing-down walkers keep their face moments in device memory when they exceed
     // the LDS (d_hoface, [Nd][faces] doubles); dynamic LDS of their launch in precision sp
     double* d_hoface = nullptr;
+    // z-split 3D scan sweeps (PassDesc::sclaim / shand; null: the split shapes are not offered)
+    unsigned long long* d_sclaim = nullptr;
+    unsigned long long* d_shand = nullptr;
     size_t ho_lds(int sp) const;
     size_t nE() const { return (size_t)Nd * Ncell * fm_e; }  // energy-face angular flux values
     int accel, restart, depth, maxit, precision, state_mode, verbose;

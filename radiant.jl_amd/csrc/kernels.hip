@@ -65,7 +65,8 @@ static void launch_timed(void (*kernel)(KArgs...), dim3 grid, dim3 block, uint32
 // ------------------------------------------------------------------------------------------
 // Launch-shape autotuner.  A sweep or contraction launch has variants that compute bit-identical
 // results and differ only in their schedule: the rows in flight of the line kernel (1D/2D), for
-// the 3D scan kernel the pencils per item with their prefetch depth and the waves per workgroup,
+// the 3D scan kernel the pencils per item with their prefetch depth, the waves per workgroup and
+// the split of each direction over 1, 2 or 4 workgroups,
 // and the pipeline depth (k steps in flight) of the M and D contractions.  In
 // mode 1 the first launches of each key (launch class, kernel, cells per lane, default waves,
 // inflow flag, log2 of the task count) cycle through the candidates with dispatch start/stop
@@ -220,8 +221,8 @@ std::string autotune_report() {
         const TuneState& st = kv.second;
         char line[256];
         static const char* knames[] = {"line", "scan3d", "mgemm", "dgemm"};
-        std::snprintf(line, sizeof line, "%s %s cpl=%d waves=%d inflow=%d tasks~2^%d: ", kinds[k.kind & 3],
-                      knames[k.kernel & 3], k.cpl, k.waves, k.bin, k.lg);
+        std::snprintf(line, sizeof line, "%s %s cpl=%d waves=%d inflow=%d splits=%d tasks~2^%d: ", kinds[k.kind & 3],
+                      knames[k.kernel & 3], k.cpl, k.waves, k.bin & 1, k.bin >> 1, k.lg);
         r += line;
         if (st.best < 0) r += "tuning";
         else {
@@ -1508,20 +1509,85 @@ __global__ __launch_bounds__(64) void sweep_line_mixed_kernel(const PassDesc* __
 // diagonal.  After the last diagonal FY and FZ hold the exit y- and z-faces, which are written
 // out at the end with coalesced stores.  PPW pencils per wave: with PPW = 2 each half of the wave
 // (32 lanes, CPL cells per lane) solves its own pencil of the item, and the scans run per half.
-template <typename T, int CPL, int U, int PF, bool DPP, bool BIN = false, int PPW = 1>
-__device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, unsigned char* smem) {
+//
+// SPL: the direction's pencils are split by z into S slabs of Zb = ceil(Nz / S) layers, one
+// workgroup each (scan_split_kernel).  Slab k's top layer hands its outgoing z-face rows to slab
+// k + 1 through device memory: every value goes out as 64-bit words that carry the launch's tag in
+// their high half (one word per float, two per double), so a word is valid exactly when its tag is
+// the current one and no fence or flag is needed.  The slab above copies the rows into its FZ in
+// LDS with one extra wave (the courier, the workgroup's last), a few rows ahead of the diagonal
+// that reads them, so the solving waves' loads and waits stay as in one workgroup per direction.
+// The workgroups of a direction take their slabs in the order they start (a counter per
+// direction, tagged the same way), so a slab only ever waits for a workgroup that is already
+// running, and the first never waits: the launch cannot deadlock whatever the dispatch order.
+struct ScanSplit {
+    int S;         // slabs per direction (1: no split)
+    unsigned tag;  // launch tag, never 0
+    int lag;       // rows the slab below is ahead when a slab starts
+};
+constexpr int SCAN_SPLIT_ROWS = 6;  // rows the courier keeps in flight
+constexpr int SCAN_SPLIT_LAG = 9;   // rows the slab below is ahead when a slab starts (default)
+constexpr int SCAN_SPLIT_SPIN = 1 << 20;  // polls of handoff rows per workgroup before values are given up (NaN)
+
+// The handed words are read and written with device-scope read-modify-write atomics, which the
+// workgroups of every XCD see at once (plain loads and stores, even agent-scope ones, can be served
+// by an XCD's own L2 for a long time).  Global (not flat) accesses: a flat access also counts on the
+// LDS counter, so a wait for the solving waves' LDS reads would wait for the handoff too.
+__device__ __forceinline__ unsigned long long split_ld(const unsigned long long* p) {
+    return __hip_atomic_fetch_or((RT_GLOBAL unsigned long long*)p, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void split_st(unsigned long long* p, unsigned long long v) {
+    (void)__hip_atomic_exchange((RT_GLOBAL unsigned long long*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T, int CPL, int U, int PF, bool DPP, bool BIN = false, int PPW = 1, bool SPL = false>
+__device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, unsigned char* smem,
+                                                const ScanSplit sp = {1, 0u, 0}) {
     const int Nx = d.Nx, Ny = d.Ny, Nz = d.Nz, Nc = d.Nc, Nmat = d.Nmat;
     // the wave index is uniform: in a scalar register, the item cursors and barrier counts are too
-    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6), W = blockDim.x >> 6;
+    // (W counts the waves that solve pencils; with SPL the courier wave comes after them)
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int W = (blockDim.x >> 6) - (SPL ? 1 : 0);
     static_assert(PPW == 1 || PPW == 2, "one or two pencils per wave");
     const int half = PPW == 2 ? lane >> 5 : 0;    // the pencil of the item this lane works on
     const int ll = PPW == 2 ? lane & 31 : lane;    // lane within the pencil
     constexpr int UP = U * PPW;                    // pencils per item
+    constexpr int WPV = sizeof(T) / 4;             // handoff words per value
+    // slab of this workgroup: layers z0 .. z0 + Zc - 1 (sweep order) of Zb per slab
+    int role = 0, z0 = 0, Zc = Nz, Zb = Nz;
+    unsigned long long* hout = nullptr;        // rows handed to the slab above (null: top slab)
+    const unsigned long long* hin = nullptr;   // rows handed from the slab below (null: first slab)
+    if constexpr (SPL) {
+        __shared__ int s_role;
+        if (tid == 0) {
+            unsigned long long* cw = d.sclaim + n;
+            unsigned long long old = split_ld(cw);
+            int r = 0;
+            for (;;) {  // at most S rounds: every failed exchange is another workgroup's success
+                const bool cur = (old >> 8) == (unsigned long long)sp.tag;
+                r = cur ? (int)(old & 255) : 0;
+                const unsigned long long want = cur ? old + 1 : (((unsigned long long)sp.tag << 8) | 1ull);
+                if (__hip_atomic_compare_exchange_strong(cw, &old, want, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT))
+                    break;
+            }
+            s_role = r;
+        }
+        __syncthreads();
+        role = __builtin_amdgcn_readfirstlane(s_role);
+        Zb = (Nz + sp.S - 1) / sp.S;
+        z0 = role * Zb;
+        Zc = Nz - z0 < Zb ? Nz - z0 : Zb;
+        const size_t hrow = (size_t)Ny * Nx * 2;  // words per handed face (sized for doubles)
+        unsigned long long* hb = d.shand + (size_t)n * (SCAN_SPLIT_MAX - 1) * hrow;
+        if (role + 1 < sp.S) hout = hb + (size_t)role * hrow;
+        if (role > 0) hin = hb + (size_t)(role - 1) * hrow;
+    }
     T* hy2 = (T*)smem;
     T* hz2 = hy2 + Ny;
     T* st = hz2 + Nz;
-    T* FY = st + Nmat;                   // [Nz][Nx]
-    T* FZ = FY + (size_t)Nz * Nx;        // [Ny][Nx]
+    T* FY = st + Nmat;                   // [Zb][Nx]: y-faces of the slab's layers
+    T* FZ = FY + (size_t)Zb * Nx;        // [Ny][Nx]
     T* FT = FZ + (size_t)Ny * Nx;        // [64*CPL] trash for inactive lanes
     const double mu = d.mu[n], eta = d.eta[n], xi = d.xi[n];
     for (int i = tid; i < Ny; i += blockDim.x) hy2[i] = (T)(2.0 * (fabs(eta) / d.dy[i]));
@@ -1529,16 +1595,18 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
     for (int i = tid; i < Nmat; i += blockDim.x) st[i] = (T)d.sigt[i];
     const bool fxs = mu >= 0, fys = eta >= 0, fzs = xi >= 0;
     const size_t npx = (size_t)Ny * Nz, npy = (size_t)Nx * Nz, npz = (size_t)Nx * Ny;  // face positions
-    for (int i = tid; i < (Nz + Ny) * Nx; i += blockDim.x) {
-        // inflow on the entry y- and z-faces: zero for void boundaries
+    // inflow on the entry y- and z-faces: zero for void boundaries; the z-faces of a slab above
+    // the first come from the slab below (written by the courier)
+    const int nfz = (Zb + (role == 0 ? Ny : 0)) * Nx;
+    for (int i = tid; i < nfz; i += blockDim.x) {
         T v = (T)0;
         if constexpr (BIN) {
-            if (i < Nz * Nx) {
-                const int jz = i / Nx, jx = i - jz * Nx;
+            if (i < Zb * Nx) {
+                const int jz = z0 + i / Nx, jx = i - (i / Nx) * Nx;
                 const int ix = fxs ? jx : Nx - 1 - jx, iz = fzs ? jz : Nz - 1 - jz;
-                v = (T)gld(d.bin_y + n * npy + ix + (size_t)Nx * iz);
+                if (jz < Nz) v = (T)gld(d.bin_y + n * npy + ix + (size_t)Nx * iz);
             } else {
-                const int k = i - Nz * Nx, jy = k / Nx, jx = k - jy * Nx;
+                const int k = i - Zb * Nx, jy = k / Nx, jx = k - jy * Nx;
                 const int ix = fxs ? jx : Nx - 1 - jx, iy = fys ? jy : Ny - 1 - jy;
                 v = (T)gld(d.bin_z + n * npz + ix + (size_t)Nx * iy);
             }
@@ -1556,19 +1624,86 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
         hx[c] = inx[c] ? (T)(2.0 * (fabs(mu) / d.dx[ixc[c]])) : (T)0;
     }
     const bool last_lane = ll == (Nx - 1) / CPL;
+    if constexpr (SPL) {
+        if (wave == W) {
+            // The courier: row jy of the slab below's top z-faces goes into FZ[jy] before barrier
+            // jy (the one that opens diagonal jy, whose pencil (jy, z0) reads it first), with the
+            // rows up to SCAN_SPLIT_ROWS ahead already requested.  It passes the same 1 + ndiag
+            // barriers as the solving waves.
+            constexpr int CR = SCAN_SPLIT_ROWS, KW = 2;  // KW words per lane hold a row (Nx * WPV <= 128)
+            const int ndg = Ny + Zc - 1, nw = Nx * WPV;
+            const unsigned long long want = (unsigned long long)sp.tag << 32;
+            auto valid = [&](unsigned long long w) { return (w >> 32) == (unsigned long long)sp.tag; };
+            auto issue = [&](unsigned long long (&w)[KW], int jy) {
+#pragma unroll
+                for (int k = 0; k < KW; ++k) {
+                    const int i = lane + 64 * k;
+                    w[k] = i < nw ? split_ld(hin + (size_t)jy * nw + i) : want;
+                }
+            };
+            // polls the words of row jy that are not valid yet, a bounded number of times in all:
+            // once the bound is spent the remaining rows are not waited for
+            int spins = 0;
+            auto settle = [&](unsigned long long (&w)[KW], int jy) {
+                bool ok = true;
+#pragma unroll
+                for (int k = 0; k < KW; ++k) ok = ok && valid(w[k]);
+                for (; spins < SCAN_SPLIT_SPIN && !__all(ok); ++spins) {
+                    __builtin_amdgcn_s_sleep(8);
+                    ok = true;
+#pragma unroll
+                    for (int k = 0; k < KW; ++k) {
+                        const int i = lane + 64 * k;
+                        if (!valid(w[k])) w[k] = split_ld(hin + (size_t)jy * nw + i);
+                        ok = ok && valid(w[k]);
+                    }
+                }
+            };
+            unsigned long long rw[CR][KW];
+            if (role > 0) {
+                const int jl = sp.lag < Ny - 1 ? sp.lag : Ny - 1;
+                issue(rw[0], jl);
+                settle(rw[0], jl);  // start once the slab below is sp.lag rows ahead
+#pragma unroll
+                for (int r = 0; r < CR; ++r)
+                    if (r < Ny) issue(rw[r], r);
+            }
+            for (int t0 = 0; t0 <= ndg; t0 += CR) {
+#pragma unroll
+                for (int r = 0; r < CR; ++r) {
+                    const int t = t0 + r;
+                    if (t > ndg) break;
+                    if (role > 0 && t < Ny) {
+                        settle(rw[r], t);
+                        unsigned char* dst = (unsigned char*)(FZ + (size_t)t * Nx);
+#pragma unroll
+                        for (int k = 0; k < KW; ++k) {
+                            const int i = lane + 64 * k;
+                            // a word still invalid after the bound becomes NaN: the solve fails loudly
+                            const unsigned v = valid(rw[r][k]) ? (unsigned)rw[r][k] : 0xffffffffu;
+                            if (i < nw) __builtin_memcpy(dst + 4 * (size_t)i, &v, 4);
+                        }
+                        if (t + CR < Ny) issue(rw[r], t + CR);
+                    }
+                    __syncthreads();
+                }
+            }
+            return;
+        }
+    }
     __syncthreads();
     T* __restrict__ psi = (T*)d.psi + (size_t)n * Nc;
     T* __restrict__ bx = (T*)d.bx + (size_t)n * Ny * Nz;
     T* __restrict__ by = (T*)d.by + (size_t)n * Nx * Nz;
     T* __restrict__ bz = (T*)d.bz + (size_t)n * Nx * Ny;
     const int* __restrict__ mat = d.mat;
-    const int ndiag = Ny + Nz - 1;
+    const int ndiag = Ny + Zc - 1;
     // Work items of this wave, in order: (diagonal t, first pencil q) for q = wave*U, wave*U + W*U,
     // ... below the diagonal's pencil count, diagonal by diagonal (diagonals where the wave has no
     // pencil are skipped).  The angular source and material of the item PF places ahead are
     // loaded while the current one is solved, into a ring of PF register slots.
     auto diag_lo = [&](int t) { return t - (Ny - 1) > 0 ? t - (Ny - 1) : 0; };
-    auto diag_cnt = [&](int t) { return (t < Nz - 1 ? t : Nz - 1) - diag_lo(t) + 1; };  // <= 0 past the end
+    auto diag_cnt = [&](int t) { return (t < Zc - 1 ? t : Zc - 1) - diag_lo(t) + 1; };  // <= 0 past the end
     auto advance = [&](int& t, int& q) {
         if (q + W * UP < diag_cnt(t)) {
             q += W * UP;
@@ -1588,7 +1723,7 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
             const int q = q0 + u * PPW + half;
             const bool ok = q < diag_cnt(tt);
             const int jz = diag_lo(tt) + (ok ? q : 0), jy = tt - jz;
-            const int iy = fys ? jy : Ny - 1 - jy, iz = fzs ? jz : Nz - 1 - jz;
+            const int iy = fys ? jy : Ny - 1 - jy, iz = fzs ? z0 + jz : Nz - 1 - z0 - jz;
             x[u] = (T)gld(d.bin_x + n * npx + iy + (size_t)Ny * iz);
         }
     };
@@ -1599,7 +1734,7 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
             const int q = q0 + u * PPW + half;
             const bool ok = q < diag_cnt(tt);
             const int jz = diag_lo(tt) + (ok ? q : 0), jy = tt - jz;
-            const int iy = fys ? jy : Ny - 1 - jy, iz = fzs ? jz : Nz - 1 - jz;
+            const int iy = fys ? jy : Ny - 1 - jy, iz = fzs ? z0 + jz : Nz - 1 - z0 - jz;
             const int ro = Nx * (iy + Ny * iz);
 #pragma unroll
             for (int c = 0; c < CPL; ++c) {
@@ -1639,7 +1774,7 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
             const int cnt = diag_cnt(t), jz_lo = diag_lo(t);
             T fy[U][CPL], fz[U][CPL], c0[U][CPL], c1[U][CPL];
             int roff[U], fyo_i[U], fzo_i[U], iyv[U], izv[U];
-            bool ok[U];
+            bool ok[U], top[U];
             T A[U], B[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
@@ -1647,7 +1782,8 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
                 ok[u] = ct < ndiag && q < cnt;
                 const int jz = jz_lo + (ok[u] ? q : 0), jy = t - jz;
                 iyv[u] = fys ? jy : Ny - 1 - jy;
-                izv[u] = fzs ? jz : Nz - 1 - jz;
+                izv[u] = fzs ? z0 + jz : Nz - 1 - z0 - jz;
+                top[u] = ok[u] && jz == Zc - 1;
                 roff[u] = Nx * (iyv[u] + Ny * izv[u]);
                 fyo_i[u] = jz * Nx;
                 fzo_i[u] = jy * Nx;
@@ -1715,6 +1851,17 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
                     fx = inx[c] ? p2 - fx : fx;
                     *(act ? FY + fyo_i[u] + jx : FT + jt) = p2 - fy[u][c];
                     *(act ? FZ + fzo_i[u] + jx : FT + jt) = p2 - fz[u][c];
+                    if constexpr (SPL) {
+                        // the top layer's outgoing z-face goes to the slab above as tagged words
+                        if (hout && act && top[u]) {
+                            const T zo = p2 - fz[u][c];
+                            unsigned wv[WPV];
+                            __builtin_memcpy(wv, &zo, sizeof(T));
+#pragma unroll
+                            for (int w = 0; w < WPV; ++w)
+                                split_st(hout + (size_t)(fzo_i[u] + jx) * WPV + w, ((unsigned long long)sp.tag << 32) | wv[w]);
+                        }
+                    }
                 }
                 gst((ok[u] && last_lane) ? bx + iyv[u] + Ny * izv[u] : trash_slot<T>(lane, 8 + u), fx);
             }
@@ -1726,17 +1873,19 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
         __syncthreads();
         ++bt;
     }
-    // exit faces: FY[jz][jx] -> by[ix + Nx iz], FZ[jy][jx] -> bz[ix + Nx iy]
-    for (int i = tid; i < Nz * Nx; i += blockDim.x) {
-        const int jz = i / Nx, jx = i - jz * Nx;
+    // exit faces: FY[jz][jx] -> by[ix + Nx iz] (the slab's layers), FZ[jy][jx] -> bz[ix + Nx iy]
+    // (the top slab)
+    for (int i = tid; i < Zc * Nx; i += 64 * W) {
+        const int jz = z0 + i / Nx, jx = i - (i / Nx) * Nx;
         const int ix = fxs ? jx : Nx - 1 - jx, iz = fzs ? jz : Nz - 1 - jz;
         gst(by + ix + Nx * iz, FY[i]);
     }
-    for (int i = tid; i < Ny * Nx; i += blockDim.x) {
-        const int jy = i / Nx, jx = i - jy * Nx;
-        const int ix = fxs ? jx : Nx - 1 - jx, iy = fys ? jy : Ny - 1 - jy;
-        gst(bz + ix + Nx * iy, FZ[i]);
-    }
+    if (!hout)
+        for (int i = tid; i < Ny * Nx; i += 64 * W) {
+            const int jy = i / Nx, jx = i - jy * Nx;
+            const int ix = fxs ? jx : Nx - 1 - jx, iy = fys ? jy : Ny - 1 - jy;
+            gst(bz + ix + Nx * iy, FZ[i]);
+        }
 }
 
 template <typename T, int CPL, int U, int PF, int MAXT, bool DPP, bool BIN = false, int PPW = 1>
@@ -1744,6 +1893,17 @@ __global__ __launch_bounds__(MAXT) void sweep_scan_kernel(const PassDesc* __rest
     extern __shared__ __align__(16) unsigned char smem[];
     const SweepTask tk = tasks[blockIdx.x];
     sweep_scan_body<T, CPL, U, PF, DPP, BIN, PPW>(pds[tk.prob], tk.dir, smem);
+}
+
+// A direction's pencils split by z over S workgroups (blockIdx / S is the task); the last wave of
+// each workgroup is the courier of its slab.
+template <typename T, int CPL, int PF, int PPW>
+__global__ __launch_bounds__(1024) void sweep_scan_split_kernel(const PassDesc* __restrict__ pds,
+                                                                const SweepTask* __restrict__ tasks, int S,
+                                                                unsigned tag, int lag) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const SweepTask tk = tasks[blockIdx.x / S];
+    sweep_scan_body<T, CPL, 1, PF, true, false, PPW, true>(pds[tk.prob], tk.dir, smem, ScanSplit{S, tag, lag});
 }
 
 template <int CPL, int U, int PF, int MAXT, bool DPP, int PPW = 1>
@@ -2787,17 +2947,89 @@ static void scan3d_launch(int kind, const PassDesc* d_pd, const SweepTask* tasks
                      d_pd, tasks);
 }
 
-// Two pencils per wave (32 lanes each): one item shape, the default waves and half of them.
+// z-split launches: RADIANT_HIP_SCAN_SPLIT=1 never splits, =2 or =4 always splits into that many
+// slabs where the grid allows it; unset, the split shapes are autotuner candidates.
+static int scan_split_env() {
+    const char* e = std::getenv("RADIANT_HIP_SCAN_SPLIT");
+    return e ? std::atoi(e) : 0;
+}
+
+// Solving waves of a slab workgroup (its courier comes on top), 0 when S slabs do not fit the grid:
+// every slab needs a layer, and a handed row must fit the courier's two words per lane.  items:
+// items per wave on the slab's longest diagonal (two pencils per item).
+int scan_split_waves(const ScanSplitGeom& g, int S, int prec, int items) {
+    if (S < 2 || S > SCAN_SPLIT_MAX || g.Nz < 2 * S) return 0;
+    if (g.Nx * (prec == P64 ? 2 : 1) > 128) return 0;
+    const int Zb = (g.Nz + S - 1) / S;
+    if ((S - 1) * Zb >= g.Nz) return 0;
+    const int diag = g.Ny < Zb ? g.Ny : Zb;
+    const int w = (diag + 2 * items - 1) / (2 * items);
+    return w < 1 ? 1 : (w > 15 ? 15 : w);
+}
+
+static std::atomic<unsigned> g_split_tag{0};
+
+template <int CPL>
+static void scan3d_split_launch(int kind, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, const ScanSplitGeom& g,
+                                int S, int items, size_t lds, hipStream_t st) {
+    const int ws = scan_split_waves(g, S, kind, items);
+    const size_t es = kind == P64 ? 8 : 4;
+    const int Zb = (g.Nz + S - 1) / S;
+    const size_t lds_s = lds - es * (size_t)(g.Nz - Zb) * g.Nx;  // FY holds the slab's layers only
+    unsigned tag = ++g_split_tag;
+    if (tag == 0) tag = ++g_split_tag;  // 0 is the value of words never written
+    auto f64 = sweep_scan_split_kernel<double, CPL, 4, 2>;
+    auto f32 = sweep_scan_split_kernel<float, CPL, 4, 2>;
+    static bool attr[2] = {false, false};
+    const int ai = kind == P64 ? 0 : 1;
+    if (!attr[ai]) {
+        // 160 KiB less the kernel's static LDS (the slab role)
+        RT_HIP(hipFuncSetAttribute(ai == 0 ? (const void*)f64 : (const void*)f32,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256));
+        attr[ai] = true;
+    }
+    static const int lag = [] {
+        const char* e = std::getenv("RADIANT_HIP_SCAN_SPLIT_LAG");
+        return e ? std::atoi(e) : SCAN_SPLIT_LAG;
+    }();
+    if (ai == 0) launch_timed(f64, dim3(ntasks * S), dim3(64 * (ws + 1)), lds_s, st, d_pd, tasks, S, tag, lag);
+    else launch_timed(f32, dim3(ntasks * S), dim3(64 * (ws + 1)), lds_s, st, d_pd, tasks, S, tag, lag);
+}
+
+// Two pencils per wave (32 lanes each): one item shape, the default waves and half of them, and
+// for single-problem passes whose PassDesc carries the split buffers (sg) the z-split into 2 and 4
+// slabs per direction.
 template <int CPL, bool BIN>
 static void scan3d_pair_inst(int kind, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int waves, size_t lds,
-                             hipStream_t st) {
+                             hipStream_t st, const ScanSplitGeom* sg) {
     const int NW = waves >= 2 ? 2 : 1;
+    // split candidates: (slabs, items per wave on the longest diagonal)
+    int splits[4], items[4], ns = 0;
+    if constexpr (!BIN && CPL <= 2) {
+        if (sg && kind != PMIX && scan_split_env() != 1)
+            for (int S : {2, 4})
+                if (scan_split_waves(*sg, S, kind, 1) > 0)
+                    for (int it : {1, 2}) { splits[ns] = S; items[ns] = it; ++ns; }
+        const int forced = scan_split_env();
+        const char* fi = std::getenv("RADIANT_HIP_SCAN_SPLIT_ITEMS");
+        const int fit = fi ? std::atoi(fi) : 1;
+        for (int k = 0; k < ns; ++k)
+            if (splits[k] == forced && items[k] == fit) {
+                scan3d_split_launch<CPL>(kind, d_pd, tasks, ntasks, *sg, forced, fit, lds, st);
+                return;
+            }
+    }
     TuneTicket t;
-    const int v = tune_choose({kind, 1, CPL | SCAN_PAIR, waves, BIN ? 1 : 0, ilog2(ntasks)}, NW, t);
+    const int v = tune_choose({kind, 1, CPL | SCAN_PAIR, waves, (BIN ? 1 : 0) + 2 * (ns / 2), ilog2(ntasks)}, NW + ns, t);
     if (t.a) set_launch_events(t.a, t.b);
-    const int w = v ? (waves + 1) / 2 : waves;
-    constexpr int PF = CPL >= 4 ? 2 : 4;
-    scan3d_launch<CPL, 1, PF, 1024, true, BIN, 2>(kind, d_pd, tasks, ntasks, w, lds, st);
+    if (v >= NW) {
+        if constexpr (!BIN && CPL <= 2)
+            scan3d_split_launch<CPL>(kind, d_pd, tasks, ntasks, *sg, splits[v - NW], items[v - NW], lds, st);
+    } else {
+        const int w = v ? (waves + 1) / 2 : waves;
+        constexpr int PF = CPL >= 4 ? 2 : 4;
+        scan3d_launch<CPL, 1, PF, 1024, true, BIN, 2>(kind, d_pd, tasks, ntasks, w, lds, st);
+    }
     tune_commit(t);
 }
 
@@ -2896,7 +3128,7 @@ static void scan_any(int kind, const PassDesc* d_pd, const SweepTask* tasks, int
 }
 
 void launch_sweep_scan(int prec, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int cpl, int waves, size_t lds,
-                       hipStream_t st, bool bin) {
+                       hipStream_t st, bool bin, const ScanSplitGeom* sg) {
     if (ntasks <= 0) return;
     const int kind = prec == PMIX ? PMIX : (prec == P64 ? P64 : P32);
     if (bin && kind == PMIX) throw std::runtime_error("boundary inflow is supported by single-problem passes only");
@@ -2904,12 +3136,12 @@ void launch_sweep_scan(int prec, const PassDesc* d_pd, const SweepTask* tasks, i
         cpl &= SCAN_PAIR - 1;
         if (waves > 0 && scan_use_dpp()) {  // the shuffle cross-check runs one pencil per wave
             switch (cpl) {
-                case 1: bin ? scan3d_pair_inst<1, true>(kind, d_pd, tasks, ntasks, waves, lds, st)
-                            : scan3d_pair_inst<1, false>(kind, d_pd, tasks, ntasks, waves, lds, st); break;
-                case 2: bin ? scan3d_pair_inst<2, true>(kind, d_pd, tasks, ntasks, waves, lds, st)
-                            : scan3d_pair_inst<2, false>(kind, d_pd, tasks, ntasks, waves, lds, st); break;
-                default: bin ? scan3d_pair_inst<4, true>(kind, d_pd, tasks, ntasks, waves, lds, st)
-                             : scan3d_pair_inst<4, false>(kind, d_pd, tasks, ntasks, waves, lds, st); break;
+                case 1: bin ? scan3d_pair_inst<1, true>(kind, d_pd, tasks, ntasks, waves, lds, st, nullptr)
+                            : scan3d_pair_inst<1, false>(kind, d_pd, tasks, ntasks, waves, lds, st, sg); break;
+                case 2: bin ? scan3d_pair_inst<2, true>(kind, d_pd, tasks, ntasks, waves, lds, st, nullptr)
+                            : scan3d_pair_inst<2, false>(kind, d_pd, tasks, ntasks, waves, lds, st, sg); break;
+                default: bin ? scan3d_pair_inst<4, true>(kind, d_pd, tasks, ntasks, waves, lds, st, nullptr)
+                             : scan3d_pair_inst<4, false>(kind, d_pd, tasks, ntasks, waves, lds, st, sg); break;
             }
             RT_HIP(hipGetLastError());
             return;

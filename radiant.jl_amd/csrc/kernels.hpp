@@ -97,6 +97,11 @@ struct PassDesc {
     // faces exceed the 160 KiB of a workgroup): [Nd][faces] in the pass's compute type, launched
     // with no dynamic LDS.  Null: LDS.
     void* hoface;
+    // 3D scan sweeps split by z over several workgroups per direction (null: not offered): a
+    // claim counter per direction [Nd] and the handed z-face rows [Nd][SCAN_SPLIT_MAX - 1][Ny][Nx][2]
+    // of tagged 64-bit words, both zero before the first launch
+    unsigned long long* sclaim;
+    unsigned long long* shand;
 };
 
 enum Prec { P64 = 0, P32 = 1, PBF16 = 2,
@@ -149,9 +154,14 @@ constexpr int SCAN_PAIR_NX = 64;
 int sweep_scan_cpl3d(int Nx);
 size_t sweep_scan_lds_bytes(int prec, int ndims, int Nx, int Ny, int Nz, int Nmat);
 int sweep_scan_waves(int ndims, int Ny, int Nz, int Nx);
-// bin: the passes take their inflow from PassDesc::bin_* (FP64 only)
+// 3D scan sweeps split by z over several workgroups per direction (PassDesc::sclaim / shand)
+constexpr int SCAN_SPLIT_MAX = 4;  // slabs per direction at most: shand holds SCAN_SPLIT_MAX - 1 faces
+struct ScanSplitGeom { int Nx, Ny, Nz; };
+int scan_split_waves(const ScanSplitGeom& g, int S, int prec, int items);
+// bin: the passes take their inflow from PassDesc::bin_* (FP64 only).  sg: the grid of a
+// single-problem pass whose PassDesc carries the split buffers (null: no split shapes).
 void launch_sweep_scan(int prec, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, int cpl, int waves, size_t lds,
-                       hipStream_t st, bool bin = false);
+                       hipStream_t st, bool bin = false, const ScanSplitGeom* sg = nullptr);
 // incoming face fluxes of non-void boundaries (PassDesc::bin_*), one problem
 void launch_bndin(const PassDesc* d_pd, int total, hipStream_t st);
 // continuous slowing down, one problem: psi += K[mat] * psiE_in before the sweep; psiE_out = 2 psi -
