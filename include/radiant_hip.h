@@ -127,9 +127,9 @@ typedef struct rt_problem_desc {
     int32_t order_y;
     const double *omega_y;
     int32_t partial_coupling;
-    /* The z axis of 3D problems likewise (orders at most 2 per axis in 3D; Nm5 = 𝒪x*𝒪y*𝒪z moments,
-     * or 𝒪x + 𝒪y + 𝒪z - 2 without full coupling; a face carries the moments over the other two
-     * axes). */
+    /* The z axis of 3D problems likewise (orders at most 3 per axis in 3D and at most 12 moments per
+     * cell; Nm5 = 𝒪x*𝒪y*𝒪z moments, or 𝒪x + 𝒪y + 𝒪z - 2 without full coupling; a face carries the
+     * moments over the other two axes). */
     int32_t order_z;
     const double *omega_z;
     /* The energy axis of the slowing-down types (csd = 1; SN.set_scheme("E", type, order),

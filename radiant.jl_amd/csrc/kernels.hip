@@ -3240,26 +3240,49 @@ static void ho3_launch(const PassDesc* d_pd, int Nd, size_t lds, hipStream_t st)
     launch_timed(fn, dim3(Nd), dim3(256), (uint32_t)lds, st, d_pd);  // with the caller's timing events
 }
 
+bool sweep_3d_ho_supported(int Kx, int Ky, int Kz, bool fc) {
+    if (Kx < 1 || Ky < 1 || Kz < 1 || Kx > HO3_KMAX || Ky > HO3_KMAX || Kz > HO3_KMAX) return false;
+    return (fc ? Kx * Ky * Kz : Kx + Ky + Kz - 2) <= HO3_NM_MAX;
+}
+
 template <typename T, int KX, int KY, int KZ>
 static void ho3_fc(bool fc, const PassDesc* d_pd, int Nd, size_t lds, hipStream_t st) {
     // partial coupling differs from full coupling only when two orders or more exceed 1
     if constexpr ((KX > 1) + (KY > 1) + (KZ > 1) >= 2) {
         if (!fc) return ho3_launch<T, KX, KY, KZ, false>(d_pd, Nd, lds, st);
     }
-    ho3_launch<T, KX, KY, KZ, true>(d_pd, Nd, lds, st);
+    // the cell system lives in registers: full coupling up to HO3_NM_MAX moments
+    if constexpr (KX * KY * KZ <= HO3_NM_MAX) ho3_launch<T, KX, KY, KZ, true>(d_pd, Nd, lds, st);
+    else throw std::runtime_error("3D closure with more than HO3_NM_MAX moments per cell");
+}
+
+template <typename T, int KX, int KY>
+static void ho3_z(int Kz, bool fc, const PassDesc* d_pd, int Nd, size_t lds, hipStream_t st) {
+    switch (Kz) {
+        case 1: return ho3_fc<T, KX, KY, 1>(fc, d_pd, Nd, lds, st);
+        case 2: return ho3_fc<T, KX, KY, 2>(fc, d_pd, Nd, lds, st);
+        case 3: return ho3_fc<T, KX, KY, 3>(fc, d_pd, Nd, lds, st);
+        default: throw std::runtime_error("3D closure orders out of range");
+    }
+}
+
+template <typename T, int KX>
+static void ho3_yz(int Ky, int Kz, bool fc, const PassDesc* d_pd, int Nd, size_t lds, hipStream_t st) {
+    switch (Ky) {
+        case 1: return ho3_z<T, KX, 1>(Kz, fc, d_pd, Nd, lds, st);
+        case 2: return ho3_z<T, KX, 2>(Kz, fc, d_pd, Nd, lds, st);
+        case 3: return ho3_z<T, KX, 3>(Kz, fc, d_pd, Nd, lds, st);
+        default: throw std::runtime_error("3D closure orders out of range");
+    }
 }
 
 template <typename T>
 static void ho3_dispatch(const PassDesc* d_pd, int Nd, int Kx, int Ky, int Kz, bool fc, size_t lds, hipStream_t st) {
-    switch (Kx * 100 + Ky * 10 + Kz) {
-        case 111: return ho3_fc<T, 1, 1, 1>(fc, d_pd, Nd, lds, st);
-        case 211: return ho3_fc<T, 2, 1, 1>(fc, d_pd, Nd, lds, st);
-        case 121: return ho3_fc<T, 1, 2, 1>(fc, d_pd, Nd, lds, st);
-        case 112: return ho3_fc<T, 1, 1, 2>(fc, d_pd, Nd, lds, st);
-        case 221: return ho3_fc<T, 2, 2, 1>(fc, d_pd, Nd, lds, st);
-        case 212: return ho3_fc<T, 2, 1, 2>(fc, d_pd, Nd, lds, st);
-        case 122: return ho3_fc<T, 1, 2, 2>(fc, d_pd, Nd, lds, st);
-        case 222: return ho3_fc<T, 2, 2, 2>(fc, d_pd, Nd, lds, st);
+    if (!sweep_3d_ho_supported(Kx, Ky, Kz, fc)) throw std::runtime_error("3D closure orders out of range");
+    switch (Kx) {
+        case 1: return ho3_yz<T, 1>(Ky, Kz, fc, d_pd, Nd, lds, st);
+        case 2: return ho3_yz<T, 2>(Ky, Kz, fc, d_pd, Nd, lds, st);
+        case 3: return ho3_yz<T, 3>(Ky, Kz, fc, d_pd, Nd, lds, st);
         default: throw std::runtime_error("3D closure orders out of range");
     }
 }

@@ -180,9 +180,12 @@ void launch_sweep_1d_ho(int prec, const PassDesc* d_pd, int Nd, int K, hipStream
 constexpr int HO2_KMAX = 3;
 size_t sweep_2d_ho_lds_bytes(int prec, int Nx, int Ny, int Kx, int Ky);
 void launch_sweep_2d_ho(int prec, const PassDesc* d_pd, int Nd, int Kx, int Ky, bool fc, size_t lds, hipStream_t st);
-// 3D sweep of a general closure of orders up to HO3_KMAX per axis: one workgroup per direction
-// walking the diagonal planes; lds = face moments of every pencil of the three axes
-constexpr int HO3_KMAX = 2;
+// 3D sweep of a general closure of orders up to HO3_KMAX per axis and at most HO3_NM_MAX moments per
+// cell (full coupling: the product of the orders; without: their sum - 2): one workgroup per
+// direction walking the diagonal planes; lds = face moments of every pencil of the three axes
+constexpr int HO3_KMAX = 3;
+constexpr int HO3_NM_MAX = 12;
+bool sweep_3d_ho_supported(int Kx, int Ky, int Kz, bool fc);
 size_t sweep_3d_ho_lds_bytes(int prec, int Nx, int Ny, int Nz, int Kx, int Ky, int Kz, bool fc);
 // Slowing-down sweep with a general closure in space and energy (sweep_bfp.hip): one workgroup per
 // direction walks the diagonal planes; orders Ke 1..2, Kx 1..4 in 1D and 1..2 per axis in 2D/3D,

@@ -40,6 +40,10 @@ def _mix():
     p.scheme_order, p.scheme_order_y, p.scheme_order_z, p.full_coupling = 2, 2, 1, False
     out.append(p)
     out.append(P.small_problem(2, N=8, G=3, L=2, tol=1e-12, seed=56))  # order 1
+    p = P.small_problem(3, dims=(3, 4, 3), G=2, L=2, tol=1e-12, seed=57, bc=[0, 1, 2, 2, 1, 0])
+    p.scheme, p.scheme_y, p.scheme_z = "DD", "DG", "DD"
+    p.scheme_order, p.scheme_order_y, p.scheme_order_z, p.full_coupling = 3, 2, 2, True  # 12 moments
+    out.append(p)
     return out
 
 

@@ -1,5 +1,6 @@
-"""Spatial closures of the 3D cell: DD and DG of orders up to 2 per axis, with and without full
-coupling (flux_3D_BTE of sn_3D_bte.jl:34-152, sn_sweep_3D.jl:57-240).
+"""Spatial closures of the 3D cell: DD and DG of orders up to 3 per axis, with and without full
+coupling (flux_3D_BTE of sn_3D_bte.jl:34-152, sn_sweep_3D.jl:57-240); the GPU sweep holds cells of
+at most 12 moments (3 x 2 x 2 with full coupling, any orders up to 3 without).
 
 CPU pins of the oracle's cell, as for 2D (psi = sum C_a C_b C_c psi_abc P_a P_b P_c on the cell):
 - polynomial transport solutions are reproduced exactly, cell and face moments, by DD up to degrees
@@ -69,7 +70,8 @@ def _cell_case(oracle, schemes, O3, fc, deg, om, rng):
 
 
 @pytest.mark.parametrize("scheme", ["DD", "DG"])
-@pytest.mark.parametrize("O3", [(1, 1, 1), (2, 1, 1), (1, 2, 1), (2, 2, 1), (1, 2, 2), (2, 2, 2)])
+@pytest.mark.parametrize("O3", [(1, 1, 1), (2, 1, 1), (1, 2, 1), (2, 2, 1), (1, 2, 2), (2, 2, 2), (3, 1, 1), (1, 3, 2),
+                                (3, 2, 2), (3, 3, 3)])
 @pytest.mark.parametrize("fc", [True, False])
 def test_cell_polynomial_exactness_and_balance(oracle, scheme, O3, fc):
     rng = np.random.default_rng(sum(O3))
@@ -114,7 +116,12 @@ def _say(*a):
 
 
 GPU_CASES = [(cfg, None) for cfg in CONFIGS] + [(("DD", "DD", "DD", 2, 2, 2, True), [1, 2, 0, 1, 2, 0]),
-                                                (("DG", "DG", "DG", 2, 2, 1, False), [1] * 6)]
+                                                (("DG", "DG", "DG", 2, 2, 1, False), [1] * 6),
+                                                # order 3: 7 moments without coupling, 12 and 6 with it
+                                                (("DD", "DD", "DD", 3, 3, 3, False), None),
+                                                (("DG", "DD", "DG", 3, 2, 2, True), [1, 2, 0, 1, 2, 0]),
+                                                (("DD", "DG", "DD", 1, 3, 2, True), [2] * 6),
+                                                (("DG", "DG", "DD", 3, 3, 2, False), [1] * 6)]
 
 
 @pytest.mark.gpu
@@ -152,6 +159,17 @@ def test_scheme3d_gpu_parity(oracle, case):
     e = rel_err(mx, ref)
     _say("  mixed-fp32", e)
     assert e < 1e-6
+
+
+@pytest.mark.gpu
+def test_scheme3d_moment_limit_refused():
+    """Full coupling of orders 3 x 3 x 2 gives 18 moments per cell, above the 12 whose cell system
+    the sweep holds in registers: refused at set-up with the limits named."""
+    from radiant_hip import problems as P, solver as S
+    from radiant_hip._lib import RTError
+    p = _set(P.small_problem(3, dims=(4, 3, 5), G=2, L=2, seed=140), "DD", "DD", "DD", 3, 3, 2, True)
+    with pytest.raises(RTError, match="at most 12 moments per cell"):
+        S.ResidentProblem(p)
 
 
 @pytest.mark.gpu
