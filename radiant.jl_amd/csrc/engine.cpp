@@ -518,6 +518,8 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
         // the 1D/2D row-scan kernel addresses a direction's angular flux through one buffer
         // resource, whose byte count is 32-bit
         if (ndims <= 2 && (uint64_t)Nc * 8u > 0xffffffffull) scan_cpl = 0;
+        // the 3D one's buffer offsets of a direction stay below 2^30 bytes
+        if (ndims == 3 && (uint64_t)Nc * 8u > 0x40000000ull) scan_cpl = 0;
         scan_waves = sweep_scan_waves(ndims, Ny, Nz, Nx);
         if (split_scan_offered(d)) {
             d_sclaim = dalloc<unsigned long long>(Nd);

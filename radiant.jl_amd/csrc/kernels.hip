@@ -1525,6 +1525,7 @@ struct ScanSplit {
     unsigned tag;  // launch tag, never 0
     int lag;       // rows the slab below is ahead when a slab starts
 };
+constexpr unsigned SCAN_OOB = 0x40000000u;  // buffer offset past every num_records of the 3D scan
 constexpr int SCAN_SPLIT_ROWS = 6;  // rows the courier keeps in flight
 constexpr int SCAN_SPLIT_LAG = 9;   // rows the slab below is ahead when a slab starts (default)
 constexpr int SCAN_SPLIT_SPIN = 1 << 20;  // polls of handoff rows per workgroup before values are given up (NaN)
@@ -1583,35 +1584,44 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
         if (role + 1 < sp.S) hout = hb + (size_t)role * hrow;
         if (role > 0) hin = hb + (size_t)(role - 1) * hrow;
     }
+    // A face row of Nx cells is held in RS slots with cell jx at slot(jx) = (jx % CPL) La + jx / CPL
+    // (La lanes hold the row's cells): lane l's cell c sits at c La + l, so the lanes of one LDS
+    // access read consecutive slots, free of bank conflicts.
+    const int La = (Nx + CPL - 1) / CPL, RS = CPL * La;
+    auto slot = [&](int jx) { return (jx % CPL) * La + jx / CPL; };
     T* hy2 = (T*)smem;
     T* hz2 = hy2 + Ny;
     T* st = hz2 + Nz;
-    T* FY = st + Nmat;                   // [Zb][Nx]: y-faces of the slab's layers
-    T* FZ = FY + (size_t)Zb * Nx;        // [Ny][Nx]
-    T* FT = FZ + (size_t)Ny * Nx;        // [64*CPL] trash for inactive lanes
+    T* FY = st + Nmat;                   // [Zb][RS]: y-faces of the slab's layers
+    T* FZ = FY + (size_t)Zb * RS;        // [Ny][RS]
+    T* FT = FZ + (size_t)Ny * RS;        // [64*CPL] trash for inactive lanes
     const double mu = d.mu[n], eta = d.eta[n], xi = d.xi[n];
     for (int i = tid; i < Ny; i += blockDim.x) hy2[i] = (T)(2.0 * (fabs(eta) / d.dy[i]));
     for (int i = tid; i < Nz; i += blockDim.x) hz2[i] = (T)(2.0 * (fabs(xi) / d.dz[i]));
     for (int i = tid; i < Nmat; i += blockDim.x) st[i] = (T)d.sigt[i];
-    const bool fxs = mu >= 0, fys = eta >= 0, fzs = xi >= 0;
+    // the octant is uniform: scalar, so that the pencil index work below stays scalar
+    const bool fxs = __builtin_amdgcn_readfirstlane((int)(mu >= 0)) != 0;
+    const bool fys = __builtin_amdgcn_readfirstlane((int)(eta >= 0)) != 0;
+    const bool fzs = __builtin_amdgcn_readfirstlane((int)(xi >= 0)) != 0;
     const size_t npx = (size_t)Ny * Nz, npy = (size_t)Nx * Nz, npz = (size_t)Nx * Ny;  // face positions
     // inflow on the entry y- and z-faces: zero for void boundaries; the z-faces of a slab above
     // the first come from the slab below (written by the courier)
     const int nfz = (Zb + (role == 0 ? Ny : 0)) * Nx;
     for (int i = tid; i < nfz; i += blockDim.x) {
+        const int r = i / Nx, jx = i - r * Nx;  // row of FY, then of FZ, and cell
         T v = (T)0;
         if constexpr (BIN) {
-            if (i < Zb * Nx) {
-                const int jz = z0 + i / Nx, jx = i - (i / Nx) * Nx;
+            if (r < Zb) {
+                const int jz = z0 + r;
                 const int ix = fxs ? jx : Nx - 1 - jx, iz = fzs ? jz : Nz - 1 - jz;
                 if (jz < Nz) v = (T)gld(d.bin_y + n * npy + ix + (size_t)Nx * iz);
             } else {
-                const int k = i - Zb * Nx, jy = k / Nx, jx = k - jy * Nx;
+                const int jy = r - Zb;
                 const int ix = fxs ? jx : Nx - 1 - jx, iy = fys ? jy : Ny - 1 - jy;
                 v = (T)gld(d.bin_z + n * npz + ix + (size_t)Nx * iy);
             }
         }
-        FY[i] = v;
+        FY[(size_t)r * RS + slot(jx)] = v;
     }
     int ixc[CPL];
     bool inx[CPL];
@@ -1675,13 +1685,14 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
                     if (t > ndg) break;
                     if (role > 0 && t < Ny) {
                         settle(rw[r], t);
-                        unsigned char* dst = (unsigned char*)(FZ + (size_t)t * Nx);
+                        unsigned char* dst = (unsigned char*)(FZ + (size_t)t * RS);
 #pragma unroll
                         for (int k = 0; k < KW; ++k) {
-                            const int i = lane + 64 * k;
+                            const int i = lane + 64 * k;  // word w of cell jx (rows are handed in cell order)
+                            const int jx = i / WPV, w = i - jx * WPV;
                             // a word still invalid after the bound becomes NaN: the solve fails loudly
                             const unsigned v = valid(rw[r][k]) ? (unsigned)rw[r][k] : 0xffffffffu;
-                            if (i < nw) __builtin_memcpy(dst + 4 * (size_t)i, &v, 4);
+                            if (i < nw) __builtin_memcpy(dst + 4 * ((size_t)slot(jx) * WPV + w), &v, 4);
                         }
                         if (t + CR < Ny) issue(rw[r], t + CR);
                     }
@@ -1696,7 +1707,34 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
     T* __restrict__ bx = (T*)d.bx + (size_t)n * Ny * Nz;
     T* __restrict__ by = (T*)d.by + (size_t)n * Nx * Nz;
     T* __restrict__ bz = (T*)d.bz + (size_t)n * Nx * Ny;
-    const int* __restrict__ mat = d.mat;
+    // The angular flux, the material map and the x exit face go through buffer resources: offsets
+    // are 32-bit, and a load or store at an offset of SCAN_OOB or more (past num_records: the host
+    // keeps a direction's cells within SCAN_OOB bytes) reads 0 or is dropped, so a pencil that does
+    // not exist and a cell past Nx need neither a clamped address nor a trash line.
+    constexpr unsigned es = sizeof(T);
+    const __amdgpu_buffer_rsrc_t rp = buf_rsrc(psi, (unsigned)((size_t)Nc * es));
+    const __amdgpu_buffer_rsrc_t rm = buf_rsrc(d.mat, (unsigned)((size_t)Nc * 4));
+    const __amdgpu_buffer_rsrc_t rb = buf_rsrc(bx, (unsigned)((size_t)Ny * Nz * es));
+    unsigned xl[CPL], xs[CPL];  // a cell's byte offset in its row, for loads and stores (SCAN_OOB: past Nx)
+#pragma unroll
+    for (int c = 0; c < CPL; ++c) {
+        xl[c] = (unsigned)ixc[c] * es;
+        xs[c] = inx[c] ? xl[c] : SCAN_OOB;
+    }
+    // Pencil indices.  In sweep order pencil (jy, jz) of diagonal t = jy + jz (jz within the slab)
+    // has iy = ay + sy jy, iz = az + sz jz and row iy + Ny iz = rb0 + sy t + dr jz.  The item's first
+    // pencil q0 is scalar work; this lane's pencil is q0 + hq (hq: its half-wave with two pencils per
+    // wave), whose indices differ from q0's by per-lane constants.
+    const int sy = fys ? 1 : -1, ay = fys ? 0 : Ny - 1;
+    const int sz = fzs ? 1 : -1, az = fzs ? z0 : Nz - 1 - z0;
+    const int dr = Ny * sz - sy, rb0 = ay + Ny * az;
+    const int hq = PPW == 2 ? half : 0;
+    const int h_row = hq * dr, h_rs = hq * RS, h_nx = hq * Nx, h_iy = -sy * hq, h_iz = sz * hq;
+    // byte offsets of the lane's pencil past q0's: angular flux row, material row, x exit face
+    const unsigned h_rp = (unsigned)(h_row * Nx) * es, h_rm = (unsigned)(h_row * Nx) * 4u, h_rb = (unsigned)h_row * es;
+    // A scalar value passed through readfirstlane stays scalar: the compiler would otherwise fold
+    // the per-lane constant in before a multiply and do the multiply per lane.
+    auto sc = [](int v) { return __builtin_amdgcn_readfirstlane(v); };
     const int ndiag = Ny + Zc - 1;
     // Work items of this wave, in order: (diagonal t, first pencil q) for q = wave*U, wave*U + W*U,
     // ... below the diagonal's pencil count, diagonal by diagonal (diagonals where the wave has no
@@ -1716,30 +1754,30 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
     T vb[PF][U][CPL];
     int mb[PF][U][CPL];
     T xb[PF][U];  // x-face inflow of each pencil of the item in a slot (BIN)
+    // the row (iy + Ny iz) of pencil q0 of diagonal t (scalar)
+    auto row_of = [&](int t, int q0) { return rb0 + sy * t + dr * (diag_lo(t) + q0); };
     auto load_inflow = [&](T (&x)[U], int t, int q0) {
         const int tt = t < ndiag ? t : ndiag - 1;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int q = q0 + u * PPW + half;
-            const bool ok = q < diag_cnt(tt);
-            const int jz = diag_lo(tt) + (ok ? q : 0), jy = tt - jz;
-            const int iy = fys ? jy : Ny - 1 - jy, iz = fzs ? z0 + jz : Nz - 1 - z0 - jz;
-            x[u] = (T)gld(d.bin_x + n * npx + iy + (size_t)Ny * iz);
+            const int qu = q0 + u * PPW;
+            const bool okl = hq < diag_cnt(tt) - qu;
+            x[u] = (T)gld(d.bin_x + n * npx + (okl ? sc(row_of(tt, qu)) + h_row : 0));
         }
     };
     auto load_item = [&](T (&v)[U][CPL], int (&m)[U][CPL], int t, int q0) {
         const int tt = t < ndiag ? t : ndiag - 1;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int q = q0 + u * PPW + half;
-            const bool ok = q < diag_cnt(tt);
-            const int jz = diag_lo(tt) + (ok ? q : 0), jy = tt - jz;
-            const int iy = fys ? jy : Ny - 1 - jy, iz = fzs ? z0 + jz : Nz - 1 - z0 - jz;
-            const int ro = Nx * (iy + Ny * iz);
+            const int qu = q0 + u * PPW, row = row_of(tt, qu);
+            const bool okl = hq < diag_cnt(tt) - qu;
+            // byte offsets of this lane's row in the angular flux and in the material map
+            const unsigned rp_o = okl ? (unsigned)sc(row * Nx * (int)es) + h_rp : SCAN_OOB;
+            const unsigned rm_o = okl ? (unsigned)sc(row * Nx * 4) + h_rm : SCAN_OOB;
 #pragma unroll
             for (int c = 0; c < CPL; ++c) {
-                v[u][c] = gld(psi + ro + ixc[c]);
-                m[u][c] = gld(mat + ro + ixc[c]);
+                v[u][c] = bld<T>(rp, rp_o + xl[c]);
+                m[u][c] = bld<int>(rm, rm_o + (unsigned)ixc[c] * 4u);
             }
         }
     };
@@ -1773,26 +1811,31 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
             const int t = ct < ndiag ? ct : ndiag - 1;
             const int cnt = diag_cnt(t), jz_lo = diag_lo(t);
             T fy[U][CPL], fz[U][CPL], c0[U][CPL], c1[U][CPL];
-            int roff[U], fyo_i[U], fzo_i[U], iyv[U], izv[U];
+            unsigned soff[U], boff[U];
+            int fyo_i[U], fzo_i[U], hro_i[U];
             bool ok[U], top[U];
             T A[U], B[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int q = cq + u * PPW + half;
-                ok[u] = ct < ndiag && q < cnt;
-                const int jz = jz_lo + (ok[u] ? q : 0), jy = t - jz;
-                iyv[u] = fys ? jy : Ny - 1 - jy;
-                izv[u] = fzs ? z0 + jz : Nz - 1 - z0 - jz;
-                top[u] = ok[u] && jz == Zc - 1;
-                roff[u] = Nx * (iyv[u] + Ny * izv[u]);
-                fyo_i[u] = jz * Nx;
-                fzo_i[u] = jy * Nx;
-                const T hy = hy2[iyv[u]], hz = hz2[izv[u]];
+                const int q0 = cq + u * PPW, jz0 = jz_lo + q0;  // the item's first pencil (scalar)
+                ok[u] = ct < ndiag && hq < cnt - q0;
+                top[u] = ok[u] && jz0 + hq == Zc - 1;
+                const int row = rb0 + sy * t + dr * jz0;
+                soff[u] = ok[u] ? (unsigned)sc(row * Nx * (int)es) + h_rp : SCAN_OOB;
+                boff[u] = (ok[u] && last_lane) ? (unsigned)sc(row * (int)es) + h_rb : SCAN_OOB;
+                // LDS indices, clamped into their arrays for a pencil that does not exist (it stores
+                // only to trash)
+                fyo_i[u] = min(sc(jz0 * RS) + h_rs, (Zc - 1) * RS);
+                fzo_i[u] = max(sc((t - jz0) * RS) - h_rs, 0);
+                if constexpr (SPL) hro_i[u] = sc((t - jz0) * Nx) - h_nx;  // handed row of jy (cell order)
+                const int iy = min(max(sc(ay + sy * (t - jz0)) + h_iy, 0), Ny - 1);
+                const int iz = min(max(sc(az + sz * jz0) + h_iz, 0), Nz - 1);
+                const T hy = hy2[iy], hz = hz2[iz];
                 A[u] = (T)0;
                 B[u] = (T)1;
 #pragma unroll
                 for (int c = 0; c < CPL; ++c) {
-                    const int jx = inx[c] ? ll * CPL + c : 0;
+                    const int jx = inx[c] ? c * La + ll : 0;  // slot of the lane's cell c
                     fy[u][c] = FY[fyo_i[u] + jx];
                     fz[u][c] = FZ[fzo_i[u] + jx];
                     const T S = ((st[mb[sl][u][c]] + hx[c]) + hy) + hz;
@@ -1844,9 +1887,9 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
 #pragma unroll
                 for (int c = 0; c < CPL; ++c) {
                     const bool act = ok[u] && inx[c];
-                    const int jx = ll * CPL + c, jt = lane * CPL + c;
+                    const int jx = c * La + ll, jt = lane * CPL + c;  // slot of cell c; trash slot
                     const T p = c0[u][c] + c1[u][c] * fx;
-                    gst(act ? psi + roff[u] + ixc[c] : trash_slot<T>(lane, c), p);
+                    bst(rp, soff[u] + xs[c], p);
                     const T p2 = p + p;
                     fx = inx[c] ? p2 - fx : fx;
                     *(act ? FY + fyo_i[u] + jx : FT + jt) = p2 - fy[u][c];
@@ -1859,11 +1902,12 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
                             __builtin_memcpy(wv, &zo, sizeof(T));
 #pragma unroll
                             for (int w = 0; w < WPV; ++w)
-                                split_st(hout + (size_t)(fzo_i[u] + jx) * WPV + w, ((unsigned long long)sp.tag << 32) | wv[w]);
+                                split_st(hout + (size_t)(hro_i[u] + ll * CPL + c) * WPV + w,
+                                         ((unsigned long long)sp.tag << 32) | wv[w]);
                         }
                     }
                 }
-                gst((ok[u] && last_lane) ? bx + iyv[u] + Ny * izv[u] : trash_slot<T>(lane, 8 + u), fx);
+                bst(rb, boff[u], fx);
             }
             if constexpr (BIN) load_inflow(xb[sl], xt, xq);
             advance(ct, cq);
@@ -1876,15 +1920,15 @@ __device__ __forceinline__ void sweep_scan_body(const PassDesc& d, const int n, 
     // exit faces: FY[jz][jx] -> by[ix + Nx iz] (the slab's layers), FZ[jy][jx] -> bz[ix + Nx iy]
     // (the top slab)
     for (int i = tid; i < Zc * Nx; i += 64 * W) {
-        const int jz = z0 + i / Nx, jx = i - (i / Nx) * Nx;
+        const int r = i / Nx, jz = z0 + r, jx = i - r * Nx;
         const int ix = fxs ? jx : Nx - 1 - jx, iz = fzs ? jz : Nz - 1 - jz;
-        gst(by + ix + Nx * iz, FY[i]);
+        gst(by + ix + Nx * iz, FY[(size_t)r * RS + slot(jx)]);
     }
     if (!hout)
         for (int i = tid; i < Ny * Nx; i += 64 * W) {
             const int jy = i / Nx, jx = i - jy * Nx;
             const int ix = fxs ? jx : Nx - 1 - jx, iy = fys ? jy : Ny - 1 - jy;
-            gst(bz + ix + Nx * iy, FZ[i]);
+            gst(bz + ix + Nx * iy, FZ[(size_t)jy * RS + slot(jx)]);
         }
 }
 
@@ -2882,8 +2926,10 @@ int sweep_scan_cpl3d(int Nx) {
 size_t sweep_scan_lds_bytes(int prec, int ndims, int Nx, int Ny, int Nz, int Nmat) {
     const size_t es = prec == P64 ? 8 : 4;
     if (ndims < 3) return es * ((size_t)Ny + Nmat);
-    const int cpl = sweep_scan_cpl3d(Nx) & (SCAN_PAIR - 1);
-    return es * ((size_t)Ny + Nz + Nmat + ((size_t)Nz + Ny) * Nx + 64 * (size_t)(cpl > 0 ? cpl : 8));
+    int cpl = sweep_scan_cpl3d(Nx) & (SCAN_PAIR - 1);
+    if (cpl <= 0) cpl = 8;
+    const size_t rs = (size_t)cpl * ((Nx + cpl - 1) / cpl);  // slots of a face row (sweep_scan_body)
+    return es * ((size_t)Ny + Nz + Nmat + ((size_t)Nz + Ny) * rs + 64 * (size_t)cpl);
 }
 
 int sweep_scan_waves(int ndims, int Ny, int Nz, int Nx) {
@@ -2975,7 +3021,8 @@ static void scan3d_split_launch(int kind, const PassDesc* d_pd, const SweepTask*
     const int ws = scan_split_waves(g, S, kind, items);
     const size_t es = kind == P64 ? 8 : 4;
     const int Zb = (g.Nz + S - 1) / S;
-    const size_t lds_s = lds - es * (size_t)(g.Nz - Zb) * g.Nx;  // FY holds the slab's layers only
+    const size_t rs = (size_t)CPL * ((g.Nx + CPL - 1) / CPL);  // slots of a face row
+    const size_t lds_s = lds - es * (size_t)(g.Nz - Zb) * rs;    // FY holds the slab's layers only
     unsigned tag = ++g_split_tag;
     if (tag == 0) tag = ++g_split_tag;  // 0 is the value of words never written
     auto f64 = sweep_scan_split_kernel<double, CPL, 4, 2>;
