@@ -51,6 +51,7 @@ void BatchSolver::init_device(int n) {
         return p;
     };
     d_pd = (PassDesc*)al(sizeof(PassDesc) * n);
+    d_wl = (unsigned*)al(128 * sizeof(unsigned));
     RT_CHECK(hipStreamCreateWithFlags(&st64, hipStreamNonBlocking));
     {
         const char* e = std::getenv("RADIANT_HIP_CGS2");  // re-orthogonalise every Arnoldi step
@@ -566,10 +567,13 @@ void BatchSolver::launch_passes(const std::vector<int>& who, const std::vector<P
             const int cpl = kv.first[0], waves = kv.first[1], bprec = kv.first[2];
             const size_t lds = slds[kv.first];
             SweepTask* sb = swbase;
-            plan.push_back([=] { launch_expand_sweep(dsm, nsm, sb, s); });
+            unsigned* wl = d_wl + (s == st64 ? 64 : 0);  // work-list counter of this stream
+            plan.push_back([=] { launch_expand_sweep(dsm, nsm, sb, s, wl); });
             plan.push_back(timed(1, tix, s_upd[kv.first], 0.0, s_by[kv.first], s, [=] {
                 const int kind = bprec >= 0 ? bprec : (ps_ == PBF16 ? (int)P32 : ps_);
+                set_sweep_worklist(wl);
                 launch_sweep_scan(kind, dpd, sb, ntask, cpl, waves, lds, s, bprec >= 0);
+                set_sweep_worklist(nullptr);
             }));
             swbase += ntask;
         }

@@ -67,6 +67,7 @@ private:
     GemmTask *d_mg[3] = {}, *d_dg[3] = {};
     SweepTask* d_sw[3] = {};
     hipStream_t st64 = nullptr;
+    unsigned* d_wl = nullptr;  // work-list counters of the line sweeps: [0] on st, [64] on st64
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     size_t cap_gemm = 0, cap_sweep = 0;
     void* task_mem[9] = {};

@@ -1504,6 +1504,32 @@ __global__ __launch_bounds__(64) void sweep_line_mixed_kernel(const PassDesc* __
     }
 }
 
+// The same as a persistent work list: a fixed grid of single-wave workgroups takes the tasks in
+// table order (largest problems first) from a counter, zeroed by the task expansion kernel that
+// precedes the launch on its stream, until the table is exhausted.  A wave's tasks run one after
+// another; every wave leaves once its fetch passes the end.
+template <int CPL, int PD, bool DPP>
+__global__ __launch_bounds__(64) void sweep_line_worklist_kernel(const PassDesc* __restrict__ pds,
+                                                                 const SweepTask* __restrict__ tasks, int ntasks,
+                                                                 unsigned* ctr) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    for (;;) {
+        unsigned k = 0;
+        if (threadIdx.x == 0) k = atomicAdd(ctr, 1u);
+        k = __builtin_amdgcn_readfirstlane(k);
+        if (k >= (unsigned)ntasks) return;
+        const SweepTask tk = tasks[k];
+        const PassDesc& d = pds[tk.prob];
+        if constexpr (DPP) {
+            if (d.prec == P64) sweep_line_fast_body<double, CPL, PD>(d, tk.dir, smem);
+            else sweep_line_fast_body<float, CPL, PD>(d, tk.dir, smem);
+        } else {
+            if (d.prec == P64) sweep_line_body<double, CPL, PD, DPP>(d, tk.dir, smem);
+            else sweep_line_body<float, CPL, PD, DPP>(d, tk.dir, smem);
+        }
+    }
+}
+
 // 3D: one workgroup per direction.  Pencils (jy, jz) advance by diagonals t = jy + jz; the
 // y/z face fluxes of the previous diagonal live in LDS (FY[jz][x], FZ[jy][x]), one barrier per
 // diagonal.  After the last diagonal FY and FZ hold the exit y- and z-faces, which are written
@@ -3106,6 +3132,30 @@ static void scan3d_inst(int kind, const PassDesc* d_pd, const SweepTask* tasks, 
     tune_commit(t);
 }
 
+// Persistent work list for the lockstep line sweeps (set_sweep_worklist): off unless
+// RADIANT_HIP_WORKLIST=1; RADIANT_HIP_WORKLIST_WG single-wave workgroups per CU (default 8), or
+// RADIANT_HIP_WORKLIST_GRID workgroups in all.
+thread_local unsigned* tl_worklist = nullptr;
+void set_sweep_worklist(unsigned* ctr) { tl_worklist = ctr; }
+static bool worklist_on() {
+    const char* e = std::getenv("RADIANT_HIP_WORKLIST");
+    return e && e[0] == '1';
+}
+static int worklist_grid() {
+    static const int cus = [] {
+        int dev = 0, n = 256;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+            (void)hipGetLastError();
+            n = 256;
+        }
+        return n;
+    }();
+    const char* g = std::getenv("RADIANT_HIP_WORKLIST_GRID");
+    if (g && std::atoi(g) > 0) return std::atoi(g);
+    const char* e = std::getenv("RADIANT_HIP_WORKLIST_WG");
+    return cus * (e && std::atoi(e) > 0 ? std::atoi(e) : 8);
+}
+
 template <int CPL, int PD, bool DPP, bool BIN>
 static void line_launch(int kind, const PassDesc* d_pd, const SweepTask* tasks, int ntasks, size_t lds, hipStream_t st) {
     if constexpr (BIN) {  // FP64 and FP32 passes of one problem
@@ -3131,7 +3181,18 @@ static void line_launch(int kind, const PassDesc* d_pd, const SweepTask* tasks, 
         RT_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         attr_set[ai] = true;
     }
-    if (kind == PMIX)
+    if (kind == PMIX && tl_worklist && worklist_on()) {
+        auto wl = sweep_line_worklist_kernel<CPL, PD, DPP>;
+        static bool wattr = false;
+        if (!wattr) {
+            RT_HIP(hipFuncSetAttribute((const void*)wl, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            wattr = true;
+        }
+        const int grid = ntasks < worklist_grid() ? ntasks : worklist_grid();
+        unsigned* ctr = tl_worklist;
+        tl_worklist = nullptr;
+        launch_timed(wl, dim3(grid), dim3(64), lds, st, d_pd, tasks, ntasks, ctr);
+    } else if (kind == PMIX)
         launch_timed(sweep_line_mixed_kernel<CPL, PD, DPP>, dim3(ntasks), dim3(64), lds, st, d_pd, tasks);
     else if (kind == P64)
         launch_timed(sweep_line_kernel<double, CPL, PD, DPP>, dim3(ntasks), dim3(64), lds, st, d_pd, tasks);
@@ -3486,9 +3547,10 @@ __global__ void expand_gemm_kernel(const int* __restrict__ meta, int n, GemmTask
     }
 }
 
-__global__ void expand_sweep_kernel(const int* __restrict__ meta, int n, SweepTask* __restrict__ out) {
+__global__ void expand_sweep_kernel(const int* __restrict__ meta, int n, SweepTask* __restrict__ out, unsigned* ctr) {
     // meta: [prob, ndir, offset] per entry
     const int e = blockIdx.x;
+    if (e == 0 && threadIdx.x == 0 && ctr) *ctr = 0u;  // work-list counter of the sweep that follows
     if (e >= n) return;
     const int prob = meta[3 * e], nd = meta[3 * e + 1], off = meta[3 * e + 2];
     for (int i = threadIdx.x; i < nd; i += blockDim.x) out[off + i] = SweepTask{prob, i};
@@ -3670,9 +3732,9 @@ void launch_expand_gemm(const int* meta, int n, GemmTask* out, hipStream_t st) {
     hipLaunchKernelGGL(expand_gemm_kernel, dim3(n), dim3(256), 0, st, meta, n, out);
     RT_HIP(hipGetLastError());
 }
-void launch_expand_sweep(const int* meta, int n, SweepTask* out, hipStream_t st) {
+void launch_expand_sweep(const int* meta, int n, SweepTask* out, hipStream_t st, unsigned* ctr) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(expand_sweep_kernel, dim3(n), dim3(256), 0, st, meta, n, out);
+    hipLaunchKernelGGL(expand_sweep_kernel, dim3(n), dim3(256), 0, st, meta, n, out, ctr);
     RT_HIP(hipGetLastError());
 }
 void launch_vecops(const VecItem* items, int nitems, int blocks_per_item, hipStream_t st) {

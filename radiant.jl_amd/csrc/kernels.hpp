@@ -275,7 +275,13 @@ struct VecItem {
     double alpha;
 };
 void launch_expand_gemm(const int* meta, int n, GemmTask* out, hipStream_t st);   // meta [prob, ntile, nrg, off]
-void launch_expand_sweep(const int* meta, int n, SweepTask* out, hipStream_t st); // meta [prob, ndir, off]
+void launch_expand_sweep(const int* meta, int n, SweepTask* out, hipStream_t st,
+                         unsigned* ctr = nullptr);  // meta [prob, ndir, off]; zeroes *ctr when given
+// Lockstep line sweeps as a persistent work list: the next mixed-precision line sweep launched by
+// launch_sweep_scan from this host thread runs a fixed grid of workgroups that take their tasks in
+// order from the counter ctr (zeroed by launch_expand_sweep on the same stream), when
+// RADIANT_HIP_WORKLIST=1; null clears it.
+void set_sweep_worklist(unsigned* ctr);
 // Batched Gram-Schmidt step, one item per problem: optionally (update) y -= sum_i coef_i V_i in
 // place, then out_i = V_i . y for i < k, or out_0 = y . y (norm).  Every item is reduced over its
 // own bpi blocks in a fixed order, so its result does not depend on the other items.

@@ -232,3 +232,26 @@ def test_server_destroy_with_uncollected_results():
     srv.close()
     phi, _ = S.compute_flux(probs[0])
     assert np.isfinite(phi).all()
+
+
+def test_worklist_sweep_equals_per_task_launch():
+    """The lockstep line sweeps as a persistent work list (RADIANT_HIP_WORKLIST=1: a fixed grid of
+    workgroups taking tasks from a counter) give the same bits as one workgroup per task, for an
+    uneven mix of 1D/2D problems in mixed FP32 and BF16, and with 7 workgroups for all the tasks."""
+    import os
+    from radiant_hip import problems as P, solver as S
+    probs = [P.small_problem(2, dims=(20 + 9 * i, 6 + 5 * i), G=2 + i % 3, L=2, seed=700 + i) for i in range(6)]
+    probs += [P.small_problem(1, N=30 + 7 * i, G=3, L=3, seed=720 + i) for i in range(3)]
+    for precision, grid in (("mixed-fp32", None), ("mixed-bf16", None), ("mixed-fp32", "7")):
+        ref, _, _ = S.solve_batch(probs, precision=precision)
+        os.environ["RADIANT_HIP_WORKLIST"] = "1"
+        if grid:
+            os.environ["RADIANT_HIP_WORKLIST_GRID"] = grid  # 7 workgroups: many tasks each
+        try:
+            got, st, _ = S.solve_batch(probs, precision=precision)
+        finally:
+            os.environ.pop("RADIANT_HIP_WORKLIST", None)
+            os.environ.pop("RADIANT_HIP_WORKLIST_GRID", None)
+        for i, (a, b) in enumerate(zip(ref, got)):
+            assert np.array_equal(a, b), (precision, i)
+            assert all(s["converged"] for s in st[i])
