@@ -111,7 +111,7 @@ typedef struct rt_problem_desc {
     const double *Mfp;
     /* Spatial closure along x (SN.set_scheme("x", type, order); scheme_weights.jl:25-90,
      * sn_1D_bte.jl:27-64): order_x = 𝒪x spatial Legendre moments per cell (0 or 1: first order, at
-     * most 4) and omega_x[𝒪x + 1] the closure weights ω of the x axis (NULL: diamond difference of
+     * most 8 in 1D) and omega_x[𝒪x + 1] the closure weights ω of the x axis (NULL: diamond difference of
      * order 1).  Any order or closure other than diamond difference of order 1 works in single
      * problems, lockstep batches and served requests, every precision and state; with the
      * slowing-down term (csd = 1) see order_e below.  The flux moments then carry Nm5 = 𝒪x
@@ -119,7 +119,8 @@ typedef struct rt_problem_desc {
      * [Np, Nm5 * Nx]; qvol keeps one value per cell (spatial moment 1, Source.jl:135). */
     int32_t order_x;
     const double *omega_x;
-    /* The y axis of 2D problems likewise (order_y at most 3, omega_y[𝒪y + 1]; with order_x at most 3),
+    /* The y axis of 2D problems likewise (order_y at most 4, omega_y[𝒪y + 1]; with order_x at most 4
+     * and at most 12 moments per cell),
      * and the coupling of the high-order moments (SN.set_is_full_coupling; 0 = full, the default:
      * Nm5 = 𝒪x*𝒪y moments per cell; 1 = 𝒪x + 𝒪y - 1).  The x-face boundary moments then carry 𝒪y
      * moments per position and the y faces 𝒪x (sn_one_speed.jl:92-106: 𝚽x12[Nps, Nm1, 2, Ny], ...),

@@ -146,6 +146,10 @@ static void validate(const rt_problem_desc& d) {
     if (d.order_x > 1 && !d.omega_x) bad("closures of order above 1 need their weights omega_x");
     if (d.ndims >= 2 && d.order_y > 1 && !d.omega_y) bad("closures of order above 1 need their weights omega_y");
     if (d.ndims >= 3 && d.order_z > 1 && !d.omega_z) bad("closures of order above 1 need their weights omega_z");
+    if (d.ndims == 2 && !d.csd &&
+        !sweep_2d_ho_supported(std::max(1, d.order_x), std::max(1, d.order_y), d.partial_coupling == 0))
+        bad("2D closures: orders up to " + std::to_string(HO2_KMAX) + " per axis and at most " +
+            std::to_string(HO2_NM_MAX) + " moments per cell");
     if (d.ndims == 3 && !d.csd &&
         !sweep_3d_ho_supported(std::max(1, d.order_x), std::max(1, d.order_y), std::max(1, d.order_z), d.partial_coupling == 0))
         bad("3D closures: orders up to " + std::to_string(HO3_KMAX) + " per axis and at most " +

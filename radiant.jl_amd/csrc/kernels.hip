@@ -3288,6 +3288,10 @@ static void ho_launch(const PassDesc* d_pd, int Nd, int K, hipStream_t st) {
         case 2: launch_timed(sweep_1d_ho_kernel<T, 2>, dim3(Nd), dim3(64), 0, st, d_pd); break;
         case 3: launch_timed(sweep_1d_ho_kernel<T, 3>, dim3(Nd), dim3(64), 0, st, d_pd); break;
         case 4: launch_timed(sweep_1d_ho_kernel<T, 4>, dim3(Nd), dim3(64), 0, st, d_pd); break;
+        case 5: launch_timed(sweep_1d_ho_kernel<T, 5>, dim3(Nd), dim3(64), 0, st, d_pd); break;
+        case 6: launch_timed(sweep_1d_ho_kernel<T, 6>, dim3(Nd), dim3(64), 0, st, d_pd); break;
+        case 7: launch_timed(sweep_1d_ho_kernel<T, 7>, dim3(Nd), dim3(64), 0, st, d_pd); break;
+        case 8: launch_timed(sweep_1d_ho_kernel<T, 8>, dim3(Nd), dim3(64), 0, st, d_pd); break;
         default: throw std::runtime_error("1D closure order out of range");
     }
 }
@@ -3306,20 +3310,41 @@ static void ho2_launch(const PassDesc* d_pd, int Nd, size_t lds, hipStream_t st)
     launch_timed(fn, dim3(Nd), dim3(256), (uint32_t)lds, st, d_pd);  // with the caller's timing events
 }
 
+bool sweep_2d_ho_supported(int Kx, int Ky, bool fc) {
+    if (Kx < 1 || Ky < 1 || Kx > HO2_KMAX || Ky > HO2_KMAX) return false;
+    return (fc ? Kx * Ky : Kx + Ky - 1) <= HO2_NM_MAX;
+}
+
+template <typename T, int KX, int KY>
+static void ho2_fc(bool fc, const PassDesc* d_pd, int Nd, size_t lds, hipStream_t st) {
+    // partial coupling differs from full coupling only when both orders exceed 1
+    if constexpr (KX > 1 && KY > 1) {
+        if (!fc) return ho2_launch<T, KX, KY, false>(d_pd, Nd, lds, st);
+    }
+    // the cell system lives in registers: full coupling up to HO2_NM_MAX moments
+    if constexpr (KX * KY <= HO2_NM_MAX) ho2_launch<T, KX, KY, true>(d_pd, Nd, lds, st);
+    else throw std::runtime_error("2D closure with more than HO2_NM_MAX moments per cell");
+}
+
+template <typename T, int KX>
+static void ho2_y(int Ky, bool fc, const PassDesc* d_pd, int Nd, size_t lds, hipStream_t st) {
+    switch (Ky) {
+        case 1: return ho2_fc<T, KX, 1>(fc, d_pd, Nd, lds, st);
+        case 2: return ho2_fc<T, KX, 2>(fc, d_pd, Nd, lds, st);
+        case 3: return ho2_fc<T, KX, 3>(fc, d_pd, Nd, lds, st);
+        case 4: return ho2_fc<T, KX, 4>(fc, d_pd, Nd, lds, st);
+        default: throw std::runtime_error("2D closure orders out of range");
+    }
+}
+
 template <typename T>
 static void ho2_dispatch(const PassDesc* d_pd, int Nd, int Kx, int Ky, bool fc, size_t lds, hipStream_t st) {
-    // partial coupling differs from full coupling only when both orders exceed 1
-    const bool full = fc || Kx == 1 || Ky == 1;
-    switch (Kx * 10 + Ky) {
-        case 11: return ho2_launch<T, 1, 1, true>(d_pd, Nd, lds, st);
-        case 12: return ho2_launch<T, 1, 2, true>(d_pd, Nd, lds, st);
-        case 13: return ho2_launch<T, 1, 3, true>(d_pd, Nd, lds, st);
-        case 21: return ho2_launch<T, 2, 1, true>(d_pd, Nd, lds, st);
-        case 31: return ho2_launch<T, 3, 1, true>(d_pd, Nd, lds, st);
-        case 22: return full ? ho2_launch<T, 2, 2, true>(d_pd, Nd, lds, st) : ho2_launch<T, 2, 2, false>(d_pd, Nd, lds, st);
-        case 23: return full ? ho2_launch<T, 2, 3, true>(d_pd, Nd, lds, st) : ho2_launch<T, 2, 3, false>(d_pd, Nd, lds, st);
-        case 32: return full ? ho2_launch<T, 3, 2, true>(d_pd, Nd, lds, st) : ho2_launch<T, 3, 2, false>(d_pd, Nd, lds, st);
-        case 33: return full ? ho2_launch<T, 3, 3, true>(d_pd, Nd, lds, st) : ho2_launch<T, 3, 3, false>(d_pd, Nd, lds, st);
+    if (!sweep_2d_ho_supported(Kx, Ky, fc)) throw std::runtime_error("2D closure orders out of range");
+    switch (Kx) {
+        case 1: return ho2_y<T, 1>(Ky, fc, d_pd, Nd, lds, st);
+        case 2: return ho2_y<T, 2>(Ky, fc, d_pd, Nd, lds, st);
+        case 3: return ho2_y<T, 3>(Ky, fc, d_pd, Nd, lds, st);
+        case 4: return ho2_y<T, 4>(Ky, fc, d_pd, Nd, lds, st);
         default: throw std::runtime_error("2D closure orders out of range");
     }
 }

@@ -173,11 +173,14 @@ void launch_csd_out(const PassDesc* d_pd, long n, hipStream_t st);
 void launch_fp_source(const PassDesc* d_pd, long n, hipStream_t st);
 // 1D sweep of a general closure with K = PassDesc::ho_K (1..HO_KMAX) spatial moments per cell, one
 // problem, one wave per direction
-constexpr int HO_KMAX = 4;
+constexpr int HO_KMAX = 8;
 void launch_sweep_1d_ho(int prec, const PassDesc* d_pd, int Nd, int K, hipStream_t st);
-// 2D sweep of a general closure of orders Kx, Ky (1..HO2_KMAX each), one problem, one workgroup per
-// direction walking the anti-diagonals; lds = face moments of every row and column
-constexpr int HO2_KMAX = 3;
+// 2D sweep of a general closure of orders Kx, Ky (1..HO2_KMAX each, at most HO2_NM_MAX moments per
+// cell), one problem, one workgroup per direction walking the anti-diagonals; lds = face moments of
+// every row and column
+constexpr int HO2_KMAX = 4;
+constexpr int HO2_NM_MAX = 12;
+bool sweep_2d_ho_supported(int Kx, int Ky, bool fc);
 size_t sweep_2d_ho_lds_bytes(int prec, int Nx, int Ny, int Kx, int Ky);
 void launch_sweep_2d_ho(int prec, const PassDesc* d_pd, int Nd, int Kx, int Ky, bool fc, size_t lds, hipStream_t st);
 // 3D sweep of a general closure of orders up to HO3_KMAX per axis and at most HO3_NM_MAX moments per

@@ -44,6 +44,12 @@ def _mix():
     p.scheme, p.scheme_y, p.scheme_z = "DD", "DG", "DD"
     p.scheme_order, p.scheme_order_y, p.scheme_order_z, p.full_coupling = 3, 2, 2, True  # 12 moments
     out.append(p)
+    p = P.small_problem(2, dims=(5, 7), G=2, L=2, tol=1e-12, seed=58, bc=[2, 2, 1, 0])
+    p.scheme, p.scheme_y, p.scheme_order, p.scheme_order_y, p.full_coupling = "DG", "DD", 4, 2, True
+    out.append(p)
+    p = P.small_problem(1, N=7, G=2, L=2, tol=1e-12, seed=59)
+    p.scheme, p.scheme_order = "DD", 7
+    out.append(p)
     return out
 
 

@@ -50,7 +50,7 @@ EXACT = {"DD": 0, "DG": 1, "DG-": 2, "DG+": 2}   # exact up to degree K - EXACT
 
 
 @pytest.mark.parametrize("scheme", ["DD", "DG", "DG-", "DG+"])
-@pytest.mark.parametrize("K", [1, 2, 3, 4])
+@pytest.mark.parametrize("K", [1, 2, 3, 4, 5, 6, 7, 8])
 def test_cell_polynomial_exactness_and_balance(oracle, scheme, K):
     if scheme in ("DG-", "DG+") and K < 2:
         pytest.skip("the modified DG closures start at order 2")
@@ -141,7 +141,8 @@ def _say(*a):
 
 
 GPU_CASES = [("DG", 1, None), ("DD", 2, None), ("DG", 3, None), ("DD", 4, None), ("DG-", 2, None),
-             ("DG+", 3, None), ("DD", 3, [1, 2]), ("DG", 2, [2, 2]), ("DD", 2, None, 300)]
+             ("DG+", 3, None), ("DD", 3, [1, 2]), ("DG", 2, [2, 2]), ("DD", 2, None, 300),
+             ("DD", 6, None), ("DG", 8, [1, 2]), ("DG+", 5, None), ("DG-", 7, [2, 2])]
 
 
 @pytest.mark.gpu

@@ -59,7 +59,7 @@ def _cell_case(oracle, schemes, ox, oy, fc, dxd, dyd, mu, eta, rng):
     return err, abs(bal)
 
 
-ORDERS = [(1, 1), (2, 1), (1, 2), (2, 2), (3, 2), (2, 3), (3, 3)]
+ORDERS = [(1, 1), (2, 1), (1, 2), (2, 2), (3, 2), (2, 3), (3, 3), (4, 1), (1, 4), (4, 3), (4, 4)]
 
 
 @pytest.mark.parametrize("schemes", [("DD", "DD"), ("DG", "DG"), ("DD", "DG")])
@@ -122,7 +122,11 @@ def _say(*a):
 
 GPU_CASES = [(cfg, None) for cfg in CONFIGS] + [(("DD", "DD", 3, 3, True), None),
                                                 (("DD", "DD", 2, 2, True), [1, 2, 0, 1]),
-                                                (("DG", "DG", 2, 3, False), [1, 1, 1, 1])]
+                                                (("DG", "DG", 2, 3, False), [1, 1, 1, 1]),
+                                                # order 4: 12 moments with full coupling, 7 without
+                                                (("DD", "DG", 4, 3, True), [2, 2, 0, 1]),
+                                                (("DG", "DD", 4, 4, False), None),
+                                                (("DD", "DD", 1, 4, True), [1, 1, 1, 1])]
 
 
 @pytest.mark.gpu
@@ -178,3 +182,15 @@ def test_scheme2d_gpu_accelerations_and_batch(oracle):
     phis, _, _ = S.solve_batch([p, p], precision="mixed-fp32")
     for ph in phis:
         assert rel_err(ph, ref) < 1e-6
+
+
+@pytest.mark.gpu
+def test_scheme2d_moment_limit_refused():
+    """Full coupling of orders 4 x 4 gives 16 moments per cell, above the 12 whose cell system the
+    sweep holds in registers: refused at set-up with the limits named (partial coupling of the same
+    orders, 7 moments, is solved above)."""
+    from radiant_hip import problems as P, solver as S
+    from radiant_hip._lib import RTError
+    p = _set(P.small_problem(2, dims=(6, 5), G=2, L=2, seed=99), "DD", "DD", 4, 4, True)
+    with pytest.raises(RTError, match="at most 12 moments per cell"):
+        S.ResidentProblem(p)
