@@ -37,7 +37,7 @@ HBM_PEAK_GBS = 8000.0
 # dispatches), each scaled by what the counter reads for a plain 1 GiB device copy.  Keyed by
 # (workload, kernel class); reported only when the library sources match the measured digest.
 MEASURED_TRAFFIC = {("C2 mixed-fp32", "sweep_fp32"): dict(
-    fetch=201.047e6, write=184.516e6, fetch_scale=2.0, write_scale=1.0, digest="044e330de3b38d2c",
+    fetch=201.119e6, write=184.669e6, fetch_scale=2.0, write_scale=1.0, digest="cb972359b4a45d12",
     name="sweep_scan_kernel<float, 2, 1, 4, 1024, true, false, 2>", summary="profiles/sweep_scan_c2_mixed.md")}
 MFMA_PEAK_TFLOPS = {0: 78.6, 1: 157.3, 2: 2500.0}
 KNAMES = ["mgemm", "sweep", "dgemm", "bndmom"]
