@@ -17,6 +17,16 @@
 
 namespace rt {
 
+// out[g][Kc * c] = in[c][g] for the G x Ncell volume source values (out zero elsewhere)
+__global__ void qvol_layout_kernel(double* __restrict__ out, const double* __restrict__ in, int G, long Ncell, int Kc,
+                                   long Nc) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;  // output order: coalesced writes
+    if (i >= (long)G * Ncell) return;
+    const int g = (int)(i / Ncell);
+    const long c = i - (long)g * Ncell;
+    out[(size_t)g * Nc + (size_t)Kc * c] = in[(size_t)G * c + g];
+}
+
 static inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
 double inner_tol_for(int prec) {
@@ -374,10 +384,27 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
         }
         d_sigw = dalloc<double>(sigw.size()); upload(d_sigw, sigw, st);
         d_sigo = dalloc<double>(sigo.size()); upload(d_sigo, sigo, st);
-        std::vector<double> qv((size_t)G * Nc, 0.0);  // spatial moment 1 of each cell
-        for (int c = 0; c < Ncell; ++c)
-            for (int g = 0; g < G; ++g) qv[(size_t)g * Nc + (size_t)Kc * c] = d.qvol[g + (size_t)G * c];
-        d_qvol = dalloc<double>(qv.size()); upload(d_qvol, qv, st);
+        // volume source: spatial moment 1 of each cell, [g][Kc * cell], rearranged on the device from
+        // the description's [cell][g] (a host transpose of G x Ncell values dominated the set-up of
+        // large requests)
+        d_qvol = dalloc<double>((size_t)G * Nc);
+        {
+            void* raw = dev_alloc((size_t)G * Ncell * sizeof(double));
+            try {
+                RT_CHECK(hipStreamSynchronize(st));
+                RT_CHECK(hipMemcpy(raw, d.qvol, (size_t)G * Ncell * sizeof(double), hipMemcpyHostToDevice));
+                if (Kc > 1) RT_CHECK(hipMemsetAsync(d_qvol, 0, (size_t)G * Nc * sizeof(double), st));
+                const size_t n = (size_t)G * Ncell;
+                hipLaunchKernelGGL(qvol_layout_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d_qvol,
+                                   (const double*)raw, G, (long)Ncell, Kc, (long)Nc);
+                RT_CHECK(hipGetLastError());
+                RT_CHECK(hipStreamSynchronize(st));
+            } catch (...) {
+                dev_free(raw);
+                throw;
+            }
+            dev_free(raw);
+        }
         // ---- angular operators (row-major device copies)
         // k-major device copies: MT[a][n] = M[n, act a], DT[n][p] = D[p, n], DTact[n][a] = D[act a, n]
         std::vector<double> Mact((size_t)Nd * Nact), D((size_t)Np * Nd), Dact((size_t)Nact * Nd);
