@@ -48,6 +48,10 @@ class Library:
     charge_deposition: list               # per particle: [G, Nmat]
     scattering: dict = field(default_factory=dict)  # (i, j) -> [Nmat, G_i, G_j, L+1] from i to j
     title: str = ""
+    # energy and charge deposited at the cut-off (the value at the lowest group boundary) per
+    # particle: [Nmat] each; MATXS files carry them (edep, cdep of length G + 1), FMAC-M files do not
+    energy_deposition_cutoff: list = field(default_factory=list)
+    charge_deposition_cutoff: list = field(default_factory=list)
 
     @property
     def Nmat(self):
