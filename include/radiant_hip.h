@@ -145,6 +145,15 @@ typedef struct rt_problem_desc {
      * Nm4 = 𝒪x*𝒪y*𝒪z moments of each cell (sn_flux.jl:216). */
     int32_t order_e;
     const double *omega_e;
+    /* The adaptive weighted diamond closure (SN scheme "AWD"; scheme_weights.jl:32-37, adaptive.jl):
+     * 1 makes every spatial axis adaptive, the closure weights of each cell then following from its
+     * incoming face fluxes and source moments in every sweep (omega_* are not used).  Orders 1 and 2
+     * in 1D (AWD0, AWD1) and order 1 on every axis in 2D and 3D (AWD0); BTE only (csd = 0),
+     * precision RT_PREC_FP64 and state_mode RT_STATE_FAITHFUL only: the closure is not linear in the
+     * flux, so the mixed-precision solve's sweep-free reconstruction T(z) = A z + T(0) does not hold
+     * for it, and the Krylov iterates depend on the state they run over (the faithful state is the
+     * reference's).  0: off. */
+    int32_t adaptive;
 } rt_problem_desc;
 
 /* Per-group result of the in-group solve (the values sn_one_speed.jl:189-193 prints). */

@@ -1002,10 +1002,10 @@ vector<double> scheme_weights_1d(const string& scheme, int K) {
     const int Mx = K - 1;
     vector<double> w(Mx + 2, 0.0);
     auto pw = [](int e) { return (e % 2 == 0) ? 1.0 : -1.0; };  // (-1)^e, e >= 0
-    if (scheme == "DD") {
+    if (scheme == "DD" || (scheme == "AWD" && Mx == 0)) {
         w[0] = pw(Mx + 1);
         for (int n = 0; n <= Mx; ++n) w[n + 1] = 1.0 - pw(Mx + 1 - n);
-    } else if (scheme == "DG") {
+    } else if (scheme == "DG" || (scheme == "AWD" && Mx == 1)) {
         w[0] = 0.0;
         for (int n = 0; n <= Mx; ++n) w[n + 1] = 1.0;
     } else if (scheme == "DG-" || scheme == "DG+") {
@@ -1045,14 +1045,61 @@ vector<double> lu_solve(vector<double> S, vector<double> b, int K) {
     return x;
 }
 
-// sn_1D_bte.jl:27-64 (non-adaptive): the K spatial Legendre moments of the cell's angular flux
-// and the outgoing face flux.  C[i] = sqrt(2i-1) (scheme_weights.jl:409-410).
-void flux_1d_bte(double mu, double St, double dx, const double* Qn, double& fx, int K, const double* om,
-                 double* psi) {
+// The adaptive weighted diamond closure (scheme "AWD", adaptive.jl): a cell's weights follow from
+// its incoming face fluxes and source moments, cell by cell in every sweep.
+// AWD0, order 1 on n axes (adaptive.jl:36-54, 127-158, 257-297): the diamond estimate
+//   phi_DD = (Q_1 + 2 h_1 f_1 + ... + 2 h_n f_n) / (2 h_1 + ... + 2 h_n + St),
+//   u_a = (f_a - phi_DD) / phi_DD,  P_a = 1 when 3|u_a| <= 1, else 1/(3|u_a|); 0 when negative or NaN;
+//   w_a = [-P_a, 1 + P_a].
+void awd0_weights(int n, const double* h, const double* f, double Q1, double St, double* const* w) {
+    double num = Q1, den = 2.0 * h[0];
+    for (int a = 0; a < n; ++a) num += 2.0 * h[a] * f[a];
+    for (int a = 1; a < n; ++a) den += 2.0 * h[a];
+    const double phi_dd = num / (den + St);
+    for (int a = 0; a < n; ++a) {
+        const double u = (f[a] - phi_dd) / phi_dd;
+        double P = 3.0 * std::fabs(u) <= 1.0 ? 1.0 : 1.0 / (3.0 * std::fabs(u));
+        if (P < 0.0 || std::isnan(P)) P = 0.0;
+        w[a][0] = -P;
+        w[a][1] = 1.0 + P;
+    }
+}
+// AWD1, order 2 in 1D (adaptive.jl:56-73): dQ = sqrt3 sx (Q_2 - sqrt3 hx sx f) / (Q_1 + hx f),
+//   Qx = (3 + dQ)/(3 (3 - dQ)) when dQ > 3/2, -1/dQ when dQ < -1, else 1;  w = [0, 1, Qx].
+void awd1_weights_1d(double hx, double sx, double f, const double* Qn, double* w) {
+    const double r3 = std::sqrt(3.0);
+    const double dQ = r3 * sx * (Qn[1] - r3 * hx * sx * f) / (Qn[0] + hx * f);
+    double Qx;
+    if (dQ > 1.5) Qx = (3.0 + dQ) / (3.0 * (3.0 - dQ));
+    else if (dQ < -1.0) Qx = -1.0 / dQ;
+    else Qx = 1.0;
+    w[0] = 0.0;
+    w[1] = 1.0;
+    w[2] = Qx + 0.0;
+}
+
+// sn_1D_bte.jl:27-64: the K spatial Legendre moments of the cell's angular flux and the outgoing
+// face flux.  C[i] = sqrt(2i-1) (scheme_weights.jl:409-410).  adapt: the AWD weights of the cell
+// replace om (not for a tangential direction, sn_1D_bte.jl:39).
+void flux_1d_bte(double mu, double St, double dx, const double* Qn, double& fx, int K, const double* om_in,
+                 double* psi, bool adapt = false) {
     double sx = mu > 0 ? 1.0 : (mu < 0 ? -1.0 : 0.0);
     const bool tang = std::fabs(mu) <= 1e-10;
     if (tang) { sx = 1.0; mu = 0.0; }
     const double hx = std::fabs(mu) / dx;
+    double oma[3];
+    const double* om = om_in;
+    if (adapt && !tang) {
+        if (K == 1) {
+            double* wp[1] = {oma};
+            awd0_weights(1, &hx, &fx, Qn[0], St, wp);
+        } else if (K == 2) {
+            awd1_weights_1d(hx, sx, fx, Qn, oma);
+        } else {
+            fail("Adaptive scheme is not available for Ox=" + std::to_string(K) + " order flux expansion over the finite-element.");
+        }
+        om = oma;
+    }
     auto C = [](int i) { return std::sqrt(2.0 * i - 1.0); };  // 1-based
     auto spow = [](double s, int e) { double v = 1.0; for (int k = 0; k < e; ++k) v *= s; return v; };
     vector<double> S((size_t)K * K, 0.0), Q(K, 0.0);
@@ -1090,10 +1137,21 @@ struct Cell2 {
     }
 };
 void flux_2d_bte(double mu, double eta, double St, double dx, double dy, const double* Qn, double* fx, double* fy,
-                 const Cell2& cl, const double* wx, const double* wy, double* psi) {
+                 const Cell2& cl, const double* wx_in, const double* wy_in, double* psi, bool adapt = false) {
     const double sx = mu > 0 ? 1.0 : (mu < 0 ? -1.0 : 0.0), sy = eta > 0 ? 1.0 : (eta < 0 ? -1.0 : 0.0);
     const double hx = std::fabs(mu) / dx, hy = std::fabs(eta) / dy;
     const int Ox = cl.Ox, Oy = cl.Oy, Nm = cl.Nm;
+    double wxa[2], wya[2];
+    const double *wx = wx_in, *wy = wy_in;
+    if (adapt) {  // adaptive.jl:121-232: AWD0 for orders (1, 1); the coupled linear AWD1 is not provided
+        if (Ox != 1 || Oy != 1)
+            fail("the adaptive closure in 2D is provided for orders (1, 1) only");
+        const double h[2] = {hx, hy}, f[2] = {fx[0], fy[0]};
+        double* wp[2] = {wxa, wya};
+        awd0_weights(2, h, f, Qn[0], St, wp);
+        wx = wxa;
+        wy = wya;
+    }
     auto C = [](int i) { return std::sqrt(2.0 * i - 1.0); };  // 1-based
     auto spow = [](double s, int e) { double v = 1.0; for (int k = 0; k < e; ++k) v *= s; return v; };
     vector<double> S((size_t)Nm * Nm, 0.0), Q(Nm, 0.0);
@@ -1153,11 +1211,23 @@ struct Cell3 {
     int fz(int a, int b) const { return fc ? O[0] * b + a : a + b + (b > 0 ? O[0] - 1 : 0); }
 };
 void flux_3d_bte(const double* om, const double* h, double St, const double* Qn, double* fx, double* fy, double* fz,
-                 const Cell3& cl, const double* const* w, double* psi) {
+                 const Cell3& cl, const double* const* w_in, double* psi, bool adapt = false) {
     double s[3], hh[3];
     for (int a = 0; a < 3; ++a) {
         s[a] = om[a] > 0 ? 1.0 : (om[a] < 0 ? -1.0 : 0.0);
         hh[a] = std::fabs(om[a]) / h[a];
+    }
+    double wa[3][2];
+    const double* wl[3] = {w_in[0], w_in[1], w_in[2]};
+    const double* const* w = w_in;
+    if (adapt) {  // adaptive.jl:250-300: AWD0 for orders (1, 1, 1) only
+        if (cl.O[0] != 1 || cl.O[1] != 1 || cl.O[2] != 1)
+            fail("the adaptive closure in 3D is provided for orders (1, 1, 1) only");
+        const double f[3] = {fx[0], fy[0], fz[0]};
+        double* wp[3] = {wa[0], wa[1], wa[2]};
+        awd0_weights(3, hh, f, Qn[0], St, wp);
+        for (int a = 0; a < 3; ++a) wl[a] = wa[a];
+        w = wl;
     }
     auto C = [](int i) { return std::sqrt(2.0 * i + 1.0); };  // 0-based
     auto spow = [](double v, int e) { double r = 1.0; for (int k = 0; k < e; ++k) r *= v; return r; };
@@ -1439,6 +1509,7 @@ struct Solver {
     double lam0 = 0.0;          // its λ₀
     int K = 1;                  // spatial moments per cell (1D scheme order; 1 elsewhere)
     bool ho = false;            // the general 1D cell solve (order > 1 or a closure other than DD)
+    bool adapt = false;         // the adaptive weighted diamond closure (scheme "AWD")
     vector<double> omx, omy;    // its ω per axis
     bool ho2 = false;           // the general 2D cell solve
     Cell2 cl2{1, 1, 1, true};
@@ -1548,14 +1619,21 @@ struct Solver {
             for (int m = 0; m < Nmat; ++m)
                 for (int g = 0; g < G; ++g) sigt_eff[g + (size_t)G * m] -= sig_s(m, g, g, 0);
         // spatial closure (SN.get_schemes, scheme_weights.jl)
-        const string scheme = p.scheme ? p.scheme : "DD";
-        const string scheme_y = p.scheme_y ? p.scheme_y : "DD";
+        string scheme = p.scheme ? p.scheme : "DD";
+        string scheme_y = p.scheme_y ? p.scheme_y : "DD";
         K = p.scheme_order > 1 ? p.scheme_order : 1;
         const int Ky = p.scheme_order_y > 1 ? p.scheme_order_y : 1;
         const string scheme_e = p.scheme_e ? p.scheme_e : "DD";
         const int Ke = p.scheme_order_e > 1 ? p.scheme_order_e : 1;
         if (!p.csd && (Ke > 1 || scheme_e != "DD"))
             fail("an energy closure needs the slowing-down term (csd)");
+        {  // scheme_weights.jl:32-37: "AWD" on any axis makes every axis adaptive
+            const string sz = p.scheme_z ? p.scheme_z : "DD";
+            adapt = scheme == "AWD" || (ndims >= 2 && scheme_y == "AWD") || (ndims >= 3 && sz == "AWD") ||
+                    (p.csd && scheme_e == "AWD");
+            if (adapt && p.csd) fail("the adaptive closure of the slowing-down cell is not provided");
+            if (adapt) scheme = scheme_y = "AWD";
+        }
         if (Ke > 2) fail("energy closures of order above 2 are not supported");
         if (p.csd) {  // scheme_weights.jl with isCSD: axes E, x[, y[, z]], DD or DG each
             const string scheme_z = p.scheme_z ? p.scheme_z : "DD";
@@ -1563,7 +1641,7 @@ struct Solver {
             const string sy = ndims >= 2 ? scheme_y : "DD", sz = ndims >= 3 ? scheme_z : "DD";
             const int oy = ndims >= 2 ? Ky : 1, oz = ndims >= 3 ? Kz : 1;
             if (Ke > 1 || K > 1 || oy > 1 || oz > 1 || scheme_e != "DD" || scheme != "DD" || sy != "DD" || sz != "DD") {
-                for (const string* sc : {&scheme_e, &scheme, &sy, &sz})
+                for (const string* sc : std::initializer_list<const string*>{&scheme_e, &scheme, &sy, &sz})
                     if (*sc != "DD" && *sc != "DG") fail("Closure relation not implemented yet.");
                 ome = scheme_weights_1d(scheme_e, Ke);
                 omx = scheme_weights_1d(scheme, K);
@@ -1579,11 +1657,13 @@ struct Solver {
                 bfp = true;
             }
         } else if (ndims == 1 && (K > 1 || scheme != "DD")) {
+            if (adapt && K > 2) fail("the adaptive closure in 1D is provided for orders 1 and 2 only");
             omx = scheme_weights_1d(scheme, K);
             ho = true;
         } else if (ndims == 2 && (K > 1 || Ky > 1 || scheme != "DD" || scheme_y != "DD")) {
             for (const string* sc : {&scheme, &scheme_y})
-                if (*sc != "DD" && *sc != "DG") fail("Closure relation not implemented yet.");
+                if (*sc != "DD" && *sc != "DG" && *sc != "AWD") fail("Closure relation not implemented yet.");
+            if (adapt && (K != 1 || Ky != 1)) fail("the adaptive closure in 2D is provided for orders (1, 1) only");
             omx = scheme_weights_1d(scheme, K);
             omy = scheme_weights_1d(scheme_y, Ky);
             const bool fc = p.partial_coupling == 0;
@@ -1593,11 +1673,13 @@ struct Solver {
             K = cl2.Nm;
             ho2 = true;
         } else if (ndims == 3) {
-            const string scheme_z = p.scheme_z ? p.scheme_z : "DD";
+            const string scheme_z = adapt ? string("AWD") : string(p.scheme_z ? p.scheme_z : "DD");
             const int Kz = p.scheme_order_z > 1 ? p.scheme_order_z : 1;
             if (K > 1 || Ky > 1 || Kz > 1 || scheme != "DD" || scheme_y != "DD" || scheme_z != "DD") {
-                for (const string* sc : {&scheme, &scheme_y, &scheme_z})
-                    if (*sc != "DD" && *sc != "DG") fail("Closure relation not implemented yet.");
+                for (const string* sc : std::initializer_list<const string*>{&scheme, &scheme_y, &scheme_z})
+                    if (*sc != "DD" && *sc != "DG" && *sc != "AWD") fail("Closure relation not implemented yet.");
+                if (adapt && (K != 1 || Ky != 1 || Kz != 1))
+                    fail("the adaptive closure in 3D is provided for orders (1, 1, 1) only");
                 omx = scheme_weights_1d(scheme, K);
                 omy = scheme_weights_1d(scheme_y, Ky);
                 omz = scheme_weights_1d(scheme_z, Kz);
@@ -1659,7 +1741,7 @@ struct Solver {
                     for (int p = 0; p < Np; ++p) q += A.B.M[n + (size_t)A.Nd * p] * Ql[p + (size_t)Np * (is + (size_t)K * ix)];
                     Qn[is] = q;
                 }
-                flux_1d_bte(mu, sig_t(g, matc[ix]), P.dx[ix], Qn.data(), fx, K, omx.data(), psi.data());
+                flux_1d_bte(mu, sig_t(g, matc[ix]), P.dx[ix], Qn.data(), fx, K, omx.data(), psi.data(), adapt);
                 for (int is = 0; is < K; ++is)
                     for (int p = 0; p < Np; ++p)
                         phi[p + (size_t)Np * (is + (size_t)K * ix)] += A.B.D[p + (size_t)Np * n] * psi[is];
@@ -1731,7 +1813,7 @@ struct Solver {
                     Qn[is] = q;
                 }
                 flux_2d_bte(mu, eta, sig_t(g, matc[c]), P.dx[ix], P.dy[iy], Qn.data(), fx, fym.data(), cl2, omx.data(),
-                            omy.data(), psi.data());
+                            omy.data(), psi.data(), adapt);
                 for (int is = 0; is < K; ++is)
                     for (int p = 0; p < Np; ++p)
                         phi[p + (size_t)Np * (is + (size_t)K * c)] += A.B.D[p + (size_t)Np * n] * psi[is];
@@ -1803,7 +1885,7 @@ struct Solver {
                         Qn[is] = q;
                     }
                     const double h[3] = {P.dx[ix], P.dy[iy], P.dz[iz]};
-                    flux_3d_bte(om, h, sig_t(g, matc[c]), Qn.data(), fx, fy, FZ.data(), cl3, w, psi.data());
+                    flux_3d_bte(om, h, sig_t(g, matc[c]), Qn.data(), fx, fy, FZ.data(), cl3, w, psi.data(), adapt);
                     for (int is = 0; is < K; ++is)
                         for (int p = 0; p < Np; ++p)
                             phi[p + (size_t)Np * (is + (size_t)K * c)] += A.B.D[p + (size_t)Np * n] * psi[is];
@@ -2562,7 +2644,7 @@ int oracle_cell_1d(const char* scheme, int K, double mu, double St, double dx, c
                    double* psi) {
     try {
         vector<double> om = scheme_weights_1d(scheme, K);
-        flux_1d_bte(mu, St, dx, Qn, *fx, K, om.data(), psi);
+        flux_1d_bte(mu, St, dx, Qn, *fx, K, om.data(), psi, std::string(scheme) == "AWD");
         return 0;
     } catch (std::exception& e) { g_err = e.what(); return -1; }
 }
@@ -2575,7 +2657,8 @@ int oracle_cell_2d(const char* sx, const char* sy, int Ox, int Oy, int partial, 
         vector<double> wx = scheme_weights_1d(sx, Ox), wy = scheme_weights_1d(sy, Oy);
         const bool fc = partial == 0;
         Cell2 cl{Ox, Oy, fc ? Ox * Oy : Ox + Oy - 1, fc};
-        flux_2d_bte(mu, eta, St, dx, dy, Qn, fx, fy, cl, wx.data(), wy.data(), psi);
+        const bool ad = std::string(sx) == "AWD" || std::string(sy) == "AWD";
+        flux_2d_bte(mu, eta, St, dx, dy, Qn, fx, fy, cl, wx.data(), wy.data(), psi, ad);
         return 0;
     } catch (std::exception& e) { g_err = e.what(); return -1; }
 }
@@ -2589,7 +2672,8 @@ int oracle_cell_3d(const char* sx, const char* sy, const char* sz, int Ox, int O
         vector<double> wx = scheme_weights_1d(sx, Ox), wy = scheme_weights_1d(sy, Oy), wz = scheme_weights_1d(sz, Oz);
         const double* w[3] = {wx.data(), wy.data(), wz.data()};
         Cell3 cl(Ox, Oy, Oz, partial == 0);
-        flux_3d_bte(om, h, St, Qn, fx, fy, fz, cl, w, psi);
+        const bool ad = std::string(sx) == "AWD" || std::string(sy) == "AWD" || std::string(sz) == "AWD";
+        flux_3d_bte(om, h, St, Qn, fx, fy, fz, cl, w, psi, ad);
         return 0;
     } catch (std::exception& e) { g_err = e.what(); return -1; }
 }

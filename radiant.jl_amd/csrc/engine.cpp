@@ -5,6 +5,7 @@
 // resident in HBM and the transport pass of sn_inner_pass.jl:212-332 done by the kernels of
 // kernels.hip.
 #include "engine.hpp"
+#include "sweep_awd.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -64,7 +65,8 @@ static int ho_order(const rt_problem_desc& d) {
     const int Kz = d.ndims >= 3 && d.order_z > 1 ? d.order_z : 1;
     const int Ke = d.csd && d.order_e > 1 ? d.order_e : 1;
     auto dd1 = [](const double* w) { return !w || (w[0] == -1.0 && w[1] == 2.0); };
-    if (Kx == 1 && Ky == 1 && Kz == 1 && Ke == 1 && dd1(d.omega_x) && (d.ndims < 2 || dd1(d.omega_y)) &&
+    // an adaptive closure walks its cells in order, at order 1 too (sweep_awd.hip)
+    if (!d.adaptive && Kx == 1 && Ky == 1 && Kz == 1 && Ke == 1 && dd1(d.omega_x) && (d.ndims < 2 || dd1(d.omega_y)) &&
         (d.ndims < 3 || dd1(d.omega_z)) && (!d.csd || dd1(d.omega_e)))
         return 0;
     if (d.csd) return bfp_cell_moments(Ke, Kx, Ky, Kz, d.partial_coupling == 0);  // moments over (E, x, y, z)
@@ -165,6 +167,19 @@ static void validate(const rt_problem_desc& d) {
         bad("3D closures: orders up to " + std::to_string(HO3_KMAX) + " per axis and at most " +
             std::to_string(HO3_NM_MAX) + " moments per cell");
     if (!d.csd && d.order_e > 1) bad("an energy closure needs the slowing-down term (csd)");
+    if (d.adaptive) {  // scheme "AWD" (adaptive.jl)
+        if (d.adaptive != 1) bad("adaptive must be 0 or 1");
+        if (d.csd) bad("the adaptive closure (AWD) of the slowing-down cell is not provided");
+        if (d.precision != RT_PREC_FP64)
+            bad("the adaptive closure (AWD) is not linear in the flux: it runs in the FP64 solve, whose "
+                "reconstruction is a full pass");
+        if (d.state_mode != RT_STATE_FAITHFUL)
+            bad("the adaptive closure (AWD) runs in the faithful state: for a pass that is not linear the "
+                "Krylov iterates depend on the state they run over");
+        if (!sweep_awd_supported(d.ndims, std::max(1, d.order_x), d.ndims >= 2 ? std::max(1, d.order_y) : 1,
+                                 d.ndims >= 3 ? std::max(1, d.order_z) : 1))
+            bad("the adaptive closure (AWD): orders 1 and 2 in 1D, order 1 on every axis in 2D and 3D");
+    }
     if (d.csd && ho_order(d)) {
         const int Ke = d.order_e > 1 ? d.order_e : 1;
         if (Ke > 1 && !d.omega_e) bad("energy closures of order above 1 need their weights omega_e");
@@ -201,6 +216,7 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
     ndims = d.ndims; Nx = d.Nx; Ny = d.Ny; Nz = d.Nz; Ncell = Nx * Ny * Nz;
     ho_K = ho_order(d);
     bfp = d.csd && ho_K;
+    adapt = d.adaptive != 0;
     if (bfp) {
         ho_KE = d.order_e > 1 ? d.order_e : 1;
         ho_Kx = d.order_x > 1 ? d.order_x : 1;
@@ -745,7 +761,10 @@ void Problem::launch_pass(PassDesc& pd, int prec, bool full_rows) {
     if (pd.psiE_in && !bfp) launch_csd_source(d_pd, (long)Nd * Nc, st);  // energy inflow (FP64 passes)
     timed(1, [&] {
         const int sp = prec == PBF16 ? P32 : prec;
-        if (bfp)
+        if (adapt) {
+            if (sp != P64) throw Error(RT_EINVAL, "the adaptive closure runs FP64 passes only");
+            launch_sweep_awd(d_pd, Nd, ndims, ndims == 1 ? ho_K : 1, ho_lds(P64), st);
+        } else if (bfp)
             launch_sweep_bfp(sp, d_pd, Nd, ho_KE, ho_Kx, ho_Ky, ho_Kz, ho_fc, ho_lds(sp), st);
         else if (ho_K && ndims == 3)
             launch_sweep_3d_ho(sp, d_pd, Nd, ho_Kx, ho_Ky, ho_Kz, ho_fc, ho_lds(sp), st);

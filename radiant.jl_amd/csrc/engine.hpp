@@ -101,6 +101,7 @@ public:
     // difference of order 1): energy order, moments of the energy face; the cell moments ho_K
     // then span (E, x, y, z)
     bool bfp = false;
+    bool adapt = false;  // adaptive weighted diamond closure (sweep_awd.hip), FP64 passes only
     int ho_KE = 1, fm_e = 1;
     // the 3D and slowing-down walkers keep their face moments in device memory when they exceed
     // the LDS (d_hoface, [Nd][faces] doubles); dynamic LDS of their launch in precision sp

@@ -1,0 +1,278 @@
+// sweep_awd.hip -- sweeps with the adaptive weighted diamond closure (SN scheme "AWD";
+// particle_transport/adaptive.jl, scheme_weights.jl:32-37), FP64, for gfx950.
+//
+// The closure weights of a cell follow from its incoming face fluxes and source moments (Alcouffe's
+// adaptive weighted diamond), recomputed cell by cell in every pass.  The outgoing face flux is then
+// no longer an affine map of the incoming one, so a sweep walks its cells in order:
+// - 1D (sn_sweep_1D.jl:51-124 with flux_1D_BTE, sn_1D_bte.jl:27-64): one lane per direction walks
+//   its row of cells; orders 1 (AWD0, adaptive.jl:36-54) and 2 (AWD1, adaptive.jl:56-73).  A
+//   direction with |mu| <= 1e-10 does not stream: its face flux passes through unchanged and the
+//   weights are not adapted (sn_1D_bte.jl:39).
+// - 2D and 3D (sn_sweep_2D.jl:54-200 / sn_sweep_3D.jl:57-240 with flux_2D_BTE / flux_3D_BTE): one
+//   workgroup per direction walks the diagonals kx + ky (+ kz) = d in sweep order, one cell per
+//   thread, with the face fluxes of every pencil in LDS (3D grids whose faces exceed it: in the
+//   direction's slice of hoface); order 1 (AWD0, adaptive.jl:127-158 and 257-297).
+// AWD0 on n axes: the diamond estimate
+//   phi_DD = (Q + 2 h_1 f_1 + ... + 2 h_n f_n) / (2 h_1 + ... + 2 h_n + St),  h_a = |Omega_a| / Delta_a,
+//   u_a = (f_a - phi_DD) / phi_DD,  P_a = 1 when 3|u_a| <= 1, else 1/(3|u_a|), and 0 when negative or NaN,
+// gives the weights w_a = (-P_a, 1 + P_a) of the order-1 cell
+//   psi = (Q - sum_a h_a (w_a0 - 1) f_a) / (St + sum_a h_a w_a1),   f_a <- w_a0 f_a + w_a1 psi.
+// AWD1 (1D, order 2): dQ = sqrt3 s (Q_1 - sqrt3 h s f) / (Q_0 + h f),  Qx = (3 + dQ) / (3 (3 - dQ)) for
+// dQ > 3/2, -1/dQ for dQ < -1, else 1; the weights (0, 1, Qx) enter the 2 x 2 cell of sn_1D_bte.jl
+//   S_ij = St d_ij - [i > j] C_i h s C_j (1 - (-1)^(i-j)) + C_i h s^i C_j s^j w_(j+1),
+//   Q_j = Qn_j - C_j h (s^j w_0 - (-s)^j) f,  f <- w_0 f + sum_j C_j s^j w_(j+1) psi_j   (C_j = sqrt(2j+1)).
+// Source and flux moments: psi[n][k + K * cell], in place (source in, flux out); incoming face fluxes
+// from bin_x/bin_y/bin_z ([Nd][pos], zero when null), exit faces into bx/by/bz.
+#include "sweep_awd.hpp"
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
+#include <cmath>
+#include <stdexcept>
+#include <string>
+
+namespace rt {
+
+#define RT_HIP(x)                                                                                 \
+    do {                                                                                          \
+        hipError_t e_ = (x);                                                                      \
+        if (e_ != hipSuccess) throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e_) + \
+                                                       " at " __FILE__ ":" + std::to_string(__LINE__));   \
+    } while (0)
+
+// The AWD0 weights of n axes: w[a][0] = -P_a, w[a][1] = 1 + P_a (adaptive.jl:36-54, 127-158, 257-297).
+template <int N>
+__device__ __forceinline__ void awd0(const double* h, const double* f, double Q, double St, double (*w)[2]) {
+    double num = Q, den = 2.0 * h[0];
+#pragma unroll
+    for (int a = 0; a < N; ++a) num += 2.0 * h[a] * f[a];
+#pragma unroll
+    for (int a = 1; a < N; ++a) den += 2.0 * h[a];
+    const double phi_dd = num / (den + St);
+#pragma unroll
+    for (int a = 0; a < N; ++a) {
+        const double u = (f[a] - phi_dd) / phi_dd;
+        double P = 3.0 * fabs(u) <= 1.0 ? 1.0 : 1.0 / (3.0 * fabs(u));
+        if (P < 0.0 || isnan(P)) P = 0.0;
+        w[a][0] = -P;
+        w[a][1] = 1.0 + P;
+    }
+}
+
+// ------------------------------------------------------------------------------------------- 1D
+template <int K>
+__global__ __launch_bounds__(64) void sweep_1d_awd_kernel(const PassDesc* __restrict__ pds) {
+    const PassDesc& d = pds[0];
+    const int n = blockIdx.x * 64 + threadIdx.x;
+    if (n >= d.Nd) return;
+    const int Nx = d.Nx, Nc = d.Nc;
+    double mu = d.mu[n];
+    const bool fwd = mu >= 0.0;
+    const bool tang = fabs(mu) <= 1e-10;
+    const double sx = (tang || mu > 0.0) ? 1.0 : -1.0;
+    if (tang) mu = 0.0;
+    double om0[K + 1], Cc[K], sp[K], sn[K];  // base weights, C_j, sx^j, (-sx)^j
+#pragma unroll
+    for (int i = 0; i <= K; ++i) om0[i] = d.ho_om[i];
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+        Cc[i] = sqrt(2.0 * i + 1.0);
+        sp[i] = i == 0 ? 1.0 : sp[i - 1] * sx;
+        sn[i] = i == 0 ? 1.0 : -sn[i - 1] * sx;
+    }
+    double* __restrict__ psi = (double*)d.psi + (size_t)n * Nc;
+    double f = d.bin_x ? d.bin_x[n] : 0.0;
+    for (int k = 0; k < Nx; ++k) {
+        const int ix = fwd ? k : Nx - 1 - k;
+        const double hx = fabs(mu) / d.dx[ix];
+        const double St = d.sigt[d.mat[(size_t)K * ix]];
+        double Qn[K], om[K + 1];
+#pragma unroll
+        for (int j = 0; j < K; ++j) Qn[j] = psi[j + (size_t)K * ix];
+#pragma unroll
+        for (int j = 0; j <= K; ++j) om[j] = om0[j];
+        if (!tang) {
+            if constexpr (K == 1) {
+                double w[1][2];
+                awd0<1>(&hx, &f, Qn[0], St, w);
+                om[0] = w[0][0];
+                om[1] = w[0][1];
+            } else {
+                const double r3 = sqrt(3.0);
+                const double dQ = r3 * sx * (Qn[1] - r3 * hx * sx * f) / (Qn[0] + hx * f);
+                double Qx;
+                if (dQ > 1.5) Qx = (3.0 + dQ) / (3.0 * (3.0 - dQ));
+                else if (dQ < -1.0) Qx = -1.0 / dQ;
+                else Qx = 1.0;
+                om[0] = 0.0;
+                om[1] = 1.0;
+                om[2] = Qx + 0.0;
+            }
+        }
+        double S[K][K], q[K];
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+                double e = i == j ? St : 0.0;
+                if (i > j && ((i - j) & 1)) e -= Cc[i] * hx * sx * Cc[j] * 2.0;
+                e += Cc[i] * hx * sp[i] * Cc[j] * sp[j] * om[j + 1];
+                S[i][j] = e;
+            }
+            q[i] = Qn[i] - Cc[i] * hx * (sp[i] * om[0] - sn[i]) * f;
+        }
+        if constexpr (K == 2) {  // partial pivoting, then elimination
+            if (fabs(S[1][0]) > fabs(S[0][0])) {
+                for (int j = 0; j < 2; ++j) { const double t = S[0][j]; S[0][j] = S[1][j]; S[1][j] = t; }
+                const double t = q[0]; q[0] = q[1]; q[1] = t;
+            }
+            const double fct = S[1][0] / S[0][0];
+            S[1][1] -= fct * S[0][1];
+            q[1] -= fct * q[0];
+            q[1] = q[1] / S[1][1];
+            q[0] = (q[0] - S[0][1] * q[1]) / S[0][0];
+        } else {
+            q[0] = q[0] / S[0][0];
+        }
+#pragma unroll
+        for (int j = 0; j < K; ++j) psi[j + (size_t)K * ix] = q[j];
+        if (!tang) {
+            double fo = om[0] * f;
+#pragma unroll
+            for (int j = 0; j < K; ++j) fo += Cc[j] * sp[j] * om[j + 1] * q[j];
+            f = fo;
+        }
+    }
+    if (d.bx) ((double*)d.bx)[n] = f;  // exit face flux (one position in 1D)
+}
+
+// ------------------------------------------------------------------------------------------- 2D
+__global__ __launch_bounds__(256) void sweep_2d_awd_kernel(const PassDesc* __restrict__ pds) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const PassDesc& d = pds[0];
+    const int n = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    const int Nx = d.Nx, Ny = d.Ny;
+    double* FX = (double*)smem;  // [Ny]
+    double* FY = FX + Ny;        // [Nx]
+    const double mu = d.mu[n], eta = d.eta[n];
+    const bool fxs = mu >= 0, fys = eta >= 0;
+    double* __restrict__ psi = (double*)d.psi + (size_t)n * d.Nc;
+    for (int i = tid; i < Ny; i += nt) FX[i] = d.bin_x ? d.bin_x[(size_t)n * Ny + i] : 0.0;
+    for (int i = tid; i < Nx; i += nt) FY[i] = d.bin_y ? d.bin_y[(size_t)n * Nx + i] : 0.0;
+    __syncthreads();
+    for (int dg = 0; dg < Nx + Ny - 1; ++dg) {
+        const int k0 = dg - (Ny - 1) > 0 ? dg - (Ny - 1) : 0, k1 = dg < Nx - 1 ? dg : Nx - 1;
+        for (int kx = k0 + tid; kx <= k1; kx += nt) {
+            const int ky = dg - kx;
+            const int ix = fxs ? kx : Nx - 1 - kx, iy = fys ? ky : Ny - 1 - ky;
+            const size_t c = (size_t)ix + (size_t)Nx * iy;
+            const double h[2] = {fabs(mu) / d.dx[ix], fabs(eta) / d.dy[iy]};
+            const double St = d.sigt[d.mat[c]];
+            const double f[2] = {FX[iy], FY[ix]};
+            const double Q = psi[c];
+            double w[2][2];
+            awd0<2>(h, f, Q, St, w);
+            const double S = St + h[0] * w[0][1] + h[1] * w[1][1];
+            const double q = Q - h[0] * (w[0][0] - 1.0) * f[0] - h[1] * (w[1][0] - 1.0) * f[1];
+            const double v = q / S;
+            psi[c] = v;
+            FX[iy] = w[0][0] * f[0] + w[0][1] * v;
+            FY[ix] = w[1][0] * f[1] + w[1][1] * v;
+        }
+        __syncthreads();
+    }
+    if (d.bx)
+        for (int i = tid; i < Ny; i += nt) ((double*)d.bx)[(size_t)n * Ny + i] = FX[i];
+    if (d.by)
+        for (int i = tid; i < Nx; i += nt) ((double*)d.by)[(size_t)n * Nx + i] = FY[i];
+}
+
+// ------------------------------------------------------------------------------------------- 3D
+__global__ __launch_bounds__(256) void sweep_3d_awd_kernel(const PassDesc* __restrict__ pds) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const PassDesc& d = pds[0];
+    const int n = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    const int Nx = d.Nx, Ny = d.Ny, Nz = d.Nz;
+    // face fluxes in LDS, or in this direction's slice of hoface (the waves of the workgroup share
+    // one CU, so __syncthreads orders their device-memory writes and reads too)
+    const size_t nface = (size_t)Ny * Nz + (size_t)Nx * Nz + (size_t)Nx * Ny;
+    double* FX = d.hoface ? (double*)d.hoface + (size_t)n * nface : (double*)smem;  // [Ny*Nz]
+    double* FY = FX + (size_t)Ny * Nz;                                             // [Nx*Nz]
+    double* FZ = FY + (size_t)Nx * Nz;                                             // [Nx*Ny]
+    const double om[3] = {d.mu[n], d.eta[n], d.xi[n]};
+    const bool fw[3] = {om[0] >= 0, om[1] >= 0, om[2] >= 0};
+    double* __restrict__ psi = (double*)d.psi + (size_t)n * d.Nc;
+    for (int i = tid; i < Ny * Nz; i += nt) FX[i] = d.bin_x ? d.bin_x[(size_t)n * Ny * Nz + i] : 0.0;
+    for (int i = tid; i < Nx * Nz; i += nt) FY[i] = d.bin_y ? d.bin_y[(size_t)n * Nx * Nz + i] : 0.0;
+    for (int i = tid; i < Nx * Ny; i += nt) FZ[i] = d.bin_z ? d.bin_z[(size_t)n * Nx * Ny + i] : 0.0;
+    __syncthreads();
+    for (int dg = 0; dg < Nx + Ny + Nz - 2; ++dg) {
+        for (int i = tid; i < Nx * Ny; i += nt) {
+            const int kx = i % Nx, ky = i / Nx, kz = dg - kx - ky;
+            if (kz < 0 || kz >= Nz) continue;
+            const int ix = fw[0] ? kx : Nx - 1 - kx, iy = fw[1] ? ky : Ny - 1 - ky, iz = fw[2] ? kz : Nz - 1 - kz;
+            const size_t c = (size_t)ix + (size_t)Nx * (iy + (size_t)Ny * iz);
+            const double h[3] = {fabs(om[0]) / d.dx[ix], fabs(om[1]) / d.dy[iy], fabs(om[2]) / d.dz[iz]};
+            const double St = d.sigt[d.mat[c]];
+            double* fx = FX + ((size_t)iy + (size_t)Ny * iz);
+            double* fy = FY + ((size_t)ix + (size_t)Nx * iz);
+            double* fz = FZ + ((size_t)ix + (size_t)Nx * iy);
+            const double f[3] = {*fx, *fy, *fz};
+            const double Q = psi[c];
+            double w[3][2];
+            awd0<3>(h, f, Q, St, w);
+            const double S = St + h[0] * w[0][1] + h[1] * w[1][1] + h[2] * w[2][1];
+            const double q = Q - h[0] * (w[0][0] - 1.0) * f[0] - h[1] * (w[1][0] - 1.0) * f[1] -
+                             h[2] * (w[2][0] - 1.0) * f[2];
+            const double v = q / S;
+            psi[c] = v;
+            *fx = w[0][0] * f[0] + w[0][1] * v;
+            *fy = w[1][0] * f[1] + w[1][1] * v;
+            *fz = w[2][0] * f[2] + w[2][1] * v;
+        }
+        __syncthreads();
+    }
+    if (d.bx)
+        for (int i = tid; i < Ny * Nz; i += nt) ((double*)d.bx)[(size_t)n * Ny * Nz + i] = FX[i];
+    if (d.by)
+        for (int i = tid; i < Nx * Nz; i += nt) ((double*)d.by)[(size_t)n * Nx * Nz + i] = FY[i];
+    if (d.bz)
+        for (int i = tid; i < Nx * Ny; i += nt) ((double*)d.bz)[(size_t)n * Nx * Ny + i] = FZ[i];
+}
+
+template <typename F>
+static void awd_launch(F fn, dim3 grid, dim3 block, size_t lds, hipStream_t st, const PassDesc* d_pd) {
+    hipEvent_t a, b;
+    take_launch_events(&a, &b);
+    if (a) hipExtLaunchKernelGGL(fn, grid, block, (uint32_t)lds, st, a, b, 0u, d_pd);
+    else hipLaunchKernelGGL(fn, grid, block, (uint32_t)lds, st, d_pd);
+}
+
+bool sweep_awd_supported(int ndims, int Kx, int Ky, int Kz) {
+    if (ndims == 1) return Kx == 1 || Kx == 2;
+    if (ndims == 2) return Kx == 1 && Ky == 1;
+    return ndims == 3 && Kx == 1 && Ky == 1 && Kz == 1;
+}
+
+void launch_sweep_awd(const PassDesc* d_pd, int Nd, int ndims, int K, size_t lds, hipStream_t st) {
+    if (Nd <= 0) return;
+    if (lds > 160 * 1024) throw std::runtime_error("adaptive-closure face fluxes exceed the LDS");
+    if (ndims == 1) {
+        const dim3 grid((unsigned)((Nd + 63) / 64));
+        if (K == 1) awd_launch(sweep_1d_awd_kernel<1>, grid, dim3(64), 0, st, d_pd);
+        else if (K == 2) awd_launch(sweep_1d_awd_kernel<2>, grid, dim3(64), 0, st, d_pd);
+        else throw std::runtime_error("1D adaptive closure of order other than 1 or 2");
+    } else {
+        if (K != 1) throw std::runtime_error("2D/3D adaptive closure of order other than 1");
+        auto fn = ndims == 2 ? sweep_2d_awd_kernel : sweep_3d_awd_kernel;
+        static bool attr[2] = {false, false};
+        if (!attr[ndims - 2]) {
+            RT_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            attr[ndims - 2] = true;
+        }
+        awd_launch(fn, dim3(Nd), dim3(256), lds, st, d_pd);
+    }
+    RT_HIP(hipGetLastError());
+}
+
+}  // namespace rt

@@ -17,6 +17,10 @@ SCHEMES = ("DD", "DG", "DG-", "DG+")
 def weights_1d(scheme, K):
     """ω [K + 1] of the x axis."""
     scheme = scheme.upper()
+    if scheme == "AWD":  # the adaptive closure's base weights (scheme_weights.jl:47,52)
+        if K > 2:
+            raise ValueError("Closure relation not implemented yet.")
+        scheme = "DD" if K == 1 else "DG"
     if scheme not in SCHEMES:
         raise ValueError("Closure relation not implemented yet.")
     Mx = K - 1

@@ -74,14 +74,18 @@ class Desc:
         K = schemes.spatial_moments(prob)
         scheme = (getattr(prob, "scheme", None) or "DD").upper()
         ox = max(1, int(getattr(prob, "scheme_order", 1) or 1))
+        axes = [scheme] + [(getattr(prob, nm, None) or "DD").upper() for nm in ("scheme_y", "scheme_z")][:prob.ndims - 1]
+        if "AWD" in axes:  # scheme_weights.jl:32-37: one adaptive axis makes every axis adaptive
+            d.adaptive = 1
+            scheme = "AWD"
         if prob.ndims >= 2:
-            scheme_y = (getattr(prob, "scheme_y", None) or "DD").upper()
+            scheme_y = "AWD" if d.adaptive else (getattr(prob, "scheme_y", None) or "DD").upper()
             oy = max(1, int(getattr(prob, "scheme_order_y", 1) or 1))
-            scheme_z = (getattr(prob, "scheme_z", None) or "DD").upper() if prob.ndims == 3 else "DD"
+            scheme_z = ("AWD" if d.adaptive else (getattr(prob, "scheme_z", None) or "DD").upper()) if prob.ndims == 3 else "DD"
             oz = max(1, int(getattr(prob, "scheme_order_z", 1) or 1)) if prob.ndims == 3 else 1
             if ox > 1 or oy > 1 or oz > 1 or scheme != "DD" or scheme_y != "DD" or scheme_z != "DD":
                 for sc in (scheme, scheme_y, scheme_z):
-                    if sc not in ("DD", "DG"):
+                    if sc not in ("DD", "DG", "AWD"):
                         raise ValueError("Closure relation not implemented yet.")
                 k["omega_x"] = np.ascontiguousarray(schemes.weights_1d(scheme, ox))
                 k["omega_y"] = np.ascontiguousarray(schemes.weights_1d(scheme_y, oy))

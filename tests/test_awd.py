@@ -1,0 +1,266 @@
+"""The adaptive weighted diamond closure, scheme "AWD" (particle_transport/adaptive.jl,
+scheme_weights.jl:32-37): AWD0 (order 1) in 1D, 2D and 3D, AWD1 (order 2) in 1D.
+
+Oracle pins (CPU):
+- an AWD0 cell equals the weighted diamond difference written in its balance form (Alcouffe 1993):
+  the balance sum_a h_a (out_a - in_a) + St psi = Q with the auxiliary relations
+  psi = (out_a + P_a in_a) / (1 + P_a), solved directly here, with P_a from the published rule
+  (P = 1 for 3|u| <= 1, else 1/(3|u|), u = (in - phi_DD)/phi_DD, phi_DD the diamond estimate);
+  in 1D, 2D and 3D, on both sweep directions, including the P = 1 (diamond) and the P = 0 (no
+  estimate) branches;
+- the AWD0 outgoing flux stays non-negative on optically thick cells where diamond difference turns
+  negative;
+- an AWD1 cell equals the DG cell of order 2 while -1 <= dQ <= 3/2, and is continuous in the source
+  slope across both ends of that range;
+- the solve: all-reflective uniform media give the infinite-medium closed form (u = 0, so every cell
+  keeps the diamond weights), equal to the DD solve; AWD is positively homogeneous (twice the source
+  gives twice the flux) although not linear; 2D/3D orders above 1 and the slowing-down cell are
+  refused as the reference refuses them or does not provide them.
+GPU: passes (1e-12) and FP64 solves in the faithful state (1e-10, iteration counts equal) against
+the oracle in 1D (orders 1, 2), 2D and 3D, with void, reflective and periodic faces and GMRES /
+Anderson / BiCGStab; mixed precision and the reduced state are refused."""
+import sys
+
+import numpy as np
+import pytest
+
+from util import rel_err
+
+
+def _O():
+    from oracle import oracle as O
+    return O
+
+
+def _P_rule(fin, phi_dd):
+    u = (fin - phi_dd) / phi_dd if phi_dd != 0 else np.inf
+    if not np.isfinite(u):
+        return 0.0
+    return 1.0 if 3 * abs(u) <= 1 else 1.0 / (3 * abs(u))
+
+
+def _wdd_cell(h, fins, Q, St):
+    """The weighted diamond cell in balance form with the AWD weights: (psi, outgoing fluxes)."""
+    phi_dd = (Q + sum(2 * hh * f for hh, f in zip(h, fins))) / (sum(2 * hh for hh in h) + St)
+    P = [_P_rule(f, phi_dd) for f in fins]
+    # out_a = (1 + P_a) psi - P_a in_a into the balance
+    psi = (Q + sum(hh * (1 + p) * f for hh, p, f in zip(h, P, fins))) / (St + sum(hh * (1 + p) for hh, p in zip(h, P)))
+    return psi, [(1 + p) * psi - p * f for p, f in zip(P, fins)], P
+
+
+CASES = [  # (|omega| per axis, widths, St, Q, inflows)
+    ((0.5,), (1.0,), 1.0, 0.2, (1.0,)),        # P = 1/2
+    ((0.9,), (0.3,), 0.4, 1.0, (1.0,)),        # thin, smooth: P = 1 (diamond)
+    ((0.2,), (2.0,), 5.0, 0.0, (1.0,)),        # thick, no source
+    ((0.7,), (1.0,), 1.0, 0.0, (0.0,)),        # nothing enters: phi_DD = 0, P = 0
+    ((0.3, 0.8), (1.0, 0.5), 2.0, 0.5, (1.0, 0.1)),
+    ((0.6, 0.6), (1.0, 1.0), 0.5, 1.0, (1.1, 0.9)),
+    ((0.2, 0.5, 0.84), (1.0, 2.0, 0.5), 3.0, 0.3, (2.0, 0.0, 0.5)),
+    ((0.57, 0.57, 0.59), (1.0, 1.0, 1.0), 1.0, 1.0, (0.8, 0.9, 1.0)),
+]
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+@pytest.mark.parametrize("sign", [1.0, -1.0])
+def test_awd0_cell_is_the_weighted_diamond(case, sign):
+    O = _O()
+    om, dx, St, Q, fins = CASES[case]
+    om = [sign * o for o in om]
+    h = [abs(o) / d for o, d in zip(om, dx)]
+    psi_ref, out_ref, P = _wdd_cell(h, fins, Q, St)
+    n = len(om)
+    if n == 1:
+        psi, fx = O.cell_1d("AWD", 1, om[0], St, dx[0], [Q], fins[0])
+        outs = [fx]
+    elif n == 2:
+        psi, fx, fy = O.cell_2d(("AWD", "AWD"), (1, 1), True, om[0], om[1], St, dx[0], dx[1], [Q], [fins[0]], [fins[1]])
+        outs = [fx[0], fy[0]]
+    else:
+        psi, fx, fy, fz = O.cell_3d(("AWD",) * 3, (1, 1, 1), True, om, dx, St, [Q], [fins[0]], [fins[1]], [fins[2]])
+        outs = [fx[0], fy[0], fz[0]]
+    assert abs(psi[0] - psi_ref) <= 1e-14 * max(1.0, abs(psi_ref))
+    assert np.allclose(outs, out_ref, rtol=1e-13, atol=1e-15)
+    if case == 1:
+        assert P == [1.0]     # the diamond branch
+        dd, fdd = O.cell_1d("DD", 1, om[0], St, dx[0], [Q], fins[0])
+        assert abs(psi[0] - dd[0]) < 1e-15 and abs(outs[0] - fdd) < 1e-15
+    if case == 3:
+        assert P == [0.0]     # no estimate: the step closure, out = psi
+
+
+def test_awd0_positive_on_thick_cells():
+    O = _O()
+    for tau in (2.5, 5.0, 20.0, 100.0):
+        for q in (0.0, 0.1):
+            dd, fdd = O.cell_1d("DD", 1, 0.5, tau, 1.0, [q], 1.0)
+            aw, faw = O.cell_1d("AWD", 1, 0.5, tau, 1.0, [q], 1.0)
+            if q == 0.0:
+                assert fdd < 0.0   # diamond difference overshoots below zero for tau > 2 |mu|/dx... here 1
+            assert faw >= 0.0 and aw[0] >= 0.0
+
+
+def _dq(mu, dx, Qn, f):
+    hx, sx = abs(mu) / dx, np.sign(mu)
+    return np.sqrt(3) * sx * (Qn[1] - np.sqrt(3) * hx * sx * f) / (Qn[0] + hx * f)
+
+
+@pytest.mark.parametrize("mu", [0.6, -0.35])
+def test_awd1_cell(mu):
+    O = _O()
+    dx, St, f = 1.0, 1.3, 0.8
+    sx, hx = np.sign(mu), abs(mu) / dx
+    q_for = lambda dq: dq * (1.0 + hx * f) / (np.sqrt(3) * sx) + np.sqrt(3) * hx * sx * f  # Qn[1] at Qn[0] = 1
+    # inside -1 <= dQ <= 3/2 the cell is the DG cell of order 2
+    for dq in (-0.9, 0.0, 1.0, 1.4):
+        Qn = [1.0, q_for(dq)]
+        assert -1 <= _dq(mu, dx, Qn, f) <= 1.5
+        a, fa = O.cell_1d("AWD", 2, mu, St, dx, Qn, f)
+        g, fg = O.cell_1d("DG", 2, mu, St, dx, Qn, f)
+        assert np.array_equal(a, g) and fa == fg
+    # continuity in the source slope across dQ = 3/2 and dQ = -1, and a change of the cell beyond
+    for target in (1.5, -1.0):
+        q_at = q_for(target)
+        lo = O.cell_1d("AWD", 2, mu, St, dx, [1.0, q_at - 1e-9], f)
+        hi = O.cell_1d("AWD", 2, mu, St, dx, [1.0, q_at + 1e-9], f)
+        assert np.allclose(lo[0], hi[0], atol=1e-7) and abs(lo[1] - hi[1]) < 1e-7
+        far = q_for(2.5 if target > 0 else -3.0)
+        a, fa = O.cell_1d("AWD", 2, mu, St, dx, [1.0, far], f)
+        g, fg = O.cell_1d("DG", 2, mu, St, dx, [1.0, far], f)
+        assert not np.allclose(a, g)
+
+
+def _awd(prob, **kw):
+    prob.scheme = "AWD"
+    for k, v in kw.items():
+        setattr(prob, k, v)
+    return prob
+
+
+@pytest.mark.parametrize("accel", ["anderson", "livolant"])
+def test_awd_uniform_reflective_medium_is_the_closed_form(accel):
+    """All-reflective faces, one material and a uniform source: the fixed point of the pass is the
+    uniform flux, where u = 0 in every cell and AWD keeps the diamond weights; the flux is the
+    infinite-medium value.  (The fixed-point accelerators converge to it; GMRES and BiCGStab treat
+    the pass as affine, z - A z = T(0) with A from homogeneous passes, which AWD is not, so the
+    Krylov solve of sn_one_speed.jl:157-172 ends elsewhere, here as in the reference.)"""
+    O = _O()
+    from radiant_hip import problems as P
+    for ndims in (1, 2, 3):
+        p = P.small_problem(ndims, N=4, G=2, L=1, Nmat=1, tol=1e-13, bc=[1] * (2 * ndims), seed=3, accel=accel)
+        p.qvol[...] = 0.0
+        p.qvol[0, :] = 1.0
+        ref, _ = O.solve(p)
+        q = _awd(P.small_problem(ndims, N=4, G=2, L=1, Nmat=1, tol=1e-13, bc=[1] * (2 * ndims), seed=3, accel=accel))
+        q.qvol = p.qvol
+        phi, st = O.solve(q)
+        assert all(s["converged"] for s in st)
+        assert rel_err(phi, ref) < 1e-11
+        s0 = p.sigt[0, 0] - p.sigs[0, 0, 0, 0]
+        assert np.allclose(phi[0, 0], 1.0 / s0, rtol=1e-10)
+
+
+def test_awd_solve_is_homogeneous_not_linear():
+    O = _O()
+    from radiant_hip import problems as P
+    for ndims in (1, 2, 3):
+        p = _awd(P.small_problem(ndims, N=5, G=2, L=1, tol=1e-13, seed=9))
+        phi, _ = O.solve(p)
+        q = _awd(P.small_problem(ndims, N=5, G=2, L=1, tol=1e-13, seed=9))
+        q.qvol = 2.0 * p.qvol
+        phi2, _ = O.solve(q)
+        assert rel_err(phi2, 2.0 * phi) < 1e-10
+        r = P.small_problem(ndims, N=5, G=2, L=1, tol=1e-13, seed=9)
+        dd, _ = O.solve(r)
+        assert rel_err(phi, dd) > 1e-4   # the vacuum faces make the weights differ from the diamond
+
+
+def test_awd_refusals():
+    O = _O()
+    from radiant_hip import problems as P
+    p = _awd(P.small_problem(2, N=4, G=1, L=1), scheme_order=2, scheme_order_y=2)
+    with pytest.raises(RuntimeError, match="orders \\(1, 1\\) only"):
+        O.solve(p)
+    p = _awd(P.small_problem(3, N=3, G=1, L=1), scheme_order_z=2)
+    with pytest.raises(RuntimeError, match="orders \\(1, 1, 1\\) only"):
+        O.solve(p)
+    p = _awd(P.small_problem(1, N=4, G=1, L=1), scheme_order=3)
+    with pytest.raises(RuntimeError):
+        O.solve(p)
+    p = P.small_problem(2, N=4, G=1, L=1)
+    p.scheme_y = "AWD"               # one adaptive axis makes both adaptive
+    q = _awd(P.small_problem(2, N=4, G=1, L=1))
+    assert rel_err(O.solve(p)[0], O.solve(q)[0]) == 0.0
+
+
+# ------------------------------------------------------------------------------------- GPU
+# AWD1 runs with the fixed-point accelerators: under GMRES or BiCGStab, which take the pass as affine,
+# these problems do not converge in the oracle (1000 iterations, growing residual), as in the
+# reference's algorithm, so they make no parity case.
+GPU_CASES = [  # (ndims, order, bc, accel)
+    (1, 1, None, "gmres"), (1, 2, None, "anderson"), (1, 1, [1, 0], "anderson"), (1, 2, [2, 2], "livolant"),
+    (1, 2, [1, 0], "anderson"),
+    (2, 1, None, "gmres"), (2, 1, [1, 0, 0, 1], "gmres"), (2, 1, None, "anderson"),
+    (3, 1, None, "gmres"), (3, 1, [1, 0, 2, 2, 0, 1], "bicgstab"),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", GPU_CASES, ids=[f"{c[0]}d-o{c[1]}-{c[3]}-{'bc' if c[2] else 'void'}" for c in GPU_CASES])
+def test_gpu_awd_solve(oracle, case):
+    from radiant_hip import problems as P, solver as S
+    ndims, order, bc, accel = case
+    p = _awd(P.small_problem(ndims, N={1: 24, 2: 10, 3: 6}[ndims], G=2, L=1, tol=1e-12, seed=70 + ndims, accel=accel,
+                             bc=bc), scheme_order=order)
+    ref, rst = oracle.solve(p)
+    phi, st = S.compute_flux(p)
+    e = rel_err(phi, ref)
+    print("awd solve", case, e, flush=True)
+    sys.stdout.flush()
+    assert e < 1e-10 and all(s["converged"] for s in st)
+    assert [s["niter"] for s in st] == [s["niter"] for s in rst]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("ndims", [1, 2, 3])
+def test_gpu_awd_pass(oracle, ndims):
+    """T(z), A z and z - A z of one group on a positive state and source (the weights depend on
+    them cell by cell), one reflective face."""
+    from radiant_hip import problems as P, solver as S
+    p = _awd(P.small_problem(ndims, N={1: 40, 2: 9, 3: 5}[ndims], G=2, L=1, seed=80 + ndims,
+                             bc=[1] + [0] * (2 * ndims - 1)))
+    h = S.ResidentProblem(p)
+    try:
+        nd, np_, nps, Sl = oracle.sizes(p)
+        assert Sl == h.state_length()
+        rng = np.random.default_rng(ndims)
+        z = rng.uniform(0.0, 1.0, Sl)
+        q = np.zeros(np_ * p.Nc)
+        for k in range(np_):
+            if h.d.ang.pl[k] <= p.Lxs:
+                q[k::np_] = rng.uniform(0.0, 1.0, p.Nc)
+        for mode in (0, 1, 2):
+            ref = oracle.transport_pass(p, 1, 0 if mode == 0 else 1, z, q)
+            if mode == 2:
+                ref = z - ref
+            got = h.transport_pass(1, mode, q, z)
+            e = rel_err(got, ref)
+            print("awd pass", ndims, mode, e, flush=True)
+            assert e < 1e-12
+    finally:
+        h.close()
+
+
+@pytest.mark.gpu
+def test_gpu_awd_refusals():
+    """Mixed precision (its reconstruction T(z) = A z + T(0) assumes a linear pass), the reduced
+    state (Krylov iterates over another state) and the orders the reference does not adapt."""
+    from radiant_hip import problems as P, solver as S
+    p = _awd(P.small_problem(2, N=6, G=1, L=1))
+    with pytest.raises(Exception, match="adaptive"):
+        S.compute_flux(p, precision="mixed-fp32")
+    with pytest.raises(Exception, match="faithful state"):
+        S.compute_flux(p, state="reduced")
+    with pytest.raises(Exception, match="adaptive"):
+        S.compute_flux(_awd(P.small_problem(2, N=6, G=1, L=1), scheme_order=2))
+    with pytest.raises(ValueError, match="Closure relation"):  # scheme_weights.jl has no AWD weights of order 3
+        S.compute_flux(_awd(P.small_problem(1, N=6, G=1, L=1), scheme_order=3))

@@ -85,8 +85,11 @@ def test_host_weights():
             assert sch.weights_1d("DG+", K)[K] == 1001
     with pytest.raises(ValueError):
         sch.weights_1d("DG-", 1)
+    # the adaptive closure's base weights: those of DD at order 1 and DG at order 2 (scheme_weights.jl:47,52)
+    assert np.array_equal(sch.weights_1d("AWD", 1), sch.weights_1d("DD", 1))
+    assert np.array_equal(sch.weights_1d("AWD", 2), sch.weights_1d("DG", 2))
     with pytest.raises(ValueError):
-        sch.weights_1d("AWD", 2)
+        sch.weights_1d("AWD", 3)
 
 
 def _slab(scheme, K, bc=None, seed=41, G=3, N=9):
