@@ -83,6 +83,8 @@ def lib():
                                      C.c_double, C.c_double, C.c_double, dp, dp, dp, dp]
         L.oracle_cell_bfp.argtypes = [C.c_char_p] * 4 + [C.c_int] * 6 + [dp, dp, C.c_double, C.c_double, C.c_double,
                                                                        dp, dp, dp, dp, dp, dp]
+        L.oracle_voronoi_neighbours.argtypes = [C.c_int, dp, dp, dp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                                C.c_int]
         L.oracle_legendre_triple.argtypes = [C.c_int, C.c_int, C.c_int]
         L.oracle_legendre_triple.restype = C.c_double
         _lib = L
@@ -223,6 +225,24 @@ def make_problem(prob, nthreads=1, verbose=0, **over):
     op.scheme_order_e = int(getattr(prob, "scheme_order_e", 1) or 1)
     op._keep = k
     return op
+
+
+def voronoi_neighbours(P):
+    """The oracle's voronoi_sphere: per node of P [Nd, 3], its neighbours in polygon order."""
+    P = np.ascontiguousarray(P, np.float64)
+    Nd = len(P)
+    x, y, z = (np.ascontiguousarray(P[:, k]) for k in range(3))
+    cnt = np.zeros(Nd, np.int32)
+    cap = 16 * Nd
+    nbr = np.zeros(cap, np.int32)
+    ip = C.POINTER(C.c_int32)
+    r = lib().oracle_voronoi_neighbours(Nd, _p(x), _p(y), _p(z), cnt.ctypes.data_as(ip), nbr.ctypes.data_as(ip), cap)
+    _check(r)
+    out, k = [], 0
+    for n in range(Nd):
+        out.append([int(v) for v in nbr[k:k + cnt[n]]])
+        k += cnt[n]
+    return out
 
 
 def fokker_planck(prob):

@@ -29,6 +29,8 @@
 #include <vector>
 #include <algorithm>
 #include <limits>
+#include <map>
+#include <array>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -972,9 +974,275 @@ vector<double> fp_dq_1d(const vector<double>& mu, double& lam0) {
     return Mn_dir;
 }
 
+// ---------------------------------------------------------------------------------------
+// Finite-difference Fokker-Planck operator on the Voronoi tessellation of the unit sphere, for the
+// sets other than gauss-legendre-chebychev in 2D/3D (tools/voronoi.jl; fokker_planck_finite_difference.jl
+// :139-179 for Qdims = 2, :255-275 for Qdims = 3).
+// ---------------------------------------------------------------------------------------
+
+// voronoi_sphere (voronoi.jl:13-205): for each node n, the neighbour across each edge of its Voronoi
+// cell, in polygon order.  The cell's vertices come from every triplet (n, j, k): Newton on
+// |x|^2 = 1 and the bisector planes of (n, j) and (n, k) from x = node n (tolerance 1e-7, 100 steps,
+// skipped when the Jacobian is singular or the iteration does not settle); vertices that agree on a
+// 1e-7 grid are merged with the union of their node sets; a vertex stays when n is its nearest node
+// (within 1e-7); the polygon is ordered from its first vertex, each next one the first unused vertex
+// sharing exactly one node other than n with the previous.  The neighbour across the edge between
+// consecutive vertices is the first other node whose polygon holds both.
+vector<vector<int>> voronoi_neighbours(const vector<double>& X, const vector<double>& Y, const vector<double>& Z) {
+    const int Nd = (int)X.size();
+    vector<vector<vector<int>>> index(Nd);
+    for (int n = 0; n < Nd; ++n) {
+        vector<std::array<double, 3>> vtx;
+        vector<vector<int>> pix;
+        const double x1 = X[n], y1 = Y[n], z1 = Z[n];
+        for (int j = 0; j < Nd; ++j) {
+            if (j == n) continue;
+            const double x2 = X[j], y2 = Y[j], z2 = Z[j];
+            const double a1 = x1 - x2, b1 = y1 - y2, c1 = z1 - z2;
+            const double d1 = 0.5 * (x1 * x1 - x2 * x2 + y1 * y1 - y2 * y2 + z1 * z1 - z2 * z2);
+            for (int k = 0; k < Nd; ++k) {
+                if (k == j || k == n) continue;
+                const double x3 = X[k], y3 = Y[k], z3 = Z[k];
+                const double a2 = x1 - x3, b2 = y1 - y3, c2 = z1 - z3;
+                const double d2 = 0.5 * (x1 * x1 - x3 * x3 + y1 * y1 - y3 * y3 + z1 * z1 - z3 * z3);
+                auto F = [&](const double* x, double* f) {
+                    f[0] = x[0] * x[0] + x[1] * x[1] + x[2] * x[2] - 1.0;
+                    f[1] = a1 * x[0] + b1 * x[1] + c1 * x[2] - d1;
+                    f[2] = a2 * x[0] + b2 * x[1] + c2 * x[2] - d2;
+                };
+                double xm[3] = {x1, y1, z1}, x[3] = {x1, y1, z1};
+                double eps = std::numeric_limits<double>::infinity();
+                int N = 1;
+                bool isinv = true;
+                while (eps > 1e-7 && N < 100) {
+                    const double J[3][3] = {{2 * xm[0], 2 * xm[1], 2 * xm[2]}, {a1, b1, c1}, {a2, b2, c2}};
+                    const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                                       J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                                       J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+                    if (det == 0.0) { isinv = false; break; }
+                    double Ji[3][3];  // adjugate / det
+                    for (int r = 0; r < 3; ++r)
+                        for (int c = 0; c < 3; ++c) {
+                            const int r1 = (c + 1) % 3, r2 = (c + 2) % 3, c1_ = (r + 1) % 3, c2_ = (r + 2) % 3;
+                            Ji[r][c] = (J[r1][c1_] * J[r2][c2_] - J[r1][c2_] * J[r2][c1_]) / det;
+                        }
+                    double f[3];
+                    F(x, f);
+                    for (int r = 0; r < 3; ++r) x[r] = xm[r] - (Ji[r][0] * f[0] + Ji[r][1] * f[1] + Ji[r][2] * f[2]);
+                    F(x, f);
+                    eps = std::max(std::fabs(f[0]), std::max(std::fabs(f[1]), std::fabs(f[2])));
+                    for (int r = 0; r < 3; ++r) xm[r] = x[r];
+                    ++N;
+                }
+                if (N < 100 && isinv) {
+                    vtx.push_back({x[0], x[1], x[2]});
+                    pix.push_back({n, j, k});
+                }
+            }
+        }
+        // merge duplicates on the 1e-7 grid (round half to even, as Julia's round)
+        vector<std::array<double, 3>> v2;
+        vector<vector<int>> p2;
+        std::map<std::array<long long, 3>, int> seen;
+        for (size_t i = 0; i < vtx.size(); ++i) {
+            const std::array<long long, 3> key = {(long long)std::nearbyint(vtx[i][0] * 1e7),
+                                                  (long long)std::nearbyint(vtx[i][1] * 1e7),
+                                                  (long long)std::nearbyint(vtx[i][2] * 1e7)};
+            auto it = seen.find(key);
+            if (it != seen.end()) {
+                vector<int>& u = p2[it->second];
+                for (int q : pix[i])
+                    if (std::find(u.begin(), u.end(), q) == u.end()) u.push_back(q);
+            } else {
+                v2.push_back(vtx[i]);
+                p2.push_back(pix[i]);
+                seen[key] = (int)v2.size() - 1;
+            }
+        }
+        // the vertices whose nearest node is n
+        vector<vector<int>> p3;
+        for (size_t i = 0; i < v2.size(); ++i) {
+            double dn = 0.0, dmin = std::numeric_limits<double>::infinity();
+            for (int q = 0; q < Nd; ++q) {
+                const double dx = X[q] - v2[i][0], dy = Y[q] - v2[i][1], dz = Z[q] - v2[i][2];
+                const double dq = std::sqrt(dx * dx + dy * dy + dz * dz);
+                if (q == n) dn = dq;
+                dmin = std::min(dmin, dq);
+            }
+            if (std::fabs(dn - dmin) < 1e-7) {
+                vector<int> srt = p2[i];
+                std::sort(srt.begin(), srt.end());
+                p3.push_back(srt);
+            }
+        }
+        // polygon order
+        vector<vector<int>> ord;
+        if (!p3.empty()) ord.push_back(p3[0]);
+        auto others = [&](const vector<int>& v) {
+            vector<int> o;
+            for (int q : v)
+                if (q != n) o.push_back(q);
+            return o;
+        };
+        for (size_t i = 1; i < p3.size(); ++i) {
+            if (ord.size() < i) break;
+            const vector<int> a = others(ord[i - 1]);
+            for (size_t j = 0; j < p3.size(); ++j) {
+                const vector<int> b = others(p3[j]);
+                int common = 0;
+                for (int q : a)
+                    if (std::find(b.begin(), b.end(), q) != b.end()) ++common;
+                if (common == 1 && std::find(ord.begin(), ord.end(), p3[j]) == ord.end()) {
+                    ord.push_back(p3[j]);
+                    break;
+                }
+            }
+        }
+        index[n] = ord;
+    }
+    vector<vector<int>> nb(Nd);
+    for (int n = 0; n < Nd; ++n) {
+        const int M = (int)index[n].size();
+        for (int m = 0; m < M; ++m) {
+            const vector<int>& i2 = index[n][m == 0 ? M - 1 : m - 1];
+            const vector<int>& i3 = index[n][m];
+            int found = -1;
+            for (int i = 0; i < Nd && found < 0; ++i) {
+                if (i == n) continue;
+                const auto& L = index[i];
+                if (std::find(L.begin(), L.end(), i2) != L.end() && std::find(L.begin(), L.end(), i3) != L.end())
+                    found = i;
+            }
+            if (found < 0) fail("Voronoi cell edge without a neighbour");
+            nb[n].push_back(found);
+        }
+    }
+    return nb;
+}
+
+// Least-squares solution of G x = q (G: m x n column-major, full column rank) by Householder QR: the
+// pinv(G) q of voronoi.jl, whose check ||pinv(G) G - I|| <= 1e-7 (full column rank) is made here on R.
+vector<double> least_squares(vector<double> G, vector<double> q, int m, int n) {
+    double rmax = 0.0;
+    vector<double> diag(n);
+    for (int k = 0; k < n; ++k) {
+        double nrm = 0.0;
+        for (int i = k; i < m; ++i) nrm += G[i + (size_t)m * k] * G[i + (size_t)m * k];
+        nrm = std::sqrt(nrm);
+        const double a = G[k + (size_t)m * k] > 0 ? -nrm : nrm;
+        diag[k] = a;
+        rmax = std::max(rmax, std::fabs(a));
+        if (nrm == 0.0) continue;
+        vector<double> v(m - k);
+        for (int i = k; i < m; ++i) v[i - k] = G[i + (size_t)m * k];
+        v[0] -= a;
+        double vv = 0.0;
+        for (double t : v) vv += t * t;
+        if (vv == 0.0) continue;
+        for (int c = k; c < n; ++c) {
+            double s = 0.0;
+            for (int i = k; i < m; ++i) s += v[i - k] * G[i + (size_t)m * c];
+            s = 2.0 * s / vv;
+            for (int i = k; i < m; ++i) G[i + (size_t)m * c] -= s * v[i - k];
+        }
+        double s = 0.0;
+        for (int i = k; i < m; ++i) s += v[i - k] * q[i];
+        s = 2.0 * s / vv;
+        for (int i = k; i < m; ++i) q[i] -= s * v[i - k];
+    }
+    for (int k = 0; k < n; ++k)
+        if (!(std::fabs(G[k + (size_t)m * k]) > 1e-10 * rmax))
+            fail("The equation system to find γ-parameters does not have a solution.");
+    vector<double> x(n);
+    for (int k = n - 1; k >= 0; --k) {
+        double v = q[k];
+        for (int c = k + 1; c < n; ++c) v -= G[k + (size_t)m * c] * x[c];
+        x[k] = v / G[k + (size_t)m * k];
+    }
+    return x;
+}
+
+// fokker_planck_weights_3D / _2D (voronoi.jl:207-353) and the operator of
+// fokker_planck_finite_difference.jl:139-179, 255-275: one unknown γ per Voronoi edge (n, m), fixed by
+// sum_m γ_nm (Ω_k[m] - Ω_k[n]) = -2 w_n Ω_k[n] for every node and axis k (the first moments are
+// eigenfunctions of eigenvalue -2); ℳ[n][m] = γ_nm / w_n off the diagonal, rows summing to zero, then
+// shifted by λ₀ = max |ℳ[n][n]|.  Qdims = 2 takes the edges of the full-sphere set (X3, Y3, Z3) between
+// nodes of the upper hemisphere, mapped onto the folded set (mu, eta, xi, w).
+vector<double> fp_fd_voronoi(int Qdims, const vector<double>& mu, const vector<double>& eta, const vector<double>& xi,
+                             const vector<double>& w, const vector<double>& X3, const vector<double>& Y3,
+                             const vector<double>& Z3, double& lam0) {
+    const int Nd = (int)w.size();
+    const vector<vector<int>> nb3 = voronoi_neighbours(X3, Y3, Z3);
+    const int N3 = (int)X3.size();
+    vector<int> to2(N3, -1), to3(Nd, -1);
+    if (Qdims == 3) {
+        if (N3 != Nd) fail("Fokker-Planck operator does not match the quadrature");
+        for (int n = 0; n < Nd; ++n) to2[n] = to3[n] = n;
+    } else {
+        for (int n3 = 0; n3 < N3; ++n3) {
+            int best = 0;
+            double bv = 0.0;
+            for (int j = 0; j < Nd; ++j) {
+                const double sz = Z3[n3] > 0 ? 1.0 : (Z3[n3] < 0 ? -1.0 : 0.0);
+                const double e = std::pow(std::fabs(X3[n3] - mu[j]), 2) + std::pow(std::fabs(Y3[n3] - eta[j]), 2) +
+                                 std::pow(std::fabs(sz * Z3[n3] - xi[j]), 2);
+                if (j == 0 || e < bv) { bv = e; best = j; }
+            }
+            to2[n3] = best;
+            if (Z3[n3] >= 0) to3[best] = n3;
+        }
+        for (int n = 0; n < Nd; ++n)
+            if (to3[n] < 0) fail("Fokker-Planck operator does not match the quadrature");
+    }
+    auto upper = [&](int n3) { return Qdims == 3 || Z3[n3] >= 0; };
+    // edges in order of discovery
+    vector<std::pair<int, int>> edges;
+    std::map<std::pair<int, int>, int> eid;
+    for (int n3 = 0; n3 < N3; ++n3) {
+        if (!upper(n3)) continue;
+        for (int m3 : nb3[n3]) {
+            if (!upper(m3)) continue;
+            const std::pair<int, int> e = {std::min(to2[n3], to2[m3]), std::max(to2[n3], to2[m3])};
+            if (!eid.count(e)) { eid[e] = (int)edges.size(); edges.push_back(e); }
+        }
+    }
+    const int Ne = (int)edges.size(), Neq = 3 * Nd;
+    const double* Om[3] = {mu.data(), eta.data(), xi.data()};
+    vector<double> G((size_t)Neq * Ne, 0.0), Q(Neq, 0.0);
+    for (int n = 0; n < Nd; ++n)
+        for (int k = 0; k < 3; ++k) {
+            const int eq = 3 * n + k;
+            for (int m3 : nb3[to3[n]]) {
+                if (!upper(m3)) continue;
+                const int m = to2[m3];
+                if (m == n) continue;
+                const int e = eid.at({std::min(n, m), std::max(n, m)});
+                G[eq + (size_t)Neq * e] -= Om[k][n];
+                G[eq + (size_t)Neq * e] += Om[k][m];
+                Q[eq] = -2.0 * w[n] * Om[k][n];
+            }
+        }
+    const vector<double> gam = least_squares(G, Q, Neq, Ne);
+    vector<double> Mn_dir((size_t)Nd * Nd, 0.0);
+    for (int n = 0; n < Nd; ++n)
+        for (int m3 : nb3[to3[n]]) {
+            if (!upper(m3)) continue;
+            const int m = to2[m3];
+            if (m == n) continue;
+            const double g = gam[eid.at({std::min(n, m), std::max(n, m)})];
+            Mn_dir[n + (size_t)Nd * m] += g / w[n];
+            Mn_dir[n + (size_t)Nd * n] -= g / w[n];
+        }
+    lam0 = 0.0;
+    for (int d = 0; d < Nd; ++d) lam0 = std::max(lam0, std::fabs(Mn_dir[d + (size_t)Nd * d]));
+    for (int d = 0; d < Nd; ++d) Mn_dir[d + (size_t)Nd * d] += lam0;
+    return Mn_dir;
+}
+
 // fokker_planck_scattering_matrix.jl:29-40 -- ℳ (Np x Np column-major) and λ₀ for one method.
 vector<double> fokker_planck_matrix(const string& method, int N, const string& qt, int Qdims,
-                                    const vector<double>& mu, const vector<double>& w, const Basis& B,
+                                    const vector<double>& mu, const vector<double>& eta, const vector<double>& xi,
+                                    const vector<double>& w, const Basis& B, const vector<double>* sphere3,
                                     double& lam0) {
     const int Nd = (int)w.size();
     if (method == "galerkin") return fp_galerkin(B, Nd, lam0);
@@ -982,7 +1250,7 @@ vector<double> fokker_planck_matrix(const string& method, int N, const string& q
     if (method == "finite-difference") {
         if (Qdims == 1) Mn_dir = fp_fd_1d(mu, w, lam0);
         else if (qt == "gauss-legendre-chebychev") Mn_dir = fp_fd_glc(N, Qdims, lam0);
-        else fail("finite-difference Fokker-Planck on the " + qt + " quadrature (Voronoi weights) is not supported");
+        else Mn_dir = fp_fd_voronoi(Qdims, mu, eta, xi, w, sphere3[0], sphere3[1], sphere3[2], lam0);
         if ((int)(Mn_dir.size()) != Nd * Nd) fail("Fokker-Planck operator does not match the quadrature");
     } else if (method == "differential-quadrature") {
         if (Qdims != 1) fail("Differential quadrature unavailable in multidimensional geometry.");
@@ -1612,7 +1880,16 @@ struct Solver {
         if (solver == 2 || solver == 4) {
             if (!p.T) fail("solver types BFP and FP need the momentum transfer");
             const string meth = p.fokker_planck ? p.fokker_planck : "finite-difference";
-            Mfp = fokker_planck_matrix(meth, p.quad_order, qt, Qdims, A.mu, A.w, A.B, lam0);
+            // the full-sphere set of the same rule (the Voronoi construction of 2D sets starts there)
+            vector<double> S3[4];
+            if (p.quad_n > 0) {
+                S3[0].assign(p.quad_mu, p.quad_mu + p.quad_n);
+                S3[1].assign(p.quad_eta, p.quad_eta + p.quad_n);
+                S3[2].assign(p.quad_xi, p.quad_xi + p.quad_n);
+            } else if (Qdims >= 2 && meth == "finite-difference" && qt != "gauss-legendre-chebychev") {
+                quadrature(p.quad_order, qt, 3, 3, S3[0], S3[1], S3[2], S3[3]);
+            }
+            Mfp = fokker_planck_matrix(meth, p.quad_order, qt, Qdims, A.mu, A.eta, A.xi, A.w, A.B, S3, lam0);
             for (size_t i = 0; i < sigt_eff.size(); ++i) sigt_eff[i] += p.T[i] * lam0;
         }
         if (solver == 6)  // elastic-free: the self-scattering moment l = 0 leaves the removal
@@ -2627,6 +2904,24 @@ int oracle_legendre(int L, double x, int half_range, double* P) {
 
 // Fokker-Planck operator of a problem (types 2 and 4): ℳ [Np x Np] column-major and λ₀;
 // returns Np.
+// The Voronoi neighbours of every node of a set on the unit sphere (voronoi_sphere): nbr receives
+// the neighbour lists one after another, cnt[n] their lengths; returns the total (-1 on error or
+// when it exceeds cap).
+int oracle_voronoi_neighbours(int Nd, const double* x, const double* y, const double* z, int* cnt, int* nbr, int cap) {
+    try {
+        const auto nb = voronoi_neighbours(vector<double>(x, x + Nd), vector<double>(y, y + Nd), vector<double>(z, z + Nd));
+        int tot = 0;
+        for (int n = 0; n < Nd; ++n) {
+            cnt[n] = (int)nb[n].size();
+            for (int m : nb[n]) {
+                if (tot >= cap) fail("capacity too small");
+                nbr[tot++] = m;
+            }
+        }
+        return tot;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
 int oracle_fokker_planck(const oracle_problem* p, double* Mfp, double* lam0, int capNp) {
     try {
         Solver s(*p);

@@ -13,8 +13,9 @@ operator ``ℳ_n`` shifted by ``λ₀`` has a non-negative diagonal.  Discretisa
   for the gauss-legendre-chebychev product set in 2D/3D the same polar operator plus a three-point
   azimuthal operator per level, periodic over the full circle (Qdims 3) or closed at the ends of
   the upper half (Qdims 2), scaled so that the first moments keep their eigenvalue
-  (fokker_planck_finite_difference.jl).  The Voronoi construction of the other 2D/3D sets is not
-  provided;
+  (fokker_planck_finite_difference.jl); for the other 2D/3D sets (lebedev, carlson) a coupling
+  γ_nm / w_n between nodes whose Voronoi cells on the unit sphere share an edge, with one γ per
+  edge fixed by the first moments being eigenfunctions of eigenvalue -2 (tools/voronoi.jl);
 * ``differential-quadrature`` -- ``(1 - mu²) W² - 2 mu W`` with W the Lagrange differentiation
   matrix on the 1D nodes (fokker_planck_differential_quadrature.jl).
 """
@@ -123,6 +124,84 @@ def _dq_1d(mu):
     return _shift_min_diag((np.eye(N) - H @ H) @ W @ W - 2.0 * H @ W)
 
 
+def voronoi_edges(P, tol=1e-7):
+    """Pairs (n, m), n < m, of nodes on the unit sphere (P [Nd, 3]) whose Voronoi cells share an
+    edge (the neighbours that voronoi.jl:13-205 finds from vertex triplets).  For nodes on a sphere
+    the Delaunay triangulation is the convex hull, and each hull facet's outward normal is the
+    Voronoi vertex of its three nodes; facets of nodes on one circle (coplanar facets) are one
+    vertex, so their diagonals are not edges.  Two cells share an edge exactly when they share two
+    vertices."""
+    from scipy.spatial import ConvexHull
+    hull = ConvexHull(P)
+    nrm = hull.equations[:, :3]
+    vid = np.empty(len(nrm), dtype=np.int64)
+    reps = []
+    for f, v in enumerate(nrm):
+        for r, rv in enumerate(reps):
+            if np.max(np.abs(rv - v)) < tol:
+                vid[f] = r
+                break
+        else:
+            reps.append(v)
+            vid[f] = len(reps) - 1
+    verts = [set() for _ in range(len(P))]
+    for f, tri in enumerate(hull.simplices):
+        for n in tri:
+            verts[n].add(int(vid[f]))
+    edges = set()
+    for tri in hull.simplices:
+        for a in range(3):
+            for b in range(a + 1, 3):
+                n, m = sorted((int(tri[a]), int(tri[b])))
+                if len(verts[n] & verts[m]) >= 2:
+                    edges.add((n, m))
+    return sorted(edges)
+
+
+def _fd_voronoi(ang, quad_type, quad_order, qdims):
+    """fokker_planck_finite_difference.jl:139-179 (qdims 2) and :255-275 (qdims 3) with
+    fokker_planck_weights_{2D,3D} of voronoi.jl: unknowns γ per Voronoi edge with
+    sum_m γ_nm (Ω_k[m] - Ω_k[n]) = -2 w_n Ω_k[n] for every node and axis (least squares, full
+    column rank required), ℳ[n, m] = γ_nm / w_n, rows summing to zero, shifted by max |ℳ[n, n]|.
+    In 2D the edges of the full-sphere rule between upper-hemisphere nodes, mapped onto the folded set."""
+    from .angular import quadrature
+    mu, eta, xi, w = (np.asarray(a, float) for a in (ang.mu, ang.eta, ang.xi, ang.w))
+    Nd = len(w)
+    X3, Y3, Z3, _ = quadrature(quad_order, quad_type, 3, 3)
+    if qdims == 3:
+        if len(X3) != Nd:
+            raise ValueError("Fokker-Planck operator does not match the quadrature")
+        to2 = np.arange(Nd)
+        up = np.ones(Nd, bool)
+    else:
+        d = (X3[:, None] - mu[None]) ** 2 + (Y3[:, None] - eta[None]) ** 2 + (np.abs(Z3)[:, None] - xi[None]) ** 2
+        to2 = np.argmin(d, axis=1)
+        up = Z3 >= 0
+    pairs = set()
+    for n3, m3 in voronoi_edges(np.stack([X3, Y3, Z3], 1)):
+        if up[n3] and up[m3] and to2[n3] != to2[m3]:
+            pairs.add((min(to2[n3], to2[m3]), max(to2[n3], to2[m3])))
+    pairs = sorted(pairs)
+    O = np.stack([mu, eta, xi], 1)
+    G = np.zeros((3 * Nd, len(pairs)))
+    Q = (-2.0 * w[:, None] * O).ravel()
+    for e, (n, m) in enumerate(pairs):
+        G[3 * n:3 * n + 3, e] += O[m] - O[n]
+        G[3 * m:3 * m + 3, e] += O[n] - O[m]
+    pG = np.linalg.pinv(G)
+    if np.linalg.norm(pG @ G - np.eye(len(pairs))) > 1e-7:
+        raise ValueError("The equation system to find γ-parameters does not have a solution.")
+    gam = pG @ Q
+    Mn = np.zeros((Nd, Nd))
+    for e, (n, m) in enumerate(pairs):
+        Mn[n, m] += gam[e] / w[n]
+        Mn[m, n] += gam[e] / w[m]
+        Mn[n, n] -= gam[e] / w[n]
+        Mn[m, m] -= gam[e] / w[m]
+    lam = float(np.max(np.abs(np.diag(Mn))))
+    return Mn + lam * np.eye(Nd), lam
+
+
 def direction_operator(ang, method, quad_type, quad_order, qdims):
     """(ℳ_n [Nd, Nd] in the direction order of the angular set, λ₀)."""
     if method == "galerkin":
@@ -135,8 +214,7 @@ def direction_operator(ang, method, quad_type, quad_order, qdims):
             if Mn.shape[0] != ang.Nd:
                 raise ValueError("Fokker-Planck operator does not match the quadrature")
             return Mn, lam
-        raise ValueError(f"finite-difference Fokker-Planck on the {quad_type} quadrature (Voronoi weights) "
-                         "is not supported")
+        return _fd_voronoi(ang, quad_type, quad_order, qdims)
     if method == "differential-quadrature":
         if qdims != 1:
             raise ValueError("Differential quadrature unavailable in multidimensional geometry.")
