@@ -83,6 +83,8 @@ def lib():
                                      C.c_double, C.c_double, C.c_double, dp, dp, dp, dp]
         L.oracle_cell_bfp.argtypes = [C.c_char_p] * 4 + [C.c_int] * 6 + [dp, dp, C.c_double, C.c_double, C.c_double,
                                                                        dp, dp, dp, dp, dp, dp]
+        L.oracle_cell_bfp_awd.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp, dp,
+                                          dp, dp, dp]
         L.oracle_voronoi_neighbours.argtypes = [C.c_int, dp, dp, dp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                                 C.c_int]
         L.oracle_legendre_triple.argtypes = [C.c_int, C.c_int, C.c_int]
@@ -319,6 +321,16 @@ def cell_bfp(schemes, orders, full_coupling, ndims, om, h, St, Sm, Sp, Qn, fe_in
     _check(lib().oracle_cell_bfp(*(s.encode() for s in schemes), oe, ox, oy, oz, int(not full_coupling), ndims,
                                  _p(o), _p(hh), St, Sm, Sp, _p(q), _p(fe), _p(fx), _p(fy), _p(fz), _p(psi)))
     return psi, fe, fx, fy, fz
+
+
+def cell_bfp_awd(ndims, om, h, St, Sm, Sp, dE, Q, fe, fx, fy, fz):
+    """One slowing-down cell of order 1 with the adaptive closure: (psi, fe, fx, fy, fz) outgoing."""
+    arr = lambda v: np.ascontiguousarray(v, np.float64).copy()
+    o, hh, q = arr(om), arr(h), arr([Q])
+    f = [arr([v]) for v in (fe, fx, fy, fz)]
+    psi = np.zeros(1)
+    _check(lib().oracle_cell_bfp_awd(ndims, _p(o), _p(hh), St, Sm, Sp, dE, _p(q), *[_p(v) for v in f], _p(psi)))
+    return psi[0], f[0][0], f[1][0], f[2][0], f[3][0]
 
 
 def legendre_triple(a, b, c):

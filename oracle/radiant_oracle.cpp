@@ -1614,15 +1614,30 @@ struct Cell4 {
 // incoming energy, x, y and z face moments, overwritten with the outgoing ones; w[0..3]: the
 // closure weights of each axis (scheme_weights_1d).  tang1d: the 1D tangential guard
 // (|μ| <= 1e-10: no streaming, the x face passes through, sn_1D_bfp.jl:52-54).
+// adapt: the adaptive closure (AWD0, order 1 on every axis; adaptive.jl on the axes (x[, y[, z]], E)
+// as sn_{1,2,3}D_bfp.jl call it, with 1/ΔE as the energy axis' h and the energy face as its inflow),
+// not for a tangential 1D direction (sn_1D_bfp.jl:52); inv_dE = 1/ΔE of the group.
 void flux_bfp(const double* om, const double* h, double St, double Sm, double Sp, const double* Sv,
-              const double* W, const double* Qn, double* const* f, const Cell4& cl, const double* const* w,
-              bool tang1d, double* psi) {
+              const double* W, const double* Qn, double* const* f, const Cell4& cl, const double* const* w_in,
+              bool tang1d, double* psi, bool adapt = false, double inv_dE = 0.0) {
     double s[4] = {-1.0, 0, 0, 0}, hh[4] = {0, 0, 0, 0};
     for (int a = 1; a < 4; ++a) {
         s[a] = om[a - 1] > 0 ? 1.0 : (om[a - 1] < 0 ? -1.0 : 0.0);
         hh[a] = h[a - 1] > 0 ? std::fabs(om[a - 1]) / h[a - 1] : 0.0;
     }
     if (tang1d) { s[1] = 1.0; hh[1] = 0.0; }
+    double wa[4][2];
+    const double* wl[4] = {w_in[0], w_in[1], w_in[2], w_in[3]};
+    const double* const* w = w_in;
+    if (adapt && !tang1d) {
+        if (cl.Nm != 1) fail("the adaptive closure with slowing down is provided for order 1 on every axis");
+        // axes x, y, z, then E; an axis a problem lacks has h = 0 and adds nothing to the estimate
+        const double h4[4] = {hh[1], hh[2], hh[3], inv_dE}, f4[4] = {f[1][0], f[2][0], f[3][0], f[0][0]};
+        double* wp[4] = {wa[1], wa[2], wa[3], wa[0]};
+        awd0_weights(4, h4, f4, Qn[0], St, wp);
+        for (int a = 0; a < 4; ++a) wl[a] = wa[a];
+        w = wl;
+    }
     auto C = [](int i) { return std::sqrt(2.0 * i + 1.0); };  // 0-based
     auto spow = [](double v, int e) { double r = 1.0; for (int k = 0; k < e; ++k) r *= v; return r; };
     const int OE = cl.O[0], Nm = cl.Nm;
@@ -1900,7 +1915,7 @@ struct Solver {
         string scheme_y = p.scheme_y ? p.scheme_y : "DD";
         K = p.scheme_order > 1 ? p.scheme_order : 1;
         const int Ky = p.scheme_order_y > 1 ? p.scheme_order_y : 1;
-        const string scheme_e = p.scheme_e ? p.scheme_e : "DD";
+        string scheme_e = p.scheme_e ? p.scheme_e : "DD";
         const int Ke = p.scheme_order_e > 1 ? p.scheme_order_e : 1;
         if (!p.csd && (Ke > 1 || scheme_e != "DD"))
             fail("an energy closure needs the slowing-down term (csd)");
@@ -1908,18 +1923,19 @@ struct Solver {
             const string sz = p.scheme_z ? p.scheme_z : "DD";
             adapt = scheme == "AWD" || (ndims >= 2 && scheme_y == "AWD") || (ndims >= 3 && sz == "AWD") ||
                     (p.csd && scheme_e == "AWD");
-            if (adapt && p.csd) fail("the adaptive closure of the slowing-down cell is not provided");
-            if (adapt) scheme = scheme_y = "AWD";
+            if (adapt) scheme = scheme_y = scheme_e = "AWD";
         }
         if (Ke > 2) fail("energy closures of order above 2 are not supported");
         if (p.csd) {  // scheme_weights.jl with isCSD: axes E, x[, y[, z]], DD or DG each
-            const string scheme_z = p.scheme_z ? p.scheme_z : "DD";
+            const string scheme_z = adapt ? string("AWD") : string(p.scheme_z ? p.scheme_z : "DD");
             const int Kz = p.scheme_order_z > 1 ? p.scheme_order_z : 1;
-            const string sy = ndims >= 2 ? scheme_y : "DD", sz = ndims >= 3 ? scheme_z : "DD";
+            const string sy = ndims >= 2 ? scheme_y : (adapt ? "AWD" : "DD"), sz = ndims >= 3 ? scheme_z : (adapt ? "AWD" : "DD");
             const int oy = ndims >= 2 ? Ky : 1, oz = ndims >= 3 ? Kz : 1;
+            if (adapt && (Ke > 1 || K > 1 || oy > 1 || oz > 1))
+                fail("the adaptive closure with slowing down is provided for order 1 on every axis");
             if (Ke > 1 || K > 1 || oy > 1 || oz > 1 || scheme_e != "DD" || scheme != "DD" || sy != "DD" || sz != "DD") {
                 for (const string* sc : std::initializer_list<const string*>{&scheme_e, &scheme, &sy, &sz})
-                    if (*sc != "DD" && *sc != "DG") fail("Closure relation not implemented yet.");
+                    if (*sc != "DD" && *sc != "DG" && *sc != "AWD") fail("Closure relation not implemented yet.");
                 ome = scheme_weights_1d(scheme_e, Ke);
                 omx = scheme_weights_1d(scheme, K);
                 omy = scheme_weights_1d(sy, oy);
@@ -2254,7 +2270,7 @@ struct Solver {
                     const int m = matc[c];
                     double* f[4] = {FE.data(), fx, fy, FZ.data()};
                     flux_bfp(om, h, sig_t(g, m), Sminus[m], Splus[m], &Svec[(size_t)cl4.O[0] * m], Wt.data(), Qn.data(),
-                             f, cl4, w, tang, psi.data());
+                             f, cl4, w, tang, psi.data(), adapt, 1.0 / P.dE[g]);
                     for (int is = 0; is < NmE; ++is) fe_out[is] = FE[is];
                     for (int is = 0; is < K; ++is)
                         for (int p = 0; p < Np; ++p)
@@ -2995,6 +3011,23 @@ int oracle_cell_bfp(const char* se, const char* sx, const char* sy, const char* 
         Cell4 cl(Oe, Ox, Oy, Oz, partial == 0);
         double* f[4] = {fe, fx, fy, fz};
         flux_bfp(om, h, St, Sm, Sp, Sv, W.data(), Qn, f, cl, w, ndims == 1 && std::fabs(om[0]) <= 1e-10, psi);
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// One slowing-down cell of order 1 on every axis with the adaptive closure (AWD0 over x[, y[, z]]
+// and E, with 1/ΔE as the energy axis' h): as oracle_cell_bfp, and dE the group width.
+int oracle_cell_bfp_awd(int ndims, const double* om, const double* h, double St, double Sm, double Sp, double dE,
+                        const double* Qn, double* fe, double* fx, double* fy, double* fz, double* psi) {
+    try {
+        double Sv[1];
+        stopping_moments(Sm, Sp, 1, Sv);
+        vector<double> we = scheme_weights_1d("AWD", 1), wx = we, wy = we, wz = we;
+        const double* w[4] = {we.data(), wx.data(), wy.data(), wz.data()};
+        const double W[1] = {legendre_triple(0, 0, 0)};
+        Cell4 cl(1, 1, 1, 1, true);
+        double* f[4] = {fe, fx, fy, fz};
+        flux_bfp(om, h, St, Sm, Sp, Sv, W, Qn, f, cl, w, ndims == 1 && std::fabs(om[0]) <= 1e-10, psi, true, 1.0 / dE);
         return 0;
     } catch (std::exception& e) { g_err = e.what(); return -1; }
 }

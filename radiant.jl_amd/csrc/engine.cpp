@@ -169,7 +169,8 @@ static void validate(const rt_problem_desc& d) {
     if (!d.csd && d.order_e > 1) bad("an energy closure needs the slowing-down term (csd)");
     if (d.adaptive) {  // scheme "AWD" (adaptive.jl)
         if (d.adaptive != 1) bad("adaptive must be 0 or 1");
-        if (d.csd) bad("the adaptive closure (AWD) of the slowing-down cell is not provided");
+        if (d.csd && (d.order_x > 1 || d.order_y > 1 || d.order_z > 1 || d.order_e > 1))
+            bad("the adaptive closure (AWD) with slowing down: order 1 on every axis, space and energy");
         if (d.precision != RT_PREC_FP64)
             bad("the adaptive closure (AWD) is not linear in the flux: it runs in the FP64 solve, whose "
                 "reconstruction is a full pass");
@@ -703,6 +704,7 @@ void Problem::fill_geometry(PassDesc& pd, int g) {
     if (bfp) {  // the general slowing-down cell needs S⁻, S⁺ in every pass (homogeneous ones too)
         pd.ho_KE = ho_KE; pd.ho_ome = d_ome; pd.fm_e = fm_e; pd.emap = d_emap;
         pd.csdS = d_csdS + (size_t)g * Nmat * 2;
+        awd_inv_dE = 1.0 / dE[g];  // the adaptive estimate's energy h (sn_1D_bfp.jl:52)
     }
 }
 
@@ -763,7 +765,8 @@ void Problem::launch_pass(PassDesc& pd, int prec, bool full_rows) {
         const int sp = prec == PBF16 ? P32 : prec;
         if (adapt) {
             if (sp != P64) throw Error(RT_EINVAL, "the adaptive closure runs FP64 passes only");
-            launch_sweep_awd(d_pd, Nd, ndims, ndims == 1 ? ho_K : 1, ho_lds(P64), st);
+            if (bfp) launch_sweep_bfp_awd(d_pd, Nd, awd_inv_dE, ho_lds(P64), st);
+            else launch_sweep_awd(d_pd, Nd, ndims, ndims == 1 ? ho_K : 1, ho_lds(P64), st);
         } else if (bfp)
             launch_sweep_bfp(sp, d_pd, Nd, ho_KE, ho_Kx, ho_Ky, ho_Kz, ho_fc, ho_lds(sp), st);
         else if (ho_K && ndims == 3)

@@ -102,6 +102,7 @@ public:
     // then span (E, x, y, z)
     bool bfp = false;
     bool adapt = false;  // adaptive weighted diamond closure (sweep_awd.hip), FP64 passes only
+    double awd_inv_dE = 0.0;  // 1/ΔE of the group of the pass being set up (adaptive slowing-down cells)
     int ho_KE = 1, fm_e = 1;
     // the 3D and slowing-down walkers keep their face moments in device memory when they exceed
     // the LDS (d_hoface, [Nd][faces] doubles); dynamic LDS of their launch in precision sp

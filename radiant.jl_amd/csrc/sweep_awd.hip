@@ -8,6 +8,7 @@
 //   its row of cells; orders 1 (AWD0, adaptive.jl:36-54) and 2 (AWD1, adaptive.jl:56-73).  A
 //   direction with |mu| <= 1e-10 does not stream: its face flux passes through unchanged and the
 //   weights are not adapted (sn_1D_bte.jl:39).
+// - slowing-down types, order 1 on every axis (flux_{1,2,3}D_BFP): below, with the energy axis.
 // - 2D and 3D (sn_sweep_2D.jl:54-200 / sn_sweep_3D.jl:57-240 with flux_2D_BTE / flux_3D_BTE): one
 //   workgroup per direction walks the diagonals kx + ky (+ kz) = d in sweep order, one cell per
 //   thread, with the face fluxes of every pencil in LDS (3D grids whose faces exceed it: in the
@@ -240,6 +241,83 @@ __global__ __launch_bounds__(256) void sweep_3d_awd_kernel(const PassDesc* __res
         for (int i = tid; i < Nx * Ny; i += nt) ((double*)d.bz)[(size_t)n * Nx * Ny + i] = FZ[i];
 }
 
+// ------------------------------------------------------------------------ slowing-down cells
+// flux_{1,2,3}D_BFP (sn_1D_bfp.jl:39-125, sn_2D_bfp.jl, sn_3D_bfp.jl) of order 1 on every axis with
+// the adaptive closure: AWD0 over the axes (x[, y[, z]], E) with h_E = 1/ΔE and the energy face as the
+// energy axis' inflow (as the reference calls adaptive), then the order-1 slowing-down cell
+//   S = St + sum_a h_a w_a1 + S+ wE1,   Q = Qn - sum_a h_a (w_a0 - 1) f_a - (S+ wE0 - S-) f_E,
+//   psi = Q / S,  f_a <- w_a0 f_a + w_a1 psi,  f_E <- wE0 f_E + wE1 psi
+// (S-, S+: the stopping powers at the group's upper and lower energy over ΔE).  A tangential 1D
+// direction keeps the base weights (sn_1D_bfp.jl:52) and its x face.  The walk, the faces (in LDS or
+// hoface, [Ny*Nz] + [Nx*Nz] (2D, 3D) + [Nx*Ny] (3D)) and the energy faces ([Nd][cells], psiE_in /
+// psiE_out, absent in homogeneous passes) are those of the general slowing-down walker.
+__global__ __launch_bounds__(256) void sweep_bfp_awd_kernel(const PassDesc* __restrict__ pds, double inv_dE) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const PassDesc& d = pds[0];
+    const int n = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    const int nd = d.ndims, Nx = d.Nx, Ny = d.Ny, Nz = d.Nz;
+    const size_t Ncell = (size_t)Nx * Ny * Nz;
+    const size_t nface = (size_t)Ny * Nz + (nd >= 2 ? (size_t)Nx * Nz : 0) + (nd >= 3 ? (size_t)Nx * Ny : 0);
+    double* FX = d.hoface ? (double*)d.hoface + (size_t)n * nface : (double*)smem;  // [Ny*Nz]
+    double* FY = FX + (size_t)Ny * Nz;                                             // [Nx*Nz] (2D, 3D)
+    double* FZ = FY + (nd >= 2 ? (size_t)Nx * Nz : 0);                             // [Nx*Ny] (3D)
+    const double om[3] = {d.mu[n], nd >= 2 ? d.eta[n] : 0.0, nd >= 3 ? d.xi[n] : 0.0};
+    const bool tang = nd == 1 && fabs(om[0]) <= 1e-10;
+    const bool fw[3] = {om[0] >= 0, om[1] >= 0, om[2] >= 0};
+    double* __restrict__ psi = (double*)d.psi + (size_t)n * d.Nc;
+    const double* Ein = d.psiE_in ? d.psiE_in + (size_t)n * Ncell : nullptr;
+    double* Eout = d.psiE_out ? d.psiE_out + (size_t)n * Ncell : nullptr;
+    for (int i = tid; i < Ny * Nz; i += nt) FX[i] = d.bin_x ? d.bin_x[(size_t)n * Ny * Nz + i] : 0.0;
+    if (nd >= 2)
+        for (int i = tid; i < Nx * Nz; i += nt) FY[i] = d.bin_y ? d.bin_y[(size_t)n * Nx * Nz + i] : 0.0;
+    if (nd >= 3)
+        for (int i = tid; i < Nx * Ny; i += nt) FZ[i] = d.bin_z ? d.bin_z[(size_t)n * Nx * Ny + i] : 0.0;
+    __syncthreads();
+    for (int dg = 0; dg < Nx + Ny + Nz - 2; ++dg) {
+        for (int i = tid; i < Nx * Ny; i += nt) {
+            const int kx = i % Nx, ky = i / Nx, kz = dg - kx - ky;
+            if (kz < 0 || kz >= Nz) continue;
+            const int ix = fw[0] ? kx : Nx - 1 - kx, iy = fw[1] ? ky : Ny - 1 - ky, iz = fw[2] ? kz : Nz - 1 - kz;
+            const size_t c = (size_t)ix + (size_t)Nx * (iy + (size_t)Ny * iz);
+            const double h[3] = {tang ? 0.0 : fabs(om[0]) / d.dx[ix], nd >= 2 ? fabs(om[1]) / d.dy[iy] : 0.0,
+                                 nd >= 3 ? fabs(om[2]) / d.dz[iz] : 0.0};
+            const int m = d.mat[c];
+            const double St = d.sigt[m], Sm = d.csdS[2 * m], Sp = d.csdS[2 * m + 1];
+            double* fx = FX + ((size_t)iy + (size_t)Ny * iz);
+            double* fy = nd >= 2 ? FY + ((size_t)ix + (size_t)Nx * iz) : nullptr;
+            double* fz = nd >= 3 ? FZ + ((size_t)ix + (size_t)Nx * iy) : nullptr;
+            const double f[3] = {*fx, fy ? *fy : 0.0, fz ? *fz : 0.0};
+            const double fe = Ein ? Ein[c] : 0.0;
+            const double Q = psi[c];
+            double w[4][2];  // x, y, z, E
+            if (!tang) {
+                const double h4[4] = {h[0], h[1], h[2], inv_dE}, f4[4] = {f[0], f[1], f[2], fe};
+                awd0<4>(h4, f4, Q, St, w);
+            } else {
+                const double* base[4] = {d.ho_om, d.ho_omy, d.ho_omz, d.ho_ome};
+#pragma unroll
+                for (int a = 0; a < 4; ++a) { w[a][0] = base[a][0]; w[a][1] = base[a][1]; }
+            }
+            const double S = St + h[0] * w[0][1] + h[1] * w[1][1] + h[2] * w[2][1] + Sp * w[3][1];
+            const double q = Q - h[0] * (w[0][0] - 1.0) * f[0] - h[1] * (w[1][0] - 1.0) * f[1] -
+                             h[2] * (w[2][0] - 1.0) * f[2] - (Sp * w[3][0] - Sm) * fe;
+            const double v = q / S;
+            psi[c] = v;
+            if (!tang) *fx = w[0][0] * f[0] + w[0][1] * v;
+            if (fy) *fy = w[1][0] * f[1] + w[1][1] * v;
+            if (fz) *fz = w[2][0] * f[2] + w[2][1] * v;
+            if (Eout) Eout[c] = w[3][0] * fe + w[3][1] * v;
+        }
+        __syncthreads();
+    }
+    if (d.bx)
+        for (int i = tid; i < Ny * Nz; i += nt) ((double*)d.bx)[(size_t)n * Ny * Nz + i] = FX[i];
+    if (nd >= 2 && d.by)
+        for (int i = tid; i < Nx * Nz; i += nt) ((double*)d.by)[(size_t)n * Nx * Nz + i] = FY[i];
+    if (nd >= 3 && d.bz)
+        for (int i = tid; i < Nx * Ny; i += nt) ((double*)d.bz)[(size_t)n * Nx * Ny + i] = FZ[i];
+}
+
 template <typename F>
 static void awd_launch(F fn, dim3 grid, dim3 block, size_t lds, hipStream_t st, const PassDesc* d_pd) {
     hipEvent_t a, b;
@@ -272,6 +350,22 @@ void launch_sweep_awd(const PassDesc* d_pd, int Nd, int ndims, int K, size_t lds
         }
         awd_launch(fn, dim3(Nd), dim3(256), lds, st, d_pd);
     }
+    RT_HIP(hipGetLastError());
+}
+
+void launch_sweep_bfp_awd(const PassDesc* d_pd, int Nd, double inv_dE, size_t lds, hipStream_t st) {
+    if (Nd <= 0) return;
+    if (lds > 160 * 1024) throw std::runtime_error("adaptive slowing-down face fluxes exceed the LDS");
+    static bool attr = false;
+    if (!attr) {
+        RT_HIP(hipFuncSetAttribute((const void*)sweep_bfp_awd_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   160 * 1024));
+        attr = true;
+    }
+    hipEvent_t a, b;
+    take_launch_events(&a, &b);
+    if (a) hipExtLaunchKernelGGL(sweep_bfp_awd_kernel, dim3(Nd), dim3(256), (uint32_t)lds, st, a, b, 0u, d_pd, inv_dE);
+    else hipLaunchKernelGGL(sweep_bfp_awd_kernel, dim3(Nd), dim3(256), (uint32_t)lds, st, d_pd, inv_dE);
     RT_HIP(hipGetLastError());
 }
 

@@ -75,6 +75,8 @@ class Desc:
         scheme = (getattr(prob, "scheme", None) or "DD").upper()
         ox = max(1, int(getattr(prob, "scheme_order", 1) or 1))
         axes = [scheme] + [(getattr(prob, nm, None) or "DD").upper() for nm in ("scheme_y", "scheme_z")][:prob.ndims - 1]
+        if getattr(prob, "csd", False):
+            axes.append((getattr(prob, "scheme_e", None) or "DD").upper())
         if "AWD" in axes:  # scheme_weights.jl:32-37: one adaptive axis makes every axis adaptive
             d.adaptive = 1
             scheme = "AWD"
@@ -103,7 +105,7 @@ class Desc:
         if oe > 1 or scheme_e != "DD":  # the energy closure of the slowing-down types
             if not getattr(prob, "csd", False):
                 raise ValueError("an energy closure needs the slowing-down term (csd)")
-            if scheme_e not in ("DD", "DG"):
+            if scheme_e not in ("DD", "DG", "AWD"):
                 raise ValueError("Closure relation not implemented yet.")
             k["omega_e"] = np.ascontiguousarray(schemes.weights_1d(scheme_e, oe))
             d.order_e, d.omega_e = oe, ptr(k["omega_e"])

@@ -16,9 +16,13 @@ Oracle pins (CPU):
   keeps the diamond weights), equal to the DD solve; AWD is positively homogeneous (twice the source
   gives twice the flux) although not linear; 2D/3D orders above 1 and the slowing-down cell are
   refused as the reference refuses them or does not provide them.
+- with slowing down (order 1 on every axis, the energy axis adapted with h = 1/dE as the reference
+  calls adaptive): the cell equals the balance form with the energy face, a tangential 1D direction
+  keeps the diamond weights; the solve is homogeneous in the sources and differs from DD.
 GPU: passes (1e-12) and FP64 solves in the faithful state (1e-10, iteration counts equal) against
 the oracle in 1D (orders 1, 2), 2D and 3D, with void, reflective and periodic faces and GMRES /
-Anderson / BiCGStab; mixed precision and the reduced state are refused."""
+Anderson / BiCGStab; BCSD and BFP solves with the cutoff flux (1e-10); mixed precision and the
+reduced state are refused."""
 import sys
 
 import numpy as np
@@ -127,6 +131,76 @@ def test_awd1_cell(mu):
         a, fa = O.cell_1d("AWD", 2, mu, St, dx, [1.0, far], f)
         g, fg = O.cell_1d("DG", 2, mu, St, dx, [1.0, far], f)
         assert not np.allclose(a, g)
+
+
+def _wdd_bfp_cell(h, fins, fe, Q, St, Sm, Sp, dE, tang=False):
+    """The slowing-down cell of order 1 in balance form, St psi + sum_a h_a (out_a - in_a) +
+    S+ out_E - S- in_E = Q, with out = (1 + P) psi - P in on every axis (energy included) and the AWD
+    weights from the diamond estimate over (x[, y[, z]], E) with 1/dE as the energy axis' h; a
+    tangential 1D direction keeps the diamond weights (P = 1)."""
+    if tang:
+        P, PE = [1.0] * len(h), 1.0
+    else:
+        hh, ff = list(h) + [1.0 / dE], list(fins) + [fe]
+        phi_dd = (Q + sum(2 * a * b for a, b in zip(hh, ff))) / (sum(2 * a for a in hh) + St)
+        Ps = [_P_rule(f, phi_dd) for f in ff]
+        P, PE = Ps[:-1], Ps[-1]
+    psi = (Q + sum(a * (1 + p) * f for a, p, f in zip(h, P, fins)) + (Sp * PE + Sm) * fe) / \
+          (St + sum(a * (1 + p) for a, p in zip(h, P)) + Sp * (1 + PE))
+    outs = [(1 + p) * psi - p * f for p, f in zip(P, fins)]
+    return psi, outs, (1 + PE) * psi - PE * fe
+
+
+@pytest.mark.parametrize("case", range(len(CASES)))
+def test_awd0_slowing_down_cell(case):
+    """flux_{1,2,3}D_BFP of order 1 with the adaptive closure (sn_1D_bfp.jl:52 and the 2D/3D calls)."""
+    O = _O()
+    om, dx, St, Q, fins = CASES[case]
+    n = len(om)
+    for (Sm, Sp, dE, fe) in [(0.8, 0.5, 0.3, 0.7), (0.2, 0.9, 2.0, 0.0), (1.5, 1.1, 1.0, 3.0)]:
+        psi_ref, out_ref, e_ref = _wdd_bfp_cell([abs(o) / d for o, d in zip(om, dx)], fins, fe, Q, St, Sm, Sp, dE)
+        om3 = list(om) + [0.0] * (3 - n)
+        h3 = list(dx) + [0.0] * (3 - n)
+        f3 = list(fins) + [0.0] * (3 - n)
+        psi, fe_o, fx, fy, fz = O.cell_bfp_awd(n, om3, h3, St, Sm, Sp, dE, Q, fe, *f3)
+        assert abs(psi - psi_ref) <= 1e-14 * max(1.0, abs(psi_ref))
+        assert np.allclose([fx, fy, fz][:n], out_ref, rtol=1e-13, atol=1e-15)
+        assert abs(fe_o - e_ref) <= 1e-13 * max(1.0, abs(e_ref))
+    # a tangential 1D direction: no adaptation, the x face passes through
+    psi_ref, _, e_ref = _wdd_bfp_cell([0.0], [0.4], 0.7, 1.0, 1.2, 0.8, 0.5, 0.3, tang=True)
+    psi, fe_o, fx, _, _ = O.cell_bfp_awd(1, [0.0, 0.0, 0.0], [1.0, 0.0, 0.0], 1.2, 0.8, 0.5, 0.3, 1.0, 0.7, 0.4, 0.0, 0.0)
+    assert abs(psi - psi_ref) < 1e-14 and abs(fe_o - e_ref) < 1e-14 and fx == 0.4
+
+
+def _slowing_down(p, stype="BCSD", seed=5):
+    rng = np.random.default_rng(seed)
+    p.csd = True
+    p.Sb = 0.7 * rng.uniform(0.5, 2.0, (p.G + 1, p.Nmat))
+    p.dE = rng.uniform(0.5, 1.5, p.G)
+    p.solver_type = stype
+    if stype == "BFP":
+        p.T = 0.3 * rng.uniform(0.2, 1.0, (p.G, p.Nmat))
+        p.fokker_planck = "galerkin"
+    return p
+
+
+def test_awd_slowing_down_solve():
+    """Positively homogeneous in the sources, different from the diamond solve, and refused above
+    order 1."""
+    O = _O()
+    from radiant_hip import problems as P
+    for ndims in (1, 2, 3):
+        p = _slowing_down(_awd(P.small_problem(ndims, N=5, G=3, L=1, tol=1e-13, seed=19, accel="anderson")))
+        phi, st = O.solve(p)
+        assert all(s["converged"] for s in st)
+        q = _slowing_down(_awd(P.small_problem(ndims, N=5, G=3, L=1, tol=1e-13, seed=19, accel="anderson")))
+        q.qvol = 2.0 * p.qvol
+        assert rel_err(O.solve(q)[0], 2.0 * phi) < 1e-10
+        r = _slowing_down(P.small_problem(ndims, N=5, G=3, L=1, tol=1e-13, seed=19, accel="anderson"))
+        assert rel_err(phi, O.solve(r)[0]) > 1e-4
+    p = _slowing_down(_awd(P.small_problem(1, N=5, G=2, L=1), scheme_order=2))
+    with pytest.raises(RuntimeError, match="order 1 on every axis"):
+        O.solve(p)
 
 
 def _awd(prob, **kw):
@@ -248,6 +322,30 @@ def test_gpu_awd_pass(oracle, ndims):
             assert e < 1e-12
     finally:
         h.close()
+
+
+SD_CASES = [  # (ndims, solver type, bc, accel)
+    (1, "BCSD", None, "gmres"), (1, "BFP", [1, 0], "anderson"), (2, "BCSD", None, "gmres"),
+    (2, "BCSD", [1, 0, 2, 2], "bicgstab"), (3, "BCSD", None, "gmres"), (3, "BFP", None, "anderson"),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", SD_CASES, ids=[f"{c[0]}d-{c[1]}-{c[3]}-{'bc' if c[2] else 'void'}" for c in SD_CASES])
+def test_gpu_awd_slowing_down(oracle, case):
+    """The slowing-down types with the adaptive closure: flux moments and the cutoff flux (the
+    energy outflow of the last group's final pass) against the oracle."""
+    from radiant_hip import problems as P, solver as S
+    ndims, stype, bc, accel = case
+    p = _slowing_down(_awd(P.small_problem(ndims, N={1: 20, 2: 8, 3: 5}[ndims], G=3, L=1, tol=1e-12, seed=90 + ndims,
+                                           accel=accel, bc=bc)), stype=stype)
+    ref, cut_ref, rst = oracle.solve_cutoff(p)
+    assert all(s["converged"] for s in rst)
+    phi, cut, st = S.compute_flux_cutoff(p)
+    e, ec = rel_err(phi, ref), rel_err(cut, cut_ref)
+    print("awd slowing down", case, e, ec, flush=True)
+    sys.stdout.flush()
+    assert e < 1e-10 and ec < 1e-10 and all(s["converged"] for s in st)
 
 
 @pytest.mark.gpu
