@@ -155,6 +155,13 @@ typedef struct rt_problem_desc {
      * for it, and the Krylov iterates depend on the state they run over (the faithful state is the
      * reference's).  0: off. */
     int32_t adaptive;
+    /* An external magnetic field (Electromagnetic_Field; sn_flux.jl:116-120): the Lorentz-force
+     * operator ℳ_EM of electromagnetic_scattering_matrix.jl, [G][Np][Np] laid out as Radiant's
+     * ℳ_EM[ig, p, q] array (column-major: g + G*(p + Np*q)); every pass adds ℳ_EM[g] Φ to the
+     * source of each cell and spatial moment (sn_inner_pass.jl:26-32).  Its λ₀_EM is zero, so Σtot
+     * is unchanged.  Every precision mode and state; the lockstep batch solver does not take such
+     * a problem, so rt_solve_batch and the server solve it on its own stream.  NULL: no field. */
+    const double *Mem;
 } rt_problem_desc;
 
 /* Per-group result of the in-group solve (the values sn_one_speed.jl:189-193 prints). */

@@ -38,6 +38,7 @@ class OracleProblem(C.Structure):
         ("quad_mu", C.POINTER(C.c_double)), ("quad_eta", C.POINTER(C.c_double)),
         ("quad_xi", C.POINTER(C.c_double)), ("quad_w", C.POINTER(C.c_double)), ("quad_n", C.c_int32),
         ("scheme_e", C.c_char_p), ("scheme_order_e", C.c_int32),
+        ("em", C.c_int32), ("em_B", C.c_double * 3), ("em_charge", C.c_double), ("Eb", C.POINTER(C.c_double)),
     ]
 
 # SN.jl get_solver_type
@@ -83,6 +84,9 @@ def lib():
                                      C.c_double, C.c_double, C.c_double, dp, dp, dp, dp]
         L.oracle_cell_bfp.argtypes = [C.c_char_p] * 4 + [C.c_int] * 6 + [dp, dp, C.c_double, C.c_double, C.c_double,
                                                                        dp, dp, dp, dp, dp, dp]
+        L.oracle_em_matrix.argtypes = [C.POINTER(OracleProblem), dp, C.c_int]
+        L.oracle_ferrer.argtypes = [C.c_int, C.c_int, C.c_double]
+        L.oracle_ferrer.restype = C.c_double
         L.oracle_cell_bfp_awd.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp, dp,
                                           dp, dp, dp]
         L.oracle_voronoi_neighbours.argtypes = [C.c_int, dp, dp, dp, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
@@ -225,6 +229,14 @@ def make_problem(prob, nthreads=1, verbose=0, **over):
     k.scheme_e = (getattr(prob, "scheme_e", None) or "DD").encode()
     op.scheme_e = k.scheme_e
     op.scheme_order_e = int(getattr(prob, "scheme_order_e", 1) or 1)
+    B = getattr(prob, "em_B", None)
+    if B is not None:  # an external magnetic field (Electromagnetic_Field)
+        op.em = 1
+        for i in range(3):
+            op.em_B[i] = float(B[i])
+        op.em_charge = float(prob.em_charge)
+        k.Eb = np.ascontiguousarray(prob.Eb, np.float64)
+        op.Eb = _p(k.Eb)
     op._keep = k
     return op
 
@@ -245,6 +257,19 @@ def voronoi_neighbours(P):
         out.append([int(v) for v in nbr[k:k + cnt[n]]])
         k += cnt[n]
     return out
+
+
+def em_matrix(prob):
+    """ℳ_EM [G, Np, Np] of a problem with a magnetic field, as the oracle builds it."""
+    op = make_problem(prob)
+    nd, np_, nps, S = sizes(prob)
+    out = np.zeros(prob.G * np_ * np_)
+    _check(lib().oracle_em_matrix(C.byref(op), _p(out), len(out)))
+    return np.stack([out[g * np_ * np_:(g + 1) * np_ * np_].reshape((np_, np_), order="F") for g in range(prob.G)])
+
+
+def ferrer(l, m, x):
+    return lib().oracle_ferrer(l, m, x)
 
 
 def fokker_planck(prob):

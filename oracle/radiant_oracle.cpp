@@ -1239,6 +1239,107 @@ vector<double> fp_fd_voronoi(int Qdims, const vector<double>& mu, const vector<d
     return Mn_dir;
 }
 
+// ---------------------------------------------------------------------------------------
+// Lorentz-force operator of an external magnetic field (electromagnetic_scattering_matrix.jl)
+// ---------------------------------------------------------------------------------------
+
+// ferrer_associated_legendre.jl: P_l^m(x) without the (-1)^m factor, by the upward recursion from
+// P_m^m = (2m-1)!! (1-x^2)^(m/2).
+double ferrer(int l, int m, double x) {
+    if (l < 0 || m < 0 || m > l || std::fabs(x) > 1) fail("invalid associated Legendre arguments");
+    double Pmm = 1.0, Pmm1 = 1.0, Pmm2 = 1.0;
+    if (m > 0) {
+        double df = 1.0;
+        for (int k = 1; k <= 2 * m - 1; k += 2) df *= k;
+        Pmm = df * std::pow(std::sqrt(1.0 - x * x), m);
+    }
+    if (l == m) return Pmm;
+    Pmm1 = (2 * m + 1) * x * Pmm;
+    if (l == m + 1) return Pmm1;
+    for (int li = m + 2; li <= l; ++li) {
+        Pmm2 = ((2 * li - 1) * x * Pmm1 - (li + m - 1) * Pmm) / (li - m);
+        Pmm = Pmm1;
+        Pmm1 = Pmm2;
+    }
+    return Pmm2;
+}
+
+// electromagnetic_scattering_matrix.jl:43-127 (magnetic field B, no electric field): per group
+// ℳ_EM[g] = -𝒯_g D M with 𝒯_g = 2/ΔE_g log((sqrt(Eb_g + 2 mc²) + sqrt(Eb_g)) / (sqrt(Eb_g+1 + 2 mc²) +
+// sqrt(Eb_g+1))) (mc² = 0.510999, c = 2.99792458) and M[n][p] the derivative of the harmonic p at
+// direction n along the Lorentz rotation, charge c (2l+1)/(4π) [(B_z η - B_y ξ) ∂Y/∂μ + (B_y μ η +
+// B_z μ ξ - B_x (1-μ²))/(1-μ²) ∂Y/∂φ], with the polar-axis limits of the reference at |μ| = 1.
+// Returns [G][Np][Np] (ℳ_EM[g] column-major at g*Np*Np); λ₀_EM is zero.
+vector<double> em_matrix(const double* Bf, double charge, const vector<double>& mu, const vector<double>& eta,
+                         const vector<double>& xi, const Basis& B, int G, const double* Eb) {
+    const int Nd = (int)mu.size(), P = B.Np;
+    const double c = 2.99792458, mc2 = 0.510999, PI = 3.14159265358979323846;
+    vector<double> T(G);
+    for (int g = 0; g < G; ++g) {
+        const double dE = Eb[g] - Eb[g + 1];
+        T[g] = 2.0 / dE * std::log((std::sqrt(Eb[g] + 2 * mc2) + std::sqrt(Eb[g])) /
+                                   (std::sqrt(Eb[g + 1] + 2 * mc2) + std::sqrt(Eb[g + 1])));
+    }
+    vector<double> M((size_t)Nd * P, 0.0);  // M[n + Nd*p]
+    auto sgn = [](double v) { return v > 0 ? 1.0 : (v < 0 ? -1.0 : 0.0); };
+    for (int n = 0; n < Nd; ++n) {
+        double phi;
+        if (xi[n] != 0) {
+            if (eta[n] > 0) phi = sgn(xi[n]) * std::atan(std::fabs(xi[n] / eta[n]));
+            else if (eta[n] == 0) phi = sgn(xi[n]) * PI / 2;
+            else phi = sgn(xi[n]) * (PI - std::atan(std::fabs(xi[n] / eta[n])));
+        } else {
+            phi = eta[n] >= 0 ? 0.0 : PI;
+        }
+        const double u = mu[n];
+        for (int p = 0; p < P; ++p) {
+            const int l = (int)B.pl[p], m = (int)B.pm[p], am = std::abs(m);
+            double lf = 0.0;
+            for (int k = 1; k <= l - am; ++k) lf += std::log((double)k);
+            for (int k = 1; k <= l + am; ++k) lf -= std::log((double)k);
+            const double Clm = std::sqrt((2.0 - (m == 0 ? 1.0 : 0.0)) * std::exp(lf));
+            const double Tm = m >= 0 ? std::cos(m * phi) : std::sin(am * phi);
+            const double nrm = (2 * l + 1) / (4 * PI) * charge * c;
+            double v = 0.0;
+            // derivative in μ
+            if (std::fabs(u) != 1) {
+                if (l > 0) {
+                    if (l > am) {
+                        const double Plm = ferrer(l, am, u), Pl1 = ferrer(l - 1, am, u);
+                        v += ((l + am) * Pl1 - l * u * Plm) / (1 - u * u) * Clm * Tm * nrm * (Bf[2] * eta[n] - Bf[1] * xi[n]);
+                    } else if (l == m) {
+                        const double Pll = ferrer(l, l, u);
+                        v += -l * u * Pll / (1 - u * u) * Clm * Tm * nrm * (Bf[2] * eta[n] - Bf[1] * xi[n]);
+                    }
+                }
+            } else if (am == 1) {
+                v += -l * (l + 1) / 2.0 * std::pow(u, l) * Clm * Tm * nrm * (Bf[2] * std::cos(phi) - Bf[1] * std::sin(phi));
+            }
+            // derivative in φ
+            const double Plm = ferrer(l, am, u);
+            if (std::fabs(u) != 1) {
+                const double fac = (Bf[1] * u * eta[n] + Bf[2] * u * xi[n] - Bf[0] * (1 - u * u));
+                if (m >= 0) v += Plm * Clm * -m * std::sin(m * phi) * (2 * l + 1) / (4 * PI) * charge * c / (1 - u * u) * fac;
+                else v += Plm * Clm * am * std::cos(am * phi) * (2 * l + 1) / (4 * PI) * charge * c / (1 - u * u) * fac;
+            } else {
+                const double fac = (Bf[1] * std::cos(phi) + Bf[2] * std::sin(phi));
+                if (m == 1) v += l * (l + 1) / 2.0 * std::pow(u, l + 1) * Clm * -m * std::sin(m * phi) * (2 * l + 1) / (4 * PI) * charge * c * fac;
+                else if (m == -1) v += l * (l + 1) / 2.0 * std::pow(u, l + 1) * Clm * am * std::cos(am * phi) * (2 * l + 1) / (4 * PI) * charge * c * fac;
+            }
+            M[n + (size_t)Nd * p] = v;
+        }
+    }
+    // G = -(D M) [P x P], ℳ_EM[g] = 𝒯_g G
+    vector<double> out((size_t)G * P * P);
+    for (int p = 0; p < P; ++p)
+        for (int q = 0; q < P; ++q) {
+            double s2 = 0.0;
+            for (int n = 0; n < Nd; ++n) s2 += B.D[p + (size_t)P * n] * M[n + (size_t)Nd * q];
+            for (int g = 0; g < G; ++g) out[(size_t)g * P * P + p + (size_t)P * q] = T[g] * -s2;
+        }
+    return out;
+}
+
 // fokker_planck_scattering_matrix.jl:29-40 -- ℳ (Np x Np column-major) and λ₀ for one method.
 vector<double> fokker_planck_matrix(const string& method, int N, const string& qt, int Qdims,
                                     const vector<double>& mu, const vector<double>& eta, const vector<double>& xi,
@@ -1765,6 +1866,13 @@ struct oracle_problem {
     // NULL, or "DG"; order 0 or 1 = first order
     const char* scheme_e;
     int32_t scheme_order_e;
+    // an external magnetic field (Electromagnetic_Field; sn_flux.jl:116-120): em = 1 adds ℳ_EM Φ to
+    // the source (sn_inner_pass.jl:26-32), with the field em_B [3] (x, y, z), the particle charge
+    // and the energy boundaries Eb [G+1] (MeV, descending)
+    int32_t em;
+    double em_B[3];
+    double em_charge;
+    const double* Eb;
 };
 
 struct oracle_stats {
@@ -1790,6 +1898,7 @@ struct Solver {
     vector<double> sigt_eff;    // Σtot as sn_flux.jl:69-112 prepares it [g + G*m]
     vector<double> Mfp;         // Fokker-Planck ℳ [Np x Np] column-major (types 2, 4)
     double lam0 = 0.0;          // its λ₀
+    vector<double> Mem;         // magnetic-field ℳ_EM [G][Np x Np] (empty without a field)
     int K = 1;                  // spatial moments per cell (1D scheme order; 1 elsewhere)
     bool ho = false;            // the general 1D cell solve (order > 1 or a closure other than DD)
     bool adapt = false;         // the adaptive weighted diamond closure (scheme "AWD")
@@ -1906,6 +2015,11 @@ struct Solver {
             }
             Mfp = fokker_planck_matrix(meth, p.quad_order, qt, Qdims, A.mu, A.eta, A.xi, A.w, A.B, S3, lam0);
             for (size_t i = 0; i < sigt_eff.size(); ++i) sigt_eff[i] += p.T[i] * lam0;
+        }
+        if (p.em) {  // sn_flux.jl:116-120: magnetic fields only, λ₀_EM = 0 leaves Σtot as it is
+            if (Qdims != 3) fail("Quadrature on the unit sphere is required for electromagnetic transport.");
+            if (!p.Eb) fail("the electromagnetic term needs the energy boundaries");
+            Mem = em_matrix(p.em_B, p.em_charge, A.mu, A.eta, A.xi, A.B, G, p.Eb);
         }
         if (solver == 6)  // elastic-free: the self-scattering moment l = 0 leaves the removal
             for (int m = 0; m < Nmat; ++m)
@@ -2450,6 +2564,14 @@ struct Solver {
                         Ql[n + (size_t)Np * c] += Tm * Mfp[n + (size_t)Np * k] * phi[k + (size_t)Np * c];
             }
         }
+        // Ql += ℳ_EM Φl   (sn_inner_pass.jl:26-32)
+        if (!Mem.empty()) {
+            const double* Me = &Mem[(size_t)g * Np * Np];
+            for (int c = 0; c < Ncol; ++c)
+                for (int n = 0; n < Np; ++n)
+                    for (int k = 0; k < Np; ++k)
+                        Ql[n + (size_t)Np * c] += Me[n + (size_t)Np * k] * phi[k + (size_t)Np * c];
+        }
         vector<double> bin(bx, bx + nbx + nby + nbz);
         int nt = std::max(1, P.nthreads);
         if (nt == 1) {
@@ -2937,6 +3059,19 @@ int oracle_voronoi_neighbours(int Nd, const double* x, const double* y, const do
         return tot;
     } catch (std::exception& e) { g_err = e.what(); return -1; }
 }
+
+// ℳ_EM of a problem with a magnetic field: [G][Np x Np]; returns Np.
+int oracle_em_matrix(const oracle_problem* p, double* Mem, int cap) {
+    try {
+        Solver s(*p);
+        if (s.Mem.empty()) fail("the problem has no electromagnetic field");
+        if ((int)s.Mem.size() > cap) fail("capacity too small");
+        std::copy(s.Mem.begin(), s.Mem.end(), Mem);
+        return s.A.Np;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+double oracle_ferrer(int l, int m, double x) { return ferrer(l, m, x); }
 
 int oracle_fokker_planck(const oracle_problem* p, double* Mfp, double* lam0, int capNp) {
     try {

@@ -6,6 +6,7 @@
 // kernels.hip.
 #include "engine.hpp"
 #include "sweep_awd.hpp"
+#include "em_source.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -95,7 +96,7 @@ static bool has_fp(const rt_problem_desc& d) {
 }
 
 // Moments that carry a source: the Legendre orders with Σs_l != 0 for some material/group pair,
-// and with the Fokker-Planck term every moment that ℳ reads or writes.
+// and with the Fokker-Planck term or a magnetic field every moment that ℳ or ℳ_EM reads or writes.
 static std::vector<int> active_moments(const rt_problem_desc& d) {
     std::vector<bool> lnz(d.Ls + 1, false);
     const size_t per_l = (size_t)d.Nmat * d.G * d.G;
@@ -110,6 +111,11 @@ static std::vector<int> active_moments(const rt_problem_desc& d) {
             for (int q = 0; q < d.Np; ++q)
                 if (d.Mfp[p + (size_t)d.Np * q] != 0.0) fp[p] = fp[q] = true;
     }
+    if (d.Mem)  // the magnetic-field operator: every moment it reads or writes
+        for (int p = 0; p < d.Np; ++p)
+            for (int q = 0; q < d.Np; ++q)
+                for (int g = 0; g < d.G; ++g)
+                    if (d.Mem[g + (size_t)d.G * (p + (size_t)d.Np * q)] != 0.0) { fp[p] = fp[q] = true; break; }
     std::vector<int> act;
     for (int p = 0; p < d.Np; ++p) {
         const int64_t l = d.pl[p];
@@ -399,6 +405,18 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
             d_fpT = dalloc<double>(Tg.size()); upload(d_fpT, Tg, st);
             d_fpq = dalloc<double>((size_t)Nact * Nc);
         }
+        if (d.Mem) {
+            // magnetic-field source ℳ_EM[g] Φ (sn_inner_pass.jl:26-32) on the active moments:
+            // [G][Nact][Nact] row-major
+            em = true;
+            std::vector<double> Me((size_t)G * Nact * Nact);
+            for (int g = 0; g < G; ++g)
+                for (int k = 0; k < Nact; ++k)
+                    for (int j = 0; j < Nact; ++j)
+                        Me[((size_t)g * Nact + k) * Nact + j] = d.Mem[g + (size_t)G * (act[k] + (size_t)Np * act[j])];
+            d_emM = dalloc<double>(Me.size()); upload(d_emM, Me, st);
+            d_emq = dalloc<double>((size_t)Nact * Nc);
+        }
         d_sigw = dalloc<double>(sigw.size()); upload(d_sigw, sigw, st);
         d_sigo = dalloc<double>(sigo.size()); upload(d_sigo, sigo, st);
         // volume source: spatial moment 1 of each cell, [g][Kc * cell], rearranged on the device from
@@ -658,6 +676,7 @@ int64_t Problem::device_bytes(const rt_problem_desc& d) {
     b += (int64_t)nvec * S * 8;
     if (d.precision != RT_PREC_FP64) b += (int64_t)d.Nd * Nc * 8;  // angular flux of c = T(0) kept by batches
     if (d.csd) b += 2LL * d.Nd * Nc * 8;                              // energy-face angular fluxes (at most Nc per direction)
+    if (d.Mem) b += (int64_t)Nact_est * Nc * 8 + (int64_t)d.G * Nact_est * Nact_est * 8;  // magnetic-field source
     if (has_fp(d)) b += (int64_t)Nact_est * (Nc + Nact_est) * 8;     // Fokker-Planck source, ℳ
     b += (int64_t)std::min<int64_t>((int64_t)d.G * d.Np * Nc, std::max<int64_t>(FLUX_STAGE, (int64_t)d.G * d.Np)) * 8;
     if (split_scan_offered(d)) b += ((int64_t)d.Nd + (int64_t)split_scan_words(d)) * 8;
@@ -694,6 +713,7 @@ void Problem::fill_geometry(PassDesc& pd, int g) {
     pd.bx = d_bex[0]; pd.by = d_bex[1]; pd.bz = d_bex[2];
     pd.Nps = Nps;
     if (fp) { pd.fpT = d_fpT + (size_t)g * Nmat; pd.fpM = d_fpM; }
+    em_g = g;
     pd.ho_K = ho_K;
     pd.ho_om = d_omx;
     pd.ho_Kx = ho_Kx; pd.ho_Ky = ho_Ky; pd.ho_fc = ho_fc ? 1 : 0; pd.ho_omy = d_omy;
@@ -744,9 +764,14 @@ void Problem::launch_pass(PassDesc& pd, int prec, bool full_rows) {
     // Fokker-Planck source of a non-zero state: the contraction reads qlout + T ℳ Φ
     const bool fpsrc = pd.fpT && pd.phi_in;
     if (fpsrc) { pd.fp_qin = pd.qlout; pd.fpq = d_fpq; pd.qlout = d_fpq; }
+    // magnetic-field source of a non-zero state: then the contraction reads qlout (+ T ℳ Φ) + ℳ_EM Φ
+    const bool emsrc = em && pd.phi_in;
+    const double* em_qin = pd.qlout;
+    if (emsrc) pd.qlout = d_emq;
     k_set_pd(pd, d_pd, st);
     if (pd.bin_x) launch_bndin(d_pd, (int)bin_total, st);  // incoming fluxes of non-void faces
     if (fpsrc) launch_fp_source(d_pd, (long)Nact * Nc, st);
+    if (emsrc) launch_em_source(pd.phi_in, pd.phi_rows, d_emM + (size_t)em_g * Nact * Nact, em_qin, d_emq, Nact, Nc, st);
     auto timed = [&](int kernel, auto&& fn) {
         if (!profiling) {
             fn();

@@ -158,6 +158,9 @@ private:
     // material, and the source qlout + T ℳ Φ the contraction reads
     bool fp = false;
     double *d_fpM = nullptr, *d_fpT = nullptr, *d_fpq = nullptr;
+    bool em = false;                                 // magnetic-field source (em_source.hip)
+    double *d_emM = nullptr, *d_emq = nullptr;       // ℳ_EM per group [G][Nact][Nact], its source rows
+    int em_g = 0;                                    // group of the pass being set up
     double *d_omx = nullptr, *d_omy = nullptr, *d_omz = nullptr;  // closure weights of the general cells per axis
     double* d_ome = nullptr;   // energy closure weights (slowing-down cells)
     double* d_csdS = nullptr;  // S⁻/ΔE, S⁺/ΔE per group and material [G][Nmat][2] (slowing-down cells)

@@ -42,7 +42,7 @@ class ProblemDesc(C.Structure):
         ("order_x", C.c_int32), ("omega_x", dp),
         ("order_y", C.c_int32), ("omega_y", dp), ("partial_coupling", C.c_int32),
         ("order_z", C.c_int32), ("omega_z", dp),
-        ("order_e", C.c_int32), ("omega_e", dp), ("adaptive", C.c_int32),
+        ("order_e", C.c_int32), ("omega_e", dp), ("adaptive", C.c_int32), ("Mem", dp),
     ]
 
 

@@ -111,6 +111,11 @@ class Desc:
             d.order_e, d.omega_e = oe, ptr(k["omega_e"])
         if getattr(prob, "csd", False):
             d.partial_coupling = 0 if getattr(prob, "full_coupling", True) else 1
+        if getattr(prob, "em_B", None) is not None:  # magnetic field: ℳ_EM [G, Np, Np] column-major
+            from . import electromagnetic
+            Mem, _ = electromagnetic.moment_operator(ang, prob.em_B, prob.em_charge, prob.Eb, qd)
+            k["Mem"] = np.ascontiguousarray(Mem.reshape(-1, order="F"))
+            d.Mem = ptr(k["Mem"])
         if Mfp is not None:  # T [G, Nmat], ℳ [Np, Np], both column-major
             k["T"] = f64(T)
             k["Mfp"] = f64(Mfp)
