@@ -84,6 +84,7 @@ def lib():
                                      C.c_double, C.c_double, C.c_double, dp, dp, dp, dp]
         L.oracle_cell_bfp.argtypes = [C.c_char_p] * 4 + [C.c_int] * 6 + [dp, dp, C.c_double, C.c_double, C.c_double,
                                                                        dp, dp, dp, dp, dp, dp]
+        L.oracle_constant_linear.argtypes = [C.c_double, C.c_double, dp]
         L.oracle_em_matrix.argtypes = [C.POINTER(OracleProblem), dp, C.c_int]
         L.oracle_ferrer.argtypes = [C.c_int, C.c_int, C.c_double]
         L.oracle_ferrer.restype = C.c_double
@@ -266,6 +267,14 @@ def em_matrix(prob):
     out = np.zeros(prob.G * np_ * np_)
     _check(lib().oracle_em_matrix(C.byref(op), _p(out), len(out)))
     return np.stack([out[g * np_ * np_:(g + 1) * np_ * np_].reshape((np_, np_), order="F") for g in range(prob.G)])
+
+
+def constant_linear(x, y):
+    """(subdomain 0..11, [Qx, Qy, Tx, Ty]) of the coupled linear closure at centroid (x, y)."""
+    out = np.zeros(4)
+    r = lib().oracle_constant_linear(float(x), float(y), _p(out))
+    _check(r)
+    return r, out
 
 
 def ferrer(l, m, x):

@@ -21,6 +21,7 @@
 // the DD cell solve, the sweep, the Krylov solvers and the group loop are each pinned by a
 // closed form, an invariant, a library routine or a brute-force dense solve.
 #include <cmath>
+#include <complex>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -1447,6 +1448,245 @@ void awd1_weights_1d(double hx, double sx, double f, const double* Qn, double* w
     w[2] = Qx + 0.0;
 }
 
+// constant_linear.jl: the weights (Qx, Qy) and (Tx, Ty) of the coupled linear closure from the centroid
+// (x, y) of the cell's constant part in [-1/2, 1/2]^2, by subdomain (A, B, C types and their signs).
+namespace cl {
+double y_in(double x) {
+    const double r = std::sqrt(36 * x * x - 60 * x + 9);
+    return (-6 * x - 3 + r) / (36 * x - 30 - 6 * r);
+}
+double y_out(double x) { return (6 * x * x - 3 * x) / (6 * x + 1); }
+// subdomains: 0 A++ 1 A+- 2 A-+ 3 A-- 4 Bx+ 5 Bx- 6 By+ 7 By- 8 C++ 9 C+- 10 C-+ 11 C--
+int which(double x, double y) {
+    if (x < -0.5 || x > 0.5 || y < -0.5 || y > 0.5)
+        fail("The centroid position of the finite element constant part is out of the finite element domain.");
+    const double s6 = 1.0 / 6.0;
+    if (-0.5 <= x && x <= -s6) {
+        if (-0.5 <= y && y <= -s6) return 11;
+        if (-s6 < y && y < s6) return 5;
+        return 10;
+    }
+    if (-s6 < x && x <= 0) {
+        const double yo = y_out(-x), yi = y_in(-x);
+        if (-0.5 <= y && y <= yo) return 7;
+        if (yo < y && y < -yi) return 3;
+        if (-yi <= y && y <= yi) return 5;
+        if (yi < y && y < -yo) return 2;
+        return 6;
+    }
+    if (0 < x && x < s6) {
+        const double yo = y_out(x), yi = y_in(x);
+        if (-0.5 <= y && y <= yo) return 7;
+        if (yo < y && y < -yi) return 1;
+        if (-yi <= y && y <= yi) return 4;
+        if (yi < y && y < -yo) return 0;
+        return 6;
+    }
+    if (-0.5 <= y && y <= -s6) return 9;
+    if (-s6 < y && y < s6) return 4;
+    return 8;
+}
+void dA(double x, double y, double& d1, double& d2) {
+    if (std::fabs(x) < 1e-10 || std::fabs(y) < 1e-10) { d1 = 0; d2 = 0; return; }
+    using cd = std::complex<double>;
+    const double a = 2 * y / x, b = -6 * y / x + 3 - 6 * y, c = 9 * y / (2 * x) - 4.5 - 9 * x + 9 * y, d = 12 * x;
+    const double D0 = b * b - 3 * a * c, D1 = 2 * b * b * b - 9 * a * b * c + 27 * a * a * d;
+    const cd xi((-1.0) / 2, -std::sqrt(3.0) / 2);
+    const cd C = std::pow((D1 + std::sqrt(cd(D1 * D1 - 4 * D0 * D0 * D0, 0.0))) / 2.0, 1.0 / 3.0) * xi;
+    if (D0 == 0 && D1 == 0) d1 = -b / (3 * a);
+    else d1 = std::real(-(b + C + D0 / C) / (3 * a));
+    d2 = (2 * d1 - 3) * y / (2 * x) + 1.5;
+}
+void dB(double x, double y, double& d1, double& d2) {
+    d1 = (12 * y * y + (6 - 12 * x) * y + 2 * x) / (12 * y * y + 1);
+    d2 = (12 * y * y - (6 - 12 * x) * y + 2 * x) / (12 * y * y + 1);
+}
+void dC(double x, double y, double& d1, double& d2) { d1 = 3 * x - 0.5; d2 = 3 * y - 0.5; }
+double QAp(double i, double j) { return (6 - 2 * i * i * j) / (3 * i * i * i * j + 6 * (1 - i * i * j)); }
+double QAm(double i, double j) { return (2 * i * i * j + 6 * (1 - i * j)) / (3 * i * i * i * j + 6 * (1 - i * i * j)); }
+double QBp(double i, double j) {
+    if (std::fabs(i - j) < 1e-5) return (1 + i) / ((1 - i) * (1 + 2 * i));
+    return (1 - (i * i + j * j + i * j) / 3) / (1 - (i * i + j * j + i * j) + 0.5 * (i * i * i + j * j * j + i * i * j + i * j * j));
+}
+double QB0(double i, double j) {
+    if (std::fabs(i - j) < 1e-5 && std::fabs(i - 1) < 1e-5) return 1.0;
+    return 2.0 / 3.0 * (i + 2 * j - 3) / (i + j - 2);
+}
+double QBm(double i, double j) {
+    if (std::fabs(i - j) < 1e-5) return 1 / (1 + 2 * i);
+    return (1 - i - j + (i * i + j * j + i * j) / 3) / (1 - (i * i + j * j + i * j) + 0.5 * (i * i * i + j * j * j + i * i * j + i * j * j));
+}
+double QCp(double i) { return 2 * (2 + i) / (3 * (1 - i * i)); }
+double QCm(double i) { return 2 / (3 * (1 + i)); }
+double TAp(double i, double j) {
+    const double den = i * i * i * j - 2 * i * i * j + 2;
+    return -i * i * j * (i * i * i * j * j + 12 * i * j - 30 * i - 20 * j + 40) / (5 * den * den);
+}
+double TAm(double i, double j) {
+    const double den = i * i * i * j - 2 * i * i * j + 2;
+    return -i * j * (i * i * i * i * j * j - 8 * i * i * i * j * j + (10 * j * j - 12 * j + 30) * i * i + (40 * j - 80) * i - 40 * j + 60) /
+           (5 * den * den);
+}
+double TBden(double i, double j) {
+    const double v = i * i * i + (j - 2) * i * i + (j * j - 2 * j) * i + j * j * j - 2 * j * j + 2;
+    return 5 * v * v;
+}
+double TBp(double i, double j) {
+    if (std::fabs(i + j - 2) < 1e-5) return 0.0;
+    return -(i - j) * ((i * i * i * i + 4 * i * i * i * j + 10 * i * i * j * j + 4 * i * j * j * j + j * j * j * j) -
+                       6 * (3 * i * i + 4 * i * j + 3 * j * j) + 20 * (i + j)) / TBden(i, j);
+}
+double TBm(double i, double j) {
+    if (std::fabs(i - j) < 1e-5) return 0.0;
+    return (i - j) * ((i * i * i * i + 4 * i * i * i * j + 10 * i * i * j * j + 4 * i * j * j * j + j * j * j * j) -
+                      8 * (i * i * i + 4 * i * i * j + 4 * i * j * j + j * j * j) + 4 * (7 * i * i + 16 * i * j + 7 * j * j) -
+                      40 * (i + j) + 20) / TBden(i, j);
+}
+double TB0(double i, double j) {
+    if (std::fabs(i - j) < 1e-5) return 0.0;
+    return -(i - j) * (i * i + 6 * i * j + 3 * j * j - 8 * i - 12 * j + 10) / (5 * (i + j - 2) * (i + j - 2));
+}
+double TCp(double i, double j) {
+    if (std::fabs(i - j) < 1e-5) return -(i * i + 6 * i + 3) / (5 * (1 + i) * (1 + i));
+    if (std::fabs(i - 1) < 1e-5) return -std::numeric_limits<double>::infinity();
+    return -(i * i + 6 * i + 3) * (j - 1) / (5 * (1 + i) * (1 + i) * (i - 1));
+}
+double TCm(double i, double j) { return -(i - 1) * (j - 1) / (5 * (1 + i) * (1 + i)); }
+
+// constant_linear.jl:19-138: subdomain, δ lengths (clamped to [0, 1] within 1e-7), then Q and T
+void constant_linear(double x, double y, double* Q, double* T) {
+    const int sub = which(x, y);
+    double d1 = 0, d2 = 0;
+    switch (sub) {
+        case 0: dA(x, y, d1, d2); break;
+        case 1: dA(x, -y, d1, d2); break;
+        case 2: dA(-x, y, d1, d2); break;
+        case 3: dA(-x, -y, d1, d2); break;
+        case 4: dB(x, y, d1, d2); break;
+        case 5: dB(-x, y, d1, d2); break;
+        case 6: dB(y, x, d1, d2); break;
+        case 7: dB(-y, x, d1, d2); break;
+        case 8: dC(x, y, d1, d2); break;
+        case 9: dC(x, -y, d1, d2); break;
+        case 10: dC(-x, y, d1, d2); break;
+        default: dC(-x, -y, d1, d2); break;
+    }
+    auto clamp01 = [](double& d) {
+        if (d > 1 && d < 1.0000001) d = 1;
+        if (d < 0 && d > -0.0000001) d = 0;
+    };
+    clamp01(d1);
+    clamp01(d2);
+    if (!std::isfinite(d1) || !std::isfinite(d2) || d1 < 0 || d1 > 1 || d2 < 0 || d2 > 1)
+        fail("constant_linear: δ lengths out of [0, 1]");
+    Q[0] = Q[1] = T[0] = T[1] = 0.0;
+    switch (sub) {
+        case 0: Q[0] += QAp(d1, d2); Q[1] += QAp(d2, d1); T[0] += TAp(d1, d2); T[1] += TAp(d2, d1); break;
+        case 1: Q[0] += QAp(d1, d2); Q[1] += QAm(d2, d1); T[0] -= TAp(d1, d2); T[1] -= TAm(d2, d1); break;
+        case 2: Q[0] += QAm(d1, d2); Q[1] += QAp(d2, d1); T[0] -= TAm(d1, d2); T[1] -= TAp(d2, d1); break;
+        case 3: Q[0] += QAm(d1, d2); Q[1] += QAm(d2, d1); T[0] += TAm(d1, d2); T[1] += TAm(d2, d1); break;
+        case 4: Q[0] += QBp(d1, d2); Q[1] += QB0(d1, d2); T[0] += TBp(d1, d2); T[1] += TB0(d1, d2); break;
+        case 5: Q[0] += QBm(d1, d2); Q[1] += QB0(d1, d2); T[0] += TBm(d1, d2); T[1] -= TB0(d1, d2); break;
+        case 6: Q[0] += QB0(d1, d2); Q[1] += QBp(d1, d2); T[0] += TB0(d1, d2); T[1] += TBp(d1, d2); break;
+        case 7: Q[0] += QB0(d1, d2); Q[1] += QBm(d1, d2); T[0] -= TB0(d1, d2); T[1] += TBm(d1, d2); break;
+        case 8: Q[0] += QCp(d1); Q[1] += QCp(d2); T[0] += TCp(d1, d2); T[1] += TCp(d2, d1); break;
+        case 9: Q[0] += QCp(d1); Q[1] += QCm(d2); T[0] -= TCp(d1, d2); T[1] -= TCm(d2, d1); break;
+        case 10: Q[0] += QCm(d1); Q[1] += QCp(d2); T[0] -= TCm(d1, d2); T[1] -= TCp(d2, d1); break;
+        default: Q[0] += QCm(d1); Q[1] += QCm(d2); T[0] += TCm(d1, d2); T[1] += TCm(d2, d1); break;
+    }
+    for (int k = 0; k < 2; ++k)
+        if (!std::isfinite(Q[k]) || !std::isfinite(T[k])) fail("constant_linear: weights not finite");
+}
+}  // namespace cl
+
+// The coupled linear AWD1 of a 2D cell of orders (2, 2) with full coupling (adaptive.jl:161-232):
+// per axis the slope of the cell's source and inflows estimates the centroid of its constant part,
+// which sets (Q, T) = constant_linear(x_CM, y_CM) and the weights
+//   wx[1][m][m] = 1, wx[2][m][m] = Qx, wx[2][0][1] = sy Tx / sqrt3 (likewise wy with Qy, Ty, sx),
+// the other entries those of DG (0 for the inflow weight wx[0][m][m]).
+void awd1_weights_2d(double hx, double hy, double sx, double sy, const double* fx, const double* fy,
+                     const double* Qn, double wx[3][2][2], double wy[3][2][2]) {
+    const double r3 = std::sqrt(3.0);
+    double xcm[2] = {0.0, 0.0};
+    for (int i = 0; i < 2; ++i) {
+        double p0, p1, dQ;
+        if (i == 0) {
+            p0 = Qn[0] + hx * fx[0] + hy * fy[0];
+            p1 = Qn[1] - r3 * hx * sx * fx[0] + hy * fy[1];
+            dQ = sx * r3 * p1;
+        } else {
+            p0 = Qn[0] + hx * fx[0] + hy * fy[0];
+            p1 = Qn[2] + hx * fx[1] - r3 * hy * sy * fy[0];
+            dQ = sy * r3 * p1;
+        }
+        if (p0 != 0) dQ = dQ / p0;
+        else if (std::fabs(dQ) < 1e-8) dQ = 0;
+        else dQ = std::numeric_limits<double>::infinity() * (dQ > 0 ? 1.0 : -1.0);
+        if (std::isnan(dQ)) dQ = 0;
+        if (dQ > 2.999) dQ = 2.999;
+        if (dQ < -2.999) dQ = -2.999;
+        const double xmax = 0.49, sg = dQ > 0 ? 1.0 : (dQ < 0 ? -1.0 : 0.0), dQt = dQ;
+        dQ += 1.5 * sg;
+        if (dQ > 1.5) {
+            if (dQ > 3) xcm[i] = xmax;
+            else xcm[i] = std::min(xmax, (2 * dQ - 3 + std::sqrt(12 * dQ * dQ - 27)) / (4 * (3 * dQ)));
+        }
+        dQ = dQt + sg;
+        if (dQ < -1) xcm[i] = std::max(-xmax, (1 + dQ) / 4);
+        if (xcm[i] < -xmax) xcm[i] = -xmax;
+        if (xcm[i] > xmax) xcm[i] = xmax;
+    }
+    double Q[2], T[2];
+    cl::constant_linear(xcm[0], xcm[1], Q, T);
+    for (int n = 0; n < 3; ++n)
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b) wx[n][a][b] = wy[n][a][b] = (a == b && n > 0) ? 1.0 : 0.0;
+    wx[1][0][0] = 1; wx[2][0][0] = Q[0]; wx[1][1][1] = 1; wx[2][1][1] = Q[0]; wx[2][0][1] = sy * T[0] / r3;
+    wy[1][0][0] = 1; wy[2][0][0] = Q[1]; wy[1][1][1] = 1; wy[2][1][1] = Q[1]; wy[2][0][1] = sx * T[1] / r3;
+}
+
+// sn_2D_bte.jl:35-116 for orders (2, 2) with full coupling and weight tensors w[n][m][k] (the
+// reference's ω[n+1, m+1, k+1]): the streaming terms couple through w[jx+1][jy][iy], the inflow uses
+// w[0][m][m], and the outgoing face moment m gathers sum_k w[jx+1][m][k].
+void flux_2d_bte_tensor(double mu, double eta, double St, double dx, double dy, const double* Qn, double* fx,
+                        double* fy, double wx[3][2][2], double wy[3][2][2], double* psi) {
+    const double sx = mu > 0 ? 1.0 : (mu < 0 ? -1.0 : 0.0), sy = eta > 0 ? 1.0 : (eta < 0 ? -1.0 : 0.0);
+    const double hx = std::fabs(mu) / dx, hy = std::fabs(eta) / dy;
+    auto C = [](int i) { return std::sqrt(2.0 * i - 1.0); };  // 1-based
+    auto spow = [](double s, int e) { double v = 1.0; for (int k = 0; k < e; ++k) v *= s; return v; };
+    const int O = 2, Nm = 4;
+    vector<double> S((size_t)Nm * Nm, 0.0), Q(Nm, 0.0);
+    for (int ix = 1; ix <= O; ++ix)
+        for (int jx = 1; jx <= O; ++jx)
+            for (int iy = 1; iy <= O; ++iy)
+                for (int jy = 1; jy <= O; ++jy) {
+                    const int i = O * (iy - 1) + ix - 1, j = O * (jy - 1) + jx - 1;
+                    double& e = S[(size_t)i * Nm + j];
+                    if (i == j) e += St;
+                    if (iy == jy && ix >= jx + 1) e -= C(ix) * hx * sx * C(jx) * (1.0 - spow(-1.0, ix - jx));
+                    e += C(ix) * hx * spow(sx, ix - 1) * C(jx) * spow(sx, jx - 1) * wx[jx][jy - 1][iy - 1];
+                    if (ix == jx && iy >= jy + 1) e -= C(iy) * hy * sy * C(jy) * (1.0 - spow(-1.0, iy - jy));
+                    e += C(iy) * hy * spow(sy, iy - 1) * C(jy) * spow(sy, jy - 1) * wy[jy][jx - 1][ix - 1];
+                }
+    for (int jx = 1; jx <= O; ++jx)
+        for (int jy = 1; jy <= O; ++jy) {
+            const int j = O * (jy - 1) + jx - 1;
+            Q[j] += Qn[j];
+            Q[j] -= C(jx) * hx * (spow(sx, jx - 1) * wx[0][jy - 1][jy - 1] - spow(-sx, jx - 1)) * fx[jy - 1];
+            Q[j] -= C(jy) * hy * (spow(sy, jy - 1) * wy[0][jx - 1][jx - 1] - spow(-sy, jy - 1)) * fy[jx - 1];
+        }
+    vector<double> x = lu_solve(S, Q, Nm);
+    for (int j = 0; j < Nm; ++j) psi[j] = x[j];
+    for (int jx = 1; jx <= O; ++jx)
+        for (int jy = 1; jy <= O; ++jy) {
+            const int j = O * (jy - 1) + jx - 1;
+            if (jx == 1) fx[jy - 1] = wx[0][jy - 1][jy - 1] * fx[jy - 1];
+            if (jy == 1) fy[jx - 1] = wy[0][jx - 1][jx - 1] * fy[jx - 1];
+            for (int iy = 1; iy <= O; ++iy) fx[jy - 1] += C(jx) * spow(sx, jx - 1) * wx[jx][jy - 1][iy - 1] * x[j];
+            for (int ix = 1; ix <= O; ++ix) fy[jx - 1] += C(jy) * spow(sy, jy - 1) * wy[jy][jx - 1][ix - 1] * x[j];
+        }
+}
+
 // sn_1D_bte.jl:27-64: the K spatial Legendre moments of the cell's angular flux and the outgoing
 // face flux.  C[i] = sqrt(2i-1) (scheme_weights.jl:409-410).  adapt: the AWD weights of the cell
 // replace om (not for a tangential direction, sn_1D_bte.jl:39).
@@ -1512,9 +1752,15 @@ void flux_2d_bte(double mu, double eta, double St, double dx, double dy, const d
     const int Ox = cl.Ox, Oy = cl.Oy, Nm = cl.Nm;
     double wxa[2], wya[2];
     const double *wx = wx_in, *wy = wy_in;
-    if (adapt) {  // adaptive.jl:121-232: AWD0 for orders (1, 1); the coupled linear AWD1 is not provided
+    if (adapt && Ox == 2 && Oy == 2 && cl.fc) {  // adaptive.jl:161-232: the coupled linear AWD1
+        double wxt[3][2][2], wyt[3][2][2];
+        awd1_weights_2d(hx, hy, sx, sy, fx, fy, Qn, wxt, wyt);
+        flux_2d_bte_tensor(mu, eta, St, dx, dy, Qn, fx, fy, wxt, wyt, psi);
+        return;
+    }
+    if (adapt) {  // adaptive.jl:121-160: AWD0 for orders (1, 1)
         if (Ox != 1 || Oy != 1)
-            fail("the adaptive closure in 2D is provided for orders (1, 1) only");
+            fail("the adaptive closure in 2D: orders (1, 1), or (2, 2) with full coupling");
         const double h[2] = {hx, hy}, f[2] = {fx[0], fy[0]};
         double* wp[2] = {wxa, wya};
         awd0_weights(2, h, f, Qn[0], St, wp);
@@ -2070,7 +2316,8 @@ struct Solver {
         } else if (ndims == 2 && (K > 1 || Ky > 1 || scheme != "DD" || scheme_y != "DD")) {
             for (const string* sc : {&scheme, &scheme_y})
                 if (*sc != "DD" && *sc != "DG" && *sc != "AWD") fail("Closure relation not implemented yet.");
-            if (adapt && (K != 1 || Ky != 1)) fail("the adaptive closure in 2D is provided for orders (1, 1) only");
+            if (adapt && !((K == 1 && Ky == 1) || (K == 2 && Ky == 2 && p.partial_coupling == 0)))
+                fail("the adaptive closure in 2D: orders (1, 1), or (2, 2) with full coupling");
             omx = scheme_weights_1d(scheme, K);
             omy = scheme_weights_1d(scheme_y, Ky);
             const bool fc = p.partial_coupling == 0;
@@ -3072,6 +3319,14 @@ int oracle_em_matrix(const oracle_problem* p, double* Mem, int cap) {
 }
 
 double oracle_ferrer(int l, int m, double x) { return ferrer(l, m, x); }
+
+// constant_linear(x, y): out = (Qx, Qy, Tx, Ty); returns the subdomain (0..11) or -1 on error.
+int oracle_constant_linear(double x, double y, double* out) {
+    try {
+        cl::constant_linear(x, y, out, out + 2);
+        return cl::which(x, y);
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
 
 int oracle_fokker_planck(const oracle_problem* p, double* Mfp, double* lam0, int capNp) {
     try {

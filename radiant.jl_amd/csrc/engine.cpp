@@ -185,7 +185,10 @@ static void validate(const rt_problem_desc& d) {
                 "Krylov iterates depend on the state they run over");
         if (!sweep_awd_supported(d.ndims, std::max(1, d.order_x), d.ndims >= 2 ? std::max(1, d.order_y) : 1,
                                  d.ndims >= 3 ? std::max(1, d.order_z) : 1))
-            bad("the adaptive closure (AWD): orders 1 and 2 in 1D, order 1 on every axis in 2D and 3D");
+            bad("the adaptive closure (AWD): orders 1 and 2 in 1D, order 1 on every axis in 2D and 3D, or (2, 2) "
+                "with full coupling in 2D");
+        if (d.ndims == 2 && d.order_x == 2 && d.partial_coupling)
+            bad("the adaptive closure (AWD) of orders (2, 2) needs full coupling");
     }
     if (d.csd && ho_order(d)) {
         const int Ke = d.order_e > 1 ? d.order_e : 1;
@@ -791,7 +794,7 @@ void Problem::launch_pass(PassDesc& pd, int prec, bool full_rows) {
         if (adapt) {
             if (sp != P64) throw Error(RT_EINVAL, "the adaptive closure runs FP64 passes only");
             if (bfp) launch_sweep_bfp_awd(d_pd, Nd, awd_inv_dE, ho_lds(P64), st);
-            else launch_sweep_awd(d_pd, Nd, ndims, ndims == 1 ? ho_K : 1, ho_lds(P64), st);
+            else launch_sweep_awd(d_pd, Nd, ndims, ho_K, ho_lds(P64), st);
         } else if (bfp)
             launch_sweep_bfp(sp, d_pd, Nd, ho_KE, ho_Kx, ho_Ky, ho_Kz, ho_fc, ho_lds(sp), st);
         else if (ho_K && ndims == 3)

@@ -188,6 +188,286 @@ __global__ __launch_bounds__(256) void sweep_2d_awd_kernel(const PassDesc* __res
         for (int i = tid; i < Nx; i += nt) ((double*)d.by)[(size_t)n * Nx + i] = FY[i];
 }
 
+// ------------------------------------------------------------------- 2D coupled linear (AWD1)
+// Orders (2, 2) with full coupling (adaptive.jl:161-232 with constant_linear.jl): per cell the
+// slopes of the source and inflows place the centroid of the cell's constant part, which sets
+// (Qx, Qy, Tx, Ty) by subdomain of [-1/2, 1/2]^2; the weights are tensors w[n][m][k] (Radiant's
+// ω[n+1, m+1, k+1]): those of DG with w[2][m][m] = Q and w[2][0][1] = s T / sqrt3 of the other axis'
+// sign.  The cell's 4 x 4 system (moment jx + 2 jy) couples through w[jx+1][jy][iy], and an outgoing
+// face moment m gathers sum_k w[jx+1][m][k] (sn_2D_bte.jl:35-116).  Faces [Ny][2] and [Nx][2] in LDS.
+namespace awd1 {
+struct cx { double re, im; };
+__device__ __forceinline__ cx add(cx a, cx b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ cx mul(cx a, cx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ cx dvd(cx a, cx b) {
+    const double d = b.re * b.re + b.im * b.im;
+    return {(a.re * b.re + a.im * b.im) / d, (a.im * b.re - a.re * b.im) / d};
+}
+__device__ __forceinline__ cx cpow(cx z, double p) {  // principal value
+    const double r = hypot(z.re, z.im), t = atan2(z.im, z.re);
+    const double m = pow(r, p);
+    return {m * cos(p * t), m * sin(p * t)};
+}
+__device__ double y_in(double x) {
+    const double r = sqrt(36 * x * x - 60 * x + 9);
+    return (-6 * x - 3 + r) / (36 * x - 30 - 6 * r);
+}
+__device__ double y_out(double x) { return (6 * x * x - 3 * x) / (6 * x + 1); }
+// 0 A++ 1 A+- 2 A-+ 3 A-- 4 Bx+ 5 Bx- 6 By+ 7 By- 8 C++ 9 C+- 10 C-+ 11 C--
+__device__ int which(double x, double y) {
+    const double s6 = 1.0 / 6.0;
+    if (x <= -s6) return y <= -s6 ? 11 : (y < s6 ? 5 : 10);
+    if (x <= 0) {
+        const double yo = y_out(-x), yi = y_in(-x);
+        if (y <= yo) return 7;
+        if (y < -yi) return 3;
+        if (y <= yi) return 5;
+        if (y < -yo) return 2;
+        return 6;
+    }
+    if (x < s6) {
+        const double yo = y_out(x), yi = y_in(x);
+        if (y <= yo) return 7;
+        if (y < -yi) return 1;
+        if (y <= yi) return 4;
+        if (y < -yo) return 0;
+        return 6;
+    }
+    return y <= -s6 ? 9 : (y < s6 ? 4 : 8);
+}
+__device__ void dA(double x, double y, double& d1, double& d2) {
+    if (fabs(x) < 1e-10 || fabs(y) < 1e-10) { d1 = 0; d2 = 0; return; }
+    const double a = 2 * y / x, b = -6 * y / x + 3 - 6 * y, c = 9 * y / (2 * x) - 4.5 - 9 * x + 9 * y, d = 12 * x;
+    const double D0 = b * b - 3 * a * c, D1 = 2 * b * b * b - 9 * a * b * c + 27 * a * a * d;
+    const cx xi = {-0.5, -sqrt(3.0) / 2};
+    const cx sq = cpow({D1 * D1 - 4 * D0 * D0 * D0, 0.0}, 0.5);
+    const cx C = mul(cpow({(D1 + sq.re) / 2, sq.im / 2}, 1.0 / 3.0), xi);
+    if (D0 == 0 && D1 == 0) d1 = -b / (3 * a);
+    else {
+        const cx t = add(add({b, 0.0}, C), dvd({D0, 0.0}, C));
+        d1 = -t.re / (3 * a);
+    }
+    d2 = (2 * d1 - 3) * y / (2 * x) + 1.5;
+}
+__device__ double QAp(double i, double j) { return (6 - 2 * i * i * j) / (3 * i * i * i * j + 6 * (1 - i * i * j)); }
+__device__ double QAm(double i, double j) { return (2 * i * i * j + 6 * (1 - i * j)) / (3 * i * i * i * j + 6 * (1 - i * i * j)); }
+__device__ double QBp(double i, double j) {
+    if (fabs(i - j) < 1e-5) return (1 + i) / ((1 - i) * (1 + 2 * i));
+    return (1 - (i * i + j * j + i * j) / 3) / (1 - (i * i + j * j + i * j) + 0.5 * (i * i * i + j * j * j + i * i * j + i * j * j));
+}
+__device__ double QB0(double i, double j) {
+    if (fabs(i - j) < 1e-5 && fabs(i - 1) < 1e-5) return 1.0;
+    return 2.0 / 3.0 * (i + 2 * j - 3) / (i + j - 2);
+}
+__device__ double QBm(double i, double j) {
+    if (fabs(i - j) < 1e-5) return 1 / (1 + 2 * i);
+    return (1 - i - j + (i * i + j * j + i * j) / 3) / (1 - (i * i + j * j + i * j) + 0.5 * (i * i * i + j * j * j + i * i * j + i * j * j));
+}
+__device__ double TAden(double i, double j) { const double v = i * i * i * j - 2 * i * i * j + 2; return 5 * v * v; }
+__device__ double TAp(double i, double j) { return -i * i * j * (i * i * i * j * j + 12 * i * j - 30 * i - 20 * j + 40) / TAden(i, j); }
+__device__ double TAm(double i, double j) {
+    return -i * j * (i * i * i * i * j * j - 8 * i * i * i * j * j + (10 * j * j - 12 * j + 30) * i * i + (40 * j - 80) * i - 40 * j + 60) /
+           TAden(i, j);
+}
+__device__ double TBden(double i, double j) {
+    const double v = i * i * i + (j - 2) * i * i + (j * j - 2 * j) * i + j * j * j - 2 * j * j + 2;
+    return 5 * v * v;
+}
+__device__ double TBp(double i, double j) {
+    if (fabs(i + j - 2) < 1e-5) return 0.0;
+    return -(i - j) * ((i * i * i * i + 4 * i * i * i * j + 10 * i * i * j * j + 4 * i * j * j * j + j * j * j * j) -
+                       6 * (3 * i * i + 4 * i * j + 3 * j * j) + 20 * (i + j)) / TBden(i, j);
+}
+__device__ double TBm(double i, double j) {
+    if (fabs(i - j) < 1e-5) return 0.0;
+    return (i - j) * ((i * i * i * i + 4 * i * i * i * j + 10 * i * i * j * j + 4 * i * j * j * j + j * j * j * j) -
+                      8 * (i * i * i + 4 * i * i * j + 4 * i * j * j + j * j * j) + 4 * (7 * i * i + 16 * i * j + 7 * j * j) -
+                      40 * (i + j) + 20) / TBden(i, j);
+}
+__device__ double TB0(double i, double j) {
+    if (fabs(i - j) < 1e-5) return 0.0;
+    return -(i - j) * (i * i + 6 * i * j + 3 * j * j - 8 * i - 12 * j + 10) / (5 * (i + j - 2) * (i + j - 2));
+}
+__device__ double TCp(double i, double j) {
+    if (fabs(i - j) < 1e-5) return -(i * i + 6 * i + 3) / (5 * (1 + i) * (1 + i));
+    if (fabs(i - 1) < 1e-5) return -INFINITY;
+    return -(i * i + 6 * i + 3) * (j - 1) / (5 * (1 + i) * (1 + i) * (i - 1));
+}
+__device__ double TCm(double i, double j) { return -(i - 1) * (j - 1) / (5 * (1 + i) * (1 + i)); }
+__device__ void constant_linear(double x, double y, double* Q, double* T) {
+    const int sub = which(x, y);
+    double d1, d2;
+    switch (sub) {
+        case 0: dA(x, y, d1, d2); break;
+        case 1: dA(x, -y, d1, d2); break;
+        case 2: dA(-x, y, d1, d2); break;
+        case 3: dA(-x, -y, d1, d2); break;
+        case 4: d1 = (12 * y * y + (6 - 12 * x) * y + 2 * x) / (12 * y * y + 1); d2 = (12 * y * y - (6 - 12 * x) * y + 2 * x) / (12 * y * y + 1); break;
+        case 5: d1 = (12 * y * y + (6 + 12 * x) * y - 2 * x) / (12 * y * y + 1); d2 = (12 * y * y - (6 + 12 * x) * y - 2 * x) / (12 * y * y + 1); break;
+        case 6: d1 = (12 * x * x + (6 - 12 * y) * x + 2 * y) / (12 * x * x + 1); d2 = (12 * x * x - (6 - 12 * y) * x + 2 * y) / (12 * x * x + 1); break;
+        case 7: d1 = (12 * x * x + (6 + 12 * y) * x - 2 * y) / (12 * x * x + 1); d2 = (12 * x * x - (6 + 12 * y) * x - 2 * y) / (12 * x * x + 1); break;
+        case 8: d1 = 3 * x - 0.5; d2 = 3 * y - 0.5; break;
+        case 9: d1 = 3 * x - 0.5; d2 = -3 * y - 0.5; break;
+        case 10: d1 = -3 * x - 0.5; d2 = 3 * y - 0.5; break;
+        default: d1 = -3 * x - 0.5; d2 = -3 * y - 0.5; break;
+    }
+    if (d1 > 1 && d1 < 1.0000001) d1 = 1;
+    if (d1 < 0 && d1 > -0.0000001) d1 = 0;
+    if (d2 > 1 && d2 < 1.0000001) d2 = 1;
+    if (d2 < 0 && d2 > -0.0000001) d2 = 0;
+    switch (sub) {
+        case 0: Q[0] = QAp(d1, d2); Q[1] = QAp(d2, d1); T[0] = TAp(d1, d2); T[1] = TAp(d2, d1); break;
+        case 1: Q[0] = QAp(d1, d2); Q[1] = QAm(d2, d1); T[0] = -TAp(d1, d2); T[1] = -TAm(d2, d1); break;
+        case 2: Q[0] = QAm(d1, d2); Q[1] = QAp(d2, d1); T[0] = -TAm(d1, d2); T[1] = -TAp(d2, d1); break;
+        case 3: Q[0] = QAm(d1, d2); Q[1] = QAm(d2, d1); T[0] = TAm(d1, d2); T[1] = TAm(d2, d1); break;
+        case 4: Q[0] = QBp(d1, d2); Q[1] = QB0(d1, d2); T[0] = TBp(d1, d2); T[1] = TB0(d1, d2); break;
+        case 5: Q[0] = QBm(d1, d2); Q[1] = QB0(d1, d2); T[0] = TBm(d1, d2); T[1] = -TB0(d1, d2); break;
+        case 6: Q[0] = QB0(d1, d2); Q[1] = QBp(d1, d2); T[0] = TB0(d1, d2); T[1] = TBp(d1, d2); break;
+        case 7: Q[0] = QB0(d1, d2); Q[1] = QBm(d1, d2); T[0] = -TB0(d1, d2); T[1] = TBm(d1, d2); break;
+        case 8: Q[0] = 2 * (2 + d1) / (3 * (1 - d1 * d1)); Q[1] = 2 * (2 + d2) / (3 * (1 - d2 * d2)); T[0] = TCp(d1, d2); T[1] = TCp(d2, d1); break;
+        case 9: Q[0] = 2 * (2 + d1) / (3 * (1 - d1 * d1)); Q[1] = 2 / (3 * (1 + d2)); T[0] = -TCp(d1, d2); T[1] = -TCm(d2, d1); break;
+        case 10: Q[0] = 2 / (3 * (1 + d1)); Q[1] = 2 * (2 + d2) / (3 * (1 - d2 * d2)); T[0] = -TCm(d1, d2); T[1] = -TCp(d2, d1); break;
+        default: Q[0] = 2 / (3 * (1 + d1)); Q[1] = 2 / (3 * (1 + d2)); T[0] = TCm(d1, d2); T[1] = TCm(d2, d1); break;
+    }
+}
+// the centroid coordinate of one axis from its slope estimate (adaptive.jl:172-209)
+__device__ double centroid(double p0, double dQ) {
+    if (p0 != 0) dQ = dQ / p0;
+    else if (fabs(dQ) < 1e-8) dQ = 0;
+    else dQ = dQ > 0 ? INFINITY : -INFINITY;
+    if (isnan(dQ)) dQ = 0;
+    if (dQ > 2.999) dQ = 2.999;
+    if (dQ < -2.999) dQ = -2.999;
+    const double xmax = 0.49, sg = dQ > 0 ? 1.0 : (dQ < 0 ? -1.0 : 0.0), dQt = dQ;
+    double x = 0.0;
+    dQ += 1.5 * sg;
+    if (dQ > 1.5) x = dQ > 3 ? xmax : fmin(xmax, (2 * dQ - 3 + sqrt(12 * dQ * dQ - 27)) / (4 * (3 * dQ)));
+    dQ = dQt + sg;
+    if (dQ < -1) x = fmax(-xmax, (1 + dQ) / 4);
+    return fmin(fmax(x, -xmax), xmax);
+}
+}  // namespace awd1
+
+__global__ __launch_bounds__(256) void sweep_2d_awd1_kernel(const PassDesc* __restrict__ pds) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const PassDesc& d = pds[0];
+    const int n = blockIdx.x, tid = threadIdx.x, nt = blockDim.x;
+    const int Nx = d.Nx, Ny = d.Ny;
+    double* FX = (double*)smem;  // [Ny][2]
+    double* FY = FX + 2 * Ny;    // [Nx][2]
+    const double mu = d.mu[n], eta = d.eta[n];
+    const bool fxs = mu >= 0, fys = eta >= 0;
+    const double sx = mu > 0 ? 1.0 : (mu < 0 ? -1.0 : 0.0), sy = eta > 0 ? 1.0 : (eta < 0 ? -1.0 : 0.0);
+    const double r3 = sqrt(3.0);
+    const double Cc[2] = {1.0, r3}, px[2] = {1.0, sx}, nx[2] = {1.0, -sx}, py[2] = {1.0, sy}, ny[2] = {1.0, -sy};
+    double* __restrict__ psi = (double*)d.psi + (size_t)n * d.Nc;
+    for (int i = tid; i < 2 * Ny; i += nt) FX[i] = d.bin_x ? d.bin_x[(size_t)n * Ny * 2 + i] : 0.0;
+    for (int i = tid; i < 2 * Nx; i += nt) FY[i] = d.bin_y ? d.bin_y[(size_t)n * Nx * 2 + i] : 0.0;
+    __syncthreads();
+    for (int dg = 0; dg < Nx + Ny - 1; ++dg) {
+        const int k0 = dg - (Ny - 1) > 0 ? dg - (Ny - 1) : 0, k1 = dg < Nx - 1 ? dg : Nx - 1;
+        for (int kx = k0 + tid; kx <= k1; kx += nt) {
+            const int ky = dg - kx;
+            const int ix = fxs ? kx : Nx - 1 - kx, iy = fys ? ky : Ny - 1 - ky;
+            const size_t c = (size_t)ix + (size_t)Nx * iy;
+            const double hx = fabs(mu) / d.dx[ix], hy = fabs(eta) / d.dy[iy];
+            const double St = d.sigt[d.mat[c * 4]];
+            double* fx = FX + 2 * (size_t)iy;
+            double* fy = FY + 2 * (size_t)ix;
+            double Qn[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) Qn[k] = psi[(size_t)k + 4 * c];
+            // weights
+            const double p0 = Qn[0] + hx * fx[0] + hy * fy[0];
+            const double xc = awd1::centroid(p0, sx * r3 * (Qn[1] - r3 * hx * sx * fx[0] + hy * fy[1]));
+            const double yc = awd1::centroid(p0, sy * r3 * (Qn[2] + hx * fx[1] - r3 * hy * sy * fy[0]));
+            double Qw[2], Tw[2];
+            awd1::constant_linear(xc, yc, Qw, Tw);
+            double wx[3][2][2], wy[3][2][2];
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b)
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) wx[a][b][e] = wy[a][b][e] = (b == e && a > 0) ? 1.0 : 0.0;
+            wx[2][0][0] = Qw[0]; wx[2][1][1] = Qw[0]; wx[2][0][1] = sy * Tw[0] / r3;
+            wy[2][0][0] = Qw[1]; wy[2][1][1] = Qw[1]; wy[2][0][1] = sx * Tw[1] / r3;
+            // the 4 x 4 system, moment jx + 2 jy
+            double S[4][4], q[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) S[i][j] = i == j ? St : 0.0;
+#pragma unroll
+            for (int ax = 0; ax < 2; ++ax)
+#pragma unroll
+                for (int ay = 0; ay < 2; ++ay) {
+                    const int i = ax + 2 * ay;
+#pragma unroll
+                    for (int bx = 0; bx < 2; ++bx)
+#pragma unroll
+                        for (int by = 0; by < 2; ++by) {
+                            const int j = bx + 2 * by;
+                            if (ay == by && ax > bx && ((ax - bx) & 1)) S[i][j] -= 2.0 * Cc[ax] * hx * sx * Cc[bx];
+                            S[i][j] += Cc[ax] * hx * px[ax] * Cc[bx] * px[bx] * wx[bx + 1][by][ay];
+                            if (ax == bx && ay > by && ((ay - by) & 1)) S[i][j] -= 2.0 * Cc[ay] * hy * sy * Cc[by];
+                            S[i][j] += Cc[ay] * hy * py[ay] * Cc[by] * py[by] * wy[by + 1][bx][ax];
+                        }
+                    q[i] = Qn[i] - Cc[ax] * hx * (px[ax] * wx[0][ay][ay] - nx[ax]) * fx[ay] -
+                           Cc[ay] * hy * (py[ay] * wy[0][ax][ax] - ny[ay]) * fy[ax];
+                }
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+#pragma unroll
+                for (int i = p + 1; i < 4; ++i) {
+                    const bool sw = fabs(S[i][p]) > fabs(S[p][p]);
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const double t = S[p][j];
+                        S[p][j] = sw ? S[i][j] : t;
+                        S[i][j] = sw ? t : S[i][j];
+                    }
+                    const double tq = q[p];
+                    q[p] = sw ? q[i] : tq;
+                    q[i] = sw ? tq : q[i];
+                }
+#pragma unroll
+                for (int i = p + 1; i < 4; ++i) {
+                    const double f = S[i][p] / S[p][p];
+#pragma unroll
+                    for (int j = p + 1; j < 4; ++j) S[i][j] -= f * S[p][j];
+                    q[i] -= f * q[p];
+                }
+            }
+#pragma unroll
+            for (int i = 3; i >= 0; --i) {
+                double v = q[i];
+#pragma unroll
+                for (int j = i + 1; j < 4; ++j) v -= S[i][j] * q[j];
+                q[i] = v / S[i][i];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) psi[(size_t)k + 4 * c] = q[k];
+            double ox[2] = {wx[0][0][0] * fx[0], wx[0][1][1] * fx[1]}, oy[2] = {wy[0][0][0] * fy[0], wy[0][1][1] * fy[1]};
+#pragma unroll
+            for (int ax = 0; ax < 2; ++ax)
+#pragma unroll
+                for (int ay = 0; ay < 2; ++ay) {
+                    const double v = q[ax + 2 * ay];
+                    ox[ay] += Cc[ax] * px[ax] * (wx[ax + 1][ay][0] + wx[ax + 1][ay][1]) * v;
+                    oy[ax] += Cc[ay] * py[ay] * (wy[ay + 1][ax][0] + wy[ay + 1][ax][1]) * v;
+                }
+            fx[0] = ox[0]; fx[1] = ox[1];
+            fy[0] = oy[0]; fy[1] = oy[1];
+        }
+        __syncthreads();
+    }
+    if (d.bx)
+        for (int i = tid; i < 2 * Ny; i += nt) ((double*)d.bx)[(size_t)n * Ny * 2 + i] = FX[i];
+    if (d.by)
+        for (int i = tid; i < 2 * Nx; i += nt) ((double*)d.by)[(size_t)n * Nx * 2 + i] = FY[i];
+}
+
 // ------------------------------------------------------------------------------------------- 3D
 __global__ __launch_bounds__(256) void sweep_3d_awd_kernel(const PassDesc* __restrict__ pds) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -328,7 +608,7 @@ static void awd_launch(F fn, dim3 grid, dim3 block, size_t lds, hipStream_t st, 
 
 bool sweep_awd_supported(int ndims, int Kx, int Ky, int Kz) {
     if (ndims == 1) return Kx == 1 || Kx == 2;
-    if (ndims == 2) return Kx == 1 && Ky == 1;
+    if (ndims == 2) return (Kx == 1 && Ky == 1) || (Kx == 2 && Ky == 2);  // (2, 2): full coupling
     return ndims == 3 && Kx == 1 && Ky == 1 && Kz == 1;
 }
 
@@ -341,12 +621,14 @@ void launch_sweep_awd(const PassDesc* d_pd, int Nd, int ndims, int K, size_t lds
         else if (K == 2) awd_launch(sweep_1d_awd_kernel<2>, grid, dim3(64), 0, st, d_pd);
         else throw std::runtime_error("1D adaptive closure of order other than 1 or 2");
     } else {
-        if (K != 1) throw std::runtime_error("2D/3D adaptive closure of order other than 1");
-        auto fn = ndims == 2 ? sweep_2d_awd_kernel : sweep_3d_awd_kernel;
-        static bool attr[2] = {false, false};
-        if (!attr[ndims - 2]) {
+        if (K != 1 && !(ndims == 2 && K == 4))
+            throw std::runtime_error("adaptive closure: order 1 in 2D/3D or (2, 2) with full coupling in 2D");
+        auto fn = ndims == 3 ? sweep_3d_awd_kernel : (K == 4 ? sweep_2d_awd1_kernel : sweep_2d_awd_kernel);
+        const int slot = ndims == 3 ? 2 : (K == 4 ? 1 : 0);
+        static bool attr[3] = {false, false, false};
+        if (!attr[slot]) {
             RT_HIP(hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-            attr[ndims - 2] = true;
+            attr[slot] = true;
         }
         awd_launch(fn, dim3(Nd), dim3(256), lds, st, d_pd);
     }

@@ -251,8 +251,12 @@ def test_awd_solve_is_homogeneous_not_linear():
 def test_awd_refusals():
     O = _O()
     from radiant_hip import problems as P
+    p = _awd(P.small_problem(2, N=4, G=1, L=1), scheme_order=2, scheme_order_y=1)
+    with pytest.raises(RuntimeError, match="orders \\(1, 1\\), or \\(2, 2\\)"):
+        O.solve(p)
     p = _awd(P.small_problem(2, N=4, G=1, L=1), scheme_order=2, scheme_order_y=2)
-    with pytest.raises(RuntimeError, match="orders \\(1, 1\\) only"):
+    p.full_coupling = False
+    with pytest.raises(RuntimeError, match="orders \\(1, 1\\), or \\(2, 2\\)"):
         O.solve(p)
     p = _awd(P.small_problem(3, N=3, G=1, L=1), scheme_order_z=2)
     with pytest.raises(RuntimeError, match="orders \\(1, 1, 1\\) only"):
@@ -264,6 +268,84 @@ def test_awd_refusals():
     p.scheme_y = "AWD"               # one adaptive axis makes both adaptive
     q = _awd(P.small_problem(2, N=4, G=1, L=1))
     assert rel_err(O.solve(p)[0], O.solve(q)[0]) == 0.0
+
+
+def _y_in(x):
+    r = np.sqrt(36 * x * x - 60 * x + 9)
+    return (-6 * x - 3 + r) / (36 * x - 30 - 6 * r)
+
+
+def _y_out(x):
+    return (6 * x * x - 3 * x) / (6 * x + 1)
+
+
+def test_constant_linear():
+    """constant_linear.jl: at the centre the closure is DG (Q = 1, T = 0); (Q, T) are continuous
+    across every boundary of the twelve subdomains (a wrong formula in one of them leaves a jump
+    there); swapping x and y swaps the axes' weights."""
+    O = _O()
+    assert np.allclose(O.constant_linear(0.0, 0.0)[1], [1.0, 1.0, 0.0, 0.0], atol=1e-15)
+    eps = 1e-9
+    pairs = []
+    for y in np.linspace(-0.45, 0.45, 37):
+        for x0 in (-1 / 6, 1 / 6, 0.0):
+            pairs.append(((x0 - eps, y), (x0 + eps, y)))
+    for x in np.linspace(0.005, 1 / 6 - 0.005, 15):
+        for sx in (1, -1):
+            for yb in (_y_in(x), -_y_in(x), _y_out(x), -_y_out(x)):
+                pairs.append(((sx * x, yb - eps), (sx * x, yb + eps)))
+    for x in list(np.linspace(-0.45, -0.17, 8)) + list(np.linspace(0.17, 0.45, 8)):
+        for y0 in (-1 / 6, 1 / 6):
+            pairs.append(((x, y0 - eps), (x, y0 + eps)))
+    subs = set()
+    for a, b in pairs:
+        sa, va = O.constant_linear(*a)
+        sb, vb = O.constant_linear(*b)
+        subs |= {sa, sb}
+        assert np.abs(va - vb).max() < 1e-5, (a, b, sa, sb, va, vb)
+    assert subs == set(range(12))            # every subdomain met
+    for x in np.linspace(-0.45, 0.45, 13):
+        for y in np.linspace(-0.45, 0.45, 13):
+            v, w = O.constant_linear(x, y)[1], O.constant_linear(y, x)[1]
+            assert np.allclose(v, w[[1, 0, 3, 2]], rtol=0, atol=1e-13)
+
+
+def test_awd1_2d_cell():
+    """The coupled linear AWD1 cell of orders (2, 2): with source slopes that cancel the inflow
+    terms of the centroid estimate (x_CM = y_CM = 0) it is the DG cell; otherwise it differs."""
+    O = _O()
+    mu, eta, St, dx, dy = 0.6, -0.45, 1.1, 0.8, 1.3
+    hx, hy, sx, sy = abs(mu) / dx, abs(eta) / dy, 1.0, -1.0
+    fx, fy = [0.7, 0.1], [0.4, -0.2]
+    r3 = np.sqrt(3.0)
+    q1 = r3 * hx * sx * fx[0] - hy * fy[1]   # the x slope estimate vanishes
+    q2 = -hx * fx[1] + r3 * hy * sy * fy[0]  # the y slope estimate vanishes
+    Qn = [1.0, q1, q2, 0.05]
+    a = O.cell_2d(("AWD", "AWD"), (2, 2), True, mu, eta, St, dx, dy, Qn, fx, fy)
+    g = O.cell_2d(("DG", "DG"), (2, 2), True, mu, eta, St, dx, dy, Qn, fx, fy)
+    for u, v in zip(a, g):
+        assert np.allclose(u, v, rtol=1e-14, atol=1e-15)
+    Qs = [1.0, q1 + 0.8, q2 - 0.5, 0.05]
+    a = O.cell_2d(("AWD", "AWD"), (2, 2), True, mu, eta, St, dx, dy, Qs, fx, fy)
+    g = O.cell_2d(("DG", "DG"), (2, 2), True, mu, eta, St, dx, dy, Qs, fx, fy)
+    assert not np.allclose(a[0], g[0])
+
+
+def test_awd1_2d_solve():
+    O = _O()
+    from radiant_hip import problems as P
+    # GMRES: here the fixed-point iterations stall short of 1e-12 (the pass is not a contraction
+    # near its fixed point once the weights adapt), GMRES and BiCGStab converge
+    kw = dict(N=5, G=2, L=1, tol=1e-13, seed=12, accel="gmres")
+    p = _awd(P.small_problem(2, **kw), scheme_order=2, scheme_order_y=2)
+    phi, st = O.solve(p)
+    assert all(s["converged"] for s in st)
+    q = _awd(P.small_problem(2, **kw), scheme_order=2, scheme_order_y=2)
+    q.qvol = 2.0 * p.qvol
+    assert rel_err(O.solve(q)[0], 2.0 * phi) < 1e-10
+    r = P.small_problem(2, **kw)
+    r.scheme, r.scheme_y, r.scheme_order, r.scheme_order_y = "DG", "DG", 2, 2
+    assert rel_err(phi, O.solve(r)[0]) > 1e-4
 
 
 # ------------------------------------------------------------------------------------- GPU
@@ -348,6 +430,26 @@ def test_gpu_awd_slowing_down(oracle, case):
     assert e < 1e-10 and ec < 1e-10 and all(s["converged"] for s in st)
 
 
+AWD1_2D = [(None, "gmres"), ([1, 0, 2, 2], "bicgstab"), ([2, 2, 1, 0], "gmres")]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", AWD1_2D, ids=["void-gmres", "bc-bicgstab", "bc-gmres"])
+def test_gpu_awd1_2d(oracle, case):
+    """The coupled linear AWD1 of orders (2, 2) in 2D: FP64 solves against the oracle."""
+    from radiant_hip import problems as P, solver as S
+    bc, accel = case
+    p = _awd(P.small_problem(2, N=8, G=2, L=1, tol=1e-12, seed=95, accel=accel, bc=bc), scheme_order=2,
+             scheme_order_y=2)
+    ref, rst = oracle.solve(p)
+    assert all(s["converged"] for s in rst)
+    phi, st = S.compute_flux(p)
+    e = rel_err(phi, ref)
+    print("awd1 2d", case, e, flush=True)
+    sys.stdout.flush()
+    assert e < 1e-10 and all(s["converged"] for s in st)
+
+
 @pytest.mark.gpu
 def test_gpu_awd_refusals():
     """Mixed precision (its reconstruction T(z) = A z + T(0) assumes a linear pass), the reduced
@@ -359,6 +461,6 @@ def test_gpu_awd_refusals():
     with pytest.raises(Exception, match="faithful state"):
         S.compute_flux(p, state="reduced")
     with pytest.raises(Exception, match="adaptive"):
-        S.compute_flux(_awd(P.small_problem(2, N=6, G=1, L=1), scheme_order=2))
+        S.compute_flux(_awd(P.small_problem(2, N=6, G=1, L=1), scheme_order=2))   # orders (2, 1)
     with pytest.raises(ValueError, match="Closure relation"):  # scheme_weights.jl has no AWD weights of order 3
         S.compute_flux(_awd(P.small_problem(1, N=6, G=1, L=1), scheme_order=3))
