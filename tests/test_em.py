@@ -8,7 +8,8 @@ CPU pins of the operator ℳ_EM[g] = -𝒯_g D M:
 - M[n, p] against the chain rule of the real harmonic p along the rotation dΩ/dt = Ω × B (partial
   derivatives by central differences, the rates checked on Rodrigues' rotation about B), times
   charge c (2l+1)/(4π), at every direction off the polar axis; for m = -l the reference's formula
-  leaves the ∂Y/∂μ term out, and so does the restatement (the pole limits stay unpinned);
+  leaves the ∂Y/∂μ term out, and so does the restatement; at the poles its limits equal the
+  rotation's derivative except for the sign of the m < 0 rows at μ = -1, kept as well;
 - host (numpy) and oracle (C++) ℳ_EM agree on four quadrature sets; its l = 0 column vanishes;
 - the pass adds ℳ_EM Φ to the source of every cell (an oracle pass with the field equals one without
   it whose source carries ℳ_EM z; ℳ_EM is not symmetric, so a transposed operator fails);
@@ -112,6 +113,37 @@ def test_derivative_matrix_is_the_lorentz_rotation(qt, N):
             assert abs(M[n, q] - want) <= 1e-6 * max(1.0, abs(want)), (n, l, m, M[n, q], want)
         checked += 1
     assert checked >= len(ang.mu) - 2
+
+
+def test_derivative_matrix_at_the_poles():
+    """The rows at the polar-axis directions (μ = ±1, where φ is undefined and the reference uses
+    limit formulas, electromagnetic_scattering_matrix.jl:94-116) against a finite difference of the
+    harmonics along Rodrigues' rotation: equal at μ = +1, and at μ = -1 for m >= 0; at μ = -1 the
+    reference's limits give the m < 0 rows the opposite sign, and the restatement keeps that."""
+    from radiant_hip import electromagnetic as EM, problems as P
+    p = P.small_problem(3, N=3, G=1, L=3, quad_type="lebedev", quad_order=7)
+    ang, _ = _ang(p)
+    M = EM.derivative_matrix(ang, B_FIELD, -1.0)
+    Bv = np.asarray(B_FIELD)
+    nB = np.linalg.norm(Bv)
+    k = Bv / nB
+    t = 1e-4
+    poles = 0
+    for n in range(len(ang.mu)):
+        O = np.array([ang.mu[n], ang.eta[n], ang.xi[n]])
+        if abs(abs(O[0]) - 1) > 1e-12:
+            continue
+        poles += 1
+        rot = lambda a: O * math.cos(a) + np.cross(k, O) * math.sin(a) + k * np.dot(k, O) * (1 - math.cos(a))
+        Op, Om = rot(-nB * t), rot(nB * t)
+        for q in range(len(ang.pl)):
+            l, m = int(ang.pl[q]), int(ang.pm[q])
+            Y = lambda v: _real_sh(l, m, max(-1.0, min(1.0, v[0])), math.atan2(v[2], v[1]))
+            want = -1.0 * EM.C_LIGHT * (2 * l + 1) / (4 * math.pi) * (Y(Op) - Y(Om)) / (2 * t)
+            if O[0] < 0 and m < 0:
+                want = -want
+            assert abs(M[n, q] - want) <= 1e-6 * max(1.0, abs(want)), (O[0], l, m, M[n, q], want)
+    assert poles == 2
 
 
 @pytest.mark.parametrize("nd,qt,N", [(3, "carlson", 4), (3, "lebedev", 5), (2, "gauss-legendre-chebychev", 4),
