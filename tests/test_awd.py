@@ -464,3 +464,14 @@ def test_gpu_awd_refusals():
         S.compute_flux(_awd(P.small_problem(2, N=6, G=1, L=1), scheme_order=2))   # orders (2, 1)
     with pytest.raises(ValueError, match="Closure relation"):  # scheme_weights.jl has no AWD weights of order 3
         S.compute_flux(_awd(P.small_problem(1, N=6, G=1, L=1), scheme_order=3))
+
+
+@pytest.mark.gpu
+def test_gpu_awd_served(oracle):
+    """An adaptive request through an FP64 server (worker thread, faithful state) equals the oracle."""
+    from radiant_hip import problems as P, solver as S
+    p = _awd(P.small_problem(2, N=8, G=2, L=1, tol=1e-12, seed=72))
+    ref, _ = oracle.solve(p)
+    e = rel_err(S.serve_alone(p, "fp64"), ref)
+    print("awd served", e, flush=True)
+    assert e < 1e-10

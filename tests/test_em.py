@@ -236,3 +236,16 @@ def test_gpu_field_in_batch(oracle):
         e = rel_err(ph, ref)
         print("field batch", e, flush=True)
         assert e < 1e-6
+
+
+@pytest.mark.gpu
+def test_gpu_field_served(oracle):
+    """A request with a field through the server (mixed FP32): the lockstep engine does not take it,
+    so it runs on its own stream; within 1e-6 of the oracle."""
+    from radiant_hip import problems as P, solver as S
+    p = _field(P.small_problem(2, N=6, G=2, L=1, tol=1e-12, quad_type="gauss-legendre-chebychev", quad_order=4),
+               B=B_SOLVE)
+    ref, _ = oracle.solve(p)
+    e = rel_err(S.serve_alone(p, "mixed-fp32"), ref)
+    print("field served", e, flush=True)
+    assert e < 1e-6
