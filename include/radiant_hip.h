@@ -150,7 +150,8 @@ typedef struct rt_problem_desc {
      * incoming face fluxes and source moments in every sweep (omega_* are not used).  Orders 1 and 2
      * in 1D (AWD0, AWD1), order 1 on every axis in 2D and 3D (AWD0), orders (2, 2) with full
      * coupling in 2D (the coupled linear AWD1 of constant_linear.jl); with slowing down (csd = 1)
-     * order 1 on every axis and in energy, the energy axis adapted too (1/ΔE its h),
+     * order 1 on every axis and in energy, the energy axis adapted too (1/ΔE its h), or orders
+     * (2, 2) in (x, E) with full coupling in 1D,
      * precision RT_PREC_FP64 and state_mode RT_STATE_FAITHFUL only: the closure is not linear in the
      * flux, so the mixed-precision solve's sweep-free reconstruction T(z) = A z + T(0) does not hold
      * for it, and the Krylov iterates depend on the state they run over (the faithful state is the

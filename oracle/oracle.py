@@ -84,6 +84,7 @@ def lib():
                                      C.c_double, C.c_double, C.c_double, dp, dp, dp, dp]
         L.oracle_cell_bfp.argtypes = [C.c_char_p] * 4 + [C.c_int] * 6 + [dp, dp, C.c_double, C.c_double, C.c_double,
                                                                        dp, dp, dp, dp, dp, dp]
+        L.oracle_cell_bfp1d_awd1.argtypes = [C.c_double] * 6 + [dp] * 4
         L.oracle_constant_linear.argtypes = [C.c_double, C.c_double, dp]
         L.oracle_em_matrix.argtypes = [C.POINTER(OracleProblem), dp, C.c_int]
         L.oracle_ferrer.argtypes = [C.c_int, C.c_int, C.c_double]
@@ -267,6 +268,15 @@ def em_matrix(prob):
     out = np.zeros(prob.G * np_ * np_)
     _check(lib().oracle_em_matrix(C.byref(op), _p(out), len(out)))
     return np.stack([out[g * np_ * np_:(g + 1) * np_ * np_].reshape((np_, np_), order="F") for g in range(prob.G)])
+
+
+def cell_bfp1d_awd1(mu, dx, St, Sm, Sp, dE, Qn, fe, fx):
+    """One 1D slowing-down cell of orders (2, 2) with the coupled linear AWD1: (psi[4], fe[2], fx[2])."""
+    arr = lambda v: np.ascontiguousarray(v, np.float64).copy()
+    q, e, x = arr(Qn), arr(fe), arr(fx)
+    psi = np.zeros(4)
+    _check(lib().oracle_cell_bfp1d_awd1(mu, dx, St, Sm, Sp, dE, _p(q), _p(e), _p(x), _p(psi)))
+    return psi, e, x
 
 
 def constant_linear(x, y):

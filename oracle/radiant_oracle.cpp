@@ -1961,6 +1961,65 @@ struct Cell4 {
 // incoming energy, x, y and z face moments, overwritten with the outgoing ones; w[0..3]: the
 // closure weights of each axis (scheme_weights_1d).  tang1d: the 1D tangential guard
 // (|μ| <= 1e-10: no streaming, the x face passes through, sn_1D_bfp.jl:52-54).
+// flux_1D_BFP (sn_1D_bfp.jl:39-125) of orders (2, 2) in (x, E) with full coupling and the coupled
+// linear AWD1: adaptive.jl's 2D closure on the axes (x, E) with h = (hx, 1/ΔE), signs (sx, -1), the x
+// face and the energy face as inflows and the cell's source moments in their own order (energy
+// fastest, as the reference hands them over); then the cell with weight tensors wx[n][jE][iE],
+// wE[n][jx][ix] and the in-group stopping-power term.  Moment (ix, iE) at 2 ix + iE; f[1] the x face
+// over the energy moments, f[0] the energy face over the x moments.
+void flux_bfp1d_awd1(double mu, double dx, double St, double Sm, double Sp, const double* Sv, const double* W,
+                     const double* Qn, double* fE, double* fx, double inv_dE, double* psi) {
+    const bool tang = std::fabs(mu) <= 1e-10;
+    const double sx = tang ? 1.0 : (mu > 0 ? 1.0 : -1.0), hx = tang ? 0.0 : std::fabs(mu) / dx;
+    double wx[3][2][2], wE[3][2][2];
+    if (tang) {  // no adaptation: DG
+        for (int n = 0; n < 3; ++n)
+            for (int a = 0; a < 2; ++a)
+                for (int b = 0; b < 2; ++b) wx[n][a][b] = wE[n][a][b] = (a == b && n > 0) ? 1.0 : 0.0;
+    } else {
+        awd1_weights_2d(hx, inv_dE, sx, -1.0, fx, fE, Qn, wx, wE);
+    }
+    auto C = [](int i) { return std::sqrt(2.0 * i - 1.0); };  // 1-based
+    auto spow = [](double v, int e) { double r = 1.0; for (int k = 0; k < e; ++k) r *= v; return r; };
+    const int O = 2, Nm = 4;
+    vector<double> S((size_t)Nm * Nm, 0.0), Q(Nm, 0.0);
+    for (int ix = 1; ix <= O; ++ix)
+        for (int jx = 1; jx <= O; ++jx)
+            for (int iE = 1; iE <= O; ++iE)
+                for (int jE = 1; jE <= O; ++jE) {
+                    const int i = O * (ix - 1) + iE - 1, j = O * (jx - 1) + jE - 1;
+                    double& e = S[(size_t)i * Nm + j];
+                    if (i == j) e += St;
+                    if (iE == jE && ix >= jx + 1) e -= C(ix) * hx * sx * C(jx) * (1.0 - spow(-1.0, ix - jx));
+                    e += C(ix) * hx * spow(sx, ix - 1) * C(jx) * spow(sx, jx - 1) * wx[jx][jE - 1][iE - 1];
+                    if (ix == jx)
+                        for (int kE = 1; kE <= iE - 1; ++kE)
+                            for (int wv = 1; wv <= O; ++wv)
+                                e += C(iE) * C(jE) * C(kE) * C(wv) * (1.0 - spow(-1.0, iE - kE)) * Sv[wv - 1] *
+                                     W[(jE - 1) + O * ((kE - 1) + O * (wv - 1))];
+                    e += C(iE) * Sp * spow(-1.0, iE - 1) * C(jE) * spow(-1.0, jE - 1) * wE[jE][jx - 1][ix - 1];
+                }
+    for (int jx = 1; jx <= O; ++jx)
+        for (int jE = 1; jE <= O; ++jE) {
+            const int j = O * (jx - 1) + jE - 1;
+            Q[j] += Qn[j];
+            Q[j] -= C(jx) * hx * (spow(sx, jx - 1) * wx[0][jE - 1][jE - 1] - spow(-sx, jx - 1)) * fx[jE - 1];
+            Q[j] -= C(jE) * (spow(-1.0, jE - 1) * Sp * wE[0][jx - 1][jx - 1] - Sm) * fE[jx - 1];
+        }
+    vector<double> x = lu_solve(S, Q, Nm);
+    for (int j = 0; j < Nm; ++j) psi[j] = x[j];
+    for (int jx = 1; jx <= O; ++jx)
+        for (int jE = 1; jE <= O; ++jE) {
+            const int j = O * (jx - 1) + jE - 1;
+            if (!tang) {
+                if (jx == 1) fx[jE - 1] = wx[0][jE - 1][jE - 1] * fx[jE - 1];
+                for (int iE = 1; iE <= O; ++iE) fx[jE - 1] += C(jx) * spow(sx, jx - 1) * wx[jx][jE - 1][iE - 1] * x[j];
+            }
+            if (jE == 1) fE[jx - 1] = wE[0][jx - 1][jx - 1] * fE[jx - 1];
+            for (int ix = 1; ix <= O; ++ix) fE[jx - 1] += C(jE) * spow(-1.0, jE - 1) * wE[jE][jx - 1][ix - 1] * x[j];
+        }
+}
+
 // adapt: the adaptive closure (AWD0, order 1 on every axis; adaptive.jl on the axes (x[, y[, z]], E)
 // as sn_{1,2,3}D_bfp.jl call it, with 1/ΔE as the energy axis' h and the energy face as its inflow),
 // not for a tangential 1D direction (sn_1D_bfp.jl:52); inv_dE = 1/ΔE of the group.
@@ -1973,11 +2032,17 @@ void flux_bfp(const double* om, const double* h, double St, double Sm, double Sp
         hh[a] = h[a - 1] > 0 ? std::fabs(om[a - 1]) / h[a - 1] : 0.0;
     }
     if (tang1d) { s[1] = 1.0; hh[1] = 0.0; }
+    if (adapt && cl.Nm == 4 && cl.fc && cl.O[0] == 2 && cl.O[1] == 2 && cl.O[2] == 1 && cl.O[3] == 1) {
+        flux_bfp1d_awd1(om[0], h[0], St, Sm, Sp, Sv, W, Qn, f[0], f[1], inv_dE, psi);
+        return;
+    }
     double wa[4][2];
     const double* wl[4] = {w_in[0], w_in[1], w_in[2], w_in[3]};
     const double* const* w = w_in;
     if (adapt && !tang1d) {
-        if (cl.Nm != 1) fail("the adaptive closure with slowing down is provided for order 1 on every axis");
+        if (cl.Nm != 1)
+            fail("the adaptive closure with slowing down is provided for order 1 on every axis, or (2, 2) in (x, E) "
+                 "with full coupling in 1D");
         // axes x, y, z, then E; an axis a problem lacks has h = 0 and adds nothing to the estimate
         const double h4[4] = {hh[1], hh[2], hh[3], inv_dE}, f4[4] = {f[1][0], f[2][0], f[3][0], f[0][0]};
         double* wp[4] = {wa[1], wa[2], wa[3], wa[0]};
@@ -2291,8 +2356,10 @@ struct Solver {
             const int Kz = p.scheme_order_z > 1 ? p.scheme_order_z : 1;
             const string sy = ndims >= 2 ? scheme_y : (adapt ? "AWD" : "DD"), sz = ndims >= 3 ? scheme_z : (adapt ? "AWD" : "DD");
             const int oy = ndims >= 2 ? Ky : 1, oz = ndims >= 3 ? Kz : 1;
-            if (adapt && (Ke > 1 || K > 1 || oy > 1 || oz > 1))
-                fail("the adaptive closure with slowing down is provided for order 1 on every axis");
+            if (adapt && (Ke > 1 || K > 1 || oy > 1 || oz > 1) &&
+                !(ndims == 1 && Ke == 2 && K == 2 && p.partial_coupling == 0))
+                fail("the adaptive closure with slowing down is provided for order 1 on every axis, or (2, 2) in "
+                     "(x, E) with full coupling in 1D");
             if (Ke > 1 || K > 1 || oy > 1 || oz > 1 || scheme_e != "DD" || scheme != "DD" || sy != "DD" || sz != "DD") {
                 for (const string* sc : std::initializer_list<const string*>{&scheme_e, &scheme, &sy, &sz})
                     if (*sc != "DD" && *sc != "DG" && *sc != "AWD") fail("Closure relation not implemented yet.");
@@ -3418,6 +3485,22 @@ int oracle_cell_bfp_awd(int ndims, const double* om, const double* h, double St,
         Cell4 cl(1, 1, 1, 1, true);
         double* f[4] = {fe, fx, fy, fz};
         flux_bfp(om, h, St, Sm, Sp, Sv, W, Qn, f, cl, w, ndims == 1 && std::fabs(om[0]) <= 1e-10, psi, true, 1.0 / dE);
+        return 0;
+    } catch (std::exception& e) { g_err = e.what(); return -1; }
+}
+
+// One 1D slowing-down cell of orders (2, 2) in (x, E), full coupling, with the coupled linear AWD1:
+// Qn[4] (moment 2 ix + iE), fe[2] (energy face over x moments), fx[2] (x face over E moments) in and out.
+int oracle_cell_bfp1d_awd1(double mu, double dx, double St, double Sm, double Sp, double dE, const double* Qn,
+                           double* fe, double* fx, double* psi) {
+    try {
+        double Sv[2];
+        stopping_moments(Sm, Sp, 2, Sv);
+        vector<double> W(8);
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                for (int c = 0; c < 2; ++c) W[a + 2 * (b + 2 * c)] = legendre_triple(a, b, c);
+        flux_bfp1d_awd1(mu, dx, St, Sm, Sp, Sv, W.data(), Qn, fe, fx, 1.0 / dE, psi);
         return 0;
     } catch (std::exception& e) { g_err = e.what(); return -1; }
 }

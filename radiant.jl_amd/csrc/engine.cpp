@@ -175,8 +175,11 @@ static void validate(const rt_problem_desc& d) {
     if (!d.csd && d.order_e > 1) bad("an energy closure needs the slowing-down term (csd)");
     if (d.adaptive) {  // scheme "AWD" (adaptive.jl)
         if (d.adaptive != 1) bad("adaptive must be 0 or 1");
-        if (d.csd && (d.order_x > 1 || d.order_y > 1 || d.order_z > 1 || d.order_e > 1))
-            bad("the adaptive closure (AWD) with slowing down: order 1 on every axis, space and energy");
+        const bool sd11 = d.order_x <= 1 && d.order_y <= 1 && d.order_z <= 1 && d.order_e <= 1;
+        const bool sd22 = d.ndims == 1 && d.order_x == 2 && d.order_e == 2 && d.partial_coupling == 0;
+        if (d.csd && !sd11 && !sd22)
+            bad("the adaptive closure (AWD) with slowing down: order 1 on every axis, space and energy, or (2, 2) "
+                "in (x, E) with full coupling in 1D");
         if (d.precision != RT_PREC_FP64)
             bad("the adaptive closure (AWD) is not linear in the flux: it runs in the FP64 solve, whose "
                 "reconstruction is a full pass");
@@ -793,7 +796,8 @@ void Problem::launch_pass(PassDesc& pd, int prec, bool full_rows) {
         const int sp = prec == PBF16 ? P32 : prec;
         if (adapt) {
             if (sp != P64) throw Error(RT_EINVAL, "the adaptive closure runs FP64 passes only");
-            if (bfp) launch_sweep_bfp_awd(d_pd, Nd, awd_inv_dE, ho_lds(P64), st);
+            if (bfp && ndims == 1 && ho_K == 4) launch_sweep_1d_bfp_awd1(d_pd, Nd, awd_inv_dE, st);
+            else if (bfp) launch_sweep_bfp_awd(d_pd, Nd, awd_inv_dE, ho_lds(P64), st);
             else launch_sweep_awd(d_pd, Nd, ndims, ho_K, ho_lds(P64), st);
         } else if (bfp)
             launch_sweep_bfp(sp, d_pd, Nd, ho_KE, ho_Kx, ho_Ky, ho_Kz, ho_fc, ho_lds(sp), st);

@@ -468,6 +468,125 @@ __global__ __launch_bounds__(256) void sweep_2d_awd1_kernel(const PassDesc* __re
         for (int i = tid; i < 2 * Nx; i += nt) ((double*)d.by)[(size_t)n * Nx * 2 + i] = FY[i];
 }
 
+// ------------------------------------------------ 1D slowing-down cell of orders (2, 2) in (x, E)
+// flux_1D_BFP (sn_1D_bfp.jl:39-125) with the coupled linear AWD1 over the axes (x, E): h = (hx, 1/ΔE),
+// signs (sx, -1), the x face and the energy face as inflows, the cell's moments (2 ix + iE, energy
+// fastest) handed to the closure as they are.  One lane per direction walks its row; the x face
+// carries the 2 energy moments ([Nd][2] in bin_x / bx), the energy face of each cell the 2 x moments
+// ([Nd][cells][2] in psiE_in / psiE_out).  The in-group stopping-power term is 2 sqrt3 Sv[jE] in the
+// rows iE = 1 with Sv = ((S- + S+)/2, (S- - S+)/(2 sqrt3)).  A tangential direction keeps DG and its x face.
+__global__ __launch_bounds__(64) void sweep_1d_bfp_awd1_kernel(const PassDesc* __restrict__ pds, double inv_dE) {
+    const PassDesc& d = pds[0];
+    const int n = blockIdx.x * 64 + threadIdx.x;
+    if (n >= d.Nd) return;
+    const int Nx = d.Nx;
+    const size_t Ncell = (size_t)Nx;
+    const double mu = d.mu[n];
+    const bool tang = fabs(mu) <= 1e-10;
+    const bool fwd = mu >= 0.0;
+    const double sx = (tang || mu > 0.0) ? 1.0 : -1.0;
+    const double r3 = sqrt(3.0);
+    const double Cc[2] = {1.0, r3}, px[2] = {1.0, sx}, nx[2] = {1.0, -sx}, pe[2] = {1.0, -1.0};
+    double* __restrict__ psi = (double*)d.psi + (size_t)n * d.Nc;
+    const double* Ein = d.psiE_in ? d.psiE_in + (size_t)n * Ncell * 2 : nullptr;
+    double* Eout = d.psiE_out ? d.psiE_out + (size_t)n * Ncell * 2 : nullptr;
+    double fx[2] = {d.bin_x ? d.bin_x[(size_t)n * 2] : 0.0, d.bin_x ? d.bin_x[(size_t)n * 2 + 1] : 0.0};
+    for (int k = 0; k < Nx; ++k) {
+        const int ix = fwd ? k : Nx - 1 - k;
+        const double hx = tang ? 0.0 : fabs(mu) / d.dx[ix];
+        const int m = d.mat[(size_t)4 * ix];
+        const double St = d.sigt[m], Sm = d.csdS[2 * m], Sp = d.csdS[2 * m + 1];
+        const double Sv[2] = {(Sm + Sp) / 2.0, (Sm - Sp) / (2.0 * r3)};
+        double Qn[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) Qn[j] = psi[(size_t)j + 4 * ix];
+        const double fE[2] = {Ein ? Ein[(size_t)ix * 2] : 0.0, Ein ? Ein[(size_t)ix * 2 + 1] : 0.0};
+        double wx[3][2][2], wE[3][2][2];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+#pragma unroll
+                for (int e = 0; e < 2; ++e) wx[a][b][e] = wE[a][b][e] = (b == e && a > 0) ? 1.0 : 0.0;
+        if (!tang) {
+            const double p0 = Qn[0] + hx * fx[0] + inv_dE * fE[0];
+            const double xc = awd1::centroid(p0, sx * r3 * (Qn[1] - r3 * hx * sx * fx[0] + inv_dE * fE[1]));
+            const double ec = awd1::centroid(p0, -r3 * (Qn[2] + hx * fx[1] + r3 * inv_dE * fE[0]));
+            double Qw[2], Tw[2];
+            awd1::constant_linear(xc, ec, Qw, Tw);
+            wx[2][0][0] = Qw[0]; wx[2][1][1] = Qw[0]; wx[2][0][1] = -Tw[0] / r3;
+            wE[2][0][0] = Qw[1]; wE[2][1][1] = Qw[1]; wE[2][0][1] = sx * Tw[1] / r3;
+        }
+        double S[4][4], q[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) S[i][j] = i == j ? St : 0.0;
+#pragma unroll
+        for (int ax = 0; ax < 2; ++ax)
+#pragma unroll
+            for (int ae = 0; ae < 2; ++ae) {
+                const int i = 2 * ax + ae;
+#pragma unroll
+                for (int bx = 0; bx < 2; ++bx)
+#pragma unroll
+                    for (int be = 0; be < 2; ++be) {
+                        const int j = 2 * bx + be;
+                        if (ae == be && ax > bx && ((ax - bx) & 1)) S[i][j] -= 2.0 * Cc[ax] * hx * sx * Cc[bx];
+                        S[i][j] += Cc[ax] * hx * px[ax] * Cc[bx] * px[bx] * wx[bx + 1][be][ae];
+                        if (ax == bx && ae == 1) S[i][j] += 2.0 * Cc[1] * Sv[be];
+                        S[i][j] += Cc[ae] * Sp * pe[ae] * Cc[be] * pe[be] * wE[be + 1][bx][ax];
+                    }
+                q[i] = Qn[i] - Cc[ax] * hx * (px[ax] * wx[0][ae][ae] - nx[ax]) * fx[ae] -
+                       Cc[ae] * (pe[ae] * Sp * wE[0][ax][ax] - Sm) * fE[ax];
+            }
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+#pragma unroll
+            for (int i = p + 1; i < 4; ++i) {
+                const bool sw = fabs(S[i][p]) > fabs(S[p][p]);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const double t = S[p][j];
+                    S[p][j] = sw ? S[i][j] : t;
+                    S[i][j] = sw ? t : S[i][j];
+                }
+                const double tq = q[p];
+                q[p] = sw ? q[i] : tq;
+                q[i] = sw ? tq : q[i];
+            }
+#pragma unroll
+            for (int i = p + 1; i < 4; ++i) {
+                const double f = S[i][p] / S[p][p];
+#pragma unroll
+                for (int j = p + 1; j < 4; ++j) S[i][j] -= f * S[p][j];
+                q[i] -= f * q[p];
+            }
+        }
+#pragma unroll
+        for (int i = 3; i >= 0; --i) {
+            double v = q[i];
+#pragma unroll
+            for (int j = i + 1; j < 4; ++j) v -= S[i][j] * q[j];
+            q[i] = v / S[i][i];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) psi[(size_t)j + 4 * ix] = q[j];
+        double ox[2] = {wx[0][0][0] * fx[0], wx[0][1][1] * fx[1]}, oE[2] = {wE[0][0][0] * fE[0], wE[0][1][1] * fE[1]};
+#pragma unroll
+        for (int ax = 0; ax < 2; ++ax)
+#pragma unroll
+            for (int ae = 0; ae < 2; ++ae) {
+                const double v = q[2 * ax + ae];
+                ox[ae] += Cc[ax] * px[ax] * (wx[ax + 1][ae][0] + wx[ax + 1][ae][1]) * v;
+                oE[ax] += Cc[ae] * pe[ae] * (wE[ae + 1][ax][0] + wE[ae + 1][ax][1]) * v;
+            }
+        if (!tang) { fx[0] = ox[0]; fx[1] = ox[1]; }
+        if (Eout) { Eout[(size_t)ix * 2] = oE[0]; Eout[(size_t)ix * 2 + 1] = oE[1]; }
+    }
+    if (d.bx) { ((double*)d.bx)[(size_t)n * 2] = fx[0]; ((double*)d.bx)[(size_t)n * 2 + 1] = fx[1]; }
+}
+
 // ------------------------------------------------------------------------------------------- 3D
 __global__ __launch_bounds__(256) void sweep_3d_awd_kernel(const PassDesc* __restrict__ pds) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -632,6 +751,16 @@ void launch_sweep_awd(const PassDesc* d_pd, int Nd, int ndims, int K, size_t lds
         }
         awd_launch(fn, dim3(Nd), dim3(256), lds, st, d_pd);
     }
+    RT_HIP(hipGetLastError());
+}
+
+void launch_sweep_1d_bfp_awd1(const PassDesc* d_pd, int Nd, double inv_dE, hipStream_t st) {
+    if (Nd <= 0) return;
+    hipEvent_t a, b;
+    take_launch_events(&a, &b);
+    const dim3 grid((unsigned)((Nd + 63) / 64));
+    if (a) hipExtLaunchKernelGGL(sweep_1d_bfp_awd1_kernel, grid, dim3(64), 0, st, a, b, 0u, d_pd, inv_dE);
+    else hipLaunchKernelGGL(sweep_1d_bfp_awd1_kernel, grid, dim3(64), 0, st, d_pd, inv_dE);
     RT_HIP(hipGetLastError());
 }
 

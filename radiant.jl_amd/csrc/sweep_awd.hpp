@@ -17,5 +17,8 @@ void launch_sweep_awd(const PassDesc* d_pd, int Nd, int ndims, int K, size_t lds
 // walker's faces and energy faces (lds = sweep_bfp_lds_bytes with orders 1, 0 with hoface);
 // inv_dE = 1/ΔE of the group, the energy axis' h in the adaptive estimate.
 void launch_sweep_bfp_awd(const PassDesc* d_pd, int Nd, double inv_dE, size_t lds, hipStream_t st);
+// The 1D slowing-down cell of orders (2, 2) in (x, E) with full coupling and the coupled linear AWD1:
+// one lane per direction, the x face [Nd][2] and the energy faces [Nd][cells][2].
+void launch_sweep_1d_bfp_awd1(const PassDesc* d_pd, int Nd, double inv_dE, hipStream_t st);
 
 }  // namespace rt
