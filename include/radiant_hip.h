@@ -119,8 +119,9 @@ typedef struct rt_problem_desc {
      * [Np, Nm5 * Nx]; qvol keeps one value per cell (spatial moment 1, Source.jl:135). */
     int32_t order_x;
     const double *omega_x;
-    /* The y axis of 2D problems likewise (order_y at most 4, omega_y[𝒪y + 1]; with order_x at most 4
-     * and at most 12 moments per cell),
+    /* The y axis of 2D problems likewise (order_y at most 8, omega_y[𝒪y + 1]; with order_x at most 8
+     * and at most 64 moments per cell: up to 12 moments and orders 4 the cell system is held in
+     * registers, beyond that it is solved in device scratch of Nd * 64 * (Nm5^2 + Nm5) doubles),
      * and the coupling of the high-order moments (SN.set_is_full_coupling; 0 = full, the default:
      * Nm5 = 𝒪x*𝒪y moments per cell; 1 = 𝒪x + 𝒪y - 1).  The x-face boundary moments then carry 𝒪y
      * moments per position and the y faces 𝒪x (sn_one_speed.jl:92-106: 𝚽x12[Nps, Nm1, 2, Ny], ...),
@@ -128,8 +129,8 @@ typedef struct rt_problem_desc {
     int32_t order_y;
     const double *omega_y;
     int32_t partial_coupling;
-    /* The z axis of 3D problems likewise (orders at most 3 per axis in 3D and at most 12 moments per
-     * cell; Nm5 = 𝒪x*𝒪y*𝒪z moments, or 𝒪x + 𝒪y + 𝒪z - 2 without full coupling; a face carries the
+    /* The z axis of 3D problems likewise (orders at most 8 per axis in 3D and at most 64 moments per
+     * cell, in registers up to orders 3 and 12 moments; Nm5 = 𝒪x*𝒪y*𝒪z moments, or 𝒪x + 𝒪y + 𝒪z - 2 without full coupling; a face carries the
      * moments over the other two axes). */
     int32_t order_z;
     const double *omega_z;

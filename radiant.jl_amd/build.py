@@ -7,7 +7,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "radiant_hip", "libradiant_hip.so")
 BUILD = os.path.join(HERE, "_build")
-SOURCES = ["kernels.hip", "sweep_bfp.hip", "sweep_awd.hip", "em_source.hip", "engine.cpp", "batch.cpp", "abi.cpp", "serve.cpp", "pool.cpp"]
+SOURCES = ["kernels.hip", "sweep_bfp.hip", "sweep_awd.hip", "em_source.hip", "sweep_hog.hip", "engine.cpp", "batch.cpp", "abi.cpp", "serve.cpp", "pool.cpp"]
 FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
          # packed FP32 instructions are disabled (they have produced wrong results on this hardware)
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -107,6 +107,7 @@ int BatchSolver::add(Problem* P) {
     if (P->precision == RT_PREC_FP64)
         throw Error(RT_EINVAL, "the batched solver runs the mixed-precision modes");
     if (P->em) throw Error(RT_EINVAL, "a magnetic field runs in the single-problem solve, not in lockstep batches");
+    if (P->hog) throw Error(RT_EINVAL, "closures beyond the register walkers run in the single-problem solve");
     if (!P->ho_K && (P->scan_cpl <= 0 || sweep_scan_lds_bytes(P64, P->ndims, P->Nx, P->Ny, P->Nz, P->Nmat) > 160 * 1024))
         throw Error(RT_EINVAL, "a batched problem does not fit the row-scan sweep");
     int slot = -1;

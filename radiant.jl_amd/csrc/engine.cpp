@@ -7,6 +7,7 @@
 #include "engine.hpp"
 #include "sweep_awd.hpp"
 #include "em_source.hpp"
+#include "sweep_hog.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -158,20 +159,18 @@ static void validate(const rt_problem_desc& d) {
     }
     if (surf && (d.Np_source < 1 || d.Np_source > d.Nps))
         bad("Np_source must lie in 1..Nps when a surface source is given");
-    const int kmax = d.ndims == 1 ? HO_KMAX : (d.ndims == 2 ? HO2_KMAX : HO3_KMAX);
+    const int kmax = d.ndims == 1 ? HO_KMAX : HOG_KMAX;
     if (d.order_x < 0 || d.order_x > kmax || d.order_y < 0 || d.order_y > kmax || d.order_z < 0 || d.order_z > kmax)
         bad("spatial closure orders must lie in 1.." + std::to_string(kmax) + " in " + std::to_string(d.ndims) + "D");
     if (d.order_x > 1 && !d.omega_x) bad("closures of order above 1 need their weights omega_x");
     if (d.ndims >= 2 && d.order_y > 1 && !d.omega_y) bad("closures of order above 1 need their weights omega_y");
     if (d.ndims >= 3 && d.order_z > 1 && !d.omega_z) bad("closures of order above 1 need their weights omega_z");
-    if (d.ndims == 2 && !d.csd &&
-        !sweep_2d_ho_supported(std::max(1, d.order_x), std::max(1, d.order_y), d.partial_coupling == 0))
-        bad("2D closures: orders up to " + std::to_string(HO2_KMAX) + " per axis and at most " +
-            std::to_string(HO2_NM_MAX) + " moments per cell");
-    if (d.ndims == 3 && !d.csd &&
-        !sweep_3d_ho_supported(std::max(1, d.order_x), std::max(1, d.order_y), std::max(1, d.order_z), d.partial_coupling == 0))
-        bad("3D closures: orders up to " + std::to_string(HO3_KMAX) + " per axis and at most " +
-            std::to_string(HO3_NM_MAX) + " moments per cell");
+    // 2D/3D closures: the register walkers up to 12 moments per cell, the general walker beyond
+    if (d.ndims >= 2 && !d.csd &&
+        !sweep_hog_supported(d.ndims, std::max(1, d.order_x), std::max(1, d.order_y),
+                             d.ndims >= 3 ? std::max(1, d.order_z) : 1, d.partial_coupling == 0))
+        bad(std::to_string(d.ndims) + "D closures: orders up to " + std::to_string(HOG_KMAX) + " per axis and at most " +
+            std::to_string(HOG_NM_MAX) + " moments per cell");
     if (!d.csd && d.order_e > 1) bad("an energy closure needs the slowing-down term (csd)");
     if (d.adaptive) {  // scheme "AWD" (adaptive.jl)
         if (d.adaptive != 1) bad("adaptive must be 0 or 1");
@@ -326,6 +325,11 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
             const size_t fb = sweep_bfp_lds_bytes(P64, ndims, Nx, Ny, Nz, ho_KE, ho_Kx, ho_Ky, ho_Kz, ho_fc);
             if (fb > 160 * 1024) d_hoface = dalloc<double>((size_t)Nd * fb / sizeof(double));
         } else if (ho_K) {  // closure weights per axis (diamond difference of order 1 when absent)
+            if (ndims >= 2 && !adapt &&
+                !(ndims == 2 ? sweep_2d_ho_supported(ho_Kx, ho_Ky, ho_fc) : sweep_3d_ho_supported(ho_Kx, ho_Ky, ho_Kz, ho_fc))) {
+                hog = true;  // beyond the register walkers: the general walker and its cell-system scratch
+                d_hog = dalloc<double>(sweep_hog_scratch(Nd, ho_Kx, ho_Ky, ndims == 3 ? ho_Kz : 1, ho_fc));
+            }
             const int kx = ndims >= 2 ? ho_Kx : ho_K;
             std::vector<double> om(d.omega_x ? d.omega_x : nullptr, d.omega_x ? d.omega_x + kx + 1 : nullptr);
             if (om.empty()) om = {-1.0, 2.0};
@@ -683,6 +687,9 @@ int64_t Problem::device_bytes(const rt_problem_desc& d) {
     if (d.precision != RT_PREC_FP64) b += (int64_t)d.Nd * Nc * 8;  // angular flux of c = T(0) kept by batches
     if (d.csd) b += 2LL * d.Nd * Nc * 8;                              // energy-face angular fluxes (at most Nc per direction)
     if (d.Mem) b += (int64_t)Nact_est * Nc * 8 + (int64_t)d.G * Nact_est * Nact_est * 8;  // magnetic-field source
+    if (ho_order(d) && !d.csd && d.ndims >= 2 && ho_order(d) > 12)                        // general walker scratch
+        b += (int64_t)sweep_hog_scratch(d.Nd, std::max(1, d.order_x), std::max(1, d.order_y),
+                                        d.ndims == 3 ? std::max(1, d.order_z) : 1, d.partial_coupling == 0) * 8;
     if (has_fp(d)) b += (int64_t)Nact_est * (Nc + Nact_est) * 8;     // Fokker-Planck source, ℳ
     b += (int64_t)std::min<int64_t>((int64_t)d.G * d.Np * Nc, std::max<int64_t>(FLUX_STAGE, (int64_t)d.G * d.Np)) * 8;
     if (split_scan_offered(d)) b += ((int64_t)d.Nd + (int64_t)split_scan_words(d)) * 8;
@@ -801,6 +808,8 @@ void Problem::launch_pass(PassDesc& pd, int prec, bool full_rows) {
             else launch_sweep_awd(d_pd, Nd, ndims, ho_K, ho_lds(P64), st);
         } else if (bfp)
             launch_sweep_bfp(sp, d_pd, Nd, ho_KE, ho_Kx, ho_Ky, ho_Kz, ho_fc, ho_lds(sp), st);
+        else if (hog)
+            launch_sweep_hog(sp, d_pd, Nd, ndims, ho_Kx, ho_Ky, ndims == 3 ? ho_Kz : 1, ho_fc, ho_lds(sp), d_hog, st);
         else if (ho_K && ndims == 3)
             launch_sweep_3d_ho(sp, d_pd, Nd, ho_Kx, ho_Ky, ho_Kz, ho_fc, ho_lds(sp), st);
         else if (ho_K && ndims == 2)

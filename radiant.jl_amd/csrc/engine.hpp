@@ -102,6 +102,8 @@ public:
     // then span (E, x, y, z)
     bool bfp = false;
     bool adapt = false;  // adaptive weighted diamond closure (sweep_awd.hip), FP64 passes only
+    bool hog = false;       // a 2D/3D closure beyond the register walkers (sweep_hog.hip)
+    double* d_hog = nullptr;  // its cell-system scratch
     double awd_inv_dE = 0.0;  // 1/ΔE of the group of the pass being set up (adaptive slowing-down cells)
     int ho_KE = 1, fm_e = 1;
     // the 3D and slowing-down walkers keep their face moments in device memory when they exceed

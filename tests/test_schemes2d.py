@@ -186,11 +186,10 @@ def test_scheme2d_gpu_accelerations_and_batch(oracle):
 
 @pytest.mark.gpu
 def test_scheme2d_moment_limit_refused():
-    """Full coupling of orders 4 x 4 gives 16 moments per cell, above the 12 whose cell system the
-    sweep holds in registers: refused at set-up with the limits named (partial coupling of the same
-    orders, 7 moments, is solved above)."""
+    """Orders above 8 per axis are refused at set-up with the limit named (orders 4 x 4 with full
+    coupling, 16 moments per cell, run on the general walker)."""
     from radiant_hip import problems as P, solver as S
     from radiant_hip._lib import RTError
-    p = _set(P.small_problem(2, dims=(6, 5), G=2, L=2, seed=99), "DD", "DD", 4, 4, True)
-    with pytest.raises(RTError, match="at most 12 moments per cell"):
+    p = _set(P.small_problem(2, dims=(6, 5), G=2, L=2, seed=99), "DD", "DD", 9, 9, False)
+    with pytest.raises(RTError, match=r"orders must lie in 1\.\.8 in 2D"):
         S.ResidentProblem(p)

@@ -163,12 +163,12 @@ def test_scheme3d_gpu_parity(oracle, case):
 
 @pytest.mark.gpu
 def test_scheme3d_moment_limit_refused():
-    """Full coupling of orders 3 x 3 x 2 gives 18 moments per cell, above the 12 whose cell system
-    the sweep holds in registers: refused at set-up with the limits named."""
+    """Full coupling of orders 4 x 4 x 5 gives 80 moments per cell, above the 64 of the general
+    walker (orders 3 x 3 x 2, 18 moments, run there): refused at set-up with the limits named."""
     from radiant_hip import problems as P, solver as S
     from radiant_hip._lib import RTError
-    p = _set(P.small_problem(3, dims=(4, 3, 5), G=2, L=2, seed=140), "DD", "DD", "DD", 3, 3, 2, True)
-    with pytest.raises(RTError, match="at most 12 moments per cell"):
+    p = _set(P.small_problem(3, dims=(4, 3, 5), G=2, L=2, seed=140), "DD", "DD", "DD", 4, 4, 5, True)
+    with pytest.raises(RTError, match="at most 64 moments per cell"):
         S.ResidentProblem(p)
 
 
