@@ -139,8 +139,9 @@ typedef struct rt_problem_desc {
      * Legendre moments per cell (0 or 1: first order, at most 2) and omega_e[𝒪E + 1] its closure
      * weights (NULL: diamond difference of order 1).  With any closure other than diamond
      * difference of order 1 on every axis (space or energy), the slowing-down types take the
-     * general cell: spatial orders up to 4 in 1D and 2 per axis in 2D/3D, DD or DG on each axis,
-     * at most 8 moments per cell.  A cell then holds Nm5 moments over (E, x[, y[, z]]), energy
+     * general cell: spatial orders up to 8 per axis, DD or DG on each axis, at most 64 moments per
+     * cell (in registers up to 8 moments and orders 4 in 1D / 2 per axis in 2D/3D; beyond that in
+     * device scratch of Nd * 64 * (Nm5^2 + Nm5) doubles).  A cell then holds Nm5 moments over (E, x[, y[, z]]), energy
      * fastest (𝒪E*𝒪x*... with full coupling, 𝒪E + 𝒪x + ... - (ndims) without); the faces carry the
      * energy moments too (Nm1 = 𝒪E*𝒪y*𝒪z, ...), and the cutoff flux fills the first
      * Nm4 = 𝒪x*𝒪y*𝒪z moments of each cell (sn_flux.jl:216). */

@@ -195,10 +195,12 @@ static void validate(const rt_problem_desc& d) {
     if (d.csd && ho_order(d)) {
         const int Ke = d.order_e > 1 ? d.order_e : 1;
         if (Ke > 1 && !d.omega_e) bad("energy closures of order above 1 need their weights omega_e");
-        if (!sweep_bfp_supported(d.ndims, Ke, std::max(1, d.order_x), d.ndims >= 2 ? std::max(1, d.order_y) : 1,
-                                 d.ndims >= 3 ? std::max(1, d.order_z) : 1, d.partial_coupling == 0))
-            bad("slowing-down closures: energy order at most 2, spatial orders up to 4 in 1D and 2 per axis in "
-                "2D/3D, at most " + std::to_string(SWEEP_BFP_NM) + " moments per cell");
+        const int kx = std::max(1, d.order_x), ky = d.ndims >= 2 ? std::max(1, d.order_y) : 1,
+                  kz = d.ndims >= 3 ? std::max(1, d.order_z) : 1;
+        if (!sweep_bfp_supported(d.ndims, Ke, kx, ky, kz, d.partial_coupling == 0) &&
+            !sweep_hog_bfp_supported(d.ndims, Ke, kx, ky, kz, d.partial_coupling == 0))
+            bad("slowing-down closures: energy order at most 2, spatial orders up to " + std::to_string(HOG_KMAX) +
+                " per axis, at most " + std::to_string(HOG_NM_MAX) + " moments per cell");
     }
     if (d.solver_type < 0 || d.solver_type > 6) bad("Unknown type of solver.");
     const int stype = solver_type_of(d);
@@ -324,6 +326,10 @@ Problem::Problem(const rt_problem_desc& d, hipStream_t stream) {
             d_emap = dalloc<int>(emap.size()); upload(d_emap, emap, st);
             const size_t fb = sweep_bfp_lds_bytes(P64, ndims, Nx, Ny, Nz, ho_KE, ho_Kx, ho_Ky, ho_Kz, ho_fc);
             if (fb > 160 * 1024) d_hoface = dalloc<double>((size_t)Nd * fb / sizeof(double));
+            if (!adapt && !sweep_bfp_supported(ndims, ho_KE, ho_Kx, ho_Ky, ho_Kz, ho_fc)) {
+                hog = true;  // beyond the register walker: the general walker and its cell-system scratch
+                d_hog = dalloc<double>(sweep_hog_bfp_scratch(Nd, ho_KE, ho_Kx, ho_Ky, ho_Kz, ho_fc));
+            }
         } else if (ho_K) {  // closure weights per axis (diamond difference of order 1 when absent)
             if (ndims >= 2 && !adapt &&
                 !(ndims == 2 ? sweep_2d_ho_supported(ho_Kx, ho_Ky, ho_fc) : sweep_3d_ho_supported(ho_Kx, ho_Ky, ho_Kz, ho_fc))) {
@@ -806,7 +812,9 @@ void Problem::launch_pass(PassDesc& pd, int prec, bool full_rows) {
             if (bfp && ndims == 1 && ho_K == 4) launch_sweep_1d_bfp_awd1(d_pd, Nd, awd_inv_dE, st);
             else if (bfp) launch_sweep_bfp_awd(d_pd, Nd, awd_inv_dE, ho_lds(P64), st);
             else launch_sweep_awd(d_pd, Nd, ndims, ho_K, ho_lds(P64), st);
-        } else if (bfp)
+        } else if (bfp && hog)
+            launch_sweep_hog_bfp(sp, d_pd, Nd, ndims, ho_KE, ho_Kx, ho_Ky, ho_Kz, ho_fc, ho_lds(sp), d_hog, st);
+        else if (bfp)
             launch_sweep_bfp(sp, d_pd, Nd, ho_KE, ho_Kx, ho_Ky, ho_Kz, ho_fc, ho_lds(sp), st);
         else if (hog)
             launch_sweep_hog(sp, d_pd, Nd, ndims, ho_Kx, ho_Ky, ndims == 3 ? ho_Kz : 1, ho_fc, ho_lds(sp), d_hog, st);

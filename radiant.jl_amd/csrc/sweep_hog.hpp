@@ -19,4 +19,12 @@ size_t sweep_hog_scratch(int Nd, int Kx, int Ky, int Kz, bool fc);
 void launch_sweep_hog(int prec, const PassDesc* d_pd, int Nd, int ndims, int Kx, int Ky, int Kz, bool fc, size_t lds,
                       double* scratch, hipStream_t st);
 
+// The slowing-down cell (E, x, y, z) of the same walk (energy order 1-2, spatial orders up to HOG_KMAX
+// per axis, at most HOG_NM_MAX moments): the cells beyond sweep_bfp.hip's register walker.
+bool sweep_hog_bfp_supported(int ndims, int Ke, int Kx, int Ky, int Kz, bool fc);
+// its scratch, in doubles
+size_t sweep_hog_bfp_scratch(int Nd, int Ke, int Kx, int Ky, int Kz, bool fc);
+void launch_sweep_hog_bfp(int prec, const PassDesc* d_pd, int Nd, int ndims, int Ke, int Kx, int Ky, int Kz, bool fc,
+                          size_t lds, double* scratch, hipStream_t st);
+
 }  // namespace rt

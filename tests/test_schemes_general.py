@@ -136,3 +136,82 @@ def test_general_closure_limits_refused(nd, orders, msg):
     p = _set(P.small_problem(nd, dims=dims, G=2, L=1, seed=198), orders, True)
     with pytest.raises(RTError, match=msg):
         S.ResidentProblem(p)
+
+
+# Slowing-down cells (E, x, y, z) beyond the register walker of 8 moments and orders 4 (1D) / 2
+# (2D, 3D): (ndims, orders (E, x, y, z), schemes, full coupling, boundary conditions)
+BFP_CASES = [(1, (2, 6, 1, 1), ("DG", "DD", "DD", "DD"), True, None),        # 12 moments
+             (1, (1, 8, 1, 1), ("DD", "DG", "DD", "DD"), True, [1, 2]),      # order 8
+             (2, (2, 3, 3, 1), ("DG", "DD", "DD", "DD"), True, None),        # 18
+             (2, (2, 4, 2, 1), ("DD", "DG", "DD", "DD"), False, [1, 1, 1, 1]),
+             (3, (2, 2, 2, 2), ("DG", "DD", "DD", "DD"), True, None),        # 16
+             (3, (2, 3, 2, 2), ("DD", "DD", "DG", "DD"), False, [1, 1, 0, 0, 2, 2])]
+
+
+def _bfp_problem(case, tol=1e-12):
+    from radiant_hip import problems as P
+    nd, orders, schemes, fc, bc = BFP_CASES[case]
+    dims = {1: (9,), 2: (5, 4), 3: (3, 4, 3)}[nd]
+    p = P.small_problem(nd, dims=dims if nd > 1 else None, N=dims[0], G=3, L=2, tol=tol, seed=220 + case, bc=bc)
+    rng = np.random.default_rng(230 + case)
+    p.csd = True
+    p.Sb = 0.8 * rng.uniform(0.5, 2.0, (p.G + 1, p.Nmat))
+    p.dE = rng.uniform(0.5, 1.5, p.G)
+    p.scheme_e, p.scheme, p.scheme_y, p.scheme_z = schemes
+    p.scheme_order_e, p.scheme_order, p.scheme_order_y, p.scheme_order_z = orders
+    p.full_coupling = fc
+    return p
+
+
+@pytest.mark.parametrize("case", range(len(BFP_CASES)))
+def test_oracle_converges_slowing_down(oracle, case):
+    phi, cut, st = oracle.solve_cutoff(_bfp_problem(case, tol=1e-10))
+    assert all(s["converged"] for s in st)
+    assert np.isfinite(phi).all() and np.abs(phi).max() > 0 and np.abs(cut).max() > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", range(len(BFP_CASES)))
+def test_general_slowing_down_closure_gpu_parity(oracle, case):
+    from radiant_hip import solver as S
+    p = _bfp_problem(case)
+    _say("general slowing-down closure", BFP_CASES[case])
+    ref, rcut, rst = oracle.solve_cutoff(p)
+    assert all(s["converged"] for s in rst)
+    K = oracle.spatial_moments(p)
+    h = S.ResidentProblem(p)
+    try:
+        nd, np_, nps, Sl = oracle.sizes(p)
+        assert Sl == h.state_length()
+        rng = np.random.default_rng(700 + case)
+        z = rng.standard_normal(Sl)
+        q = rng.standard_normal(np_ * p.Nc * K)
+        q.reshape(-1, np_)[:, [i for i in range(np_) if h.d.ang.pl[i] > p.Lxs]] = 0.0
+        for mode in (0, 1, 2):
+            want = oracle.transport_pass(p, 0, 0 if mode == 0 else 1, z, q)
+            if mode == 2:
+                want = z - want
+            e = rel_err(h.transport_pass(0, mode, q, z), want)
+            _say("  pass mode", mode, e)
+            assert e < 1e-12
+    finally:
+        h.close()
+    phi, cut, st = S.compute_flux_cutoff(p)
+    e, ec = rel_err(phi, ref), rel_err(cut, rcut)
+    _say("  solve", e, "cutoff", ec)
+    assert e < 1e-10 and ec < 1e-10 and all(s["converged"] for s in st)
+    mx, _ = S.compute_flux(p, precision="mixed-fp32", state="reduced")
+    e = rel_err(mx, ref)
+    _say("  mixed-fp32", e)
+    assert e < 1e-6
+
+
+@pytest.mark.gpu
+def test_general_slowing_down_limits_refused():
+    """Energy orders above 2 and slowing-down cells above 64 moments are refused at set-up."""
+    from radiant_hip import solver as S
+    from radiant_hip._lib import RTError
+    p = _bfp_problem(4)
+    p.scheme_order, p.scheme_order_y, p.scheme_order_z = 5, 4, 2   # 2 * 5 * 4 * 2 = 80 moments
+    with pytest.raises(RTError, match="at most 64 moments per cell"):
+        S.ResidentProblem(p)
