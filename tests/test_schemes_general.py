@@ -5,8 +5,9 @@ take any order; the register walkers hold at most 12 moments).
 CPU: the oracle's solves of these closures converge, and an infinite medium keeps its first-order
 flux with vanishing slopes at these orders too (the cell systems are those the lower-order tests pin
 by polynomial exactness).
-GPU: passes (1e-12) and solves (1e-10 FP64, 1e-6 mixed FP32) against the oracle; a batch holding
-such a problem solves it on its own stream; orders above 8 or more than 64 moments are refused."""
+GPU: passes (1e-12) and solves (1e-10 FP64, 1e-6 mixed FP32) against the oracle; a batch or the
+server holding such a problem solves it on its own stream; orders above 8 or more than 64 moments
+are refused.  The slowing-down cells (E, x, y, z) beyond 8 moments take the same checks."""
 import copy
 import sys
 
@@ -215,3 +216,16 @@ def test_general_slowing_down_limits_refused():
     p.scheme_order, p.scheme_order_y, p.scheme_order_z = 5, 4, 2   # 2 * 5 * 4 * 2 = 80 moments
     with pytest.raises(RTError, match="at most 64 moments per cell"):
         S.ResidentProblem(p)
+
+
+@pytest.mark.gpu
+def test_general_closures_served(oracle):
+    """Requests whose cells need the general walkers (a 2D 4 x 4 closure and a 3D 2 x 2 x 2 x 2
+    slowing-down cell) through the server in mixed FP32: the lockstep engine does not take them, the
+    server solves each on its own stream, and the flux matches the oracle."""
+    from radiant_hip import solver as S
+    for p in (_problem(0), _bfp_problem(4)):
+        ref = oracle.solve(p)[0]
+        e = rel_err(S.serve_alone(p, "mixed-fp32"), ref)
+        _say("general closure served", e)
+        assert e < 1e-6
