@@ -693,9 +693,16 @@ int64_t Problem::device_bytes(const rt_problem_desc& d) {
     if (d.precision != RT_PREC_FP64) b += (int64_t)d.Nd * Nc * 8;  // angular flux of c = T(0) kept by batches
     if (d.csd) b += 2LL * d.Nd * Nc * 8;                              // energy-face angular fluxes (at most Nc per direction)
     if (d.Mem) b += (int64_t)Nact_est * Nc * 8 + (int64_t)d.G * Nact_est * Nact_est * 8;  // magnetic-field source
-    if (ho_order(d) && !d.csd && d.ndims >= 2 && ho_order(d) > 12)                        // general walker scratch
-        b += (int64_t)sweep_hog_scratch(d.Nd, std::max(1, d.order_x), std::max(1, d.order_y),
-                                        d.ndims == 3 ? std::max(1, d.order_z) : 1, d.partial_coupling == 0) * 8;
+    if (ho_order(d) && !d.adaptive) {  // the general walkers' cell-system scratch
+        const int kx = std::max(1, d.order_x), ky = d.ndims >= 2 ? std::max(1, d.order_y) : 1,
+                  kz = d.ndims >= 3 ? std::max(1, d.order_z) : 1, ke = d.csd && d.order_e > 1 ? d.order_e : 1;
+        const bool fc = d.partial_coupling == 0;
+        if (d.csd && !sweep_bfp_supported(d.ndims, ke, kx, ky, kz, fc))
+            b += (int64_t)sweep_hog_bfp_scratch(d.Nd, ke, kx, ky, kz, fc) * 8;
+        else if (!d.csd && d.ndims >= 2 &&
+                 !(d.ndims == 2 ? sweep_2d_ho_supported(kx, ky, fc) : sweep_3d_ho_supported(kx, ky, kz, fc)))
+            b += (int64_t)sweep_hog_scratch(d.Nd, kx, ky, kz, fc) * 8;
+    }
     if (has_fp(d)) b += (int64_t)Nact_est * (Nc + Nact_est) * 8;     // Fokker-Planck source, ℳ
     b += (int64_t)std::min<int64_t>((int64_t)d.G * d.Np * Nc, std::max<int64_t>(FLUX_STAGE, (int64_t)d.G * d.Np)) * 8;
     if (split_scan_offered(d)) b += ((int64_t)d.Nd + (int64_t)split_scan_words(d)) * 8;
